@@ -1,0 +1,132 @@
+"""Benchmark: one Newton iteration's non-linear stages on one MI355X.
+
+Workload: four 20^3-cell lattice cubes (37,044 vertices / 160,000 tets, cell 0.1 m),
+seeded yaw rotations, stacked with 1e-3 m gaps on the ground (every interface in
+contact), E=1e5 Pa, nu=0.4, rho=1000, dt=0.01 s, relative dHat 1e-3, FP64.  A step is one
+Newton iteration of the reference's barrier subproblem without the linear solve
+(CPU IPC/IPCSolver.cpp:912-998), on a frozen state with inputs resident in HBM:
+  1. LBVH build + active set      (buildCollisionSets)
+  2. gradient + projected Hessian -> the reference's lower CSR  (computeGradientAndHessian + assemble)
+  3. step bound: ground + partial CCD + CFL + swept LBVH + full CCD
+  4. one line-search energy evaluation
+The frozen state: x as generated, each cube given a seeded velocity in [-0.5, 0.5]^3 m/s,
+x~ from velocity and gravity, direction p = x - x~ perturbed per vertex by +-5%.
+
+Prints one JSON line.  --gpus above 1 exits: every run here uses one GPU.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "cpu-ipc_amd"))
+
+METRIC = "ms per Newton iter + assembly HBM GB/s; learned-PCG vs plain at 1e-6 rel residual"
+HBM_PEAK_GBS = 8000.0
+
+
+def elastic_algorithmic_bytes(scene):
+    """Minimal HBM traffic of the inertia + tet assembly per launch: read x (24 B/vertex),
+    per tet Dm^-1 (72 B) + volume (8 B) + indices (16 B), write the gradient (24 B/vertex)
+    and the elastic lower-CSR values (8 B per nonzero: 6 per vertex + 9 per coupled
+    vertex pair)."""
+    import numpy as np
+    t = scene.tets.astype(np.int64)
+    pairs = np.concatenate([t[:, [a, b]] for a in range(4) for b in range(a + 1, 4)])
+    pairs.sort(axis=1)
+    E = len(np.unique(pairs[:, 0] * scene.V + pairs[:, 1]))
+    nnz = 6 * scene.V + 9 * E
+    return 24 * scene.V + scene.T * (72 + 8 + 16) + 24 * scene.V + 8 * nnz, nnz
+
+
+def cpu_baseline(scene, xt, p, kappa, budget_s=15.0, threads=16):
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as O
+    threads = max(1, min(threads, os.cpu_count() or 1))
+    m = O.OracleMesh(scene, xtilde=xt, threads=threads)
+    t0 = time.perf_counter()
+    n = 0
+    while True:
+        m.active_set()
+        m.assemble(kappa)
+        m.step_bound(p)
+        m.energy(kappa)
+        n += 1
+        if time.perf_counter() - t0 > budget_s or n >= 20:
+            break
+    ms = (time.perf_counter() - t0) * 1e3 / n
+    m.close()
+    return {"value": round(ms, 3), "unit": "ms/iter", "cores": threads, "kind": "port",
+            "sample": f"{n} Newton iterations of the same workload (active set + assembly + step bound + energy) "
+                      f"on the oracle (CPU restatement of the reference path), OpenMP"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=50)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--cpu-budget", type=float, default=15.0)
+    ap.add_argument("--traffic", default=None, help="JSON file with measured HBM bytes per launch of the elastic stage")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU: --gpus {args.gpus} is not supported (every run uses a single MI355X)",
+              file=sys.stderr)
+        sys.exit(2)
+
+    import numpy as np
+    from cipc import gpu, scenes
+
+    scene = scenes.config1(velocity_scale=0.5)
+    xt, p = scenes.newton_probe_state(scene)
+    kappa = scenes.suggest_kappa(scene.hhat, scene.bbox_diag2, scene.mean_mass)
+    ctx = gpu.GpuContext(scene, xtilde=xt)
+    ctx.set_direction(p)
+    for _ in range(args.warmup):
+        ctx.newton_iteration(kappa)
+    stages = []
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        stages.append(ctx.newton_iteration(kappa))
+    wall_ms = (time.perf_counter() - t0) * 1e3 / args.steps
+    mean = {k: float(np.mean([s[k] for s in stages])) for k in
+            ("broadphase_ms", "assembly_ms", "elastic_ms", "ccd_ms", "energy_ms", "total_ms")}
+    last = stages[-1]
+    ctx.close()
+
+    bytes_el, nnz_el = elastic_algorithmic_bytes(scene)
+    achieved = bytes_el / (mean["elastic_ms"] * 1e-3) / 1e9
+    traffic = None
+    if args.traffic and os.path.exists(args.traffic):
+        traffic = json.load(open(args.traffic)).get("bytes_per_launch")
+    out = {
+        "metric": METRIC,
+        "value": round(wall_ms, 4),
+        "unit": "ms/iter",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(wall_ms, 4),
+        "higher_is_better": False,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": "4x20^3 lattice cubes, 37044 verts / 160000 tets, stacked in contact, frozen Newton state",
+                   "V": scene.V, "T": scene.T, "nnz": int(last["nnz"]), "contacts": int(last["contacts"])},
+        "stages_ms": {k: round(v, 4) for k, v in mean.items()},
+        "roofline": {"kernel": "elastic assembly (snh_element_kernel + block/vertex gathers)", "bound": "hbm",
+                     "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
+                     "algorithmic_bytes": int(bytes_el), "elastic_nnz": int(nnz_el)},
+    }
+    if not args.no_cpu:
+        out["cpu_baseline"] = cpu_baseline(scene, xt, p, kappa, budget_s=args.cpu_budget)
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

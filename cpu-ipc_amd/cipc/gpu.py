@@ -1,0 +1,319 @@
+"""Python binding of the device library's C ABI (include/cipc_gpu.h).
+
+This mirrors the call sequence the reference drives per Newton iteration
+(``CPU IPC/IPCSolver.cpp:890-1064``): ``active_set`` replaces
+``SpatialHash::build`` + ``calculateActivateSet`` + ``Ground::calculateActivateSet``,
+``assemble`` replaces ``computeGradientAndHessian`` + ``NewtonLinearSystem::assemble``,
+``step_bound`` the ground/CCD/CFL block, ``energy`` ``computeEnergy``.  Errors raise the
+Python counterpart of the exception the reference throws at the same place.
+
+There is no fallback: if ``libcipc_gpu.so`` is missing or no gfx950 device is present the
+constructor raises.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(os.path.dirname(_HERE), "libcipc_gpu.so")
+
+_dp = C.POINTER(C.c_double)
+_ip = C.POINTER(C.c_int)
+_i64p = C.POINTER(C.c_int64)
+_u64p = C.POINTER(C.c_uint64)
+
+OK, ERR_INVALID_ARGUMENT, ERR_RUNTIME, ERR_LOGIC, ERR_OVERFLOW, ERR_HIP, ERR_NO_DEVICE = range(7)
+
+#: every symbol include/cipc_gpu.h declares
+EXPORTS = (
+    "cipc_gpu_last_error", "cipc_gpu_context_create", "cipc_gpu_context_destroy", "cipc_gpu_upload_scene",
+    "cipc_gpu_set_state", "cipc_gpu_get_positions", "cipc_gpu_set_direction", "cipc_gpu_active_set",
+    "cipc_gpu_get_active_set", "cipc_gpu_set_active_set", "cipc_gpu_assemble", "cipc_gpu_get_gradient",
+    "cipc_gpu_get_csr", "cipc_gpu_energy", "cipc_gpu_energy_terms", "cipc_gpu_step_bound",
+    "cipc_gpu_save_positions", "cipc_gpu_step_forward", "cipc_gpu_is_intersected", "cipc_gpu_newton_iteration",
+    "cipc_gpu_eval_snh", "cipc_gpu_eval_distance", "cipc_gpu_eval_make_pd", "cipc_gpu_eval_accd",
+    "cipc_gpu_eval_classify",
+)
+
+
+class SceneDesc(C.Structure):
+    _fields_ = [("V", C.c_int), ("T", C.c_int), ("F", C.c_int), ("Vs", C.c_int), ("Es", C.c_int),
+                ("x", _dp), ("v_rest", _dp), ("V_prev", _dp), ("xtilde", _dp), ("masses", _dp),
+                ("boundary", _ip), ("tets", _ip), ("dm_inv", _dp), ("volumes", _dp), ("faces", _ip),
+                ("surf_verts", _u64p), ("surf_edges", _u64p),
+                ("dt", C.c_double), ("length_rate", C.c_double), ("volume_rate", C.c_double),
+                ("drag", C.c_double), ("hhat", C.c_double), ("fhat", C.c_double), ("friction", C.c_double),
+                ("ground_normal", C.c_double * 3), ("ground_d", C.c_double)]
+
+
+class IterationTiming(C.Structure):
+    _fields_ = [("broadphase_ms", C.c_double), ("assembly_ms", C.c_double), ("elastic_ms", C.c_double),
+                ("ccd_ms", C.c_double),
+                ("energy_ms", C.c_double), ("total_ms", C.c_double), ("alpha", C.c_double),
+                ("energy", C.c_double), ("nnz", C.c_int64), ("contacts", C.c_int64)]
+
+
+class CipcError(RuntimeError):
+    """HIP or device failure (CIPC_ERR_HIP / CIPC_ERR_NO_DEVICE)."""
+
+
+class CipcLogicError(RuntimeError):
+    """std::logic_error in the reference."""
+
+
+_lib = None
+
+
+def load(path: str = LIB_PATH):
+    """Load the device library; raises if it has not been built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(path):
+        raise FileNotFoundError(f"{path} is missing: build it with __graft_entry__.build()")
+    L = C.CDLL(path)
+    L.cipc_gpu_last_error.restype = C.c_char_p
+    sig = {
+        "cipc_gpu_context_create": [C.c_int, C.POINTER(C.c_void_p)],
+        "cipc_gpu_context_destroy": [C.c_void_p],
+        "cipc_gpu_upload_scene": [C.c_void_p, C.POINTER(SceneDesc)],
+        "cipc_gpu_set_state": [C.c_void_p, _dp, _dp, _dp],
+        "cipc_gpu_get_positions": [C.c_void_p, _dp],
+        "cipc_gpu_set_direction": [C.c_void_p, _dp],
+        "cipc_gpu_active_set": [C.c_void_p, _i64p],
+        "cipc_gpu_get_active_set": [C.c_void_p, _ip, _ip, _ip, _ip, _ip],
+        "cipc_gpu_set_active_set": [C.c_void_p, _i64p, _ip, _ip, _ip, _ip, _ip],
+        "cipc_gpu_assemble": [C.c_void_p, C.c_double, _i64p, _ip],
+        "cipc_gpu_get_gradient": [C.c_void_p, _dp],
+        "cipc_gpu_get_csr": [C.c_void_p, _ip, _ip, _dp, _dp],
+        "cipc_gpu_energy": [C.c_void_p, C.c_double, _dp],
+        "cipc_gpu_energy_terms": [C.c_void_p, C.c_double, _dp],
+        "cipc_gpu_step_bound": [C.c_void_p, _dp],
+        "cipc_gpu_save_positions": [C.c_void_p],
+        "cipc_gpu_step_forward": [C.c_void_p, C.c_double],
+        "cipc_gpu_is_intersected": [C.c_void_p, _ip],
+        "cipc_gpu_newton_iteration": [C.c_void_p, C.c_double, C.POINTER(IterationTiming)],
+        "cipc_gpu_eval_snh": [C.c_int, _dp, C.c_double, C.c_double, _dp, _dp, _dp],
+        "cipc_gpu_eval_distance": [C.c_int, C.c_int, _dp, _dp, _dp, _dp],
+        "cipc_gpu_eval_make_pd": [C.c_int, C.c_int, _dp],
+        "cipc_gpu_eval_accd": [C.c_int, C.c_int, _dp, _dp, C.c_double, C.c_double, _dp],
+        "cipc_gpu_eval_classify": [C.c_int, C.c_int, _dp, _ip],
+    }
+    for name, args in sig.items():
+        fn = getattr(L, name)
+        fn.restype = C.c_int
+        fn.argtypes = args
+    _lib = L
+    return L
+
+
+def _check(code: int):
+    if code == OK:
+        return
+    msg = load().cipc_gpu_last_error().decode(errors="replace")
+    if code == ERR_INVALID_ARGUMENT:
+        raise ValueError(msg)
+    if code == ERR_OVERFLOW:
+        raise OverflowError(msg)
+    if code == ERR_LOGIC:
+        raise CipcLogicError(msg)
+    if code == ERR_RUNTIME:
+        raise RuntimeError(msg)
+    raise CipcError(msg)
+
+
+def _d(a):
+    return np.ascontiguousarray(a, dtype=np.float64)
+
+
+def _i(a):
+    return np.ascontiguousarray(a, dtype=np.int32)
+
+
+def _dptr(a):
+    return a.ctypes.data_as(_dp)
+
+
+def _iptr(a):
+    return a.ctypes.data_as(_ip)
+
+
+def scene_desc(scene, xtilde=None):
+    """Descriptor of a host scene; returns (desc, arrays kept alive)."""
+    xt = scene.inertial_target() if xtilde is None else xtilde
+    keep = [_d(scene.x), _d(scene.v_rest), _d(scene.V_prev), _d(xt), _d(scene.masses), _i(scene.boundary),
+            _i(scene.tets), _d(scene.dm_inv), _d(scene.volumes), _i(scene.faces),
+            np.ascontiguousarray(scene.surf_verts, dtype=np.uint64),
+            np.ascontiguousarray(scene.surf_edges, dtype=np.uint64)]
+    d = SceneDesc()
+    d.V, d.T, d.F = scene.V, scene.T, len(scene.faces)
+    d.Vs, d.Es = len(scene.surf_verts), len(scene.surf_edges)
+    d.x, d.v_rest, d.V_prev, d.xtilde, d.masses = (_dptr(keep[0]), _dptr(keep[1]), _dptr(keep[2]),
+                                                   _dptr(keep[3]), _dptr(keep[4]))
+    d.boundary, d.tets, d.dm_inv, d.volumes, d.faces = (_iptr(keep[5]), _iptr(keep[6]), _dptr(keep[7]),
+                                                        _dptr(keep[8]), _iptr(keep[9]))
+    d.surf_verts, d.surf_edges = keep[10].ctypes.data_as(_u64p), keep[11].ctypes.data_as(_u64p)
+    m = scene.material
+    d.dt, d.length_rate, d.volume_rate, d.drag = m.dt, m.length_rate, m.volume_rate, m.drag
+    d.hhat, d.fhat, d.friction = scene.hhat, scene.fhat, m.friction
+    d.ground_normal[0], d.ground_normal[1], d.ground_normal[2] = 0.0, 1.0, 0.0
+    d.ground_d = -1.0
+    return d, keep
+
+
+class GpuContext:
+    """One scene resident on one MI355X (one context, one stream)."""
+
+    def __init__(self, scene, xtilde=None, device: int = 0):
+        L = load()
+        h = C.c_void_p()
+        _check(L.cipc_gpu_context_create(device, C.byref(h)))
+        self.h = h
+        self.scene = scene
+        self.V = scene.V
+        desc, keep = scene_desc(scene, xtilde)
+        _check(L.cipc_gpu_upload_scene(self.h, C.byref(desc)))
+
+    def close(self):
+        if getattr(self, "h", None):
+            load().cipc_gpu_context_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    # state
+    def set_state(self, x=None, V_prev=None, xtilde=None):
+        a = [None if v is None else _d(v) for v in (x, V_prev, xtilde)]
+        _check(load().cipc_gpu_set_state(self.h, *[None if v is None else _dptr(v) for v in a]))
+
+    def positions(self):
+        x = np.zeros((self.V, 3))
+        _check(load().cipc_gpu_get_positions(self.h, _dptr(x)))
+        return x
+
+    def set_direction(self, p):
+        p = _d(p)
+        _check(load().cipc_gpu_set_direction(self.h, _dptr(p)))
+
+    # broad phase + active set
+    def active_set(self):
+        c = np.zeros(4, np.int64)
+        _check(load().cipc_gpu_active_set(self.h, c.ctypes.data_as(_i64p)))
+        s4 = np.zeros((max(c[0], 1), 4), np.int32)
+        e4 = np.zeros((max(c[1], 1), 4), np.int32)
+        eie = np.zeros((max(c[1], 1), 2), np.int32)
+        ccd = np.zeros((max(c[2], 1), 2), np.int32)
+        env = np.zeros(max(c[3], 1), np.int32)
+        _check(load().cipc_gpu_get_active_set(self.h, _iptr(s4), _iptr(e4), _iptr(eie), _iptr(ccd), _iptr(env)))
+        return dict(self_set=s4[:c[0]], ee_set=e4[:c[1]], ee_ie=eie[:c[1]], ccd_set=ccd[:c[2]], env_set=env[:c[3]])
+
+    def set_active_set(self, a):
+        c = np.array([len(a["self_set"]), len(a["ee_set"]), len(a["ccd_set"]), len(a["env_set"])], np.int64)
+        arr = [_i(a[k]).reshape(-1) if len(a[k]) else np.zeros(4, np.int32)
+               for k in ("self_set", "ee_set", "ee_ie", "ccd_set", "env_set")]
+        _check(load().cipc_gpu_set_active_set(self.h, c.ctypes.data_as(_i64p), *[_iptr(x) for x in arr]))
+
+    # assembly
+    def assemble(self, kappa):
+        nnz = np.zeros(1, np.int64)
+        col = np.zeros(1, np.int32)
+        _check(load().cipc_gpu_assemble(self.h, kappa, nnz.ctypes.data_as(_i64p), _iptr(col)))
+        n = 3 * self.V
+        g = np.zeros(n)
+        _check(load().cipc_gpu_get_gradient(self.h, _dptr(g)))
+        outer = np.zeros(n + 1, np.int32)
+        inner = np.zeros(max(int(nnz[0]), 1), np.int32)
+        val = np.zeros(max(int(nnz[0]), 1))
+        rhs = np.zeros(n)
+        _check(load().cipc_gpu_get_csr(self.h, _iptr(outer), _iptr(inner), _dptr(val), _dptr(rhs)))
+        k = int(nnz[0])
+        return dict(grad=g.reshape(-1, 3), outer=outer, inner=inner[:k], val=val[:k], rhs=rhs, collisions=int(col[0]))
+
+    def energy(self, kappa):
+        e = np.zeros(1)
+        _check(load().cipc_gpu_energy(self.h, kappa, _dptr(e)))
+        return float(e[0])
+
+    def energy_terms(self, kappa):
+        t = np.zeros(5)
+        _check(load().cipc_gpu_energy_terms(self.h, kappa, _dptr(t)))
+        return t
+
+    def step_bound(self, p=None):
+        if p is not None:
+            self.set_direction(p)
+        out = np.zeros(5)
+        _check(load().cipc_gpu_step_bound(self.h, _dptr(out)))
+        return dict(alpha=out[0], ground=out[1], partial=out[2], cfl=out[3], full=out[4])
+
+    def save_positions(self):
+        _check(load().cipc_gpu_save_positions(self.h))
+
+    def step_forward(self, alpha):
+        _check(load().cipc_gpu_step_forward(self.h, alpha))
+
+    def is_intersected(self):
+        r = np.zeros(1, np.int32)
+        _check(load().cipc_gpu_is_intersected(self.h, _iptr(r)))
+        return bool(r[0])
+
+    def newton_iteration(self, kappa):
+        t = IterationTiming()
+        _check(load().cipc_gpu_newton_iteration(self.h, kappa, C.byref(t)))
+        return {f: getattr(t, f) for f, _ in IterationTiming._fields_}
+
+
+# ---- element-level evaluation -----------------------------------------------------------
+def eval_snh(F, u, r):
+    """F: (n,3,3) -> PK1 (n,3,3), projected Hessian (n,9,9) over column-major vec F, psi (n,)."""
+    F = np.asarray(F, np.float64)
+    n = F.shape[0]
+    Fc = _d(F.transpose(0, 2, 1).reshape(n, 9))
+    P, H, e = np.zeros((n, 9)), np.zeros((n, 81)), np.zeros(n)
+    _check(load().cipc_gpu_eval_snh(n, _dptr(Fc), u, r, _dptr(P), _dptr(H), _dptr(e)))
+    return P.reshape(n, 3, 3).transpose(0, 2, 1), H.reshape(n, 9, 9), e
+
+
+def eval_distance(kind, pts):
+    pts = _d(np.asarray(pts).reshape(-1, 12))
+    n = pts.shape[0]
+    d, g, H = np.zeros(n), np.zeros((n, 12)), np.zeros((n, 144))
+    _check(load().cipc_gpu_eval_distance(n, kind, _dptr(pts), _dptr(d), _dptr(g), _dptr(H)))
+    N = {0: 6, 1: 9}.get(kind, 12)
+    return d, g[:, :N].copy(), H[:, :N * N].reshape(n, N, N).copy()
+
+
+def eval_make_pd(H):
+    H = _d(H).copy()
+    n, dim = H.shape[0], H.shape[1]
+    _check(load().cipc_gpu_eval_make_pd(n, dim, _dptr(H)))
+    return H
+
+
+def eval_accd(kind, pts, dirs, eta, thickness=0.0):
+    pts, dirs = _d(np.asarray(pts).reshape(-1, 12)), _d(np.asarray(dirs).reshape(-1, 12))
+    n = pts.shape[0]
+    out = np.zeros(n)
+    _check(load().cipc_gpu_eval_accd(n, kind, _dptr(pts), _dptr(dirs), eta, thickness, _dptr(out)))
+    return out
+
+
+def eval_classify(kind, pts):
+    pts = _d(np.asarray(pts).reshape(-1, 12))
+    n = pts.shape[0]
+    out = np.zeros(n, np.int32)
+    _check(load().cipc_gpu_eval_classify(n, kind, _dptr(pts), _iptr(out)))
+    return out

@@ -1,0 +1,673 @@
+// Broad phase and active set on the device.  Compiled with -ffp-contract=off.
+//
+// Reference: CPULinearBVH::build / query and CPUBroadPhaseBVH::build / buildSwept
+// (CPU IPC/LBVH.cpp:163-339), SpatialHash::calculateActivateSet
+// (CollisionBroadPhase.cpp:765-1059), Ground::calculateActivateSet (:1076-1092).
+//
+// Tree build: scene box reduction -> 30-bit Morton codes of box centres normalised to the
+// scene box (key = code << 32 | primitive, unique) -> radix sort -> Karras topology, one
+// thread per internal node -> bottom-up boxes with per-node arrival counters -> skip
+// pointers for stackless depth-first traversal.
+//
+// Active set: one thread per surface vertex (point-triangle) and per surface edge
+// (edge-edge) traverses the tree twice (count, then fill), sorts its candidates by
+// primitive id as the reference's query does, and classifies them.  Compaction
+// reproduces the reference's serial order: partial-CCD pairs, then Self_ActiveSet =
+// point-triangle entries, edge-edge entries, and the point-point/point-edge entries
+// deduplicated in std::map key order with multiplicity; Self_EE_ActiveSet for mollified
+// entries; Environment_ActiveSet in ascending surface order.
+#include "context.hpp"
+#include "contact.hpp"
+#include "lbvh.hpp"
+
+#include <hipcub/hipcub.hpp>
+
+namespace cipc {
+
+// ---------------------------------------------------------------------------------
+// LBVH build
+// ---------------------------------------------------------------------------------
+__device__ inline unsigned expand_bits(unsigned v) {       // LBVH.cpp:17-24
+    v = (v * 0x00010001u) & 0xFF0000FFu;
+    v = (v * 0x00000101u) & 0x0F00F00Fu;
+    v = (v * 0x00000011u) & 0xC30C30C3u;
+    v = (v * 0x00000005u) & 0x49249249u;
+    return v;
+}
+__device__ inline unsigned morton3(double x, double y, double z) {  // LBVH.cpp:26-37
+    const double res = 1024.0;
+    auto q = [&](double v) { v = fmax(0.0, fmin(v * res, res - 1.0)); return (unsigned)v; };
+    return (expand_bits(q(x)) << 2) | (expand_bits(q(y)) << 1) | expand_bits(q(z));
+}
+
+constexpr int kRedBlock = 256;
+constexpr int kRedParts = 256;
+__global__ __launch_bounds__(kRedBlock) void box_reduce_partial(int n, const Box* __restrict__ b, Box* __restrict__ part) {
+    __shared__ Box sh[kRedBlock];
+    Box acc{{1.7976931348623157e308, 1.7976931348623157e308, 1.7976931348623157e308},
+            {-1.7976931348623157e308, -1.7976931348623157e308, -1.7976931348623157e308}};
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
+        acc = Box{vmin(acc.lo, b[i].lo), vmax(acc.hi, b[i].hi)};
+    sh[threadIdx.x] = acc;
+    __syncthreads();
+    for (int s = blockDim.x / 2; s > 0; s >>= 1) {
+        if ((int)threadIdx.x < s)
+            sh[threadIdx.x] = Box{vmin(sh[threadIdx.x].lo, sh[threadIdx.x + s].lo), vmax(sh[threadIdx.x].hi, sh[threadIdx.x + s].hi)};
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) part[blockIdx.x] = sh[0];
+}
+__global__ void morton_keys(int n, const Box* __restrict__ b, const Box* __restrict__ part, int nparts,
+                            unsigned long long* __restrict__ keys) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    Box sc = part[0];
+    for (int k = 1; k < nparts; ++k) sc = Box{vmin(sc.lo, part[k].lo), vmax(sc.hi, part[k].hi)};
+    const d3 ext = sc.hi - sc.lo;
+    const d3 c = 0.5 * (b[i].lo + b[i].hi);
+    const d3 off = c - sc.lo;
+    const double x = ext.x > 0.0 ? off.x / ext.x : 0.0;
+    const double y = ext.y > 0.0 ? off.y / ext.y : 0.0;
+    const double z = ext.z > 0.0 ? off.z / ext.z : 0.0;
+    keys[i] = ((unsigned long long)morton3(x, y, z) << 32) | (unsigned)i;
+}
+__device__ inline int prefix(const unsigned long long* k, int n, int a, int b) {
+    if (b < 0 || b >= n) return -1;
+    return __clzll(k[a] ^ k[b]);
+}
+// Karras 2012 topology (LBVH.cpp:62-119,213-228): internal node i covers the key range
+// [first, last] and splits it at the highest differing bit.
+__global__ void karras_kernel(int n, const unsigned long long* __restrict__ k, int* __restrict__ left, int* __restrict__ right,
+                              int* __restrict__ parent, int* __restrict__ range_last, int* __restrict__ split_right) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n - 1) return;
+    int first, last;
+    if (i == 0) { first = 0; last = n - 1; }
+    else {
+        const int lp = prefix(k, n, i, i - 1), rp = prefix(k, n, i, i + 1);
+        const int dir = rp > lp ? 1 : -1;
+        const int minp = lp < rp ? lp : rp;
+        int lmax = 2;
+        while (prefix(k, n, i, i + dir * lmax) > minp) lmax <<= 1;
+        int len = 0;
+        for (int st = lmax >> 1; st > 0; st >>= 1)
+            if (prefix(k, n, i, i + dir * (len + st)) > minp) len += st;
+        const int other = i + dir * len;
+        first = i < other ? i : other;
+        last = i < other ? other : i;
+    }
+    int split;
+    const unsigned long long fc = k[first], lc = k[last];
+    if (fc == lc) split = (first + last) >> 1;
+    else {
+        const int common = __clzll(fc ^ lc);
+        int s = first, step = last - first;
+        do {
+            step = (step + 1) >> 1;
+            const int cand = s + step;
+            if (cand < last && __clzll(fc ^ k[cand]) > common) s = cand;
+        } while (step > 1);
+        split = s;
+    }
+    const int l = (split == first) ? split + n - 1 : split;
+    const int r = (split + 1 == last) ? split + 1 + n - 1 : split + 1;
+    left[i] = l;
+    right[i] = r;
+    parent[l] = i;
+    parent[r] = i;
+    range_last[i] = last;
+    split_right[split] = r;
+}
+__global__ void leaf_init(int n, const unsigned long long* __restrict__ k, const Box* __restrict__ prim, Box* __restrict__ nb,
+                          int* __restrict__ left, int* __restrict__ flag, int* __restrict__ parent) {
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= n) return;
+    const int p = (int)(k[s] & 0xffffffffull);
+    nb[n - 1 + s] = prim[p];
+    left[n - 1 + s] = p;
+    if (s < n - 1) flag[s] = 0;
+    if (s == 0) parent[0] = -1;
+}
+__global__ void skip_kernel(int n, const int* __restrict__ range_last, const int* __restrict__ split_right, int* __restrict__ skip) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= 2 * n - 1) return;
+    const int last = (i < n - 1) ? range_last[i] : i - (n - 1);
+    skip[i] = (last >= n - 1) ? -1 : split_right[last];
+}
+// Bottom-up union of child boxes; the second child to arrive at a node computes it.  Child
+// boxes written by other work-groups are read with L1-bypassing (volatile) loads after
+// the arrival counter, which is ordered after the writer's fence.
+__device__ inline Box vload(const Box* p) {
+    const volatile double* q = reinterpret_cast<const volatile double*>(p);
+    return Box{{q[0], q[1], q[2]}, {q[3], q[4], q[5]}};
+}
+__global__ void bottom_up_boxes(int n, const int* __restrict__ parent, const int* __restrict__ left,
+                                const int* __restrict__ right, Box* nb, int* flag) {
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= n || n < 2) return;
+    int node = parent[n - 1 + s];
+    for (int depth = 0; depth < n && node >= 0; ++depth) {
+        __threadfence();
+        if (atomicAdd(&flag[node], 1) == 0) return;
+        __threadfence();
+        const Box a = vload(nb + left[node]), b = vload(nb + right[node]);
+        nb[node] = Box{vmin(a.lo, b.lo), vmax(a.hi, b.hi)};
+        node = node == 0 ? -1 : parent[node];
+    }
+}
+
+void build_tree(Context& c, LBVH& t, int n) {
+    t.n = n;
+    if (n <= 0) return;
+    const int nodes = 2 * n - 1;
+    t.node_box.resize(nodes); t.left.resize(nodes); t.skip.resize(nodes); t.parent.resize(nodes);
+    t.right.resize(std::max(n - 1, 1)); t.range_last.resize(std::max(n - 1, 1));
+    t.split_right.resize(std::max(n - 1, 1)); t.flag.resize(std::max(n - 1, 1));
+    t.keys.resize(n); t.keys_sorted.resize(n);
+    const int parts = std::min<int>(kRedParts, (int)blocks_for(n, kRedBlock));
+    t.scene_partial.resize(parts);
+    hipLaunchKernelGGL(box_reduce_partial, dim3(parts), dim3(kRedBlock), 0, c.stream, n, t.prim.p, t.scene_partial.p);
+    CIPC_CHECK_LAUNCH();
+    hipLaunchKernelGGL(morton_keys, dim3(blocks_for(n, 256)), dim3(256), 0, c.stream, n, t.prim.p, t.scene_partial.p, parts, t.keys.p);
+    CIPC_CHECK_LAUNCH();
+    sort_u64(c, t.keys.p, t.keys_sorted.p, n, 64);
+    if (n > 1) {
+        hipLaunchKernelGGL(karras_kernel, dim3(blocks_for(n - 1, 256)), dim3(256), 0, c.stream, n, t.keys_sorted.p, t.left.p,
+                           t.right.p, t.parent.p, t.range_last.p, t.split_right.p);
+        CIPC_CHECK_LAUNCH();
+    }
+    hipLaunchKernelGGL(leaf_init, dim3(blocks_for(n, 256)), dim3(256), 0, c.stream, n, t.keys_sorted.p, t.prim.p, t.node_box.p,
+                       t.left.p, t.flag.p, t.parent.p);
+    CIPC_CHECK_LAUNCH();
+    hipLaunchKernelGGL(skip_kernel, dim3(blocks_for(nodes, 256)), dim3(256), 0, c.stream, n, t.range_last.p, t.split_right.p, t.skip.p);
+    CIPC_CHECK_LAUNCH();
+    if (n > 1) {
+        hipLaunchKernelGGL(bottom_up_boxes, dim3(blocks_for(n, 256)), dim3(256), 0, c.stream, n, t.parent.p, t.left.p, t.right.p,
+                           t.node_box.p, t.flag.p);
+        CIPC_CHECK_LAUNCH();
+    }
+}
+
+// Primitive boxes: static (CPUBroadPhaseBVH::build, LBVH.cpp:284-305) or swept with step
+// alpha along -p (buildSwept, LBVH.cpp:307-339).
+__global__ void face_boxes(int F, const int4* __restrict__ faces, const double* __restrict__ x, const double* __restrict__ p,
+                           double a, int swept, Box* __restrict__ out) {
+    const int f = blockIdx.x * blockDim.x + threadIdx.x;
+    if (f >= F) return;
+    const int4 v = faces[f];
+    if (!swept) { out[f] = box_tri(ld3(x, v.x), ld3(x, v.y), ld3(x, v.z)); return; }
+    Box b = swept_pt(ld3(x, v.x), ld3(p, v.x), a);
+    b = box_union(b, swept_pt(ld3(x, v.y), ld3(p, v.y), a));
+    b = box_union(b, swept_pt(ld3(x, v.z), ld3(p, v.z), a));
+    out[f] = b;
+}
+__global__ void edge_boxes(int E, const int2* __restrict__ se, const double* __restrict__ x, const double* __restrict__ p,
+                           double a, int swept, Box* __restrict__ out) {
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= E) return;
+    const int2 v = se[e];
+    if (!swept) { out[e] = box_edge(ld3(x, v.x), ld3(x, v.y)); return; }
+    out[e] = box_union(swept_pt(ld3(x, v.x), ld3(p, v.x), a), swept_pt(ld3(x, v.y), ld3(p, v.y), a));
+}
+__global__ void point_boxes(int Vs, const int* __restrict__ sv, const double* __restrict__ x, const double* __restrict__ p,
+                            double a, Box* __restrict__ out) {
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= Vs) return;
+    out[s] = swept_pt(ld3(x, sv[s]), ld3(p, sv[s]), a);
+}
+
+void build_static_trees(Context& c) {
+    Scene& s = c.sc;
+    c.face_tree.prim.resize(std::max(s.F, 1));
+    c.edge_tree.prim.resize(std::max(s.Es, 1));
+    if (s.F > 0)
+        hipLaunchKernelGGL(face_boxes, dim3(blocks_for(s.F, 256)), dim3(256), 0, c.stream, s.F, s.faces.p, s.x.p, s.x.p, 0.0, 0,
+                           c.face_tree.prim.p);
+    if (s.Es > 0)
+        hipLaunchKernelGGL(edge_boxes, dim3(blocks_for(s.Es, 256)), dim3(256), 0, c.stream, s.Es, s.se.p, s.x.p, s.x.p, 0.0, 0,
+                           c.edge_tree.prim.p);
+    CIPC_CHECK_LAUNCH();
+    build_tree(c, c.face_tree, s.F);
+    build_tree(c, c.edge_tree, s.Es);
+    c.face_tree.swept = c.edge_tree.swept = false;
+}
+void build_swept_trees(Context& c, double alpha) {
+    Scene& s = c.sc;
+    c.face_tree.prim.resize(std::max(s.F, 1));
+    c.edge_tree.prim.resize(std::max(s.Es, 1));
+    c.swept_point_box.resize(std::max(s.Vs, 1));
+    if (s.F > 0)
+        hipLaunchKernelGGL(face_boxes, dim3(blocks_for(s.F, 256)), dim3(256), 0, c.stream, s.F, s.faces.p, s.x.p, s.dir.p, alpha, 1,
+                           c.face_tree.prim.p);
+    if (s.Es > 0)
+        hipLaunchKernelGGL(edge_boxes, dim3(blocks_for(s.Es, 256)), dim3(256), 0, c.stream, s.Es, s.se.p, s.x.p, s.dir.p, alpha, 1,
+                           c.edge_tree.prim.p);
+    if (s.Vs > 0)
+        hipLaunchKernelGGL(point_boxes, dim3(blocks_for(s.Vs, 256)), dim3(256), 0, c.stream, s.Vs, s.sv.p, s.x.p, s.dir.p, alpha,
+                           c.swept_point_box.p);
+    CIPC_CHECK_LAUNCH();
+    build_tree(c, c.face_tree, s.F);
+    build_tree(c, c.edge_tree, s.Es);
+    c.face_tree.swept = c.edge_tree.swept = true;
+    c.face_tree.alpha = c.edge_tree.alpha = alpha;
+}
+
+// ---------------------------------------------------------------------------------
+// Active set (CollisionBroadPhase.cpp:765-1059)
+// ---------------------------------------------------------------------------------
+// candidate categories
+enum : int { kNone = 0, kSelfPT = 1, kMap = 2, kSelfEE = 3, kMolPE = 4, kMolEE = 5 };
+
+struct TreeView {
+    int n;
+    const Box* nb;
+    const int* left;
+    const int* skip;
+};
+__device__ inline bool pt_keep(int vI, int4 fv, const int* bt, const double* x, const Box& pb, double sq) {
+    if (is_collider(bt[vI]) && is_collider(bt[fv.x]) && is_collider(bt[fv.y]) && is_collider(bt[fv.z])) return false;
+    if (vI == fv.x || vI == fv.y || vI == fv.z) return false;
+    return overlaps(pb, box_tri(ld3(x, fv.x), ld3(x, fv.y), ld3(x, fv.z)), sq);
+}
+__device__ inline bool ee_keep(int eI, int eJ, int2 A, int2 B, const int* bt, const double* x, const Box& b1, double sq) {
+    if (eJ <= eI) return false;   // queryEdgeForEdgesWithBBoxCheck drops candidates <= eI
+    if (is_collider(bt[A.x]) && is_collider(bt[A.y]) && is_collider(bt[B.x]) && is_collider(bt[B.y])) return false;
+    if (A.x == B.x || A.x == B.y || A.y == B.x || A.y == B.y || eI > eJ) return false;
+    return overlaps(b1, box_edge(ld3(x, B.x), ld3(x, B.y)), sq);
+}
+__device__ inline void sort_segment(int* a, int n) {   // insertion sort of a short id list
+    for (int i = 1; i < n; ++i) {
+        const int v = a[i];
+        int j = i - 1;
+        while (j >= 0 && a[j] > v) { a[j + 1] = a[j]; --j; }
+        a[j + 1] = v;
+    }
+}
+
+__global__ void pt_pass(int Vs, const int* __restrict__ sv, const double* __restrict__ x, const int* __restrict__ bt,
+                        const int4* __restrict__ faces, TreeView tv, double sq, double hhat, const int* __restrict__ off,
+                        int* __restrict__ cnt, int* __restrict__ cand, int4* __restrict__ rec, int* __restrict__ cat) {
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= Vs) return;
+    const int vI = sv[s];
+    const d3 P = ld3(x, vI);
+    const Box pb = box_pt(P);
+    if (off == nullptr) {
+        int n = 0;
+        lbvh_visit(tv.n, tv.nb, tv.left, tv.skip, pb, sq, [&](int f) { if (pt_keep(vI, faces[f], bt, x, pb, sq)) ++n; });
+        cnt[s] = n;
+        return;
+    }
+    const int o = off[s], cap = off[s + 1] - o;
+    int n = 0;
+    lbvh_visit(tv.n, tv.nb, tv.left, tv.skip, pb, sq, [&](int f) {
+        if (pt_keep(vI, faces[f], bt, x, pb, sq)) { if (n < cap) cand[o + n] = f; ++n; }
+    });
+    n = n < cap ? n : cap;
+    sort_segment(cand + o, n);
+    for (int k = 0; k < n; ++k) {
+        const int f = cand[o + k];
+        const int4 fv = faces[f];
+        const d3 A = ld3(x, fv.x), B = ld3(x, fv.y), C = ld3(x, fv.z);
+        const int dt = dtype_pt(P, A, B, C);
+        int4 r = make_int4(0, 0, 0, 0);
+        int ct = kNone;
+        double d;
+        switch (dt) {
+        case 0: d = d_PP(P, A); if (d < hhat) { r = make_int4(-vI - 1, fv.x, -1, -1); ct = kMap; } break;
+        case 1: d = d_PP(P, B); if (d < hhat) { r = make_int4(-vI - 1, fv.y, -1, -1); ct = kMap; } break;
+        case 2: d = d_PP(P, C); if (d < hhat) { r = make_int4(-vI - 1, fv.z, -1, -1); ct = kMap; } break;
+        case 3: d = d_PE(P, A, B); if (d < hhat) { r = make_int4(-vI - 1, fv.x, fv.y, -1); ct = kMap; } break;
+        case 4: d = d_PE(P, B, C); if (d < hhat) { r = make_int4(-vI - 1, fv.y, fv.z, -1); ct = kMap; } break;
+        case 5: d = d_PE(P, C, A); if (d < hhat) { r = make_int4(-vI - 1, fv.z, fv.x, -1); ct = kMap; } break;
+        case 6: d = d_PT(P, A, B, C); if (d < hhat) { r = make_int4(-vI - 1, fv.x, fv.y, fv.z); ct = kSelfPT; } break;
+        default: break;
+        }
+        rec[o + k] = r;
+        cat[o + k] = ct;
+    }
+}
+
+__global__ void ee_pass(int Es, const int2* __restrict__ se, const double* __restrict__ x, const double* __restrict__ vrest,
+                        const int* __restrict__ bt, TreeView tv, double sq, double hhat, const int* __restrict__ off,
+                        int* __restrict__ cnt, int* __restrict__ cand, int4* __restrict__ rec, int2* __restrict__ aux,
+                        int* __restrict__ cat) {
+    const int eI = blockIdx.x * blockDim.x + threadIdx.x;
+    if (eI >= Es) return;
+    const int2 A = se[eI];
+    const d3 a0 = ld3(x, A.x), a1 = ld3(x, A.y);
+    const Box b1 = box_edge(a0, a1);
+    if (off == nullptr) {
+        int n = 0;
+        lbvh_visit(tv.n, tv.nb, tv.left, tv.skip, b1, sq, [&](int eJ) { if (ee_keep(eI, eJ, A, se[eJ], bt, x, b1, sq)) ++n; });
+        cnt[eI] = n;
+        return;
+    }
+    const int o = off[eI], cap = off[eI + 1] - o;
+    int n = 0;
+    lbvh_visit(tv.n, tv.nb, tv.left, tv.skip, b1, sq, [&](int eJ) {
+        if (ee_keep(eI, eJ, A, se[eJ], bt, x, b1, sq)) { if (n < cap) cand[o + n] = eJ; ++n; }
+    });
+    n = n < cap ? n : cap;
+    sort_segment(cand + o, n);
+    for (int k = 0; k < n; ++k) {
+        const int eJ = cand[o + k];
+        const int2 B = se[eJ];
+        const d3 b0 = ld3(x, B.x), b1v = ld3(x, B.y);
+        const int dt = dtype_ee(a0, a1, b0, b1v);
+        const double cr = ee_cross_sqn(a0, a1, b0, b1v);
+        const double ex = eps_x_of(ld3(vrest, A.x), ld3(vrest, A.y), ld3(vrest, B.x), ld3(vrest, B.y));
+        const int add_e = (cr < ex) ? -eJ - 2 : -1;
+        int4 r = make_int4(0, 0, 0, 0);
+        bool hit = false;
+        double d;
+        switch (dt) {
+        case 0: d = d_PP(a0, b0); if (d < hhat) { r = make_int4(-A.x - 1, B.x, -1, add_e); hit = true; } break;
+        case 1: d = d_PP(a0, b1v); if (d < hhat) { r = make_int4(-A.x - 1, B.y, -1, add_e); hit = true; } break;
+        case 2: d = d_PE(a0, b0, b1v); if (d < hhat) { r = make_int4(-A.x - 1, B.x, B.y, add_e); hit = true; } break;
+        case 3: d = d_PP(a1, b0); if (d < hhat) { r = make_int4(-A.y - 1, B.x, -1, add_e); hit = true; } break;
+        case 4: d = d_PP(a1, b1v); if (d < hhat) { r = make_int4(-A.y - 1, B.y, -1, add_e); hit = true; } break;
+        case 5: d = d_PE(a1, b0, b1v); if (d < hhat) { r = make_int4(-A.y - 1, B.x, B.y, add_e); hit = true; } break;
+        case 6: d = d_PE(b0, a0, a1); if (d < hhat) { r = make_int4(-B.x - 1, A.x, A.y, add_e); hit = true; } break;
+        case 7: d = d_PE(b1v, a0, a1); if (d < hhat) { r = make_int4(-B.y - 1, A.x, A.y, add_e); hit = true; } break;
+        case 8:
+            d = d_EE(a0, a1, b0, b1v);
+            if (d < hhat) {
+                r = (add_e <= -2) ? make_int4(A.x, A.y, B.x, -B.y - Es - 2) : make_int4(A.x, A.y, B.x, B.y);
+                hit = true;
+            }
+            break;
+        default: break;
+        }
+        int ct = kNone;
+        int2 ax = make_int2(-1, -1);
+        if (hit) {
+            // serial compaction rules, CollisionBroadPhase.cpp:1028-1052
+            if (r.w >= 0) ct = kSelfEE;
+            else if (r.w == -1) ct = kMap;
+            else if (r.w >= -Es - 1) { ax = make_int2(eI, -r.w - 2); r.w = -1; ct = kMolPE; }
+            else { r.w = -r.w - Es - 2; ct = kMolEE; }
+        }
+        rec[o + k] = r;
+        aux[o + k] = ax;
+        cat[o + k] = ct;
+    }
+}
+
+__global__ void fill_tail(int* p, int n) { p[n] = 0; }
+__global__ void flags_by_cat(int n, const int* __restrict__ cat, unsigned mask, int* __restrict__ flag) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i > n) return;
+    flag[i] = (i < n) ? (int)((mask >> cat[i]) & 1u) : 0;
+}
+__global__ void scatter_int4(int n, const int* __restrict__ flag, const int* __restrict__ pos, const int4* __restrict__ in,
+                             int base, int4* __restrict__ out) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n || !flag[i]) return;
+    out[base + pos[i]] = in[i];
+}
+__global__ void scatter_int2(int n, const int* __restrict__ flag, const int* __restrict__ pos, const int2* __restrict__ in,
+                             int base, int2* __restrict__ out) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n || !flag[i]) return;
+    out[base + pos[i]] = in[i];
+}
+// PP/PE map key: (c0 + V, c1, c2 + 1) packed with `bits` per field preserves the signed
+// lexicographic order of std::map<EncodedContact, int> (all keys have c3 = -1).
+__global__ void map_keys(int n, const int* __restrict__ flag, const int* __restrict__ pos, const int4* __restrict__ rec, int base,
+                         int V, int bits, unsigned long long* __restrict__ out) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n || !flag[i]) return;
+    const int4 r = rec[i];
+    out[base + pos[i]] = ((unsigned long long)(r.x + V) << (2 * bits)) | ((unsigned long long)r.y << bits) |
+                         (unsigned long long)(r.z + 1);
+}
+__global__ void map_decode(int n, const unsigned long long* __restrict__ keys, const int* __restrict__ counts, int V, int bits,
+                           int base, int4* __restrict__ out) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const unsigned long long k = keys[i], m = (1ull << bits) - 1;
+    out[base + i] = make_int4((int)((k >> (2 * bits)) & m) - V, (int)((k >> bits) & m), (int)(k & m) - 1, -counts[i]);
+}
+__global__ void ccd_pairs_pt(int n, const int* __restrict__ q, const int* __restrict__ cand, int2* __restrict__ out) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    out[i] = make_int2(-q[i] - 1, cand[i]);
+}
+__global__ void ccd_pairs_ee(int n, int base, const int* __restrict__ q, const int* __restrict__ cand, int2* __restrict__ out) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    out[base + i] = make_int2(q[i], cand[i]);
+}
+// query id of every candidate from the per-query offsets
+__global__ void owner_of(int nq, const int* __restrict__ off, int* __restrict__ owner) {
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= nq) return;
+    for (int i = off[s]; i < off[s + 1]; ++i) owner[i] = s;
+}
+__global__ void ground_flags(int Vs, const int* __restrict__ sv, const double* __restrict__ x, d3 gn, double gd, double hhat,
+                             int* __restrict__ flag) {
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s > Vs) return;
+    if (s == Vs) { flag[s] = 0; return; }
+    const double dist = dot(gn, ld3(x, sv[s])) - gd;
+    flag[s] = (dist * dist < hhat) ? 1 : 0;
+}
+__global__ void ground_scatter(int Vs, const int* __restrict__ sv, const int* __restrict__ flag, const int* __restrict__ pos,
+                               int* __restrict__ out) {
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= Vs || !flag[s]) return;
+    out[pos[s]] = sv[s];
+}
+
+namespace {
+int read_int(Context& c, const int* p) {
+    int v = 0;
+    CIPC_HIP(hipMemcpyAsync(&v, p, sizeof(int), hipMemcpyDeviceToHost, c.stream));
+    CIPC_HIP(hipStreamSynchronize(c.stream));
+    return v;
+}
+}  // namespace
+
+void compute_active_set(Context& c) {
+    Scene& s = c.sc;
+    build_static_trees(c);
+    ActiveSetScratch& b = c.as;
+    const double sq = sqrt(s.hhat);
+    const int Vs = s.Vs, Es = s.Es;
+    const TreeView ft{c.face_tree.n, c.face_tree.node_box.p, c.face_tree.left.p, c.face_tree.skip.p};
+    const TreeView et{c.edge_tree.n, c.edge_tree.node_box.p, c.edge_tree.left.p, c.edge_tree.skip.p};
+    b.cntPT.resize(Vs + 1); b.offPT.resize(Vs + 1); b.cntEE.resize(Es + 1); b.offEE.resize(Es + 1);
+    // count
+    if (Vs > 0)
+        hipLaunchKernelGGL(pt_pass, dim3(blocks_for(Vs, 128)), dim3(128), 0, c.stream, Vs, s.sv.p, s.x.p, s.bt.p, s.faces.p, ft, sq,
+                           s.hhat, (const int*)nullptr, b.cntPT.p, (int*)nullptr, (int4*)nullptr, (int*)nullptr);
+    if (Es > 0)
+        hipLaunchKernelGGL(ee_pass, dim3(blocks_for(Es, 128)), dim3(128), 0, c.stream, Es, s.se.p, s.x.p, s.vrest.p, s.bt.p, et, sq,
+                           s.hhat, (const int*)nullptr, b.cntEE.p, (int*)nullptr, (int4*)nullptr, (int2*)nullptr, (int*)nullptr);
+    CIPC_CHECK_LAUNCH();
+    hipLaunchKernelGGL(fill_tail, dim3(1), dim3(1), 0, c.stream, b.cntPT.p, Vs);
+    hipLaunchKernelGGL(fill_tail, dim3(1), dim3(1), 0, c.stream, b.cntEE.p, Es);
+    exclusive_scan_int(c, b.cntPT.p, b.offPT.p, Vs + 1);
+    exclusive_scan_int(c, b.cntEE.p, b.offEE.p, Es + 1);
+    const int nPT = read_int(c, b.offPT.p + Vs);
+    const int nEE = read_int(c, b.offEE.p + Es);
+    // fill
+    b.candPT.resize(nPT + 1); b.recPT.resize(nPT + 1); b.catPT.resize(nPT + 1); b.ownPT.resize(nPT + 1);
+    b.candEE.resize(nEE + 1); b.recEE.resize(nEE + 1); b.catEE.resize(nEE + 1); b.auxEE.resize(nEE + 1); b.ownEE.resize(nEE + 1);
+    if (Vs > 0) {
+        hipLaunchKernelGGL(pt_pass, dim3(blocks_for(Vs, 128)), dim3(128), 0, c.stream, Vs, s.sv.p, s.x.p, s.bt.p, s.faces.p, ft, sq,
+                           s.hhat, (const int*)b.offPT.p, b.cntPT.p, b.candPT.p, b.recPT.p, b.catPT.p);
+        hipLaunchKernelGGL(owner_of, dim3(blocks_for(Vs, 256)), dim3(256), 0, c.stream, Vs, b.offPT.p, b.ownPT.p);
+    }
+    if (Es > 0) {
+        hipLaunchKernelGGL(ee_pass, dim3(blocks_for(Es, 128)), dim3(128), 0, c.stream, Es, s.se.p, s.x.p, s.vrest.p, s.bt.p, et, sq,
+                           s.hhat, (const int*)b.offEE.p, b.cntEE.p, b.candEE.p, b.recEE.p, b.auxEE.p, b.catEE.p);
+        hipLaunchKernelGGL(owner_of, dim3(blocks_for(Es, 256)), dim3(256), 0, c.stream, Es, b.offEE.p, b.ownEE.p);
+    }
+    CIPC_CHECK_LAUNCH();
+    // category flags and their positions
+    auto flag_scan = [&](int n, DBuf<int>& cat, unsigned mask, DBuf<int>& f, DBuf<int>& p) {
+        f.resize(n + 1); p.resize(n + 1);
+        hipLaunchKernelGGL(flags_by_cat, dim3(blocks_for(n + 1, 256)), dim3(256), 0, c.stream, n, cat.p, mask, f.p);
+        CIPC_CHECK_LAUNCH();
+        exclusive_scan_int(c, f.p, p.p, n + 1);
+    };
+    flag_scan(nPT, b.catPT, 1u << kSelfPT, b.f1, b.p1);
+    flag_scan(nPT, b.catPT, 1u << kMap, b.f2, b.p2);
+    flag_scan(nEE, b.catEE, 1u << kSelfEE, b.f3, b.p3);
+    flag_scan(nEE, b.catEE, 1u << kMap, b.f4, b.p4);
+    flag_scan(nEE, b.catEE, (1u << kMolPE) | (1u << kMolEE), b.f5, b.p5);
+    b.fg.resize(Vs + 1); b.pg.resize(Vs + 1);
+    hipLaunchKernelGGL(ground_flags, dim3(blocks_for(Vs + 1, 256)), dim3(256), 0, c.stream, Vs, s.sv.p, s.x.p, s.gn, s.gd, s.hhat, b.fg.p);
+    CIPC_CHECK_LAUNCH();
+    exclusive_scan_int(c, b.fg.p, b.pg.p, Vs + 1);
+    int tot[6];
+    const int* srcs[6] = {b.p1.p + nPT, b.p2.p + nPT, b.p3.p + nEE, b.p4.p + nEE, b.p5.p + nEE, b.pg.p + Vs};
+    for (int k = 0; k < 6; ++k) CIPC_HIP(hipMemcpyAsync(&tot[k], srcs[k], sizeof(int), hipMemcpyDeviceToHost, c.stream));
+    CIPC_HIP(hipStreamSynchronize(c.stream));
+    const int nA = tot[0], n2a = tot[1], nB = tot[2], n2b = tot[3], nE = tot[4], nG = tot[5];
+    // PP/PE map: sort packed keys, run-length encode
+    int bits = 1;
+    while ((1ll << bits) < (long long)s.V + 2) ++bits;
+    if (3 * bits > 64) throw Error(CIPC_ERR_OVERFLOW, "vertex count too large for the point-point/point-edge key packing");
+    const int nm = n2a + n2b;
+    int nM = 0;
+    b.mkeys.resize(nm + 1); b.mkeys_sorted.resize(nm + 1); b.ukeys.resize(nm + 1); b.ucounts.resize(nm + 1); b.nruns.resize(1);
+    if (nm > 0) {
+        if (nPT > 0)
+            hipLaunchKernelGGL(map_keys, dim3(blocks_for(nPT, 256)), dim3(256), 0, c.stream, nPT, b.f2.p, b.p2.p, b.recPT.p, 0, s.V,
+                               bits, b.mkeys.p);
+        if (nEE > 0)
+            hipLaunchKernelGGL(map_keys, dim3(blocks_for(nEE, 256)), dim3(256), 0, c.stream, nEE, b.f4.p, b.p4.p, b.recEE.p, n2a, s.V,
+                               bits, b.mkeys.p);
+        CIPC_CHECK_LAUNCH();
+        sort_u64(c, b.mkeys.p, b.mkeys_sorted.p, nm, 3 * bits);
+        size_t bytes = 0;
+        CIPC_HIP(hipcub::DeviceRunLengthEncode::Encode(nullptr, bytes, b.mkeys_sorted.p, b.ukeys.p, b.ucounts.p, b.nruns.p, nm, c.stream));
+        c.tmp.reserve(bytes);
+        CIPC_HIP(hipcub::DeviceRunLengthEncode::Encode(c.tmp.p, bytes, b.mkeys_sorted.p, b.ukeys.p, b.ucounts.p, b.nruns.p, nm, c.stream));
+        nM = read_int(c, b.nruns.p);
+    }
+    // outputs
+    c.n_self = nA + nB + nM; c.n_ee = nE; c.n_ccd = nPT + nEE; c.n_env = nG;
+    c.self_set.resize(c.n_self + 1); c.ee_set.resize(nE + 1); c.ee_ie.resize(nE + 1);
+    c.ccd_set.resize(c.n_ccd + 1); c.env_set.resize(nG + 1);
+    if (nPT > 0) {
+        hipLaunchKernelGGL(scatter_int4, dim3(blocks_for(nPT, 256)), dim3(256), 0, c.stream, nPT, b.f1.p, b.p1.p, b.recPT.p, 0, c.self_set.p);
+        hipLaunchKernelGGL(ccd_pairs_pt, dim3(blocks_for(nPT, 256)), dim3(256), 0, c.stream, nPT, b.ownPT.p, b.candPT.p, c.ccd_set.p);
+    }
+    if (nEE > 0) {
+        hipLaunchKernelGGL(scatter_int4, dim3(blocks_for(nEE, 256)), dim3(256), 0, c.stream, nEE, b.f3.p, b.p3.p, b.recEE.p, nA, c.self_set.p);
+        hipLaunchKernelGGL(scatter_int4, dim3(blocks_for(nEE, 256)), dim3(256), 0, c.stream, nEE, b.f5.p, b.p5.p, b.recEE.p, 0, c.ee_set.p);
+        hipLaunchKernelGGL(scatter_int2, dim3(blocks_for(nEE, 256)), dim3(256), 0, c.stream, nEE, b.f5.p, b.p5.p, b.auxEE.p, 0, c.ee_ie.p);
+        hipLaunchKernelGGL(ccd_pairs_ee, dim3(blocks_for(nEE, 256)), dim3(256), 0, c.stream, nEE, nPT, b.ownEE.p, b.candEE.p, c.ccd_set.p);
+    }
+    if (nM > 0)
+        hipLaunchKernelGGL(map_decode, dim3(blocks_for(nM, 256)), dim3(256), 0, c.stream, nM, b.ukeys.p, b.ucounts.p, s.V, bits, nA + nB,
+                           c.self_set.p);
+    if (Vs > 0)
+        hipLaunchKernelGGL(ground_scatter, dim3(blocks_for(Vs, 256)), dim3(256), 0, c.stream, Vs, s.sv.p, b.fg.p, b.pg.p, c.env_set.p);
+    CIPC_CHECK_LAUNCH();
+    CIPC_HIP(hipStreamSynchronize(c.stream));
+}
+
+// ---------------------------------------------------------------------------------
+// isIntersected (IPCSolver.cpp:476-542): ground gaps of the current
+// Environment_ActiveSet, then every surface edge against every face whose static box it
+// touches, with IglUtils::segTriIntersect (ContactMechanics.h:47-71).
+// ---------------------------------------------------------------------------------
+// Eigen 3.4 FullPivLU solve of the 3x3 system of segTriIntersect: complete pivoting on
+// the first largest |a_ij| in column-major order, rank threshold 3 eps |max pivot|.
+__device__ inline bool seg_tri(d3 e0, d3 e1, d3 t0, d3 t1, d3 t2) {
+    const d3 c0 = t1 - t0, c1 = t2 - t0, c2 = e0 - e1;
+    const d3 n = cross(c0, c1);
+    if (dot(n, e0 - t0) * dot(n, e1 - t0) > 0.0) return false;
+    double A[3][3] = {{c0.x, c1.x, c2.x}, {c0.y, c1.y, c2.y}, {c0.z, c1.z, c2.z}};
+    const double det = A[0][0] * (A[1][1] * A[2][2] - A[1][2] * A[2][1]) - A[0][1] * (A[1][0] * A[2][2] - A[1][2] * A[2][0])
+                     + A[0][2] * (A[1][0] * A[2][1] - A[1][1] * A[2][0]);
+    if (det == 0.0) return false;
+    int rt[3] = {0, 1, 2}, ct[3] = {0, 1, 2}, nz = 3;
+    double maxpiv = 0.0;
+    for (int k = 0; k < 3; ++k) {
+        int br = k, bc = k;
+        double best = fabs(A[k][k]);
+        for (int cc = k; cc < 3; ++cc)
+            for (int r = k; r < 3; ++r)
+                if (fabs(A[r][cc]) > best) { best = fabs(A[r][cc]); br = r; bc = cc; }
+        if (best == 0.0) { nz = k; for (int i = k; i < 3; ++i) { rt[i] = i; ct[i] = i; } break; }
+        if (best > maxpiv) maxpiv = best;
+        rt[k] = br; ct[k] = bc;
+        if (br != k) for (int cc = 0; cc < 3; ++cc) { const double t = A[k][cc]; A[k][cc] = A[br][cc]; A[br][cc] = t; }
+        if (bc != k) for (int r = 0; r < 3; ++r) { const double t = A[r][k]; A[r][k] = A[r][bc]; A[r][bc] = t; }
+        for (int r = k + 1; r < 3; ++r) A[r][k] /= A[k][k];
+        for (int cc = k + 1; cc < 3; ++cc)
+            for (int r = k + 1; r < 3; ++r) A[r][cc] -= A[r][k] * A[k][cc];
+    }
+    int rank = 0;
+    const double thr = maxpiv * (2.220446049250313e-16 * 3.0);
+    for (int i = 0; i < nz; ++i) rank += fabs(A[i][i]) > thr;
+    const d3 rhs = e0 - t0;
+    double cv[3] = {rhs.x, rhs.y, rhs.z}, xs[3] = {0, 0, 0};
+    if (rank > 0) {
+        for (int k = 0; k < 3; ++k) { const double t = cv[k]; cv[k] = cv[rt[k]]; cv[rt[k]] = t; }
+        for (int i = 0; i < 3; ++i) if (cv[i] != 0.0) for (int r = i + 1; r < 3; ++r) cv[r] -= cv[i] * A[r][i];
+        for (int i = rank - 1; i >= 0; --i) {
+            if (cv[i] == 0.0) continue;
+            cv[i] /= A[i][i];
+            for (int r = 0; r < i; ++r) cv[r] -= cv[i] * A[r][i];
+        }
+        int q[3] = {0, 1, 2};
+        for (int k = 0; k < 3; ++k) { const int t = q[k]; q[k] = q[ct[k]]; q[ct[k]] = t; }
+        for (int i = 0; i < 3; ++i) xs[q[i]] = (i < rank) ? cv[i] : 0.0;
+    }
+    return xs[0] >= 0.0 && xs[1] >= 0.0 && xs[0] + xs[1] <= 1.0 && xs[2] >= 0.0 && xs[2] <= 1.0;
+}
+__global__ void intersect_kernel(int F, const int4* __restrict__ faces, const int2* __restrict__ se, const double* __restrict__ x,
+                                 const int* __restrict__ bt, TreeView et, int* __restrict__ hit) {
+    const int f = blockIdx.x * blockDim.x + threadIdx.x;
+    if (f >= F) return;
+    const int4 fv = faces[f];
+    const d3 t0 = ld3(x, fv.x), t1 = ld3(x, fv.y), t2 = ld3(x, fv.z);
+    const Box q = box_tri(t0, t1, t2);
+    bool any = false;
+    lbvh_visit(et.n, et.nb, et.left, et.skip, q, 0.0, [&](int e) {
+        if (any) return;
+        const int2 E = se[e];
+        if (E.x == fv.x || E.x == fv.y || E.x == fv.z || E.y == fv.x || E.y == fv.y || E.y == fv.z) return;
+        if (is_collider(bt[E.x]) && is_collider(bt[E.y]) && is_collider(bt[fv.x]) && is_collider(bt[fv.y]) && is_collider(bt[fv.z])) return;
+        if (seg_tri(ld3(x, E.x), ld3(x, E.y), t0, t1, t2)) any = true;
+    });
+    if (any) atomicOr(hit, 1);
+}
+__global__ void ground_touch_kernel(int n, const int* __restrict__ env, const double* __restrict__ x, d3 gn, double gd, int* __restrict__ hit) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const double dist = dot(gn, ld3(x, env[i])) - gd;
+    if (dist * dist <= 0.0) atomicOr(hit, 1);
+}
+__global__ void zero_int(int* p, int n) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = 0;
+}
+
+bool any_intersection(Context& c) {
+    Scene& s = c.sc;
+    c.counters.resize(4);
+    hipLaunchKernelGGL(zero_int, dim3(1), dim3(64), 0, c.stream, c.counters.p, 2);
+    if (c.n_env > 0)
+        hipLaunchKernelGGL(ground_touch_kernel, dim3(blocks_for(c.n_env, 256)), dim3(256), 0, c.stream, (int)c.n_env, c.env_set.p,
+                           s.x.p, s.gn, s.gd, c.counters.p);
+    build_static_trees(c);
+    const TreeView et{c.edge_tree.n, c.edge_tree.node_box.p, c.edge_tree.left.p, c.edge_tree.skip.p};
+    if (s.F > 0)
+        hipLaunchKernelGGL(intersect_kernel, dim3(blocks_for(s.F, 128)), dim3(128), 0, c.stream, s.F, s.faces.p, s.se.p, s.x.p, s.bt.p,
+                           et, c.counters.p + 1);
+    CIPC_CHECK_LAUNCH();
+    int h[2];
+    CIPC_HIP(hipMemcpyAsync(h, c.counters.p, 2 * sizeof(int), hipMemcpyDeviceToHost, c.stream));
+    CIPC_HIP(hipStreamSynchronize(c.stream));
+    return h[0] != 0 || h[1] != 0;
+}
+
+}  // namespace cipc

@@ -1,0 +1,185 @@
+// Newton step bound on the device.  Compiled with -ffp-contract=off.
+//
+// Reference: limitStepByGround (CPU IPC/FeasibleStep.cpp:8-40), Self_largestFeasibleStepSize
+// (ContactMechanics.cpp:3704-3752), the CFL bound (IPCSolver.cpp:960-974), the swept
+// rebuild SpatialHash::build(mesh, p, alpha) -> CPUBroadPhaseBVH::buildSwept
+// (CollisionBroadPhase.cpp:223-237, LBVH.cpp:307-339), Self_largestFeasibleStepSize_CCD
+// (ContactMechanics.cpp:3754-3867) and the merge (IPCSolver.cpp:985-989).
+//
+// Every bound is a min (or max) of per-primitive values; they are reduced with integer
+// atomics on the IEEE bit patterns of non-negative doubles, which order like the values,
+// so the result is exact and independent of scheduling.
+#include "context.hpp"
+#include "contact.hpp"
+#include "lbvh.hpp"
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+
+namespace cipc {
+
+__device__ inline unsigned long long ord_bits(double v) {
+    if (v == 0.0) v = 0.0;  // -0 -> +0
+    return (unsigned long long)__double_as_longlong(v);
+}
+__device__ inline void atomic_min_nonneg(unsigned long long* a, double v) { atomicMin(a, ord_bits(v)); }
+__device__ inline void atomic_max_nonneg(unsigned long long* a, double v) { atomicMax(a, ord_bits(v)); }
+__device__ inline double bits_to_double(unsigned long long b) { return __longlong_as_double((long long)b); }
+
+// slots: 0 ground min, 1 CFL max |p|, 2 partial CCD min, 3 full CCD min
+__global__ void init_bounds(unsigned long long* s) {
+    s[0] = ord_bits(1.0);
+    s[1] = ord_bits(0.0);
+    s[2] = ord_bits(1.0);
+    s[3] = ord_bits(1.0);
+}
+
+__global__ void ground_cfl_kernel(int Vs, const int* __restrict__ sv, const double* __restrict__ x, const double* __restrict__ p,
+                                  d3 gn, double gd, double slack, unsigned long long* __restrict__ slots) {
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= Vs) return;
+    const int v = sv[s];
+    const d3 pv = ld3(p, v);
+    double best = 1.0;
+    const double nm = dot(gn, -pv);
+    if (!(nm >= 0.0)) {
+        const double dist = dot(gn, ld3(x, v)) - gd;
+        const double cand = -dist / nm * slack;
+        if (cand >= 0.0) best = cand;
+    }
+    atomic_min_nonneg(&slots[0], best);
+    atomic_max_nonneg(&slots[1], sqrt(sqn(pv)));
+}
+
+__global__ void partial_ccd_kernel(int n, const int2* __restrict__ pairs, const int* __restrict__ sv, const int4* __restrict__ faces,
+                                   const int2* __restrict__ se, const double* __restrict__ x, const double* __restrict__ p, double eta,
+                                   unsigned long long* __restrict__ slot) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int2 pr = pairs[i];
+    double a;
+    if (pr.x < 0) {
+        const int pt = sv[-pr.x - 1];
+        const int4 f = faces[pr.y];
+        a = accd_pt(ld3(x, pt), ld3(x, f.x), ld3(x, f.y), ld3(x, f.z), -ld3(p, pt), -ld3(p, f.x), -ld3(p, f.y), -ld3(p, f.z), eta, 0.0);
+    } else {
+        const int2 e1 = se[pr.x], e2 = se[pr.y];
+        a = accd_ee(ld3(x, e1.x), ld3(x, e1.y), ld3(x, e2.x), ld3(x, e2.y), -ld3(p, e1.x), -ld3(p, e1.y), -ld3(p, e2.x), -ld3(p, e2.y),
+                    eta, 0.0);
+    }
+    atomic_min_nonneg(slot, a);
+}
+
+// Full CCD, point-triangle phase: one thread per surface vertex (ContactMechanics.cpp:3764-3816).
+__global__ void full_ccd_pt_kernel(int Vs, const int* __restrict__ sv, const int4* __restrict__ faces, const int* __restrict__ bt,
+                                   const double* __restrict__ x, const double* __restrict__ p, const Box* __restrict__ qbox,
+                                   int n, const Box* __restrict__ nb, const int* __restrict__ left, const int* __restrict__ skip,
+                                   double a, double eta, unsigned long long* __restrict__ slot) {
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= Vs) return;
+    const int vI = sv[s];
+    const d3 X = ld3(x, vI), P = ld3(p, vI);
+    const Box pb = swept_pt(X, P, a);
+    double best = 1.0;
+    lbvh_visit(n, nb, left, skip, qbox[s], 0.0, [&](int f) {
+        const int4 fv = faces[f];
+        if (is_collider(bt[vI]) && is_collider(bt[fv.x]) && is_collider(bt[fv.y]) && is_collider(bt[fv.z])) return;
+        if (vI == fv.x || vI == fv.y || vI == fv.z) return;
+        const d3 x0 = ld3(x, fv.x), x1 = ld3(x, fv.y), x2 = ld3(x, fv.z);
+        const d3 p0 = ld3(p, fv.x), p1 = ld3(p, fv.y), p2 = ld3(p, fv.z);
+        const Box fb = box_union(box_union(swept_pt(x0, p0, a), swept_pt(x1, p1, a)), swept_pt(x2, p2, a));
+        if (!overlaps(pb, fb, 0.0)) return;
+        const double la = accd_pt(X, x0, x1, x2, -P, -p0, -p1, -p2, eta, 0.0);
+        best = fmin(la, best);
+    });
+    atomic_min_nonneg(slot, best);
+}
+
+// Full CCD, edge-edge phase: one thread per surface edge eJ, candidates eI > eJ from the
+// tree built with the incoming step, exact swept-box filter with the point-triangle-reduced
+// step read from `slot` (ContactMechanics.cpp:3818-3866).
+__global__ void full_ccd_ee_kernel(int Es, const int2* __restrict__ se, const int* __restrict__ bt, const double* __restrict__ x,
+                                   const double* __restrict__ p, int n, const Box* __restrict__ nb, const int* __restrict__ left,
+                                   const int* __restrict__ skip, const Box* __restrict__ ebox, const unsigned long long* __restrict__ aslot,
+                                   double eta, unsigned long long* __restrict__ slot) {
+    const int eJ = blockIdx.x * blockDim.x + threadIdx.x;
+    if (eJ >= Es) return;
+    const double a = bits_to_double(*aslot);
+    const int2 J = se[eJ];
+    const d3 j0 = ld3(x, J.x), j1 = ld3(x, J.y), pj0 = ld3(p, J.x), pj1 = ld3(p, J.y);
+    const Box b1 = box_union(swept_pt(j0, pj0, a), swept_pt(j1, pj1, a));
+    double best = 1.0;
+    // query box: eJ's leaf box of the tree (swept with the incoming step)
+    const Box q = ebox[eJ];
+    lbvh_visit(n, nb, left, skip, q, 0.0, [&](int eI) {
+        if (eI <= eJ) return;
+        const int2 I = se[eI];
+        if (is_collider(bt[I.x]) && is_collider(bt[I.y]) && is_collider(bt[J.x]) && is_collider(bt[J.y])) return;
+        if (I.x == J.x || I.x == J.y || I.y == J.x || I.y == J.y || eI < eJ) return;
+        const d3 i0 = ld3(x, I.x), i1 = ld3(x, I.y), pi0 = ld3(p, I.x), pi1 = ld3(p, I.y);
+        const Box b2 = box_union(swept_pt(i0, pi0, a), swept_pt(i1, pi1, a));
+        if (!overlaps(b1, b2, 0.0)) return;
+        const double la = accd_ee(i0, i1, j0, j1, -pi0, -pi1, -pj0, -pj1, eta, 0.0);
+        best = fmin(la, best);
+    });
+    atomic_min_nonneg(slot, best);
+}
+__global__ void copy_slot(const unsigned long long* src, unsigned long long* dst) { *dst = *src; }
+
+void step_bound(Context& c, double out[5]) {
+    Scene& s = c.sc;
+    const double slack = 0.8;
+    const double eta = 1.0 - slack;
+    DBuf<unsigned long long>& sl = c.u64scratch;
+    sl.resize(8);
+    hipLaunchKernelGGL(init_bounds, dim3(1), dim3(1), 0, c.stream, sl.p);
+    if (s.Vs > 0)
+        hipLaunchKernelGGL(ground_cfl_kernel, dim3(blocks_for(s.Vs, 256)), dim3(256), 0, c.stream, s.Vs, s.sv.p, s.x.p, s.dir.p, s.gn,
+                           s.gd, slack, sl.p);
+    if (c.n_ccd > 0)
+        hipLaunchKernelGGL(partial_ccd_kernel, dim3(blocks_for(c.n_ccd, 64)), dim3(64), 0, c.stream, (int)c.n_ccd, c.ccd_set.p, s.sv.p,
+                           s.faces.p, s.se.p, s.x.p, s.dir.p, eta, sl.p + 2);
+    CIPC_CHECK_LAUNCH();
+    unsigned long long h[4];
+    CIPC_HIP(hipMemcpyAsync(h, sl.p, 2 * sizeof(unsigned long long), hipMemcpyDeviceToHost, c.stream));
+    CIPC_HIP(hipStreamSynchronize(c.stream));
+    double ground = 1.0;
+    std::memcpy(&ground, &h[0], sizeof(double));
+    double alpha = s.Vs > 0 ? std::min(1.0, ground) : 1.0;
+    const double aground = alpha;
+    if (alpha <= 0) alpha = 1;
+    // full CCD with the swept trees built for step alpha
+    build_swept_trees(c, alpha);
+    {
+        const unsigned long long ab = [&] { unsigned long long b; double a = alpha; std::memcpy(&b, &a, 8); return b; }();
+        CIPC_HIP(hipMemcpyAsync(sl.p + 3, &ab, sizeof(ab), hipMemcpyHostToDevice, c.stream));
+    }
+    LBVH& ft = c.face_tree;
+    LBVH& et = c.edge_tree;
+    if (s.Vs > 0 && ft.n > 0)
+        hipLaunchKernelGGL(full_ccd_pt_kernel, dim3(blocks_for(s.Vs, 64)), dim3(64), 0, c.stream, s.Vs, s.sv.p, s.faces.p, s.bt.p, s.x.p,
+                           s.dir.p, c.swept_point_box.p, ft.n, ft.node_box.p, ft.left.p, ft.skip.p, alpha, eta, sl.p + 3);
+    if (s.Es > 0 && et.n > 0) {
+        hipLaunchKernelGGL(copy_slot, dim3(1), dim3(1), 0, c.stream, sl.p + 3, sl.p + 4);
+        hipLaunchKernelGGL(full_ccd_ee_kernel, dim3(blocks_for(s.Es, 64)), dim3(64), 0, c.stream, s.Es, s.se.p, s.bt.p, s.x.p, s.dir.p,
+                           et.n, et.node_box.p, et.left.p, et.skip.p, et.prim.p, sl.p + 4, eta, sl.p + 3);
+    }
+    CIPC_CHECK_LAUNCH();
+    CIPC_HIP(hipMemcpyAsync(h, sl.p, 4 * sizeof(unsigned long long), hipMemcpyDeviceToHost, c.stream));
+    CIPC_HIP(hipStreamSynchronize(c.stream));
+    double maxmove, partial_min, full;
+    std::memcpy(&maxmove, &h[1], 8);
+    std::memcpy(&partial_min, &h[2], 8);
+    std::memcpy(&full, &h[3], 8);
+    const double partial = c.n_ccd > 0 ? std::min(alpha, partial_min) : alpha;
+    const double acfl = maxmove > 0.0 ? std::sqrt(s.hhat) / (maxmove * 2.0) : 1.0;
+    double merged = std::min(alpha, acfl);
+    if (partial > 2.0 * acfl) {
+        merged = std::min(partial, full);
+        merged = std::max(merged, acfl);
+    }
+    out[0] = merged; out[1] = aground; out[2] = partial; out[3] = acfl; out[4] = full;
+}
+
+}  // namespace cipc

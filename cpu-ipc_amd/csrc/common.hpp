@@ -1,0 +1,130 @@
+// Shared device/host definitions of the MI355X (gfx950) Newton-iteration library.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "../../include/cipc_gpu.h"
+
+namespace cipc {
+
+// ------------------------------------------------------------------------------------
+// Errors: every ABI entry point catches cipc::Error and maps it to a status code plus a
+// message retrievable with cipc_gpu_last_error().
+// ------------------------------------------------------------------------------------
+struct Error : std::runtime_error {
+    int code;
+    Error(int c, const std::string& m) : std::runtime_error(m), code(c) {}
+};
+
+#define CIPC_HIP(call)                                                                         \
+    do {                                                                                       \
+        hipError_t e_ = (call);                                                                \
+        if (e_ != hipSuccess)                                                                  \
+            throw ::cipc::Error(CIPC_ERR_HIP, std::string(#call) + ": " + hipGetErrorString(e_)); \
+    } while (0)
+
+#define CIPC_CHECK_LAUNCH() CIPC_HIP(hipGetLastError())
+
+// Grown-on-demand device buffer; never shrinks during a context's life.
+template <typename T>
+struct DBuf {
+    T* p = nullptr;
+    size_t cap = 0;
+    size_t n = 0;
+    DBuf() = default;
+    DBuf(const DBuf&) = delete;
+    DBuf& operator=(const DBuf&) = delete;
+    ~DBuf() { release(); }
+    void reserve(size_t want) {
+        if (want <= cap) return;
+        if (p) CIPC_HIP(hipFree(p));
+        size_t c = want + want / 4 + 16;
+        CIPC_HIP(hipMalloc(&p, c * sizeof(T)));
+        cap = c;
+    }
+    void resize(size_t want) { reserve(want); n = want; }
+    void release() {
+        if (p) (void)hipFree(p);
+        p = nullptr; cap = n = 0;
+    }
+    T* data() { return p; }
+    const T* data() const { return p; }
+};
+
+inline unsigned blocks_for(size_t n, unsigned bs) { return (unsigned)((n + bs - 1) / bs); }
+
+// ------------------------------------------------------------------------------------
+// Device vector helpers.  Contact classification, distances, boxes and CCD use these
+// with floating-point contraction disabled in their translation units, so that each
+// expression rounds exactly as written (and as the CPU restatement rounds it).
+// ------------------------------------------------------------------------------------
+struct d3 {
+    double x, y, z;
+};
+__host__ __device__ inline d3 mk(double a, double b, double c) { return d3{a, b, c}; }
+__host__ __device__ inline d3 operator+(d3 a, d3 b) { return {a.x + b.x, a.y + b.y, a.z + b.z}; }
+__host__ __device__ inline d3 operator-(d3 a, d3 b) { return {a.x - b.x, a.y - b.y, a.z - b.z}; }
+__host__ __device__ inline d3 operator-(d3 a) { return {-a.x, -a.y, -a.z}; }
+__host__ __device__ inline d3 operator*(double s, d3 a) { return {s * a.x, s * a.y, s * a.z}; }
+__host__ __device__ inline d3 operator/(d3 a, double s) { return {a.x / s, a.y / s, a.z / s}; }
+__host__ __device__ inline double dot(d3 a, d3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+__host__ __device__ inline double sqn(d3 a) { return a.x * a.x + a.y * a.y + a.z * a.z; }
+__host__ __device__ inline d3 cross(d3 a, d3 b) {
+    return {a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x};
+}
+__host__ __device__ inline d3 vmin(d3 a, d3 b) { return {fmin(a.x, b.x), fmin(a.y, b.y), fmin(a.z, b.z)}; }
+__host__ __device__ inline d3 vmax(d3 a, d3 b) { return {fmax(a.x, b.x), fmax(a.y, b.y), fmax(a.z, b.z)}; }
+__device__ inline d3 ld3(const double* p, int i) { return {p[3 * i], p[3 * i + 1], p[3 * i + 2]}; }
+__device__ inline void st3(double* p, int i, d3 v) { p[3 * i] = v.x; p[3 * i + 1] = v.y; p[3 * i + 2] = v.z; }
+
+__host__ __device__ inline bool is_free(int t) { return t == 0; }
+__host__ __device__ inline bool is_collider(int t) { return t >= 2; }
+
+// Axis-aligned box (lower, upper) in 48 bytes.
+struct Box {
+    d3 lo, hi;
+};
+// Touching counts as overlapping (CPU IPC/LBVH.cpp:157-161); `q` is the querying box.
+__host__ __device__ inline bool overlaps(const Box& q, const Box& b, double gap) {
+    return !((b.lo.x - q.hi.x > gap) || (b.lo.y - q.hi.y > gap) || (b.lo.z - q.hi.z > gap) ||
+             (q.lo.x - b.hi.x > gap) || (q.lo.y - b.hi.y > gap) || (q.lo.z - b.hi.z > gap));
+}
+
+// ------------------------------------------------------------------------------------
+// Device-resident scene and per-iteration state.
+// ------------------------------------------------------------------------------------
+struct Scene {
+    int V = 0, T = 0, F = 0, Vs = 0, Es = 0;
+    double dt = 0, u = 0, r = 0, drag = 0, hhat = 0, fhat = 0, fric = 0;
+    d3 gn{0, 1, 0};
+    double gd = -1;
+    DBuf<double> x, x0, vrest, vprev, xt, mass, dir;
+    DBuf<int> bt;
+    DBuf<int4> tets;           // v0..v3
+    DBuf<double> dminv;        // SoA: 9 x T (entry k of tet t at k*T + t)
+    DBuf<double> vol;
+    DBuf<int4> faces;          // v0, v1, v2, tet
+    DBuf<int> sv;              // surface vertices
+    DBuf<int2> se;             // surface edges
+    std::vector<int> h_bt;     // host copy for pattern construction
+    std::vector<int> h_tets;
+};
+
+// Static elastic/mass block pattern and gather lists (built once per scene).
+struct StaticPattern {
+    int nblocks = 0;
+    DBuf<int> col_ptr;      // V+1: blocks of block-column b in [col_ptr[b], col_ptr[b+1])
+    DBuf<int> row;          // nblocks: block row a (a >= b), ascending within a column
+    DBuf<int> gptr;         // nblocks+1: contributions of block k in [gptr[k], gptr[k+1])
+    DBuf<int> gsrc;         // tet*16 + local block (0..9), plus bit 30 when transposed
+    DBuf<int> vptr;         // V+1: tet-corner contributions to vertex gradients
+    DBuf<int> vsrc;         // tet*4 + corner
+    DBuf<int> diag_block;   // V: block index of (v, v) in the static pattern
+};
+
+}  // namespace cipc

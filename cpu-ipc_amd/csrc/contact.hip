@@ -1,0 +1,375 @@
+// Contact stage of the Newton assembly: ground barrier, self-contact barriers and the
+// mollified edge-edge barrier.  Compiled with -ffp-contract=off.
+//
+// Reference: ground loop (CPU IPC/IPCSolver.cpp:270-293), compute_g_dpt
+// (ContactMechanics.cpp:2970-3032), compute_g_dee (:3034-3080), compute_H_dpt
+// (:3466-3573), compute_H_dee (:3575-3700).
+//
+// Each contact becomes a local Hessian over its 2-4 vertices (kernel 1), is projected to
+// PSD by register-resident Jacobi (kernel 2), and is scattered as lower-triangle 3x3
+// block records keyed (column vertex << 32 | row vertex) plus per-vertex gradient
+// records (kernel 3).  The records are sorted and summed by key in sparse.hip, so the
+// result does not depend on thread scheduling.
+#include "context.hpp"
+#include "contact.hpp"
+
+namespace cipc {
+
+constexpr int kHU = 78;                  // upper triangle of a 12x12
+constexpr unsigned long long kNoKey = ~0ull;
+
+__device__ inline int nv_of(int type) { return type == 0 ? 2 : (type == 1 ? 3 : 4); }
+
+// Decode an EncodedContact (EncodedContact.h; SelfConstraintVal, ContactMechanics.cpp:2478-2503):
+// type 0 PP, 1 PE, 2 PT, 3 EE; vertices in the reference's argument order.
+__device__ inline int decode(int4 c, int v[4]) {
+    if (c.x >= 0) { v[0] = c.x; v[1] = c.y; v[2] = c.z; v[3] = c.w; return 3; }
+    v[0] = -c.x - 1; v[1] = c.y; v[2] = c.z; v[3] = c.w;
+    if (c.z < 0) return 0;
+    if (c.w < 0) return 1;
+    return 2;
+}
+__device__ inline double self_value(int type, const d3* p) {
+    switch (type) {
+    case 0: return d_PP(p[0], p[1]);
+    case 1: return d_PE(p[0], p[1], p[2]);
+    case 2: return d_PT(p[0], p[1], p[2], p[3]);
+    default: return d_EE(p[0], p[1], p[2], p[3]);
+    }
+}
+// Gradient and full Hessian (N x N row-major) of the distance of `type` at packed points.
+__device__ inline void dist_gh(int type, const double* q, double* g, double* H) {
+    if (type == 0) {
+        const double e[3] = {q[0] - q[3], q[1] - q[4], q[2] - q[5]};
+        for (int i = 0; i < 3; ++i) { g[i] = 2.0 * e[i]; g[3 + i] = -g[i]; }
+        for (int i = 0; i < 36; ++i) H[i] = 0.0;
+        for (int i = 0; i < 6; ++i) { H[i * 6 + i] = 2.0; }
+        for (int i = 0; i < 3; ++i) { H[i * 6 + 3 + i] = -2.0; H[(3 + i) * 6 + i] = -2.0; }
+    } else if (type == 1) {
+        dist_pe_gh(q, g, H);
+    } else if (type == 2) {
+        dist_pt_gh(q, g, H);
+    } else {
+        dist_ee_gh(q, g, H);
+    }
+}
+
+// Kernel 1: local Hessian (upper triangle, SoA) and local gradient of every contact.
+//   c < n_self: Self_ActiveSet entry, compute_H_dpt / compute_g_dpt
+//   otherwise : Self_EE_ActiveSet entry, compute_H_dee / compute_g_dee
+__global__ __launch_bounds__(64) void contact_local_kernel(
+    int n_self, int n_ee, const int4* __restrict__ self_set, const int4* __restrict__ ee_set,
+    const int2* __restrict__ ee_ie, const int2* __restrict__ se, const double* __restrict__ x,
+    const double* __restrict__ vrest, double hhat, double kappa, double* __restrict__ Hs, double* __restrict__ Gs,
+    int4* __restrict__ verts, int* __restrict__ dims) {
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    const int n = n_self + n_ee;
+    if (c >= n) return;
+    double H[144], g[12];
+    if (c < n_self) {
+        const int4 e = self_set[c];
+        int v[4];
+        const int type = decode(e, v);
+        const int nv = nv_of(type), N = 3 * nv;
+        d3 p[4];
+        double q[12];
+        for (int k = 0; k < nv; ++k) { p[k] = ld3(x, v[k]); q[3 * k] = p[k].x; q[3 * k + 1] = p[k].y; q[3 * k + 2] = p[k].z; }
+        const double d = self_value(type, p);
+        dist_gh(type, q, g, H);
+        const double gb = barrier_g(d, hhat), hb = barrier_h(d, hhat);
+        const double coef = (type <= 1) ? kappa * -e.w : kappa;
+        const double cH = coef * hb, cg = coef * gb;
+        int idx = 0;
+        for (int i = 0; i < N; ++i)
+            for (int j = i; j < N; ++j) Hs[(size_t)(idx++) * n + c] = (cH * g[i]) * g[j] + cg * H[i * N + j];
+        const double gs = (type <= 1) ? (kappa * -e.w) * gb : kappa * gb;
+        for (int i = 0; i < N; ++i) Gs[(size_t)i * n + c] = gs * g[i];
+        verts[c] = make_int4(v[0], v[1], type >= 1 ? v[2] : -1, type >= 2 ? v[3] : -1);
+        dims[c] = N;
+    } else {
+        const int i = c - n_self;
+        const int4 e = ee_set[i];
+        int v[4];
+        const int type = decode(e, v);
+        const int nvd = nv_of(type);
+        int e4[4];
+        if (e.w >= 0) { e4[0] = e.x; e4[1] = e.y; e4[2] = e.z; e4[3] = e.w; }
+        else {
+            const int2 ie = ee_ie[i];
+            const int2 a = se[ie.x], b = se[ie.y];
+            e4[0] = a.x; e4[1] = a.y; e4[2] = b.x; e4[3] = b.y;
+        }
+        d3 p[4], pe[4];
+        for (int k = 0; k < nvd; ++k) p[k] = ld3(x, v[k]);
+        for (int k = 0; k < 4; ++k) pe[k] = ld3(x, e4[k]);
+        const double d = self_value(type, p);
+        const double b = barrier_b(d, hhat), gb = barrier_g(d, hhat), hb = barrier_h(d, hhat);
+        const double ex = eps_x_of(ld3(vrest, e4[0]), ld3(vrest, e4[1]), ld3(vrest, e4[2]), ld3(vrest, e4[3]));
+        const double cr = ee_cross_sqn(pe[0], pe[1], pe[2], pe[3]);
+        const double mol = mol_e(cr, ex);
+        // distance gradient/Hessian embedded over e4 (indMap of ContactMechanics.cpp:3627-3685)
+        double qd[12];
+        for (int k = 0; k < nvd; ++k) { qd[3 * k] = p[k].x; qd[3 * k + 1] = p[k].y; qd[3 * k + 2] = p[k].z; }
+        double gl[12];
+        dist_gh(type, qd, gl, H);
+        int map[4] = {0, 1, 2, 3};
+        if (type != 3) {
+            for (int k = 0; k < 4; ++k) {
+                if (v[0] == e4[k]) map[0] = k;
+                else if (v[1] == e4[k]) map[1] = k;
+                else if (nvd == 3 && v[2] == e4[k]) map[2] = k;
+            }
+        }
+        const int Nd = 3 * nvd;
+        double gd[12];
+        for (int k = 0; k < 12; ++k) gd[k] = 0.0;
+        for (int a = 0; a < nvd; ++a)
+            for (int r = 0; r < 3; ++r) gd[3 * map[a] + r] = gl[3 * a + r];
+        // mollifier gradient and Hessian of the cross norm
+        double qe[12], gc[12], Hc[144];
+        for (int k = 0; k < 4; ++k) { qe[3 * k] = pe[k].x; qe[3 * k + 1] = pe[k].y; qe[3 * k + 2] = pe[k].z; }
+        ee_cross_gh(qe, gc, Hc);
+        double ge[12], qg = 0.0, qH = 0.0;
+        const bool act = cr < ex;
+        if (act) {
+            const double oe = 1.0 / ex;
+            qg = 2.0 * oe * (-oe * cr + 1.0);
+            qH = -2.0 / (ex * ex);
+        }
+        for (int k = 0; k < 12; ++k) ge[k] = act ? gc[k] * qg : 0.0;
+        const double kgb = kappa * gb, kb = kappa * b, keh = kappa * mol * hb, keg = kappa * mol * gb;
+        int idx = 0;
+        for (int r = 0; r < 12; ++r)
+            for (int s = r; s < 12; ++s) {
+                // embedded distance Hessian entry
+                double hd = 0.0;
+                const int vr = r / 3, vs = s / 3;
+                int lr = -1, ls = -1;
+                for (int a = 0; a < nvd; ++a) { if (map[a] == vr) lr = a; if (map[a] == vs) ls = a; }
+                if (lr >= 0 && ls >= 0) hd = H[(3 * lr + r % 3) * Nd + 3 * ls + s % 3];
+                const double eH = act ? Hc[r * 12 + s] * qg + (qH * gc[r]) * gc[s] : 0.0;
+                const double kgg = (kgb * gd[r]) * ge[s];
+                const double kggT = (kgb * gd[s]) * ge[r];
+                Hs[(size_t)(idx++) * n + c] = kgg + kggT + kb * eH + (keh * gd[r]) * gd[s] + keg * hd;
+            }
+        const double gsc = kappa * (gb * mol);
+        for (int k = 0; k < 12; ++k) Gs[(size_t)k * n + c] = kb * ge[k] + gsc * gd[k];
+        verts[c] = make_int4(e4[0], e4[1], e4[2], e4[3]);
+        dims[c] = 12;
+    }
+}
+
+// Kernel 2: makePD of each local Hessian, in place.
+template <int N>
+__device__ inline void pd_inplace(double* Hs, int n, int c) {
+    double H[N * N];
+    int idx = 0;
+#pragma unroll
+    for (int i = 0; i < N; ++i)
+#pragma unroll
+        for (int j = i; j < N; ++j) { const double v = Hs[(size_t)(idx++) * n + c]; H[i * N + j] = v; H[j * N + i] = v; }
+    make_pd<N>(H);
+    idx = 0;
+#pragma unroll
+    for (int i = 0; i < N; ++i)
+#pragma unroll
+        for (int j = i; j < N; ++j) Hs[(size_t)(idx++) * n + c] = H[i * N + j];
+}
+__global__ __launch_bounds__(64) void contact_pd_kernel(int n, const int* __restrict__ dims, double* __restrict__ Hs) {
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= n) return;
+    const int N = dims[c];
+    if (N == 6) pd_inplace<6>(Hs, n, c);
+    else if (N == 9) pd_inplace<9>(Hs, n, c);
+    else pd_inplace<12>(Hs, n, c);
+}
+
+// Kernel 3: block and gradient records.  Slots: 10 blocks and 4 gradients per contact.
+__global__ void contact_scatter_kernel(int n, const int4* __restrict__ verts, const int* __restrict__ dims,
+                                       const double* __restrict__ Hs, const double* __restrict__ Gs,
+                                       const int* __restrict__ bt, int slot0_b, int slot0_g,
+                                       unsigned long long* __restrict__ bkey, int* __restrict__ bidx, double* __restrict__ bval,
+                                       int* __restrict__ gkey, int* __restrict__ gidx, double* __restrict__ gval) {
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= n) return;
+    const int N = dims[c], nv = N / 3;
+    const int4 vv = verts[c];
+    const int v[4] = {vv.x, vv.y, vv.z, vv.w};
+    // upper-triangle index of (i, j), i <= j, in an N x N
+    auto ux = [N](int i, int j) { return i * N - i * (i - 1) / 2 + (j - i); };
+    int slot = 0;
+    for (int a = 0; a < 4; ++a)
+        for (int b = 0; b <= a; ++b, ++slot) {
+            const int s = slot0_b + 10 * c + slot;
+            bidx[s] = s;
+            if (a >= nv || !is_free(bt[v[a]]) || !is_free(bt[v[b]])) { bkey[s] = kNoKey; continue; }
+            // global lower block: rows of the larger vertex id
+            const int ra = v[a] >= v[b] ? a : b, cb = v[a] >= v[b] ? b : a;
+            const int R = v[ra], Cc = v[cb];
+            bkey[s] = ((unsigned long long)(unsigned)Cc << 32) | (unsigned)R;
+            for (int i = 0; i < 3; ++i)
+                for (int j = 0; j < 3; ++j) {
+                    const int gi = 3 * ra + i, gj = 3 * cb + j;
+                    const int lo = gi < gj ? gi : gj, hi = gi < gj ? gj : gi;
+                    bval[(size_t)9 * s + 3 * i + j] = Hs[(size_t)ux(lo, hi) * n + c];
+                }
+        }
+    for (int a = 0; a < 4; ++a) {
+        const int s = slot0_g + 4 * c + a;
+        gidx[s] = s;
+        if (a >= nv) { gkey[s] = 0x7fffffff; continue; }
+        gkey[s] = v[a];
+        for (int i = 0; i < 3; ++i) gval[(size_t)3 * s + i] = Gs[(size_t)(3 * a + i) * n + c];
+    }
+}
+
+// Ground (IPCSolver.cpp:270-293): one block and one gradient record per active vertex.
+__global__ void ground_kernel(int n_env, const int* __restrict__ env, const double* __restrict__ x, const int* __restrict__ bt,
+                              d3 gn, double gd, double hhat, double kappa,
+                              unsigned long long* __restrict__ bkey, int* __restrict__ bidx, double* __restrict__ bval,
+                              int* __restrict__ gkey, int* __restrict__ gidx, double* __restrict__ gval) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n_env) return;
+    const int v = env[i];
+    const double dist = dot(gn, ld3(x, v)) - gd;
+    const double d = dist * dist;
+    const double g_b = barrier_g(d, hhat), H_b = barrier_h(d, hhat);
+    const d3 g = (((kappa * g_b) * 2.0) * dist) * gn;
+    gkey[i] = v; gidx[i] = i;
+    gval[3 * i] = g.x; gval[3 * i + 1] = g.y; gval[3 * i + 2] = g.z;
+    const double param = 4.0 * H_b * d + 2.0 * g_b;
+    bidx[i] = i;
+    if (param > 0 && is_free(bt[v])) {
+        bkey[i] = ((unsigned long long)(unsigned)v << 32) | (unsigned)v;
+        const double nn[3] = {gn.x, gn.y, gn.z};
+        for (int a = 0; a < 3; ++a)
+            for (int b = 0; b < 3; ++b) bval[(size_t)9 * i + 3 * a + b] = (kappa * param) * (nn[a] * nn[b]);
+    } else {
+        bkey[i] = kNoKey;
+    }
+}
+
+void contact_assemble(Context& c, double kappa) {
+    Scene& s = c.sc;
+    const int n = (int)(c.n_self + c.n_ee);
+    const int ne = (int)c.n_env;
+    const size_t nb = (size_t)ne + 10 * (size_t)n, ng = (size_t)ne + 4 * (size_t)n;
+    c.ckey.resize(std::max<size_t>(nb, 1)); c.cidx.resize(std::max<size_t>(nb, 1));
+    c.cval.resize(9 * std::max<size_t>(nb, 1));
+    c.gkey.resize(std::max<size_t>(ng, 1)); c.gidx.resize(std::max<size_t>(ng, 1));
+    c.gval.resize(3 * std::max<size_t>(ng, 1));
+    if (ne > 0) {
+        hipLaunchKernelGGL(ground_kernel, dim3(blocks_for(ne, 256)), dim3(256), 0, c.stream, ne, c.env_set.p, s.x.p, s.bt.p,
+                           s.gn, s.gd, s.hhat, kappa, c.ckey.p, c.cidx.p, c.cval.p, c.gkey.p, c.gidx.p, c.gval.p);
+        CIPC_CHECK_LAUNCH();
+    }
+    if (n > 0) {
+        c.chess.resize((size_t)(kHU + 12) * n);
+        c.cverts.resize(n);
+        c.cdims.resize(n);
+        double* Hs = c.chess.p;
+        double* Gs = c.chess.p + (size_t)kHU * n;
+        hipLaunchKernelGGL(contact_local_kernel, dim3(blocks_for(n, 64)), dim3(64), 0, c.stream, (int)c.n_self, (int)c.n_ee,
+                           c.self_set.p, c.ee_set.p, c.ee_ie.p, s.se.p, s.x.p, s.vrest.p, s.hhat, kappa, Hs, Gs,
+                           c.cverts.p, c.cdims.p);
+        CIPC_CHECK_LAUNCH();
+        hipLaunchKernelGGL(contact_pd_kernel, dim3(blocks_for(n, 64)), dim3(64), 0, c.stream, n, c.cdims.p, Hs);
+        CIPC_CHECK_LAUNCH();
+        hipLaunchKernelGGL(contact_scatter_kernel, dim3(blocks_for(n, 128)), dim3(128), 0, c.stream, n, c.cverts.p,
+                           c.cdims.p, Hs, Gs, s.bt.p, ne, ne, c.ckey.p, c.cidx.p, c.cval.p, c.gkey.p, c.gidx.p, c.gval.p);
+        CIPC_CHECK_LAUNCH();
+    }
+    c.ckey.n = nb;
+    c.gkey.n = ng;
+}
+
+// ---------------------------------------------------------------------------------
+// Element-level evaluation entry points (tests)
+// ---------------------------------------------------------------------------------
+__global__ void distance_eval_kernel(int n, int kind, const double* __restrict__ pts, double* __restrict__ d,
+                                     double* __restrict__ g, double* __restrict__ Ho) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n) return;
+    double q[12], gg[12], H[144];
+    for (int i = 0; i < 12; ++i) q[i] = pts[12 * t + i];
+    d3 p[4];
+    for (int k = 0; k < 4; ++k) p[k] = d3{q[3 * k], q[3 * k + 1], q[3 * k + 2]};
+    int N;
+    if (kind == 4) { ee_cross_gh(q, gg, H); N = 12; d[t] = ee_cross_sqn(p[0], p[1], p[2], p[3]); }
+    else { dist_gh(kind, q, gg, H); N = 3 * nv_of(kind); d[t] = self_value(kind, p); }
+    for (int i = 0; i < 12; ++i) g[12 * t + i] = i < N ? gg[i] : 0.0;
+    for (int i = 0; i < 144; ++i) Ho[144 * t + i] = 0.0;
+    for (int i = 0; i < N; ++i)
+        for (int j = 0; j < N; ++j) Ho[144 * t + i * N + j] = H[i * N + j];
+}
+__global__ void make_pd_eval_kernel(int n, int dim, double* __restrict__ H) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n) return;
+    double* h = H + (size_t)dim * dim * t;
+    if (dim == 6) { double a[36]; for (int i = 0; i < 36; ++i) a[i] = h[i]; make_pd<6>(a); for (int i = 0; i < 36; ++i) h[i] = a[i]; }
+    else if (dim == 9) { double a[81]; for (int i = 0; i < 81; ++i) a[i] = h[i]; make_pd<9>(a); for (int i = 0; i < 81; ++i) h[i] = a[i]; }
+    else { double a[144]; for (int i = 0; i < 144; ++i) a[i] = h[i]; make_pd<12>(a); for (int i = 0; i < 144; ++i) h[i] = a[i]; }
+}
+__global__ void accd_eval_kernel(int n, int kind, const double* __restrict__ pts, const double* __restrict__ dirs, double eta,
+                                 double th, double* __restrict__ toc) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n) return;
+    d3 p[4], dd[4];
+    for (int k = 0; k < 4; ++k) { p[k] = ld3(pts + 12 * t, k); dd[k] = ld3(dirs + 12 * t, k); }
+    toc[t] = kind == 0 ? accd_pt(p[0], p[1], p[2], p[3], dd[0], dd[1], dd[2], dd[3], eta, th)
+                       : accd_ee(p[0], p[1], p[2], p[3], dd[0], dd[1], dd[2], dd[3], eta, th);
+}
+__global__ void classify_eval_kernel(int n, int kind, const double* __restrict__ pts, int* __restrict__ out) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n) return;
+    d3 p[4];
+    for (int k = 0; k < 4; ++k) p[k] = ld3(pts + 12 * t, k);
+    out[t] = kind == 0 ? dtype_pt(p[0], p[1], p[2], p[3]) : dtype_ee(p[0], p[1], p[2], p[3]);
+}
+
+template <typename T>
+struct DevArr {
+    T* p = nullptr;
+    explicit DevArr(size_t n) { CIPC_HIP(hipMalloc(&p, sizeof(T) * (n ? n : 1))); }
+    ~DevArr() { if (p) hipFree(p); }
+    void up(const T* h, size_t n) { CIPC_HIP(hipMemcpy(p, h, sizeof(T) * n, hipMemcpyHostToDevice)); }
+    void down(T* h, size_t n) { CIPC_HIP(hipMemcpy(h, p, sizeof(T) * n, hipMemcpyDeviceToHost)); }
+};
+
+}  // namespace cipc
+
+using namespace cipc;
+
+extern "C" int cipc_gpu_eval_distance_impl(int n, int kind, const double* pts, double* d, double* g, double* H) {
+    DevArr<double> dp(12 * (size_t)n), dd(n), dg(12 * (size_t)n), dh(144 * (size_t)n);
+    dp.up(pts, 12 * (size_t)n);
+    hipLaunchKernelGGL(distance_eval_kernel, dim3(blocks_for(n, 64)), dim3(64), 0, 0, n, kind, dp.p, dd.p, dg.p, dh.p);
+    CIPC_CHECK_LAUNCH();
+    dd.down(d, n); dg.down(g, 12 * (size_t)n); dh.down(H, 144 * (size_t)n);
+    return 0;
+}
+extern "C" int cipc_gpu_eval_make_pd_impl(int n, int dim, double* H) {
+    DevArr<double> dh((size_t)dim * dim * n);
+    dh.up(H, (size_t)dim * dim * n);
+    hipLaunchKernelGGL(make_pd_eval_kernel, dim3(blocks_for(n, 64)), dim3(64), 0, 0, n, dim, dh.p);
+    CIPC_CHECK_LAUNCH();
+    dh.down(H, (size_t)dim * dim * n);
+    return 0;
+}
+extern "C" int cipc_gpu_eval_accd_impl(int n, int kind, const double* pts, const double* dirs, double eta, double th, double* toc) {
+    DevArr<double> dp(12 * (size_t)n), dd(12 * (size_t)n), dt(n);
+    dp.up(pts, 12 * (size_t)n); dd.up(dirs, 12 * (size_t)n);
+    hipLaunchKernelGGL(accd_eval_kernel, dim3(blocks_for(n, 64)), dim3(64), 0, 0, n, kind, dp.p, dd.p, eta, th, dt.p);
+    CIPC_CHECK_LAUNCH();
+    dt.down(toc, n);
+    return 0;
+}
+extern "C" int cipc_gpu_eval_classify_impl(int n, int kind, const double* pts, int* type) {
+    DevArr<double> dp(12 * (size_t)n);
+    DevArr<int> dt(n);
+    dp.up(pts, 12 * (size_t)n);
+    hipLaunchKernelGGL(classify_eval_kernel, dim3(blocks_for(n, 64)), dim3(64), 0, 0, n, kind, dp.p, dt.p);
+    CIPC_CHECK_LAUNCH();
+    dt.down(type, n);
+    return 0;
+}

@@ -1,0 +1,292 @@
+// Contact geometry on the device: distance-type classification, squared distances,
+// barrier, mollifier, PSD projection and additive CCD.  Included only by translation
+// units compiled with -ffp-contract=off so that classification, distances, boxes and
+// CCD round exactly as the reference expressions are written (bit-exact pair sets).
+#pragma once
+#include "common.hpp"
+
+namespace cipc {
+
+#include "distance_gen.inc"
+
+// ---- barrier b(d) = -(d - dHat)^2 ln(d/dHat) and derivatives (ContactMechanics.cpp:775-790)
+__device__ inline double barrier_b(double d, double dh) { return -(d - dh) * (d - dh) * log(d / dh); }
+__device__ inline double barrier_g(double d, double dh) {
+    const double t = d - dh;
+    return t * log(d / dh) * -2.0 - (t * t) / d;
+}
+__device__ inline double barrier_h(double d, double dh) {
+    const double t = d - dh;
+    return (log(d / dh) * -2.0 - t * 4.0 / d) + 1.0 / (d * d) * (t * t);
+}
+
+// ---- squared distances in the reference's operation order
+__device__ inline double d_PP(d3 a, d3 b) { return sqn(a - b); }                                 // :282-287
+__device__ inline double d_PE(d3 p, d3 e0, d3 e1) { return sqn(cross(e0 - p, e1 - p)) / sqn(e1 - e0); } // :807-813
+__device__ inline double d_PT(d3 p, d3 t0, d3 t1, d3 t2) {                                       // :298-306
+    const d3 b = cross(t1 - t0, t2 - t0);
+    const double aTb = dot(p - t0, b);
+    return aTb * aTb / sqn(b);
+}
+__device__ inline double d_EE(d3 a0, d3 a1, d3 b0, d3 b1) {                                      // :764-773
+    const d3 b = cross(a1 - a0, b1 - b0);
+    const double aTb = dot(b0 - a0, b);
+    return aTb * aTb / sqn(b);
+}
+__device__ inline double ee_cross_sqn(d3 a0, d3 a1, d3 b0, d3 b1) { return sqn(cross(a1 - a0, b1 - b0)); }
+
+// Eigen 3.4 LDLT solve of a symmetric 2x2 [[a00, a10], [a10, a11]] (diagonal pivoting on
+// the larger |diagonal|, unit lower L, D pseudo-inverse below DBL_MIN), as
+// (basis * basis^T).ldlt().solve(...) evaluates it in dType_PT.
+__device__ inline void ldlt2_solve(double a00, double a10, double a11, double b0, double b1, double& x0, double& x1) {
+    bool swapped = fabs(a11) > fabs(a00);
+    double d0 = swapped ? a11 : a00, d1 = swapped ? a00 : a11, l = a10;
+    if (!(fabs(d0) > 0.0)) {
+        swapped = false; d0 = a00; d1 = a11;
+    } else {
+        l = l / d0;
+        const double tmp = d0 * l;
+        d1 = d1 - l * tmp;
+    }
+    double y0 = swapped ? b1 : b0, y1 = swapped ? b0 : b1;
+    y1 = y1 - l * y0;
+    const double tol = 2.2250738585072014e-308;
+    y0 = fabs(d0) > tol ? y0 / d0 : 0.0;
+    y1 = fabs(d1) > tol ? y1 / d1 : 0.0;
+    y0 = y0 - l * y1;
+    if (swapped) { x0 = y1; x1 = y0; } else { x0 = y0; x1 = y1; }
+}
+__device__ inline void pt_param(d3 r0, d3 n, d3 rhs, double& p0, double& p1) {
+    const d3 r1 = cross(r0, n);
+    ldlt2_solve(dot(r0, r0), dot(r1, r0), dot(r1, r1), dot(r0, rhs), dot(r1, rhs), p0, p1);
+}
+// dType_PT (ContactMechanics.cpp:143-193) with (v0 | v1 v2 v3); with the triangle
+// passed as (t0 t1 t2) it is also point_triangle_distance_type (AdditiveCCD.cpp:8-62).
+__device__ inline int dtype_pt(d3 v0, d3 v1, d3 v2, d3 v3) {
+    const d3 n = cross(v2 - v1, v3 - v1);
+    double p00, p10, p01, p11, p02, p12;
+    pt_param(v2 - v1, n, v0 - v1, p00, p10);
+    if (p00 > 0.0 && p00 < 1.0 && p10 >= 0.0) return 3;
+    pt_param(v3 - v2, n, v0 - v2, p01, p11);
+    if (p01 > 0.0 && p01 < 1.0 && p11 >= 0.0) return 4;
+    pt_param(v1 - v3, n, v0 - v3, p02, p12);
+    if (p02 > 0.0 && p02 < 1.0 && p12 >= 0.0) return 5;
+    if (p00 <= 0.0 && p02 >= 1.0) return 0;
+    if (p01 <= 0.0 && p00 >= 1.0) return 1;
+    if (p02 <= 0.0 && p01 >= 1.0) return 2;
+    return 6;
+}
+// dType_EE (ContactMechanics.cpp:195-280) == edge_edge_distance_type (AdditiveCCD.cpp:64-139)
+__device__ inline int dtype_ee(d3 v0, d3 v1, d3 v2, d3 v3) {
+    const d3 u = v1 - v0, v = v3 - v2, w = v0 - v2;
+    const double a = sqn(u), b = dot(u, v), c = sqn(v), d = dot(u, w), e = dot(v, w);
+    const double D = a * c - b * b;
+    double tD = D, sN, tN;
+    int def = 8;
+    sN = (b * e - c * d);
+    if (sN <= 0.0) { tN = e; tD = c; def = 2; }
+    else if (sN >= D) { tN = e + b; tD = c; def = 5; }
+    else {
+        tN = (a * e - b * d);
+        const d3 uv = cross(u, v);
+        if (tN > 0.0 && tN < tD && (dot(uv, w) == 0.0 || sqn(uv) < 1.0e-20 * a * c)) {
+            if (sN < D / 2) { tN = e; tD = c; def = 2; }
+            else { tN = e + b; tD = c; def = 5; }
+        }
+    }
+    if (tN <= 0.0) {
+        if (-d <= 0.0) return 0;
+        else if (-d >= a) return 3;
+        else return 6;
+    } else if (tN >= tD) {
+        if ((-d + b) <= 0.0) return 1;
+        else if ((-d + b) >= a) return 4;
+        else return 7;
+    }
+    return def;
+}
+
+// ---- mollifier (ContactMechanics.cpp:678-762)
+__device__ inline double eps_x_of(d3 ra0, d3 ra1, d3 rb0, d3 rb1) { return 1e-3 * sqn(ra0 - ra1) * sqn(rb0 - rb1); }
+__device__ inline double mol_e(double c, double ex) {
+    if (!(c < ex)) return 1.0;
+    const double t = c / ex;
+    return (-t + 2.0) * t;
+}
+
+// ---- PSD projection IglUtils::makePD<N> (ContactMechanics.h:27-45): cyclic Jacobi on the
+// upper triangle held in registers; the block is rebuilt from the eigenpairs with the
+// negative eigenvalues set to zero (when none is negative this reproduces the input up
+// to rounding, which is what "return unchanged" means at the 1e-10 parity bar).
+template <int N>
+struct Sym {
+    static constexpr int ix(int i, int j) { return i <= j ? i * N - i * (i - 1) / 2 + (j - i) : j * N - j * (j - 1) / 2 + (i - j); }
+};
+template <int N>
+__device__ inline void make_pd(double* __restrict__ H) {
+    constexpr int NU = N * (N + 1) / 2;
+    double A[NU], Z[N * N];
+#pragma unroll
+    for (int i = 0; i < N; ++i)
+#pragma unroll
+        for (int j = i; j < N; ++j) A[Sym<N>::ix(i, j)] = H[i * N + j];
+#pragma unroll
+    for (int i = 0; i < N * N; ++i) Z[i] = (i % (N + 1) == 0) ? 1.0 : 0.0;
+    for (int sweep = 0; sweep < 30; ++sweep) {
+        double off = 0.0, dg = 0.0;
+#pragma unroll
+        for (int i = 0; i < N; ++i) {
+            dg += A[Sym<N>::ix(i, i)] * A[Sym<N>::ix(i, i)];
+#pragma unroll
+            for (int j = i + 1; j < N; ++j) off += A[Sym<N>::ix(i, j)] * A[Sym<N>::ix(i, j)];
+        }
+        if (off == 0.0 || off <= 1e-34 * dg) break;
+#pragma unroll
+        for (int p = 0; p < N; ++p) {
+#pragma unroll
+            for (int q = p + 1; q < N; ++q) {
+                const double apq = A[Sym<N>::ix(p, q)];
+                if (apq != 0.0) {
+                    const double app = A[Sym<N>::ix(p, p)], aqq = A[Sym<N>::ix(q, q)];
+                    const double th = (aqq - app) / (2.0 * apq);
+                    const double t = (th >= 0 ? 1.0 : -1.0) / (fabs(th) + sqrt(th * th + 1.0));
+                    const double c = 1.0 / sqrt(t * t + 1.0), s = t * c;
+#pragma unroll
+                    for (int k = 0; k < N; ++k) {
+                        if (k == p || k == q) continue;
+                        const double akp = A[Sym<N>::ix(k, p)], akq = A[Sym<N>::ix(k, q)];
+                        A[Sym<N>::ix(k, p)] = c * akp - s * akq;
+                        A[Sym<N>::ix(k, q)] = s * akp + c * akq;
+                    }
+                    A[Sym<N>::ix(p, p)] = app - t * apq;
+                    A[Sym<N>::ix(q, q)] = aqq + t * apq;
+                    A[Sym<N>::ix(p, q)] = 0.0;
+#pragma unroll
+                    for (int k = 0; k < N; ++k) {
+                        const double zkp = Z[k * N + p], zkq = Z[k * N + q];
+                        Z[k * N + p] = c * zkp - s * zkq;
+                        Z[k * N + q] = s * zkp + c * zkq;
+                    }
+                }
+            }
+        }
+    }
+    double w[N];
+#pragma unroll
+    for (int i = 0; i < N; ++i) w[i] = fmax(A[Sym<N>::ix(i, i)], 0.0);
+#pragma unroll
+    for (int i = 0; i < N; ++i)
+#pragma unroll
+        for (int j = i; j < N; ++j) {
+            double acc = 0.0;
+#pragma unroll
+            for (int k = 0; k < N; ++k) acc += Z[i * N + k] * w[k] * Z[j * N + k];
+            H[i * N + j] = acc;
+            H[j * N + i] = acc;
+        }
+}
+
+// ---- Additive CCD (AdditiveCCD.cpp:146-329), reference operation order
+__device__ inline double ccd_pe(d3 p, d3 e0, d3 e1) { return sqn(cross(e0 - p, e1 - p)) / sqn(e1 - e0); }
+__device__ inline double ccd_pt(d3 p, d3 t0, d3 t1, d3 t2) {
+    const d3 b = cross(t1 - t0, t2 - t0);
+    const double aTb = dot(p - t0, b);
+    return aTb * aTb / sqn(b);
+}
+__device__ inline double ccd_ee(d3 a0, d3 a1, d3 b0, d3 b1) {
+    const d3 b = cross(a1 - a0, b1 - b0);
+    const double aTb = dot(b0 - a0, b);
+    return aTb * aTb / sqn(b);
+}
+__device__ inline double ee_unclassified(d3 a0, d3 a1, d3 b0, d3 b1) {
+    switch (dtype_ee(a0, a1, b0, b1)) {
+    case 0: return sqn(a0 - b0);
+    case 1: return sqn(a0 - b1);
+    case 2: return ccd_pe(a0, b0, b1);
+    case 3: return sqn(a1 - b0);
+    case 4: return sqn(a1 - b1);
+    case 5: return ccd_pe(a1, b0, b1);
+    case 6: return ccd_pe(b0, a0, a1);
+    case 7: return ccd_pe(b1, a0, a1);
+    case 8: return ccd_ee(a0, a1, b0, b1);
+    default: return 1.7976931348623157e308;
+    }
+}
+__device__ inline double pt_unclassified(d3 p, d3 t0, d3 t1, d3 t2) {
+    switch (dtype_pt(p, t0, t1, t2)) {
+    case 0: return sqn(p - t0);
+    case 1: return sqn(p - t1);
+    case 2: return sqn(p - t2);
+    case 3: return ccd_pe(p, t0, t1);
+    case 4: return ccd_pe(p, t1, t2);
+    case 5: return ccd_pe(p, t2, t0);
+    case 6: return ccd_pt(p, t0, t1, t2);
+    default: return 1.7976931348623157e308;
+    }
+}
+__device__ inline double clamp01(double toc) { return fmax(0.0, fmin(toc, 1.0)); }
+// edge_edge_ccd (AdditiveCCD.cpp:221-279)
+__device__ inline double accd_ee(d3 ea0, d3 ea1, d3 eb0, d3 eb1, d3 dea0, d3 dea1, d3 deb0, d3 deb1, double eta, double th) {
+    const d3 mov = (dea0 + dea1 + deb0 + deb1) / 4;
+    dea0 = dea0 - mov; dea1 = dea1 - mov; deb0 = deb0 - mov; deb1 = deb1 - mov;
+    const double maxd = sqrt(fmax(sqn(dea0), sqn(dea1))) + sqrt(fmax(sqn(deb0), sqn(deb1)));
+    if (maxd == 0) return 1.0;
+    double d2 = ee_unclassified(ea0, ea1, eb0, eb1);
+    double dF = d2 - th * th;
+    if (dF <= 0) {
+        d2 = fmin(fmin(sqn(ea0 - eb0), sqn(ea0 - eb1)), fmin(sqn(ea1 - eb0), sqn(ea1 - eb1)));
+        dF = d2 - th * th;
+    }
+    double dc = sqrt(d2);
+    const double gap = eta * dF / (dc + th);
+    double toc = 0.0;
+    for (int it = 0; it < 10000; ++it) {
+        const double tl = (1 - eta) * dF / ((dc + th) * maxd);
+        if (!isfinite(tl) || tl <= 0.0) return clamp01(toc);
+        ea0 = ea0 + tl * dea0; ea1 = ea1 + tl * dea1; eb0 = eb0 + tl * deb0; eb1 = eb1 + tl * deb1;
+        d2 = ee_unclassified(ea0, ea1, eb0, eb1);
+        dF = d2 - th * th;
+        if (dF <= 0) {
+            d2 = fmin(fmin(sqn(ea0 - eb0), sqn(ea0 - eb1)), fmin(sqn(ea1 - eb0), sqn(ea1 - eb1)));
+            dF = d2 - th * th;
+        }
+        dc = sqrt(d2);
+        if (toc != 0.0 && (dF / (dc + th) < gap)) break;
+        toc += tl;
+        if (toc > 1.0) return 1.0;
+    }
+    return clamp01(toc);
+}
+// point_triangle_ccd (AdditiveCCD.cpp:281-329)
+__device__ inline double accd_pt(d3 p, d3 t0, d3 t1, d3 t2, d3 dp, d3 dt0, d3 dt1, d3 dt2, double eta, double th) {
+    const d3 mov = (dt0 + dt1 + dt2 + dp) / 4;
+    dt0 = dt0 - mov; dt1 = dt1 - mov; dt2 = dt2 - mov; dp = dp - mov;
+    const double mx = fmax(fmax(sqn(dt0), sqn(dt1)), sqn(dt2));
+    const double maxd = sqrt(sqn(dp)) + sqrt(mx);
+    if (maxd == 0) return 1.0;
+    double d2 = pt_unclassified(p, t0, t1, t2);
+    double dc = sqrt(d2);
+    const double gap = eta * (d2 - th * th) / (dc + th);
+    double toc = 0.0;
+    for (int it = 0; it < 10000; ++it) {
+        const double tl = (1 - eta) * (d2 - th * th) / ((dc + th) * maxd);
+        if (!isfinite(tl) || tl <= 0.0) return clamp01(toc);
+        p = p + tl * dp; t0 = t0 + tl * dt0; t1 = t1 + tl * dt1; t2 = t2 + tl * dt2;
+        d2 = pt_unclassified(p, t0, t1, t2);
+        dc = sqrt(d2);
+        if (toc != 0.0 && ((d2 - th * th) / (dc + th) < gap)) break;
+        toc += tl;
+        if (toc > 1.0) return 1.0;
+    }
+    return clamp01(toc);
+}
+
+// Swept boxes {x, x - a p} (LBVH.cpp:120-129,400-431).
+__device__ inline Box box_pt(d3 p) { return Box{p, p}; }
+__device__ inline Box box_add(Box b, d3 p) { return Box{vmin(b.lo, p), vmax(b.hi, p)}; }
+__device__ inline Box box_edge(d3 a, d3 b) { return box_add(box_pt(a), b); }
+__device__ inline Box box_tri(d3 a, d3 b, d3 c) { return box_add(box_add(box_pt(a), b), c); }
+__device__ inline Box box_union(Box a, Box b) { return Box{vmin(a.lo, b.lo), vmax(a.hi, b.hi)}; }
+__device__ inline Box swept_pt(d3 p, d3 dir, double a) { return box_add(box_pt(p), p - a * dir); }
+
+}  // namespace cipc

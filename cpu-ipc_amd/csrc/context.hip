@@ -1,0 +1,415 @@
+// C ABI of the device path (declared in include/cipc_gpu.h).  Every entry point catches
+// cipc::Error / std::exception and returns a status code; the message is kept per thread.
+#include "context.hpp"
+
+#include <algorithm>
+#include <cstring>
+#include <string>
+
+namespace {
+thread_local std::string g_last_error;
+
+int fail(int code, const std::string& msg) {
+    g_last_error = msg;
+    return code;
+}
+
+template <typename F>
+int guard(F&& f) {
+    try {
+        f();
+        return CIPC_OK;
+    } catch (const cipc::Error& e) {
+        return fail(e.code, e.what());
+    } catch (const std::bad_alloc&) {
+        return fail(CIPC_ERR_RUNTIME, "host allocation failed");
+    } catch (const std::exception& e) {
+        return fail(CIPC_ERR_RUNTIME, e.what());
+    }
+}
+
+void require(bool ok, const char* msg) {
+    if (!ok) throw cipc::Error(CIPC_ERR_INVALID_ARGUMENT, msg);
+}
+
+template <typename T>
+void upload(cipc::DBuf<T>& d, const T* h, size_t n, hipStream_t s) {
+    d.resize(std::max<size_t>(n, 1));
+    if (n) CIPC_HIP(hipMemcpyAsync(d.p, h, n * sizeof(T), hipMemcpyHostToDevice, s));
+}
+
+void bind(cipc::Context& c) { CIPC_HIP(hipSetDevice(c.device)); }
+}  // namespace
+
+struct cipc_gpu_context : cipc::Context {};
+
+extern "C" int cipc_gpu_eval_snh_impl(int n, const double* F, double u, double r, double* P, double* H, double* e);
+extern "C" int cipc_gpu_eval_distance_impl(int n, int kind, const double* pts, double* d, double* g, double* H);
+extern "C" int cipc_gpu_eval_make_pd_impl(int n, int dim, double* H);
+extern "C" int cipc_gpu_eval_accd_impl(int n, int kind, const double* pts, const double* dirs, double eta, double th, double* toc);
+extern "C" int cipc_gpu_eval_classify_impl(int n, int kind, const double* pts, int* type);
+
+using namespace cipc;
+
+extern "C" {
+
+const char* cipc_gpu_last_error(void) { return g_last_error.c_str(); }
+
+int cipc_gpu_context_create(int device, cipc_gpu_context** out) {
+    return guard([&] {
+        require(out != nullptr, "output pointer is null");
+        int n = 0;
+        if (hipGetDeviceCount(&n) != hipSuccess || n <= 0) throw Error(CIPC_ERR_NO_DEVICE, "no HIP device is available");
+        require(device >= 0 && device < n, "device ordinal out of range");
+        CIPC_HIP(hipSetDevice(device));
+        hipDeviceProp_t prop;
+        CIPC_HIP(hipGetDeviceProperties(&prop, device));
+        if (std::strncmp(prop.gcnArchName, "gfx950", 6) != 0)
+            throw Error(CIPC_ERR_NO_DEVICE, std::string("device is ") + prop.gcnArchName + ", this library is built for gfx950");
+        auto* c = new cipc_gpu_context();
+        c->device = device;
+        CIPC_HIP(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+        *out = c;
+    });
+}
+
+int cipc_gpu_context_destroy(cipc_gpu_context* c) {
+    return guard([&] {
+        if (!c) return;
+        (void)hipSetDevice(c->device);
+        (void)hipStreamSynchronize(c->stream);
+        (void)hipStreamDestroy(c->stream);
+        delete c;  // DBuf members free on destruction of their owner only when released:
+    });
+}
+
+int cipc_gpu_upload_scene(cipc_gpu_context* c, const cipc_scene_desc* d) {
+    return guard([&] {
+        require(c && d, "null context or descriptor");
+        require(d->V > 0 && d->T >= 0 && d->F >= 0 && d->Vs >= 0 && d->Es >= 0, "invalid scene counts");
+        require(d->x && d->v_rest && d->V_prev && d->xtilde && d->masses && d->boundary, "missing vertex arrays");
+        require(d->T == 0 || (d->tets && d->dm_inv && d->volumes), "missing tet arrays");
+        require(d->F == 0 || d->faces, "missing surface faces");
+        require(d->Vs == 0 || d->surf_verts, "missing surface vertices");
+        require(d->Es == 0 || d->surf_edges, "missing surface edges");
+        require(d->hhat > 0 && d->dt > 0, "dHat and dt must be positive");
+        bind(*c);
+        Scene& s = c->sc;
+        const int V = d->V, T = d->T;
+        for (int i = 0; i < 4 * T; ++i)
+            if (d->tets[i] < 0 || d->tets[i] >= V) throw Error(CIPC_ERR_INVALID_ARGUMENT, "tet vertex index out of range");
+        for (int i = 0; i < d->F; ++i)
+            for (int k = 0; k < 3; ++k)
+                if (d->faces[4 * i + k] < 0 || d->faces[4 * i + k] >= V)
+                    throw Error(CIPC_ERR_INVALID_ARGUMENT, "surface face index out of range");
+        for (int i = 0; i < d->Vs; ++i)
+            if (d->surf_verts[i] >= (uint64_t)V) throw Error(CIPC_ERR_INVALID_ARGUMENT, "surface vertex index out of range");
+        for (int i = 0; i < 2 * d->Es; ++i)
+            if (d->surf_edges[i] >= (uint64_t)V) throw Error(CIPC_ERR_INVALID_ARGUMENT, "surface edge index out of range");
+        s.V = V; s.T = T; s.F = d->F; s.Vs = d->Vs; s.Es = d->Es;
+        s.dt = d->dt; s.u = d->length_rate; s.r = d->volume_rate; s.drag = d->drag; s.hhat = d->hhat; s.fhat = d->fhat;
+        s.fric = d->friction;
+        s.gn = d3{d->ground_normal[0], d->ground_normal[1], d->ground_normal[2]};
+        s.gd = d->ground_d;
+        const hipStream_t st = c->stream;
+        upload(s.x, d->x, 3 * (size_t)V, st);
+        upload(s.vrest, d->v_rest, 3 * (size_t)V, st);
+        upload(s.vprev, d->V_prev, 3 * (size_t)V, st);
+        upload(s.xt, d->xtilde, 3 * (size_t)V, st);
+        upload(s.mass, d->masses, (size_t)V, st);
+        upload(s.bt, d->boundary, (size_t)V, st);
+        s.x0.resize(3 * (size_t)V);
+        s.dir.resize(3 * (size_t)V);
+        CIPC_HIP(hipMemsetAsync(s.dir.p, 0, 3 * sizeof(double) * V, st));
+        s.h_tets.assign(d->tets, d->tets + 4 * (size_t)T);
+        s.h_bt.assign(d->boundary, d->boundary + V);
+        upload(s.tets, reinterpret_cast<const int4*>(d->tets), (size_t)T, st);
+        // Dm^-1 to SoA: entry k of tet t at k*T + t
+        std::vector<double> soa(9 * (size_t)std::max(T, 1));
+        for (int t = 0; t < T; ++t)
+            for (int k = 0; k < 9; ++k) soa[(size_t)k * T + t] = d->dm_inv[9 * (size_t)t + k];
+        upload(s.dminv, soa.data(), 9 * (size_t)T, st);
+        upload(s.vol, d->volumes, (size_t)T, st);
+        upload(s.faces, reinterpret_cast<const int4*>(d->faces), (size_t)d->F, st);
+        std::vector<int> sv(d->Vs), se(2 * (size_t)d->Es);
+        for (int i = 0; i < d->Vs; ++i) sv[i] = (int)d->surf_verts[i];
+        for (int i = 0; i < 2 * d->Es; ++i) se[i] = (int)d->surf_edges[i];
+        upload(s.sv, sv.data(), sv.size(), st);
+        upload(s.se, reinterpret_cast<const int2*>(se.data()), (size_t)d->Es, st);
+        CIPC_HIP(hipStreamSynchronize(st));
+        build_static_pattern(*c);
+        c->n_self = c->n_ee = c->n_ccd = c->n_env = 0;
+        c->self_set.resize(1); c->ee_set.resize(1); c->ee_ie.resize(1); c->ccd_set.resize(1); c->env_set.resize(1);
+        c->have_scene = true;
+    });
+}
+
+int cipc_gpu_set_state(cipc_gpu_context* c, const double* x, const double* vprev, const double* xt) {
+    return guard([&] {
+        require(c && c->have_scene, "no scene uploaded");
+        bind(*c);
+        const size_t n = 3 * (size_t)c->sc.V * sizeof(double);
+        if (x) CIPC_HIP(hipMemcpyAsync(c->sc.x.p, x, n, hipMemcpyHostToDevice, c->stream));
+        if (vprev) CIPC_HIP(hipMemcpyAsync(c->sc.vprev.p, vprev, n, hipMemcpyHostToDevice, c->stream));
+        if (xt) CIPC_HIP(hipMemcpyAsync(c->sc.xt.p, xt, n, hipMemcpyHostToDevice, c->stream));
+        CIPC_HIP(hipStreamSynchronize(c->stream));
+    });
+}
+
+int cipc_gpu_get_positions(cipc_gpu_context* c, double* x) {
+    return guard([&] {
+        require(c && c->have_scene && x, "no scene uploaded or null output");
+        bind(*c);
+        CIPC_HIP(hipMemcpyAsync(x, c->sc.x.p, 3 * sizeof(double) * c->sc.V, hipMemcpyDeviceToHost, c->stream));
+        CIPC_HIP(hipStreamSynchronize(c->stream));
+    });
+}
+
+int cipc_gpu_set_direction(cipc_gpu_context* c, const double* p) {
+    return guard([&] {
+        require(c && c->have_scene && p, "no scene uploaded or null direction");
+        bind(*c);
+        CIPC_HIP(hipMemcpyAsync(c->sc.dir.p, p, 3 * sizeof(double) * c->sc.V, hipMemcpyHostToDevice, c->stream));
+        CIPC_HIP(hipStreamSynchronize(c->stream));
+    });
+}
+
+int cipc_gpu_active_set(cipc_gpu_context* c, int64_t counts[4]) {
+    return guard([&] {
+        require(c && c->have_scene, "no scene uploaded");
+        bind(*c);
+        compute_active_set(*c);
+        if (counts) { counts[0] = c->n_self; counts[1] = c->n_ee; counts[2] = c->n_ccd; counts[3] = c->n_env; }
+    });
+}
+
+int cipc_gpu_get_active_set(cipc_gpu_context* c, int* self4, int* ee4, int* eeie2, int* ccd2, int* env) {
+    return guard([&] {
+        require(c && c->have_scene, "no scene uploaded");
+        bind(*c);
+        const hipStream_t st = c->stream;
+        if (self4 && c->n_self) CIPC_HIP(hipMemcpyAsync(self4, c->self_set.p, 16 * c->n_self, hipMemcpyDeviceToHost, st));
+        if (ee4 && c->n_ee) CIPC_HIP(hipMemcpyAsync(ee4, c->ee_set.p, 16 * c->n_ee, hipMemcpyDeviceToHost, st));
+        if (eeie2 && c->n_ee) CIPC_HIP(hipMemcpyAsync(eeie2, c->ee_ie.p, 8 * c->n_ee, hipMemcpyDeviceToHost, st));
+        if (ccd2 && c->n_ccd) CIPC_HIP(hipMemcpyAsync(ccd2, c->ccd_set.p, 8 * c->n_ccd, hipMemcpyDeviceToHost, st));
+        if (env && c->n_env) CIPC_HIP(hipMemcpyAsync(env, c->env_set.p, 4 * c->n_env, hipMemcpyDeviceToHost, st));
+        CIPC_HIP(hipStreamSynchronize(st));
+    });
+}
+
+int cipc_gpu_set_active_set(cipc_gpu_context* c, const int64_t n[4], const int* self4, const int* ee4, const int* eeie2,
+                            const int* ccd2, const int* env) {
+    return guard([&] {
+        require(c && c->have_scene && n, "no scene uploaded or null counts");
+        require(n[0] >= 0 && n[1] >= 0 && n[2] >= 0 && n[3] >= 0, "negative active-set count");
+        require((n[0] == 0 || self4) && (n[1] == 0 || (ee4 && eeie2)) && (n[2] == 0 || ccd2) && (n[3] == 0 || env),
+                "missing active-set array");
+        const int V = c->sc.V, Es = c->sc.Es, F = c->sc.F, Vs = c->sc.Vs;
+        for (int64_t i = 0; i < n[0]; ++i) {
+            const int* e = self4 + 4 * i;
+            const int v0 = e[0] >= 0 ? e[0] : -e[0] - 1;
+            bool ok = v0 < V && e[1] >= 0 && e[1] < V && e[2] < V && e[3] < V;
+            if (e[0] >= 0) ok = ok && e[2] >= 0 && e[3] >= 0;
+            if (!ok) throw Error(CIPC_ERR_INVALID_ARGUMENT, "Self_ActiveSet entry out of range");
+        }
+        for (int64_t i = 0; i < n[1]; ++i) {
+            const int* e = ee4 + 4 * i;
+            const int v0 = e[0] >= 0 ? e[0] : -e[0] - 1;
+            bool ok = v0 < V && e[1] >= 0 && e[1] < V && e[2] < V && e[3] < V;
+            if (e[3] < 0) ok = ok && eeie2[2 * i] >= 0 && eeie2[2 * i] < Es && eeie2[2 * i + 1] >= 0 && eeie2[2 * i + 1] < Es;
+            if (!ok) throw Error(CIPC_ERR_INVALID_ARGUMENT, "Self_EE_ActiveSet entry out of range");
+        }
+        for (int64_t i = 0; i < n[2]; ++i) {
+            const int a = ccd2[2 * i], b = ccd2[2 * i + 1];
+            const bool ok = a < 0 ? (-a - 1 < Vs && b >= 0 && b < F) : (a < Es && b >= 0 && b < Es);
+            if (!ok) throw Error(CIPC_ERR_INVALID_ARGUMENT, "Self_CCD_ActiveSet entry out of range");
+        }
+        for (int64_t i = 0; i < n[3]; ++i)
+            if (env[i] < 0 || env[i] >= V) throw Error(CIPC_ERR_INVALID_ARGUMENT, "Environment_ActiveSet entry out of range");
+        bind(*c);
+        const hipStream_t st = c->stream;
+        c->n_self = n[0]; c->n_ee = n[1]; c->n_ccd = n[2]; c->n_env = n[3];
+        upload(c->self_set, reinterpret_cast<const int4*>(self4), (size_t)n[0], st);
+        upload(c->ee_set, reinterpret_cast<const int4*>(ee4), (size_t)n[1], st);
+        upload(c->ee_ie, reinterpret_cast<const int2*>(eeie2), (size_t)n[1], st);
+        upload(c->ccd_set, reinterpret_cast<const int2*>(ccd2), (size_t)n[2], st);
+        upload(c->env_set, env, (size_t)n[3], st);
+        CIPC_HIP(hipStreamSynchronize(st));
+    });
+}
+
+static void assemble_impl(cipc_gpu_context* c, double kappa) {
+    elastic_assemble(*c);
+    contact_assemble(*c, kappa);
+    finalize_csr(*c);
+    // collision count as compute_g_dpt returns it: multiplicity of point-point/point-edge
+    // entries, one per point-triangle/edge-edge entry (ContactMechanics.cpp:2970-3032)
+    c->collisions = 0;
+    if (c->n_self) {
+        std::vector<int4> h(c->n_self);
+        CIPC_HIP(hipMemcpyAsync(h.data(), c->self_set.p, 16 * c->n_self, hipMemcpyDeviceToHost, c->stream));
+        CIPC_HIP(hipStreamSynchronize(c->stream));
+        long long n = 0;
+        for (const int4& e : h) n += (e.x < 0 && (e.z < 0 || e.w < 0)) ? -e.w : 1;
+        c->collisions = (int)n;
+    }
+    CIPC_HIP(hipStreamSynchronize(c->stream));
+}
+
+int cipc_gpu_assemble(cipc_gpu_context* c, double kappa, int64_t* nnz, int* collisions) {
+    return guard([&] {
+        require(c && c->have_scene, "no scene uploaded");
+        bind(*c);
+        assemble_impl(c, kappa);
+        if (nnz) *nnz = c->nnz;
+        if (collisions) *collisions = c->collisions;
+    });
+}
+
+int cipc_gpu_get_gradient(cipc_gpu_context* c, double* g) {
+    return guard([&] {
+        require(c && c->have_scene && g, "no scene uploaded or null output");
+        bind(*c);
+        CIPC_HIP(hipMemcpyAsync(g, c->grad.p, 3 * sizeof(double) * c->sc.V, hipMemcpyDeviceToHost, c->stream));
+        CIPC_HIP(hipStreamSynchronize(c->stream));
+    });
+}
+
+int cipc_gpu_get_csr(cipc_gpu_context* c, int* outer, int* inner, double* values, double* rhs) {
+    return guard([&] {
+        require(c && c->have_scene, "no scene uploaded");
+        bind(*c);
+        const hipStream_t st = c->stream;
+        const size_t n = 3 * (size_t)c->sc.V;
+        if (outer) CIPC_HIP(hipMemcpyAsync(outer, c->outer.p, (n + 1) * sizeof(int), hipMemcpyDeviceToHost, st));
+        if (inner && c->nnz) CIPC_HIP(hipMemcpyAsync(inner, c->inner.p, c->nnz * sizeof(int), hipMemcpyDeviceToHost, st));
+        if (values && c->nnz) CIPC_HIP(hipMemcpyAsync(values, c->val.p, c->nnz * sizeof(double), hipMemcpyDeviceToHost, st));
+        if (rhs) CIPC_HIP(hipMemcpyAsync(rhs, c->rhs.p, n * sizeof(double), hipMemcpyDeviceToHost, st));
+        CIPC_HIP(hipStreamSynchronize(st));
+    });
+}
+
+int cipc_gpu_energy_terms(cipc_gpu_context* c, double kappa, double terms[5]) {
+    return guard([&] {
+        require(c && c->have_scene && terms, "no scene uploaded or null output");
+        bind(*c);
+        energy_terms(*c, kappa, terms);
+    });
+}
+
+int cipc_gpu_energy(cipc_gpu_context* c, double kappa, double* e) {
+    return guard([&] {
+        require(c && c->have_scene && e, "no scene uploaded or null output");
+        bind(*c);
+        double t[5];
+        energy_terms(*c, kappa, t);
+        double E = t[0];
+        E *= c->sc.dt * c->sc.dt;
+        E += t[1];
+        E += kappa * (t[2] + t[3] + t[4]);
+        *e = E;
+    });
+}
+
+int cipc_gpu_step_bound(cipc_gpu_context* c, double out[5]) {
+    return guard([&] {
+        require(c && c->have_scene && out, "no scene uploaded or null output");
+        bind(*c);
+        step_bound(*c, out);
+    });
+}
+
+int cipc_gpu_save_positions(cipc_gpu_context* c) {
+    return guard([&] {
+        require(c && c->have_scene, "no scene uploaded");
+        bind(*c);
+        CIPC_HIP(hipMemcpyAsync(c->sc.x0.p, c->sc.x.p, 3 * sizeof(double) * c->sc.V, hipMemcpyDeviceToDevice, c->stream));
+        CIPC_HIP(hipStreamSynchronize(c->stream));
+    });
+}
+
+int cipc_gpu_step_forward(cipc_gpu_context* c, double alpha) {
+    return guard([&] {
+        require(c && c->have_scene, "no scene uploaded");
+        bind(*c);
+        step_forward(*c, alpha);
+        CIPC_HIP(hipStreamSynchronize(c->stream));
+    });
+}
+
+int cipc_gpu_is_intersected(cipc_gpu_context* c, int* r) {
+    return guard([&] {
+        require(c && c->have_scene && r, "no scene uploaded or null output");
+        bind(*c);
+        *r = any_intersection(*c) ? 1 : 0;
+    });
+}
+
+int cipc_gpu_newton_iteration(cipc_gpu_context* c, double kappa, cipc_iteration_timing* t) {
+    return guard([&] {
+        require(c && c->have_scene && t, "no scene uploaded or null output");
+        bind(*c);
+        hipEvent_t ev[5], ee;
+        for (auto& e : ev) CIPC_HIP(hipEventCreate(&e));
+        CIPC_HIP(hipEventCreate(&ee));
+        CIPC_HIP(hipEventRecord(ev[0], c->stream));
+        compute_active_set(*c);
+        CIPC_HIP(hipEventRecord(ev[1], c->stream));
+        elastic_assemble(*c);
+        CIPC_HIP(hipEventRecord(ee, c->stream));
+        contact_assemble(*c, kappa);
+        finalize_csr(*c);
+        CIPC_HIP(hipEventRecord(ev[2], c->stream));
+        double sb[5];
+        step_bound(*c, sb);
+        CIPC_HIP(hipEventRecord(ev[3], c->stream));
+        double et[5];
+        energy_terms(*c, kappa, et);
+        CIPC_HIP(hipEventRecord(ev[4], c->stream));
+        CIPC_HIP(hipEventSynchronize(ev[4]));
+        float ms[4];
+        for (int i = 0; i < 4; ++i) CIPC_HIP(hipEventElapsedTime(&ms[i], ev[i], ev[i + 1]));
+        float tot = 0, el = 0;
+        CIPC_HIP(hipEventElapsedTime(&tot, ev[0], ev[4]));
+        CIPC_HIP(hipEventElapsedTime(&el, ev[1], ee));
+        for (auto& e : ev) (void)hipEventDestroy(e);
+        (void)hipEventDestroy(ee);
+        t->broadphase_ms = ms[0]; t->assembly_ms = ms[1]; t->elastic_ms = el; t->ccd_ms = ms[2]; t->energy_ms = ms[3];
+        t->total_ms = tot;
+        t->alpha = sb[0];
+        double E = et[0] * (c->sc.dt * c->sc.dt) + et[1] + kappa * (et[2] + et[3] + et[4]);
+        t->energy = E;
+        t->nnz = c->nnz;
+        t->contacts = c->n_self + c->n_ee + c->n_env;
+    });
+}
+
+int cipc_gpu_eval_snh(int n, const double* F, double u, double r, double* P, double* H, double* e) {
+    return guard([&] { require(n >= 0 && (n == 0 || (F && P && H && e)), "invalid arguments"); if (n) cipc_gpu_eval_snh_impl(n, F, u, r, P, H, e); });
+}
+int cipc_gpu_eval_distance(int n, int kind, const double* pts, double* d, double* g, double* H) {
+    return guard([&] {
+        require(n >= 0 && kind >= 0 && kind <= 4 && (n == 0 || (pts && d && g && H)), "invalid arguments");
+        if (n) cipc_gpu_eval_distance_impl(n, kind, pts, d, g, H);
+    });
+}
+int cipc_gpu_eval_make_pd(int n, int dim, double* H) {
+    return guard([&] {
+        require(n >= 0 && (dim == 6 || dim == 9 || dim == 12) && (n == 0 || H), "invalid arguments");
+        if (n) cipc_gpu_eval_make_pd_impl(n, dim, H);
+    });
+}
+int cipc_gpu_eval_accd(int n, int kind, const double* pts, const double* dirs, double eta, double th, double* toc) {
+    return guard([&] {
+        require(n >= 0 && (kind == 0 || kind == 1) && (n == 0 || (pts && dirs && toc)), "invalid arguments");
+        if (n) cipc_gpu_eval_accd_impl(n, kind, pts, dirs, eta, th, toc);
+    });
+}
+int cipc_gpu_eval_classify(int n, int kind, const double* pts, int* type) {
+    return guard([&] {
+        require(n >= 0 && (kind == 0 || kind == 1) && (n == 0 || (pts && type)), "invalid arguments");
+        if (n) cipc_gpu_eval_classify_impl(n, kind, pts, type);
+    });
+}
+
+}  // extern "C"

@@ -1,0 +1,120 @@
+// Device context: one scene, one stream, all device memory.  Each translation unit
+// implements one stage of the Newton iteration on it.
+#pragma once
+#include "common.hpp"
+
+namespace cipc {
+
+// Linear BVH in the Karras layout: internal nodes [0, n-2], leaves [n-1, 2n-2] in Morton
+// order.  Traversal is stackless: `left` is the first child of an internal node (or the
+// primitive id of a leaf) and `skip` the next node in depth-first order once a subtree
+// is done (-1 at the end).
+struct LBVH {
+    int n = 0;
+    DBuf<Box> prim;            // primitive boxes (input)
+    DBuf<Box> node_box;        // 2n-1
+    DBuf<int> left;            // 2n-1
+    DBuf<int> right;           // n-1 (internal nodes)
+    DBuf<int> skip;            // 2n-1
+    DBuf<int> parent;          // 2n-1
+    DBuf<int> range_last;      // n-1
+    DBuf<int> split_right;     // n-1: right child of the internal node whose split is s
+    DBuf<int> flag;            // n-1 bottom-up visit counters
+    DBuf<unsigned long long> keys, keys_sorted;
+    DBuf<Box> scene_partial;
+    bool swept = false;
+    double alpha = 0.0;
+};
+
+// Scratch of the active-set construction (broadphase.hip).
+struct ActiveSetScratch {
+    DBuf<int> cntPT, offPT, cntEE, offEE;
+    DBuf<int> candPT, candEE, ownPT, ownEE, catPT, catEE;
+    DBuf<int4> recPT, recEE;
+    DBuf<int2> auxEE;
+    DBuf<int> f1, p1, f2, p2, f3, p3, f4, p4, f5, p5, fg, pg;
+    DBuf<unsigned long long> mkeys, mkeys_sorted, ukeys;
+    DBuf<int> ucounts, nruns;
+};
+
+struct Context {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    Scene sc;
+    StaticPattern sp;
+    bool have_scene = false;
+
+    // elastic staging (SoA: entry e of tet t at e*T + t)
+    DBuf<double> stagH, stagG;
+    DBuf<double> sbv;          // static block values, 9 per block
+    DBuf<unsigned short> sbf;  // static block nonzero-contribution masks
+    DBuf<double> grad;         // 3V
+
+    // active set
+    int64_t n_self = 0, n_ee = 0, n_ccd = 0, n_env = 0;
+    DBuf<int4> self_set, ee_set;
+    DBuf<int2> ee_ie, ccd_set;
+    DBuf<int> env_set;
+
+    // contact contributions (fixed slots, invalid key = ~0ull)
+    DBuf<unsigned long long> ckey, ckey_sorted;
+    DBuf<int> cidx, cidx_sorted;
+    DBuf<double> cval;         // 9 per record
+    DBuf<int> cdims;           // per contact: local Hessian dimension
+    DBuf<int4> cverts;         // per contact: vertices (-1 past the last)
+    DBuf<double> chess;        // per contact: 78 upper-triangle Hessian + 12 gradient entries (SoA)
+    DBuf<int> gkey, gkey_sorted, gidx, gidx_sorted;
+    DBuf<double> gval;         // 3 per record
+    DBuf<unsigned long long> ukey;  // unique contact block keys
+    DBuf<double> uval;
+    DBuf<unsigned short> uflag;
+    int64_t n_ublocks = 0;
+    DBuf<int> counters;        // small scratch integers
+
+    // merged block pattern and output CSR
+    DBuf<int> mptr, mrow;
+    DBuf<double> mval;
+    DBuf<unsigned short> mflag;
+    DBuf<int> outer, inner;
+    DBuf<double> val, rhs;
+    int64_t nnz = 0;
+    int collisions = 0;
+
+    // broad phase
+    LBVH face_tree, edge_tree;
+    ActiveSetScratch as;
+    DBuf<Box> swept_point_box;
+
+    // generic scratch
+    DBuf<unsigned char> tmp;
+    DBuf<int> iscratch, iscratch2;
+    DBuf<double> dscratch;
+    DBuf<unsigned long long> u64scratch, u64scratch2;
+    DBuf<int4> i4scratch;
+};
+
+// elastic.hip
+void build_static_pattern(Context& c);
+void elastic_assemble(Context& c);
+// contact.hip
+void contact_assemble(Context& c, double kappa);
+// sparse.hip
+void finalize_csr(Context& c);
+void exclusive_scan_int(Context& c, const int* in, int* out, int n);
+// broadphase.hip
+void build_tree(Context& c, LBVH& t, int n);
+void build_static_trees(Context& c);
+void build_swept_trees(Context& c, double alpha);
+void compute_active_set(Context& c);
+void sort_u64(Context& c, const unsigned long long* in, unsigned long long* out, int n, int bits);
+void sort_u64_pairs(Context& c, const unsigned long long* kin, unsigned long long* kout,
+                    const int* vin, int* vout, int n, int bits);
+void sort_i32_pairs(Context& c, const int* kin, int* kout, const int* vin, int* vout, int n, int bits);
+bool any_intersection(Context& c);
+// ccd.hip
+void step_bound(Context& c, double out[5]);
+// energy.hip
+void energy_terms(Context& c, double kappa, double out[5]);
+void step_forward(Context& c, double alpha);
+
+}  // namespace cipc

@@ -1,0 +1,263 @@
+// Sparse assembly: contact records + static elastic blocks -> the reference's CSR.
+//
+// Reference: BHessian::toTriplets (CPU IPC/ContactMechanics.cpp:13-134) and
+// NewtonLinearSystem::assemble (NewtonLinearSystem.cpp:126-154): lower-triangle entries
+// (row >= col) of blocks whose two vertices are free, entries exactly equal to zero
+// dropped before duplicates are summed, the mass diagonal on every DOF, Eigen's
+// column-major lower storage (outer = column starts, inner = ascending rows), and the
+// right-hand side zeroed on non-free vertices.
+//
+// Pipeline: sort contact block records by (column vertex, row vertex), sum equal keys
+// in record order; merge them per block column with the static pattern; expand the
+// merged 3x3 blocks to scalar entries keeping exactly those that received a nonzero
+// contribution.
+#include "context.hpp"
+
+#include <hipcub/hipcub.hpp>
+
+namespace cipc {
+
+constexpr unsigned long long kNoKey = ~0ull;
+
+void sort_u64_pairs(Context& c, const unsigned long long* kin, unsigned long long* kout, const int* vin, int* vout,
+                    int n, int bits) {
+    if (n <= 0) return;
+    size_t bytes = 0;
+    CIPC_HIP(hipcub::DeviceRadixSort::SortPairs(nullptr, bytes, kin, kout, vin, vout, n, 0, bits, c.stream));
+    c.tmp.reserve(bytes);
+    CIPC_HIP(hipcub::DeviceRadixSort::SortPairs(c.tmp.p, bytes, kin, kout, vin, vout, n, 0, bits, c.stream));
+}
+void sort_u64(Context& c, const unsigned long long* kin, unsigned long long* kout, int n, int bits) {
+    if (n <= 0) return;
+    size_t bytes = 0;
+    CIPC_HIP(hipcub::DeviceRadixSort::SortKeys(nullptr, bytes, kin, kout, n, 0, bits, c.stream));
+    c.tmp.reserve(bytes);
+    CIPC_HIP(hipcub::DeviceRadixSort::SortKeys(c.tmp.p, bytes, kin, kout, n, 0, bits, c.stream));
+}
+void sort_i32_pairs(Context& c, const int* kin, int* kout, const int* vin, int* vout, int n, int bits) {
+    if (n <= 0) return;
+    size_t bytes = 0;
+    CIPC_HIP(hipcub::DeviceRadixSort::SortPairs(nullptr, bytes, kin, kout, vin, vout, n, 0, bits, c.stream));
+    c.tmp.reserve(bytes);
+    CIPC_HIP(hipcub::DeviceRadixSort::SortPairs(c.tmp.p, bytes, kin, kout, vin, vout, n, 0, bits, c.stream));
+}
+// out[i] = sum in[0..i-1]; out has n+1 entries when in has n (out[n] = total).
+__global__ void set_zero_tail(int* p, int n) { p[n] = 0; }
+void exclusive_scan_int(Context& c, const int* in, int* out, int n) {
+    size_t bytes = 0;
+    CIPC_HIP(hipcub::DeviceScan::ExclusiveSum(nullptr, bytes, in, out, n, c.stream));
+    c.tmp.reserve(bytes);
+    CIPC_HIP(hipcub::DeviceScan::ExclusiveSum(c.tmp.p, bytes, in, out, n, c.stream));
+}
+
+// --- contact block records: heads of equal-key runs, their sums and nonzero masks -----
+__global__ void run_heads_u64(int n, const unsigned long long* __restrict__ k, int* __restrict__ head) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i > n) return;
+    if (i == n) { head[n] = 0; return; }
+    head[i] = (k[i] != kNoKey && (i == 0 || k[i] != k[i - 1])) ? 1 : 0;
+}
+__global__ void reduce_blocks(int n, const unsigned long long* __restrict__ k, const int* __restrict__ idx,
+                              const int* __restrict__ head, const int* __restrict__ pos, const double* __restrict__ val,
+                              unsigned long long* __restrict__ ukey, double* __restrict__ uval, unsigned short* __restrict__ uflag) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n || !head[i]) return;
+    const unsigned long long key = k[i];
+    double acc[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+    unsigned mask = 0;
+    for (int j = i; j < n && k[j] == key; ++j) {
+        const double* v = val + (size_t)9 * idx[j];
+#pragma unroll
+        for (int e = 0; e < 9; ++e)
+            if (v[e] != 0.0) { acc[e] += v[e]; mask |= 1u << e; }
+    }
+    const int o = pos[i];
+    ukey[o] = key;
+#pragma unroll
+    for (int e = 0; e < 9; ++e) uval[(size_t)9 * o + e] = acc[e];
+    uflag[o] = (unsigned short)mask;
+}
+// --- contact gradient records, summed per vertex in record order ------------------
+__global__ void add_gradient_records(int n, const int* __restrict__ k, const int* __restrict__ idx,
+                                     const double* __restrict__ val, int V, double* __restrict__ grad) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int v = k[i];
+    if (v < 0 || v >= V || (i > 0 && k[i - 1] == v)) return;
+    double g0 = grad[3 * v], g1 = grad[3 * v + 1], g2 = grad[3 * v + 2];
+    for (int j = i; j < n && k[j] == v; ++j) {
+        const double* p = val + (size_t)3 * idx[j];
+        g0 += p[0]; g1 += p[1]; g2 += p[2];
+    }
+    grad[3 * v] = g0; grad[3 * v + 1] = g1; grad[3 * v + 2] = g2;
+}
+
+// --- merge static and contact blocks per block column --------------------------------
+__device__ inline int lower_bound_u64(const unsigned long long* a, int n, unsigned long long key) {
+    int lo = 0, hi = n;
+    while (lo < hi) { const int m = (lo + hi) >> 1; if (a[m] < key) lo = m + 1; else hi = m; }
+    return lo;
+}
+__global__ void merge_count(int V, const int* __restrict__ scol, const int* __restrict__ srow,
+                            const unsigned long long* __restrict__ ukey, const int* __restrict__ nu_ptr, int* __restrict__ cnt) {
+    const int b = blockIdx.x * blockDim.x + threadIdx.x;
+    if (b >= V) return;
+    const int nu = *nu_ptr;
+    int i = scol[b], ie = scol[b + 1];
+    int j = lower_bound_u64(ukey, nu, (unsigned long long)b << 32);
+    const int je = lower_bound_u64(ukey, nu, (unsigned long long)(b + 1) << 32);
+    int n = 0;
+    while (i < ie || j < je) {
+        const int ra = i < ie ? srow[i] : 0x7fffffff;
+        const int rb = j < je ? (int)(ukey[j] & 0xffffffffull) : 0x7fffffff;
+        if (ra == rb) { ++i; ++j; } else if (ra < rb) ++i; else ++j;
+        ++n;
+    }
+    cnt[b] = n;
+}
+__global__ void merge_fill(int V, const int* __restrict__ scol, const int* __restrict__ srow, const double* __restrict__ sbv,
+                           const unsigned short* __restrict__ sbf, const unsigned long long* __restrict__ ukey,
+                           const double* __restrict__ uval, const unsigned short* __restrict__ uflag, const int* __restrict__ nu_ptr,
+                           const int* __restrict__ mptr, int* __restrict__ mrow, double* __restrict__ mval, unsigned short* __restrict__ mflag) {
+    const int b = blockIdx.x * blockDim.x + threadIdx.x;
+    if (b >= V) return;
+    const int nu = *nu_ptr;
+    int i = scol[b], ie = scol[b + 1];
+    int j = lower_bound_u64(ukey, nu, (unsigned long long)b << 32);
+    const int je = lower_bound_u64(ukey, nu, (unsigned long long)(b + 1) << 32);
+    int o = mptr[b];
+    while (i < ie || j < je) {
+        const int ra = i < ie ? srow[i] : 0x7fffffff;
+        const int rb = j < je ? (int)(ukey[j] & 0xffffffffull) : 0x7fffffff;
+        double v[9];
+        unsigned f = 0;
+        if (ra <= rb) {
+#pragma unroll
+            for (int e = 0; e < 9; ++e) v[e] = sbv[(size_t)9 * i + e];
+            f = sbf[i];
+        } else {
+#pragma unroll
+            for (int e = 0; e < 9; ++e) v[e] = 0.0;
+        }
+        if (rb <= ra) {
+#pragma unroll
+            for (int e = 0; e < 9; ++e) v[e] += uval[(size_t)9 * j + e];
+            f |= uflag[j];
+        }
+        mrow[o] = ra < rb ? ra : rb;
+#pragma unroll
+        for (int e = 0; e < 9; ++e) mval[(size_t)9 * o + e] = v[e];
+        mflag[o] = (unsigned short)f;
+        ++o;
+        if (ra == rb) { ++i; ++j; } else if (ra < rb) ++i; else ++j;
+    }
+}
+// --- scalar expansion: DOF column j = 3b + cc ------------------------------------------
+__global__ void scalar_count(int V, const int* __restrict__ mptr, const int* __restrict__ mrow,
+                             const unsigned short* __restrict__ mflag, int* __restrict__ cnt) {
+    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= 3 * V) return;
+    const int b = j / 3, cc = j % 3;
+    int n = 0;
+    for (int k = mptr[b]; k < mptr[b + 1]; ++k) {
+        const unsigned f = mflag[k];
+        const int r0 = (mrow[k] == b) ? cc : 0;
+        for (int r = r0; r < 3; ++r) n += (f >> (3 * r + cc)) & 1u;
+    }
+    cnt[j] = n;
+}
+__global__ void scalar_fill(int V, const int* __restrict__ mptr, const int* __restrict__ mrow, const double* __restrict__ mval,
+                            const unsigned short* __restrict__ mflag, const int* __restrict__ outer,
+                            int* __restrict__ inner, double* __restrict__ val) {
+    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= 3 * V) return;
+    const int b = j / 3, cc = j % 3;
+    int o = outer[j];
+    for (int k = mptr[b]; k < mptr[b + 1]; ++k) {
+        const unsigned f = mflag[k];
+        const int a = mrow[k];
+        const int r0 = (a == b) ? cc : 0;
+        for (int r = r0; r < 3; ++r) {
+            if ((f >> (3 * r + cc)) & 1u) {
+                inner[o] = 3 * a + r;
+                val[o] = mval[(size_t)9 * k + 3 * r + cc];
+                ++o;
+            }
+        }
+    }
+}
+__global__ void masked_rhs(int V, const int* __restrict__ bt, const double* __restrict__ grad, double* __restrict__ rhs) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= 3 * V) return;
+    rhs[i] = is_free(bt[i / 3]) ? grad[i] : 0.0;
+}
+
+void finalize_csr(Context& c) {
+    Scene& s = c.sc;
+    const int V = s.V;
+    // contact block records
+    const int nb = (int)c.ckey.n;
+    c.ckey_sorted.resize(std::max(nb, 1)); c.cidx_sorted.resize(std::max(nb, 1));
+    c.iscratch2.resize(std::max<size_t>((size_t)3 * V + 1, (size_t)nb + 2));
+    c.counters.resize(4);
+    c.ukey.resize(std::max(nb, 1)); c.uval.resize((size_t)9 * std::max(nb, 1)); c.uflag.resize(std::max(nb, 1));
+    int* head = c.iscratch2.p;
+    int* pos = nullptr;
+    c.iscratch.resize((size_t)std::max(nb, 1) + 1);
+    pos = c.iscratch.p;
+    if (nb > 0) {
+        sort_u64_pairs(c, c.ckey.p, c.ckey_sorted.p, c.cidx.p, c.cidx_sorted.p, nb, 64);
+        hipLaunchKernelGGL(run_heads_u64, dim3(blocks_for(nb + 1, 256)), dim3(256), 0, c.stream, nb, c.ckey_sorted.p, head);
+        CIPC_CHECK_LAUNCH();
+        exclusive_scan_int(c, head, pos, nb + 1);
+        hipLaunchKernelGGL(reduce_blocks, dim3(blocks_for(nb, 256)), dim3(256), 0, c.stream, nb, c.ckey_sorted.p,
+                           c.cidx_sorted.p, head, pos, c.cval.p, c.ukey.p, c.uval.p, c.uflag.p);
+        CIPC_CHECK_LAUNCH();
+        CIPC_HIP(hipMemcpyAsync(c.counters.p, pos + nb, sizeof(int), hipMemcpyDeviceToDevice, c.stream));
+    } else {
+        CIPC_HIP(hipMemsetAsync(c.counters.p, 0, sizeof(int), c.stream));
+    }
+    // contact gradient records
+    const int ng = (int)c.gkey.n;
+    if (ng > 0) {
+        c.gkey_sorted.resize(ng); c.gidx_sorted.resize(ng);
+        sort_i32_pairs(c, c.gkey.p, c.gkey_sorted.p, c.gidx.p, c.gidx_sorted.p, ng, 32);
+        hipLaunchKernelGGL(add_gradient_records, dim3(blocks_for(ng, 256)), dim3(256), 0, c.stream, ng, c.gkey_sorted.p,
+                           c.gidx_sorted.p, c.gval.p, V, c.grad.p);
+        CIPC_CHECK_LAUNCH();
+    }
+    // merged block pattern
+    c.mptr.resize((size_t)V + 1);
+    DBuf<int>& cnt = c.iscratch2;
+    hipLaunchKernelGGL(merge_count, dim3(blocks_for(V, 256)), dim3(256), 0, c.stream, V, c.sp.col_ptr.p, c.sp.row.p, c.ukey.p,
+                       c.counters.p, cnt.p);
+    CIPC_CHECK_LAUNCH();
+    hipLaunchKernelGGL(set_zero_tail, dim3(1), dim3(1), 0, c.stream, cnt.p, V);
+    exclusive_scan_int(c, cnt.p, c.mptr.p, V + 1);
+    int nmerged = 0;
+    CIPC_HIP(hipMemcpyAsync(&nmerged, c.mptr.p + V, sizeof(int), hipMemcpyDeviceToHost, c.stream));
+    CIPC_HIP(hipStreamSynchronize(c.stream));
+    c.mrow.resize(std::max(nmerged, 1)); c.mval.resize((size_t)9 * std::max(nmerged, 1)); c.mflag.resize(std::max(nmerged, 1));
+    hipLaunchKernelGGL(merge_fill, dim3(blocks_for(V, 256)), dim3(256), 0, c.stream, V, c.sp.col_ptr.p, c.sp.row.p, c.sbv.p,
+                       c.sbf.p, c.ukey.p, c.uval.p, c.uflag.p, c.counters.p, c.mptr.p, c.mrow.p, c.mval.p, c.mflag.p);
+    CIPC_CHECK_LAUNCH();
+    // scalar CSR
+    c.outer.resize((size_t)3 * V + 1);
+    hipLaunchKernelGGL(scalar_count, dim3(blocks_for(3 * V, 256)), dim3(256), 0, c.stream, V, c.mptr.p, c.mrow.p, c.mflag.p, cnt.p);
+    CIPC_CHECK_LAUNCH();
+    hipLaunchKernelGGL(set_zero_tail, dim3(1), dim3(1), 0, c.stream, cnt.p, 3 * V);
+    exclusive_scan_int(c, cnt.p, c.outer.p, 3 * V + 1);
+    int nnz = 0;
+    CIPC_HIP(hipMemcpyAsync(&nnz, c.outer.p + 3 * V, sizeof(int), hipMemcpyDeviceToHost, c.stream));
+    CIPC_HIP(hipStreamSynchronize(c.stream));
+    c.nnz = nnz;
+    c.inner.resize(std::max(nnz, 1)); c.val.resize(std::max(nnz, 1));
+    hipLaunchKernelGGL(scalar_fill, dim3(blocks_for(3 * V, 256)), dim3(256), 0, c.stream, V, c.mptr.p, c.mrow.p, c.mval.p,
+                       c.mflag.p, c.outer.p, c.inner.p, c.val.p);
+    CIPC_CHECK_LAUNCH();
+    c.rhs.resize((size_t)3 * V);
+    hipLaunchKernelGGL(masked_rhs, dim3(blocks_for(3 * V, 256)), dim3(256), 0, c.stream, V, s.bt.p, c.grad.p, c.rhs.p);
+    CIPC_CHECK_LAUNCH();
+}
+
+}  // namespace cipc

@@ -1,0 +1,170 @@
+/*
+ * cipc_gpu.h — C ABI of the MI355X (gfx950) per-Newton-iteration path of CPU-IPC.
+ *
+ * The reference's hot path lives behind three C++ interfaces (paths relative to the
+ * reference root, "CPU IPC/" omitted):
+ *   - the Newton step API    solveIPCStep / solveBarrierSubproblem   (IPCSolver.h:63-68, IPCSolver.cpp:890-1064)
+ *   - the broad-phase facade  SpatialHash::build / calculateActivateSet with BroadPhaseBackend
+ *                                                                      (CollisionBroadPhase.h:11-80)
+ *   - the linear system       NewtonLinearSystem::solve / assemble    (NewtonLinearSystem.h:22-28)
+ * Each entry point below replaces one whole batch function of those interfaces (never a
+ * per-primitive query, which the reference calls from inside TBB loops).  All arguments
+ * are plain pointers and sizes in the reference's own memory layouts:
+ *   std::vector<Eigen::Vector3d>  -> const double*  (3 doubles per vertex, AoS)
+ *   std::vector<Eigen::Vector4i>  -> const int*     (4 ints per tet / per surface face)
+ *   std::vector<Eigen::Matrix3d>  -> const double*  (9 doubles, column-major)
+ *   std::vector<uint64_t>, std::vector<std::pair<uint64_t,uint64_t>> -> const uint64_t*
+ *   std::vector<EncodedContact>   -> int*           (4 ints per contact)
+ *   std::vector<std::pair<int,int>> -> int*         (2 ints per pair)
+ *   Eigen::SparseMatrix<double> (lower, column-major) -> outer/inner/values arrays
+ *
+ * Every call is synchronous with respect to the host and returns CIPC_OK or an error
+ * code; cipc_gpu_last_error() returns the message of the most recent failure in the
+ * calling thread.  Error codes map to the exception types the reference throws at the
+ * same places (std::invalid_argument, std::runtime_error, std::logic_error,
+ * std::overflow_error).  One context owns one device, one stream and all device
+ * memory of one scene; calls on one context must not overlap.
+ */
+#ifndef CIPC_GPU_H
+#define CIPC_GPU_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define CIPC_OK 0
+#define CIPC_ERR_INVALID_ARGUMENT 1 /* std::invalid_argument in the reference */
+#define CIPC_ERR_RUNTIME 2          /* std::runtime_error */
+#define CIPC_ERR_LOGIC 3            /* std::logic_error */
+#define CIPC_ERR_OVERFLOW 4         /* std::overflow_error */
+#define CIPC_ERR_HIP 5              /* HIP runtime failure */
+#define CIPC_ERR_NO_DEVICE 6        /* no usable gfx950 device */
+
+typedef struct cipc_gpu_context cipc_gpu_context;
+
+/* Scene arrays of mesh3D (SimulationMesh.h:37-129) and the scalars of the barrier
+ * subproblem.  hhat is the squared, scene-scaled dHat (Simulator.cpp:486-489); the ground
+ * is the half-space n.x >= D (CollisionBroadPhase.cpp:1062-1074). */
+typedef struct {
+    int V, T, F, Vs, Es;
+    const double* x;            /* mesh3D::vertexes                    3V  */
+    const double* v_rest;       /* mesh3D::v_rest                      3V  */
+    const double* V_prev;       /* mesh3D::V_prev                      3V  */
+    const double* xtilde;       /* mesh3D::inertialTarget              3V  */
+    const double* masses;       /* mesh3D::masses                      V   */
+    const int* boundary;        /* mesh3D::boundaryTypes               V   */
+    const int* tets;            /* mesh3D::tetrahedras                 4T  */
+    const double* dm_inv;       /* mesh3D::DM_tetrahedra_inverse       9T  */
+    const double* volumes;      /* mesh3D::volum                       T   */
+    const int* faces;           /* mesh3D::surface (v0,v1,v2,tet)      4F  */
+    const uint64_t* surf_verts; /* mesh3D::surfVerts                   Vs  */
+    const uint64_t* surf_edges; /* mesh3D::surfEdges                   2Es */
+    double dt, length_rate, volume_rate, drag, hhat, fhat, friction;
+    double ground_normal[3], ground_d;
+} cipc_scene_desc;
+
+/* Stage times of one Newton iteration, milliseconds, measured with HIP events on the
+ * context's stream (the reference's StageTimer, StageTimer.h:5-27). */
+typedef struct {
+    double broadphase_ms; /* LBVH build + active set (buildCollisionSets, IPCSolver.cpp:544-556) */
+    double assembly_ms;   /* gradient + PSD Hessian -> lower CSR (IPCSolver.cpp:918-930 less the solve) */
+    double elastic_ms;    /* inertia + tet part of the assembly (element kernel and its gathers) */
+    double ccd_ms;        /* ground + partial CCD + CFL + swept LBVH + full CCD (IPCSolver.cpp:943-990) */
+    double energy_ms;     /* one line-search energy evaluation (IPCSolver.cpp:322-456) */
+    double total_ms;
+    double alpha;         /* merged step bound */
+    double energy;        /* energy at the current positions */
+    int64_t nnz;          /* CSR nonzeros */
+    int64_t contacts;     /* |Self_ActiveSet| + |Self_EE_ActiveSet| + |Environment_ActiveSet| */
+} cipc_iteration_timing;
+
+const char* cipc_gpu_last_error(void);
+
+/* Context lifetime; device is a HIP device ordinal. */
+int cipc_gpu_context_create(int device, cipc_gpu_context** out);
+int cipc_gpu_context_destroy(cipc_gpu_context* ctx);
+
+/* Upload topology, material and initial state once per scene (FEMSimulator::buildModels,
+ * Simulator.cpp:426-529, after initMesh3D and getSurface).  Builds the static elastic
+ * sparsity pattern. */
+int cipc_gpu_upload_scene(cipc_gpu_context* ctx, const cipc_scene_desc* desc);
+
+/* State: any pointer may be NULL to keep the device copy.  Positions of the current
+ * iterate, the previous step, and the inertial target (IPCSolver.cpp:55-68,1200-1204). */
+int cipc_gpu_set_state(cipc_gpu_context* ctx, const double* x, const double* V_prev, const double* xtilde);
+int cipc_gpu_get_positions(cipc_gpu_context* ctx, double* x);
+/* Newton direction p (x_new = x - alpha p), 3V doubles (moveDir, IPCSolver.cpp:904). */
+int cipc_gpu_set_direction(cipc_gpu_context* ctx, const double* p);
+
+/* Static broad phase + active set: SpatialHash::build(mesh, h) with the LBVH backend
+ * (CollisionBroadPhase.cpp:58-66; LBVH.cpp:284-305) followed by
+ * SpatialHash::calculateActivateSet (CollisionBroadPhase.cpp:765-1059) and
+ * Ground::calculateActivateSet (CollisionBroadPhase.cpp:1076-1092).
+ * counts = {|Self_ActiveSet|, |Self_EE_ActiveSet|, |Self_CCD_ActiveSet|, |Environment_ActiveSet|}. */
+int cipc_gpu_active_set(cipc_gpu_context* ctx, int64_t counts[4]);
+/* Copy the sets out in the reference's order and encodings (EncodedContact.h:8-40).
+ * Environment_ActiveSet is returned in ascending surface-vertex order (the reference's is
+ * unordered).  Any pointer may be NULL. */
+int cipc_gpu_get_active_set(cipc_gpu_context* ctx, int* self4, int* ee4, int* ee_ie2, int* ccd2, int* env);
+/* Replace the device active set (for callers that build it elsewhere). */
+int cipc_gpu_set_active_set(cipc_gpu_context* ctx, const int64_t counts[4], const int* self4, const int* ee4,
+                            const int* ee_ie2, const int* ccd2, const int* env);
+
+/* computeGradientAndHessian (IPCSolver.cpp:108-319) fused with
+ * NewtonLinearSystem::assemble (NewtonLinearSystem.cpp:126-154): gradient, PSD-projected
+ * Hessian blocks and the mass diagonal straight into the Eigen lower-triangular CSC
+ * pattern (== upper CSR), exact zeros dropped before summation, rows/columns of
+ * non-free vertices dropped, mass diagonal on every DOF.  Returns the nonzero count and
+ * the collision count the reference accumulates (compute_g_dpt's return value). */
+int cipc_gpu_assemble(cipc_gpu_context* ctx, double kappa, int64_t* nnz, int* collisions);
+/* Gradient of the last assemble (unmasked, 3V) and the CSR + masked right-hand side. */
+int cipc_gpu_get_gradient(cipc_gpu_context* ctx, double* grad);
+int cipc_gpu_get_csr(cipc_gpu_context* ctx, int* outer, int* inner, double* values, double* rhs);
+
+/* computeEnergy (IPCSolver.cpp:322-456) at the current positions with the current active
+ * set.  terms = {SNH elastic (before the dt^2 factor), inertia + drag, ground barrier sum,
+ * self barrier sum (with multiplicity), mollified edge-edge barrier sum}. */
+int cipc_gpu_energy(cipc_gpu_context* ctx, double kappa, double* energy);
+int cipc_gpu_energy_terms(cipc_gpu_context* ctx, double kappa, double terms[5]);
+
+/* Step bound of one Newton iteration for the direction set with cipc_gpu_set_direction:
+ * limitStepByGround (FeasibleStep.cpp:8-40), Self_largestFeasibleStepSize
+ * (ContactMechanics.cpp:3704-3752), CFL (IPCSolver.cpp:960-974), the swept rebuild
+ * SpatialHash::build(mesh, p, alpha) and Self_largestFeasibleStepSize_CCD
+ * (ContactMechanics.cpp:3754-3867), merged as IPCSolver.cpp:985-989.
+ * out = {alpha, ground bound, partial CCD, CFL, full CCD}. */
+int cipc_gpu_step_bound(cipc_gpu_context* ctx, double out[5]);
+
+/* Line-search support: remember the current positions as x0, x = x0 - alpha p on free
+ * vertices (stepForward, IPCSolver.cpp:458-473), and isIntersected (IPCSolver.cpp:523-542)
+ * with a static LBVH rebuilt on the current positions. */
+int cipc_gpu_save_positions(cipc_gpu_context* ctx);
+int cipc_gpu_step_forward(cipc_gpu_context* ctx, double alpha);
+int cipc_gpu_is_intersected(cipc_gpu_context* ctx, int* intersected);
+
+/* One Newton iteration's non-linear stages on the device-resident state, timed per stage:
+ * active set, assembly, step bound, energy (the unit BASELINE-style timing uses). */
+int cipc_gpu_newton_iteration(cipc_gpu_context* ctx, double kappa, cipc_iteration_timing* timing);
+
+/* Element-level evaluation of n independent inputs (host arrays in, host arrays out);
+ * used to check the device kernels element by element.
+ *  snh:      F (9n, column-major) -> PK1 (9n), projected Hessian (81n row-major), energy density (n)
+ *            (Elasticity.cpp:1144-1170,1337-1479,695-727)
+ *  distance: kind 0 PP, 1 PE, 2 PT, 3 EE, 4 EE cross norm; pts 12n -> d (n), grad (12n), Hessian (144n)
+ *            (ContactMechanics.cpp:282-316,764-2467)
+ *  make_pd:  dim in {6,9,12}, H (dim*dim*n row-major) in place (ContactMechanics.h:27-45)
+ *  accd:     kind 0 point-triangle, 1 edge-edge; pts 12n, dirs 12n -> toc (n) (AdditiveCCD.cpp:221-329)
+ *  classify: kind 0 dType_PT, 1 dType_EE; pts 12n -> type (n) (ContactMechanics.cpp:143-280) */
+int cipc_gpu_eval_snh(int n, const double* F, double length_rate, double volume_rate, double* P, double* H, double* e);
+int cipc_gpu_eval_distance(int n, int kind, const double* pts, double* d, double* grad, double* hess);
+int cipc_gpu_eval_make_pd(int n, int dim, double* H);
+int cipc_gpu_eval_accd(int n, int kind, const double* pts, const double* dirs, double eta, double thickness, double* toc);
+int cipc_gpu_eval_classify(int n, int kind, const double* pts, int* type);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* CIPC_GPU_H */

@@ -1,0 +1,1314 @@
+// ORACLE — test infrastructure only.  CPU restatement of CPU-IPC's per-Newton-iteration
+// path, used by tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg as the
+// checker; never part of the product library.  Restates the behaviour of MPL-2.0
+// licensed CPU-IPC sources, cited per function (paths relative to the reference root,
+// "CPU IPC/" omitted).  Built with -ffp-contract=off so that every expression rounds in
+// the written order.
+//
+// Parallel structure follows the reference (per-element parallel loops, serial contact
+// gradient loops) with OpenMP; every reduction and scatter is done in a fixed order, so
+// the oracle is deterministic where the reference's TBB reductions are not.
+#include "oracle.h"
+#include "oracle_math.hpp"
+
+#include <map>
+#include <stdexcept>
+#include <omp.h>
+
+using namespace oracle;
+
+namespace {
+
+inline bool isFree(int t) { return t == 0; }                 // BoundaryConditions.h:30-33
+inline bool isCollider(int t) { return t >= 2; }             // BoundaryConditions.h:35-38
+
+using I4 = std::array<int, 4>;
+
+struct Blocks {                                              // BHessian, ContactMechanics.h:8-23
+    std::vector<std::array<double, 9>> h3;   std::vector<int> d1;
+    std::vector<std::array<double, 36>> h6;  std::vector<std::array<int, 2>> d2;
+    std::vector<std::array<double, 81>> h9;  std::vector<std::array<int, 3>> d3;
+    std::vector<std::array<double, 144>> h12; std::vector<std::array<int, 4>> d4;
+};
+
+// ----------------------------------------------------------------------------------
+// Axis-aligned boxes and the LBVH (LBVH.cpp:1-431)
+// ----------------------------------------------------------------------------------
+struct AABB {
+    V3 lo{std::numeric_limits<double>::max(), std::numeric_limits<double>::max(), std::numeric_limits<double>::max()};
+    V3 hi{std::numeric_limits<double>::lowest(), std::numeric_limits<double>::lowest(), std::numeric_limits<double>::lowest()};
+    void add(const V3& p) { lo = vmin(lo, p); hi = vmax(hi, p); }           // LBVH.cpp:82-86
+    void add(const AABB& b) { lo = vmin(lo, b.lo); hi = vmax(hi, b.hi); }   // LBVH.cpp:88-92
+    V3 center() const { return 0.5 * (lo + hi); }                           // LBVH.cpp:94-97
+    // LBVH.cpp:157-161: touching counts as overlapping.
+    bool overlaps(const AABB& b, double gap) const {
+        return !((b.lo.x - hi.x > gap) || (b.lo.y - hi.y > gap) || (b.lo.z - hi.z > gap) ||
+                 (lo.x - b.hi.x > gap) || (lo.y - b.hi.y > gap) || (lo.z - b.hi.z > gap));
+    }
+};
+AABB pointBox(const V3& p) { AABB b; b.add(p); return b; }
+AABB edgeBox(const V3& a, const V3& c) { AABB b; b.add(a); b.add(c); return b; }
+AABB triBox(const V3& a, const V3& c, const V3& d) { AABB b; b.add(a); b.add(c); b.add(d); return b; }
+// LBVH.cpp:120-129,400-431: swept boxes contain x and x - alpha * p.
+AABB sweptPoint(const V3& p, const V3& dir, double a) { AABB b; b.add(p); b.add(p - a * dir); return b; }
+AABB sweptEdge(const V3& p0, const V3& d0, const V3& p1, const V3& d1, double a) {
+    AABB b = sweptPoint(p0, d0, a); b.add(sweptPoint(p1, d1, a)); return b;
+}
+AABB sweptTri(const V3& p0, const V3& d0, const V3& p1, const V3& d1, const V3& p2, const V3& d2, double a) {
+    AABB b; b.add(sweptPoint(p0, d0, a)); b.add(sweptPoint(p1, d1, a)); b.add(sweptPoint(p2, d2, a)); return b;
+}
+
+uint32_t expandBits(uint32_t v) {                            // LBVH.cpp:17-24
+    v = (v * 0x00010001u) & 0xFF0000FFu;
+    v = (v * 0x00000101u) & 0x0F00F00Fu;
+    v = (v * 0x00000011u) & 0xC30C30C3u;
+    v = (v * 0x00000005u) & 0x49249249u;
+    return v;
+}
+uint32_t morton(double x, double y, double z) {              // LBVH.cpp:26-37
+    const double res = 1024.0;
+    auto q = [&](double v) { v = std::max(0.0, std::min(v * res, res - 1.0)); return (uint32_t)v; };
+    return (expandBits(q(x)) << 2) | (expandBits(q(y)) << 1) | expandBits(q(z));
+}
+int clz64(uint64_t v) { return v == 0 ? 64 : __builtin_clzll(v); }
+
+struct LBVH {                                                // CPULinearBVH, LBVH.h:21-47
+    struct Node { uint32_t parent = ~0u, left = ~0u, right = ~0u, prim = ~0u; };
+    std::vector<Node> nodes; std::vector<AABB> boxes; std::vector<uint64_t> keys;
+
+    int prefix(int a, int b) const {
+        if (b < 0 || b >= (int)keys.size()) return -1;
+        return clz64(keys[a] ^ keys[b]);
+    }
+    std::pair<uint32_t, uint32_t> range(uint32_t idx) const {   // LBVH.cpp:62-93
+        const int n = (int)keys.size();
+        if (idx == 0) return {0u, (uint32_t)(n - 1)};
+        const int cur = (int)idx;
+        const int lp = prefix(cur, cur - 1), rp = prefix(cur, cur + 1);
+        const int dir = rp > lp ? 1 : -1;
+        const int minp = std::min(lp, rp);
+        int lmax = 2;
+        while (prefix(cur, cur + dir * lmax) > minp) lmax <<= 1;
+        int len = 0;
+        for (int st = lmax >> 1; st > 0; st >>= 1)
+            if (prefix(cur, cur + dir * (len + st)) > minp) len += st;
+        const int other = cur + dir * len;
+        return {(uint32_t)std::min(cur, other), (uint32_t)std::max(cur, other)};
+    }
+    uint32_t split(uint32_t first, uint32_t last) const {       // LBVH.cpp:95-119
+        const uint64_t fc = keys[first], lc = keys[last];
+        if (fc == lc) return (first + last) >> 1;
+        const int common = clz64(fc ^ lc);
+        uint32_t s = first, step = last - first;
+        do {
+            step = (step + 1) >> 1;
+            const uint32_t cand = s + step;
+            if (cand < last && clz64(fc ^ keys[cand]) > common) s = cand;
+        } while (step > 1);
+        return s;
+    }
+    void build(const std::vector<AABB>& prims) {                // LBVH.cpp:163-250
+        const size_t n = prims.size();
+        nodes.clear(); boxes.clear(); keys.clear();
+        if (n == 0) return;
+        AABB scene;
+        for (const auto& b : prims) scene.add(b);
+        const V3 ext = scene.hi - scene.lo;
+        std::vector<std::pair<uint64_t, uint32_t>> ent(n);
+#pragma omp parallel for schedule(static)
+        for (long i = 0; i < (long)n; ++i) {
+            const V3 off = prims[i].center() - scene.lo;
+            const double x = ext.x > 0.0 ? off.x / ext.x : 0.0;
+            const double y = ext.y > 0.0 ? off.y / ext.y : 0.0;
+            const double z = ext.z > 0.0 ? off.z / ext.z : 0.0;
+            ent[i] = {((uint64_t)morton(x, y, z) << 32) | (uint64_t)i, (uint32_t)i};
+        }
+        std::sort(ent.begin(), ent.end(), [](const auto& a, const auto& b) { return a.first < b.first; });
+        nodes.assign(2 * n - 1, Node{});
+        boxes.assign(2 * n - 1, AABB{});
+        keys.resize(n);
+        for (size_t s = 0; s < n; ++s) {
+            keys[s] = ent[s].first;
+            nodes[s + n - 1].prim = ent[s].second;
+            boxes[s + n - 1] = prims[ent[s].second];
+        }
+        if (n == 1) return;
+#pragma omp parallel for schedule(static)
+        for (long i = 0; i < (long)n - 1; ++i) {
+            auto r = range((uint32_t)i);
+            uint32_t sp = split(r.first, r.second);
+            uint32_t l = sp, rr = sp + 1;
+            if (sp == r.first) l += (uint32_t)(n - 1);
+            if (sp + 1 == r.second) rr += (uint32_t)(n - 1);
+            nodes[i].left = l; nodes[i].right = rr;
+            nodes[l].parent = (uint32_t)i; nodes[rr].parent = (uint32_t)i;
+        }
+        // post-order internal boxes (LBVH.cpp:230-249)
+        std::vector<std::pair<uint32_t, bool>> st;
+        st.emplace_back(0u, false);
+        while (!st.empty()) {
+            auto c = st.back(); st.pop_back();
+            const Node& nd = nodes[c.first];
+            if (nd.prim != ~0u) continue;
+            if (c.second) { boxes[c.first] = boxes[nd.left]; boxes[c.first].add(boxes[nd.right]); }
+            else { st.emplace_back(c.first, true); st.emplace_back(nd.right, false); st.emplace_back(nd.left, false); }
+        }
+    }
+    void query(const AABB& q, double gap, std::vector<int>& out) const {   // LBVH.cpp:252-282
+        out.clear();
+        if (nodes.empty()) return;
+        uint32_t stack[128]; size_t sz = 0;
+        stack[sz++] = 0;
+        while (sz > 0) {
+            const uint32_t ni = stack[--sz];
+            if (!q.overlaps(boxes[ni], gap)) continue;
+            const Node& nd = nodes[ni];
+            if (nd.prim != ~0u) { out.push_back((int)nd.prim); continue; }
+            if (sz + 2 > 128) throw std::runtime_error("CPU LBVH traversal stack overflow");
+            stack[sz++] = nd.right; stack[sz++] = nd.left;
+        }
+        std::sort(out.begin(), out.end());
+    }
+};
+
+// ----------------------------------------------------------------------------------
+// The mesh state (mesh3D subset, SimulationMesh.h:37-129)
+// ----------------------------------------------------------------------------------
+struct Mesh {
+    int V = 0, T = 0, F = 0, Vs = 0, Es = 0;
+    std::vector<V3> x, vrest, vprev, xt;
+    std::vector<double> mass;  std::vector<int> bt;
+    std::vector<I4> tets; std::vector<M3> dminv; std::vector<double> vol;
+    std::vector<std::array<int, 3>> faces; std::vector<int> sv; std::vector<std::pair<int, int>> se;
+    double dt = 0, u = 0, r = 0, drag = 0, hhat = 0, fhat = 0, fric = 0;
+    V3 gn{0, 1, 0}; double gd = -1;
+    std::vector<I4> selfAS, eeAS; std::vector<std::pair<int, int>> eeIe, ccdAS; std::vector<int> envAS;
+    std::vector<V3> grad;
+    std::vector<int> outer, inner; std::vector<double> val, rhs;
+    LBVH faceTree, edgeTree;
+    std::vector<AABB> sweptPointBoxes, faceBoxes, edgeBoxes;
+    int threads = 0;
+};
+
+// ----------------------------------------------------------------------------------
+// Stable neo-Hookean element (Elasticity.cpp)
+// ----------------------------------------------------------------------------------
+M3 deformation(const Mesh& m, int t) {                       // Elasticity.cpp:570-598 x Dm^-1
+    const I4& id = m.tets[t];
+    M3 Ds;
+    for (int c = 0; c < 3; ++c) {
+        const V3 e = m.x[id[c + 1]] - m.x[id[0]];
+        Ds(0, c) = e.x; Ds(1, c) = e.y; Ds(2, c) = e.z;
+    }
+    return matmul(Ds, m.dminv[t]);
+}
+M3 cofactor(const M3& F) {                                   // pI3pF, Elasticity.cpp:1155-1165
+    M3 c;
+    c(0, 0) = F(1, 1) * F(2, 2) - F(1, 2) * F(2, 1);
+    c(0, 1) = F(1, 2) * F(2, 0) - F(1, 0) * F(2, 2);
+    c(0, 2) = F(1, 0) * F(2, 1) - F(1, 1) * F(2, 0);
+    c(1, 0) = F(2, 1) * F(0, 2) - F(2, 2) * F(0, 1);
+    c(1, 1) = F(2, 2) * F(0, 0) - F(2, 0) * F(0, 2);
+    c(1, 2) = F(2, 0) * F(0, 1) - F(2, 1) * F(0, 0);
+    c(2, 0) = F(0, 1) * F(1, 2) - F(1, 1) * F(0, 2);
+    c(2, 1) = F(0, 2) * F(1, 0) - F(0, 0) * F(1, 2);
+    c(2, 2) = F(0, 0) * F(1, 1) - F(0, 1) * F(1, 0);
+    return c;
+}
+// computePEPF_StableNHK3D_2_double (Elasticity.cpp:1144-1170)
+M3 snh_pk1(const M3& F, double u, double r) {
+    M3 U, V; double s[3];
+    svd3(F, U, s, V);
+    const double I3 = s[0] * s[1] * s[2];
+    const double I2 = s[0] * s[0] + s[1] * s[1] + s[2] * s[2];
+    const M3 c = cofactor(F);
+    M3 P;
+    const double a = u * (1 - 1 / (I2 + 1)), b = r * (I3 - 1 - u * 3 / (r * 4));
+    for (int i = 0; i < 9; ++i) P.m[i] = a * F.m[i] + b * c.m[i];
+    return P;
+}
+// project_StabbleNHK_2_H_3D active branch (Elasticity.cpp:1337-1345,1436-1477), then makePD<9>.
+// H9 row-major 9x9 over column-major vec(F).
+void snh_hessian(const M3& F, double u, double r, double* H) {
+    M3 U, V; double s[3];
+    svd3(F, U, s, V);
+    const double I3 = s[0] * s[1] * s[2];
+    const double I2 = s[0] * s[0] + s[1] * s[1] + s[2] * s[2];
+    const double mu = u, lambda = r, Ic = I2, J = I3;
+    double g1[9], gJ[9];
+    for (int c = 0; c < 3; ++c)
+        for (int i = 0; i < 3; ++i) g1[3 * c + i] = 2 * F(i, c);
+    const V3 f0(F(0, 0), F(1, 0), F(2, 0)), f1(F(0, 1), F(1, 1), F(2, 1)), f2(F(0, 2), F(1, 2), F(2, 2));
+    const V3 c0 = cross(f1, f2), c1 = cross(f2, f0), c2 = cross(f0, f1);
+    for (int i = 0; i < 3; ++i) { gJ[i] = c0[i]; gJ[3 + i] = c1[i]; gJ[6 + i] = c2[i]; }
+    auto hat = [](const V3& f, double h[3][3]) {
+        h[0][0] = 0; h[0][1] = -f.z; h[0][2] = f.y;
+        h[1][0] = f.z; h[1][1] = 0; h[1][2] = -f.x;
+        h[2][0] = -f.y; h[2][1] = f.x; h[2][2] = 0;
+    };
+    double fh[3][3][3];
+    hat(f0, fh[0]); hat(f1, fh[1]); hat(f2, fh[2]);
+    double HJ[9][9] = {};
+    // blocks (0,1)=-f2hat (0,2)=f1hat (1,0)=f2hat (1,2)=-f0hat (2,0)=-f1hat (2,1)=f0hat
+    const int bk[6][2] = {{0, 1}, {0, 2}, {1, 0}, {1, 2}, {2, 0}, {2, 1}};
+    const int bf[6] = {2, 1, 2, 0, 1, 0};
+    const double bs[6] = {-1, 1, 1, -1, -1, 1};
+    for (int q = 0; q < 6; ++q)
+        for (int i = 0; i < 3; ++i)
+            for (int j = 0; j < 3; ++j) HJ[3 * bk[q][0] + i][3 * bk[q][1] + j] = bs[q] * fh[bf[q]][i][j];
+    const double a1 = (Ic * mu) / (2 * (Ic + 1));
+    const double a2 = lambda * (J - 1 - (3 * mu) / (4.0 * lambda));
+    const double a3 = mu / (2 * (Ic + 1) * (Ic + 1));
+    for (int i = 0; i < 9; ++i)
+        for (int j = 0; j < 9; ++j)
+            H[9 * i + j] = a1 * (i == j ? 2.0 : 0.0) + a2 * HJ[i][j] + a3 * g1[i] * g1[j] + lambda * gJ[i] * gJ[j];
+    make_pd<9>(H);
+}
+// getObjEnergy_StableNHK2_3D per-element term (Elasticity.cpp:695-727), without the volume.
+double snh_energy_density(const M3& F, double u, double r) {
+    M3 U, V; double s[3];
+    svd3(F, U, s, V);
+    M3 S;   // V diag(s) V^T
+    for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j) S(i, j) = V(i, 0) * s[0] * V(j, 0) + V(i, 1) * s[1] * V(j, 1) + V(i, 2) * s[2] * V(j, 2);
+    const M3 S2 = matmul(S, S);
+    const double I2 = S2(0, 0) + S2(1, 1) + S2(2, 2);
+    const double I3 = S(0, 0) * (S(1, 1) * S(2, 2) - S(1, 2) * S(2, 1)) - S(0, 1) * (S(1, 0) * S(2, 2) - S(1, 2) * S(2, 0))
+                    + S(0, 2) * (S(1, 0) * S(2, 1) - S(1, 1) * S(2, 0));
+    const double k = I3 - 1 - 3 * u / 4 / r;
+    return 0.5 * u * (I2 - 3) + 0.5 * r * k * k - 0.5 * u * std::log(I2 + 1);
+}
+// computePFPX3D_double (Elasticity.cpp:803-848): row 3c+i of vec(F) against 3a+i of x.
+void pfpx(const M3& Di, double P[9][12]) {
+    for (int i = 0; i < 9; ++i) for (int j = 0; j < 12; ++j) P[i][j] = 0;
+    for (int c = 0; c < 3; ++c) {
+        const double t = -(Di(0, c) + Di(1, c) + Di(2, c));
+        for (int i = 0; i < 3; ++i) {
+            P[3 * c + i][i] = t;
+            for (int a = 1; a < 4; ++a) P[3 * c + i][3 * a + i] = Di(a - 1, c);
+        }
+    }
+}
+
+// ----------------------------------------------------------------------------------
+// Barrier, distances and their classification (ContactMechanics.cpp)
+// ----------------------------------------------------------------------------------
+void barrier_b(double d, double dh, double& b) { b = -(d - dh) * (d - dh) * std::log(d / dh); }  // :775-778
+void barrier_g(double d, double dh, double& g) {                                                 // :780-784
+    const double t = d - dh;
+    g = t * std::log(d / dh) * -2.0 - (t * t) / d;
+}
+void barrier_h(double d, double dh, double& h) {                                                 // :786-790
+    const double t = d - dh;
+    h = (std::log(d / dh) * -2.0 - t * 4.0 / d) + 1.0 / (d * d) * (t * t);
+}
+
+double d_PP(const V3& a, const V3& b) { return sqn(a - b); }                                      // :282-287
+double d_PE(const V3& p, const V3& e0, const V3& e1) { return sqn(cross(e0 - p, e1 - p)) / sqn(e1 - e0); } // :807-813
+double d_PT(const V3& p, const V3& t0, const V3& t1, const V3& t2) {                             // :298-306
+    const V3 b = cross(t1 - t0, t2 - t0);
+    const double aTb = dot(p - t0, b);
+    return aTb * aTb / sqn(b);
+}
+double d_EE(const V3& a0, const V3& a1, const V3& b0, const V3& b1) {                            // :764-773
+    const V3 b = cross(a1 - a0, b1 - b0);
+    const double aTb = dot(b0 - a0, b);
+    return aTb * aTb / sqn(b);
+}
+double ee_cross_sqn(const V3& a0, const V3& a1, const V3& b0, const V3& b1) { return sqn(cross(a1 - a0, b1 - b0)); } // :308-316
+
+// Eigen::Matrix2d::ldlt().solve(b) for a symmetric 2x2 (Eigen 3.4 LDLT: diagonal pivoting
+// on the larger |diagonal|, unit-lower L, pseudo-inverse of D below DBL_MIN).
+void ldlt2_solve(double a00, double a10, double a11, double b0, double b1, double& x0, double& x1) {
+    bool swapped = std::fabs(a11) > std::fabs(a00);
+    double d0 = swapped ? a11 : a00, d1 = swapped ? a00 : a11, l = a10;
+    bool valid0 = std::fabs(d0) > 0.0;
+    if (!valid0) {
+        swapped = false;  // all-zero diagonal: transpositions reset to identity, L untouched
+        d0 = a00; d1 = a11;
+    } else {
+        l = l / d0;
+        const double tmp = d0 * l;
+        d1 = d1 - l * tmp;
+    }
+    double y0 = swapped ? b1 : b0, y1 = swapped ? b0 : b1;
+    y1 = y1 - l * y0;
+    const double tol = std::numeric_limits<double>::min();
+    y0 = std::fabs(d0) > tol ? y0 / d0 : 0.0;
+    y1 = std::fabs(d1) > tol ? y1 / d1 : 0.0;
+    y0 = y0 - l * y1;
+    if (swapped) { x0 = y1; x1 = y0; } else { x0 = y0; x1 = y1; }
+}
+// One projection of dType_PT: basis rows r0 = e, r1 = e x n; returns the 2 parameters.
+void pt_param(const V3& r0, const V3& nvec, const V3& rhs, double& p0, double& p1) {
+    const V3 r1 = cross(r0, nvec);
+    const double a00 = dot(r0, r0), a10 = dot(r1, r0), a11 = dot(r1, r1);
+    ldlt2_solve(a00, a10, a11, dot(r0, rhs), dot(r1, rhs), p0, p1);
+}
+// dType_PT (ContactMechanics.cpp:143-193): v0 the point, v1 v2 v3 the triangle.
+int dType_PT(const V3& v0, const V3& v1, const V3& v2, const V3& v3) {
+    const V3 n = cross(v2 - v1, v3 - v1);
+    double p00, p10, p01, p11, p02, p12;
+    pt_param(v2 - v1, n, v0 - v1, p00, p10);
+    if (p00 > 0.0 && p00 < 1.0 && p10 >= 0.0) return 3;
+    pt_param(v3 - v2, n, v0 - v2, p01, p11);
+    if (p01 > 0.0 && p01 < 1.0 && p11 >= 0.0) return 4;
+    pt_param(v1 - v3, n, v0 - v3, p02, p12);
+    if (p02 > 0.0 && p02 < 1.0 && p12 >= 0.0) return 5;
+    if (p00 <= 0.0 && p02 >= 1.0) return 0;
+    if (p01 <= 0.0 && p00 >= 1.0) return 1;
+    if (p02 <= 0.0 && p01 >= 1.0) return 2;
+    return 6;
+}
+// dType_EE (ContactMechanics.cpp:195-280) and edge_edge_distance_type (AdditiveCCD.cpp:64-139).
+int dType_EE(const V3& v0, const V3& v1, const V3& v2, const V3& v3) {
+    const V3 u = v1 - v0, v = v3 - v2, w = v0 - v2;
+    const double a = sqn(u), b = dot(u, v), c = sqn(v), d = dot(u, w), e = dot(v, w);
+    const double D = a * c - b * b;
+    double tD = D, sN, tN;
+    int def = 8;
+    sN = (b * e - c * d);
+    if (sN <= 0.0) { tN = e; tD = c; def = 2; }
+    else if (sN >= D) { tN = e + b; tD = c; def = 5; }
+    else {
+        tN = (a * e - b * d);
+        const V3 uv = cross(u, v);
+        if (tN > 0.0 && tN < tD && (dot(uv, w) == 0.0 || sqn(uv) < 1.0e-20 * a * c)) {
+            if (sN < D / 2) { tN = e; tD = c; def = 2; }
+            else { tN = e + b; tD = c; def = 5; }
+        }
+    }
+    if (tN <= 0.0) {
+        if (-d <= 0.0) return 0;
+        else if (-d >= a) return 3;
+        else return 6;
+    } else if (tN >= tD) {
+        if ((-d + b) <= 0.0) return 1;
+        else if ((-d + b) >= a) return 4;
+        else return 7;
+    }
+    return def;
+}
+
+// Exact distance derivatives through second-order AD (replaces the generated
+// g_PE/H_PE/g_PT/H_PT/g_EE/H_EE, ContactMechanics.cpp:815-2467).
+template <int N>
+void dist_derivs(int type, const V3* p, double& d, double* g, double* H) {
+    HV3<N> q[4];
+    for (int k = 0; k < N / 3; ++k) q[k] = hvar<N>(p[k], k);
+    HD<N> r;
+    switch (type) {
+    case 0: { HV3<N> e = hsub(q[0], q[1]); r = hdot(e, e); break; }
+    case 1: { HV3<N> c = hcross(hsub(q[1], q[0]), hsub(q[2], q[0])); HV3<N> e = hsub(q[2], q[1]); r = hdot(c, c) / hdot(e, e); break; }
+    case 2: { HV3<N> b = hcross(hsub(q[2], q[1]), hsub(q[3], q[1])); HD<N> t = hdot(hsub(q[0], q[1]), b); r = (t * t) / hdot(b, b); break; }
+    case 3: { HV3<N> b = hcross(hsub(q[1], q[0]), hsub(q[3], q[2])); HD<N> t = hdot(hsub(q[2], q[0]), b); r = (t * t) / hdot(b, b); break; }
+    default: { HV3<N> b = hcross(hsub(q[1], q[0]), hsub(q[3], q[2])); r = hdot(b, b); break; }
+    }
+    d = r.v;
+    for (int i = 0; i < N; ++i) { if (g) g[i] = r.g[i]; if (H) for (int j = 0; j < N; ++j) H[i * N + j] = r.h[i][j]; }
+}
+void deriv_any(int type, const V3* p, double& d, double* g, double* H) {
+    if (type == 0) dist_derivs<6>(0, p, d, g, H);
+    else if (type == 1) dist_derivs<9>(1, p, d, g, H);
+    else dist_derivs<12>(type, p, d, g, H);
+}
+
+double eps_x(const Mesh& m, int a0, int a1, int b0, int b1) {                                  // :758-762
+    return 1e-3 * sqn(m.vrest[a0] - m.vrest[a1]) * sqn(m.vrest[b0] - m.vrest[b1]);
+}
+// mollifier q(c) and derivatives (ContactMechanics.cpp:678-693)
+double mol_e(double c, double ex) { if (!(c < ex)) return 1.0; const double t = c / ex; return (-t + 2.0) * t; }
+void mol_eg(const V3* p, double ex, double* g) {                                                // :711-730
+    double c, gc[12];
+    dist_derivs<12>(4, p, c, gc, nullptr);
+    if (c < ex) {
+        const double oe = 1.0 / ex, qg = 2.0 * oe * (-oe * c + 1.0);
+        for (int i = 0; i < 12; ++i) g[i] = gc[i] * qg;
+    } else for (int i = 0; i < 12; ++i) g[i] = 0;
+}
+void mol_eH(const V3* p, double ex, double* H) {                                                // :732-756
+    double c, gc[12], Hc[144];
+    dist_derivs<12>(4, p, c, gc, Hc);
+    if (c < ex) {
+        const double oe = 1.0 / ex, qg = 2.0 * oe * (-oe * c + 1.0), qH = -2.0 / (ex * ex);
+        for (int i = 0; i < 12; ++i) for (int j = 0; j < 12; ++j) H[12 * i + j] = Hc[12 * i + j] * qg + (qH * gc[i]) * gc[j];
+    } else for (int i = 0; i < 144; ++i) H[i] = 0;
+}
+
+// SelfConstraintVal (ContactMechanics.cpp:2478-2503)
+double self_val(const Mesh& m, const I4& c) {
+    if (c[0] >= 0) return d_EE(m.x[c[0]], m.x[c[1]], m.x[c[2]], m.x[c[3]]);
+    const int v0 = -c[0] - 1;
+    if (c[2] < 0) return d_PP(m.x[v0], m.x[c[1]]);
+    if (c[3] < 0) return d_PE(m.x[v0], m.x[c[1]], m.x[c[2]]);
+    return d_PT(m.x[v0], m.x[c[1]], m.x[c[2]], m.x[c[3]]);
+}
+// vertices and distance type of an encoded contact: type 0 PP 1 PE 2 PT 3 EE
+int decode(const I4& c, int vid[4]) {
+    if (c[0] >= 0) { for (int k = 0; k < 4; ++k) vid[k] = c[k]; return 3; }
+    vid[0] = -c[0] - 1; vid[1] = c[1];
+    if (c[2] < 0) return 0;
+    vid[2] = c[2];
+    if (c[3] < 0) return 1;
+    vid[3] = c[3];
+    return 2;
+}
+
+// ----------------------------------------------------------------------------------
+// computeGradientAndHessian (IPCSolver.cpp:108-319), tet + ground + self contact subset
+// ----------------------------------------------------------------------------------
+int gradient_and_hessian(Mesh& m, double kappa, Blocks& BH) {
+    m.grad.assign(m.V, V3());
+    for (int v = 0; v < m.V; ++v) m.grad[v] = m.mass[v] * (m.x[v] - m.xt[v]);                     // :118-123
+    for (int v = 0; v < m.V; ++v) m.grad[v] = m.grad[v] + (m.drag * m.mass[v]) * (m.x[v] - m.vprev[v]); // :124-128
+    // tets (IPCSolver.cpp:139-164)
+    std::vector<std::array<double, 12>> fe(m.T);
+    BH.h12.assign(m.T, {}); BH.d4.assign(m.T, {});
+#pragma omp parallel for schedule(dynamic, 256)
+    for (int t = 0; t < m.T; ++t) {
+        double P[9][12];
+        pfpx(m.dminv[t], P);
+        const M3 F = deformation(m, t);
+        const M3 PE = snh_pk1(F, m.u, m.r);
+        for (int j = 0; j < 12; ++j) {
+            double acc = 0;
+            for (int k = 0; k < 9; ++k) acc += (m.vol[t] * P[k][j]) * PE.m[k];
+            fe[t][j] = acc;
+        }
+        double Hq[81];
+        snh_hessian(F, m.u, m.r, Hq);
+        double T1[9][12];
+        for (int i = 0; i < 9; ++i)
+            for (int j = 0; j < 12; ++j) {
+                double acc = 0;
+                for (int k = 0; k < 9; ++k) acc += Hq[9 * i + k] * P[k][j];
+                T1[i][j] = acc;
+            }
+        const double s = m.vol[t] * m.dt * m.dt;
+        for (int i = 0; i < 12; ++i)
+            for (int j = 0; j < 12; ++j) {
+                double acc = 0;
+                for (int k = 0; k < 9; ++k) acc += (s * P[k][i]) * T1[k][j];
+                BH.h12[t][12 * i + j] = acc;
+            }
+        BH.d4[t] = m.tets[t];
+    }
+    const double dt2 = m.dt * m.dt;
+    for (int t = 0; t < m.T; ++t)
+        for (int a = 0; a < 4; ++a) {
+            V3 f(fe[t][3 * a], fe[t][3 * a + 1], fe[t][3 * a + 2]);
+            m.grad[m.tets[t][a]] = m.grad[m.tets[t][a]] + dt2 * f;
+        }
+    // ground (IPCSolver.cpp:270-293)
+    for (int v : m.envAS) {
+        const double dist = dot(m.gn, m.x[v]) - m.gd;
+        const double d = dist * dist;
+        double gb, hb;
+        barrier_g(d, m.hhat, gb); barrier_h(d, m.hhat, hb);
+        m.grad[v] = m.grad[v] + (((kappa * gb) * 2.0) * dist) * m.gn;
+        const double param = 4.0 * hb * d + 2.0 * gb;
+        if (param > 0) {
+            std::array<double, 9> h;
+            for (int i = 0; i < 3; ++i) for (int j = 0; j < 3; ++j) h[3 * i + j] = (kappa * param) * (m.gn[i] * m.gn[j]);
+            BH.h3.push_back(h); BH.d1.push_back(v);
+        }
+    }
+    // self contact gradient, compute_g_dpt (ContactMechanics.cpp:2970-3032), serial
+    int collisions = 0;
+    auto add_g = [&](int type, const int* vid, double scale) {
+        V3 p[4]; const int nv = type == 0 ? 2 : (type == 1 ? 3 : 4);
+        for (int k = 0; k < nv; ++k) p[k] = m.x[vid[k]];
+        double d, g[12];
+        deriv_any(type, p, d, g, nullptr);
+        for (int k = 0; k < nv; ++k) m.grad[vid[k]] = m.grad[vid[k]] + V3(scale * g[3 * k], scale * g[3 * k + 1], scale * g[3 * k + 2]);
+    };
+    auto g_dpt = [&](const std::vector<I4>& set, const std::vector<double>& in) {
+        int n = 0;
+        for (size_t i = 0; i < set.size(); ++i) {
+            int vid[4]; const int type = decode(set[i], vid);
+            double scale;
+            if (type == 0 || type == 1) { scale = kappa * -set[i][3] * in[i]; n += -set[i][3]; }
+            else { scale = kappa * in[i]; n += 1; }
+            add_g(type, vid, scale);
+        }
+        return n;
+    };
+    std::vector<double> gb(m.selfAS.size());
+    for (size_t i = 0; i < m.selfAS.size(); ++i) barrier_g(self_val(m, m.selfAS[i]), m.hhat, gb[i]);
+    collisions += g_dpt(m.selfAS, gb);
+    // compute_g_dee (ContactMechanics.cpp:3034-3080)
+    std::vector<double> edb(m.eeAS.size());
+    for (size_t i = 0; i < m.eeAS.size(); ++i) {
+        const I4& c = m.eeAS[i];
+        const double d = self_val(m, c);
+        double b, g;
+        barrier_b(d, m.hhat, b); barrier_g(d, m.hhat, g);
+        int e4[4];
+        if (c[3] >= 0) { for (int k = 0; k < 4; ++k) e4[k] = c[k]; }
+        else {
+            const auto& eI = m.se[m.eeIe[i].first]; const auto& eJ = m.se[m.eeIe[i].second];
+            e4[0] = eI.first; e4[1] = eI.second; e4[2] = eJ.first; e4[3] = eJ.second;
+        }
+        const double ex = eps_x(m, e4[0], e4[1], e4[2], e4[3]);
+        V3 p[4] = {m.x[e4[0]], m.x[e4[1]], m.x[e4[2]], m.x[e4[3]]};
+        const double e = mol_e(ee_cross_sqn(p[0], p[1], p[2], p[3]), ex);
+        double eg[12];
+        mol_eg(p, ex, eg);
+        const double cb = kappa * b;
+        for (int k = 0; k < 4; ++k) m.grad[e4[k]] = m.grad[e4[k]] + cb * V3(eg[3 * k], eg[3 * k + 1], eg[3 * k + 2]);
+        edb[i] = g * e;
+    }
+    g_dpt(m.eeAS, edb);
+
+    // compute_H_dpt (ContactMechanics.cpp:3466-3573)
+    const size_t ns = m.selfAS.size();
+    std::vector<std::array<double, 144>> Hs(ns);
+    std::vector<int> types(ns);
+#pragma omp parallel for schedule(dynamic, 16)
+    for (long i = 0; i < (long)ns; ++i) {
+        int vid[4]; const int type = decode(m.selfAS[i], vid);
+        types[i] = type;
+        const int nv = type == 0 ? 2 : (type == 1 ? 3 : 4), n = 3 * nv;
+        V3 p[4];
+        for (int k = 0; k < nv; ++k) p[k] = m.x[vid[k]];
+        double d, g[12], H[144];
+        deriv_any(type, p, d, g, H);
+        double g_b, H_b;
+        barrier_g(d, m.hhat, g_b); barrier_h(d, m.hhat, H_b);
+        const double coef = (type <= 1) ? kappa * -m.selfAS[i][3] : kappa;
+        for (int a = 0; a < n; ++a)
+            for (int b = 0; b < n; ++b) Hs[i][n * a + b] = ((coef * H_b) * g[a]) * g[b] + (coef * g_b) * H[n * a + b];
+        if (n == 6) make_pd<6>(Hs[i].data());
+        else if (n == 9) make_pd<9>(Hs[i].data());
+        else make_pd<12>(Hs[i].data());
+    }
+    for (size_t i = 0; i < ns; ++i) {
+        int vid[4]; decode(m.selfAS[i], vid);
+        if (types[i] == 0) { std::array<double, 36> h; std::copy(Hs[i].begin(), Hs[i].begin() + 36, h.begin()); BH.h6.push_back(h); BH.d2.push_back({vid[0], vid[1]}); }
+        else if (types[i] == 1) { std::array<double, 81> h; std::copy(Hs[i].begin(), Hs[i].begin() + 81, h.begin()); BH.h9.push_back(h); BH.d3.push_back({vid[0], vid[1], vid[2]}); }
+        else { BH.h12.push_back(Hs[i]); BH.d4.push_back({vid[0], vid[1], vid[2], vid[3]}); }
+    }
+    // compute_H_dee (ContactMechanics.cpp:3575-3700)
+    const size_t ne = m.eeAS.size();
+    std::vector<std::array<double, 144>> He(ne);
+    std::vector<I4> De(ne);
+#pragma omp parallel for schedule(dynamic, 16)
+    for (long i = 0; i < (long)ne; ++i) {
+        const I4& c = m.eeAS[i];
+        const double d = self_val(m, c);
+        double b, g_b, H_b;
+        barrier_b(d, m.hhat, b); barrier_g(d, m.hhat, g_b); barrier_h(d, m.hhat, H_b);
+        int e4[4];
+        double gd[12] = {}, Hd[144] = {};
+        if (c[3] >= 0) {
+            for (int k = 0; k < 4; ++k) e4[k] = c[k];
+            V3 p[4] = {m.x[c[0]], m.x[c[1]], m.x[c[2]], m.x[c[3]]};
+            double dd;
+            dist_derivs<12>(3, p, dd, gd, Hd);
+        } else {
+            const auto& eI = m.se[m.eeIe[i].first]; const auto& eJ = m.se[m.eeIe[i].second];
+            e4[0] = eI.first; e4[1] = eI.second; e4[2] = eJ.first; e4[3] = eJ.second;
+            int vid[4]; const int type = decode(c, vid);
+            const int nv = type == 0 ? 2 : 3;
+            int map[3] = {0, 0, 0};
+            for (int k = 0; k < 4; ++k) {
+                if (vid[0] == e4[k]) map[0] = k;
+                else if (vid[1] == e4[k]) map[1] = k;
+                else if (nv == 3 && vid[2] == e4[k]) map[2] = k;
+            }
+            V3 p[3];
+            for (int k = 0; k < nv; ++k) p[k] = m.x[vid[k]];
+            double dd, gl[9], Hl[81];
+            deriv_any(type, p, dd, gl, Hl);
+            const int n = 3 * nv;
+            for (int a = 0; a < nv; ++a) {
+                for (int r = 0; r < 3; ++r) gd[3 * map[a] + r] = gl[3 * a + r];
+                for (int bb = 0; bb < nv; ++bb)
+                    for (int r = 0; r < 3; ++r)
+                        for (int s = 0; s < 3; ++s) Hd[12 * (3 * map[a] + r) + 3 * map[bb] + s] = Hl[n * (3 * a + r) + 3 * bb + s];
+            }
+        }
+        const double ex = eps_x(m, e4[0], e4[1], e4[2], e4[3]);
+        V3 p[4] = {m.x[e4[0]], m.x[e4[1]], m.x[e4[2]], m.x[e4[3]]};
+        const double e = mol_e(ee_cross_sqn(p[0], p[1], p[2], p[3]), ex);
+        double eg[12], eH[144];
+        mol_eg(p, ex, eg); mol_eH(p, ex, eH);
+        for (int a = 0; a < 12; ++a)
+            for (int bb = 0; bb < 12; ++bb) {
+                const double kgg = ((kappa * g_b) * gd[a]) * eg[bb];
+                const double kggT = ((kappa * g_b) * gd[bb]) * eg[a];
+                He[i][12 * a + bb] = kgg + kggT + (kappa * b) * eH[12 * a + bb] + ((kappa * e * H_b) * gd[a]) * gd[bb]
+                                   + (kappa * e * g_b) * Hd[12 * a + bb];
+            }
+        make_pd<12>(He[i].data());
+        De[i] = {e4[0], e4[1], e4[2], e4[3]};
+    }
+    for (size_t i = 0; i < ne; ++i) { BH.h12.push_back(He[i]); BH.d4.push_back(De[i]); }
+    return collisions;
+}
+
+// BHessian::toTriplets + NewtonLinearSystem::assemble (ContactMechanics.cpp:13-134;
+// NewtonLinearSystem.cpp:126-154).  setFromTriplets sums duplicates in insertion order.
+void assemble_csr(Mesh& m, const Blocks& BH) {
+    struct Tri { int r, c; double v; };
+    std::vector<Tri> trip;
+    auto append = [&](int nv, const double* H, const int* vid) {
+        const int n = 3 * nv;
+        for (int a = 0; a < nv; ++a) {
+            const int rv = vid[a];
+            if (rv < 0 || rv >= m.V || !isFree(m.bt[rv])) continue;
+            for (int b = 0; b < nv; ++b) {
+                const int cv = vid[b];
+                if (cv < 0 || cv >= m.V || !isFree(m.bt[cv])) continue;
+                for (int rd = 0; rd < 3; ++rd)
+                    for (int cd = 0; cd < 3; ++cd) {
+                        const int gr = 3 * rv + rd, gc = 3 * cv + cd;
+                        const double v = H[n * (3 * a + rd) + 3 * b + cd];
+                        if (gr >= gc && v != 0.0) trip.push_back({gr, gc, v});
+                    }
+            }
+        }
+    };
+    for (size_t i = 0; i < BH.h3.size(); ++i) append(1, BH.h3[i].data(), &BH.d1[i]);
+    for (size_t i = 0; i < BH.h6.size(); ++i) append(2, BH.h6[i].data(), BH.d2[i].data());
+    for (size_t i = 0; i < BH.h9.size(); ++i) append(3, BH.h9[i].data(), BH.d3[i].data());
+    for (size_t i = 0; i < BH.h12.size(); ++i) append(4, BH.h12[i].data(), BH.d4[i].data());
+    for (int dof = 0; dof < 3 * m.V; ++dof) trip.push_back({dof, dof, m.mass[dof / 3] * (1.0 + m.drag)});
+    const int n = 3 * m.V;
+    std::vector<int> cnt(n + 1, 0);
+    for (const auto& t : trip) cnt[t.c + 1]++;
+    for (int i = 0; i < n; ++i) cnt[i + 1] += cnt[i];
+    std::vector<int> pos(cnt.begin(), cnt.end() - 1);
+    std::vector<int> order(trip.size());
+    for (size_t i = 0; i < trip.size(); ++i) order[pos[trip[i].c]++] = (int)i;
+    m.outer.assign(n + 1, 0); m.inner.clear(); m.val.clear();
+    for (int c = 0; c < n; ++c) {
+        std::vector<int> ids(order.begin() + cnt[c], order.begin() + cnt[c + 1]);
+        std::stable_sort(ids.begin(), ids.end(), [&](int a, int b) { return trip[a].r < trip[b].r; });
+        for (size_t k = 0; k < ids.size(); ++k) {
+            const Tri& t = trip[ids[k]];
+            if (k > 0 && trip[ids[k - 1]].r == t.r) m.val.back() += t.v;
+            else { m.inner.push_back(t.r); m.val.push_back(t.v); }
+        }
+        m.outer[c + 1] = (int)m.inner.size();
+    }
+    m.rhs.assign(n, 0.0);
+    for (int v = 0; v < m.V; ++v)
+        if (isFree(m.bt[v])) { m.rhs[3 * v] = m.grad[v].x; m.rhs[3 * v + 1] = m.grad[v].y; m.rhs[3 * v + 2] = m.grad[v].z; }
+}
+
+// ----------------------------------------------------------------------------------
+// computeEnergy (IPCSolver.cpp:322-456), tet + inertia + barrier subset
+// ----------------------------------------------------------------------------------
+void energy_terms(Mesh& m, double kappa, double out[5]) {
+    std::vector<double> et(m.T);
+#pragma omp parallel for schedule(dynamic, 256)
+    for (int t = 0; t < m.T; ++t) et[t] = snh_energy_density(deformation(m, t), m.u, m.r) * m.vol[t];
+    double esnh = 0;
+    for (int t = 0; t < m.T; ++t) esnh += et[t];
+    double de = 0;
+    for (int v = 0; v < m.V; ++v) {
+        de += sqn(m.x[v] - m.xt[v]) * m.mass[v] / 2.0;
+        de += (sqn(m.x[v] - m.vprev[v]) * (m.drag) * m.mass[v] / 2.0);
+    }
+    double eg = 0, es = 0, ee = 0;
+    for (int v : m.envAS) {
+        const double dist = dot(m.gn, m.x[v]) - m.gd;
+        double b; barrier_b(dist * dist, m.hhat, b); eg += b;
+    }
+    for (const I4& c : m.selfAS) {
+        double b; barrier_b(self_val(m, c), m.hhat, b);
+        if (c[3] < -1) b *= -c[3];
+        es += b;
+    }
+    for (size_t i = 0; i < m.eeAS.size(); ++i) {
+        const I4& c = m.eeAS[i];
+        int e4[4];
+        if (c[3] >= 0) { for (int k = 0; k < 4; ++k) e4[k] = c[k]; }
+        else {
+            const auto& eI = m.se[m.eeIe[i].first]; const auto& eJ = m.se[m.eeIe[i].second];
+            e4[0] = eI.first; e4[1] = eI.second; e4[2] = eJ.first; e4[3] = eJ.second;
+        }
+        const double ex = eps_x(m, e4[0], e4[1], e4[2], e4[3]);
+        const double e = mol_e(ee_cross_sqn(m.x[e4[0]], m.x[e4[1]], m.x[e4[2]], m.x[e4[3]]), ex);
+        double b; barrier_b(self_val(m, c), m.hhat, b);
+        ee += b * e;
+    }
+    out[0] = esnh; out[1] = de; out[2] = eg; out[3] = es; out[4] = ee;
+    (void)kappa;
+}
+double energy(Mesh& m, double kappa) {
+    double t[5];
+    energy_terms(m, kappa, t);
+    double E = t[0];
+    E *= m.dt * m.dt;
+    E += t[1];
+    E += kappa * (t[2] + t[3] + t[4]);
+    return E;
+}
+
+// ----------------------------------------------------------------------------------
+// Broad phase and active set (CollisionBroadPhase.cpp:765-1092; LBVH.cpp:284-339)
+// ----------------------------------------------------------------------------------
+void build_static(Mesh& m) {
+    m.faceBoxes.resize(m.F);
+    for (int f = 0; f < m.F; ++f) m.faceBoxes[f] = triBox(m.x[m.faces[f][0]], m.x[m.faces[f][1]], m.x[m.faces[f][2]]);
+    m.edgeBoxes.resize(m.Es);
+    for (int e = 0; e < m.Es; ++e) m.edgeBoxes[e] = edgeBox(m.x[m.se[e].first], m.x[m.se[e].second]);
+    m.sweptPointBoxes.clear();
+    m.faceTree.build(m.faceBoxes);
+    m.edgeTree.build(m.edgeBoxes);
+}
+void build_swept(Mesh& m, const std::vector<V3>& p, double a) {
+    m.sweptPointBoxes.resize(m.Vs);
+    for (int s = 0; s < m.Vs; ++s) m.sweptPointBoxes[s] = sweptPoint(m.x[m.sv[s]], p[m.sv[s]], a);
+    m.faceBoxes.resize(m.F);
+    for (int f = 0; f < m.F; ++f) {
+        const auto& fv = m.faces[f];
+        m.faceBoxes[f] = sweptTri(m.x[fv[0]], p[fv[0]], m.x[fv[1]], p[fv[1]], m.x[fv[2]], p[fv[2]], a);
+    }
+    m.edgeBoxes.resize(m.Es);
+    for (int e = 0; e < m.Es; ++e) m.edgeBoxes[e] = sweptEdge(m.x[m.se[e].first], p[m.se[e].first], m.x[m.se[e].second], p[m.se[e].second], a);
+    m.faceTree.build(m.faceBoxes);
+    m.edgeTree.build(m.edgeBoxes);
+}
+
+void active_set(Mesh& m) {
+    build_static(m);
+    const double sq = std::sqrt(m.hhat);
+    std::vector<std::vector<I4>> csPT(m.Vs), csEE(m.Es);
+    std::vector<std::vector<int>> ccPT(m.Vs), ccEE(m.Es);
+#pragma omp parallel
+    {
+        std::vector<int> cand;
+#pragma omp for schedule(dynamic, 64)
+        for (int s = 0; s < m.Vs; ++s) {
+            const int vI = m.sv[s];
+            m.faceTree.query(pointBox(m.x[vI]), sq, cand);
+            const AABB pb = pointBox(m.x[vI]);
+            for (int f : cand) {
+                const auto& fv = m.faces[f];
+                if (isCollider(m.bt[vI]) && isCollider(m.bt[fv[0]]) && isCollider(m.bt[fv[1]]) && isCollider(m.bt[fv[2]])) continue;
+                if (vI == fv[0] || vI == fv[1] || vI == fv[2]) continue;
+                if (!pb.overlaps(triBox(m.x[fv[0]], m.x[fv[1]], m.x[fv[2]]), sq)) continue;
+                ccPT[s].push_back(f);
+                const int dt = dType_PT(m.x[vI], m.x[fv[0]], m.x[fv[1]], m.x[fv[2]]);
+                const V3 &P = m.x[vI], &A = m.x[fv[0]], &B = m.x[fv[1]], &C = m.x[fv[2]];
+                double d;
+                switch (dt) {
+                case 0: d = d_PP(P, A); if (d < m.hhat) csPT[s].push_back({-vI - 1, fv[0], -1, -1}); break;
+                case 1: d = d_PP(P, B); if (d < m.hhat) csPT[s].push_back({-vI - 1, fv[1], -1, -1}); break;
+                case 2: d = d_PP(P, C); if (d < m.hhat) csPT[s].push_back({-vI - 1, fv[2], -1, -1}); break;
+                case 3: d = d_PE(P, A, B); if (d < m.hhat) csPT[s].push_back({-vI - 1, fv[0], fv[1], -1}); break;
+                case 4: d = d_PE(P, B, C); if (d < m.hhat) csPT[s].push_back({-vI - 1, fv[1], fv[2], -1}); break;
+                case 5: d = d_PE(P, C, A); if (d < m.hhat) csPT[s].push_back({-vI - 1, fv[2], fv[0], -1}); break;
+                case 6: d = d_PT(P, A, B, C); if (d < m.hhat) csPT[s].push_back({-vI - 1, fv[0], fv[1], fv[2]}); break;
+                default: break;
+                }
+            }
+        }
+#pragma omp for schedule(dynamic, 64)
+        for (int eI = 0; eI < m.Es; ++eI) {
+            const auto& EI = m.se[eI];
+            m.edgeTree.query(edgeBox(m.x[EI.first], m.x[EI.second]), sq, cand);
+            cand.erase(std::remove_if(cand.begin(), cand.end(), [eI](int c) { return c <= eI; }), cand.end());
+            const AABB b1 = edgeBox(m.x[EI.first], m.x[EI.second]);
+            for (int eJ : cand) {
+                const auto& EJ = m.se[eJ];
+                if (isCollider(m.bt[EI.first]) && isCollider(m.bt[EI.second]) && isCollider(m.bt[EJ.first]) && isCollider(m.bt[EJ.second])) continue;
+                if (EI.first == EJ.first || EI.first == EJ.second || EI.second == EJ.first || EI.second == EJ.second || eI > eJ) continue;
+                if (!b1.overlaps(edgeBox(m.x[EJ.first], m.x[EJ.second]), sq)) continue;
+                ccEE[eI].push_back(eJ);
+                const V3 &a0 = m.x[EI.first], &a1 = m.x[EI.second], &b0 = m.x[EJ.first], &b1v = m.x[EJ.second];
+                const int dt = dType_EE(a0, a1, b0, b1v);
+                const double cr = ee_cross_sqn(a0, a1, b0, b1v);
+                const double ex = eps_x(m, EI.first, EI.second, EJ.first, EJ.second);
+                const int add_e = (cr < ex) ? -eJ - 2 : -1;
+                double d;
+                switch (dt) {
+                case 0: d = d_PP(a0, b0); if (d < m.hhat) csEE[eI].push_back({-EI.first - 1, EJ.first, -1, add_e}); break;
+                case 1: d = d_PP(a0, b1v); if (d < m.hhat) csEE[eI].push_back({-EI.first - 1, EJ.second, -1, add_e}); break;
+                case 2: d = d_PE(a0, b0, b1v); if (d < m.hhat) csEE[eI].push_back({-EI.first - 1, EJ.first, EJ.second, add_e}); break;
+                case 3: d = d_PP(a1, b0); if (d < m.hhat) csEE[eI].push_back({-EI.second - 1, EJ.first, -1, add_e}); break;
+                case 4: d = d_PP(a1, b1v); if (d < m.hhat) csEE[eI].push_back({-EI.second - 1, EJ.second, -1, add_e}); break;
+                case 5: d = d_PE(a1, b0, b1v); if (d < m.hhat) csEE[eI].push_back({-EI.second - 1, EJ.first, EJ.second, add_e}); break;
+                case 6: d = d_PE(b0, a0, a1); if (d < m.hhat) csEE[eI].push_back({-EJ.first - 1, EI.first, EI.second, add_e}); break;
+                case 7: d = d_PE(b1v, a0, a1); if (d < m.hhat) csEE[eI].push_back({-EJ.second - 1, EI.first, EI.second, add_e}); break;
+                case 8:
+                    d = d_EE(a0, a1, b0, b1v);
+                    if (d < m.hhat) {
+                        if (add_e <= -2) csEE[eI].push_back({EI.first, EI.second, EJ.first, -EJ.second - m.Es - 2});
+                        else csEE[eI].push_back({EI.first, EI.second, EJ.first, EJ.second});
+                    }
+                    break;
+                default: break;
+                }
+            }
+        }
+    }
+    // serial compaction (CollisionBroadPhase.cpp:990-1058)
+    m.ccdAS.clear();
+    for (int s = 0; s < m.Vs; ++s) for (int f : ccPT[s]) m.ccdAS.emplace_back(-s - 1, f);
+    for (int e = 0; e < m.Es; ++e) for (int f : ccEE[e]) m.ccdAS.emplace_back(e, f);
+    m.selfAS.clear(); m.eeAS.clear(); m.eeIe.clear();
+    std::map<I4, int> counter;
+    for (const auto& cs : csPT) for (const auto& c : cs) { if (c[3] < 0) ++counter[c]; else m.selfAS.push_back(c); }
+    for (int eI = 0; eI < m.Es; ++eI)
+        for (const auto& c : csEE[eI]) {
+            if (c[3] >= 0) m.selfAS.push_back(c);
+            else if (c[3] == -1) ++counter[c];
+            else if (c[3] >= -m.Es - 1) { m.eeAS.push_back({c[0], c[1], c[2], -1}); m.eeIe.emplace_back(eI, -c[3] - 2); }
+            else { m.eeAS.push_back({c[0], c[1], c[2], -c[3] - m.Es - 2}); m.eeIe.emplace_back(-1, -1); }
+        }
+    for (const auto& kv : counter) m.selfAS.push_back({kv.first[0], kv.first[1], kv.first[2], -kv.second});
+    // Ground::calculateActivateSet (CollisionBroadPhase.cpp:1076-1092), ascending surface order
+    m.envAS.clear();
+    for (int s = 0; s < m.Vs; ++s) {
+        const double dist = dot(m.gn, m.x[m.sv[s]]) - m.gd;
+        if (dist * dist < m.hhat) m.envAS.push_back(m.sv[s]);
+    }
+}
+
+// ----------------------------------------------------------------------------------
+// Additive CCD (AdditiveCCD.cpp:8-329)
+// ----------------------------------------------------------------------------------
+int pt_type_ccd(const V3& p, const V3& t0, const V3& t1, const V3& t2) {   // :8-62
+    const V3 n = cross(t1 - t0, t2 - t0);
+    double p00, p10, p01, p11, p02, p12;
+    pt_param(t1 - t0, n, p - t0, p00, p10);
+    if (p00 > 0.0 && p00 < 1.0 && p10 >= 0.0) return 3;
+    pt_param(t2 - t1, n, p - t1, p01, p11);
+    if (p01 > 0.0 && p01 < 1.0 && p11 >= 0.0) return 4;
+    pt_param(t0 - t2, n, p - t2, p02, p12);
+    if (p02 > 0.0 && p02 < 1.0 && p12 >= 0.0) return 5;
+    if (p00 <= 0.0 && p02 >= 1.0) return 0;
+    if (p01 <= 0.0 && p00 >= 1.0) return 1;
+    if (p02 <= 0.0 && p01 >= 1.0) return 2;
+    return 6;
+}
+double ccd_pe(const V3& p, const V3& e0, const V3& e1) { return sqn(cross(e0 - p, e1 - p)) / sqn(e1 - e0); }  // :146-149
+double ccd_pt(const V3& p, const V3& t0, const V3& t1, const V3& t2) {                                     // :151-156
+    const V3 b = cross(t1 - t0, t2 - t0);
+    const double aTb = dot(p - t0, b);
+    return aTb * aTb / sqn(b);
+}
+double ccd_ee(const V3& a0, const V3& a1, const V3& b0, const V3& b1) {                                     // :158-163
+    const V3 b = cross(a1 - a0, b1 - b0);
+    const double aTb = dot(b0 - a0, b);
+    return aTb * aTb / sqn(b);
+}
+double ee_unclassified(const V3& a0, const V3& a1, const V3& b0, const V3& b1) {                            // :165-191
+    switch (dType_EE(a0, a1, b0, b1)) {
+    case 0: return sqn(a0 - b0);
+    case 1: return sqn(a0 - b1);
+    case 2: return ccd_pe(a0, b0, b1);
+    case 3: return sqn(a1 - b0);
+    case 4: return sqn(a1 - b1);
+    case 5: return ccd_pe(a1, b0, b1);
+    case 6: return ccd_pe(b0, a0, a1);
+    case 7: return ccd_pe(b1, a0, a1);
+    case 8: return ccd_ee(a0, a1, b0, b1);
+    default: return std::numeric_limits<double>::max();
+    }
+}
+double pt_unclassified(const V3& p, const V3& t0, const V3& t1, const V3& t2) {                             // :193-219
+    switch (pt_type_ccd(p, t0, t1, t2)) {
+    case 0: return sqn(p - t0);
+    case 1: return sqn(p - t1);
+    case 2: return sqn(p - t2);
+    case 3: return ccd_pe(p, t0, t1);
+    case 4: return ccd_pe(p, t1, t2);
+    case 5: return ccd_pe(p, t2, t0);
+    case 6: return ccd_pt(p, t0, t1, t2);
+    default: return std::numeric_limits<double>::max();
+    }
+}
+double accd_ee(V3 ea0, V3 ea1, V3 eb0, V3 eb1, V3 dea0, V3 dea1, V3 deb0, V3 deb1, double eta, double th) {  // :221-279
+    const V3 mov = (dea0 + dea1 + deb0 + deb1) / 4;
+    dea0 = dea0 - mov; dea1 = dea1 - mov; deb0 = deb0 - mov; deb1 = deb1 - mov;
+    const double maxd = std::sqrt(std::max(sqn(dea0), sqn(dea1))) + std::sqrt(std::max(sqn(deb0), sqn(deb1)));
+    if (maxd == 0) return 1.0;
+    auto minpp = [&]() {
+        double a = sqn(ea0 - eb0), b = sqn(ea0 - eb1), c = sqn(ea1 - eb0), d = sqn(ea1 - eb1);
+        return std::min(std::min(a, b), std::min(c, d));
+    };
+    double d2 = ee_unclassified(ea0, ea1, eb0, eb1);
+    double dF = d2 - th * th;
+    if (dF <= 0) { d2 = minpp(); dF = d2 - th * th; }
+    double dc = std::sqrt(d2);
+    const double gap = eta * dF / (dc + th);
+    double toc = 0.0;
+    for (int it = 0; it < 10000; ++it) {
+        const double tl = (1 - eta) * dF / ((dc + th) * maxd);
+        if (!std::isfinite(tl) || tl <= 0.0) return std::max(0.0, std::min(toc, 1.0));
+        ea0 = ea0 + tl * dea0; ea1 = ea1 + tl * dea1; eb0 = eb0 + tl * deb0; eb1 = eb1 + tl * deb1;
+        d2 = ee_unclassified(ea0, ea1, eb0, eb1);
+        dF = d2 - th * th;
+        if (dF <= 0) { d2 = minpp(); dF = d2 - th * th; }
+        dc = std::sqrt(d2);
+        if (toc != 0.0 && (dF / (dc + th) < gap)) break;
+        toc += tl;
+        if (toc > 1.0) return 1.0;
+    }
+    return std::max(0.0, std::min(toc, 1.0));
+}
+double accd_pt(V3 p, V3 t0, V3 t1, V3 t2, V3 dp, V3 dt0, V3 dt1, V3 dt2, double eta, double th) {          // :281-329
+    const V3 mov = (dt0 + dt1 + dt2 + dp) / 4;
+    dt0 = dt0 - mov; dt1 = dt1 - mov; dt2 = dt2 - mov; dp = dp - mov;
+    const double mx = std::max(std::max(sqn(dt0), sqn(dt1)), sqn(dt2));
+    const double maxd = std::sqrt(sqn(dp)) + std::sqrt(mx);
+    if (maxd == 0) return 1.0;
+    double d2 = pt_unclassified(p, t0, t1, t2);
+    double dc = std::sqrt(d2);
+    const double gap = eta * (d2 - th * th) / (dc + th);
+    double toc = 0.0;
+    for (int it = 0; it < 10000; ++it) {
+        const double tl = (1 - eta) * (d2 - th * th) / ((dc + th) * maxd);
+        if (!std::isfinite(tl) || tl <= 0.0) return std::max(0.0, std::min(toc, 1.0));
+        p = p + tl * dp; t0 = t0 + tl * dt0; t1 = t1 + tl * dt1; t2 = t2 + tl * dt2;
+        d2 = pt_unclassified(p, t0, t1, t2);
+        dc = std::sqrt(d2);
+        if (toc != 0.0 && ((d2 - th * th) / (dc + th) < gap)) break;
+        toc += tl;
+        if (toc > 1.0) return 1.0;
+    }
+    return std::max(0.0, std::min(toc, 1.0));
+}
+
+// Newton step bound: limitStepByGround (FeasibleStep.cpp:8-40), partial CCD
+// (ContactMechanics.cpp:3704-3752), CFL (IPCSolver.cpp:960-974), swept rebuild and full
+// CCD (ContactMechanics.cpp:3754-3867), merge (IPCSolver.cpp:985-989).
+void step_bound(Mesh& m, const std::vector<V3>& p, double out[5]) {
+    const double slack = 0.8;
+    double alpha = 1.0;
+    {   // ground
+        double mn = 1.0;
+        for (int s = 0; s < m.Vs; ++s) {
+            double best = 1.0;
+            const int v = m.sv[s];
+            const double nm = dot(m.gn, -p[v]);
+            if (!(nm >= 0.0)) {
+                const double dist = dot(m.gn, m.x[v]) - m.gd;
+                const double cand = -dist / nm * slack;
+                if (cand >= 0.0) best = cand;
+            }
+            mn = std::min(mn, best);
+        }
+        if (m.Vs > 0) alpha = std::min(alpha, mn);
+    }
+    const double aground = alpha;
+    if (alpha <= 0) alpha = 1;
+    const double eta = 1.0 - slack;
+    double partial = alpha;
+    if (!m.ccdAS.empty()) {
+        std::vector<double> la(m.ccdAS.size());
+#pragma omp parallel for schedule(dynamic, 64)
+        for (long i = 0; i < (long)m.ccdAS.size(); ++i) {
+            const auto& pr = m.ccdAS[i];
+            if (pr.first < 0) {
+                const int pt = m.sv[-pr.first - 1];
+                const auto& f = m.faces[pr.second];
+                la[i] = accd_pt(m.x[pt], m.x[f[0]], m.x[f[1]], m.x[f[2]], -p[pt], -p[f[0]], -p[f[1]], -p[f[2]], eta, 0.0);
+            } else {
+                const auto& e1 = m.se[pr.first]; const auto& e2 = m.se[pr.second];
+                la[i] = accd_ee(m.x[e1.first], m.x[e1.second], m.x[e2.first], m.x[e2.second],
+                                -p[e1.first], -p[e1.second], -p[e2.first], -p[e2.second], eta, 0.0);
+            }
+        }
+        double mn = la[0];
+        for (double v : la) mn = std::min(mn, v);
+        partial = std::min(partial, mn);
+    }
+    double maxmove = 0.0;
+    for (int s = 0; s < m.Vs; ++s) maxmove = std::max(maxmove, std::sqrt(sqn(p[m.sv[s]])));
+    const double acfl = maxmove > 0.0 ? std::sqrt(m.hhat) / (maxmove * 2.0) : 1.0;
+    double full = alpha;
+    build_swept(m, p, full);
+    {   // PT phase
+        const double a = full;
+        std::vector<double> lt(m.Vs, 1.0);
+#pragma omp parallel
+        {
+            std::vector<int> cand;
+#pragma omp for schedule(dynamic, 64)
+            for (int s = 0; s < m.Vs; ++s) {
+                const int vI = m.sv[s];
+                m.faceTree.query(m.sweptPointBoxes[s], 0.0, cand);
+                const AABB pb = sweptPoint(m.x[vI], p[vI], a);
+                for (int f : cand) {
+                    const auto& fv = m.faces[f];
+                    if (isCollider(m.bt[vI]) && isCollider(m.bt[fv[0]]) && isCollider(m.bt[fv[1]]) && isCollider(m.bt[fv[2]])) continue;
+                    if (vI == fv[0] || vI == fv[1] || vI == fv[2]) continue;
+                    const AABB fb = sweptTri(m.x[fv[0]], p[fv[0]], m.x[fv[1]], p[fv[1]], m.x[fv[2]], p[fv[2]], a);
+                    if (!pb.overlaps(fb, 0.0)) continue;
+                    const double la = accd_pt(m.x[vI], m.x[fv[0]], m.x[fv[1]], m.x[fv[2]], -p[vI], -p[fv[0]], -p[fv[1]], -p[fv[2]], eta, 0);
+                    lt[s] = std::min(la, lt[s]);
+                }
+            }
+        }
+        double mn = 1.0;
+        if (m.Vs > 0) { mn = lt[0]; for (double v : lt) mn = std::min(mn, v); }
+        full = std::min(full, mn);
+    }
+    {   // EE phase
+        const double a = full;
+        std::vector<double> le(m.Es, 1.0);
+#pragma omp parallel
+        {
+            std::vector<int> cand;
+#pragma omp for schedule(dynamic, 64)
+            for (int eJ = 0; eJ < m.Es; ++eJ) {
+                const auto& EJ = m.se[eJ];
+                m.edgeTree.query(m.edgeBoxes[eJ], 0.0, cand);
+                cand.erase(std::remove_if(cand.begin(), cand.end(), [eJ](int c) { return c <= eJ; }), cand.end());
+                const AABB b1 = sweptEdge(m.x[EJ.first], p[EJ.first], m.x[EJ.second], p[EJ.second], a);
+                for (int eI : cand) {
+                    const auto& EI = m.se[eI];
+                    if (isCollider(m.bt[EI.first]) && isCollider(m.bt[EI.second]) && isCollider(m.bt[EJ.first]) && isCollider(m.bt[EJ.second])) continue;
+                    if (EI.first == EJ.first || EI.first == EJ.second || EI.second == EJ.first || EI.second == EJ.second || eI < eJ) continue;
+                    const AABB b2 = sweptEdge(m.x[EI.first], p[EI.first], m.x[EI.second], p[EI.second], a);
+                    if (!b1.overlaps(b2, 0.0)) continue;
+                    const double la = accd_ee(m.x[EI.first], m.x[EI.second], m.x[EJ.first], m.x[EJ.second],
+                                              -p[EI.first], -p[EI.second], -p[EJ.first], -p[EJ.second], eta, 0);
+                    le[eJ] = std::min(la, le[eJ]);
+                }
+            }
+        }
+        double mn = 1.0;
+        if (m.Es > 0) { mn = le[0]; for (double v : le) mn = std::min(mn, v); }
+        full = std::min(full, mn);
+    }
+    alpha = std::min(alpha, acfl);
+    if (partial > 2.0 * acfl) {
+        alpha = std::min(partial, full);
+        alpha = std::max(alpha, acfl);
+    }
+    out[0] = alpha; out[1] = aground; out[2] = partial; out[3] = acfl; out[4] = full;
+}
+
+// IglUtils::segTriIntersect (ContactMechanics.h:47-71) with Eigen 3.4 FullPivLU on the
+// 3x3 system: complete pivoting on the first largest |a_ij| in column-major order,
+// rank threshold 3 eps |max pivot|.
+bool seg_tri(const V3& e0, const V3& e1, const V3& t0, const V3& t1, const V3& t2) {
+    M3 A;
+    const V3 c0 = t1 - t0, c1 = t2 - t0, c2 = e0 - e1;
+    for (int i = 0; i < 3; ++i) { A(i, 0) = c0[i]; A(i, 1) = c1[i]; A(i, 2) = c2[i]; }
+    const V3 n = cross(c0, c1);
+    if (dot(n, e0 - t0) * dot(n, e1 - t0) > 0.0) return false;
+    const double det = A(0, 0) * (A(1, 1) * A(2, 2) - A(1, 2) * A(2, 1)) - A(0, 1) * (A(1, 0) * A(2, 2) - A(1, 2) * A(2, 0))
+                     + A(0, 2) * (A(1, 0) * A(2, 1) - A(1, 1) * A(2, 0));
+    if (det == 0.0) return false;
+    M3 lu = A;
+    int rt[3], ct[3], nz = 3;
+    double maxpiv = 0;
+    for (int k = 0; k < 3; ++k) {
+        int br = k, bc = k; double best = std::fabs(lu(k, k));
+        for (int c = k; c < 3; ++c)
+            for (int r = k; r < 3; ++r)
+                if (std::fabs(lu(r, c)) > best) { best = std::fabs(lu(r, c)); br = r; bc = c; }
+        if (best == 0.0) { nz = k; for (int i = k; i < 3; ++i) { rt[i] = i; ct[i] = i; } break; }
+        if (best > maxpiv) maxpiv = best;
+        rt[k] = br; ct[k] = bc;
+        if (br != k) for (int c = 0; c < 3; ++c) std::swap(lu(k, c), lu(br, c));
+        if (bc != k) for (int r = 0; r < 3; ++r) std::swap(lu(r, k), lu(r, bc));
+        for (int r = k + 1; r < 3; ++r) lu(r, k) /= lu(k, k);
+        for (int c = k + 1; c < 3; ++c)
+            for (int r = k + 1; r < 3; ++r) lu(r, c) -= lu(r, k) * lu(k, c);
+    }
+    int rank = 0;
+    const double thr = maxpiv * (std::numeric_limits<double>::epsilon() * 3.0);
+    for (int i = 0; i < nz; ++i) rank += std::fabs(lu(i, i)) > thr;
+    V3 rhs = e0 - t0;
+    double c[3] = {rhs.x, rhs.y, rhs.z}, x[3] = {0, 0, 0};
+    if (rank > 0) {
+        for (int k = 0; k < 3; ++k) std::swap(c[k], c[rt[k]]);
+        // Eigen's triangular_solve_vector skips a column whose right-hand side is exactly 0
+        for (int i = 0; i < 3; ++i) if (c[i] != 0.0) for (int r = i + 1; r < 3; ++r) c[r] -= c[i] * lu(r, i);
+        for (int i = rank - 1; i >= 0; --i) {
+            if (c[i] == 0.0) continue;
+            c[i] /= lu(i, i);
+            for (int r = 0; r < i; ++r) c[r] -= c[i] * lu(r, i);
+        }
+        int q[3] = {0, 1, 2};
+        for (int k = 0; k < 3; ++k) std::swap(q[k], q[ct[k]]);
+        for (int i = 0; i < rank; ++i) x[q[i]] = c[i];
+        for (int i = rank; i < 3; ++i) x[q[i]] = 0.0;
+    }
+    return x[0] >= 0.0 && x[1] >= 0.0 && x[0] + x[1] <= 1.0 && x[2] >= 0.0 && x[2] <= 1.0;
+}
+
+}  // namespace
+
+struct oracle_mesh { Mesh m; };
+
+extern "C" {
+
+oracle_mesh* oracle_create(const oracle_scene_desc* d) {
+    auto* om = new oracle_mesh;
+    Mesh& m = om->m;
+    m.V = d->V; m.T = d->T; m.F = d->F; m.Vs = d->Vs; m.Es = d->Es;
+    auto v3s = [&](const double* p, std::vector<V3>& out) {
+        out.resize(m.V);
+        for (int i = 0; i < m.V; ++i) out[i] = V3(p[3 * i], p[3 * i + 1], p[3 * i + 2]);
+    };
+    v3s(d->x, m.x); v3s(d->v_rest, m.vrest); v3s(d->V_prev, m.vprev); v3s(d->xtilde, m.xt);
+    m.mass.assign(d->masses, d->masses + m.V);
+    m.bt.assign(d->boundary, d->boundary + m.V);
+    m.tets.resize(m.T); m.dminv.resize(m.T); m.vol.assign(d->volumes, d->volumes + m.T);
+    for (int t = 0; t < m.T; ++t) {
+        for (int k = 0; k < 4; ++k) m.tets[t][k] = d->tets[4 * t + k];
+        for (int k = 0; k < 9; ++k) m.dminv[t].m[k] = d->dm_inv[9 * t + k];
+    }
+    m.faces.resize(m.F);
+    for (int f = 0; f < m.F; ++f) for (int k = 0; k < 3; ++k) m.faces[f][k] = d->faces[4 * f + k];
+    m.sv.resize(m.Vs);
+    for (int s = 0; s < m.Vs; ++s) m.sv[s] = (int)d->surf_verts[s];
+    m.se.resize(m.Es);
+    for (int e = 0; e < m.Es; ++e) m.se[e] = {(int)d->surf_edges[2 * e], (int)d->surf_edges[2 * e + 1]};
+    m.dt = d->dt; m.u = d->length_rate; m.r = d->volume_rate; m.drag = d->drag;
+    m.hhat = d->hhat; m.fhat = d->fhat; m.fric = d->friction;
+    m.gn = V3(d->ground_normal[0], d->ground_normal[1], d->ground_normal[2]); m.gd = d->ground_d;
+    return om;
+}
+void oracle_destroy(oracle_mesh* om) { delete om; }
+void oracle_set_threads(oracle_mesh* om, int threads) { om->m.threads = threads; if (threads > 0) omp_set_num_threads(threads); }
+void oracle_set_state(oracle_mesh* om, const double* x, const double* vp, const double* xt) {
+    Mesh& m = om->m;
+    for (int i = 0; i < m.V; ++i) {
+        if (x) m.x[i] = V3(x[3 * i], x[3 * i + 1], x[3 * i + 2]);
+        if (vp) m.vprev[i] = V3(vp[3 * i], vp[3 * i + 1], vp[3 * i + 2]);
+        if (xt) m.xt[i] = V3(xt[3 * i], xt[3 * i + 1], xt[3 * i + 2]);
+    }
+}
+void oracle_active_set(oracle_mesh* om, int64_t counts[4]) {
+    active_set(om->m);
+    counts[0] = om->m.selfAS.size(); counts[1] = om->m.eeAS.size(); counts[2] = om->m.ccdAS.size(); counts[3] = om->m.envAS.size();
+}
+void oracle_get_active_set(const oracle_mesh* om, int* s4, int* e4, int* eie, int* ccd, int* env) {
+    const Mesh& m = om->m;
+    for (size_t i = 0; i < m.selfAS.size(); ++i) for (int k = 0; k < 4; ++k) s4[4 * i + k] = m.selfAS[i][k];
+    for (size_t i = 0; i < m.eeAS.size(); ++i) { for (int k = 0; k < 4; ++k) e4[4 * i + k] = m.eeAS[i][k]; eie[2 * i] = m.eeIe[i].first; eie[2 * i + 1] = m.eeIe[i].second; }
+    for (size_t i = 0; i < m.ccdAS.size(); ++i) { ccd[2 * i] = m.ccdAS[i].first; ccd[2 * i + 1] = m.ccdAS[i].second; }
+    for (size_t i = 0; i < m.envAS.size(); ++i) env[i] = m.envAS[i];
+}
+void oracle_set_active_set(oracle_mesh* om, const int64_t c[4], const int* s4, const int* e4, const int* eie, const int* ccd, const int* env) {
+    Mesh& m = om->m;
+    m.selfAS.resize(c[0]); m.eeAS.resize(c[1]); m.eeIe.resize(c[1]); m.ccdAS.resize(c[2]); m.envAS.assign(env, env + c[3]);
+    for (int64_t i = 0; i < c[0]; ++i) for (int k = 0; k < 4; ++k) m.selfAS[i][k] = s4[4 * i + k];
+    for (int64_t i = 0; i < c[1]; ++i) { for (int k = 0; k < 4; ++k) m.eeAS[i][k] = e4[4 * i + k]; m.eeIe[i] = {eie[2 * i], eie[2 * i + 1]}; }
+    for (int64_t i = 0; i < c[2]; ++i) m.ccdAS[i] = {ccd[2 * i], ccd[2 * i + 1]};
+}
+int oracle_assemble(oracle_mesh* om, double kappa, int64_t* nnz) {
+    Blocks BH;
+    const int n = gradient_and_hessian(om->m, kappa, BH);
+    assemble_csr(om->m, BH);
+    *nnz = (int64_t)om->m.inner.size();
+    return n;
+}
+void oracle_get_gradient(const oracle_mesh* om, double* g) {
+    for (int v = 0; v < om->m.V; ++v) { g[3 * v] = om->m.grad[v].x; g[3 * v + 1] = om->m.grad[v].y; g[3 * v + 2] = om->m.grad[v].z; }
+}
+void oracle_get_csr(const oracle_mesh* om, int* outer, int* inner, double* val, double* rhs) {
+    const Mesh& m = om->m;
+    std::copy(m.outer.begin(), m.outer.end(), outer);
+    std::copy(m.inner.begin(), m.inner.end(), inner);
+    std::copy(m.val.begin(), m.val.end(), val);
+    if (rhs) std::copy(m.rhs.begin(), m.rhs.end(), rhs);
+}
+double oracle_energy(oracle_mesh* om, double kappa) { return energy(om->m, kappa); }
+void oracle_energy_terms(oracle_mesh* om, double kappa, double t[5]) { energy_terms(om->m, kappa, t); }
+void oracle_step_bound(oracle_mesh* om, const double* p, double out[5]) {
+    std::vector<V3> pv(om->m.V);
+    for (int i = 0; i < om->m.V; ++i) pv[i] = V3(p[3 * i], p[3 * i + 1], p[3 * i + 2]);
+    step_bound(om->m, pv, out);
+}
+void oracle_step_forward(oracle_mesh* om, const double* x0, const double* p, double a) {
+    Mesh& m = om->m;
+    for (int v = 0; v < m.V; ++v)
+        if (isFree(m.bt[v])) m.x[v] = V3(x0[3 * v], x0[3 * v + 1], x0[3 * v + 2]) - a * V3(p[3 * v], p[3 * v + 1], p[3 * v + 2]);
+}
+void oracle_get_positions(const oracle_mesh* om, double* x) {
+    for (int v = 0; v < om->m.V; ++v) { x[3 * v] = om->m.x[v].x; x[3 * v + 1] = om->m.x[v].y; x[3 * v + 2] = om->m.x[v].z; }
+}
+int oracle_is_intersected(oracle_mesh* om) {
+    Mesh& m = om->m;
+    for (int v : m.envAS) { const double d = dot(m.gn, m.x[v]) - m.gd; if (d * d <= 0.0) return 1; }
+    build_static(m);
+    int hit = 0;
+#pragma omp parallel
+    {
+        std::vector<int> cand;
+#pragma omp for schedule(dynamic, 64) reduction(| : hit)
+        for (int f = 0; f < m.F; ++f) {
+            const auto& fv = m.faces[f];
+            m.edgeTree.query(triBox(m.x[fv[0]], m.x[fv[1]], m.x[fv[2]]), 0.0, cand);
+            for (int e : cand) {
+                const auto& E = m.se[e];
+                if (E.first == fv[0] || E.first == fv[1] || E.first == fv[2] || E.second == fv[0] || E.second == fv[1] || E.second == fv[2]) continue;
+                if (isCollider(m.bt[E.first]) && isCollider(m.bt[E.second]) && isCollider(m.bt[fv[0]]) && isCollider(m.bt[fv[1]]) && isCollider(m.bt[fv[2]])) continue;
+                if (seg_tri(m.x[E.first], m.x[E.second], m.x[fv[0]], m.x[fv[1]], m.x[fv[2]])) { hit = 1; break; }
+            }
+        }
+    }
+    return hit;
+}
+
+void oracle_svd3(const double* F, double* U, double* s, double* V) {
+    M3 A, Um, Vm;
+    for (int i = 0; i < 9; ++i) A.m[i] = F[i];
+    svd3(A, Um, s, Vm);
+    for (int i = 0; i < 9; ++i) { U[i] = Um.m[i]; V[i] = Vm.m[i]; }
+}
+void oracle_snh_element(const double* Fp, double u, double r, double* P, double* H9, double* e) {
+    M3 F;
+    for (int i = 0; i < 9; ++i) F.m[i] = Fp[i];
+    const M3 Pm = snh_pk1(F, u, r);
+    for (int i = 0; i < 9; ++i) P[i] = Pm.m[i];
+    snh_hessian(F, u, r, H9);
+    *e = snh_energy_density(F, u, r);
+}
+int oracle_make_pd(int n, double* H) {
+    switch (n) {
+    case 3: return make_pd<3>(H);
+    case 6: return make_pd<6>(H);
+    case 9: return make_pd<9>(H);
+    case 12: return make_pd<12>(H);
+    default: return -1;
+    }
+}
+void oracle_barrier(double d, double dh, double* b, double* g, double* h) { barrier_b(d, dh, *b); barrier_g(d, dh, *g); barrier_h(d, dh, *h); }
+void oracle_distance(int type, const double* pts, double* d, double* g, double* H) {
+    V3 p[4];
+    for (int k = 0; k < 4; ++k) p[k] = V3(pts[3 * k], pts[3 * k + 1], pts[3 * k + 2]);
+    deriv_any(type, p, *d, g, H);
+}
+int oracle_dtype_pt(const double* q) {
+    return dType_PT(V3(q[0], q[1], q[2]), V3(q[3], q[4], q[5]), V3(q[6], q[7], q[8]), V3(q[9], q[10], q[11]));
+}
+int oracle_dtype_ee(const double* q) {
+    return dType_EE(V3(q[0], q[1], q[2]), V3(q[3], q[4], q[5]), V3(q[6], q[7], q[8]), V3(q[9], q[10], q[11]));
+}
+double oracle_accd_pt(const double* q, const double* d, double eta, double th) {
+    auto v = [](const double* a, int k) { return V3(a[3 * k], a[3 * k + 1], a[3 * k + 2]); };
+    return accd_pt(v(q, 0), v(q, 1), v(q, 2), v(q, 3), v(d, 0), v(d, 1), v(d, 2), v(d, 3), eta, th);
+}
+double oracle_accd_ee(const double* q, const double* d, double eta, double th) {
+    auto v = [](const double* a, int k) { return V3(a[3 * k], a[3 * k + 1], a[3 * k + 2]); };
+    return accd_ee(v(q, 0), v(q, 1), v(q, 2), v(q, 3), v(d, 0), v(d, 1), v(d, 2), v(d, 3), eta, th);
+}
+int oracle_seg_tri_intersect(const double* q) {
+    auto v = [&](int k) { return V3(q[3 * k], q[3 * k + 1], q[3 * k + 2]); };
+    return seg_tri(v(0), v(1), v(2), v(3), v(4)) ? 1 : 0;
+}
+int64_t oracle_lbvh_query(int n, const double* bx, const double* qb, double gap, int* out, int64_t cap) {
+    std::vector<AABB> boxes(n);
+    for (int i = 0; i < n; ++i) { boxes[i].lo = V3(bx[6 * i], bx[6 * i + 1], bx[6 * i + 2]); boxes[i].hi = V3(bx[6 * i + 3], bx[6 * i + 4], bx[6 * i + 5]); }
+    LBVH t; t.build(boxes);
+    AABB q; q.lo = V3(qb[0], qb[1], qb[2]); q.hi = V3(qb[3], qb[4], qb[5]);
+    std::vector<int> ids;
+    t.query(q, gap, ids);
+    for (int64_t i = 0; i < (int64_t)ids.size() && i < cap; ++i) out[i] = ids[i];
+    return (int64_t)ids.size();
+}
+
+}  // extern "C"

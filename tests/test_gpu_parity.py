@@ -1,0 +1,196 @@
+"""Device path vs the CPU oracle, through the C ABI, on identical inputs.
+
+Bars (stated per test): contact pair sets, classifications, CSR patterns and CCD steps
+bit-exact; FP64 energies, gradients and Hessian values within 1e-10 relative to the
+largest magnitude of the compared quantity.
+"""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+REL = 1e-10
+
+
+def rel_err(a, b):
+    a, b = np.asarray(a, np.float64), np.asarray(b, np.float64)
+    scale = max(np.abs(b).max(initial=0.0), 1e-300)
+    return np.abs(a - b).max(initial=0.0) / scale
+
+
+@pytest.fixture(scope="module")
+def gpu():
+    from cipc import gpu as g
+    g.load()
+    return g
+
+
+@pytest.fixture(scope="module")
+def O(oracle_lib):
+    return oracle_lib
+
+
+def random_F(rng, n):
+    """Deformation gradients: near rest, compressed, stretched, sheared and inverted."""
+    F = np.eye(3) + 0.05 * rng.standard_normal((n, 3, 3))
+    k = n // 5
+    F[:k] *= rng.uniform(0.5, 0.95, (k, 1, 1))            # compression
+    F[k:2 * k] *= rng.uniform(1.05, 1.6, (k, 1, 1))       # stretch
+    F[2 * k:3 * k] += 0.6 * rng.standard_normal((k, 3, 3))
+    F[3 * k:4 * k, :, 0] *= -1.0                          # inverted
+    return F
+
+
+def test_snh_element_matches_oracle(gpu, O):
+    rng = np.random.default_rng(11)
+    F = random_F(rng, 500)
+    u, r = 4 * 35714.2857 / 3, 142857.142 + 5 * 35714.2857 / 6
+    P, H, e = gpu.eval_snh(F, u, r)
+    for i in range(len(F)):
+        Po, Ho, eo = O.snh_element(F[i], u, r)
+        assert rel_err(P[i], Po) <= REL, i
+        assert rel_err(H[i], Ho) <= REL, (i, rel_err(H[i], Ho))
+        assert abs(e[i] - eo) <= REL * max(abs(eo), u), i
+
+
+@pytest.mark.parametrize("kind", [1, 2, 3, 4])
+def test_distance_derivatives_match_oracle(gpu, O, kind):
+    rng = np.random.default_rng(20 + kind)
+    pts = rng.standard_normal((300, 12))
+    d, g, H = gpu.eval_distance(kind, pts)
+    for i in range(len(pts)):
+        do, go, Ho = O.distance(kind, pts[i])
+        n = go.shape[0]
+        assert abs(d[i] - do) <= 1e-12 * max(abs(do), 1e-300) or abs(d[i] - do) <= 1e-14
+        assert rel_err(g[i][:n], go) <= REL, (kind, i)
+        assert rel_err(H[i], Ho) <= REL, (kind, i, rel_err(H[i], Ho))
+
+
+@pytest.mark.parametrize("dim", [6, 9, 12])
+def test_make_pd_matches_oracle(gpu, O, dim):
+    rng = np.random.default_rng(dim)
+    A = rng.standard_normal((200, dim, dim))
+    H = A + A.transpose(0, 2, 1)
+    H[:20] = np.einsum("nij,nkj->nik", A[:20], A[:20])   # already PSD: left unchanged
+    Hg = gpu.eval_make_pd(H)
+    for i in range(len(H)):
+        Ho = O.make_pd(H[i])
+        assert rel_err(Hg[i], Ho) <= REL, (dim, i)
+
+
+def _ccd_inputs(rng, n):
+    pts = rng.uniform(-1, 1, (n, 12))
+    dirs = rng.uniform(-1, 1, (n, 12)) * rng.uniform(0.01, 3.0, (n, 1))
+    dirs[:10] = 0.0                          # no motion -> 1
+    dirs[10:20] = np.tile(rng.uniform(-1, 1, 3), 4)  # pure translation -> 1
+    return pts, dirs
+
+
+@pytest.mark.parametrize("kind", [0, 1])
+def test_accd_bitexact(gpu, O, kind):
+    rng = np.random.default_rng(40 + kind)
+    pts, dirs = _ccd_inputs(rng, 400)
+    eta = 1.0 - 0.8
+    tg = gpu.eval_accd(kind, pts, dirs, eta)
+    f = O.accd_pt if kind == 0 else O.accd_ee
+    to = np.array([f(pts[i], dirs[i], eta) for i in range(len(pts))])
+    assert np.array_equal(tg, to)
+
+
+@pytest.mark.parametrize("kind", [0, 1])
+def test_classification_bitexact(gpu, O, kind):
+    rng = np.random.default_rng(60 + kind)
+    pts = rng.uniform(-1, 1, (2000, 12))
+    pts[:200, 9:] = pts[:200, 6:9] + 1e-9 * rng.standard_normal((200, 3))  # near-degenerate
+    pts[200:300, 3:6] = pts[200:300, 0:3] + (pts[200:300, 6:9] - pts[200:300, 0:3]) * 0.5  # collinear-ish
+    tg = gpu.eval_classify(kind, pts)
+    f = O.dtype_pt if kind == 0 else O.dtype_ee
+    to = np.array([f(p) for p in pts])
+    assert np.array_equal(tg, to)
+
+
+def _scenes():
+    from cipc import scenes
+    return {
+        "cube10": scenes.config0(),
+        "two_cubes": scenes.stacked_cubes(2, 6, 0.1, 1e-3, seed=3, rotate="yaw", velocity_scale=0.5, name="two_cubes"),
+        "four_20cubes": scenes.config1(velocity_scale=0.5),
+    }
+
+
+@pytest.fixture(scope="module", params=["cube10", "two_cubes", "four_20cubes"])
+def pair(request, gpu, O):
+    from cipc import scenes
+    s = _scenes()[request.param]
+    xt, p = scenes.newton_probe_state(s)
+    kappa = scenes.suggest_kappa(s.hhat, s.bbox_diag2, s.mean_mass)
+    g = gpu.GpuContext(s, xtilde=xt)
+    o = O.OracleMesh(s, xtilde=xt)
+    yield dict(scene=s, g=g, o=o, p=p, kappa=kappa, name=request.param)
+    g.close()
+    o.close()
+
+
+def test_active_set_bitexact(pair):
+    ag, ao = pair["g"].active_set(), pair["o"].active_set()
+    for k in ("self_set", "ee_set", "ee_ie", "ccd_set", "env_set"):
+        assert ag[k].shape == ao[k].shape, (pair["name"], k, ag[k].shape, ao[k].shape)
+        assert np.array_equal(ag[k], ao[k]), (pair["name"], k)
+
+
+def test_assembly_matches_oracle(pair):
+    g, o, kappa = pair["g"], pair["o"], pair["kappa"]
+    g.active_set()
+    o.active_set()
+    rg, ro = g.assemble(kappa), o.assemble(kappa)
+    assert np.array_equal(rg["outer"], ro["outer"]), pair["name"]
+    assert np.array_equal(rg["inner"], ro["inner"]), pair["name"]
+    assert rel_err(rg["val"], ro["val"]) <= REL, (pair["name"], rel_err(rg["val"], ro["val"]))
+    assert rel_err(rg["grad"], ro["grad"]) <= REL, (pair["name"], rel_err(rg["grad"], ro["grad"]))
+    assert rel_err(rg["rhs"], ro["rhs"]) <= REL
+    assert rg["collisions"] == ro["collisions"]
+
+
+def test_energy_matches_oracle(pair):
+    g, o, kappa = pair["g"], pair["o"], pair["kappa"]
+    g.active_set()
+    o.active_set()
+    tg, to = g.energy_terms(kappa), o.energy_terms(kappa)
+    for k in range(5):
+        assert abs(tg[k] - to[k]) <= REL * max(abs(to[k]), 1e-300) or (tg[k] == to[k]), (pair["name"], k, tg[k], to[k])
+    eg, eo = g.energy(kappa), o.energy(kappa)
+    assert abs(eg - eo) <= REL * abs(eo)
+
+
+def test_step_bound_matches_oracle(pair):
+    g, o, p = pair["g"], pair["o"], pair["p"]
+    g.active_set()
+    o.active_set()
+    bg, bo = g.step_bound(p), o.step_bound(p)
+    for k in ("alpha", "ground", "partial", "cfl", "full"):
+        assert bg[k] <= bo[k] and abs(bg[k] - bo[k]) <= 1e-6, (pair["name"], k, bg[k], bo[k])
+        assert bg[k] == bo[k], (pair["name"], k, bg[k], bo[k])
+
+
+def test_step_forward_and_intersection(pair):
+    g, o, p = pair["g"], pair["o"], pair["p"]
+    s = pair["scene"]
+    g.set_direction(p)
+    g.save_positions()
+    g.step_forward(0.5)
+    o.step_forward(s.x, p, 0.5)
+    assert np.array_equal(g.positions(), o.positions())
+    g.active_set()
+    o.active_set()
+    assert g.is_intersected() == o.is_intersected()
+    g.set_state(x=s.x)
+    o.set_state(x=s.x)
+
+
+def test_newton_iteration_timing(pair):
+    g, o, kappa = pair["g"], pair["o"], pair["kappa"]
+    g.set_direction(pair["p"])
+    t = g.newton_iteration(kappa)
+    o.active_set()
+    assert t["total_ms"] > 0 and t["nnz"] > 0
+    assert abs(t["energy"] - o.energy(kappa)) <= REL * abs(t["energy"])
