@@ -80,7 +80,7 @@ def main():
     import numpy as np
     from cipc import gpu, scenes
 
-    scene = scenes.config1(velocity_scale=0.5)
+    scene = scenes.four_cubes20(velocity_scale=0.5)
     xt, p = scenes.newton_probe_state(scene)
     kappa = scenes.suggest_kappa(scene.hhat, scene.bbox_diag2, scene.mean_mass)
     ctx = gpu.GpuContext(scene, xtilde=xt)

@@ -291,7 +291,7 @@ def stacked_cubes(n_cubes: int, cells: int, cell: float, gap: float, seed: int,
     return finalize(name, x, np.concatenate(ts), mat, velocities=np.concatenate(vs))
 
 
-def config0(seed: int = 0) -> Scene:
+def cube10(seed: int = 0) -> Scene:
     """One 10^3-cell cube (1331 verts / 5000 tets), edge 0.1 m, 0.05 m above the ground,
     +-1% jitter, E=1e5, nu=0.4, rho=1000, dt=0.01, dHat_rel=1e-3, mu=0.2."""
     mat = Material(friction=0.2)
@@ -299,10 +299,10 @@ def config0(seed: int = 0) -> Scene:
     return s
 
 
-def config1(seed: int = 1, velocity_scale: float = 0.0, rotate: str = "yaw") -> Scene:
+def four_cubes20(seed: int = 1, velocity_scale: float = 0.0, rotate: str = "yaw") -> Scene:
     """Four 20^3-cell cubes (9261 verts / 40000 tets each), cell 0.1 m, seeded random
     rotations about the stacking axis, stacked with 1e-3 m gaps on the ground, mu=0.3,
-    otherwise as config0.  The gap is below dHat (1e-3 of the scene diagonal, ~1.4e-2 m),
+    otherwise as cube10.  The gap is below dHat (1e-3 of the scene diagonal, ~8.9e-3 m),
     so every cube-cube and cube-ground interface is in contact from the first
     iteration."""
     mat = Material(friction=0.3)

@@ -112,9 +112,9 @@ def test_classification_bitexact(gpu, O, kind):
 def _scenes():
     from cipc import scenes
     return {
-        "cube10": scenes.config0(),
+        "cube10": scenes.cube10(),
         "two_cubes": scenes.stacked_cubes(2, 6, 0.1, 1e-3, seed=3, rotate="yaw", velocity_scale=0.5, name="two_cubes"),
-        "four_20cubes": scenes.config1(velocity_scale=0.5),
+        "four_20cubes": scenes.four_cubes20(velocity_scale=0.5),
     }
 
 

@@ -1,0 +1,271 @@
+"""Pins the oracle's element-level functions against what mathematics fixes: numpy's
+SVD and symmetric eigensolver, central finite differences, closed forms and invariants.
+Each check is chosen so a dropped term, a sign or index slip, or a transposed operand
+fails it."""
+import numpy as np
+import pytest
+
+
+@pytest.fixture(scope="module")
+def O(oracle_lib):
+    return oracle_lib
+
+
+def random_F(rng, n):
+    F = np.eye(3) + 0.05 * rng.standard_normal((n, 3, 3))
+    k = n // 5
+    F[:k] *= rng.uniform(0.5, 0.95, (k, 1, 1))
+    F[k:2 * k] *= rng.uniform(1.05, 1.6, (k, 1, 1))
+    F[2 * k:3 * k] += 0.6 * rng.standard_normal((k, 3, 3))
+    F[3 * k:4 * k, :, 0] *= -1.0
+    return F
+
+
+# ---------------------------------------------------------------------------------------
+def test_svd_conventions_against_numpy(O):
+    """RotationAwareSVD.h:11-13: U, V rotations, |s| descending, only s2 negative,
+    sign(s2) = sign(det F); singular values equal numpy's."""
+    rng = np.random.default_rng(1)
+    for F in random_F(rng, 300):
+        U, s, V = O.svd3(F)
+        assert np.allclose(U @ U.T, np.eye(3), atol=1e-13) and np.allclose(V @ V.T, np.eye(3), atol=1e-13)
+        assert np.linalg.det(U) > 0 and np.linalg.det(V) > 0
+        assert np.allclose(U @ np.diag(s) @ V.T, F, atol=1e-13 * np.abs(F).max())
+        assert s[0] >= s[1] >= abs(s[2]) and s[1] >= 0
+        assert np.sign(s[2]) == np.sign(np.linalg.det(F))
+        assert np.allclose(np.abs(s), np.linalg.svd(F, compute_uv=False), rtol=1e-13, atol=1e-15)
+
+
+def test_svd_degenerate_inputs(O):
+    for F in (np.zeros((3, 3)), np.diag([2.0, 0.0, 0.0]), np.diag([1.0, 1.0, 0.0]), -np.eye(3),
+              np.outer([1, 2, 3], [0.5, -1, 2.0])):
+        U, s, V = O.svd3(F)
+        assert np.allclose(U @ np.diag(s) @ V.T, F, atol=1e-13)
+        assert abs(np.linalg.det(U) - 1) < 1e-12 and abs(np.linalg.det(V) - 1) < 1e-12
+
+
+def _numpy_make_pd(H):
+    w, Z = np.linalg.eigh(H)
+    if w[0] >= 0:
+        return H
+    return Z @ np.diag(np.maximum(w, 0)) @ Z.T
+
+
+@pytest.mark.parametrize("n", [3, 6, 9, 12])
+def test_make_pd_against_numpy_eigh(O, n):
+    """ContactMechanics.h:27-45: unchanged when PSD, else negative eigenvalues zeroed."""
+    rng = np.random.default_rng(n)
+    for k in range(100):
+        A = rng.standard_normal((n, n))
+        H = A + A.T if k % 4 else A @ A.T
+        R = O.make_pd(H)
+        assert np.allclose(R, _numpy_make_pd(H), atol=1e-12 * np.abs(H).max())
+        assert np.linalg.eigvalsh(R)[0] >= -1e-12 * np.abs(H).max()
+        if k % 4 == 0:
+            assert np.array_equal(R, H)
+
+
+# ---------------------------------------------------------------------------------------
+U_RATE, R_RATE = 4 * 35714.2857 / 3, 142857.142 + 5 * 35714.2857 / 6
+
+
+def test_snh_rest_state_is_stress_free(O):
+    P, H, e = O.snh_element(np.eye(3), U_RATE, R_RATE)
+    assert np.abs(P).max() < 1e-9 * U_RATE
+
+
+def test_snh_pk1_is_energy_gradient(O):
+    """Elasticity.cpp:1144-1170 against central differences of Elasticity.cpp:695-727."""
+    rng = np.random.default_rng(3)
+    h = 1e-6
+    for F in random_F(rng, 60):
+        P, _, _ = O.snh_element(F, U_RATE, R_RATE)
+        fd = np.zeros((3, 3))
+        for i in range(3):
+            for j in range(3):
+                Fp, Fm = F.copy(), F.copy()
+                Fp[i, j] += h
+                Fm[i, j] -= h
+                fd[i, j] = (O.snh_element(Fp, U_RATE, R_RATE)[2] - O.snh_element(Fm, U_RATE, R_RATE)[2]) / (2 * h)
+        assert np.allclose(P, fd, rtol=1e-6, atol=1e-6 * np.abs(P).max())
+
+
+def test_snh_projected_hessian_is_projected_pk1_jacobian(O):
+    """Hessian of Elasticity.cpp:1436-1477 (finite differences of the PK1 over column-major
+    vec F) followed by numpy's eigen-projection equals the oracle's projected Hessian."""
+    rng = np.random.default_rng(4)
+    h = 1e-6
+    for F in random_F(rng, 40):
+        _, H, _ = O.snh_element(F, U_RATE, R_RATE)
+        J = np.zeros((9, 9))
+        for q in range(9):
+            i, j = q % 3, q // 3
+            Fp, Fm = F.copy(), F.copy()
+            Fp[i, j] += h
+            Fm[i, j] -= h
+            dP = (O.snh_element(Fp, U_RATE, R_RATE)[0] - O.snh_element(Fm, U_RATE, R_RATE)[0]) / (2 * h)
+            J[:, q] = dP.T.reshape(9)
+        J = 0.5 * (J + J.T)
+        assert np.allclose(H, _numpy_make_pd(J), atol=2e-5 * np.abs(J).max())
+
+
+def test_snh_energy_is_rotation_invariant(O):
+    rng = np.random.default_rng(5)
+    for F in random_F(rng, 30):
+        Q, _ = np.linalg.qr(rng.standard_normal((3, 3)))
+        if np.linalg.det(Q) < 0:
+            Q[:, 0] *= -1
+        e1 = O.snh_element(F, U_RATE, R_RATE)[2]
+        e2 = O.snh_element(Q @ F, U_RATE, R_RATE)[2]
+        assert abs(e1 - e2) <= 1e-12 * max(abs(e1), U_RATE)
+
+
+# ---------------------------------------------------------------------------------------
+def test_barrier_closed_forms(O):
+    """ContactMechanics.cpp:775-790: b(dHat) = b'(dHat) = 0, b' and b'' are derivatives."""
+    dh = 2.3e-4
+    b, g, H = O.barrier(dh, dh)
+    assert b == 0.0 and g == 0.0
+    for d in np.geomspace(1e-9, 0.99 * dh, 25):
+        b, g, H = O.barrier(d, dh)
+        assert b > 0 and g < 0 and H > 0
+        e = 1e-7 * d
+        assert abs((O.barrier(d + e, dh)[0] - O.barrier(d - e, dh)[0]) / (2 * e) - g) <= 1e-5 * abs(g)
+        assert abs((O.barrier(d + e, dh)[1] - O.barrier(d - e, dh)[1]) / (2 * e) - H) <= 1e-5 * abs(H)
+
+
+def _closest_pt_brute(p, t0, t1, t2, n=400):
+    u, v = np.meshgrid(np.linspace(0, 1, n), np.linspace(0, 1, n))
+    m = (u + v) <= 1
+    q = t0[None] + u[m][:, None] * (t1 - t0) + v[m][:, None] * (t2 - t0)
+    return np.min(np.sum((q - p) ** 2, axis=1))
+
+
+def _closest_ee_brute(a0, a1, b0, b1, n=600):
+    s = np.linspace(0, 1, n)
+    A = a0[None] + s[:, None] * (a1 - a0)
+    B = b0[None] + s[:, None] * (b1 - b0)
+    return np.min(np.sum((A[:, None] - B[None]) ** 2, axis=2))
+
+
+def test_point_triangle_distance_and_type_against_brute_force(O):
+    """dType_PT + d_PP/d_PE/d_PT (ContactMechanics.cpp:143-306) give the true squared
+    distance of the point to the triangle (dense sampling bound)."""
+    rng = np.random.default_rng(6)
+    for _ in range(60):
+        pts = rng.uniform(-1, 1, (4, 3))
+        t = O.dtype_pt(pts)
+        p, a, b, c = pts
+        if t in (0, 1, 2):
+            d = O.distance(O.DIST_PP, np.concatenate([p, (a, b, c)[t]]))[0]
+        elif t in (3, 4, 5):
+            e = ((a, b), (b, c), (c, a))[t - 3]
+            d = O.distance(O.DIST_PE, np.concatenate([p, e[0], e[1]]))[0]
+        else:
+            d = O.distance(O.DIST_PT, pts.reshape(-1))[0]
+        brute = _closest_pt_brute(p, a, b, c)
+        assert d <= brute + 1e-12 and brute - d <= 2e-2 * max(brute, 1e-3), (t, d, brute)
+
+
+def test_edge_edge_distance_and_type_against_brute_force(O):
+    rng = np.random.default_rng(7)
+    for _ in range(60):
+        pts = rng.uniform(-1, 1, (4, 3))
+        t = O.dtype_ee(pts)
+        a0, a1, b0, b1 = pts
+        pp = {0: (a0, b0), 1: (a0, b1), 3: (a1, b0), 4: (a1, b1)}
+        pe = {2: (a0, b0, b1), 5: (a1, b0, b1), 6: (b0, a0, a1), 7: (b1, a0, a1)}
+        if t in pp:
+            d = O.distance(O.DIST_PP, np.concatenate(pp[t]))[0]
+        elif t in pe:
+            d = O.distance(O.DIST_PE, np.concatenate(pe[t]))[0]
+        else:
+            d = O.distance(O.DIST_EE, pts.reshape(-1))[0]
+        brute = _closest_ee_brute(a0, a1, b0, b1)
+        assert d <= brute + 1e-12 and brute - d <= 2e-2 * max(brute, 1e-3), (t, d, brute)
+
+
+@pytest.mark.parametrize("kind,n", [(0, 6), (1, 9), (2, 12), (3, 12), (4, 12)])
+def test_distance_derivatives_finite_differences(O, kind, n):
+    rng = np.random.default_rng(8 + kind)
+    h = 1e-6
+    for _ in range(20):
+        x = rng.standard_normal(12)
+        d, g, H = O.distance(kind, x)
+        for j in range(n):
+            xp, xm = x.copy(), x.copy()
+            xp[j] += h
+            xm[j] -= h
+            dp, gp, _ = O.distance(kind, xp)
+            dm, gm, _ = O.distance(kind, xm)
+            assert abs((dp - dm) / (2 * h) - g[j]) <= 1e-6 * max(1.0, np.abs(g).max())
+            assert np.allclose((gp - gm) / (2 * h), H[:, j], atol=1e-5 * max(1.0, np.abs(H).max()))
+
+
+# ---------------------------------------------------------------------------------------
+def _ccd_config(rng):
+    pts = rng.uniform(-1, 1, (4, 3))
+    dirs = rng.uniform(-1, 1, (4, 3)) * rng.uniform(0.05, 2.0)
+    return pts, dirs
+
+
+def test_accd_trivial_motions_return_one(O):
+    rng = np.random.default_rng(9)
+    for _ in range(20):
+        pts, dirs = _ccd_config(rng)
+        assert O.accd_pt(pts, np.zeros((4, 3)), 0.2) == 1.0
+        assert O.accd_ee(pts, np.tile(dirs[0], (4, 1)), 0.2) == 1.0
+
+
+@pytest.mark.parametrize("kind", ["pt", "ee"])
+def test_accd_is_conservative(O, kind):
+    """AdditiveCCD.cpp:221-329: moving by toc * dir leaves a positive gap, at least about
+    eta times the starting one, never crossing a contact a dense time sampling finds."""
+    rng = np.random.default_rng(10)
+    f = O.accd_pt if kind == "pt" else O.accd_ee
+    dist = (lambda q: _closest_pt_brute(q[0], q[1], q[2], q[3], 120)) if kind == "pt" else \
+           (lambda q: _closest_ee_brute(q[0], q[1], q[2], q[3], 200))
+    for _ in range(25):
+        pts, dirs = _ccd_config(rng)
+        toc = f(pts, dirs, 0.2)
+        assert 0.0 <= toc <= 1.0
+        for s in np.linspace(0, toc, 8):
+            assert dist(pts + s * dirs) > 0.0
+
+
+def test_seg_tri_intersection_closed_form(O):
+    """IglUtils::segTriIntersect (ContactMechanics.h:47-71): a segment through the
+    interior crosses, one beside the triangle or short of the plane does not."""
+    t = [np.array([0.0, 0, 0]), np.array([1.0, 0, 0]), np.array([0.0, 1, 0])]
+    cross = np.concatenate([[0.2, 0.2, -1], [0.2, 0.2, 1], *t])
+    miss = np.concatenate([[0.8, 0.8, -1], [0.8, 0.8, 1], *t])
+    short = np.concatenate([[0.2, 0.2, 0.5], [0.2, 0.2, 1], *t])
+    assert O.seg_tri_intersect(cross) and not O.seg_tri_intersect(miss) and not O.seg_tri_intersect(short)
+
+
+# ---------------------------------------------------------------------------------------
+def _brute_overlap(boxes, q, gap):
+    lo, hi = boxes[:, :3], boxes[:, 3:]
+    ok = ~(np.any(lo - q[3:] > gap, axis=1) | np.any(q[:3] - hi > gap, axis=1))
+    return np.nonzero(ok)[0]
+
+
+@pytest.mark.parametrize("case", ["random", "duplicates", "flat", "single", "empty"])
+def test_lbvh_query_equals_brute_force(O, case):
+    """The reference's deleted LBVH tests (agent_docs/09_optimization_report.md:94):
+    random boxes, duplicate centres, a degenerate axis, one leaf and an empty tree."""
+    rng = np.random.default_rng(11)
+    n = {"random": 257, "duplicates": 200, "flat": 300, "single": 1, "empty": 0}[case]
+    c = rng.uniform(-1, 1, (n, 3))
+    if case == "duplicates":
+        c[::2] = c[1::2][: len(c[::2])]
+    if case == "flat":
+        c[:, 2] = 0.25
+    ext = rng.uniform(0, 0.1, (n, 3))
+    boxes = np.concatenate([c - ext, c + ext], axis=1)
+    for _ in range(50):
+        q0 = rng.uniform(-1, 1, 3)
+        q = np.concatenate([q0 - 0.05, q0 + 0.05])
+        gap = rng.choice([0.0, 0.02])
+        got = O.lbvh_query(boxes, q, gap) if n else np.zeros(0, np.int32)
+        assert np.array_equal(got, _brute_overlap(boxes, q, gap) if n else np.zeros(0, np.int64))
