@@ -34,7 +34,8 @@ EXPORTS = (
     "cipc_gpu_get_active_set", "cipc_gpu_set_active_set", "cipc_gpu_assemble", "cipc_gpu_get_gradient",
     "cipc_gpu_get_csr", "cipc_gpu_energy", "cipc_gpu_energy_terms", "cipc_gpu_step_bound",
     "cipc_gpu_save_positions", "cipc_gpu_step_forward", "cipc_gpu_is_intersected", "cipc_gpu_newton_iteration",
-    "cipc_gpu_eval_snh", "cipc_gpu_eval_distance", "cipc_gpu_eval_make_pd", "cipc_gpu_eval_accd",
+    "cipc_gpu_eval_snh", "cipc_gpu_eval_distance", "cipc_gpu_eval_make_pd", "cipc_gpu_eval_contact_pd",
+    "cipc_gpu_eval_accd",
     "cipc_gpu_eval_classify",
 )
 
@@ -99,6 +100,7 @@ def load(path: str = LIB_PATH):
         "cipc_gpu_eval_snh": [C.c_int, _dp, C.c_double, C.c_double, _dp, _dp, _dp],
         "cipc_gpu_eval_distance": [C.c_int, C.c_int, _dp, _dp, _dp, _dp],
         "cipc_gpu_eval_make_pd": [C.c_int, C.c_int, _dp],
+        "cipc_gpu_eval_contact_pd": [C.c_int, C.c_int, _dp],
         "cipc_gpu_eval_accd": [C.c_int, C.c_int, _dp, _dp, C.c_double, C.c_double, _dp],
         "cipc_gpu_eval_classify": [C.c_int, C.c_int, _dp, _ip],
     }
@@ -300,6 +302,14 @@ def eval_make_pd(H):
     H = _d(H).copy()
     n, dim = H.shape[0], H.shape[1]
     _check(load().cipc_gpu_eval_make_pd(n, dim, _dptr(H)))
+    return H
+
+
+def eval_contact_pd(H):
+    """Projection of translation-invariant contact Hessians (n, 3m, 3m), m vertices."""
+    H = _d(H).copy()
+    n, dim = H.shape[0], H.shape[1]
+    _check(load().cipc_gpu_eval_contact_pd(n, dim // 3, _dptr(H)))
     return H
 
 

@@ -63,7 +63,7 @@ __global__ void morton_keys(int n, const Box* __restrict__ b, const Box* __restr
     if (i >= n) return;
     Box sc = part[0];
     for (int k = 1; k < nparts; ++k) sc = Box{vmin(sc.lo, part[k].lo), vmax(sc.hi, part[k].hi)};
-    const d3 ext = sc.hi - sc.lo;
+    const d3 ext = sc.hi - sc.lo;  // nparts == 1: the partials were reduced beforehand
     const d3 c = 0.5 * (b[i].lo + b[i].hi);
     const d3 off = c - sc.lo;
     const double x = ext.x > 0.0 ? off.x / ext.x : 0.0;
@@ -78,7 +78,7 @@ __device__ inline int prefix(const unsigned long long* k, int n, int a, int b) {
 // Karras 2012 topology (LBVH.cpp:62-119,213-228): internal node i covers the key range
 // [first, last] and splits it at the highest differing bit.
 __global__ void karras_kernel(int n, const unsigned long long* __restrict__ k, int* __restrict__ left, int* __restrict__ right,
-                              int* __restrict__ parent, int* __restrict__ range_last, int* __restrict__ split_right) {
+                              int* __restrict__ range_first, int* __restrict__ range_last, int* __restrict__ split_right) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n - 1) return;
     int first, last;
@@ -113,20 +113,17 @@ __global__ void karras_kernel(int n, const unsigned long long* __restrict__ k, i
     const int r = (split + 1 == last) ? split + 1 + n - 1 : split + 1;
     left[i] = l;
     right[i] = r;
-    parent[l] = i;
-    parent[r] = i;
+    range_first[i] = first;
     range_last[i] = last;
     split_right[split] = r;
 }
 __global__ void leaf_init(int n, const unsigned long long* __restrict__ k, const Box* __restrict__ prim, Box* __restrict__ nb,
-                          int* __restrict__ left, int* __restrict__ flag, int* __restrict__ parent) {
+                          int* __restrict__ left) {
     const int s = blockIdx.x * blockDim.x + threadIdx.x;
     if (s >= n) return;
     const int p = (int)(k[s] & 0xffffffffull);
     nb[n - 1 + s] = prim[p];
     left[n - 1 + s] = p;
-    if (s < n - 1) flag[s] = 0;
-    if (s == 0) parent[0] = -1;
 }
 __global__ void skip_kernel(int n, const int* __restrict__ range_last, const int* __restrict__ split_right, int* __restrict__ skip) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -134,56 +131,108 @@ __global__ void skip_kernel(int n, const int* __restrict__ range_last, const int
     const int last = (i < n - 1) ? range_last[i] : i - (n - 1);
     skip[i] = (last >= n - 1) ? -1 : split_right[last];
 }
-// Bottom-up union of child boxes; the second child to arrive at a node computes it.  Child
-// boxes written by other work-groups are read with L1-bypassing (volatile) loads after
-// the arrival counter, which is ordered after the writer's fence.
-__device__ inline Box vload(const Box* p) {
-    const volatile double* q = reinterpret_cast<const volatile double*>(p);
-    return Box{{q[0], q[1], q[2]}, {q[3], q[4], q[5]}};
+
+// Internal boxes.  Node i of the Karras tree covers the sorted leaves [first, last], so its
+// box is the union of that range.  Power-of-two reduction levels over the sorted leaf boxes
+// (level l node j = leaves [j 2^l, (j+1) 2^l)) are built 8 levels per launch in LDS; each
+// internal box is then the union of at most 2 log2(n) level nodes.  No inter-work-group
+// communication is needed.
+constexpr int kSegBlock = 256;
+constexpr int kSegLevels = 8;   // log2(kSegBlock)
+__device__ inline Box empty_box() {
+    return Box{{1.7976931348623157e308, 1.7976931348623157e308, 1.7976931348623157e308},
+               {-1.7976931348623157e308, -1.7976931348623157e308, -1.7976931348623157e308}};
 }
-__global__ void bottom_up_boxes(int n, const int* __restrict__ parent, const int* __restrict__ left,
-                                const int* __restrict__ right, Box* nb, int* flag) {
-    const int s = blockIdx.x * blockDim.x + threadIdx.x;
-    if (s >= n || n < 2) return;
-    int node = parent[n - 1 + s];
-    for (int depth = 0; depth < n && node >= 0; ++depth) {
-        __threadfence();
-        if (atomicAdd(&flag[node], 1) == 0) return;
-        __threadfence();
-        const Box a = vload(nb + left[node]), b = vload(nb + right[node]);
-        nb[node] = Box{vmin(a.lo, b.lo), vmax(a.hi, b.hi)};
-        node = node == 0 ? -1 : parent[node];
+struct SegLevels {
+    int count[40];
+    long long offset[40];
+};
+__global__ __launch_bounds__(kSegBlock) void seg_build(const Box* __restrict__ src, int nsrc, int lvl0, SegLevels L, Box* __restrict__ seg) {
+    __shared__ Box sh[kSegBlock];
+    const int g = blockIdx.x * kSegBlock + threadIdx.x;
+    sh[threadIdx.x] = g < nsrc ? src[g] : empty_box();
+    __syncthreads();
+    for (int l = 1; l <= kSegLevels; ++l) {
+        const int half = 1 << (l - 1);
+        if ((threadIdx.x & ((1 << l) - 1)) == 0) {
+            const Box a = sh[threadIdx.x], b = sh[threadIdx.x + half];
+            sh[threadIdx.x] = Box{vmin(a.lo, b.lo), vmax(a.hi, b.hi)};
+        }
+        __syncthreads();
+        const int lv = lvl0 + l;
+        if (lv < 40 && L.count[lv] > 0 && (threadIdx.x & ((1 << l) - 1)) == 0) {
+            const int j = g >> l;
+            if (j < L.count[lv]) seg[L.offset[lv] + j] = sh[threadIdx.x];
+        }
     }
+}
+__global__ void internal_boxes(int n, const int* __restrict__ first, const int* __restrict__ last, SegLevels L,
+                               const Box* __restrict__ seg, Box* __restrict__ nb) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n - 1) return;
+    int a = first[i], b = last[i], l = 0;
+    Box acc = empty_box();
+    while (a <= b && l < 40) {
+        if (a & 1) { const Box x = seg[L.offset[l] + a]; acc = Box{vmin(acc.lo, x.lo), vmax(acc.hi, x.hi)}; ++a; }
+        if (a <= b && !(b & 1)) { const Box x = seg[L.offset[l] + b]; acc = Box{vmin(acc.lo, x.lo), vmax(acc.hi, x.hi)}; --b; }
+        a >>= 1; b >>= 1; ++l;
+    }
+    nb[i] = acc;
 }
 
 void build_tree(Context& c, LBVH& t, int n) {
     t.n = n;
     if (n <= 0) return;
     const int nodes = 2 * n - 1;
-    t.node_box.resize(nodes); t.left.resize(nodes); t.skip.resize(nodes); t.parent.resize(nodes);
-    t.right.resize(std::max(n - 1, 1)); t.range_last.resize(std::max(n - 1, 1));
-    t.split_right.resize(std::max(n - 1, 1)); t.flag.resize(std::max(n - 1, 1));
+    t.node_box.resize(nodes); t.left.resize(nodes); t.skip.resize(nodes);
+    t.right.resize(std::max(n - 1, 1)); t.range_first.resize(std::max(n - 1, 1)); t.range_last.resize(std::max(n - 1, 1));
+    t.split_right.resize(std::max(n - 1, 1));
     t.keys.resize(n); t.keys_sorted.resize(n);
     const int parts = std::min<int>(kRedParts, (int)blocks_for(n, kRedBlock));
     t.scene_partial.resize(parts);
     hipLaunchKernelGGL(box_reduce_partial, dim3(parts), dim3(kRedBlock), 0, c.stream, n, t.prim.p, t.scene_partial.p);
     CIPC_CHECK_LAUNCH();
-    hipLaunchKernelGGL(morton_keys, dim3(blocks_for(n, 256)), dim3(256), 0, c.stream, n, t.prim.p, t.scene_partial.p, parts, t.keys.p);
+    if (parts > 1) {
+        hipLaunchKernelGGL(box_reduce_partial, dim3(1), dim3(kRedBlock), 0, c.stream, parts, t.scene_partial.p, t.scene_partial.p);
+        CIPC_CHECK_LAUNCH();
+    }
+    hipLaunchKernelGGL(morton_keys, dim3(blocks_for(n, 256)), dim3(256), 0, c.stream, n, t.prim.p, t.scene_partial.p, 1, t.keys.p);
     CIPC_CHECK_LAUNCH();
     sort_u64(c, t.keys.p, t.keys_sorted.p, n, 64);
     if (n > 1) {
         hipLaunchKernelGGL(karras_kernel, dim3(blocks_for(n - 1, 256)), dim3(256), 0, c.stream, n, t.keys_sorted.p, t.left.p,
-                           t.right.p, t.parent.p, t.range_last.p, t.split_right.p);
+                           t.right.p, t.range_first.p, t.range_last.p, t.split_right.p);
         CIPC_CHECK_LAUNCH();
     }
     hipLaunchKernelGGL(leaf_init, dim3(blocks_for(n, 256)), dim3(256), 0, c.stream, n, t.keys_sorted.p, t.prim.p, t.node_box.p,
-                       t.left.p, t.flag.p, t.parent.p);
+                       t.left.p);
     CIPC_CHECK_LAUNCH();
     hipLaunchKernelGGL(skip_kernel, dim3(blocks_for(nodes, 256)), dim3(256), 0, c.stream, n, t.range_last.p, t.split_right.p, t.skip.p);
     CIPC_CHECK_LAUNCH();
     if (n > 1) {
-        hipLaunchKernelGGL(bottom_up_boxes, dim3(blocks_for(n, 256)), dim3(256), 0, c.stream, n, t.parent.p, t.left.p, t.right.p,
-                           t.node_box.p, t.flag.p);
+        // level l holds ceil(n / 2^l) boxes; level 0 is the sorted leaves themselves
+        SegLevels L{};
+        long long total = 0;
+        int levels = 0;
+        for (int l = 0; l < 40; ++l) {
+            const long long cnt = ((long long)n + (1ll << l) - 1) >> l;
+            L.count[l] = (int)cnt;
+            L.offset[l] = total;
+            total += cnt;
+            levels = l;
+            if (cnt == 1) break;
+        }
+        for (int l = levels + 1; l < 40; ++l) { L.count[l] = 0; L.offset[l] = total; }
+        t.seg.resize((size_t)total);
+        CIPC_HIP(hipMemcpyAsync(t.seg.p, t.node_box.p + (n - 1), sizeof(Box) * n, hipMemcpyDeviceToDevice, c.stream));
+        for (int l0 = 0; l0 < levels; l0 += kSegLevels) {
+            const int nsrc = L.count[l0];
+            hipLaunchKernelGGL(seg_build, dim3(blocks_for(nsrc, kSegBlock)), dim3(kSegBlock), 0, c.stream, t.seg.p + L.offset[l0], nsrc,
+                               l0, L, t.seg.p);
+            CIPC_CHECK_LAUNCH();
+        }
+        hipLaunchKernelGGL(internal_boxes, dim3(blocks_for(n - 1, 256)), dim3(256), 0, c.stream, n, t.range_first.p, t.range_last.p, L,
+                           t.seg.p, t.node_box.p);
         CIPC_CHECK_LAUNCH();
     }
 }
@@ -284,26 +333,42 @@ __device__ inline void sort_segment(int* a, int n) {   // insertion sort of a sh
     }
 }
 
-__global__ void pt_pass(int Vs, const int* __restrict__ sv, const double* __restrict__ x, const int* __restrict__ bt,
-                        const int4* __restrict__ faces, TreeView tv, double sq, double hhat, const int* __restrict__ off,
-                        int* __restrict__ cnt, int* __restrict__ cand, int4* __restrict__ rec, int* __restrict__ cat) {
-    const int s = blockIdx.x * blockDim.x + threadIdx.x;
-    if (s >= Vs) return;
+// Candidate buffering: pass A traverses once, keeps the first kCap candidates of each
+// query and its total count; pass B copies them to the packed layout (re-traversing only
+// queries with more than kCap candidates), sorts them by primitive id and classifies.
+// Queries are visited in the Morton order of their rest positions (perm) so that the
+// lanes of a wave walk the same part of the tree.
+constexpr int kCap = 32;
+
+__global__ void pt_pass(int Vs, const int* __restrict__ perm, const int* __restrict__ sv, const double* __restrict__ x,
+                        const int* __restrict__ bt, const int4* __restrict__ faces, TreeView tv, double sq, double hhat,
+                        const int* __restrict__ off, int* __restrict__ cnt, int* __restrict__ buf, int* __restrict__ cand,
+                        int4* __restrict__ rec, int* __restrict__ cat) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= Vs) return;
+    const int s = perm[t];
     const int vI = sv[s];
     const d3 P = ld3(x, vI);
     const Box pb = box_pt(P);
     if (off == nullptr) {
         int n = 0;
-        lbvh_visit(tv.n, tv.nb, tv.left, tv.skip, pb, sq, [&](int f) { if (pt_keep(vI, faces[f], bt, x, pb, sq)) ++n; });
+        int* b = buf + (size_t)s * kCap;
+        lbvh_visit(tv.n, tv.nb, tv.left, tv.skip, pb, sq, [&](int f) {
+            if (pt_keep(vI, faces[f], bt, x, pb, sq)) { if (n < kCap) b[n] = f; ++n; }
+        });
         cnt[s] = n;
         return;
     }
     const int o = off[s], cap = off[s + 1] - o;
     int n = 0;
-    lbvh_visit(tv.n, tv.nb, tv.left, tv.skip, pb, sq, [&](int f) {
-        if (pt_keep(vI, faces[f], bt, x, pb, sq)) { if (n < cap) cand[o + n] = f; ++n; }
-    });
-    n = n < cap ? n : cap;
+    if (cap <= kCap) {
+        for (; n < cap; ++n) cand[o + n] = buf[(size_t)s * kCap + n];
+    } else {
+        lbvh_visit(tv.n, tv.nb, tv.left, tv.skip, pb, sq, [&](int f) {
+            if (pt_keep(vI, faces[f], bt, x, pb, sq)) { if (n < cap) cand[o + n] = f; ++n; }
+        });
+        n = n < cap ? n : cap;
+    }
     sort_segment(cand + o, n);
     for (int k = 0; k < n; ++k) {
         const int f = cand[o + k];
@@ -328,27 +393,35 @@ __global__ void pt_pass(int Vs, const int* __restrict__ sv, const double* __rest
     }
 }
 
-__global__ void ee_pass(int Es, const int2* __restrict__ se, const double* __restrict__ x, const double* __restrict__ vrest,
-                        const int* __restrict__ bt, TreeView tv, double sq, double hhat, const int* __restrict__ off,
-                        int* __restrict__ cnt, int* __restrict__ cand, int4* __restrict__ rec, int2* __restrict__ aux,
-                        int* __restrict__ cat) {
-    const int eI = blockIdx.x * blockDim.x + threadIdx.x;
-    if (eI >= Es) return;
+__global__ void ee_pass(int Es, const int* __restrict__ perm, const int2* __restrict__ se, const double* __restrict__ x,
+                        const double* __restrict__ vrest, const int* __restrict__ bt, TreeView tv, double sq, double hhat,
+                        const int* __restrict__ off, int* __restrict__ cnt, int* __restrict__ buf, int* __restrict__ cand,
+                        int4* __restrict__ rec, int2* __restrict__ aux, int* __restrict__ cat) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= Es) return;
+    const int eI = perm[t];
     const int2 A = se[eI];
     const d3 a0 = ld3(x, A.x), a1 = ld3(x, A.y);
     const Box b1 = box_edge(a0, a1);
     if (off == nullptr) {
         int n = 0;
-        lbvh_visit(tv.n, tv.nb, tv.left, tv.skip, b1, sq, [&](int eJ) { if (ee_keep(eI, eJ, A, se[eJ], bt, x, b1, sq)) ++n; });
+        int* b = buf + (size_t)eI * kCap;
+        lbvh_visit(tv.n, tv.nb, tv.left, tv.skip, b1, sq, [&](int eJ) {
+            if (ee_keep(eI, eJ, A, se[eJ], bt, x, b1, sq)) { if (n < kCap) b[n] = eJ; ++n; }
+        });
         cnt[eI] = n;
         return;
     }
     const int o = off[eI], cap = off[eI + 1] - o;
     int n = 0;
-    lbvh_visit(tv.n, tv.nb, tv.left, tv.skip, b1, sq, [&](int eJ) {
-        if (ee_keep(eI, eJ, A, se[eJ], bt, x, b1, sq)) { if (n < cap) cand[o + n] = eJ; ++n; }
-    });
-    n = n < cap ? n : cap;
+    if (cap <= kCap) {
+        for (; n < cap; ++n) cand[o + n] = buf[(size_t)eI * kCap + n];
+    } else {
+        lbvh_visit(tv.n, tv.nb, tv.left, tv.skip, b1, sq, [&](int eJ) {
+            if (ee_keep(eI, eJ, A, se[eJ], bt, x, b1, sq)) { if (n < cap) cand[o + n] = eJ; ++n; }
+        });
+        n = n < cap ? n : cap;
+    }
     sort_segment(cand + o, n);
     for (int k = 0; k < n; ++k) {
         const int eJ = cand[o + k];
@@ -478,13 +551,15 @@ void compute_active_set(Context& c) {
     const TreeView ft{c.face_tree.n, c.face_tree.node_box.p, c.face_tree.left.p, c.face_tree.skip.p};
     const TreeView et{c.edge_tree.n, c.edge_tree.node_box.p, c.edge_tree.left.p, c.edge_tree.skip.p};
     b.cntPT.resize(Vs + 1); b.offPT.resize(Vs + 1); b.cntEE.resize(Es + 1); b.offEE.resize(Es + 1);
-    // count
+    b.bufPT.resize((size_t)kCap * std::max(Vs, 1)); b.bufEE.resize((size_t)kCap * std::max(Es, 1));
+    // traverse once: counts + first kCap candidates
     if (Vs > 0)
-        hipLaunchKernelGGL(pt_pass, dim3(blocks_for(Vs, 128)), dim3(128), 0, c.stream, Vs, s.sv.p, s.x.p, s.bt.p, s.faces.p, ft, sq,
-                           s.hhat, (const int*)nullptr, b.cntPT.p, (int*)nullptr, (int4*)nullptr, (int*)nullptr);
+        hipLaunchKernelGGL(pt_pass, dim3(blocks_for(Vs, 64)), dim3(64), 0, c.stream, Vs, s.sv_perm.p, s.sv.p, s.x.p, s.bt.p, s.faces.p,
+                           ft, sq, s.hhat, (const int*)nullptr, b.cntPT.p, b.bufPT.p, (int*)nullptr, (int4*)nullptr, (int*)nullptr);
     if (Es > 0)
-        hipLaunchKernelGGL(ee_pass, dim3(blocks_for(Es, 128)), dim3(128), 0, c.stream, Es, s.se.p, s.x.p, s.vrest.p, s.bt.p, et, sq,
-                           s.hhat, (const int*)nullptr, b.cntEE.p, (int*)nullptr, (int4*)nullptr, (int2*)nullptr, (int*)nullptr);
+        hipLaunchKernelGGL(ee_pass, dim3(blocks_for(Es, 64)), dim3(64), 0, c.stream, Es, s.se_perm.p, s.se.p, s.x.p, s.vrest.p, s.bt.p,
+                           et, sq, s.hhat, (const int*)nullptr, b.cntEE.p, b.bufEE.p, (int*)nullptr, (int4*)nullptr, (int2*)nullptr,
+                           (int*)nullptr);
     CIPC_CHECK_LAUNCH();
     hipLaunchKernelGGL(fill_tail, dim3(1), dim3(1), 0, c.stream, b.cntPT.p, Vs);
     hipLaunchKernelGGL(fill_tail, dim3(1), dim3(1), 0, c.stream, b.cntEE.p, Es);
@@ -496,13 +571,13 @@ void compute_active_set(Context& c) {
     b.candPT.resize(nPT + 1); b.recPT.resize(nPT + 1); b.catPT.resize(nPT + 1); b.ownPT.resize(nPT + 1);
     b.candEE.resize(nEE + 1); b.recEE.resize(nEE + 1); b.catEE.resize(nEE + 1); b.auxEE.resize(nEE + 1); b.ownEE.resize(nEE + 1);
     if (Vs > 0) {
-        hipLaunchKernelGGL(pt_pass, dim3(blocks_for(Vs, 128)), dim3(128), 0, c.stream, Vs, s.sv.p, s.x.p, s.bt.p, s.faces.p, ft, sq,
-                           s.hhat, (const int*)b.offPT.p, b.cntPT.p, b.candPT.p, b.recPT.p, b.catPT.p);
+        hipLaunchKernelGGL(pt_pass, dim3(blocks_for(Vs, 64)), dim3(64), 0, c.stream, Vs, s.sv_perm.p, s.sv.p, s.x.p, s.bt.p, s.faces.p,
+                           ft, sq, s.hhat, (const int*)b.offPT.p, b.cntPT.p, b.bufPT.p, b.candPT.p, b.recPT.p, b.catPT.p);
         hipLaunchKernelGGL(owner_of, dim3(blocks_for(Vs, 256)), dim3(256), 0, c.stream, Vs, b.offPT.p, b.ownPT.p);
     }
     if (Es > 0) {
-        hipLaunchKernelGGL(ee_pass, dim3(blocks_for(Es, 128)), dim3(128), 0, c.stream, Es, s.se.p, s.x.p, s.vrest.p, s.bt.p, et, sq,
-                           s.hhat, (const int*)b.offEE.p, b.cntEE.p, b.candEE.p, b.recEE.p, b.auxEE.p, b.catEE.p);
+        hipLaunchKernelGGL(ee_pass, dim3(blocks_for(Es, 64)), dim3(64), 0, c.stream, Es, s.se_perm.p, s.se.p, s.x.p, s.vrest.p, s.bt.p,
+                           et, sq, s.hhat, (const int*)b.offEE.p, b.cntEE.p, b.bufEE.p, b.candEE.p, b.recEE.p, b.auxEE.p, b.catEE.p);
         hipLaunchKernelGGL(owner_of, dim3(blocks_for(Es, 256)), dim3(256), 0, c.stream, Es, b.offEE.p, b.ownEE.p);
     }
     CIPC_CHECK_LAUNCH();

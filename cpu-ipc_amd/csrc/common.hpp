@@ -111,6 +111,7 @@ struct Scene {
     DBuf<int4> faces;          // v0, v1, v2, tet
     DBuf<int> sv;              // surface vertices
     DBuf<int2> se;             // surface edges
+    DBuf<int> sv_perm, se_perm; // query orders: surface vertices / edges by Morton code of the rest shape
     std::vector<int> h_bt;     // host copy for pattern construction
     std::vector<int> h_tets;
 };

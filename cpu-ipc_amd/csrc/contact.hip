@@ -159,29 +159,16 @@ __global__ __launch_bounds__(64) void contact_local_kernel(
     }
 }
 
-// Kernel 2: makePD of each local Hessian, in place.
-template <int N>
-__device__ inline void pd_inplace(double* Hs, int n, int c) {
-    double H[N * N];
-    int idx = 0;
-#pragma unroll
-    for (int i = 0; i < N; ++i)
-#pragma unroll
-        for (int j = i; j < N; ++j) { const double v = Hs[(size_t)(idx++) * n + c]; H[i * N + j] = v; H[j * N + i] = v; }
-    make_pd<N>(H);
-    idx = 0;
-#pragma unroll
-    for (int i = 0; i < N; ++i)
-#pragma unroll
-        for (int j = i; j < N; ++j) Hs[(size_t)(idx++) * n + c] = H[i * N + j];
-}
+// Kernel 2: makePD of each local Hessian, in place, one launch per vertex count.  The
+// projection runs on the translation-free subspace (project_contact), with the reduced
+// triangle and its eigenvectors in registers; the packed index of (i, j) is the one
+// kernel 1 writes.
+template <int M>
 __global__ __launch_bounds__(64) void contact_pd_kernel(int n, const int* __restrict__ dims, double* __restrict__ Hs) {
     const int c = blockIdx.x * blockDim.x + threadIdx.x;
-    if (c >= n) return;
-    const int N = dims[c];
-    if (N == 6) pd_inplace<6>(Hs, n, c);
-    else if (N == 9) pd_inplace<9>(Hs, n, c);
-    else pd_inplace<12>(Hs, n, c);
+    if (c >= n || dims[c] != 3 * M) return;
+    double* h = Hs + c;
+    project_contact<M>([&](int k) -> double& { return h[(size_t)k * n]; });
 }
 
 // Kernel 3: block and gradient records.  Slots: 10 blocks and 4 gradients per contact.
@@ -273,7 +260,9 @@ void contact_assemble(Context& c, double kappa) {
                            c.self_set.p, c.ee_set.p, c.ee_ie.p, s.se.p, s.x.p, s.vrest.p, s.hhat, kappa, Hs, Gs,
                            c.cverts.p, c.cdims.p);
         CIPC_CHECK_LAUNCH();
-        hipLaunchKernelGGL(contact_pd_kernel, dim3(blocks_for(n, 64)), dim3(64), 0, c.stream, n, c.cdims.p, Hs);
+        hipLaunchKernelGGL(contact_pd_kernel<4>, dim3(blocks_for(n, 64)), dim3(64), 0, c.stream, n, c.cdims.p, Hs);
+        hipLaunchKernelGGL(contact_pd_kernel<3>, dim3(blocks_for(n, 64)), dim3(64), 0, c.stream, n, c.cdims.p, Hs);
+        hipLaunchKernelGGL(contact_pd_kernel<2>, dim3(blocks_for(n, 64)), dim3(64), 0, c.stream, n, c.cdims.p, Hs);
         CIPC_CHECK_LAUNCH();
         hipLaunchKernelGGL(contact_scatter_kernel, dim3(blocks_for(n, 128)), dim3(128), 0, c.stream, n, c.cverts.p,
                            c.cdims.p, Hs, Gs, s.bt.p, ne, ne, c.ckey.p, c.cidx.p, c.cval.p, c.gkey.p, c.gidx.p, c.gval.p);
@@ -302,13 +291,29 @@ __global__ void distance_eval_kernel(int n, int kind, const double* __restrict__
     for (int i = 0; i < N; ++i)
         for (int j = 0; j < N; ++j) Ho[144 * t + i * N + j] = H[i * N + j];
 }
-__global__ void make_pd_eval_kernel(int n, int dim, double* __restrict__ H) {
+template <int N>
+__global__ __launch_bounds__(64) void make_pd_eval_kernel(int n, double* __restrict__ H) {
+    constexpr int NU = N * (N + 1) / 2;
+    __shared__ double lds[(NU + N * N) * 64];
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= n) return;
-    double* h = H + (size_t)dim * dim * t;
-    if (dim == 6) { double a[36]; for (int i = 0; i < 36; ++i) a[i] = h[i]; make_pd<6>(a); for (int i = 0; i < 36; ++i) h[i] = a[i]; }
-    else if (dim == 9) { double a[81]; for (int i = 0; i < 81; ++i) a[i] = h[i]; make_pd<9>(a); for (int i = 0; i < 81; ++i) h[i] = a[i]; }
-    else { double a[144]; for (int i = 0; i < 144; ++i) a[i] = h[i]; make_pd<12>(a); for (int i = 0; i < 144; ++i) h[i] = a[i]; }
+    double* h = H + (size_t)N * N * t;
+    double* m = lds + threadIdx.x;
+    for (int i = 0; i < N; ++i)
+        for (int j = i; j < N; ++j) m[64 * Sym<N>::ix(i, j)] = h[i * N + j];
+    make_pd_lds<N>(m);
+    for (int i = 0; i < N; ++i)
+        for (int j = i; j < N; ++j) { h[i * N + j] = m[64 * Sym<N>::ix(i, j)]; h[j * N + i] = m[64 * Sym<N>::ix(i, j)]; }
+}
+template <int M>
+__global__ __launch_bounds__(64) void contact_pd_eval_kernel(int n, double* __restrict__ H) {
+    constexpr int N = 3 * M;
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n) return;
+    double* h = H + (size_t)N * N * t;
+    if (!project_contact<M>([&](int k) -> double& { return h[Sym<N>::row(k) * N + Sym<N>::col(k)]; })) return;
+    for (int i = 0; i < N; ++i)
+        for (int j = i + 1; j < N; ++j) h[j * N + i] = h[i * N + j];
 }
 __global__ void accd_eval_kernel(int n, int kind, const double* __restrict__ pts, const double* __restrict__ dirs, double eta,
                                  double th, double* __restrict__ toc) {
@@ -351,9 +356,22 @@ extern "C" int cipc_gpu_eval_distance_impl(int n, int kind, const double* pts, d
 extern "C" int cipc_gpu_eval_make_pd_impl(int n, int dim, double* H) {
     DevArr<double> dh((size_t)dim * dim * n);
     dh.up(H, (size_t)dim * dim * n);
-    hipLaunchKernelGGL(make_pd_eval_kernel, dim3(blocks_for(n, 64)), dim3(64), 0, 0, n, dim, dh.p);
+    if (dim == 6) hipLaunchKernelGGL(make_pd_eval_kernel<6>, dim3(blocks_for(n, 64)), dim3(64), 0, 0, n, dh.p);
+    else if (dim == 9) hipLaunchKernelGGL(make_pd_eval_kernel<9>, dim3(blocks_for(n, 64)), dim3(64), 0, 0, n, dh.p);
+    else hipLaunchKernelGGL(make_pd_eval_kernel<12>, dim3(blocks_for(n, 64)), dim3(64), 0, 0, n, dh.p);
     CIPC_CHECK_LAUNCH();
     dh.down(H, (size_t)dim * dim * n);
+    return 0;
+}
+extern "C" int cipc_gpu_eval_contact_pd_impl(int n, int nverts, double* H) {
+    const size_t dim = 3 * (size_t)nverts;
+    DevArr<double> dh(dim * dim * n);
+    dh.up(H, dim * dim * n);
+    if (nverts == 2) hipLaunchKernelGGL(contact_pd_eval_kernel<2>, dim3(blocks_for(n, 64)), dim3(64), 0, 0, n, dh.p);
+    else if (nverts == 3) hipLaunchKernelGGL(contact_pd_eval_kernel<3>, dim3(blocks_for(n, 64)), dim3(64), 0, 0, n, dh.p);
+    else hipLaunchKernelGGL(contact_pd_eval_kernel<4>, dim3(blocks_for(n, 64)), dim3(64), 0, 0, n, dh.p);
+    CIPC_CHECK_LAUNCH();
+    dh.down(H, dim * dim * n);
     return 0;
 }
 extern "C" int cipc_gpu_eval_accd_impl(int n, int kind, const double* pts, const double* dirs, double eta, double th, double* toc) {

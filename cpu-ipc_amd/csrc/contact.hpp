@@ -121,17 +121,17 @@ __device__ inline double mol_e(double c, double ex) {
 template <int N>
 struct Sym {
     static constexpr int ix(int i, int j) { return i <= j ? i * N - i * (i - 1) / 2 + (j - i) : j * N - j * (j - 1) / 2 + (i - j); }
+    static constexpr int row(int k) { int i = 0; while (k >= N - i) { k -= N - i; ++i; } return i; }
+    static constexpr int col(int k) { int i = 0; while (k >= N - i) { k -= N - i; ++i; } return i + k; }
 };
-template <int N>
-__device__ inline void make_pd(double* __restrict__ H) {
-    constexpr int NU = N * (N + 1) / 2;
-    double A[NU], Z[N * N];
+// In place on the packed upper triangle A (row-major, N(N+1)/2 entries).
+// Z is the eigenvector matrix, row-major N x N, reached through zp(k) (registers, or a
+// strided LDS column of the calling thread).
+template <int N, typename ZP>
+__device__ inline void make_pd_upper_z(double* __restrict__ A, ZP zp) {
+#define Z(k) (*zp(k))
 #pragma unroll
-    for (int i = 0; i < N; ++i)
-#pragma unroll
-        for (int j = i; j < N; ++j) A[Sym<N>::ix(i, j)] = H[i * N + j];
-#pragma unroll
-    for (int i = 0; i < N * N; ++i) Z[i] = (i % (N + 1) == 0) ? 1.0 : 0.0;
+    for (int i = 0; i < N * N; ++i) Z(i) = (i % (N + 1) == 0) ? 1.0 : 0.0;
     for (int sweep = 0; sweep < 30; ++sweep) {
         double off = 0.0, dg = 0.0;
 #pragma unroll
@@ -140,7 +140,7 @@ __device__ inline void make_pd(double* __restrict__ H) {
 #pragma unroll
             for (int j = i + 1; j < N; ++j) off += A[Sym<N>::ix(i, j)] * A[Sym<N>::ix(i, j)];
         }
-        if (off == 0.0 || off <= 1e-34 * dg) break;
+        if (off == 0.0 || off <= 1e-30 * dg) break;
 #pragma unroll
         for (int p = 0; p < N; ++p) {
 #pragma unroll
@@ -163,11 +163,12 @@ __device__ inline void make_pd(double* __restrict__ H) {
                     A[Sym<N>::ix(p, q)] = 0.0;
 #pragma unroll
                     for (int k = 0; k < N; ++k) {
-                        const double zkp = Z[k * N + p], zkq = Z[k * N + q];
-                        Z[k * N + p] = c * zkp - s * zkq;
-                        Z[k * N + q] = s * zkp + c * zkq;
+                        const double zkp = Z(k * N + p), zkq = Z(k * N + q);
+                        Z(k * N + p) = c * zkp - s * zkq;
+                        Z(k * N + q) = s * zkp + c * zkq;
                     }
                 }
+
             }
         }
     }
@@ -180,10 +181,233 @@ __device__ inline void make_pd(double* __restrict__ H) {
         for (int j = i; j < N; ++j) {
             double acc = 0.0;
 #pragma unroll
-            for (int k = 0; k < N; ++k) acc += Z[i * N + k] * w[k] * Z[j * N + k];
-            H[i * N + j] = acc;
-            H[j * N + i] = acc;
+            for (int k = 0; k < N; ++k) acc += Z(i * N + k) * w[k] * Z(j * N + k);
+            A[Sym<N>::ix(i, j)] = acc;
         }
+#undef Z
+}
+template <int N>
+__device__ inline void make_pd_upper(double* __restrict__ A) {
+    double Zr[N * N];
+    make_pd_upper_z<N>(A, [&](int k) { return &Zr[k]; });
+}
+// The same rotations with the packed triangle and the eigenvectors of one thread in a
+// strided LDS column (entry k at m[k * 64]): plain loops, a handful of registers, no
+// scratch.  Used for the per-contact projection where N reaches 12.
+template <int N>
+__device__ inline void make_pd_lds(double* __restrict__ m) {
+    constexpr int NU = N * (N + 1) / 2;
+    auto A = [&](int i, int j) -> double& { return m[64 * Sym<N>::ix(i, j)]; };
+    auto Z = [&](int i, int j) -> double& { return m[64 * (NU + i * N + j)]; };
+    for (int i = 0; i < N; ++i)
+        for (int j = 0; j < N; ++j) Z(i, j) = (i == j) ? 1.0 : 0.0;
+    for (int sweep = 0; sweep < 30; ++sweep) {
+        double off = 0.0, dg = 0.0;
+        for (int i = 0; i < N; ++i) {
+            const double d = A(i, i);
+            dg += d * d;
+            for (int j = i + 1; j < N; ++j) { const double a = A(i, j); off += a * a; }
+        }
+        if (off == 0.0 || off <= 1e-30 * dg) break;
+        for (int p = 0; p < N; ++p) {
+            for (int q = p + 1; q < N; ++q) {
+                const double apq = A(p, q);
+                if (apq == 0.0) continue;
+                const double app = A(p, p), aqq = A(q, q);
+                const double th = (aqq - app) / (2.0 * apq);
+                const double t = (th >= 0 ? 1.0 : -1.0) / (fabs(th) + sqrt(th * th + 1.0));
+                const double c = 1.0 / sqrt(t * t + 1.0), s = t * c;
+                for (int k = 0; k < N; ++k) {
+                    if (k == p || k == q) continue;
+                    const double akp = A(k, p), akq = A(k, q);
+                    A(k, p) = c * akp - s * akq;
+                    A(k, q) = s * akp + c * akq;
+                }
+                A(p, p) = app - t * apq;
+                A(q, q) = aqq + t * apq;
+                A(p, q) = 0.0;
+                for (int k = 0; k < N; ++k) {
+                    const double zkp = Z(k, p), zkq = Z(k, q);
+                    Z(k, p) = c * zkp - s * zkq;
+                    Z(k, q) = s * zkp + c * zkq;
+                }
+            }
+        }
+    }
+    double w[N];
+#pragma unroll
+    for (int i = 0; i < N; ++i) w[i] = fmax(A(i, i), 0.0);
+    for (int i = 0; i < N; ++i)
+        for (int j = i; j < N; ++j) {
+            double acc = 0.0;
+#pragma unroll
+            for (int k = 0; k < N; ++k) acc += Z(i, k) * w[k] * Z(j, k);
+            A(i, j) = acc;
+        }
+}
+// ---- The same projection for contact Hessians, on the translation-free subspace.
+// A contact barrier Hessian H (3M x 3M, M = 2, 3, 4 vertices) is the Hessian of a function of
+// vertex differences only, so H (1_M (x) e_k) = 0 for the three rigid translations.  With the
+// orthonormal Helmert rows h_0..h_{M-2} of length M (all orthogonal to 1_M), B = Hm (x) I_3 is an
+// orthonormal basis of the complement, H = B^T R B with R = B H B^T of size 3(M-1), and the
+// eigenpairs of H are those of R lifted by B^T plus three zero eigenvalues.  Hence
+//   makePD(H) = H - B^T (sum_{lambda<0} lambda z z^T) B   over the eigenpairs (lambda, z) of R,
+// and H is returned unchanged when R has no negative eigenvalue.  Exact up to the rounding of
+// H's translation components (relative 1e-16 of |H|).  The 12 x 12 projection becomes 9 x 9,
+// 9 x 9 becomes 6 x 6 and 6 x 6 becomes 3 x 3: about 2.5x fewer rotations, and the packed
+// triangle plus eigenvectors (126 doubles for M = 4) fit the register file.
+__device__ constexpr double helmert(int M, int i, int a) {
+    // row i: i+1 entries 1/sqrt((i+1)(i+2)), then -(i+1)/sqrt((i+1)(i+2)), then zeros
+    constexpr double r2 = 0.70710678118654752440, r6 = 0.40824829046386301637, r12 = 0.28867513459481288225;
+    (void)M;
+    if (a <= i) return i == 0 ? r2 : i == 1 ? r6 : r12;
+    if (a == i + 1) return i == 0 ? -r2 : i == 1 ? -2.0 * r6 : -3.0 * r12;
+    return 0.0;
+}
+// Circle-method round robin over NP (even) indices: pair i of round r.
+__device__ constexpr int rr_slot(int NP, int r, int k) { return k == 0 ? 0 : ((k - 1 + r) % (NP - 1)) + 1; }
+
+// Cyclic-by-rounds Jacobi on the packed upper triangle R (Sym<NR>) with eigenvectors Z
+// (row-major NR x NR, columns are eigenvectors).  Each round rotates NP/2 disjoint pairs whose
+// angles depend only on entries no other pair of the round touches, so the angles of a round
+// are computed together (independent square roots and divisions) before the rotations.
+template <int NR>
+__device__ inline void jacobi_rounds(double* __restrict__ R, double* __restrict__ Z) {
+    constexpr int NP = NR + (NR & 1);
+    constexpr int NH = NP / 2;
+#pragma unroll
+    for (int i = 0; i < NR * NR; ++i) Z[i] = (i % (NR + 1) == 0) ? 1.0 : 0.0;
+    for (int sweep = 0; sweep < 32; ++sweep) {
+        double off = 0.0, dg = 0.0;
+#pragma unroll
+        for (int i = 0; i < NR; ++i) {
+            dg += R[Sym<NR>::ix(i, i)] * R[Sym<NR>::ix(i, i)];
+#pragma unroll
+            for (int j = i + 1; j < NR; ++j) off += R[Sym<NR>::ix(i, j)] * R[Sym<NR>::ix(i, j)];
+        }
+        if (off == 0.0 || off <= 1e-30 * dg) break;
+#pragma unroll
+        for (int r = 0; r < NP - 1; ++r) {
+            double cs[NH], sn[NH];
+#pragma unroll
+            for (int h = 0; h < NH; ++h) {
+                const int a = rr_slot(NP, r, h), b = rr_slot(NP, r, NP - 1 - h);
+                const int p = a < b ? a : b, q = a < b ? b : a;
+                cs[h] = 1.0; sn[h] = 0.0;
+                if (q >= NR) continue;
+                const double apq = R[Sym<NR>::ix(p, q)];
+                const double app = R[Sym<NR>::ix(p, p)], aqq = R[Sym<NR>::ix(q, q)];
+                const bool zero = apq == 0.0;
+                const double th = (aqq - app) / (zero ? 1.0 : 2.0 * apq);
+                double t = (th >= 0 ? 1.0 : -1.0) / (fabs(th) + sqrt(th * th + 1.0));
+                t = zero ? 0.0 : t;
+                const double c = 1.0 / sqrt(t * t + 1.0);
+                cs[h] = c; sn[h] = t * c;
+                R[Sym<NR>::ix(p, p)] = app - t * apq;
+                R[Sym<NR>::ix(q, q)] = aqq + t * apq;
+                R[Sym<NR>::ix(p, q)] = zero ? apq : 0.0;
+            }
+#pragma unroll
+            for (int h = 0; h < NH; ++h) {
+                const int a = rr_slot(NP, r, h), b = rr_slot(NP, r, NP - 1 - h);
+                const int p = a < b ? a : b, q = a < b ? b : a;
+                if (q >= NR) continue;
+                const double c = cs[h], s = sn[h];
+#pragma unroll
+                for (int k = 0; k < NR; ++k) {
+                    if (k == p || k == q) continue;
+                    const double akp = R[Sym<NR>::ix(k, p)], akq = R[Sym<NR>::ix(k, q)];
+                    R[Sym<NR>::ix(k, p)] = c * akp - s * akq;
+                    R[Sym<NR>::ix(k, q)] = s * akp + c * akq;
+                }
+#pragma unroll
+                for (int k = 0; k < NR; ++k) {
+                    const double zkp = Z[k * NR + p], zkq = Z[k * NR + q];
+                    Z[k * NR + p] = c * zkp - s * zkq;
+                    Z[k * NR + q] = s * zkp + c * zkq;
+                }
+            }
+        }
+    }
+}
+
+// In place on the packed upper triangle (Sym<3M>) reached through h(k) -> double&.
+// Returns false (and leaves h untouched) when no eigenvalue is negative.
+template <int M, typename HA>
+__device__ inline bool project_contact(HA h) {
+    constexpr int N = 3 * M, NR = 3 * (M - 1), NRU = NR * (NR + 1) / 2;
+    double R[NRU];
+    {
+        double H[N * (N + 1) / 2];
+#pragma unroll
+        for (int k = 0; k < N * (N + 1) / 2; ++k) H[k] = h(k);
+#pragma unroll
+        for (int i = 0; i < M - 1; ++i)
+#pragma unroll
+            for (int k = 0; k < 3; ++k)
+#pragma unroll
+                for (int j = i; j < M - 1; ++j)
+#pragma unroll
+                    for (int l = (j == i ? k : 0); l < 3; ++l) {
+                        double acc = 0.0;
+#pragma unroll
+                        for (int a = 0; a <= i + 1; ++a)
+#pragma unroll
+                            for (int b = 0; b <= j + 1; ++b)
+                                acc += helmert(M, i, a) * helmert(M, j, b) * H[Sym<N>::ix(3 * a + k, 3 * b + l)];
+                        R[Sym<NR>::ix(3 * i + k, 3 * j + l)] = acc;
+                    }
+    }
+    double Z[NR * NR];
+    jacobi_rounds<NR>(R, Z);
+    double w[NR];
+    bool neg = false;
+#pragma unroll
+    for (int k = 0; k < NR; ++k) { w[k] = fmin(R[Sym<NR>::ix(k, k)], 0.0); neg |= w[k] < 0.0; }
+    if (!neg) return false;
+    // negative part on the reduced space, reusing R's registers
+#pragma unroll
+    for (int p = 0; p < NR; ++p)
+#pragma unroll
+        for (int q = p; q < NR; ++q) {
+            double acc = 0.0;
+#pragma unroll
+            for (int k = 0; k < NR; ++k) acc += (w[k] * Z[p * NR + k]) * Z[q * NR + k];
+            R[Sym<NR>::ix(p, q)] = acc;
+        }
+    // lift: H_(a,k),(b,l) -= sum_{i,j} Hm[i][a] Hm[j][b] Nr_(i,k),(j,l)
+#pragma unroll
+    for (int a = 0; a < M; ++a)
+#pragma unroll
+        for (int k = 0; k < 3; ++k)
+#pragma unroll
+            for (int b = a; b < M; ++b)
+#pragma unroll
+                for (int l = (b == a ? k : 0); l < 3; ++l) {
+                    double acc = 0.0;
+#pragma unroll
+                    for (int i = (a > 0 ? a - 1 : 0); i < M - 1; ++i)
+#pragma unroll
+                        for (int j = (b > 0 ? b - 1 : 0); j < M - 1; ++j)
+                            acc += helmert(M, i, a) * helmert(M, j, b) * R[Sym<NR>::ix(3 * i + k, 3 * j + l)];
+                    h(Sym<N>::ix(3 * a + k, 3 * b + l)) -= acc;
+                }
+    return true;
+}
+
+// Full N x N row-major input/output (element-level evaluation).
+template <int N>
+__device__ inline void make_pd(double* __restrict__ H) {
+    double A[N * (N + 1) / 2];
+#pragma unroll
+    for (int i = 0; i < N; ++i)
+#pragma unroll
+        for (int j = i; j < N; ++j) A[Sym<N>::ix(i, j)] = H[i * N + j];
+    make_pd_upper<N>(A);
+#pragma unroll
+    for (int i = 0; i < N; ++i)
+#pragma unroll
+        for (int j = i; j < N; ++j) { H[i * N + j] = A[Sym<N>::ix(i, j)]; H[j * N + i] = A[Sym<N>::ix(i, j)]; }
 }
 
 // ---- Additive CCD (AdditiveCCD.cpp:146-329), reference operation order

@@ -46,6 +46,7 @@ struct cipc_gpu_context : cipc::Context {};
 extern "C" int cipc_gpu_eval_snh_impl(int n, const double* F, double u, double r, double* P, double* H, double* e);
 extern "C" int cipc_gpu_eval_distance_impl(int n, int kind, const double* pts, double* d, double* g, double* H);
 extern "C" int cipc_gpu_eval_make_pd_impl(int n, int dim, double* H);
+extern "C" int cipc_gpu_eval_contact_pd_impl(int n, int nverts, double* H);
 extern "C" int cipc_gpu_eval_accd_impl(int n, int kind, const double* pts, const double* dirs, double eta, double th, double* toc);
 extern "C" int cipc_gpu_eval_classify_impl(int n, int kind, const double* pts, int* type);
 
@@ -136,6 +137,41 @@ int cipc_gpu_upload_scene(cipc_gpu_context* c, const cipc_scene_desc* d) {
         for (int i = 0; i < 2 * d->Es; ++i) se[i] = (int)d->surf_edges[i];
         upload(s.sv, sv.data(), sv.size(), st);
         upload(s.se, reinterpret_cast<const int2*>(se.data()), (size_t)d->Es, st);
+        // spatially coherent query orders for the tree traversals
+        {
+            double lo[3] = {1e300, 1e300, 1e300}, hi[3] = {-1e300, -1e300, -1e300};
+            for (int v = 0; v < V; ++v)
+                for (int k = 0; k < 3; ++k) { lo[k] = std::min(lo[k], d->x[3 * v + k]); hi[k] = std::max(hi[k], d->x[3 * v + k]); }
+            auto code = [&](double px, double py, double pz) {
+                auto q = [&](double v, int k) {
+                    const double e = hi[k] - lo[k];
+                    const double t = e > 0 ? (v - lo[k]) / e : 0.0;
+                    return (uint32_t)std::max(0.0, std::min(t * 1024.0, 1023.0));
+                };
+                auto ex = [](uint32_t v) {
+                    v = (v * 0x00010001u) & 0xFF0000FFu; v = (v * 0x00000101u) & 0x0F00F00Fu;
+                    v = (v * 0x00000011u) & 0xC30C30C3u; v = (v * 0x00000005u) & 0x49249249u;
+                    return v;
+                };
+                return (ex(q(px, 0)) << 2) | (ex(q(py, 1)) << 1) | ex(q(pz, 2));
+            };
+            std::vector<std::pair<uint32_t, int>> ord(d->Vs);
+            for (int i = 0; i < d->Vs; ++i) ord[i] = {code(d->x[3 * sv[i]], d->x[3 * sv[i] + 1], d->x[3 * sv[i] + 2]), i};
+            std::sort(ord.begin(), ord.end());
+            std::vector<int> perm(d->Vs);
+            for (int i = 0; i < d->Vs; ++i) perm[i] = ord[i].second;
+            upload(s.sv_perm, perm.data(), perm.size(), st);
+            std::vector<std::pair<uint32_t, int>> oe(d->Es);
+            for (int i = 0; i < d->Es; ++i) {
+                const double* a = d->x + 3 * se[2 * i];
+                const double* b = d->x + 3 * se[2 * i + 1];
+                oe[i] = {code(0.5 * (a[0] + b[0]), 0.5 * (a[1] + b[1]), 0.5 * (a[2] + b[2])), i};
+            }
+            std::sort(oe.begin(), oe.end());
+            std::vector<int> pe(d->Es);
+            for (int i = 0; i < d->Es; ++i) pe[i] = oe[i].second;
+            upload(s.se_perm, pe.data(), pe.size(), st);
+        }
         CIPC_HIP(hipStreamSynchronize(st));
         build_static_pattern(*c);
         c->n_self = c->n_ee = c->n_ccd = c->n_env = 0;
@@ -397,6 +433,12 @@ int cipc_gpu_eval_make_pd(int n, int dim, double* H) {
     return guard([&] {
         require(n >= 0 && (dim == 6 || dim == 9 || dim == 12) && (n == 0 || H), "invalid arguments");
         if (n) cipc_gpu_eval_make_pd_impl(n, dim, H);
+    });
+}
+int cipc_gpu_eval_contact_pd(int n, int nverts, double* H) {
+    return guard([&] {
+        require(n >= 0 && nverts >= 2 && nverts <= 4 && (n == 0 || H), "invalid arguments");
+        if (n) cipc_gpu_eval_contact_pd_impl(n, nverts, H);
     });
 }
 int cipc_gpu_eval_accd(int n, int kind, const double* pts, const double* dirs, double eta, double th, double* toc) {

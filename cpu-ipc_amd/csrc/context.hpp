@@ -17,9 +17,10 @@ struct LBVH {
     DBuf<int> right;           // n-1 (internal nodes)
     DBuf<int> skip;            // 2n-1
     DBuf<int> parent;          // 2n-1
+    DBuf<int> range_first;     // n-1
     DBuf<int> range_last;      // n-1
     DBuf<int> split_right;     // n-1: right child of the internal node whose split is s
-    DBuf<int> flag;            // n-1 bottom-up visit counters
+    DBuf<Box> seg;             // power-of-two reduction levels over the sorted leaf boxes
     DBuf<unsigned long long> keys, keys_sorted;
     DBuf<Box> scene_partial;
     bool swept = false;
@@ -28,7 +29,7 @@ struct LBVH {
 
 // Scratch of the active-set construction (broadphase.hip).
 struct ActiveSetScratch {
-    DBuf<int> cntPT, offPT, cntEE, offEE;
+    DBuf<int> cntPT, offPT, cntEE, offEE, bufPT, bufEE;
     DBuf<int> candPT, candEE, ownPT, ownEE, catPT, catEE;
     DBuf<int4> recPT, recEE;
     DBuf<int2> auxEE;

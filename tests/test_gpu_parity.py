@@ -78,6 +78,18 @@ def test_make_pd_matches_oracle(gpu, O, dim):
         assert rel_err(Hg[i], Ho) <= REL, (dim, i)
 
 
+@pytest.mark.parametrize("kind", [0, 1, 2, 3])
+def test_contact_projection_matches_make_pd(gpu, O, kind):
+    """The projection the assembly applies to contact Hessians (on the translation-free
+    subspace) against the oracle's makePD of the full block."""
+    from contact_cases import barrier_hessians
+    H = barrier_hessians(O, kind, 300, seed=80 + kind)
+    Hg = gpu.eval_contact_pd(H)
+    for i in range(len(H)):
+        Ho = O.make_pd(H[i])
+        assert rel_err(Hg[i], Ho) <= REL, (kind, i, rel_err(Hg[i], Ho))
+
+
 def _ccd_inputs(rng, n):
     pts = rng.uniform(-1, 1, (n, 12))
     dirs = rng.uniform(-1, 1, (n, 12)) * rng.uniform(0.01, 3.0, (n, 1))

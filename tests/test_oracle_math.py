@@ -65,6 +65,30 @@ def test_make_pd_against_numpy_eigh(O, n):
             assert np.array_equal(R, H)
 
 
+@pytest.mark.parametrize("kind", [0, 1, 2, 3])
+def test_contact_make_pd_on_translation_free_subspace(O, kind):
+    """Contact Hessians annihilate rigid translations, so makePD(H) equals
+    H - B^T (sum over negative eigenpairs of R = B H B^T of lambda z z^T) B with B the
+    Helmert basis (x) I3 of the translation-free subspace; the device projects contacts
+    this way (the 12x12 solve becomes 9x9)."""
+    from contact_cases import NVERTS, barrier_hessians
+    m = NVERTS[kind]
+    hm = np.zeros((m - 1, m))
+    for i in range(m - 1):
+        hm[i, :i + 1] = 1.0 / np.sqrt((i + 1) * (i + 2))
+        hm[i, i + 1] = -(i + 1) / np.sqrt((i + 1) * (i + 2))
+    B = np.kron(hm, np.eye(3))
+    assert np.allclose(B @ B.T, np.eye(3 * m - 3)) and np.allclose(B @ np.tile(np.eye(3), (m, 1)), 0)
+    npd = 0
+    for H in barrier_hessians(O, kind, 60, seed=70 + kind):
+        assert np.abs(H @ np.tile(np.eye(3), (m, 1))).max() <= 1e-12 * np.abs(H).max()
+        w, Z = np.linalg.eigh(B @ H @ B.T)
+        P = H - B.T @ (Z @ np.diag(np.minimum(w, 0)) @ Z.T) @ B if w[0] < 0 else H
+        npd += w[0] >= 0
+        assert np.allclose(P, O.make_pd(H), atol=1e-12 * np.abs(H).max())
+    assert npd < 60   # the projection path is exercised
+
+
 # ---------------------------------------------------------------------------------------
 U_RATE, R_RATE = 4 * 35714.2857 / 3, 142857.142 + 5 * 35714.2857 / 6
 
