@@ -6,12 +6,15 @@
 //
 // Tree build: scene box reduction -> 30-bit Morton codes of box centres normalised to the
 // scene box (key = code << 32 | primitive, unique) -> radix sort -> Karras topology, one
-// thread per internal node -> bottom-up boxes with per-node arrival counters -> skip
-// pointers for stackless depth-first traversal.
+// thread per internal node -> skip pointers for stackless depth-first traversal ->
+// internal boxes as unions of power-of-two leaf ranges (no inter-work-group
+// synchronisation) -> packed 64-byte node records and a subtree frontier.
 //
-// Active set: one thread per surface vertex (point-triangle) and per surface edge
-// (edge-edge) traverses the tree twice (count, then fill), sorts its candidates by
-// primitive id as the reference's query does, and classifies them.  Compaction
+// Active set: each surface vertex (point-triangle) and surface edge (edge-edge) query is
+// split over kParts lanes, one subtree each, that buffer their first candidates; a second
+// pass per query packs them (re-walking only overflowing subtrees) and sorts them by
+// primitive id as the reference's query does; one thread per candidate classifies it.
+// Compaction
 // reproduces the reference's serial order: partial-CCD pairs, then Self_ActiveSet =
 // point-triangle entries, edge-edge entries, and the point-point/point-edge entries
 // deduplicated in std::map key order with multiplicity; Self_EE_ActiveSet for mollified
@@ -77,7 +80,7 @@ __device__ inline int prefix(const unsigned long long* k, int n, int a, int b) {
 }
 // Karras 2012 topology (LBVH.cpp:62-119,213-228): internal node i covers the key range
 // [first, last] and splits it at the highest differing bit.
-__global__ void karras_kernel(int n, const unsigned long long* __restrict__ k, int* __restrict__ left, int* __restrict__ right,
+__global__ void karras_kernel(int n, const unsigned long long* __restrict__ k, Node* __restrict__ nodes, int* __restrict__ right,
                               int* __restrict__ range_first, int* __restrict__ range_last, int* __restrict__ split_right) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n - 1) return;
@@ -111,25 +114,53 @@ __global__ void karras_kernel(int n, const unsigned long long* __restrict__ k, i
     }
     const int l = (split == first) ? split + n - 1 : split;
     const int r = (split + 1 == last) ? split + 1 + n - 1 : split + 1;
-    left[i] = l;
+    nodes[i].left = l;
     right[i] = r;
     range_first[i] = first;
     range_last[i] = last;
     split_right[split] = r;
 }
-__global__ void leaf_init(int n, const unsigned long long* __restrict__ k, const Box* __restrict__ prim, Box* __restrict__ nb,
-                          int* __restrict__ left) {
+__global__ void leaf_init(int n, const unsigned long long* __restrict__ k, const Box* __restrict__ prim, Node* __restrict__ nodes,
+                          Box* __restrict__ seg0) {
     const int s = blockIdx.x * blockDim.x + threadIdx.x;
     if (s >= n) return;
     const int p = (int)(k[s] & 0xffffffffull);
-    nb[n - 1 + s] = prim[p];
-    left[n - 1 + s] = p;
+    const Box b = prim[p];
+    Node& r = nodes[n - 1 + s];
+    r.lo = b.lo;
+    r.hi = b.hi;
+    r.left = p;
+    seg0[s] = b;
 }
-__global__ void skip_kernel(int n, const int* __restrict__ range_last, const int* __restrict__ split_right, int* __restrict__ skip) {
+__global__ void skip_kernel(int n, const int* __restrict__ range_last, const int* __restrict__ split_right, Node* __restrict__ nodes) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= 2 * n - 1) return;
     const int last = (i < n - 1) ? range_last[i] : i - (n - 1);
-    skip[i] = (last >= n - 1) ? -1 : split_right[last];
+    nodes[i].skip = (last >= n - 1) ? -1 : split_right[last];
+}
+// Subtree frontier for split queries: expand the root breadth-wise while the list fits in
+// kParts, keeping depth-first order (a leaf stays as it is).
+__global__ void frontier_kernel(int n, const Node* __restrict__ nodes, const int* __restrict__ right, int* __restrict__ fr) {
+    if (blockIdx.x != 0 || threadIdx.x != 0) return;
+    int cur[kParts], nxt[kParts];
+    int m = n > 0 ? 1 : 0;
+    cur[0] = 0;
+    for (;;) {
+        int internal = 0;
+        for (int i = 0; i < m; ++i) internal += cur[i] < n - 1;
+        if (internal == 0 || m + internal > kParts) break;
+        int k = 0;
+        for (int i = 0; i < m; ++i) {
+            if (cur[i] < n - 1) { nxt[k++] = nodes[cur[i]].left; nxt[k++] = right[cur[i]]; }
+            else nxt[k++] = cur[i];
+        }
+        m = k;
+        for (int i = 0; i < m; ++i) cur[i] = nxt[i];
+    }
+    for (int i = 0; i < kParts; ++i) {
+        fr[i] = i < m ? cur[i] : -1;
+        fr[kParts + i] = i < m ? nodes[cur[i]].skip : -1;
+    }
 }
 
 // Internal boxes.  Node i of the Karras tree covers the sorted leaves [first, last], so its
@@ -167,7 +198,7 @@ __global__ __launch_bounds__(kSegBlock) void seg_build(const Box* __restrict__ s
     }
 }
 __global__ void internal_boxes(int n, const int* __restrict__ first, const int* __restrict__ last, SegLevels L,
-                               const Box* __restrict__ seg, Box* __restrict__ nb) {
+                               const Box* __restrict__ seg, Node* __restrict__ nodes) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n - 1) return;
     int a = first[i], b = last[i], l = 0;
@@ -177,17 +208,32 @@ __global__ void internal_boxes(int n, const int* __restrict__ first, const int* 
         if (a <= b && !(b & 1)) { const Box x = seg[L.offset[l] + b]; acc = Box{vmin(acc.lo, x.lo), vmax(acc.hi, x.hi)}; --b; }
         a >>= 1; b >>= 1; ++l;
     }
-    nb[i] = acc;
+    nodes[i].lo = acc.lo;
+    nodes[i].hi = acc.hi;
 }
 
 void build_tree(Context& c, LBVH& t, int n) {
     t.n = n;
     if (n <= 0) return;
     const int nodes = 2 * n - 1;
-    t.node_box.resize(nodes); t.left.resize(nodes); t.skip.resize(nodes);
+    t.nodes.resize(nodes); t.frontier.resize(2 * kParts);
     t.right.resize(std::max(n - 1, 1)); t.range_first.resize(std::max(n - 1, 1)); t.range_last.resize(std::max(n - 1, 1));
     t.split_right.resize(std::max(n - 1, 1));
     t.keys.resize(n); t.keys_sorted.resize(n);
+    // level l of the box reduction holds ceil(n / 2^l) boxes; level 0 is the sorted leaves
+    SegLevels L{};
+    long long total = 0;
+    int levels = 0;
+    for (int l = 0; l < 40; ++l) {
+        const long long cnt = ((long long)n + (1ll << l) - 1) >> l;
+        L.count[l] = (int)cnt;
+        L.offset[l] = total;
+        total += cnt;
+        levels = l;
+        if (cnt == 1) break;
+    }
+    for (int l = levels + 1; l < 40; ++l) { L.count[l] = 0; L.offset[l] = total; }
+    t.seg.resize((size_t)total);
     const int parts = std::min<int>(kRedParts, (int)blocks_for(n, kRedBlock));
     t.scene_partial.resize(parts);
     hipLaunchKernelGGL(box_reduce_partial, dim3(parts), dim3(kRedBlock), 0, c.stream, n, t.prim.p, t.scene_partial.p);
@@ -200,31 +246,15 @@ void build_tree(Context& c, LBVH& t, int n) {
     CIPC_CHECK_LAUNCH();
     sort_u64(c, t.keys.p, t.keys_sorted.p, n, 64);
     if (n > 1) {
-        hipLaunchKernelGGL(karras_kernel, dim3(blocks_for(n - 1, 256)), dim3(256), 0, c.stream, n, t.keys_sorted.p, t.left.p,
+        hipLaunchKernelGGL(karras_kernel, dim3(blocks_for(n - 1, 256)), dim3(256), 0, c.stream, n, t.keys_sorted.p, t.nodes.p,
                            t.right.p, t.range_first.p, t.range_last.p, t.split_right.p);
         CIPC_CHECK_LAUNCH();
     }
-    hipLaunchKernelGGL(leaf_init, dim3(blocks_for(n, 256)), dim3(256), 0, c.stream, n, t.keys_sorted.p, t.prim.p, t.node_box.p,
-                       t.left.p);
+    hipLaunchKernelGGL(leaf_init, dim3(blocks_for(n, 256)), dim3(256), 0, c.stream, n, t.keys_sorted.p, t.prim.p, t.nodes.p, t.seg.p);
     CIPC_CHECK_LAUNCH();
-    hipLaunchKernelGGL(skip_kernel, dim3(blocks_for(nodes, 256)), dim3(256), 0, c.stream, n, t.range_last.p, t.split_right.p, t.skip.p);
+    hipLaunchKernelGGL(skip_kernel, dim3(blocks_for(nodes, 256)), dim3(256), 0, c.stream, n, t.range_last.p, t.split_right.p, t.nodes.p);
     CIPC_CHECK_LAUNCH();
     if (n > 1) {
-        // level l holds ceil(n / 2^l) boxes; level 0 is the sorted leaves themselves
-        SegLevels L{};
-        long long total = 0;
-        int levels = 0;
-        for (int l = 0; l < 40; ++l) {
-            const long long cnt = ((long long)n + (1ll << l) - 1) >> l;
-            L.count[l] = (int)cnt;
-            L.offset[l] = total;
-            total += cnt;
-            levels = l;
-            if (cnt == 1) break;
-        }
-        for (int l = levels + 1; l < 40; ++l) { L.count[l] = 0; L.offset[l] = total; }
-        t.seg.resize((size_t)total);
-        CIPC_HIP(hipMemcpyAsync(t.seg.p, t.node_box.p + (n - 1), sizeof(Box) * n, hipMemcpyDeviceToDevice, c.stream));
         for (int l0 = 0; l0 < levels; l0 += kSegLevels) {
             const int nsrc = L.count[l0];
             hipLaunchKernelGGL(seg_build, dim3(blocks_for(nsrc, kSegBlock)), dim3(kSegBlock), 0, c.stream, t.seg.p + L.offset[l0], nsrc,
@@ -232,9 +262,11 @@ void build_tree(Context& c, LBVH& t, int n) {
             CIPC_CHECK_LAUNCH();
         }
         hipLaunchKernelGGL(internal_boxes, dim3(blocks_for(n - 1, 256)), dim3(256), 0, c.stream, n, t.range_first.p, t.range_last.p, L,
-                           t.seg.p, t.node_box.p);
+                           t.seg.p, t.nodes.p);
         CIPC_CHECK_LAUNCH();
     }
+    hipLaunchKernelGGL(frontier_kernel, dim3(1), dim3(64), 0, c.stream, n, t.nodes.p, t.right.p, t.frontier.p);
+    CIPC_CHECK_LAUNCH();
 }
 
 // Primitive boxes: static (CPUBroadPhaseBVH::build, LBVH.cpp:284-305) or swept with step
@@ -304,26 +336,9 @@ void build_swept_trees(Context& c, double alpha) {
 // ---------------------------------------------------------------------------------
 // Active set (CollisionBroadPhase.cpp:765-1059)
 // ---------------------------------------------------------------------------------
-// candidate categories
-enum : int { kNone = 0, kSelfPT = 1, kMap = 2, kSelfEE = 3, kMolPE = 4, kMolEE = 5 };
+// candidate categories; kDrop: a box hit that the reference's candidate filters discard
+enum : int { kNone = 0, kSelfPT = 1, kMap = 2, kSelfEE = 3, kMolPE = 4, kMolEE = 5, kDrop = 6 };
 
-struct TreeView {
-    int n;
-    const Box* nb;
-    const int* left;
-    const int* skip;
-};
-__device__ inline bool pt_keep(int vI, int4 fv, const int* bt, const double* x, const Box& pb, double sq) {
-    if (is_collider(bt[vI]) && is_collider(bt[fv.x]) && is_collider(bt[fv.y]) && is_collider(bt[fv.z])) return false;
-    if (vI == fv.x || vI == fv.y || vI == fv.z) return false;
-    return overlaps(pb, box_tri(ld3(x, fv.x), ld3(x, fv.y), ld3(x, fv.z)), sq);
-}
-__device__ inline bool ee_keep(int eI, int eJ, int2 A, int2 B, const int* bt, const double* x, const Box& b1, double sq) {
-    if (eJ <= eI) return false;   // queryEdgeForEdgesWithBBoxCheck drops candidates <= eI
-    if (is_collider(bt[A.x]) && is_collider(bt[A.y]) && is_collider(bt[B.x]) && is_collider(bt[B.y])) return false;
-    if (A.x == B.x || A.x == B.y || A.y == B.x || A.y == B.y || eI > eJ) return false;
-    return overlaps(b1, box_edge(ld3(x, B.x), ld3(x, B.y)), sq);
-}
 __device__ inline void sort_segment(int* a, int n) {   // insertion sort of a short id list
     for (int i = 1; i < n; ++i) {
         const int v = a[i];
@@ -333,46 +348,84 @@ __device__ inline void sort_segment(int* a, int n) {   // insertion sort of a sh
     }
 }
 
-// Candidate buffering: pass A traverses once, keeps the first kCap candidates of each
-// query and its total count; pass B copies them to the packed layout (re-traversing only
-// queries with more than kCap candidates), sorts them by primitive id and classifies.
-// Queries are visited in the Morton order of their rest positions (perm) so that the
-// lanes of a wave walk the same part of the tree.
-constexpr int kCap = 32;
+// Candidate collection.
+//   A (traverse): each query is split over kParts lanes, one subtree each; a lane records
+//     the first kPartCap leaves whose box overlaps the query box with gap sqrt(Hhat) (for
+//     edges only later edges, as queryEdgeForEdgesWithBBoxCheck keeps) and its count; the
+//     lanes of a query add their counts.  The walk loads nothing but tree nodes.
+//   B (compact, one thread per query): the parts in order into the packed layout,
+//     re-walking only parts with more than kPartCap hits; sort by primitive id.
+//   C (classify, one thread per hit): the reference's incidence and collider filters
+//     (CollisionBroadPhase.cpp:769-776,862-870), then the distance-type classification.
+// The leaf boxes are the primitives' own boxes (face_boxes / edge_boxes), so the leaf test
+// is the box test of the reference's candidate filter.  Queries are taken in the Morton
+// order of their rest positions (perm) so that the lanes of a wave walk nearby subtrees.
+constexpr int kPartCap = 32;
+static_assert(64 % kParts == 0, "a query's lanes must share a wavefront");
 
-__global__ void pt_pass(int Vs, const int* __restrict__ perm, const int* __restrict__ sv, const double* __restrict__ x,
-                        const int* __restrict__ bt, const int4* __restrict__ faces, TreeView tv, double sq, double hhat,
-                        const int* __restrict__ off, int* __restrict__ cnt, int* __restrict__ buf, int* __restrict__ cand,
-                        int4* __restrict__ rec, int* __restrict__ cat) {
+__device__ inline int sum_over_parts(int v) {
+#pragma unroll
+    for (int m = 1; m < kParts; m <<= 1) v += __shfl_xor(v, m, kParts);
+    return v;
+}
+
+__global__ __launch_bounds__(64) void pt_traverse(int Vs, const int* __restrict__ perm, const int* __restrict__ sv,
+                                                   const double* __restrict__ x, TreeView tv, double sq, int* __restrict__ cnt,
+                                                   int* __restrict__ cntp, int* __restrict__ buf) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    const bool active = t < Vs * kParts;
+    const int s = active ? perm[t / kParts] : 0, part = t % kParts;
+    int n = 0;
+    if (active) {
+        const Box pb = box_pt(ld3(x, sv[s]));
+        int* b = buf + ((size_t)s * kParts + part) * kPartCap;
+        lbvh_visit(tv, part, pb, sq, [&](int f) { if (n < kPartCap) b[n] = f; ++n; });
+        cntp[s * kParts + part] = n;
+    }
+    const int tot = sum_over_parts(n);
+    if (active && part == 0) cnt[s] = tot;
+}
+
+__global__ void pt_compact(int Vs, const int* __restrict__ perm, const int* __restrict__ sv, const double* __restrict__ x, TreeView tv,
+                           double sq, const int* __restrict__ off, const int* __restrict__ cntp, const int* __restrict__ buf,
+                           int* __restrict__ cand, int* __restrict__ own) {
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= Vs) return;
     const int s = perm[t];
-    const int vI = sv[s];
-    const d3 P = ld3(x, vI);
-    const Box pb = box_pt(P);
-    if (off == nullptr) {
-        int n = 0;
-        int* b = buf + (size_t)s * kCap;
-        lbvh_visit(tv.n, tv.nb, tv.left, tv.skip, pb, sq, [&](int f) {
-            if (pt_keep(vI, faces[f], bt, x, pb, sq)) { if (n < kCap) b[n] = f; ++n; }
-        });
-        cnt[s] = n;
+    const int o = off[s];
+    int pos = o;
+    for (int part = 0; part < kParts; ++part) {
+        const int c = cntp[s * kParts + part];
+        if (c <= kPartCap) {
+            for (int k = 0; k < c; ++k) cand[pos + k] = buf[((size_t)s * kParts + part) * kPartCap + k];
+        } else {
+            const Box pb = box_pt(ld3(x, sv[s]));
+            int m = 0;
+            lbvh_visit(tv, part, pb, sq, [&](int f) { if (m < c) cand[pos + m] = f; ++m; });
+        }
+        pos += c;
+    }
+    const int n = pos - o;
+    sort_segment(cand + o, n);
+    for (int k = 0; k < n; ++k) own[o + k] = s;
+}
+
+// Point-triangle hits: filters, then classification (CollisionBroadPhase.cpp:765-857).
+__global__ void pt_classify(int n, const int* __restrict__ own, const int* __restrict__ sv, const double* __restrict__ x,
+                            const int* __restrict__ bt, const int4* __restrict__ faces, const int* __restrict__ cand, double hhat,
+                            int4* __restrict__ rec, int* __restrict__ cat) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n) return;
+    const int vI = sv[own[k]];
+    const int4 fv = faces[cand[k]];
+    if ((is_collider(bt[vI]) && is_collider(bt[fv.x]) && is_collider(bt[fv.y]) && is_collider(bt[fv.z])) || vI == fv.x ||
+        vI == fv.y || vI == fv.z) {
+        rec[k] = make_int4(0, 0, 0, 0);
+        cat[k] = kDrop;
         return;
     }
-    const int o = off[s], cap = off[s + 1] - o;
-    int n = 0;
-    if (cap <= kCap) {
-        for (; n < cap; ++n) cand[o + n] = buf[(size_t)s * kCap + n];
-    } else {
-        lbvh_visit(tv.n, tv.nb, tv.left, tv.skip, pb, sq, [&](int f) {
-            if (pt_keep(vI, faces[f], bt, x, pb, sq)) { if (n < cap) cand[o + n] = f; ++n; }
-        });
-        n = n < cap ? n : cap;
-    }
-    sort_segment(cand + o, n);
-    for (int k = 0; k < n; ++k) {
-        const int f = cand[o + k];
-        const int4 fv = faces[f];
+    const d3 P = ld3(x, vI);
+    {
         const d3 A = ld3(x, fv.x), B = ld3(x, fv.y), C = ld3(x, fv.z);
         const int dt = dtype_pt(P, A, B, C);
         int4 r = make_int4(0, 0, 0, 0);
@@ -388,44 +441,75 @@ __global__ void pt_pass(int Vs, const int* __restrict__ perm, const int* __restr
         case 6: d = d_PT(P, A, B, C); if (d < hhat) { r = make_int4(-vI - 1, fv.x, fv.y, fv.z); ct = kSelfPT; } break;
         default: break;
         }
-        rec[o + k] = r;
-        cat[o + k] = ct;
+        rec[k] = r;
+        cat[k] = ct;
     }
 }
 
-__global__ void ee_pass(int Es, const int* __restrict__ perm, const int2* __restrict__ se, const double* __restrict__ x,
-                        const double* __restrict__ vrest, const int* __restrict__ bt, TreeView tv, double sq, double hhat,
-                        const int* __restrict__ off, int* __restrict__ cnt, int* __restrict__ buf, int* __restrict__ cand,
-                        int4* __restrict__ rec, int2* __restrict__ aux, int* __restrict__ cat) {
+__global__ __launch_bounds__(64) void ee_traverse(int Es, const int* __restrict__ perm, const int2* __restrict__ se,
+                                                   const double* __restrict__ x, TreeView tv, double sq, int* __restrict__ cnt,
+                                                   int* __restrict__ cntp, int* __restrict__ buf) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    const bool active = t < Es * kParts;
+    const int eI = active ? perm[t / kParts] : 0, part = t % kParts;
+    int n = 0;
+    if (active) {
+        const int2 A = se[eI];
+        const Box b1 = box_edge(ld3(x, A.x), ld3(x, A.y));
+        int* b = buf + ((size_t)eI * kParts + part) * kPartCap;
+        lbvh_visit(tv, part, b1, sq, [&](int eJ) {
+            if (eJ > eI) { if (n < kPartCap) b[n] = eJ; ++n; }
+        });
+        cntp[eI * kParts + part] = n;
+    }
+    const int tot = sum_over_parts(n);
+    if (active && part == 0) cnt[eI] = tot;
+}
+
+__global__ void ee_compact(int Es, const int* __restrict__ perm, const int2* __restrict__ se, const double* __restrict__ x, TreeView tv,
+                           double sq, const int* __restrict__ off, const int* __restrict__ cntp, const int* __restrict__ buf,
+                           int* __restrict__ cand, int* __restrict__ own) {
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= Es) return;
     const int eI = perm[t];
-    const int2 A = se[eI];
-    const d3 a0 = ld3(x, A.x), a1 = ld3(x, A.y);
-    const Box b1 = box_edge(a0, a1);
-    if (off == nullptr) {
-        int n = 0;
-        int* b = buf + (size_t)eI * kCap;
-        lbvh_visit(tv.n, tv.nb, tv.left, tv.skip, b1, sq, [&](int eJ) {
-            if (ee_keep(eI, eJ, A, se[eJ], bt, x, b1, sq)) { if (n < kCap) b[n] = eJ; ++n; }
-        });
-        cnt[eI] = n;
+    const int o = off[eI];
+    int pos = o;
+    for (int part = 0; part < kParts; ++part) {
+        const int c = cntp[eI * kParts + part];
+        if (c <= kPartCap) {
+            for (int k = 0; k < c; ++k) cand[pos + k] = buf[((size_t)eI * kParts + part) * kPartCap + k];
+        } else {
+            const int2 A = se[eI];
+            const Box b1 = box_edge(ld3(x, A.x), ld3(x, A.y));
+            int m = 0;
+            lbvh_visit(tv, part, b1, sq, [&](int eJ) {
+                if (eJ > eI) { if (m < c) cand[pos + m] = eJ; ++m; }
+            });
+        }
+        pos += c;
+    }
+    const int n = pos - o;
+    sort_segment(cand + o, n);
+    for (int k = 0; k < n; ++k) own[o + k] = eI;
+}
+
+// Edge-edge hits: filters, then classification (CollisionBroadPhase.cpp:858-1052).
+__global__ void ee_classify(int n, int Es, const int* __restrict__ own, const int2* __restrict__ se, const double* __restrict__ x,
+                            const double* __restrict__ vrest, const int* __restrict__ bt, const int* __restrict__ cand, double hhat,
+                            int4* __restrict__ rec, int2* __restrict__ aux, int* __restrict__ cat) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n) return;
+    const int eI = own[k], eJ = cand[k];
+    const int2 A = se[eI], B = se[eJ];
+    if ((is_collider(bt[A.x]) && is_collider(bt[A.y]) && is_collider(bt[B.x]) && is_collider(bt[B.y])) || A.x == B.x ||
+        A.x == B.y || A.y == B.x || A.y == B.y) {
+        rec[k] = make_int4(0, 0, 0, 0);
+        aux[k] = make_int2(-1, -1);
+        cat[k] = kDrop;
         return;
     }
-    const int o = off[eI], cap = off[eI + 1] - o;
-    int n = 0;
-    if (cap <= kCap) {
-        for (; n < cap; ++n) cand[o + n] = buf[(size_t)eI * kCap + n];
-    } else {
-        lbvh_visit(tv.n, tv.nb, tv.left, tv.skip, b1, sq, [&](int eJ) {
-            if (ee_keep(eI, eJ, A, se[eJ], bt, x, b1, sq)) { if (n < cap) cand[o + n] = eJ; ++n; }
-        });
-        n = n < cap ? n : cap;
-    }
-    sort_segment(cand + o, n);
-    for (int k = 0; k < n; ++k) {
-        const int eJ = cand[o + k];
-        const int2 B = se[eJ];
+    const d3 a0 = ld3(x, A.x), a1 = ld3(x, A.y);
+    {
         const d3 b0 = ld3(x, B.x), b1v = ld3(x, B.y);
         const int dt = dtype_ee(a0, a1, b0, b1v);
         const double cr = ee_cross_sqn(a0, a1, b0, b1v);
@@ -461,39 +545,60 @@ __global__ void ee_pass(int Es, const int* __restrict__ perm, const int2* __rest
             else if (r.w >= -Es - 1) { ax = make_int2(eI, -r.w - 2); r.w = -1; ct = kMolPE; }
             else { r.w = -r.w - Es - 2; ct = kMolEE; }
         }
-        rec[o + k] = r;
-        aux[o + k] = ax;
-        cat[o + k] = ct;
+        rec[k] = r;
+        aux[k] = ax;
+        cat[k] = ct;
     }
 }
 
 __global__ void fill_tail(int* p, int n) { p[n] = 0; }
-__global__ void flags_by_cat(int n, const int* __restrict__ cat, unsigned mask, int* __restrict__ flag) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i > n) return;
-    flag[i] = (i < n) ? (int)((mask >> cat[i]) & 1u) : 0;
-}
-__global__ void scatter_int4(int n, const int* __restrict__ flag, const int* __restrict__ pos, const int4* __restrict__ in,
-                             int base, int4* __restrict__ out) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n || !flag[i]) return;
-    out[base + pos[i]] = in[i];
-}
-__global__ void scatter_int2(int n, const int* __restrict__ flag, const int* __restrict__ pos, const int2* __restrict__ in,
-                             int base, int2* __restrict__ out) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n || !flag[i]) return;
-    out[base + pos[i]] = in[i];
-}
+// Per-category output positions: one exclusive scan of four 0/1 category counts per hit
+// (int4 x, y, z, w).
+struct Cnt4Sum {
+    __host__ __device__ int4 operator()(const int4& p, const int4& q) const { return make_int4(p.x + q.x, p.y + q.y, p.z + q.z, p.w + q.w); }
+};
+struct CatFlags {
+    const int* cat;
+    int n;
+    unsigned m0, m1, m2, m3;
+    __host__ __device__ int4 operator()(int i) const {
+        if (i >= n) return make_int4(0, 0, 0, 0);
+        const unsigned bit = 1u << cat[i];
+        return make_int4((m0 & bit) ? 1 : 0, (m1 & bit) ? 1 : 0, (m2 & bit) ? 1 : 0, (m3 & bit) ? 1 : 0);
+    }
+};
 // PP/PE map key: (c0 + V, c1, c2 + 1) packed with `bits` per field preserves the signed
 // lexicographic order of std::map<EncodedContact, int> (all keys have c3 = -1).
-__global__ void map_keys(int n, const int* __restrict__ flag, const int* __restrict__ pos, const int4* __restrict__ rec, int base,
-                         int V, int bits, unsigned long long* __restrict__ out) {
+__device__ inline unsigned long long map_key(int4 r, int V, int bits) {
+    return ((unsigned long long)(r.x + V) << (2 * bits)) | ((unsigned long long)r.y << bits) | (unsigned long long)(r.z + 1);
+}
+// Point-triangle hits -> Self_ActiveSet head (a), map keys (b), partial-CCD pairs (c).
+__global__ void scatter_pt(int n, const int* __restrict__ cat, const int4* __restrict__ pos, const int4* __restrict__ rec,
+                           const int* __restrict__ own, const int* __restrict__ cand, int V, int bits, int4* __restrict__ self_set,
+                           unsigned long long* __restrict__ mkeys, int2* __restrict__ ccd) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n || !flag[i]) return;
-    const int4 r = rec[i];
-    out[base + pos[i]] = ((unsigned long long)(r.x + V) << (2 * bits)) | ((unsigned long long)r.y << bits) |
-                         (unsigned long long)(r.z + 1);
+    if (i >= n) return;
+    const int ct = cat[i];
+    if (ct == kDrop) return;
+    const int4 p = pos[i];
+    ccd[p.z] = make_int2(-own[i] - 1, cand[i]);
+    if (ct == kSelfPT) self_set[p.x] = rec[i];
+    else if (ct == kMap) mkeys[p.y] = map_key(rec[i], V, bits);
+}
+// Edge-edge hits -> Self_ActiveSet (a), map keys (b), Self_EE_ActiveSet + eIe (c), CCD pairs (d).
+__global__ void scatter_ee(int n, const int* __restrict__ cat, const int4* __restrict__ pos, const int4* __restrict__ rec,
+                           const int2* __restrict__ aux, const int* __restrict__ own, const int* __restrict__ cand, int V, int bits,
+                           int base_self, int base_map, int base_ccd, int4* __restrict__ self_set, unsigned long long* __restrict__ mkeys,
+                           int4* __restrict__ ee_set, int2* __restrict__ ee_ie, int2* __restrict__ ccd) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int ct = cat[i];
+    if (ct == kDrop) return;
+    const int4 p = pos[i];
+    ccd[base_ccd + p.w] = make_int2(own[i], cand[i]);
+    if (ct == kSelfEE) self_set[base_self + p.x] = rec[i];
+    else if (ct == kMap) mkeys[base_map + p.y] = map_key(rec[i], V, bits);
+    else if (ct == kMolPE || ct == kMolEE) { ee_set[p.z] = rec[i]; ee_ie[p.z] = aux[i]; }
 }
 __global__ void map_decode(int n, const unsigned long long* __restrict__ keys, const int* __restrict__ counts, int V, int bits,
                            int base, int4* __restrict__ out) {
@@ -501,22 +606,6 @@ __global__ void map_decode(int n, const unsigned long long* __restrict__ keys, c
     if (i >= n) return;
     const unsigned long long k = keys[i], m = (1ull << bits) - 1;
     out[base + i] = make_int4((int)((k >> (2 * bits)) & m) - V, (int)((k >> bits) & m), (int)(k & m) - 1, -counts[i]);
-}
-__global__ void ccd_pairs_pt(int n, const int* __restrict__ q, const int* __restrict__ cand, int2* __restrict__ out) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    out[i] = make_int2(-q[i] - 1, cand[i]);
-}
-__global__ void ccd_pairs_ee(int n, int base, const int* __restrict__ q, const int* __restrict__ cand, int2* __restrict__ out) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    out[base + i] = make_int2(q[i], cand[i]);
-}
-// query id of every candidate from the per-query offsets
-__global__ void owner_of(int nq, const int* __restrict__ off, int* __restrict__ owner) {
-    const int s = blockIdx.x * blockDim.x + threadIdx.x;
-    if (s >= nq) return;
-    for (int i = off[s]; i < off[s + 1]; ++i) owner[i] = s;
 }
 __global__ void ground_flags(int Vs, const int* __restrict__ sv, const double* __restrict__ x, d3 gn, double gd, double hhat,
                              int* __restrict__ flag) {
@@ -542,108 +631,112 @@ int read_int(Context& c, const int* p) {
 }
 }  // namespace
 
+void category_scan(Context& c, const int* cat, int n, unsigned m0, unsigned m1, unsigned m2, unsigned m3, DBuf<int4>& pos) {
+    pos.resize((size_t)n + 1);
+    hipcub::CountingInputIterator<int> idx(0);
+    hipcub::TransformInputIterator<int4, CatFlags, hipcub::CountingInputIterator<int>> it(idx, CatFlags{cat, n, m0, m1, m2, m3});
+    size_t bytes = 0;
+    CIPC_HIP(hipcub::DeviceScan::ExclusiveScan(nullptr, bytes, it, pos.p, Cnt4Sum(), make_int4(0, 0, 0, 0), n + 1, c.stream));
+    c.tmp.reserve(bytes);
+    CIPC_HIP(hipcub::DeviceScan::ExclusiveScan(c.tmp.p, bytes, it, pos.p, Cnt4Sum(), make_int4(0, 0, 0, 0), n + 1, c.stream));
+}
+
 void compute_active_set(Context& c) {
     Scene& s = c.sc;
     build_static_trees(c);
     ActiveSetScratch& b = c.as;
     const double sq = sqrt(s.hhat);
     const int Vs = s.Vs, Es = s.Es;
-    const TreeView ft{c.face_tree.n, c.face_tree.node_box.p, c.face_tree.left.p, c.face_tree.skip.p};
-    const TreeView et{c.edge_tree.n, c.edge_tree.node_box.p, c.edge_tree.left.p, c.edge_tree.skip.p};
+    const TreeView ft{c.face_tree.n, c.face_tree.nodes.p, c.face_tree.frontier.p};
+    const TreeView et{c.edge_tree.n, c.edge_tree.nodes.p, c.edge_tree.frontier.p};
     b.cntPT.resize(Vs + 1); b.offPT.resize(Vs + 1); b.cntEE.resize(Es + 1); b.offEE.resize(Es + 1);
-    b.bufPT.resize((size_t)kCap * std::max(Vs, 1)); b.bufEE.resize((size_t)kCap * std::max(Es, 1));
-    // traverse once: counts + first kCap candidates
+    b.cntpPT.resize((size_t)kParts * std::max(Vs, 1)); b.cntpEE.resize((size_t)kParts * std::max(Es, 1));
+    b.bufPT.resize((size_t)kParts * kPartCap * std::max(Vs, 1)); b.bufEE.resize((size_t)kParts * kPartCap * std::max(Es, 1));
+    // A: one walk per query part
     if (Vs > 0)
-        hipLaunchKernelGGL(pt_pass, dim3(blocks_for(Vs, 64)), dim3(64), 0, c.stream, Vs, s.sv_perm.p, s.sv.p, s.x.p, s.bt.p, s.faces.p,
-                           ft, sq, s.hhat, (const int*)nullptr, b.cntPT.p, b.bufPT.p, (int*)nullptr, (int4*)nullptr, (int*)nullptr);
+        hipLaunchKernelGGL(pt_traverse, dim3(blocks_for((size_t)Vs * kParts, 64)), dim3(64), 0, c.stream, Vs, s.sv_perm.p, s.sv.p, s.x.p,
+                           ft, sq, b.cntPT.p, b.cntpPT.p, b.bufPT.p);
     if (Es > 0)
-        hipLaunchKernelGGL(ee_pass, dim3(blocks_for(Es, 64)), dim3(64), 0, c.stream, Es, s.se_perm.p, s.se.p, s.x.p, s.vrest.p, s.bt.p,
-                           et, sq, s.hhat, (const int*)nullptr, b.cntEE.p, b.bufEE.p, (int*)nullptr, (int4*)nullptr, (int2*)nullptr,
-                           (int*)nullptr);
+        hipLaunchKernelGGL(ee_traverse, dim3(blocks_for((size_t)Es * kParts, 64)), dim3(64), 0, c.stream, Es, s.se_perm.p, s.se.p, s.x.p,
+                           et, sq, b.cntEE.p, b.cntpEE.p, b.bufEE.p);
     CIPC_CHECK_LAUNCH();
     hipLaunchKernelGGL(fill_tail, dim3(1), dim3(1), 0, c.stream, b.cntPT.p, Vs);
     hipLaunchKernelGGL(fill_tail, dim3(1), dim3(1), 0, c.stream, b.cntEE.p, Es);
     exclusive_scan_int(c, b.cntPT.p, b.offPT.p, Vs + 1);
     exclusive_scan_int(c, b.cntEE.p, b.offEE.p, Es + 1);
-    const int nPT = read_int(c, b.offPT.p + Vs);
-    const int nEE = read_int(c, b.offEE.p + Es);
-    // fill
+    int nh[2];
+    CIPC_HIP(hipMemcpyAsync(&nh[0], b.offPT.p + Vs, sizeof(int), hipMemcpyDeviceToHost, c.stream));
+    CIPC_HIP(hipMemcpyAsync(&nh[1], b.offEE.p + Es, sizeof(int), hipMemcpyDeviceToHost, c.stream));
+    CIPC_HIP(hipStreamSynchronize(c.stream));
+    const int nPT = nh[0], nEE = nh[1];
+    // B + C: pack, sort, filter and classify the hits
     b.candPT.resize(nPT + 1); b.recPT.resize(nPT + 1); b.catPT.resize(nPT + 1); b.ownPT.resize(nPT + 1);
     b.candEE.resize(nEE + 1); b.recEE.resize(nEE + 1); b.catEE.resize(nEE + 1); b.auxEE.resize(nEE + 1); b.ownEE.resize(nEE + 1);
     if (Vs > 0) {
-        hipLaunchKernelGGL(pt_pass, dim3(blocks_for(Vs, 64)), dim3(64), 0, c.stream, Vs, s.sv_perm.p, s.sv.p, s.x.p, s.bt.p, s.faces.p,
-                           ft, sq, s.hhat, (const int*)b.offPT.p, b.cntPT.p, b.bufPT.p, b.candPT.p, b.recPT.p, b.catPT.p);
-        hipLaunchKernelGGL(owner_of, dim3(blocks_for(Vs, 256)), dim3(256), 0, c.stream, Vs, b.offPT.p, b.ownPT.p);
+        hipLaunchKernelGGL(pt_compact, dim3(blocks_for(Vs, 64)), dim3(64), 0, c.stream, Vs, s.sv_perm.p, s.sv.p, s.x.p, ft, sq, b.offPT.p,
+                           b.cntpPT.p, b.bufPT.p, b.candPT.p, b.ownPT.p);
+        if (nPT > 0)
+            hipLaunchKernelGGL(pt_classify, dim3(blocks_for(nPT, 128)), dim3(128), 0, c.stream, nPT, b.ownPT.p, s.sv.p, s.x.p, s.bt.p,
+                               s.faces.p, b.candPT.p, s.hhat, b.recPT.p, b.catPT.p);
     }
     if (Es > 0) {
-        hipLaunchKernelGGL(ee_pass, dim3(blocks_for(Es, 64)), dim3(64), 0, c.stream, Es, s.se_perm.p, s.se.p, s.x.p, s.vrest.p, s.bt.p,
-                           et, sq, s.hhat, (const int*)b.offEE.p, b.cntEE.p, b.bufEE.p, b.candEE.p, b.recEE.p, b.auxEE.p, b.catEE.p);
-        hipLaunchKernelGGL(owner_of, dim3(blocks_for(Es, 256)), dim3(256), 0, c.stream, Es, b.offEE.p, b.ownEE.p);
+        hipLaunchKernelGGL(ee_compact, dim3(blocks_for(Es, 64)), dim3(64), 0, c.stream, Es, s.se_perm.p, s.se.p, s.x.p, et, sq, b.offEE.p,
+                           b.cntpEE.p, b.bufEE.p, b.candEE.p, b.ownEE.p);
+        if (nEE > 0)
+            hipLaunchKernelGGL(ee_classify, dim3(blocks_for(nEE, 128)), dim3(128), 0, c.stream, nEE, Es, b.ownEE.p, s.se.p, s.x.p,
+                               s.vrest.p, s.bt.p, b.candEE.p, s.hhat, b.recEE.p, b.auxEE.p, b.catEE.p);
     }
     CIPC_CHECK_LAUNCH();
-    // category flags and their positions
-    auto flag_scan = [&](int n, DBuf<int>& cat, unsigned mask, DBuf<int>& f, DBuf<int>& p) {
-        f.resize(n + 1); p.resize(n + 1);
-        hipLaunchKernelGGL(flags_by_cat, dim3(blocks_for(n + 1, 256)), dim3(256), 0, c.stream, n, cat.p, mask, f.p);
-        CIPC_CHECK_LAUNCH();
-        exclusive_scan_int(c, f.p, p.p, n + 1);
-    };
-    flag_scan(nPT, b.catPT, 1u << kSelfPT, b.f1, b.p1);
-    flag_scan(nPT, b.catPT, 1u << kMap, b.f2, b.p2);
-    flag_scan(nEE, b.catEE, 1u << kSelfEE, b.f3, b.p3);
-    flag_scan(nEE, b.catEE, 1u << kMap, b.f4, b.p4);
-    flag_scan(nEE, b.catEE, (1u << kMolPE) | (1u << kMolEE), b.f5, b.p5);
+    // category positions
+    const unsigned keptPT = (1u << kNone) | (1u << kSelfPT) | (1u << kMap);
+    const unsigned keptEE = (1u << kNone) | (1u << kSelfEE) | (1u << kMap) | (1u << kMolPE) | (1u << kMolEE);
+    category_scan(c, b.catPT.p, nPT, 1u << kSelfPT, 1u << kMap, keptPT, 0u, b.posPT);
+    category_scan(c, b.catEE.p, nEE, 1u << kSelfEE, 1u << kMap, (1u << kMolPE) | (1u << kMolEE), keptEE, b.posEE);
     b.fg.resize(Vs + 1); b.pg.resize(Vs + 1);
     hipLaunchKernelGGL(ground_flags, dim3(blocks_for(Vs + 1, 256)), dim3(256), 0, c.stream, Vs, s.sv.p, s.x.p, s.gn, s.gd, s.hhat, b.fg.p);
     CIPC_CHECK_LAUNCH();
     exclusive_scan_int(c, b.fg.p, b.pg.p, Vs + 1);
-    int tot[6];
-    const int* srcs[6] = {b.p1.p + nPT, b.p2.p + nPT, b.p3.p + nEE, b.p4.p + nEE, b.p5.p + nEE, b.pg.p + Vs};
-    for (int k = 0; k < 6; ++k) CIPC_HIP(hipMemcpyAsync(&tot[k], srcs[k], sizeof(int), hipMemcpyDeviceToHost, c.stream));
+    int4 tpt, tee;
+    int nG = 0;
+    CIPC_HIP(hipMemcpyAsync(&tpt, b.posPT.p + nPT, sizeof(int4), hipMemcpyDeviceToHost, c.stream));
+    CIPC_HIP(hipMemcpyAsync(&tee, b.posEE.p + nEE, sizeof(int4), hipMemcpyDeviceToHost, c.stream));
+    CIPC_HIP(hipMemcpyAsync(&nG, b.pg.p + Vs, sizeof(int), hipMemcpyDeviceToHost, c.stream));
     CIPC_HIP(hipStreamSynchronize(c.stream));
-    const int nA = tot[0], n2a = tot[1], nB = tot[2], n2b = tot[3], nE = tot[4], nG = tot[5];
-    // PP/PE map: sort packed keys, run-length encode
+    const int nA = tpt.x, n2a = tpt.y, nPTk = tpt.z;
+    const int nB = tee.x, n2b = tee.y, nE = tee.z, nEEk = tee.w;
+    // PP/PE map keys need the vertex count to fit three fields in 64 bits
     int bits = 1;
     while ((1ll << bits) < (long long)s.V + 2) ++bits;
     if (3 * bits > 64) throw Error(CIPC_ERR_OVERFLOW, "vertex count too large for the point-point/point-edge key packing");
     const int nm = n2a + n2b;
-    int nM = 0;
+    // outputs (the map tail of Self_ActiveSet has at most nm entries)
+    c.self_set.resize((size_t)nA + nB + nm + 1); c.ee_set.resize(nE + 1); c.ee_ie.resize(nE + 1);
+    c.ccd_set.resize((size_t)nPTk + nEEk + 1); c.env_set.resize(nG + 1);
     b.mkeys.resize(nm + 1); b.mkeys_sorted.resize(nm + 1); b.ukeys.resize(nm + 1); b.ucounts.resize(nm + 1); b.nruns.resize(1);
+    if (nPT > 0)
+        hipLaunchKernelGGL(scatter_pt, dim3(blocks_for(nPT, 256)), dim3(256), 0, c.stream, nPT, b.catPT.p, b.posPT.p, b.recPT.p, b.ownPT.p,
+                           b.candPT.p, s.V, bits, c.self_set.p, b.mkeys.p, c.ccd_set.p);
+    if (nEE > 0)
+        hipLaunchKernelGGL(scatter_ee, dim3(blocks_for(nEE, 256)), dim3(256), 0, c.stream, nEE, b.catEE.p, b.posEE.p, b.recEE.p, b.auxEE.p,
+                           b.ownEE.p, b.candEE.p, s.V, bits, nA, n2a, nPTk, c.self_set.p, b.mkeys.p, c.ee_set.p, c.ee_ie.p,
+                           c.ccd_set.p);
+    if (Vs > 0)
+        hipLaunchKernelGGL(ground_scatter, dim3(blocks_for(Vs, 256)), dim3(256), 0, c.stream, Vs, s.sv.p, b.fg.p, b.pg.p, c.env_set.p);
+    CIPC_CHECK_LAUNCH();
+    // PP/PE map: sort packed keys, run-length encode (std::map order with multiplicity)
+    int nM = 0;
     if (nm > 0) {
-        if (nPT > 0)
-            hipLaunchKernelGGL(map_keys, dim3(blocks_for(nPT, 256)), dim3(256), 0, c.stream, nPT, b.f2.p, b.p2.p, b.recPT.p, 0, s.V,
-                               bits, b.mkeys.p);
-        if (nEE > 0)
-            hipLaunchKernelGGL(map_keys, dim3(blocks_for(nEE, 256)), dim3(256), 0, c.stream, nEE, b.f4.p, b.p4.p, b.recEE.p, n2a, s.V,
-                               bits, b.mkeys.p);
-        CIPC_CHECK_LAUNCH();
         sort_u64(c, b.mkeys.p, b.mkeys_sorted.p, nm, 3 * bits);
         size_t bytes = 0;
         CIPC_HIP(hipcub::DeviceRunLengthEncode::Encode(nullptr, bytes, b.mkeys_sorted.p, b.ukeys.p, b.ucounts.p, b.nruns.p, nm, c.stream));
         c.tmp.reserve(bytes);
         CIPC_HIP(hipcub::DeviceRunLengthEncode::Encode(c.tmp.p, bytes, b.mkeys_sorted.p, b.ukeys.p, b.ucounts.p, b.nruns.p, nm, c.stream));
         nM = read_int(c, b.nruns.p);
-    }
-    // outputs
-    c.n_self = nA + nB + nM; c.n_ee = nE; c.n_ccd = nPT + nEE; c.n_env = nG;
-    c.self_set.resize(c.n_self + 1); c.ee_set.resize(nE + 1); c.ee_ie.resize(nE + 1);
-    c.ccd_set.resize(c.n_ccd + 1); c.env_set.resize(nG + 1);
-    if (nPT > 0) {
-        hipLaunchKernelGGL(scatter_int4, dim3(blocks_for(nPT, 256)), dim3(256), 0, c.stream, nPT, b.f1.p, b.p1.p, b.recPT.p, 0, c.self_set.p);
-        hipLaunchKernelGGL(ccd_pairs_pt, dim3(blocks_for(nPT, 256)), dim3(256), 0, c.stream, nPT, b.ownPT.p, b.candPT.p, c.ccd_set.p);
-    }
-    if (nEE > 0) {
-        hipLaunchKernelGGL(scatter_int4, dim3(blocks_for(nEE, 256)), dim3(256), 0, c.stream, nEE, b.f3.p, b.p3.p, b.recEE.p, nA, c.self_set.p);
-        hipLaunchKernelGGL(scatter_int4, dim3(blocks_for(nEE, 256)), dim3(256), 0, c.stream, nEE, b.f5.p, b.p5.p, b.recEE.p, 0, c.ee_set.p);
-        hipLaunchKernelGGL(scatter_int2, dim3(blocks_for(nEE, 256)), dim3(256), 0, c.stream, nEE, b.f5.p, b.p5.p, b.auxEE.p, 0, c.ee_ie.p);
-        hipLaunchKernelGGL(ccd_pairs_ee, dim3(blocks_for(nEE, 256)), dim3(256), 0, c.stream, nEE, nPT, b.ownEE.p, b.candEE.p, c.ccd_set.p);
-    }
-    if (nM > 0)
         hipLaunchKernelGGL(map_decode, dim3(blocks_for(nM, 256)), dim3(256), 0, c.stream, nM, b.ukeys.p, b.ucounts.p, s.V, bits, nA + nB,
                            c.self_set.p);
-    if (Vs > 0)
-        hipLaunchKernelGGL(ground_scatter, dim3(blocks_for(Vs, 256)), dim3(256), 0, c.stream, Vs, s.sv.p, b.fg.p, b.pg.p, c.env_set.p);
-    CIPC_CHECK_LAUNCH();
+        CIPC_CHECK_LAUNCH();
+    }
+    c.n_self = nA + nB + nM; c.n_ee = nE; c.n_ccd = nPTk + nEEk; c.n_env = nG;
     CIPC_HIP(hipStreamSynchronize(c.stream));
 }
 
@@ -706,7 +799,7 @@ __global__ void intersect_kernel(int F, const int4* __restrict__ faces, const in
     const d3 t0 = ld3(x, fv.x), t1 = ld3(x, fv.y), t2 = ld3(x, fv.z);
     const Box q = box_tri(t0, t1, t2);
     bool any = false;
-    lbvh_visit(et.n, et.nb, et.left, et.skip, q, 0.0, [&](int e) {
+    lbvh_visit(et, -1, q, 0.0, [&](int e) {
         if (any) return;
         const int2 E = se[e];
         if (E.x == fv.x || E.x == fv.y || E.x == fv.z || E.y == fv.x || E.y == fv.y || E.y == fv.z) return;
@@ -734,7 +827,7 @@ bool any_intersection(Context& c) {
         hipLaunchKernelGGL(ground_touch_kernel, dim3(blocks_for(c.n_env, 256)), dim3(256), 0, c.stream, (int)c.n_env, c.env_set.p,
                            s.x.p, s.gn, s.gd, c.counters.p);
     build_static_trees(c);
-    const TreeView et{c.edge_tree.n, c.edge_tree.node_box.p, c.edge_tree.left.p, c.edge_tree.skip.p};
+    const TreeView et{c.edge_tree.n, c.edge_tree.nodes.p, c.edge_tree.frontier.p};
     if (s.F > 0)
         hipLaunchKernelGGL(intersect_kernel, dim3(blocks_for(s.F, 128)), dim3(128), 0, c.stream, s.F, s.faces.p, s.se.p, s.x.p, s.bt.p,
                            et, c.counters.p + 1);

@@ -71,59 +71,73 @@ __global__ void partial_ccd_kernel(int n, const int2* __restrict__ pairs, const 
     atomic_min_nonneg(slot, a);
 }
 
-// Full CCD, point-triangle phase: one thread per surface vertex (ContactMechanics.cpp:3764-3816).
-__global__ void full_ccd_pt_kernel(int Vs, const int* __restrict__ sv, const int4* __restrict__ faces, const int* __restrict__ bt,
-                                   const double* __restrict__ x, const double* __restrict__ p, const Box* __restrict__ qbox,
-                                   int n, const Box* __restrict__ nb, const int* __restrict__ left, const int* __restrict__ skip,
-                                   double a, double eta, unsigned long long* __restrict__ slot) {
-    const int s = blockIdx.x * blockDim.x + threadIdx.x;
-    if (s >= Vs) return;
-    const int vI = sv[s];
-    const d3 X = ld3(x, vI), P = ld3(p, vI);
-    const Box pb = swept_pt(X, P, a);
+// Full CCD, point-triangle phase (ContactMechanics.cpp:3764-3816): each surface vertex is
+// split over kParts lanes, one subtree of the swept face tree each; the step is the min
+// over every lane, reduced across the wavefront before one atomic.
+__device__ inline double wave_min(double v) {
+#pragma unroll
+    for (int m = 1; m < 64; m <<= 1) v = fmin(v, __shfl_xor(v, m));
+    return v;
+}
+__global__ __launch_bounds__(64) void full_ccd_pt_kernel(int Vs, const int* __restrict__ perm, const int* __restrict__ sv,
+                                                          const int4* __restrict__ faces, const int* __restrict__ bt,
+                                                          const double* __restrict__ x, const double* __restrict__ p,
+                                                          const Box* __restrict__ qbox, TreeView tv, double a, double eta,
+                                                          unsigned long long* __restrict__ slot) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
     double best = 1.0;
-    lbvh_visit(n, nb, left, skip, qbox[s], 0.0, [&](int f) {
-        const int4 fv = faces[f];
-        if (is_collider(bt[vI]) && is_collider(bt[fv.x]) && is_collider(bt[fv.y]) && is_collider(bt[fv.z])) return;
-        if (vI == fv.x || vI == fv.y || vI == fv.z) return;
-        const d3 x0 = ld3(x, fv.x), x1 = ld3(x, fv.y), x2 = ld3(x, fv.z);
-        const d3 p0 = ld3(p, fv.x), p1 = ld3(p, fv.y), p2 = ld3(p, fv.z);
-        const Box fb = box_union(box_union(swept_pt(x0, p0, a), swept_pt(x1, p1, a)), swept_pt(x2, p2, a));
-        if (!overlaps(pb, fb, 0.0)) return;
-        const double la = accd_pt(X, x0, x1, x2, -P, -p0, -p1, -p2, eta, 0.0);
-        best = fmin(la, best);
-    });
-    atomic_min_nonneg(slot, best);
+    if (t < Vs * kParts) {
+        const int s = perm[t / kParts], part = t % kParts;
+        const int vI = sv[s];
+        const d3 X = ld3(x, vI), P = ld3(p, vI);
+        const Box pb = swept_pt(X, P, a);
+        lbvh_visit(tv, part, qbox[s], 0.0, [&](int f) {
+            const int4 fv = faces[f];
+            if (is_collider(bt[vI]) && is_collider(bt[fv.x]) && is_collider(bt[fv.y]) && is_collider(bt[fv.z])) return;
+            if (vI == fv.x || vI == fv.y || vI == fv.z) return;
+            const d3 x0 = ld3(x, fv.x), x1 = ld3(x, fv.y), x2 = ld3(x, fv.z);
+            const d3 p0 = ld3(p, fv.x), p1 = ld3(p, fv.y), p2 = ld3(p, fv.z);
+            const Box fb = box_union(box_union(swept_pt(x0, p0, a), swept_pt(x1, p1, a)), swept_pt(x2, p2, a));
+            if (!overlaps(pb, fb, 0.0)) return;
+            const double la = accd_pt(X, x0, x1, x2, -P, -p0, -p1, -p2, eta, 0.0);
+            best = fmin(la, best);
+        });
+    }
+    best = wave_min(best);
+    if (threadIdx.x == 0) atomic_min_nonneg(slot, best);
 }
 
-// Full CCD, edge-edge phase: one thread per surface edge eJ, candidates eI > eJ from the
-// tree built with the incoming step, exact swept-box filter with the point-triangle-reduced
-// step read from `slot` (ContactMechanics.cpp:3818-3866).
-__global__ void full_ccd_ee_kernel(int Es, const int2* __restrict__ se, const int* __restrict__ bt, const double* __restrict__ x,
-                                   const double* __restrict__ p, int n, const Box* __restrict__ nb, const int* __restrict__ left,
-                                   const int* __restrict__ skip, const Box* __restrict__ ebox, const unsigned long long* __restrict__ aslot,
-                                   double eta, unsigned long long* __restrict__ slot) {
-    const int eJ = blockIdx.x * blockDim.x + threadIdx.x;
-    if (eJ >= Es) return;
-    const double a = bits_to_double(*aslot);
-    const int2 J = se[eJ];
-    const d3 j0 = ld3(x, J.x), j1 = ld3(x, J.y), pj0 = ld3(p, J.x), pj1 = ld3(p, J.y);
-    const Box b1 = box_union(swept_pt(j0, pj0, a), swept_pt(j1, pj1, a));
+// Full CCD, edge-edge phase: each surface edge eJ (split over kParts lanes), candidates
+// eI > eJ from the tree built with the incoming step, exact swept-box filter with the
+// point-triangle-reduced step read from `slot` (ContactMechanics.cpp:3818-3866).
+__global__ __launch_bounds__(64) void full_ccd_ee_kernel(int Es, const int* __restrict__ perm, const int2* __restrict__ se,
+                                                          const int* __restrict__ bt, const double* __restrict__ x,
+                                                          const double* __restrict__ p, TreeView tv, const Box* __restrict__ ebox,
+                                                          const unsigned long long* __restrict__ aslot, double eta,
+                                                          unsigned long long* __restrict__ slot) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
     double best = 1.0;
-    // query box: eJ's leaf box of the tree (swept with the incoming step)
-    const Box q = ebox[eJ];
-    lbvh_visit(n, nb, left, skip, q, 0.0, [&](int eI) {
-        if (eI <= eJ) return;
-        const int2 I = se[eI];
-        if (is_collider(bt[I.x]) && is_collider(bt[I.y]) && is_collider(bt[J.x]) && is_collider(bt[J.y])) return;
-        if (I.x == J.x || I.x == J.y || I.y == J.x || I.y == J.y || eI < eJ) return;
-        const d3 i0 = ld3(x, I.x), i1 = ld3(x, I.y), pi0 = ld3(p, I.x), pi1 = ld3(p, I.y);
-        const Box b2 = box_union(swept_pt(i0, pi0, a), swept_pt(i1, pi1, a));
-        if (!overlaps(b1, b2, 0.0)) return;
-        const double la = accd_ee(i0, i1, j0, j1, -pi0, -pi1, -pj0, -pj1, eta, 0.0);
-        best = fmin(la, best);
-    });
-    atomic_min_nonneg(slot, best);
+    if (t < Es * kParts) {
+        const int eJ = perm[t / kParts], part = t % kParts;
+        const double a = bits_to_double(*aslot);
+        const int2 J = se[eJ];
+        const d3 j0 = ld3(x, J.x), j1 = ld3(x, J.y), pj0 = ld3(p, J.x), pj1 = ld3(p, J.y);
+        const Box b1 = box_union(swept_pt(j0, pj0, a), swept_pt(j1, pj1, a));
+        // query box: eJ's leaf box of the tree (swept with the incoming step)
+        lbvh_visit(tv, part, ebox[eJ], 0.0, [&](int eI) {
+            if (eI <= eJ) return;
+            const int2 I = se[eI];
+            if (is_collider(bt[I.x]) && is_collider(bt[I.y]) && is_collider(bt[J.x]) && is_collider(bt[J.y])) return;
+            if (I.x == J.x || I.x == J.y || I.y == J.x || I.y == J.y || eI < eJ) return;
+            const d3 i0 = ld3(x, I.x), i1 = ld3(x, I.y), pi0 = ld3(p, I.x), pi1 = ld3(p, I.y);
+            const Box b2 = box_union(swept_pt(i0, pi0, a), swept_pt(i1, pi1, a));
+            if (!overlaps(b1, b2, 0.0)) return;
+            const double la = accd_ee(i0, i1, j0, j1, -pi0, -pi1, -pj0, -pj1, eta, 0.0);
+            best = fmin(la, best);
+        });
+    }
+    best = wave_min(best);
+    if (threadIdx.x == 0) atomic_min_nonneg(slot, best);
 }
 __global__ void copy_slot(const unsigned long long* src, unsigned long long* dst) { *dst = *src; }
 
@@ -157,13 +171,14 @@ void step_bound(Context& c, double out[5]) {
     }
     LBVH& ft = c.face_tree;
     LBVH& et = c.edge_tree;
+    const TreeView ftv{ft.n, ft.nodes.p, ft.frontier.p}, etv{et.n, et.nodes.p, et.frontier.p};
     if (s.Vs > 0 && ft.n > 0)
-        hipLaunchKernelGGL(full_ccd_pt_kernel, dim3(blocks_for(s.Vs, 64)), dim3(64), 0, c.stream, s.Vs, s.sv.p, s.faces.p, s.bt.p, s.x.p,
-                           s.dir.p, c.swept_point_box.p, ft.n, ft.node_box.p, ft.left.p, ft.skip.p, alpha, eta, sl.p + 3);
+        hipLaunchKernelGGL(full_ccd_pt_kernel, dim3(blocks_for((size_t)s.Vs * kParts, 64)), dim3(64), 0, c.stream, s.Vs, s.sv_perm.p,
+                           s.sv.p, s.faces.p, s.bt.p, s.x.p, s.dir.p, c.swept_point_box.p, ftv, alpha, eta, sl.p + 3);
     if (s.Es > 0 && et.n > 0) {
         hipLaunchKernelGGL(copy_slot, dim3(1), dim3(1), 0, c.stream, sl.p + 3, sl.p + 4);
-        hipLaunchKernelGGL(full_ccd_ee_kernel, dim3(blocks_for(s.Es, 64)), dim3(64), 0, c.stream, s.Es, s.se.p, s.bt.p, s.x.p, s.dir.p,
-                           et.n, et.node_box.p, et.left.p, et.skip.p, et.prim.p, sl.p + 4, eta, sl.p + 3);
+        hipLaunchKernelGGL(full_ccd_ee_kernel, dim3(blocks_for((size_t)s.Es * kParts, 64)), dim3(64), 0, c.stream, s.Es, s.se_perm.p,
+                           s.se.p, s.bt.p, s.x.p, s.dir.p, etv, et.prim.p, sl.p + 4, eta, sl.p + 3);
     }
     CIPC_CHECK_LAUNCH();
     CIPC_HIP(hipMemcpyAsync(h, sl.p, 4 * sizeof(unsigned long long), hipMemcpyDeviceToHost, c.stream));

@@ -89,6 +89,15 @@ __host__ __device__ inline bool is_collider(int t) { return t >= 2; }
 struct Box {
     d3 lo, hi;
 };
+// Packed LBVH node, one 64-byte record: the box, then the first child (internal node) or
+// the primitive id (leaf) and the node that follows this node's subtree in depth-first
+// order (-1 after the last).  One record is read per traversal step.
+struct alignas(64) Node {
+    d3 lo;
+    int left, skip;
+    d3 hi;
+    int pad[2];
+};
 // Touching counts as overlapping (CPU IPC/LBVH.cpp:157-161); `q` is the querying box.
 __host__ __device__ inline bool overlaps(const Box& q, const Box& b, double gap) {
     return !((b.lo.x - q.hi.x > gap) || (b.lo.y - q.hi.y > gap) || (b.lo.z - q.hi.z > gap) ||

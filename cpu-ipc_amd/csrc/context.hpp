@@ -12,11 +12,9 @@ namespace cipc {
 struct LBVH {
     int n = 0;
     DBuf<Box> prim;            // primitive boxes (input)
-    DBuf<Box> node_box;        // 2n-1
-    DBuf<int> left;            // 2n-1
+    DBuf<Node> nodes;          // 2n-1: internal nodes 0..n-2 (root 0), leaves n-1..2n-2
     DBuf<int> right;           // n-1 (internal nodes)
-    DBuf<int> skip;            // 2n-1
-    DBuf<int> parent;          // 2n-1
+    DBuf<int> frontier;        // 2 x kParts: subtree roots in depth-first order, then where each ends
     DBuf<int> range_first;     // n-1
     DBuf<int> range_last;      // n-1
     DBuf<int> split_right;     // n-1: right child of the internal node whose split is s
@@ -29,11 +27,12 @@ struct LBVH {
 
 // Scratch of the active-set construction (broadphase.hip).
 struct ActiveSetScratch {
-    DBuf<int> cntPT, offPT, cntEE, offEE, bufPT, bufEE;
+    DBuf<int> cntPT, offPT, cntEE, offEE, cntpPT, cntpEE, bufPT, bufEE;
     DBuf<int> candPT, candEE, ownPT, ownEE, catPT, catEE;
     DBuf<int4> recPT, recEE;
     DBuf<int2> auxEE;
-    DBuf<int> f1, p1, f2, p2, f3, p3, f4, p4, f5, p5, fg, pg;
+    DBuf<int> fg, pg;
+    DBuf<int4> posPT, posEE;   // per-hit exclusive counts of four hit categories
     DBuf<unsigned long long> mkeys, mkeys_sorted, ukeys;
     DBuf<int> ucounts, nruns;
 };

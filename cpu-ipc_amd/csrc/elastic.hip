@@ -8,7 +8,8 @@
 // Data flow (deterministic, no atomics):
 //   1. snh_element_kernel: one thread per tet, F from x and Dm^-1 (SoA), projected
 //      12x12 stiffness written as its 78 lower-triangle entries and the 12 gradient
-//      entries to SoA staging (entry e of tet t at e*T + t: every store is coalesced).
+//      entries to per-tet staging records (tet t at t*kStageH and t*kStageG), so that
+//      each 3x3 block a gather reads is one contiguous run of 48 or 72 bytes.
 //   2. block_gather_kernel: one thread per 3x3 block of the static block pattern sums
 //      its contributions in tet order, adds the mass on diagonal blocks, and records
 //      which entries received a nonzero contribution (the reference drops zero entries
@@ -28,7 +29,7 @@ namespace cipc {
 // (row-major entries of K_ab, rows of local vertex a).
 __host__ __device__ constexpr int lb_a(int lb) { return lb < 4 ? lb : (lb < 7 ? (lb - 3) : (lb < 9 ? lb - 5 : 3)); }
 __host__ __device__ constexpr int lb_b(int lb) { return lb < 4 ? lb : (lb < 7 ? 0 : (lb < 9 ? 1 : 2)); }
-constexpr int kStageH = 78;
+constexpr int kStageH = 80;   // 78 used, padded to a multiple of 16 bytes
 constexpr int kStageG = 12;
 
 // ---------------------------------------------------------------------------------
@@ -175,9 +176,7 @@ __global__ __launch_bounds__(256) void snh_element_kernel(int T, const int4* __r
 #pragma unroll
     for (int a = 0; a < 4; ++a) {
         const d3 f = vt * (e.pa * mvec(e.F, B[a]) + e.pb * mvec(e.C, B[a]));
-        stagG[(size_t)(3 * a + 0) * T + t] = f.x;
-        stagG[(size_t)(3 * a + 1) * T + t] = f.y;
-        stagG[(size_t)(3 * a + 2) * T + t] = f.z;
+        st3(stagG + (size_t)t * kStageG, a, f);
     }
     d3 G1[4], GJ[4];
     double beta[4][3];
@@ -196,18 +195,19 @@ __global__ __launch_bounds__(256) void snh_element_kernel(int T, const int4* __r
         snh_block(e, B, G1, GJ, beta, a, b, K);
         if (lb < 4) {
             const int o = 6 * lb;
-            stagH[(size_t)(o + 0) * T + t] = sc * K[0][0];
-            stagH[(size_t)(o + 1) * T + t] = sc * K[1][0];
-            stagH[(size_t)(o + 2) * T + t] = sc * K[1][1];
-            stagH[(size_t)(o + 3) * T + t] = sc * K[2][0];
-            stagH[(size_t)(o + 4) * T + t] = sc * K[2][1];
-            stagH[(size_t)(o + 5) * T + t] = sc * K[2][2];
+            double* h = stagH + (size_t)t * kStageH + o;
+            h[0] = sc * K[0][0];
+            h[1] = sc * K[1][0];
+            h[2] = sc * K[1][1];
+            h[3] = sc * K[2][0];
+            h[4] = sc * K[2][1];
+            h[5] = sc * K[2][2];
         } else {
             const int o = 24 + 9 * (lb - 4);
 #pragma unroll
             for (int i = 0; i < 3; ++i)
 #pragma unroll
-                for (int j = 0; j < 3; ++j) stagH[(size_t)(o + 3 * i + j) * T + t] = sc * K[i][j];
+                for (int j = 0; j < 3; ++j) stagH[(size_t)t * kStageH + o + 3 * i + j] = sc * K[i][j];
         }
     }
 }
@@ -233,7 +233,7 @@ __global__ __launch_bounds__(256) void block_gather_kernel(int nb, int T, const 
             const int map[6] = {0, 3, 4, 6, 7, 8};  // (0,0) (1,0) (1,1) (2,0) (2,1) (2,2)
 #pragma unroll
             for (int i = 0; i < 6; ++i) {
-                const double v = stagH[(size_t)(o + i) * T + t];
+                const double v = stagH[(size_t)t * kStageH + o + i];
                 if (v != 0.0) { acc[map[i]] += v; mask |= 1u << map[i]; }
             }
         } else {
@@ -242,7 +242,7 @@ __global__ __launch_bounds__(256) void block_gather_kernel(int nb, int T, const 
             for (int i = 0; i < 3; ++i)
 #pragma unroll
                 for (int j = 0; j < 3; ++j) {
-                    const double v = stagH[(size_t)(o + 3 * i + j) * T + t];
+                    const double v = stagH[(size_t)t * kStageH + o + 3 * i + j];
                     const int dst = tr ? (3 * j + i) : (3 * i + j);
                     if (v != 0.0) { acc[dst] += v; mask |= 1u << dst; }
                 }
@@ -281,7 +281,7 @@ __global__ __launch_bounds__(256) void vertex_gather_kernel(int V, int T, const 
     for (int q = vptr[v]; q < vptr[v + 1]; ++q) {
         const int s = vsrc[q];
         const int t = s >> 2, a = s & 3;
-        const d3 f = {stagG[(size_t)(3 * a) * T + t], stagG[(size_t)(3 * a + 1) * T + t], stagG[(size_t)(3 * a + 2) * T + t]};
+        const d3 f = ld3(stagG + (size_t)t * kStageG, a);
         g = g + dt2 * f;
     }
     st3(grad, v, g);
