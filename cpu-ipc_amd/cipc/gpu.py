@@ -100,7 +100,7 @@ def load(path: str = LIB_PATH):
         "cipc_gpu_eval_snh": [C.c_int, _dp, C.c_double, C.c_double, _dp, _dp, _dp],
         "cipc_gpu_eval_distance": [C.c_int, C.c_int, _dp, _dp, _dp, _dp],
         "cipc_gpu_eval_make_pd": [C.c_int, C.c_int, _dp],
-        "cipc_gpu_eval_contact_pd": [C.c_int, C.c_int, _dp],
+        "cipc_gpu_eval_contact_pd": [C.c_int, C.c_int, _dp, _dp],
         "cipc_gpu_eval_accd": [C.c_int, C.c_int, _dp, _dp, C.c_double, C.c_double, _dp],
         "cipc_gpu_eval_classify": [C.c_int, C.c_int, _dp, _ip],
     }
@@ -305,11 +305,15 @@ def eval_make_pd(H):
     return H
 
 
-def eval_contact_pd(H):
-    """Projection of translation-invariant contact Hessians (n, 3m, 3m), m vertices."""
+def eval_contact_pd(H, G=None):
+    """Projection of translation-invariant contact Hessians (n, 3m, 3m), m vertices; with the
+    blocks' gradients G (n, 3m) the structured path used for distance-barrier blocks."""
     H = _d(H).copy()
     n, dim = H.shape[0], H.shape[1]
-    _check(load().cipc_gpu_eval_contact_pd(n, dim // 3, _dptr(H)))
+    g = None
+    if G is not None:
+        g = _d(np.asarray(G).reshape(n, dim))
+    _check(load().cipc_gpu_eval_contact_pd(n, dim // 3, _dptr(H), _dptr(g) if g is not None else None))
     return H
 
 

@@ -4,18 +4,18 @@
 // (CPU IPC/LBVH.cpp:163-339), SpatialHash::calculateActivateSet
 // (CollisionBroadPhase.cpp:765-1059), Ground::calculateActivateSet (:1076-1092).
 //
-// Tree build: scene box reduction -> 30-bit Morton codes of box centres normalised to the
-// scene box (key = code << 32 | primitive, unique) -> radix sort -> Karras topology, one
-// thread per internal node -> skip pointers for stackless depth-first traversal ->
-// internal boxes as unions of power-of-two leaf ranges (no inter-work-group
-// synchronisation) -> packed 64-byte node records and a subtree frontier.
+// Tree topology: scene box reduction -> 30-bit Morton codes of box centres normalised to
+// the scene box -> stable radix sort of (code, primitive) -> unique keys code << 32 |
+// primitive -> Karras splits, one thread per internal node -> skip pointers for stackless
+// depth-first traversal.  Refit (every build): leaf boxes in leaf order -> internal boxes
+// as unions of power-of-two leaf ranges (no inter-work-group synchronisation).  Query
+// results do not depend on the topology, so it is rebuilt only every kMaxRefits fits.
 //
 // Active set: each surface vertex (point-triangle) and surface edge (edge-edge) query is
 // split over kParts lanes, one subtree each, that buffer their first candidates; a second
 // pass per query packs them (re-walking only overflowing subtrees) and sorts them by
 // primitive id as the reference's query does; one thread per candidate classifies it.
-// Compaction
-// reproduces the reference's serial order: partial-CCD pairs, then Self_ActiveSet =
+// Compaction reproduces the reference's serial order: partial-CCD pairs, then Self_ActiveSet =
 // point-triangle entries, edge-edge entries, and the point-point/point-edge entries
 // deduplicated in std::map key order with multiplicity; Self_EE_ActiveSet for mollified
 // entries; Environment_ActiveSet in ascending surface order.
@@ -61,7 +61,7 @@ __global__ __launch_bounds__(kRedBlock) void box_reduce_partial(int n, const Box
     if (threadIdx.x == 0) part[blockIdx.x] = sh[0];
 }
 __global__ void morton_keys(int n, const Box* __restrict__ b, const Box* __restrict__ part, int nparts,
-                            unsigned long long* __restrict__ keys) {
+                            unsigned* __restrict__ codes, int* __restrict__ ids) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     Box sc = part[0];
@@ -72,7 +72,16 @@ __global__ void morton_keys(int n, const Box* __restrict__ b, const Box* __restr
     const double x = ext.x > 0.0 ? off.x / ext.x : 0.0;
     const double y = ext.y > 0.0 ? off.y / ext.y : 0.0;
     const double z = ext.z > 0.0 ? off.z / ext.z : 0.0;
-    keys[i] = ((unsigned long long)morton3(x, y, z) << 32) | (unsigned)i;
+    codes[i] = morton3(x, y, z);
+    ids[i] = i;
+}
+// Unique keys (code << 32 | primitive).  The radix sort of (code, primitive) pairs is stable
+// and the primitives enter in ascending order, so these keys come out sorted exactly as a
+// sort of the 64-bit keys would order them.
+__global__ void join_keys(int n, const unsigned* __restrict__ codes, const int* __restrict__ ids, unsigned long long* __restrict__ keys) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    keys[i] = ((unsigned long long)codes[i] << 32) | (unsigned)ids[i];
 }
 __device__ inline int prefix(const unsigned long long* k, int n, int a, int b) {
     if (b < 0 || b >= n) return -1;
@@ -80,8 +89,7 @@ __device__ inline int prefix(const unsigned long long* k, int n, int a, int b) {
 }
 // Karras 2012 topology (LBVH.cpp:62-119,213-228): internal node i covers the key range
 // [first, last] and splits it at the highest differing bit.
-__global__ void karras_kernel(int n, const unsigned long long* __restrict__ k, Node* __restrict__ nodes, int* __restrict__ right,
-                              int* __restrict__ range_first, int* __restrict__ range_last, int* __restrict__ split_right) {
+__global__ void karras_kernel(int n, const unsigned long long* __restrict__ k, Node* __restrict__ nodes, int* __restrict__ range_first, int* __restrict__ range_last, int* __restrict__ split_right) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n - 1) return;
     int first, last;
@@ -115,22 +123,15 @@ __global__ void karras_kernel(int n, const unsigned long long* __restrict__ k, N
     const int l = (split == first) ? split + n - 1 : split;
     const int r = (split + 1 == last) ? split + 1 + n - 1 : split + 1;
     nodes[i].left = l;
-    right[i] = r;
+    nodes[i].right = r;
     range_first[i] = first;
     range_last[i] = last;
     split_right[split] = r;
 }
-__global__ void leaf_init(int n, const unsigned long long* __restrict__ k, const Box* __restrict__ prim, Node* __restrict__ nodes,
-                          Box* __restrict__ seg0) {
+__global__ void leaf_ids(int n, const int* __restrict__ order, Node* __restrict__ nodes) {
     const int s = blockIdx.x * blockDim.x + threadIdx.x;
     if (s >= n) return;
-    const int p = (int)(k[s] & 0xffffffffull);
-    const Box b = prim[p];
-    Node& r = nodes[n - 1 + s];
-    r.lo = b.lo;
-    r.hi = b.hi;
-    r.left = p;
-    seg0[s] = b;
+    nodes[n - 1 + s].left = order[s];
 }
 __global__ void skip_kernel(int n, const int* __restrict__ range_last, const int* __restrict__ split_right, Node* __restrict__ nodes) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -138,29 +139,16 @@ __global__ void skip_kernel(int n, const int* __restrict__ range_last, const int
     const int last = (i < n - 1) ? range_last[i] : i - (n - 1);
     nodes[i].skip = (last >= n - 1) ? -1 : split_right[last];
 }
-// Subtree frontier for split queries: expand the root breadth-wise while the list fits in
-// kParts, keeping depth-first order (a leaf stays as it is).
-__global__ void frontier_kernel(int n, const Node* __restrict__ nodes, const int* __restrict__ right, int* __restrict__ fr) {
-    if (blockIdx.x != 0 || threadIdx.x != 0) return;
-    int cur[kParts], nxt[kParts];
-    int m = n > 0 ? 1 : 0;
-    cur[0] = 0;
-    for (;;) {
-        int internal = 0;
-        for (int i = 0; i < m; ++i) internal += cur[i] < n - 1;
-        if (internal == 0 || m + internal > kParts) break;
-        int k = 0;
-        for (int i = 0; i < m; ++i) {
-            if (cur[i] < n - 1) { nxt[k++] = nodes[cur[i]].left; nxt[k++] = right[cur[i]]; }
-            else nxt[k++] = cur[i];
-        }
-        m = k;
-        for (int i = 0; i < m; ++i) cur[i] = nxt[i];
-    }
-    for (int i = 0; i < kParts; ++i) {
-        fr[i] = i < m ? cur[i] : -1;
-        fr[kParts + i] = i < m ? nodes[cur[i]].skip : -1;
-    }
+// Leaf boxes of the current primitive boxes, in leaf order (node records and level 0).
+__global__ void leaf_boxes(int n, const int* __restrict__ order, const Box* __restrict__ prim, Node* __restrict__ nodes,
+                           Box* __restrict__ seg0) {
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= n) return;
+    const Box b = prim[order[s]];
+    Node& r = nodes[n - 1 + s];
+    r.lo = b.lo;
+    r.hi = b.hi;
+    seg0[s] = b;
 }
 
 // Internal boxes.  Node i of the Karras tree covers the sorted leaves [first, last], so its
@@ -174,10 +162,6 @@ __device__ inline Box empty_box() {
     return Box{{1.7976931348623157e308, 1.7976931348623157e308, 1.7976931348623157e308},
                {-1.7976931348623157e308, -1.7976931348623157e308, -1.7976931348623157e308}};
 }
-struct SegLevels {
-    int count[40];
-    long long offset[40];
-};
 __global__ __launch_bounds__(kSegBlock) void seg_build(const Box* __restrict__ src, int nsrc, int lvl0, SegLevels L, Box* __restrict__ seg) {
     __shared__ Box sh[kSegBlock];
     const int g = blockIdx.x * kSegBlock + threadIdx.x;
@@ -212,15 +196,15 @@ __global__ void internal_boxes(int n, const int* __restrict__ first, const int* 
     nodes[i].hi = acc.hi;
 }
 
-void build_tree(Context& c, LBVH& t, int n) {
+// Topology from the current primitive boxes.
+void build_topology(Context& c, LBVH& t, int n) {
     t.n = n;
-    if (n <= 0) return;
+    t.age = 0;
     const int nodes = 2 * n - 1;
-    t.nodes.resize(nodes); t.frontier.resize(2 * kParts);
-    t.right.resize(std::max(n - 1, 1)); t.range_first.resize(std::max(n - 1, 1)); t.range_last.resize(std::max(n - 1, 1));
-    t.split_right.resize(std::max(n - 1, 1));
-    t.keys.resize(n); t.keys_sorted.resize(n);
-    // level l of the box reduction holds ceil(n / 2^l) boxes; level 0 is the sorted leaves
+    t.nodes.resize(nodes);
+    t.range_first.resize(std::max(n - 1, 1)); t.range_last.resize(std::max(n - 1, 1)); t.split_right.resize(std::max(n - 1, 1));
+    t.codes.resize(n); t.codes_sorted.resize(n); t.order.resize(2 * (size_t)n); t.keys_sorted.resize(n);
+    // level l of the box reduction holds ceil(n / 2^l) boxes; level 0 is the leaves
     SegLevels L{};
     long long total = 0;
     int levels = 0;
@@ -233,6 +217,8 @@ void build_tree(Context& c, LBVH& t, int n) {
         if (cnt == 1) break;
     }
     for (int l = levels + 1; l < 40; ++l) { L.count[l] = 0; L.offset[l] = total; }
+    t.L = L;
+    t.levels = levels;
     t.seg.resize((size_t)total);
     const int parts = std::min<int>(kRedParts, (int)blocks_for(n, kRedBlock));
     t.scene_partial.resize(parts);
@@ -242,31 +228,45 @@ void build_tree(Context& c, LBVH& t, int n) {
         hipLaunchKernelGGL(box_reduce_partial, dim3(1), dim3(kRedBlock), 0, c.stream, parts, t.scene_partial.p, t.scene_partial.p);
         CIPC_CHECK_LAUNCH();
     }
-    hipLaunchKernelGGL(morton_keys, dim3(blocks_for(n, 256)), dim3(256), 0, c.stream, n, t.prim.p, t.scene_partial.p, 1, t.keys.p);
+    hipLaunchKernelGGL(morton_keys, dim3(blocks_for(n, 256)), dim3(256), 0, c.stream, n, t.prim.p, t.scene_partial.p, 1, t.codes.p,
+                       t.order.p + n);
     CIPC_CHECK_LAUNCH();
-    sort_u64(c, t.keys.p, t.keys_sorted.p, n, 64);
+    sort_u32_pairs(c, t.codes.p, t.codes_sorted.p, t.order.p + n, t.order.p, n, 30);
+    hipLaunchKernelGGL(join_keys, dim3(blocks_for(n, 256)), dim3(256), 0, c.stream, n, t.codes_sorted.p, t.order.p, t.keys_sorted.p);
+    CIPC_CHECK_LAUNCH();
     if (n > 1) {
         hipLaunchKernelGGL(karras_kernel, dim3(blocks_for(n - 1, 256)), dim3(256), 0, c.stream, n, t.keys_sorted.p, t.nodes.p,
-                           t.right.p, t.range_first.p, t.range_last.p, t.split_right.p);
+                           t.range_first.p, t.range_last.p, t.split_right.p);
         CIPC_CHECK_LAUNCH();
     }
-    hipLaunchKernelGGL(leaf_init, dim3(blocks_for(n, 256)), dim3(256), 0, c.stream, n, t.keys_sorted.p, t.prim.p, t.nodes.p, t.seg.p);
-    CIPC_CHECK_LAUNCH();
+    hipLaunchKernelGGL(leaf_ids, dim3(blocks_for(n, 256)), dim3(256), 0, c.stream, n, t.order.p, t.nodes.p);
     hipLaunchKernelGGL(skip_kernel, dim3(blocks_for(nodes, 256)), dim3(256), 0, c.stream, n, t.range_last.p, t.split_right.p, t.nodes.p);
     CIPC_CHECK_LAUNCH();
+}
+
+// Boxes of the current primitive boxes on the existing topology.
+void refit(Context& c, LBVH& t) {
+    const int n = t.n;
+    hipLaunchKernelGGL(leaf_boxes, dim3(blocks_for(n, 256)), dim3(256), 0, c.stream, n, t.order.p, t.prim.p, t.nodes.p, t.seg.p);
+    CIPC_CHECK_LAUNCH();
     if (n > 1) {
-        for (int l0 = 0; l0 < levels; l0 += kSegLevels) {
-            const int nsrc = L.count[l0];
-            hipLaunchKernelGGL(seg_build, dim3(blocks_for(nsrc, kSegBlock)), dim3(kSegBlock), 0, c.stream, t.seg.p + L.offset[l0], nsrc,
-                               l0, L, t.seg.p);
+        for (int l0 = 0; l0 < t.levels; l0 += kSegLevels) {
+            const int nsrc = t.L.count[l0];
+            hipLaunchKernelGGL(seg_build, dim3(blocks_for(nsrc, kSegBlock)), dim3(kSegBlock), 0, c.stream, t.seg.p + t.L.offset[l0],
+                               nsrc, l0, t.L, t.seg.p);
             CIPC_CHECK_LAUNCH();
         }
-        hipLaunchKernelGGL(internal_boxes, dim3(blocks_for(n - 1, 256)), dim3(256), 0, c.stream, n, t.range_first.p, t.range_last.p, L,
-                           t.seg.p, t.nodes.p);
+        hipLaunchKernelGGL(internal_boxes, dim3(blocks_for(n - 1, 256)), dim3(256), 0, c.stream, n, t.range_first.p, t.range_last.p,
+                           t.L, t.seg.p, t.nodes.p);
         CIPC_CHECK_LAUNCH();
     }
-    hipLaunchKernelGGL(frontier_kernel, dim3(1), dim3(64), 0, c.stream, n, t.nodes.p, t.right.p, t.frontier.p);
-    CIPC_CHECK_LAUNCH();
+    ++t.age;
+}
+
+void build_tree(Context& c, LBVH& t, int n) {
+    if (n <= 0) { t.n = n; return; }
+    if (t.n != n || t.age >= kMaxRefits) build_topology(c, t, n);
+    refit(c, t);
 }
 
 // Primitive boxes: static (CPUBroadPhaseBVH::build, LBVH.cpp:284-305) or swept with step
@@ -372,7 +372,7 @@ __device__ inline int sum_over_parts(int v) {
 __global__ __launch_bounds__(64) void pt_traverse(int Vs, const int* __restrict__ perm, const int* __restrict__ sv,
                                                    const double* __restrict__ x, TreeView tv, double sq, int* __restrict__ cnt,
                                                    int* __restrict__ cntp, int* __restrict__ buf) {
-    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    const int t = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
     const bool active = t < Vs * kParts;
     const int s = active ? perm[t / kParts] : 0, part = t % kParts;
     int n = 0;
@@ -449,7 +449,7 @@ __global__ void pt_classify(int n, const int* __restrict__ own, const int* __res
 __global__ __launch_bounds__(64) void ee_traverse(int Es, const int* __restrict__ perm, const int2* __restrict__ se,
                                                    const double* __restrict__ x, TreeView tv, double sq, int* __restrict__ cnt,
                                                    int* __restrict__ cntp, int* __restrict__ buf) {
-    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    const int t = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
     const bool active = t < Es * kParts;
     const int eI = active ? perm[t / kParts] : 0, part = t % kParts;
     int n = 0;
@@ -647,8 +647,8 @@ void compute_active_set(Context& c) {
     ActiveSetScratch& b = c.as;
     const double sq = sqrt(s.hhat);
     const int Vs = s.Vs, Es = s.Es;
-    const TreeView ft{c.face_tree.n, c.face_tree.nodes.p, c.face_tree.frontier.p};
-    const TreeView et{c.edge_tree.n, c.edge_tree.nodes.p, c.edge_tree.frontier.p};
+    const TreeView ft{c.face_tree.n, c.face_tree.nodes.p};
+    const TreeView et{c.edge_tree.n, c.edge_tree.nodes.p};
     b.cntPT.resize(Vs + 1); b.offPT.resize(Vs + 1); b.cntEE.resize(Es + 1); b.offEE.resize(Es + 1);
     b.cntpPT.resize((size_t)kParts * std::max(Vs, 1)); b.cntpEE.resize((size_t)kParts * std::max(Es, 1));
     b.bufPT.resize((size_t)kParts * kPartCap * std::max(Vs, 1)); b.bufEE.resize((size_t)kParts * kPartCap * std::max(Es, 1));
@@ -827,7 +827,7 @@ bool any_intersection(Context& c) {
         hipLaunchKernelGGL(ground_touch_kernel, dim3(blocks_for(c.n_env, 256)), dim3(256), 0, c.stream, (int)c.n_env, c.env_set.p,
                            s.x.p, s.gn, s.gd, c.counters.p);
     build_static_trees(c);
-    const TreeView et{c.edge_tree.n, c.edge_tree.nodes.p, c.edge_tree.frontier.p};
+    const TreeView et{c.edge_tree.n, c.edge_tree.nodes.p};
     if (s.F > 0)
         hipLaunchKernelGGL(intersect_kernel, dim3(blocks_for(s.F, 128)), dim3(128), 0, c.stream, s.F, s.faces.p, s.se.p, s.x.p, s.bt.p,
                            et, c.counters.p + 1);

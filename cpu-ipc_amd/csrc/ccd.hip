@@ -84,7 +84,7 @@ __global__ __launch_bounds__(64) void full_ccd_pt_kernel(int Vs, const int* __re
                                                           const double* __restrict__ x, const double* __restrict__ p,
                                                           const Box* __restrict__ qbox, TreeView tv, double a, double eta,
                                                           unsigned long long* __restrict__ slot) {
-    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    const int t = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
     double best = 1.0;
     if (t < Vs * kParts) {
         const int s = perm[t / kParts], part = t % kParts;
@@ -115,7 +115,7 @@ __global__ __launch_bounds__(64) void full_ccd_ee_kernel(int Es, const int* __re
                                                           const double* __restrict__ p, TreeView tv, const Box* __restrict__ ebox,
                                                           const unsigned long long* __restrict__ aslot, double eta,
                                                           unsigned long long* __restrict__ slot) {
-    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    const int t = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
     double best = 1.0;
     if (t < Es * kParts) {
         const int eJ = perm[t / kParts], part = t % kParts;
@@ -171,7 +171,7 @@ void step_bound(Context& c, double out[5]) {
     }
     LBVH& ft = c.face_tree;
     LBVH& et = c.edge_tree;
-    const TreeView ftv{ft.n, ft.nodes.p, ft.frontier.p}, etv{et.n, et.nodes.p, et.frontier.p};
+    const TreeView ftv{ft.n, ft.nodes.p}, etv{et.n, et.nodes.p};
     if (s.Vs > 0 && ft.n > 0)
         hipLaunchKernelGGL(full_ccd_pt_kernel, dim3(blocks_for((size_t)s.Vs * kParts, 64)), dim3(64), 0, c.stream, s.Vs, s.sv_perm.p,
                            s.sv.p, s.faces.p, s.bt.p, s.x.p, s.dir.p, c.swept_point_box.p, ftv, alpha, eta, sl.p + 3);

@@ -89,14 +89,15 @@ __host__ __device__ inline bool is_collider(int t) { return t >= 2; }
 struct Box {
     d3 lo, hi;
 };
-// Packed LBVH node, one 64-byte record: the box, then the first child (internal node) or
-// the primitive id (leaf) and the node that follows this node's subtree in depth-first
-// order (-1 after the last).  One record is read per traversal step.
+// Packed LBVH node, one 64-byte record: the box, the first child (internal node) or the
+// primitive id (leaf), the node that follows this node's subtree in depth-first order (-1
+// after the last) and the second child (internal node).  One record is read per traversal
+// step.
 struct alignas(64) Node {
     d3 lo;
     int left, skip;
     d3 hi;
-    int pad[2];
+    int right, pad;
 };
 // Touching counts as overlapping (CPU IPC/LBVH.cpp:157-161); `q` is the querying box.
 __host__ __device__ inline bool overlaps(const Box& q, const Box& b, double gap) {
@@ -109,6 +110,7 @@ __host__ __device__ inline bool overlaps(const Box& q, const Box& b, double gap)
 // ------------------------------------------------------------------------------------
 struct Scene {
     int V = 0, T = 0, F = 0, Vs = 0, Es = 0;
+    int kb = 1;                // block-record key: column << kb | row, 2^kb > V
     double dt = 0, u = 0, r = 0, drag = 0, hhat = 0, fhat = 0, fric = 0;
     d3 gn{0, 1, 0};
     double gd = -1;

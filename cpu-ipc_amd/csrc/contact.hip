@@ -7,7 +7,7 @@
 //
 // Each contact becomes a local Hessian over its 2-4 vertices (kernel 1), is projected to
 // PSD by register-resident Jacobi (kernel 2), and is scattered as lower-triangle 3x3
-// block records keyed (column vertex << 32 | row vertex) plus per-vertex gradient
+// block records keyed (column vertex << kb | row vertex, 2^kb > V) plus per-vertex gradient
 // records (kernel 3).  The records are sorted and summed by key in sparse.hip, so the
 // result does not depend on thread scheduling.
 #include "context.hpp"
@@ -159,22 +159,32 @@ __global__ __launch_bounds__(64) void contact_local_kernel(
     }
 }
 
-// Kernel 2: makePD of each local Hessian, in place, one launch per vertex count.  The
-// projection runs on the translation-free subspace (project_contact), with the reduced
-// triangle and its eigenvectors in registers; the packed index of (i, j) is the one
-// kernel 1 writes.
-template <int M>
-__global__ __launch_bounds__(64) void contact_pd_kernel(int n, const int* __restrict__ dims, double* __restrict__ Hs) {
-    const int c = blockIdx.x * blockDim.x + threadIdx.x;
-    if (c >= n || dims[c] != 3 * M) return;
+// Kernel 2: makePD of each local Hessian, in place, one launch per vertex count and kind,
+// on the translation-free subspace.  Self_ActiveSet blocks (point-triangle, edge-edge,
+// point-edge distances) use the structured range basis (project_contact_structured); the
+// mollified edge-edge blocks and point-point blocks the general Jacobi (project_contact).
+// The packed index of (i, j) is the one kernel 1 writes.
+template <int M, bool Structured>
+__global__ __launch_bounds__(64) void contact_pd_kernel(int c0, int c1, int n, const int* __restrict__ dims, double* __restrict__ Hs,
+                                                        const double* __restrict__ Gs) {
+    const int c = c0 + (int)(blockIdx.x * blockDim.x + threadIdx.x);
+    if (c >= c1 || dims[c] != 3 * M) return;
     double* h = Hs + c;
-    project_contact<M>([&](int k) -> double& { return h[(size_t)k * n]; });
+    auto acc = [&](int k) -> double& { return h[(size_t)k * n]; };
+    if constexpr (Structured) {
+        double G[3 * M];
+#pragma unroll
+        for (int k = 0; k < 3 * M; ++k) G[k] = Gs[(size_t)k * n + c];
+        project_contact_structured<M>(acc, G);
+    } else {
+        project_contact<M>(acc);
+    }
 }
 
 // Kernel 3: block and gradient records.  Slots: 10 blocks and 4 gradients per contact.
 __global__ void contact_scatter_kernel(int n, const int4* __restrict__ verts, const int* __restrict__ dims,
                                        const double* __restrict__ Hs, const double* __restrict__ Gs,
-                                       const int* __restrict__ bt, int slot0_b, int slot0_g,
+                                       const int* __restrict__ bt, int slot0_b, int slot0_g, int kb,
                                        unsigned long long* __restrict__ bkey, int* __restrict__ bidx, double* __restrict__ bval,
                                        int* __restrict__ gkey, int* __restrict__ gidx, double* __restrict__ gval) {
     const int c = blockIdx.x * blockDim.x + threadIdx.x;
@@ -193,7 +203,7 @@ __global__ void contact_scatter_kernel(int n, const int4* __restrict__ verts, co
             // global lower block: rows of the larger vertex id
             const int ra = v[a] >= v[b] ? a : b, cb = v[a] >= v[b] ? b : a;
             const int R = v[ra], Cc = v[cb];
-            bkey[s] = ((unsigned long long)(unsigned)Cc << 32) | (unsigned)R;
+            bkey[s] = ((unsigned long long)(unsigned)Cc << kb) | (unsigned)R;
             for (int i = 0; i < 3; ++i)
                 for (int j = 0; j < 3; ++j) {
                     const int gi = 3 * ra + i, gj = 3 * cb + j;
@@ -212,7 +222,7 @@ __global__ void contact_scatter_kernel(int n, const int4* __restrict__ verts, co
 
 // Ground (IPCSolver.cpp:270-293): one block and one gradient record per active vertex.
 __global__ void ground_kernel(int n_env, const int* __restrict__ env, const double* __restrict__ x, const int* __restrict__ bt,
-                              d3 gn, double gd, double hhat, double kappa,
+                              d3 gn, double gd, double hhat, double kappa, int kb,
                               unsigned long long* __restrict__ bkey, int* __restrict__ bidx, double* __restrict__ bval,
                               int* __restrict__ gkey, int* __restrict__ gidx, double* __restrict__ gval) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -227,7 +237,7 @@ __global__ void ground_kernel(int n_env, const int* __restrict__ env, const doub
     const double param = 4.0 * H_b * d + 2.0 * g_b;
     bidx[i] = i;
     if (param > 0 && is_free(bt[v])) {
-        bkey[i] = ((unsigned long long)(unsigned)v << 32) | (unsigned)v;
+        bkey[i] = ((unsigned long long)(unsigned)v << kb) | (unsigned)v;
         const double nn[3] = {gn.x, gn.y, gn.z};
         for (int a = 0; a < 3; ++a)
             for (int b = 0; b < 3; ++b) bval[(size_t)9 * i + 3 * a + b] = (kappa * param) * (nn[a] * nn[b]);
@@ -247,7 +257,7 @@ void contact_assemble(Context& c, double kappa) {
     c.gval.resize(3 * std::max<size_t>(ng, 1));
     if (ne > 0) {
         hipLaunchKernelGGL(ground_kernel, dim3(blocks_for(ne, 256)), dim3(256), 0, c.stream, ne, c.env_set.p, s.x.p, s.bt.p,
-                           s.gn, s.gd, s.hhat, kappa, c.ckey.p, c.cidx.p, c.cval.p, c.gkey.p, c.gidx.p, c.gval.p);
+                           s.gn, s.gd, s.hhat, kappa, s.kb, c.ckey.p, c.cidx.p, c.cval.p, c.gkey.p, c.gidx.p, c.gval.p);
         CIPC_CHECK_LAUNCH();
     }
     if (n > 0) {
@@ -260,12 +270,18 @@ void contact_assemble(Context& c, double kappa) {
                            c.self_set.p, c.ee_set.p, c.ee_ie.p, s.se.p, s.x.p, s.vrest.p, s.hhat, kappa, Hs, Gs,
                            c.cverts.p, c.cdims.p);
         CIPC_CHECK_LAUNCH();
-        hipLaunchKernelGGL(contact_pd_kernel<4>, dim3(blocks_for(n, 64)), dim3(64), 0, c.stream, n, c.cdims.p, Hs);
-        hipLaunchKernelGGL(contact_pd_kernel<3>, dim3(blocks_for(n, 64)), dim3(64), 0, c.stream, n, c.cdims.p, Hs);
-        hipLaunchKernelGGL(contact_pd_kernel<2>, dim3(blocks_for(n, 64)), dim3(64), 0, c.stream, n, c.cdims.p, Hs);
+        const int ns = (int)c.n_self;
+        if (ns > 0) {
+            hipLaunchKernelGGL((contact_pd_kernel<4, true>), dim3(blocks_for(ns, 64)), dim3(64), 0, c.stream, 0, ns, n, c.cdims.p, Hs, Gs);
+            hipLaunchKernelGGL((contact_pd_kernel<3, true>), dim3(blocks_for(ns, 64)), dim3(64), 0, c.stream, 0, ns, n, c.cdims.p, Hs, Gs);
+            hipLaunchKernelGGL((contact_pd_kernel<2, false>), dim3(blocks_for(ns, 64)), dim3(64), 0, c.stream, 0, ns, n, c.cdims.p, Hs, Gs);
+        }
+        if (n > ns)
+            hipLaunchKernelGGL((contact_pd_kernel<4, false>), dim3(blocks_for(n - ns, 64)), dim3(64), 0, c.stream, ns, n, n, c.cdims.p, Hs,
+                               Gs);
         CIPC_CHECK_LAUNCH();
         hipLaunchKernelGGL(contact_scatter_kernel, dim3(blocks_for(n, 128)), dim3(128), 0, c.stream, n, c.cverts.p,
-                           c.cdims.p, Hs, Gs, s.bt.p, ne, ne, c.ckey.p, c.cidx.p, c.cval.p, c.gkey.p, c.gidx.p, c.gval.p);
+                           c.cdims.p, Hs, Gs, s.bt.p, ne, ne, s.kb, c.ckey.p, c.cidx.p, c.cval.p, c.gkey.p, c.gidx.p, c.gval.p);
         CIPC_CHECK_LAUNCH();
     }
     c.ckey.n = nb;
@@ -306,12 +322,25 @@ __global__ __launch_bounds__(64) void make_pd_eval_kernel(int n, double* __restr
         for (int j = i; j < N; ++j) { h[i * N + j] = m[64 * Sym<N>::ix(i, j)]; h[j * N + i] = m[64 * Sym<N>::ix(i, j)]; }
 }
 template <int M>
-__global__ __launch_bounds__(64) void contact_pd_eval_kernel(int n, double* __restrict__ H) {
+__global__ __launch_bounds__(64) void contact_pd_eval_kernel(int n, double* __restrict__ H, const double* __restrict__ G) {
     constexpr int N = 3 * M;
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= n) return;
     double* h = H + (size_t)N * N * t;
-    if (!project_contact<M>([&](int k) -> double& { return h[Sym<N>::row(k) * N + Sym<N>::col(k)]; })) return;
+    auto acc = [&](int k) -> double& { return h[Sym<N>::row(k) * N + Sym<N>::col(k)]; };
+    bool changed;
+    if constexpr (M >= 3) {
+        if (G != nullptr) {
+            double g[N];
+            for (int k = 0; k < N; ++k) g[k] = G[(size_t)N * t + k];
+            changed = project_contact_structured<M>(acc, g);
+        } else {
+            changed = project_contact<M>(acc);
+        }
+    } else {
+        changed = project_contact<M>(acc);
+    }
+    if (!changed) return;
     for (int i = 0; i < N; ++i)
         for (int j = i + 1; j < N; ++j) h[j * N + i] = h[i * N + j];
 }
@@ -363,13 +392,15 @@ extern "C" int cipc_gpu_eval_make_pd_impl(int n, int dim, double* H) {
     dh.down(H, (size_t)dim * dim * n);
     return 0;
 }
-extern "C" int cipc_gpu_eval_contact_pd_impl(int n, int nverts, double* H) {
+extern "C" int cipc_gpu_eval_contact_pd_impl(int n, int nverts, double* H, const double* G) {
     const size_t dim = 3 * (size_t)nverts;
-    DevArr<double> dh(dim * dim * n);
+    DevArr<double> dh(dim * dim * n), dg(G ? dim * n : 1);
     dh.up(H, dim * dim * n);
-    if (nverts == 2) hipLaunchKernelGGL(contact_pd_eval_kernel<2>, dim3(blocks_for(n, 64)), dim3(64), 0, 0, n, dh.p);
-    else if (nverts == 3) hipLaunchKernelGGL(contact_pd_eval_kernel<3>, dim3(blocks_for(n, 64)), dim3(64), 0, 0, n, dh.p);
-    else hipLaunchKernelGGL(contact_pd_eval_kernel<4>, dim3(blocks_for(n, 64)), dim3(64), 0, 0, n, dh.p);
+    if (G) dg.up(G, dim * n);
+    const double* gp = G ? dg.p : nullptr;
+    if (nverts == 2) hipLaunchKernelGGL(contact_pd_eval_kernel<2>, dim3(blocks_for(n, 64)), dim3(64), 0, 0, n, dh.p, gp);
+    else if (nverts == 3) hipLaunchKernelGGL(contact_pd_eval_kernel<3>, dim3(blocks_for(n, 64)), dim3(64), 0, 0, n, dh.p, gp);
+    else hipLaunchKernelGGL(contact_pd_eval_kernel<4>, dim3(blocks_for(n, 64)), dim3(64), 0, 0, n, dh.p, gp);
     CIPC_CHECK_LAUNCH();
     dh.down(H, dim * dim * n);
     return 0;

@@ -331,33 +331,60 @@ __device__ inline void jacobi_rounds(double* __restrict__ R, double* __restrict_
     }
 }
 
-// In place on the packed upper triangle (Sym<3M>) reached through h(k) -> double&.
-// Returns false (and leaves h untouched) when no eigenvalue is negative.
+// R = B H B^T on the packed upper triangles (H: Sym<3M> through h(k) -> double&).
+template <int M, typename HA>
+__device__ inline void reduce_hessian(HA h, double* __restrict__ R) {
+    constexpr int N = 3 * M, NR = 3 * (M - 1);
+    double H[N * (N + 1) / 2];
+#pragma unroll
+    for (int k = 0; k < N * (N + 1) / 2; ++k) H[k] = h(k);
+#pragma unroll
+    for (int i = 0; i < M - 1; ++i)
+#pragma unroll
+        for (int k = 0; k < 3; ++k)
+#pragma unroll
+            for (int j = i; j < M - 1; ++j)
+#pragma unroll
+                for (int l = (j == i ? k : 0); l < 3; ++l) {
+                    double acc = 0.0;
+#pragma unroll
+                    for (int a = 0; a <= i + 1; ++a)
+#pragma unroll
+                        for (int b = 0; b <= j + 1; ++b)
+                            acc += helmert(M, i, a) * helmert(M, j, b) * H[Sym<N>::ix(3 * a + k, 3 * b + l)];
+                    R[Sym<NR>::ix(3 * i + k, 3 * j + l)] = acc;
+                }
+}
+// H -= B^T Nr B  (Nr: Sym<3(M-1)>): H_(a,k),(b,l) -= sum_{i,j} Hm[i][a] Hm[j][b] Nr_(i,k),(j,l)
+template <int M, typename HA>
+__device__ inline void lift_subtract(HA h, const double* __restrict__ Nr) {
+    constexpr int N = 3 * M, NR = 3 * (M - 1);
+#pragma unroll
+    for (int a = 0; a < M; ++a)
+#pragma unroll
+        for (int k = 0; k < 3; ++k)
+#pragma unroll
+            for (int b = a; b < M; ++b)
+#pragma unroll
+                for (int l = (b == a ? k : 0); l < 3; ++l) {
+                    double acc = 0.0;
+#pragma unroll
+                    for (int i = (a > 0 ? a - 1 : 0); i < M - 1; ++i)
+#pragma unroll
+                        for (int j = (b > 0 ? b - 1 : 0); j < M - 1; ++j)
+                            acc += helmert(M, i, a) * helmert(M, j, b) * Nr[Sym<NR>::ix(3 * i + k, 3 * j + l)];
+                    h(Sym<N>::ix(3 * a + k, 3 * b + l)) -= acc;
+                }
+}
+
+// General contact block: eigenpairs of R by Jacobi.  In place on the packed upper triangle
+// (Sym<3M>) reached through h(k) -> double&; returns false (h untouched) when no eigenvalue
+// is negative.
 template <int M, typename HA>
 __device__ inline bool project_contact(HA h) {
-    constexpr int N = 3 * M, NR = 3 * (M - 1), NRU = NR * (NR + 1) / 2;
+    constexpr int NR = 3 * (M - 1), NRU = NR * (NR + 1) / 2;
     double R[NRU];
-    {
-        double H[N * (N + 1) / 2];
-#pragma unroll
-        for (int k = 0; k < N * (N + 1) / 2; ++k) H[k] = h(k);
-#pragma unroll
-        for (int i = 0; i < M - 1; ++i)
-#pragma unroll
-            for (int k = 0; k < 3; ++k)
-#pragma unroll
-                for (int j = i; j < M - 1; ++j)
-#pragma unroll
-                    for (int l = (j == i ? k : 0); l < 3; ++l) {
-                        double acc = 0.0;
-#pragma unroll
-                        for (int a = 0; a <= i + 1; ++a)
-#pragma unroll
-                            for (int b = 0; b <= j + 1; ++b)
-                                acc += helmert(M, i, a) * helmert(M, j, b) * H[Sym<N>::ix(3 * a + k, 3 * b + l)];
-                        R[Sym<NR>::ix(3 * i + k, 3 * j + l)] = acc;
-                    }
-    }
+    reduce_hessian<M>(h, R);
     double Z[NR * NR];
     jacobi_rounds<NR>(R, Z);
     double w[NR];
@@ -375,23 +402,158 @@ __device__ inline bool project_contact(HA h) {
             for (int k = 0; k < NR; ++k) acc += (w[k] * Z[p * NR + k]) * Z[q * NR + k];
             R[Sym<NR>::ix(p, q)] = acc;
         }
-    // lift: H_(a,k),(b,l) -= sum_{i,j} Hm[i][a] Hm[j][b] Nr_(i,k),(j,l)
+    lift_subtract<M>(h, R);
+    return true;
+}
+
+// ---- Structured contact blocks: a smaller eigenproblem on the block's range.
+// For a point-plane (point-triangle) or line-line (edge-edge) squared distance d = s^2 the
+// gradient is grad s = c (x) n (c: per-vertex weights summing to zero, n: unit normal), and
+// the Hessian of s maps into { w (x) n : sum w = 0 } + { c (x) t : t perpendicular to n }
+// (the weights and the normal are all that move to first order).  The barrier block
+// kappa (b'' g g^T + b' H_d), g = 2 s grad s, H_d = 2 grad s grad s^T + 2 s Hess s, therefore
+// has its range in that 5-dimensional space (3 + 2), with the orthonormal basis
+// hm_i (x) n (i = 0..2, the Helmert rows) and c (x) t1, c (x) t2 (c normalised).
+// For a point-line (point-edge) squared distance d = |r|^2, grad d = 2 c (x) r and the range
+// is { c (x) t : t in R^3 } + { (0,1,-1) (x) r }: 4-dimensional.  c and n (or r) are the
+// rank-one factors of the block's own gradient G = coef b' grad d (a 3M vector, M rows of
+// 3).  The eigenpairs of H are those of the K x K matrix T = Q^T R Q lifted by Q, plus zeros,
+// up to the rounding of H (relative 1e-14 on random blocks).  The gradient vanishes only at
+// d = dHat, where b' = b'' = 0 and the block is exactly zero, or when the block is not
+// finite; the block is then left as it is.
+template <int M>
+struct RangeDim { static constexpr int K = (M == 4) ? 5 : 4; };
+
+// The basis is built in a frame V = (u, t1, t2) with u = n (or r): column j of the basis is
+// om_j (x) V e_{dir_j}, om_j a unit vector of per-vertex weights orthogonal to the ones vector.
+// T_ij = sum_{a,b} om_i[a] om_j[b] (V^T H_ab V)[dir_i][dir_j], so each 3x3 block is rotated
+// once into the frame and the vertex weights are contracted afterwards.
+template <int M, typename HA>
+__device__ inline bool project_contact_structured(HA h, const double* __restrict__ G) {
+    static_assert(M == 3 || M == 4, "structured blocks have 3 or 4 vertices");
+    constexpr int N = 3 * M, K = RangeDim<M>::K, NB = M * (M + 1) / 2;
+    // rank-one factors of G (M rows of 3): direction of the largest row, then the weights
+    double best = -1.0, ux = 0.0, uy = 0.0, uz = 0.0;
+#pragma unroll
+    for (int a = 0; a < M; ++a) {
+        const double r2 = G[3 * a] * G[3 * a] + G[3 * a + 1] * G[3 * a + 1] + G[3 * a + 2] * G[3 * a + 2];
+        if (r2 > best) { best = r2; ux = G[3 * a]; uy = G[3 * a + 1]; uz = G[3 * a + 2]; }
+    }
+    if (!(best > 0.0) || !(best < 1.7976931348623157e308)) return false;
+    const double iu = 1.0 / sqrt(best);
+    ux *= iu; uy *= iu; uz *= iu;
+    double c[M], cn = 0.0;
+#pragma unroll
+    for (int a = 0; a < M; ++a) { c[a] = G[3 * a] * ux + G[3 * a + 1] * uy + G[3 * a + 2] * uz; cn += c[a] * c[a]; }
+    const double ic = 1.0 / sqrt(cn);
+#pragma unroll
+    for (int a = 0; a < M; ++a) c[a] *= ic;
+    // frame (Duff et al. 2017, "Building an orthonormal basis, revisited")
+    double V[3][3];
+    {
+        const double sg = uz >= 0.0 ? 1.0 : -1.0;
+        const double aa = -1.0 / (sg + uz), bb = ux * uy * aa;
+        V[0][0] = ux; V[1][0] = uy; V[2][0] = uz;
+        V[0][1] = 1.0 + sg * ux * ux * aa; V[1][1] = sg * bb; V[2][1] = -sg * ux;
+        V[0][2] = bb; V[1][2] = sg + uy * uy * aa; V[2][2] = -uy;
+    }
+    // basis columns: weights om[j] and frame axis dir[j]
+    double om[K][M];
+    int dir[K];
+    if constexpr (M == 4) {
+#pragma unroll
+        for (int j = 0; j < 3; ++j) {
+#pragma unroll
+            for (int a = 0; a < M; ++a) om[j][a] = helmert(M, j, a);
+            dir[j] = 0;
+        }
+#pragma unroll
+        for (int a = 0; a < M; ++a) { om[3][a] = c[a]; om[4][a] = c[a]; }
+        dir[3] = 1; dir[4] = 2;
+    } else {
+#pragma unroll
+        for (int j = 0; j < 3; ++j) {
+#pragma unroll
+            for (int a = 0; a < M; ++a) om[j][a] = c[a];
+            dir[j] = j;
+        }
+        // (0, 1, -1)/sqrt2 made orthogonal to c
+        double dv[3] = {0.0, 0.70710678118654752440, -0.70710678118654752440};
+        const double dc = dv[0] * c[0] + dv[1] * c[1] + dv[2] * c[2];
+        double dn = 0.0;
+#pragma unroll
+        for (int a = 0; a < 3; ++a) { dv[a] -= dc * c[a]; dn += dv[a] * dv[a]; }
+        const double idn = 1.0 / sqrt(dn);
+#pragma unroll
+        for (int a = 0; a < 3; ++a) om[3][a] = dv[a] * idn;
+        dir[3] = 0;
+    }
+    // blocks in the frame, a <= b
+    double P[NB][3][3];
 #pragma unroll
     for (int a = 0; a < M; ++a)
 #pragma unroll
-        for (int k = 0; k < 3; ++k)
+        for (int b = a; b < M; ++b) {
+            double Hb[3][3], tmp[3][3];
 #pragma unroll
-            for (int b = a; b < M; ++b)
+            for (int k = 0; k < 3; ++k)
 #pragma unroll
-                for (int l = (b == a ? k : 0); l < 3; ++l) {
-                    double acc = 0.0;
+                for (int l = 0; l < 3; ++l) Hb[k][l] = h(Sym<N>::ix(3 * a + k, 3 * b + l));
 #pragma unroll
-                    for (int i = (a > 0 ? a - 1 : 0); i < M - 1; ++i)
+            for (int k = 0; k < 3; ++k)
 #pragma unroll
-                        for (int j = (b > 0 ? b - 1 : 0); j < M - 1; ++j)
-                            acc += helmert(M, i, a) * helmert(M, j, b) * R[Sym<NR>::ix(3 * i + k, 3 * j + l)];
-                    h(Sym<N>::ix(3 * a + k, 3 * b + l)) -= acc;
+                for (int y = 0; y < 3; ++y) tmp[k][y] = Hb[k][0] * V[0][y] + Hb[k][1] * V[1][y] + Hb[k][2] * V[2][y];
+#pragma unroll
+            for (int x = 0; x < 3; ++x)
+#pragma unroll
+                for (int y = 0; y < 3; ++y) P[Sym<M>::ix(a, b)][x][y] = V[0][x] * tmp[0][y] + V[1][x] * tmp[1][y] + V[2][x] * tmp[2][y];
+        }
+    double T[K * (K + 1) / 2];
+#pragma unroll
+    for (int i = 0; i < K; ++i)
+#pragma unroll
+        for (int j = i; j < K; ++j) {
+            double acc = 0.0;
+#pragma unroll
+            for (int a = 0; a < M; ++a)
+#pragma unroll
+                for (int b = 0; b < M; ++b) {
+                    const double sab = a <= b ? P[Sym<M>::ix(a, b)][dir[i]][dir[j]] : P[Sym<M>::ix(b, a)][dir[j]][dir[i]];
+                    acc += (om[i][a] * om[j][b]) * sab;
                 }
+            T[Sym<K>::ix(i, j)] = acc;
+        }
+    // compiler barrier: the block is re-read for the update below rather than held in
+    // registers through the eigensolve
+    __asm__ volatile("" ::: "memory");
+    double Y[K * K];
+    jacobi_rounds<K>(T, Y);
+    double w[K];
+    bool neg = false;
+#pragma unroll
+    for (int k = 0; k < K; ++k) { w[k] = fmin(T[Sym<K>::ix(k, k)], 0.0); neg |= w[k] < 0.0; }
+    if (!neg) return false;
+    // eigenvectors in full coordinates: u_k(a, :) = V (sum_j Y[j][k] om_j[a] e_{dir_j})
+    double U[K][N];
+#pragma unroll
+    for (int k = 0; k < K; ++k)
+#pragma unroll
+        for (int a = 0; a < M; ++a) {
+            double f[3] = {0.0, 0.0, 0.0};
+#pragma unroll
+            for (int j = 0; j < K; ++j) f[dir[j]] += Y[j * K + k] * om[j][a];
+#pragma unroll
+            for (int r = 0; r < 3; ++r) U[k][3 * a + r] = V[r][0] * f[0] + V[r][1] * f[1] + V[r][2] * f[2];
+        }
+#pragma unroll
+    for (int r = 0; r < N; ++r)
+#pragma unroll
+        for (int s2 = r; s2 < N; ++s2) {
+            double acc = 0.0;
+#pragma unroll
+            for (int k = 0; k < K; ++k) acc += (w[k] * U[k][r]) * U[k][s2];
+            h(Sym<N>::ix(r, s2)) -= acc;
+        }
     return true;
 }
 

@@ -46,7 +46,7 @@ struct cipc_gpu_context : cipc::Context {};
 extern "C" int cipc_gpu_eval_snh_impl(int n, const double* F, double u, double r, double* P, double* H, double* e);
 extern "C" int cipc_gpu_eval_distance_impl(int n, int kind, const double* pts, double* d, double* g, double* H);
 extern "C" int cipc_gpu_eval_make_pd_impl(int n, int dim, double* H);
-extern "C" int cipc_gpu_eval_contact_pd_impl(int n, int nverts, double* H);
+extern "C" int cipc_gpu_eval_contact_pd_impl(int n, int nverts, double* H, const double* G);
 extern "C" int cipc_gpu_eval_accd_impl(int n, int kind, const double* pts, const double* dirs, double eta, double th, double* toc);
 extern "C" int cipc_gpu_eval_classify_impl(int n, int kind, const double* pts, int* type);
 
@@ -108,6 +108,8 @@ int cipc_gpu_upload_scene(cipc_gpu_context* c, const cipc_scene_desc* d) {
         for (int i = 0; i < 2 * d->Es; ++i)
             if (d->surf_edges[i] >= (uint64_t)V) throw Error(CIPC_ERR_INVALID_ARGUMENT, "surface edge index out of range");
         s.V = V; s.T = T; s.F = d->F; s.Vs = d->Vs; s.Es = d->Es;
+        s.kb = 1;
+        while ((1ll << s.kb) <= (long long)V) ++s.kb;
         s.dt = d->dt; s.u = d->length_rate; s.r = d->volume_rate; s.drag = d->drag; s.hhat = d->hhat; s.fhat = d->fhat;
         s.fric = d->friction;
         s.gn = d3{d->ground_normal[0], d->ground_normal[1], d->ground_normal[2]};
@@ -177,6 +179,7 @@ int cipc_gpu_upload_scene(cipc_gpu_context* c, const cipc_scene_desc* d) {
         c->n_self = c->n_ee = c->n_ccd = c->n_env = 0;
         c->self_set.resize(1); c->ee_set.resize(1); c->ee_ie.resize(1); c->ccd_set.resize(1); c->env_set.resize(1);
         c->have_scene = true;
+        c->face_tree.age = c->edge_tree.age = kMaxRefits;
     });
 }
 
@@ -185,7 +188,10 @@ int cipc_gpu_set_state(cipc_gpu_context* c, const double* x, const double* vprev
         require(c && c->have_scene, "no scene uploaded");
         bind(*c);
         const size_t n = 3 * (size_t)c->sc.V * sizeof(double);
-        if (x) CIPC_HIP(hipMemcpyAsync(c->sc.x.p, x, n, hipMemcpyHostToDevice, c->stream));
+        if (x) {
+            CIPC_HIP(hipMemcpyAsync(c->sc.x.p, x, n, hipMemcpyHostToDevice, c->stream));
+            c->face_tree.age = c->edge_tree.age = kMaxRefits;   // new positions: fresh topology on the next build
+        }
         if (vprev) CIPC_HIP(hipMemcpyAsync(c->sc.vprev.p, vprev, n, hipMemcpyHostToDevice, c->stream));
         if (xt) CIPC_HIP(hipMemcpyAsync(c->sc.xt.p, xt, n, hipMemcpyHostToDevice, c->stream));
         CIPC_HIP(hipStreamSynchronize(c->stream));
@@ -435,10 +441,11 @@ int cipc_gpu_eval_make_pd(int n, int dim, double* H) {
         if (n) cipc_gpu_eval_make_pd_impl(n, dim, H);
     });
 }
-int cipc_gpu_eval_contact_pd(int n, int nverts, double* H) {
+int cipc_gpu_eval_contact_pd(int n, int nverts, double* H, const double* G) {
     return guard([&] {
         require(n >= 0 && nverts >= 2 && nverts <= 4 && (n == 0 || H), "invalid arguments");
-        if (n) cipc_gpu_eval_contact_pd_impl(n, nverts, H);
+        require(G == nullptr || nverts >= 3, "the structured projection needs 3 or 4 vertices");
+        if (n) cipc_gpu_eval_contact_pd_impl(n, nverts, H, G);
     });
 }
 int cipc_gpu_eval_accd(int n, int kind, const double* pts, const double* dirs, double eta, double th, double* toc) {

@@ -9,21 +9,36 @@ namespace cipc {
 // order.  Traversal is stackless: `left` is the first child of an internal node (or the
 // primitive id of a leaf) and `skip` the next node in depth-first order once a subtree
 // is done (-1 at the end).
+// Power-of-two reduction levels over the sorted leaf boxes (level l node j = leaves
+// [j 2^l, (j+1) 2^l)), used to form internal boxes without inter-work-group communication.
+struct SegLevels {
+    int count[40];
+    long long offset[40];
+};
+
+// Device LBVH.  The topology (Morton order, Karras splits, skip pointers) and the boxes are
+// built separately: a query's result set depends only on the leaf boxes, never on the
+// topology, so a topology built from earlier positions is refitted with the current boxes
+// and rebuilt only every kMaxRefits fits (or when the state is replaced).
 struct LBVH {
     int n = 0;
+    int age = 1 << 30;         // fits since the topology was built
     DBuf<Box> prim;            // primitive boxes (input)
     DBuf<Node> nodes;          // 2n-1: internal nodes 0..n-2 (root 0), leaves n-1..2n-2
-    DBuf<int> right;           // n-1 (internal nodes)
-    DBuf<int> frontier;        // 2 x kParts: subtree roots in depth-first order, then where each ends
     DBuf<int> range_first;     // n-1
     DBuf<int> range_last;      // n-1
     DBuf<int> split_right;     // n-1: right child of the internal node whose split is s
-    DBuf<Box> seg;             // power-of-two reduction levels over the sorted leaf boxes
-    DBuf<unsigned long long> keys, keys_sorted;
+    SegLevels L{};
+    int levels = 0;
+    DBuf<Box> seg;             // the reduction levels, level 0 = leaf boxes in leaf order
+    DBuf<unsigned> codes, codes_sorted;   // 30-bit Morton codes, by primitive / sorted
+    DBuf<int> order;                      // primitive ids in leaf order (+ n scratch)
+    DBuf<unsigned long long> keys_sorted; // (code << 32 | primitive), sorted and unique
     DBuf<Box> scene_partial;
     bool swept = false;
     double alpha = 0.0;
 };
+constexpr int kMaxRefits = 32;
 
 // Scratch of the active-set construction (broadphase.hip).
 struct ActiveSetScratch {
@@ -110,6 +125,7 @@ void sort_u64(Context& c, const unsigned long long* in, unsigned long long* out,
 void sort_u64_pairs(Context& c, const unsigned long long* kin, unsigned long long* kout,
                     const int* vin, int* vout, int n, int bits);
 void sort_i32_pairs(Context& c, const int* kin, int* kout, const int* vin, int* vout, int n, int bits);
+void sort_u32_pairs(Context& c, const unsigned* kin, unsigned* kout, const int* vin, int* vout, int n, int bits);
 bool any_intersection(Context& c);
 // ccd.hip
 void step_bound(Context& c, double out[5]);

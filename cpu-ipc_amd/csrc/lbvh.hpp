@@ -4,17 +4,26 @@
 
 namespace cipc {
 
-// A query can be split over kParts lanes: the frontier holds up to kParts disjoint
-// subtrees that together cover every leaf, in depth-first order (the root's descendants
-// down to depth log2(kParts), or a leaf where the tree is shallower), each with the node
-// that follows it.  Lane `part` walks only its subtree; the lanes' visits concatenated in
-// part order are the whole-tree depth-first visit.
-constexpr int kParts = 8;
+// A query can be split over kParts lanes.  Part p owns the subtree reached from the root by
+// the kPartBits bits of p, most significant first (0 = first child); where a leaf is met
+// before the bits run out, the part whose remaining bits are zero owns it and the others
+// are empty.  The parts are disjoint, cover every leaf, and in part order their visits
+// concatenate to the whole-tree depth-first visit.
+constexpr int kPartBits = 3;
+constexpr int kParts = 1 << kPartBits;
+
+// Workgroups are dispatched to the 8 XCDs of the device round-robin.  xcd_block maps the
+// physical index so that XCD k runs the k-th contiguous slice of the logical range: queries
+// taken in Morton order then walk nearby subtrees out of the same L2.
+constexpr int kXcds = 8;
+__device__ inline int xcd_block(int b, int nb) {
+    const int q = nb / kXcds, r = nb % kXcds, k = b % kXcds;
+    return k * q + (k < r ? k : r) + b / kXcds;
+}
 
 struct TreeView {
     int n;
     const Node* nodes;
-    const int* frontier;   // [kParts roots (-1 = empty part), kParts stops]
 };
 
 // Visits, in depth-first (Morton) order, every leaf of the subtree [start, stop) whose box
@@ -44,8 +53,17 @@ __device__ inline void lbvh_visit_range(int n, const Node* __restrict__ nd, int 
 // part in [0, kParts): that part's subtree; part < 0: the whole tree.
 template <typename F>
 __device__ inline void lbvh_visit(const TreeView& t, int part, const Box& q, double gap, F&& visit) {
-    if (part < 0) lbvh_visit_range(t.n, t.nodes, 0, -1, q, gap, visit);
-    else lbvh_visit_range(t.n, t.nodes, t.frontier[part], t.frontier[kParts + part], q, gap, visit);
+    if (t.n <= 0) return;
+    if (part < 0) { lbvh_visit_range(t.n, t.nodes, 0, -1, q, gap, visit); return; }
+    int node = 0;
+    for (int l = kPartBits - 1; l >= 0; --l) {
+        if (node >= t.n - 1) {
+            if (part & ((1 << (l + 1)) - 1)) return;
+            break;
+        }
+        node = ((part >> l) & 1) ? t.nodes[node].right : t.nodes[node].left;
+    }
+    lbvh_visit_range(t.n, t.nodes, node, t.nodes[node].skip, q, gap, visit);
 }
 
 }  // namespace cipc

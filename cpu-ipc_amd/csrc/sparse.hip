@@ -41,6 +41,13 @@ void sort_i32_pairs(Context& c, const int* kin, int* kout, const int* vin, int* 
     c.tmp.reserve(bytes);
     CIPC_HIP(hipcub::DeviceRadixSort::SortPairs(c.tmp.p, bytes, kin, kout, vin, vout, n, 0, bits, c.stream));
 }
+void sort_u32_pairs(Context& c, const unsigned* kin, unsigned* kout, const int* vin, int* vout, int n, int bits) {
+    if (n <= 0) return;
+    size_t bytes = 0;
+    CIPC_HIP(hipcub::DeviceRadixSort::SortPairs(nullptr, bytes, kin, kout, vin, vout, n, 0, bits, c.stream));
+    c.tmp.reserve(bytes);
+    CIPC_HIP(hipcub::DeviceRadixSort::SortPairs(c.tmp.p, bytes, kin, kout, vin, vout, n, 0, bits, c.stream));
+}
 // out[i] = sum in[0..i-1]; out has n+1 entries when in has n (out[n] = total).
 __global__ void set_zero_tail(int* p, int n) { p[n] = 0; }
 void exclusive_scan_int(Context& c, const int* in, int* out, int n) {
@@ -98,24 +105,24 @@ __device__ inline int lower_bound_u64(const unsigned long long* a, int n, unsign
     while (lo < hi) { const int m = (lo + hi) >> 1; if (a[m] < key) lo = m + 1; else hi = m; }
     return lo;
 }
-__global__ void merge_count(int V, const int* __restrict__ scol, const int* __restrict__ srow,
+__global__ void merge_count(int V, int kb, const int* __restrict__ scol, const int* __restrict__ srow,
                             const unsigned long long* __restrict__ ukey, const int* __restrict__ nu_ptr, int* __restrict__ cnt) {
     const int b = blockIdx.x * blockDim.x + threadIdx.x;
     if (b >= V) return;
     const int nu = *nu_ptr;
     int i = scol[b], ie = scol[b + 1];
-    int j = lower_bound_u64(ukey, nu, (unsigned long long)b << 32);
-    const int je = lower_bound_u64(ukey, nu, (unsigned long long)(b + 1) << 32);
+    int j = lower_bound_u64(ukey, nu, (unsigned long long)b << kb);
+    const int je = lower_bound_u64(ukey, nu, (unsigned long long)(b + 1) << kb);
     int n = 0;
     while (i < ie || j < je) {
         const int ra = i < ie ? srow[i] : 0x7fffffff;
-        const int rb = j < je ? (int)(ukey[j] & 0xffffffffull) : 0x7fffffff;
+        const int rb = j < je ? (int)(ukey[j] & ((1ull << kb) - 1)) : 0x7fffffff;
         if (ra == rb) { ++i; ++j; } else if (ra < rb) ++i; else ++j;
         ++n;
     }
     cnt[b] = n;
 }
-__global__ void merge_fill(int V, const int* __restrict__ scol, const int* __restrict__ srow, const double* __restrict__ sbv,
+__global__ void merge_fill(int V, int kb, const int* __restrict__ scol, const int* __restrict__ srow, const double* __restrict__ sbv,
                            const unsigned short* __restrict__ sbf, const unsigned long long* __restrict__ ukey,
                            const double* __restrict__ uval, const unsigned short* __restrict__ uflag, const int* __restrict__ nu_ptr,
                            const int* __restrict__ mptr, int* __restrict__ mrow, double* __restrict__ mval, unsigned short* __restrict__ mflag) {
@@ -123,12 +130,12 @@ __global__ void merge_fill(int V, const int* __restrict__ scol, const int* __res
     if (b >= V) return;
     const int nu = *nu_ptr;
     int i = scol[b], ie = scol[b + 1];
-    int j = lower_bound_u64(ukey, nu, (unsigned long long)b << 32);
-    const int je = lower_bound_u64(ukey, nu, (unsigned long long)(b + 1) << 32);
+    int j = lower_bound_u64(ukey, nu, (unsigned long long)b << kb);
+    const int je = lower_bound_u64(ukey, nu, (unsigned long long)(b + 1) << kb);
     int o = mptr[b];
     while (i < ie || j < je) {
         const int ra = i < ie ? srow[i] : 0x7fffffff;
-        const int rb = j < je ? (int)(ukey[j] & 0xffffffffull) : 0x7fffffff;
+        const int rb = j < je ? (int)(ukey[j] & ((1ull << kb) - 1)) : 0x7fffffff;
         double v[9];
         unsigned f = 0;
         if (ra <= rb) {
@@ -206,7 +213,9 @@ void finalize_csr(Context& c) {
     c.iscratch.resize((size_t)std::max(nb, 1) + 1);
     pos = c.iscratch.p;
     if (nb > 0) {
-        sort_u64_pairs(c, c.ckey.p, c.ckey_sorted.p, c.cidx.p, c.cidx_sorted.p, nb, 64);
+        // keys below the sentinel use the low 2 kb bits, and the sentinel's low bits are all
+        // ones, so sorting those bits orders them all
+        sort_u64_pairs(c, c.ckey.p, c.ckey_sorted.p, c.cidx.p, c.cidx_sorted.p, nb, 2 * s.kb);
         hipLaunchKernelGGL(run_heads_u64, dim3(blocks_for(nb + 1, 256)), dim3(256), 0, c.stream, nb, c.ckey_sorted.p, head);
         CIPC_CHECK_LAUNCH();
         exclusive_scan_int(c, head, pos, nb + 1);
@@ -221,7 +230,8 @@ void finalize_csr(Context& c) {
     const int ng = (int)c.gkey.n;
     if (ng > 0) {
         c.gkey_sorted.resize(ng); c.gidx_sorted.resize(ng);
-        sort_i32_pairs(c, c.gkey.p, c.gkey_sorted.p, c.gidx.p, c.gidx_sorted.p, ng, 32);
+        // vertex keys are below 2^kb and the unused-slot key 0x7fffffff has all low bits set
+        sort_i32_pairs(c, c.gkey.p, c.gkey_sorted.p, c.gidx.p, c.gidx_sorted.p, ng, s.kb);
         hipLaunchKernelGGL(add_gradient_records, dim3(blocks_for(ng, 256)), dim3(256), 0, c.stream, ng, c.gkey_sorted.p,
                            c.gidx_sorted.p, c.gval.p, V, c.grad.p);
         CIPC_CHECK_LAUNCH();
@@ -229,7 +239,7 @@ void finalize_csr(Context& c) {
     // merged block pattern
     c.mptr.resize((size_t)V + 1);
     DBuf<int>& cnt = c.iscratch2;
-    hipLaunchKernelGGL(merge_count, dim3(blocks_for(V, 256)), dim3(256), 0, c.stream, V, c.sp.col_ptr.p, c.sp.row.p, c.ukey.p,
+    hipLaunchKernelGGL(merge_count, dim3(blocks_for(V, 256)), dim3(256), 0, c.stream, V, s.kb, c.sp.col_ptr.p, c.sp.row.p, c.ukey.p,
                        c.counters.p, cnt.p);
     CIPC_CHECK_LAUNCH();
     hipLaunchKernelGGL(set_zero_tail, dim3(1), dim3(1), 0, c.stream, cnt.p, V);
@@ -238,7 +248,7 @@ void finalize_csr(Context& c) {
     CIPC_HIP(hipMemcpyAsync(&nmerged, c.mptr.p + V, sizeof(int), hipMemcpyDeviceToHost, c.stream));
     CIPC_HIP(hipStreamSynchronize(c.stream));
     c.mrow.resize(std::max(nmerged, 1)); c.mval.resize((size_t)9 * std::max(nmerged, 1)); c.mflag.resize(std::max(nmerged, 1));
-    hipLaunchKernelGGL(merge_fill, dim3(blocks_for(V, 256)), dim3(256), 0, c.stream, V, c.sp.col_ptr.p, c.sp.row.p, c.sbv.p,
+    hipLaunchKernelGGL(merge_fill, dim3(blocks_for(V, 256)), dim3(256), 0, c.stream, V, s.kb, c.sp.col_ptr.p, c.sp.row.p, c.sbv.p,
                        c.sbf.p, c.ukey.p, c.uval.p, c.uflag.p, c.counters.p, c.mptr.p, c.mrow.p, c.mval.p, c.mflag.p);
     CIPC_CHECK_LAUNCH();
     // scalar CSR

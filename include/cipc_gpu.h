@@ -156,15 +156,17 @@ int cipc_gpu_newton_iteration(cipc_gpu_context* ctx, double kappa, cipc_iteratio
  *  distance: kind 0 PP, 1 PE, 2 PT, 3 EE, 4 EE cross norm; pts 12n -> d (n), grad (12n), Hessian (144n)
  *            (ContactMechanics.cpp:282-316,764-2467)
  *  make_pd:  dim in {6,9,12}, H (dim*dim*n row-major) in place (ContactMechanics.h:27-45)
- *  contact_pd: the same projection as the assembly applies it to contact Hessians (functions
- *            of vertex differences only), nverts in {2,3,4}, H (9*nverts^2*n row-major) in place
+ *  contact_pd: the projection as the assembly applies it to contact Hessians (functions of
+ *            vertex differences only), nverts in {2,3,4}, H (9*nverts^2*n row-major) in place;
+ *            with G (3*nverts*n, the blocks' gradients, point-edge / point-triangle /
+ *            edge-edge distances only) the structured range-basis path, else the general one
  *            (ContactMechanics.cpp:3489,3516,3539,3561)
  *  accd:     kind 0 point-triangle, 1 edge-edge; pts 12n, dirs 12n -> toc (n) (AdditiveCCD.cpp:221-329)
  *  classify: kind 0 dType_PT, 1 dType_EE; pts 12n -> type (n) (ContactMechanics.cpp:143-280) */
 int cipc_gpu_eval_snh(int n, const double* F, double length_rate, double volume_rate, double* P, double* H, double* e);
 int cipc_gpu_eval_distance(int n, int kind, const double* pts, double* d, double* grad, double* hess);
 int cipc_gpu_eval_make_pd(int n, int dim, double* H);
-int cipc_gpu_eval_contact_pd(int n, int nverts, double* H);
+int cipc_gpu_eval_contact_pd(int n, int nverts, double* H, const double* G);
 int cipc_gpu_eval_accd(int n, int kind, const double* pts, const double* dirs, double eta, double thickness, double* toc);
 int cipc_gpu_eval_classify(int n, int kind, const double* pts, int* type);
 

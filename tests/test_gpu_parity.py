@@ -80,14 +80,19 @@ def test_make_pd_matches_oracle(gpu, O, dim):
 
 @pytest.mark.parametrize("kind", [0, 1, 2, 3])
 def test_contact_projection_matches_make_pd(gpu, O, kind):
-    """The projection the assembly applies to contact Hessians (on the translation-free
-    subspace) against the oracle's makePD of the full block."""
-    from contact_cases import barrier_hessians
-    H = barrier_hessians(O, kind, 300, seed=80 + kind)
+    """The projections the assembly applies to contact Hessians against the oracle's
+    makePD of the full block: the general one on the translation-free subspace (all kinds)
+    and the structured range-basis one (point-edge, point-triangle, edge-edge)."""
+    from contact_cases import barrier_blocks
+    H, G = barrier_blocks(O, kind, 300, seed=80 + kind)
+    Ho = [O.make_pd(H[i]) for i in range(len(H))]
     Hg = gpu.eval_contact_pd(H)
     for i in range(len(H)):
-        Ho = O.make_pd(H[i])
-        assert rel_err(Hg[i], Ho) <= REL, (kind, i, rel_err(Hg[i], Ho))
+        assert rel_err(Hg[i], Ho[i]) <= REL, (kind, i, rel_err(Hg[i], Ho[i]))
+    if kind >= 1:
+        Hs = gpu.eval_contact_pd(H, G)
+        for i in range(len(H)):
+            assert rel_err(Hs[i], Ho[i]) <= REL, ("structured", kind, i, rel_err(Hs[i], Ho[i]))
 
 
 def _ccd_inputs(rng, n):
@@ -195,6 +200,26 @@ def test_step_forward_and_intersection(pair):
     g.active_set()
     o.active_set()
     assert g.is_intersected() == o.is_intersected()
+    g.set_state(x=s.x)
+    o.set_state(x=s.x)
+
+
+def test_refitted_trees_after_a_step(pair):
+    """After step_forward the device refits trees whose topology was built at the old
+    positions; queries still return exactly the reference's sets and step bounds."""
+    g, o, p = pair["g"], pair["o"], pair["p"]
+    s = pair["scene"]
+    g.active_set()
+    g.set_direction(p)
+    g.save_positions()
+    g.step_forward(0.7)
+    o.step_forward(s.x, p, 0.7)
+    ag, ao = g.active_set(), o.active_set()
+    for k in ("self_set", "ee_set", "ee_ie", "ccd_set", "env_set"):
+        assert np.array_equal(ag[k], ao[k]), (pair["name"], k)
+    bg, bo = g.step_bound(p), o.step_bound(p)
+    for k in ("alpha", "ground", "partial", "cfl", "full"):
+        assert bg[k] == bo[k], (pair["name"], k, bg[k], bo[k])
     g.set_state(x=s.x)
     o.set_state(x=s.x)
 

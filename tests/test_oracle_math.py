@@ -89,6 +89,43 @@ def test_contact_make_pd_on_translation_free_subspace(O, kind):
     assert npd < 60   # the projection path is exercised
 
 
+@pytest.mark.parametrize("kind", [1, 2, 3])
+def test_structured_contact_blocks_live_on_a_small_range(O, kind):
+    """Point-edge, point-triangle and edge-edge barrier blocks map into a 4 / 5 / 5
+    dimensional space built from the rank-one factors c (x) u of their own gradient:
+    {w (x) u : sum w = 0} + {c (x) t : t perpendicular to u} (point-triangle, edge-edge) and
+    {c (x) t : t in R^3} + {(0, 1, -1) (x) u} (point-edge).  makePD of the block equals the
+    projection computed from the eigenpairs of the block restricted to that space."""
+    from contact_cases import NVERTS, barrier_blocks
+    m = NVERTS[kind]
+    H, G = barrier_blocks(O, kind, 200, seed=90 + kind)
+    ones = np.ones(m) / np.sqrt(m)
+    for i in range(len(H)):
+        Gm = G[i].reshape(m, 3)
+        u = Gm[np.argmax(np.sum(Gm * Gm, axis=1))]
+        u = u / np.linalg.norm(u)
+        c = Gm @ u
+        c = c / np.linalg.norm(c)
+        assert abs(c @ ones) < 1e-12
+        t = np.linalg.svd(u[None])[2][1:]           # two unit vectors perpendicular to u
+        if m == 4:
+            w = np.linalg.svd(ones[None])[2][1:]    # orthonormal basis of sum-zero weights
+            cols = [np.kron(wi, u) for wi in w] + [np.kron(c, ti) for ti in t]
+        else:
+            dv = np.array([0.0, 1.0, -1.0]) / np.sqrt(2)
+            dv = dv - (dv @ c) * c
+            cols = [np.kron(c, e) for e in np.eye(3)] + [np.kron(dv / np.linalg.norm(dv), u)]
+        Q = np.stack(cols, axis=1)
+        assert np.allclose(Q.T @ Q, np.eye(Q.shape[1]), atol=1e-12)
+        P = Q @ Q.T
+        scale = np.abs(H[i]).max()
+        assert np.abs(H[i] - P @ H[i] @ P).max() <= 1e-12 * scale
+        lam, Y = np.linalg.eigh(Q.T @ H[i] @ Q)
+        Z = Q @ Y
+        proj = H[i] - Z @ np.diag(np.minimum(lam, 0)) @ Z.T if lam[0] < 0 else H[i]
+        assert np.allclose(proj, O.make_pd(H[i]), atol=1e-11 * scale)
+
+
 # ---------------------------------------------------------------------------------------
 U_RATE, R_RATE = 4 * 35714.2857 / 3, 142857.142 + 5 * 35714.2857 / 6
 
