@@ -183,6 +183,10 @@ struct Mesh {
     double dt = 0, u = 0, r = 0, drag = 0, hhat = 0, fhat = 0, fric = 0;
     V3 gn{0, 1, 0}; double gd = -1;
     std::vector<I4> selfAS, eeAS; std::vector<std::pair<int, int>> eeIe, ccdAS; std::vector<int> envAS;
+    // lagged friction state (Friction::initialize, Friction.cpp:81-156; SimulationMesh.h:86-93)
+    std::vector<I4> selfLast; std::vector<double> selfLam;
+    std::vector<std::array<double, 2>> distCoord; std::vector<std::array<V3, 2>> tanBasis;
+    std::vector<int> envLast; std::vector<double> envLam;
     std::vector<V3> grad;
     std::vector<int> outer, inner; std::vector<double> val, rhs;
     LBVH faceTree, edgeTree;
@@ -455,6 +459,252 @@ int decode(const I4& c, int vid[4]) {
 }
 
 // ----------------------------------------------------------------------------------
+// Lagged friction (Friction.cpp, FrictionKinematics.h with SFCLAMPING_ORDER 1)
+// ----------------------------------------------------------------------------------
+void f0_SF(double x2, double e, double& f0) { f0 = x2 * (-std::sqrt(x2) / 3.0 + e) / (e * e) + e / 3.0; }    // :262-265
+void f1_SF_div(double x2, double e, double& r) { r = (-std::sqrt(x2) + 2.0 * e) / (e * e); }             // :267-270
+void f2_SF(double x2, double e, double& f2) { f2 = 2.0 * (e - std::sqrt(x2)) / (e * e); }                 // :272-275
+V3 normalized(const V3& v) {   // Eigen 3.4 normalized(): v / |v|, unchanged when |v| = 0
+    const double n2 = sqn(v);
+    if (!(n2 > 0.0)) return v;
+    const double n = std::sqrt(n2);
+    return V3(v.x / n, v.y / n, v.z / n);
+}
+
+// Tangent bases (FrictionKinematics.h:18-28,81-91,155-164,211-227) and closest points
+// (:30-41,93-114,166-172).  Columns of the 3x2 basis are b[0], b[1].
+void tangent_PT(const V3& v0, const V3& v1, const V3& v2, const V3& v3, std::array<V3, 2>& b) {
+    const V3 v12 = v2 - v1;
+    b[0] = normalized(v12);
+    b[1] = normalized(cross(cross(v12, v3 - v1), v12));
+    (void)v0;
+}
+void closest_PT(const V3& v0, const V3& v1, const V3& v2, const V3& v3, double beta[2]) {
+    const V3 r0 = v2 - v1, r1 = v3 - v1, q = v0 - v1;
+    ldlt2_solve(dot(r0, r0), dot(r1, r0), dot(r1, r1), dot(r0, q), dot(r1, q), beta[0], beta[1]);
+}
+void tangent_EE(const V3& v0, const V3& v1, const V3& v2, const V3& v3, std::array<V3, 2>& b) {
+    const V3 v01 = v1 - v0;
+    b[0] = normalized(v01);
+    b[1] = normalized(cross(cross(v01, v3 - v2), v01));
+}
+void closest_EE(const V3& v0, const V3& v1, const V3& v2, const V3& v3, double g[2]) {
+    const V3 e20 = v0 - v2, e01 = v1 - v0, e23 = v3 - v2;
+    const double c00 = sqn(e01), c01 = -dot(e23, e01), c11 = sqn(e23);
+    ldlt2_solve(c00, c01, c11, -dot(e20, e01), dot(e20, e23), g[0], g[1]);
+}
+void tangent_PE(const V3& v0, const V3& v1, const V3& v2, std::array<V3, 2>& b) {
+    const V3 v12 = v2 - v1;
+    b[0] = normalized(v12);
+    b[1] = normalized(cross(v12, v0 - v1));
+}
+double closest_PE(const V3& v0, const V3& v1, const V3& v2) { const V3 e12 = v2 - v1; return dot(v0 - v1, e12) / sqn(e12); }
+void tangent_PP(const V3& v0, const V3& v1, std::array<V3, 2>& b) {
+    const V3 v01 = v1 - v0;
+    const V3 xc = cross(V3(1, 0, 0), v01), yc = cross(V3(0, 1, 0), v01);
+    if (sqn(xc) > sqn(yc)) { b[0] = normalized(xc); b[1] = normalized(cross(v01, xc)); }
+    else { b[0] = normalized(yc); b[1] = normalized(cross(v01, yc)); }
+}
+
+// Relative tangential displacement of a lagged contact (computeRelDX_*), in 3D.
+V3 rel_dx(const Mesh& m, size_t c) {
+    const I4& id = m.selfLast[c];
+    auto dx = [&](int v) { return m.x[v] - m.vprev[v]; };
+    const double b1 = m.distCoord[c][0], b2 = m.distCoord[c][1];
+    if (id[0] >= 0) return dx(id[0]) + b1 * (dx(id[1]) - dx(id[0])) - (dx(id[2]) + b2 * (dx(id[3]) - dx(id[2])));   // :117-126
+    const int p = -id[0] - 1;
+    if (id[2] < 0) return dx(p) - dx(id[1]);                                                             // :229-235
+    if (id[3] < 0) return dx(p) - (dx(id[1]) + b1 * (dx(id[2]) - dx(id[1])));                               // :174-181
+    return dx(p) - (dx(id[1]) + b1 * (dx(id[2]) - dx(id[1])) + b2 * (dx(id[3]) - dx(id[1])));              // :43-51
+}
+// liftRelDXTanToMesh_*: the 3M-vector of a tangent direction t (2D), and its vertex ids.
+int lift(const Mesh& m, size_t c, const double t[2], double* out, int vid[4]) {
+    const I4& id = m.selfLast[c];
+    const V3 T = t[0] * m.tanBasis[c][0] + t[1] * m.tanBasis[c][1];
+    const double b1 = m.distCoord[c][0], b2 = m.distCoord[c][1];
+    auto put = [&](int k, const V3& v) { out[3 * k] = v.x; out[3 * k + 1] = v.y; out[3 * k + 2] = v.z; };
+    if (id[0] >= 0) {
+        put(0, (1.0 - b1) * T); put(1, b1 * T); put(2, (b2 - 1.0) * T); put(3, -b2 * T);                  // :128-139
+        for (int k = 0; k < 4; ++k) vid[k] = id[k];
+        return 4;
+    }
+    const int p = -id[0] - 1;
+    vid[0] = p; vid[1] = id[1];
+    put(0, T);
+    if (id[2] < 0) { put(1, -1.0 * T); return 2; }                                                          // :237-244
+    vid[2] = id[2];
+    if (id[3] < 0) { put(1, (b1 - 1.0) * T); put(2, -b1 * T); return 3; }                                 // :183-192
+    vid[3] = id[3];
+    put(1, (-1 + b1 + b2) * T); put(2, -b1 * T); put(3, -b2 * T);                                          // :53-63
+    return 4;
+}
+
+void friction_initialize(Mesh& m, double kappa) {
+    const size_t n = m.selfAS.size();
+    m.selfLam.resize(n); m.distCoord.assign(n, {0.0, 0.0}); m.tanBasis.resize(n);
+    for (size_t i = 0; i < n; ++i) {                                                                     // :88-96
+        const I4& c = m.selfAS[i];
+        const double d = self_val(m, c);
+        double lam;
+        barrier_g(d, m.hhat, lam);
+        lam *= -kappa * 2.0 * std::sqrt(d);
+        if (c[3] < -1) lam *= -c[3];
+        m.selfLam[i] = lam;
+        if (c[0] >= 0) {
+            closest_EE(m.x[c[0]], m.x[c[1]], m.x[c[2]], m.x[c[3]], m.distCoord[i].data());
+            tangent_EE(m.x[c[0]], m.x[c[1]], m.x[c[2]], m.x[c[3]], m.tanBasis[i]);
+        } else {
+            const int p = -c[0] - 1;
+            if (c[2] < 0) tangent_PP(m.x[p], m.x[c[1]], m.tanBasis[i]);
+            else if (c[3] < 0) {
+                m.distCoord[i][0] = closest_PE(m.x[p], m.x[c[1]], m.x[c[2]]);
+                tangent_PE(m.x[p], m.x[c[1]], m.x[c[2]], m.tanBasis[i]);
+            } else {
+                closest_PT(m.x[p], m.x[c[1]], m.x[c[2]], m.x[c[3]], m.distCoord[i].data());
+                tangent_PT(m.x[p], m.x[c[1]], m.x[c[2]], m.x[c[3]], m.tanBasis[i]);
+            }
+        }
+    }
+    m.selfLast = m.selfAS;
+    m.envLam.resize(m.envAS.size());                                                                     // :141-152
+    for (size_t i = 0; i < m.envAS.size(); ++i) {
+        const double dist = dot(m.gn, m.x[m.envAS[i]]) - m.gd;
+        const double d = dist * dist;
+        double lam;
+        barrier_g(d, m.hhat, lam);
+        lam *= -kappa * 2.0 * std::sqrt(d);
+        m.envLam[i] = lam;
+    }
+    m.envLast = m.envAS;
+}
+
+// Tangent-plane basis of the ground normal (Friction.cpp:66-76).
+std::array<V3, 2> ground_basis(const V3& nrm) {
+    const V3 helper = std::fabs(nrm.x) < 0.9 ? V3(1, 0, 0) : V3(0, 1, 0);
+    std::array<V3, 2> b;
+    b[0] = normalized(helper - dot(helper, nrm) * nrm);
+    b[1] = normalized(cross(nrm, b[0]));
+    return b;
+}
+
+// Friction::addGradient (Friction.cpp:158-302).
+void friction_gradient(Mesh& m, double eps2, double coef) {
+    const double eps = std::sqrt(eps2);
+    for (size_t c = 0; c < m.selfLast.size(); ++c) {
+        const V3 r3 = rel_dx(m, c);
+        double t[2] = {dot(r3, m.tanBasis[c][0]), dot(r3, m.tanBasis[c][1])};
+        const double sq = t[0] * t[0] + t[1] * t[1];
+        if (sq > eps2) { const double nr = std::sqrt(sq); t[0] /= nr; t[1] /= nr; }
+        else { double f1; f1_SF_div(sq, eps, f1); t[0] *= f1; t[1] *= f1; }
+        double L[12]; int vid[4];
+        const int nv = lift(m, c, t, L, vid);
+        const double s = coef * m.selfLam[c];
+        for (int k = 0; k < nv; ++k) m.grad[vid[k]] = m.grad[vid[k]] + V3(L[3 * k] * s, L[3 * k + 1] * s, L[3 * k + 2] * s);
+    }
+    for (size_t c = 0; c < m.envLast.size(); ++c) {
+        const int v = m.envLast[c];
+        const V3 d = m.x[v] - m.vprev[v];
+        const V3 pj = d - dot(d, m.gn) * m.gn;
+        const double mag2 = sqn(pj);
+        if (mag2 > eps2) m.grad[v] = m.grad[v] + (coef * m.envLam[c] / std::sqrt(mag2)) * pj;
+        else m.grad[v] = m.grad[v] + (coef * m.envLam[c] / eps) * pj;
+    }
+}
+
+// Friction::addHessian (Friction.cpp:304-457) with analyticFrictionHessian (:11-64).
+void friction_hessian(Mesh& m, Blocks& BH, double eps2, double coef) {
+    const double eps = std::sqrt(eps2);
+    const std::array<V3, 2> gb = ground_basis(m.gn);
+    for (size_t c = 0; c < m.envLast.size(); ++c) {
+        const int v = m.envLast[c];
+        const double mult = std::max(0.0, coef * m.envLam[c]);
+        const V3 d = m.x[v] - m.vprev[v];
+        const double r0 = dot(gb[0], d), r1 = dot(gb[1], d);
+        const double nr = std::sqrt(r0 * r0 + r1 * r1);
+        std::array<double, 9> H{};
+        if (mult > 0.0) {
+            if (nr > eps) {
+                const double p0 = r0 / nr, p1 = r1 / nr;
+                const V3 l = (-p1) * gb[0] + p0 * gb[1];
+                const double s = mult / nr;
+                for (int i = 0; i < 3; ++i) for (int j = 0; j < 3; ++j) H[3 * i + j] = (s * l[i]) * l[j];
+            } else {
+                const double s = mult / eps;
+                for (int i = 0; i < 3; ++i) for (int j = 0; j < 3; ++j) H[3 * i + j] = s * (gb[0][i] * gb[0][j] + gb[1][i] * gb[1][j]);
+            }
+        }
+        BH.h3.push_back(H); BH.d1.push_back(v);
+    }
+    for (size_t c = 0; c < m.selfLast.size(); ++c) {
+        const V3 r3 = rel_dx(m, c);
+        const double u[2] = {dot(r3, m.tanBasis[c][0]), dot(r3, m.tanBasis[c][1])};
+        const double w = std::max(0.0, coef * m.selfLam[c]);
+        double H[144] = {};
+        int vid[4] = {-1, -1, -1, -1};
+        int nv = 0;
+        {
+            double L[12];
+            const double t0[2] = {1.0, 0.0};
+            nv = lift(m, c, t0, L, vid);
+        }
+        const int N = 3 * nv;
+        auto add = [&](const double t[2], double ev) {
+            const double e = std::max(0.0, ev);
+            if (e == 0.0) return;
+            double L[12]; int vv[4];
+            lift(m, c, t, L, vv);
+            const double s = w * e;
+            for (int i = 0; i < N; ++i) for (int j = 0; j < N; ++j) H[N * i + j] += (s * L[i]) * L[j];
+        };
+        if (w != 0.0) {
+            const double sq = u[0] * u[0] + u[1] * u[1];
+            const double nr = std::sqrt(sq);
+            if (nr > 0.0) {
+                const double par[2] = {u[0] / nr, u[1] / nr};
+                const double perp[2] = {-par[1], par[0]};
+                if (nr > eps) add(perp, 1.0 / nr);
+                else {
+                    double pe, pa;
+                    f1_SF_div(sq, eps, pe); f2_SF(sq, eps, pa);
+                    add(par, pa); add(perp, pe);
+                }
+            } else {
+                double iso;
+                f1_SF_div(0.0, eps, iso);
+                const double ex[2] = {1.0, 0.0}, ey[2] = {0.0, 1.0};
+                add(ex, iso); add(ey, iso);
+            }
+        }
+        if (nv == 2) { std::array<double, 36> h; std::copy(H, H + 36, h.begin()); BH.h6.push_back(h); BH.d2.push_back({vid[0], vid[1]}); }
+        else if (nv == 3) { std::array<double, 81> h; std::copy(H, H + 81, h.begin()); BH.h9.push_back(h); BH.d3.push_back({vid[0], vid[1], vid[2]}); }
+        else { std::array<double, 144> h; std::copy(H, H + 144, h.begin()); BH.h12.push_back(h); BH.d4.push_back({vid[0], vid[1], vid[2], vid[3]}); }
+    }
+}
+
+// Friction::energy (Friction.cpp:459-545), without the friction coefficient's factor.
+double friction_energy(const Mesh& m) {
+    const double eps2 = m.fhat * m.dt * m.dt, eps = std::sqrt(eps2);
+    double sum = 0.0;
+    for (size_t c = 0; c < m.selfLast.size(); ++c) {
+        const V3 r3 = rel_dx(m, c);
+        const double a = dot(r3, m.tanBasis[c][0]), b = dot(r3, m.tanBasis[c][1]);
+        const double sq = a * a + b * b;
+        if (sq > eps2) sum += m.selfLam[c] * std::sqrt(sq);
+        else { double f0; f0_SF(sq, eps, f0); sum += m.selfLam[c] * f0; }
+    }
+    double result = m.fric * sum;
+    for (size_t c = 0; c < m.envLast.size(); ++c) {
+        const int v = m.envLast[c];
+        const V3 d = m.x[v] - m.vprev[v];
+        const V3 pj = d - dot(d, m.gn) * m.gn;
+        const double sq = sqn(pj);
+        const double cl = sq > eps2 ? std::sqrt(sq) - 0.5 * eps : 0.5 * sq / eps;
+        result += m.fric * m.envLam[c] * cl;
+    }
+    return result;
+}
+
+// ----------------------------------------------------------------------------------
 // computeGradientAndHessian (IPCSolver.cpp:108-319), tet + ground + self contact subset
 // ----------------------------------------------------------------------------------
 int gradient_and_hessian(Mesh& m, double kappa, Blocks& BH) {
@@ -644,6 +894,9 @@ int gradient_and_hessian(Mesh& m, double kappa, Blocks& BH) {
         De[i] = {e4[0], e4[1], e4[2], e4[3]};
     }
     for (size_t i = 0; i < ne; ++i) { BH.h12.push_back(He[i]); BH.d4.push_back(De[i]); }
+    // lagged friction (IPCSolver.cpp:309-317)
+    friction_gradient(m, m.fhat * m.dt * m.dt, m.fric);
+    friction_hessian(m, BH, m.fhat * m.dt * m.dt, m.fric);
     return collisions;
 }
 
@@ -700,7 +953,7 @@ void assemble_csr(Mesh& m, const Blocks& BH) {
 // ----------------------------------------------------------------------------------
 // computeEnergy (IPCSolver.cpp:322-456), tet + inertia + barrier subset
 // ----------------------------------------------------------------------------------
-void energy_terms(Mesh& m, double kappa, double out[5]) {
+void energy_terms(Mesh& m, double kappa, double out[6]) {
     std::vector<double> et(m.T);
 #pragma omp parallel for schedule(dynamic, 256)
     for (int t = 0; t < m.T; ++t) et[t] = snh_energy_density(deformation(m, t), m.u, m.r) * m.vol[t];
@@ -735,15 +988,17 @@ void energy_terms(Mesh& m, double kappa, double out[5]) {
         ee += b * e;
     }
     out[0] = esnh; out[1] = de; out[2] = eg; out[3] = es; out[4] = ee;
+    out[5] = friction_energy(m);
     (void)kappa;
 }
 double energy(Mesh& m, double kappa) {
-    double t[5];
+    double t[6];
     energy_terms(m, kappa, t);
     double E = t[0];
     E *= m.dt * m.dt;
     E += t[1];
     E += kappa * (t[2] + t[3] + t[4]);
+    E += t[5];
     return E;
 }
 
@@ -1216,7 +1471,18 @@ void oracle_get_csr(const oracle_mesh* om, int* outer, int* inner, double* val, 
     if (rhs) std::copy(m.rhs.begin(), m.rhs.end(), rhs);
 }
 double oracle_energy(oracle_mesh* om, double kappa) { return energy(om->m, kappa); }
-void oracle_energy_terms(oracle_mesh* om, double kappa, double t[5]) { energy_terms(om->m, kappa, t); }
+void oracle_energy_terms(oracle_mesh* om, double kappa, double t[6]) { energy_terms(om->m, kappa, t); }
+void oracle_friction_update(oracle_mesh* om, double kappa) { friction_initialize(om->m, kappa); }
+void oracle_get_friction(const oracle_mesh* om, int64_t counts[2], double* lam, double* coord, double* basis, double* env_lam) {
+    const Mesh& m = om->m;
+    counts[0] = (int64_t)m.selfLast.size(); counts[1] = (int64_t)m.envLast.size();
+    if (lam) std::copy(m.selfLam.begin(), m.selfLam.end(), lam);
+    if (coord) for (size_t i = 0; i < m.distCoord.size(); ++i) { coord[2 * i] = m.distCoord[i][0]; coord[2 * i + 1] = m.distCoord[i][1]; }
+    if (basis)
+        for (size_t i = 0; i < m.tanBasis.size(); ++i)
+            for (int k = 0; k < 2; ++k) { basis[6 * i + 3 * k] = m.tanBasis[i][k].x; basis[6 * i + 3 * k + 1] = m.tanBasis[i][k].y; basis[6 * i + 3 * k + 2] = m.tanBasis[i][k].z; }
+    if (env_lam) std::copy(m.envLam.begin(), m.envLam.end(), env_lam);
+}
 void oracle_step_bound(oracle_mesh* om, const double* p, double out[5]) {
     std::vector<V3> pv(om->m.V);
     for (int i = 0; i < om->m.V; ++i) pv[i] = V3(p[3 * i], p[3 * i + 1], p[3 * i + 2]);

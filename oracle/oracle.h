@@ -56,7 +56,12 @@ void oracle_get_csr(const oracle_mesh* m, int* outer, int* inner, double* val, d
 /* computeEnergy (without friction / cloth / bending terms for tet-only scenes). */
 double oracle_energy(oracle_mesh* m, double kappa);
 /* per-term energies: {SNH (undivided by dt^2), inertia+drag, ground barrier, self barrier, mollified EE} */
-void oracle_energy_terms(oracle_mesh* m, double kappa, double terms[5]);
+void oracle_energy_terms(oracle_mesh* m, double kappa, double terms[6]);
+/* Friction::initialize: lagged friction state from the current active set and positions. */
+void oracle_friction_update(oracle_mesh* m, double kappa);
+/* counts[0] self contacts, counts[1] ground contacts; lam (n), coord (2n), basis (6n: two
+ * columns of 3), env_lam (counts[1]); any output may be NULL. */
+void oracle_get_friction(const oracle_mesh* m, int64_t counts[2], double* lam, double* coord, double* basis, double* env_lam);
 
 /* Newton step bound (IPCSolver.cpp:943-990): out = {alpha, ground, partial, cfl, full} */
 void oracle_step_bound(oracle_mesh* m, const double* p, double out[5]);

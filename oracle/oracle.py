@@ -60,6 +60,8 @@ def lib():
         L.oracle_energy.restype = C.c_double
         L.oracle_energy.argtypes = [C.c_void_p, C.c_double]
         L.oracle_energy_terms.argtypes = [C.c_void_p, C.c_double, _dp]
+        L.oracle_friction_update.argtypes = [C.c_void_p, C.c_double]
+        L.oracle_get_friction.argtypes = [C.c_void_p, _i64p, _dp, _dp, _dp, _dp]
         L.oracle_step_bound.argtypes = [C.c_void_p, _dp, _dp]
         L.oracle_step_forward.argtypes = [C.c_void_p, _dp, _dp, C.c_double]
         L.oracle_get_positions.argtypes = [C.c_void_p, _dp]
@@ -183,9 +185,21 @@ class OracleMesh:
         return lib().oracle_energy(self.h, kappa)
 
     def energy_terms(self, kappa):
-        t = np.zeros(5)
+        """[elastic, inertia, ground barrier, self barrier, mollified barrier, friction]"""
+        t = np.zeros(6)
         lib().oracle_energy_terms(self.h, kappa, dptr(t))
         return t
+
+    def friction_update(self, kappa):
+        """Friction::initialize on the current active set and positions."""
+        lib().oracle_friction_update(self.h, kappa)
+
+    def friction_state(self):
+        n = np.zeros(2, np.int64)
+        lib().oracle_get_friction(self.h, n.ctypes.data_as(_i64p), None, None, None, None)
+        lam, coord, basis, env = np.zeros(n[0]), np.zeros(2 * n[0]), np.zeros(6 * n[0]), np.zeros(n[1])
+        lib().oracle_get_friction(self.h, n.ctypes.data_as(_i64p), dptr(lam), dptr(coord), dptr(basis), dptr(env))
+        return dict(lam=lam, coord=coord.reshape(-1, 2), basis=basis.reshape(-1, 2, 3), env_lam=env)
 
     def step_bound(self, p):
         p = _d(p)
