@@ -2,15 +2,18 @@
 
 Workload: four 20^3-cell lattice cubes (37,044 vertices / 160,000 tets, cell 0.1 m),
 seeded yaw rotations, stacked with 1e-3 m gaps on the ground (every interface in
-contact), E=1e5 Pa, nu=0.4, rho=1000, dt=0.01 s, relative dHat 1e-3, FP64.  A step is one
-Newton iteration of the reference's barrier subproblem without the linear solve
-(CPU IPC/IPCSolver.cpp:912-998), on a frozen state with inputs resident in HBM:
+contact), E=1e5 Pa, nu=0.4, rho=1000, dt=0.01 s, relative dHat 1e-3, friction mu=0.3,
+FP64.  A step is one Newton iteration of the reference's barrier subproblem without the
+linear solve (CPU IPC/IPCSolver.cpp:912-998), on a frozen state with inputs resident in HBM:
   1. LBVH build + active set      (buildCollisionSets)
-  2. gradient + projected Hessian -> the reference's lower CSR  (computeGradientAndHessian + assemble)
+  2. gradient + projected Hessian with lagged friction -> the reference's lower CSR
+     (computeGradientAndHessian + assemble)
   3. step bound: ground + partial CCD + CFL + swept LBVH + full CCD
-  4. one line-search energy evaluation
+  4. one line-search energy evaluation (with friction)
 The frozen state: x as generated, each cube given a seeded velocity in [-0.5, 0.5]^3 m/s,
-x~ from velocity and gravity, direction p = x - x~ perturbed per vertex by +-5%.
+positions at the step start V_prev = x - v dt, x~ from velocity and gravity, direction
+p = x - x~ perturbed per vertex by +-5%; the lagged friction state (Friction::initialize)
+is taken once at that state, as at the start of a time step.
 
 Prints one JSON line.  --gpus above 1 exits: every run here uses one GPU.
 """
@@ -41,11 +44,14 @@ def elastic_algorithmic_bytes(scene):
     return 24 * scene.V + scene.T * (72 + 8 + 16) + 24 * scene.V + 8 * nnz, nnz
 
 
-def cpu_baseline(scene, xt, p, kappa, budget_s=15.0, threads=16):
+def cpu_baseline(scene, xt, vprev, p, kappa, budget_s=15.0, threads=16):
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle as O
     threads = max(1, min(threads, os.cpu_count() or 1))
     m = O.OracleMesh(scene, xtilde=xt, threads=threads)
+    m.set_state(V_prev=vprev)
+    m.active_set()
+    m.friction_update(kappa)
     t0 = time.perf_counter()
     n = 0
     while True:
@@ -82,8 +88,12 @@ def main():
 
     scene = scenes.four_cubes20(velocity_scale=0.5)
     xt, p = scenes.newton_probe_state(scene)
+    vprev = scene.x - scene.velocities * scene.material.dt
     kappa = scenes.suggest_kappa(scene.hhat, scene.bbox_diag2, scene.mean_mass)
     ctx = gpu.GpuContext(scene, xtilde=xt)
+    ctx.set_state(V_prev=vprev)
+    ctx.active_set()
+    ctx.friction_update(kappa)
     ctx.set_direction(p)
     for _ in range(args.warmup):
         ctx.newton_iteration(kappa)
@@ -115,7 +125,7 @@ def main():
         "vs_baseline": None,
         "dtype": "f64",
         "data": "synthetic",
-        "config": {"workload": "4x20^3 lattice cubes, 37044 verts / 160000 tets, stacked in contact, frozen Newton state",
+        "config": {"workload": "4x20^3 lattice cubes, 37044 verts / 160000 tets, stacked in contact, friction 0.3, frozen Newton state",
                    "V": scene.V, "T": scene.T, "nnz": int(last["nnz"]), "contacts": int(last["contacts"])},
         "stages_ms": {k: round(v, 4) for k, v in mean.items()},
         "roofline": {"kernel": "elastic assembly (snh_element_kernel + block/vertex gathers)", "bound": "hbm",
@@ -124,7 +134,7 @@ def main():
                      "algorithmic_bytes": int(bytes_el), "elastic_nnz": int(nnz_el)},
     }
     if not args.no_cpu:
-        out["cpu_baseline"] = cpu_baseline(scene, xt, p, kappa, budget_s=args.cpu_budget)
+        out["cpu_baseline"] = cpu_baseline(scene, xt, vprev, p, kappa, budget_s=args.cpu_budget)
     print(json.dumps(out))
 
 

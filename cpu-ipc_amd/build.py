@@ -19,7 +19,8 @@ COMMON = ["--offload-arch=" + ARCH, "-O3", "-std=c++17", "-fPIC", "-Wall", "-Wno
           "-Wno-unused-variable", "-Wno-unused-lambda-capture",
           "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"]
 NO_CONTRACT = {"contact.hip", "broadphase.hip", "ccd.hip", "energy.hip", "sparse.hip"}
-SOURCES = ["context.hip", "elastic.hip", "contact.hip", "sparse.hip", "broadphase.hip", "ccd.hip", "energy.hip"]
+SOURCES = ["context.hip", "elastic.hip", "cloth.hip", "contact.hip", "sparse.hip", "broadphase.hip", "ccd.hip", "energy.hip",
+           "pcg.hip"]
 
 
 def _compile(src):

@@ -33,8 +33,9 @@ EXPORTS = (
     "cipc_gpu_set_state", "cipc_gpu_get_positions", "cipc_gpu_set_direction", "cipc_gpu_active_set",
     "cipc_gpu_get_active_set", "cipc_gpu_set_active_set", "cipc_gpu_assemble", "cipc_gpu_get_gradient",
     "cipc_gpu_get_csr", "cipc_gpu_energy", "cipc_gpu_energy_terms", "cipc_gpu_step_bound",
-    "cipc_gpu_save_positions", "cipc_gpu_step_forward", "cipc_gpu_is_intersected", "cipc_gpu_newton_iteration",
-    "cipc_gpu_eval_snh", "cipc_gpu_eval_distance", "cipc_gpu_eval_make_pd", "cipc_gpu_eval_contact_pd",
+    "cipc_gpu_friction_update", "cipc_gpu_get_friction", "cipc_gpu_solve", "cipc_gpu_save_positions", "cipc_gpu_step_forward",
+    "cipc_gpu_is_intersected", "cipc_gpu_newton_iteration",
+    "cipc_gpu_eval_snh", "cipc_gpu_eval_cloth", "cipc_gpu_eval_hinge", "cipc_gpu_eval_distance", "cipc_gpu_eval_make_pd", "cipc_gpu_eval_contact_pd",
     "cipc_gpu_eval_accd",
     "cipc_gpu_eval_classify",
 )
@@ -47,7 +48,22 @@ class SceneDesc(C.Structure):
                 ("surf_verts", _u64p), ("surf_edges", _u64p),
                 ("dt", C.c_double), ("length_rate", C.c_double), ("volume_rate", C.c_double),
                 ("drag", C.c_double), ("hhat", C.c_double), ("fhat", C.c_double), ("friction", C.c_double),
-                ("ground_normal", C.c_double * 3), ("ground_d", C.c_double)]
+                ("ground_normal", C.c_double * 3), ("ground_d", C.c_double),
+                ("Tr", C.c_int), ("triangles", _ip), ("dm_tri_inv", _dp), ("areas", _dp),
+                ("stretch", C.c_double), ("shear", C.c_double), ("strain_rate", C.c_double),
+                ("plate_rigidity", C.c_double), ("Nb", C.c_int), ("bending", C.c_int), ("bend_verts", _ip),
+                ("bend_q", _dp), ("bend_rest", _dp), ("bend_weight", _dp)]
+
+
+def _fill_cloth(d, keep, scene):
+    """Cloth fields of the descriptor; the arrays are appended to ``keep``."""
+    from cipc import scenes
+    for name, val in scenes.cloth_arrays(scene).items():
+        if isinstance(val, np.ndarray):
+            keep.append(val)
+            setattr(d, name, val.ctypes.data_as(_ip if val.dtype == np.int32 else _dp))
+        else:
+            setattr(d, name, val)
 
 
 class IterationTiming(C.Structure):
@@ -55,6 +71,10 @@ class IterationTiming(C.Structure):
                 ("ccd_ms", C.c_double),
                 ("energy_ms", C.c_double), ("total_ms", C.c_double), ("alpha", C.c_double),
                 ("energy", C.c_double), ("nnz", C.c_int64), ("contacts", C.c_int64)]
+
+
+class SolveStats(C.Structure):
+    _fields_ = [("iterations", C.c_int), ("refinements", C.c_int), ("rel_residual", C.c_double), ("solve_ms", C.c_double)]
 
 
 class CipcError(RuntimeError):
@@ -92,12 +112,17 @@ def load(path: str = LIB_PATH):
         "cipc_gpu_get_csr": [C.c_void_p, _ip, _ip, _dp, _dp],
         "cipc_gpu_energy": [C.c_void_p, C.c_double, _dp],
         "cipc_gpu_energy_terms": [C.c_void_p, C.c_double, _dp],
+        "cipc_gpu_friction_update": [C.c_void_p, C.c_double],
+        "cipc_gpu_solve": [C.c_void_p, C.c_int, C.c_double, C.c_int, _dp, C.POINTER(SolveStats)],
+        "cipc_gpu_get_friction": [C.c_void_p, C.POINTER(C.c_int64), _dp, _dp, _dp, _dp],
         "cipc_gpu_step_bound": [C.c_void_p, _dp],
         "cipc_gpu_save_positions": [C.c_void_p],
         "cipc_gpu_step_forward": [C.c_void_p, C.c_double],
         "cipc_gpu_is_intersected": [C.c_void_p, _ip],
         "cipc_gpu_newton_iteration": [C.c_void_p, C.c_double, C.POINTER(IterationTiming)],
         "cipc_gpu_eval_snh": [C.c_int, _dp, C.c_double, C.c_double, _dp, _dp, _dp],
+        "cipc_gpu_eval_cloth": [C.c_int, _dp, C.c_double, C.c_double, C.c_double, _dp, _dp, _dp],
+        "cipc_gpu_eval_hinge": [C.c_int, _dp, _dp, _dp, C.c_double, _dp, _dp, _dp],
         "cipc_gpu_eval_distance": [C.c_int, C.c_int, _dp, _dp, _dp, _dp],
         "cipc_gpu_eval_make_pd": [C.c_int, C.c_int, _dp],
         "cipc_gpu_eval_contact_pd": [C.c_int, C.c_int, _dp, _dp],
@@ -163,6 +188,7 @@ def scene_desc(scene, xtilde=None):
     d.hhat, d.fhat, d.friction = scene.hhat, scene.fhat, m.friction
     d.ground_normal[0], d.ground_normal[1], d.ground_normal[2] = 0.0, 1.0, 0.0
     d.ground_d = -1.0
+    _fill_cloth(d, keep, scene)
     return d, keep
 
 
@@ -250,9 +276,32 @@ class GpuContext:
         return float(e[0])
 
     def energy_terms(self, kappa):
-        t = np.zeros(5)
+        """[elastic, inertia, ground barrier, self barrier, mollified barrier, friction,
+        cloth triangles, bending]"""
+        t = np.zeros(8)
         _check(load().cipc_gpu_energy_terms(self.h, kappa, _dptr(t)))
         return t
+
+    def solve(self, precision=0, rel_tol=1e-6, max_iter=10000):
+        """Newton linear solve of the last assembled system; returns (direction (V,3), stats).
+        precision 0: FP64 block-Jacobi PCG; 1: FP32 PCG inside FP64 refinement."""
+        p = np.zeros((self.V, 3))
+        st = SolveStats()
+        _check(load().cipc_gpu_solve(self.h, precision, rel_tol, max_iter, _dptr(p), C.byref(st)))
+        return p, dict(iterations=st.iterations, refinements=st.refinements, rel_residual=st.rel_residual,
+                       solve_ms=st.solve_ms)
+
+    def friction_update(self, kappa):
+        """Friction::initialize on the current active set and positions."""
+        _check(load().cipc_gpu_friction_update(self.h, kappa))
+
+    def friction_state(self):
+        n = np.zeros(2, np.int64)
+        ip = n.ctypes.data_as(C.POINTER(C.c_int64))
+        _check(load().cipc_gpu_get_friction(self.h, ip, None, None, None, None))
+        lam, coord, basis, env = np.zeros(n[0]), np.zeros(2 * n[0]), np.zeros(6 * n[0]), np.zeros(n[1])
+        _check(load().cipc_gpu_get_friction(self.h, ip, _dptr(lam), _dptr(coord), _dptr(basis), _dptr(env)))
+        return dict(lam=lam, coord=coord.reshape(-1, 2), basis=basis.reshape(-1, 2, 3), env_lam=env)
 
     def step_bound(self, p=None):
         if p is not None:
@@ -287,6 +336,27 @@ def eval_snh(F, u, r):
     P, H, e = np.zeros((n, 9)), np.zeros((n, 81)), np.zeros(n)
     _check(load().cipc_gpu_eval_snh(n, _dptr(Fc), u, r, _dptr(P), _dptr(H), _dptr(e)))
     return P.reshape(n, 3, 3).transpose(0, 2, 1), H.reshape(n, 9, 9), e
+
+
+def eval_cloth(F, stretch, shear, strain_rate):
+    """F: (n,3,2) -> PK1 (n,3,2), projected 6x6 Hessian over column-major vec F, energy density."""
+    F = np.asarray(F, np.float64)
+    n = F.shape[0]
+    Fc = _d(F.transpose(0, 2, 1).reshape(n, 6))
+    P, H, e = np.zeros((n, 6)), np.zeros((n, 36)), np.zeros(n)
+    _check(load().cipc_gpu_eval_cloth(n, _dptr(Fc), stretch, shear, strain_rate, _dptr(P), _dptr(H), _dptr(e)))
+    return P.reshape(n, 2, 3).transpose(0, 2, 1), H.reshape(n, 6, 6), e
+
+
+def eval_hinge(pts, rest, weight, plate):
+    """pts: (n,4,3) hinge points -> gradient (n,12), projected Hessian (n,12,12), energy (n,),
+    before the dt^2 factor."""
+    pts = _d(np.asarray(pts).reshape(-1, 12))
+    n = pts.shape[0]
+    r, w = _d(np.broadcast_to(rest, (n,))), _d(np.broadcast_to(weight, (n,)))
+    g, H, e = np.zeros((n, 12)), np.zeros((n, 144)), np.zeros(n)
+    _check(load().cipc_gpu_eval_hinge(n, _dptr(pts), _dptr(r), _dptr(w), plate, _dptr(g), _dptr(H), _dptr(e)))
+    return g, H.reshape(n, 12, 12), e
 
 
 def eval_distance(kind, pts):

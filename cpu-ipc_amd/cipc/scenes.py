@@ -69,6 +69,59 @@ class Material:
 
 
 @dataclasses.dataclass
+class ClothMaterial:
+    """Cloth knobs and their conversions (updateMaterial, Simulator.cpp:128-141;
+    defaults of Assets/scene/parameterSetting.txt).  ``plate_rigidity`` overrides the
+    conversion E_b t^3 / (12 (1 - nu^2)) when given."""
+
+    youngs: float = 1e4
+    shear_youngs: float = 1e1
+    bend_youngs: float = 1e4
+    poisson: float = 0.4
+    thickness: float = 1e-3
+    density: float = 200.0
+    strain_rate: float = 1e2
+    plate_rigidity_override: Optional[float] = None
+
+    @property
+    def stretch(self) -> float:
+        return self.youngs / (1 - self.poisson * self.poisson)
+
+    @property
+    def shear(self) -> float:
+        return self.shear_youngs / (2 * (1 + self.poisson)) / self.thickness
+
+    @property
+    def plate_rigidity(self) -> float:
+        if self.plate_rigidity_override is not None:
+            return self.plate_rigidity_override
+        return self.bend_youngs * self.thickness ** 3 / (12 * (1 - self.poisson * self.poisson))
+
+
+@dataclasses.dataclass
+class Cloth:
+    """Triangle (cloth) elements and bending elements of a scene (mesh3D::triangles,
+    DM_triangle_inverse, areas, quadBendingInfo / hingeBendingInfo; SimulationMesh.h:
+    17-35,79-113).  ``bending`` is 0 for the quadratic form (USE_QUADRATIC_BENDING) and 1
+    for the hinge-angle form; ``bend_q`` holds Q (row-major 4x4) of the quadratic form,
+    ``bend_rest`` / ``bend_weight`` the rest angle and l^2/(2A0+2A1) of the hinge form."""
+
+    triangles: np.ndarray    # (Tr,3) int32
+    dm_tri_inv: np.ndarray   # (Tr,4) column-major 2x2 Dm^-1
+    areas: np.ndarray        # (Tr,) area x thickness
+    material: ClothMaterial
+    bending: int
+    bend_verts: np.ndarray   # (Nb,4) int32
+    bend_q: np.ndarray       # (Nb,16)
+    bend_rest: np.ndarray    # (Nb,)
+    bend_weight: np.ndarray  # (Nb,)
+
+    @property
+    def Nb(self) -> int:
+        return int(self.bend_verts.shape[0])
+
+
+@dataclasses.dataclass
 class Scene:
     """Host arrays in the reference's layouts (``mesh3D``, SimulationMesh.h:37-129).
 
@@ -99,6 +152,11 @@ class Scene:
     avg_edge_len: float
     kappa: float = 0.0
     apply_gravity: bool = True
+    cloth: Optional["Cloth"] = None
+
+    @property
+    def Tr(self) -> int:
+        return 0 if self.cloth is None else int(self.cloth.triangles.shape[0])
 
     @property
     def V(self) -> int:
@@ -154,7 +212,7 @@ def random_rotation(rng: np.random.Generator) -> np.ndarray:
                      [2 * (x * z - y * w), 2 * (y * z + x * w), 1 - 2 * (x * x + y * y)]])
 
 
-def extract_surface(x: np.ndarray, tets: np.ndarray):
+def extract_surface(x: np.ndarray, tets: np.ndarray, triangles: Optional[np.ndarray] = None):
     """Boundary faces, surface vertices and surface edges of a tet mesh.
 
     Follows the reference's rules (SimulationMesh.cpp:364-446): a face seen once is on
@@ -164,9 +222,12 @@ def extract_surface(x: np.ndarray, tets: np.ndarray):
     directed edge is skipped if its reverse is already present.  The face order itself
     is the ascending order of sorted face keys (the reference's order comes from an
     unordered_map and is not reproducible; the device path consumes these arrays as
-    given).
+    given).  Cloth triangles follow the tet faces as (v0, v1, v2, 0)
+    (SimulationMesh.cpp:418-420).
     """
     T = tets.shape[0]
+    if T == 0:
+        tets = np.zeros((0, 4), np.int64)
     local = np.array([[0, 1, 2], [1, 2, 3], [2, 3, 0], [3, 0, 1]])
     opp = np.array([3, 0, 1, 2])
     tri = tets[:, local].reshape(-1, 3)
@@ -188,21 +249,121 @@ def extract_surface(x: np.ndarray, tets: np.ndarray):
             faces.append((v0, v1, v2, tet_id[f]))
         else:
             faces.append((v0, v2, v1, tet_id[f]))
+    if triangles is not None and len(triangles):
+        faces.extend((int(a), int(b), int(c), 0) for a, b, c in triangles)
     faces = np.asarray(faces, dtype=np.int64).reshape(-1, 4)
-    seen = np.zeros(x.shape[0], bool)
-    surf_verts = []
-    for f in faces:
-        for v in f[:3]:
-            if not seen[v]:
-                seen[v] = True
-                surf_verts.append(v)
-    edges = set()
-    for f in faces:
-        for a, b in ((f[0], f[1]), (f[1], f[2]), (f[2], f[0])):
-            if (b, a) not in edges:
-                edges.add((a, b))
-    surf_edges = np.asarray(sorted(edges), dtype=np.int64).reshape(-1, 2)
-    return faces, np.asarray(surf_verts, dtype=np.int64), surf_edges
+    # surfVerts: first appearance over the face list
+    fv = faces[:, :3].reshape(-1)
+    uv, first = np.unique(fv, return_index=True)
+    surf_verts = uv[np.argsort(first, kind="stable")]
+    # surfEdges: an undirected edge keeps the direction of its first appearance (a later
+    # reversed copy finds its reverse in the set; a later equal copy is a no-op insert)
+    a = faces[:, [0, 1, 2]].reshape(-1)
+    b = faces[:, [1, 2, 0]].reshape(-1)
+    key = np.stack([np.minimum(a, b), np.maximum(a, b)], axis=1)
+    _, fi = np.unique(key, axis=0, return_index=True)
+    de = np.stack([a[fi], b[fi]], axis=1)
+    surf_edges = de[np.lexsort((de[:, 1], de[:, 0]))].reshape(-1, 2)
+    return faces, surf_verts.astype(np.int64), surf_edges.astype(np.int64)
+
+
+def triangle_rest(x: np.ndarray, tris: np.ndarray, thickness: float):
+    """Per-triangle Dm^-1 (column-major 2x2) and area x thickness (initMesh3D triangle
+    loop, Elasticity.cpp:1085-1113; calculateDms2D_double, Elasticity.cpp:610-640): the
+    triangle is rotated so that its normal maps to +y (Rodrigues' formula
+    I + [v]x + [v]x^2 / (1 + cos)) and its (x, z) coordinates are the 2D rest shape."""
+    v01 = x[tris[:, 1]] - x[tris[:, 0]]
+    v02 = x[tris[:, 2]] - x[tris[:, 0]]
+    cr = np.cross(v01, v02)
+    area = 0.5 * np.linalg.norm(cr, axis=1)
+    nrm = cr / np.linalg.norm(cr, axis=1, keepdims=True)
+    target = np.array([0.0, 1.0, 0.0])
+    vec = np.cross(nrm, target)
+    cos = nrm @ target
+    K = np.zeros((len(tris), 3, 3))
+    K[:, 0, 1], K[:, 0, 2] = -vec[:, 2], vec[:, 1]
+    K[:, 1, 0], K[:, 1, 2] = vec[:, 2], -vec[:, 0]
+    K[:, 2, 0], K[:, 2, 1] = -vec[:, 1], vec[:, 0]
+    R = np.eye(3)[None] + K + (K @ K) / (1 + cos)[:, None, None]
+    uv = [np.einsum("tij,tj->ti", R, x[tris[:, k]])[:, [0, 2]] for k in range(3)]
+    dm = np.stack([uv[1] - uv[0], uv[2] - uv[0]], axis=2)       # columns are edges
+    inv = np.linalg.inv(dm)
+    return inv.transpose(0, 2, 1).reshape(-1, 4).copy(), area * thickness
+
+
+def prepare_quadratic_bending(x: np.ndarray, tris: np.ndarray):
+    """prepareQuadraticBending (Simulator.cpp:48-109): one element per later appearance of
+    an edge over the triangle list (edge i of a triangle runs from corner i to corner
+    i+1), in the order of those appearances, with vertices (e0, e1, outer vertex of the
+    edge's first triangle, outer vertex of this one) and the constant
+    Q = 3 / (A0 + A1) K K^T from the cotangent weights."""
+    tris = np.asarray(tris, np.int64).reshape(-1, 3)
+    a = tris.reshape(-1)
+    b = tris[:, [1, 2, 0]].reshape(-1)
+    other = tris[:, [2, 0, 1]].reshape(-1)
+    lo, hi = np.minimum(a, b), np.maximum(a, b)
+    order = np.lexsort((np.arange(len(a)), hi, lo))           # by edge, then appearance
+    lo_s, hi_s = lo[order], hi[order]
+    new = np.ones(len(order), bool)
+    new[1:] = (lo_s[1:] != lo_s[:-1]) | (hi_s[1:] != hi_s[:-1])
+    first = np.maximum.accumulate(np.where(new, np.arange(len(order)), 0))
+    later = np.nonzero(~new)[0]
+    pos = order[later]                                        # appearance index of the later occurrence
+    o1 = other[order[first[later]]]
+    o2 = other[pos]
+    e0, e1 = lo[pos], hi[pos]
+    srt = np.argsort(pos, kind="stable")
+    e0, e1, o1, o2 = e0[srt], e1[srt], o1[srt], o2[srt]
+    v0, v1, v2, v3 = x[o1], x[e0], x[e1], x[o2]
+    E0, E1, E2, E3, E4 = v2 - v1, v0 - v1, v3 - v1, v0 - v2, v3 - v2
+    nrm = lambda u: np.linalg.norm(u, axis=1)
+    a0 = 0.5 * nrm(np.cross(E0, E1))
+    a1 = 0.5 * nrm(np.cross(E0, E2))
+    cot = lambda u, v: np.einsum("ij,ij->i", u, v) / nrm(np.cross(u, v))
+    c01, c02, c03, c04 = cot(E0, E1), cot(E0, E2), cot(-E0, E3), cot(-E0, E4)
+    K = np.stack([c03 + c04, c01 + c02, -c01 - c03, -c02 - c04], axis=1)
+    Q = (3.0 / (a0 + a1))[:, None, None] * (K[:, :, None] * K[:, None, :])
+    verts = np.stack([e0, e1, o1, o2], axis=1).astype(np.int32)
+    return verts.reshape(-1, 4), Q.reshape(-1, 16)
+
+
+def hinge_angle(q0, q1, q2, q3) -> float:
+    """edgeTheta's angle (Elasticity.cpp:12-40,69-85): signed dihedral angle of the
+    triangles (q0, q1, q2) and (q1, q0, q3) about the axis q1 - q0."""
+    n0 = np.cross(q0 - q2, q1 - q2)
+    n1 = np.cross(q1 - q3, q0 - q3)
+    axis = q1 - q0
+    return 2.0 * math.atan2(np.dot(np.cross(n0, n1), axis) / np.linalg.norm(axis),
+                            np.dot(n0, n1) + np.linalg.norm(n0) * np.linalg.norm(n1))
+
+
+def prepare_hinge_bending(x: np.ndarray, tris: np.ndarray):
+    """The hinge list of load_triangleMesh (SimulationMesh.cpp:659-713: interior edges in
+    ascending (min, max) order, opposite vertices in triangle order, the three edges of a
+    triangle (a,b,c) taken as (a,b), (b,c), (a,c)) and makeHingeBendingInfo
+    (HingeBending.cpp:47-77: rest angle and l^2 / (2A0 + 2A1))."""
+    tris = np.asarray(tris, np.int64).reshape(-1, 3)
+    p = tris[:, [0, 1, 0]].reshape(-1)
+    q = tris[:, [1, 2, 2]].reshape(-1)
+    o = tris[:, [2, 0, 1]].reshape(-1)
+    lo, hi = np.minimum(p, q), np.maximum(p, q)
+    order = np.lexsort((np.arange(len(p)), hi, lo))
+    lo_s, hi_s, o_s = lo[order], hi[order], o[order]
+    start = np.ones(len(order), bool)
+    start[1:] = (lo_s[1:] != lo_s[:-1]) | (hi_s[1:] != hi_s[:-1])
+    idx = np.nonzero(start)[0]
+    cnt = np.diff(np.append(idx, len(order)))
+    two = idx[cnt == 2]
+    verts = np.stack([lo_s[two], hi_s[two], o_s[two], o_s[two + 1]], axis=1)
+    q0, q1, q2, q3 = (x[verts[:, k]] for k in range(4))
+    n0 = np.cross(q0 - q2, q1 - q2)
+    n1 = np.cross(q1 - q3, q0 - q3)
+    axis = q1 - q0
+    ln = np.linalg.norm(axis, axis=1)
+    rest = 2.0 * np.arctan2(np.einsum("ij,ij->i", np.cross(n0, n1), axis) / ln,
+                            np.einsum("ij,ij->i", n0, n1) + np.linalg.norm(n0, axis=1) * np.linalg.norm(n1, axis=1))
+    s2 = np.linalg.norm(np.cross(axis, q2 - q0), axis=1) + np.linalg.norm(np.cross(axis, q3 - q0), axis=1)
+    return verts.astype(np.int32), rest, ln * ln / s2
 
 
 def dm_inverse_colmajor(x: np.ndarray, tets: np.ndarray):
@@ -218,14 +379,36 @@ def dm_inverse_colmajor(x: np.ndarray, tets: np.ndarray):
 
 def finalize(name: str, x: np.ndarray, tets: np.ndarray, mat: Material,
              boundary: Optional[np.ndarray] = None,
-             velocities: Optional[np.ndarray] = None) -> Scene:
-    """initMesh3D + getSurface + the scalar set-up of FEMSimulator::buildModels."""
-    tets = np.ascontiguousarray(tets, dtype=np.int64)
+             velocities: Optional[np.ndarray] = None,
+             triangles: Optional[np.ndarray] = None,
+             cloth_mat: Optional[ClothMaterial] = None,
+             bending: int = 0) -> Scene:
+    """initMesh3D + getSurface + the scalar set-up of FEMSimulator::buildModels; with
+    ``triangles`` also the cloth part of initMesh3D (masses area x thickness x density / 3
+    per corner) and the bending preparation (quadratic for ``bending`` 0, hinge for 1)."""
+    tets = np.ascontiguousarray(tets, dtype=np.int64).reshape(-1, 4)
     V = x.shape[0]
-    dm_inv, vol = dm_inverse_colmajor(x, tets)
+    if len(tets):
+        dm_inv, vol = dm_inverse_colmajor(x, tets)
+    else:
+        dm_inv, vol = np.zeros((0, 9)), np.zeros(0)
     masses = np.zeros(V)
     np.add.at(masses, tets.reshape(-1), np.repeat(vol * mat.density / 4.0, 4))
-    faces, surf_verts, surf_edges = extract_surface(x, tets)
+    cloth = None
+    if triangles is not None and len(triangles):
+        triangles = np.ascontiguousarray(triangles, dtype=np.int64)
+        cm = cloth_mat or ClothMaterial()
+        dmt, areas = triangle_rest(x, triangles, cm.thickness)
+        np.add.at(masses, triangles.reshape(-1), np.repeat(areas * cm.density / 3.0, 3))
+        if bending == 0:
+            bv, bq = prepare_quadratic_bending(x, triangles)
+            br, bw = np.zeros(len(bv)), np.zeros(len(bv))
+        else:
+            bv, br, bw = prepare_hinge_bending(x, triangles)
+            bq = np.zeros((len(bv), 16))
+        cloth = Cloth(triangles=triangles.astype(np.int32), dm_tri_inv=dmt, areas=areas, material=cm,
+                      bending=bending, bend_verts=bv, bend_q=bq, bend_rest=br, bend_weight=bw)
+    faces, surf_verts, surf_edges = extract_surface(x, tets, triangles)
     lo, hi = x.min(axis=0), x.max(axis=0)
     diag2 = float(np.sum((hi - lo) ** 2))
     elen = np.linalg.norm(x[surf_edges[:, 0]] - x[surf_edges[:, 1]], axis=1).sum()
@@ -241,7 +424,8 @@ def finalize(name: str, x: np.ndarray, tets: np.ndarray, mat: Material,
                  surf_edges=surf_edges.astype(np.int32), material=mat,
                  bbox_diag2=diag2, hhat=mat.rel_dhat ** 2 * diag2,
                  fhat=mat.fhat_rel * diag2, dtol=mat.dtol_rel * diag2,
-                 mean_mass=float(masses.sum() / V), avg_edge_len=float(elen / (len(surf_edges) * 3)))
+                 mean_mass=float(masses.sum() / V), avg_edge_len=float(elen / (len(surf_edges) * 3)),
+                 cloth=cloth)
 
 
 GROUND_Y = -1.0  # Ground plane y = -1, normal (0,1,0) (CollisionBroadPhase.cpp:1067-1069)
@@ -330,3 +514,119 @@ def newton_probe_state(scene: Scene, seed: int = 7, scale: float = 1.0):
     p = p * (1.0 + rng.uniform(-0.05, 0.05, p.shape))
     p[scene.boundary != 0] = 0.0
     return xt, np.ascontiguousarray(p)
+
+
+CLOTH_FIELDS = (("Tr", "int"), ("triangles", "int*"), ("dm_tri_inv", "double*"), ("areas", "double*"),
+                ("stretch", "double"), ("shear", "double"), ("strain_rate", "double"), ("plate_rigidity", "double"),
+                ("Nb", "int"), ("bending", "int"), ("bend_verts", "int*"), ("bend_q", "double*"),
+                ("bend_rest", "double*"), ("bend_weight", "double*"))
+
+
+def cloth_arrays(scene: Scene) -> dict:
+    """The cloth part of the scene descriptor (names of CLOTH_FIELDS), arrays as
+    contiguous int32 / float64 (never empty, so that every pointer is valid)."""
+    c = scene.cloth
+    i32 = lambda a: np.ascontiguousarray(np.asarray(a).reshape(-1) if np.size(a) else np.zeros(1), dtype=np.int32)
+    f64 = lambda a: np.ascontiguousarray(np.asarray(a).reshape(-1) if np.size(a) else np.zeros(1), dtype=np.float64)
+    if c is None:
+        z = np.zeros(1)
+        return dict(Tr=0, triangles=i32(z), dm_tri_inv=f64(z), areas=f64(z), stretch=0.0, shear=0.0,
+                    strain_rate=0.0, plate_rigidity=0.0, Nb=0, bending=0, bend_verts=i32(z), bend_q=f64(z),
+                    bend_rest=f64(z), bend_weight=f64(z))
+    m = c.material
+    return dict(Tr=int(len(c.triangles)), triangles=i32(c.triangles), dm_tri_inv=f64(c.dm_tri_inv),
+                areas=f64(c.areas), stretch=m.stretch, shear=m.shear, strain_rate=m.strain_rate,
+                plate_rigidity=m.plate_rigidity, Nb=c.Nb, bending=int(c.bending), bend_verts=i32(c.bend_verts),
+                bend_q=f64(c.bend_q), bend_rest=f64(c.bend_rest), bend_weight=f64(c.bend_weight))
+
+
+def cloth_grid(n: int, size: float):
+    """(n+1)^2 vertices of a ``size`` square in the xz-plane centred on the origin, and
+    2 n^2 triangles, each oriented with its normal along +y (the orientation
+    calculateDms2D_double's rotation to +y is defined for)."""
+    m = n + 1
+    i, k = np.meshgrid(np.arange(m), np.arange(m), indexing="ij")
+    x = np.stack([i.reshape(-1), np.zeros(m * m), k.reshape(-1)], axis=1).astype(np.float64)
+    x = (x / n - np.array([0.5, 0.0, 0.5])) * np.array([size, 0.0, size])
+    vid = lambda a, b: a * m + b          # vertex (x index a, z index b)
+    tris = []
+    for a in range(n):
+        for b in range(n):
+            p00, p01, p10, p11 = vid(a, b), vid(a, b + 1), vid(a + 1, b), vid(a + 1, b + 1)
+            if (a + b) % 2 == 0:
+                tris += [(p00, p01, p11), (p00, p11, p10)]
+            else:
+                tris += [(p00, p01, p10), (p01, p11, p10)]
+    tris = np.asarray(tris, np.int64)
+    nrm = np.cross(x[tris[:, 1]] - x[tris[:, 0]], x[tris[:, 2]] - x[tris[:, 0]])
+    assert np.all(nrm[:, 1] > 0)
+    return x, tris
+
+
+def lattice_ball(cells: int, radius: float):
+    """A ball of ``cells``^3 lattice cells (5 tets each) obtained by mapping the cube
+    [-1, 1]^3 radially onto the ball (x -> x |x|_inf / |x|_2 * radius)."""
+    x, tets = lattice(cells, 2.0 / cells)
+    x = x - 1.0
+    n2 = np.linalg.norm(x, axis=1)
+    ninf = np.abs(x).max(axis=1)
+    scale = np.where(n2 > 0, ninf / np.where(n2 > 0, n2, 1.0), 0.0)
+    x = x * scale[:, None] * radius
+    d = x[tets[:, 1:]] - x[tets[:, :1]]
+    neg = np.linalg.det(d) < 0
+    tets[neg, 1], tets[neg, 2] = tets[neg, 2].copy(), tets[neg, 1].copy()
+    return x, tets
+
+
+def cloth_on_sphere(n: int = 256, size: float = 1.0, sphere_cells: int = 10, radius: float = 0.3,
+                    bending: int = 0, fold: Optional[int] = None, seed: int = 0,
+                    layer_gap: float = 0.6e-3, name: Optional[str] = None) -> Scene:
+    """A ``size`` square cloth of ``n`` x ``n`` quads (2 n^2 triangles) above a fixed
+    ball collider of ``sphere_cells``^3 cells (5 sphere_cells^3 tets, boundary type 2)
+    on the ground.  Cloth: E = 1e4, thickness 1e-3, density 200, plate rigidity 1e-4;
+    contact: dHat_rel 1e-3, mu 0.2.
+
+    The rest shape is the flat square.  The returned state (x) lies ``layer_gap`` above
+    the ball's top; with ``fold`` = r the cloth rows past z-row r are folded back over
+    the rest, ``layer_gap`` above it, shifted by half a cell in x, so that the two layers
+    are in contact everywhere they overlap (a frozen, self-contact-heavy state of a
+    cloth dropped on the ball).  In-plane positions carry a seeded +-15% (of a cell)
+    jitter."""
+    rng = np.random.default_rng(seed)
+    mat = Material(youngs=1e5, friction=0.2)
+    cm = ClothMaterial(youngs=1e4, thickness=1e-3, plate_rigidity_override=1e-4)
+    xc, tris = cloth_grid(n, size)
+    xb, tb = lattice_ball(sphere_cells, radius)
+    centre_y = GROUND_Y + radius + 0.2
+    xb = xb + np.array([0.0, centre_y, 0.0])
+    top = xb[:, 1].max()
+    Vb = xb.shape[0]
+    x_rest = np.concatenate([xb, xc + np.array([0.0, top + layer_gap, 0.0])])
+    tris = tris + Vb
+    boundary = np.zeros(x_rest.shape[0], np.int32)
+    boundary[:Vb] = 2
+    s = finalize(name or f"cloth{n}_sphere", x_rest, tb, mat, boundary=boundary, triangles=tris,
+                 cloth_mat=cm, bending=bending)
+    # the state: in-plane jitter, then the fold
+    cell = size / n
+    m = n + 1
+    x = s.x.copy()
+    ci = np.arange(m * m)
+    a, b = ci // m, ci % m            # x index, z index
+    jit = rng.uniform(-0.15, 0.15, (m * m, 2)) * cell
+    xcl = x[Vb:]
+    xcl[:, 0] += jit[:, 0]
+    if fold is not None:
+        up = b > fold
+        keep_z = b == fold + 1        # the crease row stays over row `fold`
+        zi = np.where(up, 2 * fold + 1 - b, b)
+        zj = np.where(keep_z, fold, zi)
+        xcl[:, 2] = -0.5 * size + zj * cell + np.where(keep_z | (b == fold), 0.0, jit[:, 1])
+        xcl[up, 0] += 0.5 * cell
+        xcl[up, 1] += layer_gap
+    else:
+        xcl[:, 2] += jit[:, 1]
+    x[Vb:] = xcl
+    s.x = x
+    s.V_prev = x.copy()
+    return s

@@ -196,12 +196,40 @@ __global__ void internal_boxes(int n, const int* __restrict__ first, const int* 
     nodes[i].hi = acc.hi;
 }
 
+// Four-wide records from the refitted binary nodes.
+__global__ void wide_records(int n, const Node* __restrict__ nodes, Node4* __restrict__ wide) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n - 1) return;
+    int ch[4], m = 0;
+    const int c2[2] = {nodes[i].left, nodes[i].right};
+    for (int k = 0; k < 2; ++k) {
+        const int c = c2[k];
+        if (c < n - 1) { ch[m++] = nodes[c].left; ch[m++] = nodes[c].right; }
+        else ch[m++] = c;
+    }
+    Node4 r;
+    for (int k = 0; k < 4; ++k) {
+        if (k < m) {
+            const Node& x = nodes[ch[k]];
+            r.lo[k][0] = __double2float_rd(x.lo.x); r.lo[k][1] = __double2float_rd(x.lo.y); r.lo[k][2] = __double2float_rd(x.lo.z);
+            r.hi[k][0] = __double2float_ru(x.hi.x); r.hi[k][1] = __double2float_ru(x.hi.y); r.hi[k][2] = __double2float_ru(x.hi.z);
+            r.child[k] = ch[k] >= n - 1 ? -(x.left + 1) : ch[k];
+        } else {
+            r.lo[k][0] = r.lo[k][1] = r.lo[k][2] = __int_as_float(0x7f800000);
+            r.hi[k][0] = r.hi[k][1] = r.hi[k][2] = __int_as_float(0xff800000);
+            r.child[k] = kNoChild;
+        }
+        r.pad[k] = 0;
+    }
+    wide[i] = r;
+}
+
 // Topology from the current primitive boxes.
 void build_topology(Context& c, LBVH& t, int n) {
     t.n = n;
     t.age = 0;
     const int nodes = 2 * n - 1;
-    t.nodes.resize(nodes);
+    t.nodes.resize(nodes); t.wide.resize(std::max(n - 1, 1));
     t.range_first.resize(std::max(n - 1, 1)); t.range_last.resize(std::max(n - 1, 1)); t.split_right.resize(std::max(n - 1, 1));
     t.codes.resize(n); t.codes_sorted.resize(n); t.order.resize(2 * (size_t)n); t.keys_sorted.resize(n);
     // level l of the box reduction holds ceil(n / 2^l) boxes; level 0 is the leaves
@@ -259,6 +287,8 @@ void refit(Context& c, LBVH& t) {
         hipLaunchKernelGGL(internal_boxes, dim3(blocks_for(n - 1, 256)), dim3(256), 0, c.stream, n, t.range_first.p, t.range_last.p,
                            t.L, t.seg.p, t.nodes.p);
         CIPC_CHECK_LAUNCH();
+        hipLaunchKernelGGL(wide_records, dim3(blocks_for(n - 1, 256)), dim3(256), 0, c.stream, n, t.nodes.p, t.wide.p);
+        CIPC_CHECK_LAUNCH();
     }
     ++t.age;
 }
@@ -269,68 +299,72 @@ void build_tree(Context& c, LBVH& t, int n) {
     refit(c, t);
 }
 
-// Primitive boxes: static (CPUBroadPhaseBVH::build, LBVH.cpp:284-305) or swept with step
-// alpha along -p (buildSwept, LBVH.cpp:307-339).
+// Primitive boxes: static (CPUBroadPhaseBVH::build, LBVH.cpp:284-305) or swept along -p
+// with the step whose bits are at *abits (buildSwept, LBVH.cpp:307-339).  The step is read
+// on the device, so building the swept trees needs no host round trip.
+__device__ inline double step_of(const unsigned long long* abits) { return __longlong_as_double((long long)*abits); }
 __global__ void face_boxes(int F, const int4* __restrict__ faces, const double* __restrict__ x, const double* __restrict__ p,
-                           double a, int swept, Box* __restrict__ out) {
+                           const unsigned long long* __restrict__ abits, Box* __restrict__ out) {
     const int f = blockIdx.x * blockDim.x + threadIdx.x;
     if (f >= F) return;
     const int4 v = faces[f];
-    if (!swept) { out[f] = box_tri(ld3(x, v.x), ld3(x, v.y), ld3(x, v.z)); return; }
+    if (abits == nullptr) { out[f] = box_tri(ld3(x, v.x), ld3(x, v.y), ld3(x, v.z)); return; }
+    const double a = step_of(abits);
     Box b = swept_pt(ld3(x, v.x), ld3(p, v.x), a);
     b = box_union(b, swept_pt(ld3(x, v.y), ld3(p, v.y), a));
     b = box_union(b, swept_pt(ld3(x, v.z), ld3(p, v.z), a));
     out[f] = b;
 }
 __global__ void edge_boxes(int E, const int2* __restrict__ se, const double* __restrict__ x, const double* __restrict__ p,
-                           double a, int swept, Box* __restrict__ out) {
+                           const unsigned long long* __restrict__ abits, Box* __restrict__ out) {
     const int e = blockIdx.x * blockDim.x + threadIdx.x;
     if (e >= E) return;
     const int2 v = se[e];
-    if (!swept) { out[e] = box_edge(ld3(x, v.x), ld3(x, v.y)); return; }
+    if (abits == nullptr) { out[e] = box_edge(ld3(x, v.x), ld3(x, v.y)); return; }
+    const double a = step_of(abits);
     out[e] = box_union(swept_pt(ld3(x, v.x), ld3(p, v.x), a), swept_pt(ld3(x, v.y), ld3(p, v.y), a));
 }
 __global__ void point_boxes(int Vs, const int* __restrict__ sv, const double* __restrict__ x, const double* __restrict__ p,
-                            double a, Box* __restrict__ out) {
+                            const unsigned long long* __restrict__ abits, Box* __restrict__ out) {
     const int s = blockIdx.x * blockDim.x + threadIdx.x;
     if (s >= Vs) return;
-    out[s] = swept_pt(ld3(x, sv[s]), ld3(p, sv[s]), a);
+    out[s] = swept_pt(ld3(x, sv[s]), ld3(p, sv[s]), step_of(abits));
 }
 
 void build_static_trees(Context& c) {
     Scene& s = c.sc;
     c.face_tree.prim.resize(std::max(s.F, 1));
     c.edge_tree.prim.resize(std::max(s.Es, 1));
+    const unsigned long long* none = nullptr;
     if (s.F > 0)
-        hipLaunchKernelGGL(face_boxes, dim3(blocks_for(s.F, 256)), dim3(256), 0, c.stream, s.F, s.faces.p, s.x.p, s.x.p, 0.0, 0,
+        hipLaunchKernelGGL(face_boxes, dim3(blocks_for(s.F, 256)), dim3(256), 0, c.stream, s.F, s.faces.p, s.x.p, s.x.p, none,
                            c.face_tree.prim.p);
     if (s.Es > 0)
-        hipLaunchKernelGGL(edge_boxes, dim3(blocks_for(s.Es, 256)), dim3(256), 0, c.stream, s.Es, s.se.p, s.x.p, s.x.p, 0.0, 0,
+        hipLaunchKernelGGL(edge_boxes, dim3(blocks_for(s.Es, 256)), dim3(256), 0, c.stream, s.Es, s.se.p, s.x.p, s.x.p, none,
                            c.edge_tree.prim.p);
     CIPC_CHECK_LAUNCH();
     build_tree(c, c.face_tree, s.F);
     build_tree(c, c.edge_tree, s.Es);
     c.face_tree.swept = c.edge_tree.swept = false;
 }
-void build_swept_trees(Context& c, double alpha) {
+void build_swept_trees(Context& c, const unsigned long long* abits) {
     Scene& s = c.sc;
     c.face_tree.prim.resize(std::max(s.F, 1));
     c.edge_tree.prim.resize(std::max(s.Es, 1));
     c.swept_point_box.resize(std::max(s.Vs, 1));
     if (s.F > 0)
-        hipLaunchKernelGGL(face_boxes, dim3(blocks_for(s.F, 256)), dim3(256), 0, c.stream, s.F, s.faces.p, s.x.p, s.dir.p, alpha, 1,
+        hipLaunchKernelGGL(face_boxes, dim3(blocks_for(s.F, 256)), dim3(256), 0, c.stream, s.F, s.faces.p, s.x.p, s.dir.p, abits,
                            c.face_tree.prim.p);
     if (s.Es > 0)
-        hipLaunchKernelGGL(edge_boxes, dim3(blocks_for(s.Es, 256)), dim3(256), 0, c.stream, s.Es, s.se.p, s.x.p, s.dir.p, alpha, 1,
+        hipLaunchKernelGGL(edge_boxes, dim3(blocks_for(s.Es, 256)), dim3(256), 0, c.stream, s.Es, s.se.p, s.x.p, s.dir.p, abits,
                            c.edge_tree.prim.p);
     if (s.Vs > 0)
-        hipLaunchKernelGGL(point_boxes, dim3(blocks_for(s.Vs, 256)), dim3(256), 0, c.stream, s.Vs, s.sv.p, s.x.p, s.dir.p, alpha,
+        hipLaunchKernelGGL(point_boxes, dim3(blocks_for(s.Vs, 256)), dim3(256), 0, c.stream, s.Vs, s.sv.p, s.x.p, s.dir.p, abits,
                            c.swept_point_box.p);
     CIPC_CHECK_LAUNCH();
     build_tree(c, c.face_tree, s.F);
     build_tree(c, c.edge_tree, s.Es);
     c.face_tree.swept = c.edge_tree.swept = true;
-    c.face_tree.alpha = c.edge_tree.alpha = alpha;
 }
 
 // ---------------------------------------------------------------------------------
@@ -338,15 +372,6 @@ void build_swept_trees(Context& c, double alpha) {
 // ---------------------------------------------------------------------------------
 // candidate categories; kDrop: a box hit that the reference's candidate filters discard
 enum : int { kNone = 0, kSelfPT = 1, kMap = 2, kSelfEE = 3, kMolPE = 4, kMolEE = 5, kDrop = 6 };
-
-__device__ inline void sort_segment(int* a, int n) {   // insertion sort of a short id list
-    for (int i = 1; i < n; ++i) {
-        const int v = a[i];
-        int j = i - 1;
-        while (j >= 0 && a[j] > v) { a[j + 1] = a[j]; --j; }
-        a[j + 1] = v;
-    }
-}
 
 // Candidate collection.
 //   A (traverse): each query is split over kParts lanes, one subtree each; a lane records
@@ -375,20 +400,22 @@ __global__ __launch_bounds__(64) void pt_traverse(int Vs, const int* __restrict_
     const int t = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
     const bool active = t < Vs * kParts;
     const int s = active ? perm[t / kParts] : 0, part = t % kParts;
+    __shared__ int stk[kStack * 64];
     int n = 0;
     if (active) {
         const Box pb = box_pt(ld3(x, sv[s]));
         int* b = buf + ((size_t)s * kParts + part) * kPartCap;
-        lbvh_visit(tv, part, pb, sq, [&](int f) { if (n < kPartCap) b[n] = f; ++n; });
+        part_walk(tv, part, pb, sq, stk + threadIdx.x, [&](int f) { if (n < kPartCap) b[n] = f; ++n; }, [&] { n = 0; });
         cntp[s * kParts + part] = n;
     }
     const int tot = sum_over_parts(n);
     if (active && part == 0) cnt[s] = tot;
 }
 
-__global__ void pt_compact(int Vs, const int* __restrict__ perm, const int* __restrict__ sv, const double* __restrict__ x, TreeView tv,
+__global__ __launch_bounds__(64) void pt_compact(int Vs, const int* __restrict__ perm, const int* __restrict__ sv, const double* __restrict__ x, TreeView tv,
                            double sq, const int* __restrict__ off, const int* __restrict__ cntp, const int* __restrict__ buf,
-                           int* __restrict__ cand, int* __restrict__ own) {
+                           int* __restrict__ candU, int* __restrict__ own) {
+    __shared__ int stk[kStack * 64];
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= Vs) return;
     const int s = perm[t];
@@ -397,27 +424,37 @@ __global__ void pt_compact(int Vs, const int* __restrict__ perm, const int* __re
     for (int part = 0; part < kParts; ++part) {
         const int c = cntp[s * kParts + part];
         if (c <= kPartCap) {
-            for (int k = 0; k < c; ++k) cand[pos + k] = buf[((size_t)s * kParts + part) * kPartCap + k];
+            for (int k = 0; k < c; ++k) candU[pos + k] = buf[((size_t)s * kParts + part) * kPartCap + k];
         } else {
             const Box pb = box_pt(ld3(x, sv[s]));
             int m = 0;
-            lbvh_visit(tv, part, pb, sq, [&](int f) { if (m < c) cand[pos + m] = f; ++m; });
+            part_walk(tv, part, pb, sq, stk + threadIdx.x, [&](int f) { if (m < c) candU[pos + m] = f; ++m; }, [&] { m = 0; });
         }
         pos += c;
     }
-    const int n = pos - o;
-    sort_segment(cand + o, n);
-    for (int k = 0; k < n; ++k) own[o + k] = s;
+    for (int k = o; k < pos; ++k) own[k] = s;
 }
 
 // Point-triangle hits: filters, then classification (CollisionBroadPhase.cpp:765-857).
-__global__ void pt_classify(int n, const int* __restrict__ own, const int* __restrict__ sv, const double* __restrict__ x,
-                            const int* __restrict__ bt, const int4* __restrict__ faces, const int* __restrict__ cand, double hhat,
-                            int4* __restrict__ rec, int* __restrict__ cat) {
-    const int k = blockIdx.x * blockDim.x + threadIdx.x;
-    if (k >= n) return;
-    const int vI = sv[own[k]];
-    const int4 fv = faces[cand[k]];
+__global__ void pt_classify(int n, int F, const int* __restrict__ own, const int* __restrict__ off, const int* __restrict__ sv,
+                            const double* __restrict__ x, const int* __restrict__ bt, const int4* __restrict__ faces,
+                            const int* __restrict__ candU, int* __restrict__ cand, double hhat,
+                            double sq, int4* __restrict__ rec, int* __restrict__ cat) {
+    const int u = blockIdx.x * blockDim.x + threadIdx.x;
+    if (u >= n) return;
+    // sorted position: the hit's rank among its query's hits (all distinct)
+    const int q = own[u], o = off[q], e = off[q + 1];
+    const int f = candU[u];
+    int k = o;
+    for (int j = o; j < e; ++j) k += candU[j] < f;
+    cand[k] = f;
+    if ((unsigned)f >= (unsigned)F || k >= e) {   // cannot happen for a consistent walk; never index with it
+        rec[k < e ? k : o] = make_int4(0, 0, 0, 0);
+        cat[k < e ? k : o] = kDrop;
+        return;
+    }
+    const int vI = sv[q];
+    const int4 fv = faces[f];
     if ((is_collider(bt[vI]) && is_collider(bt[fv.x]) && is_collider(bt[fv.y]) && is_collider(bt[fv.z])) || vI == fv.x ||
         vI == fv.y || vI == fv.z) {
         rec[k] = make_int4(0, 0, 0, 0);
@@ -427,6 +464,11 @@ __global__ void pt_classify(int n, const int* __restrict__ own, const int* __res
     const d3 P = ld3(x, vI);
     {
         const d3 A = ld3(x, fv.x), B = ld3(x, fv.y), C = ld3(x, fv.z);
+        if (!overlaps(box_pt(P), box_tri(A, B, C), sq)) {   // the walk's float boxes admit a superset
+            rec[k] = make_int4(0, 0, 0, 0);
+            cat[k] = kDrop;
+            return;
+        }
         const int dt = dtype_pt(P, A, B, C);
         int4 r = make_int4(0, 0, 0, 0);
         int ct = kNone;
@@ -452,23 +494,25 @@ __global__ __launch_bounds__(64) void ee_traverse(int Es, const int* __restrict_
     const int t = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
     const bool active = t < Es * kParts;
     const int eI = active ? perm[t / kParts] : 0, part = t % kParts;
+    __shared__ int stk[kStack * 64];
     int n = 0;
     if (active) {
         const int2 A = se[eI];
         const Box b1 = box_edge(ld3(x, A.x), ld3(x, A.y));
         int* b = buf + ((size_t)eI * kParts + part) * kPartCap;
-        lbvh_visit(tv, part, b1, sq, [&](int eJ) {
+        part_walk(tv, part, b1, sq, stk + threadIdx.x, [&](int eJ) {
             if (eJ > eI) { if (n < kPartCap) b[n] = eJ; ++n; }
-        });
+        }, [&] { n = 0; });
         cntp[eI * kParts + part] = n;
     }
     const int tot = sum_over_parts(n);
     if (active && part == 0) cnt[eI] = tot;
 }
 
-__global__ void ee_compact(int Es, const int* __restrict__ perm, const int2* __restrict__ se, const double* __restrict__ x, TreeView tv,
+__global__ __launch_bounds__(64) void ee_compact(int Es, const int* __restrict__ perm, const int2* __restrict__ se, const double* __restrict__ x, TreeView tv,
                            double sq, const int* __restrict__ off, const int* __restrict__ cntp, const int* __restrict__ buf,
-                           int* __restrict__ cand, int* __restrict__ own) {
+                           int* __restrict__ candU, int* __restrict__ own) {
+    __shared__ int stk[kStack * 64];
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= Es) return;
     const int eI = perm[t];
@@ -477,29 +521,38 @@ __global__ void ee_compact(int Es, const int* __restrict__ perm, const int2* __r
     for (int part = 0; part < kParts; ++part) {
         const int c = cntp[eI * kParts + part];
         if (c <= kPartCap) {
-            for (int k = 0; k < c; ++k) cand[pos + k] = buf[((size_t)eI * kParts + part) * kPartCap + k];
+            for (int k = 0; k < c; ++k) candU[pos + k] = buf[((size_t)eI * kParts + part) * kPartCap + k];
         } else {
             const int2 A = se[eI];
             const Box b1 = box_edge(ld3(x, A.x), ld3(x, A.y));
             int m = 0;
-            lbvh_visit(tv, part, b1, sq, [&](int eJ) {
-                if (eJ > eI) { if (m < c) cand[pos + m] = eJ; ++m; }
-            });
+            part_walk(tv, part, b1, sq, stk + threadIdx.x, [&](int eJ) {
+                if (eJ > eI) { if (m < c) candU[pos + m] = eJ; ++m; }
+            }, [&] { m = 0; });
         }
         pos += c;
     }
-    const int n = pos - o;
-    sort_segment(cand + o, n);
-    for (int k = 0; k < n; ++k) own[o + k] = eI;
+    for (int k = o; k < pos; ++k) own[k] = eI;
 }
 
 // Edge-edge hits: filters, then classification (CollisionBroadPhase.cpp:858-1052).
-__global__ void ee_classify(int n, int Es, const int* __restrict__ own, const int2* __restrict__ se, const double* __restrict__ x,
-                            const double* __restrict__ vrest, const int* __restrict__ bt, const int* __restrict__ cand, double hhat,
-                            int4* __restrict__ rec, int2* __restrict__ aux, int* __restrict__ cat) {
-    const int k = blockIdx.x * blockDim.x + threadIdx.x;
-    if (k >= n) return;
-    const int eI = own[k], eJ = cand[k];
+__global__ void ee_classify(int n, int Es, const int* __restrict__ own, const int* __restrict__ off, const int2* __restrict__ se,
+                            const double* __restrict__ x, const double* __restrict__ vrest, const int* __restrict__ bt,
+                            const int* __restrict__ candU, int* __restrict__ cand, double hhat,
+                            double sq, int4* __restrict__ rec, int2* __restrict__ aux, int* __restrict__ cat) {
+    const int u = blockIdx.x * blockDim.x + threadIdx.x;
+    if (u >= n) return;
+    const int eI = own[u], o = off[eI], e = off[eI + 1];
+    const int eJ = candU[u];
+    int k = o;
+    for (int j = o; j < e; ++j) k += candU[j] < eJ;
+    cand[k] = eJ;
+    if ((unsigned)eJ >= (unsigned)Es || k >= e) {   // cannot happen for a consistent walk; never index with it
+        rec[k < e ? k : o] = make_int4(0, 0, 0, 0);
+        aux[k < e ? k : o] = make_int2(-1, -1);
+        cat[k < e ? k : o] = kDrop;
+        return;
+    }
     const int2 A = se[eI], B = se[eJ];
     if ((is_collider(bt[A.x]) && is_collider(bt[A.y]) && is_collider(bt[B.x]) && is_collider(bt[B.y])) || A.x == B.x ||
         A.x == B.y || A.y == B.x || A.y == B.y) {
@@ -511,6 +564,12 @@ __global__ void ee_classify(int n, int Es, const int* __restrict__ own, const in
     const d3 a0 = ld3(x, A.x), a1 = ld3(x, A.y);
     {
         const d3 b0 = ld3(x, B.x), b1v = ld3(x, B.y);
+        if (!overlaps(box_edge(a0, a1), box_edge(b0, b1v), sq)) {   // the walk's float boxes admit a superset
+            rec[k] = make_int4(0, 0, 0, 0);
+            aux[k] = make_int2(-1, -1);
+            cat[k] = kDrop;
+            return;
+        }
         const int dt = dtype_ee(a0, a1, b0, b1v);
         const double cr = ee_cross_sqn(a0, a1, b0, b1v);
         const double ex = eps_x_of(ld3(vrest, A.x), ld3(vrest, A.y), ld3(vrest, B.x), ld3(vrest, B.y));
@@ -647,8 +706,8 @@ void compute_active_set(Context& c) {
     ActiveSetScratch& b = c.as;
     const double sq = sqrt(s.hhat);
     const int Vs = s.Vs, Es = s.Es;
-    const TreeView ft{c.face_tree.n, c.face_tree.nodes.p};
-    const TreeView et{c.edge_tree.n, c.edge_tree.nodes.p};
+    const TreeView ft{c.face_tree.n, c.face_tree.nodes.p, c.face_tree.wide.p};
+    const TreeView et{c.edge_tree.n, c.edge_tree.nodes.p, c.edge_tree.wide.p};
     b.cntPT.resize(Vs + 1); b.offPT.resize(Vs + 1); b.cntEE.resize(Es + 1); b.offEE.resize(Es + 1);
     b.cntpPT.resize((size_t)kParts * std::max(Vs, 1)); b.cntpEE.resize((size_t)kParts * std::max(Es, 1));
     b.bufPT.resize((size_t)kParts * kPartCap * std::max(Vs, 1)); b.bufEE.resize((size_t)kParts * kPartCap * std::max(Es, 1));
@@ -670,21 +729,22 @@ void compute_active_set(Context& c) {
     CIPC_HIP(hipStreamSynchronize(c.stream));
     const int nPT = nh[0], nEE = nh[1];
     // B + C: pack, sort, filter and classify the hits
+    b.candUPT.resize(nPT + 1); b.candUEE.resize(nEE + 1);
     b.candPT.resize(nPT + 1); b.recPT.resize(nPT + 1); b.catPT.resize(nPT + 1); b.ownPT.resize(nPT + 1);
     b.candEE.resize(nEE + 1); b.recEE.resize(nEE + 1); b.catEE.resize(nEE + 1); b.auxEE.resize(nEE + 1); b.ownEE.resize(nEE + 1);
     if (Vs > 0) {
         hipLaunchKernelGGL(pt_compact, dim3(blocks_for(Vs, 64)), dim3(64), 0, c.stream, Vs, s.sv_perm.p, s.sv.p, s.x.p, ft, sq, b.offPT.p,
-                           b.cntpPT.p, b.bufPT.p, b.candPT.p, b.ownPT.p);
+                           b.cntpPT.p, b.bufPT.p, b.candUPT.p, b.ownPT.p);
         if (nPT > 0)
-            hipLaunchKernelGGL(pt_classify, dim3(blocks_for(nPT, 128)), dim3(128), 0, c.stream, nPT, b.ownPT.p, s.sv.p, s.x.p, s.bt.p,
-                               s.faces.p, b.candPT.p, s.hhat, b.recPT.p, b.catPT.p);
+            hipLaunchKernelGGL(pt_classify, dim3(blocks_for(nPT, 128)), dim3(128), 0, c.stream, nPT, s.F, b.ownPT.p, b.offPT.p, s.sv.p, s.x.p,
+                               s.bt.p, s.faces.p, b.candUPT.p, b.candPT.p, s.hhat, sq, b.recPT.p, b.catPT.p);
     }
     if (Es > 0) {
         hipLaunchKernelGGL(ee_compact, dim3(blocks_for(Es, 64)), dim3(64), 0, c.stream, Es, s.se_perm.p, s.se.p, s.x.p, et, sq, b.offEE.p,
-                           b.cntpEE.p, b.bufEE.p, b.candEE.p, b.ownEE.p);
+                           b.cntpEE.p, b.bufEE.p, b.candUEE.p, b.ownEE.p);
         if (nEE > 0)
-            hipLaunchKernelGGL(ee_classify, dim3(blocks_for(nEE, 128)), dim3(128), 0, c.stream, nEE, Es, b.ownEE.p, s.se.p, s.x.p,
-                               s.vrest.p, s.bt.p, b.candEE.p, s.hhat, b.recEE.p, b.auxEE.p, b.catEE.p);
+            hipLaunchKernelGGL(ee_classify, dim3(blocks_for(nEE, 128)), dim3(128), 0, c.stream, nEE, Es, b.ownEE.p, b.offEE.p, s.se.p,
+                               s.x.p, s.vrest.p, s.bt.p, b.candUEE.p, b.candEE.p, s.hhat, sq, b.recEE.p, b.auxEE.p, b.catEE.p);
     }
     CIPC_CHECK_LAUNCH();
     // category positions
@@ -827,7 +887,7 @@ bool any_intersection(Context& c) {
         hipLaunchKernelGGL(ground_touch_kernel, dim3(blocks_for(c.n_env, 256)), dim3(256), 0, c.stream, (int)c.n_env, c.env_set.p,
                            s.x.p, s.gn, s.gd, c.counters.p);
     build_static_trees(c);
-    const TreeView et{c.edge_tree.n, c.edge_tree.nodes.p};
+    const TreeView et{c.edge_tree.n, c.edge_tree.nodes.p, c.edge_tree.wide.p};
     if (s.F > 0)
         hipLaunchKernelGGL(intersect_kernel, dim3(blocks_for(s.F, 128)), dim3(128), 0, c.stream, s.F, s.faces.p, s.se.p, s.x.p, s.bt.p,
                            et, c.counters.p + 1);

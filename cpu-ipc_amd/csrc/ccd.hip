@@ -27,7 +27,8 @@ __device__ inline void atomic_min_nonneg(unsigned long long* a, double v) { atom
 __device__ inline void atomic_max_nonneg(unsigned long long* a, double v) { atomicMax(a, ord_bits(v)); }
 __device__ inline double bits_to_double(unsigned long long b) { return __longlong_as_double((long long)b); }
 
-// slots: 0 ground min, 1 CFL max |p|, 2 partial CCD min, 3 full CCD min
+// slots: 0 ground min, 1 CFL max |p|, 2 partial CCD min, 3 full CCD min, 4 the point-triangle
+// phase's result read by the edge-edge phase, 5 the swept trees' step
 __global__ void init_bounds(unsigned long long* s) {
     s[0] = ord_bits(1.0);
     s[1] = ord_bits(0.0);
@@ -82,16 +83,19 @@ __device__ inline double wave_min(double v) {
 __global__ __launch_bounds__(64) void full_ccd_pt_kernel(int Vs, const int* __restrict__ perm, const int* __restrict__ sv,
                                                           const int4* __restrict__ faces, const int* __restrict__ bt,
                                                           const double* __restrict__ x, const double* __restrict__ p,
-                                                          const Box* __restrict__ qbox, TreeView tv, double a, double eta,
+                                                          const Box* __restrict__ qbox, TreeView tv,
+                                                          const unsigned long long* __restrict__ aslot, double eta,
                                                           unsigned long long* __restrict__ slot) {
+    __shared__ int stk[kStack * 64];
     const int t = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
     double best = 1.0;
     if (t < Vs * kParts) {
         const int s = perm[t / kParts], part = t % kParts;
+        const double a = bits_to_double(*aslot);
         const int vI = sv[s];
         const d3 X = ld3(x, vI), P = ld3(p, vI);
         const Box pb = swept_pt(X, P, a);
-        lbvh_visit(tv, part, qbox[s], 0.0, [&](int f) {
+        part_walk(tv, part, qbox[s], 0.0, stk + threadIdx.x, [&](int f) {
             const int4 fv = faces[f];
             if (is_collider(bt[vI]) && is_collider(bt[fv.x]) && is_collider(bt[fv.y]) && is_collider(bt[fv.z])) return;
             if (vI == fv.x || vI == fv.y || vI == fv.z) return;
@@ -101,7 +105,7 @@ __global__ __launch_bounds__(64) void full_ccd_pt_kernel(int Vs, const int* __re
             if (!overlaps(pb, fb, 0.0)) return;
             const double la = accd_pt(X, x0, x1, x2, -P, -p0, -p1, -p2, eta, 0.0);
             best = fmin(la, best);
-        });
+        }, [] {});
     }
     best = wave_min(best);
     if (threadIdx.x == 0) atomic_min_nonneg(slot, best);
@@ -115,6 +119,7 @@ __global__ __launch_bounds__(64) void full_ccd_ee_kernel(int Es, const int* __re
                                                           const double* __restrict__ p, TreeView tv, const Box* __restrict__ ebox,
                                                           const unsigned long long* __restrict__ aslot, double eta,
                                                           unsigned long long* __restrict__ slot) {
+    __shared__ int stk[kStack * 64];
     const int t = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
     double best = 1.0;
     if (t < Es * kParts) {
@@ -124,8 +129,9 @@ __global__ __launch_bounds__(64) void full_ccd_ee_kernel(int Es, const int* __re
         const d3 j0 = ld3(x, J.x), j1 = ld3(x, J.y), pj0 = ld3(p, J.x), pj1 = ld3(p, J.y);
         const Box b1 = box_union(swept_pt(j0, pj0, a), swept_pt(j1, pj1, a));
         // query box: eJ's leaf box of the tree (swept with the incoming step)
-        lbvh_visit(tv, part, ebox[eJ], 0.0, [&](int eI) {
-            if (eI <= eJ) return;
+        const Box q = ebox[eJ];
+        part_walk(tv, part, q, 0.0, stk + threadIdx.x, [&](int eI) {
+            if (eI <= eJ || !overlaps(q, ebox[eI], 0.0)) return;   // exact leaf test (the walk's boxes are float)
             const int2 I = se[eI];
             if (is_collider(bt[I.x]) && is_collider(bt[I.y]) && is_collider(bt[J.x]) && is_collider(bt[J.y])) return;
             if (I.x == J.x || I.x == J.y || I.y == J.x || I.y == J.y || eI < eJ) return;
@@ -134,12 +140,20 @@ __global__ __launch_bounds__(64) void full_ccd_ee_kernel(int Es, const int* __re
             if (!overlaps(b1, b2, 0.0)) return;
             const double la = accd_ee(i0, i1, j0, j1, -pi0, -pi1, -pj0, -pj1, eta, 0.0);
             best = fmin(la, best);
-        });
+        }, [] {});
     }
     best = wave_min(best);
     if (threadIdx.x == 0) atomic_min_nonneg(slot, best);
 }
 __global__ void copy_slot(const unsigned long long* src, unsigned long long* dst) { *dst = *src; }
+// Step of the swept trees from the ground bound (IPCSolver.cpp:943-958): min(1, ground), or
+// 1 when that is not positive; it seeds the full-CCD minimum (slot 3) and is kept in slot 5.
+__global__ void swept_step(int has_surface, unsigned long long* s) {
+    double a = has_surface ? fmin(1.0, bits_to_double(s[0])) : 1.0;
+    if (a <= 0) a = 1;
+    s[3] = ord_bits(a);
+    s[5] = ord_bits(a);
+}
 
 void step_bound(Context& c, double out[5]) {
     Scene& s = c.sc;
@@ -155,34 +169,29 @@ void step_bound(Context& c, double out[5]) {
         hipLaunchKernelGGL(partial_ccd_kernel, dim3(blocks_for(c.n_ccd, 64)), dim3(64), 0, c.stream, (int)c.n_ccd, c.ccd_set.p, s.sv.p,
                            s.faces.p, s.se.p, s.x.p, s.dir.p, eta, sl.p + 2);
     CIPC_CHECK_LAUNCH();
-    unsigned long long h[4];
-    CIPC_HIP(hipMemcpyAsync(h, sl.p, 2 * sizeof(unsigned long long), hipMemcpyDeviceToHost, c.stream));
-    CIPC_HIP(hipStreamSynchronize(c.stream));
-    double ground = 1.0;
-    std::memcpy(&ground, &h[0], sizeof(double));
-    double alpha = s.Vs > 0 ? std::min(1.0, ground) : 1.0;
-    const double aground = alpha;
-    if (alpha <= 0) alpha = 1;
-    // full CCD with the swept trees built for step alpha
-    build_swept_trees(c, alpha);
-    {
-        const unsigned long long ab = [&] { unsigned long long b; double a = alpha; std::memcpy(&b, &a, 8); return b; }();
-        CIPC_HIP(hipMemcpyAsync(sl.p + 3, &ab, sizeof(ab), hipMemcpyHostToDevice, c.stream));
-    }
+    // full CCD with the swept trees built for the ground-limited step, all on the device
+    hipLaunchKernelGGL(swept_step, dim3(1), dim3(1), 0, c.stream, s.Vs > 0 ? 1 : 0, sl.p);
+    CIPC_CHECK_LAUNCH();
+    build_swept_trees(c, sl.p + 5);
     LBVH& ft = c.face_tree;
     LBVH& et = c.edge_tree;
-    const TreeView ftv{ft.n, ft.nodes.p}, etv{et.n, et.nodes.p};
+    const TreeView ftv{ft.n, ft.nodes.p, ft.wide.p}, etv{et.n, et.nodes.p, et.wide.p};
     if (s.Vs > 0 && ft.n > 0)
         hipLaunchKernelGGL(full_ccd_pt_kernel, dim3(blocks_for((size_t)s.Vs * kParts, 64)), dim3(64), 0, c.stream, s.Vs, s.sv_perm.p,
-                           s.sv.p, s.faces.p, s.bt.p, s.x.p, s.dir.p, c.swept_point_box.p, ftv, alpha, eta, sl.p + 3);
+                           s.sv.p, s.faces.p, s.bt.p, s.x.p, s.dir.p, c.swept_point_box.p, ftv, sl.p + 5, eta, sl.p + 3);
     if (s.Es > 0 && et.n > 0) {
         hipLaunchKernelGGL(copy_slot, dim3(1), dim3(1), 0, c.stream, sl.p + 3, sl.p + 4);
         hipLaunchKernelGGL(full_ccd_ee_kernel, dim3(blocks_for((size_t)s.Es * kParts, 64)), dim3(64), 0, c.stream, s.Es, s.se_perm.p,
                            s.se.p, s.bt.p, s.x.p, s.dir.p, etv, et.prim.p, sl.p + 4, eta, sl.p + 3);
     }
     CIPC_CHECK_LAUNCH();
+    unsigned long long h[4];
     CIPC_HIP(hipMemcpyAsync(h, sl.p, 4 * sizeof(unsigned long long), hipMemcpyDeviceToHost, c.stream));
     CIPC_HIP(hipStreamSynchronize(c.stream));
+    double ground = 1.0;
+    std::memcpy(&ground, &h[0], sizeof(double));
+    const double aground = s.Vs > 0 ? std::min(1.0, ground) : 1.0;
+    const double alpha = aground <= 0 ? 1.0 : aground;
     double maxmove, partial_min, full;
     std::memcpy(&maxmove, &h[1], 8);
     std::memcpy(&partial_min, &h[2], 8);

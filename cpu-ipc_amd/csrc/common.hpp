@@ -82,6 +82,11 @@ __host__ __device__ inline d3 vmax(d3 a, d3 b) { return {fmax(a.x, b.x), fmax(a.
 __device__ inline d3 ld3(const double* p, int i) { return {p[3 * i], p[3 * i + 1], p[3 * i + 2]}; }
 __device__ inline void st3(double* p, int i, d3 v) { p[3 * i] = v.x; p[3 * i + 1] = v.y; p[3 * i + 2] = v.z; }
 
+// Local 3x3 blocks of a 4-vertex element: 0..3 diagonal (a,a); 4..9 off-diagonal (a,b),
+// a > b: (1,0) (2,0) (3,0) (2,1) (3,1) (3,2).  A triangle uses those with a, b < 3.
+__host__ __device__ constexpr int lb_a(int lb) { return lb < 4 ? lb : (lb < 7 ? (lb - 3) : (lb < 9 ? lb - 5 : 3)); }
+__host__ __device__ constexpr int lb_b(int lb) { return lb < 4 ? lb : (lb < 7 ? 0 : (lb < 9 ? 1 : 2)); }
+
 __host__ __device__ inline bool is_free(int t) { return t == 0; }
 __host__ __device__ inline bool is_collider(int t) { return t >= 2; }
 
@@ -98,6 +103,18 @@ struct alignas(64) Node {
     int left, skip;
     d3 hi;
     int right, pad;
+};
+// Four-wide record of an internal node: the boxes of its grandchildren (its children where
+// those are leaves), rounded outward to float, and what they are: the binary node index of
+// an internal node, -(primitive + 1) for a leaf, kNoChild for an empty slot.  One 128-byte
+// line per step of a wide walk; boxes that only touch after rounding are kept, so a wide
+// walk returns a superset that the caller confirms with the exact boxes.
+constexpr int kNoChild = 0x7fffffff;
+struct alignas(128) Node4 {
+    float lo[4][3];
+    float hi[4][3];
+    int child[4];
+    int pad[4];
 };
 // Touching counts as overlapping (CPU IPC/LBVH.cpp:157-161); `q` is the querying box.
 __host__ __device__ inline bool overlaps(const Box& q, const Box& b, double gap) {
@@ -123,8 +140,17 @@ struct Scene {
     DBuf<int> sv;              // surface vertices
     DBuf<int2> se;             // surface edges
     DBuf<int> sv_perm, se_perm; // query orders: surface vertices / edges by Morton code of the rest shape
-    std::vector<int> h_bt;     // host copy for pattern construction
-    std::vector<int> h_tets;
+    // cloth: Baraff-Witkin triangles and bending elements
+    int Tr = 0, Nb = 0, bmode = 0;  // bmode 0: quadratic bending form, 1: hinge angle
+    double stretch = 0, shear = 0, srate = 0, plate = 0;
+    DBuf<int4> tris;           // v0, v1, v2, 0
+    DBuf<double> dmt;          // SoA: 4 x Tr, column-major 2x2 Dm^-1
+    DBuf<double> area;         // Tr, area x thickness
+    DBuf<int4> bverts;         // Nb
+    DBuf<double> bq;           // SoA: 16 x Nb (quadratic form Q, row-major 4x4)
+    DBuf<double> brest, bweight;  // Nb (hinge form)
+    std::vector<int> h_bt;     // host copies for pattern construction
+    std::vector<int> h_tets, h_tris, h_bverts;
 };
 
 // Static elastic/mass block pattern and gather lists (built once per scene).
@@ -132,10 +158,11 @@ struct StaticPattern {
     int nblocks = 0;
     DBuf<int> col_ptr;      // V+1: blocks of block-column b in [col_ptr[b], col_ptr[b+1])
     DBuf<int> row;          // nblocks: block row a (a >= b), ascending within a column
-    DBuf<int> gptr;         // nblocks+1: contributions of block k in [gptr[k], gptr[k+1])
-    DBuf<int> gsrc;         // tet*16 + local block (0..9), plus bit 30 when transposed
-    DBuf<int> vptr;         // V+1: tet-corner contributions to vertex gradients
-    DBuf<int> vsrc;         // tet*4 + corner
+    DBuf<int> gptr;         // nblocks+1: contribution slots of block k in [gptr[k], gptr[k+1])
+    DBuf<int> vptr;         // V+1: element-corner slots of vertex v in [vptr[v], vptr[v+1])
+    // Elements e: tets [0, T), triangles [T, T+Tr), bending elements [T+Tr, T+Tr+Nb).
+    DBuf<int> slot_h;       // 10 per element: slot of (element, local block), -1 when not stored
+    DBuf<int> slot_g;       // 4 per element: slot of (element, corner)
     DBuf<int> diag_block;   // V: block index of (v, v) in the static pattern
 };
 

@@ -246,11 +246,314 @@ __global__ void ground_kernel(int n_env, const int* __restrict__ env, const doub
     }
 }
 
+// ---------------------------------------------------------------------------------
+// Lagged friction (Friction.cpp; FrictionKinematics.h with SFCLAMPING_ORDER 1)
+// ---------------------------------------------------------------------------------
+__device__ inline double f0_SF(double x2, double e) { return x2 * (-sqrt(x2) / 3.0 + e) / (e * e) + e / 3.0; }  // :262-265
+__device__ inline double f1_SF_div(double x2, double e) { return (-sqrt(x2) + 2.0 * e) / (e * e); }           // :267-270
+__device__ inline double f2_SF(double x2, double e) { return 2.0 * (e - sqrt(x2)) / (e * e); }               // :272-275
+__device__ inline d3 normalized(d3 v) {   // Eigen normalized(): v / |v|, unchanged when |v| = 0
+    const double n2 = sqn(v);
+    if (!(n2 > 0.0)) return v;
+    const double n = sqrt(n2);
+    return d3{v.x / n, v.y / n, v.z / n};
+}
+// Friction::initialize for one Self_ActiveSet entry (Friction.cpp:88-137): lambda, closest
+// point coordinates (computeClosestPoint_*) and tangent basis (computeTangentBasis_*).
+__global__ void friction_init_self(int n, const int4* __restrict__ set, const double* __restrict__ x, double hhat, double kappa,
+                                   int4* __restrict__ fset, double* __restrict__ lam, double* __restrict__ coord,
+                                   double* __restrict__ basis) {
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= n) return;
+    const int4 e = set[c];
+    int v[4];
+    const int type = decode(e, v);
+    d3 p[4];
+    for (int k = 0; k < nv_of(type); ++k) p[k] = ld3(x, v[k]);
+    const double d = self_value(type, p);
+    double l = barrier_g(d, hhat);
+    l *= -kappa * 2.0 * sqrt(d);
+    if (e.w < -1) l *= -e.w;
+    double b1 = 0.0, b2 = 0.0;
+    d3 B0, B1;
+    if (type == 3) {            // edge-edge (FrictionKinematics.h:81-114)
+        const d3 e20 = p[0] - p[2], e01 = p[1] - p[0], e23 = p[3] - p[2];
+        ldlt2_solve(sqn(e01), -dot(e23, e01), sqn(e23), -dot(e20, e01), dot(e20, e23), b1, b2);
+        B0 = normalized(e01);
+        B1 = normalized(cross(cross(e01, p[3] - p[2]), e01));
+    } else if (type == 0) {     // point-point (:211-227)
+        const d3 v01 = p[1] - p[0];
+        const d3 xc = cross(d3{1.0, 0.0, 0.0}, v01), yc = cross(d3{0.0, 1.0, 0.0}, v01);
+        if (sqn(xc) > sqn(yc)) { B0 = normalized(xc); B1 = normalized(cross(v01, xc)); }
+        else { B0 = normalized(yc); B1 = normalized(cross(v01, yc)); }
+    } else if (type == 1) {     // point-edge (:155-172)
+        const d3 e12 = p[2] - p[1];
+        b1 = dot(p[0] - p[1], e12) / sqn(e12);
+        B0 = normalized(e12);
+        B1 = normalized(cross(e12, p[0] - p[1]));
+    } else {                    // point-triangle (:18-41)
+        const d3 r0 = p[2] - p[1], r1 = p[3] - p[1], q = p[0] - p[1];
+        ldlt2_solve(dot(r0, r0), dot(r1, r0), dot(r1, r1), dot(r0, q), dot(r1, q), b1, b2);
+        B0 = normalized(r0);
+        B1 = normalized(cross(cross(r0, r1), r0));
+    }
+    fset[c] = e;
+    lam[c] = l;
+    coord[2 * c] = b1; coord[2 * c + 1] = b2;
+    st3(basis + 6 * (size_t)c, 0, B0);
+    st3(basis + 6 * (size_t)c, 1, B1);
+}
+__global__ void friction_init_ground(int n, const int* __restrict__ env, const double* __restrict__ x, d3 gn, double gd, double hhat,
+                                     double kappa, int* __restrict__ fenv, double* __restrict__ lam) {      // Friction.cpp:141-152
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= n) return;
+    const int v = env[c];
+    const double dist = dot(gn, ld3(x, v)) - gd;
+    const double d = dist * dist;
+    double l = barrier_g(d, hhat);
+    l *= -kappa * 2.0 * sqrt(d);
+    fenv[c] = v;
+    lam[c] = l;
+}
+
+// A lagged contact's vertices, relative displacement (computeRelDX_*) and lift
+// (liftRelDXTanToMesh_*) of a tangent direction.
+struct FrictionPair {
+    int type, nv, v[4];
+    double b1, b2;
+    d3 B0, B1;
+    __device__ FrictionPair(int4 e, const double* coord, const double* basis, int c) {
+        type = decode(e, v);
+        nv = nv_of(type);
+        b1 = coord[2 * c]; b2 = coord[2 * c + 1];
+        B0 = ld3(basis + 6 * (size_t)c, 0);
+        B1 = ld3(basis + 6 * (size_t)c, 1);
+    }
+    __device__ d3 rel_dx(const double* x, const double* xp) const {
+        d3 dx[4];
+        for (int k = 0; k < nv; ++k) dx[k] = ld3(x, v[k]) - ld3(xp, v[k]);
+        if (type == 3) return dx[0] + b1 * (dx[1] - dx[0]) - (dx[2] + b2 * (dx[3] - dx[2]));          // :117-126
+        if (type == 0) return dx[0] - dx[1];                                                        // :229-235
+        if (type == 1) return dx[0] - (dx[1] + b1 * (dx[2] - dx[1]));                               // :174-181
+        return dx[0] - (dx[1] + b1 * (dx[2] - dx[1]) + b2 * (dx[3] - dx[1]));                        // :43-51
+    }
+    __device__ void lift(double t0, double t1, double* L) const {
+        const d3 T = t0 * B0 + t1 * B1;
+        d3 s[4];
+        if (type == 3) { s[0] = (1.0 - b1) * T; s[1] = b1 * T; s[2] = (b2 - 1.0) * T; s[3] = -b2 * T; }        // :128-139
+        else if (type == 0) { s[0] = T; s[1] = -1.0 * T; }                                                   // :237-244
+        else if (type == 1) { s[0] = T; s[1] = (b1 - 1.0) * T; s[2] = -b1 * T; }                            // :183-192
+        else { s[0] = T; s[1] = (-1 + b1 + b2) * T; s[2] = -b1 * T; s[3] = -b2 * T; }                      // :53-63
+        for (int k = 0; k < nv; ++k) { L[3 * k] = s[k].x; L[3 * k + 1] = s[k].y; L[3 * k + 2] = s[k].z; }
+    }
+};
+
+// Friction::addGradient / addHessian for the lagged self contacts (Friction.cpp:158-289,
+// 349-457): local gradient and the analytic PSD Hessian (analyticFrictionHessian, :11-64),
+// in the same SoA layout as the contact blocks so the contact scatter writes their records.
+__global__ __launch_bounds__(64) void friction_local_kernel(int n, const int4* __restrict__ fset, const double* __restrict__ lam,
+                                                            const double* __restrict__ coord, const double* __restrict__ basis,
+                                                            const double* __restrict__ x, const double* __restrict__ xp, double eps2,
+                                                            double coef, double* __restrict__ Hs, double* __restrict__ Gs,
+                                                            int4* __restrict__ verts, int* __restrict__ dims) {
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= n) return;
+    const double eps = sqrt(eps2);
+    const FrictionPair fp(fset[c], coord, basis, c);
+    const int N = 3 * fp.nv;
+    const d3 r3 = fp.rel_dx(x, xp);
+    const double u0 = dot(r3, fp.B0), u1 = dot(r3, fp.B1);
+    // gradient (:172-190)
+    double t0 = u0, t1 = u1;
+    const double sq = u0 * u0 + u1 * u1;
+    if (sq > eps2) { const double nr = sqrt(sq); t0 /= nr; t1 /= nr; }
+    else { const double f1 = f1_SF_div(sq, eps); t0 *= f1; t1 *= f1; }
+    double L[12];
+    fp.lift(t0, t1, L);
+    const double gs = coef * lam[c];
+    for (int i = 0; i < N; ++i) Gs[(size_t)i * n + c] = L[i] * gs;
+    // Hessian: at most two nonnegative eigenpairs in the tangent plane
+    double H[78];
+    for (int k = 0; k < N * (N + 1) / 2; ++k) H[k] = 0.0;
+    const double w = fmax(0.0, coef * lam[c]);
+    auto add = [&](double d0, double d1, double ev) {
+        const double e = fmax(0.0, ev);
+        if (e == 0.0) return;
+        double l[12];
+        fp.lift(d0, d1, l);
+        const double sc = w * e;
+        int k = 0;
+        for (int i = 0; i < N; ++i)
+            for (int j = i; j < N; ++j, ++k) H[k] += (sc * l[i]) * l[j];
+    };
+    if (w != 0.0) {
+        const double nr = sqrt(sq);
+        if (nr > 0.0) {
+            const double p0 = u0 / nr, p1 = u1 / nr;
+            if (nr > eps) add(-p1, p0, 1.0 / nr);
+            else {
+                const double pe = f1_SF_div(sq, eps), pa = f2_SF(sq, eps);
+                add(p0, p1, pa);
+                add(-p1, p0, pe);
+            }
+        } else {
+            const double iso = f1_SF_div(0.0, eps);
+            add(1.0, 0.0, iso);
+            add(0.0, 1.0, iso);
+        }
+    }
+    for (int k = 0; k < N * (N + 1) / 2; ++k) Hs[(size_t)k * n + c] = H[k];
+    verts[c] = make_int4(fp.v[0], fp.v[1], fp.nv >= 3 ? fp.v[2] : -1, fp.nv >= 4 ? fp.v[3] : -1);
+    dims[c] = N;
+}
+// Ground friction records (Friction.cpp:283-300,310-347): one gradient and one 3x3 block.
+__global__ void friction_ground_kernel(int n, const int* __restrict__ fenv, const double* __restrict__ lam, const double* __restrict__ x,
+                                       const double* __restrict__ xp, const int* __restrict__ bt, d3 gn, double eps2, double coef, int kb,
+                                       int slot_b, int slot_g, unsigned long long* __restrict__ bkey, int* __restrict__ bidx, double* __restrict__ bval,
+                                       int* __restrict__ gkey, int* __restrict__ gidx, double* __restrict__ gval) {
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= n) return;
+    const double eps = sqrt(eps2);
+    const int v = fenv[c];
+    const d3 d = ld3(x, v) - ld3(xp, v);
+    const d3 pj = d - dot(d, gn) * gn;
+    const double mag2 = sqn(pj);
+    const d3 g = mag2 > eps2 ? (coef * lam[c] / sqrt(mag2)) * pj : (coef * lam[c] / eps) * pj;
+    const int sg = slot_g + c, sb = slot_b + c;
+    gkey[sg] = v; gidx[sg] = sg;
+    gval[3 * (size_t)sg] = g.x; gval[3 * (size_t)sg + 1] = g.y; gval[3 * (size_t)sg + 2] = g.z;
+    // tangent basis of the ground (Friction.cpp:66-76)
+    const d3 helper = fabs(gn.x) < 0.9 ? d3{1.0, 0.0, 0.0} : d3{0.0, 1.0, 0.0};
+    const d3 gb0 = normalized(helper - dot(helper, gn) * gn);
+    const d3 gb1 = normalized(cross(gn, gb0));
+    const double mult = fmax(0.0, coef * lam[c]);
+    const double r0 = dot(gb0, d), r1 = dot(gb1, d);
+    const double nr = sqrt(r0 * r0 + r1 * r1);
+    double H[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+    if (mult > 0.0) {
+        const double b0[3] = {gb0.x, gb0.y, gb0.z}, b1[3] = {gb1.x, gb1.y, gb1.z};
+        if (nr > eps) {
+            const double p0 = r0 / nr, p1 = r1 / nr;
+            const d3 l = (-p1) * gb0 + p0 * gb1;
+            const double lv[3] = {l.x, l.y, l.z};
+            const double sc = mult / nr;
+            for (int i = 0; i < 3; ++i)
+                for (int j = 0; j < 3; ++j) H[3 * i + j] = (sc * lv[i]) * lv[j];
+        } else {
+            const double sc = mult / eps;
+            for (int i = 0; i < 3; ++i)
+                for (int j = 0; j < 3; ++j) H[3 * i + j] = sc * (b0[i] * b0[j] + b1[i] * b1[j]);
+        }
+    }
+    bidx[sb] = sb;
+    if (is_free(bt[v])) {
+        bkey[sb] = ((unsigned long long)(unsigned)v << kb) | (unsigned)v;
+        for (int k = 0; k < 9; ++k) bval[(size_t)9 * sb + k] = H[k];
+    } else {
+        bkey[sb] = kNoKey;
+    }
+}
+
+// Friction::energy parts (Friction.cpp:459-545): sum over self contacts of lambda times the
+// clamped tangential norm, and sum over ground contacts of mu lambda times theirs.
+__device__ inline double friction_self_term(int c, const int4* fset, const double* lam, const double* coord, const double* basis,
+                                            const double* x, const double* xp, double eps2, double eps) {
+    const FrictionPair fp(fset[c], coord, basis, c);
+    const d3 r3 = fp.rel_dx(x, xp);
+    const double a = dot(r3, fp.B0), b = dot(r3, fp.B1);
+    const double sq = a * a + b * b;
+    return sq > eps2 ? lam[c] * sqrt(sq) : lam[c] * f0_SF(sq, eps);
+}
+__device__ inline double friction_ground_term(int c, const int* fenv, const double* lam, const double* x, const double* xp, d3 gn,
+                                              double eps2, double eps, double mu) {
+    const int v = fenv[c];
+    const d3 d = ld3(x, v) - ld3(xp, v);
+    const d3 pj = d - dot(d, gn) * gn;
+    const double sq = sqn(pj);
+    const double cl = sq > eps2 ? sqrt(sq) - 0.5 * eps : 0.5 * sq / eps;
+    return mu * lam[c] * cl;
+}
+constexpr int kFB = 256;
+__global__ __launch_bounds__(kFB) void e_friction(int ns, int ng, const int4* __restrict__ fset, const double* __restrict__ lam,
+                                                  const double* __restrict__ coord, const double* __restrict__ basis,
+                                                  const int* __restrict__ fenv, const double* __restrict__ glam,
+                                                  const double* __restrict__ x, const double* __restrict__ xp, d3 gn, double eps2,
+                                                  double mu, double* __restrict__ part) {
+    // part[blockIdx.x] = self partial, part[gridDim.x + blockIdx.x] = ground partial
+    __shared__ double sh[2][kFB];
+    const double eps = sqrt(eps2);
+    double as = 0.0, ag = 0.0;
+    for (int c = blockIdx.x * kFB + threadIdx.x; c < ns; c += gridDim.x * kFB)
+        as += friction_self_term(c, fset, lam, coord, basis, x, xp, eps2, eps);
+    for (int c = blockIdx.x * kFB + threadIdx.x; c < ng; c += gridDim.x * kFB)
+        ag += friction_ground_term(c, fenv, glam, x, xp, gn, eps2, eps, mu);
+    sh[0][threadIdx.x] = as;
+    sh[1][threadIdx.x] = ag;
+    __syncthreads();
+    for (int s = kFB / 2; s > 0; s >>= 1) {
+        if ((int)threadIdx.x < s) { sh[0][threadIdx.x] += sh[0][threadIdx.x + s]; sh[1][threadIdx.x] += sh[1][threadIdx.x + s]; }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) { part[blockIdx.x] = sh[0][0]; part[gridDim.x + blockIdx.x] = sh[1][0]; }
+}
+
+void friction_update(Context& c, double kappa) {
+    Scene& s = c.sc;
+    const int ns = (int)c.n_self, ne = (int)c.n_env;
+    c.fr_self.resize(std::max(ns, 1)); c.fr_lam.resize(std::max(ns, 1));
+    c.fr_coord.resize(2 * (size_t)std::max(ns, 1)); c.fr_basis.resize(6 * (size_t)std::max(ns, 1));
+    c.fr_env.resize(std::max(ne, 1)); c.fr_env_lam.resize(std::max(ne, 1));
+    if (ns > 0)
+        hipLaunchKernelGGL(friction_init_self, dim3(blocks_for(ns, 128)), dim3(128), 0, c.stream, ns, c.self_set.p, s.x.p, s.hhat,
+                           kappa, c.fr_self.p, c.fr_lam.p, c.fr_coord.p, c.fr_basis.p);
+    if (ne > 0)
+        hipLaunchKernelGGL(friction_init_ground, dim3(blocks_for(ne, 256)), dim3(256), 0, c.stream, ne, c.env_set.p, s.x.p, s.gn, s.gd,
+                           s.hhat, kappa, c.fr_env.p, c.fr_env_lam.p);
+    CIPC_CHECK_LAUNCH();
+    c.n_fself = ns;
+    c.n_fenv = ne;
+}
+
+void friction_records(Context& c, size_t slot_b, size_t slot_g) {
+    Scene& s = c.sc;
+    const int nf = (int)c.n_fself, ngf = (int)c.n_fenv;
+    const double eps2 = s.fhat * s.dt * s.dt;
+    if (ngf > 0) {
+        hipLaunchKernelGGL(friction_ground_kernel, dim3(blocks_for(ngf, 256)), dim3(256), 0, c.stream, ngf, c.fr_env.p, c.fr_env_lam.p,
+                           s.x.p, s.vprev.p, s.bt.p, s.gn, eps2, s.fric, s.kb, (int)slot_b, (int)slot_g, c.ckey.p, c.cidx.p, c.cval.p,
+                           c.gkey.p, c.gidx.p, c.gval.p);
+        CIPC_CHECK_LAUNCH();
+    }
+    if (nf > 0) {
+        c.fhess.resize((size_t)(kHU + 12) * nf);
+        c.fverts.resize(nf);
+        c.fdims.resize(nf);
+        double* Hs = c.fhess.p;
+        double* Gs = c.fhess.p + (size_t)kHU * nf;
+        hipLaunchKernelGGL(friction_local_kernel, dim3(blocks_for(nf, 64)), dim3(64), 0, c.stream, nf, c.fr_self.p, c.fr_lam.p,
+                           c.fr_coord.p, c.fr_basis.p, s.x.p, s.vprev.p, eps2, s.fric, Hs, Gs, c.fverts.p, c.fdims.p);
+        CIPC_CHECK_LAUNCH();
+        hipLaunchKernelGGL(contact_scatter_kernel, dim3(blocks_for(nf, 128)), dim3(128), 0, c.stream, nf, c.fverts.p, c.fdims.p, Hs, Gs,
+                           s.bt.p, (int)(slot_b + ngf), (int)(slot_g + ngf), s.kb, c.ckey.p, c.cidx.p, c.cval.p, c.gkey.p, c.gidx.p,
+                           c.gval.p);
+        CIPC_CHECK_LAUNCH();
+    }
+}
+
+void friction_energy_partials(Context& c, double* part) {
+    Scene& s = c.sc;
+    hipLaunchKernelGGL(e_friction, dim3(kFB), dim3(kFB), 0, c.stream, (int)c.n_fself, (int)c.n_fenv, c.fr_self.p, c.fr_lam.p,
+                       c.fr_coord.p, c.fr_basis.p, c.fr_env.p, c.fr_env_lam.p, s.x.p, s.vprev.p, s.gn, s.fhat * s.dt * s.dt, s.fric,
+                       part);
+    CIPC_CHECK_LAUNCH();
+}
+
 void contact_assemble(Context& c, double kappa) {
     Scene& s = c.sc;
     const int n = (int)(c.n_self + c.n_ee);
     const int ne = (int)c.n_env;
-    const size_t nb = (size_t)ne + 10 * (size_t)n, ng = (size_t)ne + 4 * (size_t)n;
+    const size_t nbc = (size_t)ne + 10 * (size_t)n, ngc = (size_t)ne + 4 * (size_t)n;
+    const size_t nb = nbc + (size_t)c.n_fenv + 10 * (size_t)c.n_fself, ng = ngc + (size_t)c.n_fenv + 4 * (size_t)c.n_fself;
     c.ckey.resize(std::max<size_t>(nb, 1)); c.cidx.resize(std::max<size_t>(nb, 1));
     c.cval.resize(9 * std::max<size_t>(nb, 1));
     c.gkey.resize(std::max<size_t>(ng, 1)); c.gidx.resize(std::max<size_t>(ng, 1));
@@ -284,6 +587,8 @@ void contact_assemble(Context& c, double kappa) {
                            c.cdims.p, Hs, Gs, s.bt.p, ne, ne, s.kb, c.ckey.p, c.cidx.p, c.cval.p, c.gkey.p, c.gidx.p, c.gval.p);
         CIPC_CHECK_LAUNCH();
     }
+    // lagged friction (IPCSolver.cpp:309-317), after the barrier records
+    friction_records(c, nbc, ngc);
     c.ckey.n = nb;
     c.gkey.n = ng;
 }

@@ -3,6 +3,7 @@
 #include "context.hpp"
 
 #include <algorithm>
+#include <cmath>
 #include <cstring>
 #include <string>
 
@@ -44,6 +45,9 @@ void bind(cipc::Context& c) { CIPC_HIP(hipSetDevice(c.device)); }
 struct cipc_gpu_context : cipc::Context {};
 
 extern "C" int cipc_gpu_eval_snh_impl(int n, const double* F, double u, double r, double* P, double* H, double* e);
+extern "C" int cipc_gpu_eval_cloth_impl(int n, const double* F, double st, double sh, double k, double* P, double* H, double* e);
+extern "C" int cipc_gpu_eval_hinge_impl(int n, const double* pts, const double* rest, const double* weight, double plate, double* g,
+                                        double* H, double* e);
 extern "C" int cipc_gpu_eval_distance_impl(int n, int kind, const double* pts, double* d, double* g, double* H);
 extern "C" int cipc_gpu_eval_make_pd_impl(int n, int dim, double* H);
 extern "C" int cipc_gpu_eval_contact_pd_impl(int n, int nverts, double* H, const double* G);
@@ -70,6 +74,9 @@ int cipc_gpu_context_create(int device, cipc_gpu_context** out) {
         auto* c = new cipc_gpu_context();
         c->device = device;
         CIPC_HIP(hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking));
+        CIPC_HIP(hipStreamCreateWithFlags(&c->aux, hipStreamNonBlocking));
+        CIPC_HIP(hipEventCreateWithFlags(&c->fork, hipEventDisableTiming));
+        CIPC_HIP(hipEventCreateWithFlags(&c->join, hipEventDisableTiming));
         *out = c;
     });
 }
@@ -79,6 +86,10 @@ int cipc_gpu_context_destroy(cipc_gpu_context* c) {
         if (!c) return;
         (void)hipSetDevice(c->device);
         (void)hipStreamSynchronize(c->stream);
+        (void)hipStreamSynchronize(c->aux);
+        (void)hipEventDestroy(c->fork);
+        (void)hipEventDestroy(c->join);
+        (void)hipStreamDestroy(c->aux);
         (void)hipStreamDestroy(c->stream);
         delete c;  // DBuf members free on destruction of their owner only when released:
     });
@@ -94,6 +105,10 @@ int cipc_gpu_upload_scene(cipc_gpu_context* c, const cipc_scene_desc* d) {
         require(d->Vs == 0 || d->surf_verts, "missing surface vertices");
         require(d->Es == 0 || d->surf_edges, "missing surface edges");
         require(d->hhat > 0 && d->dt > 0, "dHat and dt must be positive");
+        require(d->Tr >= 0 && d->Nb >= 0 && (d->bending == 0 || d->bending == 1), "invalid cloth counts or bending mode");
+        require(d->Tr == 0 || (d->triangles && d->dm_tri_inv && d->areas), "missing triangle arrays");
+        require(d->Nb == 0 || (d->bend_verts && (d->bending == 0 ? d->bend_q != nullptr : (d->bend_rest && d->bend_weight))),
+                "missing bending arrays");
         bind(*c);
         Scene& s = c->sc;
         const int V = d->V, T = d->T;
@@ -103,6 +118,10 @@ int cipc_gpu_upload_scene(cipc_gpu_context* c, const cipc_scene_desc* d) {
             for (int k = 0; k < 3; ++k)
                 if (d->faces[4 * i + k] < 0 || d->faces[4 * i + k] >= V)
                     throw Error(CIPC_ERR_INVALID_ARGUMENT, "surface face index out of range");
+        for (int i = 0; i < 3 * d->Tr; ++i)
+            if (d->triangles[i] < 0 || d->triangles[i] >= V) throw Error(CIPC_ERR_INVALID_ARGUMENT, "triangle vertex index out of range");
+        for (int i = 0; i < 4 * d->Nb; ++i)
+            if (d->bend_verts[i] < 0 || d->bend_verts[i] >= V) throw Error(CIPC_ERR_INVALID_ARGUMENT, "bending vertex index out of range");
         for (int i = 0; i < d->Vs; ++i)
             if (d->surf_verts[i] >= (uint64_t)V) throw Error(CIPC_ERR_INVALID_ARGUMENT, "surface vertex index out of range");
         for (int i = 0; i < 2 * d->Es; ++i)
@@ -134,6 +153,31 @@ int cipc_gpu_upload_scene(cipc_gpu_context* c, const cipc_scene_desc* d) {
         upload(s.dminv, soa.data(), 9 * (size_t)T, st);
         upload(s.vol, d->volumes, (size_t)T, st);
         upload(s.faces, reinterpret_cast<const int4*>(d->faces), (size_t)d->F, st);
+        // cloth: triangles padded to int4, Dm^-1 and the quadratic forms to SoA
+        {
+            const int Tr = d->Tr, Nb = d->Nb;
+            s.Tr = Tr; s.Nb = Nb; s.bmode = d->bending;
+            s.stretch = d->stretch; s.shear = d->shear; s.srate = d->strain_rate; s.plate = d->plate_rigidity;
+            s.h_tris.assign(d->triangles, d->triangles + 3 * (size_t)Tr);
+            s.h_bverts.assign(d->bend_verts, d->bend_verts + 4 * (size_t)Nb);
+            std::vector<int4> tr4(std::max(Tr, 1));
+            for (int t = 0; t < Tr; ++t) tr4[t] = make_int4(d->triangles[3 * t], d->triangles[3 * t + 1], d->triangles[3 * t + 2], 0);
+            upload(s.tris, tr4.data(), (size_t)Tr, st);
+            std::vector<double> dm(4 * (size_t)std::max(Tr, 1));
+            for (int t = 0; t < Tr; ++t)
+                for (int k = 0; k < 4; ++k) dm[(size_t)k * Tr + t] = d->dm_tri_inv[4 * (size_t)t + k];
+            upload(s.dmt, dm.data(), 4 * (size_t)Tr, st);
+            upload(s.area, d->areas, (size_t)Tr, st);
+            upload(s.bverts, reinterpret_cast<const int4*>(d->bend_verts), (size_t)Nb, st);
+            std::vector<double> q(16 * (size_t)std::max(Nb, 1), 0.0);
+            if (d->bending == 0)
+                for (int h = 0; h < Nb; ++h)
+                    for (int k = 0; k < 16; ++k) q[(size_t)k * Nb + h] = d->bend_q[16 * (size_t)h + k];
+            upload(s.bq, q.data(), d->bending == 0 ? 16 * (size_t)Nb : 0, st);
+            upload(s.brest, d->bending == 1 ? d->bend_rest : nullptr, d->bending == 1 ? (size_t)Nb : 0, st);
+            upload(s.bweight, d->bending == 1 ? d->bend_weight : nullptr, d->bending == 1 ? (size_t)Nb : 0, st);
+            CIPC_HIP(hipStreamSynchronize(st));   // the host staging vectors go out of scope
+        }
         std::vector<int> sv(d->Vs), se(2 * (size_t)d->Es);
         for (int i = 0; i < d->Vs; ++i) sv[i] = (int)d->surf_verts[i];
         for (int i = 0; i < 2 * d->Es; ++i) se[i] = (int)d->surf_edges[i];
@@ -180,6 +224,9 @@ int cipc_gpu_upload_scene(cipc_gpu_context* c, const cipc_scene_desc* d) {
         c->self_set.resize(1); c->ee_set.resize(1); c->ee_ie.resize(1); c->ccd_set.resize(1); c->env_set.resize(1);
         c->have_scene = true;
         c->face_tree.age = c->edge_tree.age = kMaxRefits;
+        c->n_self = c->n_ee = c->n_ccd = c->n_env = 0;   // sets and lagged state of an earlier scene
+        c->n_fself = c->n_fenv = 0;
+        c->assembled = false;
     });
 }
 
@@ -280,10 +327,24 @@ int cipc_gpu_set_active_set(cipc_gpu_context* c, const int64_t n[4], const int* 
     });
 }
 
+// The elastic part depends only on the state: it runs on the auxiliary stream from the
+// current point of the main stream, and the main stream waits for it before the merge.
+static void elastic_fork(cipc_gpu_context* c, hipEvent_t t0 = nullptr, hipEvent_t t1 = nullptr) {
+    CIPC_HIP(hipEventRecord(c->fork, c->stream));
+    CIPC_HIP(hipStreamWaitEvent(c->aux, c->fork, 0));
+    if (t0) CIPC_HIP(hipEventRecord(t0, c->aux));
+    elastic_assemble(*c, c->aux);
+    if (t1) CIPC_HIP(hipEventRecord(t1, c->aux));
+    CIPC_HIP(hipEventRecord(c->join, c->aux));
+}
+static void elastic_join(cipc_gpu_context* c) { CIPC_HIP(hipStreamWaitEvent(c->stream, c->join, 0)); }
+
 static void assemble_impl(cipc_gpu_context* c, double kappa) {
-    elastic_assemble(*c);
+    elastic_fork(c);
     contact_assemble(*c, kappa);
+    elastic_join(c);
     finalize_csr(*c);
+    c->assembled = true;
     // collision count as compute_g_dpt returns it: multiplicity of point-point/point-edge
     // entries, one per point-triangle/edge-edge entry (ContactMechanics.cpp:2970-3032)
     c->collisions = 0;
@@ -331,7 +392,7 @@ int cipc_gpu_get_csr(cipc_gpu_context* c, int* outer, int* inner, double* values
     });
 }
 
-int cipc_gpu_energy_terms(cipc_gpu_context* c, double kappa, double terms[5]) {
+int cipc_gpu_energy_terms(cipc_gpu_context* c, double kappa, double terms[8]) {
     return guard([&] {
         require(c && c->have_scene && terms, "no scene uploaded or null output");
         bind(*c);
@@ -343,13 +404,70 @@ int cipc_gpu_energy(cipc_gpu_context* c, double kappa, double* e) {
     return guard([&] {
         require(c && c->have_scene && e, "no scene uploaded or null output");
         bind(*c);
-        double t[5];
+        double t[8];
         energy_terms(*c, kappa, t);
-        double E = t[0];
-        E *= c->sc.dt * c->sc.dt;
-        E += t[1];
-        E += kappa * (t[2] + t[3] + t[4]);
-        *e = E;
+        *e = total_energy(*c, kappa, t);
+    });
+}
+
+int cipc_gpu_solve(cipc_gpu_context* c, int precision, double rel_tol, int max_iter, double* direction, cipc_solve_stats* stats) {
+    return guard([&] {
+        require(c && c->have_scene, "no scene uploaded");
+        require(precision == 0 || precision == 1, "precision must be 0 (FP64) or 1 (FP32 with FP64 refinement)");
+        require(std::isfinite(rel_tol) && rel_tol > 0.0 && max_iter > 0, "tolerance and iteration cap must be positive");
+        require(c->assembled, "no assembled system: call cipc_gpu_assemble first");
+        bind(*c);
+        hipEvent_t e0, e1;
+        CIPC_HIP(hipEventCreate(&e0));
+        CIPC_HIP(hipEventCreate(&e1));
+        CIPC_HIP(hipEventRecord(e0, c->stream));
+        SolveResult r;
+        linear_solve(*c, precision, rel_tol, max_iter, r);
+        CIPC_HIP(hipEventRecord(e1, c->stream));
+        CIPC_HIP(hipEventSynchronize(e1));
+        float ms = 0.f;
+        CIPC_HIP(hipEventElapsedTime(&ms, e0, e1));
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+        if (direction) {
+            CIPC_HIP(hipMemcpyAsync(direction, c->sc.dir.p, 3 * sizeof(double) * c->sc.V, hipMemcpyDeviceToHost, c->stream));
+            CIPC_HIP(hipStreamSynchronize(c->stream));
+        }
+        if (stats) {
+            stats->iterations = r.iterations;
+            stats->refinements = r.refinements;
+            stats->rel_residual = r.rel_residual;
+            stats->solve_ms = ms;
+        }
+        if (!(r.rel_residual <= rel_tol))
+            throw Error(CIPC_ERR_RUNTIME, "conjugate gradients stopped after " + std::to_string(r.iterations) +
+                                              " iterations at relative residual " + std::to_string(r.rel_residual));
+    });
+}
+
+int cipc_gpu_friction_update(cipc_gpu_context* c, double kappa) {
+    return guard([&] {
+        require(c && c->have_scene, "no scene uploaded");
+        bind(*c);
+        friction_update(*c, kappa);
+        CIPC_HIP(hipStreamSynchronize(c->stream));
+    });
+}
+
+int cipc_gpu_get_friction(cipc_gpu_context* c, int64_t counts[2], double* lam, double* coord, double* basis, double* env_lam) {
+    return guard([&] {
+        require(c && c->have_scene && counts, "no scene uploaded or null output");
+        bind(*c);
+        counts[0] = c->n_fself;
+        counts[1] = c->n_fenv;
+        const hipStream_t st = c->stream;
+        const size_t n = (size_t)c->n_fself;
+        if (lam && n) CIPC_HIP(hipMemcpyAsync(lam, c->fr_lam.p, n * sizeof(double), hipMemcpyDeviceToHost, st));
+        if (coord && n) CIPC_HIP(hipMemcpyAsync(coord, c->fr_coord.p, 2 * n * sizeof(double), hipMemcpyDeviceToHost, st));
+        if (basis && n) CIPC_HIP(hipMemcpyAsync(basis, c->fr_basis.p, 6 * n * sizeof(double), hipMemcpyDeviceToHost, st));
+        if (env_lam && c->n_fenv)
+            CIPC_HIP(hipMemcpyAsync(env_lam, c->fr_env_lam.p, (size_t)c->n_fenv * sizeof(double), hipMemcpyDeviceToHost, st));
+        CIPC_HIP(hipStreamSynchronize(st));
     });
 }
 
@@ -391,21 +509,22 @@ int cipc_gpu_newton_iteration(cipc_gpu_context* c, double kappa, cipc_iteration_
     return guard([&] {
         require(c && c->have_scene && t, "no scene uploaded or null output");
         bind(*c);
-        hipEvent_t ev[5], ee;
+        hipEvent_t ev[5], es, ee;
         for (auto& e : ev) CIPC_HIP(hipEventCreate(&e));
+        CIPC_HIP(hipEventCreate(&es));
         CIPC_HIP(hipEventCreate(&ee));
         CIPC_HIP(hipEventRecord(ev[0], c->stream));
+        elastic_fork(c, es, ee);          // overlaps the active set and the contact blocks
         compute_active_set(*c);
         CIPC_HIP(hipEventRecord(ev[1], c->stream));
-        elastic_assemble(*c);
-        CIPC_HIP(hipEventRecord(ee, c->stream));
         contact_assemble(*c, kappa);
+        elastic_join(c);
         finalize_csr(*c);
         CIPC_HIP(hipEventRecord(ev[2], c->stream));
         double sb[5];
         step_bound(*c, sb);
         CIPC_HIP(hipEventRecord(ev[3], c->stream));
-        double et[5];
+        double et[8];
         energy_terms(*c, kappa, et);
         CIPC_HIP(hipEventRecord(ev[4], c->stream));
         CIPC_HIP(hipEventSynchronize(ev[4]));
@@ -413,19 +532,32 @@ int cipc_gpu_newton_iteration(cipc_gpu_context* c, double kappa, cipc_iteration_
         for (int i = 0; i < 4; ++i) CIPC_HIP(hipEventElapsedTime(&ms[i], ev[i], ev[i + 1]));
         float tot = 0, el = 0;
         CIPC_HIP(hipEventElapsedTime(&tot, ev[0], ev[4]));
-        CIPC_HIP(hipEventElapsedTime(&el, ev[1], ee));
+        CIPC_HIP(hipEventElapsedTime(&el, es, ee));
         for (auto& e : ev) (void)hipEventDestroy(e);
+        (void)hipEventDestroy(es);
         (void)hipEventDestroy(ee);
         t->broadphase_ms = ms[0]; t->assembly_ms = ms[1]; t->elastic_ms = el; t->ccd_ms = ms[2]; t->energy_ms = ms[3];
         t->total_ms = tot;
         t->alpha = sb[0];
-        double E = et[0] * (c->sc.dt * c->sc.dt) + et[1] + kappa * (et[2] + et[3] + et[4]);
-        t->energy = E;
+        t->energy = total_energy(*c, kappa, et);
         t->nnz = c->nnz;
         t->contacts = c->n_self + c->n_ee + c->n_env;
     });
 }
 
+int cipc_gpu_eval_cloth(int n, const double* F, double st, double sh, double k, double* P, double* H, double* e) {
+    return guard([&] {
+        require(n >= 0 && (n == 0 || (F && P && H && e)), "invalid arguments");
+        if (n) cipc_gpu_eval_cloth_impl(n, F, st, sh, k, P, H, e);
+    });
+}
+int cipc_gpu_eval_hinge(int n, const double* pts, const double* rest, const double* weight, double plate, double* g, double* H,
+                        double* e) {
+    return guard([&] {
+        require(n >= 0 && (n == 0 || (pts && rest && weight && g && H && e)), "invalid arguments");
+        if (n) cipc_gpu_eval_hinge_impl(n, pts, rest, weight, plate, g, H, e);
+    });
+}
 int cipc_gpu_eval_snh(int n, const double* F, double u, double r, double* P, double* H, double* e) {
     return guard([&] { require(n >= 0 && (n == 0 || (F && P && H && e)), "invalid arguments"); if (n) cipc_gpu_eval_snh_impl(n, F, u, r, P, H, e); });
 }

@@ -25,6 +25,7 @@ struct LBVH {
     int age = 1 << 30;         // fits since the topology was built
     DBuf<Box> prim;            // primitive boxes (input)
     DBuf<Node> nodes;          // 2n-1: internal nodes 0..n-2 (root 0), leaves n-1..2n-2
+    DBuf<Node4> wide;          // n-1: four-wide records of the internal nodes
     DBuf<int> range_first;     // n-1
     DBuf<int> range_last;      // n-1
     DBuf<int> split_right;     // n-1: right child of the internal node whose split is s
@@ -36,14 +37,13 @@ struct LBVH {
     DBuf<unsigned long long> keys_sorted; // (code << 32 | primitive), sorted and unique
     DBuf<Box> scene_partial;
     bool swept = false;
-    double alpha = 0.0;
 };
 constexpr int kMaxRefits = 32;
 
 // Scratch of the active-set construction (broadphase.hip).
 struct ActiveSetScratch {
     DBuf<int> cntPT, offPT, cntEE, offEE, cntpPT, cntpEE, bufPT, bufEE;
-    DBuf<int> candPT, candEE, ownPT, ownEE, catPT, catEE;
+    DBuf<int> candUPT, candUEE, candPT, candEE, ownPT, ownEE, catPT, catEE;
     DBuf<int4> recPT, recEE;
     DBuf<int2> auxEE;
     DBuf<int> fg, pg;
@@ -52,14 +52,31 @@ struct ActiveSetScratch {
     DBuf<int> ucounts, nruns;
 };
 
+// Buffers of the Newton linear solve (pcg.hip).
+struct SolveBuffers {
+    DBuf<unsigned long long> key, key_sorted;
+    DBuf<int> src, src_sorted, col, rowptr;   // full symmetric 3x3-block matrix (BSR)
+    DBuf<double> val, minv;                   // FP64 blocks and block-Jacobi inverses
+    DBuf<float> valf, minvf, xf;              // FP32 copies for the mixed form
+    DBuf<double> r, z, p, q, x, res, part, scal;
+};
+struct SolveResult {
+    int iterations = 0, refinements = 0;
+    double rel_residual = 0.0;
+};
+
 struct Context {
     int device = 0;
     hipStream_t stream = nullptr;
+    // Elastic assembly depends on the state only; it runs on `aux` (default priority) while
+    // `stream` builds the active set and the contact blocks (fork / join events).
+    hipStream_t aux = nullptr;
+    hipEvent_t fork = nullptr, join = nullptr;
     Scene sc;
     StaticPattern sp;
     bool have_scene = false;
 
-    // elastic staging (SoA: entry e of tet t at e*T + t)
+    // elastic staging: 3x3 blocks and corner forces in their pattern slots
     DBuf<double> stagH, stagG;
     DBuf<double> sbv;          // static block values, 9 per block
     DBuf<unsigned short> sbf;  // static block nonzero-contribution masks
@@ -70,6 +87,18 @@ struct Context {
     DBuf<int4> self_set, ee_set;
     DBuf<int2> ee_ie, ccd_set;
     DBuf<int> env_set;
+
+    // lagged friction state (Friction::initialize): per self contact the encoded contact,
+    // lambda, closest-point coordinates (2) and tangent basis (6: column 0 xyz, column 1 xyz);
+    // per ground contact the vertex and lambda
+    int64_t n_fself = 0, n_fenv = 0;
+    DBuf<int4> fr_self;
+    DBuf<double> fr_lam, fr_coord, fr_basis;
+    DBuf<int> fr_env;
+    DBuf<double> fr_env_lam;
+    DBuf<double> fhess;        // per friction contact: 78 upper-triangle Hessian + 12 gradient entries (SoA)
+    DBuf<int> fdims;
+    DBuf<int4> fverts;
 
     // contact contributions (fixed slots, invalid key = ~0ull)
     DBuf<unsigned long long> ckey, ckey_sorted;
@@ -87,12 +116,15 @@ struct Context {
     DBuf<int> counters;        // small scratch integers
 
     // merged block pattern and output CSR
-    DBuf<int> mptr, mrow;
+    DBuf<int> mptr, mrow, mcol;
+    DBuf<int> pcnt, ppos;      // scalar runs of (merged block, column component) pairs
     DBuf<double> mval;
     DBuf<unsigned short> mflag;
     DBuf<int> outer, inner;
     DBuf<double> val, rhs;
     int64_t nnz = 0;
+    bool assembled = false;    // a system exists for the linear solve
+    SolveBuffers solve;
     int collisions = 0;
 
     // broad phase
@@ -110,7 +142,8 @@ struct Context {
 
 // elastic.hip
 void build_static_pattern(Context& c);
-void elastic_assemble(Context& c);
+void elastic_assemble(Context& c, hipStream_t st);
+void cloth_assemble(Context& c, hipStream_t st);
 // contact.hip
 void contact_assemble(Context& c, double kappa);
 // sparse.hip
@@ -119,7 +152,7 @@ void exclusive_scan_int(Context& c, const int* in, int* out, int n);
 // broadphase.hip
 void build_tree(Context& c, LBVH& t, int n);
 void build_static_trees(Context& c);
-void build_swept_trees(Context& c, double alpha);
+void build_swept_trees(Context& c, const unsigned long long* abits);
 void compute_active_set(Context& c);
 void sort_u64(Context& c, const unsigned long long* in, unsigned long long* out, int n, int bits);
 void sort_u64_pairs(Context& c, const unsigned long long* kin, unsigned long long* kout,
@@ -130,7 +163,31 @@ bool any_intersection(Context& c);
 // ccd.hip
 void step_bound(Context& c, double out[5]);
 // energy.hip
-void energy_terms(Context& c, double kappa, double out[5]);
+void energy_terms(Context& c, double kappa, double out[8]);
+void cloth_energy_partials(Context& c, double* part_bw, double* part_bend);
+double total_energy(const Context& c, double kappa, const double t[8]);
+
+// Deterministic energy reductions: a fixed grid of kEnergyGrid work-groups of kEnergyBlock
+// threads, each summing a grid-strided subset in index order, then a shared-memory tree;
+// one partial per work-group.
+constexpr int kEnergyBlock = 256;
+constexpr int kEnergyGrid = 256;
+__device__ inline void energy_block_sum(double v, double* part) {
+    __shared__ double sh[kEnergyBlock];
+    sh[threadIdx.x] = v;
+    __syncthreads();
+    for (int s = kEnergyBlock / 2; s > 0; s >>= 1) {
+        if ((int)threadIdx.x < s) sh[threadIdx.x] += sh[threadIdx.x + s];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) part[blockIdx.x] = sh[0];
+}
+void friction_update(Context& c, double kappa);
+void build_solve_matrix(Context& c, bool fp32);
+void linear_solve(Context& c, int precision, double rel_tol, int max_iter, SolveResult& res);
+// Friction records after the contact records; returns the friction energy parts' launch
+void friction_records(Context& c, size_t slot_b, size_t slot_g);
+void friction_energy_partials(Context& c, double* part);
 void step_forward(Context& c, double alpha);
 
 }  // namespace cipc

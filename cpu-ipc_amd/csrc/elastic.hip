@@ -7,14 +7,17 @@
 //
 // Data flow (deterministic, no atomics):
 //   1. snh_element_kernel: one thread per tet, F from x and Dm^-1 (SoA), projected
-//      12x12 stiffness written as its 78 lower-triangle entries and the 12 gradient
-//      entries to per-tet staging records (tet t at t*kStageH and t*kStageG), so that
-//      each 3x3 block a gather reads is one contiguous run of 48 or 72 bytes.
+//      12x12 stiffness.  Every (tet, local block) and (tet, corner) has a slot fixed when
+//      the pattern is built: its position in the list of contributions sorted by
+//      (block column, block row, tet), or by (vertex, tet).  The kernel writes each 3x3
+//      block (9 values, in the stored block's orientation) and each corner force (3
+//      values) to its slot.
 //   2. block_gather_kernel: one thread per 3x3 block of the static block pattern sums
-//      its contributions in tet order, adds the mass on diagonal blocks, and records
-//      which entries received a nonzero contribution (the reference drops zero entries
-//      before summing duplicates).
-//   3. vertex_gather_kernel: one thread per vertex, inertia + drag + tet gradients.
+//      its contiguous run of slots in tet order and records which entries received a
+//      nonzero contribution (the reference drops zero entries before summing
+//      duplicates); mass_diag_kernel then adds the mass on diagonal blocks.
+//   3. vertex_gather_kernel: one thread per vertex, inertia + drag + its run of corner
+//      forces.  The gathers stream their slots; only the element kernel scatters.
 #include "context.hpp"
 #include "snh.hpp"
 
@@ -23,36 +26,35 @@
 
 namespace cipc {
 
-// Local block enumeration of a tet: 0..3 diagonal (a,a); 4..9 off-diagonal (a,b), a > b:
-// (1,0) (2,0) (3,0) (2,1) (3,1) (3,2).  Staging offsets: diagonal block a at 6a (lower
-// entries (0,0) (1,0) (1,1) (2,0) (2,1) (2,2)), off-diagonal block 4+m at 24 + 9m
-// (row-major entries of K_ab, rows of local vertex a).
-__host__ __device__ constexpr int lb_a(int lb) { return lb < 4 ? lb : (lb < 7 ? (lb - 3) : (lb < 9 ? lb - 5 : 3)); }
-__host__ __device__ constexpr int lb_b(int lb) { return lb < 4 ? lb : (lb < 7 ? 0 : (lb < 9 ? 1 : 2)); }
-constexpr int kStageH = 80;   // 78 used, padded to a multiple of 16 bytes
-constexpr int kStageG = 12;
-
 // ---------------------------------------------------------------------------------
 // Static pattern: block column b holds block rows a >= b; diagonal blocks exist for
-// every vertex (mass), off-diagonal ones where a tet couples two free vertices.
+// every vertex (mass), off-diagonal ones where an element (tet, triangle or bending
+// element) couples two free vertices.
 // ---------------------------------------------------------------------------------
 void build_static_pattern(Context& c) {
     Scene& s = c.sc;
-    const int V = s.V, T = s.T;
-    const std::vector<int>& tv = s.h_tets;
+    const int V = s.V, T = s.T, Tr = s.Tr, Nb = s.Nb;
+    const int NE = T + Tr + Nb;
     const std::vector<int>& bt = s.h_bt;
-    // (col, row, src) triples; src = tet*16 + lb, bit 30 when the stored block is K_ab^T
-    struct Rec { int col, row, src; };
+    // vertices of element e (tets, then triangles, then bending elements); nv[e] corners
+    auto ev = [&](int e, int a) {
+        if (e < T) return s.h_tets[4 * (size_t)e + a];
+        if (e < T + Tr) return s.h_tris[3 * (size_t)(e - T) + a];
+        return s.h_bverts[4 * (size_t)(e - T - Tr) + a];
+    };
+    auto env = [&](int e) { return (e >= T && e < T + Tr) ? 3 : 4; };
+    // (col, row, src) triples; src = element*16 + local block
+    struct Rec { int col, row; long long src; };
     std::vector<Rec> rec;
-    rec.reserve((size_t)T * 10 + V);
+    rec.reserve((size_t)NE * 10 + V);
     for (int v = 0; v < V; ++v) rec.push_back({v, v, -1});
-    for (int t = 0; t < T; ++t) {
+    for (int e = 0; e < NE; ++e) {
+        const int nv = env(e);
         for (int lb = 0; lb < 10; ++lb) {
-            const int va = tv[4 * t + lb_a(lb)], vb = tv[4 * t + lb_b(lb)];
+            if (lb_a(lb) >= nv) continue;
+            const int va = ev(e, lb_a(lb)), vb = ev(e, lb_b(lb));
             if (!is_free(bt[va]) || !is_free(bt[vb])) continue;
-            const int row = std::max(va, vb), col = std::min(va, vb);
-            const int tr = (lb >= 4 && va < vb) ? (1 << 30) : 0;
-            rec.push_back({col, row, t * 16 + lb + tr});
+            rec.push_back({std::min(va, vb), std::max(va, vb), (long long)e * 16 + lb});
         }
     }
     // counting sort by column, then by (row, src) inside a column
@@ -64,15 +66,16 @@ void build_static_pattern(Context& c) {
         std::vector<int> pos(cnt.begin(), cnt.end() - 1);
         for (const Rec& r : rec) srt[pos[r.col]++] = r;
     }
-    std::vector<int> col_ptr(V + 1, 0), row, gptr, gsrc, diag(V, -1);
+    std::vector<int> col_ptr(V + 1, 0), row, gptr, diag(V, -1);
+    std::vector<long long> gsrc;
     row.reserve(rec.size() / 2);
     gptr.reserve(rec.size() / 2);
-    gsrc.reserve((size_t)T * 10);
+    gsrc.reserve((size_t)NE * 10);
     for (int v = 0; v < V; ++v) {
         auto b = srt.begin() + cnt[v], e = srt.begin() + cnt[v + 1];
         std::sort(b, e, [](const Rec& x, const Rec& y) {
             if (x.row != y.row) return x.row < y.row;
-            return x.src < y.src;  // -1 (mass marker) first, then tet order
+            return x.src < y.src;  // -1 (mass marker) first, then element order
         });
         for (auto it = b; it != e; ++it) {
             if (it == b || it->row != (it - 1)->row) {
@@ -85,14 +88,17 @@ void build_static_pattern(Context& c) {
         col_ptr[v + 1] = (int)row.size();
     }
     gptr.push_back((int)gsrc.size());
-    // vertex gradient gather lists: tet corners of every vertex in tet order
-    std::vector<int> vptr(V + 1, 0), vsrc((size_t)T * 4);
-    for (int i = 0; i < 4 * T; ++i) vptr[tv[i] + 1]++;
+    // vertex gradient gather lists: element corners of every vertex in element order
+    std::vector<int> vptr(V + 1, 0);
+    size_t ncorner = 0;
+    for (int e = 0; e < NE; ++e)
+        for (int a = 0; a < env(e); ++a) { vptr[ev(e, a) + 1]++; ++ncorner; }
     for (int v = 0; v < V; ++v) vptr[v + 1] += vptr[v];
+    std::vector<long long> vsrc(ncorner);
     {
         std::vector<int> pos(vptr.begin(), vptr.end() - 1);
-        for (int t = 0; t < T; ++t)
-            for (int a = 0; a < 4; ++a) vsrc[pos[tv[4 * t + a]]++] = t * 4 + a;
+        for (int e = 0; e < NE; ++e)
+            for (int a = 0; a < env(e); ++a) vsrc[pos[ev(e, a)]++] = (long long)e * 4 + a;
     }
     StaticPattern& sp = c.sp;
     sp.nblocks = (int)row.size();
@@ -100,10 +106,16 @@ void build_static_pattern(Context& c) {
         d.resize(std::max<size_t>(h.size(), 1));
         if (!h.empty()) CIPC_HIP(hipMemcpy(d.p, h.data(), h.size() * sizeof(int), hipMemcpyHostToDevice));
     };
-    up(sp.col_ptr, col_ptr); up(sp.row, row); up(sp.gptr, gptr); up(sp.gsrc, gsrc);
-    up(sp.vptr, vptr); up(sp.vsrc, vsrc); up(sp.diag_block, diag);
-    c.stagH.resize((size_t)kStageH * std::max(T, 1));
-    c.stagG.resize((size_t)kStageG * std::max(T, 1));
+    // slot of every (element, local block) and (element, corner): its position in the sorted lists
+    if (gsrc.size() >= (size_t)INT32_MAX / 9 || vsrc.size() >= (size_t)INT32_MAX / 3)
+        throw Error(CIPC_ERR_OVERFLOW, "element pattern exceeds 32-bit staging offsets");
+    std::vector<int> slot_h((size_t)10 * std::max(NE, 1), -1), slot_g((size_t)4 * std::max(NE, 1), -1);
+    for (size_t j = 0; j < gsrc.size(); ++j) slot_h[(size_t)10 * (gsrc[j] >> 4) + (gsrc[j] & 15)] = (int)j;
+    for (size_t j = 0; j < vsrc.size(); ++j) slot_g[vsrc[j]] = (int)j;
+    up(sp.col_ptr, col_ptr); up(sp.row, row); up(sp.gptr, gptr); up(sp.vptr, vptr); up(sp.diag_block, diag);
+    up(sp.slot_h, slot_h); up(sp.slot_g, slot_g);
+    c.stagH.resize((size_t)9 * std::max<size_t>(gsrc.size(), 1));
+    c.stagG.resize((size_t)3 * std::max<size_t>(vsrc.size(), 1));
     c.sbv.resize((size_t)9 * std::max(sp.nblocks, 1));
     c.sbf.resize(std::max(sp.nblocks, 1));
     c.grad.resize((size_t)3 * std::max(V, 1));
@@ -163,20 +175,23 @@ __device__ inline void snh_block(const SNH& e, const d3 B[4], const d3 G1[4], co
 
 __global__ __launch_bounds__(256) void snh_element_kernel(int T, const int4* __restrict__ tets, const double* __restrict__ x,
                                                           const double* __restrict__ dminv, const double* __restrict__ vol,
-                                                          double u, double r, double dt2,
-                                                          double* __restrict__ stagH, double* __restrict__ stagG) {
+                                                          double u, double r, double dt2, const int* __restrict__ slot_h,
+                                                          const int* __restrict__ slot_g, double* __restrict__ stagH,
+                                                          double* __restrict__ stagG) {
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= T) return;
     SNH e;
     d3 B[4];
     load_tet(t, T, tets, x, dminv, e.F, B);
+    const int4 tv = tets[t];
+    const int vid[4] = {tv.x, tv.y, tv.z, tv.w};
     snh_eval(e, u, r);
     const double vt = vol[t];
     // gradient f_a = vol P B_a (dt^2 is applied by the vertex gather, as IPCSolver.cpp:149-151)
 #pragma unroll
     for (int a = 0; a < 4; ++a) {
         const d3 f = vt * (e.pa * mvec(e.F, B[a]) + e.pb * mvec(e.C, B[a]));
-        st3(stagG + (size_t)t * kStageG, a, f);
+        st3(stagG, slot_g[4 * t + a], f);
     }
     d3 G1[4], GJ[4];
     double beta[4][3];
@@ -190,62 +205,37 @@ __global__ __launch_bounds__(256) void snh_element_kernel(int T, const int4* __r
     const double sc = vt * dt2;
 #pragma unroll
     for (int lb = 0; lb < 10; ++lb) {
+        const int j = slot_h[10 * t + lb];
+        if (j < 0) continue;   // a vertex of the block is not free
         const int a = lb_a(lb), b = lb_b(lb);
         double K[3][3];
         snh_block(e, B, G1, GJ, beta, a, b, K);
-        if (lb < 4) {
-            const int o = 6 * lb;
-            double* h = stagH + (size_t)t * kStageH + o;
-            h[0] = sc * K[0][0];
-            h[1] = sc * K[1][0];
-            h[2] = sc * K[1][1];
-            h[3] = sc * K[2][0];
-            h[4] = sc * K[2][1];
-            h[5] = sc * K[2][2];
-        } else {
-            const int o = 24 + 9 * (lb - 4);
+        // stored block (row vertex max(va, vb), column vertex min): K_ab, or K_ab^T when
+        // va < vb; a diagonal block is symmetric and stored whole
+        const bool tr = vid[a] < vid[b];
+        double* o = stagH + (size_t)9 * j;
 #pragma unroll
-            for (int i = 0; i < 3; ++i)
+        for (int i = 0; i < 3; ++i)
 #pragma unroll
-                for (int j = 0; j < 3; ++j) stagH[(size_t)t * kStageH + o + 3 * i + j] = sc * K[i][j];
-        }
+            for (int k = 0; k < 3; ++k) o[3 * i + k] = sc * (tr ? K[k][i] : K[i][k]);
     }
 }
 
 // ---------------------------------------------------------------------------------
 // Gathers
 // ---------------------------------------------------------------------------------
-__global__ __launch_bounds__(256) void block_gather_kernel(int nb, int T, const int* __restrict__ col_ptr_unused,
-                                                           const int* __restrict__ gptr, const int* __restrict__ gsrc,
-                                                           const double* __restrict__ stagH, double* __restrict__ sbv,
-                                                           unsigned short* __restrict__ sbf) {
+__global__ __launch_bounds__(256) void block_gather_kernel(int nb, const int* __restrict__ gptr, const double* __restrict__ stagH,
+                                                           double* __restrict__ sbv, unsigned short* __restrict__ sbf) {
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= nb) return;
     double acc[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
     unsigned mask = 0;
     for (int q = gptr[k]; q < gptr[k + 1]; ++q) {
-        const int src = gsrc[q];
-        const bool tr = (src >> 30) & 1;
-        const int s = src & ((1 << 30) - 1);
-        const int t = s >> 4, lb = s & 15;
-        if (lb < 4) {
-            const int o = 6 * lb;
-            const int map[6] = {0, 3, 4, 6, 7, 8};  // (0,0) (1,0) (1,1) (2,0) (2,1) (2,2)
+        const double* p = stagH + (size_t)9 * q;
 #pragma unroll
-            for (int i = 0; i < 6; ++i) {
-                const double v = stagH[(size_t)t * kStageH + o + i];
-                if (v != 0.0) { acc[map[i]] += v; mask |= 1u << map[i]; }
-            }
-        } else {
-            const int o = 24 + 9 * (lb - 4);
-#pragma unroll
-            for (int i = 0; i < 3; ++i)
-#pragma unroll
-                for (int j = 0; j < 3; ++j) {
-                    const double v = stagH[(size_t)t * kStageH + o + 3 * i + j];
-                    const int dst = tr ? (3 * j + i) : (3 * i + j);
-                    if (v != 0.0) { acc[dst] += v; mask |= 1u << dst; }
-                }
+        for (int i = 0; i < 9; ++i) {
+            const double v = p[i];
+            if (v != 0.0) { acc[i] += v; mask |= 1u << i; }
         }
     }
 #pragma unroll
@@ -266,44 +256,39 @@ __global__ void mass_diag_kernel(int V, const int* __restrict__ diag_block, cons
     sbf[k] |= (1u << 0) | (1u << 4) | (1u << 8);
 }
 
-// g = m (x - x~); g += drag m (x - V_prev); g += dt^2 f over tets (IPCSolver.cpp:118-151).
+// g = m (x - x~); g += drag m (x - V_prev); g += dt^2 f over elements (IPCSolver.cpp:118-267).
 __global__ __launch_bounds__(256) void vertex_gather_kernel(int V, int T, const double* __restrict__ x, const double* __restrict__ xt,
                                                             const double* __restrict__ vprev, const double* __restrict__ mass,
                                                             double drag, double dt2, const int* __restrict__ vptr,
-                                                            const int* __restrict__ vsrc, const double* __restrict__ stagG,
-                                                            double* __restrict__ grad) {
+                                                            const double* __restrict__ stagG, double* __restrict__ grad) {
     const int v = blockIdx.x * blockDim.x + threadIdx.x;
     if (v >= V) return;
     const d3 xv = ld3(x, v);
     const double m = mass[v];
     d3 g = m * (xv - ld3(xt, v));
     g = g + (drag * m) * (xv - ld3(vprev, v));
-    for (int q = vptr[v]; q < vptr[v + 1]; ++q) {
-        const int s = vsrc[q];
-        const int t = s >> 2, a = s & 3;
-        const d3 f = ld3(stagG + (size_t)t * kStageG, a);
-        g = g + dt2 * f;
-    }
+    for (int q = vptr[v]; q < vptr[v + 1]; ++q) g = g + dt2 * ld3(stagG, q);
     st3(grad, v, g);
 }
 
-void elastic_assemble(Context& c) {
+void elastic_assemble(Context& c, hipStream_t st) {
     Scene& s = c.sc;
     const double dt2 = s.dt * s.dt;
     if (s.T > 0)
-        hipLaunchKernelGGL(snh_element_kernel, dim3(blocks_for(s.T, 256)), dim3(256), 0, c.stream, s.T, s.tets.p, s.x.p,
-                           s.dminv.p, s.vol.p, s.u, s.r, dt2, c.stagH.p, c.stagG.p);
+        hipLaunchKernelGGL(snh_element_kernel, dim3(blocks_for(s.T, 256)), dim3(256), 0, st, s.T, s.tets.p, s.x.p,
+                           s.dminv.p, s.vol.p, s.u, s.r, dt2, c.sp.slot_h.p, c.sp.slot_g.p, c.stagH.p, c.stagG.p);
     CIPC_CHECK_LAUNCH();
+    cloth_assemble(c, st);   // triangles and bending elements into the same staging
     if (c.sp.nblocks > 0) {
-        hipLaunchKernelGGL(block_gather_kernel, dim3(blocks_for(c.sp.nblocks, 256)), dim3(256), 0, c.stream, c.sp.nblocks,
-                           s.T, c.sp.col_ptr.p, c.sp.gptr.p, c.sp.gsrc.p, c.stagH.p, c.sbv.p, c.sbf.p);
+        hipLaunchKernelGGL(block_gather_kernel, dim3(blocks_for(c.sp.nblocks, 256)), dim3(256), 0, st, c.sp.nblocks,
+                           c.sp.gptr.p, c.stagH.p, c.sbv.p, c.sbf.p);
         CIPC_CHECK_LAUNCH();
-        hipLaunchKernelGGL(mass_diag_kernel, dim3(blocks_for(s.V, 256)), dim3(256), 0, c.stream, s.V, c.sp.diag_block.p,
+        hipLaunchKernelGGL(mass_diag_kernel, dim3(blocks_for(s.V, 256)), dim3(256), 0, st, s.V, c.sp.diag_block.p,
                            s.mass.p, s.drag, c.sbv.p, c.sbf.p);
         CIPC_CHECK_LAUNCH();
     }
-    hipLaunchKernelGGL(vertex_gather_kernel, dim3(blocks_for(s.V, 256)), dim3(256), 0, c.stream, s.V, s.T, s.x.p, s.xt.p,
-                       s.vprev.p, s.mass.p, s.drag, dt2, c.sp.vptr.p, c.sp.vsrc.p, c.stagG.p, c.grad.p);
+    hipLaunchKernelGGL(vertex_gather_kernel, dim3(blocks_for(s.V, 256)), dim3(256), 0, st, s.V, s.T, s.x.p, s.xt.p,
+                       s.vprev.p, s.mass.p, s.drag, dt2, c.sp.vptr.p, c.stagG.p, c.grad.p);
     CIPC_CHECK_LAUNCH();
 }
 

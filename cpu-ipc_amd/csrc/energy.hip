@@ -1,8 +1,8 @@
 // Line-search energy on the device.  Compiled with -ffp-contract=off.
 //
-// Reference: computeEnergy (CPU IPC/IPCSolver.cpp:322-456) for tet scenes:
-//   E = dt^2 E_SNH + sum m|x - x~|^2/2 + sum drag m|x - V_prev|^2/2
-//       + kappa (sum_ground b + sum_self b * multiplicity + sum_EE e b)
+// Reference: computeEnergy (CPU IPC/IPCSolver.cpp:322-456):
+//   E = dt^2 (E_SNH + E_BW + E_bend) + sum m|x - x~|^2/2 + sum drag m|x - V_prev|^2/2
+//       + kappa (sum_ground b + sum_self b * multiplicity + sum_EE e b) + E_friction
 // with getObjEnergy_StableNHK2_3D (Elasticity.cpp:695-727) and the barrier sums of
 // Evaluate_*ConstraintVals + compute_b (ContactMechanics.cpp:2478-2544,775-778).
 //
@@ -15,20 +15,11 @@
 
 namespace cipc {
 
-constexpr int kEB = 256;      // threads per work-group
-constexpr int kEG = 256;      // work-groups
+constexpr int kEB = kEnergyBlock;   // threads per work-group
+constexpr int kEG = kEnergyGrid;    // work-groups
 
 template <typename F>
-__device__ inline void block_sum_store(double v, double* part, F) {
-    __shared__ double sh[kEB];
-    sh[threadIdx.x] = v;
-    __syncthreads();
-    for (int s = kEB / 2; s > 0; s >>= 1) {
-        if ((int)threadIdx.x < s) sh[threadIdx.x] += sh[threadIdx.x + s];
-        __syncthreads();
-    }
-    if (threadIdx.x == 0) part[blockIdx.x] = sh[0];
-}
+__device__ inline void block_sum_store(double v, double* part, F) { energy_block_sum(v, part); }
 
 __global__ __launch_bounds__(kEB) void e_snh(int T, const int4* __restrict__ tets, const double* __restrict__ x,
                                              const double* __restrict__ dminv, const double* __restrict__ vol, double u, double r,
@@ -123,12 +114,12 @@ __global__ __launch_bounds__(kEB) void sum_partials(const double* __restrict__ p
     }
 }
 
-void energy_terms(Context& c, double kappa, double out[5]) {
+void energy_terms(Context& c, double kappa, double out[8]) {
     (void)kappa;
     Scene& s = c.sc;
     static_assert(kEG == kEB, "partials are summed by one work-group");
     DBuf<double>& part = c.dscratch;
-    part.resize((size_t)6 * kEG);
+    part.resize((size_t)10 * kEG);
     double* P = part.p;
     hipLaunchKernelGGL(e_snh, dim3(kEG), dim3(kEB), 0, c.stream, s.T, s.tets.p, s.x.p, s.dminv.p, s.vol.p, s.u, s.r, P);
     hipLaunchKernelGGL(e_inertia, dim3(kEG), dim3(kEB), 0, c.stream, s.V, s.x.p, s.xt.p, s.vprev.p, s.mass.p, s.drag, P + kEG);
@@ -136,10 +127,31 @@ void energy_terms(Context& c, double kappa, double out[5]) {
     hipLaunchKernelGGL(e_self, dim3(kEG), dim3(kEB), 0, c.stream, (int)c.n_self, c.self_set.p, s.x.p, s.hhat, P + 3 * kEG);
     hipLaunchKernelGGL(e_mollified, dim3(kEG), dim3(kEB), 0, c.stream, (int)c.n_ee, c.ee_set.p, c.ee_ie.p, s.se.p, s.x.p, s.vrest.p,
                        s.hhat, P + 4 * kEG);
-    hipLaunchKernelGGL(sum_partials, dim3(1), dim3(kEB), 0, c.stream, P, 5, P + 5 * kEG);
+    // lagged friction: self part (lambda times clamped norm) and ground part (mu included)
+    friction_energy_partials(c, P + 5 * kEG);
+    // cloth triangles and bending (IPCSolver.cpp:330-368)
+    cloth_energy_partials(c, P + 7 * kEG, P + 8 * kEG);
+    hipLaunchKernelGGL(sum_partials, dim3(1), dim3(kEB), 0, c.stream, P, 9, P + 9 * kEG);
     CIPC_CHECK_LAUNCH();
-    CIPC_HIP(hipMemcpyAsync(out, P + 5 * kEG, 5 * sizeof(double), hipMemcpyDeviceToHost, c.stream));
+    double t[9];
+    CIPC_HIP(hipMemcpyAsync(t, P + 9 * kEG, 9 * sizeof(double), hipMemcpyDeviceToHost, c.stream));
     CIPC_HIP(hipStreamSynchronize(c.stream));
+    for (int k = 0; k < 5; ++k) out[k] = t[k];
+    out[5] = s.fric * t[5] + t[6];   // Friction::energy: mu * sum_self + sum_ground (Friction.cpp:521-544)
+    out[6] = t[7];
+    out[7] = s.bmode == 0 ? 0.5 * s.plate * t[8] : t[8];   // quadratic form: D/2 x^T (Q (x) I3) x
+}
+
+// Total as computeEnergy adds it: dt^2 (E_SNH + E_BW + E_bend) + inertia + kappa barriers + friction.
+double total_energy(const Context& c, double kappa, const double t[8]) {
+    double E = t[0];
+    E += t[6];
+    E += t[7];
+    E *= c.sc.dt * c.sc.dt;
+    E += t[1];
+    E += kappa * (t[2] + t[3] + t[4]);
+    E += t[5];
+    return E;
 }
 
 __global__ void step_forward_kernel(int V, const int* __restrict__ bt, const double* __restrict__ x0, const double* __restrict__ p,

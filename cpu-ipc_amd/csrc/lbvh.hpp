@@ -24,6 +24,7 @@ __device__ inline int xcd_block(int b, int nb) {
 struct TreeView {
     int n;
     const Node* nodes;
+    const Node4* wide;
 };
 
 // Visits, in depth-first (Morton) order, every leaf of the subtree [start, stop) whose box
@@ -50,20 +51,75 @@ __device__ inline void lbvh_visit_range(int n, const Node* __restrict__ nd, int 
         }
     }
 }
-// part in [0, kParts): that part's subtree; part < 0: the whole tree.
+// Root of part `part` (in [0, kParts)), or -1 for an empty part.
+__device__ inline int part_root(const TreeView& t, int part) {
+    if (t.n <= 0) return -1;
+    int node = 0;
+    for (int l = kPartBits - 1; l >= 0; --l) {
+        if (node >= t.n - 1) return (part & ((1 << (l + 1)) - 1)) ? -1 : node;
+        node = ((part >> l) & 1) ? t.nodes[node].right : t.nodes[node].left;
+    }
+    return node;
+}
+// part in [0, kParts): that part's subtree; part < 0: the whole tree.  Exact.
 template <typename F>
 __device__ inline void lbvh_visit(const TreeView& t, int part, const Box& q, double gap, F&& visit) {
     if (t.n <= 0) return;
     if (part < 0) { lbvh_visit_range(t.n, t.nodes, 0, -1, q, gap, visit); return; }
-    int node = 0;
-    for (int l = kPartBits - 1; l >= 0; --l) {
-        if (node >= t.n - 1) {
-            if (part & ((1 << (l + 1)) - 1)) return;
-            break;
-        }
-        node = ((part >> l) & 1) ? t.nodes[node].right : t.nodes[node].left;
+    const int node = part_root(t, part);
+    if (node >= 0) lbvh_visit_range(t.n, t.nodes, node, t.nodes[node].skip, q, gap, visit);
+}
+
+// Box test against a float box: evaluated in double on the outward-rounded bounds, so
+// (rounding being monotone) it never rejects what the exact test accepts.
+__device__ inline bool overlaps_f(const Box& q, const float* lo, const float* hi, double gap) {
+    return !(((double)lo[0] - q.hi.x > gap) || ((double)lo[1] - q.hi.y > gap) || ((double)lo[2] - q.hi.z > gap) ||
+             (q.lo.x - (double)hi[0] > gap) || (q.lo.y - (double)hi[1] > gap) || (q.lo.z - (double)hi[2] > gap));
+}
+
+// Wide walk of the subtree at binary node `start` with a per-lane stack in LDS (entry i at
+// stk[64 i]): one 128-byte record per step, about half the steps of the binary walk.  Calls
+// hit(primitive) for a superset of the overlapping leaves (float boxes; a single-leaf
+// subtree is tested exactly).  Returns false if the stack or the step bound ran out.
+constexpr int kStack = 64;
+template <typename F>
+__device__ inline bool wide_visit(const TreeView& t, int start, const Box& q, double gap, int* __restrict__ stk, F&& hit) {
+    if (start < 0) return true;
+    if (start >= t.n - 1) {
+        const Node& r = t.nodes[start];
+        if (overlaps(q, Box{r.lo, r.hi}, gap)) hit(r.left);
+        return true;
     }
-    lbvh_visit_range(t.n, t.nodes, node, t.nodes[node].skip, q, gap, visit);
+    int sp = 1;
+    stk[0] = start;
+    for (int steps = 0; sp > 0; ++steps) {
+        if (steps >= t.n) return false;
+        const int node = stk[64 * (--sp)];
+        const Node4 r = t.wide[node];
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            const int ch = r.child[k];
+            if (ch == kNoChild || !overlaps_f(q, r.lo[k], r.hi[k], gap)) continue;
+            if (ch < 0) {
+                hit(-ch - 1);
+            } else {
+                if (sp >= kStack) return false;
+                stk[64 * (sp++)] = ch;
+            }
+        }
+    }
+    return true;
+}
+// One part of a split query: the wide walk, or, should its stack run out, reset() and the
+// exact binary walk of the same subtree.  Deterministic, so a second call returns the same
+// hits in the same order.
+template <typename F, typename R>
+__device__ inline void part_walk(const TreeView& t, int part, const Box& q, double gap, int* __restrict__ stk, F&& hit, R&& reset) {
+    const int root = part_root(t, part);
+    if (root < 0) return;
+    if (wide_visit(t, root, q, gap, stk, hit)) return;
+    reset();
+    lbvh_visit_range(t.n, t.nodes, root, t.nodes[root].skip, q, gap, hit);
 }
 
 }  // namespace cipc

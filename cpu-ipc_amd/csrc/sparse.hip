@@ -64,24 +64,36 @@ __global__ void run_heads_u64(int n, const unsigned long long* __restrict__ k, i
     if (i == n) { head[n] = 0; return; }
     head[i] = (k[i] != kNoKey && (i == 0 || k[i] != k[i - 1])) ? 1 : 0;
 }
+// End of the run of `key` that starts at i in the sorted keys (galloping, then bisection), so
+// that the summation loop has known bounds and its loads are independent.
+template <typename K>
+__device__ inline int run_end(const K* __restrict__ k, int n, int i, K key) {
+    int a = i, b = i + 1, st = 1;
+    while (b < n && k[b] == key) { a = b; st <<= 1; b = i + st; }
+    if (b > n) b = n;
+    while (b - a > 1) { const int m = (a + b) >> 1; if (k[m] == key) a = m; else b = m; }
+    return b;
+}
 __global__ void reduce_blocks(int n, const unsigned long long* __restrict__ k, const int* __restrict__ idx,
                               const int* __restrict__ head, const int* __restrict__ pos, const double* __restrict__ val,
                               unsigned long long* __restrict__ ukey, double* __restrict__ uval, unsigned short* __restrict__ uflag) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n || !head[i]) return;
     const unsigned long long key = k[i];
+    const int e = run_end(k, n, i, key);
     double acc[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
     unsigned mask = 0;
-    for (int j = i; j < n && k[j] == key; ++j) {
+#pragma unroll 2
+    for (int j = i; j < e; ++j) {
         const double* v = val + (size_t)9 * idx[j];
 #pragma unroll
-        for (int e = 0; e < 9; ++e)
-            if (v[e] != 0.0) { acc[e] += v[e]; mask |= 1u << e; }
+        for (int c = 0; c < 9; ++c)
+            if (v[c] != 0.0) { acc[c] += v[c]; mask |= 1u << c; }
     }
     const int o = pos[i];
     ukey[o] = key;
 #pragma unroll
-    for (int e = 0; e < 9; ++e) uval[(size_t)9 * o + e] = acc[e];
+    for (int c = 0; c < 9; ++c) uval[(size_t)9 * o + c] = acc[c];
     uflag[o] = (unsigned short)mask;
 }
 // --- contact gradient records, summed per vertex in record order ------------------
@@ -91,14 +103,15 @@ __global__ void add_gradient_records(int n, const int* __restrict__ k, const int
     if (i >= n) return;
     const int v = k[i];
     if (v < 0 || v >= V || (i > 0 && k[i - 1] == v)) return;
+    const int e = run_end(k, n, i, v);
     double g0 = grad[3 * v], g1 = grad[3 * v + 1], g2 = grad[3 * v + 2];
-    for (int j = i; j < n && k[j] == v; ++j) {
+#pragma unroll 2
+    for (int j = i; j < e; ++j) {
         const double* p = val + (size_t)3 * idx[j];
         g0 += p[0]; g1 += p[1]; g2 += p[2];
     }
     grad[3 * v] = g0; grad[3 * v + 1] = g1; grad[3 * v + 2] = g2;
 }
-
 // --- merge static and contact blocks per block column --------------------------------
 __device__ inline int lower_bound_u64(const unsigned long long* a, int n, unsigned long long key) {
     int lo = 0, hi = n;
@@ -125,7 +138,8 @@ __global__ void merge_count(int V, int kb, const int* __restrict__ scol, const i
 __global__ void merge_fill(int V, int kb, const int* __restrict__ scol, const int* __restrict__ srow, const double* __restrict__ sbv,
                            const unsigned short* __restrict__ sbf, const unsigned long long* __restrict__ ukey,
                            const double* __restrict__ uval, const unsigned short* __restrict__ uflag, const int* __restrict__ nu_ptr,
-                           const int* __restrict__ mptr, int* __restrict__ mrow, double* __restrict__ mval, unsigned short* __restrict__ mflag) {
+                           const int* __restrict__ mptr, int* __restrict__ mrow, int* __restrict__ mcol, double* __restrict__ mval,
+                           unsigned short* __restrict__ mflag) {
     const int b = blockIdx.x * blockDim.x + threadIdx.x;
     if (b >= V) return;
     const int nu = *nu_ptr;
@@ -152,6 +166,7 @@ __global__ void merge_fill(int V, int kb, const int* __restrict__ scol, const in
             f |= uflag[j];
         }
         mrow[o] = ra < rb ? ra : rb;
+        mcol[o] = b;
 #pragma unroll
         for (int e = 0; e < 9; ++e) mval[(size_t)9 * o + e] = v[e];
         mflag[o] = (unsigned short)f;
@@ -160,36 +175,51 @@ __global__ void merge_fill(int V, int kb, const int* __restrict__ scol, const in
     }
 }
 // --- scalar expansion: DOF column j = 3b + cc ------------------------------------------
-__global__ void scalar_count(int V, const int* __restrict__ mptr, const int* __restrict__ mrow,
-                             const unsigned short* __restrict__ mflag, int* __restrict__ cnt) {
-    const int j = blockIdx.x * blockDim.x + threadIdx.x;
-    if (j >= 3 * V) return;
-    const int b = j / 3, cc = j % 3;
-    int n = 0;
-    for (int k = mptr[b]; k < mptr[b + 1]; ++k) {
-        const unsigned f = mflag[k];
-        const int r0 = (mrow[k] == b) ? cc : 0;
-        for (int r = r0; r < 3; ++r) n += (f >> (3 * r + cc)) & 1u;
-    }
-    cnt[j] = n;
+// Scalar CSR.  Scalar column 3b + cc lists, for each merged block k of block column b in
+// row order, the rows r of k with a nonzero flag at (r, cc) (r >= cc on the diagonal block).
+// Each (block, component) pair therefore owns a contiguous run of the column; the pairs
+// are laid out (b, cc, k) so one exclusive scan of their counts gives every run's start,
+// and the column starts are the starts of each column's first run.
+__device__ inline int pair_index(const int* __restrict__ mptr, int b, int cc, int k) {
+    const int m0 = mptr[b];
+    return 3 * m0 + cc * (mptr[b + 1] - m0) + (k - m0);
 }
-__global__ void scalar_fill(int V, const int* __restrict__ mptr, const int* __restrict__ mrow, const double* __restrict__ mval,
-                            const unsigned short* __restrict__ mflag, const int* __restrict__ outer,
-                            int* __restrict__ inner, double* __restrict__ val) {
+__global__ void pair_count(const int* __restrict__ nm_ptr, const int* __restrict__ mptr, const int* __restrict__ mrow,
+                           const int* __restrict__ mcol, const unsigned short* __restrict__ mflag, int* __restrict__ cnt) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    const int nm = *nm_ptr;
+    if (t > 3 * nm) return;
+    if (t == 3 * nm) { cnt[t] = 0; return; }
+    const int k = t / 3, cc = t % 3;
+    const int b = mcol[k];
+    const unsigned f = mflag[k];
+    const int r0 = (mrow[k] == b) ? cc : 0;
+    int n = 0;
+    for (int r = r0; r < 3; ++r) n += (f >> (3 * r + cc)) & 1u;
+    cnt[pair_index(mptr, b, cc, k)] = n;
+}
+__global__ void column_starts(int V, const int* __restrict__ mptr, const int* __restrict__ ppos, const int* __restrict__ nm_ptr,
+                              int* __restrict__ outer) {
     const int j = blockIdx.x * blockDim.x + threadIdx.x;
-    if (j >= 3 * V) return;
+    if (j > 3 * V) return;
+    if (j == 3 * V) { outer[j] = ppos[3 * *nm_ptr]; return; }
     const int b = j / 3, cc = j % 3;
-    int o = outer[j];
-    for (int k = mptr[b]; k < mptr[b + 1]; ++k) {
-        const unsigned f = mflag[k];
-        const int a = mrow[k];
-        const int r0 = (a == b) ? cc : 0;
-        for (int r = r0; r < 3; ++r) {
-            if ((f >> (3 * r + cc)) & 1u) {
-                inner[o] = 3 * a + r;
-                val[o] = mval[(size_t)9 * k + 3 * r + cc];
-                ++o;
-            }
+    outer[j] = ppos[pair_index(mptr, b, cc, mptr[b])];
+}
+__global__ void pair_fill(const int* __restrict__ nm_ptr, const int* __restrict__ mptr, const int* __restrict__ mrow,
+                          const int* __restrict__ mcol, const double* __restrict__ mval, const unsigned short* __restrict__ mflag,
+                          const int* __restrict__ ppos, int* __restrict__ inner, double* __restrict__ val) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= 3 * *nm_ptr) return;
+    const int k = t / 3, cc = t % 3;
+    const int b = mcol[k], a = mrow[k];
+    const unsigned f = mflag[k];
+    int o = ppos[pair_index(mptr, b, cc, k)];
+    for (int r = (a == b) ? cc : 0; r < 3; ++r) {
+        if ((f >> (3 * r + cc)) & 1u) {
+            inner[o] = 3 * a + r;
+            val[o] = mval[(size_t)9 * k + 3 * r + cc];
+            ++o;
         }
     }
 }
@@ -236,7 +266,7 @@ void finalize_csr(Context& c) {
                            c.gidx_sorted.p, c.gval.p, V, c.grad.p);
         CIPC_CHECK_LAUNCH();
     }
-    // merged block pattern
+    // merged block pattern, sized by its bound (static blocks + unique contact blocks <= records)
     c.mptr.resize((size_t)V + 1);
     DBuf<int>& cnt = c.iscratch2;
     hipLaunchKernelGGL(merge_count, dim3(blocks_for(V, 256)), dim3(256), 0, c.stream, V, s.kb, c.sp.col_ptr.p, c.sp.row.p, c.ukey.p,
@@ -244,26 +274,29 @@ void finalize_csr(Context& c) {
     CIPC_CHECK_LAUNCH();
     hipLaunchKernelGGL(set_zero_tail, dim3(1), dim3(1), 0, c.stream, cnt.p, V);
     exclusive_scan_int(c, cnt.p, c.mptr.p, V + 1);
-    int nmerged = 0;
-    CIPC_HIP(hipMemcpyAsync(&nmerged, c.mptr.p + V, sizeof(int), hipMemcpyDeviceToHost, c.stream));
-    CIPC_HIP(hipStreamSynchronize(c.stream));
-    c.mrow.resize(std::max(nmerged, 1)); c.mval.resize((size_t)9 * std::max(nmerged, 1)); c.mflag.resize(std::max(nmerged, 1));
+    const size_t mcap = (size_t)c.sp.nblocks + (size_t)nb + 1;
+    c.mrow.resize(mcap); c.mcol.resize(mcap); c.mval.resize(9 * mcap); c.mflag.resize(mcap);
     hipLaunchKernelGGL(merge_fill, dim3(blocks_for(V, 256)), dim3(256), 0, c.stream, V, s.kb, c.sp.col_ptr.p, c.sp.row.p, c.sbv.p,
-                       c.sbf.p, c.ukey.p, c.uval.p, c.uflag.p, c.counters.p, c.mptr.p, c.mrow.p, c.mval.p, c.mflag.p);
+                       c.sbf.p, c.ukey.p, c.uval.p, c.uflag.p, c.counters.p, c.mptr.p, c.mrow.p, c.mcol.p, c.mval.p, c.mflag.p);
     CIPC_CHECK_LAUNCH();
-    // scalar CSR
+    // scalar CSR: (block, component) counts -> one scan -> column starts and runs
+    const int* nm_ptr = c.mptr.p + V;
+    c.pcnt.resize(3 * mcap + 1); c.ppos.resize(3 * mcap + 1);
+    hipLaunchKernelGGL(pair_count, dim3(blocks_for(3 * mcap + 1, 256)), dim3(256), 0, c.stream, nm_ptr, c.mptr.p, c.mrow.p, c.mcol.p,
+                       c.mflag.p, c.pcnt.p);
+    CIPC_CHECK_LAUNCH();
+    // the scan covers the bound; entries past 3 nm + 1 are never read
+    exclusive_scan_int(c, c.pcnt.p, c.ppos.p, (int)(3 * mcap + 1));
     c.outer.resize((size_t)3 * V + 1);
-    hipLaunchKernelGGL(scalar_count, dim3(blocks_for(3 * V, 256)), dim3(256), 0, c.stream, V, c.mptr.p, c.mrow.p, c.mflag.p, cnt.p);
+    hipLaunchKernelGGL(column_starts, dim3(blocks_for(3 * V + 1, 256)), dim3(256), 0, c.stream, V, c.mptr.p, c.ppos.p, nm_ptr, c.outer.p);
     CIPC_CHECK_LAUNCH();
-    hipLaunchKernelGGL(set_zero_tail, dim3(1), dim3(1), 0, c.stream, cnt.p, 3 * V);
-    exclusive_scan_int(c, cnt.p, c.outer.p, 3 * V + 1);
     int nnz = 0;
     CIPC_HIP(hipMemcpyAsync(&nnz, c.outer.p + 3 * V, sizeof(int), hipMemcpyDeviceToHost, c.stream));
     CIPC_HIP(hipStreamSynchronize(c.stream));
     c.nnz = nnz;
     c.inner.resize(std::max(nnz, 1)); c.val.resize(std::max(nnz, 1));
-    hipLaunchKernelGGL(scalar_fill, dim3(blocks_for(3 * V, 256)), dim3(256), 0, c.stream, V, c.mptr.p, c.mrow.p, c.mval.p,
-                       c.mflag.p, c.outer.p, c.inner.p, c.val.p);
+    hipLaunchKernelGGL(pair_fill, dim3(blocks_for(3 * mcap, 256)), dim3(256), 0, c.stream, nm_ptr, c.mptr.p, c.mrow.p, c.mcol.p,
+                       c.mval.p, c.mflag.p, c.ppos.p, c.inner.p, c.val.p);
     CIPC_CHECK_LAUNCH();
     c.rhs.resize((size_t)3 * V);
     hipLaunchKernelGGL(masked_rhs, dim3(blocks_for(3 * V, 256)), dim3(256), 0, c.stream, V, s.bt.p, c.grad.p, c.rhs.p);

@@ -64,6 +64,19 @@ typedef struct {
     const uint64_t* surf_edges; /* mesh3D::surfEdges                   2Es */
     double dt, length_rate, volume_rate, drag, hhat, fhat, friction;
     double ground_normal[3], ground_d;
+    /* Cloth (Baraff-Witkin triangles, IPCSolver.cpp:166-194) and bending (IPCSolver.cpp:
+     * 197-267); Tr = Nb = 0 and NULL pointers for tet-only scenes. */
+    int Tr;                     /* mesh3D::triangleNum                 */
+    const int* triangles;       /* mesh3D::triangles                   3Tr */
+    const double* dm_tri_inv;   /* mesh3D::DM_triangle_inverse         4Tr (2x2 column-major) */
+    const double* areas;        /* mesh3D::areas (area x thickness)    Tr  */
+    double stretch, shear, strain_rate, plate_rigidity; /* stretchStiffness, shearStiffness, strainRate, plateRigidity */
+    int Nb;                     /* bending elements                    */
+    int bending;                /* 0: quadratic form (USE_QUADRATIC_BENDING), 1: hinge angle */
+    const int* bend_verts;      /* QuadBendingInfo::verts / HingeBendingInfo::vertices   4Nb */
+    const double* bend_q;       /* QuadBendingInfo::Q, row-major 4x4 (bending 0)         16Nb */
+    const double* bend_rest;    /* HingeBendingInfo::restAngle (bending 1)              Nb  */
+    const double* bend_weight;  /* HingeBendingInfo::geometricWeight (bending 1)        Nb  */
 } cipc_scene_desc;
 
 /* Stage times of one Newton iteration, milliseconds, measured with HIP events on the
@@ -80,6 +93,14 @@ typedef struct {
     int64_t nnz;          /* CSR nonzeros */
     int64_t contacts;     /* |Self_ActiveSet| + |Self_EE_ActiveSet| + |Environment_ActiveSet| */
 } cipc_iteration_timing;
+
+/* Result of one Newton linear solve. */
+typedef struct {
+    int iterations;       /* conjugate-gradient iterations (inner iterations summed for precision 1) */
+    int refinements;      /* FP64 refinement steps (precision 1), 0 for precision 0 */
+    double rel_residual;  /* ||rhs - H p|| / ||rhs|| evaluated in FP64 at exit */
+    double solve_ms;      /* device time of the solve, HIP events on the context's stream */
+} cipc_solve_stats;
 
 const char* cipc_gpu_last_error(void);
 
@@ -126,9 +147,32 @@ int cipc_gpu_get_csr(cipc_gpu_context* ctx, int* outer, int* inner, double* valu
 
 /* computeEnergy (IPCSolver.cpp:322-456) at the current positions with the current active
  * set.  terms = {SNH elastic (before the dt^2 factor), inertia + drag, ground barrier sum,
- * self barrier sum (with multiplicity), mollified edge-edge barrier sum}. */
+ * self barrier sum (with multiplicity), mollified edge-edge barrier sum, lagged friction
+ * energy (Friction::energy, Friction.cpp:459-545; 0 before cipc_gpu_friction_update),
+ * Baraff-Witkin triangles (before dt^2), bending (before dt^2)}. */
 int cipc_gpu_energy(cipc_gpu_context* ctx, double kappa, double* energy);
-int cipc_gpu_energy_terms(cipc_gpu_context* ctx, double kappa, double terms[5]);
+int cipc_gpu_energy_terms(cipc_gpu_context* ctx, double kappa, double terms[8]);
+
+/* NewtonLinearSystem::solve (NewtonLinearSystem.cpp:21-56) for the system of the last
+ * cipc_gpu_assemble: H p = rhs with the relative-residual target and iteration cap of
+ * LinearSolverOptions (LinearSolverOptions.h:9-19: 1e-6, 10000).  The solve runs on the
+ * device with the 3x3 block-Jacobi preconditioner, in FP64 (precision 0) or in FP32 inside
+ * FP64 iterative refinement (precision 1).  The solution becomes the direction of
+ * cipc_gpu_step_bound / cipc_gpu_step_forward and is copied to `direction` (3V, may be
+ * NULL).  CIPC_ERR_RUNTIME when the target is not met within max_iter, as the reference's
+ * solveWithEigenConjugateGradient throws (NewtonLinearSystem.cpp:196-203). */
+int cipc_gpu_solve(cipc_gpu_context* ctx, int precision, double rel_tol, int max_iter, double* direction, cipc_solve_stats* stats);
+
+/* Friction::initialize (Friction.cpp:81-156), called at the start of a time step: the
+ * lagged friction state (per Self_ActiveSet entry lambda, closest-point coordinates and
+ * tangent basis; per Environment_ActiveSet vertex lambda) from the current active set and
+ * positions.  Every later assemble and energy adds the friction terms
+ * (Friction::addGradient / addHessian / energy, IPCSolver.cpp:309-317,453-455) with the
+ * scene's friction coefficient and epsilon^2 = fhat dt^2. */
+int cipc_gpu_friction_update(cipc_gpu_context* ctx, double kappa);
+/* The lagged state: counts = {self contacts, ground contacts}; lam (n), coord (2n),
+ * basis (6n: column 0 xyz, column 1 xyz), env_lam (counts[1]); outputs may be NULL. */
+int cipc_gpu_get_friction(cipc_gpu_context* ctx, int64_t counts[2], double* lam, double* coord, double* basis, double* env_lam);
 
 /* Step bound of one Newton iteration for the direction set with cipc_gpu_set_direction:
  * limitStepByGround (FeasibleStep.cpp:8-40), Self_largestFeasibleStepSize
@@ -164,6 +208,14 @@ int cipc_gpu_newton_iteration(cipc_gpu_context* ctx, double kappa, cipc_iteratio
  *  accd:     kind 0 point-triangle, 1 edge-edge; pts 12n, dirs 12n -> toc (n) (AdditiveCCD.cpp:221-329)
  *  classify: kind 0 dType_PT, 1 dType_EE; pts 12n -> type (n) (ContactMechanics.cpp:143-280) */
 int cipc_gpu_eval_snh(int n, const double* F, double length_rate, double volume_rate, double* P, double* H, double* e);
+/*  cloth:    F (6n, 3x2 column-major) -> PK1 (6n), projected 6x6 Hessian (36n row-major), energy
+ *            density (n) (Elasticity.cpp:729-771,1173-1205,1481-1544)
+ *  hinge:    hinge points (12n: edge start, edge end, opposite vertices), rest angle, weight (n)
+ *            -> gradient (12n), makePD'd Hessian (144n row-major), energy (n), before dt^2
+ *            (HingeBending.cpp:79-106; Elasticity.cpp:12-150; IPCSolver.cpp:238-267) */
+int cipc_gpu_eval_cloth(int n, const double* F, double stretch, double shear, double strain_rate, double* P, double* H, double* e);
+int cipc_gpu_eval_hinge(int n, const double* pts, const double* rest, const double* weight, double plate_rigidity,
+                        double* g, double* H, double* e);
 int cipc_gpu_eval_distance(int n, int kind, const double* pts, double* d, double* grad, double* hess);
 int cipc_gpu_eval_make_pd(int n, int dim, double* H);
 int cipc_gpu_eval_contact_pd(int n, int nverts, double* H, const double* G);

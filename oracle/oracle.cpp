@@ -179,6 +179,11 @@ struct Mesh {
     std::vector<V3> x, vrest, vprev, xt;
     std::vector<double> mass;  std::vector<int> bt;
     std::vector<I4> tets; std::vector<M3> dminv; std::vector<double> vol;
+    // cloth triangles and bending elements (SimulationMesh.h:17-35,79-113)
+    int Tr = 0, Nb = 0, bmode = 0;
+    std::vector<std::array<int, 3>> tris; std::vector<std::array<double, 4>> dmt; std::vector<double> area;
+    double stretchS = 0, shearS = 0, srate = 0, plate = 0;
+    std::vector<I4> bverts; std::vector<std::array<double, 16>> bq; std::vector<double> brest, bweight;
     std::vector<std::array<int, 3>> faces; std::vector<int> sv; std::vector<std::pair<int, int>> se;
     double dt = 0, u = 0, r = 0, drag = 0, hhat = 0, fhat = 0, fric = 0;
     V3 gn{0, 1, 0}; double gd = -1;
@@ -293,6 +298,205 @@ void pfpx(const M3& Di, double P[9][12]) {
         }
     }
 }
+
+// ----------------------------------------------------------------------------------
+// Cloth: Baraff-Witkin triangles (Elasticity.cpp) and bending (Simulator.cpp:48-109,
+// HingeBending.cpp, Elasticity.cpp:12-150).  A 3x2 deformation gradient is stored
+// column-major: f[0..2] = F a (a = (1,0)), f[3..5] = F b (b = (0,1)).
+// ----------------------------------------------------------------------------------
+V3 normalized(const V3& v) {   // Eigen 3.4 normalized(): v / |v|, unchanged when |v| = 0
+    const double n2 = sqn(v);
+    if (!(n2 > 0.0)) return v;
+    const double n = std::sqrt(n2);
+    return V3(v.x / n, v.y / n, v.z / n);
+}
+// calculateDs32D_double x Dm^-1 (Elasticity.cpp:601-607; IPCSolver.cpp:169-170)
+void tri_deformation(const Mesh& m, int t, double f[6]) {
+    const auto& id = m.tris[t];
+    const V3 e1 = m.x[id[1]] - m.x[id[0]], e2 = m.x[id[2]] - m.x[id[0]];
+    const auto& D = m.dmt[t];   // column-major 2x2
+    for (int j = 0; j < 2; ++j) {
+        const V3 c = D[2 * j] * e1 + D[2 * j + 1] * e2;
+        f[3 * j] = c.x; f[3 * j + 1] = c.y; f[3 * j + 2] = c.z;
+    }
+}
+// computePFPX32D_double (Elasticity.cpp:851-897): row 3j+i of vec(F) against 3c+i of x.
+void tri_pfpx(const std::array<double, 4>& D, double P[6][9]) {
+    for (int i = 0; i < 6; ++i) for (int j = 0; j < 9; ++j) P[i][j] = 0;
+    const double s0 = D[0] + D[1], s1 = D[2] + D[3];
+    for (int i = 0; i < 3; ++i) {
+        P[i][i] = -s0; P[3 + i][i] = -s1;
+        P[i][3 + i] = D[0]; P[3 + i][3 + i] = D[2];
+        P[i][6 + i] = D[1]; P[3 + i][6 + i] = D[3];
+    }
+}
+// computePEPF_baraffwitkin_double (Elasticity.cpp:1173-1205)
+void bw_pk1(const double f[6], double stretchS, double shearS, double k, double P[6]) {
+    const V3 fu(f[0], f[1], f[2]), fv(f[3], f[4], f[5]);
+    const double I6 = dot(fu, fv);
+    const double I5u = dot(fu, fu), I5v = dot(fv, fv);
+    double uc = 1.0 - 1 / std::sqrt(I5u), vc = 1.0 - 1 / std::sqrt(I5v);
+    if (I5u > 1) uc += 1.5 * k * (std::sqrt(I5u) + 1 / std::sqrt(I5u) - 2);
+    if (I5v > 1) vc += 1.5 * k * (std::sqrt(I5v) + 1 / std::sqrt(I5v) - 2);
+    // shear: 2 (I6 - a.b)(F a b^T + F b a^T) = 2 I6 [F b, F a]; stretch: 2 c_u F a a^T + 2 c_v F b b^T
+    for (int i = 0; i < 3; ++i) {
+        P[i] = stretchS * (uc * 2. * fu[i]) + shearS * (2 * I6 * fv[i]);
+        P[3 + i] = stretchS * (vc * 2. * fv[i]) + shearS * (2 * I6 * fu[i]);
+    }
+}
+// project_baraffwitkint_H_3D (Elasticity.cpp:1481-1544): 6x6 row-major over vec(F).
+void bw_hessian(const double f[6], double stretchS, double shearS, double k, double Hout[36]) {
+    const V3 fu(f[0], f[1], f[2]), fv(f[3], f[4], f[5]);
+    double H[36] = {};
+    const double I5u = dot(fu, fu), I5v = dot(fv, fv);
+    const double su = std::sqrt(I5u), sv = std::sqrt(I5v);
+    if (su > 1) { const double h = 2 * (((su - 1) * (3 * su * k - 3 * k + 2)) / (2 * su)); for (int i = 0; i < 3; ++i) H[7 * i] = h; }
+    if (sv > 1) { const double h = 2 * (((sv - 1) * (3 * sv * k - 3 * k + 2)) / (2 * sv)); for (int i = 3; i < 6; ++i) H[7 * i] = h; }
+    const V3 nu = normalized(fu), nv = normalized(fv);
+    const double uC = (su > 1.0) ? (3 * I5u * k - 3 * k + 2) / su : 2.0;
+    const double vC = (sv > 1.0) ? (3 * I5v * k - 3 * k + 2) / sv : 2.0;
+    for (int i = 0; i < 3; ++i)
+        for (int j = 0; j < 3; ++j) { H[6 * i + j] += uC * (nu[i] * nu[j]); H[6 * (3 + i) + 3 + j] += vC * (nv[i] * nv[j]); }
+    for (int i = 0; i < 36; ++i) Hout[i] = stretchS * H[i];
+    // shear: analytic PSD form from the eigensystem of the I6^2 Hessian
+    const double I6 = dot(fu, fv);
+    const double sgn = (I6 >= 0) ? 1.0 : -1.0;
+    double g[6] = {fv.x, fv.y, fv.z, fu.x, fu.y, fu.z};           // vec(F (a b^T + b a^T))
+    const double I2 = I5u + I5v;
+    const double l0 = 0.5 * (I2 + std::sqrt(I2 * I2 + 12.0 * I6 * I6));
+    double q[6], qn = 0;
+    for (int i = 0; i < 6; ++i) { q[i] = I6 * g[(i + 3) % 6] + l0 * g[i]; qn += q[i] * q[i]; }
+    if (qn > 0) { qn = std::sqrt(qn); for (int i = 0; i < 6; ++i) q[i] /= qn; }
+    double T[36], Tq[6], tn = 0;
+    for (int i = 0; i < 6; ++i)
+        for (int j = 0; j < 6; ++j) T[6 * i + j] = 0.5 * ((i == j ? 1.0 : 0.0) + sgn * ((j == (i + 3) % 6) ? 1.0 : 0.0));
+    for (int i = 0; i < 6; ++i) { double a = 0; for (int j = 0; j < 6; ++j) a += T[6 * i + j] * q[j]; Tq[i] = a; tn += a * a; }
+    for (int i = 0; i < 6; ++i)
+        for (int j = 0; j < 6; ++j) {
+            const double hs = std::fabs(I6) * (T[6 * i + j] - (Tq[i] * Tq[j]) / tn) + l0 * (q[i] * q[j]);
+            Hout[6 * i + j] += shearS * (hs * 2);
+        }
+}
+// getObjEnergy_baraffwitkin_3D per element (Elasticity.cpp:729-771), without the area.
+double bw_energy_density(const double f[6], double stretchS, double shearS, double k) {
+    const V3 fu(f[0], f[1], f[2]), fv(f[3], f[4], f[5]);
+    const double I6 = dot(fu, fv);
+    const double sh = I6 * I6;
+    const double nu = std::sqrt(sqn(fu)), nv = std::sqrt(sqn(fv));
+    const double cu = nu <= 1 ? 0.0 : 1.0, cv = nv <= 1 ? 0.0 : 1.0;
+    const double st = std::pow(nu - 1, 2) + cu * k * std::pow(nu - 1, 3) + std::pow(nv - 1, 2) + cv * k * std::pow(nv - 1, 3);
+    return stretchS * st + shearS * sh;
+}
+
+// LibShell::angle (Elasticity.cpp:28-58): angle of v to w about axis, derivative over
+// (v, w, axis) and the second-derivative blocks the reference fills ((0,0), (3,3),
+// (0,6), (3,6); the (6,0) and (6,3) blocks stay zero there).
+double shell_angle(const V3& v, const V3& w, const V3& axis, double* der, double (*hes)[9]) {
+    const double an = std::sqrt(sqn(axis));
+    const double theta = 2.0 * std::atan2(dot(cross(v, w), axis) / an, dot(v, w) + std::sqrt(sqn(v)) * std::sqrt(sqn(w)));
+    if (der) {
+        const V3 a = cross(axis, v), b = cross(axis, w);
+        const double vv = sqn(v), ww = sqn(w);
+        for (int i = 0; i < 3; ++i) { der[i] = -a[i] / vv / an; der[3 + i] = b[i] / ww / an; der[6 + i] = 0.0; }
+    }
+    if (hes) {
+        for (int i = 0; i < 9; ++i) for (int j = 0; j < 9; ++j) hes[i][j] = 0.0;
+        const double vv = sqn(v), ww = sqn(w);
+        const V3 av = cross(axis, v), aw = cross(axis, w);
+        for (int i = 0; i < 3; ++i)
+            for (int j = 0; j < 3; ++j) {
+                hes[i][j] += 2.0 * av[i] * v[j] / vv / vv / an;
+                hes[3 + i][3 + j] += -2.0 * aw[i] * w[j] / ww / ww / an;
+            }
+        const double cx[3][3] = {{0, -axis.z, axis.y}, {axis.z, 0, -axis.x}, {-axis.y, axis.x, 0}};
+        for (int i = 0; i < 3; ++i)
+            for (int j = 0; j < 3; ++j) { hes[i][j] += -cx[i][j] / vv / an; hes[3 + i][3 + j] += cx[i][j] / ww / an; }
+        double dah[3][3];
+        for (int i = 0; i < 3; ++i)
+            for (int j = 0; j < 3; ++j) dah[i][j] = (i == j ? 1.0 : 0.0) / an - axis[i] * axis[j] / an / an / an;
+        const double cv[3][3] = {{0, -v.z, v.y}, {v.z, 0, -v.x}, {-v.y, v.x, 0}};
+        const double cw[3][3] = {{0, -w.z, w.y}, {w.z, 0, -w.x}, {-w.y, w.x, 0}};
+        for (int i = 0; i < 3; ++i)
+            for (int j = 0; j < 3; ++j) {
+                double a = 0, b = 0;
+                for (int k = 0; k < 3; ++k) { a += cv[i][k] * dah[k][j]; b += cw[i][k] * dah[k][j]; }
+                hes[i][6 + j] += a / vv;
+                hes[3 + i][6 + j] += -b / ww;
+            }
+    }
+    return theta;
+}
+// edgeTheta (Elasticity.cpp:59-150): dihedral angle of the hinge (q0, q1; q2, q3), its
+// 12-gradient and its 12x12 second-derivative matrix (row-major) as the reference
+// assembles them from shell_angle's blocks.
+double edge_theta(const V3& q0, const V3& q1, const V3& q2, const V3& q3, double* der, double* hess) {
+    const V3 n0 = cross(q0 - q2, q1 - q2), n1 = cross(q1 - q3, q0 - q3), axis = q1 - q0;
+    double ad[9], ah[9][9];
+    const double theta = shell_angle(n0, n1, axis, (der || hess) ? ad : nullptr, hess ? ah : nullptr);
+    auto cm = [](const V3& v, double o[3][3]) {
+        o[0][0] = 0; o[0][1] = -v.z; o[0][2] = v.y; o[1][0] = v.z; o[1][1] = 0; o[1][2] = -v.x; o[2][0] = -v.y; o[2][1] = v.x; o[2][2] = 0;
+    };
+    // row vector r times cross matrix of e: (r [e]x)_j = sum_i r_i [e]x(i, j)
+    auto rowcm = [&](const double* r, const V3& e, double* out) {
+        double c[3][3]; cm(e, c);
+        for (int j = 0; j < 3; ++j) out[j] += r[0] * c[0][j] + r[1] * c[1][j] + r[2] * c[2][j];
+    };
+    if (der) {
+        for (int i = 0; i < 12; ++i) der[i] = 0.0;
+        rowcm(ad, q2 - q1, der + 0); rowcm(ad, q0 - q2, der + 3); rowcm(ad, q1 - q0, der + 6);
+        rowcm(ad + 3, q1 - q3, der + 0); rowcm(ad + 3, q3 - q0, der + 3); rowcm(ad + 3, q0 - q1, der + 9);
+    }
+    if (hess) {
+        for (int i = 0; i < 144; ++i) hess[i] = 0.0;
+        double vq[3][3][3], wq[3][3][3];
+        cm(q0 - q2, vq[0]); cm(q1 - q0, vq[1]); cm(q2 - q1, vq[2]);
+        cm(q0 - q1, wq[0]); cm(q1 - q3, wq[1]); cm(q3 - q0, wq[2]);
+        const int vi[3] = {3, 6, 0}, wi[3] = {9, 0, 3};
+        // blk(r0, c0) += A^T B C  /  A B  /  B C patterns over 3x3 blocks
+        auto blk = [&](int r0, int c0, const double (*L)[3], bool LT, int br, int bc, const double (*R)[3], double sg) {
+            // L^T (or L) * ah[br.., bc..] * R (R may be null = identity)
+            double t[3][3];
+            for (int i = 0; i < 3; ++i)
+                for (int j = 0; j < 3; ++j) {
+                    double a = 0;
+                    for (int k = 0; k < 3; ++k) a += (L ? (LT ? L[k][i] : L[i][k]) : (i == k ? 1.0 : 0.0)) * ah[br + k][bc + j];
+                    t[i][j] = a;
+                }
+            for (int i = 0; i < 3; ++i)
+                for (int j = 0; j < 3; ++j) {
+                    double a = 0;
+                    for (int k = 0; k < 3; ++k) a += t[i][k] * (R ? R[k][j] : (k == j ? 1.0 : 0.0));
+                    hess[12 * (r0 + i) + c0 + j] += sg * a;
+                }
+        };
+        for (int i = 0; i < 3; ++i) {
+            for (int j = 0; j < 3; ++j) {
+                blk(vi[i], vi[j], vq[i], true, 0, 0, vq[j], 1.0);
+                blk(vi[i], wi[j], vq[i], true, 0, 3, wq[j], 1.0);
+                blk(wi[i], vi[j], wq[i], true, 3, 0, vq[j], 1.0);
+                blk(wi[i], wi[j], wq[i], true, 3, 3, wq[j], 1.0);
+            }
+            blk(vi[i], 3, vq[i], true, 0, 6, nullptr, 1.0);
+            blk(3, vi[i], nullptr, false, 6, 0, vq[i], 1.0);
+            blk(vi[i], 0, vq[i], true, 0, 6, nullptr, -1.0);
+            blk(0, vi[i], nullptr, false, 6, 0, vq[i], -1.0);
+            blk(wi[i], 3, wq[i], true, 3, 6, nullptr, 1.0);
+            blk(3, wi[i], nullptr, false, 6, 3, wq[i], 1.0);
+            blk(wi[i], 0, wq[i], true, 3, 6, nullptr, -1.0);
+            blk(0, wi[i], nullptr, false, 6, 3, wq[i], -1.0);
+        }
+        double d1[3][3], d2[3][3];
+        cm(V3(ad[0], ad[1], ad[2]), d1); cm(V3(ad[3], ad[4], ad[5]), d2);
+        auto add = [&](int r0, int c0, const double (*M)[3], double sg) {
+            for (int i = 0; i < 3; ++i) for (int j = 0; j < 3; ++j) hess[12 * (r0 + i) + c0 + j] += sg * M[i][j];
+        };
+        add(6, 3, d1, 1); add(0, 3, d1, -1); add(0, 6, d1, 1); add(3, 0, d1, 1); add(3, 6, d1, -1); add(6, 0, d1, -1);
+        add(9, 0, d2, 1); add(3, 0, d2, -1); add(3, 9, d2, 1); add(0, 3, d2, 1); add(0, 9, d2, -1); add(9, 3, d2, -1);
+    }
+    return theta;
+}
+// remainder(angle - rest, 2 pi) (HingeBending.cpp:38-42)
+double hinge_difference(double angle, double rest) { return std::remainder(angle - rest, 6.28318530717958647692); }
 
 // ----------------------------------------------------------------------------------
 // Barrier, distances and their classification (ContactMechanics.cpp)
@@ -464,12 +668,6 @@ int decode(const I4& c, int vid[4]) {
 void f0_SF(double x2, double e, double& f0) { f0 = x2 * (-std::sqrt(x2) / 3.0 + e) / (e * e) + e / 3.0; }    // :262-265
 void f1_SF_div(double x2, double e, double& r) { r = (-std::sqrt(x2) + 2.0 * e) / (e * e); }             // :267-270
 void f2_SF(double x2, double e, double& f2) { f2 = 2.0 * (e - std::sqrt(x2)) / (e * e); }                 // :272-275
-V3 normalized(const V3& v) {   // Eigen 3.4 normalized(): v / |v|, unchanged when |v| = 0
-    const double n2 = sqn(v);
-    if (!(n2 > 0.0)) return v;
-    const double n = std::sqrt(n2);
-    return V3(v.x / n, v.y / n, v.z / n);
-}
 
 // Tangent bases (FrictionKinematics.h:18-28,81-91,155-164,211-227) and closest points
 // (:30-41,93-114,166-172).  Columns of the 3x2 basis are b[0], b[1].
@@ -749,6 +947,74 @@ int gradient_and_hessian(Mesh& m, double kappa, Blocks& BH) {
             V3 f(fe[t][3 * a], fe[t][3 * a + 1], fe[t][3 * a + 2]);
             m.grad[m.tets[t][a]] = m.grad[m.tets[t][a]] + dt2 * f;
         }
+    // cloth triangles (IPCSolver.cpp:166-194): f = dt^2 area PFPX^T vec(P), H9 = area dt^2 PFPX^T Hq PFPX
+    std::vector<std::array<double, 9>> ft(m.Tr);
+    BH.h9.assign(m.Tr, {}); BH.d3.assign(m.Tr, {});
+#pragma omp parallel for schedule(dynamic, 256)
+    for (int t = 0; t < m.Tr; ++t) {
+        double P[6][9], f[6], pe[6], Hq[36];
+        tri_pfpx(m.dmt[t], P);
+        tri_deformation(m, t, f);
+        bw_pk1(f, m.stretchS, m.shearS, m.srate, pe);
+        for (int j = 0; j < 9; ++j) {
+            double acc = 0;
+            for (int k = 0; k < 6; ++k) acc += P[k][j] * pe[k];
+            ft[t][j] = m.dt * m.dt * m.area[t] * acc;
+        }
+        bw_hessian(f, m.stretchS, m.shearS, m.srate, Hq);
+        double T1[6][9];
+        for (int i = 0; i < 6; ++i)
+            for (int j = 0; j < 9; ++j) {
+                double acc = 0;
+                for (int k = 0; k < 6; ++k) acc += Hq[6 * i + k] * P[k][j];
+                T1[i][j] = acc;
+            }
+        const double sc = m.area[t] * m.dt * m.dt;
+        for (int i = 0; i < 9; ++i)
+            for (int j = 0; j < 9; ++j) {
+                double acc = 0;
+                for (int k = 0; k < 6; ++k) acc += P[k][i] * T1[k][j];
+                BH.h9[t][9 * i + j] = sc * acc;
+            }
+        BH.d3[t] = m.tris[t];
+    }
+    for (int t = 0; t < m.Tr; ++t)
+        for (int a = 0; a < 3; ++a) m.grad[m.tris[t][a]] = m.grad[m.tris[t][a]] + V3(ft[t][3 * a], ft[t][3 * a + 1], ft[t][3 * a + 2]);
+    // bending (IPCSolver.cpp:197-267): quadratic form D dt^2 (Q x I3) or the hinge angle
+    std::vector<std::array<double, 12>> fb(m.Nb);
+    std::vector<std::array<double, 144>> hb(m.Nb);
+#pragma omp parallel for schedule(dynamic, 256)
+    for (int h = 0; h < m.Nb; ++h) {
+        const I4& bv = m.bverts[h];
+        if (m.bmode == 0) {
+            const double sc = m.plate * m.dt * m.dt;
+            for (int a = 0; a < 4; ++a)
+                for (int d = 0; d < 3; ++d) {
+                    double acc = 0;
+                    for (int b = 0; b < 4; ++b) acc += (sc * m.bq[h][4 * a + b]) * m.x[bv[b]][d];
+                    fb[h][3 * a + d] = acc;
+                }
+            for (int i = 0; i < 144; ++i) hb[h][i] = 0.0;
+            for (int a = 0; a < 4; ++a)
+                for (int b = 0; b < 4; ++b)
+                    for (int d = 0; d < 3; ++d) hb[h][12 * (3 * a + d) + 3 * b + d] = sc * m.bq[h][4 * a + b];
+        } else {
+            double ag[12], aH[144];
+            const double th = edge_theta(m.x[bv[0]], m.x[bv[1]], m.x[bv[2]], m.x[bv[3]], ag, aH);
+            const double df = hinge_difference(th, m.brest[h]);
+            const double sc = m.plate * m.bweight[h];
+            for (int i = 0; i < 12; ++i) fb[h][i] = sc * df * ag[i];
+            for (int i = 0; i < 12; ++i)
+                for (int j = 0; j < 12; ++j) hb[h][12 * i + j] = sc * (ag[i] * ag[j] + df * aH[12 * i + j]);
+            make_pd<12>(hb[h].data());
+            for (int i = 0; i < 12; ++i) fb[h][i] *= m.dt * m.dt;
+            for (int i = 0; i < 144; ++i) hb[h][i] *= m.dt * m.dt;
+        }
+    }
+    for (int h = 0; h < m.Nb; ++h) {
+        for (int a = 0; a < 4; ++a) m.grad[m.bverts[h][a]] = m.grad[m.bverts[h][a]] + V3(fb[h][3 * a], fb[h][3 * a + 1], fb[h][3 * a + 2]);
+        BH.h12.push_back(hb[h]); BH.d4.push_back(m.bverts[h]);
+    }
     // ground (IPCSolver.cpp:270-293)
     for (int v : m.envAS) {
         const double dist = dot(m.gn, m.x[v]) - m.gd;
@@ -953,12 +1219,37 @@ void assemble_csr(Mesh& m, const Blocks& BH) {
 // ----------------------------------------------------------------------------------
 // computeEnergy (IPCSolver.cpp:322-456), tet + inertia + barrier subset
 // ----------------------------------------------------------------------------------
-void energy_terms(Mesh& m, double kappa, double out[6]) {
+void energy_terms(Mesh& m, double kappa, double out[8]) {
     std::vector<double> et(m.T);
 #pragma omp parallel for schedule(dynamic, 256)
     for (int t = 0; t < m.T; ++t) et[t] = snh_energy_density(deformation(m, t), m.u, m.r) * m.vol[t];
     double esnh = 0;
     for (int t = 0; t < m.T; ++t) esnh += et[t];
+    // Baraff-Witkin triangles and bending, before the dt^2 factor (IPCSolver.cpp:330-368)
+    std::vector<double> etr(m.Tr), ebd(m.Nb);
+#pragma omp parallel for schedule(dynamic, 256)
+    for (int t = 0; t < m.Tr; ++t) {
+        double f[6];
+        tri_deformation(m, t, f);
+        etr[t] = bw_energy_density(f, m.stretchS, m.shearS, m.srate) * m.area[t];
+    }
+#pragma omp parallel for schedule(dynamic, 256)
+    for (int h = 0; h < m.Nb; ++h) {
+        const I4& bv = m.bverts[h];
+        if (m.bmode == 0) {
+            double acc = 0;
+            for (int a = 0; a < 4; ++a)
+                for (int b = 0; b < 4; ++b) acc += m.bq[h][4 * a + b] * dot(m.x[bv[a]], m.x[bv[b]]);
+            ebd[h] = acc;
+        } else {
+            const double df = hinge_difference(edge_theta(m.x[bv[0]], m.x[bv[1]], m.x[bv[2]], m.x[bv[3]], nullptr, nullptr), m.brest[h]);
+            ebd[h] = 0.5 * m.plate * m.bweight[h] * df * df;
+        }
+    }
+    double ebw = 0, ebend = 0;
+    for (int t = 0; t < m.Tr; ++t) ebw += etr[t];
+    for (int h = 0; h < m.Nb; ++h) ebend += ebd[h];
+    if (m.bmode == 0) ebend *= 0.5 * m.plate;
     double de = 0;
     for (int v = 0; v < m.V; ++v) {
         de += sqn(m.x[v] - m.xt[v]) * m.mass[v] / 2.0;
@@ -989,12 +1280,15 @@ void energy_terms(Mesh& m, double kappa, double out[6]) {
     }
     out[0] = esnh; out[1] = de; out[2] = eg; out[3] = es; out[4] = ee;
     out[5] = friction_energy(m);
+    out[6] = ebw; out[7] = ebend;
     (void)kappa;
 }
 double energy(Mesh& m, double kappa) {
-    double t[6];
+    double t[8];
     energy_terms(m, kappa, t);
     double E = t[0];
+    E += t[6];
+    E += t[7];
     E *= m.dt * m.dt;
     E += t[1];
     E += kappa * (t[2] + t[3] + t[4]);
@@ -1420,6 +1714,22 @@ oracle_mesh* oracle_create(const oracle_scene_desc* d) {
     for (int s = 0; s < m.Vs; ++s) m.sv[s] = (int)d->surf_verts[s];
     m.se.resize(m.Es);
     for (int e = 0; e < m.Es; ++e) m.se[e] = {(int)d->surf_edges[2 * e], (int)d->surf_edges[2 * e + 1]};
+    m.Tr = d->Tr; m.Nb = d->Nb; m.bmode = d->bending;
+    m.tris.resize(m.Tr); m.dmt.resize(m.Tr); m.area.assign(d->areas, d->areas + m.Tr);
+    for (int t = 0; t < m.Tr; ++t) {
+        for (int k = 0; k < 3; ++k) m.tris[t][k] = d->triangles[3 * t + k];
+        for (int k = 0; k < 4; ++k) m.dmt[t][k] = d->dm_tri_inv[4 * t + k];
+    }
+    m.stretchS = d->stretch; m.shearS = d->shear; m.srate = d->strain_rate; m.plate = d->plate_rigidity;
+    m.bverts.resize(m.Nb);
+    for (int h = 0; h < m.Nb; ++h) for (int k = 0; k < 4; ++k) m.bverts[h][k] = d->bend_verts[4 * h + k];
+    if (m.bmode == 0) {
+        m.bq.resize(m.Nb);
+        for (int h = 0; h < m.Nb; ++h) for (int k = 0; k < 16; ++k) m.bq[h][k] = d->bend_q[16 * h + k];
+    } else {
+        m.brest.assign(d->bend_rest, d->bend_rest + m.Nb);
+        m.bweight.assign(d->bend_weight, d->bend_weight + m.Nb);
+    }
     m.dt = d->dt; m.u = d->length_rate; m.r = d->volume_rate; m.drag = d->drag;
     m.hhat = d->hhat; m.fhat = d->fhat; m.fric = d->friction;
     m.gn = V3(d->ground_normal[0], d->ground_normal[1], d->ground_normal[2]); m.gd = d->ground_d;
@@ -1471,7 +1781,7 @@ void oracle_get_csr(const oracle_mesh* om, int* outer, int* inner, double* val, 
     if (rhs) std::copy(m.rhs.begin(), m.rhs.end(), rhs);
 }
 double oracle_energy(oracle_mesh* om, double kappa) { return energy(om->m, kappa); }
-void oracle_energy_terms(oracle_mesh* om, double kappa, double t[6]) { energy_terms(om->m, kappa, t); }
+void oracle_energy_terms(oracle_mesh* om, double kappa, double t[8]) { energy_terms(om->m, kappa, t); }
 void oracle_friction_update(oracle_mesh* om, double kappa) { friction_initialize(om->m, kappa); }
 void oracle_get_friction(const oracle_mesh* om, int64_t counts[2], double* lam, double* coord, double* basis, double* env_lam) {
     const Mesh& m = om->m;
@@ -1532,6 +1842,25 @@ void oracle_snh_element(const double* Fp, double u, double r, double* P, double*
     for (int i = 0; i < 9; ++i) P[i] = Pm.m[i];
     snh_hessian(F, u, r, H9);
     *e = snh_energy_density(F, u, r);
+}
+void oracle_bw_element(const double* F6, double stretch, double shear, double k, double* P6, double* H36, double* e) {
+    bw_pk1(F6, stretch, shear, k, P6);
+    bw_hessian(F6, stretch, shear, k, H36);
+    *e = bw_energy_density(F6, stretch, shear, k);
+}
+double oracle_edge_theta(const double* p, double* g, double* H) {
+    return edge_theta(V3(p[0], p[1], p[2]), V3(p[3], p[4], p[5]), V3(p[6], p[7], p[8]), V3(p[9], p[10], p[11]), g, H);
+}
+void oracle_hinge_element(const double* p, double rest, double weight, double plate, double* g, double* H, double* e) {
+    double ag[12], aH[144];
+    const double th = edge_theta(V3(p[0], p[1], p[2]), V3(p[3], p[4], p[5]), V3(p[6], p[7], p[8]), V3(p[9], p[10], p[11]), ag, aH);
+    const double df = hinge_difference(th, rest);
+    const double sc = plate * weight;
+    for (int i = 0; i < 12; ++i) g[i] = sc * df * ag[i];
+    for (int i = 0; i < 12; ++i)
+        for (int j = 0; j < 12; ++j) H[12 * i + j] = sc * (ag[i] * ag[j] + df * aH[12 * i + j]);
+    make_pd<12>(H);
+    *e = 0.5 * plate * weight * df * df;
 }
 int oracle_make_pd(int n, double* H) {
     switch (n) {

@@ -31,6 +31,17 @@ typedef struct {
     const uint64_t* surf_edges; /* 2Es */
     double dt, length_rate, volume_rate, drag, hhat, fhat, friction;
     double ground_normal[3], ground_d;
+    /* cloth triangles and bending elements (0 / NULL for tet-only scenes) */
+    int Tr;                      /* mesh3D::triangleNum */
+    const int* triangles;        /* 3Tr */
+    const double* dm_tri_inv;    /* 4Tr column-major 2x2 */
+    const double* areas;         /* Tr (area x thickness) */
+    double stretch, shear, strain_rate, plate_rigidity;
+    int Nb, bending;             /* bending: 0 quadratic, 1 hinge */
+    const int* bend_verts;       /* 4Nb */
+    const double* bend_q;        /* 16Nb (bending 0) */
+    const double* bend_rest;     /* Nb (bending 1) */
+    const double* bend_weight;   /* Nb (bending 1) */
 } oracle_scene_desc;
 
 typedef struct oracle_mesh oracle_mesh;
@@ -53,10 +64,11 @@ int oracle_assemble(oracle_mesh* m, double kappa, int64_t* nnz);
 void oracle_get_gradient(const oracle_mesh* m, double* grad);
 void oracle_get_csr(const oracle_mesh* m, int* outer, int* inner, double* val, double* rhs);
 
-/* computeEnergy (without friction / cloth / bending terms for tet-only scenes). */
+/* computeEnergy. */
 double oracle_energy(oracle_mesh* m, double kappa);
-/* per-term energies: {SNH (undivided by dt^2), inertia+drag, ground barrier, self barrier, mollified EE} */
-void oracle_energy_terms(oracle_mesh* m, double kappa, double terms[6]);
+/* per-term energies: {SNH (before dt^2), inertia+drag, ground barrier, self barrier, mollified EE,
+ * friction, Baraff-Witkin triangles (before dt^2), bending (before dt^2)} */
+void oracle_energy_terms(oracle_mesh* m, double kappa, double terms[8]);
 /* Friction::initialize: lagged friction state from the current active set and positions. */
 void oracle_friction_update(oracle_mesh* m, double kappa);
 /* counts[0] self contacts, counts[1] ground contacts; lam (n), coord (2n), basis (6n: two
@@ -76,6 +88,12 @@ int oracle_is_intersected(oracle_mesh* m);
 void oracle_svd3(const double* F, double* U, double* s, double* V);
 void oracle_snh_element(const double* F, double u, double r, double* P, double* H9, double* energy);
 int oracle_make_pd(int n, double* H);  /* n in {3,6,9,12}; row-major */
+/* Baraff-Witkin triangle: F 3x2 column-major -> PK1 (6), projected 6x6 Hessian, energy density */
+void oracle_bw_element(const double* F6, double stretch, double shear, double k, double* P6, double* H36, double* e);
+/* edgeTheta: hinge points q0..q3 (12) -> angle, gradient (12), second-derivative matrix (144 row-major) */
+double oracle_edge_theta(const double* pts, double* g, double* H);
+/* hinge bending element before dt^2: gradient (12), makePD'd Hessian (144), energy */
+void oracle_hinge_element(const double* pts, double rest, double weight, double plate, double* g, double* H, double* e);
 void oracle_barrier(double d, double dhat, double* b, double* g, double* h);
 /* type: 0 PP(2 pts) 1 PE(3) 2 PT(4) 3 EE(4) 4 EE cross norm(4); pts 12 doubles */
 void oracle_distance(int type, const double* pts, double* d, double* g, double* H);

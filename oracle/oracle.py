@@ -27,7 +27,22 @@ class SceneDesc(C.Structure):
                 ("surf_verts", _u64p), ("surf_edges", _u64p),
                 ("dt", C.c_double), ("length_rate", C.c_double), ("volume_rate", C.c_double),
                 ("drag", C.c_double), ("hhat", C.c_double), ("fhat", C.c_double), ("friction", C.c_double),
-                ("ground_normal", C.c_double * 3), ("ground_d", C.c_double)]
+                ("ground_normal", C.c_double * 3), ("ground_d", C.c_double),
+                ("Tr", C.c_int), ("triangles", _ip), ("dm_tri_inv", _dp), ("areas", _dp),
+                ("stretch", C.c_double), ("shear", C.c_double), ("strain_rate", C.c_double),
+                ("plate_rigidity", C.c_double), ("Nb", C.c_int), ("bending", C.c_int), ("bend_verts", _ip),
+                ("bend_q", _dp), ("bend_rest", _dp), ("bend_weight", _dp)]
+
+
+def _fill_cloth(d, keep, scene):
+    """Cloth fields of the descriptor; the arrays are appended to ``keep``."""
+    from cipc import scenes
+    for name, val in scenes.cloth_arrays(scene).items():
+        if isinstance(val, np.ndarray):
+            keep.append(val)
+            setattr(d, name, val.ctypes.data_as(_ip if val.dtype == np.int32 else _dp))
+        else:
+            setattr(d, name, val)
 
 
 def build(force: bool = False) -> str:
@@ -69,6 +84,10 @@ def lib():
         L.oracle_is_intersected.argtypes = [C.c_void_p]
         L.oracle_svd3.argtypes = [_dp, _dp, _dp, _dp]
         L.oracle_snh_element.argtypes = [_dp, C.c_double, C.c_double, _dp, _dp, _dp]
+        L.oracle_bw_element.argtypes = [_dp, C.c_double, C.c_double, C.c_double, _dp, _dp, _dp]
+        L.oracle_edge_theta.restype = C.c_double
+        L.oracle_edge_theta.argtypes = [_dp, _dp, _dp]
+        L.oracle_hinge_element.argtypes = [_dp, C.c_double, C.c_double, C.c_double, _dp, _dp, _dp]
         L.oracle_make_pd.restype = C.c_int
         L.oracle_make_pd.argtypes = [C.c_int, _dp]
         L.oracle_barrier.argtypes = [C.c_double, C.c_double, _dp, _dp, _dp]
@@ -128,6 +147,7 @@ class OracleMesh:
         d.hhat, d.fhat, d.friction = scene.hhat, scene.fhat, m.friction
         d.ground_normal[0], d.ground_normal[1], d.ground_normal[2] = 0.0, 1.0, 0.0
         d.ground_d = -1.0
+        _fill_cloth(d, self._keep, scene)
         self.h = L.oracle_create(C.byref(d))
         if threads:
             L.oracle_set_threads(self.h, threads)
@@ -185,8 +205,9 @@ class OracleMesh:
         return lib().oracle_energy(self.h, kappa)
 
     def energy_terms(self, kappa):
-        """[elastic, inertia, ground barrier, self barrier, mollified barrier, friction]"""
-        t = np.zeros(6)
+        """[elastic, inertia, ground barrier, self barrier, mollified barrier, friction,
+        cloth triangles, bending]"""
+        t = np.zeros(8)
         lib().oracle_energy_terms(self.h, kappa, dptr(t))
         return t
 
@@ -233,6 +254,29 @@ def snh_element(F, u, r):
     P, H, e = np.zeros(9), np.zeros(81), np.zeros(1)
     lib().oracle_snh_element(dptr(Fc), u, r, dptr(P), dptr(H), dptr(e))
     return P.reshape(3, 3).T, H.reshape(9, 9), e[0]
+
+
+def bw_element(F, stretch, shear, k):
+    """Baraff-Witkin triangle: F (3x2) -> PK1 (3x2), projected 6x6 Hessian over
+    column-major vec(F), energy density."""
+    Fc = _d(np.asarray(F).T.reshape(-1))
+    P, H, e = np.zeros(6), np.zeros(36), np.zeros(1)
+    lib().oracle_bw_element(dptr(Fc), stretch, shear, k, dptr(P), dptr(H), dptr(e))
+    return P.reshape(2, 3).T, H.reshape(6, 6), e[0]
+
+
+def edge_theta(pts):
+    p = _d(np.asarray(pts).reshape(-1))
+    g, H = np.zeros(12), np.zeros(144)
+    th = lib().oracle_edge_theta(dptr(p), dptr(g), dptr(H))
+    return th, g, H.reshape(12, 12)
+
+
+def hinge_element(pts, rest, weight, plate):
+    p = _d(np.asarray(pts).reshape(-1))
+    g, H, e = np.zeros(12), np.zeros(144), np.zeros(1)
+    lib().oracle_hinge_element(dptr(p), rest, weight, plate, dptr(g), dptr(H), dptr(e))
+    return g, H.reshape(12, 12), e[0]
 
 
 def make_pd(H):

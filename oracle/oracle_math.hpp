@@ -195,10 +195,15 @@ inline void sym_eig(double* A, double* w, double* Z) {
 // IglUtils::makePD (CPU IPC/ContactMechanics.h:27-45): if the smallest eigenvalue is
 // >= 0 the matrix is returned untouched; otherwise the leading (negative) eigenvalues
 // are set to zero and the matrix is rebuilt as Z D Z^T.  H is row-major N x N.
+//
+// Eigen's SelfAdjointEigenSolver reads only the lower triangle, so the decomposition is
+// of the matrix mirrored from H's lower triangle; an unprojected H is returned as given
+// (it may be non-symmetric: the hinge-bending Hessian is).
 template <int N>
 inline bool make_pd(double* H) {
     double A[N * N], w[N], Z[N * N];
-    std::memcpy(A, H, sizeof(A));
+    for (int i = 0; i < N; ++i)
+        for (int j = 0; j <= i; ++j) A[i * N + j] = A[j * N + i] = H[i * N + j];
     sym_eig<N>(A, w, Z);
     if (w[0] >= 0.0) return false;
     for (int i = 0; i < N; ++i) {

@@ -173,7 +173,7 @@ def test_energy_matches_oracle(pair):
     g.active_set()
     o.active_set()
     tg, to = g.energy_terms(kappa), o.energy_terms(kappa)
-    for k in range(5):
+    for k in range(6):
         assert abs(tg[k] - to[k]) <= REL * max(abs(to[k]), 1e-300) or (tg[k] == to[k]), (pair["name"], k, tg[k], to[k])
     eg, eo = g.energy(kappa), o.energy(kappa)
     assert abs(eg - eo) <= REL * abs(eo)
@@ -222,6 +222,46 @@ def test_refitted_trees_after_a_step(pair):
         assert bg[k] == bo[k], (pair["name"], k, bg[k], bo[k])
     g.set_state(x=s.x)
     o.set_state(x=s.x)
+
+
+@pytest.mark.parametrize("name", ["two_cubes", "four_20cubes"])
+def test_friction_matches_oracle(gpu, O, name):
+    """Lagged friction (Friction::initialize, addGradient, addHessian, energy): lagged state,
+    assembled CSR and gradient, and energy, from a state whose previous positions are slid
+    tangentially so that contacts are both sticking and sliding."""
+    from cipc import scenes
+    s = _scenes()[name]
+    rng = np.random.default_rng(3)
+    eps = np.sqrt(s.fhat) * s.material.dt
+    vprev = s.x.copy()
+    upper = s.x[:, 1] > np.median(s.x[:, 1])
+    vprev[upper] -= np.array([1.5 * eps, 0.0, -0.7 * eps])
+    vprev += 0.6 * eps * rng.standard_normal(s.x.shape)
+    xt, _ = scenes.newton_probe_state(s)
+    kappa = scenes.suggest_kappa(s.hhat, s.bbox_diag2, s.mean_mass)
+    g = gpu.GpuContext(s, xtilde=xt)
+    o = O.OracleMesh(s, xtilde=xt)
+    try:
+        g.set_state(V_prev=vprev)
+        o.set_state(V_prev=vprev)
+        g.active_set()
+        o.active_set()
+        g.friction_update(kappa)
+        o.friction_update(kappa)
+        fg, fo = g.friction_state(), o.friction_state()
+        assert len(fg["lam"]) == len(fo["lam"]) > 0 and len(fg["env_lam"]) == len(fo["env_lam"])
+        for k in ("lam", "coord", "basis", "env_lam"):
+            assert rel_err(fg[k], fo[k]) <= REL, (name, k, rel_err(fg[k], fo[k]))
+        rg, ro = g.assemble(kappa), o.assemble(kappa)
+        assert np.array_equal(rg["outer"], ro["outer"]) and np.array_equal(rg["inner"], ro["inner"]), name
+        assert rel_err(rg["val"], ro["val"]) <= REL, (name, rel_err(rg["val"], ro["val"]))
+        assert rel_err(rg["grad"], ro["grad"]) <= REL, (name, rel_err(rg["grad"], ro["grad"]))
+        tg, to = g.energy_terms(kappa), o.energy_terms(kappa)
+        assert to[5] > 0 and abs(tg[5] - to[5]) <= REL * abs(to[5]), (name, tg[5], to[5])
+        assert abs(g.energy(kappa) - o.energy(kappa)) <= REL * abs(o.energy(kappa))
+    finally:
+        g.close()
+        o.close()
 
 
 def test_newton_iteration_timing(pair):
