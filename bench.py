@@ -69,6 +69,36 @@ def cpu_baseline(scene, xt, vprev, p, kappa, budget_s=15.0, threads=16):
                       f"on the oracle (CPU restatement of the reference path), OpenMP"}
 
 
+def pcg_leg(ctx, kappa, tol=1e-6, max_iter=10000, repeats=3):
+    """Newton linear solve of the same frozen state's system (NewtonLinearSystem::solve,
+    NewtonLinearSystem.cpp:21-56) to the reference's 1e-6 relative residual: FP64
+    block-Jacobi PCG against FP32 PCG inside FP64 refinement; device time from HIP events,
+    median of `repeats` solves."""
+    import numpy as np
+    ctx.assemble(kappa)
+    out = {"tol": tol, "system": "the frozen state's assembled Newton system (lower CSR, n = 3V)"}
+    for name, prec in (("plain_fp64_jacobi", 0), ("mixed_fp32_fp64_jacobi", 1)):
+        runs = []
+        for _ in range(repeats):
+            try:
+                _, st = ctx.solve(precision=prec, rel_tol=tol, max_iter=max_iter)
+                runs.append(st)
+            except RuntimeError as e:
+                runs.append({"error": str(e)})
+                break
+        ok = [r for r in runs if "error" not in r]
+        if ok:
+            ms = float(np.median([r["solve_ms"] for r in ok]))
+            out[name] = {"ms": round(ms, 3), "iterations": int(ok[-1]["iterations"]),
+                         "refinements": int(ok[-1]["refinements"]), "rel_residual": float(ok[-1]["rel_residual"])}
+        else:
+            out[name] = {"error": runs[-1]["error"]}
+    if "ms" in out.get("plain_fp64_jacobi", {}) and "ms" in out.get("mixed_fp32_fp64_jacobi", {}):
+        out["mixed_speedup"] = round(out["plain_fp64_jacobi"]["ms"] / out["mixed_fp32_fp64_jacobi"]["ms"], 3)
+    out["learned"] = None
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -76,6 +106,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--cpu-budget", type=float, default=15.0)
+    ap.add_argument("--no-pcg", action="store_true", help="skip the linear-solve leg")
     ap.add_argument("--traffic", default=None, help="JSON file with measured HBM bytes per launch of the elastic stage")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -105,6 +136,7 @@ def main():
     mean = {k: float(np.mean([s[k] for s in stages])) for k in
             ("broadphase_ms", "assembly_ms", "elastic_ms", "ccd_ms", "energy_ms", "total_ms")}
     last = stages[-1]
+    pcg = pcg_leg(ctx, kappa) if not args.no_pcg else None
     ctx.close()
 
     bytes_el, nnz_el = elastic_algorithmic_bytes(scene)
@@ -133,6 +165,8 @@ def main():
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
                      "algorithmic_bytes": int(bytes_el), "elastic_nnz": int(nnz_el)},
     }
+    if pcg is not None:
+        out["pcg"] = pcg
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(scene, xt, vprev, p, kappa, budget_s=args.cpu_budget)
     print(json.dumps(out))

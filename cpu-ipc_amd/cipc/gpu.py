@@ -33,7 +33,8 @@ EXPORTS = (
     "cipc_gpu_set_state", "cipc_gpu_get_positions", "cipc_gpu_set_direction", "cipc_gpu_active_set",
     "cipc_gpu_get_active_set", "cipc_gpu_set_active_set", "cipc_gpu_assemble", "cipc_gpu_get_gradient",
     "cipc_gpu_get_csr", "cipc_gpu_energy", "cipc_gpu_energy_terms", "cipc_gpu_step_bound",
-    "cipc_gpu_friction_update", "cipc_gpu_get_friction", "cipc_gpu_solve", "cipc_gpu_save_positions", "cipc_gpu_step_forward",
+    "cipc_gpu_friction_update", "cipc_gpu_get_friction", "cipc_gpu_set_soft_targets", "cipc_gpu_kappa_gradients",
+    "cipc_gpu_close_constraints", "cipc_gpu_clear_close_constraints", "cipc_gpu_solve", "cipc_gpu_save_positions", "cipc_gpu_step_forward",
     "cipc_gpu_is_intersected", "cipc_gpu_newton_iteration",
     "cipc_gpu_eval_snh", "cipc_gpu_eval_cloth", "cipc_gpu_eval_hinge", "cipc_gpu_eval_distance", "cipc_gpu_eval_make_pd", "cipc_gpu_eval_contact_pd",
     "cipc_gpu_eval_accd",
@@ -113,6 +114,10 @@ def load(path: str = LIB_PATH):
         "cipc_gpu_energy": [C.c_void_p, C.c_double, _dp],
         "cipc_gpu_energy_terms": [C.c_void_p, C.c_double, _dp],
         "cipc_gpu_friction_update": [C.c_void_p, C.c_double],
+        "cipc_gpu_set_soft_targets": [C.c_void_p, C.c_int, _ip, _dp, C.c_double],
+        "cipc_gpu_kappa_gradients": [C.c_void_p, _dp, _i64p],
+        "cipc_gpu_close_constraints": [C.c_void_p, C.c_double, _ip, _i64p],
+        "cipc_gpu_clear_close_constraints": [C.c_void_p],
         "cipc_gpu_solve": [C.c_void_p, C.c_int, C.c_double, C.c_int, _dp, C.POINTER(SolveStats)],
         "cipc_gpu_get_friction": [C.c_void_p, C.POINTER(C.c_int64), _dp, _dp, _dp, _dp],
         "cipc_gpu_step_bound": [C.c_void_p, _dp],
@@ -277,10 +282,35 @@ class GpuContext:
 
     def energy_terms(self, kappa):
         """[elastic, inertia, ground barrier, self barrier, mollified barrier, friction,
-        cloth triangles, bending]"""
-        t = np.zeros(8)
+        cloth triangles, bending, soft targets]"""
+        t = np.zeros(9)
         _check(load().cipc_gpu_energy_terms(self.h, kappa, _dptr(t)))
         return t
+
+    def kappa_gradients(self):
+        """initKappa's reductions: (sum g_c . g_E, sum |g_c|^2, number of constraints)."""
+        out = np.zeros(2)
+        n = np.zeros(1, np.int64)
+        _check(load().cipc_gpu_kappa_gradients(self.h, _dptr(out), n.ctypes.data_as(_i64p)))
+        return out[0], out[1], int(n[0])
+
+    def close_constraints(self, dtol):
+        """postLineSearch's bookkeeping: (a recorded constraint got closer, (#ground, #self) recorded)."""
+        closer = C.c_int(0)
+        cnt = np.zeros(2, np.int64)
+        _check(load().cipc_gpu_close_constraints(self.h, dtol, C.byref(closer), cnt.ctypes.data_as(_i64p)))
+        return bool(closer.value), (int(cnt[0]), int(cnt[1]))
+
+    def clear_close_constraints(self):
+        _check(load().cipc_gpu_clear_close_constraints(self.h))
+
+    def set_soft_targets(self, idx, targets, weight):
+        """SoftBoundaryCondition: 0.5 weight |x_v - target|^2 on the listed (distinct, free)
+        vertices; an empty list removes it."""
+        idx = _i(np.asarray(idx).reshape(-1))
+        t = _d(np.asarray(targets).reshape(-1, 3))
+        n = len(idx)
+        _check(load().cipc_gpu_set_soft_targets(self.h, n, _iptr(idx) if n else None, _dptr(t) if n else None, weight))
 
     def solve(self, precision=0, rel_tol=1e-6, max_iter=10000):
         """Newton linear solve of the last assembled system; returns (direction (V,3), stats).

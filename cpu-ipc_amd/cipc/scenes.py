@@ -496,10 +496,8 @@ def four_cubes20(seed: int = 1, velocity_scale: float = 0.0, rotate: str = "yaw"
 
 def suggest_kappa(hhat: float, bbox_diag2: float, mean_mass: float) -> float:
     """suggestKappa (IPCSolver.cpp:751-755)."""
-    d = 1.0e-16 * bbox_diag2
-    t = d - hhat
-    h_b = (math.log(d / hhat) * -2.0 - t * 4.0 / d) + 1.0 / (d * d) * (t * t)
-    return 1e13 * mean_mass / (4.0e-16 * bbox_diag2 * h_b)
+    from cipc.kappa import suggest_kappa as sk
+    return sk(hhat, bbox_diag2, mean_mass)
 
 
 def newton_probe_state(scene: Scene, seed: int = 7, scale: float = 1.0):
@@ -591,7 +589,9 @@ def cloth_on_sphere(n: int = 256, size: float = 1.0, sphere_cells: int = 10, rad
     the rest, ``layer_gap`` above it, shifted by half a cell in x, so that the two layers
     are in contact everywhere they overlap (a frozen, self-contact-heavy state of a
     cloth dropped on the ball).  In-plane positions carry a seeded +-15% (of a cell)
-    jitter."""
+    jitter, heights a +-5% (of ``layer_gap``) one: exactly planar layers make many
+    Hessian entries mathematically zero, and whether such an entry comes out as an exact
+    zero (and is dropped from the CSR) then depends on rounding order."""
     rng = np.random.default_rng(seed)
     mat = Material(youngs=1e5, friction=0.2)
     cm = ClothMaterial(youngs=1e4, thickness=1e-3, plate_rigidity_override=1e-4)
@@ -616,12 +616,15 @@ def cloth_on_sphere(n: int = 256, size: float = 1.0, sphere_cells: int = 10, rad
     jit = rng.uniform(-0.15, 0.15, (m * m, 2)) * cell
     xcl = x[Vb:]
     xcl[:, 0] += jit[:, 0]
+    xcl[:, 1] += rng.uniform(-0.05, 0.05, m * m) * layer_gap
     if fold is not None:
         up = b > fold
         keep_z = b == fold + 1        # the crease row stays over row `fold`
         zi = np.where(up, 2 * fold + 1 - b, b)
         zj = np.where(keep_z, fold, zi)
-        xcl[:, 2] = -0.5 * size + zj * cell + np.where(keep_z | (b == fold), 0.0, jit[:, 1])
+        crease = keep_z | (b == fold)   # the crease rows share a z, up to a +-5% (of layer_gap) jitter
+        zjit = np.where(crease, rng.uniform(-0.05, 0.05, m * m) * layer_gap, jit[:, 1])
+        xcl[:, 2] = -0.5 * size + zj * cell + zjit
         xcl[up, 0] += 0.5 * cell
         xcl[up, 1] += layer_gap
     else:

@@ -163,6 +163,7 @@ struct StaticPattern {
     // Elements e: tets [0, T), triangles [T, T+Tr), bending elements [T+Tr, T+Tr+Nb).
     DBuf<int> slot_h;       // 10 per element: slot of (element, local block), -1 when not stored
     DBuf<int> slot_g;       // 4 per element: slot of (element, corner)
+    DBuf<int> vtet;         // V: tet corners of vertex v (the first slots of its run)
     DBuf<int> diag_block;   // V: block index of (v, v) in the static pattern
 };
 

@@ -226,6 +226,8 @@ int cipc_gpu_upload_scene(cipc_gpu_context* c, const cipc_scene_desc* d) {
         c->face_tree.age = c->edge_tree.age = kMaxRefits;
         c->n_self = c->n_ee = c->n_ccd = c->n_env = 0;   // sets and lagged state of an earlier scene
         c->n_fself = c->n_fenv = 0;
+        c->n_soft = 0; c->soft_w = 0.0;
+        c->n_cenv = c->n_cself = 0;
         c->assembled = false;
     });
 }
@@ -392,7 +394,59 @@ int cipc_gpu_get_csr(cipc_gpu_context* c, int* outer, int* inner, double* values
     });
 }
 
-int cipc_gpu_energy_terms(cipc_gpu_context* c, double kappa, double terms[8]) {
+int cipc_gpu_set_soft_targets(cipc_gpu_context* c, int n, const int* vertices, const double* targets, double weight) {
+    return guard([&] {
+        require(c && c->have_scene, "no scene uploaded");
+        require(n >= 0 && (n == 0 || (vertices && targets)), "invalid soft boundary arguments");
+        bind(*c);
+        if (n > 0) {
+            // validateSoftState / initialize (BoundaryConditions.cpp:36-58,64-110)
+            require(std::isfinite(weight) && weight > 0.0, "soft boundary weight must be finite and positive");
+            std::vector<char> seen(c->sc.V, 0);
+            for (int i = 0; i < n; ++i) {
+                const int v = vertices[i];
+                require(v >= 0 && v < c->sc.V, "soft boundary vertex index out of range");
+                require(is_free(c->sc.h_bt[v]), "soft boundary vertices must be free");
+                require(!seen[v], "soft boundary vertices must be distinct");
+                seen[v] = 1;
+                for (int k = 0; k < 3; ++k) require(std::isfinite(targets[3 * i + k]), "soft boundary target must be finite");
+            }
+        }
+        upload(c->soft_idx, vertices, (size_t)n, c->stream);
+        upload(c->soft_t, targets, 3 * (size_t)n, c->stream);
+        CIPC_HIP(hipStreamSynchronize(c->stream));
+        c->n_soft = n;
+        c->soft_w = n > 0 ? weight : 0.0;
+    });
+}
+
+int cipc_gpu_kappa_gradients(cipc_gpu_context* c, double out[2], int64_t* n_constraints) {
+    return guard([&] {
+        require(c && c->have_scene && out, "no scene uploaded or null output");
+        bind(*c);
+        const int64_t n = kappa_gradients(*c, out);
+        if (n_constraints) *n_constraints = n;
+    });
+}
+
+int cipc_gpu_close_constraints(cipc_gpu_context* c, double dtol, int* closer, int64_t counts[2]) {
+    return guard([&] {
+        require(c && c->have_scene && closer, "no scene uploaded or null output");
+        bind(*c);
+        int64_t cnt[2];
+        *closer = close_constraints(*c, dtol, cnt);
+        if (counts) { counts[0] = cnt[0]; counts[1] = cnt[1]; }
+    });
+}
+
+int cipc_gpu_clear_close_constraints(cipc_gpu_context* c) {
+    return guard([&] {
+        require(c && c->have_scene, "no scene uploaded");
+        c->n_cenv = c->n_cself = 0;
+    });
+}
+
+int cipc_gpu_energy_terms(cipc_gpu_context* c, double kappa, double terms[9]) {
     return guard([&] {
         require(c && c->have_scene && terms, "no scene uploaded or null output");
         bind(*c);
@@ -404,7 +458,7 @@ int cipc_gpu_energy(cipc_gpu_context* c, double kappa, double* e) {
     return guard([&] {
         require(c && c->have_scene && e, "no scene uploaded or null output");
         bind(*c);
-        double t[8];
+        double t[9];
         energy_terms(*c, kappa, t);
         *e = total_energy(*c, kappa, t);
     });
@@ -524,7 +578,7 @@ int cipc_gpu_newton_iteration(cipc_gpu_context* c, double kappa, cipc_iteration_
         double sb[5];
         step_bound(*c, sb);
         CIPC_HIP(hipEventRecord(ev[3], c->stream));
-        double et[8];
+        double et[9];
         energy_terms(*c, kappa, et);
         CIPC_HIP(hipEventRecord(ev[4], c->stream));
         CIPC_HIP(hipEventSynchronize(ev[4]));

@@ -92,6 +92,17 @@ struct Context {
     // lambda, closest-point coordinates (2) and tangent basis (6: column 0 xyz, column 1 xyz);
     // per ground contact the vertex and lambda
     int64_t n_fself = 0, n_fenv = 0;
+    // soft target-position penalty (SoftBoundaryCondition, BoundaryConditions.h:54-67)
+    int n_soft = 0;
+    double soft_w = 0.0;
+    DBuf<int> soft_idx;
+    DBuf<double> soft_t;       // 3 n_soft
+    // barrier-stiffness control: initKappa vectors, postLineSearch close-constraint records
+    DBuf<double> kgE, kgc, kvals;
+    int64_t n_cenv = 0, n_cself = 0;
+    DBuf<int> cenv;
+    DBuf<double> cenv_val, cself_val;
+    DBuf<int4> cself;
     DBuf<int4> fr_self;
     DBuf<double> fr_lam, fr_coord, fr_basis;
     DBuf<int> fr_env;
@@ -159,13 +170,19 @@ void sort_u64_pairs(Context& c, const unsigned long long* kin, unsigned long lon
                     const int* vin, int* vout, int n, int bits);
 void sort_i32_pairs(Context& c, const int* kin, int* kout, const int* vin, int* vout, int n, int bits);
 void sort_u32_pairs(Context& c, const unsigned* kin, unsigned* kout, const int* vin, int* vout, int n, int bits);
+// sums vertex-keyed 3-vector records (keys sorted, record order within a key) into grad
+void add_gradient_records_into(Context& c, int n, const int* k, const int* idx, const double* val, double* grad);
+// initKappa's reductions {sum g_c.g_E, sum |g_c|^2}; returns the ground + self constraint count
+int64_t kappa_gradients(Context& c, double out[2]);
+// postLineSearch's close-constraint check and records; returns 1 when a record got closer
+int close_constraints(Context& c, double dtol, int64_t counts[2]);
 bool any_intersection(Context& c);
 // ccd.hip
 void step_bound(Context& c, double out[5]);
 // energy.hip
-void energy_terms(Context& c, double kappa, double out[8]);
+void energy_terms(Context& c, double kappa, double out[9]);
 void cloth_energy_partials(Context& c, double* part_bw, double* part_bend);
-double total_energy(const Context& c, double kappa, const double t[8]);
+double total_energy(const Context& c, double kappa, const double t[9]);
 
 // Deterministic energy reductions: a fixed grid of kEnergyGrid work-groups of kEnergyBlock
 // threads, each summing a grid-strided subset in index order, then a shared-memory tree;

@@ -114,6 +114,9 @@ void build_static_pattern(Context& c) {
     for (size_t j = 0; j < vsrc.size(); ++j) slot_g[vsrc[j]] = (int)j;
     up(sp.col_ptr, col_ptr); up(sp.row, row); up(sp.gptr, gptr); up(sp.vptr, vptr); up(sp.diag_block, diag);
     up(sp.slot_h, slot_h); up(sp.slot_g, slot_g);
+    std::vector<int> vtet(V, 0);
+    for (int i = 0; i < 4 * T; ++i) vtet[s.h_tets[i]]++;
+    up(sp.vtet, vtet);
     c.stagH.resize((size_t)9 * std::max<size_t>(gsrc.size(), 1));
     c.stagG.resize((size_t)3 * std::max<size_t>(vsrc.size(), 1));
     c.sbv.resize((size_t)9 * std::max(sp.nblocks, 1));
@@ -271,6 +274,22 @@ __global__ __launch_bounds__(256) void vertex_gather_kernel(int V, int T, const 
     st3(grad, v, g);
 }
 
+// Soft targets: g += w (x - t), w I on the vertex's diagonal block (addSoftDerivatives,
+// BoundaryConditions.cpp:175-208).  Soft vertices are distinct and free.
+__global__ void soft_kernel(int n, const int* __restrict__ idx, const double* __restrict__ tgt, double w,
+                            const double* __restrict__ x, const int* __restrict__ diag_block, double* __restrict__ grad,
+                            double* __restrict__ sbv, unsigned short* __restrict__ sbf) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int v = idx[i];
+    st3(grad, v, ld3(grad, v) + w * (ld3(x, v) - ld3(tgt, i)));
+    const int k = diag_block[v];
+    sbv[(size_t)9 * k + 0] += w;
+    sbv[(size_t)9 * k + 4] += w;
+    sbv[(size_t)9 * k + 8] += w;
+    sbf[k] |= (1u << 0) | (1u << 4) | (1u << 8);
+}
+
 void elastic_assemble(Context& c, hipStream_t st) {
     Scene& s = c.sc;
     const double dt2 = s.dt * s.dt;
@@ -290,6 +309,11 @@ void elastic_assemble(Context& c, hipStream_t st) {
     hipLaunchKernelGGL(vertex_gather_kernel, dim3(blocks_for(s.V, 256)), dim3(256), 0, st, s.V, s.T, s.x.p, s.xt.p,
                        s.vprev.p, s.mass.p, s.drag, dt2, c.sp.vptr.p, c.stagG.p, c.grad.p);
     CIPC_CHECK_LAUNCH();
+    if (c.n_soft > 0) {
+        hipLaunchKernelGGL(soft_kernel, dim3(blocks_for(c.n_soft, 256)), dim3(256), 0, st, c.n_soft, c.soft_idx.p, c.soft_t.p,
+                           c.soft_w, s.x.p, c.sp.diag_block.p, c.grad.p, c.sbv.p, c.sbf.p);
+        CIPC_CHECK_LAUNCH();
+    }
 }
 
 // ---------------------------------------------------------------------------------

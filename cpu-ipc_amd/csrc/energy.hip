@@ -1,7 +1,7 @@
 // Line-search energy on the device.  Compiled with -ffp-contract=off.
 //
 // Reference: computeEnergy (CPU IPC/IPCSolver.cpp:322-456):
-//   E = dt^2 (E_SNH + E_BW + E_bend) + sum m|x - x~|^2/2 + sum drag m|x - V_prev|^2/2
+//   E = dt^2 (E_SNH + E_BW + E_bend) + sum m|x - x~|^2/2 + sum drag m|x - V_prev|^2/2 + E_soft
 //       + kappa (sum_ground b + sum_self b * multiplicity + sum_EE e b) + E_friction
 // with getObjEnergy_StableNHK2_3D (Elasticity.cpp:695-727) and the barrier sums of
 // Evaluate_*ConstraintVals + compute_b (ContactMechanics.cpp:2478-2544,775-778).
@@ -100,6 +100,12 @@ __global__ __launch_bounds__(kEB) void e_mollified(int n, const int4* __restrict
     }
     block_sum_store(acc, part, 0);
 }
+__global__ __launch_bounds__(kEB) void e_soft(int n, const int* __restrict__ idx, const double* __restrict__ tgt, double w,
+                                              const double* __restrict__ x, double* __restrict__ part) {
+    double acc = 0.0;
+    for (int i = blockIdx.x * kEB + threadIdx.x; i < n; i += kEG * kEB) acc += 0.5 * w * sqn(ld3(x, idx[i]) - ld3(tgt, i));
+    block_sum_store(acc, part, 0);
+}
 __global__ __launch_bounds__(kEB) void sum_partials(const double* __restrict__ part, int nterms, double* __restrict__ out) {
     __shared__ double sh[kEB];
     for (int t = 0; t < nterms; ++t) {
@@ -114,12 +120,12 @@ __global__ __launch_bounds__(kEB) void sum_partials(const double* __restrict__ p
     }
 }
 
-void energy_terms(Context& c, double kappa, double out[8]) {
+void energy_terms(Context& c, double kappa, double out[9]) {
     (void)kappa;
     Scene& s = c.sc;
     static_assert(kEG == kEB, "partials are summed by one work-group");
     DBuf<double>& part = c.dscratch;
-    part.resize((size_t)10 * kEG);
+    part.resize((size_t)11 * kEG);
     double* P = part.p;
     hipLaunchKernelGGL(e_snh, dim3(kEG), dim3(kEB), 0, c.stream, s.T, s.tets.p, s.x.p, s.dminv.p, s.vol.p, s.u, s.r, P);
     hipLaunchKernelGGL(e_inertia, dim3(kEG), dim3(kEB), 0, c.stream, s.V, s.x.p, s.xt.p, s.vprev.p, s.mass.p, s.drag, P + kEG);
@@ -131,24 +137,29 @@ void energy_terms(Context& c, double kappa, double out[8]) {
     friction_energy_partials(c, P + 5 * kEG);
     // cloth triangles and bending (IPCSolver.cpp:330-368)
     cloth_energy_partials(c, P + 7 * kEG, P + 8 * kEG);
-    hipLaunchKernelGGL(sum_partials, dim3(1), dim3(kEB), 0, c.stream, P, 9, P + 9 * kEG);
+    // soft targets (softEnergy, BoundaryConditions.cpp:210-229)
+    hipLaunchKernelGGL(e_soft, dim3(kEG), dim3(kEB), 0, c.stream, c.n_soft, c.soft_idx.p, c.soft_t.p, c.soft_w, s.x.p, P + 9 * kEG);
+    hipLaunchKernelGGL(sum_partials, dim3(1), dim3(kEB), 0, c.stream, P, 10, P + 10 * kEG);
     CIPC_CHECK_LAUNCH();
-    double t[9];
-    CIPC_HIP(hipMemcpyAsync(t, P + 9 * kEG, 9 * sizeof(double), hipMemcpyDeviceToHost, c.stream));
+    double t[10];
+    CIPC_HIP(hipMemcpyAsync(t, P + 10 * kEG, 10 * sizeof(double), hipMemcpyDeviceToHost, c.stream));
     CIPC_HIP(hipStreamSynchronize(c.stream));
     for (int k = 0; k < 5; ++k) out[k] = t[k];
     out[5] = s.fric * t[5] + t[6];   // Friction::energy: mu * sum_self + sum_ground (Friction.cpp:521-544)
     out[6] = t[7];
     out[7] = s.bmode == 0 ? 0.5 * s.plate * t[8] : t[8];   // quadratic form: D/2 x^T (Q (x) I3) x
+    out[8] = t[9];
 }
 
-// Total as computeEnergy adds it: dt^2 (E_SNH + E_BW + E_bend) + inertia + kappa barriers + friction.
-double total_energy(const Context& c, double kappa, const double t[8]) {
+// Total as computeEnergy adds it: dt^2 (E_SNH + E_BW + E_bend) + inertia + soft + kappa
+// barriers + friction.
+double total_energy(const Context& c, double kappa, const double t[9]) {
     double E = t[0];
     E += t[6];
     E += t[7];
     E *= c.sc.dt * c.sc.dt;
     E += t[1];
+    E += t[8];
     E += kappa * (t[2] + t[3] + t[4]);
     E += t[5];
     return E;

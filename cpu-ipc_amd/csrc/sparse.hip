@@ -112,6 +112,11 @@ __global__ void add_gradient_records(int n, const int* __restrict__ k, const int
     }
     grad[3 * v] = g0; grad[3 * v + 1] = g1; grad[3 * v + 2] = g2;
 }
+void add_gradient_records_into(Context& c, int n, const int* k, const int* idx, const double* val, double* grad) {
+    if (n <= 0) return;
+    hipLaunchKernelGGL(add_gradient_records, dim3(blocks_for(n, 256)), dim3(256), 0, c.stream, n, k, idx, val, c.sc.V, grad);
+    CIPC_CHECK_LAUNCH();
+}
 // --- merge static and contact blocks per block column --------------------------------
 __device__ inline int lower_bound_u64(const unsigned long long* a, int n, unsigned long long key) {
     int lo = 0, hi = n;

@@ -149,9 +149,33 @@ int cipc_gpu_get_csr(cipc_gpu_context* ctx, int* outer, int* inner, double* valu
  * set.  terms = {SNH elastic (before the dt^2 factor), inertia + drag, ground barrier sum,
  * self barrier sum (with multiplicity), mollified edge-edge barrier sum, lagged friction
  * energy (Friction::energy, Friction.cpp:459-545; 0 before cipc_gpu_friction_update),
- * Baraff-Witkin triangles (before dt^2), bending (before dt^2)}. */
+ * Baraff-Witkin triangles (before dt^2), bending (before dt^2), soft targets (softEnergy,
+ * BoundaryConditions.cpp:210-229)}. */
 int cipc_gpu_energy(cipc_gpu_context* ctx, double kappa, double* energy);
-int cipc_gpu_energy_terms(cipc_gpu_context* ctx, double kappa, double terms[8]);
+int cipc_gpu_energy_terms(cipc_gpu_context* ctx, double kappa, double terms[9]);
+
+/* The data-parallel parts of the barrier-stiffness control; the scalar logic of
+ * suggestKappa / upperBoundKappa / initKappa / postLineSearch (IPCSolver.cpp:751-869) stays
+ * with the caller.
+ *  kappa_gradients: initKappa's sums (IPCSolver.cpp:766-799) over the current positions and
+ *    sets: out = {sum_v g_c.g_E, sum_v |g_c|^2} with g_E = computeEGradient (inertia without
+ *    drag, tet elastic forces times dt^2; IPCSolver.cpp:74-106) and g_c the ground and
+ *    Self_ActiveSet barrier gradients at kappa = 1; n_constraints = |Environment| + |Self|.
+ *  close_constraints: postLineSearch's else-branch (IPCSolver.cpp:815-866): *closer = 1 when
+ *    a recorded ground or self constraint is now at or below its recorded value, then the
+ *    records are replaced by the current constraints with value < dtol (counts = {ground,
+ *    self}).  clear_close_constraints empties the records (IPCSolver.cpp:1163-1166). */
+int cipc_gpu_kappa_gradients(cipc_gpu_context* ctx, double out[2], int64_t* n_constraints);
+int cipc_gpu_close_constraints(cipc_gpu_context* ctx, double dtol, int* closer, int64_t counts[2]);
+int cipc_gpu_clear_close_constraints(cipc_gpu_context* ctx);
+
+/* SoftBoundaryCondition (BoundaryConditions.h:54-67) after BoundaryConditionOps::
+ * initialize / updateSoftTargets (BoundaryConditions.cpp:64-140): the penalty
+ * 0.5 weight |x_v - target|^2 on n distinct free vertices, added to every later assemble
+ * (addSoftDerivatives, BoundaryConditions.cpp:175-208) and energy.  n = 0 removes it.
+ * CIPC_ERR_INVALID_ARGUMENT for a non-positive weight, non-finite targets, or a vertex
+ * out of range, repeated or not free. */
+int cipc_gpu_set_soft_targets(cipc_gpu_context* ctx, int n, const int* vertices, const double* targets, double weight);
 
 /* NewtonLinearSystem::solve (NewtonLinearSystem.cpp:21-56) for the system of the last
  * cipc_gpu_assemble: H p = rhs with the relative-residual target and iteration cap of

@@ -184,6 +184,8 @@ struct Mesh {
     std::vector<std::array<int, 3>> tris; std::vector<std::array<double, 4>> dmt; std::vector<double> area;
     double stretchS = 0, shearS = 0, srate = 0, plate = 0;
     std::vector<I4> bverts; std::vector<std::array<double, 16>> bq; std::vector<double> brest, bweight;
+    // soft target-position penalty (SoftBoundaryCondition, BoundaryConditions.h:54-67)
+    std::vector<int> softIdx; std::vector<V3> softT; double softW = 0;
     std::vector<std::array<int, 3>> faces; std::vector<int> sv; std::vector<std::pair<int, int>> se;
     double dt = 0, u = 0, r = 0, drag = 0, hhat = 0, fhat = 0, fric = 0;
     V3 gn{0, 1, 0}; double gd = -1;
@@ -192,6 +194,9 @@ struct Mesh {
     std::vector<I4> selfLast; std::vector<double> selfLam;
     std::vector<std::array<double, 2>> distCoord; std::vector<std::array<V3, 2>> tanBasis;
     std::vector<int> envLast; std::vector<double> envLam;
+    // close constraints recorded by postLineSearch (SimulationMesh.h; IPCSolver.cpp:844-866)
+    std::vector<int> closeEnv; std::vector<double> closeEnvVal;
+    std::vector<I4> closeSelf; std::vector<double> closeSelfVal;
     std::vector<V3> grad;
     std::vector<int> outer, inner; std::vector<double> val, rhs;
     LBVH faceTree, edgeTree;
@@ -909,6 +914,14 @@ int gradient_and_hessian(Mesh& m, double kappa, Blocks& BH) {
     m.grad.assign(m.V, V3());
     for (int v = 0; v < m.V; ++v) m.grad[v] = m.mass[v] * (m.x[v] - m.xt[v]);                     // :118-123
     for (int v = 0; v < m.V; ++v) m.grad[v] = m.grad[v] + (m.drag * m.mass[v]) * (m.x[v] - m.vprev[v]); // :124-128
+    // soft targets (addSoftDerivatives, BoundaryConditions.cpp:175-208; IPCSolver.cpp:130)
+    for (size_t i = 0; i < m.softIdx.size(); ++i) {
+        const int v = m.softIdx[i];
+        m.grad[v] = m.grad[v] + m.softW * (m.x[v] - m.softT[i]);
+        std::array<double, 9> h{};
+        h[0] = h[4] = h[8] = m.softW;
+        BH.h3.push_back(h); BH.d1.push_back(v);
+    }
     // tets (IPCSolver.cpp:139-164)
     std::vector<std::array<double, 12>> fe(m.T);
     BH.h12.assign(m.T, {}); BH.d4.assign(m.T, {});
@@ -1219,7 +1232,7 @@ void assemble_csr(Mesh& m, const Blocks& BH) {
 // ----------------------------------------------------------------------------------
 // computeEnergy (IPCSolver.cpp:322-456), tet + inertia + barrier subset
 // ----------------------------------------------------------------------------------
-void energy_terms(Mesh& m, double kappa, double out[8]) {
+void energy_terms(Mesh& m, double kappa, double out[9]) {
     std::vector<double> et(m.T);
 #pragma omp parallel for schedule(dynamic, 256)
     for (int t = 0; t < m.T; ++t) et[t] = snh_energy_density(deformation(m, t), m.u, m.r) * m.vol[t];
@@ -1255,6 +1268,9 @@ void energy_terms(Mesh& m, double kappa, double out[8]) {
         de += sqn(m.x[v] - m.xt[v]) * m.mass[v] / 2.0;
         de += (sqn(m.x[v] - m.vprev[v]) * (m.drag) * m.mass[v] / 2.0);
     }
+    // softEnergy (BoundaryConditions.cpp:210-229)
+    double esoft = 0;
+    for (size_t i = 0; i < m.softIdx.size(); ++i) esoft += 0.5 * m.softW * sqn(m.x[m.softIdx[i]] - m.softT[i]);
     double eg = 0, es = 0, ee = 0;
     for (int v : m.envAS) {
         const double dist = dot(m.gn, m.x[v]) - m.gd;
@@ -1280,17 +1296,90 @@ void energy_terms(Mesh& m, double kappa, double out[8]) {
     }
     out[0] = esnh; out[1] = de; out[2] = eg; out[3] = es; out[4] = ee;
     out[5] = friction_energy(m);
-    out[6] = ebw; out[7] = ebend;
+    out[6] = ebw; out[7] = ebend; out[8] = esoft;
     (void)kappa;
 }
+// initKappa's reductions (IPCSolver.cpp:766-799): g_E from computeEGradient (inertia
+// without drag, tet elastic forces times dt^2; IPCSolver.cpp:74-106), g_c the ground and
+// self barrier gradients with kappa = 1 (compute_g_dpt with multiplicity).
+// out = {sum_v g_c . g_E, sum_v |g_c|^2}; returns the number of ground + self constraints.
+int kappa_gradients(Mesh& m, double out[2]) {
+    const int nc = (int)(m.envAS.size() + m.selfAS.size());
+    out[0] = out[1] = 0.0;
+    if (!nc) return 0;
+    std::vector<V3> gE(m.V, V3()), gc(m.V, V3());
+    for (int v = 0; v < m.V; ++v) gE[v] = gE[v] + m.mass[v] * (m.x[v] - m.xt[v]);
+    const double dt2 = m.dt * m.dt;
+    std::vector<std::array<double, 12>> fe(m.T);
+#pragma omp parallel for schedule(dynamic, 256)
+    for (int t = 0; t < m.T; ++t) {
+        double P[9][12];
+        pfpx(m.dminv[t], P);
+        const M3 PE = snh_pk1(deformation(m, t), m.u, m.r);
+        for (int j = 0; j < 12; ++j) {
+            double acc = 0;
+            for (int k = 0; k < 9; ++k) acc += (m.vol[t] * P[k][j]) * PE.m[k];
+            fe[t][j] = acc;
+        }
+    }
+    for (int t = 0; t < m.T; ++t)
+        for (int a = 0; a < 4; ++a) gE[m.tets[t][a]] = gE[m.tets[t][a]] + dt2 * V3(fe[t][3 * a], fe[t][3 * a + 1], fe[t][3 * a + 2]);
+    for (int v : m.envAS) {
+        const double dist = dot(m.gn, m.x[v]) - m.gd;
+        double gb;
+        barrier_g(dist * dist, m.hhat, gb);
+        gc[v] = gc[v] + ((gb * 2.0) * dist) * m.gn;
+    }
+    for (const I4& c : m.selfAS) {
+        int vid[4];
+        const int type = decode(c, vid);
+        double gb;
+        barrier_g(self_val(m, c), m.hhat, gb);
+        const double scale = (type == 0 || type == 1) ? -c[3] * gb : gb;
+        V3 p[4];
+        const int nv = type == 0 ? 2 : (type == 1 ? 3 : 4);
+        for (int k = 0; k < nv; ++k) p[k] = m.x[vid[k]];
+        double d, g[12];
+        deriv_any(type, p, d, g, nullptr);
+        for (int k = 0; k < nv; ++k) gc[vid[k]] = gc[vid[k]] + V3(scale * g[3 * k], scale * g[3 * k + 1], scale * g[3 * k + 2]);
+    }
+    for (int v = 0; v < m.V; ++v) { out[0] += dot(gc[v], gE[v]); out[1] += sqn(gc[v]); }
+    return nc;
+}
+
+// postLineSearch's constraint bookkeeping (IPCSolver.cpp:815-866): whether a recorded
+// close constraint is now at or below its recorded value, then the new records (ground
+// and self constraints with value < dTol, in set order).
+int close_constraints(Mesh& m, double dtol) {
+    int closer = 0;
+    for (size_t i = 0; i < m.closeEnv.size() && !closer; ++i) {
+        const double dist = dot(m.gn, m.x[m.closeEnv[i]]) - m.gd;
+        if (dist * dist <= m.closeEnvVal[i]) closer = 1;
+    }
+    for (size_t i = 0; i < m.closeSelf.size() && !closer; ++i)
+        if (self_val(m, m.closeSelf[i]) <= m.closeSelfVal[i]) closer = 1;
+    m.closeEnv.clear(); m.closeEnvVal.clear(); m.closeSelf.clear(); m.closeSelfVal.clear();
+    for (int v : m.envAS) {
+        const double dist = dot(m.gn, m.x[v]) - m.gd;
+        const double d = dist * dist;
+        if (d < dtol) { m.closeEnv.push_back(v); m.closeEnvVal.push_back(d); }
+    }
+    for (const I4& c : m.selfAS) {
+        const double d = self_val(m, c);
+        if (d < dtol) { m.closeSelf.push_back(c); m.closeSelfVal.push_back(d); }
+    }
+    return closer;
+}
+
 double energy(Mesh& m, double kappa) {
-    double t[8];
+    double t[9];
     energy_terms(m, kappa, t);
     double E = t[0];
     E += t[6];
     E += t[7];
     E *= m.dt * m.dt;
     E += t[1];
+    E += t[8];
     E += kappa * (t[2] + t[3] + t[4]);
     E += t[5];
     return E;
@@ -1781,7 +1870,24 @@ void oracle_get_csr(const oracle_mesh* om, int* outer, int* inner, double* val, 
     if (rhs) std::copy(m.rhs.begin(), m.rhs.end(), rhs);
 }
 double oracle_energy(oracle_mesh* om, double kappa) { return energy(om->m, kappa); }
-void oracle_energy_terms(oracle_mesh* om, double kappa, double t[8]) { energy_terms(om->m, kappa, t); }
+void oracle_energy_terms(oracle_mesh* om, double kappa, double t[9]) { energy_terms(om->m, kappa, t); }
+int oracle_kappa_gradients(oracle_mesh* om, double out[2]) { return kappa_gradients(om->m, out); }
+int oracle_close_constraints(oracle_mesh* om, double dtol, int64_t counts[2]) {
+    const int r = close_constraints(om->m, dtol);
+    counts[0] = (int64_t)om->m.closeEnv.size(); counts[1] = (int64_t)om->m.closeSelf.size();
+    return r;
+}
+void oracle_clear_close_constraints(oracle_mesh* om) {
+    Mesh& m = om->m;
+    m.closeEnv.clear(); m.closeEnvVal.clear(); m.closeSelf.clear(); m.closeSelfVal.clear();
+}
+void oracle_set_soft(oracle_mesh* om, int n, const int* idx, const double* targets, double weight) {
+    Mesh& m = om->m;
+    m.softIdx.assign(idx, idx + n);
+    m.softT.resize(n);
+    for (int i = 0; i < n; ++i) m.softT[i] = V3(targets[3 * i], targets[3 * i + 1], targets[3 * i + 2]);
+    m.softW = weight;
+}
 void oracle_friction_update(oracle_mesh* om, double kappa) { friction_initialize(om->m, kappa); }
 void oracle_get_friction(const oracle_mesh* om, int64_t counts[2], double* lam, double* coord, double* basis, double* env_lam) {
     const Mesh& m = om->m;

@@ -67,8 +67,16 @@ void oracle_get_csr(const oracle_mesh* m, int* outer, int* inner, double* val, d
 /* computeEnergy. */
 double oracle_energy(oracle_mesh* m, double kappa);
 /* per-term energies: {SNH (before dt^2), inertia+drag, ground barrier, self barrier, mollified EE,
- * friction, Baraff-Witkin triangles (before dt^2), bending (before dt^2)} */
-void oracle_energy_terms(oracle_mesh* m, double kappa, double terms[8]);
+ * friction, Baraff-Witkin triangles (before dt^2), bending (before dt^2), soft targets} */
+void oracle_energy_terms(oracle_mesh* m, double kappa, double terms[9]);
+/* initKappa's reductions: out = {sum g_c . g_E, sum |g_c|^2}; returns #ground + #self constraints */
+int oracle_kappa_gradients(oracle_mesh* m, double out[2]);
+/* postLineSearch's bookkeeping: returns 1 when a recorded close constraint got closer, then
+ * records the constraints below dtol; counts = {ground, self} recorded */
+int oracle_close_constraints(oracle_mesh* m, double dtol, int64_t counts[2]);
+void oracle_clear_close_constraints(oracle_mesh* m);
+/* soft target-position penalty 0.5 w |x_v - t|^2 on n vertices (SoftBoundaryCondition) */
+void oracle_set_soft(oracle_mesh* m, int n, const int* idx, const double* targets, double weight);
 /* Friction::initialize: lagged friction state from the current active set and positions. */
 void oracle_friction_update(oracle_mesh* m, double kappa);
 /* counts[0] self contacts, counts[1] ground contacts; lam (n), coord (2n), basis (6n: two

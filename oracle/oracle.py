@@ -76,6 +76,12 @@ def lib():
         L.oracle_energy.argtypes = [C.c_void_p, C.c_double]
         L.oracle_energy_terms.argtypes = [C.c_void_p, C.c_double, _dp]
         L.oracle_friction_update.argtypes = [C.c_void_p, C.c_double]
+        L.oracle_set_soft.argtypes = [C.c_void_p, C.c_int, _ip, _dp, C.c_double]
+        L.oracle_kappa_gradients.restype = C.c_int
+        L.oracle_kappa_gradients.argtypes = [C.c_void_p, _dp]
+        L.oracle_close_constraints.restype = C.c_int
+        L.oracle_close_constraints.argtypes = [C.c_void_p, C.c_double, _i64p]
+        L.oracle_clear_close_constraints.argtypes = [C.c_void_p]
         L.oracle_get_friction.argtypes = [C.c_void_p, _i64p, _dp, _dp, _dp, _dp]
         L.oracle_step_bound.argtypes = [C.c_void_p, _dp, _dp]
         L.oracle_step_forward.argtypes = [C.c_void_p, _dp, _dp, C.c_double]
@@ -206,10 +212,31 @@ class OracleMesh:
 
     def energy_terms(self, kappa):
         """[elastic, inertia, ground barrier, self barrier, mollified barrier, friction,
-        cloth triangles, bending]"""
-        t = np.zeros(8)
+        cloth triangles, bending, soft targets]"""
+        t = np.zeros(9)
         lib().oracle_energy_terms(self.h, kappa, dptr(t))
         return t
+
+    def set_soft_targets(self, idx, targets, weight):
+        """SoftBoundaryCondition: 0.5 weight |x_v - target|^2 on the listed vertices."""
+        idx, t = _i(np.asarray(idx).reshape(-1)), _d(np.asarray(targets).reshape(-1, 3))
+        self._soft = (idx, t)
+        lib().oracle_set_soft(self.h, len(idx), iptr(idx), dptr(t), weight)
+
+    def kappa_gradients(self):
+        """initKappa's reductions: (sum g_c . g_E, sum |g_c|^2, number of constraints)."""
+        out = np.zeros(2)
+        n = lib().oracle_kappa_gradients(self.h, dptr(out))
+        return out[0], out[1], n
+
+    def close_constraints(self, dtol):
+        """postLineSearch's bookkeeping: (a recorded constraint got closer, (#ground, #self) recorded)."""
+        c = np.zeros(2, np.int64)
+        r = lib().oracle_close_constraints(self.h, dtol, c.ctypes.data_as(_i64p))
+        return bool(r), (int(c[0]), int(c[1]))
+
+    def clear_close_constraints(self):
+        lib().oracle_clear_close_constraints(self.h)
 
     def friction_update(self, kappa):
         """Friction::initialize on the current active set and positions."""

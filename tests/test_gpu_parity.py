@@ -271,3 +271,73 @@ def test_newton_iteration_timing(pair):
     o.active_set()
     assert t["total_ms"] > 0 and t["nnz"] > 0
     assert abs(t["energy"] - o.energy(kappa)) <= REL * abs(t["energy"])
+
+
+def test_soft_targets_match_oracle(gpu, O):
+    """SoftBoundaryCondition on the device: assembled system and energy against the
+    oracle, and the argument checks of validateSoftState."""
+    from cipc import scenes
+    s = _scenes()["two_cubes"]
+    xt, _ = scenes.newton_probe_state(s)
+    kappa = scenes.suggest_kappa(s.hhat, s.bbox_diag2, s.mean_mass)
+    rng = np.random.default_rng(9)
+    idx = rng.choice(s.V, 40, replace=False)
+    tgt = s.x[idx] + 0.02 * rng.standard_normal((40, 3))
+    g = gpu.GpuContext(s, xtilde=xt)
+    o = O.OracleMesh(s, xtilde=xt)
+    try:
+        g.set_soft_targets(idx, tgt, 5e2)
+        o.set_soft_targets(idx, tgt, 5e2)
+        g.active_set()
+        o.active_set()
+        rg, ro = g.assemble(kappa), o.assemble(kappa)
+        assert np.array_equal(rg["outer"], ro["outer"]) and np.array_equal(rg["inner"], ro["inner"])
+        assert rel_err(rg["val"], ro["val"]) <= REL and rel_err(rg["grad"], ro["grad"]) <= REL
+        tg, to = g.energy_terms(kappa), o.energy_terms(kappa)
+        assert to[8] > 0 and abs(tg[8] - to[8]) <= REL * to[8]
+        assert abs(g.energy(kappa) - o.energy(kappa)) <= REL * abs(o.energy(kappa))
+        with pytest.raises(ValueError):
+            g.set_soft_targets([idx[0], idx[0]], tgt[:2], 5e2)      # repeated vertex
+        with pytest.raises(ValueError):
+            g.set_soft_targets(idx[:2], tgt[:2], 0.0)               # weight must be positive
+        g.set_soft_targets([], np.zeros((0, 3)), 0.0)               # removal
+        assert g.energy_terms(kappa)[8] == 0.0
+    finally:
+        g.close()
+        o.close()
+
+
+@pytest.mark.parametrize("name", ["two_cubes", "four_20cubes"])
+def test_kappa_control_matches_oracle(gpu, O, name):
+    """initKappa's sums and postLineSearch's close-constraint records on the device."""
+    from cipc import kappa as K
+    from cipc import scenes
+    s = _scenes()[name]
+    xt, p = scenes.newton_probe_state(s)
+    g = gpu.GpuContext(s, xtilde=xt)
+    o = O.OracleMesh(s, xtilde=xt)
+    try:
+        g.active_set()
+        o.active_set()
+        sg, so = g.kappa_gradients(), o.kappa_gradients()
+        assert sg[2] == so[2] > 0
+        assert abs(sg[0] - so[0]) <= REL * abs(so[0]) and abs(sg[1] - so[1]) <= REL * abs(so[1]), (sg, so)
+        kg = K.KappaControl(g, s.hhat, s.bbox_diag2, s.mean_mass, s.dtol)
+        ko = K.KappaControl(o, s.hhat, s.bbox_diag2, s.mean_mass, s.dtol)
+        assert abs(kg.step_start() - ko.step_start()) <= REL * ko.mesh_kappa
+        # records at a threshold that catches part of the constraints, then a step closer
+        dtol = s.hhat * 0.5
+        assert g.close_constraints(dtol) == o.close_constraints(dtol)
+        g.set_direction(p)
+        g.save_positions()
+        g.step_forward(0.3)
+        o.step_forward(s.x, p, 0.3)
+        g.active_set()
+        o.active_set()
+        assert g.close_constraints(dtol) == o.close_constraints(dtol)
+        g.clear_close_constraints()
+        o.clear_close_constraints()
+        assert g.close_constraints(dtol) == o.close_constraints(dtol)
+    finally:
+        g.close()
+        o.close()
