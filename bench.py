@@ -71,17 +71,24 @@ def cpu_baseline(scene, xt, vprev, p, kappa, budget_s=15.0, threads=16):
 
 def pcg_leg(ctx, kappa, tol=1e-6, max_iter=10000, repeats=3):
     """Newton linear solve of the same frozen state's system (NewtonLinearSystem::solve,
-    NewtonLinearSystem.cpp:21-56) to the reference's 1e-6 relative residual: FP64
-    block-Jacobi PCG against FP32 PCG inside FP64 refinement; device time from HIP events,
-    median of `repeats` solves."""
+    NewtonLinearSystem.cpp:21-56) to the reference's 1e-6 relative residual: FP64 PCG and
+    FP32 PCG inside FP64 refinement, each with block-Jacobi, block FSAI and (when trained
+    weights are present) the learned factor; device time (factor set-up and, for the learned
+    factor, network inference included) from HIP events, median of `repeats` solves."""
     import numpy as np
     ctx.assemble(kappa)
     out = {"tol": tol, "system": "the frozen state's assembled Newton system (lower CSR, n = 3V)"}
-    for name, prec in (("plain_fp64_jacobi", 0), ("mixed_fp32_fp64_jacobi", 1)):
+    legs = [("plain_fp64_jacobi", 0, 0), ("mixed_fp32_fp64_jacobi", 1, 0), ("plain_fp64_fsai", 0, 1), ("mixed_fp32_fp64_fsai", 1, 1)]
+    from cipc import train_precond
+    weights = train_precond.DEFAULT_WEIGHTS
+    if os.path.exists(weights):
+        ctx.set_preconditioner_weights(np.load(weights))
+        legs += [("plain_fp64_learned", 0, 2), ("mixed_fp32_fp64_learned", 1, 2)]
+    for name, prec, pc in legs:
         runs = []
         for _ in range(repeats):
             try:
-                _, st = ctx.solve(precision=prec, rel_tol=tol, max_iter=max_iter)
+                _, st = ctx.solve(precision=prec, rel_tol=tol, max_iter=max_iter, preconditioner=pc)
                 runs.append(st)
             except RuntimeError as e:
                 runs.append({"error": str(e)})
@@ -93,10 +100,37 @@ def pcg_leg(ctx, kappa, tol=1e-6, max_iter=10000, repeats=3):
                          "refinements": int(ok[-1]["refinements"]), "rel_residual": float(ok[-1]["rel_residual"])}
         else:
             out[name] = {"error": runs[-1]["error"]}
-    if "ms" in out.get("plain_fp64_jacobi", {}) and "ms" in out.get("mixed_fp32_fp64_jacobi", {}):
-        out["mixed_speedup"] = round(out["plain_fp64_jacobi"]["ms"] / out["mixed_fp32_fp64_jacobi"]["ms"], 3)
-    out["learned"] = None
+    base = out.get("plain_fp64_jacobi", {}).get("ms")
+    if base:
+        out["speedup_vs_plain_fp64_jacobi"] = {name: round(base / out[name]["ms"], 3) for name, _, _ in legs[1:]
+                                               if "ms" in out.get(name, {})}
+    out["learned_weights"] = os.path.relpath(weights, ROOT) if os.path.exists(weights) else None
     return out
+
+
+def traffic_probe(n=5):
+    """Workload for the HBM-traffic counters of the elastic stage: n Newton iterations of the
+    frozen state (each launches the elastic kernels once), then one 256 MiB device-to-device
+    copy through torch, a plain copy kernel of known traffic (256 MiB read, 256 MiB written)
+    that scales the counters.  Profiled once with --pmc FETCH_SIZE and once with WRITE_SIZE."""
+    import torch
+    from cipc import gpu, scenes
+    scene = scenes.four_cubes20(velocity_scale=0.5)
+    xt, p = scenes.newton_probe_state(scene)
+    kappa = scenes.suggest_kappa(scene.hhat, scene.bbox_diag2, scene.mean_mass)
+    ctx = gpu.GpuContext(scene, xtilde=xt)
+    ctx.set_state(V_prev=scene.x - scene.velocities * scene.material.dt)
+    ctx.active_set()
+    ctx.friction_update(kappa)
+    ctx.set_direction(p)
+    for _ in range(n):
+        ctx.newton_iteration(kappa)
+    ctx.close()
+    a = torch.empty(256 * 1024 * 1024 // 4, dtype=torch.float32, device="cuda").fill_(1.0)
+    b = torch.empty_like(a)
+    b.copy_(a)
+    torch.cuda.synchronize()
+    print("traffic probe done", flush=True)
 
 
 def main():
@@ -108,7 +142,11 @@ def main():
     ap.add_argument("--cpu-budget", type=float, default=15.0)
     ap.add_argument("--no-pcg", action="store_true", help="skip the linear-solve leg")
     ap.add_argument("--traffic", default=None, help="JSON file with measured HBM bytes per launch of the elastic stage")
+    ap.add_argument("--traffic-probe", action="store_true", help="run the HBM-counter workload instead of the benchmark")
     args = ap.parse_args()
+    if args.traffic_probe:
+        traffic_probe()
+        return
     if args.gpus != 1:
         print(f"bench.py measures one GPU: --gpus {args.gpus} is not supported (every run uses a single MI355X)",
               file=sys.stderr)

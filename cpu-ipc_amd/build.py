@@ -20,7 +20,7 @@ COMMON = ["--offload-arch=" + ARCH, "-O3", "-std=c++17", "-fPIC", "-Wall", "-Wno
           "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"]
 NO_CONTRACT = {"contact.hip", "broadphase.hip", "ccd.hip", "energy.hip", "sparse.hip", "kappa.hip"}
 SOURCES = ["context.hip", "elastic.hip", "cloth.hip", "contact.hip", "sparse.hip", "broadphase.hip", "ccd.hip", "energy.hip",
-           "pcg.hip", "kappa.hip"]
+           "pcg.hip", "kappa.hip", "learned.hip"]
 
 
 def _compile(src):

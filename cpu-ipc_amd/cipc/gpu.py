@@ -24,6 +24,7 @@ _dp = C.POINTER(C.c_double)
 _ip = C.POINTER(C.c_int)
 _i64p = C.POINTER(C.c_int64)
 _u64p = C.POINTER(C.c_uint64)
+_fp = C.POINTER(C.c_float)
 
 OK, ERR_INVALID_ARGUMENT, ERR_RUNTIME, ERR_LOGIC, ERR_OVERFLOW, ERR_HIP, ERR_NO_DEVICE = range(7)
 
@@ -34,7 +35,8 @@ EXPORTS = (
     "cipc_gpu_get_active_set", "cipc_gpu_set_active_set", "cipc_gpu_assemble", "cipc_gpu_get_gradient",
     "cipc_gpu_get_csr", "cipc_gpu_energy", "cipc_gpu_energy_terms", "cipc_gpu_step_bound",
     "cipc_gpu_friction_update", "cipc_gpu_get_friction", "cipc_gpu_set_soft_targets", "cipc_gpu_kappa_gradients",
-    "cipc_gpu_close_constraints", "cipc_gpu_clear_close_constraints", "cipc_gpu_solve", "cipc_gpu_save_positions", "cipc_gpu_step_forward",
+    "cipc_gpu_close_constraints", "cipc_gpu_clear_close_constraints", "cipc_gpu_preconditioner_param_count",
+    "cipc_gpu_set_preconditioner_weights", "cipc_gpu_preconditioner_gradient", "cipc_gpu_preconditioner_train_step", "cipc_gpu_solve", "cipc_gpu_solve_ex", "cipc_gpu_save_positions", "cipc_gpu_step_forward",
     "cipc_gpu_is_intersected", "cipc_gpu_newton_iteration",
     "cipc_gpu_eval_snh", "cipc_gpu_eval_cloth", "cipc_gpu_eval_hinge", "cipc_gpu_eval_distance", "cipc_gpu_eval_make_pd", "cipc_gpu_eval_contact_pd",
     "cipc_gpu_eval_accd",
@@ -116,9 +118,13 @@ def load(path: str = LIB_PATH):
         "cipc_gpu_friction_update": [C.c_void_p, C.c_double],
         "cipc_gpu_set_soft_targets": [C.c_void_p, C.c_int, _ip, _dp, C.c_double],
         "cipc_gpu_kappa_gradients": [C.c_void_p, _dp, _i64p],
+        "cipc_gpu_set_preconditioner_weights": [C.c_void_p, _fp, C.c_int64],
+        "cipc_gpu_preconditioner_gradient": [C.c_void_p, C.c_int, _fp, _fp, C.c_int, _dp, _fp],
+        "cipc_gpu_preconditioner_train_step": [C.c_void_p, C.c_int, _fp, _fp, _fp, C.c_int, C.c_double, C.c_int, C.c_uint64, _dp],
         "cipc_gpu_close_constraints": [C.c_void_p, C.c_double, _ip, _i64p],
         "cipc_gpu_clear_close_constraints": [C.c_void_p],
         "cipc_gpu_solve": [C.c_void_p, C.c_int, C.c_double, C.c_int, _dp, C.POINTER(SolveStats)],
+        "cipc_gpu_solve_ex": [C.c_void_p, C.c_int, C.c_int, C.c_double, C.c_int, _dp, C.POINTER(SolveStats)],
         "cipc_gpu_get_friction": [C.c_void_p, C.POINTER(C.c_int64), _dp, _dp, _dp, _dp],
         "cipc_gpu_step_bound": [C.c_void_p, _dp],
         "cipc_gpu_save_positions": [C.c_void_p],
@@ -138,6 +144,8 @@ def load(path: str = LIB_PATH):
         fn = getattr(L, name)
         fn.restype = C.c_int
         fn.argtypes = args
+    L.cipc_gpu_preconditioner_param_count.restype = C.c_int64
+    L.cipc_gpu_preconditioner_param_count.argtypes = []
     _lib = L
     return L
 
@@ -287,6 +295,34 @@ class GpuContext:
         _check(load().cipc_gpu_energy_terms(self.h, kappa, _dptr(t)))
         return t
 
+    def set_preconditioner_weights(self, w):
+        """Weights of the learned preconditioner (solve(preconditioner=2))."""
+        w = np.ascontiguousarray(w, dtype=np.float32)
+        self._pw = w
+        _check(load().cipc_gpu_set_preconditioner_weights(self.h, w.ctypes.data_as(_fp), len(w)))
+
+    def preconditioner_gradient(self, params, probes=None, objective=0):
+        """Training loss of the last assembled system and its gradient: objective 0 for probes
+        (n_z, V, 3), objective 1 the regression on the block FSAI factor."""
+        params = np.ascontiguousarray(params, dtype=np.float32)
+        g = np.zeros_like(params)
+        loss = C.c_double(0.0)
+        if probes is None:
+            zp, nz = None, 0
+        else:
+            z = np.ascontiguousarray(probes, dtype=np.float32).reshape(len(probes), -1)
+            zp, nz = z.ctypes.data_as(_fp), len(z)
+        _check(load().cipc_gpu_preconditioner_gradient(self.h, objective, params.ctypes.data_as(_fp), zp, nz, C.byref(loss),
+                                                       g.ctypes.data_as(_fp)))
+        return loss.value, g
+
+    def preconditioner_train_step(self, params, m1, m2, step, lr, n_probes=4, seed=0, objective=1):
+        """One Adam step on the last assembled system; float32 arrays updated in place."""
+        loss = C.c_double(0.0)
+        _check(load().cipc_gpu_preconditioner_train_step(self.h, objective, params.ctypes.data_as(_fp), m1.ctypes.data_as(_fp),
+                                                         m2.ctypes.data_as(_fp), step, lr, n_probes, seed, C.byref(loss)))
+        return loss.value
+
     def kappa_gradients(self):
         """initKappa's reductions: (sum g_c . g_E, sum |g_c|^2, number of constraints)."""
         out = np.zeros(2)
@@ -312,12 +348,13 @@ class GpuContext:
         n = len(idx)
         _check(load().cipc_gpu_set_soft_targets(self.h, n, _iptr(idx) if n else None, _dptr(t) if n else None, weight))
 
-    def solve(self, precision=0, rel_tol=1e-6, max_iter=10000):
+    def solve(self, precision=0, rel_tol=1e-6, max_iter=10000, preconditioner=0):
         """Newton linear solve of the last assembled system; returns (direction (V,3), stats).
-        precision 0: FP64 block-Jacobi PCG; 1: FP32 PCG inside FP64 refinement."""
+        precision 0: FP64 PCG; 1: FP32 PCG inside FP64 refinement.  preconditioner 0:
+        block-Jacobi; 1: block FSAI; 2: the learned factor."""
         p = np.zeros((self.V, 3))
         st = SolveStats()
-        _check(load().cipc_gpu_solve(self.h, precision, rel_tol, max_iter, _dptr(p), C.byref(st)))
+        _check(load().cipc_gpu_solve_ex(self.h, precision, preconditioner, rel_tol, max_iter, _dptr(p), C.byref(st)))
         return p, dict(iterations=st.iterations, refinements=st.refinements, rel_residual=st.rel_residual,
                        solve_ms=st.solve_ms)
 
@@ -366,6 +403,10 @@ def eval_snh(F, u, r):
     P, H, e = np.zeros((n, 9)), np.zeros((n, 81)), np.zeros(n)
     _check(load().cipc_gpu_eval_snh(n, _dptr(Fc), u, r, _dptr(P), _dptr(H), _dptr(e)))
     return P.reshape(n, 3, 3).transpose(0, 2, 1), H.reshape(n, 9, 9), e
+
+
+def preconditioner_param_count():
+    return int(load().cipc_gpu_preconditioner_param_count())
 
 
 def eval_cloth(F, stretch, shear, strain_rate):

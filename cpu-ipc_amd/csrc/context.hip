@@ -91,6 +91,7 @@ int cipc_gpu_context_destroy(cipc_gpu_context* c) {
         (void)hipEventDestroy(c->join);
         (void)hipStreamDestroy(c->aux);
         (void)hipStreamDestroy(c->stream);
+        destroy_learned(*c);
         delete c;  // DBuf members free on destruction of their owner only when released:
     });
 }
@@ -420,6 +421,41 @@ int cipc_gpu_set_soft_targets(cipc_gpu_context* c, int n, const int* vertices, c
     });
 }
 
+int64_t cipc_gpu_preconditioner_param_count(void) { return learned_param_count(); }
+
+int cipc_gpu_set_preconditioner_weights(cipc_gpu_context* c, const float* weights, int64_t n) {
+    return guard([&] {
+        require(c && weights, "null context or weights");
+        bind(*c);
+        learned_set_weights(*c, weights, n);
+    });
+}
+
+int cipc_gpu_preconditioner_gradient(cipc_gpu_context* c, int objective, const float* params, const float* probes, int n_probes,
+                                     double* loss, float* grad) {
+    return guard([&] {
+        require(c && c->have_scene && params && loss, "invalid arguments");
+        require(objective == 0 || objective == 1, "objective must be 0 (probe loss) or 1 (block FSAI regression)");
+        require(objective == 1 || (probes && n_probes > 0), "the probe loss needs probes");
+        require(c->assembled, "no assembled system: call cipc_gpu_assemble first");
+        bind(*c);
+        *loss = learned_loss_grad(*c, params, probes, n_probes, 0, grad, objective);
+    });
+}
+
+int cipc_gpu_preconditioner_train_step(cipc_gpu_context* c, int objective, float* params, float* adam_m, float* adam_v, int step,
+                                       double lr, int n_probes, uint64_t seed, double* loss) {
+    return guard([&] {
+        require(c && c->have_scene && params && adam_m && adam_v, "invalid arguments");
+        require(objective == 0 || objective == 1, "objective must be 0 (probe loss) or 1 (block FSAI regression)");
+        require(step >= 1 && (objective == 1 || n_probes > 0) && lr > 0, "step >= 1, probes > 0 and lr > 0");
+        require(c->assembled, "no assembled system: call cipc_gpu_assemble first");
+        bind(*c);
+        const double l = learned_train_step(*c, params, adam_m, adam_v, step, lr, n_probes, seed, objective);
+        if (loss) *loss = l;
+    });
+}
+
 int cipc_gpu_kappa_gradients(cipc_gpu_context* c, double out[2], int64_t* n_constraints) {
     return guard([&] {
         require(c && c->have_scene && out, "no scene uploaded or null output");
@@ -465,9 +501,15 @@ int cipc_gpu_energy(cipc_gpu_context* c, double kappa, double* e) {
 }
 
 int cipc_gpu_solve(cipc_gpu_context* c, int precision, double rel_tol, int max_iter, double* direction, cipc_solve_stats* stats) {
+    return cipc_gpu_solve_ex(c, precision, 0, rel_tol, max_iter, direction, stats);
+}
+
+int cipc_gpu_solve_ex(cipc_gpu_context* c, int precision, int preconditioner, double rel_tol, int max_iter, double* direction,
+                      cipc_solve_stats* stats) {
     return guard([&] {
         require(c && c->have_scene, "no scene uploaded");
         require(precision == 0 || precision == 1, "precision must be 0 (FP64) or 1 (FP32 with FP64 refinement)");
+        require(preconditioner >= 0 && preconditioner <= 2, "preconditioner must be 0 (block-Jacobi), 1 (block FSAI) or 2 (learned)");
         require(std::isfinite(rel_tol) && rel_tol > 0.0 && max_iter > 0, "tolerance and iteration cap must be positive");
         require(c->assembled, "no assembled system: call cipc_gpu_assemble first");
         bind(*c);
@@ -476,7 +518,7 @@ int cipc_gpu_solve(cipc_gpu_context* c, int precision, double rel_tol, int max_i
         CIPC_HIP(hipEventCreate(&e1));
         CIPC_HIP(hipEventRecord(e0, c->stream));
         SolveResult r;
-        linear_solve(*c, precision, rel_tol, max_iter, r);
+        linear_solve(*c, precision, preconditioner, rel_tol, max_iter, r);
         CIPC_HIP(hipEventRecord(e1, c->stream));
         CIPC_HIP(hipEventSynchronize(e1));
         float ms = 0.f;

@@ -58,15 +58,22 @@ struct SolveBuffers {
     DBuf<int> src, src_sorted, col, rowptr;   // full symmetric 3x3-block matrix (BSR)
     DBuf<double> val, minv;                   // FP64 blocks and block-Jacobi inverses
     DBuf<float> valf, minvf, xf;              // FP32 copies for the mixed form
-    DBuf<double> r, z, p, q, x, res, part, scal;
+    DBuf<double> r, z, p, q, x, res, part, scal, t;
+    DBuf<double> G;                           // factor preconditioner at the lower BSR entries
+    DBuf<float> Gf;
+    DBuf<int> mirror;                         // index of the transposed entry
+    size_t ne = 0;                            // BSR entry capacity (sentinels included)
 };
 struct SolveResult {
     int iterations = 0, refinements = 0;
     double rel_residual = 0.0;
 };
 
+struct LearnedState;
+
 struct Context {
     int device = 0;
+    LearnedState* learned = nullptr;    // learned preconditioner state (learned.hip)
     hipStream_t stream = nullptr;
     // Elastic assembly depends on the state only; it runs on `aux` (default priority) while
     // `stream` builds the active set and the contact blocks (fork / join events).
@@ -201,7 +208,17 @@ __device__ inline void energy_block_sum(double v, double* part) {
 }
 void friction_update(Context& c, double kappa);
 void build_solve_matrix(Context& c, bool fp32);
-void linear_solve(Context& c, int precision, double rel_tol, int max_iter, SolveResult& res);
+// preconditioner: 0 block-Jacobi, 1 block FSAI, 2 learned factor
+void linear_solve(Context& c, int precision, int preconditioner, double rel_tol, int max_iter, SolveResult& res);
+void build_fsai(Context& c, bool fp32);
+void build_learned(Context& c, bool fp32);
+void learned_set_weights(Context& c, const float* w, int64_t n);
+int64_t learned_param_count();
+void destroy_learned(Context& c);
+double learned_loss_grad(Context& c, const float* params, const float* host_probes, int nz, unsigned long long seed, float* grad_out,
+                         int objective);
+double learned_train_step(Context& c, float* params, float* m1, float* m2, int step, double lr, int nz, unsigned long long seed,
+                          int objective);
 // Friction records after the contact records; returns the friction energy parts' launch
 void friction_records(Context& c, size_t slot_b, size_t slot_g);
 void friction_energy_partials(Context& c, double* part);

@@ -7,15 +7,18 @@
 // the entries are sorted by (block row, block column) with one radix sort.  Zero entries
 // that the CSR export drops contribute nothing to a product, so the two agree.
 //
-// Two forms:
-//   precision 0: FP64 conjugate gradients with the 3x3 block-Jacobi preconditioner;
+// Two precisions:
+//   precision 0: FP64 conjugate gradients;
 //   precision 1: the same iteration in FP32 (blocks, vectors) as the inner solver of FP64
-//                iterative refinement: r = rhs - H x in FP64, solve H d = r in FP32 to a
-//                loose relative tolerance, x += d, until the FP64 residual meets the target.
+//                iterative refinement: r = rhs - H x in FP64, solve H d = r in FP32, x += d,
+//                until the FP64 residual meets the target.
+// Three preconditioners: 3x3 block-Jacobi; a block lower-triangular factor G with
+// M^-1 = G^T G, either the block FSAI of the matrix or the learned factor (learned.hip).
 //
-// One iteration is three kernels; scalars (alpha, beta, norms) stay on the device and are
-// combined from per-work-group partials in a fixed order (deterministic).  The host checks
-// convergence every kCheck iterations; kernels launched past convergence return at once.
+// Scalars (alpha, beta, norms) stay on the device and are combined from per-work-group
+// partials in a fixed order (deterministic).  kCheck iterations are captured once into a
+// HIP graph and replayed; the host checks convergence after each replay and kernels
+// launched past convergence return at once.
 #include "context.hpp"
 
 #include <hipcub/hipcub.hpp>
@@ -110,6 +113,31 @@ __device__ inline void block_mv(const T* __restrict__ B, const T* __restrict__ x
     y[1] = (double)B[3] * x[0] + (double)B[4] * x[1] + (double)B[5] * x[2];
     y[2] = (double)B[6] * x[0] + (double)B[7] * x[1] + (double)B[8] * x[2];
 }
+// Row-parallel sparse products: kGL lanes per 3x3-block row, so that the lanes of a row read
+// consecutive blocks (coalesced), combined with cross-lane shuffles.  kRows rows per work-group.
+constexpr int kGL = 8;
+constexpr int kRows = kPB / kGL;
+template <typename T>
+__device__ inline void group_reduce3(double y[3]) {
+#pragma unroll
+    for (int o = kGL / 2; o > 0; o >>= 1) {
+        y[0] += __shfl_xor(y[0], o, kGL);
+        y[1] += __shfl_xor(y[1], o, kGL);
+        y[2] += __shfl_xor(y[2], o, kGL);
+    }
+}
+// y = (A x)_a over entries rowptr[a] + sub, step kGL (all lanes of the group get the sum)
+template <typename T, typename X>
+__device__ inline void row_product(int a, int sub, const int* __restrict__ rowptr, const int* __restrict__ col, const T* __restrict__ val,
+                                   const X* __restrict__ x, double y[3]) {
+    y[0] = y[1] = y[2] = 0.0;
+    for (int e = rowptr[a] + sub; e < rowptr[a + 1]; e += kGL) {
+        double t[3];
+        block_mv(val + (size_t)9 * e, x + 3 * (size_t)col[e], t);
+        y[0] += t[0]; y[1] += t[1]; y[2] += t[2];
+    }
+    group_reduce3<T>(y);
+}
 // work-group sum of up to three values; thread 0 stores them at part[k * gridDim.x + blockIdx.x]
 template <int K>
 __device__ inline void group_partials(double v[K], double* __restrict__ part) {
@@ -126,6 +154,18 @@ __device__ inline void group_partials(double v[K], double* __restrict__ part) {
     if (threadIdx.x == 0)
 #pragma unroll
         for (int k = 0; k < K; ++k) part[k * gridDim.x + blockIdx.x] = sh[k][0];
+}
+// work-group sums of two values stored to two partial arrays
+__device__ inline void group_partials2(double v[2], double* __restrict__ pa, double* __restrict__ pb) {
+    __shared__ double sh[2][kPB];
+    sh[0][threadIdx.x] = v[0];
+    sh[1][threadIdx.x] = v[1];
+    __syncthreads();
+    for (int s = kPB / 2; s > 0; s >>= 1) {
+        if ((int)threadIdx.x < s) { sh[0][threadIdx.x] += sh[0][threadIdx.x + s]; sh[1][threadIdx.x] += sh[1][threadIdx.x + s]; }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) { pa[blockIdx.x] = sh[0][0]; pb[blockIdx.x] = sh[1][0]; }
 }
 // every work-group sums the partials of the previous kernel in the same order
 __device__ inline double sum_partials_fixed(const double* __restrict__ part, int n) {
@@ -144,40 +184,108 @@ __device__ inline double sum_partials_fixed(const double* __restrict__ part, int
 }
 
 // ---- PCG kernels (T: storage precision of blocks and vectors; scalars FP64) ------------
-// r = b - A x (A x skipped when x is zero), z = M^-1 r, p = z; partials of r.z, r.r, b.b
+// Partials layout (ng work-groups each): [0] p.q, [1] r.z, [2] r.r, [3] b.b
+// r = b - A x (A x skipped when x is zero); partials of r.r and b.b
+// (grid: kRows block rows per work-group, kGL lanes per row)
 template <typename T>
-__global__ __launch_bounds__(kPB) void pcg_start(int V, const int* __restrict__ rowptr, const int* __restrict__ col,
-                                                 const T* __restrict__ val, const T* __restrict__ minv, const double* __restrict__ b,
-                                                 const T* __restrict__ x, int x_zero, T* __restrict__ r, T* __restrict__ z,
-                                                 T* __restrict__ p, double* __restrict__ part) {
-    const int a = blockIdx.x * blockDim.x + threadIdx.x;
-    double acc[3] = {0, 0, 0};
+__global__ __launch_bounds__(kPB) void pcg_residual(int V, const int* __restrict__ rowptr, const int* __restrict__ col,
+                                                    const T* __restrict__ val, const double* __restrict__ b, const T* __restrict__ x,
+                                                    int x_zero, T* __restrict__ r, double* __restrict__ part_rr,
+                                                    double* __restrict__ part_bb) {
+    const int a = blockIdx.x * kRows + threadIdx.x / kGL, sub = threadIdx.x % kGL;
+    double acc[2] = {0, 0};
     if (a < V) {
-        double rv[3] = {b[3 * a], b[3 * a + 1], b[3 * a + 2]};
-        const double bb = rv[0] * rv[0] + rv[1] * rv[1] + rv[2] * rv[2];
-        if (!x_zero) {
-            for (int e = rowptr[a]; e < rowptr[a + 1]; ++e) {
-                double y[3];
-                block_mv(val + (size_t)9 * e, x + 3 * (size_t)col[e], y);
-                rv[0] -= y[0]; rv[1] -= y[1]; rv[2] -= y[2];
-            }
+        double y[3] = {0, 0, 0};
+        if (!x_zero) row_product<T, T>(a, sub, rowptr, col, val, x, y);
+        if (sub == 0) {
+            const double rv[3] = {b[3 * a] - y[0], b[3 * a + 1] - y[1], b[3 * a + 2] - y[2]};
+            acc[1] = b[3 * a] * b[3 * a] + b[3 * a + 1] * b[3 * a + 1] + b[3 * a + 2] * b[3 * a + 2];
+            for (int k = 0; k < 3; ++k) r[3 * a + k] = (T)rv[k];
+            acc[0] = (double)(T)rv[0] * (T)rv[0] + (double)(T)rv[1] * (T)rv[1] + (double)(T)rv[2] * (T)rv[2];
         }
-        double zv[3];
-        const T rt[3] = {(T)rv[0], (T)rv[1], (T)rv[2]};
-        block_mv(minv + (size_t)9 * a, rt, zv);
-        for (int k = 0; k < 3; ++k) { r[3 * a + k] = rt[k]; z[3 * a + k] = (T)zv[k]; p[3 * a + k] = (T)zv[k]; }
-        acc[0] = (double)rt[0] * (T)zv[0] + (double)rt[1] * (T)zv[1] + (double)rt[2] * (T)zv[2];
-        acc[1] = (double)rt[0] * rt[0] + (double)rt[1] * rt[1] + (double)rt[2] * rt[2];
-        acc[2] = bb;
     }
-    group_partials<3>(acc, part);
+    group_partials2(acc, part_rr, part_bb);
+}
+// block-Jacobi: z = M^-1 r; partial of r.z; p = z when `init`
+template <typename T>
+__global__ __launch_bounds__(kPB) void prec_jacobi(int V, const T* __restrict__ minv, const T* __restrict__ r, T* __restrict__ z,
+                                                   T* __restrict__ p, int init, const PcgScalars* __restrict__ sc,
+                                                   double* __restrict__ part_rz) {
+    if (sc && sc->done) return;
+    const int a = blockIdx.x * kRows + threadIdx.x / kGL, sub = threadIdx.x % kGL;
+    double acc[1] = {0.0};
+    if (a < V && sub == 0) {
+        const T rt[3] = {r[3 * a], r[3 * a + 1], r[3 * a + 2]};
+        double zv[3];
+        block_mv(minv + (size_t)9 * a, rt, zv);
+        for (int k = 0; k < 3; ++k) {
+            z[3 * a + k] = (T)zv[k];
+            if (init) p[3 * a + k] = (T)zv[k];
+        }
+        acc[0] = (double)rt[0] * (T)zv[0] + (double)rt[1] * (T)zv[1] + (double)rt[2] * (T)zv[2];
+    }
+    group_partials<1>(acc, part_rz);
+}
+// factor preconditioner M^-1 = G^T G, G block lower-triangular stored at the lower entries of
+// the BSR (the entry of (i, j), j <= i, holds G_ij):  t = G r
+template <typename T>
+__global__ __launch_bounds__(kPB) void prec_g1(int V, const int* __restrict__ rowptr, const int* __restrict__ col, const T* __restrict__ G,
+                                               const T* __restrict__ r, T* __restrict__ t, const PcgScalars* __restrict__ sc) {
+    if (sc && sc->done) return;
+    const int a = blockIdx.x * kRows + threadIdx.x / kGL, sub = threadIdx.x % kGL;
+    if (a >= V) return;
+    double y[3] = {0, 0, 0};
+    for (int e = rowptr[a] + sub; e < rowptr[a + 1]; e += kGL) {
+        const int j = col[e];
+        if (j > a) break;
+        double w[3];
+        block_mv(G + (size_t)9 * e, r + 3 * (size_t)j, w);
+        y[0] += w[0]; y[1] += w[1]; y[2] += w[2];
+    }
+    group_reduce3<T>(y);
+    if (sub == 0)
+        for (int k = 0; k < 3; ++k) t[3 * a + k] = (T)y[k];
+}
+// z = G^T t: row j gathers G_ij^T t_i over its upper entries (i >= j) through the mirror
+// index of each entry; partial of r.z; p = z when `init`
+template <typename T>
+__global__ __launch_bounds__(kPB) void prec_g2(int V, const int* __restrict__ rowptr, const int* __restrict__ col,
+                                               const int* __restrict__ mirror, const T* __restrict__ G, const T* __restrict__ t,
+                                               const T* __restrict__ r, T* __restrict__ z, T* __restrict__ p, int init,
+                                               const PcgScalars* __restrict__ sc, double* __restrict__ part_rz) {
+    if (sc && sc->done) return;
+    const int a = blockIdx.x * kRows + threadIdx.x / kGL, sub = threadIdx.x % kGL;
+    double acc[1] = {0.0};
+    if (a < V) {
+        double y[3] = {0, 0, 0};
+        for (int e = rowptr[a + 1] - 1 - sub; e >= rowptr[a]; e -= kGL) {
+            const int i = col[e];
+            if (i < a) break;
+            const T* B = G + (size_t)9 * mirror[e];          // G_ia, a <= i
+            const T* ti = t + 3 * (size_t)i;
+            y[0] += (double)B[0] * ti[0] + (double)B[3] * ti[1] + (double)B[6] * ti[2];
+            y[1] += (double)B[1] * ti[0] + (double)B[4] * ti[1] + (double)B[7] * ti[2];
+            y[2] += (double)B[2] * ti[0] + (double)B[5] * ti[1] + (double)B[8] * ti[2];
+        }
+        group_reduce3<T>(y);
+        if (sub == 0) {
+            const T zt[3] = {(T)y[0], (T)y[1], (T)y[2]};
+            for (int k = 0; k < 3; ++k) {
+                z[3 * a + k] = zt[k];
+                if (init) p[3 * a + k] = zt[k];
+            }
+            acc[0] = (double)r[3 * a] * zt[0] + (double)r[3 * a + 1] * zt[1] + (double)r[3 * a + 2] * zt[2];
+        }
+    }
+    group_partials<1>(acc, part_rz);
 }
 // scalars from the start partials: rz, rr, ||b|| (or ||r0|| as the reference of an inner solve)
-__global__ __launch_bounds__(kPB) void pcg_init_scalars(const double* __restrict__ part, int ng, double rel_tol, int rel_to_r0,
+__global__ __launch_bounds__(kPB) void pcg_init_scalars(const double* __restrict__ part_rz, const double* __restrict__ part_rr,
+                                                        const double* __restrict__ part_bb, int ng, double rel_tol, int rel_to_r0,
                                                         PcgScalars* __restrict__ sc) {
-    const double rz = sum_partials_fixed(part, ng);
-    const double rr = sum_partials_fixed(part + ng, ng);
-    const double bb = sum_partials_fixed(part + 2 * ng, ng);
+    const double rz = sum_partials_fixed(part_rz, ng);
+    const double rr = sum_partials_fixed(part_rr, ng);
+    const double bb = sum_partials_fixed(part_bb, ng);
     if (threadIdx.x == 0) {
         sc->rz = rz;
         sc->rr = rr;
@@ -191,65 +299,70 @@ __global__ __launch_bounds__(kPB) void pcg_init_scalars(const double* __restrict
 template <typename T>
 __global__ __launch_bounds__(kPB) void pcg_spmv(int V, const int* __restrict__ rowptr, const int* __restrict__ col,
                                                 const T* __restrict__ val, const T* __restrict__ p, T* __restrict__ q,
-                                                const PcgScalars* __restrict__ sc, double* __restrict__ part) {
+                                                const PcgScalars* __restrict__ sc, double* __restrict__ part_pq) {
     if (sc->done) return;
-    const int a = blockIdx.x * blockDim.x + threadIdx.x;
+    const int a = blockIdx.x * kRows + threadIdx.x / kGL, sub = threadIdx.x % kGL;
     double acc[1] = {0.0};
     if (a < V) {
-        double y[3] = {0, 0, 0};
-        for (int e = rowptr[a]; e < rowptr[a + 1]; ++e) {
-            double t[3];
-            block_mv(val + (size_t)9 * e, p + 3 * (size_t)col[e], t);
-            y[0] += t[0]; y[1] += t[1]; y[2] += t[2];
+        double y[3];
+        row_product<T, T>(a, sub, rowptr, col, val, p, y);
+        if (sub == 0) {
+            const T yt[3] = {(T)y[0], (T)y[1], (T)y[2]};
+            for (int k = 0; k < 3; ++k) q[3 * a + k] = yt[k];
+            acc[0] = (double)p[3 * a] * yt[0] + (double)p[3 * a + 1] * yt[1] + (double)p[3 * a + 2] * yt[2];
         }
-        const T yt[3] = {(T)y[0], (T)y[1], (T)y[2]};
-        for (int k = 0; k < 3; ++k) q[3 * a + k] = yt[k];
-        acc[0] = (double)p[3 * a] * yt[0] + (double)p[3 * a + 1] * yt[1] + (double)p[3 * a + 2] * yt[2];
     }
-    group_partials<1>(acc, part);
+    group_partials<1>(acc, part_pq);
 }
-// alpha = rz / p.q; x += alpha p; r -= alpha q; z = M^-1 r; partials of r.z and r.r
+// alpha = rz / p.q; x += alpha p; r -= alpha q; partial of r.r; with block-Jacobi (minv set)
+// also z = M^-1 r and its r.z partial
 template <typename T>
 __global__ __launch_bounds__(kPB) void pcg_update(int V, const double* __restrict__ part_pq, int ng, const T* __restrict__ minv,
                                                   const T* __restrict__ p, const T* __restrict__ q, T* __restrict__ x, T* __restrict__ r,
-                                                  T* __restrict__ z, const PcgScalars* __restrict__ sc, double* __restrict__ part) {
+                                                  T* __restrict__ z, const PcgScalars* __restrict__ sc, double* __restrict__ part_rz,
+                                                  double* __restrict__ part_rr) {
     if (sc->done) return;
     const double pq = sum_partials_fixed(part_pq, ng);
     const double alpha = sc->rz / pq;
-    const int a = blockIdx.x * blockDim.x + threadIdx.x;
+    const int a = blockIdx.x * kRows + threadIdx.x / kGL, sub = threadIdx.x % kGL;
     double acc[2] = {0.0, 0.0};
-    if (a < V) {
+    if (a < V && sub == 0) {
         T rt[3];
         for (int k = 0; k < 3; ++k) {
             x[3 * a + k] = (T)((double)x[3 * a + k] + alpha * (double)p[3 * a + k]);
             rt[k] = (T)((double)r[3 * a + k] - alpha * (double)q[3 * a + k]);
             r[3 * a + k] = rt[k];
         }
-        double zv[3];
-        block_mv(minv + (size_t)9 * a, rt, zv);
-        for (int k = 0; k < 3; ++k) z[3 * a + k] = (T)zv[k];
-        acc[0] = (double)rt[0] * (T)zv[0] + (double)rt[1] * (T)zv[1] + (double)rt[2] * (T)zv[2];
+        if (minv) {
+            double zv[3];
+            block_mv(minv + (size_t)9 * a, rt, zv);
+            for (int k = 0; k < 3; ++k) z[3 * a + k] = (T)zv[k];
+            acc[0] = (double)rt[0] * (T)zv[0] + (double)rt[1] * (T)zv[1] + (double)rt[2] * (T)zv[2];
+        }
         acc[1] = (double)rt[0] * rt[0] + (double)rt[1] * rt[1] + (double)rt[2] * rt[2];
     }
-    group_partials<2>(acc, part);
+    if (minv) group_partials2(acc, part_rz, part_rr);
+    else { double a1[1] = {acc[1]}; group_partials<1>(a1, part_rr); }
 }
-// beta = rz' / rz; p = z + beta p.  Every work-group reads the previous r.z here; the
+// beta = rz' / rz; p = z + beta p (one DOF per thread: the grid covers 3V / kPB work-groups,
+// more than the row kernels' ng, so every work-group reads all ng partials).  Every
+// work-group reads the previous r.z here; the
 // scalars (and the convergence flag) are replaced by the following one-work-group kernel,
 // so no work-group can see a value another one has changed.
 template <typename T>
-__global__ __launch_bounds__(kPB) void pcg_direction(int V, const double* __restrict__ part, int ng, const T* __restrict__ z,
+__global__ __launch_bounds__(kPB) void pcg_direction(int V, const double* __restrict__ part_rz, int ng, const T* __restrict__ z,
                                                      T* __restrict__ p, const PcgScalars* __restrict__ sc) {
     if (sc->done) return;
-    const double rz1 = sum_partials_fixed(part, ng);
+    const double rz1 = sum_partials_fixed(part_rz, ng);
     const double beta = rz1 / sc->rz;
-    const int a = blockIdx.x * blockDim.x + threadIdx.x;
-    if (a < V)
-        for (int k = 0; k < 3; ++k) p[3 * a + k] = (T)((double)z[3 * a + k] + beta * (double)p[3 * a + k]);
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;       // one DOF per thread
+    if (t < 3 * V) p[t] = (T)((double)z[t] + beta * (double)p[t]);
 }
-__global__ void pcg_scalars(const double* __restrict__ part, int ng, PcgScalars* __restrict__ sc, int max_iter) {
+__global__ void pcg_scalars(const double* __restrict__ part_rz, const double* __restrict__ part_rr, int ng,
+                            PcgScalars* __restrict__ sc, int max_iter) {
     if (sc->done) return;
-    const double rz1 = sum_partials_fixed(part, ng);
-    const double rr = sum_partials_fixed(part + ng, ng);
+    const double rz1 = sum_partials_fixed(part_rz, ng);
+    const double rr = sum_partials_fixed(part_rr, ng);
     if (threadIdx.x == 0) {
         sc->rz = rz1;
         sc->rr = rr;
@@ -263,8 +376,8 @@ __global__ void add_correction(int n, const float* __restrict__ d, double* __res
     if (i < n) x[i] += (double)d[i];
 }
 template <typename S, typename D>
-__global__ void convert(int n, const S* __restrict__ s, D* __restrict__ d) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+__global__ void convert(size_t n, const S* __restrict__ s, D* __restrict__ d) {
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) d[i] = (D)s[i];
 }
 __global__ void zero_d(int n, double* __restrict__ x) {
@@ -276,6 +389,139 @@ __global__ void zero_f(int n, float* __restrict__ x) {
     if (i < n) x[i] = 0.0f;
 }
 
+// ---- block FSAI factor ----------------------------------------------------------------
+// For block row i with lower pattern P (its entries j <= i, at most kFsaiMax blocks: the
+// diagonal and the largest off-diagonal blocks by Frobenius norm), G_{i,P} is the last block
+// row of L^-1, L the Cholesky factor of A[P,P]: then (G A G^T)_ii = I and G minimizes the
+// Kaporin condition number over the pattern (Kolotilina & Yeremin's factorized sparse
+// approximate inverse, in 3x3 blocks).  One wavefront per block row; A[P,P] in LDS.
+constexpr int kFsaiMax = 24;                   // blocks per row pattern
+constexpr int kFsaiN = 3 * kFsaiMax;           // dense order
+constexpr int kFsaiLd = kFsaiN + 1;            // padded leading dimension
+
+__device__ inline int find_col(const int* __restrict__ col, int lo, int hi, int c) {
+    while (lo < hi) { const int m = (lo + hi) >> 1; if (col[m] < c) lo = m + 1; else hi = m; }
+    return lo;
+}
+
+__global__ __launch_bounds__(64) void fsai_rows(int V, const int* __restrict__ rowptr, const int* __restrict__ col,
+                                                const double* __restrict__ val, double* __restrict__ G) {
+    __shared__ double M[kFsaiN * kFsaiLd];
+    __shared__ double Z[kFsaiN * 3];
+    __shared__ int P[kFsaiMax], Pe[kFsaiMax];
+    __shared__ int np_sh;
+    __shared__ double red[64];
+    const int lane = threadIdx.x;
+    for (int i = blockIdx.x; i < V; i += gridDim.x) {
+        const int e0 = rowptr[i], e1 = rowptr[i + 1];
+        const int ediag = find_col(col, e0, e1, i);           // diagonal entry of row i (present)
+        const int nl = ediag - e0;                              // off-diagonal lower candidates
+        // choose the candidates: all, or the kFsaiMax-1 largest by Frobenius norm (ties by column)
+        for (int e = e0 + lane; e < e1; e += 64) {              // clear this row's G entries
+            for (int k = 0; k < 9; ++k) G[(size_t)9 * e + k] = 0.0;
+        }
+        __syncthreads();
+        if (lane == 0) np_sh = 0;
+        __syncthreads();
+        for (int c = lane; c < nl; c += 64) {
+            bool keep = true;
+            if (nl > kFsaiMax - 1) {
+                double nc = 0;
+                for (int k = 0; k < 9; ++k) nc += val[(size_t)9 * (e0 + c) * 1 + k] * val[(size_t)9 * (e0 + c) + k];
+                int rank = 0;
+                for (int o = 0; o < nl; ++o) {
+                    double no = 0;
+                    for (int k = 0; k < 9; ++k) no += val[(size_t)9 * (e0 + o) + k] * val[(size_t)9 * (e0 + o) + k];
+                    rank += (no > nc) || (no == nc && o < c);
+                }
+                keep = rank < kFsaiMax - 1;
+            }
+            if (keep) {
+                const int slot = atomicAdd(&np_sh, 1);
+                Pe[slot] = e0 + c;
+            }
+        }
+        __syncthreads();
+        const int k = np_sh + 1;
+        if (lane == 0) {
+            // sort the kept entries by column (insertion sort, k <= kFsaiMax), diagonal last
+            for (int a = 1; a < k - 1; ++a) {
+                const int v = Pe[a];
+                int b = a - 1;
+                while (b >= 0 && Pe[b] > v) { Pe[b + 1] = Pe[b]; --b; }
+                Pe[b + 1] = v;
+            }
+            Pe[k - 1] = ediag;
+            for (int a = 0; a < k; ++a) P[a] = col[Pe[a]];
+        }
+        __syncthreads();
+        const int n = 3 * k;
+        // M = A[P,P] (lower blocks looked up in the block rows of P; upper by symmetry)
+        for (int ab = lane; ab < k * k; ab += 64) {
+            const int a = ab / k, b = ab % k;
+            if (b > a) continue;
+            const int ra = P[a];
+            const int rs = rowptr[ra], re = rowptr[ra + 1];
+            const int f = find_col(col, rs, re, P[b]);
+            const bool has = f < re && col[f] == P[b];
+            for (int u = 0; u < 3; ++u)
+                for (int v = 0; v < 3; ++v) {
+                    const double m = has ? val[(size_t)9 * f + 3 * u + v] : 0.0;   // A_{P_a P_b}(u, v)
+                    M[(3 * a + u) * kFsaiLd + 3 * b + v] = m;
+                    M[(3 * b + v) * kFsaiLd + 3 * a + u] = m;
+                }
+        }
+        __syncthreads();
+        // Cholesky M = L L^T in place (lower triangle)
+        for (int c = 0; c < n; ++c) {
+            if (lane == 0) M[c * kFsaiLd + c] = sqrt(fmax(M[c * kFsaiLd + c], 1e-300));
+            __syncthreads();
+            const double dc = M[c * kFsaiLd + c];
+            for (int r = c + 1 + lane; r < n; r += 64) M[r * kFsaiLd + c] /= dc;
+            __syncthreads();
+            const int m = n - c - 1;
+            for (int idx = lane; idx < m * m; idx += 64) {
+                const int r = c + 1 + idx / m, q = c + 1 + idx % m;
+                if (q <= r) M[r * kFsaiLd + q] -= M[r * kFsaiLd + c] * M[q * kFsaiLd + c];
+            }
+            __syncthreads();
+        }
+        // L^T Z = E (last three unit columns), bottom-up; lanes split the dot products
+        for (int r = n - 1; r >= 0; --r) {
+            for (int qq = 0; qq < 3; ++qq) {
+                double acc = 0.0;
+                for (int s2 = r + 1 + lane; s2 < n; s2 += 64) acc += M[s2 * kFsaiLd + r] * Z[s2 * 3 + qq];
+                red[lane] = acc;
+                __syncthreads();
+                for (int w = 32; w > 0; w >>= 1) {
+                    if (lane < w) red[lane] += red[lane + w];
+                    __syncthreads();
+                }
+                if (lane == 0) {
+                    const double rhs = (r == n - 3 + qq) ? 1.0 : 0.0;
+                    Z[r * 3 + qq] = (rhs - red[0]) / M[r * kFsaiLd + r];
+                }
+                __syncthreads();
+            }
+        }
+        // G_{i, P_a} (row u, column v) = Z[3a + v][u]
+        for (int idx = lane; idx < 9 * k; idx += 64) {
+            const int a = idx / 9, u = (idx % 9) / 3, v = idx % 3;
+            G[(size_t)9 * Pe[a] + 3 * u + v] = Z[(3 * a + v) * 3 + u];
+        }
+        __syncthreads();
+    }
+}
+// mirror[e] = index of the entry (col(e), row(e))
+__global__ void bsr_mirror(int V, const int* __restrict__ rowptr, const int* __restrict__ col, int* __restrict__ mirror) {
+    const int a = blockIdx.x * blockDim.x + threadIdx.x;
+    if (a >= V) return;
+    for (int e = rowptr[a]; e < rowptr[a + 1]; ++e) {
+        const int j = col[e];
+        mirror[e] = find_col(col, rowptr[j], rowptr[j + 1], a);
+    }
+}
+
 // ---- host ---------------------------------------------------------------------------
 namespace {
 PcgScalars read_scalars(Context& c, const PcgScalars* d) {
@@ -285,33 +531,78 @@ PcgScalars read_scalars(Context& c, const PcgScalars* d) {
     return h;
 }
 
-// One PCG solve of A x = b (b FP64; A, x and the work vectors in T).  Returns iterations.
+// The preconditioner of one solve: block-Jacobi (minv) or the factor G (with mirror).
 template <typename T>
-int pcg_run(Context& c, const T* val, const T* minv, const double* b, T* x, int x_zero, double rel_tol, int rel_to_r0, int max_iter,
-            double& rr_out, double& bn_out) {
+struct Prec {
+    const T* minv = nullptr;
+    const T* G = nullptr;
+    const int* mirror = nullptr;
+};
+
+template <typename T>
+void apply_prec(Context& c, const Prec<T>& M, const T* r, T* z, T* p, int init, const PcgScalars* sc, double* part_rz) {
+    const int V = c.sc.V, ng = (int)blocks_for(V, kRows);
+    SolveBuffers& w = c.solve;
+    if (M.G) {
+        T* t = reinterpret_cast<T*>(w.t.p);
+        hipLaunchKernelGGL(prec_g1<T>, dim3(ng), dim3(kPB), 0, c.stream, V, w.rowptr.p, w.col.p, M.G, r, t, sc);
+        hipLaunchKernelGGL(prec_g2<T>, dim3(ng), dim3(kPB), 0, c.stream, V, w.rowptr.p, w.col.p, M.mirror, M.G, t, r, z, p, init, sc,
+                           part_rz);
+    } else {
+        hipLaunchKernelGGL(prec_jacobi<T>, dim3(ng), dim3(kPB), 0, c.stream, V, M.minv, r, z, p, init, sc, part_rz);
+    }
+}
+
+// One PCG solve of A x = b (b FP64; A, x and the work vectors in T).  kCheck iterations are
+// captured once into a HIP graph and replayed until the device flag says converged.
+template <typename T>
+int pcg_run(Context& c, const T* val, const Prec<T>& M, const double* b, T* x, int x_zero, double rel_tol, int rel_to_r0,
+            int max_iter, double& rr_out, double& bn_out) {
     const int V = c.sc.V;
-    const int ng = (int)blocks_for(V, kPB);
+    const int ng = (int)blocks_for(V, kRows);
+    const int ngd = (int)blocks_for(3 * V, kPB);
     SolveBuffers& w = c.solve;
     T* r = reinterpret_cast<T*>(w.r.p);
     T* z = reinterpret_cast<T*>(w.z.p);
     T* p = reinterpret_cast<T*>(w.p.p);
     T* q = reinterpret_cast<T*>(w.q.p);
-    double* part = w.part.p;
+    double* part_pq = w.part.p;
+    double* part_rz = w.part.p + ng;
+    double* part_rr = w.part.p + 2 * ng;
+    double* part_bb = w.part.p + 3 * ng;
     PcgScalars* sc = reinterpret_cast<PcgScalars*>(w.scal.p);
-    hipLaunchKernelGGL(pcg_start<T>, dim3(ng), dim3(kPB), 0, c.stream, V, w.rowptr.p, w.col.p, val, minv, b, x, x_zero, r, z, p, part);
-    hipLaunchKernelGGL(pcg_init_scalars, dim3(1), dim3(kPB), 0, c.stream, part, ng, rel_tol, rel_to_r0, sc);
+    hipLaunchKernelGGL(pcg_residual<T>, dim3(ng), dim3(kPB), 0, c.stream, V, w.rowptr.p, w.col.p, val, b, x, x_zero, r, part_rr, part_bb);
+    apply_prec<T>(c, M, r, z, p, 1, nullptr, part_rz);
+    hipLaunchKernelGGL(pcg_init_scalars, dim3(1), dim3(kPB), 0, c.stream, part_rz, part_rr, part_bb, ng, rel_tol, rel_to_r0, sc);
     CIPC_CHECK_LAUNCH();
     PcgScalars h = read_scalars(c, sc);
-    while (!h.done) {
-        for (int k = 0; k < kCheck; ++k) {
-            hipLaunchKernelGGL(pcg_spmv<T>, dim3(ng), dim3(kPB), 0, c.stream, V, w.rowptr.p, w.col.p, val, p, q, sc, part);
-            hipLaunchKernelGGL(pcg_update<T>, dim3(ng), dim3(kPB), 0, c.stream, V, part, ng, minv, p, q, x, r, z, sc, part + ng);
-            hipLaunchKernelGGL(pcg_direction<T>, dim3(ng), dim3(kPB), 0, c.stream, V, part + ng, ng, z, p, sc);
-            hipLaunchKernelGGL(pcg_scalars, dim3(1), dim3(kPB), 0, c.stream, part + ng, ng, sc, max_iter);
-        }
-        CIPC_CHECK_LAUNCH();
-        h = read_scalars(c, sc);
+    if (h.done) { rr_out = h.rr; bn_out = h.bn; return h.iters; }
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    CIPC_HIP(hipStreamBeginCapture(c.stream, hipStreamCaptureModeThreadLocal));
+    for (int k = 0; k < kCheck; ++k) {
+        hipLaunchKernelGGL(pcg_spmv<T>, dim3(ng), dim3(kPB), 0, c.stream, V, w.rowptr.p, w.col.p, val, p, q, sc, part_pq);
+        hipLaunchKernelGGL(pcg_update<T>, dim3(ng), dim3(kPB), 0, c.stream, V, part_pq, ng, M.G ? nullptr : M.minv, p, q, x, r, z, sc,
+                           part_rz, part_rr);
+        if (M.G) apply_prec<T>(c, M, r, z, p, 0, sc, part_rz);
+        hipLaunchKernelGGL(pcg_direction<T>, dim3(ngd), dim3(kPB), 0, c.stream, V, part_rz, ng, z, p, sc);
+        hipLaunchKernelGGL(pcg_scalars, dim3(1), dim3(kPB), 0, c.stream, part_rz, part_rr, ng, sc, max_iter);
     }
+    const hipError_t ce = hipStreamEndCapture(c.stream, &graph);
+    CIPC_HIP(ce);
+    CIPC_HIP(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
+    try {
+        while (!h.done) {
+            CIPC_HIP(hipGraphLaunch(exec, c.stream));
+            h = read_scalars(c, sc);
+        }
+    } catch (...) {
+        (void)hipGraphExecDestroy(exec);
+        (void)hipGraphDestroy(graph);
+        throw;
+    }
+    (void)hipGraphExecDestroy(exec);
+    (void)hipGraphDestroy(graph);
     rr_out = h.rr;
     bn_out = h.bn;
     return h.iters;
@@ -341,56 +632,79 @@ void build_solve_matrix(Context& c, bool fp32) {
     CIPC_CHECK_LAUNCH();
     if (fp32) {
         w.valf.resize(9 * ne); w.minvf.resize((size_t)9 * V);
-        hipLaunchKernelGGL((convert<double, float>), dim3(blocks_for(9 * ne, 256)), dim3(256), 0, c.stream, (int)(9 * ne), w.val.p,
-                           w.valf.p);
+        hipLaunchKernelGGL((convert<double, float>), dim3(blocks_for(9 * ne, 256)), dim3(256), 0, c.stream, 9 * ne, w.val.p, w.valf.p);
         hipLaunchKernelGGL(diag_inverse<float>, dim3(blocks_for(V, 256)), dim3(256), 0, c.stream, V, w.rowptr.p, w.col.p, w.valf.p,
                            w.minvf.p);
         CIPC_CHECK_LAUNCH();
     }
+    w.ne = ne;
 }
 
-void linear_solve(Context& c, int precision, double rel_tol, int max_iter, SolveResult& res) {
+// The block FSAI factor of the current BSR (and its FP32 copy for the mixed form).
+void build_fsai(Context& c, bool fp32) {
+    SolveBuffers& w = c.solve;
+    const int V = c.sc.V;
+    const size_t ne = w.ne;
+    w.G.resize(9 * ne);
+    w.mirror.resize(ne);
+    hipLaunchKernelGGL(bsr_mirror, dim3(blocks_for(V, 256)), dim3(256), 0, c.stream, V, w.rowptr.p, w.col.p, w.mirror.p);
+    const int grid = std::min(V, 256 * 16);
+    hipLaunchKernelGGL(fsai_rows, dim3(grid), dim3(64), 0, c.stream, V, w.rowptr.p, w.col.p, w.val.p, w.G.p);
+    CIPC_CHECK_LAUNCH();
+    if (fp32) {
+        w.Gf.resize(9 * ne);
+        hipLaunchKernelGGL((convert<double, float>), dim3(blocks_for(9 * ne, 256)), dim3(256), 0, c.stream, 9 * ne, w.G.p, w.Gf.p);
+        CIPC_CHECK_LAUNCH();
+    }
+}
+
+void linear_solve(Context& c, int precision, int preconditioner, double rel_tol, int max_iter, SolveResult& res) {
     Scene& s = c.sc;
     const int V = s.V, n = 3 * V;
     SolveBuffers& w = c.solve;
-    const int ng = (int)blocks_for(V, kPB);
-    w.r.resize((size_t)n); w.z.resize((size_t)n); w.p.resize((size_t)n); w.q.resize((size_t)n);
+    const int ng = (int)blocks_for(V, kRows);
+    w.r.resize((size_t)n); w.z.resize((size_t)n); w.p.resize((size_t)n); w.q.resize((size_t)n); w.t.resize((size_t)n);
     w.part.resize((size_t)4 * ng);
     w.scal.resize((sizeof(PcgScalars) + 7) / 8);
     w.x.resize((size_t)n);
     build_solve_matrix(c, precision == 1);
+    if (preconditioner == 1) build_fsai(c, precision == 1);
+    else if (preconditioner == 2) build_learned(c, precision == 1);
+    Prec<double> Md;
+    Prec<float> Mf;
+    if (preconditioner == 0) { Md.minv = w.minv.p; Mf.minv = w.minvf.p; }
+    else { Md.G = w.G.p; Md.mirror = w.mirror.p; Mf.G = w.Gf.p; Mf.mirror = w.mirror.p; }
     res = SolveResult{};
     if (precision == 0) {
         double rr = 0, bn = 0;
         hipLaunchKernelGGL(zero_d, dim3(blocks_for(n, 256)), dim3(256), 0, c.stream, n, w.x.p);
-        res.iterations = pcg_run<double>(c, w.val.p, w.minv.p, c.rhs.p, w.x.p, 1, rel_tol, 0, max_iter, rr, bn);
+        res.iterations = pcg_run<double>(c, w.val.p, Md, c.rhs.p, w.x.p, 1, rel_tol, 0, max_iter, rr, bn);
         res.rel_residual = bn > 0 ? std::sqrt(rr) / bn : 0.0;
     } else {
-        // FP64 refinement around FP32 PCG
+        // FP64 refinement around FP32 PCG; each inner solve aims just past the remaining
+        // reduction (floored at the FP32 reach), so one or two refinements usually suffice
         w.xf.resize((size_t)n); w.res.resize((size_t)n);
         hipLaunchKernelGGL(zero_d, dim3(blocks_for(n, 256)), dim3(256), 0, c.stream, n, w.x.p);
-        const double inner_tol = 1e-3;
         double rr = 0, bn = 0;
         int iters = 0, outer = 0;
+        double* part_rr = w.part.p + 2 * ng;
+        double* part_bb = w.part.p + 3 * ng;
         for (;;) {
-            // r = b - A x in FP64, ||r|| / ||b||
             res.iterations = iters;
             double* rd = w.res.p;
-            {
-                // reuse the FP64 start kernel to form r (and its norms) without a solve
-                hipLaunchKernelGGL(pcg_start<double>, dim3(ng), dim3(kPB), 0, c.stream, V, w.rowptr.p, w.col.p, w.val.p, w.minv.p,
-                                   c.rhs.p, w.x.p, outer == 0 ? 1 : 0, rd, w.q.p, w.p.p, w.part.p);
-                hipLaunchKernelGGL(pcg_init_scalars, dim3(1), dim3(kPB), 0, c.stream, w.part.p, ng, rel_tol, 0,
-                                   reinterpret_cast<PcgScalars*>(w.scal.p));
-                CIPC_CHECK_LAUNCH();
-                const PcgScalars h = read_scalars(c, reinterpret_cast<PcgScalars*>(w.scal.p));
-                rr = h.rr; bn = h.bn;
-                if (!(std::sqrt(rr) > rel_tol * bn) || iters >= max_iter) break;
-            }
-            // inner FP32 solve of A d = r
+            hipLaunchKernelGGL(pcg_residual<double>, dim3(ng), dim3(kPB), 0, c.stream, V, w.rowptr.p, w.col.p, w.val.p, c.rhs.p, w.x.p,
+                               outer == 0 ? 1 : 0, rd, part_rr, part_bb);
+            hipLaunchKernelGGL(pcg_init_scalars, dim3(1), dim3(kPB), 0, c.stream, part_rr, part_rr, part_bb, ng, rel_tol, 0,
+                               reinterpret_cast<PcgScalars*>(w.scal.p));
+            CIPC_CHECK_LAUNCH();
+            const PcgScalars h = read_scalars(c, reinterpret_cast<PcgScalars*>(w.scal.p));
+            rr = h.rr; bn = h.bn;
+            if (!(std::sqrt(rr) > rel_tol * bn) || iters >= max_iter) break;
+            const double need = rel_tol * bn / std::sqrt(rr);      // reduction still needed
+            const double inner_tol = std::max(0.3 * need, 2e-6);
             hipLaunchKernelGGL(zero_f, dim3(blocks_for(n, 256)), dim3(256), 0, c.stream, n, w.xf.p);
             double irr = 0, ibn = 0;
-            const int it = pcg_run<float>(c, w.valf.p, w.minvf.p, rd, w.xf.p, 1, inner_tol, 1, max_iter - iters, irr, ibn);
+            const int it = pcg_run<float>(c, w.valf.p, Mf, rd, w.xf.p, 1, inner_tol, 1, max_iter - iters, irr, ibn);
             if (it == 0 || outer >= kMaxRefine) break;   // no progress possible (breakdown) or refinement cap
             iters += it;
             hipLaunchKernelGGL(add_correction, dim3(blocks_for(n, 256)), dim3(256), 0, c.stream, n, w.xf.p, w.x.p);

@@ -154,6 +154,27 @@ int cipc_gpu_get_csr(cipc_gpu_context* ctx, int* outer, int* inner, double* valu
 int cipc_gpu_energy(cipc_gpu_context* ctx, double kappa, double* energy);
 int cipc_gpu_energy_terms(cipc_gpu_context* ctx, double kappa, double terms[9]);
 
+/* Learned block preconditioner (solve preconditioner 2).  No counterpart in the reference.
+ * A message-passing network (hidden width 64, 3 layers, bfloat16 weights with float32
+ * accumulation) over the vertex-block graph of the Newton matrix predicts the block
+ * lower-triangular factor G of M^-1 = G^T G as a correction of the first-order block FSAI
+ * of the symmetrically scaled matrix.  Weights are a flat float32 vector of
+ * cipc_gpu_preconditioner_param_count() values (layout of oracle/gnn_oracle.py).
+ *  set_preconditioner_weights: weights for later solves with preconditioner 2;
+ *  preconditioner_gradient: a training loss of the last assembled system and its gradient
+ *    (param_count): objective 0, sum_z |G^ A^ G^T z - z|^2 / (n_z 3V) for the given probes
+ *    (n_probes x 3V); objective 1, the mean squared difference of G^ from the block FSAI factor
+ *    of the same matrix in the scaled basis (probes unused);
+ *  preconditioner_train_step: one Adam step (beta 0.9 / 0.999) on the last assembled system
+ *    (objective 0 with n_probes Gaussian probes drawn from `seed`); params and moments updated
+ *    in place. */
+int64_t cipc_gpu_preconditioner_param_count(void);
+int cipc_gpu_set_preconditioner_weights(cipc_gpu_context* ctx, const float* weights, int64_t n);
+int cipc_gpu_preconditioner_gradient(cipc_gpu_context* ctx, int objective, const float* params, const float* probes, int n_probes,
+                                     double* loss, float* grad);
+int cipc_gpu_preconditioner_train_step(cipc_gpu_context* ctx, int objective, float* params, float* adam_m, float* adam_v, int step,
+                                       double lr, int n_probes, uint64_t seed, double* loss);
+
 /* The data-parallel parts of the barrier-stiffness control; the scalar logic of
  * suggestKappa / upperBoundKappa / initKappa / postLineSearch (IPCSolver.cpp:751-869) stays
  * with the caller.
@@ -186,6 +207,13 @@ int cipc_gpu_set_soft_targets(cipc_gpu_context* ctx, int n, const int* vertices,
  * NULL).  CIPC_ERR_RUNTIME when the target is not met within max_iter, as the reference's
  * solveWithEigenConjugateGradient throws (NewtonLinearSystem.cpp:196-203). */
 int cipc_gpu_solve(cipc_gpu_context* ctx, int precision, double rel_tol, int max_iter, double* direction, cipc_solve_stats* stats);
+/* The same solve with a choice of preconditioner: 0 block-Jacobi (as cipc_gpu_solve), 1 the
+ * 3x3-block factorized sparse approximate inverse M^-1 = G^T G with G on the matrix's lower
+ * block pattern (at most 24 blocks per row, the largest kept), 2 the learned factor of the
+ * weights set with cipc_gpu_set_preconditioner_weights.  No counterpart in the reference,
+ * whose iterative solver is Eigen CG with incomplete Cholesky (NewtonLinearSystem.cpp:176-207). */
+int cipc_gpu_solve_ex(cipc_gpu_context* ctx, int precision, int preconditioner, double rel_tol, int max_iter, double* direction,
+                      cipc_solve_stats* stats);
 
 /* Friction::initialize (Friction.cpp:81-156), called at the start of a time step: the
  * lagged friction state (per Self_ActiveSet entry lambda, closest-point coordinates and

@@ -23,7 +23,9 @@ model, shared with the device:
 * G^_ii = T_i B_ii, G^_ij = B_ij + D_ij: block lower-triangular on A's pattern, equal to
   the base when the decoders are zero;
 * preconditioner M^-1 = S G^T G S;
-* loss L = sum_z |G^ A^ G^T z - z|^2 / (n_z 3n) over Gaussian probes z.
+* losses: L = sum_z |G^ A^ G^T z - z|^2 / (n_z 3n) over Gaussian probes z (objective 0), or
+  the mean squared difference of G^ from the block FSAI factor of A in the scaled basis
+  (objective 1, oracle/fsai_oracle.py).
 
 Storage roundings of the device are reproduced with ``bf16=True``: features, embeddings,
 hidden states, messages, aggregates and every GEMM input are rounded to bfloat16 (round to
@@ -243,7 +245,37 @@ def loss_and_grad(flat, g: Graph, val, Z, use_bf16=True):
     return L, pack(grads), Gd, Go
 
 
+def fsai_targets(g: Graph, val, d):
+    """The block FSAI factor (oracle/fsai_oracle.py) in the scaled basis, F^_ij = F_ij sqrt(d_j),
+    as (diagonal blocks (n,3,3), lower off-diagonal blocks (n_low,3,3))."""
+    import fsai_oracle
+    F = fsai_oracle.block_fsai(g.rowptr, g.col, val).toarray()
+    n = g.n
+    Fd = np.stack([F[3 * i:3 * i + 3, 3 * i:3 * i + 3] * np.sqrt(d[i]) for i in range(n)])
+    i, j = g.src[g.low], g.dst[g.low]
+    Fo = np.stack([F[3 * a:3 * a + 3, 3 * b:3 * b + 3] * np.sqrt(d[b]) for a, b in zip(i, j)]) if len(i) else np.zeros((0, 3, 3))
+    return Fd, Fo
+
+
+def loss_and_grad_fsai(flat, g: Graph, val, use_bf16=True):
+    """Objective 1: mean squared difference of G^ from the scaled block FSAI factor over the
+    lower blocks (diagonal included), per entry."""
+    p = unpack(flat)
+    X, E, d = features(g, val)
+    An = normalized_blocks(g, val, d)
+    Bd, Bo = base_factor(g, An)
+    Gd, Go, c = forward(p, g, X, E, Bd, Bo, use_bf16)
+    Fd, Fo = fsai_targets(g, val, d)
+    cst = 1.0 / (9.0 * (g.n + len(g.low)))
+    L = cst * (np.sum((Gd - Fd) ** 2) + np.sum((Go - Fo) ** 2))
+    grads = backward(p, g, c, 2 * cst * (Gd - Fd), 2 * cst * (Go - Fo), use_bf16)
+    return L, pack(grads), Gd, Go
+
+
 def backward(p, g: Graph, c, dGd, dGo, use_bf16=True):
+    """Parameter gradients.  With ``use_bf16`` the upstream gradient of every GEMM is rounded
+    to bfloat16 as the GEMM operand (weight gradient X^T dY and input gradient dY W^T), as
+    on the device; bias gradients use the unrounded values."""
     R = bf16 if use_bf16 else (lambda v: np.asarray(v, np.float64))
     gr = {k: np.zeros_like(v) for k, v in p.items()}
     n = g.n
@@ -259,12 +291,12 @@ def backward(p, g: Graph, c, dGd, dGo, use_bf16=True):
     doo[:, :9] = dGo.reshape(-1, 9)
     h = c["h"][-1]
     e = c["e"]
-    gr["Wd"] = h.T @ dod
+    gr["Wd"] = h.T @ R(dod)
     gr["bd"] = dod.sum(axis=0)
-    gr["Wo"] = c["cat_o"].T @ doo
+    gr["Wo"] = c["cat_o"].T @ R(doo)
     gr["bo"] = doo.sum(axis=0)
-    dh = dod @ R(p["Wd"]).T
-    dcat = doo @ R(p["Wo"]).T
+    dh = R(dod) @ R(p["Wd"]).T
+    dcat = R(doo) @ R(p["Wo"]).T
     i, j = g.src[g.low], g.dst[g.low]
     de = np.zeros_like(e)
     de[g.low] += dcat[:, H:]
@@ -274,23 +306,23 @@ def backward(p, g: Graph, c, dGd, dGo, use_bf16=True):
         hprev, m, a, pre = c["h"][l], c[f"m{l}"], c[f"a{l}"], c[f"pre_u{l}"]
         dpre = dh * (pre > 0)
         cat = np.concatenate([hprev, a], axis=1)
-        gr[f"Wu{l}"] = cat.T @ dpre
+        gr[f"Wu{l}"] = cat.T @ R(dpre)
         gr[f"bu{l}"] = dpre.sum(axis=0)
-        dcat_u = dpre @ R(p[f"Wu{l}"]).T
+        dcat_u = R(dpre) @ R(p[f"Wu{l}"]).T
         dhp = dh + dcat_u[:, :H]
         da = dcat_u[:, H:]
         # a_i = sum_j e_ij m_j
         de += da[g.src] * m[g.dst]
         dm = np.zeros_like(m)
         np.add.at(dm, g.dst, da[g.src] * e)
-        gr[f"Wm{l}"] = hprev.T @ dm
-        dhp += dm @ R(p[f"Wm{l}"]).T
+        gr[f"Wm{l}"] = hprev.T @ R(dm)
+        dhp += R(dm) @ R(p[f"Wm{l}"]).T
         dh = dhp
     dpre_e = de * (c["pre_e"] > 0)
-    gr["We"] = c["Eb"].T @ dpre_e
+    gr["We"] = c["Eb"].T @ R(dpre_e)
     gr["be"] = dpre_e.sum(axis=0)
     dpre_n = dh * (c["pre_n"] > 0)
-    gr["Wn"] = c["Xb"].T @ dpre_n
+    gr["Wn"] = c["Xb"].T @ R(dpre_n)
     gr["bn"] = dpre_n.sum(axis=0)
     return gr
 

@@ -35,8 +35,9 @@ def gpu():
     return g
 
 
+@pytest.mark.parametrize("preconditioner", [0, 1])
 @pytest.mark.parametrize("precision", [0, 1])
-def test_solve_small_scene_matches_direct(gpu, precision):
+def test_solve_small_scene_matches_direct(gpu, precision, preconditioner):
     import scipy.sparse.linalg as spla
     from cipc import scenes
     s = scenes.stacked_cubes(2, 6, 0.1, 1e-3, seed=3, rotate="yaw", velocity_scale=0.5, name="two_cubes")
@@ -44,10 +45,10 @@ def test_solve_small_scene_matches_direct(gpu, precision):
     try:
         A = _sym(r)
         b = r["rhs"]
-        p, st = g.solve(precision=precision, rel_tol=1e-6)
+        p, st = g.solve(precision=precision, rel_tol=1e-6, preconditioner=preconditioner)
         p = p.reshape(-1)
         res = np.linalg.norm(b - A @ p) / np.linalg.norm(b)
-        assert res <= 1.5e-6, (precision, res, st)
+        assert res <= 1.5e-6, (precision, preconditioner, res, st)
         assert st["rel_residual"] <= 1e-6 and st["iterations"] > 0
         assert (st["refinements"] > 0) == (precision == 1)
         ref = spla.spsolve(A.tocsc(), b)
@@ -59,17 +60,53 @@ def test_solve_small_scene_matches_direct(gpu, precision):
         g.close()
 
 
+@pytest.mark.parametrize("preconditioner", [0, 1])
 @pytest.mark.parametrize("precision", [0, 1])
-def test_solve_four_cubes_residual(gpu, precision):
+def test_solve_four_cubes_residual(gpu, precision, preconditioner):
     from cipc import scenes
     s = scenes.four_cubes20(velocity_scale=0.5)
     g, r = _system(gpu, s)
     try:
         A = _sym(r)
         b = r["rhs"]
-        p, st = g.solve(precision=precision, rel_tol=1e-6)
+        p, st = g.solve(precision=precision, rel_tol=1e-6, preconditioner=preconditioner)
         res = np.linalg.norm(b - A @ p.reshape(-1)) / np.linalg.norm(b)
-        assert res <= 1.5e-6, (precision, res, st)
+        assert res <= 1.5e-6, (precision, preconditioner, res, st)
+    finally:
+        g.close()
+
+
+def test_block_fsai_matches_oracle_iterations(gpu):
+    """The device's block FSAI (FP64 PCG) against the numpy restatement of the same factor on
+    the same system: the same iteration count to 1e-6 (up to one), and far fewer than
+    block-Jacobi."""
+    import scipy.sparse as sp
+    from cipc import scenes
+    import fsai_oracle as F
+    s = scenes.stacked_cubes(2, 6, 0.1, 1e-3, seed=3, rotate="yaw", velocity_scale=0.5, name="two_cubes")
+    g, r = _system(gpu, s)
+    try:
+        A = _sym(r)
+        b = r["rhs"]
+        B = sp.bsr_matrix(A, blocksize=(3, 3))
+        B.sort_indices()
+        Gm = F.block_fsai(B.indptr, B.indices, np.asarray(B.data))
+        it_o, _ = F.pcg_iterations(A, b, lambda v: Gm.T @ (Gm @ v))
+        _, st1 = g.solve(precision=0, rel_tol=1e-6, preconditioner=1)
+        _, st0 = g.solve(precision=0, rel_tol=1e-6, preconditioner=0)
+        assert abs(st1["iterations"] - it_o) <= 1, (st1, it_o)
+        assert st1["iterations"] * 2 < st0["iterations"], (st1, st0)
+    finally:
+        g.close()
+
+
+def test_learned_without_weights_is_an_error(gpu):
+    from cipc import scenes
+    s = scenes.stacked_cubes(1, 3, 0.1, 0.5, seed=2, rotate="none")
+    g, r = _system(gpu, s)
+    try:
+        with pytest.raises(gpu.CipcLogicError):
+            g.solve(preconditioner=2)
     finally:
         g.close()
 

@@ -102,3 +102,23 @@ def test_bf16_rounding():
     assert r[3] == 1.0 + 2 ** -6                           # tie: to the even neighbour (up)
     assert r[4] == 1.0                                     # below half a spacing: down
     assert abs(r[5] - x[5]) <= 2 ** -8 * abs(x[5]) and r[6] == 0.0
+
+
+def test_product_layout_matches_oracle():
+    import gnn_oracle as G
+    from cipc import gnn_layout
+    assert gnn_layout.param_shapes() == G.param_shapes() and gnn_layout.n_params() == G.n_params()
+
+
+def test_fsai_objective_gradient_matches_finite_differences(G, system):
+    rowptr, col, val, _ = system
+    g = G.Graph(rowptr, col)
+    rng = np.random.default_rng(4)
+    flat = G.init_params(1) + 0.05 * rng.standard_normal(G.n_params())
+    L, grad, _, _ = G.loss_and_grad_fsai(flat, g, val, use_bf16=False)
+    h = 1e-6
+    for k in rng.choice(len(flat), 30, replace=False):
+        e = np.zeros_like(flat)
+        e[k] = h
+        fd = (G.loss_and_grad_fsai(flat + e, g, val, use_bf16=False)[0] - G.loss_and_grad_fsai(flat - e, g, val, use_bf16=False)[0]) / (2 * h)
+        assert fd == pytest.approx(grad[k], rel=1e-5, abs=1e-8 * np.abs(grad).max()), k
