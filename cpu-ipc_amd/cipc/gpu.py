@@ -38,6 +38,9 @@ EXPORTS = (
     "cipc_gpu_close_constraints", "cipc_gpu_clear_close_constraints", "cipc_gpu_preconditioner_param_count",
     "cipc_gpu_set_preconditioner_weights", "cipc_gpu_preconditioner_gradient", "cipc_gpu_preconditioner_train_step", "cipc_gpu_solve", "cipc_gpu_solve_ex", "cipc_gpu_save_positions", "cipc_gpu_step_forward",
     "cipc_gpu_is_intersected", "cipc_gpu_newton_iteration",
+    "cipc_gpu_upload_batch", "cipc_gpu_batch_size", "cipc_gpu_batch_assemble", "cipc_gpu_batch_friction_update",
+    "cipc_gpu_batch_energy", "cipc_gpu_batch_step_bound", "cipc_gpu_batch_step_forward", "cipc_gpu_batch_is_intersected",
+    "cipc_gpu_batch_solve", "cipc_gpu_batch_newton_iteration",
     "cipc_gpu_eval_snh", "cipc_gpu_eval_cloth", "cipc_gpu_eval_hinge", "cipc_gpu_eval_distance", "cipc_gpu_eval_make_pd", "cipc_gpu_eval_contact_pd",
     "cipc_gpu_eval_accd",
     "cipc_gpu_eval_classify",
@@ -131,6 +134,16 @@ def load(path: str = LIB_PATH):
         "cipc_gpu_step_forward": [C.c_void_p, C.c_double],
         "cipc_gpu_is_intersected": [C.c_void_p, _ip],
         "cipc_gpu_newton_iteration": [C.c_void_p, C.c_double, C.POINTER(IterationTiming)],
+        "cipc_gpu_upload_batch": [C.c_void_p, C.POINTER(C.POINTER(SceneDesc)), C.c_int],
+        "cipc_gpu_batch_size": [C.c_void_p, _ip, _ip, _ip],
+        "cipc_gpu_batch_assemble": [C.c_void_p, _dp, _i64p, _ip],
+        "cipc_gpu_batch_friction_update": [C.c_void_p, _dp],
+        "cipc_gpu_batch_energy": [C.c_void_p, _dp, _dp, _dp],
+        "cipc_gpu_batch_step_bound": [C.c_void_p, _dp],
+        "cipc_gpu_batch_step_forward": [C.c_void_p, _dp],
+        "cipc_gpu_batch_is_intersected": [C.c_void_p, _ip],
+        "cipc_gpu_batch_solve": [C.c_void_p, C.c_int, C.c_int, C.c_double, C.c_int, _ip, _dp, C.POINTER(SolveStats)],
+        "cipc_gpu_batch_newton_iteration": [C.c_void_p, _dp, C.POINTER(IterationTiming), _dp, _dp],
         "cipc_gpu_eval_snh": [C.c_int, _dp, C.c_double, C.c_double, _dp, _dp, _dp],
         "cipc_gpu_eval_cloth": [C.c_int, _dp, C.c_double, C.c_double, C.c_double, _dp, _dp, _dp],
         "cipc_gpu_eval_hinge": [C.c_int, _dp, _dp, _dp, C.c_double, _dp, _dp, _dp],
@@ -392,6 +405,114 @@ class GpuContext:
         t = IterationTiming()
         _check(load().cipc_gpu_newton_iteration(self.h, kappa, C.byref(t)))
         return {f: getattr(t, f) for f, _ in IterationTiming._fields_}
+
+
+class BatchContext(GpuContext):
+    """Independent tet scenes packed into one context (cipc_gpu_upload_batch): each scene
+    is evaluated as a context holding it alone would evaluate it.  Arrays over the whole
+    batch (positions, gradient, CSR) list scene k's vertices after scene k-1's; the
+    per-scene methods take and return one value per scene."""
+
+    def __init__(self, scenes, xtildes=None, device: int = 0):
+        L = load()
+        h = C.c_void_p()
+        _check(L.cipc_gpu_context_create(device, C.byref(h)))
+        self.h = h
+        self.scenes = list(scenes)
+        self.S = len(self.scenes)
+        xts = xtildes if xtildes is not None else [None] * self.S
+        descs, self._keep = [], []
+        for sc, xt in zip(self.scenes, xts):
+            d, keep = scene_desc(sc, xt)
+            descs.append(d)
+            self._keep.append(keep)
+        ptrs = (C.POINTER(SceneDesc) * self.S)(*[C.pointer(d) for d in descs])
+        _check(L.cipc_gpu_upload_batch(self.h, ptrs, self.S))
+        self._keep = None
+        n = np.zeros(1, np.int32)
+        self.voff = np.zeros(self.S + 1, np.int32)
+        self.toff = np.zeros(self.S + 1, np.int32)
+        _check(L.cipc_gpu_batch_size(self.h, _iptr(n), _iptr(self.voff), _iptr(self.toff)))
+        assert int(n[0]) == self.S
+        self.V = int(self.voff[-1])
+        self.scene = None
+
+    def _k(self, kappa):
+        k = np.broadcast_to(np.asarray(kappa, np.float64), (self.S,))
+        return np.ascontiguousarray(k)
+
+    def split(self, a):
+        """Per-scene views of a per-vertex array of the batch."""
+        return [a[self.voff[k]:self.voff[k + 1]] for k in range(self.S)]
+
+    def assemble(self, kappa):
+        k = self._k(kappa)
+        nnz = np.zeros(1, np.int64)
+        col = np.zeros(self.S, np.int32)
+        _check(load().cipc_gpu_batch_assemble(self.h, _dptr(k), nnz.ctypes.data_as(_i64p), _iptr(col)))
+        n = 3 * self.V
+        g = np.zeros(n)
+        _check(load().cipc_gpu_get_gradient(self.h, _dptr(g)))
+        outer = np.zeros(n + 1, np.int32)
+        inner = np.zeros(max(int(nnz[0]), 1), np.int32)
+        val = np.zeros(max(int(nnz[0]), 1))
+        rhs = np.zeros(n)
+        _check(load().cipc_gpu_get_csr(self.h, _iptr(outer), _iptr(inner), _dptr(val), _dptr(rhs)))
+        m = int(nnz[0])
+        return dict(grad=g.reshape(-1, 3), outer=outer, inner=inner[:m], val=val[:m], rhs=rhs, collisions=col)
+
+    def friction_update(self, kappa):
+        k = self._k(kappa)
+        _check(load().cipc_gpu_batch_friction_update(self.h, _dptr(k)))
+
+    def energy(self, kappa):
+        k = self._k(kappa)
+        e = np.zeros(self.S)
+        _check(load().cipc_gpu_batch_energy(self.h, _dptr(k), _dptr(e), None))
+        return e
+
+    def energy_terms(self, kappa=0.0):
+        k = self._k(kappa)
+        t = np.zeros((self.S, 9))
+        _check(load().cipc_gpu_batch_energy(self.h, _dptr(k), None, _dptr(t)))
+        return t
+
+    def step_bound(self, p=None):
+        if p is not None:
+            self.set_direction(p)
+        out = np.zeros((self.S, 5))
+        _check(load().cipc_gpu_batch_step_bound(self.h, _dptr(out)))
+        return dict(alpha=out[:, 0], ground=out[:, 1], partial=out[:, 2], cfl=out[:, 3], full=out[:, 4])
+
+    def step_forward(self, alpha):
+        a = self._k(alpha)
+        _check(load().cipc_gpu_batch_step_forward(self.h, _dptr(a)))
+
+    def is_intersected(self):
+        r = np.zeros(self.S, np.int32)
+        _check(load().cipc_gpu_batch_is_intersected(self.h, _iptr(r)))
+        return r.astype(bool)
+
+    def solve(self, precision=0, rel_tol=1e-6, max_iter=10000, preconditioner=0, active=None):
+        """Per-scene Newton solves in lock step; returns (direction (V,3), list of stats)."""
+        p = np.zeros((self.V, 3))
+        st = (SolveStats * self.S)()
+        act = None if active is None else _i(np.asarray(active).astype(np.int32))
+        _check(load().cipc_gpu_batch_solve(self.h, precision, preconditioner, rel_tol, max_iter,
+                                           None if act is None else _iptr(act), _dptr(p), st))
+        return p, [dict(iterations=s.iterations, refinements=s.refinements, rel_residual=s.rel_residual,
+                        solve_ms=s.solve_ms) for s in st]
+
+    def newton_iteration(self, kappa):
+        k = self._k(kappa)
+        t = IterationTiming()
+        a = np.zeros(self.S)
+        e = np.zeros(self.S)
+        _check(load().cipc_gpu_batch_newton_iteration(self.h, _dptr(k), C.byref(t), _dptr(a), _dptr(e)))
+        out = {f: getattr(t, f) for f, _ in IterationTiming._fields_}
+        out["alpha_scenes"] = a
+        out["energy_scenes"] = e
+        return out
 
 
 # ---- element-level evaluation -----------------------------------------------------------

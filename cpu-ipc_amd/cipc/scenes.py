@@ -494,6 +494,42 @@ def four_cubes20(seed: int = 1, velocity_scale: float = 0.0, rotate: str = "yaw"
                          velocity_scale=velocity_scale, name="4x20cubes")
 
 
+def batched_cubes(n_scenes: int = 256, seed: int = 0, cells_lo: int = 8, cells_hi: int = 24, edge: float = 1.0,
+                  spacing: float = 3.0, jitter: float = 0.01) -> list:
+    """Independent single-cube scenes for a batch: each a lattice cube of edge ``edge`` with
+    a cell count per axis log-uniform in [cells_lo, cells_hi] (8^3..24^3 cells: 2,560 to
+    69,120 tets), seeded Young's modulus log-uniform in [1e4, 1e6] Pa, Poisson ratio
+    uniform in [0.3, 0.45], dropped from rest with its lowest point 0.02-0.2 m above the
+    ground; mu = 0.2, +-1% vertex jitter.  dHat is relative to the nominal cube diagonal
+    (sqrt(3) edge) in every scene, so the scenes share hhat, fhat and dtol as a batch
+    requires.  Scene k sits at a grid position in the ground plane, ``spacing`` apart
+    (the scenes never interact in any case)."""
+    rng = np.random.default_rng(seed)
+    cache = {}
+    cols = int(math.ceil(math.sqrt(n_scenes)))
+    out = []
+    diag2 = 3.0 * edge * edge
+    for k in range(n_scenes):
+        n = int(round(math.exp(rng.uniform(math.log(cells_lo), math.log(cells_hi)))))
+        n = min(max(n, cells_lo), cells_hi)
+        if n not in cache:
+            cache[n] = lattice(n, edge / n)
+        base_x, base_t = cache[n]
+        E = math.exp(rng.uniform(math.log(1e4), math.log(1e6)))
+        nu = rng.uniform(0.3, 0.45)
+        drop = rng.uniform(0.02, 0.2)
+        mat = Material(youngs=E, poisson=nu, friction=0.2)
+        x = base_x + rng.uniform(-jitter, jitter, base_x.shape) * (edge / n)
+        lo = x.min(axis=0)
+        x = x + np.array([(k % cols) * spacing - lo[0], GROUND_Y + drop - lo[1], (k // cols) * spacing - lo[2]])
+        sc = finalize(f"batch{k}_{n}", x, base_t, mat)
+        sc.hhat = mat.rel_dhat ** 2 * diag2
+        sc.fhat = mat.fhat_rel * diag2
+        sc.dtol = mat.dtol_rel * diag2
+        out.append(sc)
+    return out
+
+
 def suggest_kappa(hhat: float, bbox_diag2: float, mean_mass: float) -> float:
     """suggestKappa (IPCSolver.cpp:751-755)."""
     from cipc.kappa import suggest_kappa as sk

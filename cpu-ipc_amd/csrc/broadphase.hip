@@ -303,32 +303,34 @@ void build_tree(Context& c, LBVH& t, int n) {
 // with the step whose bits are at *abits (buildSwept, LBVH.cpp:307-339).  The step is read
 // on the device, so building the swept trees needs no host round trip.
 __device__ inline double step_of(const unsigned long long* abits) { return __longlong_as_double((long long)*abits); }
+// Swept boxes use the step of the primitive's scene: abits[kSlots * scene] (one scene: abits[0]).
 __global__ void face_boxes(int F, const int4* __restrict__ faces, const double* __restrict__ x, const double* __restrict__ p,
-                           const unsigned long long* __restrict__ abits, Box* __restrict__ out) {
+                           const unsigned long long* __restrict__ abits, const int* __restrict__ vs, Box* __restrict__ out) {
     const int f = blockIdx.x * blockDim.x + threadIdx.x;
     if (f >= F) return;
     const int4 v = faces[f];
     if (abits == nullptr) { out[f] = box_tri(ld3(x, v.x), ld3(x, v.y), ld3(x, v.z)); return; }
-    const double a = step_of(abits);
+    const double a = step_of(vs ? abits + kSlots * vs[v.x] : abits);
     Box b = swept_pt(ld3(x, v.x), ld3(p, v.x), a);
     b = box_union(b, swept_pt(ld3(x, v.y), ld3(p, v.y), a));
     b = box_union(b, swept_pt(ld3(x, v.z), ld3(p, v.z), a));
     out[f] = b;
 }
 __global__ void edge_boxes(int E, const int2* __restrict__ se, const double* __restrict__ x, const double* __restrict__ p,
-                           const unsigned long long* __restrict__ abits, Box* __restrict__ out) {
+                           const unsigned long long* __restrict__ abits, const int* __restrict__ vs, Box* __restrict__ out) {
     const int e = blockIdx.x * blockDim.x + threadIdx.x;
     if (e >= E) return;
     const int2 v = se[e];
     if (abits == nullptr) { out[e] = box_edge(ld3(x, v.x), ld3(x, v.y)); return; }
-    const double a = step_of(abits);
+    const double a = step_of(vs ? abits + kSlots * vs[v.x] : abits);
     out[e] = box_union(swept_pt(ld3(x, v.x), ld3(p, v.x), a), swept_pt(ld3(x, v.y), ld3(p, v.y), a));
 }
 __global__ void point_boxes(int Vs, const int* __restrict__ sv, const double* __restrict__ x, const double* __restrict__ p,
-                            const unsigned long long* __restrict__ abits, Box* __restrict__ out) {
+                            const unsigned long long* __restrict__ abits, const int* __restrict__ vs, Box* __restrict__ out) {
     const int s = blockIdx.x * blockDim.x + threadIdx.x;
     if (s >= Vs) return;
-    out[s] = swept_pt(ld3(x, sv[s]), ld3(p, sv[s]), step_of(abits));
+    const int v = sv[s];
+    out[s] = swept_pt(ld3(x, v), ld3(p, v), step_of(vs ? abits + kSlots * vs[v] : abits));
 }
 
 void build_static_trees(Context& c) {
@@ -337,10 +339,10 @@ void build_static_trees(Context& c) {
     c.edge_tree.prim.resize(std::max(s.Es, 1));
     const unsigned long long* none = nullptr;
     if (s.F > 0)
-        hipLaunchKernelGGL(face_boxes, dim3(blocks_for(s.F, 256)), dim3(256), 0, c.stream, s.F, s.faces.p, s.x.p, s.x.p, none,
+        hipLaunchKernelGGL(face_boxes, dim3(blocks_for(s.F, 256)), dim3(256), 0, c.stream, s.F, s.faces.p, s.x.p, s.x.p, none, nullptr,
                            c.face_tree.prim.p);
     if (s.Es > 0)
-        hipLaunchKernelGGL(edge_boxes, dim3(blocks_for(s.Es, 256)), dim3(256), 0, c.stream, s.Es, s.se.p, s.x.p, s.x.p, none,
+        hipLaunchKernelGGL(edge_boxes, dim3(blocks_for(s.Es, 256)), dim3(256), 0, c.stream, s.Es, s.se.p, s.x.p, s.x.p, none, nullptr,
                            c.edge_tree.prim.p);
     CIPC_CHECK_LAUNCH();
     build_tree(c, c.face_tree, s.F);
@@ -353,13 +355,13 @@ void build_swept_trees(Context& c, const unsigned long long* abits) {
     c.edge_tree.prim.resize(std::max(s.Es, 1));
     c.swept_point_box.resize(std::max(s.Vs, 1));
     if (s.F > 0)
-        hipLaunchKernelGGL(face_boxes, dim3(blocks_for(s.F, 256)), dim3(256), 0, c.stream, s.F, s.faces.p, s.x.p, s.dir.p, abits,
+        hipLaunchKernelGGL(face_boxes, dim3(blocks_for(s.F, 256)), dim3(256), 0, c.stream, s.F, s.faces.p, s.x.p, s.dir.p, abits, scene_of_vertex(s),
                            c.face_tree.prim.p);
     if (s.Es > 0)
-        hipLaunchKernelGGL(edge_boxes, dim3(blocks_for(s.Es, 256)), dim3(256), 0, c.stream, s.Es, s.se.p, s.x.p, s.dir.p, abits,
+        hipLaunchKernelGGL(edge_boxes, dim3(blocks_for(s.Es, 256)), dim3(256), 0, c.stream, s.Es, s.se.p, s.x.p, s.dir.p, abits, scene_of_vertex(s),
                            c.edge_tree.prim.p);
     if (s.Vs > 0)
-        hipLaunchKernelGGL(point_boxes, dim3(blocks_for(s.Vs, 256)), dim3(256), 0, c.stream, s.Vs, s.sv.p, s.x.p, s.dir.p, abits,
+        hipLaunchKernelGGL(point_boxes, dim3(blocks_for(s.Vs, 256)), dim3(256), 0, c.stream, s.Vs, s.sv.p, s.x.p, s.dir.p, abits, scene_of_vertex(s),
                            c.swept_point_box.p);
     CIPC_CHECK_LAUNCH();
     build_tree(c, c.face_tree, s.F);
@@ -439,7 +441,7 @@ __global__ __launch_bounds__(64) void pt_compact(int Vs, const int* __restrict__
 __global__ void pt_classify(int n, int F, const int* __restrict__ own, const int* __restrict__ off, const int* __restrict__ sv,
                             const double* __restrict__ x, const int* __restrict__ bt, const int4* __restrict__ faces,
                             const int* __restrict__ candU, int* __restrict__ cand, double hhat,
-                            double sq, int4* __restrict__ rec, int* __restrict__ cat) {
+                            double sq, const int* __restrict__ vs, int4* __restrict__ rec, int* __restrict__ cat) {
     const int u = blockIdx.x * blockDim.x + threadIdx.x;
     if (u >= n) return;
     // sorted position: the hit's rank among its query's hits (all distinct)
@@ -456,7 +458,7 @@ __global__ void pt_classify(int n, int F, const int* __restrict__ own, const int
     const int vI = sv[q];
     const int4 fv = faces[f];
     if ((is_collider(bt[vI]) && is_collider(bt[fv.x]) && is_collider(bt[fv.y]) && is_collider(bt[fv.z])) || vI == fv.x ||
-        vI == fv.y || vI == fv.z) {
+        vI == fv.y || vI == fv.z || (vs && vs[vI] != vs[fv.x])) {   // (scenes of a batch never touch)
         rec[k] = make_int4(0, 0, 0, 0);
         cat[k] = kDrop;
         return;
@@ -539,7 +541,8 @@ __global__ __launch_bounds__(64) void ee_compact(int Es, const int* __restrict__
 __global__ void ee_classify(int n, int Es, const int* __restrict__ own, const int* __restrict__ off, const int2* __restrict__ se,
                             const double* __restrict__ x, const double* __restrict__ vrest, const int* __restrict__ bt,
                             const int* __restrict__ candU, int* __restrict__ cand, double hhat,
-                            double sq, int4* __restrict__ rec, int2* __restrict__ aux, int* __restrict__ cat) {
+                            double sq, const int* __restrict__ vs, int4* __restrict__ rec, int2* __restrict__ aux,
+                            int* __restrict__ cat) {
     const int u = blockIdx.x * blockDim.x + threadIdx.x;
     if (u >= n) return;
     const int eI = own[u], o = off[eI], e = off[eI + 1];
@@ -555,7 +558,7 @@ __global__ void ee_classify(int n, int Es, const int* __restrict__ own, const in
     }
     const int2 A = se[eI], B = se[eJ];
     if ((is_collider(bt[A.x]) && is_collider(bt[A.y]) && is_collider(bt[B.x]) && is_collider(bt[B.y])) || A.x == B.x ||
-        A.x == B.y || A.y == B.x || A.y == B.y) {
+        A.x == B.y || A.y == B.x || A.y == B.y || (vs && vs[A.x] != vs[B.x])) {
         rec[k] = make_int4(0, 0, 0, 0);
         aux[k] = make_int2(-1, -1);
         cat[k] = kDrop;
@@ -737,14 +740,15 @@ void compute_active_set(Context& c) {
                            b.cntpPT.p, b.bufPT.p, b.candUPT.p, b.ownPT.p);
         if (nPT > 0)
             hipLaunchKernelGGL(pt_classify, dim3(blocks_for(nPT, 128)), dim3(128), 0, c.stream, nPT, s.F, b.ownPT.p, b.offPT.p, s.sv.p, s.x.p,
-                               s.bt.p, s.faces.p, b.candUPT.p, b.candPT.p, s.hhat, sq, b.recPT.p, b.catPT.p);
+                               s.bt.p, s.faces.p, b.candUPT.p, b.candPT.p, s.hhat, sq, scene_of_vertex(s), b.recPT.p, b.catPT.p);
     }
     if (Es > 0) {
         hipLaunchKernelGGL(ee_compact, dim3(blocks_for(Es, 64)), dim3(64), 0, c.stream, Es, s.se_perm.p, s.se.p, s.x.p, et, sq, b.offEE.p,
                            b.cntpEE.p, b.bufEE.p, b.candUEE.p, b.ownEE.p);
         if (nEE > 0)
             hipLaunchKernelGGL(ee_classify, dim3(blocks_for(nEE, 128)), dim3(128), 0, c.stream, nEE, Es, b.ownEE.p, b.offEE.p, s.se.p,
-                               s.x.p, s.vrest.p, s.bt.p, b.candUEE.p, b.candEE.p, s.hhat, sq, b.recEE.p, b.auxEE.p, b.catEE.p);
+                               s.x.p, s.vrest.p, s.bt.p, b.candUEE.p, b.candEE.p, s.hhat, sq, scene_of_vertex(s), b.recEE.p, b.auxEE.p,
+                               b.catEE.p);
     }
     CIPC_CHECK_LAUNCH();
     // category positions
@@ -851,8 +855,9 @@ __device__ inline bool seg_tri(d3 e0, d3 e1, d3 t0, d3 t1, d3 t2) {
     }
     return xs[0] >= 0.0 && xs[1] >= 0.0 && xs[0] + xs[1] <= 1.0 && xs[2] >= 0.0 && xs[2] <= 1.0;
 }
+// hit[0] (one scene) or hit[scene] (batch) is set for a face that an edge crosses
 __global__ void intersect_kernel(int F, const int4* __restrict__ faces, const int2* __restrict__ se, const double* __restrict__ x,
-                                 const int* __restrict__ bt, TreeView et, int* __restrict__ hit) {
+                                 const int* __restrict__ bt, TreeView et, const int* __restrict__ vs, int* __restrict__ hit) {
     const int f = blockIdx.x * blockDim.x + threadIdx.x;
     if (f >= F) return;
     const int4 fv = faces[f];
@@ -863,39 +868,51 @@ __global__ void intersect_kernel(int F, const int4* __restrict__ faces, const in
         if (any) return;
         const int2 E = se[e];
         if (E.x == fv.x || E.x == fv.y || E.x == fv.z || E.y == fv.x || E.y == fv.y || E.y == fv.z) return;
+        if (vs && vs[E.x] != vs[fv.x]) return;
         if (is_collider(bt[E.x]) && is_collider(bt[E.y]) && is_collider(bt[fv.x]) && is_collider(bt[fv.y]) && is_collider(bt[fv.z])) return;
         if (seg_tri(ld3(x, E.x), ld3(x, E.y), t0, t1, t2)) any = true;
     });
-    if (any) atomicOr(hit, 1);
+    if (any) atomicOr(hit + (vs ? vs[fv.x] : 0), 1);
 }
-__global__ void ground_touch_kernel(int n, const int* __restrict__ env, const double* __restrict__ x, d3 gn, double gd, int* __restrict__ hit) {
+__global__ void ground_touch_kernel(int n, const int* __restrict__ env, const double* __restrict__ x, d3 gn, double gd,
+                                    const int* __restrict__ vs, int* __restrict__ hit) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
-    const double dist = dot(gn, ld3(x, env[i])) - gd;
-    if (dist * dist <= 0.0) atomicOr(hit, 1);
+    const int v = env[i];
+    const double dist = dot(gn, ld3(x, v)) - gd;
+    if (dist * dist <= 0.0) atomicOr(hit + (vs ? vs[v] : 0), 1);
 }
 __global__ void zero_int(int* p, int n) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) p[i] = 0;
 }
 
-bool any_intersection(Context& c) {
+void scene_intersections(Context& c, int* out) {
     Scene& s = c.sc;
-    c.counters.resize(4);
-    hipLaunchKernelGGL(zero_int, dim3(1), dim3(64), 0, c.stream, c.counters.p, 2);
+    const int S = s.S;
+    c.counters.resize(2 * (size_t)S);
+    hipLaunchKernelGGL(zero_int, dim3(blocks_for(2 * S, 64)), dim3(64), 0, c.stream, c.counters.p, 2 * S);
     if (c.n_env > 0)
         hipLaunchKernelGGL(ground_touch_kernel, dim3(blocks_for(c.n_env, 256)), dim3(256), 0, c.stream, (int)c.n_env, c.env_set.p,
-                           s.x.p, s.gn, s.gd, c.counters.p);
+                           s.x.p, s.gn, s.gd, scene_of_vertex(s), c.counters.p);
     build_static_trees(c);
     const TreeView et{c.edge_tree.n, c.edge_tree.nodes.p, c.edge_tree.wide.p};
     if (s.F > 0)
         hipLaunchKernelGGL(intersect_kernel, dim3(blocks_for(s.F, 128)), dim3(128), 0, c.stream, s.F, s.faces.p, s.se.p, s.x.p, s.bt.p,
-                           et, c.counters.p + 1);
+                           et, scene_of_vertex(s), c.counters.p + S);
     CIPC_CHECK_LAUNCH();
-    int h[2];
-    CIPC_HIP(hipMemcpyAsync(h, c.counters.p, 2 * sizeof(int), hipMemcpyDeviceToHost, c.stream));
+    std::vector<int> h(2 * (size_t)S);
+    CIPC_HIP(hipMemcpyAsync(h.data(), c.counters.p, 2 * (size_t)S * sizeof(int), hipMemcpyDeviceToHost, c.stream));
     CIPC_HIP(hipStreamSynchronize(c.stream));
-    return h[0] != 0 || h[1] != 0;
+    for (int k = 0; k < S; ++k) out[k] = (h[k] != 0 || h[S + k] != 0) ? 1 : 0;
+}
+
+bool any_intersection(Context& c) {
+    std::vector<int> r(c.sc.S);
+    scene_intersections(c, r.data());
+    for (int v : r)
+        if (v) return true;
+    return false;
 }
 
 }  // namespace cipc

@@ -27,78 +27,123 @@ __device__ inline void atomic_min_nonneg(unsigned long long* a, double v) { atom
 __device__ inline void atomic_max_nonneg(unsigned long long* a, double v) { atomicMax(a, ord_bits(v)); }
 __device__ inline double bits_to_double(unsigned long long b) { return __longlong_as_double((long long)b); }
 
-// slots: 0 ground min, 1 CFL max |p|, 2 partial CCD min, 3 full CCD min, 4 the point-triangle
-// phase's result read by the edge-edge phase, 5 the swept trees' step
-__global__ void init_bounds(unsigned long long* s) {
+// slots (kSlots per scene, scene k at k * kSlots): 0 ground min, 1 CFL max |p|, 2 partial
+// CCD min, 3 full CCD min, 4 the point-triangle phase's result read by the edge-edge phase,
+// 5 the swept trees' step
+__global__ void init_bounds(int S, unsigned long long* sl) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= S) return;
+    unsigned long long* s = sl + kSlots * k;
     s[0] = ord_bits(1.0);
     s[1] = ord_bits(0.0);
     s[2] = ord_bits(1.0);
     s[3] = ord_bits(1.0);
 }
-
-__global__ void ground_cfl_kernel(int Vs, const int* __restrict__ sv, const double* __restrict__ x, const double* __restrict__ p,
-                                  d3 gn, double gd, double slack, unsigned long long* __restrict__ slots) {
-    const int s = blockIdx.x * blockDim.x + threadIdx.x;
-    if (s >= Vs) return;
-    const int v = sv[s];
-    const d3 pv = ld3(p, v);
-    double best = 1.0;
-    const double nm = dot(gn, -pv);
-    if (!(nm >= 0.0)) {
-        const double dist = dot(gn, ld3(x, v)) - gd;
-        const double cand = -dist / nm * slack;
-        if (cand >= 0.0) best = cand;
-    }
-    atomic_min_nonneg(&slots[0], best);
-    atomic_max_nonneg(&slots[1], sqrt(sqn(pv)));
-}
-
-__global__ void partial_ccd_kernel(int n, const int2* __restrict__ pairs, const int* __restrict__ sv, const int4* __restrict__ faces,
-                                   const int2* __restrict__ se, const double* __restrict__ x, const double* __restrict__ p, double eta,
-                                   unsigned long long* __restrict__ slot) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    const int2 pr = pairs[i];
-    double a;
-    if (pr.x < 0) {
-        const int pt = sv[-pr.x - 1];
-        const int4 f = faces[pr.y];
-        a = accd_pt(ld3(x, pt), ld3(x, f.x), ld3(x, f.y), ld3(x, f.z), -ld3(p, pt), -ld3(p, f.x), -ld3(p, f.y), -ld3(p, f.z), eta, 0.0);
-    } else {
-        const int2 e1 = se[pr.x], e2 = se[pr.y];
-        a = accd_ee(ld3(x, e1.x), ld3(x, e1.y), ld3(x, e2.x), ld3(x, e2.y), -ld3(p, e1.x), -ld3(p, e1.y), -ld3(p, e2.x), -ld3(p, e2.y),
-                    eta, 0.0);
-    }
-    atomic_min_nonneg(slot, a);
-}
-
-// Full CCD, point-triangle phase (ContactMechanics.cpp:3764-3816): each surface vertex is
-// split over kParts lanes, one subtree of the swept face tree each; the step is the min
-// over every lane, reduced across the wavefront before one atomic.
+__device__ inline int scene_of(const int* vs, int v) { return vs ? vs[v] : 0; }
 __device__ inline double wave_min(double v) {
 #pragma unroll
     for (int m = 1; m < 64; m <<= 1) v = fmin(v, __shfl_xor(v, m));
     return v;
 }
+__device__ inline double wave_max(double v) {
+#pragma unroll
+    for (int m = 1; m < 64; m <<= 1) v = fmax(v, __shfl_xor(v, m));
+    return v;
+}
+constexpr int kNoScene = -1;   // a lane without a value (past the end of its range)
+// min (Max: max) of v into slot k of the lane's scene.  One atomic per wavefront when all of
+// its lanes with a value share the scene (always for one scene; the usual case in a batch,
+// whose primitives are ordered by scene), else one per lane.  Every lane of the wavefront
+// calls it; a lane without a value passes kNoScene and the neutral value.
+template <bool Max>
+__device__ inline void scene_atomic(unsigned long long* sl, int scene, int k, double v) {
+    const unsigned long long has = __ballot(scene != kNoScene);
+    if (!has) return;
+    const int s0 = __shfl(scene, __ffsll((long long)has) - 1);
+    if (__all(scene == s0 || scene == kNoScene)) {
+        const double w = Max ? wave_max(v) : wave_min(v);
+        if ((threadIdx.x & 63) == (unsigned)(__ffsll((long long)has) - 1)) {
+            if (Max) atomic_max_nonneg(sl + kSlots * s0 + k, w);
+            else atomic_min_nonneg(sl + kSlots * s0 + k, w);
+        }
+    } else if (scene != kNoScene) {
+        if (Max) atomic_max_nonneg(sl + kSlots * scene + k, v);
+        else atomic_min_nonneg(sl + kSlots * scene + k, v);
+    }
+}
+
+__global__ void ground_cfl_kernel(int Vs, const int* __restrict__ sv, const double* __restrict__ x, const double* __restrict__ p,
+                                  d3 gn, double gd, double slack, const int* __restrict__ vs, unsigned long long* __restrict__ sl) {
+    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+    double best = 1.0, move = 0.0;
+    int scene = kNoScene;
+    if (s < Vs) {
+        const int v = sv[s];
+        scene = scene_of(vs, v);
+        const d3 pv = ld3(p, v);
+        const double nm = dot(gn, -pv);
+        if (!(nm >= 0.0)) {
+            const double dist = dot(gn, ld3(x, v)) - gd;
+            const double cand = -dist / nm * slack;
+            if (cand >= 0.0) best = cand;
+        }
+        move = sqrt(sqn(pv));
+    }
+    scene_atomic<false>(sl, scene, 0, best);
+    scene_atomic<true>(sl, scene, 1, move);
+}
+
+__global__ void partial_ccd_kernel(int n, const int2* __restrict__ pairs, const int* __restrict__ sv, const int4* __restrict__ faces,
+                                   const int2* __restrict__ se, const double* __restrict__ x, const double* __restrict__ p, double eta,
+                                   const int* __restrict__ vs, unsigned long long* __restrict__ sl) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    double a = 1.0;
+    int scene = kNoScene;
+    if (i < n) {
+        const int2 pr = pairs[i];
+        int v0;
+        if (pr.x < 0) {
+            const int pt = sv[-pr.x - 1];
+            v0 = pt;
+            const int4 f = faces[pr.y];
+            a = accd_pt(ld3(x, pt), ld3(x, f.x), ld3(x, f.y), ld3(x, f.z), -ld3(p, pt), -ld3(p, f.x), -ld3(p, f.y), -ld3(p, f.z), eta,
+                        0.0);
+        } else {
+            const int2 e1 = se[pr.x], e2 = se[pr.y];
+            v0 = e1.x;
+            a = accd_ee(ld3(x, e1.x), ld3(x, e1.y), ld3(x, e2.x), ld3(x, e2.y), -ld3(p, e1.x), -ld3(p, e1.y), -ld3(p, e2.x),
+                        -ld3(p, e2.y), eta, 0.0);
+        }
+        scene = scene_of(vs, v0);
+    }
+    scene_atomic<false>(sl, scene, 2, a);
+}
+
+// Full CCD, point-triangle phase (ContactMechanics.cpp:3764-3816): each surface vertex is
+// split over kParts lanes, one subtree of the swept face tree each; the step is the min
+// over every lane, reduced across the wavefront before one atomic.
 __global__ __launch_bounds__(64) void full_ccd_pt_kernel(int Vs, const int* __restrict__ perm, const int* __restrict__ sv,
                                                           const int4* __restrict__ faces, const int* __restrict__ bt,
                                                           const double* __restrict__ x, const double* __restrict__ p,
                                                           const Box* __restrict__ qbox, TreeView tv,
-                                                          const unsigned long long* __restrict__ aslot, double eta,
-                                                          unsigned long long* __restrict__ slot) {
+                                                          unsigned long long* __restrict__ sl, double eta,
+                                                          const int* __restrict__ vs) {
     __shared__ int stk[kStack * 64];
     const int t = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
     double best = 1.0;
+    int scene = kNoScene;
     if (t < Vs * kParts) {
         const int s = perm[t / kParts], part = t % kParts;
-        const double a = bits_to_double(*aslot);
         const int vI = sv[s];
+        scene = scene_of(vs, vI);
+        const double a = bits_to_double(sl[kSlots * scene + 5]);
         const d3 X = ld3(x, vI), P = ld3(p, vI);
         const Box pb = swept_pt(X, P, a);
         part_walk(tv, part, qbox[s], 0.0, stk + threadIdx.x, [&](int f) {
             const int4 fv = faces[f];
             if (is_collider(bt[vI]) && is_collider(bt[fv.x]) && is_collider(bt[fv.y]) && is_collider(bt[fv.z])) return;
             if (vI == fv.x || vI == fv.y || vI == fv.z) return;
+            if (vs && vs[fv.x] != scene) return;          // scenes of a batch never touch
             const d3 x0 = ld3(x, fv.x), x1 = ld3(x, fv.y), x2 = ld3(x, fv.z);
             const d3 p0 = ld3(p, fv.x), p1 = ld3(p, fv.y), p2 = ld3(p, fv.z);
             const Box fb = box_union(box_union(swept_pt(x0, p0, a), swept_pt(x1, p1, a)), swept_pt(x2, p2, a));
@@ -107,8 +152,7 @@ __global__ __launch_bounds__(64) void full_ccd_pt_kernel(int Vs, const int* __re
             best = fmin(la, best);
         }, [] {});
     }
-    best = wave_min(best);
-    if (threadIdx.x == 0) atomic_min_nonneg(slot, best);
+    scene_atomic<false>(sl, scene, 3, best);
 }
 
 // Full CCD, edge-edge phase: each surface edge eJ (split over kParts lanes), candidates
@@ -117,15 +161,16 @@ __global__ __launch_bounds__(64) void full_ccd_pt_kernel(int Vs, const int* __re
 __global__ __launch_bounds__(64) void full_ccd_ee_kernel(int Es, const int* __restrict__ perm, const int2* __restrict__ se,
                                                           const int* __restrict__ bt, const double* __restrict__ x,
                                                           const double* __restrict__ p, TreeView tv, const Box* __restrict__ ebox,
-                                                          const unsigned long long* __restrict__ aslot, double eta,
-                                                          unsigned long long* __restrict__ slot) {
+                                                          unsigned long long* __restrict__ sl, double eta, const int* __restrict__ vs) {
     __shared__ int stk[kStack * 64];
     const int t = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
     double best = 1.0;
+    int scene = kNoScene;
     if (t < Es * kParts) {
         const int eJ = perm[t / kParts], part = t % kParts;
-        const double a = bits_to_double(*aslot);
         const int2 J = se[eJ];
+        scene = scene_of(vs, J.x);
+        const double a = bits_to_double(sl[kSlots * scene + 4]);
         const d3 j0 = ld3(x, J.x), j1 = ld3(x, J.y), pj0 = ld3(p, J.x), pj1 = ld3(p, J.y);
         const Box b1 = box_union(swept_pt(j0, pj0, a), swept_pt(j1, pj1, a));
         // query box: eJ's leaf box of the tree (swept with the incoming step)
@@ -135,6 +180,7 @@ __global__ __launch_bounds__(64) void full_ccd_ee_kernel(int Es, const int* __re
             const int2 I = se[eI];
             if (is_collider(bt[I.x]) && is_collider(bt[I.y]) && is_collider(bt[J.x]) && is_collider(bt[J.y])) return;
             if (I.x == J.x || I.x == J.y || I.y == J.x || I.y == J.y || eI < eJ) return;
+            if (vs && vs[I.x] != scene) return;
             const d3 i0 = ld3(x, I.x), i1 = ld3(x, I.y), pi0 = ld3(p, I.x), pi1 = ld3(p, I.y);
             const Box b2 = box_union(swept_pt(i0, pi0, a), swept_pt(i1, pi1, a));
             if (!overlaps(b1, b2, 0.0)) return;
@@ -142,52 +188,27 @@ __global__ __launch_bounds__(64) void full_ccd_ee_kernel(int Es, const int* __re
             best = fmin(la, best);
         }, [] {});
     }
-    best = wave_min(best);
-    if (threadIdx.x == 0) atomic_min_nonneg(slot, best);
+    scene_atomic<false>(sl, scene, 3, best);
 }
-__global__ void copy_slot(const unsigned long long* src, unsigned long long* dst) { *dst = *src; }
+// slot 4 = slot 3 of every scene: the point-triangle result that the edge-edge phase filters with
+__global__ void copy_pt_result(int S, unsigned long long* sl) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k < S) sl[kSlots * k + 4] = sl[kSlots * k + 3];
+}
 // Step of the swept trees from the ground bound (IPCSolver.cpp:943-958): min(1, ground), or
 // 1 when that is not positive; it seeds the full-CCD minimum (slot 3) and is kept in slot 5.
-__global__ void swept_step(int has_surface, unsigned long long* s) {
+__global__ void swept_step(int S, int has_surface, unsigned long long* sl) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= S) return;
+    unsigned long long* s = sl + kSlots * k;
     double a = has_surface ? fmin(1.0, bits_to_double(s[0])) : 1.0;
     if (a <= 0) a = 1;
     s[3] = ord_bits(a);
     s[5] = ord_bits(a);
 }
 
-void step_bound(Context& c, double out[5]) {
-    Scene& s = c.sc;
-    const double slack = 0.8;
-    const double eta = 1.0 - slack;
-    DBuf<unsigned long long>& sl = c.u64scratch;
-    sl.resize(8);
-    hipLaunchKernelGGL(init_bounds, dim3(1), dim3(1), 0, c.stream, sl.p);
-    if (s.Vs > 0)
-        hipLaunchKernelGGL(ground_cfl_kernel, dim3(blocks_for(s.Vs, 256)), dim3(256), 0, c.stream, s.Vs, s.sv.p, s.x.p, s.dir.p, s.gn,
-                           s.gd, slack, sl.p);
-    if (c.n_ccd > 0)
-        hipLaunchKernelGGL(partial_ccd_kernel, dim3(blocks_for(c.n_ccd, 64)), dim3(64), 0, c.stream, (int)c.n_ccd, c.ccd_set.p, s.sv.p,
-                           s.faces.p, s.se.p, s.x.p, s.dir.p, eta, sl.p + 2);
-    CIPC_CHECK_LAUNCH();
-    // full CCD with the swept trees built for the ground-limited step, all on the device
-    hipLaunchKernelGGL(swept_step, dim3(1), dim3(1), 0, c.stream, s.Vs > 0 ? 1 : 0, sl.p);
-    CIPC_CHECK_LAUNCH();
-    build_swept_trees(c, sl.p + 5);
-    LBVH& ft = c.face_tree;
-    LBVH& et = c.edge_tree;
-    const TreeView ftv{ft.n, ft.nodes.p, ft.wide.p}, etv{et.n, et.nodes.p, et.wide.p};
-    if (s.Vs > 0 && ft.n > 0)
-        hipLaunchKernelGGL(full_ccd_pt_kernel, dim3(blocks_for((size_t)s.Vs * kParts, 64)), dim3(64), 0, c.stream, s.Vs, s.sv_perm.p,
-                           s.sv.p, s.faces.p, s.bt.p, s.x.p, s.dir.p, c.swept_point_box.p, ftv, sl.p + 5, eta, sl.p + 3);
-    if (s.Es > 0 && et.n > 0) {
-        hipLaunchKernelGGL(copy_slot, dim3(1), dim3(1), 0, c.stream, sl.p + 3, sl.p + 4);
-        hipLaunchKernelGGL(full_ccd_ee_kernel, dim3(blocks_for((size_t)s.Es * kParts, 64)), dim3(64), 0, c.stream, s.Es, s.se_perm.p,
-                           s.se.p, s.bt.p, s.x.p, s.dir.p, etv, et.prim.p, sl.p + 4, eta, sl.p + 3);
-    }
-    CIPC_CHECK_LAUNCH();
-    unsigned long long h[4];
-    CIPC_HIP(hipMemcpyAsync(h, sl.p, 4 * sizeof(unsigned long long), hipMemcpyDeviceToHost, c.stream));
-    CIPC_HIP(hipStreamSynchronize(c.stream));
+// The merge of IPCSolver.cpp:938-989 from one scene's slots.
+static void merge_bounds(const Scene& s, const unsigned long long* h, bool any_ccd, double out[5]) {
     double ground = 1.0;
     std::memcpy(&ground, &h[0], sizeof(double));
     const double aground = s.Vs > 0 ? std::min(1.0, ground) : 1.0;
@@ -196,7 +217,7 @@ void step_bound(Context& c, double out[5]) {
     std::memcpy(&maxmove, &h[1], 8);
     std::memcpy(&partial_min, &h[2], 8);
     std::memcpy(&full, &h[3], 8);
-    const double partial = c.n_ccd > 0 ? std::min(alpha, partial_min) : alpha;
+    const double partial = any_ccd ? std::min(alpha, partial_min) : alpha;
     const double acfl = maxmove > 0.0 ? std::sqrt(s.hhat) / (maxmove * 2.0) : 1.0;
     double merged = std::min(alpha, acfl);
     if (partial > 2.0 * acfl) {
@@ -204,6 +225,49 @@ void step_bound(Context& c, double out[5]) {
         merged = std::max(merged, acfl);
     }
     out[0] = merged; out[1] = aground; out[2] = partial; out[3] = acfl; out[4] = full;
+}
+
+void step_bound_scenes(Context& c, double* out) {
+    Scene& s = c.sc;
+    const int S = s.S;
+    const int* vs = scene_of_vertex(s);
+    const double slack = 0.8;
+    const double eta = 1.0 - slack;
+    DBuf<unsigned long long>& sl = c.u64scratch;
+    sl.resize((size_t)kSlots * S);
+    hipLaunchKernelGGL(init_bounds, dim3(blocks_for(S, 64)), dim3(64), 0, c.stream, S, sl.p);
+    if (s.Vs > 0)
+        hipLaunchKernelGGL(ground_cfl_kernel, dim3(blocks_for(s.Vs, 256)), dim3(256), 0, c.stream, s.Vs, s.sv.p, s.x.p, s.dir.p, s.gn,
+                           s.gd, slack, vs, sl.p);
+    if (c.n_ccd > 0)
+        hipLaunchKernelGGL(partial_ccd_kernel, dim3(blocks_for(c.n_ccd, 64)), dim3(64), 0, c.stream, (int)c.n_ccd, c.ccd_set.p, s.sv.p,
+                           s.faces.p, s.se.p, s.x.p, s.dir.p, eta, vs, sl.p);
+    CIPC_CHECK_LAUNCH();
+    // full CCD with the swept trees built for each scene's ground-limited step, on the device
+    hipLaunchKernelGGL(swept_step, dim3(blocks_for(S, 64)), dim3(64), 0, c.stream, S, s.Vs > 0 ? 1 : 0, sl.p);
+    CIPC_CHECK_LAUNCH();
+    build_swept_trees(c, sl.p + 5);
+    LBVH& ft = c.face_tree;
+    LBVH& et = c.edge_tree;
+    const TreeView ftv{ft.n, ft.nodes.p, ft.wide.p}, etv{et.n, et.nodes.p, et.wide.p};
+    if (s.Vs > 0 && ft.n > 0)
+        hipLaunchKernelGGL(full_ccd_pt_kernel, dim3(blocks_for((size_t)s.Vs * kParts, 64)), dim3(64), 0, c.stream, s.Vs, s.sv_perm.p,
+                           s.sv.p, s.faces.p, s.bt.p, s.x.p, s.dir.p, c.swept_point_box.p, ftv, sl.p, eta, vs);
+    if (s.Es > 0 && et.n > 0) {
+        hipLaunchKernelGGL(copy_pt_result, dim3(blocks_for(S, 64)), dim3(64), 0, c.stream, S, sl.p);
+        hipLaunchKernelGGL(full_ccd_ee_kernel, dim3(blocks_for((size_t)s.Es * kParts, 64)), dim3(64), 0, c.stream, s.Es, s.se_perm.p,
+                           s.se.p, s.bt.p, s.x.p, s.dir.p, etv, et.prim.p, sl.p, eta, vs);
+    }
+    CIPC_CHECK_LAUNCH();
+    std::vector<unsigned long long> h((size_t)kSlots * S);
+    CIPC_HIP(hipMemcpyAsync(h.data(), sl.p, h.size() * sizeof(unsigned long long), hipMemcpyDeviceToHost, c.stream));
+    CIPC_HIP(hipStreamSynchronize(c.stream));
+    for (int k = 0; k < S; ++k) merge_bounds(s, h.data() + (size_t)kSlots * k, c.n_ccd > 0, out + 5 * (size_t)k);
+}
+
+void step_bound(Context& c, double out[5]) {
+    if (c.sc.S != 1) throw Error(CIPC_ERR_LOGIC, "a batch has one step bound per scene");
+    step_bound_scenes(c, out);
 }
 
 }  // namespace cipc

@@ -151,7 +151,37 @@ struct Scene {
     DBuf<double> brest, bweight;  // Nb (hinge form)
     std::vector<int> h_bt;     // host copies for pattern construction
     std::vector<int> h_tets, h_tris, h_bverts;
+    // Batch of independent scenes packed into this one (S > 1): vertices, tets, surface
+    // vertices, edges and faces of scene k follow those of scene k - 1.  Contacts never join
+    // two scenes; stiffness, energies, step bounds and solves are per scene.
+    int S = 1;
+    DBuf<int> vscene;          // V: scene of each vertex (batch only)
+    DBuf<double> tmu, tlam;    // T: per-tet Lame parameters u, r (batch only)
+    DBuf<double> kap;          // S: barrier stiffness of the current call (batch only)
+    std::vector<int> h_voff, h_toff;   // S + 1 vertex / tet offsets
 };
+
+// Barrier stiffness of the scene a vertex belongs to: one value, or the per-scene value of
+// a batch.
+struct Kap {
+    double k = 0.0;
+    const double* ks = nullptr;
+    const int* vs = nullptr;
+    __device__ double at(int v) const { return ks ? ks[vs[v]] : k; }
+};
+// Lame parameters of a tet: one pair, or per tet in a batch.
+struct Lame {
+    double u = 0.0, r = 0.0;
+    const double* tu = nullptr;
+    const double* tr = nullptr;
+    __device__ void at(int t, double& uu, double& rr) const {
+        if (tu) { uu = tu[t]; rr = tr[t]; }
+        else { uu = u; rr = r; }
+    }
+};
+inline Kap scene_kappa(const Scene& s, double k) { return Kap{k, s.S > 1 ? s.kap.p : nullptr, s.S > 1 ? s.vscene.p : nullptr}; }
+inline Lame scene_lame(const Scene& s) { return Lame{s.u, s.r, s.S > 1 ? s.tmu.p : nullptr, s.S > 1 ? s.tlam.p : nullptr}; }
+inline const int* scene_of_vertex(const Scene& s) { return s.S > 1 ? s.vscene.p : nullptr; }
 
 // Static elastic/mass block pattern and gather lists (built once per scene).
 struct StaticPattern {

@@ -60,7 +60,7 @@ __device__ inline void dist_gh(int type, const double* q, double* g, double* H) 
 __global__ __launch_bounds__(64) void contact_local_kernel(
     int n_self, int n_ee, const int4* __restrict__ self_set, const int4* __restrict__ ee_set,
     const int2* __restrict__ ee_ie, const int2* __restrict__ se, const double* __restrict__ x,
-    const double* __restrict__ vrest, double hhat, double kappa, double* __restrict__ Hs, double* __restrict__ Gs,
+    const double* __restrict__ vrest, double hhat, Kap kap, double* __restrict__ Hs, double* __restrict__ Gs,
     int4* __restrict__ verts, int* __restrict__ dims) {
     const int c = blockIdx.x * blockDim.x + threadIdx.x;
     const int n = n_self + n_ee;
@@ -70,6 +70,7 @@ __global__ __launch_bounds__(64) void contact_local_kernel(
         const int4 e = self_set[c];
         int v[4];
         const int type = decode(e, v);
+        const double kappa = kap.at(v[0]);
         const int nv = nv_of(type), N = 3 * nv;
         d3 p[4];
         double q[12];
@@ -91,6 +92,7 @@ __global__ __launch_bounds__(64) void contact_local_kernel(
         const int4 e = ee_set[i];
         int v[4];
         const int type = decode(e, v);
+        const double kappa = kap.at(v[0]);
         const int nvd = nv_of(type);
         int e4[4];
         if (e.w >= 0) { e4[0] = e.x; e4[1] = e.y; e4[2] = e.z; e4[3] = e.w; }
@@ -222,12 +224,13 @@ __global__ void contact_scatter_kernel(int n, const int4* __restrict__ verts, co
 
 // Ground (IPCSolver.cpp:270-293): one block and one gradient record per active vertex.
 __global__ void ground_kernel(int n_env, const int* __restrict__ env, const double* __restrict__ x, const int* __restrict__ bt,
-                              d3 gn, double gd, double hhat, double kappa, int kb,
+                              d3 gn, double gd, double hhat, Kap kap, int kb,
                               unsigned long long* __restrict__ bkey, int* __restrict__ bidx, double* __restrict__ bval,
                               int* __restrict__ gkey, int* __restrict__ gidx, double* __restrict__ gval) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n_env) return;
     const int v = env[i];
+    const double kappa = kap.at(v);
     const double dist = dot(gn, ld3(x, v)) - gd;
     const double d = dist * dist;
     const double g_b = barrier_g(d, hhat), H_b = barrier_h(d, hhat);
@@ -260,7 +263,7 @@ __device__ inline d3 normalized(d3 v) {   // Eigen normalized(): v / |v|, unchan
 }
 // Friction::initialize for one Self_ActiveSet entry (Friction.cpp:88-137): lambda, closest
 // point coordinates (computeClosestPoint_*) and tangent basis (computeTangentBasis_*).
-__global__ void friction_init_self(int n, const int4* __restrict__ set, const double* __restrict__ x, double hhat, double kappa,
+__global__ void friction_init_self(int n, const int4* __restrict__ set, const double* __restrict__ x, double hhat, Kap kap,
                                    int4* __restrict__ fset, double* __restrict__ lam, double* __restrict__ coord,
                                    double* __restrict__ basis) {
     const int c = blockIdx.x * blockDim.x + threadIdx.x;
@@ -268,6 +271,7 @@ __global__ void friction_init_self(int n, const int4* __restrict__ set, const do
     const int4 e = set[c];
     int v[4];
     const int type = decode(e, v);
+    const double kappa = kap.at(v[0]);
     d3 p[4];
     for (int k = 0; k < nv_of(type); ++k) p[k] = ld3(x, v[k]);
     const double d = self_value(type, p);
@@ -304,10 +308,11 @@ __global__ void friction_init_self(int n, const int4* __restrict__ set, const do
     st3(basis + 6 * (size_t)c, 1, B1);
 }
 __global__ void friction_init_ground(int n, const int* __restrict__ env, const double* __restrict__ x, d3 gn, double gd, double hhat,
-                                     double kappa, int* __restrict__ fenv, double* __restrict__ lam) {      // Friction.cpp:141-152
+                                     Kap kap, int* __restrict__ fenv, double* __restrict__ lam) {      // Friction.cpp:141-152
     const int c = blockIdx.x * blockDim.x + threadIdx.x;
     if (c >= n) return;
     const int v = env[c];
+    const double kappa = kap.at(v);
     const double dist = dot(gn, ld3(x, v)) - gd;
     const double d = dist * dist;
     double l = barrier_g(d, hhat);
@@ -478,15 +483,24 @@ __global__ __launch_bounds__(kFB) void e_friction(int ns, int ng, const int4* __
                                                   const double* __restrict__ coord, const double* __restrict__ basis,
                                                   const int* __restrict__ fenv, const double* __restrict__ glam,
                                                   const double* __restrict__ x, const double* __restrict__ xp, d3 gn, double eps2,
-                                                  double mu, double* __restrict__ part) {
-    // part[blockIdx.x] = self partial, part[gridDim.x + blockIdx.x] = ground partial
+                                                  double mu, double* __restrict__ part, double* __restrict__ es,
+                                                  double* __restrict__ eg) {
+    // part[blockIdx.x] = self partial, part[gridDim.x + blockIdx.x] = ground partial; with
+    // es / eg the per-contact values are stored instead
     __shared__ double sh[2][kFB];
     const double eps = sqrt(eps2);
     double as = 0.0, ag = 0.0;
-    for (int c = blockIdx.x * kFB + threadIdx.x; c < ns; c += gridDim.x * kFB)
-        as += friction_self_term(c, fset, lam, coord, basis, x, xp, eps2, eps);
-    for (int c = blockIdx.x * kFB + threadIdx.x; c < ng; c += gridDim.x * kFB)
-        ag += friction_ground_term(c, fenv, glam, x, xp, gn, eps2, eps, mu);
+    for (int c = blockIdx.x * kFB + threadIdx.x; c < ns; c += gridDim.x * kFB) {
+        const double v = friction_self_term(c, fset, lam, coord, basis, x, xp, eps2, eps);
+        if (es) es[c] = v;
+        else as += v;
+    }
+    for (int c = blockIdx.x * kFB + threadIdx.x; c < ng; c += gridDim.x * kFB) {
+        const double v = friction_ground_term(c, fenv, glam, x, xp, gn, eps2, eps, mu);
+        if (eg) eg[c] = v;
+        else ag += v;
+    }
+    if (part == nullptr) return;   // per-contact values only (uniform over the work-group)
     sh[0][threadIdx.x] = as;
     sh[1][threadIdx.x] = ag;
     __syncthreads();
@@ -505,10 +519,10 @@ void friction_update(Context& c, double kappa) {
     c.fr_env.resize(std::max(ne, 1)); c.fr_env_lam.resize(std::max(ne, 1));
     if (ns > 0)
         hipLaunchKernelGGL(friction_init_self, dim3(blocks_for(ns, 128)), dim3(128), 0, c.stream, ns, c.self_set.p, s.x.p, s.hhat,
-                           kappa, c.fr_self.p, c.fr_lam.p, c.fr_coord.p, c.fr_basis.p);
+                           scene_kappa(s, kappa), c.fr_self.p, c.fr_lam.p, c.fr_coord.p, c.fr_basis.p);
     if (ne > 0)
         hipLaunchKernelGGL(friction_init_ground, dim3(blocks_for(ne, 256)), dim3(256), 0, c.stream, ne, c.env_set.p, s.x.p, s.gn, s.gd,
-                           s.hhat, kappa, c.fr_env.p, c.fr_env_lam.p);
+                           s.hhat, scene_kappa(s, kappa), c.fr_env.p, c.fr_env_lam.p);
     CIPC_CHECK_LAUNCH();
     c.n_fself = ns;
     c.n_fenv = ne;
@@ -540,11 +554,11 @@ void friction_records(Context& c, size_t slot_b, size_t slot_g) {
     }
 }
 
-void friction_energy_partials(Context& c, double* part) {
+void friction_energy_partials(Context& c, double* part, double* elem_self, double* elem_ground) {
     Scene& s = c.sc;
     hipLaunchKernelGGL(e_friction, dim3(kFB), dim3(kFB), 0, c.stream, (int)c.n_fself, (int)c.n_fenv, c.fr_self.p, c.fr_lam.p,
                        c.fr_coord.p, c.fr_basis.p, c.fr_env.p, c.fr_env_lam.p, s.x.p, s.vprev.p, s.gn, s.fhat * s.dt * s.dt, s.fric,
-                       part);
+                       part, elem_self, elem_ground);
     CIPC_CHECK_LAUNCH();
 }
 
@@ -560,7 +574,7 @@ void contact_assemble(Context& c, double kappa) {
     c.gval.resize(3 * std::max<size_t>(ng, 1));
     if (ne > 0) {
         hipLaunchKernelGGL(ground_kernel, dim3(blocks_for(ne, 256)), dim3(256), 0, c.stream, ne, c.env_set.p, s.x.p, s.bt.p,
-                           s.gn, s.gd, s.hhat, kappa, s.kb, c.ckey.p, c.cidx.p, c.cval.p, c.gkey.p, c.gidx.p, c.gval.p);
+                           s.gn, s.gd, s.hhat, scene_kappa(s, kappa), s.kb, c.ckey.p, c.cidx.p, c.cval.p, c.gkey.p, c.gidx.p, c.gval.p);
         CIPC_CHECK_LAUNCH();
     }
     if (n > 0) {
@@ -570,7 +584,7 @@ void contact_assemble(Context& c, double kappa) {
         double* Hs = c.chess.p;
         double* Gs = c.chess.p + (size_t)kHU * n;
         hipLaunchKernelGGL(contact_local_kernel, dim3(blocks_for(n, 64)), dim3(64), 0, c.stream, (int)c.n_self, (int)c.n_ee,
-                           c.self_set.p, c.ee_set.p, c.ee_ie.p, s.se.p, s.x.p, s.vrest.p, s.hhat, kappa, Hs, Gs,
+                           c.self_set.p, c.ee_set.p, c.ee_ie.p, s.se.p, s.x.p, s.vrest.p, s.hhat, scene_kappa(s, kappa), Hs, Gs,
                            c.cverts.p, c.cdims.p);
         CIPC_CHECK_LAUNCH();
         const int ns = (int)c.n_self;

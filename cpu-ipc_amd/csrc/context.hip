@@ -96,8 +96,11 @@ int cipc_gpu_context_destroy(cipc_gpu_context* c) {
     });
 }
 
-int cipc_gpu_upload_scene(cipc_gpu_context* c, const cipc_scene_desc* d) {
-    return guard([&] {
+}  // extern "C"
+
+namespace {
+void upload_scene_impl(cipc_gpu_context* c, const cipc_scene_desc* d) {
+    {
         require(c && d, "null context or descriptor");
         require(d->V > 0 && d->T >= 0 && d->F >= 0 && d->Vs >= 0 && d->Es >= 0, "invalid scene counts");
         require(d->x && d->v_rest && d->V_prev && d->xtilde && d->masses && d->boundary, "missing vertex arrays");
@@ -230,6 +233,123 @@ int cipc_gpu_upload_scene(cipc_gpu_context* c, const cipc_scene_desc* d) {
         c->n_soft = 0; c->soft_w = 0.0;
         c->n_cenv = c->n_cself = 0;
         c->assembled = false;
+        s.S = 1;
+        s.h_voff = {0, V};
+        s.h_toff = {0, T};
+    }
+}
+
+// One descriptor holding the scenes of a batch one after another: vertex, tet and face
+// indices shifted by the scene's offsets.  The storage lives in `keep`.
+struct BatchPack {
+    std::vector<double> x, vr, vp, xt, m, dm, vol;
+    std::vector<int> bt, tets, faces;
+    std::vector<uint64_t> sv, se;
+    std::vector<double> tmu, tlam;
+    std::vector<int> vscene, voff, toff;
+};
+void pack_batch(const cipc_scene_desc* const* ds, int S, BatchPack& b, cipc_scene_desc& u) {
+    const cipc_scene_desc& f = *ds[0];
+    u = cipc_scene_desc{};
+    b.voff.assign(1, 0);
+    b.toff.assign(1, 0);
+    for (int k = 0; k < S; ++k) {
+        const cipc_scene_desc* d = ds[k];
+        require(d != nullptr, "null scene descriptor in the batch");
+        require(d->V > 0 && d->T > 0 && d->F >= 0 && d->Vs >= 0 && d->Es >= 0, "invalid scene counts in the batch");
+        require(d->x && d->v_rest && d->V_prev && d->xtilde && d->masses && d->boundary && d->tets && d->dm_inv && d->volumes,
+                "missing arrays in a batch scene");
+        require(d->F == 0 || d->faces, "missing surface faces");
+        require(d->Vs == 0 || d->surf_verts, "missing surface vertices");
+        require(d->Es == 0 || d->surf_edges, "missing surface edges");
+        require(d->Tr == 0 && d->Nb == 0, "batched scenes hold tets only");
+        require(d->dt == f.dt && d->hhat == f.hhat && d->fhat == f.fhat && d->drag == f.drag && d->friction == f.friction &&
+                    d->ground_d == f.ground_d && d->ground_normal[0] == f.ground_normal[0] &&
+                    d->ground_normal[1] == f.ground_normal[1] && d->ground_normal[2] == f.ground_normal[2],
+                "the scenes of a batch share dt, dHat, the friction parameters and the ground");
+        const int V = d->V, T = d->T, vo = b.voff.back(), to = b.toff.back();
+        if ((long long)vo + V > (1ll << 30) || (long long)to + T > (1ll << 30)) throw Error(CIPC_ERR_OVERFLOW, "batch too large");
+        for (int i = 0; i < 4 * T; ++i)
+            if (d->tets[i] < 0 || d->tets[i] >= V) throw Error(CIPC_ERR_INVALID_ARGUMENT, "tet vertex index out of range");
+        for (int i = 0; i < d->F; ++i) {
+            for (int q = 0; q < 3; ++q)
+                if (d->faces[4 * i + q] < 0 || d->faces[4 * i + q] >= V)
+                    throw Error(CIPC_ERR_INVALID_ARGUMENT, "surface face index out of range");
+        }
+        for (int i = 0; i < d->Vs; ++i)
+            if (d->surf_verts[i] >= (uint64_t)V) throw Error(CIPC_ERR_INVALID_ARGUMENT, "surface vertex index out of range");
+        for (int i = 0; i < 2 * d->Es; ++i)
+            if (d->surf_edges[i] >= (uint64_t)V) throw Error(CIPC_ERR_INVALID_ARGUMENT, "surface edge index out of range");
+        b.x.insert(b.x.end(), d->x, d->x + 3 * (size_t)V);
+        b.vr.insert(b.vr.end(), d->v_rest, d->v_rest + 3 * (size_t)V);
+        b.vp.insert(b.vp.end(), d->V_prev, d->V_prev + 3 * (size_t)V);
+        b.xt.insert(b.xt.end(), d->xtilde, d->xtilde + 3 * (size_t)V);
+        b.m.insert(b.m.end(), d->masses, d->masses + V);
+        b.bt.insert(b.bt.end(), d->boundary, d->boundary + V);
+        for (int i = 0; i < 4 * T; ++i) b.tets.push_back(d->tets[i] + vo);
+        b.dm.insert(b.dm.end(), d->dm_inv, d->dm_inv + 9 * (size_t)T);
+        b.vol.insert(b.vol.end(), d->volumes, d->volumes + T);
+        for (int i = 0; i < d->F; ++i) {
+            for (int q = 0; q < 3; ++q) b.faces.push_back(d->faces[4 * i + q] + vo);
+            b.faces.push_back(d->faces[4 * i + 3] + to);
+        }
+        for (int i = 0; i < d->Vs; ++i) b.sv.push_back(d->surf_verts[i] + vo);
+        for (int i = 0; i < 2 * d->Es; ++i) b.se.push_back(d->surf_edges[i] + vo);
+        b.tmu.insert(b.tmu.end(), (size_t)T, d->length_rate);
+        b.tlam.insert(b.tlam.end(), (size_t)T, d->volume_rate);
+        b.vscene.insert(b.vscene.end(), (size_t)V, k);
+        b.voff.push_back(vo + V);
+        b.toff.push_back(to + T);
+    }
+    u.V = b.voff.back(); u.T = b.toff.back(); u.F = (int)(b.faces.size() / 4); u.Vs = (int)b.sv.size(); u.Es = (int)(b.se.size() / 2);
+    u.x = b.x.data(); u.v_rest = b.vr.data(); u.V_prev = b.vp.data(); u.xtilde = b.xt.data(); u.masses = b.m.data();
+    u.boundary = b.bt.data(); u.tets = b.tets.data(); u.dm_inv = b.dm.data(); u.volumes = b.vol.data();
+    u.faces = b.faces.data(); u.surf_verts = b.sv.data(); u.surf_edges = b.se.data();
+    u.dt = f.dt; u.length_rate = f.length_rate; u.volume_rate = f.volume_rate; u.drag = f.drag; u.hhat = f.hhat; u.fhat = f.fhat;
+    u.friction = f.friction;
+    for (int q = 0; q < 3; ++q) u.ground_normal[q] = f.ground_normal[q];
+    u.ground_d = f.ground_d;
+}
+
+void require_batch(cipc_gpu_context* c) { require(c && c->have_scene, "no scene uploaded"); }
+}  // namespace
+
+extern "C" {
+
+int cipc_gpu_upload_scene(cipc_gpu_context* c, const cipc_scene_desc* d) {
+    return guard([&] { upload_scene_impl(c, d); });
+}
+
+int cipc_gpu_upload_batch(cipc_gpu_context* c, const cipc_scene_desc* const* scenes, int n_scenes) {
+    return guard([&] {
+        require(c && scenes && n_scenes > 0, "null context or descriptors, or no scenes");
+        BatchPack b;
+        cipc_scene_desc u;
+        pack_batch(scenes, n_scenes, b, u);
+        upload_scene_impl(c, &u);
+        Scene& s = c->sc;
+        const hipStream_t st = c->stream;
+        s.S = n_scenes;
+        s.h_voff = b.voff;
+        s.h_toff = b.toff;
+        upload(s.vscene, b.vscene.data(), b.vscene.size(), st);
+        upload(s.tmu, b.tmu.data(), b.tmu.size(), st);
+        upload(s.tlam, b.tlam.data(), b.tlam.size(), st);
+        s.kap.resize(n_scenes);
+        std::vector<int> off(b.toff);
+        off.insert(off.end(), b.voff.begin(), b.voff.end());
+        upload(c->batch_off, off.data(), off.size(), st);
+        CIPC_HIP(hipStreamSynchronize(st));
+    });
+}
+
+int cipc_gpu_batch_size(cipc_gpu_context* c, int* n_scenes, int* vertex_offsets, int* tet_offsets) {
+    return guard([&] {
+        require_batch(c);
+        const Scene& s = c->sc;
+        if (n_scenes) *n_scenes = s.S;
+        if (vertex_offsets) std::copy(s.h_voff.begin(), s.h_voff.end(), vertex_offsets);
+        if (tet_offsets) std::copy(s.h_toff.begin(), s.h_toff.end(), tet_offsets);
     });
 }
 
@@ -342,7 +462,23 @@ static void elastic_fork(cipc_gpu_context* c, hipEvent_t t0 = nullptr, hipEvent_
 }
 static void elastic_join(cipc_gpu_context* c) { CIPC_HIP(hipStreamWaitEvent(c->stream, c->join, 0)); }
 
-static void assemble_impl(cipc_gpu_context* c, double kappa) {
+// Barrier stiffness per scene of a batch (ks, S values; NULL: k for every scene).
+static void set_scene_kappa(cipc_gpu_context* c, const double* ks, double k) {
+    Scene& s = c->sc;
+    if (s.S == 1) return;
+    std::vector<double> h(s.S, k);
+    if (ks)
+        for (int i = 0; i < s.S; ++i) h[i] = ks[i];
+    for (double v : h) require(std::isfinite(v) && v >= 0.0, "barrier stiffness must be finite and non-negative");
+    CIPC_HIP(hipMemcpyAsync(s.kap.p, h.data(), s.S * sizeof(double), hipMemcpyHostToDevice, c->stream));
+    CIPC_HIP(hipStreamSynchronize(c->stream));
+}
+static void require_single(cipc_gpu_context* c, const char* what) {
+    if (c->sc.S != 1) throw Error(CIPC_ERR_LOGIC, std::string(what) + ": a batch context takes the per-scene entry points");
+}
+
+static void assemble_impl(cipc_gpu_context* c, double kappa, const double* ks = nullptr) {
+    set_scene_kappa(c, ks, kappa);
     elastic_fork(c);
     contact_assemble(*c, kappa);
     elastic_join(c);
@@ -459,6 +595,7 @@ int cipc_gpu_preconditioner_train_step(cipc_gpu_context* c, int objective, float
 int cipc_gpu_kappa_gradients(cipc_gpu_context* c, double out[2], int64_t* n_constraints) {
     return guard([&] {
         require(c && c->have_scene && out, "no scene uploaded or null output");
+        require_single(c, "cipc_gpu_kappa_gradients");
         bind(*c);
         const int64_t n = kappa_gradients(*c, out);
         if (n_constraints) *n_constraints = n;
@@ -468,6 +605,7 @@ int cipc_gpu_kappa_gradients(cipc_gpu_context* c, double out[2], int64_t* n_cons
 int cipc_gpu_close_constraints(cipc_gpu_context* c, double dtol, int* closer, int64_t counts[2]) {
     return guard([&] {
         require(c && c->have_scene && closer, "no scene uploaded or null output");
+        require_single(c, "cipc_gpu_close_constraints");
         bind(*c);
         int64_t cnt[2];
         *closer = close_constraints(*c, dtol, cnt);
@@ -541,10 +679,189 @@ int cipc_gpu_solve_ex(cipc_gpu_context* c, int precision, int preconditioner, do
     });
 }
 
+// ---- batched scenes -------------------------------------------------------------------
+int cipc_gpu_batch_assemble(cipc_gpu_context* c, const double* kappa, int64_t* nnz, int* collisions) {
+    return guard([&] {
+        require_batch(c);
+        require(kappa != nullptr, "null stiffness array");
+        bind(*c);
+        assemble_impl(c, kappa[0], kappa);
+        if (nnz) *nnz = c->nnz;
+        if (collisions) {
+            // per scene, counted as compute_g_dpt returns it (scene of the entry's point / first edge vertex)
+            const Scene& s = c->sc;
+            std::fill(collisions, collisions + s.S, 0);
+            if (c->n_self) {
+                std::vector<int4> h(c->n_self);
+                CIPC_HIP(hipMemcpyAsync(h.data(), c->self_set.p, 16 * c->n_self, hipMemcpyDeviceToHost, c->stream));
+                CIPC_HIP(hipStreamSynchronize(c->stream));
+                for (const int4& e : h) {
+                    const int v = e.x >= 0 ? e.x : -e.x - 1;
+                    const int k = (int)(std::upper_bound(s.h_voff.begin(), s.h_voff.end(), v) - s.h_voff.begin()) - 1;
+                    collisions[k] += (e.x < 0 && (e.z < 0 || e.w < 0)) ? -e.w : 1;
+                }
+            }
+        }
+    });
+}
+
+int cipc_gpu_batch_friction_update(cipc_gpu_context* c, const double* kappa) {
+    return guard([&] {
+        require_batch(c);
+        require(kappa != nullptr, "null stiffness array");
+        bind(*c);
+        set_scene_kappa(c, kappa, kappa[0]);
+        friction_update(*c, kappa[0]);
+        CIPC_HIP(hipStreamSynchronize(c->stream));
+    });
+}
+
+int cipc_gpu_batch_energy(cipc_gpu_context* c, const double* kappa, double* energy, double* terms) {
+    return guard([&] {
+        require_batch(c);
+        require(kappa && (energy || terms), "null stiffness array or outputs");
+        bind(*c);
+        const int S = c->sc.S;
+        std::vector<double> t(9 * (size_t)S);
+        energy_terms_scenes(*c, t.data());
+        for (int k = 0; k < S; ++k) {
+            if (energy) energy[k] = total_energy(*c, kappa[k], t.data() + 9 * (size_t)k);
+            if (terms) std::copy(t.begin() + 9 * (size_t)k, t.begin() + 9 * (size_t)k + 9, terms + 9 * (size_t)k);
+        }
+    });
+}
+
+int cipc_gpu_batch_step_bound(cipc_gpu_context* c, double* out) {
+    return guard([&] {
+        require_batch(c);
+        require(out != nullptr, "null output");
+        bind(*c);
+        step_bound_scenes(*c, out);
+    });
+}
+
+int cipc_gpu_batch_step_forward(cipc_gpu_context* c, const double* alpha) {
+    return guard([&] {
+        require_batch(c);
+        require(alpha != nullptr, "null step array");
+        for (int k = 0; k < c->sc.S; ++k) require(std::isfinite(alpha[k]), "steps must be finite");
+        bind(*c);
+        step_forward_scenes(*c, alpha);
+        CIPC_HIP(hipStreamSynchronize(c->stream));
+    });
+}
+
+int cipc_gpu_batch_is_intersected(cipc_gpu_context* c, int* r) {
+    return guard([&] {
+        require_batch(c);
+        require(r != nullptr, "null output");
+        bind(*c);
+        scene_intersections(*c, r);
+    });
+}
+
+int cipc_gpu_batch_solve(cipc_gpu_context* c, int precision, int preconditioner, double rel_tol, int max_iter, const int* active,
+                         double* direction, cipc_solve_stats* stats) {
+    return guard([&] {
+        require_batch(c);
+        require(precision == 0 || precision == 1, "precision must be 0 (FP64) or 1 (FP32 with FP64 refinement)");
+        require(preconditioner >= 0 && preconditioner <= 2, "preconditioner must be 0 (block-Jacobi), 1 (block FSAI) or 2 (learned)");
+        require(std::isfinite(rel_tol) && rel_tol > 0.0 && max_iter > 0, "tolerance and iteration cap must be positive");
+        require(c->assembled, "no assembled system: call cipc_gpu_batch_assemble first");
+        bind(*c);
+        const int S = c->sc.S;
+        hipEvent_t e0, e1;
+        CIPC_HIP(hipEventCreate(&e0));
+        CIPC_HIP(hipEventCreate(&e1));
+        CIPC_HIP(hipEventRecord(e0, c->stream));
+        std::vector<SolveResult> r(S);
+        linear_solve_scenes(*c, precision, preconditioner, rel_tol, max_iter, active, r.data());
+        CIPC_HIP(hipEventRecord(e1, c->stream));
+        CIPC_HIP(hipEventSynchronize(e1));
+        float ms = 0.f;
+        CIPC_HIP(hipEventElapsedTime(&ms, e0, e1));
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+        if (direction) {
+            CIPC_HIP(hipMemcpyAsync(direction, c->sc.dir.p, 3 * sizeof(double) * c->sc.V, hipMemcpyDeviceToHost, c->stream));
+            CIPC_HIP(hipStreamSynchronize(c->stream));
+        }
+        int missed = -1;
+        for (int k = 0; k < S; ++k) {
+            if (stats) {
+                stats[k].iterations = r[k].iterations;
+                stats[k].refinements = r[k].refinements;
+                stats[k].rel_residual = r[k].rel_residual;
+                stats[k].solve_ms = ms;
+            }
+            if ((!active || active[k]) && !(r[k].rel_residual <= rel_tol) && missed < 0) missed = k;
+        }
+        if (missed >= 0)
+            throw Error(CIPC_ERR_RUNTIME, "conjugate gradients of scene " + std::to_string(missed) + " stopped after " +
+                                              std::to_string(r[missed].iterations) + " iterations at relative residual " +
+                                              std::to_string(r[missed].rel_residual));
+    });
+}
+
+// One Newton iteration of every scene: active set, assembly with each scene's stiffness, step
+// bounds and energies (per scene), timed as cipc_gpu_newton_iteration; total->alpha and
+// total->energy hold the minimum step and the summed energy.
+int cipc_gpu_batch_newton_iteration(cipc_gpu_context* c, const double* kappa, cipc_iteration_timing* t, double* alpha, double* energy) {
+    return guard([&] {
+        require_batch(c);
+        require(kappa && t, "null stiffness array or output");
+        bind(*c);
+        const int S = c->sc.S;
+        set_scene_kappa(c, kappa, kappa[0]);
+        hipEvent_t ev[5], es, ee;
+        for (auto& e : ev) CIPC_HIP(hipEventCreate(&e));
+        CIPC_HIP(hipEventCreate(&es));
+        CIPC_HIP(hipEventCreate(&ee));
+        CIPC_HIP(hipEventRecord(ev[0], c->stream));
+        elastic_fork(c, es, ee);
+        compute_active_set(*c);
+        CIPC_HIP(hipEventRecord(ev[1], c->stream));
+        contact_assemble(*c, kappa[0]);
+        elastic_join(c);
+        finalize_csr(*c);
+        c->assembled = true;
+        CIPC_HIP(hipEventRecord(ev[2], c->stream));
+        std::vector<double> sb(5 * (size_t)S), et(9 * (size_t)S);
+        step_bound_scenes(*c, sb.data());
+        CIPC_HIP(hipEventRecord(ev[3], c->stream));
+        energy_terms_scenes(*c, et.data());
+        CIPC_HIP(hipEventRecord(ev[4], c->stream));
+        CIPC_HIP(hipEventSynchronize(ev[4]));
+        float ms[4];
+        for (int i = 0; i < 4; ++i) CIPC_HIP(hipEventElapsedTime(&ms[i], ev[i], ev[i + 1]));
+        float tot = 0, el = 0;
+        CIPC_HIP(hipEventElapsedTime(&tot, ev[0], ev[4]));
+        CIPC_HIP(hipEventElapsedTime(&el, es, ee));
+        for (auto& e : ev) (void)hipEventDestroy(e);
+        (void)hipEventDestroy(es);
+        (void)hipEventDestroy(ee);
+        t->broadphase_ms = ms[0]; t->assembly_ms = ms[1]; t->elastic_ms = el; t->ccd_ms = ms[2]; t->energy_ms = ms[3];
+        t->total_ms = tot;
+        double amin = 1.0, esum = 0.0;
+        for (int k = 0; k < S; ++k) {
+            const double a = sb[5 * (size_t)k], e = total_energy(*c, kappa[k], et.data() + 9 * (size_t)k);
+            if (alpha) alpha[k] = a;
+            if (energy) energy[k] = e;
+            amin = std::min(amin, a);
+            esum += e;
+        }
+        t->alpha = amin;
+        t->energy = esum;
+        t->nnz = c->nnz;
+        t->contacts = c->n_self + c->n_ee + c->n_env;
+    });
+}
+
 int cipc_gpu_friction_update(cipc_gpu_context* c, double kappa) {
     return guard([&] {
         require(c && c->have_scene, "no scene uploaded");
         bind(*c);
+        set_scene_kappa(c, nullptr, kappa);
         friction_update(*c, kappa);
         CIPC_HIP(hipStreamSynchronize(c->stream));
     });
@@ -604,6 +921,7 @@ int cipc_gpu_is_intersected(cipc_gpu_context* c, int* r) {
 int cipc_gpu_newton_iteration(cipc_gpu_context* c, double kappa, cipc_iteration_timing* t) {
     return guard([&] {
         require(c && c->have_scene && t, "no scene uploaded or null output");
+        require_single(c, "cipc_gpu_newton_iteration");
         bind(*c);
         hipEvent_t ev[5], es, ee;
         for (auto& e : ev) CIPC_HIP(hipEventCreate(&e));

@@ -63,6 +63,12 @@ struct SolveBuffers {
     DBuf<float> Gf;
     DBuf<int> mirror;                         // index of the transposed entry
     size_t ne = 0;                            // BSR entry capacity (sentinels included)
+    // batch: work-group tables of the row kernels (per vertex partition) and per-scene controls
+    std::vector<int> layout_key;
+    int layout_ng = 0;
+    DBuf<int4> bl;
+    DBuf<int> gb, ctl_i;
+    DBuf<double> ctl_d;
 };
 struct SolveResult {
     int iterations = 0, refinements = 0;
@@ -153,7 +159,10 @@ struct Context {
     // generic scratch
     DBuf<unsigned char> tmp;
     DBuf<int> iscratch, iscratch2;
-    DBuf<double> dscratch;
+    DBuf<double> dscratch, dscratch2;
+    // batch: device tet offsets (S + 1) then vertex offsets (S + 1); per-scene steps
+    DBuf<int> batch_off;
+    DBuf<double> batch_alpha;
     DBuf<unsigned long long> u64scratch, u64scratch2;
     DBuf<int4> i4scratch;
 };
@@ -184,12 +193,20 @@ int64_t kappa_gradients(Context& c, double out[2]);
 // postLineSearch's close-constraint check and records; returns 1 when a record got closer
 int close_constraints(Context& c, double dtol, int64_t counts[2]);
 bool any_intersection(Context& c);
-// ccd.hip
+// per scene: 1 when a ground gap is zero or an edge crosses a face
+void scene_intersections(Context& c, int* out);
+// ccd.hip: step-bound slots per scene (ground, CFL, partial CCD, full CCD, point-triangle
+// result, swept step)
+constexpr int kSlots = 8;
 void step_bound(Context& c, double out[5]);
+// per scene: out[5 k .. 5 k + 4] as step_bound's five values
+void step_bound_scenes(Context& c, double* out);
 // energy.hip
 void energy_terms(Context& c, double kappa, double out[9]);
 void cloth_energy_partials(Context& c, double* part_bw, double* part_bend);
 double total_energy(const Context& c, double kappa, const double t[9]);
+// per scene: out[9 k .. 9 k + 8] as energy_terms' nine values
+void energy_terms_scenes(Context& c, double* out);
 
 // Deterministic energy reductions: a fixed grid of kEnergyGrid work-groups of kEnergyBlock
 // threads, each summing a grid-strided subset in index order, then a shared-memory tree;
@@ -221,7 +238,12 @@ double learned_train_step(Context& c, float* params, float* m1, float* m2, int s
                           int objective);
 // Friction records after the contact records; returns the friction energy parts' launch
 void friction_records(Context& c, size_t slot_b, size_t slot_g);
-void friction_energy_partials(Context& c, double* part);
+void friction_energy_partials(Context& c, double* part, double* elem_self = nullptr, double* elem_ground = nullptr);
 void step_forward(Context& c, double alpha);
+// per scene: x = x0 - alpha[scene] p
+void step_forward_scenes(Context& c, const double* alpha);
+// per scene lock-step solves; active[k] == 0 leaves scene k's direction zero
+void linear_solve_scenes(Context& c, int precision, int preconditioner, double rel_tol, int max_iter, const int* active,
+                         SolveResult* res);
 
 }  // namespace cipc

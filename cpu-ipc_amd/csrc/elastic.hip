@@ -178,7 +178,7 @@ __device__ inline void snh_block(const SNH& e, const d3 B[4], const d3 G1[4], co
 
 __global__ __launch_bounds__(256) void snh_element_kernel(int T, const int4* __restrict__ tets, const double* __restrict__ x,
                                                           const double* __restrict__ dminv, const double* __restrict__ vol,
-                                                          double u, double r, double dt2, const int* __restrict__ slot_h,
+                                                          Lame lame, double dt2, const int* __restrict__ slot_h,
                                                           const int* __restrict__ slot_g, double* __restrict__ stagH,
                                                           double* __restrict__ stagG) {
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
@@ -188,6 +188,8 @@ __global__ __launch_bounds__(256) void snh_element_kernel(int T, const int4* __r
     load_tet(t, T, tets, x, dminv, e.F, B);
     const int4 tv = tets[t];
     const int vid[4] = {tv.x, tv.y, tv.z, tv.w};
+    double u, r;
+    lame.at(t, u, r);
     snh_eval(e, u, r);
     const double vt = vol[t];
     // gradient f_a = vol P B_a (dt^2 is applied by the vertex gather, as IPCSolver.cpp:149-151)
@@ -295,7 +297,7 @@ void elastic_assemble(Context& c, hipStream_t st) {
     const double dt2 = s.dt * s.dt;
     if (s.T > 0)
         hipLaunchKernelGGL(snh_element_kernel, dim3(blocks_for(s.T, 256)), dim3(256), 0, st, s.T, s.tets.p, s.x.p,
-                           s.dminv.p, s.vol.p, s.u, s.r, dt2, c.sp.slot_h.p, c.sp.slot_g.p, c.stagH.p, c.stagG.p);
+                           s.dminv.p, s.vol.p, scene_lame(s), dt2, c.sp.slot_h.p, c.sp.slot_g.p, c.stagH.p, c.stagG.p);
     CIPC_CHECK_LAUNCH();
     cloth_assemble(c, st);   // triangles and bending elements into the same staging
     if (c.sp.nblocks > 0) {

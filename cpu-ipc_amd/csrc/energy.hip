@@ -18,14 +18,20 @@ namespace cipc {
 constexpr int kEB = kEnergyBlock;   // threads per work-group
 constexpr int kEG = kEnergyGrid;    // work-groups
 
+// one partial per work-group; none when the kernel stores per-element values instead (part
+// null: the same for every thread of the work-group, so the barriers stay uniform)
 template <typename F>
-__device__ inline void block_sum_store(double v, double* part, F) { energy_block_sum(v, part); }
+__device__ inline void block_sum_store(double v, double* part, F) {
+    if (part) energy_block_sum(v, part);
+}
 
 __global__ __launch_bounds__(kEB) void e_snh(int T, const int4* __restrict__ tets, const double* __restrict__ x,
-                                             const double* __restrict__ dminv, const double* __restrict__ vol, double u, double r,
-                                             double* __restrict__ part) {
+                                             const double* __restrict__ dminv, const double* __restrict__ vol, Lame lame,
+                                             double* __restrict__ part, double* __restrict__ elem) {
     double acc = 0.0;
     for (int t = blockIdx.x * kEB + threadIdx.x; t < T; t += kEG * kEB) {
+        double u, r;
+        lame.at(t, u, r);
         const int4 id = tets[t];
         const d3 p0 = ld3(x, id.x);
         const d3 e1 = ld3(x, id.y) - p0, e2 = ld3(x, id.z) - p0, e3 = ld3(x, id.w) - p0;
@@ -40,27 +46,35 @@ __global__ __launch_bounds__(kEB) void e_snh(int T, const int4* __restrict__ tet
         const double I2 = sqn(f[0]) + sqn(f[1]) + sqn(f[2]);
         const double I3 = dot(f[0], cross(f[1], f[2]));
         const double k = I3 - 1 - 3 * u / 4 / r;
-        acc += (0.5 * u * (I2 - 3) + 0.5 * r * k * k - 0.5 * u * log(I2 + 1)) * vol[t];
+        const double et = (0.5 * u * (I2 - 3) + 0.5 * r * k * k - 0.5 * u * log(I2 + 1)) * vol[t];
+        if (elem) elem[t] = et;
+        else acc += et;
     }
     block_sum_store(acc, part, 0);
 }
 __global__ __launch_bounds__(kEB) void e_inertia(int V, const double* __restrict__ x, const double* __restrict__ xt,
                                                  const double* __restrict__ vp, const double* __restrict__ m, double drag,
-                                                 double* __restrict__ part) {
+                                                 double* __restrict__ part, double* __restrict__ elem) {
     double acc = 0.0;
     for (int v = blockIdx.x * kEB + threadIdx.x; v < V; v += kEG * kEB) {
         const d3 xv = ld3(x, v);
+        if (elem) {
+            elem[v] = sqn(xv - ld3(xt, v)) * m[v] / 2.0 + (sqn(xv - ld3(vp, v)) * (drag) * m[v] / 2.0);
+            continue;
+        }
         acc += sqn(xv - ld3(xt, v)) * m[v] / 2.0;
         acc += (sqn(xv - ld3(vp, v)) * (drag) * m[v] / 2.0);
     }
     block_sum_store(acc, part, 0);
 }
 __global__ __launch_bounds__(kEB) void e_ground(int n, const int* __restrict__ env, const double* __restrict__ x, d3 gn, double gd,
-                                                double hhat, double* __restrict__ part) {
+                                                double hhat, double* __restrict__ part, double* __restrict__ elem) {
     double acc = 0.0;
     for (int i = blockIdx.x * kEB + threadIdx.x; i < n; i += kEG * kEB) {
         const double dist = dot(gn, ld3(x, env[i])) - gd;
-        acc += barrier_b(dist * dist, hhat);
+        const double b = barrier_b(dist * dist, hhat);
+        if (elem) elem[i] = b;
+        else acc += b;
     }
     block_sum_store(acc, part, 0);
 }
@@ -72,19 +86,21 @@ __device__ inline double self_d(int4 c, const double* x) {
     return d_PT(p, ld3(x, c.y), ld3(x, c.z), ld3(x, c.w));
 }
 __global__ __launch_bounds__(kEB) void e_self(int n, const int4* __restrict__ set, const double* __restrict__ x, double hhat,
-                                              double* __restrict__ part) {
+                                              double* __restrict__ part, double* __restrict__ elem) {
     double acc = 0.0;
     for (int i = blockIdx.x * kEB + threadIdx.x; i < n; i += kEG * kEB) {
         const int4 c = set[i];
         double b = barrier_b(self_d(c, x), hhat);
         if (c.w < -1) b *= -c.w;
-        acc += b;
+        if (elem) elem[i] = b;
+        else acc += b;
     }
     block_sum_store(acc, part, 0);
 }
 __global__ __launch_bounds__(kEB) void e_mollified(int n, const int4* __restrict__ set, const int2* __restrict__ ie,
                                                    const int2* __restrict__ se, const double* __restrict__ x,
-                                                   const double* __restrict__ vrest, double hhat, double* __restrict__ part) {
+                                                   const double* __restrict__ vrest, double hhat, double* __restrict__ part,
+                                                   double* __restrict__ elem) {
     double acc = 0.0;
     for (int i = blockIdx.x * kEB + threadIdx.x; i < n; i += kEG * kEB) {
         const int4 c = set[i];
@@ -96,7 +112,9 @@ __global__ __launch_bounds__(kEB) void e_mollified(int n, const int4* __restrict
         }
         const double ex = eps_x_of(ld3(vrest, e4[0]), ld3(vrest, e4[1]), ld3(vrest, e4[2]), ld3(vrest, e4[3]));
         const double e = mol_e(ee_cross_sqn(ld3(x, e4[0]), ld3(x, e4[1]), ld3(x, e4[2]), ld3(x, e4[3])), ex);
-        acc += barrier_b(self_d(c, x), hhat) * e;
+        const double b = barrier_b(self_d(c, x), hhat) * e;
+        if (elem) elem[i] = b;
+        else acc += b;
     }
     block_sum_store(acc, part, 0);
 }
@@ -123,16 +141,20 @@ __global__ __launch_bounds__(kEB) void sum_partials(const double* __restrict__ p
 void energy_terms(Context& c, double kappa, double out[9]) {
     (void)kappa;
     Scene& s = c.sc;
+    if (s.S != 1) throw Error(CIPC_ERR_LOGIC, "a batch has one energy per scene");
     static_assert(kEG == kEB, "partials are summed by one work-group");
     DBuf<double>& part = c.dscratch;
     part.resize((size_t)11 * kEG);
     double* P = part.p;
-    hipLaunchKernelGGL(e_snh, dim3(kEG), dim3(kEB), 0, c.stream, s.T, s.tets.p, s.x.p, s.dminv.p, s.vol.p, s.u, s.r, P);
-    hipLaunchKernelGGL(e_inertia, dim3(kEG), dim3(kEB), 0, c.stream, s.V, s.x.p, s.xt.p, s.vprev.p, s.mass.p, s.drag, P + kEG);
-    hipLaunchKernelGGL(e_ground, dim3(kEG), dim3(kEB), 0, c.stream, (int)c.n_env, c.env_set.p, s.x.p, s.gn, s.gd, s.hhat, P + 2 * kEG);
-    hipLaunchKernelGGL(e_self, dim3(kEG), dim3(kEB), 0, c.stream, (int)c.n_self, c.self_set.p, s.x.p, s.hhat, P + 3 * kEG);
+    hipLaunchKernelGGL(e_snh, dim3(kEG), dim3(kEB), 0, c.stream, s.T, s.tets.p, s.x.p, s.dminv.p, s.vol.p, scene_lame(s), P,
+                       nullptr);
+    hipLaunchKernelGGL(e_inertia, dim3(kEG), dim3(kEB), 0, c.stream, s.V, s.x.p, s.xt.p, s.vprev.p, s.mass.p, s.drag, P + kEG,
+                       nullptr);
+    hipLaunchKernelGGL(e_ground, dim3(kEG), dim3(kEB), 0, c.stream, (int)c.n_env, c.env_set.p, s.x.p, s.gn, s.gd, s.hhat, P + 2 * kEG,
+                       nullptr);
+    hipLaunchKernelGGL(e_self, dim3(kEG), dim3(kEB), 0, c.stream, (int)c.n_self, c.self_set.p, s.x.p, s.hhat, P + 3 * kEG, nullptr);
     hipLaunchKernelGGL(e_mollified, dim3(kEG), dim3(kEB), 0, c.stream, (int)c.n_ee, c.ee_set.p, c.ee_ie.p, s.se.p, s.x.p, s.vrest.p,
-                       s.hhat, P + 4 * kEG);
+                       s.hhat, P + 4 * kEG, nullptr);
     // lagged friction: self part (lambda times clamped norm) and ground part (mu included)
     friction_energy_partials(c, P + 5 * kEG);
     // cloth triangles and bending (IPCSolver.cpp:330-368)
@@ -149,6 +171,164 @@ void energy_terms(Context& c, double kappa, double out[9]) {
     out[6] = t[7];
     out[7] = s.bmode == 0 ? 0.5 * s.plate * t[8] : t[8];   // quadratic form: D/2 x^T (Q (x) I3) x
     out[8] = t[9];
+}
+
+// ---- per-scene sums of a batch ---------------------------------------------------------
+// Each term's per-element values are summed per scene.  Tets and vertices of a scene are
+// contiguous ranges.  A contact list is ordered by scene within each of its parts (the
+// queries run over surface vertices and edges in scene order), so it splits into runs of
+// one scene each: every run is summed by one work-group and the runs of a scene are added
+// in list order.  The order of every addition is fixed, so the sums are deterministic.
+constexpr int kRunGrid = 1024;
+__device__ inline int first_vertex(int4 c) { return c.x >= 0 ? c.x : -c.x - 1; }
+__device__ inline int first_vertex(int v) { return v; }
+// block sum of one value per thread (all threads get it)
+__device__ inline double block_total(double v) {
+    __shared__ double sh[kEB];
+    sh[threadIdx.x] = v;
+    __syncthreads();
+    for (int s = kEB / 2; s > 0; s >>= 1) {
+        if ((int)threadIdx.x < s) sh[threadIdx.x] += sh[threadIdx.x + s];
+        __syncthreads();
+    }
+    const double r = sh[0];
+    __syncthreads();
+    return r;
+}
+// out[k * 9 + term] = sum of elem[off[k] .. off[k + 1]) (one work-group per scene)
+__global__ __launch_bounds__(kEB) void range_sums(const double* __restrict__ elem, const int* __restrict__ off, int term,
+                                                  double* __restrict__ out) {
+    const int k = blockIdx.x;
+    double acc = 0.0;
+    for (int i = off[k] + threadIdx.x; i < off[k + 1]; i += kEB) acc += elem[i];
+    const double t = block_total(acc);
+    if (threadIdx.x == 0) out[9 * k + term] = t;
+}
+// head[i] = 1 where the scene of record i differs from record i - 1
+template <typename R>
+__global__ void scene_run_heads(int n, const R* __restrict__ rec, const int* __restrict__ vs, int* __restrict__ key,
+                                int* __restrict__ head) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i > n) return;
+    if (i == n) { head[n] = 0; return; }
+    const int k = vs[first_vertex(rec[i])];
+    key[i] = k;
+    head[i] = (i == 0 || vs[first_vertex(rec[i - 1])] != k) ? 1 : 0;
+}
+__global__ void run_starts(int n, const int* __restrict__ head, const int* __restrict__ pos, int* __restrict__ start,
+                           int* __restrict__ nruns) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i > n) return;
+    if (i == n) { const int r = n > 0 ? pos[n - 1] + head[n - 1] : 0; start[r] = n; *nruns = r; return; }
+    if (head[i]) start[pos[i]] = i;
+}
+// run sums: work-group b sums runs b, b + kRunGrid, ...
+__global__ __launch_bounds__(kEB) void run_sums(const double* __restrict__ elem, const int* __restrict__ start,
+                                                const int* __restrict__ nruns, double* __restrict__ rsum) {
+    const int R = *nruns;
+    for (int r = blockIdx.x; r < R; r += gridDim.x) {
+        double acc = 0.0;
+        for (int i = start[r] + threadIdx.x; i < start[r + 1]; i += kEB) acc += elem[i];
+        const double t = block_total(acc);
+        if (threadIdx.x == 0) rsum[r] = t;
+    }
+}
+// out[k * 9 + term] = sum over scene k's runs in list order (one work-group per scene)
+__global__ __launch_bounds__(kEB) void scene_from_runs(const double* __restrict__ rsum, const int* __restrict__ start,
+                                                       const int* __restrict__ key, const int* __restrict__ nruns, int term,
+                                                       double* __restrict__ out) {
+    const int k = blockIdx.x, R = *nruns;
+    double acc = 0.0;
+    for (int r = threadIdx.x; r < R; r += kEB)
+        if (key[start[r]] == k) acc += rsum[r];
+    const double t = block_total(acc);
+    if (threadIdx.x == 0) out[9 * k + term] = t;
+}
+__global__ void zero_term(int S, int term, double* __restrict__ out) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k < S) out[9 * k + term] = 0.0;
+}
+
+namespace {
+template <typename R>
+void scene_list_sums(Context& c, int n, const R* rec, const double* elem, int term, double* out) {
+    Scene& s = c.sc;
+    const int S = s.S;
+    if (n == 0) {
+        hipLaunchKernelGGL(zero_term, dim3(blocks_for(S, 64)), dim3(64), 0, c.stream, S, term, out);
+        CIPC_CHECK_LAUNCH();
+        return;
+    }
+    DBuf<int>& key = c.iscratch;
+    DBuf<int>& aux = c.iscratch2;   // head (n + 1), pos (n + 1), start (n + 1), count (1)
+    key.resize((size_t)n + 1);
+    aux.resize(3 * ((size_t)n + 1) + 1);
+    int* head = aux.p;
+    int* pos = aux.p + (n + 1);
+    int* start = aux.p + 2 * (size_t)(n + 1);
+    int* cnt = aux.p + 3 * (size_t)(n + 1);
+    c.dscratch2.resize((size_t)n + 1);
+    hipLaunchKernelGGL(scene_run_heads<R>, dim3(blocks_for((size_t)n + 1, 256)), dim3(256), 0, c.stream, n, rec, s.vscene.p, key.p, head);
+    CIPC_CHECK_LAUNCH();
+    exclusive_scan_int(c, head, pos, n + 1);
+    hipLaunchKernelGGL(run_starts, dim3(blocks_for((size_t)n + 1, 256)), dim3(256), 0, c.stream, n, head, pos, start, cnt);
+    hipLaunchKernelGGL(run_sums, dim3(kRunGrid), dim3(kEB), 0, c.stream, elem, start, cnt, c.dscratch2.p);
+    hipLaunchKernelGGL(scene_from_runs, dim3(S), dim3(kEB), 0, c.stream, c.dscratch2.p, start, key.p, cnt, term, out);
+    CIPC_CHECK_LAUNCH();
+}
+}  // namespace
+
+void energy_terms_scenes(Context& c, double* out) {
+    Scene& s = c.sc;
+    const int S = s.S;
+    if (S == 1) {
+        energy_terms(c, 0.0, out);
+        return;
+    }
+    if (s.Tr > 0 || s.Nb > 0 || c.n_soft > 0) throw Error(CIPC_ERR_LOGIC, "batched scenes hold tets only");
+    // per-element values: tets, vertices, ground, self, mollified, friction self, friction ground
+    const size_t nT = s.T, nV = s.V, nG = c.n_env, nS = c.n_self, nE = c.n_ee, nFs = c.n_fself, nFg = c.n_fenv;
+    DBuf<double>& el = c.dscratch;
+    el.resize(nT + nV + nG + nS + nE + nFs + nFg + (size_t)9 * S + 1);
+    double* eT = el.p;
+    double* eV = eT + nT;
+    double* eG = eV + nV;
+    double* eS = eG + nG;
+    double* eE = eS + nS;
+    double* eFs = eE + nE;
+    double* eFg = eFs + nFs;
+    double* dev_out = eFg + nFg;
+    hipLaunchKernelGGL(e_snh, dim3(kEG), dim3(kEB), 0, c.stream, s.T, s.tets.p, s.x.p, s.dminv.p, s.vol.p, scene_lame(s), nullptr, eT);
+    hipLaunchKernelGGL(e_inertia, dim3(kEG), dim3(kEB), 0, c.stream, s.V, s.x.p, s.xt.p, s.vprev.p, s.mass.p, s.drag, nullptr, eV);
+    if (nG) hipLaunchKernelGGL(e_ground, dim3(kEG), dim3(kEB), 0, c.stream, (int)nG, c.env_set.p, s.x.p, s.gn, s.gd, s.hhat, nullptr, eG);
+    if (nS) hipLaunchKernelGGL(e_self, dim3(kEG), dim3(kEB), 0, c.stream, (int)nS, c.self_set.p, s.x.p, s.hhat, nullptr, eS);
+    if (nE)
+        hipLaunchKernelGGL(e_mollified, dim3(kEG), dim3(kEB), 0, c.stream, (int)nE, c.ee_set.p, c.ee_ie.p, s.se.p, s.x.p, s.vrest.p,
+                           s.hhat, nullptr, eE);
+    if (nFs + nFg) friction_energy_partials(c, nullptr, eFs, eFg);
+    CIPC_CHECK_LAUNCH();
+    // device offsets of tets and vertices follow the output block
+    DBuf<int>& off = c.batch_off;
+    hipLaunchKernelGGL(range_sums, dim3(S), dim3(kEB), 0, c.stream, eT, off.p, 0, dev_out);
+    hipLaunchKernelGGL(range_sums, dim3(S), dim3(kEB), 0, c.stream, eV, off.p + (S + 1), 1, dev_out);
+    CIPC_CHECK_LAUNCH();
+    scene_list_sums<int>(c, (int)nG, c.env_set.p, eG, 2, dev_out);
+    scene_list_sums<int4>(c, (int)nS, c.self_set.p, eS, 3, dev_out);
+    scene_list_sums<int4>(c, (int)nE, c.ee_set.p, eE, 4, dev_out);
+    scene_list_sums<int4>(c, (int)nFs, c.fr_self.p, eFs, 5, dev_out);
+    scene_list_sums<int>(c, (int)nFg, c.fr_env.p, eFg, 6, dev_out);
+    std::vector<double> h((size_t)9 * S);
+    CIPC_HIP(hipMemcpyAsync(h.data(), dev_out, h.size() * sizeof(double), hipMemcpyDeviceToHost, c.stream));
+    CIPC_HIP(hipStreamSynchronize(c.stream));
+    for (int k = 0; k < S; ++k) {
+        const double* t = h.data() + 9 * (size_t)k;
+        double* o = out + 9 * (size_t)k;
+        for (int q = 0; q < 5; ++q) o[q] = t[q];
+        o[5] = s.fric * t[5] + t[6];   // as energy_terms: mu * sum_self + sum_ground
+        o[6] = 0.0;
+        o[7] = 0.0;
+        o[8] = 0.0;
+    }
 }
 
 // Total as computeEnergy adds it: dt^2 (E_SNH + E_BW + E_bend) + inertia + soft + kappa
@@ -175,6 +355,22 @@ void step_forward(Context& c, double alpha) {
     Scene& s = c.sc;
     hipLaunchKernelGGL(step_forward_kernel, dim3(blocks_for(s.V, 256)), dim3(256), 0, c.stream, s.V, s.bt.p, s.x0.p, s.dir.p, alpha, s.x.p);
     CIPC_CHECK_LAUNCH();
+}
+__global__ void step_forward_scenes_kernel(int V, const int* __restrict__ bt, const int* __restrict__ vs, const double* __restrict__ x0,
+                                           const double* __restrict__ p, const double* __restrict__ a, double* __restrict__ x) {
+    const int v = blockIdx.x * blockDim.x + threadIdx.x;
+    if (v >= V || !is_free(bt[v])) return;
+    st3(x, v, ld3(x0, v) - a[vs[v]] * ld3(p, v));
+}
+void step_forward_scenes(Context& c, const double* alpha) {
+    Scene& s = c.sc;
+    if (s.S == 1) { step_forward(c, alpha[0]); return; }
+    c.batch_alpha.resize(s.S);
+    CIPC_HIP(hipMemcpyAsync(c.batch_alpha.p, alpha, s.S * sizeof(double), hipMemcpyHostToDevice, c.stream));
+    hipLaunchKernelGGL(step_forward_scenes_kernel, dim3(blocks_for(s.V, 256)), dim3(256), 0, c.stream, s.V, s.bt.p, s.vscene.p, s.x0.p,
+                       s.dir.p, c.batch_alpha.p, s.x.p);
+    CIPC_CHECK_LAUNCH();
+    CIPC_HIP(hipStreamSynchronize(c.stream));
 }
 
 }  // namespace cipc

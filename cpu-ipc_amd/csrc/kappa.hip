@@ -41,10 +41,12 @@ __device__ inline double constraint_value(int4 c, const double* x) {
 
 // Tet corner forces vol (pa F + pb cof F) B_a into the tet slots (no Hessian, no SVD).
 __global__ __launch_bounds__(256) void tet_force_kernel(int T, const int4* __restrict__ tets, const double* __restrict__ x,
-                                                        const double* __restrict__ dminv, const double* __restrict__ vol, double u,
-                                                        double r, const int* __restrict__ slot_g, double* __restrict__ stagG) {
+                                                        const double* __restrict__ dminv, const double* __restrict__ vol, Lame lame,
+                                                        const int* __restrict__ slot_g, double* __restrict__ stagG) {
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= T) return;
+    double u, r;
+    lame.at(t, u, r);
     const int4 id = tets[t];
     const d3 p0 = ld3(x, id.x);
     const d3 e1 = ld3(x, id.y) - p0, e2 = ld3(x, id.z) - p0, e3 = ld3(x, id.w) - p0;
@@ -167,7 +169,7 @@ int64_t kappa_gradients(Context& c, double out[2]) {
     c.kgc.resize(3 * (size_t)V);
     if (s.T > 0) {
         hipLaunchKernelGGL(tet_force_kernel, dim3(blocks_for(s.T, 256)), dim3(256), 0, c.stream, s.T, s.tets.p, s.x.p, s.dminv.p,
-                           s.vol.p, s.u, s.r, c.sp.slot_g.p, c.stagG.p);
+                           s.vol.p, scene_lame(s), c.sp.slot_g.p, c.stagG.p);
         CIPC_CHECK_LAUNCH();
     }
     hipLaunchKernelGGL(egrad_gather_kernel, dim3(blocks_for(V, 256)), dim3(256), 0, c.stream, V, s.x.p, s.xt.p, s.mass.p,

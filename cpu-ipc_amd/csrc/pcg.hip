@@ -183,18 +183,44 @@ __device__ inline double sum_partials_fixed(const double* __restrict__ part, int
     return r;
 }
 
+// ---- work-group layout of the row kernels ----------------------------------------------
+// One system: work-group b covers block rows [b kRows, b kRows + kRows) of [0, V).  A batch of
+// independent systems (block-diagonal matrix, one block per scene): bl[b] = {scene, first
+// row, end row}; the work-groups of a scene are consecutive and never cover two scenes, and
+// gb[k] .. gb[k + 1] are scene k's work-groups, so that the partials of one scene are summed
+// exactly as those of the same system solved alone.
+struct Layout {
+    int V = 0, ng = 0;
+    const int4* bl = nullptr;
+    const int* gb = nullptr;
+    __device__ void block(int& scene, int& row0, int& rend) const {
+        if (bl) {
+            const int4 t = bl[blockIdx.x];
+            scene = t.x; row0 = t.y; rend = t.z;
+        } else {
+            scene = 0; row0 = blockIdx.x * kRows; rend = V;
+        }
+    }
+    __device__ void parts(int scene, int& lo, int& n) const {
+        if (gb) { lo = gb[scene]; n = gb[scene + 1] - lo; }
+        else { lo = 0; n = ng; }
+    }
+};
+
 // ---- PCG kernels (T: storage precision of blocks and vectors; scalars FP64) ------------
 // Partials layout (ng work-groups each): [0] p.q, [1] r.z, [2] r.r, [3] b.b
 // r = b - A x (A x skipped when x is zero); partials of r.r and b.b
 // (grid: kRows block rows per work-group, kGL lanes per row)
 template <typename T>
-__global__ __launch_bounds__(kPB) void pcg_residual(int V, const int* __restrict__ rowptr, const int* __restrict__ col,
+__global__ __launch_bounds__(kPB) void pcg_residual(Layout L, const int* __restrict__ rowptr, const int* __restrict__ col,
                                                     const T* __restrict__ val, const double* __restrict__ b, const T* __restrict__ x,
                                                     int x_zero, T* __restrict__ r, double* __restrict__ part_rr,
                                                     double* __restrict__ part_bb) {
-    const int a = blockIdx.x * kRows + threadIdx.x / kGL, sub = threadIdx.x % kGL;
+    int scene, row0, rend;
+    L.block(scene, row0, rend);
+    const int a = row0 + threadIdx.x / kGL, sub = threadIdx.x % kGL;
     double acc[2] = {0, 0};
-    if (a < V) {
+    if (a < rend) {
         double y[3] = {0, 0, 0};
         if (!x_zero) row_product<T, T>(a, sub, rowptr, col, val, x, y);
         if (sub == 0) {
@@ -208,13 +234,15 @@ __global__ __launch_bounds__(kPB) void pcg_residual(int V, const int* __restrict
 }
 // block-Jacobi: z = M^-1 r; partial of r.z; p = z when `init`
 template <typename T>
-__global__ __launch_bounds__(kPB) void prec_jacobi(int V, const T* __restrict__ minv, const T* __restrict__ r, T* __restrict__ z,
+__global__ __launch_bounds__(kPB) void prec_jacobi(Layout L, const T* __restrict__ minv, const T* __restrict__ r, T* __restrict__ z,
                                                    T* __restrict__ p, int init, const PcgScalars* __restrict__ sc,
                                                    double* __restrict__ part_rz) {
-    if (sc && sc->done) return;
-    const int a = blockIdx.x * kRows + threadIdx.x / kGL, sub = threadIdx.x % kGL;
+    int scene, row0, rend;
+    L.block(scene, row0, rend);
+    if (sc && sc[scene].done) return;
+    const int a = row0 + threadIdx.x / kGL, sub = threadIdx.x % kGL;
     double acc[1] = {0.0};
-    if (a < V && sub == 0) {
+    if (a < rend && sub == 0) {
         const T rt[3] = {r[3 * a], r[3 * a + 1], r[3 * a + 2]};
         double zv[3];
         block_mv(minv + (size_t)9 * a, rt, zv);
@@ -229,11 +257,13 @@ __global__ __launch_bounds__(kPB) void prec_jacobi(int V, const T* __restrict__ 
 // factor preconditioner M^-1 = G^T G, G block lower-triangular stored at the lower entries of
 // the BSR (the entry of (i, j), j <= i, holds G_ij):  t = G r
 template <typename T>
-__global__ __launch_bounds__(kPB) void prec_g1(int V, const int* __restrict__ rowptr, const int* __restrict__ col, const T* __restrict__ G,
+__global__ __launch_bounds__(kPB) void prec_g1(Layout L, const int* __restrict__ rowptr, const int* __restrict__ col, const T* __restrict__ G,
                                                const T* __restrict__ r, T* __restrict__ t, const PcgScalars* __restrict__ sc) {
-    if (sc && sc->done) return;
-    const int a = blockIdx.x * kRows + threadIdx.x / kGL, sub = threadIdx.x % kGL;
-    if (a >= V) return;
+    int scene, row0, rend;
+    L.block(scene, row0, rend);
+    if (sc && sc[scene].done) return;
+    const int a = row0 + threadIdx.x / kGL, sub = threadIdx.x % kGL;
+    if (a >= rend) return;
     double y[3] = {0, 0, 0};
     for (int e = rowptr[a] + sub; e < rowptr[a + 1]; e += kGL) {
         const int j = col[e];
@@ -249,14 +279,16 @@ __global__ __launch_bounds__(kPB) void prec_g1(int V, const int* __restrict__ ro
 // z = G^T t: row j gathers G_ij^T t_i over its upper entries (i >= j) through the mirror
 // index of each entry; partial of r.z; p = z when `init`
 template <typename T>
-__global__ __launch_bounds__(kPB) void prec_g2(int V, const int* __restrict__ rowptr, const int* __restrict__ col,
+__global__ __launch_bounds__(kPB) void prec_g2(Layout L, const int* __restrict__ rowptr, const int* __restrict__ col,
                                                const int* __restrict__ mirror, const T* __restrict__ G, const T* __restrict__ t,
                                                const T* __restrict__ r, T* __restrict__ z, T* __restrict__ p, int init,
                                                const PcgScalars* __restrict__ sc, double* __restrict__ part_rz) {
-    if (sc && sc->done) return;
-    const int a = blockIdx.x * kRows + threadIdx.x / kGL, sub = threadIdx.x % kGL;
+    int scene, row0, rend;
+    L.block(scene, row0, rend);
+    if (sc && sc[scene].done) return;
+    const int a = row0 + threadIdx.x / kGL, sub = threadIdx.x % kGL;
     double acc[1] = {0.0};
-    if (a < V) {
+    if (a < rend) {
         double y[3] = {0, 0, 0};
         for (int e = rowptr[a + 1] - 1 - sub; e >= rowptr[a]; e -= kGL) {
             const int i = col[e];
@@ -279,31 +311,46 @@ __global__ __launch_bounds__(kPB) void prec_g2(int V, const int* __restrict__ ro
     }
     group_partials<1>(acc, part_rz);
 }
-// scalars from the start partials: rz, rr, ||b|| (or ||r0|| as the reference of an inner solve)
-__global__ __launch_bounds__(kPB) void pcg_init_scalars(const double* __restrict__ part_rz, const double* __restrict__ part_rr,
-                                                        const double* __restrict__ part_bb, int ng, double rel_tol, int rel_to_r0,
-                                                        PcgScalars* __restrict__ sc) {
-    const double rz = sum_partials_fixed(part_rz, ng);
-    const double rr = sum_partials_fixed(part_rr, ng);
-    const double bb = sum_partials_fixed(part_bb, ng);
+// Per-scene controls of a batch solve (null for one system): relative tolerance, iteration
+// cap and whether the scene takes part at all.
+struct SceneControl {
+    const double* rel_tol = nullptr;
+    const int* max_iter = nullptr;
+    const int* active = nullptr;
+};
+// scalars from the start partials: rz, rr, ||b|| (or ||r0|| as the reference of an inner
+// solve); one work-group per scene
+__global__ __launch_bounds__(kPB) void pcg_init_scalars(Layout L, const double* __restrict__ part_rz, const double* __restrict__ part_rr,
+                                                        const double* __restrict__ part_bb, double rel_tol, int rel_to_r0,
+                                                        SceneControl ctl, PcgScalars* __restrict__ sc) {
+    const int k = blockIdx.x;
+    int lo, n;
+    L.parts(k, lo, n);
+    const double rz = sum_partials_fixed(part_rz + lo, n);
+    const double rr = sum_partials_fixed(part_rr + lo, n);
+    const double bb = sum_partials_fixed(part_bb + lo, n);
     if (threadIdx.x == 0) {
-        sc->rz = rz;
-        sc->rr = rr;
-        sc->bn = sqrt(bb);
-        sc->tol = rel_tol * (rel_to_r0 ? sqrt(rr) : sqrt(bb));
-        sc->done = !(sqrt(rr) > sc->tol) || !(rz > 0.0);
-        sc->iters = 0;
+        const double tol = ctl.rel_tol ? ctl.rel_tol[k] : rel_tol;
+        PcgScalars& s = sc[k];
+        s.rz = rz;
+        s.rr = rr;
+        s.bn = sqrt(bb);
+        s.tol = tol * (rel_to_r0 ? sqrt(rr) : sqrt(bb));
+        s.done = !(sqrt(rr) > s.tol) || !(rz > 0.0) || (ctl.active && !ctl.active[k]);
+        s.iters = 0;
     }
 }
 // q = A p and partials of p.q
 template <typename T>
-__global__ __launch_bounds__(kPB) void pcg_spmv(int V, const int* __restrict__ rowptr, const int* __restrict__ col,
+__global__ __launch_bounds__(kPB) void pcg_spmv(Layout L, const int* __restrict__ rowptr, const int* __restrict__ col,
                                                 const T* __restrict__ val, const T* __restrict__ p, T* __restrict__ q,
                                                 const PcgScalars* __restrict__ sc, double* __restrict__ part_pq) {
-    if (sc->done) return;
-    const int a = blockIdx.x * kRows + threadIdx.x / kGL, sub = threadIdx.x % kGL;
+    int scene, row0, rend;
+    L.block(scene, row0, rend);
+    if (sc[scene].done) return;
+    const int a = row0 + threadIdx.x / kGL, sub = threadIdx.x % kGL;
     double acc[1] = {0.0};
-    if (a < V) {
+    if (a < rend) {
         double y[3];
         row_product<T, T>(a, sub, rowptr, col, val, p, y);
         if (sub == 0) {
@@ -317,16 +364,20 @@ __global__ __launch_bounds__(kPB) void pcg_spmv(int V, const int* __restrict__ r
 // alpha = rz / p.q; x += alpha p; r -= alpha q; partial of r.r; with block-Jacobi (minv set)
 // also z = M^-1 r and its r.z partial
 template <typename T>
-__global__ __launch_bounds__(kPB) void pcg_update(int V, const double* __restrict__ part_pq, int ng, const T* __restrict__ minv,
+__global__ __launch_bounds__(kPB) void pcg_update(Layout L, const double* __restrict__ part_pq, const T* __restrict__ minv,
                                                   const T* __restrict__ p, const T* __restrict__ q, T* __restrict__ x, T* __restrict__ r,
                                                   T* __restrict__ z, const PcgScalars* __restrict__ sc, double* __restrict__ part_rz,
                                                   double* __restrict__ part_rr) {
-    if (sc->done) return;
-    const double pq = sum_partials_fixed(part_pq, ng);
-    const double alpha = sc->rz / pq;
-    const int a = blockIdx.x * kRows + threadIdx.x / kGL, sub = threadIdx.x % kGL;
+    int scene, row0, rend;
+    L.block(scene, row0, rend);
+    if (sc[scene].done) return;
+    int lo, n;
+    L.parts(scene, lo, n);
+    const double pq = sum_partials_fixed(part_pq + lo, n);
+    const double alpha = sc[scene].rz / pq;
+    const int a = row0 + threadIdx.x / kGL, sub = threadIdx.x % kGL;
     double acc[2] = {0.0, 0.0};
-    if (a < V && sub == 0) {
+    if (a < rend && sub == 0) {
         T rt[3];
         for (int k = 0; k < 3; ++k) {
             x[3 * a + k] = (T)((double)x[3 * a + k] + alpha * (double)p[3 * a + k]);
@@ -358,22 +409,43 @@ __global__ __launch_bounds__(kPB) void pcg_direction(int V, const double* __rest
     const int t = blockIdx.x * blockDim.x + threadIdx.x;       // one DOF per thread
     if (t < 3 * V) p[t] = (T)((double)z[t] + beta * (double)p[t]);
 }
-__global__ void pcg_scalars(const double* __restrict__ part_rz, const double* __restrict__ part_rr, int ng,
-                            PcgScalars* __restrict__ sc, int max_iter) {
-    if (sc->done) return;
-    const double rz1 = sum_partials_fixed(part_rz, ng);
-    const double rr = sum_partials_fixed(part_rr, ng);
+// the same on the row layout of a batch: the 3 kRows DOFs of the work-group's rows
+template <typename T>
+__global__ __launch_bounds__(kPB) void pcg_direction_rows(Layout L, const double* __restrict__ part_rz, const T* __restrict__ z,
+                                                          T* __restrict__ p, const PcgScalars* __restrict__ sc) {
+    int scene, row0, rend;
+    L.block(scene, row0, rend);
+    if (sc[scene].done) return;
+    int lo, n;
+    L.parts(scene, lo, n);
+    const double rz1 = sum_partials_fixed(part_rz + lo, n);
+    const double beta = rz1 / sc[scene].rz;
+    const int t = 3 * row0 + threadIdx.x;
+    if ((int)threadIdx.x < 3 * kRows && t < 3 * rend) p[t] = (T)((double)z[t] + beta * (double)p[t]);
+}
+// one work-group per scene
+__global__ void pcg_scalars(Layout L, const double* __restrict__ part_rz, const double* __restrict__ part_rr,
+                            PcgScalars* __restrict__ sc, int max_iter, SceneControl ctl) {
+    const int k = blockIdx.x;
+    if (sc[k].done) return;
+    int lo, n;
+    L.parts(k, lo, n);
+    const double rz1 = sum_partials_fixed(part_rz + lo, n);
+    const double rr = sum_partials_fixed(part_rr + lo, n);
     if (threadIdx.x == 0) {
-        sc->rz = rz1;
-        sc->rr = rr;
-        sc->iters += 1;
-        if (!(sqrt(rr) > sc->tol) || sc->iters >= max_iter || !(rz1 > 0.0)) sc->done = 1;
+        PcgScalars& s = sc[k];
+        const int cap = ctl.max_iter ? ctl.max_iter[k] : max_iter;
+        s.rz = rz1;
+        s.rr = rr;
+        s.iters += 1;
+        if (!(sqrt(rr) > s.tol) || s.iters >= cap || !(rz1 > 0.0)) s.done = 1;
     }
 }
-// x (FP64) += d (FP32), for the refinement
-__global__ void add_correction(int n, const float* __restrict__ d, double* __restrict__ x) {
+// x (FP64) += d (FP32), for the refinement (scenes with add[scene] != 0 when add is set)
+__global__ void add_correction(int n, const float* __restrict__ d, double* __restrict__ x, const int* __restrict__ vs,
+                               const int* __restrict__ add) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) x[i] += (double)d[i];
+    if (i < n && (!add || add[vs[i / 3]])) x[i] += (double)d[i];
 }
 template <typename S, typename D>
 __global__ void convert(size_t n, const S* __restrict__ s, D* __restrict__ d) {
@@ -524,11 +596,15 @@ __global__ void bsr_mirror(int V, const int* __restrict__ rowptr, const int* __r
 
 // ---- host ---------------------------------------------------------------------------
 namespace {
-PcgScalars read_scalars(Context& c, const PcgScalars* d) {
-    PcgScalars h;
-    CIPC_HIP(hipMemcpyAsync(&h, d, sizeof(PcgScalars), hipMemcpyDeviceToHost, c.stream));
+void read_scalars(Context& c, const PcgScalars* d, int S, std::vector<PcgScalars>& h) {
+    h.resize(S);
+    CIPC_HIP(hipMemcpyAsync(h.data(), d, S * sizeof(PcgScalars), hipMemcpyDeviceToHost, c.stream));
     CIPC_HIP(hipStreamSynchronize(c.stream));
-    return h;
+}
+bool all_done(const std::vector<PcgScalars>& h) {
+    for (const PcgScalars& s : h)
+        if (!s.done) return false;
+    return true;
 }
 
 // The preconditioner of one solve: block-Jacobi (minv) or the factor G (with mirror).
@@ -539,27 +615,60 @@ struct Prec {
     const int* mirror = nullptr;
 };
 
+// Work-group layout of the current system: kRows block rows per work-group; for a batch
+// the tables of per-scene work-groups (cached per vertex partition).
+Layout make_layout(Context& c) {
+    Scene& s = c.sc;
+    SolveBuffers& w = c.solve;
+    Layout L;
+    L.V = s.V;
+    if (s.S == 1) {
+        L.ng = (int)blocks_for(s.V, kRows);
+        return L;
+    }
+    if (w.layout_key != s.h_voff) {
+        std::vector<int4> bl;
+        std::vector<int> gb(s.S + 1, 0);
+        for (int k = 0; k < s.S; ++k) {
+            gb[k] = (int)bl.size();
+            for (int r = s.h_voff[k]; r < s.h_voff[k + 1]; r += kRows) bl.push_back(make_int4(k, r, s.h_voff[k + 1], 0));
+        }
+        gb[s.S] = (int)bl.size();
+        w.bl.resize(std::max<size_t>(bl.size(), 1));
+        w.gb.resize(gb.size());
+        CIPC_HIP(hipMemcpyAsync(w.bl.p, bl.data(), bl.size() * sizeof(int4), hipMemcpyHostToDevice, c.stream));
+        CIPC_HIP(hipMemcpyAsync(w.gb.p, gb.data(), gb.size() * sizeof(int), hipMemcpyHostToDevice, c.stream));
+        CIPC_HIP(hipStreamSynchronize(c.stream));
+        w.layout_key = s.h_voff;
+        w.layout_ng = (int)bl.size();
+    }
+    L.ng = w.layout_ng;
+    L.bl = w.bl.p;
+    L.gb = w.gb.p;
+    return L;
+}
+
 template <typename T>
-void apply_prec(Context& c, const Prec<T>& M, const T* r, T* z, T* p, int init, const PcgScalars* sc, double* part_rz) {
-    const int V = c.sc.V, ng = (int)blocks_for(V, kRows);
+void apply_prec(Context& c, const Layout& L, const Prec<T>& M, const T* r, T* z, T* p, int init, const PcgScalars* sc, double* part_rz) {
     SolveBuffers& w = c.solve;
     if (M.G) {
         T* t = reinterpret_cast<T*>(w.t.p);
-        hipLaunchKernelGGL(prec_g1<T>, dim3(ng), dim3(kPB), 0, c.stream, V, w.rowptr.p, w.col.p, M.G, r, t, sc);
-        hipLaunchKernelGGL(prec_g2<T>, dim3(ng), dim3(kPB), 0, c.stream, V, w.rowptr.p, w.col.p, M.mirror, M.G, t, r, z, p, init, sc,
+        hipLaunchKernelGGL(prec_g1<T>, dim3(L.ng), dim3(kPB), 0, c.stream, L, w.rowptr.p, w.col.p, M.G, r, t, sc);
+        hipLaunchKernelGGL(prec_g2<T>, dim3(L.ng), dim3(kPB), 0, c.stream, L, w.rowptr.p, w.col.p, M.mirror, M.G, t, r, z, p, init, sc,
                            part_rz);
     } else {
-        hipLaunchKernelGGL(prec_jacobi<T>, dim3(ng), dim3(kPB), 0, c.stream, V, M.minv, r, z, p, init, sc, part_rz);
+        hipLaunchKernelGGL(prec_jacobi<T>, dim3(L.ng), dim3(kPB), 0, c.stream, L, M.minv, r, z, p, init, sc, part_rz);
     }
 }
 
-// One PCG solve of A x = b (b FP64; A, x and the work vectors in T).  kCheck iterations are
-// captured once into a HIP graph and replayed until the device flag says converged.
+// PCG solves of A x = b (b FP64; A, x and the work vectors in T), one per scene in lock step.
+// kCheck iterations are captured once into a HIP graph and replayed until the device flags
+// say that every scene has converged.  Returns the scenes' final scalars.
 template <typename T>
-int pcg_run(Context& c, const T* val, const Prec<T>& M, const double* b, T* x, int x_zero, double rel_tol, int rel_to_r0,
-            int max_iter, double& rr_out, double& bn_out) {
+void pcg_run(Context& c, const Layout& L, int S, const T* val, const Prec<T>& M, const double* b, T* x, int x_zero, double rel_tol,
+             int rel_to_r0, int max_iter, SceneControl ctl, std::vector<PcgScalars>& h) {
     const int V = c.sc.V;
-    const int ng = (int)blocks_for(V, kRows);
+    const int ng = L.ng;
     const int ngd = (int)blocks_for(3 * V, kPB);
     SolveBuffers& w = c.solve;
     T* r = reinterpret_cast<T*>(w.r.p);
@@ -571,30 +680,31 @@ int pcg_run(Context& c, const T* val, const Prec<T>& M, const double* b, T* x, i
     double* part_rr = w.part.p + 2 * ng;
     double* part_bb = w.part.p + 3 * ng;
     PcgScalars* sc = reinterpret_cast<PcgScalars*>(w.scal.p);
-    hipLaunchKernelGGL(pcg_residual<T>, dim3(ng), dim3(kPB), 0, c.stream, V, w.rowptr.p, w.col.p, val, b, x, x_zero, r, part_rr, part_bb);
-    apply_prec<T>(c, M, r, z, p, 1, nullptr, part_rz);
-    hipLaunchKernelGGL(pcg_init_scalars, dim3(1), dim3(kPB), 0, c.stream, part_rz, part_rr, part_bb, ng, rel_tol, rel_to_r0, sc);
+    hipLaunchKernelGGL(pcg_residual<T>, dim3(ng), dim3(kPB), 0, c.stream, L, w.rowptr.p, w.col.p, val, b, x, x_zero, r, part_rr, part_bb);
+    apply_prec<T>(c, L, M, r, z, p, 1, nullptr, part_rz);
+    hipLaunchKernelGGL(pcg_init_scalars, dim3(S), dim3(kPB), 0, c.stream, L, part_rz, part_rr, part_bb, rel_tol, rel_to_r0, ctl, sc);
     CIPC_CHECK_LAUNCH();
-    PcgScalars h = read_scalars(c, sc);
-    if (h.done) { rr_out = h.rr; bn_out = h.bn; return h.iters; }
+    read_scalars(c, sc, S, h);
+    if (all_done(h)) return;
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
     CIPC_HIP(hipStreamBeginCapture(c.stream, hipStreamCaptureModeThreadLocal));
     for (int k = 0; k < kCheck; ++k) {
-        hipLaunchKernelGGL(pcg_spmv<T>, dim3(ng), dim3(kPB), 0, c.stream, V, w.rowptr.p, w.col.p, val, p, q, sc, part_pq);
-        hipLaunchKernelGGL(pcg_update<T>, dim3(ng), dim3(kPB), 0, c.stream, V, part_pq, ng, M.G ? nullptr : M.minv, p, q, x, r, z, sc,
+        hipLaunchKernelGGL(pcg_spmv<T>, dim3(ng), dim3(kPB), 0, c.stream, L, w.rowptr.p, w.col.p, val, p, q, sc, part_pq);
+        hipLaunchKernelGGL(pcg_update<T>, dim3(ng), dim3(kPB), 0, c.stream, L, part_pq, M.G ? nullptr : M.minv, p, q, x, r, z, sc,
                            part_rz, part_rr);
-        if (M.G) apply_prec<T>(c, M, r, z, p, 0, sc, part_rz);
-        hipLaunchKernelGGL(pcg_direction<T>, dim3(ngd), dim3(kPB), 0, c.stream, V, part_rz, ng, z, p, sc);
-        hipLaunchKernelGGL(pcg_scalars, dim3(1), dim3(kPB), 0, c.stream, part_rz, part_rr, ng, sc, max_iter);
+        if (M.G) apply_prec<T>(c, L, M, r, z, p, 0, sc, part_rz);
+        if (L.bl) hipLaunchKernelGGL(pcg_direction_rows<T>, dim3(ng), dim3(kPB), 0, c.stream, L, part_rz, z, p, sc);
+        else hipLaunchKernelGGL(pcg_direction<T>, dim3(ngd), dim3(kPB), 0, c.stream, V, part_rz, ng, z, p, sc);
+        hipLaunchKernelGGL(pcg_scalars, dim3(S), dim3(kPB), 0, c.stream, L, part_rz, part_rr, sc, max_iter, ctl);
     }
     const hipError_t ce = hipStreamEndCapture(c.stream, &graph);
     CIPC_HIP(ce);
     CIPC_HIP(hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0));
     try {
-        while (!h.done) {
+        while (!all_done(h)) {
             CIPC_HIP(hipGraphLaunch(exec, c.stream));
-            h = read_scalars(c, sc);
+            read_scalars(c, sc, S, h);
         }
     } catch (...) {
         (void)hipGraphExecDestroy(exec);
@@ -603,10 +713,14 @@ int pcg_run(Context& c, const T* val, const Prec<T>& M, const double* b, T* x, i
     }
     (void)hipGraphExecDestroy(exec);
     (void)hipGraphDestroy(graph);
-    rr_out = h.rr;
-    bn_out = h.bn;
-    return h.iters;
 }
+
+// Per-scene state of the FP64 refinement.
+struct Refine {
+    int iters = 0, outer = 0;
+    bool finished = false;
+    double rr = 0.0, bn = 0.0;
+};
 }  // namespace
 
 // Full BSR (FP64 and, for the mixed form, FP32) and block-Jacobi inverses of the last assembly.
@@ -658,14 +772,18 @@ void build_fsai(Context& c, bool fp32) {
     }
 }
 
-void linear_solve(Context& c, int precision, int preconditioner, double rel_tol, int max_iter, SolveResult& res) {
+// One solve per scene of the last assembly (one scene: the plain solve).  Each scene runs
+// the same iteration as it would alone: its own scalars, convergence test and refinement.
+void linear_solve_scenes(Context& c, int precision, int preconditioner, double rel_tol, int max_iter, const int* active,
+                         SolveResult* res) {
     Scene& s = c.sc;
-    const int V = s.V, n = 3 * V;
+    const int V = s.V, n = 3 * V, S = s.S;
     SolveBuffers& w = c.solve;
-    const int ng = (int)blocks_for(V, kRows);
+    const Layout L = make_layout(c);
+    const int ng = L.ng;
     w.r.resize((size_t)n); w.z.resize((size_t)n); w.p.resize((size_t)n); w.q.resize((size_t)n); w.t.resize((size_t)n);
     w.part.resize((size_t)4 * ng);
-    w.scal.resize((sizeof(PcgScalars) + 7) / 8);
+    w.scal.resize((size_t)S * ((sizeof(PcgScalars) + 7) / 8));
     w.x.resize((size_t)n);
     build_solve_matrix(c, precision == 1);
     if (preconditioner == 1) build_fsai(c, precision == 1);
@@ -674,51 +792,105 @@ void linear_solve(Context& c, int precision, int preconditioner, double rel_tol,
     Prec<float> Mf;
     if (preconditioner == 0) { Md.minv = w.minv.p; Mf.minv = w.minvf.p; }
     else { Md.G = w.G.p; Md.mirror = w.mirror.p; Mf.G = w.Gf.p; Mf.mirror = w.mirror.p; }
-    res = SolveResult{};
+    // per-scene controls (a batch): participation, inner tolerances and caps, correction mask
+    SceneControl ctl;
+    std::vector<int> act(S, 1);
+    if (active)
+        for (int k = 0; k < S; ++k) act[k] = active[k] ? 1 : 0;
+    if (S > 1) {
+        w.ctl_d.resize((size_t)S);
+        w.ctl_i.resize(3 * (size_t)S);
+        CIPC_HIP(hipMemcpyAsync(w.ctl_i.p, act.data(), S * sizeof(int), hipMemcpyHostToDevice, c.stream));
+        ctl.active = w.ctl_i.p;
+    }
+    for (int k = 0; k < S; ++k) res[k] = SolveResult{};
+    std::vector<PcgScalars> h;
+    hipLaunchKernelGGL(zero_d, dim3(blocks_for(n, 256)), dim3(256), 0, c.stream, n, w.x.p);
     if (precision == 0) {
-        double rr = 0, bn = 0;
-        hipLaunchKernelGGL(zero_d, dim3(blocks_for(n, 256)), dim3(256), 0, c.stream, n, w.x.p);
-        res.iterations = pcg_run<double>(c, w.val.p, Md, c.rhs.p, w.x.p, 1, rel_tol, 0, max_iter, rr, bn);
-        res.rel_residual = bn > 0 ? std::sqrt(rr) / bn : 0.0;
+        pcg_run<double>(c, L, S, w.val.p, Md, c.rhs.p, w.x.p, 1, rel_tol, 0, max_iter, ctl, h);
+        for (int k = 0; k < S; ++k) {
+            res[k].iterations = h[k].iters;
+            res[k].rel_residual = h[k].bn > 0 ? std::sqrt(h[k].rr) / h[k].bn : 0.0;
+        }
     } else {
         // FP64 refinement around FP32 PCG; each inner solve aims just past the remaining
         // reduction (floored at the FP32 reach), so one or two refinements usually suffice
         w.xf.resize((size_t)n); w.res.resize((size_t)n);
-        hipLaunchKernelGGL(zero_d, dim3(blocks_for(n, 256)), dim3(256), 0, c.stream, n, w.x.p);
-        double rr = 0, bn = 0;
-        int iters = 0, outer = 0;
         double* part_rr = w.part.p + 2 * ng;
         double* part_bb = w.part.p + 3 * ng;
-        for (;;) {
-            res.iterations = iters;
+        PcgScalars* sc = reinterpret_cast<PcgScalars*>(w.scal.p);
+        std::vector<Refine> st(S);
+        for (int k = 0; k < S; ++k) st[k].finished = !act[k];
+        std::vector<double> itol(S, 0.0);
+        std::vector<int> icap(S, 0), need_inner(S, 0), add(S, 0);   // host sources of async copies: kept alive
+        for (int round = 0;; ++round) {
             double* rd = w.res.p;
-            hipLaunchKernelGGL(pcg_residual<double>, dim3(ng), dim3(kPB), 0, c.stream, V, w.rowptr.p, w.col.p, w.val.p, c.rhs.p, w.x.p,
-                               outer == 0 ? 1 : 0, rd, part_rr, part_bb);
-            hipLaunchKernelGGL(pcg_init_scalars, dim3(1), dim3(kPB), 0, c.stream, part_rr, part_rr, part_bb, ng, rel_tol, 0,
-                               reinterpret_cast<PcgScalars*>(w.scal.p));
+            hipLaunchKernelGGL(pcg_residual<double>, dim3(ng), dim3(kPB), 0, c.stream, L, w.rowptr.p, w.col.p, w.val.p, c.rhs.p, w.x.p,
+                               round == 0 ? 1 : 0, rd, part_rr, part_bb);
+            hipLaunchKernelGGL(pcg_init_scalars, dim3(S), dim3(kPB), 0, c.stream, L, part_rr, part_rr, part_bb, rel_tol, 0, SceneControl{},
+                               sc);
             CIPC_CHECK_LAUNCH();
-            const PcgScalars h = read_scalars(c, reinterpret_cast<PcgScalars*>(w.scal.p));
-            rr = h.rr; bn = h.bn;
-            if (!(std::sqrt(rr) > rel_tol * bn) || iters >= max_iter) break;
-            const double need = rel_tol * bn / std::sqrt(rr);      // reduction still needed
-            const double inner_tol = std::max(0.3 * need, 2e-6);
+            read_scalars(c, sc, S, h);
+            bool any = false;
+            for (int k = 0; k < S; ++k) {
+                need_inner[k] = 0;
+                if (st[k].finished) continue;
+                st[k].rr = h[k].rr; st[k].bn = h[k].bn;
+                if (!(std::sqrt(h[k].rr) > rel_tol * h[k].bn) || st[k].iters >= max_iter) { st[k].finished = true; continue; }
+                const double need = rel_tol * h[k].bn / std::sqrt(h[k].rr);      // reduction still needed
+                itol[k] = std::max(0.3 * need, 2e-6);
+                icap[k] = max_iter - st[k].iters;
+                need_inner[k] = 1;
+                any = true;
+            }
+            if (!any) break;
+            SceneControl ictl;
+            if (S > 1) {
+                CIPC_HIP(hipMemcpyAsync(w.ctl_d.p, itol.data(), S * sizeof(double), hipMemcpyHostToDevice, c.stream));
+                CIPC_HIP(hipMemcpyAsync(w.ctl_i.p + S, icap.data(), S * sizeof(int), hipMemcpyHostToDevice, c.stream));
+                CIPC_HIP(hipMemcpyAsync(w.ctl_i.p, need_inner.data(), S * sizeof(int), hipMemcpyHostToDevice, c.stream));
+                ictl.rel_tol = w.ctl_d.p;
+                ictl.max_iter = w.ctl_i.p + S;
+                ictl.active = w.ctl_i.p;
+            }
             hipLaunchKernelGGL(zero_f, dim3(blocks_for(n, 256)), dim3(256), 0, c.stream, n, w.xf.p);
-            double irr = 0, ibn = 0;
-            const int it = pcg_run<float>(c, w.valf.p, Mf, rd, w.xf.p, 1, inner_tol, 1, max_iter - iters, irr, ibn);
-            if (it == 0 || outer >= kMaxRefine) break;   // no progress possible (breakdown) or refinement cap
-            iters += it;
-            hipLaunchKernelGGL(add_correction, dim3(blocks_for(n, 256)), dim3(256), 0, c.stream, n, w.xf.p, w.x.p);
+            std::vector<PcgScalars> hi;
+            pcg_run<float>(c, L, S, w.valf.p, Mf, rd, w.xf.p, 1, itol[0], 1, icap[0], ictl, hi);
+            // scenes that made no progress (breakdown) or hit the refinement cap stop here,
+            // without their correction
+            std::fill(add.begin(), add.end(), 0);
+            for (int k = 0; k < S; ++k) {
+                if (!need_inner[k]) continue;
+                const int it = hi[k].iters;
+                if (it == 0 || st[k].outer >= kMaxRefine) { st[k].finished = true; continue; }
+                st[k].iters += it;
+                ++st[k].outer;
+                add[k] = 1;
+            }
+            if (S > 1) {
+                CIPC_HIP(hipMemcpyAsync(w.ctl_i.p + 2 * S, add.data(), S * sizeof(int), hipMemcpyHostToDevice, c.stream));
+                hipLaunchKernelGGL(add_correction, dim3(blocks_for(n, 256)), dim3(256), 0, c.stream, n, w.xf.p, w.x.p, s.vscene.p,
+                                   w.ctl_i.p + 2 * S);
+            } else if (add[0]) {
+                hipLaunchKernelGGL(add_correction, dim3(blocks_for(n, 256)), dim3(256), 0, c.stream, n, w.xf.p, w.x.p, nullptr, nullptr);
+            }
             CIPC_CHECK_LAUNCH();
-            ++outer;
         }
-        res.iterations = iters;
-        res.refinements = outer;
-        res.rel_residual = bn > 0 ? std::sqrt(rr) / bn : 0.0;
+        for (int k = 0; k < S; ++k) {
+            res[k].iterations = st[k].iters;
+            res[k].refinements = st[k].outer;
+            res[k].rel_residual = st[k].bn > 0 ? std::sqrt(st[k].rr) / st[k].bn : 0.0;
+        }
     }
     // the Newton direction for the step bound and line search
     s.dir.resize((size_t)n);
     CIPC_HIP(hipMemcpyAsync(s.dir.p, w.x.p, (size_t)n * sizeof(double), hipMemcpyDeviceToDevice, c.stream));
     CIPC_HIP(hipStreamSynchronize(c.stream));
+}
+
+void linear_solve(Context& c, int precision, int preconditioner, double rel_tol, int max_iter, SolveResult& res) {
+    if (c.sc.S != 1) throw Error(CIPC_ERR_LOGIC, "a batch is solved per scene");
+    linear_solve_scenes(c, precision, preconditioner, rel_tol, max_iter, nullptr, &res);
 }
 
 }  // namespace cipc

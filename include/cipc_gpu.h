@@ -245,6 +245,46 @@ int cipc_gpu_is_intersected(cipc_gpu_context* ctx, int* intersected);
  * active set, assembly, step bound, energy (the unit BASELINE-style timing uses). */
 int cipc_gpu_newton_iteration(cipc_gpu_context* ctx, double kappa, cipc_iteration_timing* timing);
 
+/* Batched scenes.  No single reference interface: the reference runs independent scenes
+ * as separate FEMSimulator / IPCSolverContext instances (IPCSolver.h:46-61), each with
+ * its own mesh3D, contact sets, barrier stiffness, step and solve.  A batch context holds
+ * n_scenes such scenes in one set of device arrays (vertices, tets and surface primitives
+ * of scene k after those of scene k-1, indices shifted) and evaluates each one exactly as
+ * a context holding that scene alone would: contacts never join two scenes, and step
+ * bounds, energies and linear solves are per scene.  The scenes are tet meshes and share
+ * dt, hhat, fhat, drag, friction and the ground; the Lame parameters (length_rate,
+ * volume_rate) and the masses are per scene.  Scene-agnostic entry points (set_state,
+ * get_positions, set_direction, active_set, get_active_set, get_gradient, get_csr,
+ * save_positions, step_forward, is_intersected, assemble and friction_update with one
+ * stiffness for every scene) act on the whole batch; the per-scene ones below replace
+ * energy, step_bound, solve and newton_iteration, which refuse a batch
+ * (CIPC_ERR_LOGIC), as do kappa_gradients and close_constraints. */
+int cipc_gpu_upload_batch(cipc_gpu_context* ctx, const cipc_scene_desc* const* scenes, int n_scenes);
+/* n_scenes; vertex_offsets and tet_offsets (n_scenes + 1 each) may be NULL. */
+int cipc_gpu_batch_size(cipc_gpu_context* ctx, int* n_scenes, int* vertex_offsets, int* tet_offsets);
+/* As cipc_gpu_assemble with kappa[k] for scene k; collisions (n_scenes, may be NULL) per scene. */
+int cipc_gpu_batch_assemble(cipc_gpu_context* ctx, const double* kappa, int64_t* nnz, int* collisions);
+/* As cipc_gpu_friction_update with kappa[k] for scene k. */
+int cipc_gpu_batch_friction_update(cipc_gpu_context* ctx, const double* kappa);
+/* As cipc_gpu_energy / cipc_gpu_energy_terms per scene: energy (n_scenes) and terms
+ * (9 n_scenes), either may be NULL. */
+int cipc_gpu_batch_energy(cipc_gpu_context* ctx, const double* kappa, double* energy, double* terms);
+/* As cipc_gpu_step_bound per scene: out[5k .. 5k+4]. */
+int cipc_gpu_batch_step_bound(cipc_gpu_context* ctx, double* out);
+/* x = x0 - alpha[k] p on scene k's vertices (a converged scene takes alpha 0). */
+int cipc_gpu_batch_step_forward(cipc_gpu_context* ctx, const double* alpha);
+/* r[k] = 1 when scene k is intersected (cipc_gpu_is_intersected per scene). */
+int cipc_gpu_batch_is_intersected(cipc_gpu_context* ctx, int* r);
+/* As cipc_gpu_solve_ex per scene, all scenes in lock step: active (n_scenes, NULL = all)
+ * masks converged scenes, whose direction is left zero; stats (n_scenes, may be NULL).
+ * CIPC_ERR_RUNTIME names the first active scene that misses the tolerance. */
+int cipc_gpu_batch_solve(cipc_gpu_context* ctx, int precision, int preconditioner, double rel_tol, int max_iter, const int* active,
+                         double* direction, cipc_solve_stats* stats);
+/* As cipc_gpu_newton_iteration with kappa[k] for scene k; alpha and energy (n_scenes
+ * each, may be NULL) per scene; timing->alpha is their minimum, timing->energy their sum. */
+int cipc_gpu_batch_newton_iteration(cipc_gpu_context* ctx, const double* kappa, cipc_iteration_timing* timing, double* alpha,
+                                    double* energy);
+
 /* Element-level evaluation of n independent inputs (host arrays in, host arrays out);
  * used to check the device kernels element by element.
  *  snh:      F (9n, column-major) -> PK1 (9n), projected Hessian (81n row-major), energy density (n)

@@ -179,3 +179,18 @@ def test_step_bound_leaves_a_feasible_step(O):
     assert not m.is_intersected()
     x = m.positions()
     assert np.all(x[s.surf_verts, 1] > -1.0)
+
+
+def test_batched_cubes_share_the_batch_scalars():
+    """The batched workload: cell counts in [8, 24] per axis, per-scene materials, drops of
+    0.02-0.2 m, and the hhat / fhat / dtol that a batch requires to be shared."""
+    from cipc import scenes
+    b = scenes.batched_cubes(6, seed=2)
+    assert len(b) == 6
+    for s in b:
+        assert 5 * 8 ** 3 <= s.T <= 5 * 24 ** 3
+        assert s.hhat == b[0].hhat and s.fhat == b[0].fhat and s.dtol == b[0].dtol
+        gap = s.x[:, 1].min() - scenes.GROUND_Y
+        assert 0.02 - 1e-12 <= gap <= 0.2 + 1e-12
+        assert 1e4 <= s.material.youngs <= 1e6 and 0.3 <= s.material.poisson <= 0.45
+    assert len({s.material.youngs for s in b}) == 6
