@@ -141,7 +141,9 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--cpu-budget", type=float, default=15.0)
     ap.add_argument("--no-pcg", action="store_true", help="skip the linear-solve leg")
-    ap.add_argument("--traffic", default=None, help="JSON file with measured HBM bytes per launch of the elastic stage")
+    ap.add_argument("--traffic", default=os.path.join(ROOT, "profiles", "elastic_traffic_4x20cubes.json"),
+                    help="JSON file with the measured HBM bytes per launch of the elastic stage (rocprofv3 FETCH_SIZE + "
+                         "WRITE_SIZE of --traffic-probe, scaled by a copy of known size)")
     ap.add_argument("--traffic-probe", action="store_true", help="run the HBM-counter workload instead of the benchmark")
     args = ap.parse_args()
     if args.traffic_probe:

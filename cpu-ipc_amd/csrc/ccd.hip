@@ -103,18 +103,26 @@ __global__ void partial_ccd_kernel(int n, const int2* __restrict__ pairs, const 
         const int2 pr = pairs[i];
         int v0;
         if (pr.x < 0) {
-            const int pt = sv[-pr.x - 1];
-            v0 = pt;
-            const int4 f = faces[pr.y];
-            a = accd_pt(ld3(x, pt), ld3(x, f.x), ld3(x, f.y), ld3(x, f.z), -ld3(p, pt), -ld3(p, f.x), -ld3(p, f.y), -ld3(p, f.z), eta,
-                        0.0);
+            v0 = sv[-pr.x - 1];
         } else {
-            const int2 e1 = se[pr.x], e2 = se[pr.y];
-            v0 = e1.x;
-            a = accd_ee(ld3(x, e1.x), ld3(x, e1.y), ld3(x, e2.x), ld3(x, e2.y), -ld3(p, e1.x), -ld3(p, e1.y), -ld3(p, e2.x),
-                        -ld3(p, e2.y), eta, 0.0);
+            v0 = se[pr.x].x;
         }
         scene = scene_of(vs, v0);
+        // the merge takes min(ground step, partial CCD): a pair at or past the ground step
+        // cannot change it (slot 0 is final: the ground kernel ran before this one)
+        double g = bits_to_double(sl[kSlots * scene]);
+        g = fmin(1.0, g);
+        const double cap = g <= 0.0 ? 1.0 : g;
+        if (pr.x < 0) {
+            const int pt = v0;
+            const int4 f = faces[pr.y];
+            a = accd_pt(ld3(x, pt), ld3(x, f.x), ld3(x, f.y), ld3(x, f.z), -ld3(p, pt), -ld3(p, f.x), -ld3(p, f.y), -ld3(p, f.z), eta,
+                        0.0, cap);
+        } else {
+            const int2 e1 = se[pr.x], e2 = se[pr.y];
+            a = accd_ee(ld3(x, e1.x), ld3(x, e1.y), ld3(x, e2.x), ld3(x, e2.y), -ld3(p, e1.x), -ld3(p, e1.y), -ld3(p, e2.x),
+                        -ld3(p, e2.y), eta, 0.0, cap);
+        }
     }
     scene_atomic<false>(sl, scene, 2, a);
 }
@@ -148,7 +156,9 @@ __global__ __launch_bounds__(64) void full_ccd_pt_kernel(int Vs, const int* __re
             const d3 p0 = ld3(p, fv.x), p1 = ld3(p, fv.y), p2 = ld3(p, fv.z);
             const Box fb = box_union(box_union(swept_pt(x0, p0, a), swept_pt(x1, p1, a)), swept_pt(x2, p2, a));
             if (!overlaps(pb, fb, 0.0)) return;
-            const double la = accd_pt(X, x0, x1, x2, -P, -p0, -p1, -p2, eta, 0.0);
+            // capped at the lane's best so far, itself at most the seeded step a: the
+            // minimum over all pairs is unchanged
+            const double la = accd_pt(X, x0, x1, x2, -P, -p0, -p1, -p2, eta, 0.0, fmin(best, a));
             best = fmin(la, best);
         }, [] {});
     }
@@ -184,7 +194,7 @@ __global__ __launch_bounds__(64) void full_ccd_ee_kernel(int Es, const int* __re
             const d3 i0 = ld3(x, I.x), i1 = ld3(x, I.y), pi0 = ld3(p, I.x), pi1 = ld3(p, I.y);
             const Box b2 = box_union(swept_pt(i0, pi0, a), swept_pt(i1, pi1, a));
             if (!overlaps(b1, b2, 0.0)) return;
-            const double la = accd_ee(i0, i1, j0, j1, -pi0, -pi1, -pj0, -pj1, eta, 0.0);
+            const double la = accd_ee(i0, i1, j0, j1, -pi0, -pi1, -pj0, -pj1, eta, 0.0, fmin(best, a));
             best = fmin(la, best);
         }, [] {});
     }

@@ -611,8 +611,11 @@ __device__ inline double pt_unclassified(d3 p, d3 t0, d3 t1, d3 t2) {
     }
 }
 __device__ inline double clamp01(double toc) { return fmax(0.0, fmin(toc, 1.0)); }
-// edge_edge_ccd (AdditiveCCD.cpp:221-279)
-__device__ inline double accd_ee(d3 ea0, d3 ea1, d3 eb0, d3 eb1, d3 dea0, d3 dea1, d3 deb0, d3 deb1, double eta, double th) {
+// edge_edge_ccd (AdditiveCCD.cpp:221-279).  `cap`: the caller only needs the result when it
+// is below cap (a step bound already known); the advance stops once it reaches cap and
+// returns a value >= cap.  Results below cap are those of the uncapped loop.
+__device__ inline double accd_ee(d3 ea0, d3 ea1, d3 eb0, d3 eb1, d3 dea0, d3 dea1, d3 deb0, d3 deb1, double eta, double th,
+                                 double cap = __builtin_inf()) {
     const d3 mov = (dea0 + dea1 + deb0 + deb1) / 4;
     dea0 = dea0 - mov; dea1 = dea1 - mov; deb0 = deb0 - mov; deb1 = deb1 - mov;
     const double maxd = sqrt(fmax(sqn(dea0), sqn(dea1))) + sqrt(fmax(sqn(deb0), sqn(deb1)));
@@ -640,11 +643,13 @@ __device__ inline double accd_ee(d3 ea0, d3 ea1, d3 eb0, d3 eb1, d3 dea0, d3 dea
         if (toc != 0.0 && (dF / (dc + th) < gap)) break;
         toc += tl;
         if (toc > 1.0) return 1.0;
+        if (toc >= cap) return toc;
     }
     return clamp01(toc);
 }
-// point_triangle_ccd (AdditiveCCD.cpp:281-329)
-__device__ inline double accd_pt(d3 p, d3 t0, d3 t1, d3 t2, d3 dp, d3 dt0, d3 dt1, d3 dt2, double eta, double th) {
+// point_triangle_ccd (AdditiveCCD.cpp:281-329); `cap` as for accd_ee
+__device__ inline double accd_pt(d3 p, d3 t0, d3 t1, d3 t2, d3 dp, d3 dt0, d3 dt1, d3 dt2, double eta, double th,
+                                 double cap = __builtin_inf()) {
     const d3 mov = (dt0 + dt1 + dt2 + dp) / 4;
     dt0 = dt0 - mov; dt1 = dt1 - mov; dt2 = dt2 - mov; dp = dp - mov;
     const double mx = fmax(fmax(sqn(dt0), sqn(dt1)), sqn(dt2));
@@ -663,6 +668,7 @@ __device__ inline double accd_pt(d3 p, d3 t0, d3 t1, d3 t2, d3 dp, d3 dt0, d3 dt
         if (toc != 0.0 && ((d2 - th * th) / (dc + th) < gap)) break;
         toc += tl;
         if (toc > 1.0) return 1.0;
+        if (toc >= cap) return toc;
     }
     return clamp01(toc);
 }
