@@ -108,6 +108,41 @@ def pcg_leg(ctx, kappa, tol=1e-6, max_iter=10000, repeats=3):
     return out
 
 
+def batch_leg(n_scenes=256, steps=5, seed=0):
+    """The batched workload: n independent lattice cubes (8^3..24^3 cells, per-scene E, nu,
+    drop height) in one batch context, each at its first Newton iteration (frozen state,
+    probe direction).  Times the batched Newton iteration (HIP events) and one per-scene
+    lock-step solve (FP32 PCG with FP64 refinement, block FSAI, 1e-6)."""
+    import numpy as np
+    from cipc import gpu, scenes
+    t0 = time.perf_counter()
+    sc = scenes.batched_cubes(n_scenes, seed=seed)
+    probes = [scenes.newton_probe_state(s) for s in sc]
+    kappa = np.array([scenes.suggest_kappa(s.hhat, s.bbox_diag2, s.mean_mass) for s in sc])
+    b = gpu.BatchContext(sc, xtildes=[xt for xt, _ in probes])
+    b.set_state(V_prev=np.concatenate([s.x - s.velocities * s.material.dt for s in sc]))
+    b.active_set()
+    b.friction_update(kappa)
+    b.set_direction(np.concatenate([p for _, p in probes]))
+    setup_s = time.perf_counter() - t0
+    b.newton_iteration(kappa)
+    it = [b.newton_iteration(kappa) for _ in range(steps)]
+    ms = float(np.mean([t["total_ms"] for t in it]))
+    b.assemble(kappa)
+    _, st = b.solve(precision=1, rel_tol=1e-6, preconditioner=1)
+    solve_ms = st[0]["solve_ms"]
+    iters = [s["iterations"] for s in st]
+    out = {"scenes": n_scenes, "V": int(b.V), "T": int(sum(s.T for s in sc)), "newton_iteration_ms": round(ms, 3),
+           "scene_iterations_per_s": round(n_scenes / (ms * 1e-3), 1),
+           "stages_ms": {k: round(float(np.mean([t[k] for t in it])), 3) for k in
+                         ("broadphase_ms", "assembly_ms", "elastic_ms", "ccd_ms", "energy_ms")},
+           "contacts": int(it[-1]["contacts"]), "min_alpha": float(it[-1]["alpha"]),
+           "solve_mixed_fsai_ms": round(solve_ms, 3), "solve_iterations_min_max": [int(min(iters)), int(max(iters))],
+           "setup_s": round(setup_s, 1)}
+    b.close()
+    return out
+
+
 def traffic_probe(n=5):
     """Workload for the HBM-traffic counters of the elastic stage: n Newton iterations of the
     frozen state (each launches the elastic kernels once), then one 256 MiB device-to-device
@@ -145,6 +180,7 @@ def main():
                     help="JSON file with the measured HBM bytes per launch of the elastic stage (rocprofv3 FETCH_SIZE + "
                          "WRITE_SIZE of --traffic-probe, scaled by a copy of known size)")
     ap.add_argument("--traffic-probe", action="store_true", help="run the HBM-counter workload instead of the benchmark")
+    ap.add_argument("--batch", type=int, default=0, help="also time a batch of this many independent scenes (256: the batched workload)")
     args = ap.parse_args()
     if args.traffic_probe:
         traffic_probe()
@@ -207,6 +243,8 @@ def main():
     }
     if pcg is not None:
         out["pcg"] = pcg
+    if args.batch > 0:
+        out["batch"] = batch_leg(args.batch)
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(scene, xt, vprev, p, kappa, budget_s=args.cpu_budget)
     print(json.dumps(out))

@@ -237,7 +237,7 @@ static void merge_bounds(const Scene& s, const unsigned long long* h, bool any_c
     out[0] = merged; out[1] = aground; out[2] = partial; out[3] = acfl; out[4] = full;
 }
 
-void step_bound_scenes(Context& c, double* out) {
+void step_bound_launch(Context& c) {
     Scene& s = c.sc;
     const int S = s.S;
     const int* vs = scene_of_vertex(s);
@@ -269,10 +269,17 @@ void step_bound_scenes(Context& c, double* out) {
                            s.se.p, s.bt.p, s.x.p, s.dir.p, etv, et.prim.p, sl.p, eta, vs);
     }
     CIPC_CHECK_LAUNCH();
-    std::vector<unsigned long long> h((size_t)kSlots * S);
-    CIPC_HIP(hipMemcpyAsync(h.data(), sl.p, h.size() * sizeof(unsigned long long), hipMemcpyDeviceToHost, c.stream));
+    c.pin_slots.reserve((size_t)kSlots * S);
+    CIPC_HIP(hipMemcpyAsync(c.pin_slots.p, sl.p, (size_t)kSlots * S * sizeof(unsigned long long), hipMemcpyDeviceToHost, c.stream));
+}
+void step_bound_finish(Context& c, double* out) {
+    const Scene& s = c.sc;
+    for (int k = 0; k < s.S; ++k) merge_bounds(s, c.pin_slots.p + (size_t)kSlots * k, c.n_ccd > 0, out + 5 * (size_t)k);
+}
+void step_bound_scenes(Context& c, double* out) {
+    step_bound_launch(c);
     CIPC_HIP(hipStreamSynchronize(c.stream));
-    for (int k = 0; k < S; ++k) merge_bounds(s, h.data() + (size_t)kSlots * k, c.n_ccd > 0, out + 5 * (size_t)k);
+    step_bound_finish(c, out);
 }
 
 void step_bound(Context& c, double out[5]) {

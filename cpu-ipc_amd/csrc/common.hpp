@@ -56,6 +56,28 @@ struct DBuf {
     const T* data() const { return p; }
 };
 
+// Grown-on-demand pinned host buffer: the destination of asynchronous device-to-host copies
+// that the host reads after a later synchronisation of the stream (so that one
+// synchronisation serves several results).  Grown only while no copy into it is pending.
+template <typename T>
+struct PinnedBuf {
+    T* p = nullptr;
+    size_t cap = 0;
+    PinnedBuf() = default;
+    PinnedBuf(const PinnedBuf&) = delete;
+    PinnedBuf& operator=(const PinnedBuf&) = delete;
+    ~PinnedBuf() {
+        if (p) (void)hipHostFree(p);
+    }
+    void reserve(size_t want) {
+        if (want <= cap) return;
+        if (p) CIPC_HIP(hipHostFree(p));
+        p = nullptr;
+        CIPC_HIP(hipHostMalloc(reinterpret_cast<void**>(&p), want * sizeof(T), hipHostMallocDefault));
+        cap = want;
+    }
+};
+
 inline unsigned blocks_for(size_t n, unsigned bs) { return (unsigned)((n + bs - 1) / bs); }
 
 // ------------------------------------------------------------------------------------

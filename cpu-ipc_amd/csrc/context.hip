@@ -496,6 +496,7 @@ static void assemble_impl(cipc_gpu_context* c, double kappa, const double* ks = 
         c->collisions = (int)n;
     }
     CIPC_HIP(hipStreamSynchronize(c->stream));
+    settle_nnz(*c);
 }
 
 int cipc_gpu_assemble(cipc_gpu_context* c, double kappa, int64_t* nnz, int* collisions) {
@@ -523,6 +524,8 @@ int cipc_gpu_get_csr(cipc_gpu_context* c, int* outer, int* inner, double* values
         bind(*c);
         const hipStream_t st = c->stream;
         const size_t n = 3 * (size_t)c->sc.V;
+        CIPC_HIP(hipStreamSynchronize(st));
+        settle_nnz(*c);
         if (outer) CIPC_HIP(hipMemcpyAsync(outer, c->outer.p, (n + 1) * sizeof(int), hipMemcpyDeviceToHost, st));
         if (inner && c->nnz) CIPC_HIP(hipMemcpyAsync(inner, c->inner.p, c->nnz * sizeof(int), hipMemcpyDeviceToHost, st));
         if (values && c->nnz) CIPC_HIP(hipMemcpyAsync(values, c->val.p, c->nnz * sizeof(double), hipMemcpyDeviceToHost, st));
@@ -827,11 +830,13 @@ int cipc_gpu_batch_newton_iteration(cipc_gpu_context* c, const double* kappa, ci
         c->assembled = true;
         CIPC_HIP(hipEventRecord(ev[2], c->stream));
         std::vector<double> sb(5 * (size_t)S), et(9 * (size_t)S);
-        step_bound_scenes(*c, sb.data());
+        step_bound_launch(*c);
         CIPC_HIP(hipEventRecord(ev[3], c->stream));
-        energy_terms_scenes(*c, et.data());
+        energy_terms_scenes(*c, et.data());          // synchronises the stream
         CIPC_HIP(hipEventRecord(ev[4], c->stream));
         CIPC_HIP(hipEventSynchronize(ev[4]));
+        settle_nnz(*c);
+        step_bound_finish(*c, sb.data());
         float ms[4];
         for (int i = 0; i < 4; ++i) CIPC_HIP(hipEventElapsedTime(&ms[i], ev[i], ev[i + 1]));
         float tot = 0, el = 0;
@@ -935,13 +940,16 @@ int cipc_gpu_newton_iteration(cipc_gpu_context* c, double kappa, cipc_iteration_
         elastic_join(c);
         finalize_csr(*c);
         CIPC_HIP(hipEventRecord(ev[2], c->stream));
-        double sb[5];
-        step_bound(*c, sb);
+        // step bound and energy with one synchronisation: their reads are asynchronous
+        step_bound_launch(*c);
         CIPC_HIP(hipEventRecord(ev[3], c->stream));
-        double et[9];
-        energy_terms(*c, kappa, et);
+        energy_terms_launch(*c);
         CIPC_HIP(hipEventRecord(ev[4], c->stream));
         CIPC_HIP(hipEventSynchronize(ev[4]));
+        settle_nnz(*c);
+        double sb[5], et[9];
+        step_bound_finish(*c, sb);
+        energy_terms_finish(*c, et);
         float ms[4];
         for (int i = 0; i < 4; ++i) CIPC_HIP(hipEventElapsedTime(&ms[i], ev[i], ev[i + 1]));
         float tot = 0, el = 0;

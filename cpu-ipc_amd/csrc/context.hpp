@@ -147,6 +147,11 @@ struct Context {
     DBuf<int> outer, inner;
     DBuf<double> val, rhs;
     int64_t nnz = 0;
+    bool nnz_pending = false;  // nnz is in pin_nnz once the stream has been synchronised
+    // pinned destinations of the asynchronous reads that one final synchronisation collects
+    PinnedBuf<int> pin_nnz;
+    PinnedBuf<unsigned long long> pin_slots;
+    PinnedBuf<double> pin_energy;
     bool assembled = false;    // a system exists for the linear solve
     SolveBuffers solve;
     int collisions = 0;
@@ -199,10 +204,21 @@ void scene_intersections(Context& c, int* out);
 // result, swept step)
 constexpr int kSlots = 8;
 void step_bound(Context& c, double out[5]);
+// step_bound in two halves: the kernels and an asynchronous read of the slots, then (after
+// the caller has synchronised the stream) the merge
+void step_bound_launch(Context& c);
+void step_bound_finish(Context& c, double* out);
 // per scene: out[5 k .. 5 k + 4] as step_bound's five values
 void step_bound_scenes(Context& c, double* out);
 // energy.hip
 void energy_terms(Context& c, double kappa, double out[9]);
+// energy_terms in two halves, as step_bound_launch / step_bound_finish
+void energy_terms_launch(Context& c);
+void energy_terms_finish(Context& c, double out[9]);
+// after a synchronisation of the stream: the CSR nonzero count of the last finalize_csr
+inline void settle_nnz(Context& c) {
+    if (c.nnz_pending) { c.nnz = *c.pin_nnz.p; c.nnz_pending = false; }
+}
 void cloth_energy_partials(Context& c, double* part_bw, double* part_bend);
 double total_energy(const Context& c, double kappa, const double t[9]);
 // per scene: out[9 k .. 9 k + 8] as energy_terms' nine values

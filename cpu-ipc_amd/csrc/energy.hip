@@ -140,8 +140,14 @@ __global__ __launch_bounds__(kEB) void sum_partials(const double* __restrict__ p
 
 void energy_terms(Context& c, double kappa, double out[9]) {
     (void)kappa;
+    if (c.sc.S != 1) throw Error(CIPC_ERR_LOGIC, "a batch has one energy per scene");
+    energy_terms_launch(c);
+    CIPC_HIP(hipStreamSynchronize(c.stream));
+    energy_terms_finish(c, out);
+}
+
+void energy_terms_launch(Context& c) {
     Scene& s = c.sc;
-    if (s.S != 1) throw Error(CIPC_ERR_LOGIC, "a batch has one energy per scene");
     static_assert(kEG == kEB, "partials are summed by one work-group");
     DBuf<double>& part = c.dscratch;
     part.resize((size_t)11 * kEG);
@@ -163,9 +169,13 @@ void energy_terms(Context& c, double kappa, double out[9]) {
     hipLaunchKernelGGL(e_soft, dim3(kEG), dim3(kEB), 0, c.stream, c.n_soft, c.soft_idx.p, c.soft_t.p, c.soft_w, s.x.p, P + 9 * kEG);
     hipLaunchKernelGGL(sum_partials, dim3(1), dim3(kEB), 0, c.stream, P, 10, P + 10 * kEG);
     CIPC_CHECK_LAUNCH();
-    double t[10];
-    CIPC_HIP(hipMemcpyAsync(t, P + 10 * kEG, 10 * sizeof(double), hipMemcpyDeviceToHost, c.stream));
-    CIPC_HIP(hipStreamSynchronize(c.stream));
+    c.pin_energy.reserve(10);
+    CIPC_HIP(hipMemcpyAsync(c.pin_energy.p, P + 10 * kEG, 10 * sizeof(double), hipMemcpyDeviceToHost, c.stream));
+}
+
+void energy_terms_finish(Context& c, double out[9]) {
+    const Scene& s = c.sc;
+    const double* t = c.pin_energy.p;
     for (int k = 0; k < 5; ++k) out[k] = t[k];
     out[5] = s.fric * t[5] + t[6];   // Friction::energy: mu * sum_self + sum_ground (Friction.cpp:521-544)
     out[6] = t[7];

@@ -295,11 +295,12 @@ void finalize_csr(Context& c) {
     c.outer.resize((size_t)3 * V + 1);
     hipLaunchKernelGGL(column_starts, dim3(blocks_for(3 * V + 1, 256)), dim3(256), 0, c.stream, V, c.mptr.p, c.ppos.p, nm_ptr, c.outer.p);
     CIPC_CHECK_LAUNCH();
-    int nnz = 0;
-    CIPC_HIP(hipMemcpyAsync(&nnz, c.outer.p + 3 * V, sizeof(int), hipMemcpyDeviceToHost, c.stream));
-    CIPC_HIP(hipStreamSynchronize(c.stream));
-    c.nnz = nnz;
-    c.inner.resize(std::max(nnz, 1)); c.val.resize(std::max(nnz, 1));
+    // the count is read with the caller's next synchronisation; the entries are sized by
+    // their bound (9 per merged block)
+    c.pin_nnz.reserve(1);
+    CIPC_HIP(hipMemcpyAsync(c.pin_nnz.p, c.outer.p + 3 * V, sizeof(int), hipMemcpyDeviceToHost, c.stream));
+    c.nnz_pending = true;
+    c.inner.resize(9 * mcap); c.val.resize(9 * mcap);
     hipLaunchKernelGGL(pair_fill, dim3(blocks_for(3 * mcap, 256)), dim3(256), 0, c.stream, nm_ptr, c.mptr.p, c.mrow.p, c.mcol.p,
                        c.mval.p, c.mflag.p, c.ppos.p, c.inner.p, c.val.p);
     CIPC_CHECK_LAUNCH();
