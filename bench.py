@@ -212,6 +212,17 @@ def main():
     mean = {k: float(np.mean([s[k] for s in stages])) for k in
             ("broadphase_ms", "assembly_ms", "elastic_ms", "ccd_ms", "energy_ms", "total_ms")}
     last = stages[-1]
+    # the mixed form: FP32 elastic Hessian blocks, FP64 barriers / friction / gradient
+    ctx.set_hessian_precision(1)
+    for _ in range(args.warmup):
+        ctx.newton_iteration(kappa)
+    t1 = time.perf_counter()
+    mixed = [ctx.newton_iteration(kappa) for _ in range(args.steps)]
+    mixed_wall = (time.perf_counter() - t1) * 1e3 / args.steps
+    ctx.set_hessian_precision(0)
+    mixed_out = {"ms_per_iter": round(mixed_wall, 4),
+                 "stages_ms": {k: round(float(np.mean([s[k] for s in mixed])), 4) for k in
+                               ("broadphase_ms", "assembly_ms", "elastic_ms", "ccd_ms", "energy_ms", "total_ms")}}
     pcg = pcg_leg(ctx, kappa) if not args.no_pcg else None
     ctx.close()
 
@@ -241,6 +252,7 @@ def main():
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
                      "algorithmic_bytes": int(bytes_el), "elastic_nnz": int(nnz_el)},
     }
+    out["mixed_fp32_hessian"] = mixed_out
     if pcg is not None:
         out["pcg"] = pcg
     if args.batch > 0:

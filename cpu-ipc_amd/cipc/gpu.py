@@ -32,7 +32,8 @@ OK, ERR_INVALID_ARGUMENT, ERR_RUNTIME, ERR_LOGIC, ERR_OVERFLOW, ERR_HIP, ERR_NO_
 EXPORTS = (
     "cipc_gpu_last_error", "cipc_gpu_context_create", "cipc_gpu_context_destroy", "cipc_gpu_upload_scene",
     "cipc_gpu_set_state", "cipc_gpu_get_positions", "cipc_gpu_set_direction", "cipc_gpu_active_set",
-    "cipc_gpu_get_active_set", "cipc_gpu_set_active_set", "cipc_gpu_assemble", "cipc_gpu_get_gradient",
+    "cipc_gpu_get_active_set", "cipc_gpu_set_active_set", "cipc_gpu_assemble", "cipc_gpu_set_hessian_precision",
+    "cipc_gpu_get_gradient",
     "cipc_gpu_get_csr", "cipc_gpu_energy", "cipc_gpu_energy_terms", "cipc_gpu_step_bound",
     "cipc_gpu_friction_update", "cipc_gpu_get_friction", "cipc_gpu_set_soft_targets", "cipc_gpu_kappa_gradients",
     "cipc_gpu_close_constraints", "cipc_gpu_clear_close_constraints", "cipc_gpu_preconditioner_param_count",
@@ -114,6 +115,7 @@ def load(path: str = LIB_PATH):
         "cipc_gpu_get_active_set": [C.c_void_p, _ip, _ip, _ip, _ip, _ip],
         "cipc_gpu_set_active_set": [C.c_void_p, _i64p, _ip, _ip, _ip, _ip, _ip],
         "cipc_gpu_assemble": [C.c_void_p, C.c_double, _i64p, _ip],
+        "cipc_gpu_set_hessian_precision": [C.c_void_p, C.c_int],
         "cipc_gpu_get_gradient": [C.c_void_p, _dp],
         "cipc_gpu_get_csr": [C.c_void_p, _ip, _ip, _dp, _dp],
         "cipc_gpu_energy": [C.c_void_p, C.c_double, _dp],
@@ -281,6 +283,11 @@ class GpuContext:
         _check(load().cipc_gpu_set_active_set(self.h, c.ctypes.data_as(_i64p), *[_iptr(x) for x in arr]))
 
     # assembly
+    def set_hessian_precision(self, precision):
+        """0: FP64 elastic Hessian blocks; 1: the mixed form (FP32 elastic blocks, FP64
+        barriers, friction, gradient)."""
+        _check(load().cipc_gpu_set_hessian_precision(self.h, precision))
+
     def assemble(self, kappa):
         nnz = np.zeros(1, np.int64)
         col = np.zeros(1, np.int32)

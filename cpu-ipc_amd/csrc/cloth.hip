@@ -19,11 +19,12 @@ namespace cipc {
 // One thread per triangle.  F = [x1 - x0, x2 - x0] Dm^-1; dF/dx_c = B_c with
 // B_0 = -(B_1 + B_2), B_1 = row 0 of Dm^-1, B_2 = row 1 (computePFPX32D_double,
 // Elasticity.cpp:851-897).  K_ab[i][k] = area dt^2 sum_{j,l} B_a[j] B_b[l] H[3j+i][3l+k].
+template <typename HT>
 __global__ __launch_bounds__(256) void bw_element_kernel(int Tr, int e0, const int4* __restrict__ tris, const double* __restrict__ x,
                                                          const double* __restrict__ dmt, const double* __restrict__ area,
                                                          double st, double sh, double k, double dt2,
                                                          const int* __restrict__ slot_h, const int* __restrict__ slot_g,
-                                                         double* __restrict__ stagH, double* __restrict__ stagG) {
+                                                         HT* __restrict__ stagH, double* __restrict__ stagG) {
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= Tr) return;
     const int4 id = tris[t];
@@ -66,20 +67,21 @@ __global__ __launch_bounds__(256) void bw_element_kernel(int Tr, int e0, const i
             }
         // stored block (row vertex max, column vertex min): K_ab, or K_ab^T when vid[a] < vid[b]
         const bool tr = vid[a] < vid[b];
-        double* o = stagH + (size_t)9 * j;
+        HT* o = stagH + (size_t)9 * j;
 #pragma unroll
         for (int i = 0; i < 3; ++i)
 #pragma unroll
-            for (int kk = 0; kk < 3; ++kk) o[3 * i + kk] = tr ? K[kk][i] : K[i][kk];
+            for (int kk = 0; kk < 3; ++kk) o[3 * i + kk] = (HT)(tr ? K[kk][i] : K[i][kk]);
     }
 }
 
 // One thread per bending element of the quadratic form: forces D (Q (x) I3) x, blocks
 // D dt^2 Q_ab I3 (IPCSolver.cpp:197-237).  Q is read as 16 coalesced SoA streams.
+template <typename HT>
 __global__ __launch_bounds__(256) void bend_quad_kernel(int Nb, int e0, const int4* __restrict__ bv, const double* __restrict__ x,
                                                         const double* __restrict__ bq, double plate, double dt2,
                                                         const int* __restrict__ slot_h, const int* __restrict__ slot_g,
-                                                        double* __restrict__ stagH, double* __restrict__ stagG) {
+                                                        HT* __restrict__ stagH, double* __restrict__ stagG) {
     const int h = blockIdx.x * blockDim.x + threadIdx.x;
     if (h >= Nb) return;
     const int4 id = bv[h];
@@ -104,9 +106,9 @@ __global__ __launch_bounds__(256) void bend_quad_kernel(int Nb, int e0, const in
         const int j = slot_h[10 * e + lb];
         if (j < 0) continue;
         const double q = sc * Q[4 * lb_a(lb) + lb_b(lb)];
-        double* o = stagH + (size_t)9 * j;
+        HT* o = stagH + (size_t)9 * j;
 #pragma unroll
-        for (int i = 0; i < 9; ++i) o[i] = (i % 4 == 0) ? q : 0.0;
+        for (int i = 0; i < 9; ++i) o[i] = (HT)((i % 4 == 0) ? q : 0.0);
     }
 }
 
@@ -114,10 +116,11 @@ __global__ __launch_bounds__(256) void bend_quad_kernel(int Nb, int e0, const in
 // g = D w dtheta grad theta, H = makePD(D w (grad grad^T + dtheta Hess theta)), both times
 // dt^2 (HingeBending.cpp:79-106; IPCSolver.cpp:238-267).  The angle is translation-invariant,
 // so the projection runs on the translation-free subspace like the contact blocks.
+template <typename HT>
 __global__ __launch_bounds__(64) void bend_hinge_kernel(int Nb, int e0, const int4* __restrict__ bv, const double* __restrict__ x,
                                                         const double* __restrict__ rest, const double* __restrict__ weight,
                                                         double plate, double dt2, const int* __restrict__ slot_h,
-                                                        const int* __restrict__ slot_g, double* __restrict__ stagH,
+                                                        const int* __restrict__ slot_g, HT* __restrict__ stagH,
                                                         double* __restrict__ stagG) {
     const int h = blockIdx.x * blockDim.x + threadIdx.x;
     if (h >= Nb) return;
@@ -157,32 +160,37 @@ __global__ __launch_bounds__(64) void bend_hinge_kernel(int Nb, int e0, const in
         if (j < 0) continue;
         const int a = lb_a(lb), b = lb_b(lb);
         const bool tr = vid[a] < vid[b];
-        double* o = stagH + (size_t)9 * j;
+        HT* o = stagH + (size_t)9 * j;
 #pragma unroll
         for (int i = 0; i < 3; ++i)
 #pragma unroll
             for (int kk = 0; kk < 3; ++kk)
-                o[3 * i + kk] = dt2 * (tr ? Hs[Sym<12>::ix(3 * a + kk, 3 * b + i)] : Hs[Sym<12>::ix(3 * a + i, 3 * b + kk)]);
+                o[3 * i + kk] = (HT)(dt2 * (tr ? Hs[Sym<12>::ix(3 * a + kk, 3 * b + i)] : Hs[Sym<12>::ix(3 * a + i, 3 * b + kk)]));
     }
 }
 
-void cloth_assemble(Context& c, hipStream_t st) {
+template <typename HT>
+static void cloth_assemble_t(Context& c, hipStream_t st, HT* stagH) {
     Scene& s = c.sc;
     const double dt2 = s.dt * s.dt;
     if (s.Tr > 0) {
-        hipLaunchKernelGGL(bw_element_kernel, dim3(blocks_for(s.Tr, 256)), dim3(256), 0, st, s.Tr, s.T, s.tris.p, s.x.p, s.dmt.p,
-                           s.area.p, s.stretch, s.shear, s.srate, dt2, c.sp.slot_h.p, c.sp.slot_g.p, c.stagH.p, c.stagG.p);
+        hipLaunchKernelGGL(bw_element_kernel<HT>, dim3(blocks_for(s.Tr, 256)), dim3(256), 0, st, s.Tr, s.T, s.tris.p, s.x.p, s.dmt.p,
+                           s.area.p, s.stretch, s.shear, s.srate, dt2, c.sp.slot_h.p, c.sp.slot_g.p, stagH, c.stagG.p);
         CIPC_CHECK_LAUNCH();
     }
     if (s.Nb > 0) {
         if (s.bmode == 0)
-            hipLaunchKernelGGL(bend_quad_kernel, dim3(blocks_for(s.Nb, 256)), dim3(256), 0, st, s.Nb, s.T + s.Tr, s.bverts.p, s.x.p,
-                               s.bq.p, s.plate, dt2, c.sp.slot_h.p, c.sp.slot_g.p, c.stagH.p, c.stagG.p);
+            hipLaunchKernelGGL(bend_quad_kernel<HT>, dim3(blocks_for(s.Nb, 256)), dim3(256), 0, st, s.Nb, s.T + s.Tr, s.bverts.p, s.x.p,
+                               s.bq.p, s.plate, dt2, c.sp.slot_h.p, c.sp.slot_g.p, stagH, c.stagG.p);
         else
-            hipLaunchKernelGGL(bend_hinge_kernel, dim3(blocks_for(s.Nb, 64)), dim3(64), 0, st, s.Nb, s.T + s.Tr, s.bverts.p, s.x.p,
-                               s.brest.p, s.bweight.p, s.plate, dt2, c.sp.slot_h.p, c.sp.slot_g.p, c.stagH.p, c.stagG.p);
+            hipLaunchKernelGGL(bend_hinge_kernel<HT>, dim3(blocks_for(s.Nb, 64)), dim3(64), 0, st, s.Nb, s.T + s.Tr, s.bverts.p, s.x.p,
+                               s.brest.p, s.bweight.p, s.plate, dt2, c.sp.slot_h.p, c.sp.slot_g.p, stagH, c.stagG.p);
         CIPC_CHECK_LAUNCH();
     }
+}
+void cloth_assemble(Context& c, hipStream_t st) {
+    if (c.hessian_fp32) cloth_assemble_t<float>(c, st, reinterpret_cast<float*>(c.stagH.p));
+    else cloth_assemble_t<double>(c, st, c.stagH.p);
 }
 
 // ---------------------------------------------------------------------------------

@@ -499,6 +499,15 @@ static void assemble_impl(cipc_gpu_context* c, double kappa, const double* ks = 
     settle_nnz(*c);
 }
 
+int cipc_gpu_set_hessian_precision(cipc_gpu_context* c, int precision) {
+    return guard([&] {
+        require(c != nullptr, "null context");
+        require(precision == 0 || precision == 1, "precision must be 0 (FP64) or 1 (FP32 elastic Hessian blocks)");
+        c->hessian_fp32 = precision == 1;
+        c->assembled = false;
+    });
+}
+
 int cipc_gpu_assemble(cipc_gpu_context* c, double kappa, int64_t* nnz, int* collisions) {
     return guard([&] {
         require(c && c->have_scene, "no scene uploaded");

@@ -89,8 +89,11 @@ struct Context {
     StaticPattern sp;
     bool have_scene = false;
 
-    // elastic staging: 3x3 blocks and corner forces in their pattern slots
+    // elastic staging: 3x3 blocks and corner forces in their pattern slots; with
+    // hessian_fp32 (the mixed form: FP32 elastic Hessian, FP64 barriers, friction and
+    // gradient) the blocks are staged as float in the same buffer
     DBuf<double> stagH, stagG;
+    bool hessian_fp32 = false;
     DBuf<double> sbv;          // static block values, 9 per block
     DBuf<unsigned short> sbf;  // static block nonzero-contribution masks
     DBuf<double> grad;         // 3V

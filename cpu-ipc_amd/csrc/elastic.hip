@@ -176,10 +176,12 @@ __device__ inline void snh_block(const SNH& e, const d3 B[4], const d3 G1[4], co
     }
 }
 
+// H: staging type of the Hessian blocks (double; float for the mixed form)
+template <typename H>
 __global__ __launch_bounds__(256) void snh_element_kernel(int T, const int4* __restrict__ tets, const double* __restrict__ x,
                                                           const double* __restrict__ dminv, const double* __restrict__ vol,
                                                           Lame lame, double dt2, const int* __restrict__ slot_h,
-                                                          const int* __restrict__ slot_g, double* __restrict__ stagH,
+                                                          const int* __restrict__ slot_g, H* __restrict__ stagH,
                                                           double* __restrict__ stagG) {
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= T) return;
@@ -218,28 +220,29 @@ __global__ __launch_bounds__(256) void snh_element_kernel(int T, const int4* __r
         // stored block (row vertex max(va, vb), column vertex min): K_ab, or K_ab^T when
         // va < vb; a diagonal block is symmetric and stored whole
         const bool tr = vid[a] < vid[b];
-        double* o = stagH + (size_t)9 * j;
+        H* o = stagH + (size_t)9 * j;
 #pragma unroll
         for (int i = 0; i < 3; ++i)
 #pragma unroll
-            for (int k = 0; k < 3; ++k) o[3 * i + k] = sc * (tr ? K[k][i] : K[i][k]);
+            for (int k = 0; k < 3; ++k) o[3 * i + k] = (H)(sc * (tr ? K[k][i] : K[i][k]));
     }
 }
 
 // ---------------------------------------------------------------------------------
 // Gathers
 // ---------------------------------------------------------------------------------
-__global__ __launch_bounds__(256) void block_gather_kernel(int nb, const int* __restrict__ gptr, const double* __restrict__ stagH,
+template <typename H>
+__global__ __launch_bounds__(256) void block_gather_kernel(int nb, const int* __restrict__ gptr, const H* __restrict__ stagH,
                                                            double* __restrict__ sbv, unsigned short* __restrict__ sbf) {
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= nb) return;
     double acc[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
     unsigned mask = 0;
     for (int q = gptr[k]; q < gptr[k + 1]; ++q) {
-        const double* p = stagH + (size_t)9 * q;
+        const H* p = stagH + (size_t)9 * q;
 #pragma unroll
         for (int i = 0; i < 9; ++i) {
-            const double v = p[i];
+            const double v = (double)p[i];
             if (v != 0.0) { acc[i] += v; mask |= 1u << i; }
         }
     }
@@ -295,14 +298,24 @@ __global__ void soft_kernel(int n, const int* __restrict__ idx, const double* __
 void elastic_assemble(Context& c, hipStream_t st) {
     Scene& s = c.sc;
     const double dt2 = s.dt * s.dt;
-    if (s.T > 0)
-        hipLaunchKernelGGL(snh_element_kernel, dim3(blocks_for(s.T, 256)), dim3(256), 0, st, s.T, s.tets.p, s.x.p,
-                           s.dminv.p, s.vol.p, scene_lame(s), dt2, c.sp.slot_h.p, c.sp.slot_g.p, c.stagH.p, c.stagG.p);
+    if (s.T > 0) {
+        if (c.hessian_fp32)
+            hipLaunchKernelGGL(snh_element_kernel<float>, dim3(blocks_for(s.T, 256)), dim3(256), 0, st, s.T, s.tets.p, s.x.p,
+                               s.dminv.p, s.vol.p, scene_lame(s), dt2, c.sp.slot_h.p, c.sp.slot_g.p,
+                               reinterpret_cast<float*>(c.stagH.p), c.stagG.p);
+        else
+            hipLaunchKernelGGL(snh_element_kernel<double>, dim3(blocks_for(s.T, 256)), dim3(256), 0, st, s.T, s.tets.p, s.x.p,
+                               s.dminv.p, s.vol.p, scene_lame(s), dt2, c.sp.slot_h.p, c.sp.slot_g.p, c.stagH.p, c.stagG.p);
+    }
     CIPC_CHECK_LAUNCH();
     cloth_assemble(c, st);   // triangles and bending elements into the same staging
     if (c.sp.nblocks > 0) {
-        hipLaunchKernelGGL(block_gather_kernel, dim3(blocks_for(c.sp.nblocks, 256)), dim3(256), 0, st, c.sp.nblocks,
-                           c.sp.gptr.p, c.stagH.p, c.sbv.p, c.sbf.p);
+        if (c.hessian_fp32)
+            hipLaunchKernelGGL(block_gather_kernel<float>, dim3(blocks_for(c.sp.nblocks, 256)), dim3(256), 0, st, c.sp.nblocks,
+                               c.sp.gptr.p, reinterpret_cast<const float*>(c.stagH.p), c.sbv.p, c.sbf.p);
+        else
+            hipLaunchKernelGGL(block_gather_kernel<double>, dim3(blocks_for(c.sp.nblocks, 256)), dim3(256), 0, st, c.sp.nblocks,
+                               c.sp.gptr.p, c.stagH.p, c.sbv.p, c.sbf.p);
         CIPC_CHECK_LAUNCH();
         hipLaunchKernelGGL(mass_diag_kernel, dim3(blocks_for(s.V, 256)), dim3(256), 0, st, s.V, c.sp.diag_block.p,
                            s.mass.p, s.drag, c.sbv.p, c.sbf.p);

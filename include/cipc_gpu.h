@@ -141,6 +141,11 @@ int cipc_gpu_set_active_set(cipc_gpu_context* ctx, const int64_t counts[4], cons
  * non-free vertices dropped, mass diagonal on every DOF.  Returns the nonzero count and
  * the collision count the reference accumulates (compute_g_dpt's return value). */
 int cipc_gpu_assemble(cipc_gpu_context* ctx, double kappa, int64_t* nnz, int* collisions);
+/* Precision of the elastic Hessian blocks of the next assemblies.  No reference counterpart
+ * (the reference assembles in FP64).  0: FP64 (default); 1: the mixed form, tet, triangle
+ * and bending blocks rounded to FP32 where they are staged and summed in FP64, with the
+ * barrier, friction and mass parts, the gradient and the right-hand side in FP64. */
+int cipc_gpu_set_hessian_precision(cipc_gpu_context* ctx, int precision);
 /* Gradient of the last assemble (unmasked, 3V) and the CSR + masked right-hand side. */
 int cipc_gpu_get_gradient(cipc_gpu_context* ctx, double* grad);
 int cipc_gpu_get_csr(cipc_gpu_context* ctx, int* outer, int* inner, double* values, double* rhs);
