@@ -143,6 +143,42 @@ def batch_leg(n_scenes=256, steps=5, seed=0):
     return out
 
 
+def pcg_large_leg(tol=1e-6, max_iter=20000):
+    """The large linear-solve workload: eight 32^3-cell lattice cubes (287,496 vertices /
+    1,310,720 tets) stacked in contact with seeded velocities; the assembled Newton system of
+    a frozen state solved to 1e-6 relative residual with FP64 block-Jacobi PCG (the plain
+    baseline), FP64 and mixed block FSAI, and the learned factor when weights exist."""
+    import numpy as np
+    from cipc import gpu, scenes, train_precond
+    sc = scenes.stacked_cubes(8, 32, 0.1, 1e-3, seed=4, rotate="yaw", velocity_scale=0.5, name="8x32cubes")
+    xt, p = scenes.newton_probe_state(sc)
+    kappa = scenes.suggest_kappa(sc.hhat, sc.bbox_diag2, sc.mean_mass)
+    ctx = gpu.GpuContext(sc, xtilde=xt)
+    ctx.set_state(V_prev=sc.x - sc.velocities * sc.material.dt)
+    a = ctx.active_set()
+    ctx.friction_update(kappa)
+    r = ctx.assemble(kappa)
+    out = {"V": sc.V, "T": sc.T, "n": 3 * sc.V, "nnz": int(len(r["val"])),
+           "contacts": int(len(a["self_set"]) + len(a["ee_set"]) + len(a["env_set"]))}
+    legs = [("plain_fp64_jacobi", 0, 0), ("mixed_fp32_fp64_jacobi", 1, 0), ("plain_fp64_fsai", 0, 1), ("mixed_fp32_fp64_fsai", 1, 1)]
+    if os.path.exists(train_precond.DEFAULT_WEIGHTS):
+        ctx.set_preconditioner_weights(np.load(train_precond.DEFAULT_WEIGHTS))
+        legs += [("plain_fp64_learned", 0, 2), ("mixed_fp32_fp64_learned", 1, 2)]
+    for name, prec, pc in legs:
+        try:
+            ctx.solve(precision=prec, rel_tol=tol, max_iter=max_iter, preconditioner=pc)   # warm-up
+            _, st = ctx.solve(precision=prec, rel_tol=tol, max_iter=max_iter, preconditioner=pc)
+            out[name] = {"ms": round(st["solve_ms"], 3), "iterations": st["iterations"], "refinements": st["refinements"],
+                         "rel_residual": st["rel_residual"]}
+        except RuntimeError as e:
+            out[name] = {"error": str(e)}
+    base = out["plain_fp64_jacobi"].get("ms")
+    if base:
+        out["speedup_vs_plain_fp64_jacobi"] = {n: round(base / out[n]["ms"], 3) for n, _, _ in legs[1:] if "ms" in out[n]}
+    ctx.close()
+    return out
+
+
 def traffic_probe(n=5):
     """Workload for the HBM-traffic counters of the elastic stage: n Newton iterations of the
     frozen state (each launches the elastic kernels once), then one 256 MiB device-to-device
@@ -181,6 +217,7 @@ def main():
                          "WRITE_SIZE of --traffic-probe, scaled by a copy of known size)")
     ap.add_argument("--traffic-probe", action="store_true", help="run the HBM-counter workload instead of the benchmark")
     ap.add_argument("--batch", type=int, default=0, help="also time a batch of this many independent scenes (256: the batched workload)")
+    ap.add_argument("--pcg-large", action="store_true", help="also time the linear solves of the eight-32^3-cube system")
     args = ap.parse_args()
     if args.traffic_probe:
         traffic_probe()
@@ -257,6 +294,8 @@ def main():
         out["pcg"] = pcg
     if args.batch > 0:
         out["batch"] = batch_leg(args.batch)
+    if args.pcg_large:
+        out["pcg_large"] = pcg_large_leg()
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(scene, xt, vprev, p, kappa, budget_s=args.cpu_budget)
     print(json.dumps(out))

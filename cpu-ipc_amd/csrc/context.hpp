@@ -62,6 +62,8 @@ struct SolveBuffers {
     DBuf<double> G;                           // factor preconditioner at the lower BSR entries
     DBuf<float> Gf;
     DBuf<int> mirror;                         // index of the transposed entry
+    DBuf<double> vil, Gil;                    // interleaved copies for the row products
+    DBuf<float> vilf, Gilf;
     size_t ne = 0;                            // BSR entry capacity (sentinels included)
     // batch: work-group tables of the row kernels (per vertex partition) and per-scene controls
     std::vector<int> layout_key;

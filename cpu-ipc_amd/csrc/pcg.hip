@@ -38,6 +38,8 @@ struct PcgScalars {
     double rr;        // r . r
     double bn;        // ||b||
     double tol;       // absolute residual target
+    double pq;        // p . q of the current iteration
+    double beta;      // r.z' / r.z of the current iteration
     int done;         // converged (or broken down)
     int iters;
 };
@@ -126,14 +128,34 @@ __device__ inline void group_reduce3(double y[3]) {
         y[2] += __shfl_xor(y[2], o, kGL);
     }
 }
-// y = (A x)_a over entries rowptr[a] + sub, step kGL (all lanes of the group get the sum)
+// Interleaved copy of the BSR values for the products: entries in groups of kGL, each
+// group stored component-major (entry e, component k at (e / kGL) 9 kGL + k kGL + e % kGL), so
+// that the kGL lanes of a row, which read consecutive entries, load each component with one
+// contiguous access instead of nine 72-byte-strided ones.
+__host__ __device__ inline size_t il_index(size_t e, int k) { return (e / kGL) * (9 * kGL) + (size_t)k * kGL + e % kGL; }
+template <typename T>
+__global__ void interleave_blocks(size_t ne, const T* __restrict__ val, T* __restrict__ out) {
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;   // (entry, component)
+    if (i >= 9 * ne) return;
+    const size_t e = i / 9;
+    out[il_index(e, (int)(i % 9))] = val[i];
+}
+template <typename T, typename X>
+__device__ inline void block_mv_il(const T* __restrict__ B, size_t e, const X* __restrict__ x, double y[3]) {
+    const T* b = B + (e / kGL) * (9 * kGL) + e % kGL;
+    y[0] = (double)b[0] * x[0] + (double)b[kGL] * x[1] + (double)b[2 * kGL] * x[2];
+    y[1] = (double)b[3 * kGL] * x[0] + (double)b[4 * kGL] * x[1] + (double)b[5 * kGL] * x[2];
+    y[2] = (double)b[6 * kGL] * x[0] + (double)b[7 * kGL] * x[1] + (double)b[8 * kGL] * x[2];
+}
+// y = (A x)_a over entries rowptr[a] + sub, step kGL (all lanes of the group get the sum);
+// val interleaved
 template <typename T, typename X>
 __device__ inline void row_product(int a, int sub, const int* __restrict__ rowptr, const int* __restrict__ col, const T* __restrict__ val,
                                    const X* __restrict__ x, double y[3]) {
     y[0] = y[1] = y[2] = 0.0;
     for (int e = rowptr[a] + sub; e < rowptr[a + 1]; e += kGL) {
         double t[3];
-        block_mv(val + (size_t)9 * e, x + 3 * (size_t)col[e], t);
+        block_mv_il(val, (size_t)e, x + 3 * (size_t)col[e], t);
         y[0] += t[0]; y[1] += t[1]; y[2] += t[2];
     }
     group_reduce3<T>(y);
@@ -255,7 +277,7 @@ __global__ __launch_bounds__(kPB) void prec_jacobi(Layout L, const T* __restrict
     group_partials<1>(acc, part_rz);
 }
 // factor preconditioner M^-1 = G^T G, G block lower-triangular stored at the lower entries of
-// the BSR (the entry of (i, j), j <= i, holds G_ij):  t = G r
+// the BSR (the entry of (i, j), j <= i, holds G_ij):  t = G r (G interleaved)
 template <typename T>
 __global__ __launch_bounds__(kPB) void prec_g1(Layout L, const int* __restrict__ rowptr, const int* __restrict__ col, const T* __restrict__ G,
                                                const T* __restrict__ r, T* __restrict__ t, const PcgScalars* __restrict__ sc) {
@@ -269,7 +291,7 @@ __global__ __launch_bounds__(kPB) void prec_g1(Layout L, const int* __restrict__
         const int j = col[e];
         if (j > a) break;
         double w[3];
-        block_mv(G + (size_t)9 * e, r + 3 * (size_t)j, w);
+        block_mv_il(G, (size_t)e, r + 3 * (size_t)j, w);
         y[0] += w[0]; y[1] += w[1]; y[2] += w[2];
     }
     group_reduce3<T>(y);
@@ -277,7 +299,7 @@ __global__ __launch_bounds__(kPB) void prec_g1(Layout L, const int* __restrict__
         for (int k = 0; k < 3; ++k) t[3 * a + k] = (T)y[k];
 }
 // z = G^T t: row j gathers G_ij^T t_i over its upper entries (i >= j) through the mirror
-// index of each entry; partial of r.z; p = z when `init`
+// index of each entry (G in its plain block layout); partial of r.z; p = z when `init`
 template <typename T>
 __global__ __launch_bounds__(kPB) void prec_g2(Layout L, const int* __restrict__ rowptr, const int* __restrict__ col,
                                                const int* __restrict__ mirror, const T* __restrict__ G, const T* __restrict__ t,
@@ -361,20 +383,26 @@ __global__ __launch_bounds__(kPB) void pcg_spmv(Layout L, const int* __restrict_
     }
     group_partials<1>(acc, part_pq);
 }
+// p.q of every scene from the product's partials, once (one work-group per scene)
+__global__ __launch_bounds__(kPB) void pcg_reduce_pq(Layout L, const double* __restrict__ part_pq, PcgScalars* __restrict__ sc) {
+    const int k = blockIdx.x;
+    if (sc[k].done) return;
+    int lo, n;
+    L.parts(k, lo, n);
+    const double pq = sum_partials_fixed(part_pq + lo, n);
+    if (threadIdx.x == 0) sc[k].pq = pq;
+}
 // alpha = rz / p.q; x += alpha p; r -= alpha q; partial of r.r; with block-Jacobi (minv set)
 // also z = M^-1 r and its r.z partial
 template <typename T>
-__global__ __launch_bounds__(kPB) void pcg_update(Layout L, const double* __restrict__ part_pq, const T* __restrict__ minv,
+__global__ __launch_bounds__(kPB) void pcg_update(Layout L, const T* __restrict__ minv,
                                                   const T* __restrict__ p, const T* __restrict__ q, T* __restrict__ x, T* __restrict__ r,
                                                   T* __restrict__ z, const PcgScalars* __restrict__ sc, double* __restrict__ part_rz,
                                                   double* __restrict__ part_rr) {
     int scene, row0, rend;
     L.block(scene, row0, rend);
     if (sc[scene].done) return;
-    int lo, n;
-    L.parts(scene, lo, n);
-    const double pq = sum_partials_fixed(part_pq + lo, n);
-    const double alpha = sc[scene].rz / pq;
+    const double alpha = sc[scene].rz / sc[scene].pq;
     const int a = row0 + threadIdx.x / kGL, sub = threadIdx.x % kGL;
     double acc[2] = {0.0, 0.0};
     if (a < rend && sub == 0) {
@@ -395,35 +423,28 @@ __global__ __launch_bounds__(kPB) void pcg_update(Layout L, const double* __rest
     if (minv) group_partials2(acc, part_rz, part_rr);
     else { double a1[1] = {acc[1]}; group_partials<1>(a1, part_rr); }
 }
-// beta = rz' / rz; p = z + beta p (one DOF per thread: the grid covers 3V / kPB work-groups,
-// more than the row kernels' ng, so every work-group reads all ng partials).  Every
-// work-group reads the previous r.z here; the
-// scalars (and the convergence flag) are replaced by the following one-work-group kernel,
-// so no work-group can see a value another one has changed.
+// p = z + beta p with the beta of pcg_scalars (one DOF per thread); skipped by a scene that
+// has just converged, whose direction is no longer used
 template <typename T>
-__global__ __launch_bounds__(kPB) void pcg_direction(int V, const double* __restrict__ part_rz, int ng, const T* __restrict__ z,
-                                                     T* __restrict__ p, const PcgScalars* __restrict__ sc) {
+__global__ __launch_bounds__(kPB) void pcg_direction(int V, const T* __restrict__ z, T* __restrict__ p, const PcgScalars* __restrict__ sc) {
     if (sc->done) return;
-    const double rz1 = sum_partials_fixed(part_rz, ng);
-    const double beta = rz1 / sc->rz;
+    const double beta = sc->beta;
     const int t = blockIdx.x * blockDim.x + threadIdx.x;       // one DOF per thread
     if (t < 3 * V) p[t] = (T)((double)z[t] + beta * (double)p[t]);
 }
 // the same on the row layout of a batch: the 3 kRows DOFs of the work-group's rows
 template <typename T>
-__global__ __launch_bounds__(kPB) void pcg_direction_rows(Layout L, const double* __restrict__ part_rz, const T* __restrict__ z,
-                                                          T* __restrict__ p, const PcgScalars* __restrict__ sc) {
+__global__ __launch_bounds__(kPB) void pcg_direction_rows(Layout L, const T* __restrict__ z, T* __restrict__ p,
+                                                          const PcgScalars* __restrict__ sc) {
     int scene, row0, rend;
     L.block(scene, row0, rend);
     if (sc[scene].done) return;
-    int lo, n;
-    L.parts(scene, lo, n);
-    const double rz1 = sum_partials_fixed(part_rz + lo, n);
-    const double beta = rz1 / sc[scene].rz;
+    const double beta = sc[scene].beta;
     const int t = 3 * row0 + threadIdx.x;
     if ((int)threadIdx.x < 3 * kRows && t < 3 * rend) p[t] = (T)((double)z[t] + beta * (double)p[t]);
 }
-// one work-group per scene
+// r.z', r.r and beta = r.z' / r.z from the partials, once per iteration (one work-group per
+// scene); the convergence test
 __global__ void pcg_scalars(Layout L, const double* __restrict__ part_rz, const double* __restrict__ part_rr,
                             PcgScalars* __restrict__ sc, int max_iter, SceneControl ctl) {
     const int k = blockIdx.x;
@@ -435,6 +456,7 @@ __global__ void pcg_scalars(Layout L, const double* __restrict__ part_rz, const 
     if (threadIdx.x == 0) {
         PcgScalars& s = sc[k];
         const int cap = ctl.max_iter ? ctl.max_iter[k] : max_iter;
+        s.beta = rz1 / s.rz;
         s.rz = rz1;
         s.rr = rr;
         s.iters += 1;
@@ -611,7 +633,8 @@ bool all_done(const std::vector<PcgScalars>& h) {
 template <typename T>
 struct Prec {
     const T* minv = nullptr;
-    const T* G = nullptr;
+    const T* G = nullptr;      // plain blocks (transposed products)
+    const T* Gi = nullptr;     // interleaved (row products)
     const int* mirror = nullptr;
 };
 
@@ -653,7 +676,7 @@ void apply_prec(Context& c, const Layout& L, const Prec<T>& M, const T* r, T* z,
     SolveBuffers& w = c.solve;
     if (M.G) {
         T* t = reinterpret_cast<T*>(w.t.p);
-        hipLaunchKernelGGL(prec_g1<T>, dim3(L.ng), dim3(kPB), 0, c.stream, L, w.rowptr.p, w.col.p, M.G, r, t, sc);
+        hipLaunchKernelGGL(prec_g1<T>, dim3(L.ng), dim3(kPB), 0, c.stream, L, w.rowptr.p, w.col.p, M.Gi, r, t, sc);
         hipLaunchKernelGGL(prec_g2<T>, dim3(L.ng), dim3(kPB), 0, c.stream, L, w.rowptr.p, w.col.p, M.mirror, M.G, t, r, z, p, init, sc,
                            part_rz);
     } else {
@@ -691,12 +714,13 @@ void pcg_run(Context& c, const Layout& L, int S, const T* val, const Prec<T>& M,
     CIPC_HIP(hipStreamBeginCapture(c.stream, hipStreamCaptureModeThreadLocal));
     for (int k = 0; k < kCheck; ++k) {
         hipLaunchKernelGGL(pcg_spmv<T>, dim3(ng), dim3(kPB), 0, c.stream, L, w.rowptr.p, w.col.p, val, p, q, sc, part_pq);
-        hipLaunchKernelGGL(pcg_update<T>, dim3(ng), dim3(kPB), 0, c.stream, L, part_pq, M.G ? nullptr : M.minv, p, q, x, r, z, sc,
+        hipLaunchKernelGGL(pcg_reduce_pq, dim3(S), dim3(kPB), 0, c.stream, L, part_pq, sc);
+        hipLaunchKernelGGL(pcg_update<T>, dim3(ng), dim3(kPB), 0, c.stream, L, M.G ? nullptr : M.minv, p, q, x, r, z, sc,
                            part_rz, part_rr);
         if (M.G) apply_prec<T>(c, L, M, r, z, p, 0, sc, part_rz);
-        if (L.bl) hipLaunchKernelGGL(pcg_direction_rows<T>, dim3(ng), dim3(kPB), 0, c.stream, L, part_rz, z, p, sc);
-        else hipLaunchKernelGGL(pcg_direction<T>, dim3(ngd), dim3(kPB), 0, c.stream, V, part_rz, ng, z, p, sc);
         hipLaunchKernelGGL(pcg_scalars, dim3(S), dim3(kPB), 0, c.stream, L, part_rz, part_rr, sc, max_iter, ctl);
+        if (L.bl) hipLaunchKernelGGL(pcg_direction_rows<T>, dim3(ng), dim3(kPB), 0, c.stream, L, z, p, sc);
+        else hipLaunchKernelGGL(pcg_direction<T>, dim3(ngd), dim3(kPB), 0, c.stream, V, z, p, sc);
     }
     const hipError_t ce = hipStreamEndCapture(c.stream, &graph);
     CIPC_HIP(ce);
@@ -788,10 +812,33 @@ void linear_solve_scenes(Context& c, int precision, int preconditioner, double r
     build_solve_matrix(c, precision == 1);
     if (preconditioner == 1) build_fsai(c, precision == 1);
     else if (preconditioner == 2) build_learned(c, precision == 1);
+    // interleaved copies for the row products
+    const size_t ne = w.ne, nil = (ne + kGL - 1) / kGL * 9 * kGL;
+    const bool fp32 = precision == 1;
+    w.vil.resize(nil);           // FP64: the solve, or the refinement's residuals
+    hipLaunchKernelGGL(interleave_blocks<double>, dim3(blocks_for(9 * ne, 256)), dim3(256), 0, c.stream, ne, w.val.p, w.vil.p);
+    if (fp32) {
+        w.vilf.resize(nil);
+        hipLaunchKernelGGL(interleave_blocks<float>, dim3(blocks_for(9 * ne, 256)), dim3(256), 0, c.stream, ne, w.valf.p, w.vilf.p);
+    }
+    if (preconditioner != 0) {
+        if (fp32) {
+            w.Gilf.resize(nil);
+            hipLaunchKernelGGL(interleave_blocks<float>, dim3(blocks_for(9 * ne, 256)), dim3(256), 0, c.stream, ne, w.Gf.p, w.Gilf.p);
+        } else {
+            w.Gil.resize(nil);
+            hipLaunchKernelGGL(interleave_blocks<double>, dim3(blocks_for(9 * ne, 256)), dim3(256), 0, c.stream, ne, w.G.p, w.Gil.p);
+        }
+    }
+    CIPC_CHECK_LAUNCH();
     Prec<double> Md;
     Prec<float> Mf;
     if (preconditioner == 0) { Md.minv = w.minv.p; Mf.minv = w.minvf.p; }
-    else { Md.G = w.G.p; Md.mirror = w.mirror.p; Mf.G = w.Gf.p; Mf.mirror = w.mirror.p; }
+    else {
+        Md.G = w.G.p; Md.mirror = w.mirror.p; Mf.G = w.Gf.p; Mf.mirror = w.mirror.p;
+        Md.Gi = fp32 ? nullptr : w.Gil.p;
+        Mf.Gi = fp32 ? w.Gilf.p : nullptr;
+    }
     // per-scene controls (a batch): participation, inner tolerances and caps, correction mask
     SceneControl ctl;
     std::vector<int> act(S, 1);
@@ -807,7 +854,7 @@ void linear_solve_scenes(Context& c, int precision, int preconditioner, double r
     std::vector<PcgScalars> h;
     hipLaunchKernelGGL(zero_d, dim3(blocks_for(n, 256)), dim3(256), 0, c.stream, n, w.x.p);
     if (precision == 0) {
-        pcg_run<double>(c, L, S, w.val.p, Md, c.rhs.p, w.x.p, 1, rel_tol, 0, max_iter, ctl, h);
+        pcg_run<double>(c, L, S, w.vil.p, Md, c.rhs.p, w.x.p, 1, rel_tol, 0, max_iter, ctl, h);
         for (int k = 0; k < S; ++k) {
             res[k].iterations = h[k].iters;
             res[k].rel_residual = h[k].bn > 0 ? std::sqrt(h[k].rr) / h[k].bn : 0.0;
@@ -825,7 +872,7 @@ void linear_solve_scenes(Context& c, int precision, int preconditioner, double r
         std::vector<int> icap(S, 0), need_inner(S, 0), add(S, 0);   // host sources of async copies: kept alive
         for (int round = 0;; ++round) {
             double* rd = w.res.p;
-            hipLaunchKernelGGL(pcg_residual<double>, dim3(ng), dim3(kPB), 0, c.stream, L, w.rowptr.p, w.col.p, w.val.p, c.rhs.p, w.x.p,
+            hipLaunchKernelGGL(pcg_residual<double>, dim3(ng), dim3(kPB), 0, c.stream, L, w.rowptr.p, w.col.p, w.vil.p, c.rhs.p, w.x.p,
                                round == 0 ? 1 : 0, rd, part_rr, part_bb);
             hipLaunchKernelGGL(pcg_init_scalars, dim3(S), dim3(kPB), 0, c.stream, L, part_rr, part_rr, part_bb, rel_tol, 0, SceneControl{},
                                sc);
@@ -855,7 +902,7 @@ void linear_solve_scenes(Context& c, int precision, int preconditioner, double r
             }
             hipLaunchKernelGGL(zero_f, dim3(blocks_for(n, 256)), dim3(256), 0, c.stream, n, w.xf.p);
             std::vector<PcgScalars> hi;
-            pcg_run<float>(c, L, S, w.valf.p, Mf, rd, w.xf.p, 1, itol[0], 1, icap[0], ictl, hi);
+            pcg_run<float>(c, L, S, w.vilf.p, Mf, rd, w.xf.p, 1, itol[0], 1, icap[0], ictl, hi);
             // scenes that made no progress (breakdown) or hit the refinement cap stop here,
             // without their correction
             std::fill(add.begin(), add.end(), 0);
