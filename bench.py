@@ -128,6 +128,10 @@ def batch_leg(n_scenes=256, steps=5, seed=0):
     b.newton_iteration(kappa)
     it = [b.newton_iteration(kappa) for _ in range(steps)]
     ms = float(np.mean([t["total_ms"] for t in it]))
+    b.set_hessian_precision(1)                        # the mixed form
+    b.newton_iteration(kappa)
+    itm = [b.newton_iteration(kappa) for _ in range(steps)]
+    b.set_hessian_precision(0)
     b.assemble(kappa)
     _, st = b.solve(precision=1, rel_tol=1e-6, preconditioner=1)
     solve_ms = st[0]["solve_ms"]
@@ -136,6 +140,8 @@ def batch_leg(n_scenes=256, steps=5, seed=0):
            "scene_iterations_per_s": round(n_scenes / (ms * 1e-3), 1),
            "stages_ms": {k: round(float(np.mean([t[k] for t in it])), 3) for k in
                          ("broadphase_ms", "assembly_ms", "elastic_ms", "ccd_ms", "energy_ms")},
+           "mixed_fp32_hessian": {"newton_iteration_ms": round(float(np.mean([t["total_ms"] for t in itm])), 3),
+                                  "elastic_ms": round(float(np.mean([t["elastic_ms"] for t in itm])), 3)},
            "contacts": int(it[-1]["contacts"]), "min_alpha": float(it[-1]["alpha"]),
            "solve_mixed_fsai_ms": round(solve_ms, 3), "solve_iterations_min_max": [int(min(iters)), int(max(iters))],
            "setup_s": round(setup_s, 1)}
