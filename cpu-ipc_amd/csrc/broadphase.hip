@@ -128,10 +128,11 @@ __global__ void karras_kernel(int n, const unsigned long long* __restrict__ k, N
     range_last[i] = last;
     split_right[split] = r;
 }
-__global__ void leaf_ids(int n, const int* __restrict__ order, Node* __restrict__ nodes) {
+__global__ void leaf_ids(int n, const int* __restrict__ order, Node* __restrict__ nodes, int* __restrict__ leafpos) {
     const int s = blockIdx.x * blockDim.x + threadIdx.x;
     if (s >= n) return;
     nodes[n - 1 + s].left = order[s];
+    leafpos[order[s]] = s;
 }
 __global__ void skip_kernel(int n, const int* __restrict__ range_last, const int* __restrict__ split_right, Node* __restrict__ nodes) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -197,7 +198,8 @@ __global__ void internal_boxes(int n, const int* __restrict__ first, const int* 
 }
 
 // Four-wide records from the refitted binary nodes.
-__global__ void wide_records(int n, const Node* __restrict__ nodes, Node4* __restrict__ wide) {
+// pad[k]: the last leaf position under child k (wide_visit_after)
+__global__ void wide_records(int n, const Node* __restrict__ nodes, const int* __restrict__ range_last, Node4* __restrict__ wide) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n - 1) return;
     int ch[4], m = 0;
@@ -214,12 +216,13 @@ __global__ void wide_records(int n, const Node* __restrict__ nodes, Node4* __res
             r.lo[k][0] = __double2float_rd(x.lo.x); r.lo[k][1] = __double2float_rd(x.lo.y); r.lo[k][2] = __double2float_rd(x.lo.z);
             r.hi[k][0] = __double2float_ru(x.hi.x); r.hi[k][1] = __double2float_ru(x.hi.y); r.hi[k][2] = __double2float_ru(x.hi.z);
             r.child[k] = ch[k] >= n - 1 ? -(x.left + 1) : ch[k];
+            r.pad[k] = ch[k] >= n - 1 ? ch[k] - (n - 1) : range_last[ch[k]];
         } else {
             r.lo[k][0] = r.lo[k][1] = r.lo[k][2] = __int_as_float(0x7f800000);
             r.hi[k][0] = r.hi[k][1] = r.hi[k][2] = __int_as_float(0xff800000);
             r.child[k] = kNoChild;
+            r.pad[k] = -1;
         }
-        r.pad[k] = 0;
     }
     wide[i] = r;
 }
@@ -267,7 +270,8 @@ void build_topology(Context& c, LBVH& t, int n) {
                            t.range_first.p, t.range_last.p, t.split_right.p);
         CIPC_CHECK_LAUNCH();
     }
-    hipLaunchKernelGGL(leaf_ids, dim3(blocks_for(n, 256)), dim3(256), 0, c.stream, n, t.order.p, t.nodes.p);
+    t.leafpos.resize(n);
+    hipLaunchKernelGGL(leaf_ids, dim3(blocks_for(n, 256)), dim3(256), 0, c.stream, n, t.order.p, t.nodes.p, t.leafpos.p);
     hipLaunchKernelGGL(skip_kernel, dim3(blocks_for(nodes, 256)), dim3(256), 0, c.stream, n, t.range_last.p, t.split_right.p, t.nodes.p);
     CIPC_CHECK_LAUNCH();
 }
@@ -287,7 +291,7 @@ void refit(Context& c, LBVH& t) {
         hipLaunchKernelGGL(internal_boxes, dim3(blocks_for(n - 1, 256)), dim3(256), 0, c.stream, n, t.range_first.p, t.range_last.p,
                            t.L, t.seg.p, t.nodes.p);
         CIPC_CHECK_LAUNCH();
-        hipLaunchKernelGGL(wide_records, dim3(blocks_for(n - 1, 256)), dim3(256), 0, c.stream, n, t.nodes.p, t.wide.p);
+        hipLaunchKernelGGL(wide_records, dim3(blocks_for(n - 1, 256)), dim3(256), 0, c.stream, n, t.nodes.p, t.range_last.p, t.wide.p);
         CIPC_CHECK_LAUNCH();
     }
     ++t.age;

@@ -165,9 +165,11 @@ __global__ __launch_bounds__(64) void full_ccd_pt_kernel(int Vs, const int* __re
     scene_atomic<false>(sl, scene, 3, best);
 }
 
-// Full CCD, edge-edge phase: each surface edge eJ (split over kParts lanes), candidates
-// eI > eJ from the tree built with the incoming step, exact swept-box filter with the
-// point-triangle-reduced step read from `slot` (ContactMechanics.cpp:3818-3866).
+// Full CCD, edge-edge phase (ContactMechanics.cpp:3818-3866): every unordered pair of
+// surface edges whose swept boxes (built with the incoming step) overlap, the pair ordered as
+// the reference orders it (I the larger edge id, J the smaller) and filtered with the exact
+// swept boxes of the point-triangle-reduced step read from the slots.  Each edge (split over
+// kParts lanes) walks only the leaves after its own leaf, so each pair is met once.
 __global__ __launch_bounds__(64) void full_ccd_ee_kernel(int Es, const int* __restrict__ perm, const int2* __restrict__ se,
                                                           const int* __restrict__ bt, const double* __restrict__ x,
                                                           const double* __restrict__ p, TreeView tv, const Box* __restrict__ ebox,
@@ -177,23 +179,25 @@ __global__ __launch_bounds__(64) void full_ccd_ee_kernel(int Es, const int* __re
     double best = 1.0;
     int scene = kNoScene;
     if (t < Es * kParts) {
-        const int eJ = perm[t / kParts], part = t % kParts;
-        const int2 J = se[eJ];
-        scene = scene_of(vs, J.x);
+        const int eQ = perm[t / kParts], part = t % kParts;
+        const int2 Q = se[eQ];
+        scene = scene_of(vs, Q.x);
         const double a = bits_to_double(sl[kSlots * scene + 4]);
-        const d3 j0 = ld3(x, J.x), j1 = ld3(x, J.y), pj0 = ld3(p, J.x), pj1 = ld3(p, J.y);
-        const Box b1 = box_union(swept_pt(j0, pj0, a), swept_pt(j1, pj1, a));
-        // query box: eJ's leaf box of the tree (swept with the incoming step)
-        const Box q = ebox[eJ];
-        part_walk(tv, part, q, 0.0, stk + threadIdx.x, [&](int eI) {
-            if (eI <= eJ || !overlaps(q, ebox[eI], 0.0)) return;   // exact leaf test (the walk's boxes are float)
-            const int2 I = se[eI];
+        const Box bq = box_union(swept_pt(ld3(x, Q.x), ld3(p, Q.x), a), swept_pt(ld3(x, Q.y), ld3(p, Q.y), a));
+        // query box: the edge's leaf box of the tree (swept with the incoming step)
+        const Box q = ebox[eQ];
+        part_walk_after(tv, part, q, 0.0, tv.leafpos[eQ], stk + threadIdx.x, [&](int eH) {
+            if (!overlaps(q, ebox[eH], 0.0)) return;   // exact leaf test (the walk's boxes are float)
+            const int eI = eH > eQ ? eH : eQ, eJ = eH > eQ ? eQ : eH;
+            const int2 I = se[eI], J = se[eJ];
             if (is_collider(bt[I.x]) && is_collider(bt[I.y]) && is_collider(bt[J.x]) && is_collider(bt[J.y])) return;
-            if (I.x == J.x || I.x == J.y || I.y == J.x || I.y == J.y || eI < eJ) return;
-            if (vs && vs[I.x] != scene) return;
+            if (I.x == J.x || I.x == J.y || I.y == J.x || I.y == J.y) return;
+            const int2 H = se[eH];
+            if (vs && vs[H.x] != scene) return;            // scenes of a batch never touch
+            const Box bh = box_union(swept_pt(ld3(x, H.x), ld3(p, H.x), a), swept_pt(ld3(x, H.y), ld3(p, H.y), a));
+            if (!overlaps(bq, bh, 0.0)) return;
             const d3 i0 = ld3(x, I.x), i1 = ld3(x, I.y), pi0 = ld3(p, I.x), pi1 = ld3(p, I.y);
-            const Box b2 = box_union(swept_pt(i0, pi0, a), swept_pt(i1, pi1, a));
-            if (!overlaps(b1, b2, 0.0)) return;
+            const d3 j0 = ld3(x, J.x), j1 = ld3(x, J.y), pj0 = ld3(p, J.x), pj1 = ld3(p, J.y);
             const double la = accd_ee(i0, i1, j0, j1, -pi0, -pi1, -pj0, -pj1, eta, 0.0, fmin(best, a));
             best = fmin(la, best);
         }, [] {});
@@ -259,7 +263,7 @@ void step_bound_launch(Context& c) {
     build_swept_trees(c, sl.p + 5);
     LBVH& ft = c.face_tree;
     LBVH& et = c.edge_tree;
-    const TreeView ftv{ft.n, ft.nodes.p, ft.wide.p}, etv{et.n, et.nodes.p, et.wide.p};
+    const TreeView ftv{ft.n, ft.nodes.p, ft.wide.p}, etv{et.n, et.nodes.p, et.wide.p, et.leafpos.p};
     if (s.Vs > 0 && ft.n > 0)
         hipLaunchKernelGGL(full_ccd_pt_kernel, dim3(blocks_for((size_t)s.Vs * kParts, 64)), dim3(64), 0, c.stream, s.Vs, s.sv_perm.p,
                            s.sv.p, s.faces.p, s.bt.p, s.x.p, s.dir.p, c.swept_point_box.p, ftv, sl.p, eta, vs);

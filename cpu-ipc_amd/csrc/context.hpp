@@ -34,6 +34,7 @@ struct LBVH {
     DBuf<Box> seg;             // the reduction levels, level 0 = leaf boxes in leaf order
     DBuf<unsigned> codes, codes_sorted;   // 30-bit Morton codes, by primitive / sorted
     DBuf<int> order;                      // primitive ids in leaf order (+ n scratch)
+    DBuf<int> leafpos;                    // leaf position of each primitive
     DBuf<unsigned long long> keys_sorted; // (code << 32 | primitive), sorted and unique
     DBuf<Box> scene_partial;
     bool swept = false;

@@ -25,6 +25,7 @@ struct TreeView {
     int n;
     const Node* nodes;
     const Node4* wide;
+    const int* leafpos = nullptr;   // leaf position of each primitive (the forward walks)
 };
 
 // Visits, in depth-first (Morton) order, every leaf of the subtree [start, stop) whose box
@@ -110,6 +111,42 @@ __device__ inline bool wide_visit(const TreeView& t, int start, const Box& q, do
     }
     return true;
 }
+// wide_visit restricted to the leaves after leaf position `after` in depth-first order:
+// Node4::pad[k] holds the last leaf position under child k, so whole subtrees at or before
+// `after` are skipped without a box test.  For a symmetric pair search in which every
+// primitive queries the same tree, each unordered pair is then met once, from the primitive
+// whose leaf comes first.
+template <typename F>
+__device__ inline bool wide_visit_after(const TreeView& t, int start, const Box& q, double gap, int after, int* __restrict__ stk,
+                                        F&& hit) {
+    if (start < 0) return true;
+    if (start >= t.n - 1) {
+        if (start - (t.n - 1) <= after) return true;
+        const Node& r = t.nodes[start];
+        if (overlaps(q, Box{r.lo, r.hi}, gap)) hit(r.left);
+        return true;
+    }
+    int sp = 1;
+    stk[0] = start;
+    for (int steps = 0; sp > 0; ++steps) {
+        if (steps >= t.n) return false;
+        const int node = stk[64 * (--sp)];
+        const Node4 r = t.wide[node];
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            const int ch = r.child[k];
+            if (ch == kNoChild || r.pad[k] <= after || !overlaps_f(q, r.lo[k], r.hi[k], gap)) continue;
+            if (ch < 0) {
+                hit(-ch - 1);
+            } else {
+                if (sp >= kStack) return false;
+                stk[64 * (sp++)] = ch;
+            }
+        }
+    }
+    return true;
+}
+
 // One part of a split query: the wide walk, or, should its stack run out, reset() and the
 // exact binary walk of the same subtree.  Deterministic, so a second call returns the same
 // hits in the same order.
@@ -120,6 +157,18 @@ __device__ inline void part_walk(const TreeView& t, int part, const Box& q, doub
     if (wide_visit(t, root, q, gap, stk, hit)) return;
     reset();
     lbvh_visit_range(t.n, t.nodes, root, t.nodes[root].skip, q, gap, hit);
+}
+// part_walk over the leaves after position `after` only (wide_visit_after)
+template <typename F, typename R>
+__device__ inline void part_walk_after(const TreeView& t, int part, const Box& q, double gap, int after, int* __restrict__ stk,
+                                       F&& hit, R&& reset) {
+    const int root = part_root(t, part);
+    if (root < 0) return;
+    if (wide_visit_after(t, root, q, gap, after, stk, hit)) return;
+    reset();
+    lbvh_visit_range(t.n, t.nodes, root, t.nodes[root].skip, q, gap, [&](int prim) {
+        if (t.leafpos[prim] > after) hit(prim);
+    });
 }
 
 }  // namespace cipc
