@@ -82,7 +82,7 @@ __device__ inline bool overlaps_f(const Box& q, const float* lo, const float* hi
 // stk[64 i]): one 128-byte record per step, about half the steps of the binary walk.  Calls
 // hit(primitive) for a superset of the overlapping leaves (float boxes; a single-leaf
 // subtree is tested exactly).  Returns false if the stack or the step bound ran out.
-constexpr int kStack = 64;
+constexpr int kStack = 32;
 template <typename F>
 __device__ inline bool wide_visit(const TreeView& t, int start, const Box& q, double gap, int* __restrict__ stk, F&& hit) {
     if (start < 0) return true;
