@@ -232,11 +232,11 @@ __global__ __launch_bounds__(kEnergyBlock) void e_bend(int Nb, int mode, const i
     energy_block_sum(acc, part);
 }
 
-void cloth_energy_partials(Context& c, double* part_bw, double* part_bend) {
+void cloth_energy_partials(Context& c, double* part_bw, double* part_bend, hipStream_t st) {
     Scene& s = c.sc;
-    hipLaunchKernelGGL(e_bw, dim3(kEnergyGrid), dim3(kEnergyBlock), 0, c.stream, s.Tr, s.tris.p, s.x.p, s.dmt.p, s.area.p, s.stretch,
+    hipLaunchKernelGGL(e_bw, dim3(kEnergyGrid), dim3(kEnergyBlock), 0, st, s.Tr, s.tris.p, s.x.p, s.dmt.p, s.area.p, s.stretch,
                        s.shear, s.srate, part_bw);
-    hipLaunchKernelGGL(e_bend, dim3(kEnergyGrid), dim3(kEnergyBlock), 0, c.stream, s.Nb, s.bmode, s.bverts.p, s.x.p, s.bq.p,
+    hipLaunchKernelGGL(e_bend, dim3(kEnergyGrid), dim3(kEnergyBlock), 0, st, s.Nb, s.bmode, s.bverts.p, s.x.p, s.bq.p,
                        s.brest.p, s.bweight.p, s.plate, part_bend);
     CIPC_CHECK_LAUNCH();
 }

@@ -554,9 +554,9 @@ void friction_records(Context& c, size_t slot_b, size_t slot_g) {
     }
 }
 
-void friction_energy_partials(Context& c, double* part, double* elem_self, double* elem_ground) {
+void friction_energy_partials(Context& c, double* part, double* elem_self, double* elem_ground, hipStream_t st) {
     Scene& s = c.sc;
-    hipLaunchKernelGGL(e_friction, dim3(kFB), dim3(kFB), 0, c.stream, (int)c.n_fself, (int)c.n_fenv, c.fr_self.p, c.fr_lam.p,
+    hipLaunchKernelGGL(e_friction, dim3(kFB), dim3(kFB), 0, st, (int)c.n_fself, (int)c.n_fenv, c.fr_self.p, c.fr_lam.p,
                        c.fr_coord.p, c.fr_basis.p, c.fr_env.p, c.fr_env_lam.p, s.x.p, s.vprev.p, s.gn, s.fhat * s.dt * s.dt, s.fric,
                        part, elem_self, elem_ground);
     CIPC_CHECK_LAUNCH();

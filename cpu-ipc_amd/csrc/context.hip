@@ -937,10 +937,12 @@ int cipc_gpu_newton_iteration(cipc_gpu_context* c, double kappa, cipc_iteration_
         require(c && c->have_scene && t, "no scene uploaded or null output");
         require_single(c, "cipc_gpu_newton_iteration");
         bind(*c);
-        hipEvent_t ev[5], es, ee;
+        hipEvent_t ev[5], es, ee, e0, en;
         for (auto& e : ev) CIPC_HIP(hipEventCreate(&e));
         CIPC_HIP(hipEventCreate(&es));
         CIPC_HIP(hipEventCreate(&ee));
+        CIPC_HIP(hipEventCreate(&e0));
+        CIPC_HIP(hipEventCreate(&en));
         CIPC_HIP(hipEventRecord(ev[0], c->stream));
         elastic_fork(c, es, ee);          // overlaps the active set and the contact blocks
         compute_active_set(*c);
@@ -949,10 +951,17 @@ int cipc_gpu_newton_iteration(cipc_gpu_context* c, double kappa, cipc_iteration_
         elastic_join(c);
         finalize_csr(*c);
         CIPC_HIP(hipEventRecord(ev[2], c->stream));
-        // step bound and energy with one synchronisation: their reads are asynchronous
+        // step bound and energy with one synchronisation: both read the state and the active
+        // set only, so the energy runs on the auxiliary stream beside the CCD kernels, and
+        // their results come back through asynchronous pinned reads
+        CIPC_HIP(hipEventRecord(c->fork, c->stream));
+        CIPC_HIP(hipStreamWaitEvent(c->aux, c->fork, 0));
+        CIPC_HIP(hipEventRecord(e0, c->aux));
+        energy_terms_launch(*c, c->aux);
+        CIPC_HIP(hipEventRecord(en, c->aux));
         step_bound_launch(*c);
         CIPC_HIP(hipEventRecord(ev[3], c->stream));
-        energy_terms_launch(*c);
+        CIPC_HIP(hipStreamWaitEvent(c->stream, en, 0));
         CIPC_HIP(hipEventRecord(ev[4], c->stream));
         CIPC_HIP(hipEventSynchronize(ev[4]));
         settle_nnz(*c);
@@ -960,13 +969,16 @@ int cipc_gpu_newton_iteration(cipc_gpu_context* c, double kappa, cipc_iteration_
         step_bound_finish(*c, sb);
         energy_terms_finish(*c, et);
         float ms[4];
-        for (int i = 0; i < 4; ++i) CIPC_HIP(hipEventElapsedTime(&ms[i], ev[i], ev[i + 1]));
+        for (int i = 0; i < 3; ++i) CIPC_HIP(hipEventElapsedTime(&ms[i], ev[i], ev[i + 1]));
+        CIPC_HIP(hipEventElapsedTime(&ms[3], e0, en));  // energy, overlapped with the step bound
         float tot = 0, el = 0;
         CIPC_HIP(hipEventElapsedTime(&tot, ev[0], ev[4]));
         CIPC_HIP(hipEventElapsedTime(&el, es, ee));
         for (auto& e : ev) (void)hipEventDestroy(e);
         (void)hipEventDestroy(es);
         (void)hipEventDestroy(ee);
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(en);
         t->broadphase_ms = ms[0]; t->assembly_ms = ms[1]; t->elastic_ms = el; t->ccd_ms = ms[2]; t->energy_ms = ms[3];
         t->total_ms = tot;
         t->alpha = sb[0];

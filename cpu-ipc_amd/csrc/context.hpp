@@ -219,13 +219,13 @@ void step_bound_scenes(Context& c, double* out);
 // energy.hip
 void energy_terms(Context& c, double kappa, double out[9]);
 // energy_terms in two halves, as step_bound_launch / step_bound_finish
-void energy_terms_launch(Context& c);
+void energy_terms_launch(Context& c, hipStream_t st);
 void energy_terms_finish(Context& c, double out[9]);
 // after a synchronisation of the stream: the CSR nonzero count of the last finalize_csr
 inline void settle_nnz(Context& c) {
     if (c.nnz_pending) { c.nnz = *c.pin_nnz.p; c.nnz_pending = false; }
 }
-void cloth_energy_partials(Context& c, double* part_bw, double* part_bend);
+void cloth_energy_partials(Context& c, double* part_bw, double* part_bend, hipStream_t st);
 double total_energy(const Context& c, double kappa, const double t[9]);
 // per scene: out[9 k .. 9 k + 8] as energy_terms' nine values
 void energy_terms_scenes(Context& c, double* out);
@@ -260,7 +260,7 @@ double learned_train_step(Context& c, float* params, float* m1, float* m2, int s
                           int objective);
 // Friction records after the contact records; returns the friction energy parts' launch
 void friction_records(Context& c, size_t slot_b, size_t slot_g);
-void friction_energy_partials(Context& c, double* part, double* elem_self = nullptr, double* elem_ground = nullptr);
+void friction_energy_partials(Context& c, double* part, double* elem_self, double* elem_ground, hipStream_t st);
 void step_forward(Context& c, double alpha);
 // per scene: x = x0 - alpha[scene] p
 void step_forward_scenes(Context& c, const double* alpha);

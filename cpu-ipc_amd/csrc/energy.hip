@@ -141,36 +141,36 @@ __global__ __launch_bounds__(kEB) void sum_partials(const double* __restrict__ p
 void energy_terms(Context& c, double kappa, double out[9]) {
     (void)kappa;
     if (c.sc.S != 1) throw Error(CIPC_ERR_LOGIC, "a batch has one energy per scene");
-    energy_terms_launch(c);
+    energy_terms_launch(c, c.stream);
     CIPC_HIP(hipStreamSynchronize(c.stream));
     energy_terms_finish(c, out);
 }
 
-void energy_terms_launch(Context& c) {
+void energy_terms_launch(Context& c, hipStream_t st) {
     Scene& s = c.sc;
     static_assert(kEG == kEB, "partials are summed by one work-group");
     DBuf<double>& part = c.dscratch;
     part.resize((size_t)11 * kEG);
     double* P = part.p;
-    hipLaunchKernelGGL(e_snh, dim3(kEG), dim3(kEB), 0, c.stream, s.T, s.tets.p, s.x.p, s.dminv.p, s.vol.p, scene_lame(s), P,
+    hipLaunchKernelGGL(e_snh, dim3(kEG), dim3(kEB), 0, st, s.T, s.tets.p, s.x.p, s.dminv.p, s.vol.p, scene_lame(s), P,
                        nullptr);
-    hipLaunchKernelGGL(e_inertia, dim3(kEG), dim3(kEB), 0, c.stream, s.V, s.x.p, s.xt.p, s.vprev.p, s.mass.p, s.drag, P + kEG,
+    hipLaunchKernelGGL(e_inertia, dim3(kEG), dim3(kEB), 0, st, s.V, s.x.p, s.xt.p, s.vprev.p, s.mass.p, s.drag, P + kEG,
                        nullptr);
-    hipLaunchKernelGGL(e_ground, dim3(kEG), dim3(kEB), 0, c.stream, (int)c.n_env, c.env_set.p, s.x.p, s.gn, s.gd, s.hhat, P + 2 * kEG,
+    hipLaunchKernelGGL(e_ground, dim3(kEG), dim3(kEB), 0, st, (int)c.n_env, c.env_set.p, s.x.p, s.gn, s.gd, s.hhat, P + 2 * kEG,
                        nullptr);
-    hipLaunchKernelGGL(e_self, dim3(kEG), dim3(kEB), 0, c.stream, (int)c.n_self, c.self_set.p, s.x.p, s.hhat, P + 3 * kEG, nullptr);
-    hipLaunchKernelGGL(e_mollified, dim3(kEG), dim3(kEB), 0, c.stream, (int)c.n_ee, c.ee_set.p, c.ee_ie.p, s.se.p, s.x.p, s.vrest.p,
+    hipLaunchKernelGGL(e_self, dim3(kEG), dim3(kEB), 0, st, (int)c.n_self, c.self_set.p, s.x.p, s.hhat, P + 3 * kEG, nullptr);
+    hipLaunchKernelGGL(e_mollified, dim3(kEG), dim3(kEB), 0, st, (int)c.n_ee, c.ee_set.p, c.ee_ie.p, s.se.p, s.x.p, s.vrest.p,
                        s.hhat, P + 4 * kEG, nullptr);
     // lagged friction: self part (lambda times clamped norm) and ground part (mu included)
-    friction_energy_partials(c, P + 5 * kEG);
+    friction_energy_partials(c, P + 5 * kEG, nullptr, nullptr, st);
     // cloth triangles and bending (IPCSolver.cpp:330-368)
-    cloth_energy_partials(c, P + 7 * kEG, P + 8 * kEG);
+    cloth_energy_partials(c, P + 7 * kEG, P + 8 * kEG, st);
     // soft targets (softEnergy, BoundaryConditions.cpp:210-229)
-    hipLaunchKernelGGL(e_soft, dim3(kEG), dim3(kEB), 0, c.stream, c.n_soft, c.soft_idx.p, c.soft_t.p, c.soft_w, s.x.p, P + 9 * kEG);
-    hipLaunchKernelGGL(sum_partials, dim3(1), dim3(kEB), 0, c.stream, P, 10, P + 10 * kEG);
+    hipLaunchKernelGGL(e_soft, dim3(kEG), dim3(kEB), 0, st, c.n_soft, c.soft_idx.p, c.soft_t.p, c.soft_w, s.x.p, P + 9 * kEG);
+    hipLaunchKernelGGL(sum_partials, dim3(1), dim3(kEB), 0, st, P, 10, P + 10 * kEG);
     CIPC_CHECK_LAUNCH();
     c.pin_energy.reserve(10);
-    CIPC_HIP(hipMemcpyAsync(c.pin_energy.p, P + 10 * kEG, 10 * sizeof(double), hipMemcpyDeviceToHost, c.stream));
+    CIPC_HIP(hipMemcpyAsync(c.pin_energy.p, P + 10 * kEG, 10 * sizeof(double), hipMemcpyDeviceToHost, st));
 }
 
 void energy_terms_finish(Context& c, double out[9]) {
@@ -315,7 +315,7 @@ void energy_terms_scenes(Context& c, double* out) {
     if (nE)
         hipLaunchKernelGGL(e_mollified, dim3(kEG), dim3(kEB), 0, c.stream, (int)nE, c.ee_set.p, c.ee_ie.p, s.se.p, s.x.p, s.vrest.p,
                            s.hhat, nullptr, eE);
-    if (nFs + nFg) friction_energy_partials(c, nullptr, eFs, eFg);
+    if (nFs + nFg) friction_energy_partials(c, nullptr, eFs, eFg, c.stream);
     CIPC_CHECK_LAUNCH();
     // device offsets of tets and vertices follow the output block
     DBuf<int>& off = c.batch_off;

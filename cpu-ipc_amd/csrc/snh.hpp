@@ -50,7 +50,9 @@ __device__ inline void svd3(const M33& F, M33& U, double s[3], M33& V) {
     d3 w0 = {1, 0, 0}, w1 = {0, 1, 0}, w2 = {0, 0, 1};
     auto rot = [](d3& bp, d3& bq, d3& wp, d3& wq) -> bool {
         const double al = sqn(bp), be = sqn(bq), ga = dot(bp, bq);
-        if (ga == 0.0 || fabs(ga) <= 1e-17 * sqrt(al * be)) return false;
+        // converged to the rounding level: |ga| <= 4e-16 sqrt(al be), in squares (one sqrt less);
+        // a threshold below the FP64 unit roundoff is never met and would run every sweep
+        if (ga * ga <= 1.6e-31 * (al * be)) return false;
         const double zeta = (be - al) / (2.0 * ga);
         const double t = (zeta >= 0 ? 1.0 : -1.0) / (fabs(zeta) + sqrt(1.0 + zeta * zeta));
         const double c = 1.0 / sqrt(1.0 + t * t), sn = c * t;

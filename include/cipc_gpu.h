@@ -86,7 +86,7 @@ typedef struct {
     double assembly_ms;   /* gradient + PSD Hessian -> lower CSR (IPCSolver.cpp:918-930 less the solve) */
     double elastic_ms;    /* inertia + tet part of the assembly (element kernel and its gathers) */
     double ccd_ms;        /* ground + partial CCD + CFL + swept LBVH + full CCD (IPCSolver.cpp:943-990) */
-    double energy_ms;     /* one line-search energy evaluation (IPCSolver.cpp:322-456) */
+    double energy_ms;     /* one line-search energy evaluation (IPCSolver.cpp:322-456), overlapped with the step bound on a second stream */
     double total_ms;
     double alpha;         /* merged step bound */
     double energy;        /* energy at the current positions */
