@@ -205,6 +205,8 @@ inline Kap scene_kappa(const Scene& s, double k) { return Kap{k, s.S > 1 ? s.kap
 inline Lame scene_lame(const Scene& s) { return Lame{s.u, s.r, s.S > 1 ? s.tmu.p : nullptr, s.S > 1 ? s.tlam.p : nullptr}; }
 inline const int* scene_of_vertex(const Scene& s) { return s.S > 1 ? s.vscene.p : nullptr; }
 
+constexpr int kChunk = 64;   // tets per chunk of the chunked tet assembly: one wavefront
+
 // Static elastic/mass block pattern and gather lists (built once per scene).
 struct StaticPattern {
     int nblocks = 0;
@@ -217,6 +219,25 @@ struct StaticPattern {
     DBuf<int> slot_g;       // 4 per element: slot of (element, corner)
     DBuf<int> vtet;         // V: tet corners of vertex v (the first slots of its run)
     DBuf<int> diag_block;   // V: block index of (v, v) in the static pattern
+    // host copies for the chunk plan: the block of every contribution slot, and slot_h
+    std::vector<int> h_blk_of_slot, h_slot_h;
+
+    // Chunked tet assembly (scenes whose elements are all tets): the tets of each scene in
+    // Morton order of their centroids, cut into chunks of kChunk tets, one wavefront per
+    // chunk and one lane per tet.  A chunk sums its tets' blocks in LDS accumulators, one per
+    // distinct block it touches; a block touched by one chunk only is written straight into
+    // the block values, the others leave one partial per chunk that the completion pass adds
+    // in chunk order.
+    bool chunked = false;
+    int chunk = 0, nchunks = 0, lds_bytes = 0, ncomplete = 0;
+    DBuf<int> ch_tet;               // nchunks*chunk: tet of each lane, -1 for padding
+    DBuf<unsigned short> ch_code;   // ((ch*10 + lb)*chunk + lane): accumulator << 6 | round; 0xFFFF: none
+    DBuf<unsigned char> ch_rounds;  // ch*16 + lb: rounds of that phase (16 bytes per chunk)
+    DBuf<int> ch_acc_ptr;           // nchunks+1: accumulators of chunk ch in [ptr[ch], ptr[ch+1])
+    DBuf<int> ch_acc_dst;           // per accumulator: block index (>= 0), or -(partial + 1)
+    DBuf<int> cp_blk, cp_ptr;       // completion: block, and its partials in [cp_ptr[i], cp_ptr[i+1])
+    DBuf<double> part;              // 9 per partial
+    DBuf<unsigned short> partf;     // nonzero masks of the partials
 };
 
 }  // namespace cipc

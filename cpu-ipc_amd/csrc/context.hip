@@ -99,7 +99,7 @@ int cipc_gpu_context_destroy(cipc_gpu_context* c) {
 }  // extern "C"
 
 namespace {
-void upload_scene_impl(cipc_gpu_context* c, const cipc_scene_desc* d) {
+void upload_scene_impl(cipc_gpu_context* c, const cipc_scene_desc* d, bool plan = true) {
     {
         require(c && d, "null context or descriptor");
         require(d->V > 0 && d->T >= 0 && d->F >= 0 && d->Vs >= 0 && d->Es >= 0, "invalid scene counts");
@@ -236,6 +236,7 @@ void upload_scene_impl(cipc_gpu_context* c, const cipc_scene_desc* d) {
         s.S = 1;
         s.h_voff = {0, V};
         s.h_toff = {0, T};
+        if (plan) build_tet_chunks(*c, d->x);
     }
 }
 
@@ -326,12 +327,13 @@ int cipc_gpu_upload_batch(cipc_gpu_context* c, const cipc_scene_desc* const* sce
         BatchPack b;
         cipc_scene_desc u;
         pack_batch(scenes, n_scenes, b, u);
-        upload_scene_impl(c, &u);
+        upload_scene_impl(c, &u, false);
         Scene& s = c->sc;
         const hipStream_t st = c->stream;
         s.S = n_scenes;
         s.h_voff = b.voff;
         s.h_toff = b.toff;
+        build_tet_chunks(*c, b.x.data());
         upload(s.vscene, b.vscene.data(), b.vscene.size(), st);
         upload(s.tmu, b.tmu.data(), b.tmu.size(), st);
         upload(s.tlam, b.tlam.data(), b.tlam.size(), st);

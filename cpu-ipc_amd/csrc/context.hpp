@@ -180,6 +180,8 @@ struct Context {
 
 // elastic.hip
 void build_static_pattern(Context& c);
+// chunk plan of the tet assembly from the host positions (after the scene ranges are set)
+void build_tet_chunks(Context& c, const double* hx);
 void elastic_assemble(Context& c, hipStream_t st);
 void cloth_assemble(Context& c, hipStream_t st);
 // contact.hip

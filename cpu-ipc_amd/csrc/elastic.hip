@@ -117,11 +117,162 @@ void build_static_pattern(Context& c) {
     std::vector<int> vtet(V, 0);
     for (int i = 0; i < 4 * T; ++i) vtet[s.h_tets[i]]++;
     up(sp.vtet, vtet);
-    c.stagH.resize((size_t)9 * std::max<size_t>(gsrc.size(), 1));
+    sp.h_blk_of_slot.assign(gsrc.size(), 0);
+    for (int k = 0; k < sp.nblocks; ++k)
+        for (int j = gptr[k]; j < gptr[k + 1]; ++j) sp.h_blk_of_slot[j] = k;
+    sp.h_slot_h.swap(slot_h);
+    sp.chunked = false;     // build_tet_chunks decides, once the scene ranges are known
     c.stagG.resize((size_t)3 * std::max<size_t>(vsrc.size(), 1));
     c.sbv.resize((size_t)9 * std::max(sp.nblocks, 1));
     c.sbf.resize(std::max(sp.nblocks, 1));
     c.grad.resize((size_t)3 * std::max(V, 1));
+}
+
+// ---------------------------------------------------------------------------------
+// Chunk plan of the tet assembly.  The summation order of a block is fixed by the plan:
+// within a chunk, phase by phase (local block 0..9), and within a phase by rounds that
+// take the chunk's lanes in order; across chunks, in chunk order.  The plan depends only
+// on each scene's own tets, so a scene packed into a batch is summed exactly as alone.
+// ---------------------------------------------------------------------------------
+namespace {
+constexpr int kAccBytes = 9 * 8 + 2;   // one accumulator: 9 doubles and a 16-bit mask
+
+uint32_t morton30(double px, double py, double pz, const double lo[3], double ext) {
+    auto q = [&](double v, int k) {
+        const double t = ext > 0 ? (v - lo[k]) / ext : 0.0;
+        return (uint32_t)std::max(0.0, std::min(t * 1024.0, 1023.0));
+    };
+    auto ex = [](uint32_t v) {
+        v = (v * 0x00010001u) & 0xFF0000FFu; v = (v * 0x00000101u) & 0x0F00F00Fu;
+        v = (v * 0x00000011u) & 0xC30C30C3u; v = (v * 0x00000005u) & 0x49249249u;
+        return v;
+    };
+    return (ex(q(px, 0)) << 2) | (ex(q(py, 1)) << 1) | ex(q(pz, 2));
+}
+}  // namespace
+
+void build_tet_chunks(Context& c, const double* hx) {
+    Scene& s = c.sc;
+    StaticPattern& sp = c.sp;
+    const int T = s.T;
+    sp.chunked = false;
+    auto staged = [&] { c.stagH.resize((size_t)9 * std::max<size_t>(sp.h_blk_of_slot.size(), 1)); };
+    if (T == 0 || s.Tr > 0 || s.Nb > 0 || !hx) { staged(); return; }
+    // tets of each scene in Morton order of their centroids (the scene's own box), ties by index
+    std::vector<int> order(T);
+    const std::vector<int>& toff = s.h_toff;
+    for (size_t k = 0; k + 1 < toff.size(); ++k) {
+        const int t0 = toff[k], t1 = toff[k + 1];
+        if (t1 <= t0) continue;
+        std::vector<double> cen(3 * (size_t)(t1 - t0));
+        double lo[3] = {1e300, 1e300, 1e300}, hi[3] = {-1e300, -1e300, -1e300};
+        for (int t = t0; t < t1; ++t)
+            for (int q = 0; q < 3; ++q) {
+                double m = 0;
+                for (int a = 0; a < 4; ++a) m += hx[3 * (size_t)s.h_tets[4 * (size_t)t + a] + q];
+                m *= 0.25;
+                cen[3 * (size_t)(t - t0) + q] = m;
+                lo[q] = std::min(lo[q], m); hi[q] = std::max(hi[q], m);
+            }
+        const double ext = std::max(hi[0] - lo[0], std::max(hi[1] - lo[1], hi[2] - lo[2]));
+        std::vector<std::pair<uint32_t, int>> o((size_t)(t1 - t0));
+        for (int t = t0; t < t1; ++t) {
+            const double* m = &cen[3 * (size_t)(t - t0)];
+            o[t - t0] = {morton30(m[0], m[1], m[2], lo, ext), t};
+        }
+        std::sort(o.begin(), o.end());
+        for (int t = t0; t < t1; ++t) order[t] = o[t - t0].second;
+    }
+    auto target = [&](int t, int lb) {
+        const int j = sp.h_slot_h[10 * (size_t)t + lb];
+        return j < 0 ? -1 : sp.h_blk_of_slot[j];
+    };
+    // one wavefront per chunk: the rounds need no work-group barrier
+    for (int C : {kChunk}) {
+        std::vector<int> ch_first;   // first position in `order` of every chunk
+        for (size_t k = 0; k + 1 < toff.size(); ++k)
+            for (int p = toff[k]; p < toff[k + 1]; p += C) ch_first.push_back(p);
+        const int nch = (int)ch_first.size();
+        auto chunk_end = [&](int ch) {
+            const int p = ch_first[ch];
+            const int k = (int)(std::upper_bound(toff.begin(), toff.end(), p) - toff.begin()) - 1;
+            return std::min(p + C, toff[k + 1]);
+        };
+        std::vector<int> ch_tet((size_t)nch * C, -1), acc_ptr(nch + 1, 0), acc_blk;
+        std::vector<unsigned short> code((size_t)nch * 10 * C, 0xFFFF);
+        std::vector<unsigned char> rounds((size_t)nch * 16, 0);   // 10 used, padded to 16 bytes
+        bool fits = true;
+        std::vector<int> tg, cnt;
+        for (int ch = 0; ch < nch && fits; ++ch) {
+            const int p0 = ch_first[ch], p1 = chunk_end(ch), n = p1 - p0;
+            tg.clear();
+            for (int l = 0; l < n; ++l) {
+                ch_tet[(size_t)ch * C + l] = order[p0 + l];
+                for (int lb = 0; lb < 10; ++lb) {
+                    const int b = target(order[p0 + l], lb);
+                    if (b >= 0) tg.push_back(b);
+                }
+            }
+            std::sort(tg.begin(), tg.end());
+            tg.erase(std::unique(tg.begin(), tg.end()), tg.end());
+            const int na = (int)tg.size();
+            if (na > 1023) { fits = false; break; }   // the 10-bit accumulator field (64 x 10 = 640 at most)
+            acc_ptr[ch + 1] = acc_ptr[ch] + na;
+            acc_blk.insert(acc_blk.end(), tg.begin(), tg.end());
+            cnt.assign(na, 0);
+            for (int lb = 0; lb < 10; ++lb) {
+                std::fill(cnt.begin(), cnt.end(), 0);
+                int rmax = 0;
+                for (int l = 0; l < n; ++l) {
+                    const int b = target(order[p0 + l], lb);
+                    if (b < 0) continue;
+                    const int a = (int)(std::lower_bound(tg.begin(), tg.end(), b) - tg.begin());
+                    const int r = cnt[a]++;
+                    if (r > 63) { fits = false; break; }
+                    code[((size_t)ch * 10 + lb) * C + l] = (unsigned short)((a << 6) | r);
+                    rmax = std::max(rmax, r + 1);
+                }
+                rounds[(size_t)ch * 16 + lb] = (unsigned char)rmax;
+            }
+        }
+        if (!fits) continue;
+        // blocks touched by one chunk are written by it; the others (and blocks no tet
+        // touches) are completed from their partials, in chunk order
+        const int nb = sp.nblocks;
+        std::vector<int> nch_of(nb, 0);
+        for (int b : acc_blk) nch_of[b]++;
+        std::vector<int> cp_blk, cp_ptr(1, 0), cursor(nb, -1);
+        for (int b = 0; b < nb; ++b)
+            if (nch_of[b] != 1) {
+                cursor[b] = cp_ptr.back();
+                cp_blk.push_back(b);
+                cp_ptr.push_back(cp_ptr.back() + nch_of[b]);
+            }
+        std::vector<int> acc_dst(acc_blk.size());
+        for (size_t i = 0; i < acc_blk.size(); ++i) {   // chunks in order: partial runs in chunk order
+            const int b = acc_blk[i];
+            acc_dst[i] = nch_of[b] == 1 ? b : -(cursor[b]++ + 1);
+        }
+        int maxacc = 0;
+        for (int ch = 0; ch < nch; ++ch) maxacc = std::max(maxacc, acc_ptr[ch + 1] - acc_ptr[ch]);
+        auto up = [&](auto& d, const auto& h) {
+            d.resize(std::max<size_t>(h.size(), 1));
+            if (!h.empty()) CIPC_HIP(hipMemcpy(d.p, h.data(), h.size() * sizeof(h[0]), hipMemcpyHostToDevice));
+        };
+        up(sp.ch_tet, ch_tet); up(sp.ch_code, code); up(sp.ch_rounds, rounds);
+        up(sp.ch_acc_ptr, acc_ptr); up(sp.ch_acc_dst, acc_dst);
+        up(sp.cp_blk, cp_blk); up(sp.cp_ptr, cp_ptr);
+        sp.part.resize((size_t)9 * std::max(cp_ptr.back(), 1));
+        sp.partf.resize((size_t)std::max(cp_ptr.back(), 1));
+        sp.chunk = C;
+        sp.nchunks = nch;
+        sp.ncomplete = (int)cp_blk.size();
+        sp.lds_bytes = ((maxacc * 9 * 8 + maxacc * 2 + 15) / 16) * 16;
+        sp.chunked = true;
+        c.stagH.release();
+        return;
+    }
+    staged();
 }
 
 // ---------------------------------------------------------------------------------
@@ -148,34 +299,123 @@ __device__ inline void load_tet(int t, int T, const int4* __restrict__ tets, con
     B[3] = d3{D[2][0], D[2][1], D[2][2]};
 }
 
+// Per-tet state of the block formula, in the frames of F = U diag(s) V^T (U, V rotations):
+// with beta_a = V^T B_a, F B_a = U (s o beta_a), cof F B_a = U (s' o beta_a) (s' = s1 s2,
+// s0 s2, s0 s1), B_a . B_b = beta_a . beta_b and F (B_a x B_b) = U (s o (beta_a x beta_b)),
+// so every block is K_ab = U X_ab U^T with
+//   X_ab = A (beta_a . beta_b) I - a2 hat(s o (beta_a x beta_b)) + 4 a3 (s o beta_a)(s o beta_b)^T
+//          + r (s' o beta_a)(s' o beta_b)^T + P_ab,
+// P_ab the projection of the negative modes (SNH::Sw).  Identical in real arithmetic to the
+// world-frame form of the element stiffness; 44 doubles of state per tet instead of 90.
+struct TetSetup {
+    double beta[4][3];
+    M33 U;
+    double s[3], sc3[3];   // singular values (s2 signed) and their cofactor products s'
+    double A, a2, c1, r;   // Hessian coefficients, c1 = 4 a3
+    double Sw[3][3], pp[3], pm[3];
+    bool project;
+    int vid[4];
+    double sc;             // vol dt^2
+};
+
+// Per-tet set-up shared by the staged and the chunked kernels: SNH evaluation and the corner
+// forces f_a = vol P B_a written to their gradient slots (dt^2 is applied by the vertex
+// gather, as IPCSolver.cpp:149-151).
+__device__ inline void tet_setup(TetSetup& q, int t, int T, const int4* __restrict__ tets, const double* __restrict__ x,
+                                 const double* __restrict__ dminv, const double* __restrict__ vol, const Lame& lame, double dt2,
+                                 const int* __restrict__ slot_g, double* __restrict__ stagG, bool write_forces) {
+    SNH e;
+    d3 B[4];
+    load_tet(t, T, tets, x, dminv, e.F, B);
+    const int4 tv = tets[t];
+    q.vid[0] = tv.x; q.vid[1] = tv.y; q.vid[2] = tv.z; q.vid[3] = tv.w;
+    double u, r;
+    lame.at(t, u, r);
+    snh_eval(e, u, r);
+    const double vt = vol[t];
+    if (write_forces) {
+#pragma unroll
+        for (int a = 0; a < 4; ++a) {
+            const d3 f = vt * (e.pa * mvec(e.F, B[a]) + e.pb * mvec(e.C, B[a]));
+            st3(stagG, slot_g[4 * t + a], f);
+        }
+    }
+#pragma unroll
+    for (int a = 0; a < 4; ++a)
+#pragma unroll
+        for (int k = 0; k < 3; ++k) q.beta[a][k] = dot(mcol(e.V, k), B[a]);
+    q.U = e.U;
+#pragma unroll
+    for (int k = 0; k < 3; ++k) q.s[k] = e.s[k];
+    q.sc3[0] = e.s[1] * e.s[2]; q.sc3[1] = e.s[0] * e.s[2]; q.sc3[2] = e.s[0] * e.s[1];
+    q.A = e.A; q.a2 = e.a2; q.c1 = 4.0 * e.a3; q.r = e.r;
+    q.project = e.project;
+#pragma unroll
+    for (int i = 0; i < 3; ++i) {
+        q.pp[i] = e.pp[i]; q.pm[i] = e.pm[i];
+#pragma unroll
+        for (int j = 0; j < 3; ++j) q.Sw[i][j] = e.Sw[i][j];
+    }
+    q.sc = vt * dt2;
+}
+
 // Block K_ab (3x3) of the projected element stiffness before the vol dt^2 factor.
-__device__ inline void snh_block(const SNH& e, const d3 B[4], const d3 G1[4], const d3 GJ[4],
-                                 const double beta[4][3], int a, int b, double K[3][3]) {
-    const double bb = dot(B[a], B[b]);
-    const d3 w = mvec(e.F, cross(B[a], B[b]));
-    const double h[3][3] = {{0, -w.z, w.y}, {w.z, 0, -w.x}, {-w.y, w.x, 0}};
-    const double c1 = 4.0 * e.a3;
+__device__ inline void snh_block(const TetSetup& q, int a, int b, double K[3][3]) {
+    const double* ba = q.beta[a];
+    const double* bb = q.beta[b];
+    const double dab = ba[0] * bb[0] + ba[1] * bb[1] + ba[2] * bb[2];
+    const double w[3] = {q.s[0] * (ba[1] * bb[2] - ba[2] * bb[1]), q.s[1] * (ba[2] * bb[0] - ba[0] * bb[2]),
+                         q.s[2] * (ba[0] * bb[1] - ba[1] * bb[0])};
+    const double h[3][3] = {{0, -w[2], w[1]}, {w[2], 0, -w[0]}, {-w[1], w[0], 0}};
+    double X[3][3];
 #pragma unroll
     for (int i = 0; i < 3; ++i)
 #pragma unroll
         for (int j = 0; j < 3; ++j)
-            K[i][j] = (i == j ? e.A * bb : 0.0) - e.a2 * h[i][j] + c1 * comp(G1[a], i) * comp(G1[b], j)
-                    + e.r * comp(GJ[a], i) * comp(GJ[b], j);
-    if (e.project) {
+            X[i][j] = (i == j ? q.A * dab : 0.0) - q.a2 * h[i][j] + q.c1 * (q.s[i] * ba[i]) * (q.s[j] * bb[j])
+                    + q.r * (q.sc3[i] * ba[i]) * (q.sc3[j] * bb[j]);
+    if (q.project) {
 #pragma unroll
-        for (int m = 0; m < 9; ++m) {
-            if (e.lam[m] < 0.0) {
-                const d3 qa = snh_mode_apply(e, m, beta[a]);
-                const d3 qb = snh_mode_apply(e, m, beta[b]);
+        for (int i = 0; i < 3; ++i)
 #pragma unroll
-                for (int i = 0; i < 3; ++i)
+            for (int j = 0; j < 3; ++j) X[i][j] += q.Sw[i][j] * ba[i] * bb[j];
 #pragma unroll
-                    for (int j = 0; j < 3; ++j) K[i][j] -= e.lam[m] * comp(qa, i) * comp(qb, j);
-            }
+        for (int k = 0; k < 3; ++k) {
+            const int i = (k == 0) ? 1 : 0, j = (k == 2) ? 1 : 2;
+            X[i][i] += q.pp[k] * ba[j] * bb[j];
+            X[j][j] += q.pp[k] * ba[i] * bb[i];
+            X[i][j] += q.pm[k] * ba[j] * bb[i];
+            X[j][i] += q.pm[k] * ba[i] * bb[j];
         }
     }
+    double M[3][3];   // U X
+#pragma unroll
+    for (int r = 0; r < 3; ++r)
+#pragma unroll
+        for (int j = 0; j < 3; ++j) M[r][j] = q.U.a[r][0] * X[0][j] + q.U.a[r][1] * X[1][j] + q.U.a[r][2] * X[2][j];
+#pragma unroll
+    for (int r = 0; r < 3; ++r)
+#pragma unroll
+        for (int c = 0; c < 3; ++c) K[r][c] = M[r][0] * q.U.a[c][0] + M[r][1] * q.U.a[c][1] + M[r][2] * q.U.a[c][2];
 }
 
+// Local block lb as stored (row vertex max(va, vb), column vertex min): vol dt^2 K_ab, or
+// its transpose when va < vb (a diagonal block is symmetric and stored whole), rounded to
+// the staging type H.
+template <typename H>
+__device__ inline void tet_block(const TetSetup& q, int lb, double o[9]) {
+    const int a = lb_a(lb), b = lb_b(lb);
+    double K[3][3];
+    snh_block(q, a, b, K);
+    const bool tr = q.vid[a] < q.vid[b];
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+        for (int k = 0; k < 3; ++k) o[3 * i + k] = (double)(H)(q.sc * (tr ? K[k][i] : K[i][k]));
+}
+
+// Staged form (scenes with cloth elements): every (tet, local block) and (tet, corner) has
+// a slot fixed when the pattern is built; the block gather sums the slots of a block.
 // H: staging type of the Hessian blocks (double; float for the mixed form)
 template <typename H>
 __global__ __launch_bounds__(256) void snh_element_kernel(int T, const int4* __restrict__ tets, const double* __restrict__ x,
@@ -185,47 +425,134 @@ __global__ __launch_bounds__(256) void snh_element_kernel(int T, const int4* __r
                                                           double* __restrict__ stagG) {
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= T) return;
-    SNH e;
-    d3 B[4];
-    load_tet(t, T, tets, x, dminv, e.F, B);
-    const int4 tv = tets[t];
-    const int vid[4] = {tv.x, tv.y, tv.z, tv.w};
-    double u, r;
-    lame.at(t, u, r);
-    snh_eval(e, u, r);
-    const double vt = vol[t];
-    // gradient f_a = vol P B_a (dt^2 is applied by the vertex gather, as IPCSolver.cpp:149-151)
-#pragma unroll
-    for (int a = 0; a < 4; ++a) {
-        const d3 f = vt * (e.pa * mvec(e.F, B[a]) + e.pb * mvec(e.C, B[a]));
-        st3(stagG, slot_g[4 * t + a], f);
-    }
-    d3 G1[4], GJ[4];
-    double beta[4][3];
-#pragma unroll
-    for (int a = 0; a < 4; ++a) {
-        G1[a] = mvec(e.F, B[a]);
-        GJ[a] = mvec(e.C, B[a]);
-#pragma unroll
-        for (int k = 0; k < 3; ++k) beta[a][k] = dot(mcol(e.V, k), B[a]);
-    }
-    const double sc = vt * dt2;
+    TetSetup q;
+    tet_setup(q, t, T, tets, x, dminv, vol, lame, dt2, slot_g, stagG, true);
 #pragma unroll
     for (int lb = 0; lb < 10; ++lb) {
         const int j = slot_h[10 * t + lb];
         if (j < 0) continue;   // a vertex of the block is not free
-        const int a = lb_a(lb), b = lb_b(lb);
-        double K[3][3];
-        snh_block(e, B, G1, GJ, beta, a, b, K);
-        // stored block (row vertex max(va, vb), column vertex min): K_ab, or K_ab^T when
-        // va < vb; a diagonal block is symmetric and stored whole
-        const bool tr = vid[a] < vid[b];
-        H* o = stagH + (size_t)9 * j;
+        double o[9];
+        tet_block<H>(q, lb, o);
+        H* d = stagH + (size_t)9 * j;
 #pragma unroll
-        for (int i = 0; i < 3; ++i)
-#pragma unroll
-            for (int k = 0; k < 3; ++k) o[3 * i + k] = (H)(sc * (tr ? K[k][i] : K[i][k]));
+        for (int i = 0; i < 9; ++i) d[i] = (H)o[i];
     }
+}
+
+// Orders the LDS accesses of one wavefront: a compiler fence only.  A wavefront's LDS
+// instructions execute in issue order, so what a lane wrote before this point is what any
+// lane of the same wavefront reads after it.
+__device__ inline void wave_lds_order() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// One phase of the chunked form: local block LB of every lane's tet into the chunk's
+// accumulators, round by round (LB is a template argument so that every index into the
+// tet's state is static).
+template <typename H, int LB>
+__device__ __forceinline__ void chunk_phase(const TetSetup& q, const unsigned cdp[5], const unsigned rdp[4], double* acc,
+                                            unsigned short* msk) {
+    const unsigned cd = (cdp[LB / 2] >> (16 * (LB % 2))) & 0xFFFFu;
+    const int R = (int)((rdp[LB / 4] >> (8 * (LB % 4))) & 0xFFu);
+    double o[9];
+    // (a block outside the pattern, or a padding lane, computes nothing; the branch also
+    // keeps the phases' arithmetic apart, which bounds the register pressure)
+    if (cd != 0xFFFFu) tet_block<H>(q, LB, o);
+    const int mine = cd == 0xFFFFu ? -1 : (int)(cd & 63u);
+    const unsigned a = cd >> 6;
+    for (int r = 0; r < R; ++r) {
+        if (mine == r) {
+            unsigned m = 0;
+#pragma unroll
+            for (int i = 0; i < 9; ++i)
+                if (o[i] != 0.0) { acc[9 * a + i] += o[i]; m |= 1u << i; }
+            msk[a] |= (unsigned short)m;
+        }
+        wave_lds_order();
+    }
+    // keep the next phase's arithmetic out of this one (it would only raise register pressure)
+    __builtin_amdgcn_sched_barrier(0);
+}
+
+// Chunked form: one wavefront per chunk of 64 tets, one lane per tet.  For each local block
+// lb the lanes add their block to the chunk's LDS accumulators in the planned rounds (no two
+// lanes of a round share an accumulator), skipping zero entries and recording which entries
+// received a nonzero value (the reference drops zero entries before summing duplicates).
+// Then each accumulator goes to its block, or to its partial when other chunks touch the
+// block too.
+template <typename H>
+__global__ __launch_bounds__(kChunk) void snh_chunk_kernel(int T, const int4* __restrict__ tets, const double* __restrict__ x,
+                                                           const double* __restrict__ dminv, const double* __restrict__ vol,
+                                                           Lame lame, double dt2, const int* __restrict__ slot_g,
+                                                           double* __restrict__ stagG, const int* __restrict__ ch_tet,
+                                                           const unsigned short* __restrict__ ch_code,
+                                                           const unsigned char* __restrict__ ch_rounds,
+                                                           const int* __restrict__ acc_ptr, const int* __restrict__ acc_dst,
+                                                           double* __restrict__ sbv, unsigned short* __restrict__ sbf,
+                                                           double* __restrict__ part, unsigned short* __restrict__ partf) {
+    extern __shared__ double lds_acc[];
+    const int ch = blockIdx.x, lane = threadIdx.x;
+    const int a0 = acc_ptr[ch], na = acc_ptr[ch + 1] - a0;
+    double* acc = lds_acc;
+    unsigned short* msk = reinterpret_cast<unsigned short*>(lds_acc + 9 * (size_t)na);
+    for (int k = lane; k < 9 * na; k += kChunk) acc[k] = 0.0;
+    for (int k = lane; k < na; k += kChunk) msk[k] = 0;
+    const int t = ch_tet[(size_t)ch * kChunk + lane];
+    const bool live = t >= 0;
+    // padding lanes (a scene's last chunk) evaluate the chunk's first tet and add nothing
+    const int tt = live ? t : ch_tet[(size_t)ch * kChunk];
+    // the lane's ten codes and the chunk's ten round counts, loaded before any LDS ordering
+    // point so that their latency overlaps the tet's evaluation
+    unsigned cdp[5], rdp[4];
+#pragma unroll
+    for (int i = 0; i < 5; ++i)
+        cdp[i] = (unsigned)ch_code[((size_t)ch * 10 + 2 * i) * kChunk + lane] |
+                 ((unsigned)ch_code[((size_t)ch * 10 + 2 * i + 1) * kChunk + lane] << 16);
+    const uint4 rd = reinterpret_cast<const uint4*>(ch_rounds)[ch];
+    rdp[0] = rd.x; rdp[1] = rd.y; rdp[2] = rd.z; rdp[3] = rd.w;
+    TetSetup q;
+    tet_setup(q, tt, T, tets, x, dminv, vol, lame, dt2, slot_g, stagG, live);
+    wave_lds_order();
+    chunk_phase<H, 0>(q, cdp, rdp, acc, msk);
+    chunk_phase<H, 1>(q, cdp, rdp, acc, msk);
+    chunk_phase<H, 2>(q, cdp, rdp, acc, msk);
+    chunk_phase<H, 3>(q, cdp, rdp, acc, msk);
+    chunk_phase<H, 4>(q, cdp, rdp, acc, msk);
+    chunk_phase<H, 5>(q, cdp, rdp, acc, msk);
+    chunk_phase<H, 6>(q, cdp, rdp, acc, msk);
+    chunk_phase<H, 7>(q, cdp, rdp, acc, msk);
+    chunk_phase<H, 8>(q, cdp, rdp, acc, msk);
+    chunk_phase<H, 9>(q, cdp, rdp, acc, msk);
+    for (int k = lane; k < na; k += kChunk) {
+        const int d = acc_dst[a0 + k];
+        double* o = d >= 0 ? sbv + (size_t)9 * d : part + (size_t)9 * (-d - 1);
+#pragma unroll
+        for (int i = 0; i < 9; ++i) o[i] = acc[9 * (size_t)k + i];
+        if (d >= 0) sbf[d] = msk[k];
+        else partf[-d - 1] = msk[k];
+    }
+}
+
+// Blocks touched by several chunks (or by none): the sum of their partials in chunk order.
+__global__ __launch_bounds__(256) void chunk_complete_kernel(int n, const int* __restrict__ blk, const int* __restrict__ ptr,
+                                                             const double* __restrict__ part,
+                                                             const unsigned short* __restrict__ partf, double* __restrict__ sbv,
+                                                             unsigned short* __restrict__ sbf) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    double acc[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+    unsigned m = 0;
+    for (int p = ptr[i]; p < ptr[i + 1]; ++p) {
+#pragma unroll
+        for (int k = 0; k < 9; ++k) acc[k] += part[(size_t)9 * p + k];
+        m |= partf[p];
+    }
+    const int b = blk[i];
+#pragma unroll
+    for (int k = 0; k < 9; ++k) sbv[(size_t)9 * b + k] = acc[k];
+    sbf[b] = (unsigned short)m;
 }
 
 // ---------------------------------------------------------------------------------
@@ -295,9 +622,41 @@ __global__ void soft_kernel(int n, const int* __restrict__ idx, const double* __
     sbf[k] |= (1u << 0) | (1u << 4) | (1u << 8);
 }
 
+static void elastic_staged(Context& c, hipStream_t st, double dt2);
+
 void elastic_assemble(Context& c, hipStream_t st) {
     Scene& s = c.sc;
     const double dt2 = s.dt * s.dt;
+    StaticPattern& sp = c.sp;
+    if (sp.chunked) {
+        auto k = c.hessian_fp32 ? snh_chunk_kernel<float> : snh_chunk_kernel<double>;
+        hipLaunchKernelGGL(k, dim3(sp.nchunks), dim3(kChunk), sp.lds_bytes, st, s.T, s.tets.p, s.x.p, s.dminv.p, s.vol.p,
+                           scene_lame(s), dt2, sp.slot_g.p, c.stagG.p, sp.ch_tet.p, sp.ch_code.p, sp.ch_rounds.p,
+                           sp.ch_acc_ptr.p, sp.ch_acc_dst.p, c.sbv.p, c.sbf.p, sp.part.p, sp.partf.p);
+        CIPC_CHECK_LAUNCH();
+        if (sp.ncomplete > 0)
+            hipLaunchKernelGGL(chunk_complete_kernel, dim3(blocks_for(sp.ncomplete, 256)), dim3(256), 0, st, sp.ncomplete,
+                               sp.cp_blk.p, sp.cp_ptr.p, sp.part.p, sp.partf.p, c.sbv.p, c.sbf.p);
+        CIPC_CHECK_LAUNCH();
+        hipLaunchKernelGGL(mass_diag_kernel, dim3(blocks_for(s.V, 256)), dim3(256), 0, st, s.V, sp.diag_block.p, s.mass.p,
+                           s.drag, c.sbv.p, c.sbf.p);
+        CIPC_CHECK_LAUNCH();
+    } else {
+        elastic_staged(c, st, dt2);
+    }
+    hipLaunchKernelGGL(vertex_gather_kernel, dim3(blocks_for(s.V, 256)), dim3(256), 0, st, s.V, s.T, s.x.p, s.xt.p,
+                       s.vprev.p, s.mass.p, s.drag, dt2, c.sp.vptr.p, c.stagG.p, c.grad.p);
+    CIPC_CHECK_LAUNCH();
+    if (c.n_soft > 0) {
+        hipLaunchKernelGGL(soft_kernel, dim3(blocks_for(c.n_soft, 256)), dim3(256), 0, st, c.n_soft, c.soft_idx.p, c.soft_t.p,
+                           c.soft_w, s.x.p, c.sp.diag_block.p, c.grad.p, c.sbv.p, c.sbf.p);
+        CIPC_CHECK_LAUNCH();
+    }
+}
+
+// Staged form: element kernels write every contribution to its slot, the gathers sum them.
+static void elastic_staged(Context& c, hipStream_t st, double dt2) {
+    Scene& s = c.sc;
     if (s.T > 0) {
         if (c.hessian_fp32)
             hipLaunchKernelGGL(snh_element_kernel<float>, dim3(blocks_for(s.T, 256)), dim3(256), 0, st, s.T, s.tets.p, s.x.p,
@@ -319,14 +678,6 @@ void elastic_assemble(Context& c, hipStream_t st) {
         CIPC_CHECK_LAUNCH();
         hipLaunchKernelGGL(mass_diag_kernel, dim3(blocks_for(s.V, 256)), dim3(256), 0, st, s.V, c.sp.diag_block.p,
                            s.mass.p, s.drag, c.sbv.p, c.sbf.p);
-        CIPC_CHECK_LAUNCH();
-    }
-    hipLaunchKernelGGL(vertex_gather_kernel, dim3(blocks_for(s.V, 256)), dim3(256), 0, st, s.V, s.T, s.x.p, s.xt.p,
-                       s.vprev.p, s.mass.p, s.drag, dt2, c.sp.vptr.p, c.stagG.p, c.grad.p);
-    CIPC_CHECK_LAUNCH();
-    if (c.n_soft > 0) {
-        hipLaunchKernelGGL(soft_kernel, dim3(blocks_for(c.n_soft, 256)), dim3(256), 0, st, c.n_soft, c.soft_idx.p, c.soft_t.p,
-                           c.soft_w, s.x.p, c.sp.diag_block.p, c.grad.p, c.sbv.p, c.sbf.p);
         CIPC_CHECK_LAUNCH();
     }
 }

@@ -25,6 +25,8 @@
 //      K_ab = A (B_a.B_b) I3 - a2 hat(F (B_a x B_b)) + 4 a3 (F B_a)(F B_b)^T
 //             + r (cof F B_a)(cof F B_b)^T - sum_{lambda<0} lambda (M B_a)(M B_b)^T
 //    with hat(w) the cross-product matrix of w, and M B_a = sum_k (v_k . B_a) (M v_k).
+//    Every M B_a lies in the frame of U, so the last sum is U P_ab U^T with a 3x3 P_ab that
+//    is bilinear in V^T B_a and V^T B_b (SNH::Sw): no branch per mode.
 #pragma once
 #include "common.hpp"
 
@@ -53,9 +55,12 @@ __device__ inline void svd3(const M33& F, M33& U, double s[3], M33& V) {
         // converged to the rounding level: |ga| <= 4e-16 sqrt(al be), in squares (one sqrt less);
         // a threshold below the FP64 unit roundoff is never met and would run every sweep
         if (ga * ga <= 1.6e-31 * (al * be)) return false;
-        const double zeta = (be - al) / (2.0 * ga);
-        const double t = (zeta >= 0 ? 1.0 : -1.0) / (fabs(zeta) + sqrt(1.0 + zeta * zeta));
-        const double c = 1.0 / sqrt(1.0 + t * t), sn = c * t;
+        // tangent of the smaller rotation angle, tan = 2 ga sgn(d) / (|d| + sqrt(d^2 + 4 ga^2)),
+        // d = be - al (the textbook sgn(z)/(|z| + sqrt(1 + z^2)), z = d / 2ga, without its two
+        // divisions); one square root, one division and one reciprocal square root
+        const double d = be - al, g2 = 2.0 * ga;
+        const double t = (d >= 0 ? g2 : -g2) / (fabs(d) + sqrt(d * d + g2 * g2));
+        const double c = rsqrt(1.0 + t * t), sn = c * t;
         const d3 np = c * bp - sn * bq, nq = sn * bp + c * bq;
         const d3 vp = c * wp - sn * wq, vq = sn * wp + c * wq;
         bp = np; bq = nq; wp = vp; wq = vq;
@@ -105,9 +110,10 @@ __device__ inline void sym3_eig(double S[3][3], double l[3], double W[3][3]) {
             const int p = (k == 2) ? 1 : 0, q = (k == 0) ? 1 : 2;
             const double apq = S[p][q];
             if (apq == 0.0) continue;
-            const double th = (S[q][q] - S[p][p]) / (2.0 * apq);
-            const double t = (th >= 0 ? 1.0 : -1.0) / (fabs(th) + sqrt(th * th + 1.0));
-            const double c = 1.0 / sqrt(t * t + 1.0), s = t * c;
+            // as in svd3: tan = 2 apq sgn(d) / (|d| + sqrt(d^2 + 4 apq^2)), d = S_qq - S_pp
+            const double d = S[q][q] - S[p][p], g2 = 2.0 * apq;
+            const double t = (d >= 0 ? g2 : -g2) / (fabs(d) + sqrt(d * d + g2 * g2));
+            const double c = rsqrt(t * t + 1.0), s = t * c;
 #pragma unroll
             for (int m = 0; m < 3; ++m) {
                 const double a = S[m][p], b = S[m][q];
@@ -140,6 +146,14 @@ struct SNH {
     double W[3][3];       // scaling-mode eigenvectors (columns)
     double lam[9];        // eigenvalues: flip k (0..2), twist k (3..5), scaling m (6..8)
     bool project;         // any eigenvalue < 0
+    // The projection in the frame of U (set when project): with w = max(-lambda, 0) and
+    // beta = V^T B_a, the correction of block (a, b) is U P_ab U^T,
+    //   P_ab[i][j] = Sw[i][j] beta_a[i] beta_b[j]                   (scaling modes)
+    //   + for axis k, (i, j) its other two axes (flip and twist modes of k):
+    //     P[i][i] += pp[k] beta_a[j] beta_b[j],  P[j][j] += pp[k] beta_a[i] beta_b[i],
+    //     P[i][j] += pm[k] beta_a[j] beta_b[i],  P[j][i] += pm[k] beta_a[i] beta_b[j],
+    //   Sw = W diag(w_scaling) W^T, pp = (w_flip + w_twist)/2, pm = (w_flip - w_twist)/2.
+    double Sw[3][3], pp[3], pm[3];
 };
 
 __device__ inline void snh_eval(SNH& e, double u, double r) {
@@ -177,6 +191,19 @@ __device__ inline void snh_eval(SNH& e, double u, double r) {
 #pragma unroll
     for (int m = 0; m < 9; ++m) neg |= e.lam[m] < 0.0;
     e.project = neg;
+    double w[9];
+#pragma unroll
+    for (int m = 0; m < 9; ++m) w[m] = e.lam[m] < 0.0 ? -e.lam[m] : 0.0;
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        e.pp[k] = 0.5 * (w[k] + w[3 + k]);
+        e.pm[k] = 0.5 * (w[k] - w[3 + k]);
+    }
+#pragma unroll
+    for (int i = 0; i < 3; ++i)
+#pragma unroll
+        for (int j = 0; j < 3; ++j)
+            e.Sw[i][j] = w[6] * e.W[i][0] * e.W[j][0] + w[7] * e.W[i][1] * e.W[j][1] + w[8] * e.W[i][2] * e.W[j][2];
 }
 
 // M_m applied to a vector x, given beta_k = v_k . x (mode index m static after unrolling).
