@@ -228,16 +228,26 @@ struct StaticPattern {
     // distinct block it touches; a block touched by one chunk only is written straight into
     // the block values, the others leave one partial per chunk that the completion pass adds
     // in chunk order.
+    // Vertex gradients go the same way: the tets' corner forces into per-chunk vertex
+    // accumulators, a vertex interior to one chunk written with its inertia term by that
+    // chunk, the others completed from their partials.
     bool chunked = false;
-    int chunk = 0, nchunks = 0, lds_bytes = 0, ncomplete = 0;
+    int chunk = 0, nchunks = 0, lds_bytes = 0, ncomplete = 0, ncomplete_v = 0;
     DBuf<int> ch_tet;               // nchunks*chunk: tet of each lane, -1 for padding
-    DBuf<unsigned short> ch_code;   // ((ch*10 + lb)*chunk + lane): accumulator << 6 | round; 0xFFFF: none
-    DBuf<unsigned char> ch_rounds;  // ch*16 + lb: rounds of that phase (16 bytes per chunk)
-    DBuf<int> ch_acc_ptr;           // nchunks+1: accumulators of chunk ch in [ptr[ch], ptr[ch+1])
-    DBuf<int> ch_acc_dst;           // per accumulator: block index (>= 0), or -(partial + 1)
+    DBuf<unsigned short> ch_code;   // ((ch*14 + phase)*chunk + lane): accumulator << 6 | round; 0xFFFF: none
+                                    // (phases 0..9 the local blocks, 10..13 the corners)
+    DBuf<unsigned char> ch_rounds;  // ch*16 + phase: rounds of that phase (16 bytes per chunk)
+    DBuf<int> ch_acc_ptr;           // nchunks+1: block accumulators of chunk ch in [ptr[ch], ptr[ch+1])
+    DBuf<int> ch_acc_dst;           // per block accumulator: block index (>= 0), or -(partial + 1)
+    DBuf<int> ch_vacc_ptr;          // nchunks+1: vertex accumulators of chunk ch
+    DBuf<int> ch_vacc_dst;          // per vertex accumulator: vertex (>= 0), or -(vertex partial + 1)
     DBuf<int> cp_blk, cp_ptr;       // completion: block, and its partials in [cp_ptr[i], cp_ptr[i+1])
-    DBuf<double> part;              // 9 per partial
-    DBuf<unsigned short> partf;     // nonzero masks of the partials
+    DBuf<int> cpv_vtx, cpv_ptr;     // completion: vertex, and its partials
+    DBuf<double> part;              // 9 per block partial
+    DBuf<unsigned short> partf;     // nonzero masks of the block partials
+    DBuf<double> vpart;             // 3 per vertex partial
+    DBuf<int4> ctet;                // chunk order copies of the tets' static data (padding: the
+    DBuf<double> cdm, cvol;         //   chunk's first tet); cdm SoA, entry k of lane p at k*NP + p
 };
 
 }  // namespace cipc

@@ -136,6 +136,7 @@ void build_static_pattern(Context& c) {
 // ---------------------------------------------------------------------------------
 namespace {
 constexpr int kAccBytes = 9 * 8 + 2;   // one accumulator: 9 doubles and a 16-bit mask
+constexpr int kChunkAcc = 216;         // block accumulators per chunk: 216 x 74 B < 16 KiB of LDS
 
 uint32_t morton30(double px, double py, double pz, const double lo[3], double ext) {
     auto q = [&](double v, int k) {
@@ -150,6 +151,19 @@ uint32_t morton30(double px, double py, double pz, const double lo[3], double ex
     return (ex(q(px, 0)) << 2) | (ex(q(py, 1)) << 1) | ex(q(pz, 2));
 }
 }  // namespace
+
+__global__ void chunk_static_gather(int NP, int T, const int* __restrict__ ch_tet, const int4* __restrict__ tets,
+                                    const double* __restrict__ dminv, const double* __restrict__ vol, int4* __restrict__ ctet,
+                                    double* __restrict__ cdm, double* __restrict__ cvol) {
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= NP) return;
+    int t = ch_tet[p];
+    if (t < 0) t = ch_tet[p - p % kChunk];
+    ctet[p] = tets[t];
+    cvol[p] = vol[t];
+#pragma unroll
+    for (int k = 0; k < 9; ++k) cdm[(size_t)k * NP + p] = dminv[(size_t)k * T + t];
+}
 
 void build_tet_chunks(Context& c, const double* hx) {
     Scene& s = c.sc;
@@ -187,92 +201,142 @@ void build_tet_chunks(Context& c, const double* hx) {
         const int j = sp.h_slot_h[10 * (size_t)t + lb];
         return j < 0 ? -1 : sp.h_blk_of_slot[j];
     };
-    // one wavefront per chunk: the rounds need no work-group barrier
-    for (int C : {kChunk}) {
-        std::vector<int> ch_first;   // first position in `order` of every chunk
-        for (size_t k = 0; k + 1 < toff.size(); ++k)
-            for (int p = toff[k]; p < toff[k + 1]; p += C) ch_first.push_back(p);
-        const int nch = (int)ch_first.size();
-        auto chunk_end = [&](int ch) {
-            const int p = ch_first[ch];
-            const int k = (int)(std::upper_bound(toff.begin(), toff.end(), p) - toff.begin()) - 1;
-            return std::min(p + C, toff[k + 1]);
-        };
-        std::vector<int> ch_tet((size_t)nch * C, -1), acc_ptr(nch + 1, 0), acc_blk;
-        std::vector<unsigned short> code((size_t)nch * 10 * C, 0xFFFF);
-        std::vector<unsigned char> rounds((size_t)nch * 16, 0);   // 10 used, padded to 16 bytes
-        bool fits = true;
-        std::vector<int> tg, cnt;
-        for (int ch = 0; ch < nch && fits; ++ch) {
-            const int p0 = ch_first[ch], p1 = chunk_end(ch), n = p1 - p0;
-            tg.clear();
-            for (int l = 0; l < n; ++l) {
-                ch_tet[(size_t)ch * C + l] = order[p0 + l];
-                for (int lb = 0; lb < 10; ++lb) {
-                    const int b = target(order[p0 + l], lb);
-                    if (b >= 0) tg.push_back(b);
+    // one wavefront (up to kChunk tets) per chunk: the rounds need no work-group barrier.  A
+    // chunk also ends before its block accumulators would exceed kChunkAcc, which bounds the
+    // LDS of a work-group (10 work-groups per CU) so that every chunk is resident at once.
+    constexpr int C = kChunk;
+    std::vector<int> ch_first, ch_end;   // positions in `order` of every chunk
+    {
+        std::vector<int> mark(sp.nblocks, -1);
+        for (size_t k = 0; k + 1 < toff.size(); ++k) {
+            int p = toff[k];
+            while (p < toff[k + 1]) {
+                const int ch = (int)ch_first.size();
+                ch_first.push_back(p);
+                int na = 0;
+                while (p < toff[k + 1] && p - ch_first.back() < C) {
+                    int fresh = 0;
+                    for (int lb = 0; lb < 10; ++lb) {
+                        const int b = target(order[p], lb);
+                        if (b >= 0 && mark[b] != ch) {
+                            bool dup = false;   // a block twice in one tet counts once
+                            for (int l2 = 0; l2 < lb; ++l2) dup |= target(order[p], l2) == b;
+                            fresh += dup ? 0 : 1;
+                        }
+                    }
+                    if (na + fresh > kChunkAcc) break;
+                    for (int lb = 0; lb < 10; ++lb) {
+                        const int b = target(order[p], lb);
+                        if (b >= 0) mark[b] = ch;
+                    }
+                    na += fresh;
+                    ++p;
                 }
-            }
-            std::sort(tg.begin(), tg.end());
-            tg.erase(std::unique(tg.begin(), tg.end()), tg.end());
-            const int na = (int)tg.size();
-            if (na > 1023) { fits = false; break; }   // the 10-bit accumulator field (64 x 10 = 640 at most)
-            acc_ptr[ch + 1] = acc_ptr[ch] + na;
-            acc_blk.insert(acc_blk.end(), tg.begin(), tg.end());
-            cnt.assign(na, 0);
-            for (int lb = 0; lb < 10; ++lb) {
-                std::fill(cnt.begin(), cnt.end(), 0);
-                int rmax = 0;
-                for (int l = 0; l < n; ++l) {
-                    const int b = target(order[p0 + l], lb);
-                    if (b < 0) continue;
-                    const int a = (int)(std::lower_bound(tg.begin(), tg.end(), b) - tg.begin());
-                    const int r = cnt[a]++;
-                    if (r > 63) { fits = false; break; }
-                    code[((size_t)ch * 10 + lb) * C + l] = (unsigned short)((a << 6) | r);
-                    rmax = std::max(rmax, r + 1);
-                }
-                rounds[(size_t)ch * 16 + lb] = (unsigned char)rmax;
+                ch_end.push_back(p);
             }
         }
-        if (!fits) continue;
-        // blocks touched by one chunk are written by it; the others (and blocks no tet
-        // touches) are completed from their partials, in chunk order
-        const int nb = sp.nblocks;
-        std::vector<int> nch_of(nb, 0);
-        for (int b : acc_blk) nch_of[b]++;
-        std::vector<int> cp_blk, cp_ptr(1, 0), cursor(nb, -1);
-        for (int b = 0; b < nb; ++b)
-            if (nch_of[b] != 1) {
-                cursor[b] = cp_ptr.back();
-                cp_blk.push_back(b);
-                cp_ptr.push_back(cp_ptr.back() + nch_of[b]);
-            }
-        std::vector<int> acc_dst(acc_blk.size());
-        for (size_t i = 0; i < acc_blk.size(); ++i) {   // chunks in order: partial runs in chunk order
-            const int b = acc_blk[i];
-            acc_dst[i] = nch_of[b] == 1 ? b : -(cursor[b]++ + 1);
-        }
-        int maxacc = 0;
-        for (int ch = 0; ch < nch; ++ch) maxacc = std::max(maxacc, acc_ptr[ch + 1] - acc_ptr[ch]);
-        auto up = [&](auto& d, const auto& h) {
-            d.resize(std::max<size_t>(h.size(), 1));
-            if (!h.empty()) CIPC_HIP(hipMemcpy(d.p, h.data(), h.size() * sizeof(h[0]), hipMemcpyHostToDevice));
-        };
-        up(sp.ch_tet, ch_tet); up(sp.ch_code, code); up(sp.ch_rounds, rounds);
-        up(sp.ch_acc_ptr, acc_ptr); up(sp.ch_acc_dst, acc_dst);
-        up(sp.cp_blk, cp_blk); up(sp.cp_ptr, cp_ptr);
-        sp.part.resize((size_t)9 * std::max(cp_ptr.back(), 1));
-        sp.partf.resize((size_t)std::max(cp_ptr.back(), 1));
-        sp.chunk = C;
-        sp.nchunks = nch;
-        sp.ncomplete = (int)cp_blk.size();
-        sp.lds_bytes = ((maxacc * 9 * 8 + maxacc * 2 + 15) / 16) * 16;
-        sp.chunked = true;
-        c.stagH.release();
-        return;
     }
-    staged();
+    const int nch = (int)ch_first.size();
+    auto chunk_end = [&](int ch) { return ch_end[ch]; };
+    // per chunk: block accumulators (distinct blocks of its tets) and vertex accumulators
+    // (distinct vertices); per lane 10 block codes then 4 corner codes, accumulator << 6 |
+    // round; per chunk 10 + 4 round counts
+    std::vector<int> ch_tet((size_t)nch * C, -1), acc_ptr(nch + 1, 0), acc_blk, vacc_ptr(nch + 1, 0), vacc_vtx;
+    std::vector<unsigned short> code((size_t)nch * 14 * C, 0xFFFF);
+    std::vector<unsigned char> rounds((size_t)nch * 16, 0);   // 14 used, padded to 16 bytes
+    std::vector<int> tg, vg, cnt;
+    int maxlds = 0;
+    for (int ch = 0; ch < nch; ++ch) {
+        const int p0 = ch_first[ch], p1 = chunk_end(ch), n = p1 - p0;
+        tg.clear();
+        vg.clear();
+        for (int l = 0; l < n; ++l) {
+            const int t = order[p0 + l];
+            ch_tet[(size_t)ch * C + l] = t;
+            for (int lb = 0; lb < 10; ++lb) {
+                const int b = target(t, lb);
+                if (b >= 0) tg.push_back(b);
+            }
+            for (int a = 0; a < 4; ++a) vg.push_back(s.h_tets[4 * (size_t)t + a]);
+        }
+        std::sort(tg.begin(), tg.end());
+        tg.erase(std::unique(tg.begin(), tg.end()), tg.end());
+        std::sort(vg.begin(), vg.end());
+        vg.erase(std::unique(vg.begin(), vg.end()), vg.end());
+        const int na = (int)tg.size(), nv = (int)vg.size();   // na <= 640, nv <= 256: 10-bit fields
+        acc_ptr[ch + 1] = acc_ptr[ch] + na;
+        acc_blk.insert(acc_blk.end(), tg.begin(), tg.end());
+        vacc_ptr[ch + 1] = vacc_ptr[ch] + nv;
+        vacc_vtx.insert(vacc_vtx.end(), vg.begin(), vg.end());
+        maxlds = std::max(maxlds, std::max(na * kAccBytes, nv * 24));   // vertex accumulators alias the block ones
+        for (int ph = 0; ph < 14; ++ph) {
+            const bool blk = ph < 10;
+            const std::vector<int>& keys = blk ? tg : vg;
+            cnt.assign(keys.size(), 0);
+            int rmax = 0;
+            for (int l = 0; l < n; ++l) {
+                const int t = order[p0 + l];
+                const int key = blk ? target(t, ph) : s.h_tets[4 * (size_t)t + (ph - 10)];
+                if (key < 0) continue;
+                const int a = (int)(std::lower_bound(keys.begin(), keys.end(), key) - keys.begin());
+                const int r = cnt[a]++;
+                if (r > 63) { staged(); return; }   // the 6-bit round field
+                code[((size_t)ch * 14 + ph) * C + l] = (unsigned short)((a << 6) | r);
+                rmax = std::max(rmax, r + 1);
+            }
+            rounds[(size_t)ch * 16 + ph] = (unsigned char)rmax;
+        }
+    }
+    // blocks and vertices touched by one chunk are written by it; the others (and those no
+    // tet touches) are completed from their partials, in chunk order
+    auto runs = [](const std::vector<int>& keys, int n, std::vector<int>& list, std::vector<int>& ptr, std::vector<int>& dst) {
+        std::vector<int> nof(n, 0), cursor(n, -1);
+        for (int k : keys) nof[k]++;
+        ptr.assign(1, 0);
+        list.clear();
+        auto completed = [&](int k) { return nof[k] != 1; };
+        for (int k = 0; k < n; ++k)
+            if (completed(k)) {
+                cursor[k] = ptr.back();
+                list.push_back(k);
+                ptr.push_back(ptr.back() + nof[k]);
+            }
+        dst.resize(keys.size());
+        for (size_t i = 0; i < keys.size(); ++i)   // chunks in order: partial runs in chunk order
+            dst[i] = completed(keys[i]) ? -(cursor[keys[i]]++ + 1) : keys[i];
+    };
+    std::vector<int> cp_blk, cp_ptr, acc_dst, cpv_vtx, cpv_ptr, vacc_dst;
+    runs(acc_blk, sp.nblocks, cp_blk, cp_ptr, acc_dst);
+    runs(vacc_vtx, s.V, cpv_vtx, cpv_ptr, vacc_dst);
+    auto up = [&](auto& d, const auto& h) {
+        d.resize(std::max<size_t>(h.size(), 1));
+        if (!h.empty()) CIPC_HIP(hipMemcpy(d.p, h.data(), h.size() * sizeof(h[0]), hipMemcpyHostToDevice));
+    };
+    up(sp.ch_tet, ch_tet); up(sp.ch_code, code); up(sp.ch_rounds, rounds);
+    up(sp.ch_acc_ptr, acc_ptr); up(sp.ch_acc_dst, acc_dst);
+    up(sp.ch_vacc_ptr, vacc_ptr); up(sp.ch_vacc_dst, vacc_dst);
+    up(sp.cp_blk, cp_blk); up(sp.cp_ptr, cp_ptr);
+    up(sp.cpv_vtx, cpv_vtx); up(sp.cpv_ptr, cpv_ptr);
+    sp.part.resize((size_t)9 * std::max(cp_ptr.back(), 1));
+    sp.partf.resize((size_t)std::max(cp_ptr.back(), 1));
+    sp.vpart.resize((size_t)3 * std::max(cpv_ptr.back(), 1));
+    sp.chunk = C;
+    sp.nchunks = nch;
+    sp.ncomplete = (int)cp_blk.size();
+    sp.ncomplete_v = (int)cpv_vtx.size();
+    sp.lds_bytes = ((maxlds + 15) / 16) * 16;
+    // the tets' static data in chunk order (coalesced loads in the chunk kernel); a padding
+    // lane holds its chunk's first tet
+    const size_t NP = (size_t)nch * C;
+    sp.ctet.resize(NP);
+    sp.cdm.resize(9 * NP);
+    sp.cvol.resize(NP);
+    hipLaunchKernelGGL(chunk_static_gather, dim3(blocks_for(NP, 256)), dim3(256), 0, c.stream, (int)NP, T, sp.ch_tet.p,
+                       s.tets.p, s.dminv.p, s.vol.p, sp.ctet.p, sp.cdm.p, sp.cvol.p);
+    CIPC_CHECK_LAUNCH();
+    CIPC_HIP(hipStreamSynchronize(c.stream));
+    sp.chunked = true;
+    c.stagH.release();
 }
 
 // ---------------------------------------------------------------------------------
@@ -318,28 +382,28 @@ struct TetSetup {
     double sc;             // vol dt^2
 };
 
-// Per-tet set-up shared by the staged and the chunked kernels: SNH evaluation and the corner
-// forces f_a = vol P B_a written to their gradient slots (dt^2 is applied by the vertex
-// gather, as IPCSolver.cpp:149-151).
-__device__ inline void tet_setup(TetSetup& q, int t, int T, const int4* __restrict__ tets, const double* __restrict__ x,
-                                 const double* __restrict__ dminv, const double* __restrict__ vol, const Lame& lame, double dt2,
-                                 const int* __restrict__ slot_g, double* __restrict__ stagG, bool write_forces) {
-    SNH e;
-    d3 B[4];
-    load_tet(t, T, tets, x, dminv, e.F, B);
-    const int4 tv = tets[t];
+// Per-tet set-up shared by the staged and the chunked kernels, in two halves so that the
+// corner forces can be used (and dropped) before the SVD.  First half: F, the invariants and
+// the corner forces f_a = vol P B_a (dt^2 is applied where they are summed, as
+// IPCSolver.cpp:149-151).  The tet's static data (indices, Dm^-1 with stride NP, volume) is
+// at position p; its material (a batch's per-tet Lame parameters) at tet index t.
+__device__ inline void tet_setup_forces(TetSetup& q, SNH& e, d3 B[4], d3 f[4], double& u, double& r, int t, int p, int NP,
+                                        const int4* __restrict__ tets, const double* __restrict__ x,
+                                        const double* __restrict__ dminv, const double* __restrict__ vol, const Lame& lame,
+                                        double dt2) {
+    load_tet(p, NP, tets, x, dminv, e.F, B);
+    const int4 tv = tets[p];
     q.vid[0] = tv.x; q.vid[1] = tv.y; q.vid[2] = tv.z; q.vid[3] = tv.w;
-    double u, r;
     lame.at(t, u, r);
-    snh_eval(e, u, r);
-    const double vt = vol[t];
-    if (write_forces) {
+    snh_invariants(e, u, r);
+    const double vt = vol[p];
 #pragma unroll
-        for (int a = 0; a < 4; ++a) {
-            const d3 f = vt * (e.pa * mvec(e.F, B[a]) + e.pb * mvec(e.C, B[a]));
-            st3(stagG, slot_g[4 * t + a], f);
-        }
-    }
+    for (int a = 0; a < 4; ++a) f[a] = vt * (e.pa * mvec(e.F, B[a]) + e.pb * mvec(e.C, B[a]));
+    q.sc = vt * dt2;
+}
+// Second half: SVD frames, eigenvalues and the block formula's state.
+__device__ inline void tet_setup_modes(TetSetup& q, SNH& e, const d3 B[4], double u, double r) {
+    snh_modes(e, u, r);
 #pragma unroll
     for (int a = 0; a < 4; ++a)
 #pragma unroll
@@ -356,7 +420,6 @@ __device__ inline void tet_setup(TetSetup& q, int t, int T, const int4* __restri
 #pragma unroll
         for (int j = 0; j < 3; ++j) q.Sw[i][j] = e.Sw[i][j];
     }
-    q.sc = vt * dt2;
 }
 
 // Block K_ab (3x3) of the projected element stiffness before the vol dt^2 factor.
@@ -426,7 +489,13 @@ __global__ __launch_bounds__(256) void snh_element_kernel(int T, const int4* __r
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= T) return;
     TetSetup q;
-    tet_setup(q, t, T, tets, x, dminv, vol, lame, dt2, slot_g, stagG, true);
+    SNH e;
+    d3 B[4], f[4];
+    double u, r;
+    tet_setup_forces(q, e, B, f, u, r, t, t, T, tets, x, dminv, vol, lame, dt2);
+#pragma unroll
+    for (int a = 0; a < 4; ++a) st3(stagG, slot_g[4 * t + a], f[a]);
+    tet_setup_modes(q, e, B, u, r);
 #pragma unroll
     for (int lb = 0; lb < 10; ++lb) {
         const int j = slot_h[10 * t + lb];
@@ -448,14 +517,36 @@ __device__ inline void wave_lds_order() {
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
-// One phase of the chunked form: local block LB of every lane's tet into the chunk's
+// g = m (x - x~) + drag m (x - V_prev) (IPCSolver.cpp:118-128), to which the elastic forces
+// are added.
+__device__ inline d3 inertia_gradient(int v, const double* __restrict__ x, const double* __restrict__ xt,
+                                      const double* __restrict__ vprev, const double* __restrict__ mass, double drag) {
+    const d3 xv = ld3(x, v);
+    const double m = mass[v];
+    return m * (xv - ld3(xt, v)) + (drag * m) * (xv - ld3(vprev, v));
+}
+
+// Code and round count of phase PH (0..9 local blocks, 10..13 corners) from the packed words.
+template <int PH>
+__device__ __forceinline__ unsigned phase_code(const unsigned cdp[7]) { return (cdp[PH / 2] >> (16 * (PH % 2))) & 0xFFFFu; }
+template <int PH>
+__device__ __forceinline__ int phase_rounds(const unsigned rdp[4]) { return (int)((rdp[PH / 4] >> (8 * (PH % 4))) & 0xFFu); }
+
+// One block phase of the chunked form: local block LB of every lane's tet into the chunk's
 // accumulators, round by round (LB is a template argument so that every index into the
 // tet's state is static).
 template <typename H, int LB>
-__device__ __forceinline__ void chunk_phase(const TetSetup& q, const unsigned cdp[5], const unsigned rdp[4], double* acc,
+__device__ __forceinline__ void chunk_phase(TetSetup& q, const unsigned cdp[7], const unsigned rdp[4], double* acc,
                                             unsigned short* msk) {
-    const unsigned cd = (cdp[LB / 2] >> (16 * (LB % 2))) & 0xFFFFu;
-    const int R = (int)((rdp[LB / 4] >> (8 * (LB % 4))) & 0xFFu);
+    // the frame coordinates pass through an empty asm statement: products that several blocks
+    // share are then recomputed in each phase rather than kept live across all ten (which
+    // would double the register footprint and halve the occupancy)
+#pragma unroll
+    for (int a = 0; a < 4; ++a)
+#pragma unroll
+        for (int k = 0; k < 3; ++k) __asm__ volatile("" : "+v"(q.beta[a][k]));
+    const unsigned cd = phase_code<LB>(cdp);
+    const int R = phase_rounds<LB>(rdp);
     double o[9];
     // (a block outside the pattern, or a padding lane, computes nothing; the branch also
     // keeps the phases' arithmetic apart, which bounds the register pressure)
@@ -476,45 +567,89 @@ __device__ __forceinline__ void chunk_phase(const TetSetup& q, const unsigned cd
     __builtin_amdgcn_sched_barrier(0);
 }
 
-// Chunked form: one wavefront per chunk of 64 tets, one lane per tet.  For each local block
-// lb the lanes add their block to the chunk's LDS accumulators in the planned rounds (no two
-// lanes of a round share an accumulator), skipping zero entries and recording which entries
-// received a nonzero value (the reference drops zero entries before summing duplicates).
-// Then each accumulator goes to its block, or to its partial when other chunks touch the
-// block too.
+// One corner phase: dt^2 f_A of every lane's tet into the chunk's vertex accumulators.
+template <int A>
+__device__ __forceinline__ void corner_phase(const d3 f[4], double dt2, const unsigned cdp[7], const unsigned rdp[4],
+                                             double* vacc) {
+    const unsigned cd = phase_code<10 + A>(cdp);
+    const int R = phase_rounds<10 + A>(rdp);
+    const int mine = cd == 0xFFFFu ? -1 : (int)(cd & 63u);
+    const unsigned a = cd >> 6;
+    const d3 g = dt2 * f[A];
+    for (int r = 0; r < R; ++r) {
+        if (mine == r) {
+            vacc[3 * a] += g.x;
+            vacc[3 * a + 1] += g.y;
+            vacc[3 * a + 2] += g.z;
+        }
+        wave_lds_order();
+    }
+}
+
+// Chunked form: one wavefront per chunk of 64 tets, one lane per tet.  The corner forces go
+// to the chunk's vertex accumulators and then, for each local block lb, the lanes add their
+// block to the chunk's block accumulators, in the planned rounds (no two lanes of a round
+// share an accumulator).  Block entries: zero entries are skipped and the entries that
+// received a nonzero value recorded (the reference drops zero entries before summing
+// duplicates).  Then each accumulator goes to its block or vertex (a vertex with its inertia
+// term), or to its partial when other chunks touch the block or vertex too.
 template <typename H>
-__global__ __launch_bounds__(kChunk) void snh_chunk_kernel(int T, const int4* __restrict__ tets, const double* __restrict__ x,
-                                                           const double* __restrict__ dminv, const double* __restrict__ vol,
-                                                           Lame lame, double dt2, const int* __restrict__ slot_g,
-                                                           double* __restrict__ stagG, const int* __restrict__ ch_tet,
+__global__ __launch_bounds__(kChunk) void snh_chunk_kernel(int NP, const int4* __restrict__ ctet, const double* __restrict__ x,
+                                                           const double* __restrict__ cdm, const double* __restrict__ cvol,
+                                                           Lame lame, double dt2, const int* __restrict__ ch_tet,
                                                            const unsigned short* __restrict__ ch_code,
                                                            const unsigned char* __restrict__ ch_rounds,
                                                            const int* __restrict__ acc_ptr, const int* __restrict__ acc_dst,
-                                                           double* __restrict__ sbv, unsigned short* __restrict__ sbf,
-                                                           double* __restrict__ part, unsigned short* __restrict__ partf) {
+                                                           const int* __restrict__ vacc_ptr, const int* __restrict__ vacc_dst,
+                                                           const double* __restrict__ xt, const double* __restrict__ vprev,
+                                                           const double* __restrict__ mass, double drag, double* __restrict__ sbv,
+                                                           unsigned short* __restrict__ sbf, double* __restrict__ part,
+                                                           unsigned short* __restrict__ partf, double* __restrict__ grad,
+                                                           double* __restrict__ vpart) {
     extern __shared__ double lds_acc[];
     const int ch = blockIdx.x, lane = threadIdx.x;
     const int a0 = acc_ptr[ch], na = acc_ptr[ch + 1] - a0;
+    const int v0 = vacc_ptr[ch], nv = vacc_ptr[ch + 1] - v0;
     double* acc = lds_acc;
-    unsigned short* msk = reinterpret_cast<unsigned short*>(lds_acc + 9 * (size_t)na);
-    for (int k = lane; k < 9 * na; k += kChunk) acc[k] = 0.0;
-    for (int k = lane; k < na; k += kChunk) msk[k] = 0;
-    const int t = ch_tet[(size_t)ch * kChunk + lane];
-    const bool live = t >= 0;
-    // padding lanes (a scene's last chunk) evaluate the chunk's first tet and add nothing
-    const int tt = live ? t : ch_tet[(size_t)ch * kChunk];
-    // the lane's ten codes and the chunk's ten round counts, loaded before any LDS ordering
-    // point so that their latency overlaps the tet's evaluation
-    unsigned cdp[5], rdp[4];
+    double* vacc = lds_acc;   // the corner phases finish before the block phases start
+    unsigned short* msk = reinterpret_cast<unsigned short*>(acc + 9 * (size_t)na);
+    const int p = ch * kChunk + lane;
+    // the lane's 14 codes and the chunk's 14 round counts, loaded before any LDS ordering point
+    // so that their latency overlaps the tet's evaluation
+    unsigned cdp[7], rdp[4];
 #pragma unroll
-    for (int i = 0; i < 5; ++i)
-        cdp[i] = (unsigned)ch_code[((size_t)ch * 10 + 2 * i) * kChunk + lane] |
-                 ((unsigned)ch_code[((size_t)ch * 10 + 2 * i + 1) * kChunk + lane] << 16);
+    for (int i = 0; i < 7; ++i)
+        cdp[i] = (unsigned)ch_code[((size_t)ch * 14 + 2 * i) * kChunk + lane] |
+                 ((unsigned)ch_code[((size_t)ch * 14 + 2 * i + 1) * kChunk + lane] << 16);
     const uint4 rd = reinterpret_cast<const uint4*>(ch_rounds)[ch];
     rdp[0] = rd.x; rdp[1] = rd.y; rdp[2] = rd.z; rdp[3] = rd.w;
+    for (int k = lane; k < 3 * nv; k += kChunk) vacc[k] = 0.0;
+    // padding lanes (a scene's last chunk) evaluate the chunk's first tet (their static data
+    // is that tet's) and add nothing: their codes are all 0xFFFF
+    const int t = ch_tet[p];
+    const int tt = t >= 0 ? t : ch_tet[ch * kChunk];
     TetSetup q;
-    tet_setup(q, tt, T, tets, x, dminv, vol, lame, dt2, slot_g, stagG, live);
+    SNH e;
+    d3 B[4], f[4];
+    double u, r;
+    tet_setup_forces(q, e, B, f, u, r, tt, p, NP, ctet, x, cdm, cvol, lame, dt2);
     wave_lds_order();
+    corner_phase<0>(f, dt2, cdp, rdp, vacc);
+    corner_phase<1>(f, dt2, cdp, rdp, vacc);
+    corner_phase<2>(f, dt2, cdp, rdp, vacc);
+    corner_phase<3>(f, dt2, cdp, rdp, vacc);
+    for (int k = lane; k < nv; k += kChunk) {
+        const int d = vacc_dst[v0 + k];
+        const d3 g = {vacc[3 * k], vacc[3 * k + 1], vacc[3 * k + 2]};
+        if (d >= 0) st3(grad, d, inertia_gradient(d, x, xt, vprev, mass, drag) + g);
+        else st3(vpart, -d - 1, g);
+    }
+    wave_lds_order();
+    for (int k = lane; k < 9 * na; k += kChunk) acc[k] = 0.0;
+    for (int k = lane; k < na; k += kChunk) msk[k] = 0;
+    wave_lds_order();
+    __builtin_amdgcn_sched_barrier(0);
+    tet_setup_modes(q, e, B, u, r);
     chunk_phase<H, 0>(q, cdp, rdp, acc, msk);
     chunk_phase<H, 1>(q, cdp, rdp, acc, msk);
     chunk_phase<H, 2>(q, cdp, rdp, acc, msk);
@@ -535,24 +670,36 @@ __global__ __launch_bounds__(kChunk) void snh_chunk_kernel(int T, const int4* __
     }
 }
 
-// Blocks touched by several chunks (or by none): the sum of their partials in chunk order.
-__global__ __launch_bounds__(256) void chunk_complete_kernel(int n, const int* __restrict__ blk, const int* __restrict__ ptr,
+// Blocks (i < nb) and vertices (i >= nb) touched by several chunks or by none: the sum of
+// their partials in chunk order; a vertex with its inertia term.
+__global__ __launch_bounds__(256) void chunk_complete_kernel(int nb, int nv, const int* __restrict__ blk, const int* __restrict__ ptr,
                                                              const double* __restrict__ part,
                                                              const unsigned short* __restrict__ partf, double* __restrict__ sbv,
-                                                             unsigned short* __restrict__ sbf) {
+                                                             unsigned short* __restrict__ sbf, const int* __restrict__ vtx,
+                                                             const int* __restrict__ vptr, const double* __restrict__ vpart,
+                                                             const double* __restrict__ x, const double* __restrict__ xt,
+                                                             const double* __restrict__ vprev, const double* __restrict__ mass,
+                                                             double drag, double* __restrict__ grad) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    double acc[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
-    unsigned m = 0;
-    for (int p = ptr[i]; p < ptr[i + 1]; ++p) {
+    if (i < nb) {
+        double acc[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+        unsigned m = 0;
+        for (int q = ptr[i]; q < ptr[i + 1]; ++q) {
 #pragma unroll
-        for (int k = 0; k < 9; ++k) acc[k] += part[(size_t)9 * p + k];
-        m |= partf[p];
+            for (int k = 0; k < 9; ++k) acc[k] += part[(size_t)9 * q + k];
+            m |= partf[q];
+        }
+        const int b = blk[i];
+#pragma unroll
+        for (int k = 0; k < 9; ++k) sbv[(size_t)9 * b + k] = acc[k];
+        sbf[b] = (unsigned short)m;
+    } else if (i < nb + nv) {
+        const int j = i - nb;
+        d3 g = {0, 0, 0};
+        for (int q = vptr[j]; q < vptr[j + 1]; ++q) g = g + ld3(vpart, q);
+        const int v = vtx[j];
+        st3(grad, v, inertia_gradient(v, x, xt, vprev, mass, drag) + g);
     }
-    const int b = blk[i];
-#pragma unroll
-    for (int k = 0; k < 9; ++k) sbv[(size_t)9 * b + k] = acc[k];
-    sbf[b] = (unsigned short)m;
 }
 
 // ---------------------------------------------------------------------------------
@@ -630,23 +777,26 @@ void elastic_assemble(Context& c, hipStream_t st) {
     StaticPattern& sp = c.sp;
     if (sp.chunked) {
         auto k = c.hessian_fp32 ? snh_chunk_kernel<float> : snh_chunk_kernel<double>;
-        hipLaunchKernelGGL(k, dim3(sp.nchunks), dim3(kChunk), sp.lds_bytes, st, s.T, s.tets.p, s.x.p, s.dminv.p, s.vol.p,
-                           scene_lame(s), dt2, sp.slot_g.p, c.stagG.p, sp.ch_tet.p, sp.ch_code.p, sp.ch_rounds.p,
-                           sp.ch_acc_ptr.p, sp.ch_acc_dst.p, c.sbv.p, c.sbf.p, sp.part.p, sp.partf.p);
+        hipLaunchKernelGGL(k, dim3(sp.nchunks), dim3(kChunk), sp.lds_bytes, st, sp.nchunks * kChunk, sp.ctet.p, s.x.p, sp.cdm.p,
+                           sp.cvol.p, scene_lame(s), dt2, sp.ch_tet.p, sp.ch_code.p, sp.ch_rounds.p, sp.ch_acc_ptr.p,
+                           sp.ch_acc_dst.p, sp.ch_vacc_ptr.p, sp.ch_vacc_dst.p, s.xt.p, s.vprev.p, s.mass.p, s.drag, c.sbv.p,
+                           c.sbf.p, sp.part.p, sp.partf.p, c.grad.p, sp.vpart.p);
         CIPC_CHECK_LAUNCH();
-        if (sp.ncomplete > 0)
-            hipLaunchKernelGGL(chunk_complete_kernel, dim3(blocks_for(sp.ncomplete, 256)), dim3(256), 0, st, sp.ncomplete,
-                               sp.cp_blk.p, sp.cp_ptr.p, sp.part.p, sp.partf.p, c.sbv.p, c.sbf.p);
+        const int nc = sp.ncomplete + sp.ncomplete_v;
+        if (nc > 0)
+            hipLaunchKernelGGL(chunk_complete_kernel, dim3(blocks_for(nc, 256)), dim3(256), 0, st, sp.ncomplete, sp.ncomplete_v,
+                               sp.cp_blk.p, sp.cp_ptr.p, sp.part.p, sp.partf.p, c.sbv.p, c.sbf.p, sp.cpv_vtx.p, sp.cpv_ptr.p,
+                               sp.vpart.p, s.x.p, s.xt.p, s.vprev.p, s.mass.p, s.drag, c.grad.p);
         CIPC_CHECK_LAUNCH();
         hipLaunchKernelGGL(mass_diag_kernel, dim3(blocks_for(s.V, 256)), dim3(256), 0, st, s.V, sp.diag_block.p, s.mass.p,
                            s.drag, c.sbv.p, c.sbf.p);
         CIPC_CHECK_LAUNCH();
     } else {
         elastic_staged(c, st, dt2);
+        hipLaunchKernelGGL(vertex_gather_kernel, dim3(blocks_for(s.V, 256)), dim3(256), 0, st, s.V, s.T, s.x.p, s.xt.p,
+                           s.vprev.p, s.mass.p, s.drag, dt2, c.sp.vptr.p, c.stagG.p, c.grad.p);
+        CIPC_CHECK_LAUNCH();
     }
-    hipLaunchKernelGGL(vertex_gather_kernel, dim3(blocks_for(s.V, 256)), dim3(256), 0, st, s.V, s.T, s.x.p, s.xt.p,
-                       s.vprev.p, s.mass.p, s.drag, dt2, c.sp.vptr.p, c.stagG.p, c.grad.p);
-    CIPC_CHECK_LAUNCH();
     if (c.n_soft > 0) {
         hipLaunchKernelGGL(soft_kernel, dim3(blocks_for(c.n_soft, 256)), dim3(256), 0, st, c.n_soft, c.soft_idx.p, c.soft_t.p,
                            c.soft_w, s.x.p, c.sp.diag_block.p, c.grad.p, c.sbv.p, c.sbf.p);

@@ -156,7 +156,16 @@ struct SNH {
     double Sw[3][3], pp[3], pm[3];
 };
 
+// snh_eval in two halves: the invariants, Hessian coefficients and PK1 coefficients (all a
+// gradient needs), then the SVD frames, eigenvalues and projection data.
+__device__ inline void snh_invariants(SNH& e, double u, double r);
+__device__ inline void snh_modes(SNH& e, double u, double r);
 __device__ inline void snh_eval(SNH& e, double u, double r) {
+    snh_invariants(e, u, r);
+    snh_modes(e, u, r);
+}
+
+__device__ inline void snh_invariants(SNH& e, double u, double r) {
     const d3 f0 = mcol(e.F, 0), f1 = mcol(e.F, 1), f2 = mcol(e.F, 2);
     setcol(e.C, 0, cross(f1, f2)); setcol(e.C, 1, cross(f2, f0)); setcol(e.C, 2, cross(f0, f1));
     e.I2 = sqn(f0) + sqn(f1) + sqn(f2);
@@ -168,6 +177,10 @@ __device__ inline void snh_eval(SNH& e, double u, double r) {
     e.r = r;
     e.pa = u * (1 - 1 / (e.I2 + 1));
     e.pb = r * (e.I3 - 1 - u * 3 / (r * 4));
+}
+
+__device__ inline void snh_modes(SNH& e, double u, double r) {
+    const double Ic = e.I2, mu = u;
     svd3(e.F, e.U, e.s, e.V);
 #pragma unroll
     for (int k = 0; k < 3; ++k) {
