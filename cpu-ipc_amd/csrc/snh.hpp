@@ -144,7 +144,8 @@ struct SNH {
     M33 U, V;             // SVD frames
     double s[3];
     double W[3][3];       // scaling-mode eigenvectors (columns)
-    double lam[9];        // eigenvalues: flip k (0..2), twist k (3..5), scaling m (6..8)
+    double lam[9];        // eigenvalues: flip k (0..2), twist k (3..5), scaling m (6..8); the
+                          // scaling ones hold S's diagonal when S is positive definite
     bool project;         // any eigenvalue < 0
     // The projection in the frame of U (set when project): with w = max(-lambda, 0) and
     // beta = V^T B_a, the correction of block (a, b) is U P_ab U^T,
@@ -198,7 +199,23 @@ __device__ inline void snh_modes(SNH& e, double u, double r) {
             S[a][b] = (a == b ? e.A : 0.0) + e.a2 * off + cc * e.s[a] * e.s[b] + r * cv[a] * cv[b];
         }
     double l[3];
-    sym3_eig(S, l, e.W);
+    // The scaling modes matter only through their negative eigenvalues.  When S is positive
+    // definite (Sylvester's criterion: positive leading minors) there are none and their
+    // eigenvectors are not needed: l takes S's (positive) diagonal and W the identity.  Near
+    // rest and under moderate strain S is positive definite, which skips the Jacobi sweeps.
+    const double m2 = S[0][0] * S[1][1] - S[0][1] * S[0][1];
+    const double m3 = S[0][0] * (S[1][1] * S[2][2] - S[1][2] * S[1][2]) - S[0][1] * (S[0][1] * S[2][2] - S[1][2] * S[0][2]) +
+                      S[0][2] * (S[0][1] * S[1][2] - S[1][1] * S[0][2]);
+    if (S[0][0] > 0.0 && m2 > 0.0 && m3 > 0.0) {
+#pragma unroll
+        for (int i = 0; i < 3; ++i) {
+            l[i] = S[i][i];
+#pragma unroll
+            for (int j = 0; j < 3; ++j) e.W[i][j] = (i == j) ? 1.0 : 0.0;
+        }
+    } else {
+        sym3_eig(S, l, e.W);
+    }
     e.lam[6] = l[0]; e.lam[7] = l[1]; e.lam[8] = l[2];
     bool neg = false;
 #pragma unroll
