@@ -553,13 +553,26 @@ __device__ __forceinline__ void chunk_phase(TetSetup& q, const unsigned cdp[7], 
     if (cd != 0xFFFFu) tet_block<H>(q, LB, o);
     const int mine = cd == 0xFFFFu ? -1 : (int)(cd & 63u);
     const unsigned a = cd >> 6;
+    // zero entries add nothing (an accumulator starts at +0 and never holds -0, so adding +0
+    // in their place is exact) and set no mask bit; the update is branch-free so that its
+    // nine LDS reads go out together
+    double v[9];
+    unsigned m = 0;
+#pragma unroll
+    for (int i = 0; i < 9; ++i) {
+        const bool nz = o[i] != 0.0;
+        v[i] = nz ? o[i] : 0.0;
+        m |= nz ? (1u << i) : 0u;
+    }
     for (int r = 0; r < R; ++r) {
         if (mine == r) {
-            unsigned m = 0;
+            double cur[9];
 #pragma unroll
-            for (int i = 0; i < 9; ++i)
-                if (o[i] != 0.0) { acc[9 * a + i] += o[i]; m |= 1u << i; }
-            msk[a] |= (unsigned short)m;
+            for (int i = 0; i < 9; ++i) cur[i] = acc[9 * a + i];
+            const unsigned short mo = msk[a];
+#pragma unroll
+            for (int i = 0; i < 9; ++i) acc[9 * a + i] = cur[i] + v[i];
+            msk[a] = (unsigned short)(mo | m);
         }
         wave_lds_order();
     }
