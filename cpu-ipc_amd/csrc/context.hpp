@@ -139,6 +139,7 @@ struct Context {
     DBuf<double> chess;        // per contact: 78 upper-triangle Hessian + 12 gradient entries (SoA)
     DBuf<int> gkey, gkey_sorted, gidx, gidx_sorted;
     DBuf<double> gval;         // 3 per record
+    DBuf<int> run_start;       // start of each run of equal sorted contact block keys
     DBuf<unsigned long long> ukey;  // unique contact block keys
     DBuf<double> uval;
     DBuf<unsigned short> uflag;

@@ -74,23 +74,37 @@ __device__ inline int run_end(const K* __restrict__ k, int n, int i, K key) {
     while (b - a > 1) { const int m = (a + b) >> 1; if (k[m] == key) a = m; else b = m; }
     return b;
 }
-__global__ void reduce_blocks(int n, const unsigned long long* __restrict__ k, const int* __restrict__ idx,
-                              const int* __restrict__ head, const int* __restrict__ pos, const double* __restrict__ val,
-                              unsigned long long* __restrict__ ukey, double* __restrict__ uval, unsigned short* __restrict__ uflag) {
+// Start of every run, in run order (start[pos[i]] = i at each head), and one past the last
+// keyed record after the last run, so that run o is [start[o], start[o + 1]).
+__global__ void run_starts_u64(int n, const unsigned long long* __restrict__ k, const int* __restrict__ head,
+                               const int* __restrict__ pos, int* __restrict__ start) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n || !head[i]) return;
+    if (i > n) return;
+    if (i < n && head[i]) start[pos[i]] = i;
+    const bool end = (i == n) ? (n == 0 || k[n - 1] != kNoKey) : (k[i] == kNoKey && (i == 0 || k[i - 1] != kNoKey));
+    if (end) start[pos[n]] = i;
+}
+// One thread per run (unique block): the records' 3x3 values summed in record order, entries
+// equal to zero skipped, and the mask of entries that received a nonzero value.
+__global__ void reduce_blocks(int n, const unsigned long long* __restrict__ k, const int* __restrict__ idx,
+                              const int* __restrict__ start, const int* __restrict__ pos, const double* __restrict__ val,
+                              unsigned long long* __restrict__ ukey, double* __restrict__ uval, unsigned short* __restrict__ uflag) {
+    const int o = blockIdx.x * blockDim.x + threadIdx.x;
+    if (o >= pos[n]) return;
+    const int i = start[o], e = start[o + 1];
     const unsigned long long key = k[i];
-    const int e = run_end(k, n, i, key);
     double acc[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
     unsigned mask = 0;
 #pragma unroll 2
     for (int j = i; j < e; ++j) {
         const double* v = val + (size_t)9 * idx[j];
 #pragma unroll
-        for (int c = 0; c < 9; ++c)
-            if (v[c] != 0.0) { acc[c] += v[c]; mask |= 1u << c; }
+        for (int c = 0; c < 9; ++c) {
+            const bool nz = v[c] != 0.0;
+            acc[c] += nz ? v[c] : 0.0;   // exact: an accumulator starting at +0 never holds -0
+            mask |= nz ? (1u << c) : 0u;
+        }
     }
-    const int o = pos[i];
     ukey[o] = key;
 #pragma unroll
     for (int c = 0; c < 9; ++c) uval[(size_t)9 * o + c] = acc[c];
@@ -254,8 +268,11 @@ void finalize_csr(Context& c) {
         hipLaunchKernelGGL(run_heads_u64, dim3(blocks_for(nb + 1, 256)), dim3(256), 0, c.stream, nb, c.ckey_sorted.p, head);
         CIPC_CHECK_LAUNCH();
         exclusive_scan_int(c, head, pos, nb + 1);
+        c.run_start.resize((size_t)nb + 2);
+        hipLaunchKernelGGL(run_starts_u64, dim3(blocks_for(nb + 1, 256)), dim3(256), 0, c.stream, nb, c.ckey_sorted.p, head, pos,
+                           c.run_start.p);
         hipLaunchKernelGGL(reduce_blocks, dim3(blocks_for(nb, 256)), dim3(256), 0, c.stream, nb, c.ckey_sorted.p,
-                           c.cidx_sorted.p, head, pos, c.cval.p, c.ukey.p, c.uval.p, c.uflag.p);
+                           c.cidx_sorted.p, c.run_start.p, pos, c.cval.p, c.ukey.p, c.uval.p, c.uflag.p);
         CIPC_CHECK_LAUNCH();
         CIPC_HIP(hipMemcpyAsync(c.counters.p, pos + nb, sizeof(int), hipMemcpyDeviceToDevice, c.stream));
     } else {
