@@ -528,26 +528,23 @@ void friction_update(Context& c, double kappa) {
     c.n_fenv = ne;
 }
 
-void friction_records(Context& c, size_t slot_b, size_t slot_g) {
+void friction_records(Context& c, size_t slot_b, size_t slot_g, hipStream_t st) {
     Scene& s = c.sc;
     const int nf = (int)c.n_fself, ngf = (int)c.n_fenv;
     const double eps2 = s.fhat * s.dt * s.dt;
     if (ngf > 0) {
-        hipLaunchKernelGGL(friction_ground_kernel, dim3(blocks_for(ngf, 256)), dim3(256), 0, c.stream, ngf, c.fr_env.p, c.fr_env_lam.p,
+        hipLaunchKernelGGL(friction_ground_kernel, dim3(blocks_for(ngf, 256)), dim3(256), 0, st, ngf, c.fr_env.p, c.fr_env_lam.p,
                            s.x.p, s.vprev.p, s.bt.p, s.gn, eps2, s.fric, s.kb, (int)slot_b, (int)slot_g, c.ckey.p, c.cidx.p, c.cval.p,
                            c.gkey.p, c.gidx.p, c.gval.p);
         CIPC_CHECK_LAUNCH();
     }
     if (nf > 0) {
-        c.fhess.resize((size_t)(kHU + 12) * nf);
-        c.fverts.resize(nf);
-        c.fdims.resize(nf);
         double* Hs = c.fhess.p;
         double* Gs = c.fhess.p + (size_t)kHU * nf;
-        hipLaunchKernelGGL(friction_local_kernel, dim3(blocks_for(nf, 64)), dim3(64), 0, c.stream, nf, c.fr_self.p, c.fr_lam.p,
+        hipLaunchKernelGGL(friction_local_kernel, dim3(blocks_for(nf, 64)), dim3(64), 0, st, nf, c.fr_self.p, c.fr_lam.p,
                            c.fr_coord.p, c.fr_basis.p, s.x.p, s.vprev.p, eps2, s.fric, Hs, Gs, c.fverts.p, c.fdims.p);
         CIPC_CHECK_LAUNCH();
-        hipLaunchKernelGGL(contact_scatter_kernel, dim3(blocks_for(nf, 128)), dim3(128), 0, c.stream, nf, c.fverts.p, c.fdims.p, Hs, Gs,
+        hipLaunchKernelGGL(contact_scatter_kernel, dim3(blocks_for(nf, 128)), dim3(128), 0, st, nf, c.fverts.p, c.fdims.p, Hs, Gs,
                            s.bt.p, (int)(slot_b + ngf), (int)(slot_g + ngf), s.kb, c.ckey.p, c.cidx.p, c.cval.p, c.gkey.p, c.gidx.p,
                            c.gval.p);
         CIPC_CHECK_LAUNCH();
@@ -572,15 +569,29 @@ void contact_assemble(Context& c, double kappa) {
     c.cval.resize(9 * std::max<size_t>(nb, 1));
     c.gkey.resize(std::max<size_t>(ng, 1)); c.gidx.resize(std::max<size_t>(ng, 1));
     c.gval.resize(3 * std::max<size_t>(ng, 1));
+    // lagged friction (IPCSolver.cpp:309-317): its records go after the barrier records and
+    // depend on the lagged state and the positions only, so they are built on the auxiliary
+    // stream beside the barrier blocks (buffers sized first: a reallocation synchronises)
+    if (c.n_fself > 0) {
+        c.fhess.resize((size_t)(kHU + 12) * c.n_fself);
+        c.fverts.resize(c.n_fself);
+        c.fdims.resize(c.n_fself);
+    }
+    if (n > 0) {
+        c.chess.resize((size_t)(kHU + 12) * n);
+        c.cverts.resize(n);
+        c.cdims.resize(n);
+    }
+    CIPC_HIP(hipEventRecord(c.fr_fork, c.stream));
+    CIPC_HIP(hipStreamWaitEvent(c.aux, c.fr_fork, 0));
+    friction_records(c, nbc, ngc, c.aux);
+    CIPC_HIP(hipEventRecord(c.fr_join, c.aux));
     if (ne > 0) {
         hipLaunchKernelGGL(ground_kernel, dim3(blocks_for(ne, 256)), dim3(256), 0, c.stream, ne, c.env_set.p, s.x.p, s.bt.p,
                            s.gn, s.gd, s.hhat, scene_kappa(s, kappa), s.kb, c.ckey.p, c.cidx.p, c.cval.p, c.gkey.p, c.gidx.p, c.gval.p);
         CIPC_CHECK_LAUNCH();
     }
     if (n > 0) {
-        c.chess.resize((size_t)(kHU + 12) * n);
-        c.cverts.resize(n);
-        c.cdims.resize(n);
         double* Hs = c.chess.p;
         double* Gs = c.chess.p + (size_t)kHU * n;
         hipLaunchKernelGGL(contact_local_kernel, dim3(blocks_for(n, 64)), dim3(64), 0, c.stream, (int)c.n_self, (int)c.n_ee,
@@ -588,21 +599,27 @@ void contact_assemble(Context& c, double kappa) {
                            c.cverts.p, c.cdims.p);
         CIPC_CHECK_LAUNCH();
         const int ns = (int)c.n_self;
+        // the projections of 2- and 3-vertex contacts run on the auxiliary stream (after the
+        // friction records) beside those of the 4-vertex ones: each kernel touches only the
+        // contacts of its size
         if (ns > 0) {
+            CIPC_HIP(hipEventRecord(c.cl_done, c.stream));
+            CIPC_HIP(hipStreamWaitEvent(c.aux, c.cl_done, 0));
+            hipLaunchKernelGGL((contact_pd_kernel<3, true>), dim3(blocks_for(ns, 64)), dim3(64), 0, c.aux, 0, ns, n, c.cdims.p, Hs, Gs);
+            hipLaunchKernelGGL((contact_pd_kernel<2, false>), dim3(blocks_for(ns, 64)), dim3(64), 0, c.aux, 0, ns, n, c.cdims.p, Hs, Gs);
+            CIPC_HIP(hipEventRecord(c.fr_join, c.aux));
             hipLaunchKernelGGL((contact_pd_kernel<4, true>), dim3(blocks_for(ns, 64)), dim3(64), 0, c.stream, 0, ns, n, c.cdims.p, Hs, Gs);
-            hipLaunchKernelGGL((contact_pd_kernel<3, true>), dim3(blocks_for(ns, 64)), dim3(64), 0, c.stream, 0, ns, n, c.cdims.p, Hs, Gs);
-            hipLaunchKernelGGL((contact_pd_kernel<2, false>), dim3(blocks_for(ns, 64)), dim3(64), 0, c.stream, 0, ns, n, c.cdims.p, Hs, Gs);
         }
         if (n > ns)
             hipLaunchKernelGGL((contact_pd_kernel<4, false>), dim3(blocks_for(n - ns, 64)), dim3(64), 0, c.stream, ns, n, n, c.cdims.p, Hs,
                                Gs);
         CIPC_CHECK_LAUNCH();
+        CIPC_HIP(hipStreamWaitEvent(c.stream, c.fr_join, 0));
         hipLaunchKernelGGL(contact_scatter_kernel, dim3(blocks_for(n, 128)), dim3(128), 0, c.stream, n, c.cverts.p,
                            c.cdims.p, Hs, Gs, s.bt.p, ne, ne, s.kb, c.ckey.p, c.cidx.p, c.cval.p, c.gkey.p, c.gidx.p, c.gval.p);
         CIPC_CHECK_LAUNCH();
     }
-    // lagged friction (IPCSolver.cpp:309-317), after the barrier records
-    friction_records(c, nbc, ngc);
+    CIPC_HIP(hipStreamWaitEvent(c.stream, c.fr_join, 0));
     c.ckey.n = nb;
     c.gkey.n = ng;
 }

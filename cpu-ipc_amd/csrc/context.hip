@@ -77,6 +77,9 @@ int cipc_gpu_context_create(int device, cipc_gpu_context** out) {
         CIPC_HIP(hipStreamCreateWithFlags(&c->aux, hipStreamNonBlocking));
         CIPC_HIP(hipEventCreateWithFlags(&c->fork, hipEventDisableTiming));
         CIPC_HIP(hipEventCreateWithFlags(&c->join, hipEventDisableTiming));
+        CIPC_HIP(hipEventCreateWithFlags(&c->fr_fork, hipEventDisableTiming));
+        CIPC_HIP(hipEventCreateWithFlags(&c->fr_join, hipEventDisableTiming));
+        CIPC_HIP(hipEventCreateWithFlags(&c->cl_done, hipEventDisableTiming));
         *out = c;
     });
 }
@@ -89,6 +92,9 @@ int cipc_gpu_context_destroy(cipc_gpu_context* c) {
         (void)hipStreamSynchronize(c->aux);
         (void)hipEventDestroy(c->fork);
         (void)hipEventDestroy(c->join);
+        (void)hipEventDestroy(c->fr_fork);
+        (void)hipEventDestroy(c->fr_join);
+        (void)hipEventDestroy(c->cl_done);
         (void)hipStreamDestroy(c->aux);
         (void)hipStreamDestroy(c->stream);
         destroy_learned(*c);

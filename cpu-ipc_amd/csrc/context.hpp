@@ -88,6 +88,8 @@ struct Context {
     // `stream` builds the active set and the contact blocks (fork / join events).
     hipStream_t aux = nullptr;
     hipEvent_t fork = nullptr, join = nullptr;
+    // friction records and the small contact projections on `aux` beside the barrier blocks
+    hipEvent_t fr_fork = nullptr, fr_join = nullptr, cl_done = nullptr;
     Scene sc;
     StaticPattern sp;
     bool have_scene = false;
@@ -262,7 +264,7 @@ double learned_loss_grad(Context& c, const float* params, const float* host_prob
 double learned_train_step(Context& c, float* params, float* m1, float* m2, int step, double lr, int nz, unsigned long long seed,
                           int objective);
 // Friction records after the contact records; returns the friction energy parts' launch
-void friction_records(Context& c, size_t slot_b, size_t slot_g);
+void friction_records(Context& c, size_t slot_b, size_t slot_g, hipStream_t st);
 void friction_energy_partials(Context& c, double* part, double* elem_self, double* elem_ground, hipStream_t st);
 void step_forward(Context& c, double alpha);
 // per scene: x = x0 - alpha[scene] p
