@@ -689,11 +689,13 @@ __global__ void ground_scatter(int Vs, const int* __restrict__ sv, const int* __
 }
 
 namespace {
+// The active set's counts come back through pinned host memory: a device-to-host copy into
+// pageable memory is staged by the runtime and costs several times the copy itself.
 int read_int(Context& c, const int* p) {
-    int v = 0;
-    CIPC_HIP(hipMemcpyAsync(&v, p, sizeof(int), hipMemcpyDeviceToHost, c.stream));
+    c.pin_as.reserve(1);
+    CIPC_HIP(hipMemcpyAsync(c.pin_as.p, p, sizeof(int), hipMemcpyDeviceToHost, c.stream));
     CIPC_HIP(hipStreamSynchronize(c.stream));
-    return v;
+    return c.pin_as.p[0];
 }
 }  // namespace
 
@@ -730,7 +732,8 @@ void compute_active_set(Context& c) {
     hipLaunchKernelGGL(fill_tail, dim3(1), dim3(1), 0, c.stream, b.cntEE.p, Es);
     exclusive_scan_int(c, b.cntPT.p, b.offPT.p, Vs + 1);
     exclusive_scan_int(c, b.cntEE.p, b.offEE.p, Es + 1);
-    int nh[2];
+    c.pin_as.reserve(16);
+    int* nh = c.pin_as.p;
     CIPC_HIP(hipMemcpyAsync(&nh[0], b.offPT.p + Vs, sizeof(int), hipMemcpyDeviceToHost, c.stream));
     CIPC_HIP(hipMemcpyAsync(&nh[1], b.offEE.p + Es, sizeof(int), hipMemcpyDeviceToHost, c.stream));
     CIPC_HIP(hipStreamSynchronize(c.stream));
@@ -764,12 +767,13 @@ void compute_active_set(Context& c) {
     hipLaunchKernelGGL(ground_flags, dim3(blocks_for(Vs + 1, 256)), dim3(256), 0, c.stream, Vs, s.sv.p, s.x.p, s.gn, s.gd, s.hhat, b.fg.p);
     CIPC_CHECK_LAUNCH();
     exclusive_scan_int(c, b.fg.p, b.pg.p, Vs + 1);
-    int4 tpt, tee;
-    int nG = 0;
-    CIPC_HIP(hipMemcpyAsync(&tpt, b.posPT.p + nPT, sizeof(int4), hipMemcpyDeviceToHost, c.stream));
-    CIPC_HIP(hipMemcpyAsync(&tee, b.posEE.p + nEE, sizeof(int4), hipMemcpyDeviceToHost, c.stream));
-    CIPC_HIP(hipMemcpyAsync(&nG, b.pg.p + Vs, sizeof(int), hipMemcpyDeviceToHost, c.stream));
+    int* hc = c.pin_as.p;   // tpt (4), tee (4), nG (1)
+    CIPC_HIP(hipMemcpyAsync(hc, b.posPT.p + nPT, sizeof(int4), hipMemcpyDeviceToHost, c.stream));
+    CIPC_HIP(hipMemcpyAsync(hc + 4, b.posEE.p + nEE, sizeof(int4), hipMemcpyDeviceToHost, c.stream));
+    CIPC_HIP(hipMemcpyAsync(hc + 8, b.pg.p + Vs, sizeof(int), hipMemcpyDeviceToHost, c.stream));
     CIPC_HIP(hipStreamSynchronize(c.stream));
+    const int4 tpt = make_int4(hc[0], hc[1], hc[2], hc[3]), tee = make_int4(hc[4], hc[5], hc[6], hc[7]);
+    const int nG = hc[8];
     const int nA = tpt.x, n2a = tpt.y, nPTk = tpt.z;
     const int nB = tee.x, n2b = tee.y, nE = tee.z, nEEk = tee.w;
     // PP/PE map keys need the vertex count to fit three fields in 64 bits

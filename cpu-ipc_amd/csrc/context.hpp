@@ -159,6 +159,7 @@ struct Context {
     bool nnz_pending = false;  // nnz is in pin_nnz once the stream has been synchronised
     // pinned destinations of the asynchronous reads that one final synchronisation collects
     PinnedBuf<int> pin_nnz;
+    PinnedBuf<int> pin_as;     // counts of the active-set construction
     PinnedBuf<unsigned long long> pin_slots;
     PinnedBuf<double> pin_energy;
     bool assembled = false;    // a system exists for the linear solve
