@@ -910,8 +910,7 @@ void scene_intersections(Context& c, int* out) {
                            et, scene_of_vertex(s), c.counters.p + S);
     CIPC_CHECK_LAUNCH();
     std::vector<int> h(2 * (size_t)S);
-    CIPC_HIP(hipMemcpyAsync(h.data(), c.counters.p, 2 * (size_t)S * sizeof(int), hipMemcpyDeviceToHost, c.stream));
-    CIPC_HIP(hipStreamSynchronize(c.stream));
+    read_back(c, h.data(), (const int*)c.counters.p, h.size());
     for (int k = 0; k < S; ++k) out[k] = (h[k] != 0 || h[S + k] != 0) ? 1 : 0;
 }
 

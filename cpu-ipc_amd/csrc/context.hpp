@@ -3,6 +3,8 @@
 #pragma once
 #include "common.hpp"
 
+#include <cstring>
+
 namespace cipc {
 
 // Linear BVH in the Karras layout: internal nodes [0, n-2], leaves [n-1, 2n-2] in Morton
@@ -160,6 +162,7 @@ struct Context {
     // pinned destinations of the asynchronous reads that one final synchronisation collects
     PinnedBuf<int> pin_nnz;
     PinnedBuf<int> pin_as;     // counts of the active-set construction
+    PinnedBuf<unsigned char> pin_rb;   // staging of read_back
     PinnedBuf<unsigned long long> pin_slots;
     PinnedBuf<double> pin_energy;
     bool assembled = false;    // a system exists for the linear solve
@@ -181,6 +184,17 @@ struct Context {
     DBuf<unsigned long long> u64scratch, u64scratch2;
     DBuf<int4> i4scratch;
 };
+
+// n values from the device to the host through pinned memory (a copy into pageable memory
+// is staged by the runtime and costs several times the copy), after the stream's pending work
+template <typename T>
+void read_back(Context& c, T* host, const T* dev, size_t n) {
+    if (n == 0) return;
+    c.pin_rb.reserve(n * sizeof(T));
+    CIPC_HIP(hipMemcpyAsync(c.pin_rb.p, dev, n * sizeof(T), hipMemcpyDeviceToHost, c.stream));
+    CIPC_HIP(hipStreamSynchronize(c.stream));
+    std::memcpy(static_cast<void*>(host), c.pin_rb.p, n * sizeof(T));
+}
 
 // elastic.hip
 void build_static_pattern(Context& c);

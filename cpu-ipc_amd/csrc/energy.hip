@@ -328,8 +328,7 @@ void energy_terms_scenes(Context& c, double* out) {
     scene_list_sums<int4>(c, (int)nFs, c.fr_self.p, eFs, 5, dev_out);
     scene_list_sums<int>(c, (int)nFg, c.fr_env.p, eFg, 6, dev_out);
     std::vector<double> h((size_t)9 * S);
-    CIPC_HIP(hipMemcpyAsync(h.data(), dev_out, h.size() * sizeof(double), hipMemcpyDeviceToHost, c.stream));
-    CIPC_HIP(hipStreamSynchronize(c.stream));
+    read_back(c, h.data(), dev_out, h.size());
     for (int k = 0; k < S; ++k) {
         const double* t = h.data() + 9 * (size_t)k;
         double* o = out + 9 * (size_t)k;

@@ -620,8 +620,7 @@ __global__ void bsr_mirror(int V, const int* __restrict__ rowptr, const int* __r
 namespace {
 void read_scalars(Context& c, const PcgScalars* d, int S, std::vector<PcgScalars>& h) {
     h.resize(S);
-    CIPC_HIP(hipMemcpyAsync(h.data(), d, S * sizeof(PcgScalars), hipMemcpyDeviceToHost, c.stream));
-    CIPC_HIP(hipStreamSynchronize(c.stream));
+    read_back(c, h.data(), d, (size_t)S);
 }
 bool all_done(const std::vector<PcgScalars>& h) {
     for (const PcgScalars& s : h)
