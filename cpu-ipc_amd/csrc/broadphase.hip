@@ -808,8 +808,9 @@ void compute_active_set(Context& c) {
                            c.self_set.p);
         CIPC_CHECK_LAUNCH();
     }
+    // the counts are known on the host; the sets themselves are read in stream order by the
+    // next stages, so no synchronisation here
     c.n_self = nA + nB + nM; c.n_ee = nE; c.n_ccd = nPTk + nEEk; c.n_env = nG;
-    CIPC_HIP(hipStreamSynchronize(c.stream));
 }
 
 // ---------------------------------------------------------------------------------

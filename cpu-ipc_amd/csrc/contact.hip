@@ -583,9 +583,6 @@ void contact_assemble(Context& c, double kappa) {
         c.cdims.resize(n);
     }
     CIPC_HIP(hipEventRecord(c.fr_fork, c.stream));
-    CIPC_HIP(hipStreamWaitEvent(c.aux, c.fr_fork, 0));
-    friction_records(c, nbc, ngc, c.aux);
-    CIPC_HIP(hipEventRecord(c.fr_join, c.aux));
     if (ne > 0) {
         hipLaunchKernelGGL(ground_kernel, dim3(blocks_for(ne, 256)), dim3(256), 0, c.stream, ne, c.env_set.p, s.x.p, s.bt.p,
                            s.gn, s.gd, s.hhat, scene_kappa(s, kappa), s.kb, c.ckey.p, c.cidx.p, c.cval.p, c.gkey.p, c.gidx.p, c.gval.p);
@@ -598,6 +595,14 @@ void contact_assemble(Context& c, double kappa) {
                            c.self_set.p, c.ee_set.p, c.ee_ie.p, s.se.p, s.x.p, s.vrest.p, s.hhat, scene_kappa(s, kappa), Hs, Gs,
                            c.cverts.p, c.cdims.p);
         CIPC_CHECK_LAUNCH();
+    }
+    // (enqueued after the first barrier kernels so that the main stream starts at once)
+    CIPC_HIP(hipStreamWaitEvent(c.aux, c.fr_fork, 0));
+    friction_records(c, nbc, ngc, c.aux);
+    CIPC_HIP(hipEventRecord(c.fr_join, c.aux));
+    if (n > 0) {
+        double* Hs = c.chess.p;
+        double* Gs = c.chess.p + (size_t)kHU * n;
         const int ns = (int)c.n_self;
         // the projections of 2- and 3-vertex contacts run on the auxiliary stream (after the
         // friction records) beside those of the 4-vertex ones: each kernel touches only the

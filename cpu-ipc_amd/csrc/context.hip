@@ -399,6 +399,7 @@ int cipc_gpu_active_set(cipc_gpu_context* c, int64_t counts[4]) {
         require(c && c->have_scene, "no scene uploaded");
         bind(*c);
         compute_active_set(*c);
+        CIPC_HIP(hipStreamSynchronize(c->stream));   // the call returns with the sets built
         if (counts) { counts[0] = c->n_self; counts[1] = c->n_ee; counts[2] = c->n_ccd; counts[3] = c->n_env; }
     });
 }
@@ -962,13 +963,15 @@ int cipc_gpu_newton_iteration(cipc_gpu_context* c, double kappa, cipc_iteration_
         // step bound and energy with one synchronisation: both read the state and the active
         // set only, so the energy runs on the auxiliary stream beside the CCD kernels, and
         // their results come back through asynchronous pinned reads
+        // (the step bound's launches go first so that the main stream is never idle while the
+        // host enqueues the energy kernels)
         CIPC_HIP(hipEventRecord(c->fork, c->stream));
+        step_bound_launch(*c);
+        CIPC_HIP(hipEventRecord(ev[3], c->stream));
         CIPC_HIP(hipStreamWaitEvent(c->aux, c->fork, 0));
         CIPC_HIP(hipEventRecord(e0, c->aux));
         energy_terms_launch(*c, c->aux);
         CIPC_HIP(hipEventRecord(en, c->aux));
-        step_bound_launch(*c);
-        CIPC_HIP(hipEventRecord(ev[3], c->stream));
         CIPC_HIP(hipStreamWaitEvent(c->stream, en, 0));
         CIPC_HIP(hipEventRecord(ev[4], c->stream));
         CIPC_HIP(hipEventSynchronize(ev[4]));
