@@ -596,35 +596,37 @@ void contact_assemble(Context& c, double kappa) {
                            c.cverts.p, c.cdims.p);
         CIPC_CHECK_LAUNCH();
     }
-    // (enqueued after the first barrier kernels so that the main stream starts at once)
-    CIPC_HIP(hipStreamWaitEvent(c.aux, c.fr_fork, 0));
-    friction_records(c, nbc, ngc, c.aux);
-    CIPC_HIP(hipEventRecord(c.fr_join, c.aux));
+    // Main stream: the projections of 4-vertex contacts.  Second stream (enqueued after them,
+    // so that the main stream never waits on the host): the friction records, then the
+    // projections of 2- and 3-vertex contacts.  Each projection kernel touches only the
+    // contacts of its size.
+    const int ns = (int)c.n_self;
+    double* Hs = c.chess.p;
+    double* Gs = c.chess.p + (size_t)kHU * n;
     if (n > 0) {
-        double* Hs = c.chess.p;
-        double* Gs = c.chess.p + (size_t)kHU * n;
-        const int ns = (int)c.n_self;
-        // the projections of 2- and 3-vertex contacts run on the auxiliary stream (after the
-        // friction records) beside those of the 4-vertex ones: each kernel touches only the
-        // contacts of its size
-        if (ns > 0) {
-            CIPC_HIP(hipEventRecord(c.cl_done, c.stream));
-            CIPC_HIP(hipStreamWaitEvent(c.aux, c.cl_done, 0));
-            hipLaunchKernelGGL((contact_pd_kernel<3, true>), dim3(blocks_for(ns, 64)), dim3(64), 0, c.aux, 0, ns, n, c.cdims.p, Hs, Gs);
-            hipLaunchKernelGGL((contact_pd_kernel<2, false>), dim3(blocks_for(ns, 64)), dim3(64), 0, c.aux, 0, ns, n, c.cdims.p, Hs, Gs);
-            CIPC_HIP(hipEventRecord(c.fr_join, c.aux));
+        CIPC_HIP(hipEventRecord(c.cl_done, c.stream));
+        if (ns > 0)
             hipLaunchKernelGGL((contact_pd_kernel<4, true>), dim3(blocks_for(ns, 64)), dim3(64), 0, c.stream, 0, ns, n, c.cdims.p, Hs, Gs);
-        }
         if (n > ns)
             hipLaunchKernelGGL((contact_pd_kernel<4, false>), dim3(blocks_for(n - ns, 64)), dim3(64), 0, c.stream, ns, n, n, c.cdims.p, Hs,
                                Gs);
         CIPC_CHECK_LAUNCH();
-        CIPC_HIP(hipStreamWaitEvent(c.stream, c.fr_join, 0));
+    }
+    CIPC_HIP(hipStreamWaitEvent(c.aux, c.fr_fork, 0));
+    friction_records(c, nbc, ngc, c.aux);
+    if (n > 0 && ns > 0) {
+        CIPC_HIP(hipStreamWaitEvent(c.aux, c.cl_done, 0));
+        hipLaunchKernelGGL((contact_pd_kernel<3, true>), dim3(blocks_for(ns, 64)), dim3(64), 0, c.aux, 0, ns, n, c.cdims.p, Hs, Gs);
+        hipLaunchKernelGGL((contact_pd_kernel<2, false>), dim3(blocks_for(ns, 64)), dim3(64), 0, c.aux, 0, ns, n, c.cdims.p, Hs, Gs);
+        CIPC_CHECK_LAUNCH();
+    }
+    CIPC_HIP(hipEventRecord(c.fr_join, c.aux));
+    CIPC_HIP(hipStreamWaitEvent(c.stream, c.fr_join, 0));
+    if (n > 0) {
         hipLaunchKernelGGL(contact_scatter_kernel, dim3(blocks_for(n, 128)), dim3(128), 0, c.stream, n, c.cverts.p,
                            c.cdims.p, Hs, Gs, s.bt.p, ne, ne, s.kb, c.ckey.p, c.cidx.p, c.cval.p, c.gkey.p, c.gidx.p, c.gval.p);
         CIPC_CHECK_LAUNCH();
     }
-    CIPC_HIP(hipStreamWaitEvent(c.stream, c.fr_join, 0));
     c.ckey.n = nb;
     c.gkey.n = ng;
 }
