@@ -140,18 +140,6 @@ __global__ void skip_kernel(int n, const int* __restrict__ range_last, const int
     const int last = (i < n - 1) ? range_last[i] : i - (n - 1);
     nodes[i].skip = (last >= n - 1) ? -1 : split_right[last];
 }
-// Leaf boxes of the current primitive boxes, in leaf order (node records and level 0).
-__global__ void leaf_boxes(int n, const int* __restrict__ order, const Box* __restrict__ prim, Node* __restrict__ nodes,
-                           Box* __restrict__ seg0) {
-    const int s = blockIdx.x * blockDim.x + threadIdx.x;
-    if (s >= n) return;
-    const Box b = prim[order[s]];
-    Node& r = nodes[n - 1 + s];
-    r.lo = b.lo;
-    r.hi = b.hi;
-    seg0[s] = b;
-}
-
 // Internal boxes.  Node i of the Karras tree covers the sorted leaves [first, last], so its
 // box is the union of that range.  Power-of-two reduction levels over the sorted leaf boxes
 // (level l node j = leaves [j 2^l, (j+1) 2^l)) are built 8 levels per launch in LDS; each
@@ -182,17 +170,52 @@ __global__ __launch_bounds__(kSegBlock) void seg_build(const Box* __restrict__ s
         }
     }
 }
-__global__ void internal_boxes(int n, const int* __restrict__ first, const int* __restrict__ last, SegLevels L,
-                               const Box* __restrict__ seg, Node* __restrict__ nodes) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n - 1) return;
-    int a = first[i], b = last[i], l = 0;
+// Box of the sorted leaves [a, b]: the union of at most 2 log2(n) level nodes (exact: the
+// union is a min / max).
+__device__ inline Box range_box(int a, int b, const SegLevels& L, const Box* __restrict__ seg) {
+    int l = 0;
     Box acc = empty_box();
     while (a <= b && l < 40) {
         if (a & 1) { const Box x = seg[L.offset[l] + a]; acc = Box{vmin(acc.lo, x.lo), vmax(acc.hi, x.hi)}; ++a; }
         if (a <= b && !(b & 1)) { const Box x = seg[L.offset[l] + b]; acc = Box{vmin(acc.lo, x.lo), vmax(acc.hi, x.hi)}; --b; }
         a >>= 1; b >>= 1; ++l;
     }
+    return acc;
+}
+// Leaf boxes in leaf order (node records and level 0) and the first kSegLevels reduction
+// levels, in one launch.
+__global__ __launch_bounds__(kSegBlock) void leaf_seg_build(int n, const int* __restrict__ order, const Box* __restrict__ prim,
+                                                            Node* __restrict__ nodes, SegLevels L, Box* __restrict__ seg) {
+    __shared__ Box sh[kSegBlock];
+    const int g = blockIdx.x * kSegBlock + threadIdx.x;
+    Box b = empty_box();
+    if (g < n) {
+        b = prim[order[g]];
+        Node& r = nodes[n - 1 + g];
+        r.lo = b.lo;
+        r.hi = b.hi;
+        seg[g] = b;
+    }
+    sh[threadIdx.x] = b;
+    __syncthreads();
+    for (int l = 1; l <= kSegLevels; ++l) {
+        const int half = 1 << (l - 1);
+        if ((threadIdx.x & ((1 << l) - 1)) == 0) {
+            const Box a = sh[threadIdx.x], c = sh[threadIdx.x + half];
+            sh[threadIdx.x] = Box{vmin(a.lo, c.lo), vmax(a.hi, c.hi)};
+        }
+        __syncthreads();
+        if (l < 40 && L.count[l] > 0 && (threadIdx.x & ((1 << l) - 1)) == 0) {
+            const int j = g >> l;
+            if (j < L.count[l]) seg[L.offset[l] + j] = sh[threadIdx.x];
+        }
+    }
+}
+__global__ void internal_boxes(int n, const int* __restrict__ first, const int* __restrict__ last, SegLevels L,
+                               const Box* __restrict__ seg, Node* __restrict__ nodes) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n - 1) return;
+    const Box acc = range_box(first[i], last[i], L, seg);
     nodes[i].lo = acc.lo;
     nodes[i].hi = acc.hi;
 }
@@ -279,10 +302,12 @@ void build_topology(Context& c, LBVH& t, int n) {
 // Boxes of the current primitive boxes on the existing topology.
 void refit(Context& c, LBVH& t) {
     const int n = t.n;
-    hipLaunchKernelGGL(leaf_boxes, dim3(blocks_for(n, 256)), dim3(256), 0, c.stream, n, t.order.p, t.prim.p, t.nodes.p, t.seg.p);
+    // leaves and levels 1..8 in one launch, the higher levels 8 per launch
+    hipLaunchKernelGGL(leaf_seg_build, dim3(blocks_for(n, kSegBlock)), dim3(kSegBlock), 0, c.stream, n, t.order.p, t.prim.p, t.nodes.p,
+                       t.L, t.seg.p);
     CIPC_CHECK_LAUNCH();
     if (n > 1) {
-        for (int l0 = 0; l0 < t.levels; l0 += kSegLevels) {
+        for (int l0 = kSegLevels; l0 < t.levels; l0 += kSegLevels) {
             const int nsrc = t.L.count[l0];
             hipLaunchKernelGGL(seg_build, dim3(blocks_for(nsrc, kSegBlock)), dim3(kSegBlock), 0, c.stream, t.seg.p + t.L.offset[l0],
                                nsrc, l0, t.L, t.seg.p);
