@@ -170,6 +170,43 @@ __global__ __launch_bounds__(kSegBlock) void seg_build(const Box* __restrict__ s
         }
     }
 }
+
+// Primitive boxes: static (CPUBroadPhaseBVH::build, LBVH.cpp:284-305) or swept along -p
+// with the step whose bits are at *abits (buildSwept, LBVH.cpp:307-339).  The step is read
+// on the device, so building the swept trees needs no host round trip.
+__device__ inline double step_of(const unsigned long long* abits) { return __longlong_as_double((long long)*abits); }
+// Where a tree's primitive boxes come from: a face or an edge of the surface, static (the
+// box of the positions) or swept (the box of x and x - a p, a the scene's step in abits).
+struct PrimSrc {
+    int kind = 0;   // 0: the prim array holds them; 1: faces; 2: surface edges
+    const int4* faces = nullptr;
+    const int2* se = nullptr;
+    const double* x = nullptr;
+    const double* p = nullptr;
+    const unsigned long long* abits = nullptr;
+    const int* vs = nullptr;
+};
+__device__ inline Box prim_box(const PrimSrc& s, int i) {
+    if (s.kind == 1) {
+        const int4 v = s.faces[i];
+        if (s.abits == nullptr) return box_tri(ld3(s.x, v.x), ld3(s.x, v.y), ld3(s.x, v.z));
+        const double a = step_of(s.vs ? s.abits + kSlots * s.vs[v.x] : s.abits);
+        Box b = swept_pt(ld3(s.x, v.x), ld3(s.p, v.x), a);
+        b = box_union(b, swept_pt(ld3(s.x, v.y), ld3(s.p, v.y), a));
+        b = box_union(b, swept_pt(ld3(s.x, v.z), ld3(s.p, v.z), a));
+        return b;
+    }
+    const int2 v = s.se[i];
+    if (s.abits == nullptr) return box_edge(ld3(s.x, v.x), ld3(s.x, v.y));
+    const double a = step_of(s.vs ? s.abits + kSlots * s.vs[v.x] : s.abits);
+    return box_union(swept_pt(ld3(s.x, v.x), ld3(s.p, v.x), a), swept_pt(ld3(s.x, v.y), ld3(s.p, v.y), a));
+}
+
+__global__ void prim_boxes(int n, PrimSrc src, Box* __restrict__ out) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) out[i] = prim_box(src, i);
+}
+
 // Box of the sorted leaves [a, b]: the union of at most 2 log2(n) level nodes (exact: the
 // union is a min / max).
 __device__ inline Box range_box(int a, int b, const SegLevels& L, const Box* __restrict__ seg) {
@@ -183,14 +220,22 @@ __device__ inline Box range_box(int a, int b, const SegLevels& L, const Box* __r
     return acc;
 }
 // Leaf boxes in leaf order (node records and level 0) and the first kSegLevels reduction
-// levels, in one launch.
-__global__ __launch_bounds__(kSegBlock) void leaf_seg_build(int n, const int* __restrict__ order, const Box* __restrict__ prim,
-                                                            Node* __restrict__ nodes, SegLevels L, Box* __restrict__ seg) {
+// levels, in one launch; with a primitive source, the primitive boxes are computed here too
+// (and stored, for the exact leaf tests).
+__global__ __launch_bounds__(kSegBlock) void leaf_seg_build(int n, const int* __restrict__ order, Box* __restrict__ prim,
+                                                            PrimSrc src, Node* __restrict__ nodes, SegLevels L,
+                                                            Box* __restrict__ seg) {
     __shared__ Box sh[kSegBlock];
     const int g = blockIdx.x * kSegBlock + threadIdx.x;
     Box b = empty_box();
     if (g < n) {
-        b = prim[order[g]];
+        const int q = order[g];
+        if (src.kind != 0) {
+            b = prim_box(src, q);
+            prim[q] = b;
+        } else {
+            b = prim[q];
+        }
         Node& r = nodes[n - 1 + g];
         r.lo = b.lo;
         r.hi = b.hi;
@@ -300,11 +345,11 @@ void build_topology(Context& c, LBVH& t, int n) {
 }
 
 // Boxes of the current primitive boxes on the existing topology.
-void refit(Context& c, LBVH& t) {
+void refit(Context& c, LBVH& t, const PrimSrc& src) {
     const int n = t.n;
     // leaves and levels 1..8 in one launch, the higher levels 8 per launch
-    hipLaunchKernelGGL(leaf_seg_build, dim3(blocks_for(n, kSegBlock)), dim3(kSegBlock), 0, c.stream, n, t.order.p, t.prim.p, t.nodes.p,
-                       t.L, t.seg.p);
+    hipLaunchKernelGGL(leaf_seg_build, dim3(blocks_for(n, kSegBlock)), dim3(kSegBlock), 0, c.stream, n, t.order.p, t.prim.p, src,
+                       t.nodes.p, t.L, t.seg.p);
     CIPC_CHECK_LAUNCH();
     if (n > 1) {
         for (int l0 = kSegLevels; l0 < t.levels; l0 += kSegLevels) {
@@ -322,38 +367,27 @@ void refit(Context& c, LBVH& t) {
     ++t.age;
 }
 
+// A tree over n primitives whose boxes come from src: a new topology (from boxes computed
+// first) when none fits, else a refit that computes the boxes itself.
+static void build_tree_from(Context& c, LBVH& t, int n, const PrimSrc& src) {
+    if (n <= 0) { t.n = n; return; }
+    t.prim.resize(n);
+    if (t.n != n || t.age >= kMaxRefits) {
+        hipLaunchKernelGGL(prim_boxes, dim3(blocks_for(n, 256)), dim3(256), 0, c.stream, n, src, t.prim.p);
+        CIPC_CHECK_LAUNCH();
+        build_topology(c, t, n);
+        refit(c, t, PrimSrc{});
+    } else {
+        refit(c, t, src);
+    }
+}
 void build_tree(Context& c, LBVH& t, int n) {
     if (n <= 0) { t.n = n; return; }
     if (t.n != n || t.age >= kMaxRefits) build_topology(c, t, n);
-    refit(c, t);
+    refit(c, t, PrimSrc{});
 }
 
-// Primitive boxes: static (CPUBroadPhaseBVH::build, LBVH.cpp:284-305) or swept along -p
-// with the step whose bits are at *abits (buildSwept, LBVH.cpp:307-339).  The step is read
-// on the device, so building the swept trees needs no host round trip.
-__device__ inline double step_of(const unsigned long long* abits) { return __longlong_as_double((long long)*abits); }
 // Swept boxes use the step of the primitive's scene: abits[kSlots * scene] (one scene: abits[0]).
-__global__ void face_boxes(int F, const int4* __restrict__ faces, const double* __restrict__ x, const double* __restrict__ p,
-                           const unsigned long long* __restrict__ abits, const int* __restrict__ vs, Box* __restrict__ out) {
-    const int f = blockIdx.x * blockDim.x + threadIdx.x;
-    if (f >= F) return;
-    const int4 v = faces[f];
-    if (abits == nullptr) { out[f] = box_tri(ld3(x, v.x), ld3(x, v.y), ld3(x, v.z)); return; }
-    const double a = step_of(vs ? abits + kSlots * vs[v.x] : abits);
-    Box b = swept_pt(ld3(x, v.x), ld3(p, v.x), a);
-    b = box_union(b, swept_pt(ld3(x, v.y), ld3(p, v.y), a));
-    b = box_union(b, swept_pt(ld3(x, v.z), ld3(p, v.z), a));
-    out[f] = b;
-}
-__global__ void edge_boxes(int E, const int2* __restrict__ se, const double* __restrict__ x, const double* __restrict__ p,
-                           const unsigned long long* __restrict__ abits, const int* __restrict__ vs, Box* __restrict__ out) {
-    const int e = blockIdx.x * blockDim.x + threadIdx.x;
-    if (e >= E) return;
-    const int2 v = se[e];
-    if (abits == nullptr) { out[e] = box_edge(ld3(x, v.x), ld3(x, v.y)); return; }
-    const double a = step_of(vs ? abits + kSlots * vs[v.x] : abits);
-    out[e] = box_union(swept_pt(ld3(x, v.x), ld3(p, v.x), a), swept_pt(ld3(x, v.y), ld3(p, v.y), a));
-}
 __global__ void point_boxes(int Vs, const int* __restrict__ sv, const double* __restrict__ x, const double* __restrict__ p,
                             const unsigned long long* __restrict__ abits, const int* __restrict__ vs, Box* __restrict__ out) {
     const int s = blockIdx.x * blockDim.x + threadIdx.x;
@@ -366,16 +400,11 @@ void build_static_trees(Context& c) {
     Scene& s = c.sc;
     c.face_tree.prim.resize(std::max(s.F, 1));
     c.edge_tree.prim.resize(std::max(s.Es, 1));
-    const unsigned long long* none = nullptr;
-    if (s.F > 0)
-        hipLaunchKernelGGL(face_boxes, dim3(blocks_for(s.F, 256)), dim3(256), 0, c.stream, s.F, s.faces.p, s.x.p, s.x.p, none, nullptr,
-                           c.face_tree.prim.p);
-    if (s.Es > 0)
-        hipLaunchKernelGGL(edge_boxes, dim3(blocks_for(s.Es, 256)), dim3(256), 0, c.stream, s.Es, s.se.p, s.x.p, s.x.p, none, nullptr,
-                           c.edge_tree.prim.p);
-    CIPC_CHECK_LAUNCH();
-    build_tree(c, c.face_tree, s.F);
-    build_tree(c, c.edge_tree, s.Es);
+    PrimSrc fs, es;
+    fs.kind = 1; fs.faces = s.faces.p; fs.x = s.x.p;
+    es.kind = 2; es.se = s.se.p; es.x = s.x.p;
+    build_tree_from(c, c.face_tree, s.F, fs);
+    build_tree_from(c, c.edge_tree, s.Es, es);
     c.face_tree.swept = c.edge_tree.swept = false;
 }
 void build_swept_trees(Context& c, const unsigned long long* abits) {
@@ -383,18 +412,15 @@ void build_swept_trees(Context& c, const unsigned long long* abits) {
     c.face_tree.prim.resize(std::max(s.F, 1));
     c.edge_tree.prim.resize(std::max(s.Es, 1));
     c.swept_point_box.resize(std::max(s.Vs, 1));
-    if (s.F > 0)
-        hipLaunchKernelGGL(face_boxes, dim3(blocks_for(s.F, 256)), dim3(256), 0, c.stream, s.F, s.faces.p, s.x.p, s.dir.p, abits, scene_of_vertex(s),
-                           c.face_tree.prim.p);
-    if (s.Es > 0)
-        hipLaunchKernelGGL(edge_boxes, dim3(blocks_for(s.Es, 256)), dim3(256), 0, c.stream, s.Es, s.se.p, s.x.p, s.dir.p, abits, scene_of_vertex(s),
-                           c.edge_tree.prim.p);
     if (s.Vs > 0)
         hipLaunchKernelGGL(point_boxes, dim3(blocks_for(s.Vs, 256)), dim3(256), 0, c.stream, s.Vs, s.sv.p, s.x.p, s.dir.p, abits, scene_of_vertex(s),
                            c.swept_point_box.p);
     CIPC_CHECK_LAUNCH();
-    build_tree(c, c.face_tree, s.F);
-    build_tree(c, c.edge_tree, s.Es);
+    PrimSrc fs, es;
+    fs.kind = 1; fs.faces = s.faces.p; fs.x = s.x.p; fs.p = s.dir.p; fs.abits = abits; fs.vs = scene_of_vertex(s);
+    es.kind = 2; es.se = s.se.p; es.x = s.x.p; es.p = s.dir.p; es.abits = abits; es.vs = scene_of_vertex(s);
+    build_tree_from(c, c.face_tree, s.F, fs);
+    build_tree_from(c, c.edge_tree, s.Es, es);
     c.face_tree.swept = c.edge_tree.swept = true;
 }
 
@@ -413,7 +439,7 @@ enum : int { kNone = 0, kSelfPT = 1, kMap = 2, kSelfEE = 3, kMolPE = 4, kMolEE =
 //     re-walking only parts with more than kPartCap hits; sort by primitive id.
 //   C (classify, one thread per hit): the reference's incidence and collider filters
 //     (CollisionBroadPhase.cpp:769-776,862-870), then the distance-type classification.
-// The leaf boxes are the primitives' own boxes (face_boxes / edge_boxes), so the leaf test
+// The leaf boxes are the primitives' own boxes (prim_box), so the leaf test
 // is the box test of the reference's candidate filter.  Queries are taken in the Morton
 // order of their rest positions (perm) so that the lanes of a wave walk nearby subtrees.
 constexpr int kPartCap = 32;
