@@ -467,6 +467,7 @@ __global__ __launch_bounds__(64) void pt_traverse(int Vs, const int* __restrict_
     }
     const int tot = sum_over_parts(n);
     if (active && part == 0) cnt[s] = tot;
+    if (blockIdx.x == 0 && threadIdx.x == 0) cnt[Vs] = 0;   // the scan's tail
 }
 
 __global__ __launch_bounds__(64) void pt_compact(int Vs, const int* __restrict__ perm, const int* __restrict__ sv, const double* __restrict__ x, TreeView tv,
@@ -564,6 +565,7 @@ __global__ __launch_bounds__(64) void ee_traverse(int Es, const int* __restrict_
     }
     const int tot = sum_over_parts(n);
     if (active && part == 0) cnt[eI] = tot;
+    if (blockIdx.x == 0 && threadIdx.x == 0) cnt[Es] = 0;   // the scan's tail
 }
 
 __global__ __launch_bounds__(64) void ee_compact(int Es, const int* __restrict__ perm, const int2* __restrict__ se, const double* __restrict__ x, TreeView tv,
@@ -779,8 +781,9 @@ void compute_active_set(Context& c) {
         hipLaunchKernelGGL(ee_traverse, dim3(blocks_for((size_t)Es * kParts, 64)), dim3(64), 0, c.stream, Es, s.se_perm.p, s.se.p, s.x.p,
                            et, sq, b.cntEE.p, b.cntpEE.p, b.bufEE.p);
     CIPC_CHECK_LAUNCH();
-    hipLaunchKernelGGL(fill_tail, dim3(1), dim3(1), 0, c.stream, b.cntPT.p, Vs);
-    hipLaunchKernelGGL(fill_tail, dim3(1), dim3(1), 0, c.stream, b.cntEE.p, Es);
+    // (the traversals also zero the scans' tails)
+    if (Vs == 0) hipLaunchKernelGGL(fill_tail, dim3(1), dim3(1), 0, c.stream, b.cntPT.p, Vs);
+    if (Es == 0) hipLaunchKernelGGL(fill_tail, dim3(1), dim3(1), 0, c.stream, b.cntEE.p, Es);
     exclusive_scan_int(c, b.cntPT.p, b.offPT.p, Vs + 1);
     exclusive_scan_int(c, b.cntEE.p, b.offEE.p, Es + 1);
     c.pin_as.reserve(16);

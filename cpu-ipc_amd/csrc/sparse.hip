@@ -49,7 +49,6 @@ void sort_u32_pairs(Context& c, const unsigned* kin, unsigned* kout, const int* 
     CIPC_HIP(hipcub::DeviceRadixSort::SortPairs(c.tmp.p, bytes, kin, kout, vin, vout, n, 0, bits, c.stream));
 }
 // out[i] = sum in[0..i-1]; out has n+1 entries when in has n (out[n] = total).
-__global__ void set_zero_tail(int* p, int n) { p[n] = 0; }
 void exclusive_scan_int(Context& c, const int* in, int* out, int n) {
     size_t bytes = 0;
     CIPC_HIP(hipcub::DeviceScan::ExclusiveSum(nullptr, bytes, in, out, n, c.stream));
@@ -140,6 +139,7 @@ __device__ inline int lower_bound_u64(const unsigned long long* a, int n, unsign
 __global__ void merge_count(int V, int kb, const int* __restrict__ scol, const int* __restrict__ srow,
                             const unsigned long long* __restrict__ ukey, const int* __restrict__ nu_ptr, int* __restrict__ cnt) {
     const int b = blockIdx.x * blockDim.x + threadIdx.x;
+    if (b == 0) cnt[V] = 0;   // the scan's tail
     if (b >= V) return;
     const int nu = *nu_ptr;
     int i = scol[b], ie = scol[b + 1];
@@ -294,7 +294,6 @@ void finalize_csr(Context& c) {
     hipLaunchKernelGGL(merge_count, dim3(blocks_for(V, 256)), dim3(256), 0, c.stream, V, s.kb, c.sp.col_ptr.p, c.sp.row.p, c.ukey.p,
                        c.counters.p, cnt.p);
     CIPC_CHECK_LAUNCH();
-    hipLaunchKernelGGL(set_zero_tail, dim3(1), dim3(1), 0, c.stream, cnt.p, V);
     exclusive_scan_int(c, cnt.p, c.mptr.p, V + 1);
     const size_t mcap = (size_t)c.sp.nblocks + (size_t)nb + 1;
     c.mrow.resize(mcap); c.mcol.resize(mcap); c.mval.resize(9 * mcap); c.mflag.resize(mcap);
