@@ -303,6 +303,20 @@ class GpuContext:
         k = int(nnz[0])
         return dict(grad=g.reshape(-1, 3), outer=outer, inner=inner[:k], val=val[:k], rhs=rhs, collisions=int(col[0]))
 
+    def system(self, nnz):
+        """The gradient and lower CSR held by the context (from the last assembly or Newton
+        iteration), without assembling again; nnz as that call returned it."""
+        n = 3 * self.V
+        g = np.zeros(n)
+        _check(load().cipc_gpu_get_gradient(self.h, _dptr(g)))
+        outer = np.zeros(n + 1, np.int32)
+        inner = np.zeros(max(int(nnz), 1), np.int32)
+        val = np.zeros(max(int(nnz), 1))
+        rhs = np.zeros(n)
+        _check(load().cipc_gpu_get_csr(self.h, _iptr(outer), _iptr(inner), _dptr(val), _dptr(rhs)))
+        k = int(nnz)
+        return dict(grad=g.reshape(-1, 3), outer=outer, inner=inner[:k], val=val[:k], rhs=rhs)
+
     def energy(self, kappa):
         e = np.zeros(1)
         _check(load().cipc_gpu_energy(self.h, kappa, _dptr(e)))

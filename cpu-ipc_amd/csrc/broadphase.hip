@@ -296,7 +296,7 @@ __global__ void wide_records(int n, const Node* __restrict__ nodes, const int* _
 }
 
 // Topology from the current primitive boxes.
-void build_topology(Context& c, LBVH& t, int n) {
+void build_topology(Context& c, LBVH& t, int n, hipStream_t st, DBuf<unsigned char>& tmp) {
     t.n = n;
     t.age = 0;
     const int nodes = 2 * n - 1;
@@ -321,47 +321,47 @@ void build_topology(Context& c, LBVH& t, int n) {
     t.seg.resize((size_t)total);
     const int parts = std::min<int>(kRedParts, (int)blocks_for(n, kRedBlock));
     t.scene_partial.resize(parts);
-    hipLaunchKernelGGL(box_reduce_partial, dim3(parts), dim3(kRedBlock), 0, c.stream, n, t.prim.p, t.scene_partial.p);
+    hipLaunchKernelGGL(box_reduce_partial, dim3(parts), dim3(kRedBlock), 0, st, n, t.prim.p, t.scene_partial.p);
     CIPC_CHECK_LAUNCH();
     if (parts > 1) {
-        hipLaunchKernelGGL(box_reduce_partial, dim3(1), dim3(kRedBlock), 0, c.stream, parts, t.scene_partial.p, t.scene_partial.p);
+        hipLaunchKernelGGL(box_reduce_partial, dim3(1), dim3(kRedBlock), 0, st, parts, t.scene_partial.p, t.scene_partial.p);
         CIPC_CHECK_LAUNCH();
     }
-    hipLaunchKernelGGL(morton_keys, dim3(blocks_for(n, 256)), dim3(256), 0, c.stream, n, t.prim.p, t.scene_partial.p, 1, t.codes.p,
+    hipLaunchKernelGGL(morton_keys, dim3(blocks_for(n, 256)), dim3(256), 0, st, n, t.prim.p, t.scene_partial.p, 1, t.codes.p,
                        t.order.p + n);
     CIPC_CHECK_LAUNCH();
-    sort_u32_pairs(c, t.codes.p, t.codes_sorted.p, t.order.p + n, t.order.p, n, 30);
-    hipLaunchKernelGGL(join_keys, dim3(blocks_for(n, 256)), dim3(256), 0, c.stream, n, t.codes_sorted.p, t.order.p, t.keys_sorted.p);
+    sort_u32_pairs(t.codes.p, t.codes_sorted.p, t.order.p + n, t.order.p, n, 30, st, tmp);
+    hipLaunchKernelGGL(join_keys, dim3(blocks_for(n, 256)), dim3(256), 0, st, n, t.codes_sorted.p, t.order.p, t.keys_sorted.p);
     CIPC_CHECK_LAUNCH();
     if (n > 1) {
-        hipLaunchKernelGGL(karras_kernel, dim3(blocks_for(n - 1, 256)), dim3(256), 0, c.stream, n, t.keys_sorted.p, t.nodes.p,
+        hipLaunchKernelGGL(karras_kernel, dim3(blocks_for(n - 1, 256)), dim3(256), 0, st, n, t.keys_sorted.p, t.nodes.p,
                            t.range_first.p, t.range_last.p, t.split_right.p);
         CIPC_CHECK_LAUNCH();
     }
     t.leafpos.resize(n);
-    hipLaunchKernelGGL(leaf_ids, dim3(blocks_for(n, 256)), dim3(256), 0, c.stream, n, t.order.p, t.nodes.p, t.leafpos.p);
-    hipLaunchKernelGGL(skip_kernel, dim3(blocks_for(nodes, 256)), dim3(256), 0, c.stream, n, t.range_last.p, t.split_right.p, t.nodes.p);
+    hipLaunchKernelGGL(leaf_ids, dim3(blocks_for(n, 256)), dim3(256), 0, st, n, t.order.p, t.nodes.p, t.leafpos.p);
+    hipLaunchKernelGGL(skip_kernel, dim3(blocks_for(nodes, 256)), dim3(256), 0, st, n, t.range_last.p, t.split_right.p, t.nodes.p);
     CIPC_CHECK_LAUNCH();
 }
 
 // Boxes of the current primitive boxes on the existing topology.
-void refit(Context& c, LBVH& t, const PrimSrc& src) {
+void refit(Context& c, LBVH& t, const PrimSrc& src, hipStream_t st) {
     const int n = t.n;
     // leaves and levels 1..8 in one launch, the higher levels 8 per launch
-    hipLaunchKernelGGL(leaf_seg_build, dim3(blocks_for(n, kSegBlock)), dim3(kSegBlock), 0, c.stream, n, t.order.p, t.prim.p, src,
+    hipLaunchKernelGGL(leaf_seg_build, dim3(blocks_for(n, kSegBlock)), dim3(kSegBlock), 0, st, n, t.order.p, t.prim.p, src,
                        t.nodes.p, t.L, t.seg.p);
     CIPC_CHECK_LAUNCH();
     if (n > 1) {
         for (int l0 = kSegLevels; l0 < t.levels; l0 += kSegLevels) {
             const int nsrc = t.L.count[l0];
-            hipLaunchKernelGGL(seg_build, dim3(blocks_for(nsrc, kSegBlock)), dim3(kSegBlock), 0, c.stream, t.seg.p + t.L.offset[l0],
+            hipLaunchKernelGGL(seg_build, dim3(blocks_for(nsrc, kSegBlock)), dim3(kSegBlock), 0, st, t.seg.p + t.L.offset[l0],
                                nsrc, l0, t.L, t.seg.p);
             CIPC_CHECK_LAUNCH();
         }
-        hipLaunchKernelGGL(internal_boxes, dim3(blocks_for(n - 1, 256)), dim3(256), 0, c.stream, n, t.range_first.p, t.range_last.p,
+        hipLaunchKernelGGL(internal_boxes, dim3(blocks_for(n - 1, 256)), dim3(256), 0, st, n, t.range_first.p, t.range_last.p,
                            t.L, t.seg.p, t.nodes.p);
         CIPC_CHECK_LAUNCH();
-        hipLaunchKernelGGL(wide_records, dim3(blocks_for(n - 1, 256)), dim3(256), 0, c.stream, n, t.nodes.p, t.range_last.p, t.wide.p);
+        hipLaunchKernelGGL(wide_records, dim3(blocks_for(n - 1, 256)), dim3(256), 0, st, n, t.nodes.p, t.range_last.p, t.wide.p);
         CIPC_CHECK_LAUNCH();
     }
     ++t.age;
@@ -369,22 +369,22 @@ void refit(Context& c, LBVH& t, const PrimSrc& src) {
 
 // A tree over n primitives whose boxes come from src: a new topology (from boxes computed
 // first) when none fits, else a refit that computes the boxes itself.
-static void build_tree_from(Context& c, LBVH& t, int n, const PrimSrc& src) {
+static void build_tree_from(Context& c, LBVH& t, int n, const PrimSrc& src, hipStream_t st, DBuf<unsigned char>& tmp) {
     if (n <= 0) { t.n = n; return; }
     t.prim.resize(n);
     if (t.n != n || t.age >= kMaxRefits) {
-        hipLaunchKernelGGL(prim_boxes, dim3(blocks_for(n, 256)), dim3(256), 0, c.stream, n, src, t.prim.p);
+        hipLaunchKernelGGL(prim_boxes, dim3(blocks_for(n, 256)), dim3(256), 0, st, n, src, t.prim.p);
         CIPC_CHECK_LAUNCH();
-        build_topology(c, t, n);
-        refit(c, t, PrimSrc{});
+        build_topology(c, t, n, st, tmp);
+        refit(c, t, PrimSrc{}, st);
     } else {
-        refit(c, t, src);
+        refit(c, t, src, st);
     }
 }
 void build_tree(Context& c, LBVH& t, int n) {
     if (n <= 0) { t.n = n; return; }
-    if (t.n != n || t.age >= kMaxRefits) build_topology(c, t, n);
-    refit(c, t, PrimSrc{});
+    if (t.n != n || t.age >= kMaxRefits) build_topology(c, t, n, c.stream, c.tmp);
+    refit(c, t, PrimSrc{}, c.stream);
 }
 
 // Swept boxes use the step of the primitive's scene: abits[kSlots * scene] (one scene: abits[0]).
@@ -403,24 +403,24 @@ void build_static_trees(Context& c) {
     PrimSrc fs, es;
     fs.kind = 1; fs.faces = s.faces.p; fs.x = s.x.p;
     es.kind = 2; es.se = s.se.p; es.x = s.x.p;
-    build_tree_from(c, c.face_tree, s.F, fs);
-    build_tree_from(c, c.edge_tree, s.Es, es);
+    build_tree_from(c, c.face_tree, s.F, fs, c.stream, c.tmp);
+    build_tree_from(c, c.edge_tree, s.Es, es, c.stream, c.tmp);
     c.face_tree.swept = c.edge_tree.swept = false;
 }
-void build_swept_trees(Context& c, const unsigned long long* abits) {
+void build_swept_trees(Context& c, const unsigned long long* abits, hipStream_t st) {
     Scene& s = c.sc;
     c.face_tree.prim.resize(std::max(s.F, 1));
     c.edge_tree.prim.resize(std::max(s.Es, 1));
     c.swept_point_box.resize(std::max(s.Vs, 1));
     if (s.Vs > 0)
-        hipLaunchKernelGGL(point_boxes, dim3(blocks_for(s.Vs, 256)), dim3(256), 0, c.stream, s.Vs, s.sv.p, s.x.p, s.dir.p, abits, scene_of_vertex(s),
+        hipLaunchKernelGGL(point_boxes, dim3(blocks_for(s.Vs, 256)), dim3(256), 0, st, s.Vs, s.sv.p, s.x.p, s.dir.p, abits, scene_of_vertex(s),
                            c.swept_point_box.p);
     CIPC_CHECK_LAUNCH();
     PrimSrc fs, es;
     fs.kind = 1; fs.faces = s.faces.p; fs.x = s.x.p; fs.p = s.dir.p; fs.abits = abits; fs.vs = scene_of_vertex(s);
     es.kind = 2; es.se = s.se.p; es.x = s.x.p; es.p = s.dir.p; es.abits = abits; es.vs = scene_of_vertex(s);
-    build_tree_from(c, c.face_tree, s.F, fs);
-    build_tree_from(c, c.edge_tree, s.Es, es);
+    build_tree_from(c, c.face_tree, s.F, fs, st, c.tmp_sb);
+    build_tree_from(c, c.edge_tree, s.Es, es, st, c.tmp_sb);
     c.face_tree.swept = c.edge_tree.swept = true;
 }
 

@@ -80,6 +80,9 @@ int cipc_gpu_context_create(int device, cipc_gpu_context** out) {
         CIPC_HIP(hipEventCreateWithFlags(&c->fr_fork, hipEventDisableTiming));
         CIPC_HIP(hipEventCreateWithFlags(&c->fr_join, hipEventDisableTiming));
         CIPC_HIP(hipEventCreateWithFlags(&c->cl_done, hipEventDisableTiming));
+        CIPC_HIP(hipStreamCreateWithFlags(&c->sbs, hipStreamNonBlocking));
+        CIPC_HIP(hipEventCreateWithFlags(&c->as_done, hipEventDisableTiming));
+        CIPC_HIP(hipEventCreateWithFlags(&c->sb_done, hipEventDisableTiming));
         *out = c;
     });
 }
@@ -90,11 +93,15 @@ int cipc_gpu_context_destroy(cipc_gpu_context* c) {
         (void)hipSetDevice(c->device);
         (void)hipStreamSynchronize(c->stream);
         (void)hipStreamSynchronize(c->aux);
+        (void)hipStreamSynchronize(c->sbs);
         (void)hipEventDestroy(c->fork);
         (void)hipEventDestroy(c->join);
         (void)hipEventDestroy(c->fr_fork);
         (void)hipEventDestroy(c->fr_join);
         (void)hipEventDestroy(c->cl_done);
+        (void)hipEventDestroy(c->as_done);
+        (void)hipEventDestroy(c->sb_done);
+        (void)hipStreamDestroy(c->sbs);
         (void)hipStreamDestroy(c->aux);
         (void)hipStreamDestroy(c->stream);
         destroy_learned(*c);
@@ -848,7 +855,7 @@ int cipc_gpu_batch_newton_iteration(cipc_gpu_context* c, const double* kappa, ci
         c->assembled = true;
         CIPC_HIP(hipEventRecord(ev[2], c->stream));
         std::vector<double> sb(5 * (size_t)S), et(9 * (size_t)S);
-        step_bound_launch(*c);
+        step_bound_launch(*c, c->stream);
         CIPC_HIP(hipEventRecord(ev[3], c->stream));
         energy_terms_scenes(*c, et.data());          // synchronises the stream
         CIPC_HIP(hipEventRecord(ev[4], c->stream));
@@ -946,50 +953,46 @@ int cipc_gpu_newton_iteration(cipc_gpu_context* c, double kappa, cipc_iteration_
         require(c && c->have_scene && t, "no scene uploaded or null output");
         require_single(c, "cipc_gpu_newton_iteration");
         bind(*c);
-        hipEvent_t ev[5], es, ee, e0, en;
+        // Three streams.  Main: active set, barrier blocks, CSR.  Second (`aux`): the elastic
+        // stage beside the active set, the friction records and small projections beside the
+        // barrier blocks, then the energy.  Step-bound stream (`sbs`): the step bound, which
+        // needs only the positions, the direction and the active set, beside the assembly.
+        // Every result comes back through pinned memory read after one synchronisation.
+        hipEvent_t ev[4], es, ee, e0, en, s0, s1;
         for (auto& e : ev) CIPC_HIP(hipEventCreate(&e));
-        CIPC_HIP(hipEventCreate(&es));
-        CIPC_HIP(hipEventCreate(&ee));
-        CIPC_HIP(hipEventCreate(&e0));
-        CIPC_HIP(hipEventCreate(&en));
+        for (hipEvent_t* e : {&es, &ee, &e0, &en, &s0, &s1}) CIPC_HIP(hipEventCreate(e));
         CIPC_HIP(hipEventRecord(ev[0], c->stream));
-        elastic_fork(c, es, ee);          // overlaps the active set and the contact blocks
+        elastic_fork(c, es, ee);
         compute_active_set(*c);
         CIPC_HIP(hipEventRecord(ev[1], c->stream));
+        CIPC_HIP(hipEventRecord(c->as_done, c->stream));
+        CIPC_HIP(hipStreamWaitEvent(c->sbs, c->as_done, 0));
+        CIPC_HIP(hipEventRecord(s0, c->sbs));
+        step_bound_launch(*c, c->sbs);
+        CIPC_HIP(hipEventRecord(s1, c->sbs));
         contact_assemble(*c, kappa);
-        elastic_join(c);
-        finalize_csr(*c);
-        CIPC_HIP(hipEventRecord(ev[2], c->stream));
-        // step bound and energy with one synchronisation: both read the state and the active
-        // set only, so the energy runs on the auxiliary stream beside the CCD kernels, and
-        // their results come back through asynchronous pinned reads
-        // (the step bound's launches go first so that the main stream is never idle while the
-        // host enqueues the energy kernels)
-        CIPC_HIP(hipEventRecord(c->fork, c->stream));
-        step_bound_launch(*c);
-        CIPC_HIP(hipEventRecord(ev[3], c->stream));
-        CIPC_HIP(hipStreamWaitEvent(c->aux, c->fork, 0));
         CIPC_HIP(hipEventRecord(e0, c->aux));
         energy_terms_launch(*c, c->aux);
         CIPC_HIP(hipEventRecord(en, c->aux));
+        elastic_join(c);
+        finalize_csr(*c);
+        CIPC_HIP(hipEventRecord(ev[2], c->stream));
+        CIPC_HIP(hipStreamWaitEvent(c->stream, s1, 0));
         CIPC_HIP(hipStreamWaitEvent(c->stream, en, 0));
-        CIPC_HIP(hipEventRecord(ev[4], c->stream));
-        CIPC_HIP(hipEventSynchronize(ev[4]));
+        CIPC_HIP(hipEventRecord(ev[3], c->stream));
+        CIPC_HIP(hipEventSynchronize(ev[3]));
         settle_nnz(*c);
         double sb[5], et[9];
         step_bound_finish(*c, sb);
         energy_terms_finish(*c, et);
-        float ms[4];
-        for (int i = 0; i < 3; ++i) CIPC_HIP(hipEventElapsedTime(&ms[i], ev[i], ev[i + 1]));
-        CIPC_HIP(hipEventElapsedTime(&ms[3], e0, en));  // energy, overlapped with the step bound
-        float tot = 0, el = 0;
-        CIPC_HIP(hipEventElapsedTime(&tot, ev[0], ev[4]));
+        float ms[4], tot = 0, el = 0;
+        for (int i = 0; i < 2; ++i) CIPC_HIP(hipEventElapsedTime(&ms[i], ev[i], ev[i + 1]));
+        CIPC_HIP(hipEventElapsedTime(&ms[2], s0, s1));  // step bound, beside the assembly
+        CIPC_HIP(hipEventElapsedTime(&ms[3], e0, en));  // energy, beside the assembly's tail
+        CIPC_HIP(hipEventElapsedTime(&tot, ev[0], ev[3]));
         CIPC_HIP(hipEventElapsedTime(&el, es, ee));
         for (auto& e : ev) (void)hipEventDestroy(e);
-        (void)hipEventDestroy(es);
-        (void)hipEventDestroy(ee);
-        (void)hipEventDestroy(e0);
-        (void)hipEventDestroy(en);
+        for (hipEvent_t e : {es, ee, e0, en, s0, s1}) (void)hipEventDestroy(e);
         t->broadphase_ms = ms[0]; t->assembly_ms = ms[1]; t->elastic_ms = el; t->ccd_ms = ms[2]; t->energy_ms = ms[3];
         t->total_ms = tot;
         t->alpha = sb[0];

@@ -92,6 +92,12 @@ struct Context {
     hipEvent_t fork = nullptr, join = nullptr;
     // friction records and the small contact projections on `aux` beside the barrier blocks
     hipEvent_t fr_fork = nullptr, fr_join = nullptr, cl_done = nullptr;
+    // In a Newton iteration the step bound runs on `sbs` (default priority) beside the
+    // assembly: it reads the positions, the direction and the active set only.  It refits
+    // the trees (the assembly does not read them) and sorts with its own temporary storage.
+    hipStream_t sbs = nullptr;
+    hipEvent_t as_done = nullptr, sb_done = nullptr;
+    DBuf<unsigned char> tmp_sb;
     Scene sc;
     StaticPattern sp;
     bool have_scene = false;
@@ -210,13 +216,16 @@ void exclusive_scan_int(Context& c, const int* in, int* out, int n);
 // broadphase.hip
 void build_tree(Context& c, LBVH& t, int n);
 void build_static_trees(Context& c);
-void build_swept_trees(Context& c, const unsigned long long* abits);
+// on stream st (the step bound may run beside the assembly); temporary storage of its own
+void build_swept_trees(Context& c, const unsigned long long* abits, hipStream_t st);
 void compute_active_set(Context& c);
 void sort_u64(Context& c, const unsigned long long* in, unsigned long long* out, int n, int bits);
 void sort_u64_pairs(Context& c, const unsigned long long* kin, unsigned long long* kout,
                     const int* vin, int* vout, int n, int bits);
 void sort_i32_pairs(Context& c, const int* kin, int* kout, const int* vin, int* vout, int n, int bits);
 void sort_u32_pairs(Context& c, const unsigned* kin, unsigned* kout, const int* vin, int* vout, int n, int bits);
+void sort_u32_pairs(const unsigned* kin, unsigned* kout, const int* vin, int* vout, int n, int bits, hipStream_t st,
+                    DBuf<unsigned char>& tmp);
 // sums vertex-keyed 3-vector records (keys sorted, record order within a key) into grad
 void add_gradient_records_into(Context& c, int n, const int* k, const int* idx, const double* val, double* grad);
 // initKappa's reductions {sum g_c.g_E, sum |g_c|^2}; returns the ground + self constraint count
@@ -232,7 +241,7 @@ constexpr int kSlots = 8;
 void step_bound(Context& c, double out[5]);
 // step_bound in two halves: the kernels and an asynchronous read of the slots, then (after
 // the caller has synchronised the stream) the merge
-void step_bound_launch(Context& c);
+void step_bound_launch(Context& c, hipStream_t st);
 void step_bound_finish(Context& c, double* out);
 // per scene: out[5 k .. 5 k + 4] as step_bound's five values
 void step_bound_scenes(Context& c, double* out);

@@ -42,11 +42,15 @@ void sort_i32_pairs(Context& c, const int* kin, int* kout, const int* vin, int* 
     CIPC_HIP(hipcub::DeviceRadixSort::SortPairs(c.tmp.p, bytes, kin, kout, vin, vout, n, 0, bits, c.stream));
 }
 void sort_u32_pairs(Context& c, const unsigned* kin, unsigned* kout, const int* vin, int* vout, int n, int bits) {
+    sort_u32_pairs(kin, kout, vin, vout, n, bits, c.stream, c.tmp);
+}
+void sort_u32_pairs(const unsigned* kin, unsigned* kout, const int* vin, int* vout, int n, int bits, hipStream_t st,
+                    DBuf<unsigned char>& tmp) {
     if (n <= 0) return;
     size_t bytes = 0;
-    CIPC_HIP(hipcub::DeviceRadixSort::SortPairs(nullptr, bytes, kin, kout, vin, vout, n, 0, bits, c.stream));
-    c.tmp.reserve(bytes);
-    CIPC_HIP(hipcub::DeviceRadixSort::SortPairs(c.tmp.p, bytes, kin, kout, vin, vout, n, 0, bits, c.stream));
+    CIPC_HIP(hipcub::DeviceRadixSort::SortPairs(nullptr, bytes, kin, kout, vin, vout, n, 0, bits, st));
+    tmp.reserve(bytes);
+    CIPC_HIP(hipcub::DeviceRadixSort::SortPairs(tmp.p, bytes, kin, kout, vin, vout, n, 0, bits, st));
 }
 // out[i] = sum in[0..i-1]; out has n+1 entries when in has n (out[n] = total).
 void exclusive_scan_int(Context& c, const int* in, int* out, int n) {

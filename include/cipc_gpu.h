@@ -79,14 +79,17 @@ typedef struct {
     const double* bend_weight;  /* HingeBendingInfo::geometricWeight (bending 1)        Nb  */
 } cipc_scene_desc;
 
-/* Stage times of one Newton iteration, milliseconds, measured with HIP events on the
- * context's stream (the reference's StageTimer, StageTimer.h:5-27). */
+/* Stage times of one Newton iteration, milliseconds, measured with HIP events (the
+ * reference's StageTimer, StageTimer.h:5-27).  The broad phase and the assembly are timed on
+ * the context's stream; the elastic part of the assembly, the step bound and the energy run
+ * on streams of their own beside it, and are timed there, so the stages overlap and
+ * total_ms is less than their sum. */
 typedef struct {
     double broadphase_ms; /* LBVH build + active set (buildCollisionSets, IPCSolver.cpp:544-556) */
     double assembly_ms;   /* gradient + PSD Hessian -> lower CSR (IPCSolver.cpp:918-930 less the solve) */
     double elastic_ms;    /* inertia + tet part of the assembly (element kernel and its gathers) */
-    double ccd_ms;        /* ground + partial CCD + CFL + swept LBVH + full CCD (IPCSolver.cpp:943-990) */
-    double energy_ms;     /* one line-search energy evaluation (IPCSolver.cpp:322-456), overlapped with the step bound on a second stream */
+    double ccd_ms;        /* ground + partial CCD + CFL + swept LBVH + full CCD (IPCSolver.cpp:943-990), beside the assembly */
+    double energy_ms;     /* one line-search energy evaluation (IPCSolver.cpp:322-456), beside the assembly's tail */
     double total_ms;
     double alpha;         /* merged step bound */
     double energy;        /* energy at the current positions */

@@ -264,13 +264,24 @@ def test_friction_matches_oracle(gpu, O, name):
         o.close()
 
 
-def test_newton_iteration_timing(pair):
+def test_newton_iteration_matches_oracle(pair):
+    """The fused iteration (active set, assembly, step bound on its own stream beside the
+    assembly, energy on a second stream): its system, step bound and energy against the
+    oracle's stage-by-stage results on the same state."""
     g, o, kappa = pair["g"], pair["o"], pair["kappa"]
     g.set_direction(pair["p"])
-    t = g.newton_iteration(kappa)
-    o.active_set()
-    assert t["total_ms"] > 0 and t["nnz"] > 0
-    assert abs(t["energy"] - o.energy(kappa)) <= REL * abs(t["energy"])
+    for _ in range(2):   # the second call refits the trees the first one built
+        t = g.newton_iteration(kappa)
+        assert t["total_ms"] > 0 and t["nnz"] > 0
+        rg = g.system(t["nnz"])
+        o.active_set()
+        ro = o.assemble(kappa)
+        assert np.array_equal(rg["outer"], ro["outer"]) and np.array_equal(rg["inner"], ro["inner"]), pair["name"]
+        assert rel_err(rg["val"], ro["val"]) <= REL, (pair["name"], rel_err(rg["val"], ro["val"]))
+        assert rel_err(rg["grad"], ro["grad"]) <= REL, (pair["name"], rel_err(rg["grad"], ro["grad"]))
+        bo = o.step_bound(pair["p"])
+        assert t["alpha"] <= bo["alpha"] and abs(t["alpha"] - bo["alpha"]) <= 1e-6, (pair["name"], t["alpha"], bo["alpha"])
+        assert abs(t["energy"] - o.energy(kappa)) <= REL * abs(t["energy"])
 
 
 def test_soft_targets_match_oracle(gpu, O):
