@@ -841,35 +841,42 @@ int cipc_gpu_batch_newton_iteration(cipc_gpu_context* c, const double* kappa, ci
         bind(*c);
         const int S = c->sc.S;
         set_scene_kappa(c, kappa, kappa[0]);
-        hipEvent_t ev[5], es, ee;
+        // streams as in cipc_gpu_newton_iteration; the per-scene energies follow the CSR on the
+        // main stream
+        hipEvent_t ev[5], es, ee, s0, s1;
         for (auto& e : ev) CIPC_HIP(hipEventCreate(&e));
-        CIPC_HIP(hipEventCreate(&es));
-        CIPC_HIP(hipEventCreate(&ee));
+        for (hipEvent_t* e : {&es, &ee, &s0, &s1}) CIPC_HIP(hipEventCreate(e));
         CIPC_HIP(hipEventRecord(ev[0], c->stream));
         elastic_fork(c, es, ee);
         compute_active_set(*c);
         CIPC_HIP(hipEventRecord(ev[1], c->stream));
+        CIPC_HIP(hipEventRecord(c->as_done, c->stream));
+        CIPC_HIP(hipStreamWaitEvent(c->sbs, c->as_done, 0));
+        CIPC_HIP(hipEventRecord(s0, c->sbs));
+        step_bound_launch(*c, c->sbs);
+        CIPC_HIP(hipEventRecord(s1, c->sbs));
         contact_assemble(*c, kappa[0]);
         elastic_join(c);
         finalize_csr(*c);
         c->assembled = true;
         CIPC_HIP(hipEventRecord(ev[2], c->stream));
         std::vector<double> sb(5 * (size_t)S), et(9 * (size_t)S);
-        step_bound_launch(*c, c->stream);
+        energy_terms_scenes(*c, et.data());          // synchronises the main stream
         CIPC_HIP(hipEventRecord(ev[3], c->stream));
-        energy_terms_scenes(*c, et.data());          // synchronises the stream
+        CIPC_HIP(hipStreamWaitEvent(c->stream, s1, 0));
         CIPC_HIP(hipEventRecord(ev[4], c->stream));
         CIPC_HIP(hipEventSynchronize(ev[4]));
         settle_nnz(*c);
         step_bound_finish(*c, sb.data());
         float ms[4];
-        for (int i = 0; i < 4; ++i) CIPC_HIP(hipEventElapsedTime(&ms[i], ev[i], ev[i + 1]));
+        for (int i = 0; i < 2; ++i) CIPC_HIP(hipEventElapsedTime(&ms[i], ev[i], ev[i + 1]));
+        CIPC_HIP(hipEventElapsedTime(&ms[2], s0, s1));
+        CIPC_HIP(hipEventElapsedTime(&ms[3], ev[2], ev[3]));
         float tot = 0, el = 0;
         CIPC_HIP(hipEventElapsedTime(&tot, ev[0], ev[4]));
         CIPC_HIP(hipEventElapsedTime(&el, es, ee));
         for (auto& e : ev) (void)hipEventDestroy(e);
-        (void)hipEventDestroy(es);
-        (void)hipEventDestroy(ee);
+        for (hipEvent_t e : {es, ee, s0, s1}) (void)hipEventDestroy(e);
         t->broadphase_ms = ms[0]; t->assembly_ms = ms[1]; t->elastic_ms = el; t->ccd_ms = ms[2]; t->energy_ms = ms[3];
         t->total_ms = tot;
         double amin = 1.0, esum = 0.0;
