@@ -405,12 +405,11 @@ void build_static_trees(Context& c) {
     es.kind = 2; es.se = s.se.p; es.x = s.x.p;
     build_tree_from(c, c.face_tree, s.F, fs, c.stream, c.tmp);
     build_tree_from(c, c.edge_tree, s.Es, es, c.stream, c.tmp);
-    c.face_tree.swept = c.edge_tree.swept = false;
 }
 void build_swept_trees(Context& c, const unsigned long long* abits, hipStream_t st) {
     Scene& s = c.sc;
-    c.face_tree.prim.resize(std::max(s.F, 1));
-    c.edge_tree.prim.resize(std::max(s.Es, 1));
+    c.sface_tree.prim.resize(std::max(s.F, 1));
+    c.sedge_tree.prim.resize(std::max(s.Es, 1));
     c.swept_point_box.resize(std::max(s.Vs, 1));
     if (s.Vs > 0)
         hipLaunchKernelGGL(point_boxes, dim3(blocks_for(s.Vs, 256)), dim3(256), 0, st, s.Vs, s.sv.p, s.x.p, s.dir.p, abits, scene_of_vertex(s),
@@ -419,9 +418,8 @@ void build_swept_trees(Context& c, const unsigned long long* abits, hipStream_t 
     PrimSrc fs, es;
     fs.kind = 1; fs.faces = s.faces.p; fs.x = s.x.p; fs.p = s.dir.p; fs.abits = abits; fs.vs = scene_of_vertex(s);
     es.kind = 2; es.se = s.se.p; es.x = s.x.p; es.p = s.dir.p; es.abits = abits; es.vs = scene_of_vertex(s);
-    build_tree_from(c, c.face_tree, s.F, fs, st, c.tmp_sb);
-    build_tree_from(c, c.edge_tree, s.Es, es, st, c.tmp_sb);
-    c.face_tree.swept = c.edge_tree.swept = true;
+    build_tree_from(c, c.sface_tree, s.F, fs, st, c.tmp_sb);
+    build_tree_from(c, c.sedge_tree, s.Es, es, st, c.tmp_sb);
 }
 
 // ---------------------------------------------------------------------------------

@@ -241,7 +241,7 @@ static void merge_bounds(const Scene& s, const unsigned long long* h, bool any_c
     out[0] = merged; out[1] = aground; out[2] = partial; out[3] = acfl; out[4] = full;
 }
 
-void step_bound_launch(Context& c, hipStream_t st) {
+void step_bound_early(Context& c, hipStream_t st) {
     Scene& s = c.sc;
     const int S = s.S;
     const int* vs = scene_of_vertex(s);
@@ -253,16 +253,13 @@ void step_bound_launch(Context& c, hipStream_t st) {
     if (s.Vs > 0)
         hipLaunchKernelGGL(ground_cfl_kernel, dim3(blocks_for(s.Vs, 256)), dim3(256), 0, st, s.Vs, s.sv.p, s.x.p, s.dir.p, s.gn,
                            s.gd, slack, vs, sl.p);
-    if (c.n_ccd > 0)
-        hipLaunchKernelGGL(partial_ccd_kernel, dim3(blocks_for(c.n_ccd, 64)), dim3(64), 0, st, (int)c.n_ccd, c.ccd_set.p, s.sv.p,
-                           s.faces.p, s.se.p, s.x.p, s.dir.p, eta, vs, sl.p);
     CIPC_CHECK_LAUNCH();
     // full CCD with the swept trees built for each scene's ground-limited step, on the device
     hipLaunchKernelGGL(swept_step, dim3(blocks_for(S, 64)), dim3(64), 0, st, S, s.Vs > 0 ? 1 : 0, sl.p);
     CIPC_CHECK_LAUNCH();
     build_swept_trees(c, sl.p + 5, st);
-    LBVH& ft = c.face_tree;
-    LBVH& et = c.edge_tree;
+    LBVH& ft = c.sface_tree;
+    LBVH& et = c.sedge_tree;
     const TreeView ftv{ft.n, ft.nodes.p, ft.wide.p}, etv{et.n, et.nodes.p, et.wide.p, et.leafpos.p};
     if (s.Vs > 0 && ft.n > 0)
         hipLaunchKernelGGL(full_ccd_pt_kernel, dim3(blocks_for((size_t)s.Vs * kParts, 64)), dim3(64), 0, st, s.Vs, s.sv_perm.p,
@@ -273,8 +270,23 @@ void step_bound_launch(Context& c, hipStream_t st) {
                            s.se.p, s.bt.p, s.x.p, s.dir.p, etv, et.prim.p, sl.p, eta, vs);
     }
     CIPC_CHECK_LAUNCH();
+}
+void step_bound_partial(Context& c, hipStream_t st) {
+    Scene& s = c.sc;
+    const int S = s.S;
+    const double eta = 1.0 - 0.8;
+    DBuf<unsigned long long>& sl = c.u64scratch;
+    // (its cap is the ground step, slot 0, final once step_bound_early has run)
+    if (c.n_ccd > 0)
+        hipLaunchKernelGGL(partial_ccd_kernel, dim3(blocks_for(c.n_ccd, 64)), dim3(64), 0, st, (int)c.n_ccd, c.ccd_set.p, s.sv.p,
+                           s.faces.p, s.se.p, s.x.p, s.dir.p, eta, scene_of_vertex(s), sl.p);
+    CIPC_CHECK_LAUNCH();
     c.pin_slots.reserve((size_t)kSlots * S);
     CIPC_HIP(hipMemcpyAsync(c.pin_slots.p, sl.p, (size_t)kSlots * S * sizeof(unsigned long long), hipMemcpyDeviceToHost, st));
+}
+void step_bound_launch(Context& c, hipStream_t st) {
+    step_bound_early(c, st);
+    step_bound_partial(c, st);
 }
 void step_bound_finish(Context& c, double* out) {
     const Scene& s = c.sc;

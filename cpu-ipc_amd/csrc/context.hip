@@ -240,7 +240,7 @@ void upload_scene_impl(cipc_gpu_context* c, const cipc_scene_desc* d, bool plan 
         c->n_self = c->n_ee = c->n_ccd = c->n_env = 0;
         c->self_set.resize(1); c->ee_set.resize(1); c->ee_ie.resize(1); c->ccd_set.resize(1); c->env_set.resize(1);
         c->have_scene = true;
-        c->face_tree.age = c->edge_tree.age = kMaxRefits;
+        c->face_tree.age = c->edge_tree.age = c->sface_tree.age = c->sedge_tree.age = kMaxRefits;
         c->n_self = c->n_ee = c->n_ccd = c->n_env = 0;   // sets and lagged state of an earlier scene
         c->n_fself = c->n_fenv = 0;
         c->n_soft = 0; c->soft_w = 0.0;
@@ -375,7 +375,8 @@ int cipc_gpu_set_state(cipc_gpu_context* c, const double* x, const double* vprev
         const size_t n = 3 * (size_t)c->sc.V * sizeof(double);
         if (x) {
             CIPC_HIP(hipMemcpyAsync(c->sc.x.p, x, n, hipMemcpyHostToDevice, c->stream));
-            c->face_tree.age = c->edge_tree.age = kMaxRefits;   // new positions: fresh topology on the next build
+            // new positions: fresh topologies on the next builds
+            c->face_tree.age = c->edge_tree.age = c->sface_tree.age = c->sedge_tree.age = kMaxRefits;
         }
         if (vprev) CIPC_HIP(hipMemcpyAsync(c->sc.vprev.p, vprev, n, hipMemcpyHostToDevice, c->stream));
         if (xt) CIPC_HIP(hipMemcpyAsync(c->sc.xt.p, xt, n, hipMemcpyHostToDevice, c->stream));
@@ -848,12 +849,14 @@ int cipc_gpu_batch_newton_iteration(cipc_gpu_context* c, const double* kappa, ci
         for (hipEvent_t* e : {&es, &ee, &s0, &s1}) CIPC_HIP(hipEventCreate(e));
         CIPC_HIP(hipEventRecord(ev[0], c->stream));
         elastic_fork(c, es, ee);
+        CIPC_HIP(hipStreamWaitEvent(c->sbs, ev[0], 0));
+        CIPC_HIP(hipEventRecord(s0, c->sbs));
+        step_bound_early(*c, c->sbs);
         compute_active_set(*c);
         CIPC_HIP(hipEventRecord(ev[1], c->stream));
         CIPC_HIP(hipEventRecord(c->as_done, c->stream));
         CIPC_HIP(hipStreamWaitEvent(c->sbs, c->as_done, 0));
-        CIPC_HIP(hipEventRecord(s0, c->sbs));
-        step_bound_launch(*c, c->sbs);
+        step_bound_partial(*c, c->sbs);
         CIPC_HIP(hipEventRecord(s1, c->sbs));
         contact_assemble(*c, kappa[0]);
         elastic_join(c);
@@ -962,20 +965,24 @@ int cipc_gpu_newton_iteration(cipc_gpu_context* c, double kappa, cipc_iteration_
         bind(*c);
         // Three streams.  Main: active set, barrier blocks, CSR.  Second (`aux`): the elastic
         // stage beside the active set, the friction records and small projections beside the
-        // barrier blocks, then the energy.  Step-bound stream (`sbs`): the step bound, which
-        // needs only the positions, the direction and the active set, beside the assembly.
-        // Every result comes back through pinned memory read after one synchronisation.
+        // barrier blocks, then the energy.  Step-bound stream (`sbs`): the ground bound, swept
+        // trees and full CCD (positions and direction only) from the start, the partial CCD
+        // once the active set exists.  Every result comes back through pinned memory read
+        // after one synchronisation.
         hipEvent_t ev[4], es, ee, e0, en, s0, s1;
         for (auto& e : ev) CIPC_HIP(hipEventCreate(&e));
         for (hipEvent_t* e : {&es, &ee, &e0, &en, &s0, &s1}) CIPC_HIP(hipEventCreate(e));
         CIPC_HIP(hipEventRecord(ev[0], c->stream));
         elastic_fork(c, es, ee);
+        // the ground bound, the swept trees and the full CCD need only x and p: from the start
+        CIPC_HIP(hipStreamWaitEvent(c->sbs, ev[0], 0));
+        CIPC_HIP(hipEventRecord(s0, c->sbs));
+        step_bound_early(*c, c->sbs);
         compute_active_set(*c);
         CIPC_HIP(hipEventRecord(ev[1], c->stream));
         CIPC_HIP(hipEventRecord(c->as_done, c->stream));
         CIPC_HIP(hipStreamWaitEvent(c->sbs, c->as_done, 0));
-        CIPC_HIP(hipEventRecord(s0, c->sbs));
-        step_bound_launch(*c, c->sbs);
+        step_bound_partial(*c, c->sbs);
         CIPC_HIP(hipEventRecord(s1, c->sbs));
         contact_assemble(*c, kappa);
         CIPC_HIP(hipEventRecord(e0, c->aux));

@@ -176,7 +176,9 @@ struct Context {
     int collisions = 0;
 
     // broad phase
-    LBVH face_tree, edge_tree;
+    LBVH face_tree, edge_tree;     // static boxes: the active set, isIntersected
+    LBVH sface_tree, sedge_tree;   // swept boxes: the full CCD (their own, so that the step
+                                   // bound can run beside the active set)
     ActiveSetScratch as;
     DBuf<Box> swept_point_box;
 
@@ -242,6 +244,11 @@ void step_bound(Context& c, double out[5]);
 // step_bound in two halves: the kernels and an asynchronous read of the slots, then (after
 // the caller has synchronised the stream) the merge
 void step_bound_launch(Context& c, hipStream_t st);
+// step_bound_launch in two parts: what needs only the positions and the direction (ground
+// bound, CFL, swept trees, full CCD), then the partial CCD over the active set's CCD pairs and
+// the read-back of the bounds
+void step_bound_early(Context& c, hipStream_t st);
+void step_bound_partial(Context& c, hipStream_t st);
 void step_bound_finish(Context& c, double* out);
 // per scene: out[5 k .. 5 k + 4] as step_bound's five values
 void step_bound_scenes(Context& c, double* out);
