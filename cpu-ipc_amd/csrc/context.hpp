@@ -1,4 +1,4 @@
-// Device context: one scene, one stream, all device memory.  Each translation unit
+// Device context: one scene, its streams (main, auxiliary, step bound), all device memory.  Each translation unit
 // implements one stage of the Newton iteration on it.
 #pragma once
 #include "common.hpp"

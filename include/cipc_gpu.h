@@ -22,8 +22,9 @@
  * code; cipc_gpu_last_error() returns the message of the most recent failure in the
  * calling thread.  Error codes map to the exception types the reference throws at the
  * same places (std::invalid_argument, std::runtime_error, std::logic_error,
- * std::overflow_error).  One context owns one device, one stream and all device
- * memory of one scene; calls on one context must not overlap.
+ * std::overflow_error).  One context owns one device, its streams (all at the default
+ * priority) and all device memory of one scene; every call returns with its work done;
+ * calls on one context must not overlap.
  */
 #ifndef CIPC_GPU_H
 #define CIPC_GPU_H
