@@ -184,10 +184,13 @@ __global__ __launch_bounds__(64) void full_ccd_ee_kernel(int Es, const int* __re
         scene = scene_of(vs, Q.x);
         const double a = bits_to_double(sl[kSlots * scene + 4]);
         const Box bq = box_union(swept_pt(ld3(x, Q.x), ld3(p, Q.x), a), swept_pt(ld3(x, Q.y), ld3(p, Q.y), a));
-        // query box: the edge's leaf box of the tree (swept with the incoming step)
-        const Box q = ebox[eQ];
-        part_walk_after(tv, part, q, 0.0, tv.leafpos[eQ], stk + threadIdx.x, [&](int eH) {
-            if (!overlaps(q, ebox[eH], 0.0)) return;   // exact leaf test (the walk's boxes are float)
+        // Query box: bq, swept with the point-triangle-reduced step, which is at most the
+        // tree's step.  The swept box of a smaller step lies inside that of a larger one (the
+        // rounded end point x - a p moves monotonically with a), so bq lies inside the edge's
+        // leaf box and every H's filter box bh inside its leaf box: a pair that passes the
+        // final bq/bh test passes the leaf-box test too, and walking with bq finds it.
+        part_walk_after(tv, part, bq, 0.0, tv.leafpos[eQ], stk + threadIdx.x, [&](int eH) {
+            if (!overlaps(bq, ebox[eH], 0.0)) return;  // exact leaf test (the walk's boxes are float)
             const int eI = eH > eQ ? eH : eQ, eJ = eH > eQ ? eQ : eH;
             const int2 I = se[eI], J = se[eJ];
             if (is_collider(bt[I.x]) && is_collider(bt[I.y]) && is_collider(bt[J.x]) && is_collider(bt[J.y])) return;
