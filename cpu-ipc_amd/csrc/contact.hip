@@ -54,40 +54,63 @@ __device__ inline void dist_gh(int type, const double* q, double* g, double* H) 
     }
 }
 
-// Kernel 1: local Hessian (upper triangle, SoA) and local gradient of every contact.
-//   c < n_self: Self_ActiveSet entry, compute_H_dpt / compute_g_dpt
-//   otherwise : Self_EE_ActiveSet entry, compute_H_dee / compute_g_dee
-__global__ __launch_bounds__(64) void contact_local_kernel(
-    int n_self, int n_ee, const int4* __restrict__ self_set, const int4* __restrict__ ee_set,
+// Kernel 1: local Hessian (upper triangle, SoA) and local gradient of every contact, in
+// two launches over the contact range [0, n_self + n_ee):
+//   contact_self_kernel, c < n_self: Self_ActiveSet entry, compute_H_dpt / compute_g_dpt
+//   contact_ee_kernel, otherwise   : Self_EE_ActiveSet entry, compute_H_dee / compute_g_dee
+// The self form is instantiated per distance type, so that its arrays have indices fixed at
+// compile time and stay in registers.
+template <int Type>
+__device__ inline void self_local(int c, int n, int4 e, const int (&v)[4], const double* __restrict__ x, double hhat, double kappa,
+                                  double* __restrict__ Hs, double* __restrict__ Gs) {
+    constexpr int nv = Type == 0 ? 2 : (Type == 1 ? 3 : 4), N = 3 * nv;
+    double H[144], g[12];       // indices fixed at compile time: only the first N (N x N) live
+    d3 p[4];
+    double q[12];
+#pragma unroll
+    for (int k = 0; k < nv; ++k) { p[k] = ld3(x, v[k]); q[3 * k] = p[k].x; q[3 * k + 1] = p[k].y; q[3 * k + 2] = p[k].z; }
+    const double d = self_value(Type, p);
+    dist_gh(Type, q, g, H);
+    const double gb = barrier_g(d, hhat), hb = barrier_h(d, hhat);
+    const double coef = (Type <= 1) ? kappa * -e.w : kappa;
+    const double cH = coef * hb, cg = coef * gb;
+    int idx = 0;
+#pragma unroll
+    for (int i = 0; i < N; ++i)
+#pragma unroll
+        for (int j = i; j < N; ++j) Hs[(size_t)(idx++) * n + c] = (cH * g[i]) * g[j] + cg * H[i * N + j];
+    const double gs = (Type <= 1) ? (kappa * -e.w) * gb : kappa * gb;
+#pragma unroll
+    for (int i = 0; i < N; ++i) Gs[(size_t)i * n + c] = gs * g[i];
+}
+__global__ __launch_bounds__(64) void contact_self_kernel(int n_self, int n, const int4* __restrict__ self_set,
+                                                          const double* __restrict__ x, double hhat, Kap kap, double* __restrict__ Hs,
+                                                          double* __restrict__ Gs, int4* __restrict__ verts, int* __restrict__ dims) {
+    const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= n_self) return;
+    const int4 e = self_set[c];
+    int v[4];
+    const int type = decode(e, v);
+    const double kappa = kap.at(v[0]);
+    switch (type) {
+    case 0: self_local<0>(c, n, e, v, x, hhat, kappa, Hs, Gs); break;
+    case 1: self_local<1>(c, n, e, v, x, hhat, kappa, Hs, Gs); break;
+    case 2: self_local<2>(c, n, e, v, x, hhat, kappa, Hs, Gs); break;
+    default: self_local<3>(c, n, e, v, x, hhat, kappa, Hs, Gs); break;
+    }
+    verts[c] = make_int4(v[0], v[1], type >= 1 ? v[2] : -1, type >= 2 ? v[3] : -1);
+    dims[c] = 3 * nv_of(type);
+}
+__global__ __launch_bounds__(64) void contact_ee_kernel(
+    int n_self, int n_ee, const int4* __restrict__ ee_set,
     const int2* __restrict__ ee_ie, const int2* __restrict__ se, const double* __restrict__ x,
     const double* __restrict__ vrest, double hhat, Kap kap, double* __restrict__ Hs, double* __restrict__ Gs,
     int4* __restrict__ verts, int* __restrict__ dims) {
-    const int c = blockIdx.x * blockDim.x + threadIdx.x;
     const int n = n_self + n_ee;
+    const int c = n_self + (int)(blockIdx.x * blockDim.x + threadIdx.x);
     if (c >= n) return;
-    double H[144], g[12];
-    if (c < n_self) {
-        const int4 e = self_set[c];
-        int v[4];
-        const int type = decode(e, v);
-        const double kappa = kap.at(v[0]);
-        const int nv = nv_of(type), N = 3 * nv;
-        d3 p[4];
-        double q[12];
-        for (int k = 0; k < nv; ++k) { p[k] = ld3(x, v[k]); q[3 * k] = p[k].x; q[3 * k + 1] = p[k].y; q[3 * k + 2] = p[k].z; }
-        const double d = self_value(type, p);
-        dist_gh(type, q, g, H);
-        const double gb = barrier_g(d, hhat), hb = barrier_h(d, hhat);
-        const double coef = (type <= 1) ? kappa * -e.w : kappa;
-        const double cH = coef * hb, cg = coef * gb;
-        int idx = 0;
-        for (int i = 0; i < N; ++i)
-            for (int j = i; j < N; ++j) Hs[(size_t)(idx++) * n + c] = (cH * g[i]) * g[j] + cg * H[i * N + j];
-        const double gs = (type <= 1) ? (kappa * -e.w) * gb : kappa * gb;
-        for (int i = 0; i < N; ++i) Gs[(size_t)i * n + c] = gs * g[i];
-        verts[c] = make_int4(v[0], v[1], type >= 1 ? v[2] : -1, type >= 2 ? v[3] : -1);
-        dims[c] = N;
-    } else {
+    double H[144];
+    {
         const int i = c - n_self;
         const int4 e = ee_set[i];
         int v[4];
@@ -342,14 +365,18 @@ struct FrictionPair {
         if (type == 1) return dx[0] - (dx[1] + b1 * (dx[2] - dx[1]));                               // :174-181
         return dx[0] - (dx[1] + b1 * (dx[2] - dx[1]) + b2 * (dx[3] - dx[1]));                        // :43-51
     }
-    __device__ void lift(double t0, double t1, double* L) const {
+    // L holds all four vertices' rows, those past nv zero (a fixed-size array the compiler
+    // keeps in registers).
+    __device__ void lift(double t0, double t1, double (&L)[12]) const {
         const d3 T = t0 * B0 + t1 * B1;
-        d3 s[4];
+        const d3 z{0.0, 0.0, 0.0};
+        d3 s[4] = {z, z, z, z};
         if (type == 3) { s[0] = (1.0 - b1) * T; s[1] = b1 * T; s[2] = (b2 - 1.0) * T; s[3] = -b2 * T; }        // :128-139
         else if (type == 0) { s[0] = T; s[1] = -1.0 * T; }                                                   // :237-244
         else if (type == 1) { s[0] = T; s[1] = (b1 - 1.0) * T; s[2] = -b1 * T; }                            // :183-192
         else { s[0] = T; s[1] = (-1 + b1 + b2) * T; s[2] = -b1 * T; s[3] = -b2 * T; }                      // :53-63
-        for (int k = 0; k < nv; ++k) { L[3 * k] = s[k].x; L[3 * k + 1] = s[k].y; L[3 * k + 2] = s[k].z; }
+#pragma unroll
+        for (int k = 0; k < 4; ++k) { L[3 * k] = s[k].x; L[3 * k + 1] = s[k].y; L[3 * k + 2] = s[k].z; }
     }
 };
 
@@ -376,38 +403,48 @@ __global__ __launch_bounds__(64) void friction_local_kernel(int n, const int4* _
     double L[12];
     fp.lift(t0, t1, L);
     const double gs = coef * lam[c];
-    for (int i = 0; i < N; ++i) Gs[(size_t)i * n + c] = L[i] * gs;
-    // Hessian: at most two nonnegative eigenpairs in the tangent plane
-    double H[78];
-    for (int k = 0; k < N * (N + 1) / 2; ++k) H[k] = 0.0;
+#pragma unroll
+    for (int i = 0; i < 12; ++i)
+        if (i < N) Gs[(size_t)i * n + c] = L[i] * gs;
+    // Hessian: at most two nonnegative eigenpairs in the tangent plane, each w e l l^T with l
+    // the lift of its tangent direction; the packed upper triangle is written term by term, the
+    // first pair's term then the second's, with every array index fixed at compile time
     const double w = fmax(0.0, coef * lam[c]);
-    auto add = [&](double d0, double d1, double ev) {
+    double la[12], lb[12], sa = 0.0, sb = 0.0;
+    bool ha = false, hb = false;
+    auto pair = [&](double d0, double d1, double ev, double (&l)[12], double& sc, bool& has) {
         const double e = fmax(0.0, ev);
-        if (e == 0.0) return;
-        double l[12];
+        has = e != 0.0;
+        sc = w * e;
         fp.lift(d0, d1, l);
-        const double sc = w * e;
-        int k = 0;
-        for (int i = 0; i < N; ++i)
-            for (int j = i; j < N; ++j, ++k) H[k] += (sc * l[i]) * l[j];
     };
     if (w != 0.0) {
         const double nr = sqrt(sq);
         if (nr > 0.0) {
             const double p0 = u0 / nr, p1 = u1 / nr;
-            if (nr > eps) add(-p1, p0, 1.0 / nr);
+            if (nr > eps) pair(-p1, p0, 1.0 / nr, la, sa, ha);
             else {
                 const double pe = f1_SF_div(sq, eps), pa = f2_SF(sq, eps);
-                add(p0, p1, pa);
-                add(-p1, p0, pe);
+                pair(p0, p1, pa, la, sa, ha);
+                pair(-p1, p0, pe, lb, sb, hb);
             }
         } else {
             const double iso = f1_SF_div(0.0, eps);
-            add(1.0, 0.0, iso);
-            add(0.0, 1.0, iso);
+            pair(1.0, 0.0, iso, la, sa, ha);
+            pair(0.0, 1.0, iso, lb, sb, hb);
         }
     }
-    for (int k = 0; k < N * (N + 1) / 2; ++k) Hs[(size_t)k * n + c] = H[k];
+#pragma unroll
+    for (int i = 0; i < 12; ++i)
+#pragma unroll
+        for (int j = i; j < 12; ++j)
+            if (j < N) {
+                double h = 0.0;
+                if (ha) h += (sa * la[i]) * la[j];
+                if (hb) h += (sb * lb[i]) * lb[j];
+                const int k = i * N - i * (i - 1) / 2 + (j - i);     // packed (i, j), rows i < N
+                Hs[(size_t)k * n + c] = h;
+            }
     verts[c] = make_int4(fp.v[0], fp.v[1], fp.nv >= 3 ? fp.v[2] : -1, fp.nv >= 4 ? fp.v[3] : -1);
     dims[c] = N;
 }
@@ -591,9 +628,13 @@ void contact_assemble(Context& c, double kappa) {
     if (n > 0) {
         double* Hs = c.chess.p;
         double* Gs = c.chess.p + (size_t)kHU * n;
-        hipLaunchKernelGGL(contact_local_kernel, dim3(blocks_for(n, 64)), dim3(64), 0, c.stream, (int)c.n_self, (int)c.n_ee,
-                           c.self_set.p, c.ee_set.p, c.ee_ie.p, s.se.p, s.x.p, s.vrest.p, s.hhat, scene_kappa(s, kappa), Hs, Gs,
-                           c.cverts.p, c.cdims.p);
+        if (c.n_self > 0)
+            hipLaunchKernelGGL(contact_self_kernel, dim3(blocks_for(c.n_self, 64)), dim3(64), 0, c.stream, (int)c.n_self, (int)n,
+                               c.self_set.p, s.x.p, s.hhat, scene_kappa(s, kappa), Hs, Gs, c.cverts.p, c.cdims.p);
+        if (c.n_ee > 0)
+            hipLaunchKernelGGL(contact_ee_kernel, dim3(blocks_for(c.n_ee, 64)), dim3(64), 0, c.stream, (int)c.n_self, (int)c.n_ee,
+                               c.ee_set.p, c.ee_ie.p, s.se.p, s.x.p, s.vrest.p, s.hhat, scene_kappa(s, kappa), Hs, Gs, c.cverts.p,
+                               c.cdims.p);
         CIPC_CHECK_LAUNCH();
     }
     // Main stream: the projections of 4-vertex contacts.  Second stream (enqueued after them,
