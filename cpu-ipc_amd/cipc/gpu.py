@@ -37,7 +37,7 @@ EXPORTS = (
     "cipc_gpu_get_csr", "cipc_gpu_energy", "cipc_gpu_energy_terms", "cipc_gpu_step_bound",
     "cipc_gpu_friction_update", "cipc_gpu_get_friction", "cipc_gpu_set_soft_targets", "cipc_gpu_kappa_gradients",
     "cipc_gpu_close_constraints", "cipc_gpu_clear_close_constraints", "cipc_gpu_preconditioner_param_count",
-    "cipc_gpu_set_preconditioner_weights", "cipc_gpu_preconditioner_gradient", "cipc_gpu_preconditioner_train_step", "cipc_gpu_solve", "cipc_gpu_solve_ex", "cipc_gpu_save_positions", "cipc_gpu_step_forward",
+    "cipc_gpu_set_preconditioner_weights", "cipc_gpu_preconditioner_gradient", "cipc_gpu_preconditioner_train_step", "cipc_gpu_solve", "cipc_gpu_solve_ex", "cipc_gpu_get_solve_factor", "cipc_gpu_save_positions", "cipc_gpu_step_forward",
     "cipc_gpu_is_intersected", "cipc_gpu_newton_iteration",
     "cipc_gpu_upload_batch", "cipc_gpu_batch_size", "cipc_gpu_batch_assemble", "cipc_gpu_batch_friction_update",
     "cipc_gpu_batch_energy", "cipc_gpu_batch_step_bound", "cipc_gpu_batch_step_forward", "cipc_gpu_batch_is_intersected",
@@ -77,7 +77,9 @@ class IterationTiming(C.Structure):
     _fields_ = [("broadphase_ms", C.c_double), ("assembly_ms", C.c_double), ("elastic_ms", C.c_double),
                 ("ccd_ms", C.c_double),
                 ("energy_ms", C.c_double), ("total_ms", C.c_double), ("alpha", C.c_double),
-                ("energy", C.c_double), ("nnz", C.c_int64), ("contacts", C.c_int64)]
+                ("energy", C.c_double), ("nnz", C.c_int64), ("contacts", C.c_int64), ("collisions", C.c_int64),
+                ("bound_ground", C.c_double), ("bound_partial", C.c_double), ("bound_cfl", C.c_double),
+                ("bound_full", C.c_double)]
 
 
 class SolveStats(C.Structure):
@@ -130,6 +132,7 @@ def load(path: str = LIB_PATH):
         "cipc_gpu_clear_close_constraints": [C.c_void_p],
         "cipc_gpu_solve": [C.c_void_p, C.c_int, C.c_double, C.c_int, _dp, C.POINTER(SolveStats)],
         "cipc_gpu_solve_ex": [C.c_void_p, C.c_int, C.c_int, C.c_double, C.c_int, _dp, C.POINTER(SolveStats)],
+        "cipc_gpu_get_solve_factor": [C.c_void_p, _i64p, _ip, _ip, _dp],
         "cipc_gpu_get_friction": [C.c_void_p, C.POINTER(C.c_int64), _dp, _dp, _dp, _dp],
         "cipc_gpu_step_bound": [C.c_void_p, _dp],
         "cipc_gpu_save_positions": [C.c_void_p],
@@ -391,6 +394,16 @@ class GpuContext:
         _check(load().cipc_gpu_solve_ex(self.h, precision, preconditioner, rel_tol, max_iter, _dptr(p), C.byref(st)))
         return p, dict(iterations=st.iterations, refinements=st.refinements, rel_residual=st.rel_residual,
                        solve_ms=st.solve_ms)
+
+    def solve_factor(self):
+        """The factor G of the last solve with preconditioner 1 or 2 as a scipy CSR matrix (3V x 3V)."""
+        import scipy.sparse as sp
+        nb = np.zeros(1, np.int64)
+        _check(load().cipc_gpu_get_solve_factor(self.h, nb.ctypes.data_as(_i64p), None, None, None))
+        rowptr, col, val = np.zeros(self.V + 1, np.int32), np.zeros(max(nb[0], 1), np.int32), np.zeros((max(nb[0], 1), 3, 3))
+        _check(load().cipc_gpu_get_solve_factor(self.h, nb.ctypes.data_as(_i64p), _iptr(rowptr), _iptr(col), _dptr(val)))
+        n = int(nb[0])
+        return sp.bsr_matrix((val[:n], col[:n], rowptr), shape=(3 * self.V, 3 * self.V)).tocsr()
 
     def friction_update(self, kappa):
         """Friction::initialize on the current active set and positions."""

@@ -495,7 +495,7 @@ def four_cubes20(seed: int = 1, velocity_scale: float = 0.0, rotate: str = "yaw"
 
 
 def batched_cubes(n_scenes: int = 256, seed: int = 0, cells_lo: int = 8, cells_hi: int = 24, edge: float = 1.0,
-                  spacing: float = 3.0, jitter: float = 0.01) -> list:
+                  spacing: float = 3.0, jitter: float = 0.01, landed: bool = False) -> list:
     """Independent single-cube scenes for a batch: each a lattice cube of edge ``edge`` with
     a cell count per axis log-uniform in [cells_lo, cells_hi] (8^3..24^3 cells: 2,560 to
     69,120 tets), seeded Young's modulus log-uniform in [1e4, 1e6] Pa, Poisson ratio
@@ -503,8 +503,13 @@ def batched_cubes(n_scenes: int = 256, seed: int = 0, cells_lo: int = 8, cells_h
     ground; mu = 0.2, +-1% vertex jitter.  dHat is relative to the nominal cube diagonal
     (sqrt(3) edge) in every scene, so the scenes share hhat, fhat and dtol as a batch
     requires.  Scene k sits at a grid position in the ground plane, ``spacing`` apart
-    (the scenes never interact in any case)."""
+    (the scenes never interact in any case).
+
+    ``landed``: each cube instead at the frame where its drop reaches the ground: the lowest
+    point a seeded 0.2-0.9 dHat above it (so part of the bottom face is in contact), moving
+    down at the speed the drop gives, sqrt(2 g h).  The other draws are unchanged."""
     rng = np.random.default_rng(seed)
+    land = np.random.default_rng(seed + 7919)
     cache = {}
     cols = int(math.ceil(math.sqrt(n_scenes)))
     out = []
@@ -521,8 +526,14 @@ def batched_cubes(n_scenes: int = 256, seed: int = 0, cells_lo: int = 8, cells_h
         mat = Material(youngs=E, poisson=nu, friction=0.2)
         x = base_x + rng.uniform(-jitter, jitter, base_x.shape) * (edge / n)
         lo = x.min(axis=0)
-        x = x + np.array([(k % cols) * spacing - lo[0], GROUND_Y + drop - lo[1], (k // cols) * spacing - lo[2]])
-        sc = finalize(f"batch{k}_{n}", x, base_t, mat)
+        if landed:
+            height = land.uniform(0.2, 0.9) * mat.rel_dhat * math.sqrt(diag2)
+            x = x + np.array([(k % cols) * spacing - lo[0], GROUND_Y + height - lo[1], (k // cols) * spacing - lo[2]])
+            vel = np.tile([0.0, -math.sqrt(2 * 9.8 * drop), 0.0], (x.shape[0], 1))
+            sc = finalize(f"landed{k}_{n}", x, base_t, mat, velocities=vel)
+        else:
+            x = x + np.array([(k % cols) * spacing - lo[0], GROUND_Y + drop - lo[1], (k // cols) * spacing - lo[2]])
+            sc = finalize(f"batch{k}_{n}", x, base_t, mat)
         sc.hhat = mat.rel_dhat ** 2 * diag2
         sc.fhat = mat.fhat_rel * diag2
         sc.dtol = mat.dtol_rel * diag2

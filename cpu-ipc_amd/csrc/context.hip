@@ -501,19 +501,8 @@ static void assemble_impl(cipc_gpu_context* c, double kappa, const double* ks = 
     elastic_join(c);
     finalize_csr(*c);
     c->assembled = true;
-    // collision count as compute_g_dpt returns it: multiplicity of point-point/point-edge
-    // entries, one per point-triangle/edge-edge entry (ContactMechanics.cpp:2970-3032)
-    c->collisions = 0;
-    if (c->n_self) {
-        std::vector<int4> h(c->n_self);
-        CIPC_HIP(hipMemcpyAsync(h.data(), c->self_set.p, 16 * c->n_self, hipMemcpyDeviceToHost, c->stream));
-        CIPC_HIP(hipStreamSynchronize(c->stream));
-        long long n = 0;
-        for (const int4& e : h) n += (e.x < 0 && (e.z < 0 || e.w < 0)) ? -e.w : 1;
-        c->collisions = (int)n;
-    }
     CIPC_HIP(hipStreamSynchronize(c->stream));
-    settle_nnz(*c);
+    settle_nnz(*c);   // the nonzero and collision counts
 }
 
 int cipc_gpu_set_hessian_precision(cipc_gpu_context* c, int precision) {
@@ -708,6 +697,23 @@ int cipc_gpu_solve_ex(cipc_gpu_context* c, int precision, int preconditioner, do
     });
 }
 
+int cipc_gpu_get_solve_factor(cipc_gpu_context* c, int64_t* nblocks, int* rowptr, int* col, double* values) {
+    return guard([&] {
+        require(c && c->have_scene && nblocks, "no scene uploaded or null output");
+        const SolveBuffers& w = c->solve;
+        if (w.factor == 0) throw Error(CIPC_ERR_LOGIC, "no factor: call cipc_gpu_solve_ex with preconditioner 1 or 2 first");
+        bind(*c);
+        const int V = c->sc.V;
+        int nb = 0;
+        read_back(*c, &nb, w.rowptr.p + V, 1);
+        *nblocks = nb;
+        if (rowptr) CIPC_HIP(hipMemcpyAsync(rowptr, w.rowptr.p, sizeof(int) * ((size_t)V + 1), hipMemcpyDeviceToHost, c->stream));
+        if (col && nb) CIPC_HIP(hipMemcpyAsync(col, w.col.p, sizeof(int) * (size_t)nb, hipMemcpyDeviceToHost, c->stream));
+        if (values && nb) CIPC_HIP(hipMemcpyAsync(values, w.G.p, sizeof(double) * 9 * (size_t)nb, hipMemcpyDeviceToHost, c->stream));
+        CIPC_HIP(hipStreamSynchronize(c->stream));
+    });
+}
+
 // ---- batched scenes -------------------------------------------------------------------
 int cipc_gpu_batch_assemble(cipc_gpu_context* c, const double* kappa, int64_t* nnz, int* collisions) {
     return guard([&] {
@@ -882,18 +888,21 @@ int cipc_gpu_batch_newton_iteration(cipc_gpu_context* c, const double* kappa, ci
         for (hipEvent_t e : {es, ee, s0, s1}) (void)hipEventDestroy(e);
         t->broadphase_ms = ms[0]; t->assembly_ms = ms[1]; t->elastic_ms = el; t->ccd_ms = ms[2]; t->energy_ms = ms[3];
         t->total_ms = tot;
-        double amin = 1.0, esum = 0.0;
+        double amin = 1.0, esum = 0.0, bmin[4] = {1.0, 1.0, 1.0, 1.0};
         for (int k = 0; k < S; ++k) {
             const double a = sb[5 * (size_t)k], e = total_energy(*c, kappa[k], et.data() + 9 * (size_t)k);
             if (alpha) alpha[k] = a;
             if (energy) energy[k] = e;
             amin = std::min(amin, a);
             esum += e;
+            for (int j = 0; j < 4; ++j) bmin[j] = std::min(bmin[j], sb[5 * (size_t)k + 1 + j]);
         }
         t->alpha = amin;
         t->energy = esum;
         t->nnz = c->nnz;
         t->contacts = c->n_self + c->n_ee + c->n_env;
+        t->collisions = c->collisions;
+        t->bound_ground = bmin[0]; t->bound_partial = bmin[1]; t->bound_cfl = bmin[2]; t->bound_full = bmin[3];
     });
 }
 
@@ -996,6 +1005,7 @@ int cipc_gpu_newton_iteration(cipc_gpu_context* c, double kappa, cipc_iteration_
         CIPC_HIP(hipEventRecord(ev[3], c->stream));
         CIPC_HIP(hipEventSynchronize(ev[3]));
         settle_nnz(*c);
+        c->assembled = true;
         double sb[5], et[9];
         step_bound_finish(*c, sb);
         energy_terms_finish(*c, et);
@@ -1013,6 +1023,8 @@ int cipc_gpu_newton_iteration(cipc_gpu_context* c, double kappa, cipc_iteration_
         t->energy = total_energy(*c, kappa, et);
         t->nnz = c->nnz;
         t->contacts = c->n_self + c->n_ee + c->n_env;
+        t->collisions = c->collisions;
+        t->bound_ground = sb[1]; t->bound_partial = sb[2]; t->bound_cfl = sb[3]; t->bound_full = sb[4];
     });
 }
 

@@ -68,6 +68,7 @@ struct SolveBuffers {
     DBuf<double> vil, Gil;                    // interleaved copies for the row products
     DBuf<float> vilf, Gilf;
     size_t ne = 0;                            // BSR entry capacity (sentinels included)
+    int factor = 0;                           // factor in G: 0 none, 1 block FSAI, 2 learned
     // batch: work-group tables of the row kernels (per vertex partition) and per-scene controls
     std::vector<int> layout_key;
     int layout_ng = 0;
@@ -164,7 +165,8 @@ struct Context {
     DBuf<int> outer, inner;
     DBuf<double> val, rhs;
     int64_t nnz = 0;
-    bool nnz_pending = false;  // nnz is in pin_nnz once the stream has been synchronised
+    DBuf<int> coll;            // collision count of the last finalize_csr (device)
+    bool nnz_pending = false;  // nnz and the collision count are in pin_nnz once the stream has been synchronised
     // pinned destinations of the asynchronous reads that one final synchronisation collects
     PinnedBuf<int> pin_nnz;
     PinnedBuf<int> pin_as;     // counts of the active-set construction
@@ -257,9 +259,10 @@ void energy_terms(Context& c, double kappa, double out[9]);
 // energy_terms in two halves, as step_bound_launch / step_bound_finish
 void energy_terms_launch(Context& c, hipStream_t st);
 void energy_terms_finish(Context& c, double out[9]);
-// after a synchronisation of the stream: the CSR nonzero count of the last finalize_csr
+// after a synchronisation of the stream: the CSR nonzero count and the collision count of the
+// last finalize_csr
 inline void settle_nnz(Context& c) {
-    if (c.nnz_pending) { c.nnz = *c.pin_nnz.p; c.nnz_pending = false; }
+    if (c.nnz_pending) { c.nnz = c.pin_nnz.p[0]; c.collisions = c.pin_nnz.p[1]; c.nnz_pending = false; }
 }
 void cloth_energy_partials(Context& c, double* part_bw, double* part_bend, hipStream_t st);
 double total_energy(const Context& c, double kappa, const double t[9]);

@@ -809,8 +809,10 @@ void linear_solve_scenes(Context& c, int precision, int preconditioner, double r
     w.scal.resize((size_t)S * ((sizeof(PcgScalars) + 7) / 8));
     w.x.resize((size_t)n);
     build_solve_matrix(c, precision == 1);
+    w.factor = 0;
     if (preconditioner == 1) build_fsai(c, precision == 1);
     else if (preconditioner == 2) build_learned(c, precision == 1);
+    w.factor = preconditioner;
     // interleaved copies for the row products
     const size_t ne = w.ne, nil = (ne + kGL - 1) / kGL * 9 * kGL;
     const bool fp32 = precision == 1;

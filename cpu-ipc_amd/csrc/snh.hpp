@@ -1,3 +1,7 @@
+// This Source Code Form is subject to the terms of the Mozilla Public License, v. 2.0.
+// If a copy of the MPL was not distributed with this file, You can obtain one at
+// https://mozilla.org/MPL/2.0/.  Parts of this file restate expressions of CPU-IPC
+// (MPL-2.0) in the reference's operation order; each such part cites the source it follows.
 // Stable neo-Hookean tet element on the device: energy, PK1 and the PSD-projected element
 // stiffness, all in FP64.
 //

@@ -252,6 +252,25 @@ __global__ void masked_rhs(int V, const int* __restrict__ bt, const double* __re
     rhs[i] = is_free(bt[i / 3]) ? grad[i] : 0.0;
 }
 
+// Collision count as compute_g_dpt returns it (ContactMechanics.cpp:2970-3032): the
+// multiplicity of a point-point / point-edge entry, one for every other Self_ActiveSet
+// entry.  One work-group, integer sums: the result does not depend on the order.
+__global__ void collision_count(int n, const int4* __restrict__ set, int* __restrict__ out) {
+    __shared__ long long sh[256];
+    long long acc = 0;
+    for (int i = threadIdx.x; i < n; i += 256) {
+        const int4 e = set[i];
+        acc += (e.x < 0 && (e.z < 0 || e.w < 0)) ? -(long long)e.w : 1ll;
+    }
+    sh[threadIdx.x] = acc;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) sh[threadIdx.x] += sh[threadIdx.x + w];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) out[0] = (int)sh[0];
+}
+
 void finalize_csr(Context& c) {
     Scene& s = c.sc;
     const int V = s.V;
@@ -317,8 +336,12 @@ void finalize_csr(Context& c) {
     CIPC_CHECK_LAUNCH();
     // the count is read with the caller's next synchronisation; the entries are sized by
     // their bound (9 per merged block)
-    c.pin_nnz.reserve(1);
+    c.pin_nnz.reserve(2);
     CIPC_HIP(hipMemcpyAsync(c.pin_nnz.p, c.outer.p + 3 * V, sizeof(int), hipMemcpyDeviceToHost, c.stream));
+    c.coll.resize(1);
+    hipLaunchKernelGGL(collision_count, dim3(1), dim3(256), 0, c.stream, (int)c.n_self, c.self_set.p, c.coll.p);
+    CIPC_CHECK_LAUNCH();
+    CIPC_HIP(hipMemcpyAsync(c.pin_nnz.p + 1, c.coll.p, sizeof(int), hipMemcpyDeviceToHost, c.stream));
     c.nnz_pending = true;
     c.inner.resize(9 * mcap); c.val.resize(9 * mcap);
     hipLaunchKernelGGL(pair_fill, dim3(blocks_for(3 * mcap, 256)), dim3(256), 0, c.stream, nm_ptr, c.mptr.p, c.mrow.p, c.mcol.p,

@@ -96,6 +96,9 @@ typedef struct {
     double energy;        /* energy at the current positions */
     int64_t nnz;          /* CSR nonzeros */
     int64_t contacts;     /* |Self_ActiveSet| + |Self_EE_ActiveSet| + |Environment_ActiveSet| */
+    int64_t collisions;   /* compute_g_dpt's return value (ContactMechanics.cpp:2970-3032), as cipc_gpu_assemble */
+    /* the parts of alpha, as cipc_gpu_step_bound's out[1..4]: ground bound, partial CCD, CFL, full CCD */
+    double bound_ground, bound_partial, bound_cfl, bound_full;
 } cipc_iteration_timing;
 
 /* Result of one Newton linear solve. */
@@ -224,6 +227,14 @@ int cipc_gpu_solve(cipc_gpu_context* ctx, int precision, double rel_tol, int max
 int cipc_gpu_solve_ex(cipc_gpu_context* ctx, int precision, int preconditioner, double rel_tol, int max_iter, double* direction,
                       cipc_solve_stats* stats);
 
+/* The factor G (M^-1 = G^T G) of the last solve with preconditioner 1 or 2, on the block
+ * pattern of the full symmetric Newton matrix: nblocks 3x3 blocks, rowptr (V + 1), col
+ * (nblocks, ascending within a row) and values (9 nblocks, row-major blocks; blocks above
+ * the diagonal and blocks outside the factor's pattern are zero).  Call with NULL arrays to
+ * get nblocks first.  CIPC_ERR_LOGIC when the last solve built no factor.  No counterpart
+ * in the reference; it exposes the preconditioner for checking. */
+int cipc_gpu_get_solve_factor(cipc_gpu_context* ctx, int64_t* nblocks, int* rowptr, int* col, double* values);
+
 /* Friction::initialize (Friction.cpp:81-156), called at the start of a time step: the
  * lagged friction state (per Self_ActiveSet entry lambda, closest-point coordinates and
  * tangent basis; per Environment_ActiveSet vertex lambda) from the current active set and
@@ -251,7 +262,8 @@ int cipc_gpu_step_forward(cipc_gpu_context* ctx, double alpha);
 int cipc_gpu_is_intersected(cipc_gpu_context* ctx, int* intersected);
 
 /* One Newton iteration's non-linear stages on the device-resident state, timed per stage:
- * active set, assembly, step bound, energy (the unit BASELINE-style timing uses). */
+ * active set, assembly, step bound, energy.  The assembled system is then the one that
+ * cipc_gpu_solve / cipc_gpu_get_csr see, as after cipc_gpu_assemble. */
 int cipc_gpu_newton_iteration(cipc_gpu_context* ctx, double kappa, cipc_iteration_timing* timing);
 
 /* Batched scenes.  No single reference interface: the reference runs independent scenes
@@ -290,7 +302,8 @@ int cipc_gpu_batch_is_intersected(cipc_gpu_context* ctx, int* r);
 int cipc_gpu_batch_solve(cipc_gpu_context* ctx, int precision, int preconditioner, double rel_tol, int max_iter, const int* active,
                          double* direction, cipc_solve_stats* stats);
 /* As cipc_gpu_newton_iteration with kappa[k] for scene k; alpha and energy (n_scenes
- * each, may be NULL) per scene; timing->alpha is their minimum, timing->energy their sum. */
+ * each, may be NULL) per scene; timing->alpha and the four bound parts are minima over the
+ * scenes, timing->energy and timing->collisions sums. */
 int cipc_gpu_batch_newton_iteration(cipc_gpu_context* ctx, const double* kappa, cipc_iteration_timing* timing, double* alpha,
                                     double* energy);
 

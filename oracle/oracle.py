@@ -1,3 +1,7 @@
+# This Source Code Form is subject to the terms of the Mozilla Public License, v. 2.0.
+# If a copy of the MPL was not distributed with this file, You can obtain one at
+# https://mozilla.org/MPL/2.0/.  Parts of this file restate logic of CPU-IPC (MPL-2.0);
+# each such part cites the source it follows.
 """ctypes binding of the oracle (test infrastructure only).
 
 The oracle is the CPU restatement of CPU-IPC's per-Newton-iteration path; only

@@ -1,3 +1,7 @@
+// This Source Code Form is subject to the terms of the Mozilla Public License, v. 2.0.
+// If a copy of the MPL was not distributed with this file, You can obtain one at
+// https://mozilla.org/MPL/2.0/.  Parts of this file restate expressions of CPU-IPC
+// (MPL-2.0) in the reference's operation order; each such part cites the source it follows.
 // ORACLE — test infrastructure only.  CPU restatement of CPU-IPC's per-Newton-iteration
 // path, used by tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg as the
 // checker.  It is never linked into, loaded by, or called from the product library.

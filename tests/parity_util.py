@@ -1,0 +1,55 @@
+"""Per-entry error measures of the device-vs-oracle comparisons.
+
+Every entry is judged relative to its own magnitude.  An entry far below the other
+entries of its 3x3 block (or of its vertex's 3-vector) is the result of cancellation, and
+its rounding error is set by the block's magnitude, not by its own; such entries are judged
+against a floor of ``floor`` times the block's largest entry.  With floor 1e-6 and a bar of
+1e-10, an entry at the floor may carry a rounding error of 1e-16 of its block.
+"""
+import numpy as np
+
+
+def csr_columns(outer):
+    outer = np.asarray(outer, np.int64)
+    return np.repeat(np.arange(len(outer) - 1), np.diff(outer))
+
+
+def csr_entry_err(outer, inner, a, b, floor=1e-6):
+    """max over the CSR entries of |a - b| / max(|b|, floor * blockmax), where blockmax is
+    the largest |b| in the entry's 3x3 block (vertex row, vertex column)."""
+    a, b = np.asarray(a, np.float64), np.asarray(b, np.float64)
+    if len(b) == 0:
+        return 0.0
+    col = csr_columns(outer)
+    row = np.asarray(inner, np.int64)
+    nb = (int(col.max()) // 3 + 1) if len(col) else 1
+    key = (row // 3) * nb + col // 3
+    uk, inv = np.unique(key, return_inverse=True)
+    bmax = np.zeros(len(uk))
+    np.maximum.at(bmax, inv, np.abs(b))
+    den = np.maximum(np.abs(b), floor * bmax[inv])
+    den = np.where(den > 0, den, 1.0)
+    return float((np.abs(a - b) / den).max())
+
+
+def vec3_entry_err(a, b, floor=1e-6):
+    """Per-entry relative error of (n,3) arrays with the floor taken per row."""
+    a = np.asarray(a, np.float64).reshape(-1, 3)
+    b = np.asarray(b, np.float64).reshape(-1, 3)
+    if b.size == 0:
+        return 0.0
+    rmax = np.abs(b).max(axis=1, keepdims=True)
+    den = np.maximum(np.abs(b), floor * rmax)
+    den = np.where(den > 0, den, 1.0)
+    err = np.abs(a - b) / den
+    return float(err.max())
+
+
+def entry_err(a, b, floor=1e-6):
+    """Per-entry relative error of two arrays, the floor taken from the largest |b|."""
+    a, b = np.asarray(a, np.float64).ravel(), np.asarray(b, np.float64).ravel()
+    if b.size == 0:
+        return 0.0
+    den = np.maximum(np.abs(b), floor * np.abs(b).max())
+    den = np.where(den > 0, den, 1.0)
+    return float((np.abs(a - b) / den).max())

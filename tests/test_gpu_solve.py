@@ -78,8 +78,8 @@ def test_solve_four_cubes_residual(gpu, precision, preconditioner):
 
 def test_block_fsai_matches_oracle_iterations(gpu):
     """The device's block FSAI (FP64 PCG) against the numpy restatement of the same factor on
-    the same system: the same iteration count to 1e-6 (up to one), and far fewer than
-    block-Jacobi."""
+    the same system: the factor entry by entry, the same iteration count to 1e-6 (up to one),
+    and far fewer than block-Jacobi."""
     import scipy.sparse as sp
     from cipc import scenes
     import fsai_oracle as F
@@ -96,6 +96,24 @@ def test_block_fsai_matches_oracle_iterations(gpu):
         _, st0 = g.solve(precision=0, rel_tol=1e-6, preconditioner=0)
         assert abs(st1["iterations"] - it_o) <= 1, (st1, it_o)
         assert st1["iterations"] * 2 < st0["iterations"], (st1, st0)
+        # the factor itself, entry by entry (each relative to itself, with a floor of 1e-6 of
+        # its block row's largest entry), and the defining identity diag(G A G^T) = I
+        g.solve(precision=0, rel_tol=1e-6, preconditioner=1)
+        Gd = g.solve_factor().tocsr()
+        Go = Gm.tocsr()
+        assert Gd.nnz > 0
+        D = (Gd - Go).tocoo()
+        rmax = np.asarray(abs(Go).max(axis=1).todense()).ravel()
+        row3 = rmax.reshape(-1, 3).max(axis=1).repeat(3)
+        ref = np.asarray(Go[D.row, D.col]).ravel()
+        den = np.maximum(np.abs(ref), 1e-6 * row3[D.row])
+        err = float((np.abs(D.data) / den).max(initial=0.0))
+        print(f"block FSAI factor: per-entry error {err:.2e}, nnz device {Gd.nnz} oracle {Go.nnz}")
+        assert err <= 1e-9, err
+        S = (Gd @ A @ Gd.T).tocsr()
+        n = A.shape[0]
+        for i in range(0, n, 3):
+            assert np.allclose(S[i:i + 3, i:i + 3].toarray(), np.eye(3), atol=1e-9), i
     finally:
         g.close()
 

@@ -294,6 +294,75 @@ def test_accd_is_conservative(O, kind):
             assert dist(pts + s * dirs) > 0.0
 
 
+def _random_frame(rng):
+    q, _ = np.linalg.qr(rng.standard_normal((3, 3)))
+    return q * np.sign(np.linalg.det(q))
+
+
+@pytest.mark.parametrize("eta", [0.2, 0.5])
+def test_accd_point_triangle_head_on_closed_form(O, eta):
+    """Additive CCD (AdditiveCCD.cpp:281-329; Li et al. 2021, Codimensional IPC, Alg. 1) for
+    a point moving along the triangle's normal onto an interior point: the motion bound is
+    the relative speed, the first advance covers (1 - eta) of the gap and leaves eta of it,
+    and the next one ends the loop, so toc = (1 - eta) t*, t* the time of impact, or 1 once
+    that exceeds 1.  Moving the triangle instead of, or as well as, the point gives the same
+    toc for the same relative motion.  A step of half, or twice, the right size fails."""
+    rng = np.random.default_rng(int(eta * 100))
+    for trial in range(40):
+        R = _random_frame(rng)
+        tri = np.array([[-0.6, 0.0, -0.5], [0.7, 0.0, -0.4], [0.0, 0.0, 0.8]])
+        foot = np.array([rng.uniform(-0.1, 0.1), 0.0, rng.uniform(-0.1, 0.1)])
+        d0 = rng.uniform(0.01, 0.5)
+        speed = rng.uniform(0.05, 2.0)
+        tstar = d0 / speed
+        share = [1.0, 0.0, 0.5][trial % 3]           # part of the approach done by the point
+        pts = np.vstack([foot + [0.0, d0, 0.0], tri]) @ R.T + rng.uniform(-2, 2, 3)
+        dp = np.array([0.0, -speed * share, 0.0]) @ R.T
+        dt = np.array([0.0, speed * (1.0 - share), 0.0]) @ R.T
+        dirs = np.vstack([dp, dt, dt, dt])
+        toc = O.accd_pt(pts, dirs, eta)
+        want = min(1.0, (1.0 - eta) * tstar)
+        assert toc <= tstar
+        assert abs(toc - want) <= 1e-12 * max(want, 1e-300) + 1e-15, (trial, toc, want, tstar)
+
+
+@pytest.mark.parametrize("eta", [0.2, 0.5])
+def test_accd_edge_edge_crossing_closed_form(O, eta):
+    """Additive CCD for two edges crossing perpendicularly (AdditiveCCD.cpp:221-279): one
+    edge approaches the other along their common normal, closest points interior to both;
+    toc = (1 - eta) t*, or 1 once that exceeds 1, with either or both edges moving."""
+    rng = np.random.default_rng(7 + int(eta * 100))
+    for trial in range(40):
+        R = _random_frame(rng)
+        d0 = rng.uniform(0.01, 0.5)
+        speed = rng.uniform(0.05, 2.0)
+        tstar = d0 / speed
+        a = rng.uniform(-0.2, 0.2, 2)
+        ea = np.array([[-1.0, d0, a[0]], [1.0, d0, a[0]]])     # along x, above
+        eb = np.array([[a[1], 0.0, -1.0], [a[1], 0.0, 1.0]])   # along z, below
+        share = [1.0, 0.0, 0.5][trial % 3]
+        pts = np.vstack([ea, eb]) @ R.T + rng.uniform(-2, 2, 3)
+        da = np.array([0.0, -speed * share, 0.0]) @ R.T
+        db = np.array([0.0, speed * (1.0 - share), 0.0]) @ R.T
+        dirs = np.vstack([da, da, db, db])
+        toc = O.accd_ee(pts, dirs, eta)
+        want = min(1.0, (1.0 - eta) * tstar)
+        assert toc <= tstar
+        assert abs(toc - want) <= 1e-12 * max(want, 1e-300) + 1e-15, (trial, toc, want, tstar)
+
+
+def test_accd_sliding_motion_returns_one(O):
+    """A point sliding parallel to a triangle's plane, and an edge sliding along its own
+    direction past another, never approach: the loop runs to toc > 1 and returns 1."""
+    tri = np.array([[-5.0, 0.0, -5.0], [5.0, 0.0, -5.0], [0.0, 0.0, 5.0]])
+    pts = np.vstack([[0.0, 0.1, 0.0], tri])
+    dirs = np.vstack([[0.3, 0.0, 0.2], np.zeros((3, 3))])
+    assert O.accd_pt(pts, dirs, 0.2) == 1.0
+    pts = np.array([[-1.0, 0.1, 0.0], [1.0, 0.1, 0.0], [0.0, 0.0, -1.0], [0.0, 0.0, 1.0]])
+    dirs = np.array([[0.5, 0.0, 0.0], [0.5, 0.0, 0.0], [0.0, 0.0, 0.0], [0.0, 0.0, 0.0]])
+    assert O.accd_ee(pts, dirs, 0.2) == 1.0
+
+
 def test_seg_tri_intersection_closed_form(O):
     """IglUtils::segTriIntersect (ContactMechanics.h:47-71): a segment through the
     interior crosses, one beside the triangle or short of the plane does not."""

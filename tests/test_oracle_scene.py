@@ -194,3 +194,73 @@ def test_batched_cubes_share_the_batch_scalars():
         assert 0.02 - 1e-12 <= gap <= 0.2 + 1e-12
         assert 1e4 <= s.material.youngs <= 1e6 and 0.3 <= s.material.poisson <= 0.45
     assert len({s.material.youngs for s in b}) == 6
+
+
+def test_intersection_check_finds_interpenetration(O):
+    """isIntersected (IPCSolver.cpp:476-542): false for the stacked cubes as built and after a
+    step smaller than their gap; true once the upper cube is pushed 5 cm into the lower one,
+    where its vertical side edges cross the lower cube's top faces."""
+    s = small_stack()
+    o = O.OracleMesh(s)
+    try:
+        o.active_set()
+        assert not o.is_intersected()
+        top = s.x[:, 1] > np.median(s.x[:, 1])
+        for depth, expect in ((1e-4, False), (0.05, True)):
+            p = np.zeros_like(s.x)
+            p[top, 1] = depth
+            o.step_forward(s.x, p, 1.0)
+            o.active_set()
+            assert o.is_intersected() == expect, depth
+            o.set_state(x=s.x)
+    finally:
+        o.close()
+
+
+def test_step_bound_closed_forms(O):
+    """The step bound's parts and both branches of its merge (IPCSolver.cpp:945-989) on hand-
+    built motions of three axis-aligned, exactly planar cubes stacked 3e-4 m apart (below
+    dHat), the lowest 3e-4 m above the ground:
+      * all cubes moving down at v: no relative motion, so the CCD bounds equal the ground
+        bound, limitStepByGround's closed form 0.8 h / v (h the lowest height above the
+        ground; FeasibleStep.cpp:8-40);
+      * the middle cube moving down onto the lowest one at v1 and the top one moving up at
+        v2 = v1: the face pairs close head-on, so the partial and full CCD are
+        (1 - eta) gap / v1 = 0.8 gap / v1 (the ACCD closed form), which is not above twice
+        the CFL bound sqrt(hhat) / (2 max|p|): alpha = min(ground, CFL) = CFL;
+      * the same with v2 = 1000 v1: the CFL bound shrinks below half the partial CCD, and
+        alpha = max(min(partial, full), CFL) = 0.8 gap / v1."""
+    s = scenes.stacked_cubes(3, 3, 0.1, 3e-4, seed=2, rotate="none", jitter=0.0, name="three")
+    n = s.V // 3
+    ids = [np.arange(k * n, (k + 1) * n) for k in range(3)]
+    gap = s.x[ids[1], 1].min() - s.x[ids[0], 1].max()
+    sq = np.sqrt(s.hhat)
+    o = O.OracleMesh(s)
+    try:
+        o.active_set()
+        v = 1.0
+        p = np.zeros_like(s.x)
+        p[:, 1] = v
+        b = o.step_bound(p)
+        h = s.x[:, 1].min() + 1.0
+        assert b["ground"] == h / v * 0.8 and b["alpha"] == b["ground"]
+        assert b["partial"] == b["ground"] and b["full"] == b["ground"]
+        p[:, 1] = -v                                     # moving up: the ground does not limit
+        assert o.step_bound(p)["ground"] == 1.0
+        v1 = 1e-3
+        for v2, branch in ((v1, "cfl"), (1000 * v1, "ccd")):
+            p = np.zeros_like(s.x)
+            p[ids[1], 1] = v1
+            p[ids[2], 1] = -v2
+            b = o.step_bound(p)
+            head_on = 0.8 * gap / v1
+            cfl = sq / (max(v1, v2) * 2.0)
+            assert b["ground"] == 1.0
+            assert abs(b["partial"] - head_on) <= 1e-12 * head_on and abs(b["full"] - head_on) <= 1e-12 * head_on
+            assert b["cfl"] == cfl
+            if branch == "cfl":
+                assert b["partial"] <= 2 * cfl and b["alpha"] == cfl
+            else:
+                assert b["partial"] > 2 * cfl and b["alpha"] == min(b["partial"], b["full"])
+    finally:
+        o.close()
