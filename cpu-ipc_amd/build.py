@@ -18,9 +18,9 @@ ARCH = "gfx950"
 COMMON = ["--offload-arch=" + ARCH, "-O3", "-std=c++17", "-fPIC", "-Wall", "-Wno-unused-function",
           "-Wno-unused-variable", "-Wno-unused-lambda-capture",
           "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"]
-NO_CONTRACT = {"contact.hip", "broadphase.hip", "ccd.hip", "energy.hip", "sparse.hip", "kappa.hip"}
+NO_CONTRACT = {"contact.hip", "broadphase.hip", "ccd.hip", "energy.hip", "sparse.hip", "kappa.hip", "solver.hip"}
 SOURCES = ["context.hip", "elastic.hip", "cloth.hip", "contact.hip", "sparse.hip", "broadphase.hip", "ccd.hip", "energy.hip",
-           "pcg.hip", "kappa.hip", "learned.hip"]
+           "pcg.hip", "kappa.hip", "learned.hip", "solver.hip"]
 
 
 def _compile(src):

@@ -38,7 +38,8 @@ EXPORTS = (
     "cipc_gpu_friction_update", "cipc_gpu_get_friction", "cipc_gpu_set_soft_targets", "cipc_gpu_kappa_gradients",
     "cipc_gpu_close_constraints", "cipc_gpu_clear_close_constraints", "cipc_gpu_preconditioner_param_count",
     "cipc_gpu_set_preconditioner_weights", "cipc_gpu_preconditioner_gradient", "cipc_gpu_preconditioner_train_step", "cipc_gpu_solve", "cipc_gpu_solve_ex", "cipc_gpu_get_solve_factor", "cipc_gpu_save_positions", "cipc_gpu_step_forward",
-    "cipc_gpu_is_intersected", "cipc_gpu_newton_iteration",
+    "cipc_gpu_is_intersected", "cipc_gpu_newton_iteration", "cipc_gpu_set_velocities", "cipc_gpu_get_state",
+    "cipc_gpu_time_step",
     "cipc_gpu_upload_batch", "cipc_gpu_batch_size", "cipc_gpu_batch_assemble", "cipc_gpu_batch_friction_update",
     "cipc_gpu_batch_energy", "cipc_gpu_batch_step_bound", "cipc_gpu_batch_step_forward", "cipc_gpu_batch_is_intersected",
     "cipc_gpu_batch_solve", "cipc_gpu_batch_newton_iteration",
@@ -84,6 +85,29 @@ class IterationTiming(C.Structure):
 
 class SolveStats(C.Structure):
     _fields_ = [("iterations", C.c_int), ("refinements", C.c_int), ("rel_residual", C.c_double), ("solve_ms", C.c_double)]
+
+
+class StepParams(C.Structure):
+    _fields_ = [("bbox_diag2", C.c_double), ("mean_mass", C.c_double), ("dtol", C.c_double), ("newton_threshold", C.c_double),
+                ("apply_gravity", C.c_int), ("quasi_static", C.c_int), ("precision", C.c_int), ("preconditioner", C.c_int),
+                ("solve_rel_tol", C.c_double), ("solve_max_iter", C.c_int), ("max_newton", C.c_int)]
+
+
+class StepStats(C.Structure):
+    _fields_ = [(n, C.c_int) for n in ("newton_iterations", "kappa_iterations", "energy_backtracks", "intersection_backtracks",
+                                       "max_energy_backtracks_per_newton", "newton_steps_over_two_backtracks")] + \
+               [(n, C.c_int64) for n in ("pcg_iterations", "collisions", "matrix_nnz", "ground_contacts", "self_contacts",
+                                         "mollified_contacts")] + \
+               [(n, C.c_double) for n in ("kappa", "min_distance2", "dirichlet_alpha", "alpha_sum", "alpha_min", "alpha_max",
+                                          "step_ms", "assembly_ms", "linear_ms", "ccd_ms", "line_search_ms", "post_ms")]
+
+
+DirichletFn = C.CFUNCTYPE(C.c_int, C.c_void_p, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_double), C.POINTER(C.c_double),
+                          C.c_int, C.c_double, C.c_double, C.POINTER(C.c_double))
+
+
+class Dirichlet(C.Structure):
+    _fields_ = [("n", C.c_int), ("vertices", C.POINTER(C.c_int)), ("step", C.c_int), ("fn", DirichletFn), ("user", C.c_void_p)]
 
 
 class CipcError(RuntimeError):
@@ -139,6 +163,9 @@ def load(path: str = LIB_PATH):
         "cipc_gpu_step_forward": [C.c_void_p, C.c_double],
         "cipc_gpu_is_intersected": [C.c_void_p, _ip],
         "cipc_gpu_newton_iteration": [C.c_void_p, C.c_double, C.POINTER(IterationTiming)],
+        "cipc_gpu_set_velocities": [C.c_void_p, _dp],
+        "cipc_gpu_get_state": [C.c_void_p, _dp, _dp, _dp, _dp],
+        "cipc_gpu_time_step": [C.c_void_p, C.POINTER(StepParams), _dp, C.POINTER(Dirichlet), C.POINTER(StepStats)],
         "cipc_gpu_upload_batch": [C.c_void_p, C.POINTER(C.POINTER(SceneDesc)), C.c_int],
         "cipc_gpu_batch_size": [C.c_void_p, _ip, _ip, _ip],
         "cipc_gpu_batch_assemble": [C.c_void_p, _dp, _i64p, _ip],
@@ -435,10 +462,49 @@ class GpuContext:
         _check(load().cipc_gpu_is_intersected(self.h, _iptr(r)))
         return bool(r[0])
 
+    def set_velocities(self, v):
+        _check(load().cipc_gpu_set_velocities(self.h, _dptr(_d(v))))
+
+    def state(self):
+        """(x, V_prev, x~, velocities), each (V, 3)."""
+        out = [np.zeros((self.V, 3)) for _ in range(4)]
+        _check(load().cipc_gpu_get_state(self.h, *[_dptr(a) for a in out]))
+        return tuple(out)
+
+    def time_step(self, kappa, params, dirichlet=None):
+        """One implicit time step (solveIPCStep) on the device; returns (mesh.Kappa after the
+        step, stats dict).  `dirichlet` = (vertices, step, fn(x (n,3), v_rest (n,3), step,
+        fraction, dt) -> directions (n,3)) for animated Dirichlet vertices."""
+        k = np.array([kappa], dtype=np.float64)
+        st = StepStats()
+        dp = None
+        if dirichlet is not None:
+            verts, step, fn = dirichlet
+            verts = np.ascontiguousarray(verts, dtype=np.int32)
+
+            def cb(user, n, vi, x, vr, stp, frac, dt, out):
+                xs = np.ctypeslib.as_array(x, shape=(n, 3)).copy()
+                rs = np.ctypeslib.as_array(vr, shape=(n, 3)).copy()
+                d = np.asarray(fn(xs, rs, stp, frac, dt), dtype=np.float64).reshape(n, 3)
+                np.ctypeslib.as_array(out, shape=(n, 3))[:] = d
+                return 0
+
+            cfn = DirichletFn(cb)
+            dp = Dirichlet(len(verts), verts.ctypes.data_as(C.POINTER(C.c_int)), step, cfn, None)
+            self._keep = (verts, cfn)
+        _check(load().cipc_gpu_time_step(self.h, C.byref(params), _dptr(k), None if dp is None else C.byref(dp), C.byref(st)))
+        return float(k[0]), {f: getattr(st, f) for f, _ in StepStats._fields_}
+
     def newton_iteration(self, kappa):
         t = IterationTiming()
         _check(load().cipc_gpu_newton_iteration(self.h, kappa, C.byref(t)))
         return {f: getattr(t, f) for f, _ in IterationTiming._fields_}
+
+
+def step_params(scene, precision=0, preconditioner=1, rel_tol=1e-6, max_iter=20000, max_newton=10000):
+    """The time step's scalars of a scene (mesh3D fields the descriptor does not carry)."""
+    return StepParams(scene.bbox_diag2, scene.mean_mass, scene.dtol, scene.material.newton_threshold,
+                      1 if scene.apply_gravity else 0, 0, precision, preconditioner, rel_tol, max_iter, max_newton)
 
 
 class BatchContext(GpuContext):

@@ -248,7 +248,7 @@ static void merge_bounds(const Scene& s, const unsigned long long* h, bool any_c
     out[0] = merged; out[1] = aground; out[2] = partial; out[3] = acfl; out[4] = full;
 }
 
-void step_bound_early(Context& c, hipStream_t st) {
+void step_bound_early(Context& c, hipStream_t st, bool ground) {
     Scene& s = c.sc;
     const int S = s.S;
     const int* vs = scene_of_vertex(s);
@@ -257,7 +257,7 @@ void step_bound_early(Context& c, hipStream_t st) {
     DBuf<unsigned long long>& sl = c.u64scratch;
     sl.resize((size_t)kSlots * S);
     hipLaunchKernelGGL(init_bounds, dim3(blocks_for(S, 64)), dim3(64), 0, st, S, sl.p);
-    if (s.Vs > 0)
+    if (s.Vs > 0 && ground)
         hipLaunchKernelGGL(ground_cfl_kernel, dim3(blocks_for(s.Vs, 256)), dim3(256), 0, st, s.Vs, s.sv.p, s.x.p, s.dir.p, s.gn,
                            s.gd, slack, vs, sl.p);
     CIPC_CHECK_LAUNCH();
@@ -303,6 +303,21 @@ void step_bound_scenes(Context& c, double* out) {
     step_bound_launch(c, c.stream);
     CIPC_HIP(hipStreamSynchronize(c.stream));
     step_bound_finish(c, out);
+}
+
+// Self_largestFeasibleStepSize_CCD seeded with step 1 (the animated Dirichlet update,
+// IPCSolver.cpp:1131-1134): the swept trees and the full CCD of step_bound_early with the
+// ground slot left at its initial 1.
+double full_ccd_bound(Context& c) {
+    Scene& s = c.sc;
+    if (s.S != 1) throw Error(CIPC_ERR_LOGIC, "a batch has one step bound per scene");
+    step_bound_early(c, c.stream, false);
+    c.pin_slots.reserve(kSlots);
+    CIPC_HIP(hipMemcpyAsync(c.pin_slots.p, c.u64scratch.p, kSlots * sizeof(unsigned long long), hipMemcpyDeviceToHost, c.stream));
+    CIPC_HIP(hipStreamSynchronize(c.stream));
+    double full;
+    std::memcpy(&full, &c.pin_slots.p[3], 8);
+    return full;
 }
 
 void step_bound(Context& c, double out[5]) {

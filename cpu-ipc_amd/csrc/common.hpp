@@ -154,6 +154,7 @@ struct Scene {
     d3 gn{0, 1, 0};
     double gd = -1;
     DBuf<double> x, x0, vrest, vprev, xt, mass, dir;
+    DBuf<double> vel;          // mesh3D::velocities (3V), advanced by a time step
     DBuf<int> bt;
     DBuf<int4> tets;           // v0..v3
     DBuf<double> dminv;        // SoA: 9 x T (entry k of tet t at k*T + t)

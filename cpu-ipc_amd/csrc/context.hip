@@ -155,6 +155,8 @@ void upload_scene_impl(cipc_gpu_context* c, const cipc_scene_desc* d, bool plan 
         upload(s.vrest, d->v_rest, 3 * (size_t)V, st);
         upload(s.vprev, d->V_prev, 3 * (size_t)V, st);
         upload(s.xt, d->xtilde, 3 * (size_t)V, st);
+        s.vel.resize(3 * (size_t)V);
+        CIPC_HIP(hipMemsetAsync(s.vel.p, 0, 3 * sizeof(double) * (size_t)V, st));
         upload(s.mass, d->masses, (size_t)V, st);
         upload(s.bt, d->boundary, (size_t)V, st);
         s.x0.resize(3 * (size_t)V);
@@ -1028,6 +1030,68 @@ int cipc_gpu_newton_iteration(cipc_gpu_context* c, double kappa, cipc_iteration_
     });
 }
 
+int cipc_gpu_set_velocities(cipc_gpu_context* c, const double* v) {
+    return guard([&] {
+        require(c && c->have_scene && v, "no scene uploaded or null velocities");
+        bind(*c);
+        CIPC_HIP(hipMemcpyAsync(c->sc.vel.p, v, 3 * sizeof(double) * c->sc.V, hipMemcpyHostToDevice, c->stream));
+        CIPC_HIP(hipStreamSynchronize(c->stream));
+    });
+}
+
+int cipc_gpu_get_state(cipc_gpu_context* c, double* x, double* vprev, double* xt, double* v) {
+    return guard([&] {
+        require(c && c->have_scene, "no scene uploaded");
+        bind(*c);
+        const size_t n = 3 * sizeof(double) * c->sc.V;
+        if (x) CIPC_HIP(hipMemcpyAsync(x, c->sc.x.p, n, hipMemcpyDeviceToHost, c->stream));
+        if (vprev) CIPC_HIP(hipMemcpyAsync(vprev, c->sc.vprev.p, n, hipMemcpyDeviceToHost, c->stream));
+        if (xt) CIPC_HIP(hipMemcpyAsync(xt, c->sc.xt.p, n, hipMemcpyDeviceToHost, c->stream));
+        if (v) CIPC_HIP(hipMemcpyAsync(v, c->sc.vel.p, n, hipMemcpyDeviceToHost, c->stream));
+        CIPC_HIP(hipStreamSynchronize(c->stream));
+    });
+}
+
+int cipc_gpu_time_step(cipc_gpu_context* c, const cipc_step_params* p, double* kappa, const cipc_dirichlet* d, cipc_step_stats* out) {
+    return guard([&] {
+        require(c && c->have_scene && p && kappa, "no scene uploaded or null parameters");
+        require_single(c, "cipc_gpu_time_step");
+        require(p->bbox_diag2 > 0 && p->mean_mass > 0 && p->newton_threshold > 0 && std::isfinite(*kappa) && *kappa >= 0,
+                "invalid step parameters");
+        require(p->precision == 0 || p->precision == 1, "precision must be 0 (FP64) or 1 (FP32 with FP64 refinement)");
+        require(p->preconditioner >= 0 && p->preconditioner <= 2, "preconditioner must be 0, 1 or 2");
+        require(p->solve_rel_tol > 0 && p->solve_max_iter > 0 && p->max_newton > 0, "solve tolerance and limits must be positive");
+        DirichletHook hook;
+        if (d && d->n > 0) {
+            require(d->vertices && d->fn, "animated Dirichlet update without vertices or callback");
+            for (int i = 0; i < d->n; ++i)
+                require(d->vertices[i] >= 0 && d->vertices[i] < c->sc.V, "Dirichlet vertex index out of range");
+            hook.n = d->n; hook.vertices = d->vertices; hook.step = d->step; hook.fn = d->fn; hook.user = d->user;
+        }
+        bind(*c);
+        StepParams P;
+        P.bbox_diag2 = p->bbox_diag2; P.mean_mass = p->mean_mass; P.dtol = p->dtol; P.newton_threshold = p->newton_threshold;
+        P.apply_gravity = p->apply_gravity; P.quasi_static = p->quasi_static;
+        P.precision = p->precision; P.preconditioner = p->preconditioner;
+        P.solve_tol = p->solve_rel_tol; P.solve_max_iter = p->solve_max_iter; P.max_newton = p->max_newton;
+        StepStats st;
+        time_step(*c, P, *kappa, st, hook.n ? &hook : nullptr);
+        if (out) {
+            out->newton_iterations = st.newton_iterations; out->kappa_iterations = st.kappa_iterations;
+            out->energy_backtracks = st.energy_backtracks; out->intersection_backtracks = st.intersection_backtracks;
+            out->max_energy_backtracks_per_newton = st.max_energy_backtracks_per_newton;
+            out->newton_steps_over_two_backtracks = st.newton_over_two_backtracks;
+            out->pcg_iterations = st.pcg_iterations; out->collisions = st.collisions; out->matrix_nnz = st.matrix_nnz;
+            out->ground_contacts = st.ground_contacts; out->self_contacts = st.self_contacts;
+            out->mollified_contacts = st.mollified_contacts;
+            out->kappa = st.kappa; out->min_distance2 = st.min_distance2; out->dirichlet_alpha = st.dirichlet_alpha;
+            out->alpha_sum = st.alpha_sum; out->alpha_min = st.alpha_min; out->alpha_max = st.alpha_max;
+            out->step_ms = st.step_ms; out->assembly_ms = st.assembly_ms; out->linear_ms = st.linear_ms; out->ccd_ms = st.ccd_ms;
+            out->line_search_ms = st.line_search_ms; out->post_ms = st.post_ms;
+        }
+    });
+}
+
 int cipc_gpu_eval_cloth(int n, const double* F, double st, double sh, double k, double* P, double* H, double* e) {
     return guard([&] {
         require(n >= 0 && (n == 0 || (F && P && H && e)), "invalid arguments");
@@ -1077,3 +1141,7 @@ int cipc_gpu_eval_classify(int n, int kind, const double* pts, int* type) {
 }
 
 }  // extern "C"
+
+namespace cipc {
+void assemble(Context& c, double kappa) { assemble_impl(static_cast<cipc_gpu_context*>(&c), kappa); }
+}  // namespace cipc

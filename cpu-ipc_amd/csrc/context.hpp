@@ -4,6 +4,7 @@
 #include "common.hpp"
 
 #include <cstring>
+#include <limits>
 
 namespace cipc {
 
@@ -236,6 +237,9 @@ void add_gradient_records_into(Context& c, int n, const int* k, const int* idx, 
 int64_t kappa_gradients(Context& c, double out[2]);
 // postLineSearch's close-constraint check and records; returns 1 when a record got closer
 int close_constraints(Context& c, double dtol, int64_t counts[2]);
+// the kappa loop's exit test: out = {min, max} over the ground then self constraint values;
+// returns their number (0: out is {0, 0})
+int64_t constraint_extrema(Context& c, double out[2]);
 bool any_intersection(Context& c);
 // per scene: 1 when a ground gap is zero or an edge crosses a face
 void scene_intersections(Context& c, int* out);
@@ -249,9 +253,14 @@ void step_bound_launch(Context& c, hipStream_t st);
 // step_bound_launch in two parts: what needs only the positions and the direction (ground
 // bound, CFL, swept trees, full CCD), then the partial CCD over the active set's CCD pairs and
 // the read-back of the bounds
-void step_bound_early(Context& c, hipStream_t st);
+// (ground false: the ground bound and CFL slots are left at 1 and 0, so the full CCD starts
+// from step 1)
+void step_bound_early(Context& c, hipStream_t st, bool ground = true);
 void step_bound_partial(Context& c, hipStream_t st);
 void step_bound_finish(Context& c, double* out);
+// Self_largestFeasibleStepSize_CCD alone (FeasibleStep.cpp), seeded with step 1 and
+// slackness 0.8, over the current direction: the animated Dirichlet update's bound
+double full_ccd_bound(Context& c);
 // per scene: out[5 k .. 5 k + 4] as step_bound's five values
 void step_bound_scenes(Context& c, double* out);
 // energy.hip
@@ -303,6 +312,40 @@ void friction_energy_partials(Context& c, double* part, double* elem_self, doubl
 void step_forward(Context& c, double alpha);
 // per scene: x = x0 - alpha[scene] p
 void step_forward_scenes(Context& c, const double* alpha);
+// computeGradientAndHessian + NewtonLinearSystem::assemble on the context's stream (synchronised)
+void assemble(Context& c, double kappa);
+
+// One implicit time step (solver.hip): scene scalars that the step reads besides the scene
+// descriptor, and the linear-solve settings of its Newton iterations.
+struct StepParams {
+    double bbox_diag2 = 0, mean_mass = 0, dtol = 0, newton_threshold = 1e-2;
+    int apply_gravity = 1, quasi_static = 0;
+    int precision = 0, preconditioner = 1;
+    double solve_tol = 1e-6;
+    int solve_max_iter = 10000;
+    int max_newton = 10000;
+};
+// IPCStepStats (IPCSolver.h) fields that the device step produces
+struct StepStats {
+    int newton_iterations = 0, kappa_iterations = 0;
+    int energy_backtracks = 0, intersection_backtracks = 0, max_energy_backtracks_per_newton = 0, newton_over_two_backtracks = 0;
+    int64_t pcg_iterations = 0, collisions = 0, matrix_nnz = 0;
+    int64_t ground_contacts = 0, self_contacts = 0, mollified_contacts = 0;
+    double kappa = 0, min_distance2 = 0, dirichlet_alpha = 1.0;
+    double alpha_sum = 0, alpha_min = std::numeric_limits<double>::infinity(), alpha_max = 0;
+    double step_ms = 0, assembly_ms = 0, linear_ms = 0, ccd_ms = 0, line_search_ms = 0, post_ms = 0;
+};
+// The prescribed motion of animated Dirichlet vertices (DirichletBoundaryCondition::
+// updateDirection, BoundaryConditions.h), evaluated by the caller on the host
+struct DirichletHook {
+    int n = 0;
+    const int* vertices = nullptr;
+    int step = 0;
+    cipc_dirichlet_fn fn = nullptr;
+    void* user = nullptr;
+};
+void time_step(Context& c, const StepParams& P, double& kappa, StepStats& st, const DirichletHook* D);
+
 // per scene lock-step solves; active[k] == 0 leaves scene k's direction zero
 void linear_solve_scenes(Context& c, int precision, int preconditioner, double rel_tol, int max_iter, const int* active,
                          SolveResult* res);

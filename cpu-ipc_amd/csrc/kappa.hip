@@ -199,6 +199,71 @@ int64_t kappa_gradients(Context& c, double out[2]) {
 }
 
 // ---------------------------------------------------------------------------------
+// The kappa loop's exit test of solveIPCStep (IPCSolver.cpp:1171-1188): the smallest and
+// largest constraint value over the ground constraints and then the self constraints.  A
+// minimum and a maximum do not depend on the order of their operands, so a grid-strided
+// pass with one partial per work-group gives the oracle's values exactly.
+// ---------------------------------------------------------------------------------
+__global__ __launch_bounds__(kEnergyBlock) void extrema_partials(int ne, int ns, const int* __restrict__ env,
+                                                                 const int4* __restrict__ self_set, const double* __restrict__ x, d3 gn,
+                                                                 double gd, double* __restrict__ part) {
+    double lo = INFINITY, hi = -INFINITY;
+    for (int i = blockIdx.x * kEnergyBlock + threadIdx.x; i < ne + ns; i += kEnergyGrid * kEnergyBlock) {
+        double d;
+        if (i < ne) {
+            const double dist = dot(gn, ld3(x, env[i])) - gd;
+            d = dist * dist;
+        } else {
+            d = constraint_value(self_set[i - ne], x);
+        }
+        lo = fmin(lo, d);
+        hi = fmax(hi, d);
+    }
+    __shared__ double sl[kEnergyBlock], sh[kEnergyBlock];
+    sl[threadIdx.x] = lo;
+    sh[threadIdx.x] = hi;
+    __syncthreads();
+    for (int s = kEnergyBlock / 2; s > 0; s >>= 1) {
+        if ((int)threadIdx.x < s) {
+            sl[threadIdx.x] = fmin(sl[threadIdx.x], sl[threadIdx.x + s]);
+            sh[threadIdx.x] = fmax(sh[threadIdx.x], sh[threadIdx.x + s]);
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) { part[blockIdx.x] = sl[0]; part[kEnergyGrid + blockIdx.x] = sh[0]; }
+}
+__global__ __launch_bounds__(kEnergyBlock) void extrema_final(const double* __restrict__ part, double* __restrict__ out) {
+    static_assert(kEnergyGrid == kEnergyBlock, "one partial per thread");
+    __shared__ double sl[kEnergyBlock], sh[kEnergyBlock];
+    sl[threadIdx.x] = part[threadIdx.x];
+    sh[threadIdx.x] = part[kEnergyGrid + threadIdx.x];
+    __syncthreads();
+    for (int s = kEnergyBlock / 2; s > 0; s >>= 1) {
+        if ((int)threadIdx.x < s) {
+            sl[threadIdx.x] = fmin(sl[threadIdx.x], sl[threadIdx.x + s]);
+            sh[threadIdx.x] = fmax(sh[threadIdx.x], sh[threadIdx.x + s]);
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) { out[0] = sl[0]; out[1] = sh[0]; }
+}
+
+int64_t constraint_extrema(Context& c, double out[2]) {
+    Scene& s = c.sc;
+    const int ne = (int)c.n_env, ns = (int)c.n_self;
+    out[0] = out[1] = 0.0;
+    if (ne + ns == 0) return 0;
+    DBuf<double>& part = c.dscratch2;
+    part.resize(2 * (size_t)kEnergyGrid + 2);
+    hipLaunchKernelGGL(extrema_partials, dim3(kEnergyGrid), dim3(kEnergyBlock), 0, c.stream, ne, ns, c.env_set.p, c.self_set.p, s.x.p,
+                       s.gn, s.gd, part.p);
+    hipLaunchKernelGGL(extrema_final, dim3(1), dim3(kEnergyBlock), 0, c.stream, part.p, part.p + 2 * kEnergyGrid);
+    CIPC_CHECK_LAUNCH();
+    read_back(c, out, part.p + 2 * kEnergyGrid, 2);
+    return ne + ns;
+}
+
+// ---------------------------------------------------------------------------------
 // Close constraints (postLineSearch)
 // ---------------------------------------------------------------------------------
 __global__ void close_check(int nge, const int* __restrict__ cenv, const double* __restrict__ cenv_val, int ngs,

@@ -266,6 +266,65 @@ int cipc_gpu_is_intersected(cipc_gpu_context* ctx, int* intersected);
  * cipc_gpu_solve / cipc_gpu_get_csr see, as after cipc_gpu_assemble. */
 int cipc_gpu_newton_iteration(cipc_gpu_context* ctx, double kappa, cipc_iteration_timing* timing);
 
+/* Velocities of the scene's vertices (mesh3D::velocities, 3V; zero after an upload) and the
+ * full state: positions, V_prev, inertial target, velocities (3V each, any may be NULL). */
+int cipc_gpu_set_velocities(cipc_gpu_context* ctx, const double* velocities);
+int cipc_gpu_get_state(cipc_gpu_context* ctx, double* x, double* V_prev, double* xtilde, double* velocities);
+
+/* Scene scalars that a time step reads besides the descriptor (mesh3D::bboxDiagSize2,
+ * meanMass, dTol, Newton_Solver_Threshold, apply_gravity, is_quasi_static;
+ * SimulationMesh.h:37-129) and the Newton linear-solve settings: the device PCG replaces
+ * NewtonLinearSystem::solve (NewtonLinearSystem.cpp:21-56) with precision 0 (FP64) or 1
+ * (FP32 inside FP64 refinement), preconditioner 0 (block-Jacobi), 1 (block FSAI) or 2
+ * (learned), to solve_rel_tol relative residual.  max_newton is solveBarrierSubproblem's
+ * iterationLimit (IPCSolver.cpp:901, 10000). */
+typedef struct {
+    double bbox_diag2, mean_mass, dtol, newton_threshold;
+    int apply_gravity, quasi_static;
+    int precision, preconditioner;
+    double solve_rel_tol;
+    int solve_max_iter, max_newton;
+} cipc_step_params;
+
+/* IPCStepStats (IPCSolver.h) of one time step.  Stage times are host wall-clock, as the
+ * reference's StageTimer; pcg_iterations sums the Newton solves' iterations. */
+typedef struct {
+    int newton_iterations, kappa_iterations;
+    int energy_backtracks, intersection_backtracks, max_energy_backtracks_per_newton, newton_steps_over_two_backtracks;
+    int64_t pcg_iterations, collisions, matrix_nnz;
+    int64_t ground_contacts, self_contacts, mollified_contacts;
+    double kappa, min_distance2, dirichlet_alpha;
+    double alpha_sum, alpha_min, alpha_max;
+    double step_ms, assembly_ms, linear_ms, ccd_ms, line_search_ms, post_ms;
+} cipc_step_stats;
+
+/* DirichletBoundaryCondition::updateDirection (BoundaryConditions.h) for n vertices: from
+ * their current positions x (3n) and rest positions v_rest (3n), the step index, the
+ * fraction of the step's prescribed motion and dt, write the directions (3n; the update
+ * moves x to x - direction).  Returns 0, or non-zero to abort the step. */
+typedef int (*cipc_dirichlet_fn)(void* user, int n, const int* vertices, const double* x, const double* v_rest, int step,
+                                 double fraction, double dt, double* directions);
+typedef struct {
+    int n;                  /* animated Dirichlet vertices (DirichletBoundaryCondition::vertexIndices) */
+    const int* vertices;
+    int step;               /* the step index passed to the callback */
+    cipc_dirichlet_fn fn;
+    void* user;
+} cipc_dirichlet;
+
+/* One implicit time step of the uploaded scene on the device: solveIPCStep
+ * (IPCSolver.cpp:1079-1336) with initKappa and Friction::initialize at the step start, the
+ * animated Dirichlet update with its CCD bound and intersection backoff when dirichlet is
+ * not NULL (IPCSolver.cpp:1127-1155), the kappa loop of barrier subproblems
+ * (solveBarrierSubproblem, IPCSolver.cpp:890-1064: assembly, the device solve, the
+ * |p|_inf convergence test, the merged step bound, lineSearch IPCSolver.cpp:560-749 and
+ * postLineSearch :811-869), then updateVelocity, V_prev = x and updateInertialTarget
+ * (:1196-1204).  kappa is mesh.Kappa, read and updated.  The reference's exceptions
+ * (non-finite energy, no non-intersecting step, no energy decrease, the kappa-loop limit)
+ * return CIPC_ERR_RUNTIME with its message. */
+int cipc_gpu_time_step(cipc_gpu_context* ctx, const cipc_step_params* params, double* kappa, const cipc_dirichlet* dirichlet,
+                       cipc_step_stats* stats);
+
 /* Batched scenes.  No single reference interface: the reference runs independent scenes
  * as separate FEMSimulator / IPCSolverContext instances (IPCSolver.h:46-61), each with
  * its own mesh3D, contact sets, barrier stiffness, step and solve.  A batch context holds

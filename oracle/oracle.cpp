@@ -202,7 +202,9 @@ struct Mesh {
     std::vector<int> closeEnv; std::vector<double> closeEnvVal;
     std::vector<I4> closeSelf; std::vector<double> closeSelfVal;
     std::vector<V3> grad;
+    std::vector<V3> gabs;      // per entry, the sum of |addends| of grad (its rounding scale)
     std::vector<int> outer, inner; std::vector<double> val, rhs;
+    std::vector<double> vscale;    // per CSR entry, the summed magnitude of its contributing blocks
     LBVH faceTree, edgeTree;
     std::vector<AABB> sweptPointBoxes, faceBoxes, edgeBoxes;
     int threads = 0;
@@ -244,6 +246,24 @@ M3 snh_pk1(const M3& F, double u, double r) {
     const double a = u * (1 - 1 / (I2 + 1)), b = r * (I3 - 1 - u * 3 / (r * 4));
     for (int i = 0; i < 9; ++i) P.m[i] = a * F.m[i] + b * c.m[i];
     return P;
+}
+// Rounding scale of each entry of snh_pk1: the magnitudes of the terms a F and b cof(F)
+// and of the cancellations inside b (r (I3 - 1) - 3u/4) and inside each cofactor.
+void snh_pk1_scale(const M3& F, double u, double r, double ps[9]) {
+    M3 U, V; double s[3];
+    svd3(F, U, s, V);
+    const double I3 = s[0] * s[1] * s[2];
+    const double I2 = s[0] * s[0] + s[1] * s[1] + s[2] * s[2];
+    const M3 c = cofactor(F);
+    const double a = u * (1 - 1 / (I2 + 1)), b = r * (I3 - 1 - u * 3 / (r * 4));
+    const double bs = std::fabs(r) * (std::fabs(I3) + 1) + 0.75 * std::fabs(u);
+    for (int col = 0; col < 3; ++col)
+        for (int row = 0; row < 3; ++row) {
+            const int r1 = (row + 1) % 3, r2 = (row + 2) % 3, c1 = (col + 1) % 3, c2 = (col + 2) % 3;
+            const double cm = std::fabs(F(r1, c1) * F(r2, c2)) + std::fabs(F(r1, c2) * F(r2, c1));
+            const int k = row + 3 * col;
+            ps[k] = std::fabs(a * F.m[k]) + bs * std::fabs(c.m[k]) + std::fabs(b) * cm;
+        }
 }
 // project_StabbleNHK_2_H_3D active branch (Elasticity.cpp:1337-1345,1436-1477), then makePD<9>.
 // H9 row-major 9x9 over column-major vec(F).
@@ -351,6 +371,19 @@ void bw_pk1(const double f[6], double stretchS, double shearS, double k, double 
     for (int i = 0; i < 3; ++i) {
         P[i] = stretchS * (uc * 2. * fu[i]) + shearS * (2 * I6 * fv[i]);
         P[3 + i] = stretchS * (vc * 2. * fv[i]) + shearS * (2 * I6 * fu[i]);
+    }
+}
+// Rounding scale of each entry of bw_pk1: the stretch coefficients' terms (1 and 1/|f|,
+// and the strain-rate terms) and the shear term's products.
+void bw_pk1_scale(const double f[6], double stretchS, double shearS, double k, double ps[6]) {
+    const V3 fu(f[0], f[1], f[2]), fv(f[3], f[4], f[5]);
+    const double su = std::sqrt(dot(fu, fu)), sv = std::sqrt(dot(fv, fv));
+    const double cu = 1 + 1 / su + (su > 1 ? 1.5 * std::fabs(k) * (su + 1 / su + 2) : 0.0);
+    const double cv = 1 + 1 / sv + (sv > 1 ? 1.5 * std::fabs(k) * (sv + 1 / sv + 2) : 0.0);
+    const double i6 = std::fabs(fu.x * fv.x) + std::fabs(fu.y * fv.y) + std::fabs(fu.z * fv.z);
+    for (int i = 0; i < 3; ++i) {
+        ps[i] = std::fabs(stretchS) * 2 * cu * std::fabs(fu[i]) + std::fabs(shearS) * 2 * i6 * std::fabs(fv[i]);
+        ps[3 + i] = std::fabs(stretchS) * 2 * cv * std::fabs(fv[i]) + std::fabs(shearS) * 2 * i6 * std::fabs(fu[i]);
     }
 }
 // project_baraffwitkint_H_3D (Elasticity.cpp:1481-1544): 6x6 row-major over vec(F).
@@ -793,6 +826,20 @@ std::array<V3, 2> ground_basis(const V3& nrm) {
     b[1] = normalized(cross(nrm, b[0]));
     return b;
 }
+// grad[v] += c; gabs[v] += the rounding scale of c: its Euclidean length (the terms are
+// vectors formed through rotations, projections and distance gradients, whose components
+// carry errors relative to the vector's length), or an explicit per-component scale `sc`
+// (element forces, whose components are sums of products that cancel)
+inline void gadd(Mesh& m, int v, const V3& c) {
+    m.grad[v] = m.grad[v] + c;
+    const double n = std::sqrt(sqn(c));
+    m.gabs[v] = m.gabs[v] + V3(n, n, n);
+}
+inline void gadd(Mesh& m, int v, const V3& c, const V3& sc) {
+    m.grad[v] = m.grad[v] + c;
+    m.gabs[v] = m.gabs[v] + sc;
+}
+
 
 // Friction::addGradient (Friction.cpp:158-302).
 void friction_gradient(Mesh& m, double eps2, double coef) {
@@ -806,15 +853,15 @@ void friction_gradient(Mesh& m, double eps2, double coef) {
         double L[12]; int vid[4];
         const int nv = lift(m, c, t, L, vid);
         const double s = coef * m.selfLam[c];
-        for (int k = 0; k < nv; ++k) m.grad[vid[k]] = m.grad[vid[k]] + V3(L[3 * k] * s, L[3 * k + 1] * s, L[3 * k + 2] * s);
+        for (int k = 0; k < nv; ++k) gadd(m, vid[k], V3(L[3 * k] * s, L[3 * k + 1] * s, L[3 * k + 2] * s));
     }
     for (size_t c = 0; c < m.envLast.size(); ++c) {
         const int v = m.envLast[c];
         const V3 d = m.x[v] - m.vprev[v];
         const V3 pj = d - dot(d, m.gn) * m.gn;
         const double mag2 = sqn(pj);
-        if (mag2 > eps2) m.grad[v] = m.grad[v] + (coef * m.envLam[c] / std::sqrt(mag2)) * pj;
-        else m.grad[v] = m.grad[v] + (coef * m.envLam[c] / eps) * pj;
+        if (mag2 > eps2) gadd(m, v, (coef * m.envLam[c] / std::sqrt(mag2)) * pj);
+        else gadd(m, v, (coef * m.envLam[c] / eps) * pj);
     }
 }
 
@@ -916,18 +963,19 @@ double friction_energy(const Mesh& m) {
 // ----------------------------------------------------------------------------------
 int gradient_and_hessian(Mesh& m, double kappa, Blocks& BH) {
     m.grad.assign(m.V, V3());
-    for (int v = 0; v < m.V; ++v) m.grad[v] = m.mass[v] * (m.x[v] - m.xt[v]);                     // :118-123
-    for (int v = 0; v < m.V; ++v) m.grad[v] = m.grad[v] + (m.drag * m.mass[v]) * (m.x[v] - m.vprev[v]); // :124-128
+    m.gabs.assign(m.V, V3());
+    for (int v = 0; v < m.V; ++v) gadd(m, v, m.mass[v] * (m.x[v] - m.xt[v]));                     // :118-123
+    for (int v = 0; v < m.V; ++v) gadd(m, v, (m.drag * m.mass[v]) * (m.x[v] - m.vprev[v])); // :124-128
     // soft targets (addSoftDerivatives, BoundaryConditions.cpp:175-208; IPCSolver.cpp:130)
     for (size_t i = 0; i < m.softIdx.size(); ++i) {
         const int v = m.softIdx[i];
-        m.grad[v] = m.grad[v] + m.softW * (m.x[v] - m.softT[i]);
+        gadd(m, v, m.softW * (m.x[v] - m.softT[i]));
         std::array<double, 9> h{};
         h[0] = h[4] = h[8] = m.softW;
         BH.h3.push_back(h); BH.d1.push_back(v);
     }
     // tets (IPCSolver.cpp:139-164)
-    std::vector<std::array<double, 12>> fe(m.T);
+    std::vector<std::array<double, 12>> fe(m.T), fs(m.T);
     BH.h12.assign(m.T, {}); BH.d4.assign(m.T, {});
 #pragma omp parallel for schedule(dynamic, 256)
     for (int t = 0; t < m.T; ++t) {
@@ -935,10 +983,16 @@ int gradient_and_hessian(Mesh& m, double kappa, Blocks& BH) {
         pfpx(m.dminv[t], P);
         const M3 F = deformation(m, t);
         const M3 PE = snh_pk1(F, m.u, m.r);
+        double ps[9];
+        snh_pk1_scale(F, m.u, m.r, ps);
         for (int j = 0; j < 12; ++j) {
-            double acc = 0;
-            for (int k = 0; k < 9; ++k) acc += (m.vol[t] * P[k][j]) * PE.m[k];
+            double acc = 0, sc = 0;
+            for (int k = 0; k < 9; ++k) {
+                acc += (m.vol[t] * P[k][j]) * PE.m[k];
+                sc += std::fabs(m.vol[t] * P[k][j]) * ps[k];
+            }
             fe[t][j] = acc;
+            fs[t][j] = sc;
         }
         double Hq[81];
         snh_hessian(F, m.u, m.r, Hq);
@@ -962,21 +1016,23 @@ int gradient_and_hessian(Mesh& m, double kappa, Blocks& BH) {
     for (int t = 0; t < m.T; ++t)
         for (int a = 0; a < 4; ++a) {
             V3 f(fe[t][3 * a], fe[t][3 * a + 1], fe[t][3 * a + 2]);
-            m.grad[m.tets[t][a]] = m.grad[m.tets[t][a]] + dt2 * f;
+            gadd(m, m.tets[t][a], dt2 * f, dt2 * V3(fs[t][3 * a], fs[t][3 * a + 1], fs[t][3 * a + 2]));
         }
     // cloth triangles (IPCSolver.cpp:166-194): f = dt^2 area PFPX^T vec(P), H9 = area dt^2 PFPX^T Hq PFPX
-    std::vector<std::array<double, 9>> ft(m.Tr);
+    std::vector<std::array<double, 9>> ft(m.Tr), fts(m.Tr);
     BH.h9.assign(m.Tr, {}); BH.d3.assign(m.Tr, {});
 #pragma omp parallel for schedule(dynamic, 256)
     for (int t = 0; t < m.Tr; ++t) {
-        double P[6][9], f[6], pe[6], Hq[36];
+        double P[6][9], f[6], pe[6], Hq[36], ps[6];
         tri_pfpx(m.dmt[t], P);
         tri_deformation(m, t, f);
         bw_pk1(f, m.stretchS, m.shearS, m.srate, pe);
+        bw_pk1_scale(f, m.stretchS, m.shearS, m.srate, ps);
         for (int j = 0; j < 9; ++j) {
-            double acc = 0;
-            for (int k = 0; k < 6; ++k) acc += P[k][j] * pe[k];
+            double acc = 0, sc = 0;
+            for (int k = 0; k < 6; ++k) { acc += P[k][j] * pe[k]; sc += std::fabs(P[k][j]) * ps[k]; }
             ft[t][j] = m.dt * m.dt * m.area[t] * acc;
+            fts[t][j] = m.dt * m.dt * m.area[t] * sc;
         }
         bw_hessian(f, m.stretchS, m.shearS, m.srate, Hq);
         double T1[6][9];
@@ -996,9 +1052,10 @@ int gradient_and_hessian(Mesh& m, double kappa, Blocks& BH) {
         BH.d3[t] = m.tris[t];
     }
     for (int t = 0; t < m.Tr; ++t)
-        for (int a = 0; a < 3; ++a) m.grad[m.tris[t][a]] = m.grad[m.tris[t][a]] + V3(ft[t][3 * a], ft[t][3 * a + 1], ft[t][3 * a + 2]);
+        for (int a = 0; a < 3; ++a)
+            gadd(m, m.tris[t][a], V3(ft[t][3 * a], ft[t][3 * a + 1], ft[t][3 * a + 2]), V3(fts[t][3 * a], fts[t][3 * a + 1], fts[t][3 * a + 2]));
     // bending (IPCSolver.cpp:197-267): quadratic form D dt^2 (Q x I3) or the hinge angle
-    std::vector<std::array<double, 12>> fb(m.Nb);
+    std::vector<std::array<double, 12>> fb(m.Nb), fbs(m.Nb);
     std::vector<std::array<double, 144>> hb(m.Nb);
 #pragma omp parallel for schedule(dynamic, 256)
     for (int h = 0; h < m.Nb; ++h) {
@@ -1007,9 +1064,13 @@ int gradient_and_hessian(Mesh& m, double kappa, Blocks& BH) {
             const double sc = m.plate * m.dt * m.dt;
             for (int a = 0; a < 4; ++a)
                 for (int d = 0; d < 3; ++d) {
-                    double acc = 0;
-                    for (int b = 0; b < 4; ++b) acc += (sc * m.bq[h][4 * a + b]) * m.x[bv[b]][d];
+                    double acc = 0, as = 0;
+                    for (int b = 0; b < 4; ++b) {
+                        acc += (sc * m.bq[h][4 * a + b]) * m.x[bv[b]][d];
+                        as += std::fabs((sc * m.bq[h][4 * a + b]) * m.x[bv[b]][d]);
+                    }
                     fb[h][3 * a + d] = acc;
+                    fbs[h][3 * a + d] = as;
                 }
             for (int i = 0; i < 144; ++i) hb[h][i] = 0.0;
             for (int a = 0; a < 4; ++a)
@@ -1025,11 +1086,17 @@ int gradient_and_hessian(Mesh& m, double kappa, Blocks& BH) {
                 for (int j = 0; j < 12; ++j) hb[h][12 * i + j] = sc * (ag[i] * ag[j] + df * aH[12 * i + j]);
             make_pd<12>(hb[h].data());
             for (int i = 0; i < 12; ++i) fb[h][i] *= m.dt * m.dt;
+            for (int a = 0; a < 4; ++a) {
+                const double gn = std::sqrt(ag[3 * a] * ag[3 * a] + ag[3 * a + 1] * ag[3 * a + 1] + ag[3 * a + 2] * ag[3 * a + 2]);
+                for (int d = 0; d < 3; ++d)
+                    fbs[h][3 * a + d] = std::fabs(sc) * (std::fabs(df) + std::fabs(th) + std::fabs(m.brest[h])) * gn * m.dt * m.dt;
+            }
             for (int i = 0; i < 144; ++i) hb[h][i] *= m.dt * m.dt;
         }
     }
     for (int h = 0; h < m.Nb; ++h) {
-        for (int a = 0; a < 4; ++a) m.grad[m.bverts[h][a]] = m.grad[m.bverts[h][a]] + V3(fb[h][3 * a], fb[h][3 * a + 1], fb[h][3 * a + 2]);
+        for (int a = 0; a < 4; ++a)
+            gadd(m, m.bverts[h][a], V3(fb[h][3 * a], fb[h][3 * a + 1], fb[h][3 * a + 2]), V3(fbs[h][3 * a], fbs[h][3 * a + 1], fbs[h][3 * a + 2]));
         BH.h12.push_back(hb[h]); BH.d4.push_back(m.bverts[h]);
     }
     // ground (IPCSolver.cpp:270-293)
@@ -1038,7 +1105,7 @@ int gradient_and_hessian(Mesh& m, double kappa, Blocks& BH) {
         const double d = dist * dist;
         double gb, hb;
         barrier_g(d, m.hhat, gb); barrier_h(d, m.hhat, hb);
-        m.grad[v] = m.grad[v] + (((kappa * gb) * 2.0) * dist) * m.gn;
+        gadd(m, v, (((kappa * gb) * 2.0) * dist) * m.gn);
         const double param = 4.0 * hb * d + 2.0 * gb;
         if (param > 0) {
             std::array<double, 9> h;
@@ -1053,7 +1120,7 @@ int gradient_and_hessian(Mesh& m, double kappa, Blocks& BH) {
         for (int k = 0; k < nv; ++k) p[k] = m.x[vid[k]];
         double d, g[12];
         deriv_any(type, p, d, g, nullptr);
-        for (int k = 0; k < nv; ++k) m.grad[vid[k]] = m.grad[vid[k]] + V3(scale * g[3 * k], scale * g[3 * k + 1], scale * g[3 * k + 2]);
+        for (int k = 0; k < nv; ++k) gadd(m, vid[k], V3(scale * g[3 * k], scale * g[3 * k + 1], scale * g[3 * k + 2]));
     };
     auto g_dpt = [&](const std::vector<I4>& set, const std::vector<double>& in) {
         int n = 0;
@@ -1088,7 +1155,7 @@ int gradient_and_hessian(Mesh& m, double kappa, Blocks& BH) {
         double eg[12];
         mol_eg(p, ex, eg);
         const double cb = kappa * b;
-        for (int k = 0; k < 4; ++k) m.grad[e4[k]] = m.grad[e4[k]] + cb * V3(eg[3 * k], eg[3 * k + 1], eg[3 * k + 2]);
+        for (int k = 0; k < 4; ++k) gadd(m, e4[k], cb * V3(eg[3 * k], eg[3 * k + 1], eg[3 * k + 2]));
         edb[i] = g * e;
     }
     g_dpt(m.eeAS, edb);
@@ -1186,10 +1253,12 @@ int gradient_and_hessian(Mesh& m, double kappa, Blocks& BH) {
 // BHessian::toTriplets + NewtonLinearSystem::assemble (ContactMechanics.cpp:13-134;
 // NewtonLinearSystem.cpp:126-154).  setFromTriplets sums duplicates in insertion order.
 void assemble_csr(Mesh& m, const Blocks& BH) {
-    struct Tri { int r, c; double v; };
+    struct Tri { int r, c; double v, s; };
     std::vector<Tri> trip;
     auto append = [&](int nv, const double* H, const int* vid) {
         const int n = 3 * nv;
+        double hmax = 0.0;                      // rounding scale of every entry of this block
+        for (int i = 0; i < n * n; ++i) hmax = std::max(hmax, std::fabs(H[i]));
         for (int a = 0; a < nv; ++a) {
             const int rv = vid[a];
             if (rv < 0 || rv >= m.V || !isFree(m.bt[rv])) continue;
@@ -1200,7 +1269,7 @@ void assemble_csr(Mesh& m, const Blocks& BH) {
                     for (int cd = 0; cd < 3; ++cd) {
                         const int gr = 3 * rv + rd, gc = 3 * cv + cd;
                         const double v = H[n * (3 * a + rd) + 3 * b + cd];
-                        if (gr >= gc && v != 0.0) trip.push_back({gr, gc, v});
+                        if (gr >= gc && v != 0.0) trip.push_back({gr, gc, v, hmax});
                     }
             }
         }
@@ -1209,7 +1278,10 @@ void assemble_csr(Mesh& m, const Blocks& BH) {
     for (size_t i = 0; i < BH.h6.size(); ++i) append(2, BH.h6[i].data(), BH.d2[i].data());
     for (size_t i = 0; i < BH.h9.size(); ++i) append(3, BH.h9[i].data(), BH.d3[i].data());
     for (size_t i = 0; i < BH.h12.size(); ++i) append(4, BH.h12[i].data(), BH.d4[i].data());
-    for (int dof = 0; dof < 3 * m.V; ++dof) trip.push_back({dof, dof, m.mass[dof / 3] * (1.0 + m.drag)});
+    for (int dof = 0; dof < 3 * m.V; ++dof) {
+        const double v = m.mass[dof / 3] * (1.0 + m.drag);
+        trip.push_back({dof, dof, v, std::fabs(v)});
+    }
     const int n = 3 * m.V;
     std::vector<int> cnt(n + 1, 0);
     for (const auto& t : trip) cnt[t.c + 1]++;
@@ -1217,14 +1289,14 @@ void assemble_csr(Mesh& m, const Blocks& BH) {
     std::vector<int> pos(cnt.begin(), cnt.end() - 1);
     std::vector<int> order(trip.size());
     for (size_t i = 0; i < trip.size(); ++i) order[pos[trip[i].c]++] = (int)i;
-    m.outer.assign(n + 1, 0); m.inner.clear(); m.val.clear();
+    m.outer.assign(n + 1, 0); m.inner.clear(); m.val.clear(); m.vscale.clear();
     for (int c = 0; c < n; ++c) {
         std::vector<int> ids(order.begin() + cnt[c], order.begin() + cnt[c + 1]);
         std::stable_sort(ids.begin(), ids.end(), [&](int a, int b) { return trip[a].r < trip[b].r; });
         for (size_t k = 0; k < ids.size(); ++k) {
             const Tri& t = trip[ids[k]];
-            if (k > 0 && trip[ids[k - 1]].r == t.r) m.val.back() += t.v;
-            else { m.inner.push_back(t.r); m.val.push_back(t.v); }
+            if (k > 0 && trip[ids[k - 1]].r == t.r) { m.val.back() += t.v; m.vscale.back() += t.s; }
+            else { m.inner.push_back(t.r); m.val.push_back(t.v); m.vscale.push_back(t.s); }
         }
         m.outer[c + 1] = (int)m.inner.size();
     }
@@ -1373,6 +1445,26 @@ int close_constraints(Mesh& m, double dtol) {
         if (d < dtol) { m.closeSelf.push_back(c); m.closeSelfVal.push_back(d); }
     }
     return closer;
+}
+
+// The kappa loop's exit test of solveIPCStep (IPCSolver.cpp:1171-1188): the smallest and
+// largest value over the ground constraints (Evaluate_GroundConstraintVals) followed by the
+// self constraints (Evaluate_SelfPTConstraintVals); returns their number.
+int constraint_extrema(Mesh& m, double out[2]) {
+    int n = 0;
+    double lo = 0.0, hi = 0.0;
+    auto take = [&](double d) {
+        if (n == 0 || d < lo) lo = d;
+        if (n == 0 || d > hi) hi = d;
+        ++n;
+    };
+    for (int v : m.envAS) {
+        const double dist = dot(m.gn, m.x[v]) - m.gd;
+        take(dist * dist);
+    }
+    for (const I4& c : m.selfAS) take(self_val(m, c));
+    out[0] = lo; out[1] = hi;
+    return n;
 }
 
 double energy(Mesh& m, double kappa) {
@@ -1620,51 +1712,12 @@ double accd_pt(V3 p, V3 t0, V3 t1, V3 t2, V3 dp, V3 dt0, V3 dt1, V3 dt2, double 
 // Newton step bound: limitStepByGround (FeasibleStep.cpp:8-40), partial CCD
 // (ContactMechanics.cpp:3704-3752), CFL (IPCSolver.cpp:960-974), swept rebuild and full
 // CCD (ContactMechanics.cpp:3754-3867), merge (IPCSolver.cpp:985-989).
-void step_bound(Mesh& m, const std::vector<V3>& p, double out[5]) {
-    const double slack = 0.8;
-    double alpha = 1.0;
-    {   // ground
-        double mn = 1.0;
-        for (int s = 0; s < m.Vs; ++s) {
-            double best = 1.0;
-            const int v = m.sv[s];
-            const double nm = dot(m.gn, -p[v]);
-            if (!(nm >= 0.0)) {
-                const double dist = dot(m.gn, m.x[v]) - m.gd;
-                const double cand = -dist / nm * slack;
-                if (cand >= 0.0) best = cand;
-            }
-            mn = std::min(mn, best);
-        }
-        if (m.Vs > 0) alpha = std::min(alpha, mn);
-    }
-    const double aground = alpha;
-    if (alpha <= 0) alpha = 1;
-    const double eta = 1.0 - slack;
-    double partial = alpha;
-    if (!m.ccdAS.empty()) {
-        std::vector<double> la(m.ccdAS.size());
-#pragma omp parallel for schedule(dynamic, 64)
-        for (long i = 0; i < (long)m.ccdAS.size(); ++i) {
-            const auto& pr = m.ccdAS[i];
-            if (pr.first < 0) {
-                const int pt = m.sv[-pr.first - 1];
-                const auto& f = m.faces[pr.second];
-                la[i] = accd_pt(m.x[pt], m.x[f[0]], m.x[f[1]], m.x[f[2]], -p[pt], -p[f[0]], -p[f[1]], -p[f[2]], eta, 0.0);
-            } else {
-                const auto& e1 = m.se[pr.first]; const auto& e2 = m.se[pr.second];
-                la[i] = accd_ee(m.x[e1.first], m.x[e1.second], m.x[e2.first], m.x[e2.second],
-                                -p[e1.first], -p[e1.second], -p[e2.first], -p[e2.second], eta, 0.0);
-            }
-        }
-        double mn = la[0];
-        for (double v : la) mn = std::min(mn, v);
-        partial = std::min(partial, mn);
-    }
-    double maxmove = 0.0;
-    for (int s = 0; s < m.Vs; ++s) maxmove = std::max(maxmove, std::sqrt(sqn(p[m.sv[s]])));
-    const double acfl = maxmove > 0.0 ? std::sqrt(m.hhat) / (maxmove * 2.0) : 1.0;
-    double full = alpha;
+// Self_largestFeasibleStepSize_CCD (ContactMechanics.cpp:3754-3867) after
+// SpatialHash::build(mesh, p, alpha0): the swept trees at step alpha0, then the
+// point-triangle and edge-edge phases; returns the full-CCD step.
+double full_ccd(Mesh& m, const std::vector<V3>& p, double alpha0) {
+    const double eta = 1.0 - 0.8;
+    double full = alpha0;
     build_swept(m, p, full);
     {   // PT phase
         const double a = full;
@@ -1720,6 +1773,54 @@ void step_bound(Mesh& m, const std::vector<V3>& p, double out[5]) {
         if (m.Es > 0) { mn = le[0]; for (double v : le) mn = std::min(mn, v); }
         full = std::min(full, mn);
     }
+    return full;
+}
+
+void step_bound(Mesh& m, const std::vector<V3>& p, double out[5]) {
+    const double slack = 0.8;
+    double alpha = 1.0;
+    {   // ground
+        double mn = 1.0;
+        for (int s = 0; s < m.Vs; ++s) {
+            double best = 1.0;
+            const int v = m.sv[s];
+            const double nm = dot(m.gn, -p[v]);
+            if (!(nm >= 0.0)) {
+                const double dist = dot(m.gn, m.x[v]) - m.gd;
+                const double cand = -dist / nm * slack;
+                if (cand >= 0.0) best = cand;
+            }
+            mn = std::min(mn, best);
+        }
+        if (m.Vs > 0) alpha = std::min(alpha, mn);
+    }
+    const double aground = alpha;
+    if (alpha <= 0) alpha = 1;
+    const double eta = 1.0 - slack;
+    double partial = alpha;
+    if (!m.ccdAS.empty()) {
+        std::vector<double> la(m.ccdAS.size());
+#pragma omp parallel for schedule(dynamic, 64)
+        for (long i = 0; i < (long)m.ccdAS.size(); ++i) {
+            const auto& pr = m.ccdAS[i];
+            if (pr.first < 0) {
+                const int pt = m.sv[-pr.first - 1];
+                const auto& f = m.faces[pr.second];
+                la[i] = accd_pt(m.x[pt], m.x[f[0]], m.x[f[1]], m.x[f[2]], -p[pt], -p[f[0]], -p[f[1]], -p[f[2]], eta, 0.0);
+            } else {
+                const auto& e1 = m.se[pr.first]; const auto& e2 = m.se[pr.second];
+                la[i] = accd_ee(m.x[e1.first], m.x[e1.second], m.x[e2.first], m.x[e2.second],
+                                -p[e1.first], -p[e1.second], -p[e2.first], -p[e2.second], eta, 0.0);
+            }
+        }
+        double mn = la[0];
+        for (double v : la) mn = std::min(mn, v);
+        partial = std::min(partial, mn);
+    }
+    double maxmove = 0.0;
+    for (int s = 0; s < m.Vs; ++s) maxmove = std::max(maxmove, std::sqrt(sqn(p[m.sv[s]])));
+    const double acfl = maxmove > 0.0 ? std::sqrt(m.hhat) / (maxmove * 2.0) : 1.0;
+    const double full = full_ccd(m, p, alpha);
     alpha = std::min(alpha, acfl);
     if (partial > 2.0 * acfl) {
         alpha = std::min(partial, full);
@@ -1866,6 +1967,12 @@ int oracle_assemble(oracle_mesh* om, double kappa, int64_t* nnz) {
 void oracle_get_gradient(const oracle_mesh* om, double* g) {
     for (int v = 0; v < om->m.V; ++v) { g[3 * v] = om->m.grad[v].x; g[3 * v + 1] = om->m.grad[v].y; g[3 * v + 2] = om->m.grad[v].z; }
 }
+void oracle_get_csr_scale(const oracle_mesh* om, double* s) {
+    for (size_t i = 0; i < om->m.vscale.size(); ++i) s[i] = om->m.vscale[i];
+}
+void oracle_get_gradient_scale(const oracle_mesh* om, double* g) {
+    for (int v = 0; v < om->m.V; ++v) { g[3 * v] = om->m.gabs[v].x; g[3 * v + 1] = om->m.gabs[v].y; g[3 * v + 2] = om->m.gabs[v].z; }
+}
 void oracle_get_csr(const oracle_mesh* om, int* outer, int* inner, double* val, double* rhs) {
     const Mesh& m = om->m;
     std::copy(m.outer.begin(), m.outer.end(), outer);
@@ -1881,6 +1988,7 @@ int oracle_close_constraints(oracle_mesh* om, double dtol, int64_t counts[2]) {
     counts[0] = (int64_t)om->m.closeEnv.size(); counts[1] = (int64_t)om->m.closeSelf.size();
     return r;
 }
+int oracle_constraint_extrema(oracle_mesh* om, double out[2]) { return constraint_extrema(om->m, out); }
 void oracle_clear_close_constraints(oracle_mesh* om) {
     Mesh& m = om->m;
     m.closeEnv.clear(); m.closeEnvVal.clear(); m.closeSelf.clear(); m.closeSelfVal.clear();
@@ -1907,6 +2015,11 @@ void oracle_step_bound(oracle_mesh* om, const double* p, double out[5]) {
     std::vector<V3> pv(om->m.V);
     for (int i = 0; i < om->m.V; ++i) pv[i] = V3(p[3 * i], p[3 * i + 1], p[3 * i + 2]);
     step_bound(om->m, pv, out);
+}
+double oracle_full_ccd(oracle_mesh* om, const double* p, double alpha0) {
+    std::vector<V3> pv(om->m.V);
+    for (int i = 0; i < om->m.V; ++i) pv[i] = V3(p[3 * i], p[3 * i + 1], p[3 * i + 2]);
+    return full_ccd(om->m, pv, alpha0);
 }
 void oracle_step_forward(oracle_mesh* om, const double* x0, const double* p, double a) {
     Mesh& m = om->m;

@@ -66,6 +66,12 @@ void oracle_set_active_set(oracle_mesh* m, const int64_t counts[4], const int* s
  * nnz via out pointer; the CSR is the Eigen lower CSC (== upper CSR). */
 int oracle_assemble(oracle_mesh* m, double kappa, int64_t* nnz);
 void oracle_get_gradient(const oracle_mesh* m, double* grad);
+/* per gradient entry, the sum of the magnitudes of the terms added into it (3V): the scale of
+ * its rounding error where the terms cancel */
+void oracle_get_gradient_scale(const oracle_mesh* m, double* scale);
+/* per CSR entry (nnz), the sum over the element / contact / mass blocks added into it of
+ * each block's largest magnitude: the scale of its rounding error where the blocks cancel */
+void oracle_get_csr_scale(const oracle_mesh* m, double* scale);
 void oracle_get_csr(const oracle_mesh* m, int* outer, int* inner, double* val, double* rhs);
 
 /* computeEnergy. */
@@ -79,6 +85,9 @@ int oracle_kappa_gradients(oracle_mesh* m, double out[2]);
  * records the constraints below dtol; counts = {ground, self} recorded */
 int oracle_close_constraints(oracle_mesh* m, double dtol, int64_t counts[2]);
 void oracle_clear_close_constraints(oracle_mesh* m);
+/* the kappa loop's exit test (IPCSolver.cpp:1171-1188): out = {min, max} over the ground then
+ * self constraint values; returns their number (0: out undefined) */
+int oracle_constraint_extrema(oracle_mesh* m, double out[2]);
 /* soft target-position penalty 0.5 w |x_v - t|^2 on n vertices (SoftBoundaryCondition) */
 void oracle_set_soft(oracle_mesh* m, int n, const int* idx, const double* targets, double weight);
 /* Friction::initialize: lagged friction state from the current active set and positions. */
@@ -89,6 +98,9 @@ void oracle_get_friction(const oracle_mesh* m, int64_t counts[2], double* lam, d
 
 /* Newton step bound (IPCSolver.cpp:943-990): out = {alpha, ground, partial, cfl, full} */
 void oracle_step_bound(oracle_mesh* m, const double* p, double out[5]);
+/* Self_largestFeasibleStepSize_CCD alone with slackness 0.8 from step alpha0 (the animated
+ * Dirichlet update, IPCSolver.cpp:1131-1134) */
+double oracle_full_ccd(oracle_mesh* m, const double* p, double alpha0);
 
 /* x <- x0 - alpha p on free vertices (stepForward, IPCSolver.cpp:458-473) */
 void oracle_step_forward(oracle_mesh* m, const double* x0, const double* p, double alpha);

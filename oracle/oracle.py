@@ -75,6 +75,8 @@ def lib():
         L.oracle_assemble.restype = C.c_int
         L.oracle_assemble.argtypes = [C.c_void_p, C.c_double, _i64p]
         L.oracle_get_gradient.argtypes = [C.c_void_p, _dp]
+        L.oracle_get_gradient_scale.argtypes = [C.c_void_p, _dp]
+        L.oracle_get_csr_scale.argtypes = [C.c_void_p, _dp]
         L.oracle_get_csr.argtypes = [C.c_void_p, _ip, _ip, _dp, _dp]
         L.oracle_energy.restype = C.c_double
         L.oracle_energy.argtypes = [C.c_void_p, C.c_double]
@@ -86,8 +88,12 @@ def lib():
         L.oracle_close_constraints.restype = C.c_int
         L.oracle_close_constraints.argtypes = [C.c_void_p, C.c_double, _i64p]
         L.oracle_clear_close_constraints.argtypes = [C.c_void_p]
+        L.oracle_constraint_extrema.restype = C.c_int
+        L.oracle_constraint_extrema.argtypes = [C.c_void_p, _dp]
         L.oracle_get_friction.argtypes = [C.c_void_p, _i64p, _dp, _dp, _dp, _dp]
         L.oracle_step_bound.argtypes = [C.c_void_p, _dp, _dp]
+        L.oracle_full_ccd.restype = C.c_double
+        L.oracle_full_ccd.argtypes = [C.c_void_p, _dp, C.c_double]
         L.oracle_step_forward.argtypes = [C.c_void_p, _dp, _dp, C.c_double]
         L.oracle_get_positions.argtypes = [C.c_void_p, _dp]
         L.oracle_is_intersected.restype = C.c_int
@@ -204,12 +210,17 @@ class OracleMesh:
         n = 3 * self.V
         g = np.zeros(n)
         lib().oracle_get_gradient(self.h, dptr(g))
+        gs = np.zeros(n)
+        lib().oracle_get_gradient_scale(self.h, dptr(gs))
         outer = np.zeros(n + 1, np.int32)
         inner = np.zeros(int(nnz[0]), np.int32)
         val = np.zeros(int(nnz[0]))
         rhs = np.zeros(n)
         lib().oracle_get_csr(self.h, iptr(outer), iptr(inner), dptr(val), dptr(rhs))
-        return dict(grad=g.reshape(-1, 3), outer=outer, inner=inner, val=val, rhs=rhs, collisions=ncol)
+        vs = np.zeros(int(nnz[0]))
+        lib().oracle_get_csr_scale(self.h, dptr(vs))
+        return dict(grad=g.reshape(-1, 3), grad_scale=gs.reshape(-1, 3), outer=outer, inner=inner, val=val, val_scale=vs, rhs=rhs,
+                    collisions=ncol)
 
     def energy(self, kappa):
         return lib().oracle_energy(self.h, kappa)
@@ -242,6 +253,12 @@ class OracleMesh:
     def clear_close_constraints(self):
         lib().oracle_clear_close_constraints(self.h)
 
+    def constraint_extrema(self):
+        """The kappa loop's exit test: (min, max, n) over the ground then self constraint values."""
+        out = np.zeros(2)
+        n = lib().oracle_constraint_extrema(self.h, dptr(out))
+        return out[0], out[1], n
+
     def friction_update(self, kappa):
         """Friction::initialize on the current active set and positions."""
         lib().oracle_friction_update(self.h, kappa)
@@ -258,6 +275,10 @@ class OracleMesh:
         out = np.zeros(5)
         lib().oracle_step_bound(self.h, dptr(p), dptr(out))
         return dict(alpha=out[0], ground=out[1], partial=out[2], cfl=out[3], full=out[4])
+
+    def full_ccd(self, p, alpha0=1.0):
+        """Self_largestFeasibleStepSize_CCD alone from step alpha0."""
+        return lib().oracle_full_ccd(self.h, dptr(_d(p)), alpha0)
 
     def step_forward(self, x0, p, alpha):
         x0, p = _d(x0), _d(p)

@@ -118,8 +118,8 @@ def test_cloth_assembly_matches_oracle(cloth):
     name = cloth["name"]
     same = np.array_equal(rg["outer"], ro["outer"]) and np.array_equal(rg["inner"], ro["inner"])
     assert same, (name, _pattern_report(rg, ro, 3 * cloth["scene"].V))
-    ev = csr_entry_err(ro["outer"], ro["inner"], rg["val"], ro["val"])
-    eg, er = vec3_entry_err(rg["grad"], ro["grad"]), vec3_entry_err(rg["rhs"], ro["rhs"])
+    ev = csr_entry_err(ro["outer"], ro["inner"], rg["val"], ro["val"], scale=ro["val_scale"])
+    eg, er = vec3_entry_err(rg["grad"], ro["grad"], scale=ro["grad_scale"]), vec3_entry_err(rg["rhs"], ro["rhs"], scale=ro["grad_scale"])
     print(f"{name}: per-entry CSR {ev:.2e} gradient {eg:.2e} rhs {er:.2e}")
     assert ev <= REL and eg <= REL and er <= REL, (name, ev, eg, er)
     assert rg["collisions"] == ro["collisions"]

@@ -49,9 +49,9 @@ def test_assembly_with_friction_matches_oracle(big):
     o.friction_update(kappa)
     rg, ro = g.assemble(kappa), o.assemble(kappa)
     assert np.array_equal(rg["outer"], ro["outer"]) and np.array_equal(rg["inner"], ro["inner"])
-    ev = csr_entry_err(ro["outer"], ro["inner"], rg["val"], ro["val"])
-    eg = vec3_entry_err(rg["grad"], ro["grad"])
-    er = vec3_entry_err(rg["rhs"], ro["rhs"])
+    ev = csr_entry_err(ro["outer"], ro["inner"], rg["val"], ro["val"], scale=ro["val_scale"])
+    eg = vec3_entry_err(rg["grad"], ro["grad"], scale=ro["grad_scale"])
+    er = vec3_entry_err(rg["rhs"], ro["rhs"], scale=ro["grad_scale"])
     print(f"8x32^3: nnz {len(ro['val'])}, per-entry CSR {ev:.2e} gradient {eg:.2e} rhs {er:.2e}")
     assert ev <= 1e-10 and eg <= 1e-10 and er <= 1e-10, (ev, eg, er)
     assert rg["collisions"] == ro["collisions"]

@@ -19,9 +19,9 @@ REL = 1e-10
 def check_system(rg, ro, name):
     """CSR pattern bit-exact; values, gradient and right-hand side per entry."""
     assert np.array_equal(rg["outer"], ro["outer"]) and np.array_equal(rg["inner"], ro["inner"]), name
-    ev = csr_entry_err(ro["outer"], ro["inner"], rg["val"], ro["val"])
-    eg = vec3_entry_err(rg["grad"], ro["grad"])
-    er = vec3_entry_err(rg["rhs"], ro["rhs"])
+    ev = csr_entry_err(ro["outer"], ro["inner"], rg["val"], ro["val"], scale=ro["val_scale"])
+    eg = vec3_entry_err(rg["grad"], ro["grad"], scale=ro["grad_scale"])
+    er = vec3_entry_err(rg["rhs"], ro["rhs"], scale=ro["grad_scale"])
     print(f"{name}: per-entry CSR {ev:.2e} gradient {eg:.2e} rhs {er:.2e} (nnz {len(ro['val'])})")
     assert ev <= REL, (name, "CSR", ev)
     assert eg <= REL, (name, "gradient", eg)
