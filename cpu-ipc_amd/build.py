@@ -36,6 +36,21 @@ def _compile(src):
     return obj, msg
 
 
+HOST = os.path.join(HERE, "host")
+HOST_SOURCES = ["mesh.cpp", "abi.cpp"]
+HEADLESS = os.path.join(HERE, "cipc_headless")
+
+
+def _compile_host(src):
+    """Host-only C++ (scene set-up and its C ABI): g++, rounding as written."""
+    obj = os.path.join(BUILD, "host_" + src.replace(".cpp", ".o"))
+    cmd = ["g++", "-O2", "-std=c++17", "-fPIC", "-Wall", "-ffp-contract=off", "-c", os.path.join(HOST, src), "-o", obj]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"g++ failed for {src}:\n{r.stdout}{r.stderr}")
+    return obj, r.stdout + r.stderr
+
+
 def build(jobs=None, verbose=False):
     os.makedirs(BUILD, exist_ok=True)
     jobs = jobs or min(8, os.cpu_count() or 4)
@@ -45,12 +60,20 @@ def build(jobs=None, verbose=False):
         for obj, msg in results:
             if msg.strip():
                 print(msg)
+    results += [_compile_host(src) for src in HOST_SOURCES]
     objs = [o for o, _ in results]
     tmp = LIB + ".tmp"
     r = subprocess.run([HIPCC, "--offload-arch=" + ARCH, "-shared", "-o", tmp] + objs, capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError("link failed:\n" + r.stdout + r.stderr)
     os.replace(tmp, LIB)
+    # the headless driver (the reference's apps/cipc_headless), linked to the library
+    tmp = HEADLESS + ".tmp"
+    r = subprocess.run(["g++", "-O2", "-std=c++17", "-Wall", "-ffp-contract=off", os.path.join(HOST, "headless.cpp"), "-o", tmp,
+                        "-L" + HERE, "-lcipc_gpu", "-Wl,-rpath,$ORIGIN"], capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError("headless link failed:\n" + r.stdout + r.stderr)
+    os.replace(tmp, HEADLESS)
     return LIB
 
 

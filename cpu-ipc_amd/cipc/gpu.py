@@ -481,15 +481,17 @@ class GpuContext:
         if dirichlet is not None:
             verts, step, fn = dirichlet
             verts = np.ascontiguousarray(verts, dtype=np.int32)
+            if isinstance(fn, int):            # the address of a C cipc_dirichlet_fn
+                cfn = DirichletFn(fn)
+            else:
+                def cb(user, n, vi, x, vr, stp, frac, dt, out):
+                    xs = np.ctypeslib.as_array(x, shape=(n, 3)).copy()
+                    rs = np.ctypeslib.as_array(vr, shape=(n, 3)).copy()
+                    d = np.asarray(fn(xs, rs, stp, frac, dt), dtype=np.float64).reshape(n, 3)
+                    np.ctypeslib.as_array(out, shape=(n, 3))[:] = d
+                    return 0
 
-            def cb(user, n, vi, x, vr, stp, frac, dt, out):
-                xs = np.ctypeslib.as_array(x, shape=(n, 3)).copy()
-                rs = np.ctypeslib.as_array(vr, shape=(n, 3)).copy()
-                d = np.asarray(fn(xs, rs, stp, frac, dt), dtype=np.float64).reshape(n, 3)
-                np.ctypeslib.as_array(out, shape=(n, 3))[:] = d
-                return 0
-
-            cfn = DirichletFn(cb)
+                cfn = DirichletFn(cb)
             dp = Dirichlet(len(verts), verts.ctypes.data_as(C.POINTER(C.c_int)), step, cfn, None)
             self._keep = (verts, cfn)
         _check(load().cipc_gpu_time_step(self.h, C.byref(params), _dptr(k), None if dp is None else C.byref(dp), C.byref(st)))

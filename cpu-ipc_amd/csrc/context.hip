@@ -59,6 +59,8 @@ using namespace cipc;
 extern "C" {
 
 const char* cipc_gpu_last_error(void) { return g_last_error.c_str(); }
+// the error channel of the host-side entry points (include/cipc_host.h)
+int cipc_internal_fail(int code, const char* msg) { return fail(code, msg ? msg : "unknown error"); }
 
 int cipc_gpu_context_create(int device, cipc_gpu_context** out) {
     return guard([&] {

@@ -474,6 +474,10 @@ void build_scene(Mesh& m, const std::string& scene, const std::string& asset_dir
     } else {
         throw std::invalid_argument("unknown scene: " + scene);
     }
+    finalize_scene(m);
+}
+
+void finalize_scene(Mesh& m) {
     // buildModels after the scene (Simulator.cpp:468-506)
     init_mesh(m);
     m.v_rest = m.x;

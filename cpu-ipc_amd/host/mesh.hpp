@@ -69,6 +69,9 @@ void load_settings(Mesh& m, const std::string& path);
 // FEMSimulator::buildModels (Simulator.cpp:385-540) for "twisting-mat", "twisting-mat-soft",
 // "bunny2" and "cloth-bunny"; assets under asset_dir (the reference's Assets/ layout)
 void build_scene(Mesh& m, const std::string& scene, const std::string& asset_dir, const std::string& settings_path);
+// the tail of buildModels (Simulator.cpp:468-506): initMesh3D, v_rest, V_prev, the inertial
+// target, the scene-scaled hhat / fhat / dTol, getSurface, the average surface edge length
+void finalize_scene(Mesh& m);
 // prepareQuadraticBending (Simulator.cpp:48-109) on the current positions
 void prepare_quadratic_bending(Mesh& m);
 // twistingBoundaryDirection (Simulator.cpp:24-46)
