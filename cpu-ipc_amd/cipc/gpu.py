@@ -34,7 +34,7 @@ EXPORTS = (
     "cipc_gpu_set_state", "cipc_gpu_get_positions", "cipc_gpu_set_direction", "cipc_gpu_active_set",
     "cipc_gpu_get_active_set", "cipc_gpu_set_active_set", "cipc_gpu_assemble", "cipc_gpu_set_hessian_precision",
     "cipc_gpu_get_gradient",
-    "cipc_gpu_get_csr", "cipc_gpu_energy", "cipc_gpu_energy_terms", "cipc_gpu_step_bound",
+    "cipc_gpu_get_csr", "cipc_gpu_energy", "cipc_gpu_energy_terms", "cipc_gpu_energy_terms_noalloc", "cipc_gpu_step_bound",
     "cipc_gpu_friction_update", "cipc_gpu_get_friction", "cipc_gpu_set_soft_targets", "cipc_gpu_kappa_gradients",
     "cipc_gpu_close_constraints", "cipc_gpu_clear_close_constraints", "cipc_gpu_preconditioner_param_count",
     "cipc_gpu_set_preconditioner_weights", "cipc_gpu_preconditioner_gradient", "cipc_gpu_preconditioner_train_step", "cipc_gpu_solve", "cipc_gpu_solve_ex", "cipc_gpu_get_solve_factor", "cipc_gpu_save_positions", "cipc_gpu_step_forward",
@@ -146,6 +146,7 @@ def load(path: str = LIB_PATH):
         "cipc_gpu_get_csr": [C.c_void_p, _ip, _ip, _dp, _dp],
         "cipc_gpu_energy": [C.c_void_p, C.c_double, _dp],
         "cipc_gpu_energy_terms": [C.c_void_p, C.c_double, _dp],
+        "cipc_gpu_energy_terms_noalloc": [C.c_void_p, C.c_double, C.c_void_p],
         "cipc_gpu_friction_update": [C.c_void_p, C.c_double],
         "cipc_gpu_set_soft_targets": [C.c_void_p, C.c_int, _ip, _dp, C.c_double],
         "cipc_gpu_kappa_gradients": [C.c_void_p, _dp, _i64p],

@@ -204,7 +204,7 @@ __global__ __launch_bounds__(kEnergyBlock) void e_bw(int Tr, const int4* __restr
                                                      const double* __restrict__ dmt, const double* __restrict__ area, double st,
                                                      double sh, double k, double* __restrict__ part) {
     double acc = 0.0;
-    for (int t = blockIdx.x * kEnergyBlock + threadIdx.x; t < Tr; t += kEnergyGrid * kEnergyBlock) {
+    for (int t = blockIdx.x * kEnergyBlock + threadIdx.x; t < Tr; t += gridDim.x * kEnergyBlock) {
         const int4 id = tris[t];
         const d3 p0 = ld3(x, id.x);
         const d3 e1 = ld3(x, id.y) - p0, e2 = ld3(x, id.z) - p0;
@@ -218,7 +218,7 @@ __global__ __launch_bounds__(kEnergyBlock) void e_bend(int Nb, int mode, const i
                                                        const double* __restrict__ bq, const double* __restrict__ rest,
                                                        const double* __restrict__ weight, double plate, double* __restrict__ part) {
     double acc = 0.0;
-    for (int h = blockIdx.x * kEnergyBlock + threadIdx.x; h < Nb; h += kEnergyGrid * kEnergyBlock) {
+    for (int h = blockIdx.x * kEnergyBlock + threadIdx.x; h < Nb; h += gridDim.x * kEnergyBlock) {
         const int4 id = bv[h];
         const d3 q[4] = {ld3(x, id.x), ld3(x, id.y), ld3(x, id.z), ld3(x, id.w)};
         if (mode == 0) {

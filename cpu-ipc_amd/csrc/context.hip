@@ -642,6 +642,20 @@ int cipc_gpu_clear_close_constraints(cipc_gpu_context* c) {
     });
 }
 
+int cipc_gpu_energy_terms_noalloc(cipc_gpu_context* c, double kappa, double* terms_device) {
+    return guard([&] {
+        (void)kappa;
+        require(c && c->have_scene && terms_device, "no scene uploaded or null output");
+        bind(*c);
+        hipPointerAttribute_t at;
+        const hipError_t pe = hipPointerGetAttributes(&at, terms_device);
+        if (pe != hipSuccess) (void)hipGetLastError();   // an unknown (host) pointer: clear the error it leaves
+        if (pe != hipSuccess || at.type != hipMemoryTypeDevice)
+            throw Error(CIPC_ERR_INVALID_ARGUMENT, "terms_device must be device memory (9 doubles)");
+        energy_terms_noalloc(*c, terms_device);
+    });
+}
+
 int cipc_gpu_energy_terms(cipc_gpu_context* c, double kappa, double terms[9]) {
     return guard([&] {
         require(c && c->have_scene && terms, "no scene uploaded or null output");

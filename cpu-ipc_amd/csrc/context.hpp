@@ -268,6 +268,12 @@ void energy_terms(Context& c, double kappa, double out[9]);
 // energy_terms in two halves, as step_bound_launch / step_bound_finish
 void energy_terms_launch(Context& c, hipStream_t st);
 void energy_terms_finish(Context& c, double out[9]);
+// the partials and sums of energy_terms_launch into P: kEnergyScratch doubles
+constexpr int kEnergyScratch = 11 * 256;
+void energy_partials_launch(Context& c, hipStream_t st, double* P);
+// energy_terms without allocating: partials in a file-scope device array; the nine terms
+// written to out_device (device memory, 9 doubles); synchronises the context's stream
+void energy_terms_noalloc(Context& c, double* out_device);
 // after a synchronisation of the stream: the CSR nonzero count and the collision count of the
 // last finalize_csr
 inline void settle_nnz(Context& c) {

@@ -17,6 +17,8 @@
 #include "context.hpp"
 #include "contact.hpp"
 
+#include <mutex>
+
 namespace cipc {
 
 constexpr int kEB = kEnergyBlock;   // threads per work-group
@@ -33,7 +35,7 @@ __global__ __launch_bounds__(kEB) void e_snh(int T, const int4* __restrict__ tet
                                              const double* __restrict__ dminv, const double* __restrict__ vol, Lame lame,
                                              double* __restrict__ part, double* __restrict__ elem) {
     double acc = 0.0;
-    for (int t = blockIdx.x * kEB + threadIdx.x; t < T; t += kEG * kEB) {
+    for (int t = blockIdx.x * kEB + threadIdx.x; t < T; t += gridDim.x * kEB) {
         double u, r;
         lame.at(t, u, r);
         const int4 id = tets[t];
@@ -60,7 +62,7 @@ __global__ __launch_bounds__(kEB) void e_inertia(int V, const double* __restrict
                                                  const double* __restrict__ vp, const double* __restrict__ m, double drag,
                                                  double* __restrict__ part, double* __restrict__ elem) {
     double acc = 0.0;
-    for (int v = blockIdx.x * kEB + threadIdx.x; v < V; v += kEG * kEB) {
+    for (int v = blockIdx.x * kEB + threadIdx.x; v < V; v += gridDim.x * kEB) {
         const d3 xv = ld3(x, v);
         if (elem) {
             elem[v] = sqn(xv - ld3(xt, v)) * m[v] / 2.0 + (sqn(xv - ld3(vp, v)) * (drag) * m[v] / 2.0);
@@ -74,7 +76,7 @@ __global__ __launch_bounds__(kEB) void e_inertia(int V, const double* __restrict
 __global__ __launch_bounds__(kEB) void e_ground(int n, const int* __restrict__ env, const double* __restrict__ x, d3 gn, double gd,
                                                 double hhat, double* __restrict__ part, double* __restrict__ elem) {
     double acc = 0.0;
-    for (int i = blockIdx.x * kEB + threadIdx.x; i < n; i += kEG * kEB) {
+    for (int i = blockIdx.x * kEB + threadIdx.x; i < n; i += gridDim.x * kEB) {
         const double dist = dot(gn, ld3(x, env[i])) - gd;
         const double b = barrier_b(dist * dist, hhat);
         if (elem) elem[i] = b;
@@ -92,7 +94,7 @@ __device__ inline double self_d(int4 c, const double* x) {
 __global__ __launch_bounds__(kEB) void e_self(int n, const int4* __restrict__ set, const double* __restrict__ x, double hhat,
                                               double* __restrict__ part, double* __restrict__ elem) {
     double acc = 0.0;
-    for (int i = blockIdx.x * kEB + threadIdx.x; i < n; i += kEG * kEB) {
+    for (int i = blockIdx.x * kEB + threadIdx.x; i < n; i += gridDim.x * kEB) {
         const int4 c = set[i];
         double b = barrier_b(self_d(c, x), hhat);
         if (c.w < -1) b *= -c.w;
@@ -106,7 +108,7 @@ __global__ __launch_bounds__(kEB) void e_mollified(int n, const int4* __restrict
                                                    const double* __restrict__ vrest, double hhat, double* __restrict__ part,
                                                    double* __restrict__ elem) {
     double acc = 0.0;
-    for (int i = blockIdx.x * kEB + threadIdx.x; i < n; i += kEG * kEB) {
+    for (int i = blockIdx.x * kEB + threadIdx.x; i < n; i += gridDim.x * kEB) {
         const int4 c = set[i];
         int e4[4];
         if (c.w >= 0) { e4[0] = c.x; e4[1] = c.y; e4[2] = c.z; e4[3] = c.w; }
@@ -125,7 +127,7 @@ __global__ __launch_bounds__(kEB) void e_mollified(int n, const int4* __restrict
 __global__ __launch_bounds__(kEB) void e_soft(int n, const int* __restrict__ idx, const double* __restrict__ tgt, double w,
                                               const double* __restrict__ x, double* __restrict__ part) {
     double acc = 0.0;
-    for (int i = blockIdx.x * kEB + threadIdx.x; i < n; i += kEG * kEB) acc += 0.5 * w * sqn(ld3(x, idx[i]) - ld3(tgt, i));
+    for (int i = blockIdx.x * kEB + threadIdx.x; i < n; i += gridDim.x * kEB) acc += 0.5 * w * sqn(ld3(x, idx[i]) - ld3(tgt, i));
     block_sum_store(acc, part, 0);
 }
 __global__ __launch_bounds__(kEB) void sum_partials(const double* __restrict__ part, int nterms, double* __restrict__ out) {
@@ -151,11 +153,18 @@ void energy_terms(Context& c, double kappa, double out[9]) {
 }
 
 void energy_terms_launch(Context& c, hipStream_t st) {
+    DBuf<double>& part = c.dscratch;
+    part.resize((size_t)kEnergyScratch);
+    energy_partials_launch(c, st, part.p);
+    c.pin_energy.reserve(10);
+    CIPC_HIP(hipMemcpyAsync(c.pin_energy.p, part.p + 10 * kEG, 10 * sizeof(double), hipMemcpyDeviceToHost, st));
+}
+
+// The partial sums of the ten energy terms (kEG per term) and, after them, the ten sums,
+// into P (kEnergyScratch doubles); every entry of P is written before it is read.
+void energy_partials_launch(Context& c, hipStream_t st, double* P) {
     Scene& s = c.sc;
     static_assert(kEG == kEB, "partials are summed by one work-group");
-    DBuf<double>& part = c.dscratch;
-    part.resize((size_t)11 * kEG);
-    double* P = part.p;
     hipLaunchKernelGGL(e_snh, dim3(kEG), dim3(kEB), 0, st, s.T, s.tets.p, s.x.p, s.dminv.p, s.vol.p, scene_lame(s), P,
                        nullptr);
     hipLaunchKernelGGL(e_inertia, dim3(kEG), dim3(kEB), 0, st, s.V, s.x.p, s.xt.p, s.vprev.p, s.mass.p, s.drag, P + kEG,
@@ -173,8 +182,50 @@ void energy_terms_launch(Context& c, hipStream_t st) {
     hipLaunchKernelGGL(e_soft, dim3(kEG), dim3(kEB), 0, st, c.n_soft, c.soft_idx.p, c.soft_t.p, c.soft_w, s.x.p, P + 9 * kEG);
     hipLaunchKernelGGL(sum_partials, dim3(1), dim3(kEB), 0, st, P, 10, P + 10 * kEG);
     CIPC_CHECK_LAUNCH();
-    c.pin_energy.reserve(10);
-    CIPC_HIP(hipMemcpyAsync(c.pin_energy.p, P + 10 * kEG, 10 * sizeof(double), hipMemcpyDeviceToHost, st));
+}
+
+// energy_terms_finish on the device: the nine terms from the ten sums
+__global__ void finish_terms_kernel(const double* __restrict__ t, double fric, int quad_bending, double plate, double* __restrict__ out) {
+    if (threadIdx.x != 0) return;
+    for (int k = 0; k < 5; ++k) out[k] = t[k];
+    out[5] = fric * t[5] + t[6];
+    out[6] = t[7];
+    out[7] = quad_bending ? 0.5 * plate * t[8] : t[8];
+    out[8] = t[9];
+}
+
+// ---- the allocation-free energy ------------------------------------------------------
+// Its partial sums live in a file-scope device array instead of the context's scratch.  The
+// array's host-side name is not a device address: each device's copy is found once, with
+// that device current, by hipGetSymbolAddress, and only that address reaches the kernels.
+// Every call on a device shares its copy: calls that overlap on one device race.
+static_assert(kEnergyScratch == 11 * kEG, "ten terms of kEG partials and the ten sums");
+__device__ __align__(256) double scratch[kEnergyScratch];
+
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mutex;
+double* g_scratch_ptr[kMaxDevices] = {};
+double* scratch_on(int device) {
+    if (device < 0 || device >= kMaxDevices) throw Error(CIPC_ERR_INVALID_ARGUMENT, "device ordinal out of range");
+    std::lock_guard<std::mutex> lock(g_scratch_mutex);
+    if (!g_scratch_ptr[device]) {
+        void* p = nullptr;
+        CIPC_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        g_scratch_ptr[device] = static_cast<double*>(p);
+    }
+    return g_scratch_ptr[device];
+}
+}  // namespace
+
+void energy_terms_noalloc(Context& c, double* out_device) {
+    const Scene& s = c.sc;
+    if (s.S != 1) throw Error(CIPC_ERR_LOGIC, "a batch has one energy per scene: the allocation-free energy takes one scene");
+    double* P = scratch_on(c.device);
+    energy_partials_launch(c, c.stream, P);
+    hipLaunchKernelGGL(finish_terms_kernel, dim3(1), dim3(64), 0, c.stream, P + 10 * kEG, s.fric, s.bmode == 0 ? 1 : 0, s.plate, out_device);
+    CIPC_CHECK_LAUNCH();
+    CIPC_HIP(hipStreamSynchronize(c.stream));
 }
 
 void energy_terms_finish(Context& c, double out[9]) {
@@ -194,6 +245,9 @@ void energy_terms_finish(Context& c, double out[9]) {
 // one scene each: every run is summed by one work-group and the runs of a scene are added
 // in list order.  The order of every addition is fixed, so the sums are deterministic.
 constexpr int kRunGrid = 1024;
+// per-element values (no partial sums): one thread per element, so that a large batch fills
+// the GPU (the order of the later sums does not depend on this grid)
+inline unsigned elem_grid(int64_t n) { return std::max(1u, blocks_for((size_t)n, kEB)); }
 __device__ inline int first_vertex(int4 c) { return c.x >= 0 ? c.x : -c.x - 1; }
 __device__ inline int first_vertex(int v) { return v; }
 // block sum of one value per thread (all threads get it)
@@ -312,12 +366,12 @@ void energy_terms_scenes(Context& c, double* out) {
     double* eFs = eE + nE;
     double* eFg = eFs + nFs;
     double* dev_out = eFg + nFg;
-    hipLaunchKernelGGL(e_snh, dim3(kEG), dim3(kEB), 0, c.stream, s.T, s.tets.p, s.x.p, s.dminv.p, s.vol.p, scene_lame(s), nullptr, eT);
-    hipLaunchKernelGGL(e_inertia, dim3(kEG), dim3(kEB), 0, c.stream, s.V, s.x.p, s.xt.p, s.vprev.p, s.mass.p, s.drag, nullptr, eV);
-    if (nG) hipLaunchKernelGGL(e_ground, dim3(kEG), dim3(kEB), 0, c.stream, (int)nG, c.env_set.p, s.x.p, s.gn, s.gd, s.hhat, nullptr, eG);
-    if (nS) hipLaunchKernelGGL(e_self, dim3(kEG), dim3(kEB), 0, c.stream, (int)nS, c.self_set.p, s.x.p, s.hhat, nullptr, eS);
+    hipLaunchKernelGGL(e_snh, dim3(elem_grid(s.T)), dim3(kEB), 0, c.stream, s.T, s.tets.p, s.x.p, s.dminv.p, s.vol.p, scene_lame(s), nullptr, eT);
+    hipLaunchKernelGGL(e_inertia, dim3(elem_grid(s.V)), dim3(kEB), 0, c.stream, s.V, s.x.p, s.xt.p, s.vprev.p, s.mass.p, s.drag, nullptr, eV);
+    if (nG) hipLaunchKernelGGL(e_ground, dim3(elem_grid(nG)), dim3(kEB), 0, c.stream, (int)nG, c.env_set.p, s.x.p, s.gn, s.gd, s.hhat, nullptr, eG);
+    if (nS) hipLaunchKernelGGL(e_self, dim3(elem_grid(nS)), dim3(kEB), 0, c.stream, (int)nS, c.self_set.p, s.x.p, s.hhat, nullptr, eS);
     if (nE)
-        hipLaunchKernelGGL(e_mollified, dim3(kEG), dim3(kEB), 0, c.stream, (int)nE, c.ee_set.p, c.ee_ie.p, s.se.p, s.x.p, s.vrest.p,
+        hipLaunchKernelGGL(e_mollified, dim3(elem_grid(nE)), dim3(kEB), 0, c.stream, (int)nE, c.ee_set.p, c.ee_ie.p, s.se.p, s.x.p, s.vrest.p,
                            s.hhat, nullptr, eE);
     if (nFs + nFg) friction_energy_partials(c, nullptr, eFs, eFg, c.stream);
     CIPC_CHECK_LAUNCH();

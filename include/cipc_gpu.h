@@ -165,6 +165,15 @@ int cipc_gpu_get_csr(cipc_gpu_context* ctx, int* outer, int* inner, double* valu
  * BoundaryConditions.cpp:210-229)}. */
 int cipc_gpu_energy(cipc_gpu_context* ctx, double kappa, double* energy);
 int cipc_gpu_energy_terms(cipc_gpu_context* ctx, double kappa, double terms[9]);
+/* No counterpart in the reference: cipc_gpu_energy_terms for callers that cannot allocate
+ * (code under HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).  The
+ * same nine terms, written to terms_device (device memory, 9 doubles); the call allocates
+ * nothing: its reduction partials live in a fixed device array of the library (11 x 256
+ * doubles, 22.5 KiB per GPU, held until the process ends), whose size does not depend on
+ * the scene.  One scene only (a batch context is refused, CIPC_ERR_LOGIC); every call on a
+ * GPU shares that GPU's array, so calls that overlap on one GPU race.  Synchronises the
+ * context's stream. */
+int cipc_gpu_energy_terms_noalloc(cipc_gpu_context* ctx, double kappa, double* terms_device);
 
 /* Learned block preconditioner (solve preconditioner 2).  No counterpart in the reference.
  * A message-passing network (hidden width 64, 3 layers, bfloat16 weights with float32

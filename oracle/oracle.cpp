@@ -1257,8 +1257,11 @@ void assemble_csr(Mesh& m, const Blocks& BH) {
     std::vector<Tri> trip;
     auto append = [&](int nv, const double* H, const int* vid) {
         const int n = 3 * nv;
-        double hmax = 0.0;                      // rounding scale of every entry of this block
+        // rounding scale of every entry of this block: n max|H|, a bound of its 2-norm, to which
+        // the error of its eigendecomposition (the PSD projection) is proportional
+        double hmax = 0.0;
         for (int i = 0; i < n * n; ++i) hmax = std::max(hmax, std::fabs(H[i]));
+        hmax *= n;
         for (int a = 0; a < nv; ++a) {
             const int rv = vid[a];
             if (rv < 0 || rv >= m.V || !isFree(m.bt[rv])) continue;

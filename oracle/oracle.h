@@ -70,7 +70,8 @@ void oracle_get_gradient(const oracle_mesh* m, double* grad);
  * its rounding error where the terms cancel */
 void oracle_get_gradient_scale(const oracle_mesh* m, double* scale);
 /* per CSR entry (nnz), the sum over the element / contact / mass blocks added into it of
- * each block's largest magnitude: the scale of its rounding error where the blocks cancel */
+ * n times the block's largest magnitude (n its order, a bound of its 2-norm): the scale of
+ * its rounding error where the blocks cancel */
 void oracle_get_csr_scale(const oracle_mesh* m, double* scale);
 void oracle_get_csr(const oracle_mesh* m, int* outer, int* inner, double* val, double* rhs);
 

@@ -10,15 +10,15 @@ A gradient or matrix entry is a sum of terms from inertia, elements, contacts an
 which can cancel to far below every one of them, and each element's term is itself a sum
 of products that can cancel.  The oracle reports, per entry, the magnitude of what was
 summed into it (``grad_scale``: per term the magnitudes of the products inside an element
-force, or the length of a contact or inertia vector; ``val_scale``: per block the largest
-entry of the element or contact matrix).  A sum formed in another order, or with fused
+force, or the length of a contact or inertia vector; ``val_scale``: per block n times the
+largest entry of the element or contact matrix of order n, a bound of its 2-norm).  A sum formed in another order, or with fused
 multiply-adds, differs by up to a few hundred units in the last place of that magnitude.
 An entry is judged relative to its own magnitude, floored at ``scale_floor`` of that scale:
 1e-4 for the gradient (with a bar of 1e-10, an entry that cancels may differ by 1e-14 of
 what was summed) and 1e-3 for the matrix, whose element and contact blocks come out of
 eigendecompositions (the PSD projections, computed by different methods on the device and
-in the oracle) that are accurate to a few tens of units in the last place of the block's
-largest entry (1e-13).
+in the oracle) whose error is a small multiple of the unit roundoff times the block's
+2-norm (1e-13 of the scale).
 """
 import numpy as np
 
