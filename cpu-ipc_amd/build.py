@@ -63,7 +63,8 @@ def build(jobs=None, verbose=False):
     results += [_compile_host(src) for src in HOST_SOURCES]
     objs = [o for o, _ in results]
     tmp = LIB + ".tmp"
-    r = subprocess.run([HIPCC, "--offload-arch=" + ARCH, "-shared", "-o", tmp] + objs, capture_output=True, text=True)
+    r = subprocess.run([HIPCC, "--offload-arch=" + ARCH, "-shared", "-o", tmp] + objs + ["-lrocsolver", "-lrocblas"],
+                       capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError("link failed:\n" + r.stdout + r.stderr)
     os.replace(tmp, LIB)

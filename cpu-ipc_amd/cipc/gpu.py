@@ -96,8 +96,8 @@ class StepParams(C.Structure):
 class StepStats(C.Structure):
     _fields_ = [(n, C.c_int) for n in ("newton_iterations", "kappa_iterations", "energy_backtracks", "intersection_backtracks",
                                        "max_energy_backtracks_per_newton", "newton_steps_over_two_backtracks")] + \
-               [(n, C.c_int64) for n in ("pcg_iterations", "collisions", "matrix_nnz", "ground_contacts", "self_contacts",
-                                         "mollified_contacts")] + \
+               [(n, C.c_int64) for n in ("pcg_iterations", "collisions", "matrix_nnz", "direct_solves", "ground_contacts",
+                                         "self_contacts", "mollified_contacts")] + \
                [(n, C.c_double) for n in ("kappa", "min_distance2", "dirichlet_alpha", "alpha_sum", "alpha_min", "alpha_max",
                                           "step_ms", "assembly_ms", "linear_ms", "ccd_ms", "line_search_ms", "post_ms")]
 

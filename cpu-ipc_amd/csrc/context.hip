@@ -107,6 +107,7 @@ int cipc_gpu_context_destroy(cipc_gpu_context* c) {
         (void)hipStreamDestroy(c->aux);
         (void)hipStreamDestroy(c->stream);
         destroy_learned(*c);
+        destroy_direct(*c);
         delete c;  // DBuf members free on destruction of their owner only when released:
     });
 }
@@ -683,7 +684,8 @@ int cipc_gpu_solve_ex(cipc_gpu_context* c, int precision, int preconditioner, do
     return guard([&] {
         require(c && c->have_scene, "no scene uploaded");
         require(precision == 0 || precision == 1, "precision must be 0 (FP64) or 1 (FP32 with FP64 refinement)");
-        require(preconditioner >= 0 && preconditioner <= 2, "preconditioner must be 0 (block-Jacobi), 1 (block FSAI) or 2 (learned)");
+        require(preconditioner >= 0 && preconditioner <= 3,
+                "preconditioner must be 0 (block-Jacobi), 1 (block FSAI), 2 (learned) or 3 (direct)");
         require(std::isfinite(rel_tol) && rel_tol > 0.0 && max_iter > 0, "tolerance and iteration cap must be positive");
         require(c->assembled, "no assembled system: call cipc_gpu_assemble first");
         bind(*c);
@@ -1075,7 +1077,7 @@ int cipc_gpu_time_step(cipc_gpu_context* c, const cipc_step_params* p, double* k
         require(p->bbox_diag2 > 0 && p->mean_mass > 0 && p->newton_threshold > 0 && std::isfinite(*kappa) && *kappa >= 0,
                 "invalid step parameters");
         require(p->precision == 0 || p->precision == 1, "precision must be 0 (FP64) or 1 (FP32 with FP64 refinement)");
-        require(p->preconditioner >= 0 && p->preconditioner <= 2, "preconditioner must be 0, 1 or 2");
+        require(p->preconditioner >= 0 && p->preconditioner <= 3, "preconditioner must be 0, 1, 2 or 3 (direct)");
         require(p->solve_rel_tol > 0 && p->solve_max_iter > 0 && p->max_newton > 0, "solve tolerance and limits must be positive");
         DirichletHook hook;
         if (d && d->n > 0) {
@@ -1098,6 +1100,7 @@ int cipc_gpu_time_step(cipc_gpu_context* c, const cipc_step_params* p, double* k
             out->max_energy_backtracks_per_newton = st.max_energy_backtracks_per_newton;
             out->newton_steps_over_two_backtracks = st.newton_over_two_backtracks;
             out->pcg_iterations = st.pcg_iterations; out->collisions = st.collisions; out->matrix_nnz = st.matrix_nnz;
+            out->direct_solves = st.direct_solves;
             out->ground_contacts = st.ground_contacts; out->self_contacts = st.self_contacts;
             out->mollified_contacts = st.mollified_contacts;
             out->kappa = st.kappa; out->min_distance2 = st.min_distance2; out->dirichlet_alpha = st.dirichlet_alpha;

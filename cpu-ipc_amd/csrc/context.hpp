@@ -76,6 +76,8 @@ struct SolveBuffers {
     DBuf<int4> bl;
     DBuf<int> gb, ctl_i;
     DBuf<double> ctl_d;
+    DBuf<double> dense;                       // direct tier: the dense lower factor (n x n, column-major)
+    DBuf<int> info;
 };
 struct SolveResult {
     int iterations = 0, refinements = 0;
@@ -87,6 +89,7 @@ struct LearnedState;
 struct Context {
     int device = 0;
     LearnedState* learned = nullptr;    // learned preconditioner state (learned.hip)
+    void* blas = nullptr;               // rocBLAS handle of the direct tier (created on first use)
     hipStream_t stream = nullptr;
     // Elastic assembly depends on the state only; it runs on `aux` (default priority) while
     // `stream` builds the active set and the contact blocks (fork / join events).
@@ -302,7 +305,12 @@ __device__ inline void energy_block_sum(double v, double* part) {
 void friction_update(Context& c, double kappa);
 void build_solve_matrix(Context& c, bool fp32);
 // preconditioner: 0 block-Jacobi, 1 block FSAI, 2 learned factor
+// preconditioner kDirectSolve: the dense Cholesky direct solve (precision ignored)
+constexpr int kDirectSolve = 3;
+constexpr int kDirectMaxDofs = 60000;   // 28.8 GB of dense factor at the limit
 void linear_solve(Context& c, int precision, int preconditioner, double rel_tol, int max_iter, SolveResult& res);
+void direct_solve(Context& c, SolveResult& res);
+void destroy_direct(Context& c);
 void build_fsai(Context& c, bool fp32);
 void build_learned(Context& c, bool fp32);
 void learned_set_weights(Context& c, const float* w, int64_t n);
@@ -335,7 +343,7 @@ struct StepParams {
 struct StepStats {
     int newton_iterations = 0, kappa_iterations = 0;
     int energy_backtracks = 0, intersection_backtracks = 0, max_energy_backtracks_per_newton = 0, newton_over_two_backtracks = 0;
-    int64_t pcg_iterations = 0, collisions = 0, matrix_nnz = 0;
+    int64_t pcg_iterations = 0, collisions = 0, matrix_nnz = 0, direct_solves = 0;
     int64_t ground_contacts = 0, self_contacts = 0, mollified_contacts = 0;
     double kappa = 0, min_distance2 = 0, dirichlet_alpha = 1.0;
     double alpha_sum = 0, alpha_min = std::numeric_limits<double>::infinity(), alpha_max = 0;

@@ -19,6 +19,7 @@
 // partials in a fixed order (deterministic).  kCheck iterations are captured once into a
 // HIP graph and replayed; the host checks convergence after each replay and kernels
 // launched past convergence return at once.
+#include <rocsolver/rocsolver.h>
 #include "context.hpp"
 
 #include <hipcub/hipcub.hpp>
@@ -936,8 +937,89 @@ void linear_solve_scenes(Context& c, int precision, int preconditioner, double r
     CIPC_HIP(hipStreamSynchronize(c.stream));
 }
 
+// ---- direct tier ------------------------------------------------------------------------
+// The host direct solvers (PARDISO, CHOLMOD; NewtonLinearSystem.cpp:150-174) on the GPU: the
+// assembled lower CSC scattered into a dense column-major matrix, factored by a dense
+// Cholesky (rocSOLVER potrf) and solved (potrs); the FP64 residual of the solution is
+// evaluated on the BSR as for PCG.  For systems up to kDirectMaxDofs unknowns.
+__global__ void csc_lower_to_dense(int n, const int* __restrict__ outer, const int* __restrict__ inner, const double* __restrict__ val,
+                                   double* __restrict__ A) {
+    const int col = blockIdx.x * blockDim.x + threadIdx.x;
+    if (col >= n) return;
+    for (int k = outer[col]; k < outer[col + 1]; ++k) A[(size_t)col * n + inner[k]] = val[k];
+}
+__global__ void zero_dense(size_t n, double* __restrict__ A) {
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) A[i] = 0.0;
+}
+
+void direct_solve(Context& c, SolveResult& res) {
+    Scene& s = c.sc;
+    if (s.S != 1) throw Error(CIPC_ERR_LOGIC, "a batch is solved per scene");
+    const int V = s.V, n = 3 * V;
+    if (n > kDirectMaxDofs)
+        throw Error(CIPC_ERR_INVALID_ARGUMENT, "the direct solve takes at most " + std::to_string(kDirectMaxDofs) + " unknowns, the system has " +
+                                                   std::to_string(n));
+    SolveBuffers& w = c.solve;
+    w.dense.resize((size_t)n * n);
+    w.x.resize((size_t)n);
+    w.info.resize(1);
+    if (!c.blas) {
+        rocblas_handle h = nullptr;
+        if (rocblas_create_handle(&h) != rocblas_status_success) throw Error(CIPC_ERR_HIP, "rocblas_create_handle failed");
+        c.blas = h;
+    }
+    rocblas_handle h = static_cast<rocblas_handle>(c.blas);
+    if (rocblas_set_stream(h, c.stream) != rocblas_status_success) throw Error(CIPC_ERR_HIP, "rocblas_set_stream failed");
+    hipLaunchKernelGGL(zero_dense, dim3(4096), dim3(256), 0, c.stream, (size_t)n * n, w.dense.p);
+    hipLaunchKernelGGL(csc_lower_to_dense, dim3(blocks_for(n, 256)), dim3(256), 0, c.stream, n, c.outer.p, c.inner.p, c.val.p, w.dense.p);
+    CIPC_HIP(hipMemcpyAsync(w.x.p, c.rhs.p, (size_t)n * sizeof(double), hipMemcpyDeviceToDevice, c.stream));
+    CIPC_CHECK_LAUNCH();
+    if (rocsolver_dpotrf(h, rocblas_fill_lower, n, w.dense.p, n, w.info.p) != rocblas_status_success)
+        throw Error(CIPC_ERR_HIP, "rocsolver_dpotrf failed");
+    int info = 0;
+    read_back(c, &info, w.info.p, 1);
+    if (info != 0)
+        throw Error(CIPC_ERR_RUNTIME, "the Newton system is not positive definite (Cholesky pivot " + std::to_string(info) + ")");
+    if (rocsolver_dpotrs(h, rocblas_fill_lower, n, 1, w.dense.p, n, w.x.p, n) != rocblas_status_success)
+        throw Error(CIPC_ERR_HIP, "rocsolver_dpotrs failed");
+    // FP64 residual on the BSR of the same system
+    build_solve_matrix(c, false);
+    const Layout L = make_layout(c);
+    const int ng = L.ng;
+    const size_t ne = w.ne, nil = (ne + kGL - 1) / kGL * 9 * kGL;
+    w.vil.resize(nil);
+    w.r.resize((size_t)n);
+    w.part.resize((size_t)4 * ng);
+    w.scal.resize((sizeof(PcgScalars) + 7) / 8);
+    hipLaunchKernelGGL(interleave_blocks<double>, dim3(blocks_for(9 * ne, 256)), dim3(256), 0, c.stream, ne, w.val.p, w.vil.p);
+    double* part_rr = w.part.p + 2 * ng;
+    double* part_bb = w.part.p + 3 * ng;
+    PcgScalars* sc = reinterpret_cast<PcgScalars*>(w.scal.p);
+    hipLaunchKernelGGL(pcg_residual<double>, dim3(ng), dim3(kPB), 0, c.stream, L, w.rowptr.p, w.col.p, w.vil.p, c.rhs.p, w.x.p, 0, w.r.p,
+                       part_rr, part_bb);
+    hipLaunchKernelGGL(pcg_init_scalars, dim3(1), dim3(kPB), 0, c.stream, L, part_rr, part_rr, part_bb, 1.0, 0, SceneControl{}, sc);
+    CIPC_CHECK_LAUNCH();
+    std::vector<PcgScalars> hs;
+    read_scalars(c, sc, 1, hs);
+    res.iterations = 0;
+    res.refinements = 0;
+    res.rel_residual = hs[0].bn > 0 ? std::sqrt(hs[0].rr) / hs[0].bn : 0.0;
+    s.dir.resize((size_t)n);
+    CIPC_HIP(hipMemcpyAsync(s.dir.p, w.x.p, (size_t)n * sizeof(double), hipMemcpyDeviceToDevice, c.stream));
+    CIPC_HIP(hipStreamSynchronize(c.stream));
+}
+
+void destroy_direct(Context& c) {
+    if (c.blas) (void)rocblas_destroy_handle(static_cast<rocblas_handle>(c.blas));
+    c.blas = nullptr;
+}
+
 void linear_solve(Context& c, int precision, int preconditioner, double rel_tol, int max_iter, SolveResult& res) {
     if (c.sc.S != 1) throw Error(CIPC_ERR_LOGIC, "a batch is solved per scene");
+    if (preconditioner == kDirectSolve) {
+        direct_solve(c, res);
+        return;
+    }
     linear_solve_scenes(c, precision, preconditioner, rel_tol, max_iter, nullptr, &res);
 }
 

@@ -273,8 +273,15 @@ struct Stepper {
             SolveResult r;
             linear_solve(c, P.precision, P.preconditioner, P.solve_tol, P.solve_max_iter, r);
             st.pcg_iterations += r.iterations;
+            if (!(r.rel_residual <= P.solve_tol) && P.preconditioner != kDirectSolve && 3 * s.V <= kDirectMaxDofs) {
+                // the direct tier takes over where conjugate gradients stall
+                const int it = r.iterations;
+                direct_solve(c, r);
+                r.iterations = it;
+                ++st.direct_solves;
+            }
             if (!(r.rel_residual <= P.solve_tol))
-                throw Error(CIPC_ERR_RUNTIME, "conjugate gradients stopped after " + std::to_string(r.iterations) +
+                throw Error(CIPC_ERR_RUNTIME, "the Newton solve stopped after " + std::to_string(r.iterations) +
                                                   " iterations at relative residual " + std::to_string(r.rel_residual));
             double sc[2];
             search_scalars(sc);

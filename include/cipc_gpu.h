@@ -231,8 +231,12 @@ int cipc_gpu_solve(cipc_gpu_context* ctx, int precision, double rel_tol, int max
 /* The same solve with a choice of preconditioner: 0 block-Jacobi (as cipc_gpu_solve), 1 the
  * 3x3-block factorized sparse approximate inverse M^-1 = G^T G with G on the matrix's lower
  * block pattern (at most 24 blocks per row, the largest kept), 2 the learned factor of the
- * weights set with cipc_gpu_set_preconditioner_weights.  No counterpart in the reference,
- * whose iterative solver is Eigen CG with incomplete Cholesky (NewtonLinearSystem.cpp:176-207). */
+ * weights set with cipc_gpu_set_preconditioner_weights, 3 no preconditioner but the direct
+ * tier: the dense Cholesky factorisation of the system on the GPU (the host direct solvers
+ * PARDISO / CHOLMOD, NewtonLinearSystem.cpp:150-174; at most 60,000 unknowns, precision,
+ * rel_tol and max_iter ignored, CIPC_ERR_RUNTIME when the system is not positive definite).
+ * The iterative choices have no counterpart in the reference, whose iterative solver is Eigen
+ * CG with incomplete Cholesky (NewtonLinearSystem.cpp:176-207). */
 int cipc_gpu_solve_ex(cipc_gpu_context* ctx, int precision, int preconditioner, double rel_tol, int max_iter, double* direction,
                       cipc_solve_stats* stats);
 
@@ -296,11 +300,14 @@ typedef struct {
 } cipc_step_params;
 
 /* IPCStepStats (IPCSolver.h) of one time step.  Stage times are host wall-clock, as the
- * reference's StageTimer; pcg_iterations sums the Newton solves' iterations. */
+ * reference's StageTimer; pcg_iterations sums the Newton solves' iterations.  A solve whose
+ * PCG stops above solve_rel_tol is repeated by the direct tier (preconditioner 3) when the
+ * system has at most 60,000 unknowns. */
 typedef struct {
     int newton_iterations, kappa_iterations;
     int energy_backtracks, intersection_backtracks, max_energy_backtracks_per_newton, newton_steps_over_two_backtracks;
     int64_t pcg_iterations, collisions, matrix_nnz;
+    int64_t direct_solves;   /* Newton solves that fell back to the direct tier (PCG above tolerance) */
     int64_t ground_contacts, self_contacts, mollified_contacts;
     double kappa, min_distance2, dirichlet_alpha;
     double alpha_sum, alpha_min, alpha_max;
