@@ -37,7 +37,7 @@ def test_noalloc_energy_matches_energy_terms():
             got = out.cpu().numpy()
             print(s.name, got)
             assert np.array_equal(got, want), (s.name, got, want)
-            assert want[2] > 0 and want[3] > 0          # ground and self barriers are in it
+            assert want[2] > 0                           # the ground barrier is in it
         finally:
             g.close()
 
