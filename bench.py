@@ -158,7 +158,8 @@ def pcg_leg(ctx, kappa, tol=1e-6, max_iter=10000, repeats=3):
 def batch_leg(n_scenes=256, steps=5, seed=0):
     """The batched workload: n independent lattice cubes (8^3..24^3 cells, per-scene E, nu,
     drop height) in one batch context, each at its first Newton iteration (frozen state,
-    probe direction).  Times the batched Newton iteration (HIP events) and one per-scene
+    probe direction).  Times the batched Newton iteration (HIP events) with the default
+    full-CCD split and again after the autotuner has picked one, and one per-scene
     lock-step solve (FP32 PCG with FP64 refinement, block FSAI, 1e-6)."""
     import numpy as np
     from cipc import gpu, scenes
@@ -175,6 +176,8 @@ def batch_leg(n_scenes=256, steps=5, seed=0):
     b.newton_iteration(kappa)
     it = [b.newton_iteration(kappa) for _ in range(steps)]
     ms = float(np.mean([t["total_ms"] for t in it]))
+    tune = b.autotune()                               # full-CCD lanes per query, on this state
+    itt = [b.newton_iteration(kappa) for _ in range(steps)]
     b.set_hessian_precision(1)                        # the mixed form
     b.newton_iteration(kappa)
     itm = [b.newton_iteration(kappa) for _ in range(steps)]
@@ -187,6 +190,10 @@ def batch_leg(n_scenes=256, steps=5, seed=0):
            "scene_iterations_per_s": round(n_scenes / (ms * 1e-3), 1),
            "stages_ms": {k: round(float(np.mean([t[k] for t in it])), 3) for k in
                          ("broadphase_ms", "assembly_ms", "elastic_ms", "ccd_ms", "energy_ms")},
+           "autotuned": {"newton_iteration_ms": round(float(np.mean([t["total_ms"] for t in itt])), 3),
+                         "ccd_ms": round(float(np.mean([t["ccd_ms"] for t in itt])), 3),
+                         "lanes_per_query": {k: 1 << c for k, (c, _) in tune.items()},
+                         "candidate_ms": {k: [round(m, 3) for m in ms_] for k, (_, ms_) in tune.items()}},
            "mixed_fp32_hessian": {"newton_iteration_ms": round(float(np.mean([t["total_ms"] for t in itm])), 3),
                                   "elastic_ms": round(float(np.mean([t["elastic_ms"] for t in itm])), 3)},
            "contacts": int(it[-1]["contacts"]), "min_alpha": float(it[-1]["alpha"]),

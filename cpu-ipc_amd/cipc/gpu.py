@@ -35,6 +35,7 @@ EXPORTS = (
     "cipc_gpu_get_active_set", "cipc_gpu_set_active_set", "cipc_gpu_assemble", "cipc_gpu_set_hessian_precision",
     "cipc_gpu_get_gradient",
     "cipc_gpu_get_csr", "cipc_gpu_energy", "cipc_gpu_energy_terms", "cipc_gpu_energy_terms_noalloc", "cipc_gpu_step_bound",
+    "cipc_gpu_autotune", "cipc_gpu_set_tune",
     "cipc_gpu_friction_update", "cipc_gpu_get_friction", "cipc_gpu_set_soft_targets", "cipc_gpu_kappa_gradients",
     "cipc_gpu_close_constraints", "cipc_gpu_clear_close_constraints", "cipc_gpu_preconditioner_param_count",
     "cipc_gpu_set_preconditioner_weights", "cipc_gpu_preconditioner_gradient", "cipc_gpu_preconditioner_train_step", "cipc_gpu_solve", "cipc_gpu_solve_ex", "cipc_gpu_get_solve_factor", "cipc_gpu_save_positions", "cipc_gpu_step_forward",
@@ -102,6 +103,10 @@ class StepStats(C.Structure):
                                           "step_ms", "assembly_ms", "linear_ms", "ccd_ms", "line_search_ms", "post_ms")]
 
 
+class TuneEntry(C.Structure):
+    _fields_ = [("kernel", C.c_int), ("chosen", C.c_int), ("n_candidates", C.c_int), ("ms", C.c_double * 8)]
+
+
 DirichletFn = C.CFUNCTYPE(C.c_int, C.c_void_p, C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_double), C.POINTER(C.c_double),
                           C.c_int, C.c_double, C.c_double, C.POINTER(C.c_double))
 
@@ -160,6 +165,8 @@ def load(path: str = LIB_PATH):
         "cipc_gpu_get_solve_factor": [C.c_void_p, _i64p, _ip, _ip, _dp],
         "cipc_gpu_get_friction": [C.c_void_p, C.POINTER(C.c_int64), _dp, _dp, _dp, _dp],
         "cipc_gpu_step_bound": [C.c_void_p, _dp],
+        "cipc_gpu_autotune": [C.c_void_p, C.POINTER(TuneEntry), C.c_int, C.POINTER(C.c_int)],
+        "cipc_gpu_set_tune": [C.c_void_p, C.c_int, C.c_int],
         "cipc_gpu_save_positions": [C.c_void_p],
         "cipc_gpu_step_forward": [C.c_void_p, C.c_double],
         "cipc_gpu_is_intersected": [C.c_void_p, _ip],
@@ -451,6 +458,20 @@ class GpuContext:
         out = np.zeros(5)
         _check(load().cipc_gpu_step_bound(self.h, _dptr(out)))
         return dict(alpha=out[0], ground=out[1], partial=out[2], cfl=out[3], full=out[4])
+
+    def autotune(self, p=None):
+        """Times the full-CCD launch splits on the current state (direction p if given) and
+        keeps the fastest; returns {kernel: (chosen, [mean ms per candidate])}."""
+        if p is not None:
+            self.set_direction(p)
+        e = (TuneEntry * 2)()
+        n = C.c_int()
+        _check(load().cipc_gpu_autotune(self.h, e, 2, C.byref(n)))
+        names = ("full_ccd_pt", "full_ccd_ee")
+        return {names[e[k].kernel]: (e[k].chosen, list(e[k].ms[:e[k].n_candidates])) for k in range(n.value)}
+
+    def set_tune(self, kernel, choice):
+        _check(load().cipc_gpu_set_tune(self.h, kernel, choice))
 
     def save_positions(self):
         _check(load().cipc_gpu_save_positions(self.h))

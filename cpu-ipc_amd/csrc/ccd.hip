@@ -132,25 +132,26 @@ __global__ void partial_ccd_kernel(int n, const int2* __restrict__ pairs, const 
 }
 
 // Full CCD, point-triangle phase (ContactMechanics.cpp:3764-3816): each surface vertex is
-// split over kParts lanes, one subtree of the swept face tree each; the step is the min
-// over every lane, reduced across the wavefront before one atomic.
+// split over 2^pb lanes, one subtree of the swept face tree each; the step is the min
+// over every lane, reduced across the wavefront before one atomic.  The minimum does not
+// depend on the split, which the autotuner picks per context.
 __global__ __launch_bounds__(64) void full_ccd_pt_kernel(int Vs, const int* __restrict__ perm, const int* __restrict__ sv,
                                                           const int4* __restrict__ faces, const int* __restrict__ bt,
                                                           const double* __restrict__ x, const double* __restrict__ p,
                                                           const Box* __restrict__ qbox, TreeView tv,
                                                           unsigned long long* __restrict__ sl, double eta,
-                                                          const int* __restrict__ vs) {
+                                                          const int* __restrict__ vs, int pb) {
     __shared__ int stk[kStack * 64];
     const int t = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
     double best = 1.0;
     int scene = kNoScene;
-    if (t < Vs * kParts) {
-        const int s = perm[t / kParts], part = t % kParts;
+    if (t < (Vs << pb)) {
+        const int s = perm[t >> pb], part = t & ((1 << pb) - 1);
         const int vI = sv[s];
         scene = scene_of(vs, vI);
         const double a = bits_to_double(sl[kSlots * scene + 5]);
         const d3 X = ld3(x, vI), P = ld3(p, vI);
-        const Box pb = swept_pt(X, P, a);
+        const Box vb = swept_pt(X, P, a);
         part_walk(tv, part, qbox[s], 0.0, stk + threadIdx.x, [&](int f) {
             const int4 fv = faces[f];
             if (is_collider(bt[vI]) && is_collider(bt[fv.x]) && is_collider(bt[fv.y]) && is_collider(bt[fv.z])) return;
@@ -159,12 +160,12 @@ __global__ __launch_bounds__(64) void full_ccd_pt_kernel(int Vs, const int* __re
             const d3 x0 = ld3(x, fv.x), x1 = ld3(x, fv.y), x2 = ld3(x, fv.z);
             const d3 p0 = ld3(p, fv.x), p1 = ld3(p, fv.y), p2 = ld3(p, fv.z);
             const Box fb = box_union(box_union(swept_pt(x0, p0, a), swept_pt(x1, p1, a)), swept_pt(x2, p2, a));
-            if (!overlaps(pb, fb, 0.0)) return;
+            if (!overlaps(vb, fb, 0.0)) return;
             // capped at the lane's best so far, itself at most the seeded step a: the
             // minimum over all pairs is unchanged
             const double la = accd_pt(X, x0, x1, x2, -P, -p0, -p1, -p2, eta, 0.0, fmin(best, a));
             best = fmin(la, best);
-        }, [] {});
+        }, [] {}, pb);
     }
     scene_atomic<false>(sl, scene, 3, best);
 }
@@ -173,17 +174,18 @@ __global__ __launch_bounds__(64) void full_ccd_pt_kernel(int Vs, const int* __re
 // surface edges whose swept boxes (built with the incoming step) overlap, the pair ordered as
 // the reference orders it (I the larger edge id, J the smaller) and filtered with the exact
 // swept boxes of the point-triangle-reduced step read from the slots.  Each edge (split over
-// kParts lanes) walks only the leaves after its own leaf, so each pair is met once.
+// 2^pb lanes) walks only the leaves after its own leaf, so each pair is met once.
 __global__ __launch_bounds__(64) void full_ccd_ee_kernel(int Es, const int* __restrict__ perm, const int2* __restrict__ se,
                                                           const int* __restrict__ bt, const double* __restrict__ x,
                                                           const double* __restrict__ p, TreeView tv, const Box* __restrict__ ebox,
-                                                          unsigned long long* __restrict__ sl, double eta, const int* __restrict__ vs) {
+                                                          unsigned long long* __restrict__ sl, double eta, const int* __restrict__ vs,
+                                                          int pb) {
     __shared__ int stk[kStack * 64];
     const int t = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
     double best = 1.0;
     int scene = kNoScene;
-    if (t < Es * kParts) {
-        const int eQ = perm[t / kParts], part = t % kParts;
+    if (t < (Es << pb)) {
+        const int eQ = perm[t >> pb], part = t & ((1 << pb) - 1);
         const int2 Q = se[eQ];
         scene = scene_of(vs, Q.x);
         const double a = bits_to_double(sl[kSlots * scene + 4]);
@@ -207,7 +209,7 @@ __global__ __launch_bounds__(64) void full_ccd_ee_kernel(int Es, const int* __re
             const d3 j0 = ld3(x, J.x), j1 = ld3(x, J.y), pj0 = ld3(p, J.x), pj1 = ld3(p, J.y);
             const double la = accd_ee(i0, i1, j0, j1, -pi0, -pi1, -pj0, -pj1, eta, 0.0, fmin(best, a));
             best = fmin(la, best);
-        }, [] {});
+        }, [] {}, pb);
     }
     scene_atomic<false>(sl, scene, 3, best);
 }
@@ -268,15 +270,85 @@ void step_bound_early(Context& c, hipStream_t st, bool ground) {
     LBVH& ft = c.sface_tree;
     LBVH& et = c.sedge_tree;
     const TreeView ftv{ft.n, ft.nodes.p, ft.wide.p}, etv{et.n, et.nodes.p, et.wide.p, et.leafpos.p};
-    if (s.Vs > 0 && ft.n > 0)
-        hipLaunchKernelGGL(full_ccd_pt_kernel, dim3(blocks_for((size_t)s.Vs * kParts, 64)), dim3(64), 0, st, s.Vs, s.sv_perm.p,
-                           s.sv.p, s.faces.p, s.bt.p, s.x.p, s.dir.p, c.swept_point_box.p, ftv, sl.p, eta, vs);
+    if (s.Vs > 0 && ft.n > 0) launch_full_ccd_pt(c, st, c.tune.ccd_pt_bits);
     if (s.Es > 0 && et.n > 0) {
         hipLaunchKernelGGL(copy_pt_result, dim3(blocks_for(S, 64)), dim3(64), 0, st, S, sl.p);
-        hipLaunchKernelGGL(full_ccd_ee_kernel, dim3(blocks_for((size_t)s.Es * kParts, 64)), dim3(64), 0, st, s.Es, s.se_perm.p,
-                           s.se.p, s.bt.p, s.x.p, s.dir.p, etv, et.prim.p, sl.p, eta, vs);
+        launch_full_ccd_ee(c, st, c.tune.ccd_ee_bits);
     }
     CIPC_CHECK_LAUNCH();
+}
+
+// The two full-CCD walks with each surface vertex / edge split over 2^pb lanes.
+void launch_full_ccd_pt(Context& c, hipStream_t st, int pb) {
+    Scene& s = c.sc;
+    const LBVH& ft = c.sface_tree;
+    const TreeView ftv{ft.n, ft.nodes.p, ft.wide.p};
+    hipLaunchKernelGGL(full_ccd_pt_kernel, dim3(blocks_for((size_t)s.Vs << pb, 64)), dim3(64), 0, st, s.Vs, s.sv_perm.p, s.sv.p,
+                       s.faces.p, s.bt.p, s.x.p, s.dir.p, c.swept_point_box.p, ftv, c.u64scratch.p, 1.0 - 0.8, scene_of_vertex(s), pb);
+}
+void launch_full_ccd_ee(Context& c, hipStream_t st, int pb) {
+    Scene& s = c.sc;
+    const LBVH& et = c.sedge_tree;
+    const TreeView etv{et.n, et.nodes.p, et.wide.p, et.leafpos.p};
+    hipLaunchKernelGGL(full_ccd_ee_kernel, dim3(blocks_for((size_t)s.Es << pb, 64)), dim3(64), 0, st, s.Es, s.se_perm.p, s.se.p, s.bt.p,
+                       s.x.p, s.dir.p, etv, et.prim.p, c.u64scratch.p, 1.0 - 0.8, scene_of_vertex(s), pb);
+}
+
+// ---- autotuner ----------------------------------------------------------------------
+__global__ void copy_slot(int S, int dst, int src, unsigned long long* sl) {
+    const int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k < S) sl[kSlots * k + dst] = sl[kSlots * k + src];
+}
+
+// Times every split of the two full-CCD walks (1 to 16 lanes per query) on the current
+// positions and direction, three runs each after one warm-up, with HIP events on the
+// context's stream, and keeps the fastest for the context's later step bounds.  Each run
+// starts from the minimum the walk starts from in a step bound (slot 3 reseeded from the
+// swept step, slot 5, for the point-triangle walk and from the point-triangle result, slot
+// 4, for the edge-edge walk), so the pruning is the one a step bound sees.  Each walk's
+// result is a minimum over pairs, so every split gives the same step.
+int autotune(Context& c, TuneEntry* out, int max_out) {
+    Scene& s = c.sc;
+    const int S = s.S;
+    if (!s.dir.p) throw Error(CIPC_ERR_LOGIC, "the autotuner times the step bound: set a direction first");
+    step_bound_early(c, c.stream, true);        // trees, seeded slots, swept boxes
+    CIPC_HIP(hipStreamSynchronize(c.stream));
+    hipEvent_t e0, e1;
+    CIPC_HIP(hipEventCreate(&e0));
+    CIPC_HIP(hipEventCreate(&e1));
+    int n = 0;
+    for (int k = 0; k < 2; ++k) {
+        const bool pt = k == 0;
+        if (pt ? (s.Vs == 0 || c.sface_tree.n == 0) : (s.Es == 0 || c.sedge_tree.n == 0)) continue;
+        TuneEntry e{};
+        e.kernel = k;
+        e.n = kTuneSplits;
+        double best = 1e300;
+        for (int pb = 0; pb < kTuneSplits; ++pb) {
+            float total = 0.f;
+            for (int r = 0; r < 4; ++r) {
+                hipLaunchKernelGGL(copy_slot, dim3(blocks_for(S, 64)), dim3(64), 0, c.stream, S, 3, pt ? 5 : 4, c.u64scratch.p);
+                CIPC_HIP(hipEventRecord(e0, c.stream));
+                if (pt) launch_full_ccd_pt(c, c.stream, pb);
+                else launch_full_ccd_ee(c, c.stream, pb);
+                CIPC_HIP(hipEventRecord(e1, c.stream));
+                CIPC_CHECK_LAUNCH();
+                CIPC_HIP(hipEventSynchronize(e1));
+                float ms = 0.f;
+                CIPC_HIP(hipEventElapsedTime(&ms, e0, e1));
+                if (r > 0) total += ms;
+            }
+            e.ms[pb] = total / 3.0;
+            if (e.ms[pb] < best) { best = e.ms[pb]; e.chosen = pb; }
+        }
+        if (pt) c.tune.ccd_pt_bits = e.chosen;
+        else c.tune.ccd_ee_bits = e.chosen;
+        if (n < max_out && out) out[n] = e;
+        ++n;
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    return n;
 }
 void step_bound_partial(Context& c, hipStream_t st) {
     Scene& s = c.sc;

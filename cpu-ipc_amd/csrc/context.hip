@@ -961,6 +961,33 @@ int cipc_gpu_step_bound(cipc_gpu_context* c, double out[5]) {
     });
 }
 
+int cipc_gpu_autotune(cipc_gpu_context* c, cipc_tune_entry* entries, int max_entries, int* n_entries) {
+    return guard([&] {
+        require(c && c->have_scene, "no scene uploaded");
+        require(max_entries >= 0 && (max_entries == 0 || entries), "null entries");
+        bind(*c);
+        TuneEntry e[2];
+        const int n = autotune(*c, e, 2);
+        for (int k = 0; k < n && k < max_entries; ++k) {
+            entries[k] = cipc_tune_entry{};
+            entries[k].kernel = e[k].kernel;
+            entries[k].chosen = e[k].chosen;
+            entries[k].n_candidates = e[k].n;
+            for (int j = 0; j < 8; ++j) entries[k].ms[j] = e[k].ms[j];
+        }
+        if (n_entries) *n_entries = n;
+    });
+}
+
+int cipc_gpu_set_tune(cipc_gpu_context* c, int kernel, int choice) {
+    return guard([&] {
+        require(c != nullptr, "null context");
+        require(kernel == 0 || kernel == 1, "kernel is 0 (point-triangle CCD) or 1 (edge-edge CCD)");
+        require(choice >= 0 && choice < kTuneSplits, "choice is 0..4 (1 to 16 lanes per query)");
+        (kernel == 0 ? c->tune.ccd_pt_bits : c->tune.ccd_ee_bits) = choice;
+    });
+}
+
 int cipc_gpu_save_positions(cipc_gpu_context* c) {
     return guard([&] {
         require(c && c->have_scene, "no scene uploaded");

@@ -103,6 +103,10 @@ struct Context {
     hipStream_t sbs = nullptr;
     hipEvent_t as_done = nullptr, sb_done = nullptr;
     DBuf<unsigned char> tmp_sb;
+    // tile choices of the autotuner (ccd.hip): lanes per query of the full-CCD walks (2^bits)
+    struct Tune {
+        int ccd_pt_bits = 3, ccd_ee_bits = 3;
+    } tune;
     Scene sc;
     StaticPattern sp;
     bool have_scene = false;
@@ -261,6 +265,16 @@ void step_bound_launch(Context& c, hipStream_t st);
 void step_bound_early(Context& c, hipStream_t st, bool ground = true);
 void step_bound_partial(Context& c, hipStream_t st);
 void step_bound_finish(Context& c, double* out);
+void launch_full_ccd_pt(Context& c, hipStream_t st, int pb);
+void launch_full_ccd_ee(Context& c, hipStream_t st, int pb);
+// the autotuner: one entry per tuned kernel (0 full CCD point-triangle, 1 edge-edge), the
+// mean ms of each candidate split and the chosen one; returns the number of entries
+constexpr int kTuneSplits = 5;     // 1, 2, 4, 8 or 16 lanes per query
+struct TuneEntry {
+    int kernel = 0, chosen = 0, n = 0;
+    double ms[8] = {};
+};
+int autotune(Context& c, TuneEntry* out, int max_out);
 // Self_largestFeasibleStepSize_CCD alone (FeasibleStep.cpp), seeded with step 1 and
 // slackness 0.8, over the current direction: the animated Dirichlet update's bound
 double full_ccd_bound(Context& c);

@@ -56,11 +56,11 @@ __device__ inline void lbvh_visit_range(int n, const Node* __restrict__ nd, int 
         }
     }
 }
-// Root of part `part` (in [0, kParts)), or -1 for an empty part.
-__device__ inline int part_root(const TreeView& t, int part) {
+// Root of part `part` (in [0, 2^bits)), or -1 for an empty part.
+__device__ inline int part_root(const TreeView& t, int part, int bits = kPartBits) {
     if (t.n <= 0) return -1;
     int node = 0;
-    for (int l = kPartBits - 1; l >= 0; --l) {
+    for (int l = bits - 1; l >= 0; --l) {
         if (node >= t.n - 1) return (part & ((1 << (l + 1)) - 1)) ? -1 : node;
         node = ((part >> l) & 1) ? t.nodes[node].right : t.nodes[node].left;
     }
@@ -155,8 +155,9 @@ __device__ inline bool wide_visit_after(const TreeView& t, int start, const Box&
 // exact binary walk of the same subtree.  Deterministic, so a second call returns the same
 // hits in the same order.
 template <typename F, typename R>
-__device__ inline void part_walk(const TreeView& t, int part, const Box& q, double gap, int* __restrict__ stk, F&& hit, R&& reset) {
-    const int root = part_root(t, part);
+__device__ inline void part_walk(const TreeView& t, int part, const Box& q, double gap, int* __restrict__ stk, F&& hit, R&& reset,
+                                 int bits = kPartBits) {
+    const int root = part_root(t, part, bits);
     if (root < 0) return;
     if (wide_visit(t, root, q, gap, stk, hit)) return;
     reset();
@@ -165,8 +166,8 @@ __device__ inline void part_walk(const TreeView& t, int part, const Box& q, doub
 // part_walk over the leaves after position `after` only (wide_visit_after)
 template <typename F, typename R>
 __device__ inline void part_walk_after(const TreeView& t, int part, const Box& q, double gap, int after, int* __restrict__ stk,
-                                       F&& hit, R&& reset) {
-    const int root = part_root(t, part);
+                                       F&& hit, R&& reset, int bits = kPartBits) {
+    const int root = part_root(t, part, bits);
     if (root < 0) return;
     if (wide_visit_after(t, root, q, gap, after, stk, hit)) return;
     reset();

@@ -267,6 +267,22 @@ int cipc_gpu_get_friction(cipc_gpu_context* ctx, int64_t counts[2], double* lam,
  * out = {alpha, ground bound, partial CCD, CFL, full CCD}. */
 int cipc_gpu_step_bound(cipc_gpu_context* ctx, double out[5]);
 
+/* The launch-shape autotuner.  No reference counterpart: the reference's CCD loops are CPU
+ * loops (ContactMechanics.cpp:3754-3867) with no launch shape to choose.  It times each
+ * candidate split of the two full-CCD walks of the step bound (kernel 0 point-triangle,
+ * 1 edge-edge; candidate c runs 2^c lanes per query, c = 0..4) on the context's current
+ * positions and direction (cipc_gpu_set_direction; a batch context times its whole batch),
+ * and keeps the fastest for every later step bound of the context.  Every split returns
+ * the same step bound, bit for bit: each walk's result is a minimum over pairs.  entries
+ * (may be NULL) receives up to max_entries records, n_entries the number of tuned kernels.
+ * cipc_gpu_set_tune sets a choice directly (the default is 3, 8 lanes per query). */
+typedef struct {
+    int kernel, chosen, n_candidates;
+    double ms[8]; /* mean ms of each candidate over three timed runs */
+} cipc_tune_entry;
+int cipc_gpu_autotune(cipc_gpu_context* ctx, cipc_tune_entry* entries, int max_entries, int* n_entries);
+int cipc_gpu_set_tune(cipc_gpu_context* ctx, int kernel, int choice);
+
 /* Line-search support: remember the current positions as x0, x = x0 - alpha p on free
  * vertices (stepForward, IPCSolver.cpp:458-473), and isIntersected (IPCSolver.cpp:523-542)
  * with a static LBVH rebuilt on the current positions. */
