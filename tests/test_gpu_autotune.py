@@ -58,9 +58,9 @@ def test_batch_tuned_step_bound_unchanged(gpu):
     sc = []
     for n, seed in ((3, 0), (5, 2), (2, 4)):
         sc.append(scenes.stacked_cubes(2, n, 0.1, 1e-3, seed=seed, rotate="yaw", velocity_scale=0.5, name=f"b{n}"))
-    hh = max(s.hhat for s in sc)
+    hh, fh, dt = max(s.hhat for s in sc), max(s.fhat for s in sc), max(s.dtol for s in sc)
     for s in sc:
-        s.hhat = hh
+        s.hhat, s.fhat, s.dtol = hh, fh, dt
     probes = [scenes.newton_probe_state(s, seed=3 + k) for k, s in enumerate(sc)]
     b = gpu.BatchContext(sc, xtildes=[xt for xt, _ in probes])
     try:
