@@ -301,6 +301,7 @@ __global__ void wide_records(int n, const Node* __restrict__ nodes, const int* _
 
 // Topology from the current primitive boxes.
 void build_topology(Context& c, LBVH& t, int n, hipStream_t st, DBuf<unsigned char>& tmp) {
+    ++c.topo_gen;
     t.n = n;
     t.age = 0;
     const int nodes = 2 * n - 1;
@@ -474,10 +475,10 @@ __global__ __launch_bounds__(64) void pt_traverse(int Vs, const int* __restrict_
 
 __global__ __launch_bounds__(64) void pt_compact(int Vs, const int* __restrict__ perm, const int* __restrict__ sv, const double* __restrict__ x, TreeView tv,
                            double sq, const int* __restrict__ off, const int* __restrict__ cntp, const int* __restrict__ buf,
-                           int* __restrict__ candU, int* __restrict__ own) {
+                           int* __restrict__ candU, int* __restrict__ own, const int* __restrict__ dc) {
     __shared__ int stk[kStack * 64];
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= Vs) return;
+    if (t >= Vs || dc[kDcFlags]) return;   // over capacity: the host builds the set again
     const int s = perm[t];
     const int o = off[s];
     int pos = o;
@@ -496,12 +497,12 @@ __global__ __launch_bounds__(64) void pt_compact(int Vs, const int* __restrict__
 }
 
 // Point-triangle hits: filters, then classification (CollisionBroadPhase.cpp:765-857).
-__global__ void pt_classify(int n, int F, const int* __restrict__ own, const int* __restrict__ off, const int* __restrict__ sv,
+__global__ void pt_classify(const int* __restrict__ dc, int F, const int* __restrict__ own, const int* __restrict__ off, const int* __restrict__ sv,
                             const double* __restrict__ x, const int* __restrict__ bt, const int4* __restrict__ faces,
                             const int* __restrict__ candU, int* __restrict__ cand, double hhat,
                             double sq, const int* __restrict__ vs, int4* __restrict__ rec, int* __restrict__ cat) {
     const int u = blockIdx.x * blockDim.x + threadIdx.x;
-    if (u >= n) return;
+    if (dc[kDcFlags] || u >= dc[kDcPT]) return;
     // sorted position: the hit's rank among its query's hits (all distinct)
     const int q = own[u], o = off[q], e = off[q + 1];
     const int f = candU[u];
@@ -572,10 +573,10 @@ __global__ __launch_bounds__(64) void ee_traverse(int Es, const int* __restrict_
 
 __global__ __launch_bounds__(64) void ee_compact(int Es, const int* __restrict__ perm, const int2* __restrict__ se, const double* __restrict__ x, TreeView tv,
                            double sq, const int* __restrict__ off, const int* __restrict__ cntp, const int* __restrict__ buf,
-                           int* __restrict__ candU, int* __restrict__ own) {
+                           int* __restrict__ candU, int* __restrict__ own, const int* __restrict__ dc) {
     __shared__ int stk[kStack * 64];
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= Es) return;
+    if (t >= Es || dc[kDcFlags]) return;
     const int eI = perm[t];
     const int o = off[eI];
     int pos = o;
@@ -597,13 +598,13 @@ __global__ __launch_bounds__(64) void ee_compact(int Es, const int* __restrict__
 }
 
 // Edge-edge hits: filters, then classification (CollisionBroadPhase.cpp:858-1052).
-__global__ void ee_classify(int n, int Es, const int* __restrict__ own, const int* __restrict__ off, const int2* __restrict__ se,
+__global__ void ee_classify(const int* __restrict__ dc, int Es, const int* __restrict__ own, const int* __restrict__ off, const int2* __restrict__ se,
                             const double* __restrict__ x, const double* __restrict__ vrest, const int* __restrict__ bt,
                             const int* __restrict__ candU, int* __restrict__ cand, double hhat,
                             double sq, const int* __restrict__ vs, int4* __restrict__ rec, int2* __restrict__ aux,
                             int* __restrict__ cat) {
     const int u = blockIdx.x * blockDim.x + threadIdx.x;
-    if (u >= n) return;
+    if (dc[kDcFlags] || u >= dc[kDcEE]) return;
     const int eI = own[u], o = off[eI], e = off[eI + 1];
     const int eJ = candU[u];
     int k = o;
@@ -680,10 +681,11 @@ struct Cnt4Sum {
 };
 struct CatFlags {
     const int* cat;
-    int n;
+    const int* dc;   // the hit count is dc[which]; nothing is counted over capacity
+    int which;
     unsigned m0, m1, m2, m3;
     __host__ __device__ int4 operator()(int i) const {
-        if (i >= n) return make_int4(0, 0, 0, 0);
+        if (dc[kDcFlags] || i >= dc[which]) return make_int4(0, 0, 0, 0);
         const unsigned bit = 1u << cat[i];
         return make_int4((m0 & bit) ? 1 : 0, (m1 & bit) ? 1 : 0, (m2 & bit) ? 1 : 0, (m3 & bit) ? 1 : 0);
     }
@@ -694,11 +696,11 @@ __device__ inline unsigned long long map_key(int4 r, int V, int bits) {
     return ((unsigned long long)(r.x + V) << (2 * bits)) | ((unsigned long long)r.y << bits) | (unsigned long long)(r.z + 1);
 }
 // Point-triangle hits -> Self_ActiveSet head (a), map keys (b), partial-CCD pairs (c).
-__global__ void scatter_pt(int n, const int* __restrict__ cat, const int4* __restrict__ pos, const int4* __restrict__ rec,
+__global__ void scatter_pt(const int* __restrict__ dc, const int* __restrict__ cat, const int4* __restrict__ pos, const int4* __restrict__ rec,
                            const int* __restrict__ own, const int* __restrict__ cand, int V, int bits, int4* __restrict__ self_set,
                            unsigned long long* __restrict__ mkeys, int2* __restrict__ ccd) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
+    if (dc[kDcFlags] || i >= dc[kDcPT]) return;
     const int ct = cat[i];
     if (ct == kDrop) return;
     const int4 p = pos[i];
@@ -707,12 +709,13 @@ __global__ void scatter_pt(int n, const int* __restrict__ cat, const int4* __res
     else if (ct == kMap) mkeys[p.y] = map_key(rec[i], V, bits);
 }
 // Edge-edge hits -> Self_ActiveSet (a), map keys (b), Self_EE_ActiveSet + eIe (c), CCD pairs (d).
-__global__ void scatter_ee(int n, const int* __restrict__ cat, const int4* __restrict__ pos, const int4* __restrict__ rec,
+__global__ void scatter_ee(const int* __restrict__ dc, const int* __restrict__ cat, const int4* __restrict__ pos, const int4* __restrict__ rec,
                            const int2* __restrict__ aux, const int* __restrict__ own, const int* __restrict__ cand, int V, int bits,
-                           int base_self, int base_map, int base_ccd, int4* __restrict__ self_set, unsigned long long* __restrict__ mkeys,
+                           int4* __restrict__ self_set, unsigned long long* __restrict__ mkeys,
                            int4* __restrict__ ee_set, int2* __restrict__ ee_ie, int2* __restrict__ ccd) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
+    if (dc[kDcFlags] || i >= dc[kDcEE]) return;
+    const int base_self = dc[kDcA], base_map = dc[kDcA2], base_ccd = dc[kDcAK];
     const int ct = cat[i];
     if (ct == kDrop) return;
     const int4 p = pos[i];
@@ -721,10 +724,41 @@ __global__ void scatter_ee(int n, const int* __restrict__ cat, const int4* __res
     else if (ct == kMap) mkeys[base_map + p.y] = map_key(rec[i], V, bits);
     else if (ct == kMolPE || ct == kMolEE) { ee_set[p.z] = rec[i]; ee_ie[p.z] = aux[i]; }
 }
-__global__ void map_decode(int n, const unsigned long long* __restrict__ keys, const int* __restrict__ counts, int V, int bits,
-                           int base, int4* __restrict__ out) {
+// Counts after the category scans: the per-category totals are the scans' values at the hit
+// counts, the ground count the ground scan's total.  The map keys fill [0, nm) of a
+// buffer of capM; past nm they are the all-ones key, which sorts last.
+__global__ void category_totals(int Vs, const int4* __restrict__ posPT, const int4* __restrict__ posEE, const int* __restrict__ pg,
+                                int capM, int* __restrict__ dc) {
+    if (blockIdx.x != 0 || threadIdx.x != 0 || dc[kDcFlags]) return;
+    const int4 a = posPT[dc[kDcPT]], b = posEE[dc[kDcEE]];
+    const int nm = a.y + b.y;
+    dc[kDcMapKeys] = nm;
+    if (nm > capM) { dc[kDcFlags] |= 4; return; }
+    // the output counts are published only for a set that fits: over capacity they stay zero,
+    // so that later stages read no entry of an unfinished set
+    dc[kDcA] = a.x; dc[kDcA2] = a.y; dc[kDcAK] = a.z;
+    dc[kDcB] = b.x; dc[kDcB2] = b.y; dc[kDcMol] = b.z; dc[kDcBK] = b.w;
+    dc[kDcCCD] = a.z + b.w;
+    dc[kDcEnv] = pg[Vs];
+}
+__global__ void map_tail(int capM, const int* __restrict__ dc, unsigned long long* __restrict__ mkeys) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
+    if (i >= capM || dc[kDcFlags]) return;
+    if (i >= dc[kDcMapKeys]) mkeys[i] = ~0ull;
+}
+// After the run-length encoding of the sorted keys: the runs without the all-ones run (there
+// is one exactly when nm < capM) are the merged map entries.
+__global__ void map_total(int capM, const int* __restrict__ nruns, int* __restrict__ dc) {
+    if (blockIdx.x != 0 || threadIdx.x != 0 || dc[kDcFlags]) return;
+    const int nM = *nruns - (dc[kDcMapKeys] < capM ? 1 : 0);
+    dc[kDcSelf] = dc[kDcA] + dc[kDcB] + nM;
+}
+__global__ void map_decode(const int* __restrict__ dc, const unsigned long long* __restrict__ keys, const int* __restrict__ counts, int V, int bits,
+                           int4* __restrict__ out) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (dc[kDcFlags]) return;
+    const int base = dc[kDcA] + dc[kDcB];
+    if (i >= dc[kDcSelf] - base) return;
     const unsigned long long k = keys[i], m = (1ull << bits) - 1;
     out[base + i] = make_int4((int)((k >> (2 * bits)) & m) - V, (int)((k >> bits) & m), (int)(k & m) - 1, -counts[i]);
 }
@@ -743,35 +777,53 @@ __global__ void ground_scatter(int Vs, const int* __restrict__ sv, const int* __
     out[pos[s]] = sv[s];
 }
 
-namespace {
-// The active set's counts come back through pinned host memory: a device-to-host copy into
-// pageable memory is staged by the runtime and costs several times the copy itself.
-int read_int(Context& c, const int* p) {
-    c.pin_as.reserve(1);
-    CIPC_HIP(hipMemcpyAsync(c.pin_as.p, p, sizeof(int), hipMemcpyDeviceToHost, c.stream));
-    CIPC_HIP(hipStreamSynchronize(c.stream));
-    return c.pin_as.p[0];
+__global__ void dc_reset(int* __restrict__ dc) {
+    if (threadIdx.x < kDcWords) dc[threadIdx.x] = 0;
 }
-}  // namespace
+// The candidate totals (the per-query scans' last entries) against their capacities.
+__global__ void candidate_totals(int Vs, int Es, const int* __restrict__ offPT, const int* __restrict__ offEE, int capPT, int capEE,
+                                 int* __restrict__ dc) {
+    if (blockIdx.x != 0 || threadIdx.x != 0) return;
+    const int nPT = offPT[Vs], nEE = offEE[Es];
+    dc[kDcPT] = nPT;
+    dc[kDcEE] = nEE;
+    dc[kDcFlags] = (nPT > capPT ? 1 : 0) | (nEE > capEE ? 2 : 0);
+}
 
-void category_scan(Context& c, const int* cat, int n, unsigned m0, unsigned m1, unsigned m2, unsigned m3, DBuf<int4>& pos) {
+// Exclusive scan of the four category flags of hits [0, dc[which]) over a buffer of n hits
+// (flags past the count are zero, so pos[count] holds the totals).
+void category_scan(Context& c, const int* cat, int n, const int* dc, int which, unsigned m0, unsigned m1, unsigned m2, unsigned m3,
+                   DBuf<int4>& pos) {
     pos.resize((size_t)n + 1);
     hipcub::CountingInputIterator<int> idx(0);
-    hipcub::TransformInputIterator<int4, CatFlags, hipcub::CountingInputIterator<int>> it(idx, CatFlags{cat, n, m0, m1, m2, m3});
+    hipcub::TransformInputIterator<int4, CatFlags, hipcub::CountingInputIterator<int>> it(idx, CatFlags{cat, dc, which, m0, m1, m2, m3});
     size_t bytes = 0;
     CIPC_HIP(hipcub::DeviceScan::ExclusiveScan(nullptr, bytes, it, pos.p, Cnt4Sum(), make_int4(0, 0, 0, 0), n + 1, c.stream));
     c.tmp.reserve(bytes);
     CIPC_HIP(hipcub::DeviceScan::ExclusiveScan(c.tmp.p, bytes, it, pos.p, Cnt4Sum(), make_int4(0, 0, 0, 0), n + 1, c.stream));
 }
 
+// Buffers of the hits and of the outputs are sized by the capacities, so no stage needs a
+// count on the host: every kernel reads its count from dcnt, and does nothing once a
+// capacity is exceeded (the host then grows it and builds the set again).  Bounds: a
+// category of hits is at most the hits of its type, so the outputs are bounded by the hit
+// capacities (Self_ActiveSet head nA + nB, map tail nM <= nm <= capM, Self_EE_ActiveSet
+// nE <= nEE, CCD pairs nPTk + nEEk <= nPT + nEE, ground nG <= Vs).
 void compute_active_set(Context& c) {
     Scene& s = c.sc;
     build_static_trees(c);
     ActiveSetScratch& b = c.as;
     const double sq = sqrt(s.hhat);
     const int Vs = s.Vs, Es = s.Es;
+    if (b.capPT == 0) b.capPT = std::max(1024, 4 * Vs);
+    if (b.capEE == 0) b.capEE = std::max(1024, 4 * Es);
+    if (b.capM == 0) b.capM = 1024;
+    const int capPT = b.capPT, capEE = b.capEE, capM = b.capM;
     const TreeView ft{c.face_tree.n, c.face_tree.nodes.p, c.face_tree.wide.p};
     const TreeView et{c.edge_tree.n, c.edge_tree.nodes.p, c.edge_tree.wide.p};
+    c.dcnt.resize(kDcWords);
+    int* dc = c.dcnt.p;
+    hipLaunchKernelGGL(dc_reset, dim3(1), dim3(64), 0, c.stream, dc);
     b.cntPT.resize(Vs + 1); b.offPT.resize(Vs + 1); b.cntEE.resize(Es + 1); b.offEE.resize(Es + 1);
     b.cntpPT.resize((size_t)kParts * std::max(Vs, 1)); b.cntpEE.resize((size_t)kParts * std::max(Es, 1));
     b.bufPT.resize((size_t)kParts * kPartCap * std::max(Vs, 1)); b.bufEE.resize((size_t)kParts * kPartCap * std::max(Es, 1));
@@ -788,85 +840,110 @@ void compute_active_set(Context& c) {
     if (Es == 0) hipLaunchKernelGGL(fill_tail, dim3(1), dim3(1), 0, c.stream, b.cntEE.p, Es);
     exclusive_scan_int(c, b.cntPT.p, b.offPT.p, Vs + 1);
     exclusive_scan_int(c, b.cntEE.p, b.offEE.p, Es + 1);
-    c.pin_as.reserve(16);
-    int* nh = c.pin_as.p;
-    CIPC_HIP(hipMemcpyAsync(&nh[0], b.offPT.p + Vs, sizeof(int), hipMemcpyDeviceToHost, c.stream));
-    CIPC_HIP(hipMemcpyAsync(&nh[1], b.offEE.p + Es, sizeof(int), hipMemcpyDeviceToHost, c.stream));
-    CIPC_HIP(hipStreamSynchronize(c.stream));
-    const int nPT = nh[0], nEE = nh[1];
+    hipLaunchKernelGGL(candidate_totals, dim3(1), dim3(64), 0, c.stream, Vs, Es, b.offPT.p, b.offEE.p, capPT, capEE, dc);
+    CIPC_CHECK_LAUNCH();
     // B + C: pack, sort, filter and classify the hits
-    b.candUPT.resize(nPT + 1); b.candUEE.resize(nEE + 1);
-    b.candPT.resize(nPT + 1); b.recPT.resize(nPT + 1); b.catPT.resize(nPT + 1); b.ownPT.resize(nPT + 1);
-    b.candEE.resize(nEE + 1); b.recEE.resize(nEE + 1); b.catEE.resize(nEE + 1); b.auxEE.resize(nEE + 1); b.ownEE.resize(nEE + 1);
+    b.candUPT.resize(capPT + 1); b.candUEE.resize(capEE + 1);
+    b.candPT.resize(capPT + 1); b.recPT.resize(capPT + 1); b.catPT.resize(capPT + 1); b.ownPT.resize(capPT + 1);
+    b.candEE.resize(capEE + 1); b.recEE.resize(capEE + 1); b.catEE.resize(capEE + 1); b.auxEE.resize(capEE + 1); b.ownEE.resize(capEE + 1);
     if (Vs > 0) {
         hipLaunchKernelGGL(pt_compact, dim3(blocks_for(Vs, 64)), dim3(64), 0, c.stream, Vs, s.sv_perm.p, s.sv.p, s.x.p, ft, sq, b.offPT.p,
-                           b.cntpPT.p, b.bufPT.p, b.candUPT.p, b.ownPT.p);
-        if (nPT > 0)
-            hipLaunchKernelGGL(pt_classify, dim3(blocks_for(nPT, 128)), dim3(128), 0, c.stream, nPT, s.F, b.ownPT.p, b.offPT.p, s.sv.p, s.x.p,
-                               s.bt.p, s.faces.p, b.candUPT.p, b.candPT.p, s.hhat, sq, scene_of_vertex(s), b.recPT.p, b.catPT.p);
+                           b.cntpPT.p, b.bufPT.p, b.candUPT.p, b.ownPT.p, dc);
+        hipLaunchKernelGGL(pt_classify, dim3(blocks_for(capPT, 128)), dim3(128), 0, c.stream, dc, s.F, b.ownPT.p, b.offPT.p, s.sv.p, s.x.p,
+                           s.bt.p, s.faces.p, b.candUPT.p, b.candPT.p, s.hhat, sq, scene_of_vertex(s), b.recPT.p, b.catPT.p);
     }
     if (Es > 0) {
         hipLaunchKernelGGL(ee_compact, dim3(blocks_for(Es, 64)), dim3(64), 0, c.stream, Es, s.se_perm.p, s.se.p, s.x.p, et, sq, b.offEE.p,
-                           b.cntpEE.p, b.bufEE.p, b.candUEE.p, b.ownEE.p);
-        if (nEE > 0)
-            hipLaunchKernelGGL(ee_classify, dim3(blocks_for(nEE, 128)), dim3(128), 0, c.stream, nEE, Es, b.ownEE.p, b.offEE.p, s.se.p,
-                               s.x.p, s.vrest.p, s.bt.p, b.candUEE.p, b.candEE.p, s.hhat, sq, scene_of_vertex(s), b.recEE.p, b.auxEE.p,
-                               b.catEE.p);
+                           b.cntpEE.p, b.bufEE.p, b.candUEE.p, b.ownEE.p, dc);
+        hipLaunchKernelGGL(ee_classify, dim3(blocks_for(capEE, 128)), dim3(128), 0, c.stream, dc, Es, b.ownEE.p, b.offEE.p, s.se.p,
+                           s.x.p, s.vrest.p, s.bt.p, b.candUEE.p, b.candEE.p, s.hhat, sq, scene_of_vertex(s), b.recEE.p, b.auxEE.p,
+                           b.catEE.p);
     }
     CIPC_CHECK_LAUNCH();
     // category positions
     const unsigned keptPT = (1u << kNone) | (1u << kSelfPT) | (1u << kMap);
     const unsigned keptEE = (1u << kNone) | (1u << kSelfEE) | (1u << kMap) | (1u << kMolPE) | (1u << kMolEE);
-    category_scan(c, b.catPT.p, nPT, 1u << kSelfPT, 1u << kMap, keptPT, 0u, b.posPT);
-    category_scan(c, b.catEE.p, nEE, 1u << kSelfEE, 1u << kMap, (1u << kMolPE) | (1u << kMolEE), keptEE, b.posEE);
+    category_scan(c, b.catPT.p, capPT, dc, kDcPT, 1u << kSelfPT, 1u << kMap, keptPT, 0u, b.posPT);
+    category_scan(c, b.catEE.p, capEE, dc, kDcEE, 1u << kSelfEE, 1u << kMap, (1u << kMolPE) | (1u << kMolEE), keptEE, b.posEE);
     b.fg.resize(Vs + 1); b.pg.resize(Vs + 1);
     hipLaunchKernelGGL(ground_flags, dim3(blocks_for(Vs + 1, 256)), dim3(256), 0, c.stream, Vs, s.sv.p, s.x.p, s.gn, s.gd, s.hhat, b.fg.p);
     CIPC_CHECK_LAUNCH();
     exclusive_scan_int(c, b.fg.p, b.pg.p, Vs + 1);
-    int* hc = c.pin_as.p;   // tpt (4), tee (4), nG (1)
-    CIPC_HIP(hipMemcpyAsync(hc, b.posPT.p + nPT, sizeof(int4), hipMemcpyDeviceToHost, c.stream));
-    CIPC_HIP(hipMemcpyAsync(hc + 4, b.posEE.p + nEE, sizeof(int4), hipMemcpyDeviceToHost, c.stream));
-    CIPC_HIP(hipMemcpyAsync(hc + 8, b.pg.p + Vs, sizeof(int), hipMemcpyDeviceToHost, c.stream));
-    CIPC_HIP(hipStreamSynchronize(c.stream));
-    const int4 tpt = make_int4(hc[0], hc[1], hc[2], hc[3]), tee = make_int4(hc[4], hc[5], hc[6], hc[7]);
-    const int nG = hc[8];
-    const int nA = tpt.x, n2a = tpt.y, nPTk = tpt.z;
-    const int nB = tee.x, n2b = tee.y, nE = tee.z, nEEk = tee.w;
+    hipLaunchKernelGGL(category_totals, dim3(1), dim3(64), 0, c.stream, Vs, b.posPT.p, b.posEE.p, b.pg.p, capM, dc);
+    CIPC_CHECK_LAUNCH();
     // PP/PE map keys need the vertex count to fit three fields in 64 bits
     int bits = 1;
     while ((1ll << bits) < (long long)s.V + 2) ++bits;
     if (3 * bits > 64) throw Error(CIPC_ERR_OVERFLOW, "vertex count too large for the point-point/point-edge key packing");
-    const int nm = n2a + n2b;
-    // outputs (the map tail of Self_ActiveSet has at most nm entries)
-    c.self_set.resize((size_t)nA + nB + nm + 1); c.ee_set.resize(nE + 1); c.ee_ie.resize(nE + 1);
-    c.ccd_set.resize((size_t)nPTk + nEEk + 1); c.env_set.resize(nG + 1);
-    b.mkeys.resize(nm + 1); b.mkeys_sorted.resize(nm + 1); b.ukeys.resize(nm + 1); b.ucounts.resize(nm + 1); b.nruns.resize(1);
-    if (nPT > 0)
-        hipLaunchKernelGGL(scatter_pt, dim3(blocks_for(nPT, 256)), dim3(256), 0, c.stream, nPT, b.catPT.p, b.posPT.p, b.recPT.p, b.ownPT.p,
-                           b.candPT.p, s.V, bits, c.self_set.p, b.mkeys.p, c.ccd_set.p);
-    if (nEE > 0)
-        hipLaunchKernelGGL(scatter_ee, dim3(blocks_for(nEE, 256)), dim3(256), 0, c.stream, nEE, b.catEE.p, b.posEE.p, b.recEE.p, b.auxEE.p,
-                           b.ownEE.p, b.candEE.p, s.V, bits, nA, n2a, nPTk, c.self_set.p, b.mkeys.p, c.ee_set.p, c.ee_ie.p,
-                           c.ccd_set.p);
+    // outputs, sized by their bounds (the map tail of Self_ActiveSet has at most capM entries)
+    c.self_set.resize((size_t)capPT + capEE + capM + 1); c.ee_set.resize((size_t)capEE + 1); c.ee_ie.resize((size_t)capEE + 1);
+    c.ccd_set.resize((size_t)capPT + capEE + 1); c.env_set.resize((size_t)Vs + 1);
+    b.mkeys.resize(capM); b.mkeys_sorted.resize(capM); b.ukeys.resize(capM); b.ucounts.resize(capM); b.nruns.resize(1);
+    hipLaunchKernelGGL(scatter_pt, dim3(blocks_for(capPT, 256)), dim3(256), 0, c.stream, dc, b.catPT.p, b.posPT.p, b.recPT.p, b.ownPT.p,
+                       b.candPT.p, s.V, bits, c.self_set.p, b.mkeys.p, c.ccd_set.p);
+    hipLaunchKernelGGL(scatter_ee, dim3(blocks_for(capEE, 256)), dim3(256), 0, c.stream, dc, b.catEE.p, b.posEE.p, b.recEE.p, b.auxEE.p,
+                       b.ownEE.p, b.candEE.p, s.V, bits, c.self_set.p, b.mkeys.p, c.ee_set.p, c.ee_ie.p, c.ccd_set.p);
     if (Vs > 0)
         hipLaunchKernelGGL(ground_scatter, dim3(blocks_for(Vs, 256)), dim3(256), 0, c.stream, Vs, s.sv.p, b.fg.p, b.pg.p, c.env_set.p);
+    hipLaunchKernelGGL(map_tail, dim3(blocks_for(capM, 256)), dim3(256), 0, c.stream, capM, dc, b.mkeys.p);
     CIPC_CHECK_LAUNCH();
-    // PP/PE map: sort packed keys, run-length encode (std::map order with multiplicity)
-    int nM = 0;
-    if (nm > 0) {
-        sort_u64(c, b.mkeys.p, b.mkeys_sorted.p, nm, 3 * bits);
-        size_t bytes = 0;
-        CIPC_HIP(hipcub::DeviceRunLengthEncode::Encode(nullptr, bytes, b.mkeys_sorted.p, b.ukeys.p, b.ucounts.p, b.nruns.p, nm, c.stream));
-        c.tmp.reserve(bytes);
-        CIPC_HIP(hipcub::DeviceRunLengthEncode::Encode(c.tmp.p, bytes, b.mkeys_sorted.p, b.ukeys.p, b.ucounts.p, b.nruns.p, nm, c.stream));
-        nM = read_int(c, b.nruns.p);
-        hipLaunchKernelGGL(map_decode, dim3(blocks_for(nM, 256)), dim3(256), 0, c.stream, nM, b.ukeys.p, b.ucounts.p, s.V, bits, nA + nB,
-                           c.self_set.p);
-        CIPC_CHECK_LAUNCH();
+    // PP/PE map: sort packed keys, run-length encode (std::map order with multiplicity); the
+    // all-ones keys past nm form the last run
+    sort_u64(c, b.mkeys.p, b.mkeys_sorted.p, capM, 3 * bits);
+    size_t bytes = 0;
+    CIPC_HIP(hipcub::DeviceRunLengthEncode::Encode(nullptr, bytes, b.mkeys_sorted.p, b.ukeys.p, b.ucounts.p, b.nruns.p, capM, c.stream));
+    c.tmp.reserve(bytes);
+    CIPC_HIP(hipcub::DeviceRunLengthEncode::Encode(c.tmp.p, bytes, b.mkeys_sorted.p, b.ukeys.p, b.ucounts.p, b.nruns.p, capM, c.stream));
+    hipLaunchKernelGGL(map_total, dim3(1), dim3(64), 0, c.stream, capM, b.nruns.p, dc);
+    hipLaunchKernelGGL(map_decode, dim3(blocks_for(capM, 256)), dim3(256), 0, c.stream, dc, b.ukeys.p, b.ucounts.p, s.V, bits,
+                       c.self_set.p);
+    CIPC_CHECK_LAUNCH();
+    // the counts are read with the host's next synchronisation (settle_active_set)
+    c.pin_dc.reserve(kDcWords);
+    CIPC_HIP(hipMemcpyAsync(c.pin_dc.p, dc, kDcWords * sizeof(int), hipMemcpyDeviceToHost, c.stream));
+    c.dc_pending = true;
+}
+
+namespace {
+int grown(int cap, int need) {
+    if (need > INT32_MAX / 2) throw Error(CIPC_ERR_OVERFLOW, "active-set count beyond the 32-bit index range");
+    return need > cap ? need + need / 4 + 256 : cap;
+}
+}  // namespace
+
+bool grow_capacities(Context& c) {
+    if (!c.dc_pending) return false;
+    ActiveSetScratch& b = c.as;
+    const int* h = c.pin_dc.p;
+    const int flags = h[kDcFlags];
+    // the hit counts are final whenever they were counted; the map and contact counts only
+    // once the hits fitted
+    if (flags & 1) b.capPT = grown(b.capPT, h[kDcPT]);
+    if (flags & 2) b.capEE = grown(b.capEE, h[kDcEE]);
+    if (flags & 4) b.capM = grown(b.capM, h[kDcMapKeys]);
+    if (flags == 0 && h[kDcContactOver]) c.capS = grown(c.capS, h[kDcSelf] + h[kDcMol]);
+    return flags != 0 || h[kDcContactOver] != 0;
+}
+
+bool settle_active_set(Context& c) {
+    if (!c.dc_pending) return false;
+    bool rebuilt = false;
+    for (int round = 0;; ++round) {
+        CIPC_HIP(hipStreamSynchronize(c.stream));
+        if (!grow_capacities(c)) break;
+        if (round >= 3) throw Error(CIPC_ERR_RUNTIME, "active-set capacities kept overflowing");
+        compute_active_set(c);
+        rebuilt = true;
     }
-    // the counts are known on the host; the sets themselves are read in stream order by the
-    // next stages, so no synchronisation here
-    c.n_self = nA + nB + nM; c.n_ee = nE; c.n_ccd = nPTk + nEEk; c.n_env = nG;
+    const int* h = c.pin_dc.p;
+    c.n_self = h[kDcSelf]; c.n_ee = h[kDcMol]; c.n_ccd = h[kDcCCD]; c.n_env = h[kDcEnv];
+    c.dc_pending = false;
+    return rebuilt;
+}
+
+void active_set_now(Context& c) {
+    compute_active_set(c);
+    settle_active_set(c);
 }
 
 // ---------------------------------------------------------------------------------

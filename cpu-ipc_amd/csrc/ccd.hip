@@ -97,10 +97,12 @@ __global__ void ground_cfl_kernel(int Vs, const int* __restrict__ sv, const doub
     scene_atomic<true>(sl, scene, 1, move);
 }
 
-__global__ void partial_ccd_kernel(int n, const int2* __restrict__ pairs, const int* __restrict__ sv, const int4* __restrict__ faces,
+// The pairs are the active set's CCD pairs: their count is the device count n_dev.
+__global__ void partial_ccd_kernel(const int* __restrict__ n_dev, const int2* __restrict__ pairs, const int* __restrict__ sv, const int4* __restrict__ faces,
                                    const int2* __restrict__ se, const double* __restrict__ x, const double* __restrict__ p, double eta,
                                    const int* __restrict__ vs, unsigned long long* __restrict__ sl) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int n = *n_dev;
     double a = 1.0;
     int scene = kNoScene;
     if (i < n) {
@@ -356,8 +358,10 @@ void step_bound_partial(Context& c, hipStream_t st) {
     const double eta = 1.0 - 0.8;
     DBuf<unsigned long long>& sl = c.u64scratch;
     // (its cap is the ground step, slot 0, final once step_bound_early has run)
-    if (c.n_ccd > 0)
-        hipLaunchKernelGGL(partial_ccd_kernel, dim3(blocks_for(c.n_ccd, 64)), dim3(64), 0, st, (int)c.n_ccd, c.ccd_set.p, s.sv.p,
+    // grid: the pairs' capacity (the active set's hit capacities), or the settled count
+    const size_t cap = c.dc_pending ? (size_t)c.as.capPT + c.as.capEE : (size_t)c.n_ccd;
+    if (cap > 0)
+        hipLaunchKernelGGL(partial_ccd_kernel, dim3(blocks_for(cap, 64)), dim3(64), 0, st, c.dcnt.p + kDcCCD, c.ccd_set.p, s.sv.p,
                            s.faces.p, s.se.p, s.x.p, s.dir.p, eta, scene_of_vertex(s), sl.p);
     CIPC_CHECK_LAUNCH();
     c.pin_slots.reserve((size_t)kSlots * S);

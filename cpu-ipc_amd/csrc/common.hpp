@@ -3,6 +3,8 @@
 
 #include <hip/hip_runtime.h>
 
+#include <atomic>
+
 #include <cstdint>
 #include <stdexcept>
 #include <string>
@@ -31,6 +33,13 @@ struct Error : std::runtime_error {
 #define CIPC_CHECK_LAUNCH() CIPC_HIP(hipGetLastError())
 
 // Grown-on-demand device buffer; never shrinks during a context's life.
+// Bumped by every device or pinned allocation and free of the buffers below: a captured HIP
+// graph holds buffer addresses, so it is replayed only while this count is unchanged.
+inline std::atomic<unsigned long long>& alloc_generation() {
+    static std::atomic<unsigned long long> g{0};
+    return g;
+}
+
 template <typename T>
 struct DBuf {
     T* p = nullptr;
@@ -42,14 +51,20 @@ struct DBuf {
     ~DBuf() { release(); }
     void reserve(size_t want) {
         if (want <= cap) return;
+        ++alloc_generation();
         if (p) CIPC_HIP(hipFree(p));
+        p = nullptr;
+        cap = 0;
         size_t c = want + want / 4 + 16;
         CIPC_HIP(hipMalloc(&p, c * sizeof(T)));
         cap = c;
     }
     void resize(size_t want) { reserve(want); n = want; }
     void release() {
-        if (p) (void)hipFree(p);
+        if (p) {
+            ++alloc_generation();
+            (void)hipFree(p);
+        }
         p = nullptr; cap = n = 0;
     }
     T* data() { return p; }
@@ -71,8 +86,10 @@ struct PinnedBuf {
     }
     void reserve(size_t want) {
         if (want <= cap) return;
+        ++alloc_generation();
         if (p) CIPC_HIP(hipHostFree(p));
         p = nullptr;
+        cap = 0;
         CIPC_HIP(hipHostMalloc(reinterpret_cast<void**>(&p), want * sizeof(T), hipHostMallocDefault));
         cap = want;
     }

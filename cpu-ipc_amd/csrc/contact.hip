@@ -87,10 +87,23 @@ __device__ inline void self_local(int c, int n, int4 e, const int (&v)[4], const
 #pragma unroll
     for (int i = 0; i < N; ++i) Gs[(size_t)i * n + c] = gs * g[i];
 }
-__global__ __launch_bounds__(64) void contact_self_kernel(int n_self, int n, const int4* __restrict__ self_set,
+// Contacts of the records: Self_ActiveSet entries [0, ns), then Self_EE_ActiveSet entries,
+// from the active set's device counts; none when they exceed the records' capacity (the
+// host grows it and assembles again).
+__device__ inline int record_contacts(const int* __restrict__ dc, int& ns) {
+    if (dc[kDcContactOver]) { ns = 0; return 0; }
+    ns = dc[kDcSelf];
+    return ns + dc[kDcMol];
+}
+__global__ void contact_totals(int capS, int* __restrict__ dc) {
+    if (blockIdx.x == 0 && threadIdx.x == 0) dc[kDcContactOver] = dc[kDcSelf] + dc[kDcMol] > capS ? 1 : 0;
+}
+__global__ __launch_bounds__(64) void contact_self_kernel(const int* __restrict__ dc, int n, const int4* __restrict__ self_set,
                                                           const double* __restrict__ x, double hhat, Kap kap, double* __restrict__ Hs,
                                                           double* __restrict__ Gs, int4* __restrict__ verts, int* __restrict__ dims) {
     const int c = blockIdx.x * blockDim.x + threadIdx.x;
+    int n_self;
+    record_contacts(dc, n_self);
     if (c >= n_self) return;
     const int4 e = self_set[c];
     int v[4];
@@ -106,13 +119,14 @@ __global__ __launch_bounds__(64) void contact_self_kernel(int n_self, int n, con
     dims[c] = 3 * nv_of(type);
 }
 __global__ __launch_bounds__(64) void contact_ee_kernel(
-    int n_self, int n_ee, const int4* __restrict__ ee_set,
+    const int* __restrict__ dc, int n, const int4* __restrict__ ee_set,
     const int2* __restrict__ ee_ie, const int2* __restrict__ se, const double* __restrict__ x,
     const double* __restrict__ vrest, double hhat, Kap kap, double* __restrict__ Hs, double* __restrict__ Gs,
     int4* __restrict__ verts, int* __restrict__ dims) {
-    const int n = n_self + n_ee;
+    int n_self;
+    const int nc = record_contacts(dc, n_self);
     const int c = n_self + (int)(blockIdx.x * blockDim.x + threadIdx.x);
-    if (c >= n) return;
+    if (c >= nc) return;
     double H[144];
     {
         const int i = c - n_self;
@@ -193,9 +207,13 @@ __global__ __launch_bounds__(64) void contact_ee_kernel(
 // point-edge distances) use the structured range basis (project_contact_structured); the
 // mollified edge-edge blocks and point-point blocks the general Jacobi (project_contact).
 // The packed index of (i, j) is the one kernel 1 writes.
+// range 0: the Self_ActiveSet contacts, 1: the Self_EE_ActiveSet ones.
 template <int M, bool Structured>
-__global__ __launch_bounds__(64) void contact_pd_kernel(int c0, int c1, int n, const int* __restrict__ dims, double* __restrict__ Hs,
-                                                        const double* __restrict__ Gs) {
+__global__ __launch_bounds__(64) void contact_pd_kernel(const int* __restrict__ dc, int range, int n, const int* __restrict__ dims,
+                                                        double* __restrict__ Hs, const double* __restrict__ Gs) {
+    int ns;
+    const int nc = record_contacts(dc, ns);
+    const int c0 = range == 0 ? 0 : ns, c1 = range == 0 ? ns : nc;
     const int c = c0 + (int)(blockIdx.x * blockDim.x + threadIdx.x);
     if (c >= c1 || dims[c] != 3 * M) return;
     double* h = Hs + c;
@@ -210,14 +228,32 @@ __global__ __launch_bounds__(64) void contact_pd_kernel(int c0, int c1, int n, c
     }
 }
 
-// Kernel 3: block and gradient records.  Slots: 10 blocks and 4 gradients per contact.
-__global__ void contact_scatter_kernel(int n, const int4* __restrict__ verts, const int* __restrict__ dims,
+// Kernel 3: block and gradient records.  Slots: 10 blocks and 4 gradients per contact.  With
+// dc, the contacts are the active set's (device count) in slots of n contacts, and the slots
+// past the count are written empty; without, n contacts.
+__global__ void contact_scatter_kernel(int n, const int* __restrict__ dc, const int4* __restrict__ verts, const int* __restrict__ dims,
                                        const double* __restrict__ Hs, const double* __restrict__ Gs,
                                        const int* __restrict__ bt, int slot0_b, int slot0_g, int kb,
                                        unsigned long long* __restrict__ bkey, int* __restrict__ bidx, double* __restrict__ bval,
                                        int* __restrict__ gkey, int* __restrict__ gidx, double* __restrict__ gval) {
     const int c = blockIdx.x * blockDim.x + threadIdx.x;
     if (c >= n) return;
+    if (dc) {
+        int ns;
+        if (c >= record_contacts(dc, ns)) {
+            for (int k = 0; k < 10; ++k) {
+                const int s = slot0_b + 10 * c + k;
+                bidx[s] = s;
+                bkey[s] = kNoKey;
+            }
+            for (int k = 0; k < 4; ++k) {
+                const int s = slot0_g + 4 * c + k;
+                gidx[s] = s;
+                gkey[s] = 0x7fffffff;
+            }
+            return;
+        }
+    }
     const int N = dims[c], nv = N / 3;
     const int4 vv = verts[c];
     const int v[4] = {vv.x, vv.y, vv.z, vv.w};
@@ -249,13 +285,19 @@ __global__ void contact_scatter_kernel(int n, const int4* __restrict__ verts, co
     }
 }
 
-// Ground (IPCSolver.cpp:270-293): one block and one gradient record per active vertex.
-__global__ void ground_kernel(int n_env, const int* __restrict__ env, const double* __restrict__ x, const int* __restrict__ bt,
-                              d3 gn, double gd, double hhat, Kap kap, int kb,
+// Ground (IPCSolver.cpp:270-293): one block and one gradient record per active vertex, in
+// slots [0, cap); slots past the device count are written empty.
+__global__ void ground_kernel(int cap, const int* __restrict__ dc, const int* __restrict__ env, const double* __restrict__ x,
+                              const int* __restrict__ bt, d3 gn, double gd, double hhat, Kap kap, int kb,
                               unsigned long long* __restrict__ bkey, int* __restrict__ bidx, double* __restrict__ bval,
                               int* __restrict__ gkey, int* __restrict__ gidx, double* __restrict__ gval) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n_env) return;
+    if (i >= cap) return;
+    if (i >= dc[kDcEnv]) {
+        bidx[i] = i; bkey[i] = kNoKey;
+        gidx[i] = i; gkey[i] = 0x7fffffff;
+        return;
+    }
     const int v = env[i];
     const double kappa = kap.at(v);
     const double dist = dot(gn, ld3(x, v)) - gd;
@@ -585,7 +627,7 @@ void friction_records(Context& c, size_t slot_b, size_t slot_g, hipStream_t st) 
         hipLaunchKernelGGL(friction_local_kernel, dim3(blocks_for(nf, 64)), dim3(64), 0, st, nf, c.fr_self.p, c.fr_lam.p,
                            c.fr_coord.p, c.fr_basis.p, s.x.p, s.vprev.p, eps2, s.fric, Hs, Gs, c.fverts.p, c.fdims.p);
         CIPC_CHECK_LAUNCH();
-        hipLaunchKernelGGL(contact_scatter_kernel, dim3(blocks_for(nf, 128)), dim3(128), 0, st, nf, c.fverts.p, c.fdims.p, Hs, Gs,
+        hipLaunchKernelGGL(contact_scatter_kernel, dim3(blocks_for(nf, 128)), dim3(128), 0, st, nf, nullptr, c.fverts.p, c.fdims.p, Hs, Gs,
                            s.bt.p, (int)(slot_b + ngf), (int)(slot_g + ngf), s.kb, c.ckey.p, c.cidx.p, c.cval.p, c.gkey.p, c.gidx.p,
                            c.gval.p);
         CIPC_CHECK_LAUNCH();
@@ -600,11 +642,27 @@ void friction_energy_partials(Context& c, double* part, double* elem_self, doubl
     CIPC_CHECK_LAUNCH();
 }
 
+void size_contact_records(Context& c) {
+    const int64_t need = c.n_self + c.n_ee;
+    if (need > INT32_MAX / 16) throw Error(CIPC_ERR_OVERFLOW, "too many contacts for the contact records");
+    if (c.capS == 0) c.capS = 1024;
+    if (need > c.capS) c.capS = (int)(need + need / 4 + 256);
+}
+
+// Record slots: ground [0, capG), capG = Vs (at most one entry per surface vertex); the
+// contacts' 10 block and 4 gradient slots each for capS contacts; then the lagged friction
+// records.  Slots past a device count are written empty, so the counts stay on the device;
+// records keep the order ground, contacts, friction, and the sort is stable, so every block
+// and gradient is summed in the same order as with packed records.
 void contact_assemble(Context& c, double kappa) {
     Scene& s = c.sc;
-    const int n = (int)(c.n_self + c.n_ee);
-    const int ne = (int)c.n_env;
-    const size_t nbc = (size_t)ne + 10 * (size_t)n, ngc = (size_t)ne + 4 * (size_t)n;
+    // with the counts still on the device, the capacity of the last assembly (grown by the
+    // host when the device reports it exceeded)
+    if (!c.dc_pending) size_contact_records(c);
+    else if (c.capS == 0) c.capS = 1024;
+    const int capS = c.capS, capG = s.Vs;
+    const int* dc = c.dcnt.p;
+    const size_t nbc = (size_t)capG + 10 * (size_t)capS, ngc = (size_t)capG + 4 * (size_t)capS;
     const size_t nb = nbc + (size_t)c.n_fenv + 10 * (size_t)c.n_fself, ng = ngc + (size_t)c.n_fenv + 4 * (size_t)c.n_fself;
     c.ckey.resize(std::max<size_t>(nb, 1)); c.cidx.resize(std::max<size_t>(nb, 1));
     c.cval.resize(9 * std::max<size_t>(nb, 1));
@@ -618,62 +676,55 @@ void contact_assemble(Context& c, double kappa) {
         c.fverts.resize(c.n_fself);
         c.fdims.resize(c.n_fself);
     }
-    if (n > 0) {
-        c.chess.resize((size_t)(kHU + 12) * n);
-        c.cverts.resize(n);
-        c.cdims.resize(n);
-    }
+    c.chess.resize((size_t)(kHU + 12) * capS);
+    c.cverts.resize(capS);
+    c.cdims.resize(capS);
+    hipLaunchKernelGGL(contact_totals, dim3(1), dim3(64), 0, c.stream, capS, c.dcnt.p);
     CIPC_HIP(hipEventRecord(c.fr_fork, c.stream));
-    if (ne > 0) {
-        hipLaunchKernelGGL(ground_kernel, dim3(blocks_for(ne, 256)), dim3(256), 0, c.stream, ne, c.env_set.p, s.x.p, s.bt.p,
+    if (capG > 0) {
+        hipLaunchKernelGGL(ground_kernel, dim3(blocks_for(capG, 256)), dim3(256), 0, c.stream, capG, dc, c.env_set.p, s.x.p, s.bt.p,
                            s.gn, s.gd, s.hhat, scene_kappa(s, kappa), s.kb, c.ckey.p, c.cidx.p, c.cval.p, c.gkey.p, c.gidx.p, c.gval.p);
         CIPC_CHECK_LAUNCH();
     }
-    if (n > 0) {
-        double* Hs = c.chess.p;
-        double* Gs = c.chess.p + (size_t)kHU * n;
-        if (c.n_self > 0)
-            hipLaunchKernelGGL(contact_self_kernel, dim3(blocks_for(c.n_self, 64)), dim3(64), 0, c.stream, (int)c.n_self, (int)n,
-                               c.self_set.p, s.x.p, s.hhat, scene_kappa(s, kappa), Hs, Gs, c.cverts.p, c.cdims.p);
-        if (c.n_ee > 0)
-            hipLaunchKernelGGL(contact_ee_kernel, dim3(blocks_for(c.n_ee, 64)), dim3(64), 0, c.stream, (int)c.n_self, (int)c.n_ee,
-                               c.ee_set.p, c.ee_ie.p, s.se.p, s.x.p, s.vrest.p, s.hhat, scene_kappa(s, kappa), Hs, Gs, c.cverts.p,
-                               c.cdims.p);
-        CIPC_CHECK_LAUNCH();
-    }
+    double* Hs = c.chess.p;
+    double* Gs = c.chess.p + (size_t)kHU * capS;
+    hipLaunchKernelGGL(contact_self_kernel, dim3(blocks_for(capS, 64)), dim3(64), 0, c.stream, dc, capS, c.self_set.p, s.x.p, s.hhat,
+                       scene_kappa(s, kappa), Hs, Gs, c.cverts.p, c.cdims.p);
+    hipLaunchKernelGGL(contact_ee_kernel, dim3(blocks_for(capS, 64)), dim3(64), 0, c.stream, dc, capS, c.ee_set.p, c.ee_ie.p, s.se.p,
+                       s.x.p, s.vrest.p, s.hhat, scene_kappa(s, kappa), Hs, Gs, c.cverts.p, c.cdims.p);
+    CIPC_CHECK_LAUNCH();
     // Main stream: the projections of 4-vertex contacts.  Second stream (enqueued after them,
     // so that the main stream never waits on the host): the friction records, then the
     // projections of 2- and 3-vertex contacts.  Each projection kernel touches only the
     // contacts of its size.
-    const int ns = (int)c.n_self;
-    double* Hs = c.chess.p;
-    double* Gs = c.chess.p + (size_t)kHU * n;
-    if (n > 0) {
-        CIPC_HIP(hipEventRecord(c.cl_done, c.stream));
-        if (ns > 0)
-            hipLaunchKernelGGL((contact_pd_kernel<4, true>), dim3(blocks_for(ns, 64)), dim3(64), 0, c.stream, 0, ns, n, c.cdims.p, Hs, Gs);
-        if (n > ns)
-            hipLaunchKernelGGL((contact_pd_kernel<4, false>), dim3(blocks_for(n - ns, 64)), dim3(64), 0, c.stream, ns, n, n, c.cdims.p, Hs,
-                               Gs);
-        CIPC_CHECK_LAUNCH();
-    }
+    CIPC_HIP(hipEventRecord(c.cl_done, c.stream));
+    hipLaunchKernelGGL((contact_pd_kernel<4, true>), dim3(blocks_for(capS, 64)), dim3(64), 0, c.stream, dc, 0, capS, c.cdims.p, Hs, Gs);
+    hipLaunchKernelGGL((contact_pd_kernel<4, false>), dim3(blocks_for(capS, 64)), dim3(64), 0, c.stream, dc, 1, capS, c.cdims.p, Hs, Gs);
+    CIPC_CHECK_LAUNCH();
     CIPC_HIP(hipStreamWaitEvent(c.aux, c.fr_fork, 0));
     friction_records(c, nbc, ngc, c.aux);
-    if (n > 0 && ns > 0) {
-        CIPC_HIP(hipStreamWaitEvent(c.aux, c.cl_done, 0));
-        hipLaunchKernelGGL((contact_pd_kernel<3, true>), dim3(blocks_for(ns, 64)), dim3(64), 0, c.aux, 0, ns, n, c.cdims.p, Hs, Gs);
-        hipLaunchKernelGGL((contact_pd_kernel<2, false>), dim3(blocks_for(ns, 64)), dim3(64), 0, c.aux, 0, ns, n, c.cdims.p, Hs, Gs);
-        CIPC_CHECK_LAUNCH();
-    }
+    CIPC_HIP(hipStreamWaitEvent(c.aux, c.cl_done, 0));
+    hipLaunchKernelGGL((contact_pd_kernel<3, true>), dim3(blocks_for(capS, 64)), dim3(64), 0, c.aux, dc, 0, capS, c.cdims.p, Hs, Gs);
+    hipLaunchKernelGGL((contact_pd_kernel<2, false>), dim3(blocks_for(capS, 64)), dim3(64), 0, c.aux, dc, 0, capS, c.cdims.p, Hs, Gs);
+    CIPC_CHECK_LAUNCH();
     CIPC_HIP(hipEventRecord(c.fr_join, c.aux));
     CIPC_HIP(hipStreamWaitEvent(c.stream, c.fr_join, 0));
-    if (n > 0) {
-        hipLaunchKernelGGL(contact_scatter_kernel, dim3(blocks_for(n, 128)), dim3(128), 0, c.stream, n, c.cverts.p,
-                           c.cdims.p, Hs, Gs, s.bt.p, ne, ne, s.kb, c.ckey.p, c.cidx.p, c.cval.p, c.gkey.p, c.gidx.p, c.gval.p);
-        CIPC_CHECK_LAUNCH();
-    }
+    hipLaunchKernelGGL(contact_scatter_kernel, dim3(blocks_for(capS, 128)), dim3(128), 0, c.stream, capS, dc, c.cverts.p, c.cdims.p, Hs,
+                       Gs, s.bt.p, capG, capG, s.kb, c.ckey.p, c.cidx.p, c.cval.p, c.gkey.p, c.gidx.p, c.gval.p);
+    CIPC_CHECK_LAUNCH();
     c.ckey.n = nb;
     c.gkey.n = ng;
+    // the counts again, now with the contact-capacity check, for the host's next synchronisation
+    if (c.dc_pending) CIPC_HIP(hipMemcpyAsync(c.pin_dc.p, c.dcnt.p, kDcWords * sizeof(int), hipMemcpyDeviceToHost, c.stream));
+}
+
+void upload_counts(Context& c) {
+    int h[kDcWords] = {};
+    h[kDcSelf] = (int)c.n_self; h[kDcMol] = (int)c.n_ee; h[kDcCCD] = (int)c.n_ccd; h[kDcEnv] = (int)c.n_env;
+    c.dcnt.resize(kDcWords);
+    CIPC_HIP(hipMemcpyAsync(c.dcnt.p, h, sizeof h, hipMemcpyHostToDevice, c.stream));
+    CIPC_HIP(hipStreamSynchronize(c.stream));
+    c.dc_pending = false;
 }
 
 // ---------------------------------------------------------------------------------
@@ -753,7 +804,7 @@ template <typename T>
 struct DevArr {
     T* p = nullptr;
     explicit DevArr(size_t n) { CIPC_HIP(hipMalloc(&p, sizeof(T) * (n ? n : 1))); }
-    ~DevArr() { if (p) hipFree(p); }
+    ~DevArr() { if (p) (void)hipFree(p); }
     void up(const T* h, size_t n) { CIPC_HIP(hipMemcpy(p, h, sizeof(T) * n, hipMemcpyHostToDevice)); }
     void down(T* h, size_t n) { CIPC_HIP(hipMemcpy(h, p, sizeof(T) * n, hipMemcpyDeviceToHost)); }
 };

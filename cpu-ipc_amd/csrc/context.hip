@@ -85,6 +85,8 @@ int cipc_gpu_context_create(int device, cipc_gpu_context** out) {
         CIPC_HIP(hipStreamCreateWithFlags(&c->sbs, hipStreamNonBlocking));
         CIPC_HIP(hipEventCreateWithFlags(&c->as_done, hipEventDisableTiming));
         CIPC_HIP(hipEventCreateWithFlags(&c->sb_done, hipEventDisableTiming));
+        CIPC_HIP(hipEventCreateWithFlags(&c->it_start, hipEventDisableTiming));
+        CIPC_HIP(hipEventCreateWithFlags(&c->en_done, hipEventDisableTiming));
         *out = c;
     });
 }
@@ -103,6 +105,9 @@ int cipc_gpu_context_destroy(cipc_gpu_context* c) {
         (void)hipEventDestroy(c->cl_done);
         (void)hipEventDestroy(c->as_done);
         (void)hipEventDestroy(c->sb_done);
+        (void)hipEventDestroy(c->it_start);
+        (void)hipEventDestroy(c->en_done);
+        if (c->graph.exec) (void)hipGraphExecDestroy(c->graph.exec);
         (void)hipStreamDestroy(c->sbs);
         (void)hipStreamDestroy(c->aux);
         (void)hipStreamDestroy(c->stream);
@@ -247,6 +252,12 @@ void upload_scene_impl(cipc_gpu_context* c, const cipc_scene_desc* d, bool plan 
         c->have_scene = true;
         c->face_tree.age = c->edge_tree.age = c->sface_tree.age = c->sedge_tree.age = kMaxRefits;
         c->n_self = c->n_ee = c->n_ccd = c->n_env = 0;   // sets and lagged state of an earlier scene
+        upload_counts(*c);
+        c->as.capPT = c->as.capEE = c->as.capM = 0;   // capacities of an earlier scene
+        c->capS = 0;
+        if (c->graph.exec) (void)hipGraphExecDestroy(c->graph.exec);   // launches of an earlier scene
+        c->graph.exec = nullptr;
+        c->graph.key.clear();
         c->n_fself = c->n_fenv = 0;
         c->n_soft = 0; c->soft_w = 0.0;
         c->n_cenv = c->n_cself = 0;
@@ -411,8 +422,7 @@ int cipc_gpu_active_set(cipc_gpu_context* c, int64_t counts[4]) {
     return guard([&] {
         require(c && c->have_scene, "no scene uploaded");
         bind(*c);
-        compute_active_set(*c);
-        CIPC_HIP(hipStreamSynchronize(c->stream));   // the call returns with the sets built
+        active_set_now(*c);   // the call returns with the sets built
         if (counts) { counts[0] = c->n_self; counts[1] = c->n_ee; counts[2] = c->n_ccd; counts[3] = c->n_env; }
     });
 }
@@ -469,6 +479,7 @@ int cipc_gpu_set_active_set(cipc_gpu_context* c, const int64_t n[4], const int* 
         upload(c->ccd_set, reinterpret_cast<const int2*>(ccd2), (size_t)n[2], st);
         upload(c->env_set, env, (size_t)n[3], st);
         CIPC_HIP(hipStreamSynchronize(st));
+        upload_counts(*c);
     });
 }
 
@@ -858,6 +869,139 @@ int cipc_gpu_batch_solve(cipc_gpu_context* c, int precision, int preconditioner,
     });
 }
 
+}  // extern "C"
+
+namespace {
+// Timing events of an iteration enqueued eagerly (a captured one is timed as a whole).
+struct IterEvents {
+    hipEvent_t ev[4] = {}, es = nullptr, ee = nullptr, e0 = nullptr, en = nullptr, s0 = nullptr, s1 = nullptr;
+    IterEvents() {
+        for (auto& e : ev) CIPC_HIP(hipEventCreate(&e));
+        for (hipEvent_t* e : {&es, &ee, &e0, &en, &s0, &s1}) CIPC_HIP(hipEventCreate(e));
+    }
+    ~IterEvents() {
+        for (auto& e : ev) (void)hipEventDestroy(e);
+        for (hipEvent_t e : {es, ee, e0, en, s0, s1}) (void)hipEventDestroy(e);
+    }
+    IterEvents(const IterEvents&) = delete;
+    IterEvents& operator=(const IterEvents&) = delete;
+};
+
+// One Newton iteration's launches on the three streams, without a host synchronisation.
+// Main: active set, barrier blocks, CSR.  Second (`aux`): the elastic stage beside the active
+// set, the friction records and small projections beside the barrier blocks, then (with
+// `energy`) the energy.  Step-bound stream (`sbs`): the ground bound, swept trees and full CCD
+// (positions and direction only) from the start, the partial CCD once the active set exists.
+// The streams fork from and join into the main stream through the context's events, so the
+// same launches can be captured into a graph.  t (eager only): per-stage timing events.
+void enqueue_iteration(cipc_gpu_context* c, double kappa, bool energy, const IterEvents* t) {
+    if (t) CIPC_HIP(hipEventRecord(t->ev[0], c->stream));
+    CIPC_HIP(hipEventRecord(c->it_start, c->stream));
+    elastic_fork(c, t ? t->es : nullptr, t ? t->ee : nullptr);
+    CIPC_HIP(hipStreamWaitEvent(c->sbs, c->it_start, 0));
+    if (t) CIPC_HIP(hipEventRecord(t->s0, c->sbs));
+    step_bound_early(*c, c->sbs);
+    compute_active_set(*c);
+    if (t) CIPC_HIP(hipEventRecord(t->ev[1], c->stream));
+    CIPC_HIP(hipEventRecord(c->as_done, c->stream));
+    CIPC_HIP(hipStreamWaitEvent(c->sbs, c->as_done, 0));
+    step_bound_partial(*c, c->sbs);
+    if (t) CIPC_HIP(hipEventRecord(t->s1, c->sbs));
+    CIPC_HIP(hipEventRecord(c->sb_done, c->sbs));
+    contact_assemble(*c, kappa);
+    if (energy) {
+        if (t) CIPC_HIP(hipEventRecord(t->e0, c->aux));
+        energy_terms_launch(*c, c->aux);
+        if (t) CIPC_HIP(hipEventRecord(t->en, c->aux));
+    }
+    CIPC_HIP(hipEventRecord(c->en_done, c->aux));
+    elastic_join(c);
+    finalize_csr(*c);
+    if (t) CIPC_HIP(hipEventRecord(t->ev[2], c->stream));
+    CIPC_HIP(hipStreamWaitEvent(c->stream, c->sb_done, 0));
+    CIPC_HIP(hipStreamWaitEvent(c->stream, c->en_done, 0));
+    if (t) CIPC_HIP(hipEventRecord(t->ev[3], c->stream));
+}
+
+// What the captured launches depend on besides device memory.
+struct IterKey {
+    double kappa, soft_w;
+    long long nfs, nfe, gen, topo;
+    int fp32, capPT, capEE, capM, capS, nsoft, tune_pt, tune_ee, S, energy;
+};
+std::vector<unsigned char> iter_key(const cipc_gpu_context* c, double kappa, bool energy) {
+    IterKey k;
+    std::memset(&k, 0, sizeof k);
+    k.kappa = kappa; k.soft_w = c->soft_w;
+    k.nfs = c->n_fself; k.nfe = c->n_fenv; k.gen = (long long)alloc_generation().load(); k.topo = (long long)c->topo_gen;
+    k.fp32 = c->hessian_fp32 ? 1 : 0; k.capPT = c->as.capPT; k.capEE = c->as.capEE; k.capM = c->as.capM; k.capS = c->capS;
+    k.nsoft = c->n_soft; k.tune_pt = c->tune.ccd_pt_bits; k.tune_ee = c->tune.ccd_ee_bits; k.S = c->sc.S; k.energy = energy ? 1 : 0;
+    const unsigned char* b = reinterpret_cast<const unsigned char*>(&k);
+    return std::vector<unsigned char>(b, b + sizeof k);
+}
+
+// The four trees are refitted, not rebuilt, by the next iteration (a captured iteration holds
+// the refit launches of the topologies it was captured with).
+bool trees_refit_only(const cipc_gpu_context* c) {
+    const Scene& s = c->sc;
+    auto ok = [](const LBVH& t, int n) { return n <= 0 || (t.n == n && t.age < kMaxRefits); };
+    return ok(c->face_tree, s.F) && ok(c->edge_tree, s.Es) && ok(c->sface_tree, s.F) && ok(c->sedge_tree, s.Es);
+}
+
+void drop_graph(cipc_gpu_context* c) {
+    if (c->graph.exec) (void)hipGraphExecDestroy(c->graph.exec);
+    c->graph.exec = nullptr;
+    c->graph.key.clear();
+}
+
+// Runs the iteration as one graph launch on the main stream when it can: a replay when the
+// key matches, else a new capture (discarded, and false returned, when an allocation happened
+// while capturing, or the capture failed).  t0/t1 bracket the launch.
+bool iteration_graph(cipc_gpu_context* c, double kappa, bool energy, hipEvent_t t0, hipEvent_t t1) {
+    if (!c->graph.enabled || !trees_refit_only(c) || c->capS == 0 || c->as.capPT == 0) return false;
+    std::vector<unsigned char> key = iter_key(c, kappa, energy);
+    if (!c->graph.exec || key != c->graph.key) {
+        drop_graph(c);
+        const unsigned long long gen0 = alloc_generation().load();
+        hipGraph_t g = nullptr;
+        CIPC_HIP(hipStreamBeginCapture(c->stream, hipStreamCaptureModeRelaxed));
+        bool ok = true;
+        try {
+            enqueue_iteration(c, kappa, energy, nullptr);
+        } catch (const std::exception&) {
+            ok = false;
+        }
+        const hipError_t e = hipStreamEndCapture(c->stream, &g);
+        if (e != hipSuccess || !g) ok = false;
+        (void)hipGetLastError();
+        if (ok && alloc_generation().load() == gen0) {
+            hipGraphExec_t exec = nullptr;
+            if (hipGraphInstantiate(&exec, g, nullptr, nullptr, 0) == hipSuccess && exec) {
+                c->graph.exec = exec;
+                c->graph.key = iter_key(c, kappa, energy);
+                ++c->graph.captures;
+            }
+            (void)hipGetLastError();
+        }
+        if (g) (void)hipGraphDestroy(g);
+        if (!c->graph.exec) return false;   // this call runs eagerly
+    } else {
+        // the host side of a replayed iteration: what enqueue_iteration records on the host
+        for (LBVH* t : {&c->face_tree, &c->edge_tree, &c->sface_tree, &c->sedge_tree})
+            if (t->n > 0) ++t->age;
+        c->dc_pending = true;
+        c->nnz_pending = true;
+        ++c->graph.replays;
+    }
+    CIPC_HIP(hipEventRecord(t0, c->stream));
+    CIPC_HIP(hipGraphLaunch(c->graph.exec, c->stream));
+    CIPC_HIP(hipEventRecord(t1, c->stream));
+    return true;
+}
+}  // namespace
+
+extern "C" {
+
 // One Newton iteration of every scene: active set, assembly with each scene's stiffness, step
 // bounds and energies (per scene), timed as cipc_gpu_newton_iteration; total->alpha and
 // total->energy hold the minimum step and the summed energy.
@@ -868,46 +1012,46 @@ int cipc_gpu_batch_newton_iteration(cipc_gpu_context* c, const double* kappa, ci
         bind(*c);
         const int S = c->sc.S;
         set_scene_kappa(c, kappa, kappa[0]);
-        // streams as in cipc_gpu_newton_iteration; the per-scene energies follow the CSR on the
-        // main stream
-        hipEvent_t ev[5], es, ee, s0, s1;
-        for (auto& e : ev) CIPC_HIP(hipEventCreate(&e));
-        for (hipEvent_t* e : {&es, &ee, &s0, &s1}) CIPC_HIP(hipEventCreate(e));
-        CIPC_HIP(hipEventRecord(ev[0], c->stream));
-        elastic_fork(c, es, ee);
-        CIPC_HIP(hipStreamWaitEvent(c->sbs, ev[0], 0));
-        CIPC_HIP(hipEventRecord(s0, c->sbs));
-        step_bound_early(*c, c->sbs);
-        compute_active_set(*c);
-        CIPC_HIP(hipEventRecord(ev[1], c->stream));
-        CIPC_HIP(hipEventRecord(c->as_done, c->stream));
-        CIPC_HIP(hipStreamWaitEvent(c->sbs, c->as_done, 0));
-        step_bound_partial(*c, c->sbs);
-        CIPC_HIP(hipEventRecord(s1, c->sbs));
-        contact_assemble(*c, kappa[0]);
-        elastic_join(c);
-        finalize_csr(*c);
-        c->assembled = true;
-        CIPC_HIP(hipEventRecord(ev[2], c->stream));
-        std::vector<double> sb(5 * (size_t)S), et(9 * (size_t)S);
-        energy_terms_scenes(*c, et.data());          // synchronises the main stream
-        CIPC_HIP(hipEventRecord(ev[3], c->stream));
-        CIPC_HIP(hipStreamWaitEvent(c->stream, s1, 0));
-        CIPC_HIP(hipEventRecord(ev[4], c->stream));
-        CIPC_HIP(hipEventSynchronize(ev[4]));
+        // the launches of cipc_gpu_newton_iteration for the whole batch (one graph when they
+        // can go as one), then the per-scene energies, whose element arrays are sized by the
+        // settled counts
+        IterEvents E;
+        hipEvent_t e_en0, e_en1;
+        CIPC_HIP(hipEventCreate(&e_en0));
+        CIPC_HIP(hipEventCreate(&e_en1));
+        bool graphed = false;
+        for (int round = 0;; ++round) {
+            graphed = round == 0 && iteration_graph(c, kappa[0], false, E.ev[0], E.ev[3]);
+            if (!graphed) enqueue_iteration(c, kappa[0], false, &E);
+            CIPC_HIP(hipEventSynchronize(E.ev[3]));
+            if (!grow_capacities(*c)) break;
+            if (round >= 3) throw Error(CIPC_ERR_RUNTIME, "active-set capacities kept overflowing");
+        }
+        settle_active_set(*c);
         settle_nnz(*c);
+        c->assembled = true;
+        std::vector<double> sb(5 * (size_t)S), et(9 * (size_t)S);
+        CIPC_HIP(hipEventRecord(e_en0, c->stream));
+        energy_terms_scenes(*c, et.data());          // synchronises the main stream
+        CIPC_HIP(hipEventRecord(e_en1, c->stream));
+        CIPC_HIP(hipEventSynchronize(e_en1));
         step_bound_finish(*c, sb.data());
-        float ms[4];
-        for (int i = 0; i < 2; ++i) CIPC_HIP(hipEventElapsedTime(&ms[i], ev[i], ev[i + 1]));
-        CIPC_HIP(hipEventElapsedTime(&ms[2], s0, s1));
-        CIPC_HIP(hipEventElapsedTime(&ms[3], ev[2], ev[3]));
-        float tot = 0, el = 0;
-        CIPC_HIP(hipEventElapsedTime(&tot, ev[0], ev[4]));
-        CIPC_HIP(hipEventElapsedTime(&el, es, ee));
-        for (auto& e : ev) (void)hipEventDestroy(e);
-        for (hipEvent_t e : {es, ee, s0, s1}) (void)hipEventDestroy(e);
-        t->broadphase_ms = ms[0]; t->assembly_ms = ms[1]; t->elastic_ms = el; t->ccd_ms = ms[2]; t->energy_ms = ms[3];
-        t->total_ms = tot;
+        *t = cipc_iteration_timing{};
+        float tot = 0, en = 0;
+        CIPC_HIP(hipEventElapsedTime(&tot, E.ev[0], E.ev[3]));
+        CIPC_HIP(hipEventElapsedTime(&en, e_en0, e_en1));
+        if (!graphed) {
+            float ms[3], el = 0;
+            for (int i = 0; i < 2; ++i) CIPC_HIP(hipEventElapsedTime(&ms[i], E.ev[i], E.ev[i + 1]));
+            CIPC_HIP(hipEventElapsedTime(&ms[2], E.s0, E.s1));
+            CIPC_HIP(hipEventElapsedTime(&el, E.es, E.ee));
+            t->broadphase_ms = ms[0]; t->assembly_ms = ms[1]; t->elastic_ms = el; t->ccd_ms = ms[2];
+        }
+        (void)hipEventDestroy(e_en0);
+        (void)hipEventDestroy(e_en1);
+        t->energy_ms = en;
+        t->total_ms = tot + en;
+        t->graph = graphed ? 1 : 0;
         double amin = 1.0, esum = 0.0, bmin[4] = {1.0, 1.0, 1.0, 1.0};
         for (int k = 0; k < S; ++k) {
             const double a = sb[5 * (size_t)k], e = total_energy(*c, kappa[k], et.data() + 9 * (size_t)k);
@@ -982,7 +1126,13 @@ int cipc_gpu_autotune(cipc_gpu_context* c, cipc_tune_entry* entries, int max_ent
 int cipc_gpu_set_tune(cipc_gpu_context* c, int kernel, int choice) {
     return guard([&] {
         require(c != nullptr, "null context");
-        require(kernel == 0 || kernel == 1, "kernel is 0 (point-triangle CCD) or 1 (edge-edge CCD)");
+        require(kernel >= 0 && kernel <= 2, "kernel is 0 (point-triangle CCD), 1 (edge-edge CCD) or 2 (iteration graph)");
+        if (kernel == 2) {
+            require(choice == 0 || choice == 1, "the iteration graph is 0 (off) or 1 (on)");
+            c->graph.enabled = choice == 1;
+            if (!c->graph.enabled) drop_graph(c);
+            return;
+        }
         require(choice >= 0 && choice < kTuneSplits, "choice is 0..4 (1 to 16 lanes per query)");
         (kernel == 0 ? c->tune.ccd_pt_bits : c->tune.ccd_ee_bits) = choice;
     });
@@ -1019,53 +1169,38 @@ int cipc_gpu_newton_iteration(cipc_gpu_context* c, double kappa, cipc_iteration_
         require(c && c->have_scene && t, "no scene uploaded or null output");
         require_single(c, "cipc_gpu_newton_iteration");
         bind(*c);
-        // Three streams.  Main: active set, barrier blocks, CSR.  Second (`aux`): the elastic
-        // stage beside the active set, the friction records and small projections beside the
-        // barrier blocks, then the energy.  Step-bound stream (`sbs`): the ground bound, swept
-        // trees and full CCD (positions and direction only) from the start, the partial CCD
-        // once the active set exists.  Every result comes back through pinned memory read
-        // after one synchronisation.
-        hipEvent_t ev[4], es, ee, e0, en, s0, s1;
-        for (auto& e : ev) CIPC_HIP(hipEventCreate(&e));
-        for (hipEvent_t* e : {&es, &ee, &e0, &en, &s0, &s1}) CIPC_HIP(hipEventCreate(e));
-        CIPC_HIP(hipEventRecord(ev[0], c->stream));
-        elastic_fork(c, es, ee);
-        // the ground bound, the swept trees and the full CCD need only x and p: from the start
-        CIPC_HIP(hipStreamWaitEvent(c->sbs, ev[0], 0));
-        CIPC_HIP(hipEventRecord(s0, c->sbs));
-        step_bound_early(*c, c->sbs);
-        compute_active_set(*c);
-        CIPC_HIP(hipEventRecord(ev[1], c->stream));
-        CIPC_HIP(hipEventRecord(c->as_done, c->stream));
-        CIPC_HIP(hipStreamWaitEvent(c->sbs, c->as_done, 0));
-        step_bound_partial(*c, c->sbs);
-        CIPC_HIP(hipEventRecord(s1, c->sbs));
-        contact_assemble(*c, kappa);
-        CIPC_HIP(hipEventRecord(e0, c->aux));
-        energy_terms_launch(*c, c->aux);
-        CIPC_HIP(hipEventRecord(en, c->aux));
-        elastic_join(c);
-        finalize_csr(*c);
-        CIPC_HIP(hipEventRecord(ev[2], c->stream));
-        CIPC_HIP(hipStreamWaitEvent(c->stream, s1, 0));
-        CIPC_HIP(hipStreamWaitEvent(c->stream, en, 0));
-        CIPC_HIP(hipEventRecord(ev[3], c->stream));
-        CIPC_HIP(hipEventSynchronize(ev[3]));
+        // No stage waits on the host: the active set's counts stay on the device, and the
+        // one synchronisation at the end also reports whether a capacity was exceeded (then
+        // the capacity is grown and the iteration enqueued again, eagerly).  The launches
+        // go as one graph when they can (stage times are then not taken: zero).
+        IterEvents E;
+        bool graphed = false;
+        for (int round = 0;; ++round) {
+            graphed = round == 0 && iteration_graph(c, kappa, true, E.ev[0], E.ev[3]);
+            if (!graphed) enqueue_iteration(c, kappa, true, &E);
+            CIPC_HIP(hipEventSynchronize(E.ev[3]));
+            if (!grow_capacities(*c)) break;
+            if (round >= 3) throw Error(CIPC_ERR_RUNTIME, "active-set capacities kept overflowing");
+        }
+        settle_active_set(*c);
         settle_nnz(*c);
         c->assembled = true;
         double sb[5], et[9];
         step_bound_finish(*c, sb);
         energy_terms_finish(*c, et);
-        float ms[4], tot = 0, el = 0;
-        for (int i = 0; i < 2; ++i) CIPC_HIP(hipEventElapsedTime(&ms[i], ev[i], ev[i + 1]));
-        CIPC_HIP(hipEventElapsedTime(&ms[2], s0, s1));  // step bound, beside the assembly
-        CIPC_HIP(hipEventElapsedTime(&ms[3], e0, en));  // energy, beside the assembly's tail
-        CIPC_HIP(hipEventElapsedTime(&tot, ev[0], ev[3]));
-        CIPC_HIP(hipEventElapsedTime(&el, es, ee));
-        for (auto& e : ev) (void)hipEventDestroy(e);
-        for (hipEvent_t e : {es, ee, e0, en, s0, s1}) (void)hipEventDestroy(e);
-        t->broadphase_ms = ms[0]; t->assembly_ms = ms[1]; t->elastic_ms = el; t->ccd_ms = ms[2]; t->energy_ms = ms[3];
+        float tot = 0;
+        CIPC_HIP(hipEventElapsedTime(&tot, E.ev[0], E.ev[3]));
+        *t = cipc_iteration_timing{};
+        if (!graphed) {
+            float ms[4], el = 0;
+            for (int i = 0; i < 2; ++i) CIPC_HIP(hipEventElapsedTime(&ms[i], E.ev[i], E.ev[i + 1]));
+            CIPC_HIP(hipEventElapsedTime(&ms[2], E.s0, E.s1));  // step bound, beside the assembly
+            CIPC_HIP(hipEventElapsedTime(&ms[3], E.e0, E.en));  // energy, beside the assembly's tail
+            CIPC_HIP(hipEventElapsedTime(&el, E.es, E.ee));
+            t->broadphase_ms = ms[0]; t->assembly_ms = ms[1]; t->elastic_ms = el; t->ccd_ms = ms[2]; t->energy_ms = ms[3];
+        }
         t->total_ms = tot;
+        t->graph = graphed ? 1 : 0;
         t->alpha = sb[0];
         t->energy = total_energy(*c, kappa, et);
         t->nnz = c->nnz;

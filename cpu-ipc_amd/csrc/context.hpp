@@ -45,7 +45,27 @@ struct LBVH {
 constexpr int kMaxRefits = 32;
 
 // Scratch of the active-set construction (broadphase.hip).
+// Device counts of the active set and of the contact records (Context::dcnt): written by the
+// kernels that find them, read by every later kernel from device memory, and by the host only
+// at its next synchronisation (settle_active_set), so that no stage waits on the host.
+enum : int {
+    kDcPT = 0,        // point-triangle candidate hits
+    kDcEE,            // edge-edge candidate hits
+    kDcSelf,          // Self_ActiveSet entries
+    kDcMol,           // Self_EE_ActiveSet entries
+    kDcCCD,           // partial-CCD pairs
+    kDcEnv,           // Environment_ActiveSet entries
+    kDcFlags,         // overflow bits of the active set: 1 PT hits, 2 EE hits, 4 map keys
+    kDcContactOver,   // 1 when n_self + n_ee exceeds the contact records' capacity
+    kDcMapKeys,       // point-point / point-edge map keys before merging
+    kDcA, kDcA2, kDcAK, kDcB, kDcB2, kDcBK,   // per-category totals: the scatters' bases
+    kDcWords = 16
+};
+
 struct ActiveSetScratch {
+    // capacities of the candidate hits (per type) and of the map keys; grown by
+    // settle_active_set when the device reports more
+    int capPT = 0, capEE = 0, capM = 0;
     DBuf<int> cntPT, offPT, cntEE, offEE, cntpPT, cntpEE, bufPT, bufEE;
     DBuf<int> candUPT, candUEE, candPT, candEE, ownPT, ownEE, catPT, catEE;
     DBuf<int4> recPT, recEE;
@@ -102,7 +122,20 @@ struct Context {
     // the trees (the assembly does not read them) and sorts with its own temporary storage.
     hipStream_t sbs = nullptr;
     hipEvent_t as_done = nullptr, sb_done = nullptr;
+    // start of an iteration (the step-bound stream forks there) and end of its energy (joins
+    // the main stream): a captured iteration forks and joins through these
+    hipEvent_t it_start = nullptr, en_done = nullptr;
     DBuf<unsigned char> tmp_sb;
+    // The Newton iteration's launches as one HIP graph (context.hip), replayed while nothing
+    // they depend on changes (the key: stiffness, precision, capacities, lagged and soft
+    // counts, launch shapes, allocations, tree topologies), else captured again.
+    struct IterGraph {
+        hipGraphExec_t exec = nullptr;
+        std::vector<unsigned char> key;
+        bool enabled = true;
+        long long captures = 0, replays = 0;
+    } graph;
+    unsigned long long topo_gen = 0;   // tree topologies built (broadphase.hip)
     // tile choices of the autotuner (ccd.hip): lanes per query of the full-CCD walks (2^bits)
     struct Tune {
         int ccd_pt_bits = 3, ccd_ee_bits = 3;
@@ -120,8 +153,12 @@ struct Context {
     DBuf<unsigned short> sbf;  // static block nonzero-contribution masks
     DBuf<double> grad;         // 3V
 
-    // active set
+    // active set (host copies of the device counts, valid after settle_active_set)
     int64_t n_self = 0, n_ee = 0, n_ccd = 0, n_env = 0;
+    DBuf<int> dcnt;            // kDcWords device counts
+    PinnedBuf<int> pin_dc;
+    bool dc_pending = false;   // the device counts are newer than the host copies
+    int capS = 0;              // contact (self + mollified) capacity of the contact records
     DBuf<int4> self_set, ee_set;
     DBuf<int2> ee_ie, ccd_set;
     DBuf<int> env_set;
@@ -230,7 +267,23 @@ void build_tree(Context& c, LBVH& t, int n);
 void build_static_trees(Context& c);
 // on stream st (the step bound may run beside the assembly); temporary storage of its own
 void build_swept_trees(Context& c, const unsigned long long* abits, hipStream_t st);
+// Enqueues the active set on the main stream; its counts stay on the device.
 void compute_active_set(Context& c);
+// Reads the device counts (one synchronisation of the main stream) into the host copies.
+// When a capacity was exceeded it is grown from the count the device found and the active
+// set is built again.  Returns true when it was rebuilt (work enqueued after the first build
+// then has to be enqueued again).
+bool settle_active_set(Context& c);
+// compute_active_set + settle_active_set
+void active_set_now(Context& c);
+// After a synchronisation that covers the last copy of the device counts (the active set's or
+// the contact records'): grows every capacity that the device reports exceeded; true when one
+// grew (the work that used it has to be enqueued again).
+bool grow_capacities(Context& c);
+// Device counts from the host copies (after the sets were uploaded or reset).
+void upload_counts(Context& c);
+// Contact-record capacity for the next assembly (at least the settled contact count).
+void size_contact_records(Context& c);
 void sort_u64(Context& c, const unsigned long long* in, unsigned long long* out, int n, int bits);
 void sort_u64_pairs(Context& c, const unsigned long long* kin, unsigned long long* kout,
                     const int* vin, int* vout, int n, int bits);

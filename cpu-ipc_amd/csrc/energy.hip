@@ -73,9 +73,12 @@ __global__ __launch_bounds__(kEB) void e_inertia(int V, const double* __restrict
     }
     block_sum_store(acc, part, 0);
 }
-__global__ __launch_bounds__(kEB) void e_ground(int n, const int* __restrict__ env, const double* __restrict__ x, d3 gn, double gd,
-                                                double hhat, double* __restrict__ part, double* __restrict__ elem) {
+// The contact terms take their count from device memory (ndev, the active set's device
+// count) when it is given, else n.
+__global__ __launch_bounds__(kEB) void e_ground(int n, const int* __restrict__ ndev, const int* __restrict__ env, const double* __restrict__ x,
+                                                d3 gn, double gd, double hhat, double* __restrict__ part, double* __restrict__ elem) {
     double acc = 0.0;
+    if (ndev) n = *ndev;
     for (int i = blockIdx.x * kEB + threadIdx.x; i < n; i += gridDim.x * kEB) {
         const double dist = dot(gn, ld3(x, env[i])) - gd;
         const double b = barrier_b(dist * dist, hhat);
@@ -91,9 +94,10 @@ __device__ inline double self_d(int4 c, const double* x) {
     if (c.w < 0) return d_PE(p, ld3(x, c.y), ld3(x, c.z));
     return d_PT(p, ld3(x, c.y), ld3(x, c.z), ld3(x, c.w));
 }
-__global__ __launch_bounds__(kEB) void e_self(int n, const int4* __restrict__ set, const double* __restrict__ x, double hhat,
-                                              double* __restrict__ part, double* __restrict__ elem) {
+__global__ __launch_bounds__(kEB) void e_self(int n, const int* __restrict__ ndev, const int4* __restrict__ set, const double* __restrict__ x,
+                                              double hhat, double* __restrict__ part, double* __restrict__ elem) {
     double acc = 0.0;
+    if (ndev) n = *ndev;
     for (int i = blockIdx.x * kEB + threadIdx.x; i < n; i += gridDim.x * kEB) {
         const int4 c = set[i];
         double b = barrier_b(self_d(c, x), hhat);
@@ -103,11 +107,12 @@ __global__ __launch_bounds__(kEB) void e_self(int n, const int4* __restrict__ se
     }
     block_sum_store(acc, part, 0);
 }
-__global__ __launch_bounds__(kEB) void e_mollified(int n, const int4* __restrict__ set, const int2* __restrict__ ie,
-                                                   const int2* __restrict__ se, const double* __restrict__ x,
+__global__ __launch_bounds__(kEB) void e_mollified(int n, const int* __restrict__ ndev, const int4* __restrict__ set,
+                                                   const int2* __restrict__ ie, const int2* __restrict__ se, const double* __restrict__ x,
                                                    const double* __restrict__ vrest, double hhat, double* __restrict__ part,
                                                    double* __restrict__ elem) {
     double acc = 0.0;
+    if (ndev) n = *ndev;
     for (int i = blockIdx.x * kEB + threadIdx.x; i < n; i += gridDim.x * kEB) {
         const int4 c = set[i];
         int e4[4];
@@ -169,10 +174,12 @@ void energy_partials_launch(Context& c, hipStream_t st, double* P) {
                        nullptr);
     hipLaunchKernelGGL(e_inertia, dim3(kEG), dim3(kEB), 0, st, s.V, s.x.p, s.xt.p, s.vprev.p, s.mass.p, s.drag, P + kEG,
                        nullptr);
-    hipLaunchKernelGGL(e_ground, dim3(kEG), dim3(kEB), 0, st, (int)c.n_env, c.env_set.p, s.x.p, s.gn, s.gd, s.hhat, P + 2 * kEG,
+    // the contact counts are the active set's device counts
+    const int* dc = c.dcnt.p;
+    hipLaunchKernelGGL(e_ground, dim3(kEG), dim3(kEB), 0, st, 0, dc + kDcEnv, c.env_set.p, s.x.p, s.gn, s.gd, s.hhat, P + 2 * kEG,
                        nullptr);
-    hipLaunchKernelGGL(e_self, dim3(kEG), dim3(kEB), 0, st, (int)c.n_self, c.self_set.p, s.x.p, s.hhat, P + 3 * kEG, nullptr);
-    hipLaunchKernelGGL(e_mollified, dim3(kEG), dim3(kEB), 0, st, (int)c.n_ee, c.ee_set.p, c.ee_ie.p, s.se.p, s.x.p, s.vrest.p,
+    hipLaunchKernelGGL(e_self, dim3(kEG), dim3(kEB), 0, st, 0, dc + kDcSelf, c.self_set.p, s.x.p, s.hhat, P + 3 * kEG, nullptr);
+    hipLaunchKernelGGL(e_mollified, dim3(kEG), dim3(kEB), 0, st, 0, dc + kDcMol, c.ee_set.p, c.ee_ie.p, s.se.p, s.x.p, s.vrest.p,
                        s.hhat, P + 4 * kEG, nullptr);
     // lagged friction: self part (lambda times clamped norm) and ground part (mu included)
     friction_energy_partials(c, P + 5 * kEG, nullptr, nullptr, st);
@@ -368,10 +375,10 @@ void energy_terms_scenes(Context& c, double* out) {
     double* dev_out = eFg + nFg;
     hipLaunchKernelGGL(e_snh, dim3(elem_grid(s.T)), dim3(kEB), 0, c.stream, s.T, s.tets.p, s.x.p, s.dminv.p, s.vol.p, scene_lame(s), nullptr, eT);
     hipLaunchKernelGGL(e_inertia, dim3(elem_grid(s.V)), dim3(kEB), 0, c.stream, s.V, s.x.p, s.xt.p, s.vprev.p, s.mass.p, s.drag, nullptr, eV);
-    if (nG) hipLaunchKernelGGL(e_ground, dim3(elem_grid(nG)), dim3(kEB), 0, c.stream, (int)nG, c.env_set.p, s.x.p, s.gn, s.gd, s.hhat, nullptr, eG);
-    if (nS) hipLaunchKernelGGL(e_self, dim3(elem_grid(nS)), dim3(kEB), 0, c.stream, (int)nS, c.self_set.p, s.x.p, s.hhat, nullptr, eS);
+    if (nG) hipLaunchKernelGGL(e_ground, dim3(elem_grid(nG)), dim3(kEB), 0, c.stream, (int)nG, nullptr, c.env_set.p, s.x.p, s.gn, s.gd, s.hhat, nullptr, eG);
+    if (nS) hipLaunchKernelGGL(e_self, dim3(elem_grid(nS)), dim3(kEB), 0, c.stream, (int)nS, nullptr, c.self_set.p, s.x.p, s.hhat, nullptr, eS);
     if (nE)
-        hipLaunchKernelGGL(e_mollified, dim3(elem_grid(nE)), dim3(kEB), 0, c.stream, (int)nE, c.ee_set.p, c.ee_ie.p, s.se.p, s.x.p, s.vrest.p,
+        hipLaunchKernelGGL(e_mollified, dim3(elem_grid(nE)), dim3(kEB), 0, c.stream, (int)nE, nullptr, c.ee_set.p, c.ee_ie.p, s.se.p, s.x.p, s.vrest.p,
                            s.hhat, nullptr, eE);
     if (nFs + nFg) friction_energy_partials(c, nullptr, eFs, eFg, c.stream);
     CIPC_CHECK_LAUNCH();

@@ -146,10 +146,7 @@ struct Stepper {
         return total_energy(c, kappa, t);
     }
 
-    void active_set() {
-        compute_active_set(c);
-        CIPC_HIP(hipStreamSynchronize(c.stream));
-    }
+    void active_set() { active_set_now(c); }
 
     void step(double alpha) {
         step_forward(c, alpha);

@@ -255,9 +255,10 @@ __global__ void masked_rhs(int V, const int* __restrict__ bt, const double* __re
 // Collision count as compute_g_dpt returns it (ContactMechanics.cpp:2970-3032): the
 // multiplicity of a point-point / point-edge entry, one for every other Self_ActiveSet
 // entry.  One work-group, integer sums: the result does not depend on the order.
-__global__ void collision_count(int n, const int4* __restrict__ set, int* __restrict__ out) {
+__global__ void collision_count(const int* __restrict__ n_dev, const int4* __restrict__ set, int* __restrict__ out) {
     __shared__ long long sh[256];
     long long acc = 0;
+    const int n = *n_dev;
     for (int i = threadIdx.x; i < n; i += 256) {
         const int4 e = set[i];
         acc += (e.x < 0 && (e.z < 0 || e.w < 0)) ? -(long long)e.w : 1ll;
@@ -269,6 +270,11 @@ __global__ void collision_count(int n, const int4* __restrict__ set, int* __rest
         __syncthreads();
     }
     if (threadIdx.x == 0) out[0] = (int)sh[0];
+}
+
+__global__ void zero_ints(int* __restrict__ p, int n) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = 0;
 }
 
 void finalize_csr(Context& c) {
@@ -299,7 +305,8 @@ void finalize_csr(Context& c) {
         CIPC_CHECK_LAUNCH();
         CIPC_HIP(hipMemcpyAsync(c.counters.p, pos + nb, sizeof(int), hipMemcpyDeviceToDevice, c.stream));
     } else {
-        CIPC_HIP(hipMemsetAsync(c.counters.p, 0, sizeof(int), c.stream));
+        hipLaunchKernelGGL(zero_ints, dim3(1), dim3(64), 0, c.stream, c.counters.p, 1);
+        CIPC_CHECK_LAUNCH();
     }
     // contact gradient records
     const int ng = (int)c.gkey.n;
@@ -339,7 +346,7 @@ void finalize_csr(Context& c) {
     c.pin_nnz.reserve(2);
     CIPC_HIP(hipMemcpyAsync(c.pin_nnz.p, c.outer.p + 3 * V, sizeof(int), hipMemcpyDeviceToHost, c.stream));
     c.coll.resize(1);
-    hipLaunchKernelGGL(collision_count, dim3(1), dim3(256), 0, c.stream, (int)c.n_self, c.self_set.p, c.coll.p);
+    hipLaunchKernelGGL(collision_count, dim3(1), dim3(256), 0, c.stream, c.dcnt.p + kDcSelf, c.self_set.p, c.coll.p);
     CIPC_CHECK_LAUNCH();
     CIPC_HIP(hipMemcpyAsync(c.pin_nnz.p + 1, c.coll.p, sizeof(int), hipMemcpyDeviceToHost, c.stream));
     c.nnz_pending = true;

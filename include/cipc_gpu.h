@@ -99,6 +99,9 @@ typedef struct {
     int64_t collisions;   /* compute_g_dpt's return value (ContactMechanics.cpp:2970-3032), as cipc_gpu_assemble */
     /* the parts of alpha, as cipc_gpu_step_bound's out[1..4]: ground bound, partial CCD, CFL, full CCD */
     double bound_ground, bound_partial, bound_cfl, bound_full;
+    /* 1 when the stages ran as one HIP graph launch (then only total_ms and, for a batch,
+     * energy_ms are timed; the other stage times are zero) */
+    int graph;
 } cipc_iteration_timing;
 
 /* Result of one Newton linear solve. */
@@ -275,7 +278,10 @@ int cipc_gpu_step_bound(cipc_gpu_context* ctx, double out[5]);
  * and keeps the fastest for every later step bound of the context.  Every split returns
  * the same step bound, bit for bit: each walk's result is a minimum over pairs.  entries
  * (may be NULL) receives up to max_entries records, n_entries the number of tuned kernels.
- * cipc_gpu_set_tune sets a choice directly (the default is 3, 8 lanes per query). */
+ * cipc_gpu_set_tune sets a choice directly (the default is 3, 8 lanes per query); its
+ * kernel 2 switches the Newton iteration's single graph launch (cipc_gpu_newton_iteration,
+ * cipc_gpu_batch_newton_iteration) off (choice 0: every stage launched on its own and timed)
+ * or on (1, the default). */
 typedef struct {
     int kernel, chosen, n_candidates;
     double ms[8]; /* mean ms of each candidate over three timed runs */
@@ -292,7 +298,11 @@ int cipc_gpu_is_intersected(cipc_gpu_context* ctx, int* intersected);
 
 /* One Newton iteration's non-linear stages on the device-resident state, timed per stage:
  * active set, assembly, step bound, energy.  The assembled system is then the one that
- * cipc_gpu_solve / cipc_gpu_get_csr see, as after cipc_gpu_assemble. */
+ * cipc_gpu_solve / cipc_gpu_get_csr see, as after cipc_gpu_assemble.  No stage waits on the
+ * host (the active set's counts stay in device memory; one synchronisation ends the call),
+ * so the stages are captured once into a HIP graph and replayed by later calls while the
+ * stiffness, the buffers' capacities, the lagged friction counts and the tree topologies stay
+ * the same. */
 int cipc_gpu_newton_iteration(cipc_gpu_context* ctx, double kappa, cipc_iteration_timing* timing);
 
 /* Velocities of the scene's vertices (mesh3D::velocities, 3V; zero after an upload) and the

@@ -81,8 +81,10 @@ def test_set_tune_rejects_bad_choices(gpu):
     g = gpu.GpuContext(scenes.cube10())
     try:
         with pytest.raises(ValueError):
-            g.set_tune(2, 0)
+            g.set_tune(3, 0)
         with pytest.raises(ValueError):
             g.set_tune(0, 5)
+        with pytest.raises(ValueError):
+            g.set_tune(2, 2)      # the iteration graph is off (0) or on (1)
     finally:
         g.close()

@@ -22,24 +22,27 @@ def _ctx(s):
 
 
 def test_noalloc_energy_matches_energy_terms():
-    import torch
     from cipc import gpu, scenes
+    from hipmem import DeviceArray
     L = gpu.load()
-    out = torch.full((9,), float("nan"), dtype=torch.float64, device="cuda")
+    out = DeviceArray(9)
     big = scenes.stacked_cubes(2, 8, 0.1, 3e-4, seed=5, rotate="yaw", velocity_scale=0.3, name="big")
     small = scenes.stacked_cubes(1, 3, 0.1, 3e-4, seed=6, rotate="yaw", velocity_scale=0.3, name="small")
-    for s in (big, small):
-        g, kappa = _ctx(s)
-        try:
-            want = np.array(g.energy_terms(kappa))
-            torch.cuda.synchronize()
-            gpu._check(L.cipc_gpu_energy_terms_noalloc(g.h, kappa, C.c_void_p(out.data_ptr())))
-            got = out.cpu().numpy()
-            print(s.name, got)
-            assert np.array_equal(got, want), (s.name, got, want)
-            assert want[2] > 0                           # the ground barrier is in it
-        finally:
-            g.close()
+    try:
+        for s in (big, small):
+            g, kappa = _ctx(s)
+            try:
+                out.fill(np.nan)
+                want = np.array(g.energy_terms(kappa))
+                gpu._check(L.cipc_gpu_energy_terms_noalloc(g.h, kappa, out.ptr))
+                got = out.numpy()
+                print(s.name, got)
+                assert np.array_equal(got, want), (s.name, got, want)
+                assert want[2] > 0                           # the ground barrier is in it
+            finally:
+                g.close()
+    finally:
+        out.close()
 
 
 def test_noalloc_energy_refuses_host_output_and_batches():
@@ -54,10 +57,11 @@ def test_noalloc_energy_refuses_host_output_and_batches():
     finally:
         g.close()
     b = gpu.BatchContext(scenes.batched_cubes(2, seed=1, landed=True))
-    import torch
-    out = torch.zeros(9, dtype=torch.float64, device="cuda")
+    from hipmem import DeviceArray
+    out = DeviceArray(9)
     try:
         with pytest.raises(gpu.CipcLogicError):
-            gpu._check(L.cipc_gpu_energy_terms_noalloc(b.h, 1.0, C.c_void_p(out.data_ptr())))
+            gpu._check(L.cipc_gpu_energy_terms_noalloc(b.h, 1.0, out.ptr))
     finally:
         b.close()
+        out.close()
