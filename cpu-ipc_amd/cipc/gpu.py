@@ -464,13 +464,15 @@ class GpuContext:
         keeps the fastest; returns {kernel: (chosen, [mean ms per candidate])}."""
         if p is not None:
             self.set_direction(p)
-        e = (TuneEntry * 2)()
+        e = (TuneEntry * 4)()
         n = C.c_int()
-        _check(load().cipc_gpu_autotune(self.h, e, 2, C.byref(n)))
-        names = ("full_ccd_pt", "full_ccd_ee")
+        _check(load().cipc_gpu_autotune(self.h, e, 4, C.byref(n)))
+        names = {0: "full_ccd_pt", 1: "full_ccd_ee", 3: "active_pt", 4: "active_ee"}
         return {names[e[k].kernel]: (e[k].chosen, list(e[k].ms[:e[k].n_candidates])) for k in range(n.value)}
 
     def set_tune(self, kernel, choice):
+        """kernel 0 / 1: full-CCD walk splits, 3 / 4: the active set's walk splits (2^choice
+        lanes per query); kernel 2: the iteration graph off (0) or on (1)."""
         _check(load().cipc_gpu_set_tune(self.h, kernel, choice))
 
     def save_positions(self):

@@ -446,50 +446,58 @@ enum : int { kNone = 0, kSelfPT = 1, kMap = 2, kSelfEE = 3, kMolPE = 4, kMolEE =
 // is the box test of the reference's candidate filter.  Queries are taken in the Morton
 // order of their rest positions (perm) so that the lanes of a wave walk nearby subtrees.
 constexpr int kPartCap = 32;
-static_assert(64 % kParts == 0, "a query's lanes must share a wavefront");
+// The split (2^PB lanes per query, PB = 0..4) is a template argument, chosen per context by
+// the autotuner; the hits and their order do not depend on it (they are ranked by primitive).
+constexpr int kMaxSplitBits = 4;
+static_assert(64 % (1 << kMaxSplitBits) == 0, "a query's lanes must share a wavefront");
 
+template <int PB>
 __device__ inline int sum_over_parts(int v) {
 #pragma unroll
-    for (int m = 1; m < kParts; m <<= 1) v += __shfl_xor(v, m, kParts);
+    for (int m = 1; m < (1 << PB); m <<= 1) v += __shfl_xor(v, m, 1 << PB);
     return v;
 }
 
+template <int PB>
 __global__ __launch_bounds__(64) void pt_traverse(int Vs, const int* __restrict__ perm, const int* __restrict__ sv,
                                                    const double* __restrict__ x, TreeView tv, double sq, int* __restrict__ cnt,
                                                    int* __restrict__ cntp, int* __restrict__ buf) {
+    constexpr int P = 1 << PB;
     const int t = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
-    const bool active = t < Vs * kParts;
-    const int s = active ? perm[t / kParts] : 0, part = t % kParts;
+    const bool active = t < Vs * P;
+    const int s = active ? perm[t >> PB] : 0, part = t & (P - 1);
     __shared__ int stk[kStack * 64];
     int n = 0;
     if (active) {
-        const Box pb = box_pt(ld3(x, sv[s]));
-        int* b = buf + ((size_t)s * kParts + part) * kPartCap;
-        part_walk(tv, part, pb, sq, stk + threadIdx.x, [&](int f) { if (n < kPartCap) b[n] = f; ++n; }, [&] { n = 0; });
-        cntp[s * kParts + part] = n;
+        const Box qb = box_pt(ld3(x, sv[s]));
+        int* b = buf + ((size_t)s * P + part) * kPartCap;
+        part_walk(tv, part, qb, sq, stk + threadIdx.x, [&](int f) { if (n < kPartCap) b[n] = f; ++n; }, [&] { n = 0; }, PB);
+        cntp[s * P + part] = n;
     }
-    const int tot = sum_over_parts(n);
+    const int tot = sum_over_parts<PB>(n);
     if (active && part == 0) cnt[s] = tot;
     if (blockIdx.x == 0 && threadIdx.x == 0) cnt[Vs] = 0;   // the scan's tail
 }
 
+template <int PB>
 __global__ __launch_bounds__(64) void pt_compact(int Vs, const int* __restrict__ perm, const int* __restrict__ sv, const double* __restrict__ x, TreeView tv,
                            double sq, const int* __restrict__ off, const int* __restrict__ cntp, const int* __restrict__ buf,
                            int* __restrict__ candU, int* __restrict__ own, const int* __restrict__ dc) {
+    constexpr int P = 1 << PB;
     __shared__ int stk[kStack * 64];
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= Vs || dc[kDcFlags]) return;   // over capacity: the host builds the set again
     const int s = perm[t];
     const int o = off[s];
     int pos = o;
-    for (int part = 0; part < kParts; ++part) {
-        const int c = cntp[s * kParts + part];
+    for (int part = 0; part < P; ++part) {
+        const int c = cntp[s * P + part];
         if (c <= kPartCap) {
-            for (int k = 0; k < c; ++k) candU[pos + k] = buf[((size_t)s * kParts + part) * kPartCap + k];
+            for (int k = 0; k < c; ++k) candU[pos + k] = buf[((size_t)s * P + part) * kPartCap + k];
         } else {
-            const Box pb = box_pt(ld3(x, sv[s]));
+            const Box qb = box_pt(ld3(x, sv[s]));
             int m = 0;
-            part_walk(tv, part, pb, sq, stk + threadIdx.x, [&](int f) { if (m < c) candU[pos + m] = f; ++m; }, [&] { m = 0; });
+            part_walk(tv, part, qb, sq, stk + threadIdx.x, [&](int f) { if (m < c) candU[pos + m] = f; ++m; }, [&] { m = 0; }, PB);
         }
         pos += c;
     }
@@ -549,48 +557,52 @@ __global__ void pt_classify(const int* __restrict__ dc, int F, const int* __rest
     }
 }
 
+template <int PB>
 __global__ __launch_bounds__(64) void ee_traverse(int Es, const int* __restrict__ perm, const int2* __restrict__ se,
                                                    const double* __restrict__ x, TreeView tv, double sq, int* __restrict__ cnt,
                                                    int* __restrict__ cntp, int* __restrict__ buf) {
+    constexpr int P = 1 << PB;
     const int t = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
-    const bool active = t < Es * kParts;
-    const int eI = active ? perm[t / kParts] : 0, part = t % kParts;
+    const bool active = t < Es * P;
+    const int eI = active ? perm[t >> PB] : 0, part = t & (P - 1);
     __shared__ int stk[kStack * 64];
     int n = 0;
     if (active) {
         const int2 A = se[eI];
         const Box b1 = box_edge(ld3(x, A.x), ld3(x, A.y));
-        int* b = buf + ((size_t)eI * kParts + part) * kPartCap;
+        int* b = buf + ((size_t)eI * P + part) * kPartCap;
         part_walk(tv, part, b1, sq, stk + threadIdx.x, [&](int eJ) {
             if (eJ > eI) { if (n < kPartCap) b[n] = eJ; ++n; }
-        }, [&] { n = 0; });
-        cntp[eI * kParts + part] = n;
+        }, [&] { n = 0; }, PB);
+        cntp[eI * P + part] = n;
     }
-    const int tot = sum_over_parts(n);
+    const int tot = sum_over_parts<PB>(n);
     if (active && part == 0) cnt[eI] = tot;
     if (blockIdx.x == 0 && threadIdx.x == 0) cnt[Es] = 0;   // the scan's tail
 }
 
+template <int PB>
 __global__ __launch_bounds__(64) void ee_compact(int Es, const int* __restrict__ perm, const int2* __restrict__ se, const double* __restrict__ x, TreeView tv,
                            double sq, const int* __restrict__ off, const int* __restrict__ cntp, const int* __restrict__ buf,
                            int* __restrict__ candU, int* __restrict__ own, const int* __restrict__ dc) {
+    constexpr int P = 1 << PB;
     __shared__ int stk[kStack * 64];
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= Es || dc[kDcFlags]) return;
     const int eI = perm[t];
     const int o = off[eI];
     int pos = o;
-    for (int part = 0; part < kParts; ++part) {
-        const int c = cntp[eI * kParts + part];
+    for (int part = 0; part < P; ++part) {
+        const int c = cntp[eI * P + part];
         if (c <= kPartCap) {
-            for (int k = 0; k < c; ++k) candU[pos + k] = buf[((size_t)eI * kParts + part) * kPartCap + k];
+            for (int k = 0; k < c; ++k) candU[pos + k] = buf[((size_t)eI * P + part) * kPartCap + k];
         } else {
             const int2 A = se[eI];
             const Box b1 = box_edge(ld3(x, A.x), ld3(x, A.y));
             int m = 0;
             part_walk(tv, part, b1, sq, stk + threadIdx.x, [&](int eJ) {
                 if (eJ > eI) { if (m < c) candU[pos + m] = eJ; ++m; }
-            }, [&] { m = 0; });
+            }, [&] { m = 0; }, PB);
         }
         pos += c;
     }
@@ -803,6 +815,50 @@ void category_scan(Context& c, const int* cat, int n, const int* dc, int which, 
     CIPC_HIP(hipcub::DeviceScan::ExclusiveScan(c.tmp.p, bytes, it, pos.p, Cnt4Sum(), make_int4(0, 0, 0, 0), n + 1, c.stream));
 }
 
+// The walks' per-part buffers for 2^pt and 2^ee lanes per query.
+void size_active_walks(Context& c, int pt, int ee) {
+    ActiveSetScratch& b = c.as;
+    const size_t Vs = std::max(c.sc.Vs, 1), Es = std::max(c.sc.Es, 1);
+    b.cntpPT.resize(Vs << pt); b.cntpEE.resize(Es << ee);
+    b.bufPT.resize((Vs << pt) * kPartCap); b.bufEE.resize((Es << ee) * kPartCap);
+}
+
+// The active set's walks with 2^pb lanes per query: the traversal (compact = false) or the
+// compaction of its hits (compact = true), point-triangle (pt) or edge-edge.
+void launch_active_walk(Context& c, bool pt, bool compact, int pb) {
+    Scene& s = c.sc;
+    ActiveSetScratch& b = c.as;
+    const double sq = sqrt(s.hhat);
+    const TreeView ft{c.face_tree.n, c.face_tree.nodes.p, c.face_tree.wide.p};
+    const TreeView et{c.edge_tree.n, c.edge_tree.nodes.p, c.edge_tree.wide.p};
+    const int* dc = c.dcnt.p;
+    const int n = pt ? s.Vs : s.Es;
+    if (n <= 0) return;
+    const dim3 grid(blocks_for(compact ? (size_t)n : (size_t)n << pb, 64)), block(64);
+    auto trav_pt = [&](auto k) { hipLaunchKernelGGL(k, grid, block, 0, c.stream, s.Vs, s.sv_perm.p, s.sv.p, s.x.p, ft, sq, b.cntPT.p,
+                                                    b.cntpPT.p, b.bufPT.p); };
+    auto trav_ee = [&](auto k) { hipLaunchKernelGGL(k, grid, block, 0, c.stream, s.Es, s.se_perm.p, s.se.p, s.x.p, et, sq, b.cntEE.p,
+                                                    b.cntpEE.p, b.bufEE.p); };
+    auto comp_pt = [&](auto k) { hipLaunchKernelGGL(k, grid, block, 0, c.stream, s.Vs, s.sv_perm.p, s.sv.p, s.x.p, ft, sq, b.offPT.p,
+                                                    b.cntpPT.p, b.bufPT.p, b.candUPT.p, b.ownPT.p, dc); };
+    auto comp_ee = [&](auto k) { hipLaunchKernelGGL(k, grid, block, 0, c.stream, s.Es, s.se_perm.p, s.se.p, s.x.p, et, sq, b.offEE.p,
+                                                    b.cntpEE.p, b.bufEE.p, b.candUEE.p, b.ownEE.p, dc); };
+#define CIPC_SPLIT(F, K)                 \
+    switch (pb) {                        \
+    case 0: F(K<0>); break;              \
+    case 1: F(K<1>); break;              \
+    case 2: F(K<2>); break;              \
+    case 3: F(K<3>); break;              \
+    default: F(K<4>); break;             \
+    }
+    if (pt && !compact) CIPC_SPLIT(trav_pt, pt_traverse)
+    else if (!pt && !compact) CIPC_SPLIT(trav_ee, ee_traverse)
+    else if (pt) CIPC_SPLIT(comp_pt, pt_compact)
+    else CIPC_SPLIT(comp_ee, ee_compact)
+#undef CIPC_SPLIT
+    CIPC_CHECK_LAUNCH();
+}
+
 // Buffers of the hits and of the outputs are sized by the capacities, so no stage needs a
 // count on the host: every kernel reads its count from dcnt, and does nothing once a
 // capacity is exceeded (the host then grows it and builds the set again).  Bounds: a
@@ -825,16 +881,10 @@ void compute_active_set(Context& c) {
     int* dc = c.dcnt.p;
     hipLaunchKernelGGL(dc_reset, dim3(1), dim3(64), 0, c.stream, dc);
     b.cntPT.resize(Vs + 1); b.offPT.resize(Vs + 1); b.cntEE.resize(Es + 1); b.offEE.resize(Es + 1);
-    b.cntpPT.resize((size_t)kParts * std::max(Vs, 1)); b.cntpEE.resize((size_t)kParts * std::max(Es, 1));
-    b.bufPT.resize((size_t)kParts * kPartCap * std::max(Vs, 1)); b.bufEE.resize((size_t)kParts * kPartCap * std::max(Es, 1));
+    size_active_walks(c, c.tune.as_pt_bits, c.tune.as_ee_bits);
     // A: one walk per query part
-    if (Vs > 0)
-        hipLaunchKernelGGL(pt_traverse, dim3(blocks_for((size_t)Vs * kParts, 64)), dim3(64), 0, c.stream, Vs, s.sv_perm.p, s.sv.p, s.x.p,
-                           ft, sq, b.cntPT.p, b.cntpPT.p, b.bufPT.p);
-    if (Es > 0)
-        hipLaunchKernelGGL(ee_traverse, dim3(blocks_for((size_t)Es * kParts, 64)), dim3(64), 0, c.stream, Es, s.se_perm.p, s.se.p, s.x.p,
-                           et, sq, b.cntEE.p, b.cntpEE.p, b.bufEE.p);
-    CIPC_CHECK_LAUNCH();
+    if (Vs > 0) launch_active_walk(c, true, false, c.tune.as_pt_bits);
+    if (Es > 0) launch_active_walk(c, false, false, c.tune.as_ee_bits);
     // (the traversals also zero the scans' tails)
     if (Vs == 0) hipLaunchKernelGGL(fill_tail, dim3(1), dim3(1), 0, c.stream, b.cntPT.p, Vs);
     if (Es == 0) hipLaunchKernelGGL(fill_tail, dim3(1), dim3(1), 0, c.stream, b.cntEE.p, Es);
@@ -847,14 +897,12 @@ void compute_active_set(Context& c) {
     b.candPT.resize(capPT + 1); b.recPT.resize(capPT + 1); b.catPT.resize(capPT + 1); b.ownPT.resize(capPT + 1);
     b.candEE.resize(capEE + 1); b.recEE.resize(capEE + 1); b.catEE.resize(capEE + 1); b.auxEE.resize(capEE + 1); b.ownEE.resize(capEE + 1);
     if (Vs > 0) {
-        hipLaunchKernelGGL(pt_compact, dim3(blocks_for(Vs, 64)), dim3(64), 0, c.stream, Vs, s.sv_perm.p, s.sv.p, s.x.p, ft, sq, b.offPT.p,
-                           b.cntpPT.p, b.bufPT.p, b.candUPT.p, b.ownPT.p, dc);
+        launch_active_walk(c, true, true, c.tune.as_pt_bits);
         hipLaunchKernelGGL(pt_classify, dim3(blocks_for(capPT, 128)), dim3(128), 0, c.stream, dc, s.F, b.ownPT.p, b.offPT.p, s.sv.p, s.x.p,
                            s.bt.p, s.faces.p, b.candUPT.p, b.candPT.p, s.hhat, sq, scene_of_vertex(s), b.recPT.p, b.catPT.p);
     }
     if (Es > 0) {
-        hipLaunchKernelGGL(ee_compact, dim3(blocks_for(Es, 64)), dim3(64), 0, c.stream, Es, s.se_perm.p, s.se.p, s.x.p, et, sq, b.offEE.p,
-                           b.cntpEE.p, b.bufEE.p, b.candUEE.p, b.ownEE.p, dc);
+        launch_active_walk(c, false, true, c.tune.as_ee_bits);
         hipLaunchKernelGGL(ee_classify, dim3(blocks_for(capEE, 128)), dim3(128), 0, c.stream, dc, Es, b.ownEE.p, b.offEE.p, s.se.p,
                            s.x.p, s.vrest.p, s.bt.p, b.candUEE.p, b.candEE.p, s.hhat, sq, scene_of_vertex(s), b.recEE.p, b.auxEE.p,
                            b.catEE.p);

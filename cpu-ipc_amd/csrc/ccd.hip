@@ -302,8 +302,8 @@ __global__ void copy_slot(int S, int dst, int src, unsigned long long* sl) {
     if (k < S) sl[kSlots * k + dst] = sl[kSlots * k + src];
 }
 
-// Times every split of the two full-CCD walks (1 to 16 lanes per query) on the current
-// positions and direction, three runs each after one warm-up, with HIP events on the
+// Times every split of the two full-CCD walks and of the active set's two walks (1 to 16
+// lanes per query) on the current positions and direction, three runs each after one warm-up, with HIP events on the
 // context's stream, and keeps the fastest for the context's later step bounds.  Each run
 // starts from the minimum the walk starts from in a step bound (slot 3 reseeded from the
 // swept step, slot 5, for the point-triangle walk and from the point-triangle result, slot
@@ -345,6 +345,36 @@ int autotune(Context& c, TuneEntry* out, int max_out) {
         }
         if (pt) c.tune.ccd_pt_bits = e.chosen;
         else c.tune.ccd_ee_bits = e.chosen;
+        if (n < max_out && out) out[n] = e;
+        ++n;
+    }
+    // the active set's walks, on the static trees of the current positions (the set is built
+    // once first; each walk writes only its hit counts and buffers, so runs repeat exactly)
+    active_set_now(c);
+    size_active_walks(c, kTuneSplits - 1, kTuneSplits - 1);
+    for (int k = 3; k <= 4; ++k) {
+        const bool pt = k == 3;
+        if (pt ? (s.Vs == 0 || c.face_tree.n <= 0) : (s.Es == 0 || c.edge_tree.n <= 0)) continue;
+        TuneEntry e{};
+        e.kernel = k;
+        e.n = kTuneSplits;
+        double best = 1e300;
+        for (int pb = 0; pb < kTuneSplits; ++pb) {
+            float total = 0.f;
+            for (int r = 0; r < 4; ++r) {
+                CIPC_HIP(hipEventRecord(e0, c.stream));
+                launch_active_walk(c, pt, false, pb);
+                CIPC_HIP(hipEventRecord(e1, c.stream));
+                CIPC_HIP(hipEventSynchronize(e1));
+                float ms = 0.f;
+                CIPC_HIP(hipEventElapsedTime(&ms, e0, e1));
+                if (r > 0) total += ms;
+            }
+            e.ms[pb] = total / 3.0;
+            if (e.ms[pb] < best) { best = e.ms[pb]; e.chosen = pb; }
+        }
+        if (pt) c.tune.as_pt_bits = e.chosen;
+        else c.tune.as_ee_bits = e.chosen;
         if (n < max_out && out) out[n] = e;
         ++n;
     }

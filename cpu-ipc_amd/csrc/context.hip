@@ -927,7 +927,7 @@ void enqueue_iteration(cipc_gpu_context* c, double kappa, bool energy, const Ite
 struct IterKey {
     double kappa, soft_w;
     long long nfs, nfe, gen, topo;
-    int fp32, capPT, capEE, capM, capS, nsoft, tune_pt, tune_ee, S, energy;
+    int fp32, capPT, capEE, capM, capS, nsoft, tune_pt, tune_ee, tune_apt, tune_aee, S, energy;
 };
 std::vector<unsigned char> iter_key(const cipc_gpu_context* c, double kappa, bool energy) {
     IterKey k;
@@ -935,7 +935,8 @@ std::vector<unsigned char> iter_key(const cipc_gpu_context* c, double kappa, boo
     k.kappa = kappa; k.soft_w = c->soft_w;
     k.nfs = c->n_fself; k.nfe = c->n_fenv; k.gen = (long long)alloc_generation().load(); k.topo = (long long)c->topo_gen;
     k.fp32 = c->hessian_fp32 ? 1 : 0; k.capPT = c->as.capPT; k.capEE = c->as.capEE; k.capM = c->as.capM; k.capS = c->capS;
-    k.nsoft = c->n_soft; k.tune_pt = c->tune.ccd_pt_bits; k.tune_ee = c->tune.ccd_ee_bits; k.S = c->sc.S; k.energy = energy ? 1 : 0;
+    k.nsoft = c->n_soft; k.tune_pt = c->tune.ccd_pt_bits; k.tune_ee = c->tune.ccd_ee_bits;
+    k.tune_apt = c->tune.as_pt_bits; k.tune_aee = c->tune.as_ee_bits; k.S = c->sc.S; k.energy = energy ? 1 : 0;
     const unsigned char* b = reinterpret_cast<const unsigned char*>(&k);
     return std::vector<unsigned char>(b, b + sizeof k);
 }
@@ -958,7 +959,10 @@ void drop_graph(cipc_gpu_context* c) {
 // key matches, else a new capture (discarded, and false returned, when an allocation happened
 // while capturing, or the capture failed).  t0/t1 bracket the launch.
 bool iteration_graph(cipc_gpu_context* c, double kappa, bool energy, hipEvent_t t0, hipEvent_t t1) {
-    if (!c->graph.enabled || !trees_refit_only(c) || c->capS == 0 || c->as.capPT == 0) return false;
+    // by default a batch goes as one graph launch and a single scene eagerly (measured no
+    // faster as a graph: launching the graph's ~120 nodes costs the host what the launches do)
+    const bool on = c->graph.mode < 0 ? c->sc.S > 1 : c->graph.mode == 1;
+    if (!on || !trees_refit_only(c) || c->capS == 0 || c->as.capPT == 0) return false;
     std::vector<unsigned char> key = iter_key(c, kappa, energy);
     if (!c->graph.exec || key != c->graph.key) {
         drop_graph(c);
@@ -974,6 +978,21 @@ bool iteration_graph(cipc_gpu_context* c, double kappa, bool energy, hipEvent_t 
         const hipError_t e = hipStreamEndCapture(c->stream, &g);
         if (e != hipSuccess || !g) ok = false;
         (void)hipGetLastError();
+        if (!ok) {
+            // a launch that cannot be captured: no graph for this context from now on; every
+            // stream leaves capture and drains before the launches go eagerly
+            c->graph.mode = 0;
+            for (hipStream_t st : {c->stream, c->aux, c->sbs}) {
+                hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+                if (hipStreamIsCapturing(st, &cs) == hipSuccess && cs != hipStreamCaptureStatusNone) {
+                    hipGraph_t dropped = nullptr;
+                    (void)hipStreamEndCapture(st, &dropped);
+                    if (dropped) (void)hipGraphDestroy(dropped);
+                }
+                (void)hipStreamSynchronize(st);
+            }
+            (void)hipGetLastError();
+        }
         if (ok && alloc_generation().load() == gen0) {
             hipGraphExec_t exec = nullptr;
             if (hipGraphInstantiate(&exec, g, nullptr, nullptr, 0) == hipSuccess && exec) {
@@ -1110,8 +1129,8 @@ int cipc_gpu_autotune(cipc_gpu_context* c, cipc_tune_entry* entries, int max_ent
         require(c && c->have_scene, "no scene uploaded");
         require(max_entries >= 0 && (max_entries == 0 || entries), "null entries");
         bind(*c);
-        TuneEntry e[2];
-        const int n = autotune(*c, e, 2);
+        TuneEntry e[4];
+        const int n = autotune(*c, e, 4);
         for (int k = 0; k < n && k < max_entries; ++k) {
             entries[k] = cipc_tune_entry{};
             entries[k].kernel = e[k].kernel;
@@ -1126,15 +1145,18 @@ int cipc_gpu_autotune(cipc_gpu_context* c, cipc_tune_entry* entries, int max_ent
 int cipc_gpu_set_tune(cipc_gpu_context* c, int kernel, int choice) {
     return guard([&] {
         require(c != nullptr, "null context");
-        require(kernel >= 0 && kernel <= 2, "kernel is 0 (point-triangle CCD), 1 (edge-edge CCD) or 2 (iteration graph)");
+        require(kernel >= 0 && kernel <= 4,
+                "kernel is 0 / 1 (point-triangle / edge-edge full CCD), 2 (iteration graph), 3 / 4 (point-triangle / edge-edge "
+                "active-set walk)");
         if (kernel == 2) {
             require(choice == 0 || choice == 1, "the iteration graph is 0 (off) or 1 (on)");
-            c->graph.enabled = choice == 1;
-            if (!c->graph.enabled) drop_graph(c);
+            c->graph.mode = choice;
+            if (choice == 0) drop_graph(c);
             return;
         }
         require(choice >= 0 && choice < kTuneSplits, "choice is 0..4 (1 to 16 lanes per query)");
-        (kernel == 0 ? c->tune.ccd_pt_bits : c->tune.ccd_ee_bits) = choice;
+        int* slot[5] = {&c->tune.ccd_pt_bits, &c->tune.ccd_ee_bits, nullptr, &c->tune.as_pt_bits, &c->tune.as_ee_bits};
+        *slot[kernel] = choice;
     });
 }
 

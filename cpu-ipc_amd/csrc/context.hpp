@@ -132,13 +132,15 @@ struct Context {
     struct IterGraph {
         hipGraphExec_t exec = nullptr;
         std::vector<unsigned char> key;
-        bool enabled = true;
+        int mode = -1;   // 1 on, 0 off, -1 the default: on for a batch, off for one scene
         long long captures = 0, replays = 0;
     } graph;
     unsigned long long topo_gen = 0;   // tree topologies built (broadphase.hip)
     // tile choices of the autotuner (ccd.hip): lanes per query of the full-CCD walks (2^bits)
+    // and of the active set's walks
     struct Tune {
         int ccd_pt_bits = 3, ccd_ee_bits = 3;
+        int as_pt_bits = 3, as_ee_bits = 3;
     } tune;
     Scene sc;
     StaticPattern sp;
@@ -194,6 +196,9 @@ struct Context {
     DBuf<int4> cverts;         // per contact: vertices (-1 past the last)
     DBuf<double> chess;        // per contact: 78 upper-triangle Hessian + 12 gradient entries (SoA)
     DBuf<int> gkey, gkey_sorted, gidx, gidx_sorted;
+    // vertex buckets that order the records (sparse.hip): counts and cursors, starts, entries
+    DBuf<int> bucket_cnt, bucket_start;
+    DBuf<unsigned long long> bucket_entries;
     DBuf<double> gval;         // 3 per record
     DBuf<int> run_start;       // start of each run of equal sorted contact block keys
     DBuf<unsigned long long> ukey;  // unique contact block keys
@@ -318,10 +323,13 @@ void step_bound_launch(Context& c, hipStream_t st);
 void step_bound_early(Context& c, hipStream_t st, bool ground = true);
 void step_bound_partial(Context& c, hipStream_t st);
 void step_bound_finish(Context& c, double* out);
+void size_active_walks(Context& c, int pt, int ee);
+void launch_active_walk(Context& c, bool pt, bool compact, int pb);
 void launch_full_ccd_pt(Context& c, hipStream_t st, int pb);
 void launch_full_ccd_ee(Context& c, hipStream_t st, int pb);
-// the autotuner: one entry per tuned kernel (0 full CCD point-triangle, 1 edge-edge), the
-// mean ms of each candidate split and the chosen one; returns the number of entries
+// the autotuner: one entry per tuned kernel (0 full CCD point-triangle, 1 edge-edge, 3 the
+// active set's point-triangle walk, 4 its edge-edge walk), the mean ms of each candidate
+// split and the chosen one; returns the number of entries
 constexpr int kTuneSplits = 5;     // 1, 2, 4, 8 or 16 lanes per query
 struct TuneEntry {
     int kernel = 0, chosen = 0, n = 0;

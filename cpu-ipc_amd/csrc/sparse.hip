@@ -277,6 +277,138 @@ __global__ void zero_ints(int* __restrict__ p, int n) {
     if (i < n) p[i] = 0;
 }
 
+
+// --- contact records bucketed by vertex: the order of a stable sort by key ----------
+// The block records are bucketed by block column (the key's high field) and the gradient
+// records by vertex: counts per bucket (integer atomics, order-free), one scan of both count
+// arrays for the bucket starts, a scatter (atomic slot within the bucket), and one wavefront
+// per bucket that orders it by (row, record) / record and writes the keys and record indices
+// as a stable sort by key over all records would (records without a key last).  This
+// replaces a global radix sort: every bucket is small, and the whole takes five launches.
+__global__ void bucket_count(int n, const unsigned long long* __restrict__ bkey, int kb, int ng, const int* __restrict__ gkey, int V,
+                             int* __restrict__ bcnt, int* __restrict__ gcnt) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) {
+        const unsigned long long k = bkey[i];
+        if (k != kNoKey) atomicAdd(bcnt + (int)(k >> kb), 1);
+    }
+    if (i < ng) {
+        const int v = gkey[i];
+        if (v >= 0 && v < V) atomicAdd(gcnt + v, 1);
+    }
+}
+// bucket entries in one array: block records as (row, record) at [0, blocks), gradient
+// records as their record index after them, where the joint scan's starts place them
+__global__ void bucket_scatter(int n, const unsigned long long* __restrict__ bkey, int kb, const int* __restrict__ bstart,
+                               int* __restrict__ bfill, int ng, const int* __restrict__ gkey, int V, const int* __restrict__ gstart,
+                               int* __restrict__ gfill, unsigned long long* __restrict__ entries) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) {
+        const unsigned long long k = bkey[i];
+        if (k != kNoKey) {
+            const int b = (int)(k >> kb);
+            const unsigned row = (unsigned)(k & ((1ull << kb) - 1));
+            entries[bstart[b] + atomicAdd(bfill + b, 1)] = ((unsigned long long)row << 32) | (unsigned)i;
+        }
+    }
+    if (i < ng) {
+        const int v = gkey[i];
+        if (v >= 0 && v < V) entries[gstart[v] + atomicAdd(gfill + v, 1)] = (unsigned long long)(unsigned)i;
+    }
+}
+constexpr int kBucketWaves = 2;
+constexpr int kBucketLds = 2048;
+__device__ inline unsigned long long shfl_xor_u64(unsigned long long v, int m) {
+    const unsigned lo = __shfl_xor((unsigned)v, m), hi = __shfl_xor((unsigned)(v >> 32), m);
+    return ((unsigned long long)hi << 32) | lo;
+}
+__device__ inline void bucket_wave_order() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+// Output of entry t of bucket bk (sorted entry e): a block record's key and index, or a
+// gradient record's vertex and index.
+struct BucketOut {
+    int V, kb, gbase;
+    unsigned long long* bkey;
+    int* bidx;
+    int* gkey;
+    int* gidx;
+    __device__ void put(int bk, int pos, unsigned long long e) const {
+        if (bk < V) {
+            bkey[pos] = ((unsigned long long)(unsigned)bk << kb) | (e >> 32);
+            bidx[pos] = (int)(e & 0xffffffffu);
+        } else {
+            gkey[pos - gbase] = bk - V - 1;
+            gidx[pos - gbase] = (int)(e & 0xffffffffu);
+        }
+    }
+};
+// Bucket bk = [starts[bk], starts[bk + 1]) of the 2V + 1 buckets, one wavefront each: up to
+// 64 entries sorted in registers (bitonic network over the lanes), up to kBucketLds in LDS
+// (bitonic network, the wave ordering its own LDS accesses between steps), longer buckets by
+// one lane in place.
+__global__ __launch_bounds__(64 * kBucketWaves) void bucket_sort(int nbk, const int* __restrict__ starts,
+                                                                 unsigned long long* __restrict__ keys, BucketOut out) {
+    __shared__ unsigned long long sh[kBucketWaves][kBucketLds];
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int bk = blockIdx.x * kBucketWaves + w;
+    if (bk >= nbk) return;   // the whole wavefront: no block-wide barrier follows
+    out.gbase = starts[out.V + 1];   // the gradient entries follow the block entries
+    const int s0 = starts[bk], k = starts[bk + 1] - s0;
+    if (k <= 0) return;
+    if (k <= 64) {
+        unsigned long long v = lane < k ? keys[s0 + lane] : ~0ull;
+#pragma unroll
+        for (int size = 2; size <= 64; size <<= 1)
+#pragma unroll
+            for (int stride = size >> 1; stride > 0; stride >>= 1) {
+                const unsigned long long o = shfl_xor_u64(v, stride);
+                const bool keep_min = ((lane & stride) == 0) == ((lane & size) == 0);
+                v = keep_min ? (o < v ? o : v) : (o > v ? o : v);
+            }
+        if (lane < k) out.put(bk, s0 + lane, v);
+        return;
+    }
+    if (k <= kBucketLds) {
+        unsigned long long* a = sh[w];
+        int P = 128;
+        while (P < k) P <<= 1;
+        for (int i = lane; i < P; i += 64) a[i] = i < k ? keys[s0 + i] : ~0ull;
+        bucket_wave_order();
+        for (int size = 2; size <= P; size <<= 1)
+            for (int stride = size >> 1; stride > 0; stride >>= 1) {
+                for (int t = lane; t < (P >> 1); t += 64) {
+                    const int i = 2 * stride * (t / stride) + (t % stride), j = i + stride;
+                    const unsigned long long x = a[i], y = a[j];
+                    const bool asc = (i & size) == 0;
+                    if ((x > y) == asc) { a[i] = y; a[j] = x; }
+                }
+                bucket_wave_order();
+            }
+        for (int i = lane; i < k; i += 64) out.put(bk, s0 + i, a[i]);
+        return;
+    }
+    if (lane == 0) {
+        for (int i = s0 + 1; i < s0 + k; ++i) {
+            const unsigned long long x = keys[i];
+            int j = i - 1;
+            while (j >= s0 && keys[j] > x) { keys[j + 1] = keys[j]; --j; }
+            keys[j + 1] = x;
+        }
+        for (int i = s0; i < s0 + k; ++i) out.put(bk, i, keys[i]);
+    }
+}
+// Records without a key, after the keyed ones: block keys kNoKey, gradient keys past every vertex.
+__global__ void bucket_tails(int nb, int ng, const int* __restrict__ starts, int V, unsigned long long* __restrict__ bkey,
+                             int* __restrict__ bidx, int* __restrict__ gkey, int* __restrict__ gidx) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int tb = starts[V], tg = starts[2 * V + 1] - starts[V + 1];
+    if (i >= tb && i < nb) { bkey[i] = kNoKey; bidx[i] = i; }
+    if (i >= tg && i < ng) { gkey[i] = 0x7fffffff; gidx[i] = i; }
+}
+
 void finalize_csr(Context& c) {
     Scene& s = c.sc;
     const int V = s.V;
@@ -290,10 +422,39 @@ void finalize_csr(Context& c) {
     int* pos = nullptr;
     c.iscratch.resize((size_t)std::max(nb, 1) + 1);
     pos = c.iscratch.p;
+    const int ng = (int)c.gkey.n;
+    // the records of both kinds in key order, bucketed by vertex (counts of the block columns
+    // and of the gradient vertices adjacent: one scan)
+    {
+        c.bucket_cnt.resize(4 * ((size_t)V + 1));
+        int* bcnt = c.bucket_cnt.p;
+        int* gcnt = bcnt + (V + 1);
+        int* bfill = gcnt + (V + 1);
+        int* gfill = bfill + (V + 1);
+        c.bucket_start.resize(2 * ((size_t)V + 1));
+        int* bstart = c.bucket_start.p;
+        int* gstart = bstart + (V + 1);
+        c.bucket_entries.resize((size_t)nb + ng + 1);
+        c.gkey_sorted.resize(std::max(ng, 1)); c.gidx_sorted.resize(std::max(ng, 1));
+        hipLaunchKernelGGL(zero_ints, dim3(blocks_for(4 * ((size_t)V + 1), 256)), dim3(256), 0, c.stream, bcnt, 4 * (V + 1));
+        const int nmax = std::max(nb, ng);
+        if (nmax > 0)
+            hipLaunchKernelGGL(bucket_count, dim3(blocks_for(nmax, 256)), dim3(256), 0, c.stream, nb, c.ckey.p, s.kb, ng, c.gkey.p, V,
+                               bcnt, gcnt);
+        CIPC_CHECK_LAUNCH();
+        exclusive_scan_int(c, bcnt, bstart, 2 * (V + 1));
+        if (nmax > 0) {
+            hipLaunchKernelGGL(bucket_scatter, dim3(blocks_for(nmax, 256)), dim3(256), 0, c.stream, nb, c.ckey.p, s.kb, bstart, bfill,
+                               ng, c.gkey.p, V, gstart, gfill, c.bucket_entries.p);
+            const BucketOut bo{V, s.kb, 0, c.ckey_sorted.p, c.cidx_sorted.p, c.gkey_sorted.p, c.gidx_sorted.p};
+            hipLaunchKernelGGL(bucket_sort, dim3(blocks_for(2 * V + 1, kBucketWaves)), dim3(64 * kBucketWaves), 0, c.stream, 2 * V + 1,
+                               bstart, c.bucket_entries.p, bo);
+            hipLaunchKernelGGL(bucket_tails, dim3(blocks_for(nmax, 256)), dim3(256), 0, c.stream, nb, ng, bstart, V, c.ckey_sorted.p,
+                               c.cidx_sorted.p, c.gkey_sorted.p, c.gidx_sorted.p);
+        }
+        CIPC_CHECK_LAUNCH();
+    }
     if (nb > 0) {
-        // keys below the sentinel use the low 2 kb bits, and the sentinel's low bits are all
-        // ones, so sorting those bits orders them all
-        sort_u64_pairs(c, c.ckey.p, c.ckey_sorted.p, c.cidx.p, c.cidx_sorted.p, nb, 2 * s.kb);
         hipLaunchKernelGGL(run_heads_u64, dim3(blocks_for(nb + 1, 256)), dim3(256), 0, c.stream, nb, c.ckey_sorted.p, head);
         CIPC_CHECK_LAUNCH();
         exclusive_scan_int(c, head, pos, nb + 1);
@@ -309,11 +470,7 @@ void finalize_csr(Context& c) {
         CIPC_CHECK_LAUNCH();
     }
     // contact gradient records
-    const int ng = (int)c.gkey.n;
     if (ng > 0) {
-        c.gkey_sorted.resize(ng); c.gidx_sorted.resize(ng);
-        // vertex keys are below 2^kb and the unused-slot key 0x7fffffff has all low bits set
-        sort_i32_pairs(c, c.gkey.p, c.gkey_sorted.p, c.gidx.p, c.gidx_sorted.p, ng, s.kb);
         hipLaunchKernelGGL(add_gradient_records, dim3(blocks_for(ng, 256)), dim3(256), 0, c.stream, ng, c.gkey_sorted.p,
                            c.gidx_sorted.p, c.gval.p, V, c.grad.p);
         CIPC_CHECK_LAUNCH();

@@ -273,15 +273,17 @@ int cipc_gpu_step_bound(cipc_gpu_context* ctx, double out[5]);
 /* The launch-shape autotuner.  No reference counterpart: the reference's CCD loops are CPU
  * loops (ContactMechanics.cpp:3754-3867) with no launch shape to choose.  It times each
  * candidate split of the two full-CCD walks of the step bound (kernel 0 point-triangle,
- * 1 edge-edge; candidate c runs 2^c lanes per query, c = 0..4) on the context's current
- * positions and direction (cipc_gpu_set_direction; a batch context times its whole batch),
- * and keeps the fastest for every later step bound of the context.  Every split returns
- * the same step bound, bit for bit: each walk's result is a minimum over pairs.  entries
- * (may be NULL) receives up to max_entries records, n_entries the number of tuned kernels.
+ * 1 edge-edge) and of the active set's two walks (kernel 3 point-triangle, 4 edge-edge;
+ * candidate c runs 2^c lanes per query, c = 0..4) on the context's current positions and
+ * direction (cipc_gpu_set_direction; a batch context times its whole batch), and keeps the
+ * fastest for every later call of the context.  Every split returns the same results, bit
+ * for bit: a full-CCD walk's result is a minimum over pairs, and the active set ranks its
+ * hits by primitive.  entries (may be NULL) receives up to max_entries records (4 at most),
+ * n_entries the number of tuned kernels.
  * cipc_gpu_set_tune sets a choice directly (the default is 3, 8 lanes per query); its
  * kernel 2 switches the Newton iteration's single graph launch (cipc_gpu_newton_iteration,
  * cipc_gpu_batch_newton_iteration) off (choice 0: every stage launched on its own and timed)
- * or on (1, the default). */
+ * or on (1); by default a batch context uses it and a single-scene context does not. */
 typedef struct {
     int kernel, chosen, n_candidates;
     double ms[8]; /* mean ms of each candidate over three timed runs */
@@ -300,9 +302,9 @@ int cipc_gpu_is_intersected(cipc_gpu_context* ctx, int* intersected);
  * active set, assembly, step bound, energy.  The assembled system is then the one that
  * cipc_gpu_solve / cipc_gpu_get_csr see, as after cipc_gpu_assemble.  No stage waits on the
  * host (the active set's counts stay in device memory; one synchronisation ends the call),
- * so the stages are captured once into a HIP graph and replayed by later calls while the
+ * so the stages can be captured once into a HIP graph and replayed by later calls while the
  * stiffness, the buffers' capacities, the lagged friction counts and the tree topologies stay
- * the same. */
+ * the same (cipc_gpu_set_tune kernel 2). */
 int cipc_gpu_newton_iteration(cipc_gpu_context* ctx, double kappa, cipc_iteration_timing* timing);
 
 /* Velocities of the scene's vertices (mesh3D::velocities, 3V; zero after an upload) and the

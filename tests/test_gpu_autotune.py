@@ -1,10 +1,13 @@
 """The launch-shape autotuner (cipc_gpu_autotune / cipc_gpu_set_tune): every split of the
-full-CCD walks (1 to 16 lanes per query) returns the oracle's step bound bit for bit, and the
-tuner keeps the fastest candidate and leaves the step bound unchanged.
+full-CCD walks and of the active set's walks (1 to 16 lanes per query) returns the oracle's
+step bound and active set bit for bit, and the tuner keeps the fastest candidate and leaves
+the results unchanged.
 
-The full-CCD minimum over pairs does not depend on which lane meets which pair, so the bar
-is bit-exact against the oracle (Self_largestFeasibleStepSize_CCD,
-ContactMechanics.cpp:3754-3867) for every choice, on single scenes and on a batch.
+The full-CCD minimum over pairs does not depend on which lane meets which pair, and the
+active set ranks its hits by primitive whatever lane found them, so the bar is bit-exact
+against the oracle (Self_largestFeasibleStepSize_CCD, ContactMechanics.cpp:3754-3867;
+buildCollisionSets, CollisionBroadPhase.cpp:765-1059) for every choice, on single scenes
+and on a batch.
 """
 import numpy as np
 import pytest
@@ -30,21 +33,28 @@ def test_every_split_matches_oracle(gpu, oracle_lib, name):
     g = gpu.GpuContext(s, xtilde=xt)
     o = oracle_lib.OracleMesh(s, xtilde=xt)
     try:
-        g.active_set()
-        o.active_set()
+        ao = o.active_set()
         bo = o.step_bound(p)
         for choice in range(5):
             g.set_tune(0, choice)
             g.set_tune(1, 4 - choice)
+            g.set_tune(3, choice)
+            g.set_tune(4, 4 - choice)
+            ag = g.active_set()
+            for k in ("self_set", "ee_set", "ee_ie", "ccd_set", "env_set"):
+                assert np.array_equal(ag[k], ao[k]), (name, choice, k)
             bg = g.step_bound(p)
             for k in KEYS:
                 assert bg[k] == bo[k], (name, choice, k, bg[k], bo[k])
         t = g.autotune(p)
         print(name, {k: (c, [f"{m:.3f}" for m in ms]) for k, (c, ms) in t.items()})
-        assert set(t) == {"full_ccd_pt", "full_ccd_ee"}
+        assert set(t) == {"full_ccd_pt", "full_ccd_ee", "active_pt", "active_ee"}
         for chosen, ms in t.values():
             assert len(ms) == 5 and all(m > 0 for m in ms)
             assert ms[chosen] == min(ms)
+        ag = g.active_set()
+        for k in ("self_set", "ee_set", "ee_ie", "ccd_set", "env_set"):
+            assert np.array_equal(ag[k], ao[k]), (name, "tuned", k)
         bg = g.step_bound(p)
         for k in KEYS:
             assert bg[k] == bo[k], (name, "tuned", k, bg[k], bo[k])
@@ -68,7 +78,7 @@ def test_batch_tuned_step_bound_unchanged(gpu):
         b.active_set()
         before = b.step_bound(p)
         t = b.autotune(p)
-        assert len(t) == 2
+        assert len(t) == 4
         after = b.step_bound(p)
         for k in KEYS:
             assert np.array_equal(before[k], after[k]), k
@@ -81,7 +91,7 @@ def test_set_tune_rejects_bad_choices(gpu):
     g = gpu.GpuContext(scenes.cube10())
     try:
         with pytest.raises(ValueError):
-            g.set_tune(3, 0)
+            g.set_tune(5, 0)
         with pytest.raises(ValueError):
             g.set_tune(0, 5)
         with pytest.raises(ValueError):

@@ -40,6 +40,7 @@ def test_replays_equal_eager_launches(gpu, name):
     kappa = scenes.suggest_kappa(s.hhat, s.bbox_diag2, s.mean_mass)
     a = gpu.GpuContext(s, xtilde=xt)
     b = gpu.GpuContext(s, xtilde=xt)
+    a.set_tune(2, 1)      # a single scene launches eagerly unless asked
     b.set_tune(2, 0)
     try:
         graphs = 0
@@ -83,7 +84,7 @@ def test_batch_replays_equal_eager_launches(gpu):
         s.hhat, s.fhat, s.dtol = hh, fh, dt
     probes = [scenes.newton_probe_state(s, seed=3 + k) for k, s in enumerate(sc)]
     kappa = np.array([scenes.suggest_kappa(s.hhat, s.bbox_diag2, s.mean_mass) for s in sc])
-    a = gpu.BatchContext(sc, xtildes=[xt for xt, _ in probes])
+    a = gpu.BatchContext(sc, xtildes=[xt for xt, _ in probes])   # a batch: one graph launch by default
     b = gpu.BatchContext(sc, xtildes=[xt for xt, _ in probes])
     b.set_tune(2, 0)
     try:
