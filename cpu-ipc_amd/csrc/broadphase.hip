@@ -458,10 +458,13 @@ __device__ inline int sum_over_parts(int v) {
     return v;
 }
 
+// The point walk also sets each surface vertex's ground flag (distance^2 to the ground below
+// Hhat, Ground::calculateActivateSet, CollisionBroadPhase.cpp:1076-1092), for the same scan.
 template <int PB>
 __global__ __launch_bounds__(64) void pt_traverse(int Vs, const int* __restrict__ perm, const int* __restrict__ sv,
                                                    const double* __restrict__ x, TreeView tv, double sq, int* __restrict__ cnt,
-                                                   int* __restrict__ cntp, int* __restrict__ buf) {
+                                                   int* __restrict__ cntp, int* __restrict__ buf, int* __restrict__ fg, d3 gn,
+                                                   double gd, double hhat) {
     constexpr int P = 1 << PB;
     const int t = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
     const bool active = t < Vs * P;
@@ -475,8 +478,12 @@ __global__ __launch_bounds__(64) void pt_traverse(int Vs, const int* __restrict_
         cntp[s * P + part] = n;
     }
     const int tot = sum_over_parts<PB>(n);
-    if (active && part == 0) cnt[s] = tot;
-    if (blockIdx.x == 0 && threadIdx.x == 0) cnt[Vs] = 0;   // the scan's tail
+    if (active && part == 0) {
+        cnt[s] = tot;
+        const double dist = dot(gn, ld3(x, sv[s])) - gd;
+        fg[s] = (dist * dist < hhat) ? 1 : 0;
+    }
+    if (blockIdx.x == 0 && threadIdx.x == 0) { cnt[Vs] = 0; fg[Vs] = 0; }   // the scans' tails
 }
 
 template <int PB>
@@ -590,7 +597,7 @@ __global__ __launch_bounds__(64) void ee_compact(int Es, const int* __restrict__
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= Es || dc[kDcFlags]) return;
     const int eI = perm[t];
-    const int o = off[eI];
+    const int o = off[eI] - off[0];   // the joint scan's edge starts begin at the point total
     int pos = o;
     for (int part = 0; part < P; ++part) {
         const int c = cntp[eI * P + part];
@@ -617,7 +624,7 @@ __global__ void ee_classify(const int* __restrict__ dc, int Es, const int* __res
                             int* __restrict__ cat) {
     const int u = blockIdx.x * blockDim.x + threadIdx.x;
     if (dc[kDcFlags] || u >= dc[kDcEE]) return;
-    const int eI = own[u], o = off[eI], e = off[eI + 1];
+    const int eI = own[u], o = off[eI] - off[0], e = off[eI + 1] - off[0];
     const int eJ = candU[u];
     int k = o;
     for (int j = o; j < e; ++j) k += candU[j] < eJ;
@@ -691,14 +698,22 @@ __global__ void fill_tail(int* p, int n) { p[n] = 0; }
 struct Cnt4Sum {
     __host__ __device__ int4 operator()(const int4& p, const int4& q) const { return make_int4(p.x + q.x, p.y + q.y, p.z + q.z, p.w + q.w); }
 };
+// Both hit kinds in one scan: items [0, n1) the point-triangle hits (dc[kDcPT] of them), the
+// rest the edge-edge hits (dc[kDcEE]), each with its own category masks; nothing is counted
+// over capacity.
 struct CatFlags {
-    const int* cat;
-    const int* dc;   // the hit count is dc[which]; nothing is counted over capacity
-    int which;
-    unsigned m0, m1, m2, m3;
+    const int* catPT;
+    const int* catEE;
+    const int* dc;
+    int n1;
+    unsigned a0, a1, a2, a3, b0, b1, b2, b3;
     __host__ __device__ int4 operator()(int i) const {
-        if (dc[kDcFlags] || i >= dc[which]) return make_int4(0, 0, 0, 0);
-        const unsigned bit = 1u << cat[i];
+        if (dc[kDcFlags]) return make_int4(0, 0, 0, 0);
+        const bool pt = i < n1;
+        const int j = pt ? i : i - n1;
+        if (j >= dc[pt ? kDcPT : kDcEE]) return make_int4(0, 0, 0, 0);
+        const unsigned bit = 1u << (pt ? catPT : catEE)[j];
+        const unsigned m0 = pt ? a0 : b0, m1 = pt ? a1 : b1, m2 = pt ? a2 : b2, m3 = pt ? a3 : b3;
         return make_int4((m0 & bit) ? 1 : 0, (m1 & bit) ? 1 : 0, (m2 & bit) ? 1 : 0, (m3 & bit) ? 1 : 0);
     }
 };
@@ -730,7 +745,8 @@ __global__ void scatter_ee(const int* __restrict__ dc, const int* __restrict__ c
     const int base_self = dc[kDcA], base_map = dc[kDcA2], base_ccd = dc[kDcAK];
     const int ct = cat[i];
     if (ct == kDrop) return;
-    const int4 p = pos[i];
+    const int4 q = pos[i], z = pos[0];   // the joint scan's edge segment starts at the point totals
+    const int4 p = make_int4(q.x - z.x, q.y - z.y, q.z - z.z, q.w - z.w);
     ccd[base_ccd + p.w] = make_int2(own[i], cand[i]);
     if (ct == kSelfEE) self_set[base_self + p.x] = rec[i];
     else if (ct == kMap) mkeys[base_map + p.y] = map_key(rec[i], V, bits);
@@ -742,7 +758,8 @@ __global__ void scatter_ee(const int* __restrict__ dc, const int* __restrict__ c
 __global__ void category_totals(int Vs, const int4* __restrict__ posPT, const int4* __restrict__ posEE, const int* __restrict__ pg,
                                 int capM, int* __restrict__ dc) {
     if (blockIdx.x != 0 || threadIdx.x != 0 || dc[kDcFlags]) return;
-    const int4 a = posPT[dc[kDcPT]], b = posEE[dc[kDcEE]];
+    const int4 a = posPT[dc[kDcPT]], q = posEE[dc[kDcEE]], z = posEE[0];
+    const int4 b = make_int4(q.x - z.x, q.y - z.y, q.z - z.z, q.w - z.w);
     const int nm = a.y + b.y;
     dc[kDcMapKeys] = nm;
     if (nm > capM) { dc[kDcFlags] |= 4; return; }
@@ -751,7 +768,7 @@ __global__ void category_totals(int Vs, const int4* __restrict__ posPT, const in
     dc[kDcA] = a.x; dc[kDcA2] = a.y; dc[kDcAK] = a.z;
     dc[kDcB] = b.x; dc[kDcB2] = b.y; dc[kDcMol] = b.z; dc[kDcBK] = b.w;
     dc[kDcCCD] = a.z + b.w;
-    dc[kDcEnv] = pg[Vs];
+    dc[kDcEnv] = pg[Vs] - pg[0];
 }
 __global__ void map_tail(int capM, const int* __restrict__ dc, unsigned long long* __restrict__ mkeys) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -774,19 +791,16 @@ __global__ void map_decode(const int* __restrict__ dc, const unsigned long long*
     const unsigned long long k = keys[i], m = (1ull << bits) - 1;
     out[base + i] = make_int4((int)((k >> (2 * bits)) & m) - V, (int)((k >> bits) & m), (int)(k & m) - 1, -counts[i]);
 }
-__global__ void ground_flags(int Vs, const int* __restrict__ sv, const double* __restrict__ x, d3 gn, double gd, double hhat,
-                             int* __restrict__ flag) {
-    const int s = blockIdx.x * blockDim.x + threadIdx.x;
-    if (s > Vs) return;
-    if (s == Vs) { flag[s] = 0; return; }
-    const double dist = dot(gn, ld3(x, sv[s])) - gd;
-    flag[s] = (dist * dist < hhat) ? 1 : 0;
-}
+// pos: the joint scan's ground starts, offset by its first entry
 __global__ void ground_scatter(int Vs, const int* __restrict__ sv, const int* __restrict__ flag, const int* __restrict__ pos,
                                int* __restrict__ out) {
     const int s = blockIdx.x * blockDim.x + threadIdx.x;
     if (s >= Vs || !flag[s]) return;
-    out[pos[s]] = sv[s];
+    out[pos[s] - pos[0]] = sv[s];
+}
+// the tails of the joint scan's point and ground segments when there is no point walk
+__global__ void point_tails(int Vs, int* __restrict__ cnt, int* __restrict__ fg) {
+    if (blockIdx.x == 0 && threadIdx.x == 0) { cnt[Vs] = 0; fg[Vs] = 0; }
 }
 
 __global__ void dc_reset(int* __restrict__ dc) {
@@ -796,19 +810,18 @@ __global__ void dc_reset(int* __restrict__ dc) {
 __global__ void candidate_totals(int Vs, int Es, const int* __restrict__ offPT, const int* __restrict__ offEE, int capPT, int capEE,
                                  int* __restrict__ dc) {
     if (blockIdx.x != 0 || threadIdx.x != 0) return;
-    const int nPT = offPT[Vs], nEE = offEE[Es];
+    const int nPT = offPT[Vs], nEE = offEE[Es] - offEE[0];
     dc[kDcPT] = nPT;
     dc[kDcEE] = nEE;
     dc[kDcFlags] = (nPT > capPT ? 1 : 0) | (nEE > capEE ? 2 : 0);
 }
 
-// Exclusive scan of the four category flags of hits [0, dc[which]) over a buffer of n hits
-// (flags past the count are zero, so pos[count] holds the totals).
-void category_scan(Context& c, const int* cat, int n, const int* dc, int which, unsigned m0, unsigned m1, unsigned m2, unsigned m3,
-                   DBuf<int4>& pos) {
+// Exclusive scan of the four category flags over n items (flags past a kind's count are
+// zero, so each kind's totals follow its last hit).
+void category_scan(Context& c, const CatFlags& f, int n, DBuf<int4>& pos) {
     pos.resize((size_t)n + 1);
     hipcub::CountingInputIterator<int> idx(0);
-    hipcub::TransformInputIterator<int4, CatFlags, hipcub::CountingInputIterator<int>> it(idx, CatFlags{cat, dc, which, m0, m1, m2, m3});
+    hipcub::TransformInputIterator<int4, CatFlags, hipcub::CountingInputIterator<int>> it(idx, f);
     size_t bytes = 0;
     CIPC_HIP(hipcub::DeviceScan::ExclusiveScan(nullptr, bytes, it, pos.p, Cnt4Sum(), make_int4(0, 0, 0, 0), n + 1, c.stream));
     c.tmp.reserve(bytes);
@@ -835,13 +848,19 @@ void launch_active_walk(Context& c, bool pt, bool compact, int pb) {
     const int n = pt ? s.Vs : s.Es;
     if (n <= 0) return;
     const dim3 grid(blocks_for(compact ? (size_t)n : (size_t)n << pb, 64)), block(64);
-    auto trav_pt = [&](auto k) { hipLaunchKernelGGL(k, grid, block, 0, c.stream, s.Vs, s.sv_perm.p, s.sv.p, s.x.p, ft, sq, b.cntPT.p,
-                                                    b.cntpPT.p, b.bufPT.p); };
-    auto trav_ee = [&](auto k) { hipLaunchKernelGGL(k, grid, block, 0, c.stream, s.Es, s.se_perm.p, s.se.p, s.x.p, et, sq, b.cntEE.p,
+    // the joint count / start arrays: points (Vs + 1), edges (Es + 1), ground flags (Vs + 1)
+    int* cPT = b.cnt_all.p;
+    int* cEE = cPT + (s.Vs + 1);
+    int* cG = cEE + (s.Es + 1);
+    const int* oPT = b.off_all.p;
+    const int* oEE = oPT + (s.Vs + 1);
+    auto trav_pt = [&](auto k) { hipLaunchKernelGGL(k, grid, block, 0, c.stream, s.Vs, s.sv_perm.p, s.sv.p, s.x.p, ft, sq, cPT,
+                                                    b.cntpPT.p, b.bufPT.p, cG, s.gn, s.gd, s.hhat); };
+    auto trav_ee = [&](auto k) { hipLaunchKernelGGL(k, grid, block, 0, c.stream, s.Es, s.se_perm.p, s.se.p, s.x.p, et, sq, cEE,
                                                     b.cntpEE.p, b.bufEE.p); };
-    auto comp_pt = [&](auto k) { hipLaunchKernelGGL(k, grid, block, 0, c.stream, s.Vs, s.sv_perm.p, s.sv.p, s.x.p, ft, sq, b.offPT.p,
+    auto comp_pt = [&](auto k) { hipLaunchKernelGGL(k, grid, block, 0, c.stream, s.Vs, s.sv_perm.p, s.sv.p, s.x.p, ft, sq, oPT,
                                                     b.cntpPT.p, b.bufPT.p, b.candUPT.p, b.ownPT.p, dc); };
-    auto comp_ee = [&](auto k) { hipLaunchKernelGGL(k, grid, block, 0, c.stream, s.Es, s.se_perm.p, s.se.p, s.x.p, et, sq, b.offEE.p,
+    auto comp_ee = [&](auto k) { hipLaunchKernelGGL(k, grid, block, 0, c.stream, s.Es, s.se_perm.p, s.se.p, s.x.p, et, sq, oEE,
                                                     b.cntpEE.p, b.bufEE.p, b.candUEE.p, b.ownEE.p, dc); };
 #define CIPC_SPLIT(F, K)                 \
     switch (pb) {                        \
@@ -865,7 +884,7 @@ void launch_active_walk(Context& c, bool pt, bool compact, int pb) {
 // category of hits is at most the hits of its type, so the outputs are bounded by the hit
 // capacities (Self_ActiveSet head nA + nB, map tail nM <= nm <= capM, Self_EE_ActiveSet
 // nE <= nEE, CCD pairs nPTk + nEEk <= nPT + nEE, ground nG <= Vs).
-void compute_active_set(Context& c) {
+void compute_active_set(Context& c, bool publish) {
     Scene& s = c.sc;
     build_static_trees(c);
     ActiveSetScratch& b = c.as;
@@ -880,17 +899,25 @@ void compute_active_set(Context& c) {
     c.dcnt.resize(kDcWords);
     int* dc = c.dcnt.p;
     hipLaunchKernelGGL(dc_reset, dim3(1), dim3(64), 0, c.stream, dc);
-    b.cntPT.resize(Vs + 1); b.offPT.resize(Vs + 1); b.cntEE.resize(Es + 1); b.offEE.resize(Es + 1);
+    // hit counts per point and per edge query and the ground flags, in one array for one scan
+    const int nall = (Vs + 1) + (Es + 1) + (Vs + 1);
+    b.cnt_all.resize(nall); b.off_all.resize(nall);
+    int* cPT = b.cnt_all.p;
+    int* cEE = cPT + (Vs + 1);
+    int* cG = cEE + (Es + 1);
+    const int* oPT = b.off_all.p;
+    const int* oEE = oPT + (Vs + 1);
+    const int* oG = oEE + (Es + 1);
     size_active_walks(c, c.tune.as_pt_bits, c.tune.as_ee_bits);
-    // A: one walk per query part
+    // A: one walk per query part (the point walk also sets the ground flags)
     if (Vs > 0) launch_active_walk(c, true, false, c.tune.as_pt_bits);
     if (Es > 0) launch_active_walk(c, false, false, c.tune.as_ee_bits);
-    // (the traversals also zero the scans' tails)
-    if (Vs == 0) hipLaunchKernelGGL(fill_tail, dim3(1), dim3(1), 0, c.stream, b.cntPT.p, Vs);
-    if (Es == 0) hipLaunchKernelGGL(fill_tail, dim3(1), dim3(1), 0, c.stream, b.cntEE.p, Es);
-    exclusive_scan_int(c, b.cntPT.p, b.offPT.p, Vs + 1);
-    exclusive_scan_int(c, b.cntEE.p, b.offEE.p, Es + 1);
-    hipLaunchKernelGGL(candidate_totals, dim3(1), dim3(64), 0, c.stream, Vs, Es, b.offPT.p, b.offEE.p, capPT, capEE, dc);
+    // (the walks also zero the scan segments' tails)
+    if (Vs == 0) hipLaunchKernelGGL(point_tails, dim3(1), dim3(1), 0, c.stream, Vs, cPT, cG);
+    if (Es == 0) hipLaunchKernelGGL(fill_tail, dim3(1), dim3(1), 0, c.stream, cEE, Es);
+    // one scan: the edge and ground segments' starts come out offset by their first entry
+    exclusive_scan_int(c, b.cnt_all.p, b.off_all.p, nall);
+    hipLaunchKernelGGL(candidate_totals, dim3(1), dim3(64), 0, c.stream, Vs, Es, oPT, oEE, capPT, capEE, dc);
     CIPC_CHECK_LAUNCH();
     // B + C: pack, sort, filter and classify the hits
     b.candUPT.resize(capPT + 1); b.candUEE.resize(capEE + 1);
@@ -898,12 +925,12 @@ void compute_active_set(Context& c) {
     b.candEE.resize(capEE + 1); b.recEE.resize(capEE + 1); b.catEE.resize(capEE + 1); b.auxEE.resize(capEE + 1); b.ownEE.resize(capEE + 1);
     if (Vs > 0) {
         launch_active_walk(c, true, true, c.tune.as_pt_bits);
-        hipLaunchKernelGGL(pt_classify, dim3(blocks_for(capPT, 128)), dim3(128), 0, c.stream, dc, s.F, b.ownPT.p, b.offPT.p, s.sv.p, s.x.p,
+        hipLaunchKernelGGL(pt_classify, dim3(blocks_for(capPT, 128)), dim3(128), 0, c.stream, dc, s.F, b.ownPT.p, oPT, s.sv.p, s.x.p,
                            s.bt.p, s.faces.p, b.candUPT.p, b.candPT.p, s.hhat, sq, scene_of_vertex(s), b.recPT.p, b.catPT.p);
     }
     if (Es > 0) {
         launch_active_walk(c, false, true, c.tune.as_ee_bits);
-        hipLaunchKernelGGL(ee_classify, dim3(blocks_for(capEE, 128)), dim3(128), 0, c.stream, dc, Es, b.ownEE.p, b.offEE.p, s.se.p,
+        hipLaunchKernelGGL(ee_classify, dim3(blocks_for(capEE, 128)), dim3(128), 0, c.stream, dc, Es, b.ownEE.p, oEE, s.se.p,
                            s.x.p, s.vrest.p, s.bt.p, b.candUEE.p, b.candEE.p, s.hhat, sq, scene_of_vertex(s), b.recEE.p, b.auxEE.p,
                            b.catEE.p);
     }
@@ -911,13 +938,16 @@ void compute_active_set(Context& c) {
     // category positions
     const unsigned keptPT = (1u << kNone) | (1u << kSelfPT) | (1u << kMap);
     const unsigned keptEE = (1u << kNone) | (1u << kSelfEE) | (1u << kMap) | (1u << kMolPE) | (1u << kMolEE);
-    category_scan(c, b.catPT.p, capPT, dc, kDcPT, 1u << kSelfPT, 1u << kMap, keptPT, 0u, b.posPT);
-    category_scan(c, b.catEE.p, capEE, dc, kDcEE, 1u << kSelfEE, 1u << kMap, (1u << kMolPE) | (1u << kMolEE), keptEE, b.posEE);
-    b.fg.resize(Vs + 1); b.pg.resize(Vs + 1);
-    hipLaunchKernelGGL(ground_flags, dim3(blocks_for(Vs + 1, 256)), dim3(256), 0, c.stream, Vs, s.sv.p, s.x.p, s.gn, s.gd, s.hhat, b.fg.p);
-    CIPC_CHECK_LAUNCH();
-    exclusive_scan_int(c, b.fg.p, b.pg.p, Vs + 1);
-    hipLaunchKernelGGL(category_totals, dim3(1), dim3(64), 0, c.stream, Vs, b.posPT.p, b.posEE.p, b.pg.p, capM, dc);
+    // both kinds of hit in one scan: points [0, capPT], edges after them
+    b.pos_all.resize((size_t)capPT + 1 + capEE + 1);
+    {
+        const CatFlags f{b.catPT.p, b.catEE.p, dc, capPT + 1, 1u << kSelfPT, 1u << kMap, keptPT, 0u,
+                         1u << kSelfEE, 1u << kMap, (1u << kMolPE) | (1u << kMolEE), keptEE};
+        category_scan(c, f, capPT + 1 + capEE + 1, b.pos_all);
+    }
+    const int4* posPT = b.pos_all.p;
+    const int4* posEE = posPT + (capPT + 1);
+    hipLaunchKernelGGL(category_totals, dim3(1), dim3(64), 0, c.stream, Vs, posPT, posEE, oG, capM, dc);
     CIPC_CHECK_LAUNCH();
     // PP/PE map keys need the vertex count to fit three fields in 64 bits
     int bits = 1;
@@ -927,12 +957,12 @@ void compute_active_set(Context& c) {
     c.self_set.resize((size_t)capPT + capEE + capM + 1); c.ee_set.resize((size_t)capEE + 1); c.ee_ie.resize((size_t)capEE + 1);
     c.ccd_set.resize((size_t)capPT + capEE + 1); c.env_set.resize((size_t)Vs + 1);
     b.mkeys.resize(capM); b.mkeys_sorted.resize(capM); b.ukeys.resize(capM); b.ucounts.resize(capM); b.nruns.resize(1);
-    hipLaunchKernelGGL(scatter_pt, dim3(blocks_for(capPT, 256)), dim3(256), 0, c.stream, dc, b.catPT.p, b.posPT.p, b.recPT.p, b.ownPT.p,
+    hipLaunchKernelGGL(scatter_pt, dim3(blocks_for(capPT, 256)), dim3(256), 0, c.stream, dc, b.catPT.p, posPT, b.recPT.p, b.ownPT.p,
                        b.candPT.p, s.V, bits, c.self_set.p, b.mkeys.p, c.ccd_set.p);
-    hipLaunchKernelGGL(scatter_ee, dim3(blocks_for(capEE, 256)), dim3(256), 0, c.stream, dc, b.catEE.p, b.posEE.p, b.recEE.p, b.auxEE.p,
+    hipLaunchKernelGGL(scatter_ee, dim3(blocks_for(capEE, 256)), dim3(256), 0, c.stream, dc, b.catEE.p, posEE, b.recEE.p, b.auxEE.p,
                        b.ownEE.p, b.candEE.p, s.V, bits, c.self_set.p, b.mkeys.p, c.ee_set.p, c.ee_ie.p, c.ccd_set.p);
     if (Vs > 0)
-        hipLaunchKernelGGL(ground_scatter, dim3(blocks_for(Vs, 256)), dim3(256), 0, c.stream, Vs, s.sv.p, b.fg.p, b.pg.p, c.env_set.p);
+        hipLaunchKernelGGL(ground_scatter, dim3(blocks_for(Vs, 256)), dim3(256), 0, c.stream, Vs, s.sv.p, cG, oG, c.env_set.p);
     hipLaunchKernelGGL(map_tail, dim3(blocks_for(capM, 256)), dim3(256), 0, c.stream, capM, dc, b.mkeys.p);
     CIPC_CHECK_LAUNCH();
     // PP/PE map: sort packed keys, run-length encode (std::map order with multiplicity); the
@@ -948,7 +978,7 @@ void compute_active_set(Context& c) {
     CIPC_CHECK_LAUNCH();
     // the counts are read with the host's next synchronisation (settle_active_set)
     c.pin_dc.reserve(kDcWords);
-    CIPC_HIP(hipMemcpyAsync(c.pin_dc.p, dc, kDcWords * sizeof(int), hipMemcpyDeviceToHost, c.stream));
+    if (publish) CIPC_HIP(hipMemcpyAsync(c.pin_dc.p, dc, kDcWords * sizeof(int), hipMemcpyDeviceToHost, c.stream));
     c.dc_pending = true;
 }
 

@@ -287,11 +287,13 @@ __global__ void contact_scatter_kernel(int n, const int* __restrict__ dc, const 
 
 // Ground (IPCSolver.cpp:270-293): one block and one gradient record per active vertex, in
 // slots [0, cap); slots past the device count are written empty.
-__global__ void ground_kernel(int cap, const int* __restrict__ dc, const int* __restrict__ env, const double* __restrict__ x,
+// Its first thread also checks the contacts against their capacity (contact_totals).
+__global__ void ground_kernel(int cap, int* __restrict__ dc, int capS, const int* __restrict__ env, const double* __restrict__ x,
                               const int* __restrict__ bt, d3 gn, double gd, double hhat, Kap kap, int kb,
                               unsigned long long* __restrict__ bkey, int* __restrict__ bidx, double* __restrict__ bval,
                               int* __restrict__ gkey, int* __restrict__ gidx, double* __restrict__ gval) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i == 0) dc[kDcContactOver] = dc[kDcSelf] + dc[kDcMol] > capS ? 1 : 0;
     if (i >= cap) return;
     if (i >= dc[kDcEnv]) {
         bidx[i] = i; bkey[i] = kNoKey;
@@ -679,10 +681,10 @@ void contact_assemble(Context& c, double kappa) {
     c.chess.resize((size_t)(kHU + 12) * capS);
     c.cverts.resize(capS);
     c.cdims.resize(capS);
-    hipLaunchKernelGGL(contact_totals, dim3(1), dim3(64), 0, c.stream, capS, c.dcnt.p);
+    if (capG == 0) hipLaunchKernelGGL(contact_totals, dim3(1), dim3(64), 0, c.stream, capS, c.dcnt.p);
     CIPC_HIP(hipEventRecord(c.fr_fork, c.stream));
     if (capG > 0) {
-        hipLaunchKernelGGL(ground_kernel, dim3(blocks_for(capG, 256)), dim3(256), 0, c.stream, capG, dc, c.env_set.p, s.x.p, s.bt.p,
+        hipLaunchKernelGGL(ground_kernel, dim3(blocks_for(capG, 256)), dim3(256), 0, c.stream, capG, c.dcnt.p, capS, c.env_set.p, s.x.p, s.bt.p,
                            s.gn, s.gd, s.hhat, scene_kappa(s, kappa), s.kb, c.ckey.p, c.cidx.p, c.cval.p, c.gkey.p, c.gidx.p, c.gval.p);
         CIPC_CHECK_LAUNCH();
     }

@@ -901,7 +901,7 @@ void enqueue_iteration(cipc_gpu_context* c, double kappa, bool energy, const Ite
     CIPC_HIP(hipStreamWaitEvent(c->sbs, c->it_start, 0));
     if (t) CIPC_HIP(hipEventRecord(t->s0, c->sbs));
     step_bound_early(*c, c->sbs);
-    compute_active_set(*c);
+    compute_active_set(*c, false);   // contact_assemble publishes the counts
     if (t) CIPC_HIP(hipEventRecord(t->ev[1], c->stream));
     CIPC_HIP(hipEventRecord(c->as_done, c->stream));
     CIPC_HIP(hipStreamWaitEvent(c->sbs, c->as_done, 0));

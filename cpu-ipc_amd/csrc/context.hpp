@@ -66,12 +66,13 @@ struct ActiveSetScratch {
     // capacities of the candidate hits (per type) and of the map keys; grown by
     // settle_active_set when the device reports more
     int capPT = 0, capEE = 0, capM = 0;
-    DBuf<int> cntPT, offPT, cntEE, offEE, cntpPT, cntpEE, bufPT, bufEE;
+    // per point query, per edge query and per surface vertex (ground flag), in one array
+    DBuf<int> cnt_all, off_all;
+    DBuf<int> cntpPT, cntpEE, bufPT, bufEE;
     DBuf<int> candUPT, candUEE, candPT, candEE, ownPT, ownEE, catPT, catEE;
     DBuf<int4> recPT, recEE;
     DBuf<int2> auxEE;
-    DBuf<int> fg, pg;
-    DBuf<int4> posPT, posEE;   // per-hit exclusive counts of four hit categories
+    DBuf<int4> pos_all;   // per-hit exclusive counts of four categories: point hits, then edge hits
     DBuf<unsigned long long> mkeys, mkeys_sorted, ukeys;
     DBuf<int> ucounts, nruns;
 };
@@ -272,8 +273,10 @@ void build_tree(Context& c, LBVH& t, int n);
 void build_static_trees(Context& c);
 // on stream st (the step bound may run beside the assembly); temporary storage of its own
 void build_swept_trees(Context& c, const unsigned long long* abits, hipStream_t st);
-// Enqueues the active set on the main stream; its counts stay on the device.
-void compute_active_set(Context& c);
+// Enqueues the active set on the main stream; its counts stay on the device (publish: also
+// copied to pinned memory for the host's next synchronisation; a caller that enqueues the
+// contact assembly next leaves that to it).
+void compute_active_set(Context& c, bool publish = true);
 // Reads the device counts (one synchronisation of the main stream) into the host copies.
 // When a capacity was exceeded it is grown from the count the device found and the active
 // set is built again.  Returns true when it was rebuilt (work enqueued after the first build

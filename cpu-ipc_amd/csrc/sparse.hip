@@ -255,7 +255,9 @@ __global__ void masked_rhs(int V, const int* __restrict__ bt, const double* __re
 // Collision count as compute_g_dpt returns it (ContactMechanics.cpp:2970-3032): the
 // multiplicity of a point-point / point-edge entry, one for every other Self_ActiveSet
 // entry.  One work-group, integer sums: the result does not depend on the order.
-__global__ void collision_count(const int* __restrict__ n_dev, const int4* __restrict__ set, int* __restrict__ out) {
+// out[0] = the CSR's nonzeros (nnz_dev), out[1] = the collision count: read by the host together.
+__global__ void collision_count(const int* __restrict__ n_dev, const int4* __restrict__ set, const int* __restrict__ nnz_dev,
+                                int* __restrict__ out) {
     __shared__ long long sh[256];
     long long acc = 0;
     const int n = *n_dev;
@@ -269,7 +271,7 @@ __global__ void collision_count(const int* __restrict__ n_dev, const int4* __res
         if ((int)threadIdx.x < w) sh[threadIdx.x] += sh[threadIdx.x + w];
         __syncthreads();
     }
-    if (threadIdx.x == 0) out[0] = (int)sh[0];
+    if (threadIdx.x == 0) { out[0] = *nnz_dev; out[1] = (int)sh[0]; }
 }
 
 __global__ void zero_ints(int* __restrict__ p, int n) {
@@ -464,11 +466,12 @@ void finalize_csr(Context& c) {
         hipLaunchKernelGGL(reduce_blocks, dim3(blocks_for(nb, 256)), dim3(256), 0, c.stream, nb, c.ckey_sorted.p,
                            c.cidx_sorted.p, c.run_start.p, pos, c.cval.p, c.ukey.p, c.uval.p, c.uflag.p);
         CIPC_CHECK_LAUNCH();
-        CIPC_HIP(hipMemcpyAsync(c.counters.p, pos + nb, sizeof(int), hipMemcpyDeviceToDevice, c.stream));
     } else {
         hipLaunchKernelGGL(zero_ints, dim3(1), dim3(64), 0, c.stream, c.counters.p, 1);
         CIPC_CHECK_LAUNCH();
     }
+    // the number of unique contact blocks: the run scan's total (zero without records)
+    const int* nu_ptr = nb > 0 ? pos + nb : c.counters.p;
     // contact gradient records
     if (ng > 0) {
         hipLaunchKernelGGL(add_gradient_records, dim3(blocks_for(ng, 256)), dim3(256), 0, c.stream, ng, c.gkey_sorted.p,
@@ -479,13 +482,13 @@ void finalize_csr(Context& c) {
     c.mptr.resize((size_t)V + 1);
     DBuf<int>& cnt = c.iscratch2;
     hipLaunchKernelGGL(merge_count, dim3(blocks_for(V, 256)), dim3(256), 0, c.stream, V, s.kb, c.sp.col_ptr.p, c.sp.row.p, c.ukey.p,
-                       c.counters.p, cnt.p);
+                       nu_ptr, cnt.p);
     CIPC_CHECK_LAUNCH();
     exclusive_scan_int(c, cnt.p, c.mptr.p, V + 1);
     const size_t mcap = (size_t)c.sp.nblocks + (size_t)nb + 1;
     c.mrow.resize(mcap); c.mcol.resize(mcap); c.mval.resize(9 * mcap); c.mflag.resize(mcap);
     hipLaunchKernelGGL(merge_fill, dim3(blocks_for(V, 256)), dim3(256), 0, c.stream, V, s.kb, c.sp.col_ptr.p, c.sp.row.p, c.sbv.p,
-                       c.sbf.p, c.ukey.p, c.uval.p, c.uflag.p, c.counters.p, c.mptr.p, c.mrow.p, c.mcol.p, c.mval.p, c.mflag.p);
+                       c.sbf.p, c.ukey.p, c.uval.p, c.uflag.p, nu_ptr, c.mptr.p, c.mrow.p, c.mcol.p, c.mval.p, c.mflag.p);
     CIPC_CHECK_LAUNCH();
     // scalar CSR: (block, component) counts -> one scan -> column starts and runs
     const int* nm_ptr = c.mptr.p + V;
@@ -501,11 +504,10 @@ void finalize_csr(Context& c) {
     // the count is read with the caller's next synchronisation; the entries are sized by
     // their bound (9 per merged block)
     c.pin_nnz.reserve(2);
-    CIPC_HIP(hipMemcpyAsync(c.pin_nnz.p, c.outer.p + 3 * V, sizeof(int), hipMemcpyDeviceToHost, c.stream));
-    c.coll.resize(1);
-    hipLaunchKernelGGL(collision_count, dim3(1), dim3(256), 0, c.stream, c.dcnt.p + kDcSelf, c.self_set.p, c.coll.p);
+    c.coll.resize(2);
+    hipLaunchKernelGGL(collision_count, dim3(1), dim3(256), 0, c.stream, c.dcnt.p + kDcSelf, c.self_set.p, c.outer.p + 3 * V, c.coll.p);
     CIPC_CHECK_LAUNCH();
-    CIPC_HIP(hipMemcpyAsync(c.pin_nnz.p + 1, c.coll.p, sizeof(int), hipMemcpyDeviceToHost, c.stream));
+    CIPC_HIP(hipMemcpyAsync(c.pin_nnz.p, c.coll.p, 2 * sizeof(int), hipMemcpyDeviceToHost, c.stream));
     c.nnz_pending = true;
     c.inner.resize(9 * mcap); c.val.resize(9 * mcap);
     hipLaunchKernelGGL(pair_fill, dim3(blocks_for(3 * mcap, 256)), dim3(256), 0, c.stream, nm_ptr, c.mptr.p, c.mrow.p, c.mcol.p,
