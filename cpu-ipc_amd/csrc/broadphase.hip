@@ -401,16 +401,19 @@ __global__ void point_boxes(int Vs, const int* __restrict__ sv, const double* __
     out[s] = swept_pt(ld3(x, v), ld3(p, v), step_of(vs ? abits + kSlots * vs[v] : abits));
 }
 
-void build_static_trees(Context& c) {
+// The static face tree on st_f and edge tree on st_e (each with its own temporary storage).
+static void build_static_trees_on(Context& c, hipStream_t st_f, DBuf<unsigned char>& tmp_f, hipStream_t st_e,
+                                  DBuf<unsigned char>& tmp_e) {
     Scene& s = c.sc;
     c.face_tree.prim.resize(std::max(s.F, 1));
     c.edge_tree.prim.resize(std::max(s.Es, 1));
     PrimSrc fs, es;
     fs.kind = 1; fs.faces = s.faces.p; fs.x = s.x.p;
     es.kind = 2; es.se = s.se.p; es.x = s.x.p;
-    build_tree_from(c, c.face_tree, s.F, fs, c.stream, c.tmp);
-    build_tree_from(c, c.edge_tree, s.Es, es, c.stream, c.tmp);
+    build_tree_from(c, c.face_tree, s.F, fs, st_f, tmp_f);
+    build_tree_from(c, c.edge_tree, s.Es, es, st_e, tmp_e);
 }
+void build_static_trees(Context& c) { build_static_trees_on(c, c.stream, c.tmp, c.stream, c.tmp); }
 void build_swept_trees(Context& c, const unsigned long long* abits, hipStream_t st) {
     Scene& s = c.sc;
     c.sface_tree.prim.resize(std::max(s.F, 1));
@@ -838,7 +841,8 @@ void size_active_walks(Context& c, int pt, int ee) {
 
 // The active set's walks with 2^pb lanes per query: the traversal (compact = false) or the
 // compaction of its hits (compact = true), point-triangle (pt) or edge-edge.
-void launch_active_walk(Context& c, bool pt, bool compact, int pb) {
+void launch_active_walk(Context& c, bool pt, bool compact, int pb, hipStream_t st) {
+    if (!st) st = c.stream;
     Scene& s = c.sc;
     ActiveSetScratch& b = c.as;
     const double sq = sqrt(s.hhat);
@@ -854,13 +858,13 @@ void launch_active_walk(Context& c, bool pt, bool compact, int pb) {
     int* cG = cEE + (s.Es + 1);
     const int* oPT = b.off_all.p;
     const int* oEE = oPT + (s.Vs + 1);
-    auto trav_pt = [&](auto k) { hipLaunchKernelGGL(k, grid, block, 0, c.stream, s.Vs, s.sv_perm.p, s.sv.p, s.x.p, ft, sq, cPT,
+    auto trav_pt = [&](auto k) { hipLaunchKernelGGL(k, grid, block, 0, st, s.Vs, s.sv_perm.p, s.sv.p, s.x.p, ft, sq, cPT,
                                                     b.cntpPT.p, b.bufPT.p, cG, s.gn, s.gd, s.hhat); };
-    auto trav_ee = [&](auto k) { hipLaunchKernelGGL(k, grid, block, 0, c.stream, s.Es, s.se_perm.p, s.se.p, s.x.p, et, sq, cEE,
+    auto trav_ee = [&](auto k) { hipLaunchKernelGGL(k, grid, block, 0, st, s.Es, s.se_perm.p, s.se.p, s.x.p, et, sq, cEE,
                                                     b.cntpEE.p, b.bufEE.p); };
-    auto comp_pt = [&](auto k) { hipLaunchKernelGGL(k, grid, block, 0, c.stream, s.Vs, s.sv_perm.p, s.sv.p, s.x.p, ft, sq, oPT,
+    auto comp_pt = [&](auto k) { hipLaunchKernelGGL(k, grid, block, 0, st, s.Vs, s.sv_perm.p, s.sv.p, s.x.p, ft, sq, oPT,
                                                     b.cntpPT.p, b.bufPT.p, b.candUPT.p, b.ownPT.p, dc); };
-    auto comp_ee = [&](auto k) { hipLaunchKernelGGL(k, grid, block, 0, c.stream, s.Es, s.se_perm.p, s.se.p, s.x.p, et, sq, oEE,
+    auto comp_ee = [&](auto k) { hipLaunchKernelGGL(k, grid, block, 0, st, s.Es, s.se_perm.p, s.se.p, s.x.p, et, sq, oEE,
                                                     b.cntpEE.p, b.bufEE.p, b.candUEE.p, b.ownEE.p, dc); };
 #define CIPC_SPLIT(F, K)                 \
     switch (pb) {                        \
@@ -886,7 +890,11 @@ void launch_active_walk(Context& c, bool pt, bool compact, int pb) {
 // nE <= nEE, CCD pairs nPTk + nEEk <= nPT + nEE, ground nG <= Vs).
 void compute_active_set(Context& c, bool publish) {
     Scene& s = c.sc;
-    build_static_trees(c);
+    // the face tree and the point walk on the main stream, the edge tree and the edge walk on
+    // the second active-set stream, beside them (small launches that leave the GPU room)
+    CIPC_HIP(hipEventRecord(c.as2_fork, c.stream));
+    CIPC_HIP(hipStreamWaitEvent(c.as2, c.as2_fork, 0));
+    build_static_trees_on(c, c.stream, c.tmp, c.as2, c.tmp_as2);
     ActiveSetScratch& b = c.as;
     const double sq = sqrt(s.hhat);
     const int Vs = s.Vs, Es = s.Es;
@@ -911,10 +919,13 @@ void compute_active_set(Context& c, bool publish) {
     size_active_walks(c, c.tune.as_pt_bits, c.tune.as_ee_bits);
     // A: one walk per query part (the point walk also sets the ground flags)
     if (Vs > 0) launch_active_walk(c, true, false, c.tune.as_pt_bits);
-    if (Es > 0) launch_active_walk(c, false, false, c.tune.as_ee_bits);
+    if (Es > 0) launch_active_walk(c, false, false, c.tune.as_ee_bits, c.as2);
     // (the walks also zero the scan segments' tails)
     if (Vs == 0) hipLaunchKernelGGL(point_tails, dim3(1), dim3(1), 0, c.stream, Vs, cPT, cG);
-    if (Es == 0) hipLaunchKernelGGL(fill_tail, dim3(1), dim3(1), 0, c.stream, cEE, Es);
+    if (Es == 0) hipLaunchKernelGGL(fill_tail, dim3(1), dim3(1), 0, c.as2, cEE, Es);
+    CIPC_CHECK_LAUNCH();
+    CIPC_HIP(hipEventRecord(c.as2_join, c.as2));
+    CIPC_HIP(hipStreamWaitEvent(c.stream, c.as2_join, 0));
     // one scan: the edge and ground segments' starts come out offset by their first entry
     exclusive_scan_int(c, b.cnt_all.p, b.off_all.p, nall);
     hipLaunchKernelGGL(candidate_totals, dim3(1), dim3(64), 0, c.stream, Vs, Es, oPT, oEE, capPT, capEE, dc);

@@ -656,23 +656,20 @@ void size_contact_records(Context& c) {
 // records.  Slots past a device count are written empty, so the counts stay on the device;
 // records keep the order ground, contacts, friction, and the sort is stable, so every block
 // and gradient is summed in the same order as with packed records.
-void contact_assemble(Context& c, double kappa) {
+// Record slots and buffers of the next contact assembly (host side only).  With the counts
+// still on the device, the capacity of the last assembly (grown by the host when the device
+// reports it exceeded).
+void contact_records_prepare(Context& c) {
     Scene& s = c.sc;
-    // with the counts still on the device, the capacity of the last assembly (grown by the
-    // host when the device reports it exceeded)
     if (!c.dc_pending) size_contact_records(c);
     else if (c.capS == 0) c.capS = 1024;
     const int capS = c.capS, capG = s.Vs;
-    const int* dc = c.dcnt.p;
     const size_t nbc = (size_t)capG + 10 * (size_t)capS, ngc = (size_t)capG + 4 * (size_t)capS;
     const size_t nb = nbc + (size_t)c.n_fenv + 10 * (size_t)c.n_fself, ng = ngc + (size_t)c.n_fenv + 4 * (size_t)c.n_fself;
     c.ckey.resize(std::max<size_t>(nb, 1)); c.cidx.resize(std::max<size_t>(nb, 1));
     c.cval.resize(9 * std::max<size_t>(nb, 1));
     c.gkey.resize(std::max<size_t>(ng, 1)); c.gidx.resize(std::max<size_t>(ng, 1));
     c.gval.resize(3 * std::max<size_t>(ng, 1));
-    // lagged friction (IPCSolver.cpp:309-317): its records go after the barrier records and
-    // depend on the lagged state and the positions only, so they are built on the auxiliary
-    // stream beside the barrier blocks (buffers sized first: a reallocation synchronises)
     if (c.n_fself > 0) {
         c.fhess.resize((size_t)(kHU + 12) * c.n_fself);
         c.fverts.resize(c.n_fself);
@@ -681,6 +678,25 @@ void contact_assemble(Context& c, double kappa) {
     c.chess.resize((size_t)(kHU + 12) * capS);
     c.cverts.resize(capS);
     c.cdims.resize(capS);
+    c.ckey.n = nb;
+    c.gkey.n = ng;
+    c.records_capS = capS;
+}
+
+// Lagged friction (IPCSolver.cpp:309-317): its records go after the barrier records and depend
+// on the lagged state and the positions only, so they can be built on `st` from the start of
+// an iteration (slots of contact_records_prepare).
+void friction_records_launch(Context& c, hipStream_t st) {
+    const size_t nbc = (size_t)c.sc.Vs + 10 * (size_t)c.records_capS, ngc = (size_t)c.sc.Vs + 4 * (size_t)c.records_capS;
+    friction_records(c, nbc, ngc, st);
+}
+
+void contact_assemble(Context& c, double kappa, bool friction_launched) {
+    Scene& s = c.sc;
+    if (!friction_launched) contact_records_prepare(c);
+    const int capS = c.records_capS, capG = s.Vs;
+    const int* dc = c.dcnt.p;
+    const size_t nbc = (size_t)capG + 10 * (size_t)capS, ngc = (size_t)capG + 4 * (size_t)capS;
     if (capG == 0) hipLaunchKernelGGL(contact_totals, dim3(1), dim3(64), 0, c.stream, capS, c.dcnt.p);
     CIPC_HIP(hipEventRecord(c.fr_fork, c.stream));
     if (capG > 0) {
@@ -704,7 +720,7 @@ void contact_assemble(Context& c, double kappa) {
     hipLaunchKernelGGL((contact_pd_kernel<4, false>), dim3(blocks_for(capS, 64)), dim3(64), 0, c.stream, dc, 1, capS, c.cdims.p, Hs, Gs);
     CIPC_CHECK_LAUNCH();
     CIPC_HIP(hipStreamWaitEvent(c.aux, c.fr_fork, 0));
-    friction_records(c, nbc, ngc, c.aux);
+    if (!friction_launched) friction_records(c, nbc, ngc, c.aux);
     CIPC_HIP(hipStreamWaitEvent(c.aux, c.cl_done, 0));
     hipLaunchKernelGGL((contact_pd_kernel<3, true>), dim3(blocks_for(capS, 64)), dim3(64), 0, c.aux, dc, 0, capS, c.cdims.p, Hs, Gs);
     hipLaunchKernelGGL((contact_pd_kernel<2, false>), dim3(blocks_for(capS, 64)), dim3(64), 0, c.aux, dc, 0, capS, c.cdims.p, Hs, Gs);
@@ -714,8 +730,6 @@ void contact_assemble(Context& c, double kappa) {
     hipLaunchKernelGGL(contact_scatter_kernel, dim3(blocks_for(capS, 128)), dim3(128), 0, c.stream, capS, dc, c.cverts.p, c.cdims.p, Hs,
                        Gs, s.bt.p, capG, capG, s.kb, c.ckey.p, c.cidx.p, c.cval.p, c.gkey.p, c.gidx.p, c.gval.p);
     CIPC_CHECK_LAUNCH();
-    c.ckey.n = nb;
-    c.gkey.n = ng;
     // the counts again, now with the contact-capacity check, for the host's next synchronisation
     if (c.dc_pending) CIPC_HIP(hipMemcpyAsync(c.pin_dc.p, c.dcnt.p, kDcWords * sizeof(int), hipMemcpyDeviceToHost, c.stream));
 }

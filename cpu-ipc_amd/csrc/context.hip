@@ -87,6 +87,9 @@ int cipc_gpu_context_create(int device, cipc_gpu_context** out) {
         CIPC_HIP(hipEventCreateWithFlags(&c->sb_done, hipEventDisableTiming));
         CIPC_HIP(hipEventCreateWithFlags(&c->it_start, hipEventDisableTiming));
         CIPC_HIP(hipEventCreateWithFlags(&c->en_done, hipEventDisableTiming));
+        CIPC_HIP(hipStreamCreateWithFlags(&c->as2, hipStreamNonBlocking));
+        CIPC_HIP(hipEventCreateWithFlags(&c->as2_fork, hipEventDisableTiming));
+        CIPC_HIP(hipEventCreateWithFlags(&c->as2_join, hipEventDisableTiming));
         *out = c;
     });
 }
@@ -98,6 +101,10 @@ int cipc_gpu_context_destroy(cipc_gpu_context* c) {
         (void)hipStreamSynchronize(c->stream);
         (void)hipStreamSynchronize(c->aux);
         (void)hipStreamSynchronize(c->sbs);
+        (void)hipStreamSynchronize(c->as2);
+        (void)hipEventDestroy(c->as2_fork);
+        (void)hipEventDestroy(c->as2_join);
+        (void)hipStreamDestroy(c->as2);
         (void)hipEventDestroy(c->fork);
         (void)hipEventDestroy(c->join);
         (void)hipEventDestroy(c->fr_fork);
@@ -895,20 +902,32 @@ struct IterEvents {
 // The streams fork from and join into the main stream through the context's events, so the
 // same launches can be captured into a graph.  t (eager only): per-stage timing events.
 void enqueue_iteration(cipc_gpu_context* c, double kappa, bool energy, const IterEvents* t) {
+    // The host enqueues the critical path first (the active set on the main stream and its
+    // edge half on `as2`), then the streams with slack: the elastic stage and the lagged
+    // friction records on `aux` (they read the state and the lagged friction only), the
+    // ground bound, swept trees and full CCD on `sbs`; then the barrier records and the CSR
+    // on the main stream, the partial CCD on `sbs` and the energy on `aux` once the active
+    // set exists.
     if (t) CIPC_HIP(hipEventRecord(t->ev[0], c->stream));
     CIPC_HIP(hipEventRecord(c->it_start, c->stream));
-    elastic_fork(c, t ? t->es : nullptr, t ? t->ee : nullptr);
-    CIPC_HIP(hipStreamWaitEvent(c->sbs, c->it_start, 0));
-    if (t) CIPC_HIP(hipEventRecord(t->s0, c->sbs));
-    step_bound_early(*c, c->sbs);
+    contact_records_prepare(*c);
     compute_active_set(*c, false);   // contact_assemble publishes the counts
     if (t) CIPC_HIP(hipEventRecord(t->ev[1], c->stream));
     CIPC_HIP(hipEventRecord(c->as_done, c->stream));
+    CIPC_HIP(hipStreamWaitEvent(c->aux, c->it_start, 0));
+    if (t) CIPC_HIP(hipEventRecord(t->es, c->aux));
+    elastic_assemble(*c, c->aux);
+    if (t) CIPC_HIP(hipEventRecord(t->ee, c->aux));
+    friction_records_launch(*c, c->aux);
+    CIPC_HIP(hipEventRecord(c->join, c->aux));
+    CIPC_HIP(hipStreamWaitEvent(c->sbs, c->it_start, 0));
+    if (t) CIPC_HIP(hipEventRecord(t->s0, c->sbs));
+    step_bound_early(*c, c->sbs);
     CIPC_HIP(hipStreamWaitEvent(c->sbs, c->as_done, 0));
     step_bound_partial(*c, c->sbs);
     if (t) CIPC_HIP(hipEventRecord(t->s1, c->sbs));
     CIPC_HIP(hipEventRecord(c->sb_done, c->sbs));
-    contact_assemble(*c, kappa);
+    contact_assemble(*c, kappa, true);
     if (energy) {
         if (t) CIPC_HIP(hipEventRecord(t->e0, c->aux));
         energy_terms_launch(*c, c->aux);
@@ -982,7 +1001,7 @@ bool iteration_graph(cipc_gpu_context* c, double kappa, bool energy, hipEvent_t 
             // a launch that cannot be captured: no graph for this context from now on; every
             // stream leaves capture and drains before the launches go eagerly
             c->graph.mode = 0;
-            for (hipStream_t st : {c->stream, c->aux, c->sbs}) {
+            for (hipStream_t st : {c->stream, c->aux, c->sbs, c->as2}) {
                 hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
                 if (hipStreamIsCapturing(st, &cs) == hipSuccess && cs != hipStreamCaptureStatusNone) {
                     hipGraph_t dropped = nullptr;

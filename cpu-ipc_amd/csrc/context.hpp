@@ -123,6 +123,11 @@ struct Context {
     // the trees (the assembly does not read them) and sorts with its own temporary storage.
     hipStream_t sbs = nullptr;
     hipEvent_t as_done = nullptr, sb_done = nullptr;
+    // The active set builds the edge tree and walks it on `as2` (default priority) while the
+    // main stream builds the face tree and walks it (fork / join events; own temporary storage).
+    hipStream_t as2 = nullptr;
+    hipEvent_t as2_fork = nullptr, as2_join = nullptr;
+    DBuf<unsigned char> tmp_as2;
     // start of an iteration (the step-bound stream forks there) and end of its energy (joins
     // the main stream): a captured iteration forks and joins through these
     hipEvent_t it_start = nullptr, en_done = nullptr;
@@ -162,6 +167,7 @@ struct Context {
     PinnedBuf<int> pin_dc;
     bool dc_pending = false;   // the device counts are newer than the host copies
     int capS = 0;              // contact (self + mollified) capacity of the contact records
+    int records_capS = 0;      // the capacity the current record slots were laid out with
     DBuf<int4> self_set, ee_set;
     DBuf<int2> ee_ie, ccd_set;
     DBuf<int> env_set;
@@ -264,7 +270,12 @@ void build_tet_chunks(Context& c, const double* hx);
 void elastic_assemble(Context& c, hipStream_t st);
 void cloth_assemble(Context& c, hipStream_t st);
 // contact.hip
-void contact_assemble(Context& c, double kappa);
+// contact_records_prepare sizes the records (host only); friction_records_launch builds the
+// lagged friction records on a stream; contact_assemble the barrier records (and, unless
+// friction_launched, prepares and launches the friction records itself on `aux`).
+void contact_records_prepare(Context& c);
+void friction_records_launch(Context& c, hipStream_t st);
+void contact_assemble(Context& c, double kappa, bool friction_launched = false);
 // sparse.hip
 void finalize_csr(Context& c);
 void exclusive_scan_int(Context& c, const int* in, int* out, int n);
@@ -327,7 +338,7 @@ void step_bound_early(Context& c, hipStream_t st, bool ground = true);
 void step_bound_partial(Context& c, hipStream_t st);
 void step_bound_finish(Context& c, double* out);
 void size_active_walks(Context& c, int pt, int ee);
-void launch_active_walk(Context& c, bool pt, bool compact, int pb);
+void launch_active_walk(Context& c, bool pt, bool compact, int pb, hipStream_t st = nullptr);
 void launch_full_ccd_pt(Context& c, hipStream_t st, int pb);
 void launch_full_ccd_ee(Context& c, hipStream_t st, int pb);
 // the autotuner: one entry per tuned kernel (0 full CCD point-triangle, 1 edge-edge, 3 the
