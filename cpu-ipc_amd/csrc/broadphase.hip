@@ -934,17 +934,24 @@ void compute_active_set(Context& c, bool publish) {
     b.candUPT.resize(capPT + 1); b.candUEE.resize(capEE + 1);
     b.candPT.resize(capPT + 1); b.recPT.resize(capPT + 1); b.catPT.resize(capPT + 1); b.ownPT.resize(capPT + 1);
     b.candEE.resize(capEE + 1); b.recEE.resize(capEE + 1); b.catEE.resize(capEE + 1); b.auxEE.resize(capEE + 1); b.ownEE.resize(capEE + 1);
+    // the edge hits on the second active-set stream (forked here, before the point hits are
+    // enqueued on the main stream), beside the point hits
+    if (Es > 0) {
+        CIPC_HIP(hipEventRecord(c.as2_fork2, c.stream));
+        CIPC_HIP(hipStreamWaitEvent(c.as2, c.as2_fork2, 0));
+        launch_active_walk(c, false, true, c.tune.as_ee_bits, c.as2);
+        hipLaunchKernelGGL(ee_classify, dim3(blocks_for(capEE, 128)), dim3(128), 0, c.as2, dc, Es, b.ownEE.p, oEE, s.se.p,
+                           s.x.p, s.vrest.p, s.bt.p, b.candUEE.p, b.candEE.p, s.hhat, sq, scene_of_vertex(s), b.recEE.p, b.auxEE.p,
+                           b.catEE.p);
+        CIPC_CHECK_LAUNCH();
+        CIPC_HIP(hipEventRecord(c.as2_join2, c.as2));
+    }
     if (Vs > 0) {
         launch_active_walk(c, true, true, c.tune.as_pt_bits);
         hipLaunchKernelGGL(pt_classify, dim3(blocks_for(capPT, 128)), dim3(128), 0, c.stream, dc, s.F, b.ownPT.p, oPT, s.sv.p, s.x.p,
                            s.bt.p, s.faces.p, b.candUPT.p, b.candPT.p, s.hhat, sq, scene_of_vertex(s), b.recPT.p, b.catPT.p);
     }
-    if (Es > 0) {
-        launch_active_walk(c, false, true, c.tune.as_ee_bits);
-        hipLaunchKernelGGL(ee_classify, dim3(blocks_for(capEE, 128)), dim3(128), 0, c.stream, dc, Es, b.ownEE.p, oEE, s.se.p,
-                           s.x.p, s.vrest.p, s.bt.p, b.candUEE.p, b.candEE.p, s.hhat, sq, scene_of_vertex(s), b.recEE.p, b.auxEE.p,
-                           b.catEE.p);
-    }
+    if (Es > 0) CIPC_HIP(hipStreamWaitEvent(c.stream, c.as2_join2, 0));
     CIPC_CHECK_LAUNCH();
     // category positions
     const unsigned keptPT = (1u << kNone) | (1u << kSelfPT) | (1u << kMap);

@@ -90,6 +90,8 @@ int cipc_gpu_context_create(int device, cipc_gpu_context** out) {
         CIPC_HIP(hipStreamCreateWithFlags(&c->as2, hipStreamNonBlocking));
         CIPC_HIP(hipEventCreateWithFlags(&c->as2_fork, hipEventDisableTiming));
         CIPC_HIP(hipEventCreateWithFlags(&c->as2_join, hipEventDisableTiming));
+        CIPC_HIP(hipEventCreateWithFlags(&c->as2_fork2, hipEventDisableTiming));
+        CIPC_HIP(hipEventCreateWithFlags(&c->as2_join2, hipEventDisableTiming));
         *out = c;
     });
 }
@@ -104,6 +106,8 @@ int cipc_gpu_context_destroy(cipc_gpu_context* c) {
         (void)hipStreamSynchronize(c->as2);
         (void)hipEventDestroy(c->as2_fork);
         (void)hipEventDestroy(c->as2_join);
+        (void)hipEventDestroy(c->as2_fork2);
+        (void)hipEventDestroy(c->as2_join2);
         (void)hipStreamDestroy(c->as2);
         (void)hipEventDestroy(c->fork);
         (void)hipEventDestroy(c->join);
