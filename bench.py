@@ -45,6 +45,36 @@ def elastic_algorithmic_bytes(scene):
     return 24 * scene.V + scene.T * (72 + 8 + 16) + 24 * scene.V + 8 * nnz, nnz
 
 
+def stage_rooflines(scene, a, nnz, nnz_el, mean):
+    """HBM rate of each stage of the iteration against its minimal traffic (each input read
+    once, each output written once), from the stage times of the HIP events.  a: the active
+    set's arrays (their bytes are the active set's output and the contact stages' input).
+    Surface positions 24 B per surface vertex, faces 12 B, edges 8 B; the lagged friction
+    state 80 B per contact (basis, coordinates, multiplier); the CSR 4 B per row start,
+    12 B per nonzero; gradient and right-hand side 24 B per vertex each."""
+    Vs, F, Es, V, T = len(scene.surf_verts), len(scene.faces), len(scene.surf_edges), scene.V, scene.T
+    topo = 24 * Vs + 12 * F + 8 * Es
+    sets = sum(int(v.nbytes) for v in a.values())
+    contact_sets = sets - int(a["ccd_set"].nbytes)
+    contacts = len(a["self_set"]) + len(a["ee_set"]) + len(a["env_set"])
+    b = {
+        "broadphase_ms": ("active set: tree refit and walks, candidate filters, sets", topo + sets),
+        "assembly_ms": ("contact blocks, friction, merge into the CSR with the elastic values",
+                        24 * Vs + contact_sets + 80 * contacts + 8 * nnz_el + 24 * V
+                        + 4 * (3 * V + 1) + 12 * nnz + 48 * V),
+        "ccd_ms": ("step bound: ground, CFL, swept trees, full and partial CCD",
+                   topo + 24 * Vs + int(a["ccd_set"].nbytes) + 8),
+        "energy_ms": ("line-search energy: inertia, elasticity, contacts, friction",
+                      56 * V + 96 * T + 24 * Vs + contact_sets + 80 * contacts + 8),
+    }
+    out = {}
+    for k, (what, nbytes) in b.items():
+        gbs = nbytes / (mean[k] * 1e-3) / 1e9
+        out[k[:-3]] = {"what": what, "algorithmic_bytes": int(nbytes), "ms": round(mean[k], 4),
+                       "achieved_GBs": round(gbs, 2), "frac": round(gbs / HBM_PEAK_GBS, 5)}
+    return out
+
+
 def cpu_model():
     try:
         for line in open("/proc/cpuinfo"):
@@ -155,7 +185,7 @@ def pcg_leg(ctx, kappa, tol=1e-6, max_iter=10000, repeats=3):
     return out
 
 
-def batch_leg(n_scenes=256, steps=5, seed=0):
+def batch_leg(n_scenes=256, steps=5, seed=0, solve=True):
     """The batched workload: n independent lattice cubes (8^3..24^3 cells, per-scene E, nu,
     drop height) in one batch context, each at its first Newton iteration (frozen state,
     probe direction).  Times the batched Newton iteration (HIP events) with the default
@@ -182,10 +212,12 @@ def batch_leg(n_scenes=256, steps=5, seed=0):
     b.newton_iteration(kappa)
     itm = [b.newton_iteration(kappa) for _ in range(steps)]
     b.set_hessian_precision(0)
-    b.assemble(kappa)
-    _, st = b.solve(precision=1, rel_tol=1e-6, preconditioner=1)
-    solve_ms = st[0]["solve_ms"]
-    iters = [s["iterations"] for s in st]
+    solve_ms, iters = None, None
+    if solve:
+        b.assemble(kappa)
+        _, st = b.solve(precision=1, rel_tol=1e-6, preconditioner=1)
+        solve_ms = round(st[0]["solve_ms"], 3)
+        iters = [int(min(s["iterations"] for s in st)), int(max(s["iterations"] for s in st))]
     out = {"scenes": n_scenes, "V": int(b.V), "T": int(sum(s.T for s in sc)), "newton_iteration_ms": round(ms, 3),
            "scene_iterations_per_s": round(n_scenes / (ms * 1e-3), 1),
            "stages_ms": {k: round(float(np.mean([t[k] for t in it])), 3) for k in
@@ -197,7 +229,7 @@ def batch_leg(n_scenes=256, steps=5, seed=0):
            "mixed_fp32_hessian": {"newton_iteration_ms": round(float(np.mean([t["total_ms"] for t in itm])), 3),
                                   "elastic_ms": round(float(np.mean([t["elastic_ms"] for t in itm])), 3)},
            "contacts": int(it[-1]["contacts"]), "min_alpha": float(it[-1]["alpha"]),
-           "solve_mixed_fsai_ms": round(solve_ms, 3), "solve_iterations_min_max": [int(min(iters)), int(max(iters))],
+           "solve_mixed_fsai_ms": solve_ms, "solve_iterations_min_max": iters,
            "setup_s": round(setup_s, 1)}
     b.close()
     return out
@@ -270,7 +302,7 @@ def main():
     ap.add_argument("--steps", type=int, default=50)
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
-    ap.add_argument("--no-pcg", action="store_true", help="skip the linear-solve leg")
+    ap.add_argument("--no-pcg", action="store_true", help="skip the linear solves (the single-scene leg and the batch's)")
     ap.add_argument("--no-time-step", action="store_true", help="skip the whole-time-step leg")
     ap.add_argument("--traffic", default=os.path.join(ROOT, "profiles", "elastic_traffic_4x20cubes.json"),
                     help="JSON file with the measured HBM bytes per launch of the elastic stage (rocprofv3 FETCH_SIZE + "
@@ -325,6 +357,7 @@ def main():
     ctx.set_tune(2, 0)
     mixed_out = {"ms_per_iter": round(mixed_wall, 4), "graph_ms_per_iter": round(mixed_graph_ms, 4),
                  "stages_ms": {k: round(float(np.mean([s[k] for s in mixed])), 4) for k in stage_keys}}
+    aset = ctx.active_set()
     pcg = pcg_leg(ctx, kappa) if not args.no_pcg else None
     ctx.close()
 
@@ -378,6 +411,8 @@ def main():
                                       "vector/matrix rate"}
     else:
         out["roofline"] = dict(out["assembly_hbm"], bound="hbm")
+    # every other stage against HBM: all far below it (launch- and dependency-bound)
+    out["stages_roofline"] = stage_rooflines(scene, aset, int(last["nnz"]), nnz_el, mean)
     out["mixed_fp32_hessian"] = mixed_out
     if pcg is not None:
         out["pcg"] = pcg
@@ -388,7 +423,7 @@ def main():
     if not args.no_time_step:
         out["time_step"] = time_step_leg(scene)
     if args.batch > 0:
-        out["batch"] = batch_leg(args.batch)
+        out["batch"] = batch_leg(args.batch, solve=not args.no_pcg)
     if args.pcg_large:
         out["pcg_large"] = pcg_large_leg()
     if not args.no_cpu:

@@ -81,7 +81,7 @@ class IterationTiming(C.Structure):
                 ("energy_ms", C.c_double), ("total_ms", C.c_double), ("alpha", C.c_double),
                 ("energy", C.c_double), ("nnz", C.c_int64), ("contacts", C.c_int64), ("collisions", C.c_int64),
                 ("bound_ground", C.c_double), ("bound_partial", C.c_double), ("bound_cfl", C.c_double),
-                ("bound_full", C.c_double), ("graph", C.c_int)]
+                ("bound_full", C.c_double), ("graph", C.c_int), ("enqueue_ms", C.c_double)]
 
 
 class SolveStats(C.Structure):

@@ -146,35 +146,15 @@ __global__ void skip_kernel(int n, const int* __restrict__ range_last, const int
 }
 // Internal boxes.  Node i of the Karras tree covers the sorted leaves [first, last], so its
 // box is the union of that range.  Power-of-two reduction levels over the sorted leaf boxes
-// (level l node j = leaves [j 2^l, (j+1) 2^l)) are built 8 levels per launch in LDS; each
-// internal box is then the union of at most 2 log2(n) level nodes.  No inter-work-group
-// communication is needed.
+// (level l node j = leaves [j 2^l, (j+1) 2^l)), levels 1..8 built in LDS by the launch that
+// writes the leaves; each internal box is then the union of a few level nodes.  No
+// inter-work-group communication is needed.
 constexpr int kSegBlock = 256;
 constexpr int kSegLevels = 8;   // log2(kSegBlock)
 __device__ inline Box empty_box() {
     return Box{{1.7976931348623157e308, 1.7976931348623157e308, 1.7976931348623157e308},
                {-1.7976931348623157e308, -1.7976931348623157e308, -1.7976931348623157e308}};
 }
-__global__ __launch_bounds__(kSegBlock) void seg_build(const Box* __restrict__ src, int nsrc, int lvl0, SegLevels L, Box* __restrict__ seg) {
-    __shared__ Box sh[kSegBlock];
-    const int g = blockIdx.x * kSegBlock + threadIdx.x;
-    sh[threadIdx.x] = g < nsrc ? src[g] : empty_box();
-    __syncthreads();
-    for (int l = 1; l <= kSegLevels; ++l) {
-        const int half = 1 << (l - 1);
-        if ((threadIdx.x & ((1 << l) - 1)) == 0) {
-            const Box a = sh[threadIdx.x], b = sh[threadIdx.x + half];
-            sh[threadIdx.x] = Box{vmin(a.lo, b.lo), vmax(a.hi, b.hi)};
-        }
-        __syncthreads();
-        const int lv = lvl0 + l;
-        if (lv < 40 && L.count[lv] > 0 && (threadIdx.x & ((1 << l) - 1)) == 0) {
-            const int j = g >> l;
-            if (j < L.count[lv]) seg[L.offset[lv] + j] = sh[threadIdx.x];
-        }
-    }
-}
-
 // Primitive boxes: static (CPUBroadPhaseBVH::build, LBVH.cpp:284-305) or swept along -p
 // with the step whose bits are at *abits (buildSwept, LBVH.cpp:307-339).  The step is read
 // on the device, so building the swept trees needs no host round trip.
@@ -211,39 +191,79 @@ __global__ void prim_boxes(int n, PrimSrc src, Box* __restrict__ out) {
     if (i < n) out[i] = prim_box(src, i);
 }
 
-// Box of the sorted leaves [a, b]: the union of at most 2 log2(n) level nodes (exact: the
-// union is a min / max).
+// Box of the sorted leaves [a, b] from reduction levels 0..kSegLevels: the union of at most
+// 2 kSegLevels level nodes below level kSegLevels and of the level-kSegLevels nodes the range
+// still spans there (ceil(n / 256) at most: only ranges of more than 512 leaves reach it).
+// Exact: the union is a min / max.
 __device__ inline Box range_box(int a, int b, const SegLevels& L, const Box* __restrict__ seg) {
     int l = 0;
     Box acc = empty_box();
-    while (a <= b && l < 40) {
+    while (a <= b && l < kSegLevels) {
         if (a & 1) { const Box x = seg[L.offset[l] + a]; acc = Box{vmin(acc.lo, x.lo), vmax(acc.hi, x.hi)}; ++a; }
         if (a <= b && !(b & 1)) { const Box x = seg[L.offset[l] + b]; acc = Box{vmin(acc.lo, x.lo), vmax(acc.hi, x.hi)}; --b; }
         a >>= 1; b >>= 1; ++l;
     }
+    for (; a <= b; ++a) { const Box x = seg[L.offset[l] + a]; acc = Box{vmin(acc.lo, x.lo), vmax(acc.hi, x.hi)}; }
     return acc;
 }
-// Leaf boxes in leaf order (node records and level 0) and the first kSegLevels reduction
-// levels, in one launch; with a primitive source, the primitive boxes are computed here too
-// (and stored, for the exact leaf tests).
-__global__ __launch_bounds__(kSegBlock) void leaf_seg_build(int n, const int* __restrict__ order, Box* __restrict__ prim,
-                                                            PrimSrc src, Node* __restrict__ nodes, SegLevels L,
-                                                            Box* __restrict__ seg) {
+// A box into its slot of a four-wide record: slot code (record << 2 | k), or -1 for none.
+__device__ inline void put_wide_box(Node4* __restrict__ wide, int code, const Box& x) {
+    if (code < 0) return;
+    Node4& r = wide[code >> 2];
+    const int k = code & 3;
+    r.lo[k][0] = __double2float_rd(x.lo.x); r.lo[k][1] = __double2float_rd(x.lo.y); r.lo[k][2] = __double2float_rd(x.lo.z);
+    r.hi[k][0] = __double2float_ru(x.hi.x); r.hi[k][1] = __double2float_ru(x.hi.y); r.hi[k][2] = __double2float_ru(x.hi.z);
+}
+// A tree's refit in two launches, for one tree or two at once (a work-group range each):
+//   leaves: the leaf boxes in leaf order (node records and level 0) and reduction levels
+//     1..kSegLevels, each work-group over its kSegBlock leaves; with a primitive source the
+//     primitive boxes are computed here too (and stored, for the exact leaf tests).
+//   internal: each internal node's box, the union of the level boxes that cover its leaf
+//     range (range_box).
+// No work-group waits for another inside a launch: on this GPU that would take a device-wide
+// fence, which writes back and invalidates an XCD's L2.
+// Each box also goes, rounded outward to float, into the four-wide record slots that hold
+// its node (wslot, set with the topology), so the records need no launch of their own.
+struct RefitTree {
+    int n = 0, levels = 0, leaf_blocks = 0, int_blocks = 0;
+    const int* order = nullptr;
+    Box* prim = nullptr;
+    PrimSrc src;
+    Node* nodes = nullptr;
+    Node4* wide = nullptr;
+    const int* first = nullptr;
+    const int* last = nullptr;
+    Box* seg = nullptr;
+    const int2* wslot = nullptr;
+    SegLevels L;
+};
+struct RefitSet {
+    RefitTree t[2];
+    int ntrees = 1;
+};
+__global__ __launch_bounds__(kSegBlock) void refit_leaves(RefitSet rs) {
+    const int which = (rs.ntrees > 1 && (int)blockIdx.x >= rs.t[0].leaf_blocks) ? 1 : 0;
+    const RefitTree& t = rs.t[which];
+    const int blk = (int)blockIdx.x - (which ? rs.t[0].leaf_blocks : 0);
     __shared__ Box sh[kSegBlock];
-    const int g = blockIdx.x * kSegBlock + threadIdx.x;
+    const int n = t.n;
+    const int g = blk * kSegBlock + threadIdx.x;
     Box b = empty_box();
     if (g < n) {
-        const int q = order[g];
-        if (src.kind != 0) {
-            b = prim_box(src, q);
-            prim[q] = b;
+        const int q = t.order[g];
+        if (t.src.kind != 0) {
+            b = prim_box(t.src, q);
+            t.prim[q] = b;
         } else {
-            b = prim[q];
+            b = t.prim[q];
         }
-        Node& r = nodes[n - 1 + g];
+        Node& r = t.nodes[n - 1 + g];
         r.lo = b.lo;
         r.hi = b.hi;
-        seg[g] = b;
+        t.seg[g] = b;
+        const int2 ws = t.wslot[n - 1 + g];
+        put_wide_box(t.wide, ws.x, b);
+        put_wide_box(t.wide, ws.y, b);
     }
     sh[threadIdx.x] = b;
     __syncthreads();
@@ -254,44 +274,59 @@ __global__ __launch_bounds__(kSegBlock) void leaf_seg_build(int n, const int* __
             sh[threadIdx.x] = Box{vmin(a.lo, c.lo), vmax(a.hi, c.hi)};
         }
         __syncthreads();
-        if (l < 40 && L.count[l] > 0 && (threadIdx.x & ((1 << l) - 1)) == 0) {
+        if (l < 40 && t.L.count[l] > 0 && (threadIdx.x & ((1 << l) - 1)) == 0) {
             const int j = g >> l;
-            if (j < L.count[l]) seg[L.offset[l] + j] = sh[threadIdx.x];
+            if (j < t.L.count[l]) t.seg[t.L.offset[l] + j] = sh[threadIdx.x];
         }
     }
 }
-__global__ void internal_boxes(int n, const int* __restrict__ first, const int* __restrict__ last, SegLevels L,
-                               const Box* __restrict__ seg, Node* __restrict__ nodes) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n - 1) return;
-    const Box acc = range_box(first[i], last[i], L, seg);
-    nodes[i].lo = acc.lo;
-    nodes[i].hi = acc.hi;
+__global__ void refit_internal(RefitSet rs) {
+    const int which = (rs.ntrees > 1 && (int)blockIdx.x >= rs.t[0].int_blocks) ? 1 : 0;
+    const RefitTree& t = rs.t[which];
+    const int blk = (int)blockIdx.x - (which ? rs.t[0].int_blocks : 0);
+    const int i = blk * blockDim.x + threadIdx.x;
+    if (i >= t.n - 1) return;
+    const Box own = range_box(t.first[i], t.last[i], t.L, t.seg);
+    t.nodes[i].lo = own.lo;
+    t.nodes[i].hi = own.hi;
+    put_wide_box(t.wide, t.wslot[i].y, own);
 }
 
-// Four-wide records from the refitted binary nodes.
-// pad[k]: the last leaf position under child k (wide_visit_after)
-__global__ void wide_records(int n, const Node* __restrict__ nodes, const int* __restrict__ range_last, Node4* __restrict__ wide) {
+// The parts of the four-wide records that depend on the topology only, once per topology:
+// what each slot holds and, for every node, the slots its box goes to (wslot.x: the record
+// of its parent, for a leaf child; wslot.y: the record of its grandparent).  A record holds
+// its node's grandchildren, or the children themselves where those are leaves; unused slots
+// hold an empty box.  pad[k]: the last leaf position under child k (wide_visit_after).
+__global__ void fill_int2(int2* __restrict__ p, int n, int2 v) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+__global__ void wide_static(int n, const Node* __restrict__ nodes, const int* __restrict__ range_last, Node4* __restrict__ wide,
+                            int2* __restrict__ wslot) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n - 1) return;
-    int ch[4], m = 0;
+    int ch[4], via[4], m = 0;
     const int c2[2] = {nodes[i].left, nodes[i].right};
     for (int k = 0; k < 2; ++k) {
         const int c = c2[k];
-        if (c < n - 1) { ch[m++] = nodes[c].left; ch[m++] = nodes[c].right; }
-        else ch[m++] = c;
+        if (c < n - 1) {
+            ch[m] = nodes[c].left; via[m++] = 1;
+            ch[m] = nodes[c].right; via[m++] = 1;
+        } else {
+            ch[m] = c; via[m++] = 0;
+        }
     }
     Node4 r;
     for (int k = 0; k < 4; ++k) {
+        r.lo[k][0] = r.lo[k][1] = r.lo[k][2] = __int_as_float(0x7f800000);
+        r.hi[k][0] = r.hi[k][1] = r.hi[k][2] = __int_as_float(0xff800000);
         if (k < m) {
-            const Node& x = nodes[ch[k]];
-            r.lo[k][0] = __double2float_rd(x.lo.x); r.lo[k][1] = __double2float_rd(x.lo.y); r.lo[k][2] = __double2float_rd(x.lo.z);
-            r.hi[k][0] = __double2float_ru(x.hi.x); r.hi[k][1] = __double2float_ru(x.hi.y); r.hi[k][2] = __double2float_ru(x.hi.z);
-            r.child[k] = ch[k] >= n - 1 ? -(x.left + 1) : ch[k];
-            r.pad[k] = ch[k] >= n - 1 ? ch[k] - (n - 1) : range_last[ch[k]];
+            const int c = ch[k];
+            r.child[k] = c >= n - 1 ? -(nodes[c].left + 1) : c;
+            r.pad[k] = c >= n - 1 ? c - (n - 1) : range_last[c];
+            if (via[k]) wslot[c].y = i << 2 | k;
+            else wslot[c].x = i << 2 | k;
         } else {
-            r.lo[k][0] = r.lo[k][1] = r.lo[k][2] = __int_as_float(0x7f800000);
-            r.hi[k][0] = r.hi[k][1] = r.hi[k][2] = __int_as_float(0xff800000);
             r.child[k] = kNoChild;
             r.pad[k] = -1;
         }
@@ -346,32 +381,52 @@ void build_topology(Context& c, LBVH& t, int n, hipStream_t st, DBuf<unsigned ch
     t.leafpos.resize(n);
     hipLaunchKernelGGL(leaf_ids, dim3(blocks_for(n, 256)), dim3(256), 0, st, n, t.order.p, t.nodes.p, t.leafpos.p);
     hipLaunchKernelGGL(skip_kernel, dim3(blocks_for(nodes, 256)), dim3(256), 0, st, n, t.range_last.p, t.split_right.p, t.nodes.p);
+    t.wslot.resize(nodes);
+    hipLaunchKernelGGL(fill_int2, dim3(blocks_for(nodes, 256)), dim3(256), 0, st, t.wslot.p, nodes, make_int2(-1, -1));
+    if (n > 1)
+        hipLaunchKernelGGL(wide_static, dim3(blocks_for(n - 1, 256)), dim3(256), 0, st, n, t.nodes.p, t.range_last.p, t.wide.p,
+                           t.wslot.p);
     CIPC_CHECK_LAUNCH();
 }
 
-// Boxes of the current primitive boxes on the existing topology.
-void refit(Context& c, LBVH& t, const PrimSrc& src, hipStream_t st) {
-    const int n = t.n;
-    // leaves and levels 1..8 in one launch, the higher levels 8 per launch
-    hipLaunchKernelGGL(leaf_seg_build, dim3(blocks_for(n, kSegBlock)), dim3(kSegBlock), 0, st, n, t.order.p, t.prim.p, src,
-                       t.nodes.p, t.L, t.seg.p);
+// Boxes of the current primitive boxes on the existing topologies: one tree, or two in the
+// same two launches.
+static RefitTree refit_view(LBVH& t, const PrimSrc& src) {
+    RefitTree r;
+    r.n = t.n; r.levels = t.levels;
+    r.leaf_blocks = (int)blocks_for(t.n, kSegBlock);
+    r.int_blocks = t.n > 1 ? (int)blocks_for(t.n - 1, 256) : 0;
+    r.order = t.order.p; r.prim = t.prim.p; r.src = src; r.nodes = t.nodes.p; r.wide = t.wide.p;
+    r.first = t.range_first.p; r.last = t.range_last.p; r.seg = t.seg.p; r.wslot = t.wslot.p; r.L = t.L;
+    return r;
+}
+static void refit_set(const RefitSet& rs, hipStream_t st) {
+    int lb = 0, ib = 0;
+    for (int k = 0; k < rs.ntrees; ++k) { lb += rs.t[k].leaf_blocks; ib += rs.t[k].int_blocks; }
+    if (lb > 0) hipLaunchKernelGGL(refit_leaves, dim3(lb), dim3(kSegBlock), 0, st, rs);
+    if (ib > 0) hipLaunchKernelGGL(refit_internal, dim3(ib), dim3(256), 0, st, rs);
     CIPC_CHECK_LAUNCH();
-    if (n > 1) {
-        for (int l0 = kSegLevels; l0 < t.levels; l0 += kSegLevels) {
-            const int nsrc = t.L.count[l0];
-            hipLaunchKernelGGL(seg_build, dim3(blocks_for(nsrc, kSegBlock)), dim3(kSegBlock), 0, st, t.seg.p + t.L.offset[l0],
-                               nsrc, l0, t.L, t.seg.p);
-            CIPC_CHECK_LAUNCH();
-        }
-        hipLaunchKernelGGL(internal_boxes, dim3(blocks_for(n - 1, 256)), dim3(256), 0, st, n, t.range_first.p, t.range_last.p,
-                           t.L, t.seg.p, t.nodes.p);
-        CIPC_CHECK_LAUNCH();
-        hipLaunchKernelGGL(wide_records, dim3(blocks_for(n - 1, 256)), dim3(256), 0, st, n, t.nodes.p, t.range_last.p, t.wide.p);
-        CIPC_CHECK_LAUNCH();
-    }
+}
+void refit(Context& c, LBVH& t, const PrimSrc& src, hipStream_t st) {
+    (void)c;
+    RefitSet rs;
+    rs.t[0] = refit_view(t, src);
+    rs.ntrees = 1;
+    refit_set(rs, st);
     ++t.age;
 }
+static void refit_two(LBVH& a, const PrimSrc& sa, LBVH& b, const PrimSrc& sb, hipStream_t st) {
+    RefitSet rs;
+    rs.t[0] = refit_view(a, sa);
+    rs.t[1] = refit_view(b, sb);
+    rs.ntrees = 2;
+    refit_set(rs, st);
+    ++a.age;
+    ++b.age;
+}
 
+// The tree over n > 0 primitives has a topology that the next build refits.
+static bool refit_only(const LBVH& t, int n) { return n > 0 && t.n == n && t.age < kMaxRefits; }
 // A tree over n primitives whose boxes come from src: a new topology (from boxes computed
 // first) when none fits, else a refit that computes the boxes itself.
 static void build_tree_from(Context& c, LBVH& t, int n, const PrimSrc& src, hipStream_t st, DBuf<unsigned char>& tmp) {
@@ -392,15 +447,6 @@ void build_tree(Context& c, LBVH& t, int n) {
     refit(c, t, PrimSrc{}, c.stream);
 }
 
-// Swept boxes use the step of the primitive's scene: abits[kSlots * scene] (one scene: abits[0]).
-__global__ void point_boxes(int Vs, const int* __restrict__ sv, const double* __restrict__ x, const double* __restrict__ p,
-                            const unsigned long long* __restrict__ abits, const int* __restrict__ vs, Box* __restrict__ out) {
-    const int s = blockIdx.x * blockDim.x + threadIdx.x;
-    if (s >= Vs) return;
-    const int v = sv[s];
-    out[s] = swept_pt(ld3(x, v), ld3(p, v), step_of(vs ? abits + kSlots * vs[v] : abits));
-}
-
 // The static face tree on st_f and edge tree on st_e (each with its own temporary storage).
 static void build_static_trees_on(Context& c, hipStream_t st_f, DBuf<unsigned char>& tmp_f, hipStream_t st_e,
                                   DBuf<unsigned char>& tmp_e) {
@@ -418,14 +464,13 @@ void build_swept_trees(Context& c, const unsigned long long* abits, hipStream_t 
     Scene& s = c.sc;
     c.sface_tree.prim.resize(std::max(s.F, 1));
     c.sedge_tree.prim.resize(std::max(s.Es, 1));
-    c.swept_point_box.resize(std::max(s.Vs, 1));
-    if (s.Vs > 0)
-        hipLaunchKernelGGL(point_boxes, dim3(blocks_for(s.Vs, 256)), dim3(256), 0, st, s.Vs, s.sv.p, s.x.p, s.dir.p, abits, scene_of_vertex(s),
-                           c.swept_point_box.p);
-    CIPC_CHECK_LAUNCH();
     PrimSrc fs, es;
     fs.kind = 1; fs.faces = s.faces.p; fs.x = s.x.p; fs.p = s.dir.p; fs.abits = abits; fs.vs = scene_of_vertex(s);
     es.kind = 2; es.se = s.se.p; es.x = s.x.p; es.p = s.dir.p; es.abits = abits; es.vs = scene_of_vertex(s);
+    if (refit_only(c.sface_tree, s.F) && refit_only(c.sedge_tree, s.Es)) {
+        refit_two(c.sface_tree, fs, c.sedge_tree, es, st);   // both trees in the same two launches
+        return;
+    }
     build_tree_from(c, c.sface_tree, s.F, fs, st, c.tmp_sb);
     build_tree_from(c, c.sedge_tree, s.Es, es, st, c.tmp_sb);
 }
@@ -726,10 +771,9 @@ __device__ inline unsigned long long map_key(int4 r, int V, int bits) {
     return ((unsigned long long)(r.x + V) << (2 * bits)) | ((unsigned long long)r.y << bits) | (unsigned long long)(r.z + 1);
 }
 // Point-triangle hits -> Self_ActiveSet head (a), map keys (b), partial-CCD pairs (c).
-__global__ void scatter_pt(const int* __restrict__ dc, const int* __restrict__ cat, const int4* __restrict__ pos, const int4* __restrict__ rec,
-                           const int* __restrict__ own, const int* __restrict__ cand, int V, int bits, int4* __restrict__ self_set,
-                           unsigned long long* __restrict__ mkeys, int2* __restrict__ ccd) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+__device__ inline void scatter_pt(int i, const int* __restrict__ dc, const int* __restrict__ cat, const int4* __restrict__ pos,
+                                  const int4* __restrict__ rec, const int* __restrict__ own, const int* __restrict__ cand, int V, int bits,
+                                  int4* __restrict__ self_set, unsigned long long* __restrict__ mkeys, int2* __restrict__ ccd) {
     if (dc[kDcFlags] || i >= dc[kDcPT]) return;
     const int ct = cat[i];
     if (ct == kDrop) return;
@@ -739,11 +783,11 @@ __global__ void scatter_pt(const int* __restrict__ dc, const int* __restrict__ c
     else if (ct == kMap) mkeys[p.y] = map_key(rec[i], V, bits);
 }
 // Edge-edge hits -> Self_ActiveSet (a), map keys (b), Self_EE_ActiveSet + eIe (c), CCD pairs (d).
-__global__ void scatter_ee(const int* __restrict__ dc, const int* __restrict__ cat, const int4* __restrict__ pos, const int4* __restrict__ rec,
-                           const int2* __restrict__ aux, const int* __restrict__ own, const int* __restrict__ cand, int V, int bits,
-                           int4* __restrict__ self_set, unsigned long long* __restrict__ mkeys,
-                           int4* __restrict__ ee_set, int2* __restrict__ ee_ie, int2* __restrict__ ccd) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+__device__ inline void scatter_ee(int i, const int* __restrict__ dc, const int* __restrict__ cat, const int4* __restrict__ pos,
+                                  const int4* __restrict__ rec, const int2* __restrict__ aux, const int* __restrict__ own,
+                                  const int* __restrict__ cand, int V, int bits, int4* __restrict__ self_set,
+                                  unsigned long long* __restrict__ mkeys, int4* __restrict__ ee_set, int2* __restrict__ ee_ie,
+                                  int2* __restrict__ ccd) {
     if (dc[kDcFlags] || i >= dc[kDcEE]) return;
     const int base_self = dc[kDcA], base_map = dc[kDcA2], base_ccd = dc[kDcAK];
     const int ct = cat[i];
@@ -773,8 +817,7 @@ __global__ void category_totals(int Vs, const int4* __restrict__ posPT, const in
     dc[kDcCCD] = a.z + b.w;
     dc[kDcEnv] = pg[Vs] - pg[0];
 }
-__global__ void map_tail(int capM, const int* __restrict__ dc, unsigned long long* __restrict__ mkeys) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+__device__ inline void map_tail(int i, int capM, const int* __restrict__ dc, unsigned long long* __restrict__ mkeys) {
     if (i >= capM || dc[kDcFlags]) return;
     if (i >= dc[kDcMapKeys]) mkeys[i] = ~0ull;
 }
@@ -795,11 +838,45 @@ __global__ void map_decode(const int* __restrict__ dc, const unsigned long long*
     out[base + i] = make_int4((int)((k >> (2 * bits)) & m) - V, (int)((k >> bits) & m), (int)(k & m) - 1, -counts[i]);
 }
 // pos: the joint scan's ground starts, offset by its first entry
-__global__ void ground_scatter(int Vs, const int* __restrict__ sv, const int* __restrict__ flag, const int* __restrict__ pos,
-                               int* __restrict__ out) {
-    const int s = blockIdx.x * blockDim.x + threadIdx.x;
+__device__ inline void ground_scatter(int s, int Vs, const int* __restrict__ sv, const int* __restrict__ flag, const int* __restrict__ pos,
+                                      int* __restrict__ out) {
     if (s >= Vs || !flag[s]) return;
     out[pos[s] - pos[0]] = sv[s];
+}
+// After the category scan, in one launch (the four parts write disjoint entries): work-groups
+// [0, bPT) place the point hits, the next bEE the edge hits, the next bG the ground entries,
+// the rest write the map keys' all-ones tail.
+struct ScatterArgs {
+    const int* dc;
+    int V, bits, Vs, capM, bPT, bEE, bG;
+    const int *catPT, *ownPT, *candPT, *catEE, *ownEE, *candEE, *sv, *fg, *pg;
+    const int4 *posPT, *recPT, *posEE, *recEE;
+    const int2* auxEE;
+    int4 *self_set, *ee_set;
+    int2 *ee_ie, *ccd;
+    unsigned long long* mkeys;
+    int* env;
+};
+__global__ void scatter_all(ScatterArgs a) {
+    int b = blockIdx.x;
+    if (b < a.bPT) {
+        scatter_pt(b * blockDim.x + threadIdx.x, a.dc, a.catPT, a.posPT, a.recPT, a.ownPT, a.candPT, a.V, a.bits, a.self_set,
+                   a.mkeys, a.ccd);
+        return;
+    }
+    b -= a.bPT;
+    if (b < a.bEE) {
+        scatter_ee(b * blockDim.x + threadIdx.x, a.dc, a.catEE, a.posEE, a.recEE, a.auxEE, a.ownEE, a.candEE, a.V, a.bits,
+                   a.self_set, a.mkeys, a.ee_set, a.ee_ie, a.ccd);
+        return;
+    }
+    b -= a.bEE;
+    if (b < a.bG) {
+        ground_scatter(b * blockDim.x + threadIdx.x, a.Vs, a.sv, a.fg, a.pg, a.env);
+        return;
+    }
+    b -= a.bG;
+    map_tail(b * blockDim.x + threadIdx.x, a.capM, a.dc, a.mkeys);
 }
 // the tails of the joint scan's point and ground segments when there is no point walk
 __global__ void point_tails(int Vs, int* __restrict__ cnt, int* __restrict__ fg) {
@@ -975,14 +1052,18 @@ void compute_active_set(Context& c, bool publish) {
     c.self_set.resize((size_t)capPT + capEE + capM + 1); c.ee_set.resize((size_t)capEE + 1); c.ee_ie.resize((size_t)capEE + 1);
     c.ccd_set.resize((size_t)capPT + capEE + 1); c.env_set.resize((size_t)Vs + 1);
     b.mkeys.resize(capM); b.mkeys_sorted.resize(capM); b.ukeys.resize(capM); b.ucounts.resize(capM); b.nruns.resize(1);
-    hipLaunchKernelGGL(scatter_pt, dim3(blocks_for(capPT, 256)), dim3(256), 0, c.stream, dc, b.catPT.p, posPT, b.recPT.p, b.ownPT.p,
-                       b.candPT.p, s.V, bits, c.self_set.p, b.mkeys.p, c.ccd_set.p);
-    hipLaunchKernelGGL(scatter_ee, dim3(blocks_for(capEE, 256)), dim3(256), 0, c.stream, dc, b.catEE.p, posEE, b.recEE.p, b.auxEE.p,
-                       b.ownEE.p, b.candEE.p, s.V, bits, c.self_set.p, b.mkeys.p, c.ee_set.p, c.ee_ie.p, c.ccd_set.p);
-    if (Vs > 0)
-        hipLaunchKernelGGL(ground_scatter, dim3(blocks_for(Vs, 256)), dim3(256), 0, c.stream, Vs, s.sv.p, cG, oG, c.env_set.p);
-    hipLaunchKernelGGL(map_tail, dim3(blocks_for(capM, 256)), dim3(256), 0, c.stream, capM, dc, b.mkeys.p);
-    CIPC_CHECK_LAUNCH();
+    {
+        ScatterArgs a;
+        a.dc = dc; a.V = s.V; a.bits = bits; a.Vs = Vs; a.capM = capM;
+        a.bPT = (int)blocks_for(capPT, 256); a.bEE = (int)blocks_for(capEE, 256); a.bG = (int)blocks_for(Vs, 256);
+        a.catPT = b.catPT.p; a.ownPT = b.ownPT.p; a.candPT = b.candPT.p; a.posPT = posPT; a.recPT = b.recPT.p;
+        a.catEE = b.catEE.p; a.ownEE = b.ownEE.p; a.candEE = b.candEE.p; a.posEE = posEE; a.recEE = b.recEE.p; a.auxEE = b.auxEE.p;
+        a.sv = s.sv.p; a.fg = cG; a.pg = oG; a.self_set = c.self_set.p; a.ee_set = c.ee_set.p; a.ee_ie = c.ee_ie.p;
+        a.ccd = c.ccd_set.p; a.mkeys = b.mkeys.p; a.env = c.env_set.p;
+        const int blocks = a.bPT + a.bEE + a.bG + (int)blocks_for(capM, 256);
+        hipLaunchKernelGGL(scatter_all, dim3(blocks), dim3(256), 0, c.stream, a);
+        CIPC_CHECK_LAUNCH();
+    }
     // PP/PE map: sort packed keys, run-length encode (std::map order with multiplicity); the
     // all-ones keys past nm form the last run
     sort_u64(c, b.mkeys.p, b.mkeys_sorted.p, capM, 3 * bits);

@@ -140,8 +140,7 @@ __global__ void partial_ccd_kernel(const int* __restrict__ n_dev, const int2* __
 __global__ __launch_bounds__(64) void full_ccd_pt_kernel(int Vs, const int* __restrict__ perm, const int* __restrict__ sv,
                                                           const int4* __restrict__ faces, const int* __restrict__ bt,
                                                           const double* __restrict__ x, const double* __restrict__ p,
-                                                          const Box* __restrict__ qbox, TreeView tv,
-                                                          unsigned long long* __restrict__ sl, double eta,
+                                                          TreeView tv, unsigned long long* __restrict__ sl, double eta,
                                                           const int* __restrict__ vs, int pb) {
     __shared__ int stk[kStack * 64];
     const int t = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
@@ -153,8 +152,8 @@ __global__ __launch_bounds__(64) void full_ccd_pt_kernel(int Vs, const int* __re
         scene = scene_of(vs, vI);
         const double a = bits_to_double(sl[kSlots * scene + 5]);
         const d3 X = ld3(x, vI), P = ld3(p, vI);
-        const Box vb = swept_pt(X, P, a);
-        part_walk(tv, part, qbox[s], 0.0, stk + threadIdx.x, [&](int f) {
+        const Box vb = swept_pt(X, P, a);   // the query box: the point swept with the trees' step
+        part_walk(tv, part, vb, 0.0, stk + threadIdx.x, [&](int f) {
             const int4 fv = faces[f];
             if (is_collider(bt[vI]) && is_collider(bt[fv.x]) && is_collider(bt[fv.y]) && is_collider(bt[fv.z])) return;
             if (vI == fv.x || vI == fv.y || vI == fv.z) return;
@@ -286,7 +285,7 @@ void launch_full_ccd_pt(Context& c, hipStream_t st, int pb) {
     const LBVH& ft = c.sface_tree;
     const TreeView ftv{ft.n, ft.nodes.p, ft.wide.p};
     hipLaunchKernelGGL(full_ccd_pt_kernel, dim3(blocks_for((size_t)s.Vs << pb, 64)), dim3(64), 0, st, s.Vs, s.sv_perm.p, s.sv.p,
-                       s.faces.p, s.bt.p, s.x.p, s.dir.p, c.swept_point_box.p, ftv, c.u64scratch.p, 1.0 - 0.8, scene_of_vertex(s), pb);
+                       s.faces.p, s.bt.p, s.x.p, s.dir.p, ftv, c.u64scratch.p, 1.0 - 0.8, scene_of_vertex(s), pb);
 }
 void launch_full_ccd_ee(Context& c, hipStream_t st, int pb) {
     Scene& s = c.sc;

@@ -3,6 +3,7 @@
 #include "context.hpp"
 
 #include <algorithm>
+#include <chrono>
 #include <cmath>
 #include <cstring>
 #include <string>
@@ -907,11 +908,11 @@ struct IterEvents {
 // same launches can be captured into a graph.  t (eager only): per-stage timing events.
 void enqueue_iteration(cipc_gpu_context* c, double kappa, bool energy, const IterEvents* t) {
     // The host enqueues the critical path first (the active set on the main stream and its
-    // edge half on `as2`), then the streams with slack: the elastic stage and the lagged
-    // friction records on `aux` (they read the state and the lagged friction only), the
-    // ground bound, swept trees and full CCD on `sbs`; then the barrier records and the CSR
-    // on the main stream, the partial CCD on `sbs` and the energy on `aux` once the active
-    // set exists.
+    // edge half on `as2`), the elastic stage and the lagged friction records on `aux` (they
+    // read the state and the lagged friction only), the barrier records on the main stream;
+    // then what has slack: the ground bound, swept trees, full and partial CCD on `sbs`, the
+    // energy on `aux`; then the CSR on the main stream.  Host enqueue time is close to half
+    // the iteration's GPU time, so its order decides how long the main stream waits.
     if (t) CIPC_HIP(hipEventRecord(t->ev[0], c->stream));
     CIPC_HIP(hipEventRecord(c->it_start, c->stream));
     contact_records_prepare(*c);
@@ -924,6 +925,8 @@ void enqueue_iteration(cipc_gpu_context* c, double kappa, bool energy, const Ite
     if (t) CIPC_HIP(hipEventRecord(t->ee, c->aux));
     friction_records_launch(*c, c->aux);
     CIPC_HIP(hipEventRecord(c->join, c->aux));
+    contact_assemble(*c, kappa, true);
+    // the step bound after the barrier records: it has slack, the main stream has none
     CIPC_HIP(hipStreamWaitEvent(c->sbs, c->it_start, 0));
     if (t) CIPC_HIP(hipEventRecord(t->s0, c->sbs));
     step_bound_early(*c, c->sbs);
@@ -931,7 +934,6 @@ void enqueue_iteration(cipc_gpu_context* c, double kappa, bool energy, const Ite
     step_bound_partial(*c, c->sbs);
     if (t) CIPC_HIP(hipEventRecord(t->s1, c->sbs));
     CIPC_HIP(hipEventRecord(c->sb_done, c->sbs));
-    contact_assemble(*c, kappa, true);
     if (energy) {
         if (t) CIPC_HIP(hipEventRecord(t->e0, c->aux));
         energy_terms_launch(*c, c->aux);
@@ -1062,9 +1064,12 @@ int cipc_gpu_batch_newton_iteration(cipc_gpu_context* c, const double* kappa, ci
         CIPC_HIP(hipEventCreate(&e_en0));
         CIPC_HIP(hipEventCreate(&e_en1));
         bool graphed = false;
+        double enq_ms = 0;
         for (int round = 0;; ++round) {
+            const auto h0 = std::chrono::steady_clock::now();
             graphed = round == 0 && iteration_graph(c, kappa[0], false, E.ev[0], E.ev[3]);
             if (!graphed) enqueue_iteration(c, kappa[0], false, &E);
+            if (round == 0) enq_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - h0).count();
             CIPC_HIP(hipEventSynchronize(E.ev[3]));
             if (!grow_capacities(*c)) break;
             if (round >= 3) throw Error(CIPC_ERR_RUNTIME, "active-set capacities kept overflowing");
@@ -1094,6 +1099,7 @@ int cipc_gpu_batch_newton_iteration(cipc_gpu_context* c, const double* kappa, ci
         t->energy_ms = en;
         t->total_ms = tot + en;
         t->graph = graphed ? 1 : 0;
+        t->enqueue_ms = enq_ms;
         double amin = 1.0, esum = 0.0, bmin[4] = {1.0, 1.0, 1.0, 1.0};
         for (int k = 0; k < S; ++k) {
             const double a = sb[5 * (size_t)k], e = total_energy(*c, kappa[k], et.data() + 9 * (size_t)k);
@@ -1220,9 +1226,12 @@ int cipc_gpu_newton_iteration(cipc_gpu_context* c, double kappa, cipc_iteration_
         // go as one graph when they can (stage times are then not taken: zero).
         IterEvents E;
         bool graphed = false;
+        double enq_ms = 0;
         for (int round = 0;; ++round) {
+            const auto h0 = std::chrono::steady_clock::now();
             graphed = round == 0 && iteration_graph(c, kappa, true, E.ev[0], E.ev[3]);
             if (!graphed) enqueue_iteration(c, kappa, true, &E);
+            if (round == 0) enq_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - h0).count();
             CIPC_HIP(hipEventSynchronize(E.ev[3]));
             if (!grow_capacities(*c)) break;
             if (round >= 3) throw Error(CIPC_ERR_RUNTIME, "active-set capacities kept overflowing");
@@ -1246,6 +1255,7 @@ int cipc_gpu_newton_iteration(cipc_gpu_context* c, double kappa, cipc_iteration_
         }
         t->total_ms = tot;
         t->graph = graphed ? 1 : 0;
+        t->enqueue_ms = enq_ms;
         t->alpha = sb[0];
         t->energy = total_energy(*c, kappa, et);
         t->nnz = c->nnz;

@@ -40,6 +40,7 @@ struct LBVH {
     DBuf<int> leafpos;                    // leaf position of each primitive
     DBuf<unsigned long long> keys_sorted; // (code << 32 | primitive), sorted and unique
     DBuf<Box> scene_partial;
+    DBuf<int2> wslot;                     // 2n-1: the four-wide record slots of each node's box, (record << 2 | k) or -1
     bool swept = false;
 };
 constexpr int kMaxRefits = 32;
@@ -239,7 +240,6 @@ struct Context {
     LBVH sface_tree, sedge_tree;   // swept boxes: the full CCD (their own, so that the step
                                    // bound can run beside the active set)
     ActiveSetScratch as;
-    DBuf<Box> swept_point_box;
 
     // generic scratch
     DBuf<unsigned char> tmp;

@@ -31,9 +31,9 @@ __device__ inline void block_sum_store(double v, double* part, F) {
     if (part) energy_block_sum(v, part);
 }
 
-__global__ __launch_bounds__(kEB) void e_snh(int T, const int4* __restrict__ tets, const double* __restrict__ x,
-                                             const double* __restrict__ dminv, const double* __restrict__ vol, Lame lame,
-                                             double* __restrict__ part, double* __restrict__ elem) {
+__device__ inline void snh_sum(int T, const int4* __restrict__ tets, const double* __restrict__ x,
+                               const double* __restrict__ dminv, const double* __restrict__ vol, Lame lame,
+                               double* __restrict__ part, double* __restrict__ elem) {
     double acc = 0.0;
     for (int t = blockIdx.x * kEB + threadIdx.x; t < T; t += gridDim.x * kEB) {
         double u, r;
@@ -58,9 +58,9 @@ __global__ __launch_bounds__(kEB) void e_snh(int T, const int4* __restrict__ tet
     }
     block_sum_store(acc, part, 0);
 }
-__global__ __launch_bounds__(kEB) void e_inertia(int V, const double* __restrict__ x, const double* __restrict__ xt,
-                                                 const double* __restrict__ vp, const double* __restrict__ m, double drag,
-                                                 double* __restrict__ part, double* __restrict__ elem) {
+__device__ inline void inertia_sum(int V, const double* __restrict__ x, const double* __restrict__ xt,
+                                   const double* __restrict__ vp, const double* __restrict__ m, double drag,
+                                   double* __restrict__ part, double* __restrict__ elem) {
     double acc = 0.0;
     for (int v = blockIdx.x * kEB + threadIdx.x; v < V; v += gridDim.x * kEB) {
         const d3 xv = ld3(x, v);
@@ -75,8 +75,8 @@ __global__ __launch_bounds__(kEB) void e_inertia(int V, const double* __restrict
 }
 // The contact terms take their count from device memory (ndev, the active set's device
 // count) when it is given, else n.
-__global__ __launch_bounds__(kEB) void e_ground(int n, const int* __restrict__ ndev, const int* __restrict__ env, const double* __restrict__ x,
-                                                d3 gn, double gd, double hhat, double* __restrict__ part, double* __restrict__ elem) {
+__device__ inline void ground_sum(int n, const int* __restrict__ ndev, const int* __restrict__ env, const double* __restrict__ x,
+                                  d3 gn, double gd, double hhat, double* __restrict__ part, double* __restrict__ elem) {
     double acc = 0.0;
     if (ndev) n = *ndev;
     for (int i = blockIdx.x * kEB + threadIdx.x; i < n; i += gridDim.x * kEB) {
@@ -94,8 +94,8 @@ __device__ inline double self_d(int4 c, const double* x) {
     if (c.w < 0) return d_PE(p, ld3(x, c.y), ld3(x, c.z));
     return d_PT(p, ld3(x, c.y), ld3(x, c.z), ld3(x, c.w));
 }
-__global__ __launch_bounds__(kEB) void e_self(int n, const int* __restrict__ ndev, const int4* __restrict__ set, const double* __restrict__ x,
-                                              double hhat, double* __restrict__ part, double* __restrict__ elem) {
+__device__ inline void self_sum(int n, const int* __restrict__ ndev, const int4* __restrict__ set, const double* __restrict__ x,
+                                double hhat, double* __restrict__ part, double* __restrict__ elem) {
     double acc = 0.0;
     if (ndev) n = *ndev;
     for (int i = blockIdx.x * kEB + threadIdx.x; i < n; i += gridDim.x * kEB) {
@@ -107,10 +107,10 @@ __global__ __launch_bounds__(kEB) void e_self(int n, const int* __restrict__ nde
     }
     block_sum_store(acc, part, 0);
 }
-__global__ __launch_bounds__(kEB) void e_mollified(int n, const int* __restrict__ ndev, const int4* __restrict__ set,
-                                                   const int2* __restrict__ ie, const int2* __restrict__ se, const double* __restrict__ x,
-                                                   const double* __restrict__ vrest, double hhat, double* __restrict__ part,
-                                                   double* __restrict__ elem) {
+__device__ inline void mollified_sum(int n, const int* __restrict__ ndev, const int4* __restrict__ set,
+                                     const int2* __restrict__ ie, const int2* __restrict__ se, const double* __restrict__ x,
+                                     const double* __restrict__ vrest, double hhat, double* __restrict__ part,
+                                     double* __restrict__ elem) {
     double acc = 0.0;
     if (ndev) n = *ndev;
     for (int i = blockIdx.x * kEB + threadIdx.x; i < n; i += gridDim.x * kEB) {
@@ -129,15 +129,75 @@ __global__ __launch_bounds__(kEB) void e_mollified(int n, const int* __restrict_
     }
     block_sum_store(acc, part, 0);
 }
+// Per-element values (elem, one grid-strided thread per element) or one partial per
+// work-group (part); the batch path takes the first, the single-scene sums the second.
+__global__ __launch_bounds__(kEB) void e_snh(int T, const int4* __restrict__ tets, const double* __restrict__ x,
+                                             const double* __restrict__ dminv, const double* __restrict__ vol, Lame lame,
+                                             double* __restrict__ part, double* __restrict__ elem) {
+    snh_sum(T, tets, x, dminv, vol, lame, part, elem);
+}
+__global__ __launch_bounds__(kEB) void e_inertia(int V, const double* __restrict__ x, const double* __restrict__ xt,
+                                                 const double* __restrict__ vp, const double* __restrict__ m, double drag,
+                                                 double* __restrict__ part, double* __restrict__ elem) {
+    inertia_sum(V, x, xt, vp, m, drag, part, elem);
+}
+__global__ __launch_bounds__(kEB) void e_ground(int n, const int* __restrict__ ndev, const int* __restrict__ env, const double* __restrict__ x,
+                                                d3 gn, double gd, double hhat, double* __restrict__ part, double* __restrict__ elem) {
+    ground_sum(n, ndev, env, x, gn, gd, hhat, part, elem);
+}
+__global__ __launch_bounds__(kEB) void e_self(int n, const int* __restrict__ ndev, const int4* __restrict__ set, const double* __restrict__ x,
+                                              double hhat, double* __restrict__ part, double* __restrict__ elem) {
+    self_sum(n, ndev, set, x, hhat, part, elem);
+}
+__global__ __launch_bounds__(kEB) void e_mollified(int n, const int* __restrict__ ndev, const int4* __restrict__ set,
+                                                   const int2* __restrict__ ie, const int2* __restrict__ se, const double* __restrict__ x,
+                                                   const double* __restrict__ vrest, double hhat, double* __restrict__ part,
+                                                   double* __restrict__ elem) {
+    mollified_sum(n, ndev, set, ie, se, x, vrest, hhat, part, elem);
+}
+// The five state and contact terms of one scene in one launch: work-group row y sums term y
+// exactly as the term's own kernel on a kEG-wide grid would (the same grid stride, the same
+// work-group index), so the partials are the same; the contact counts are the active set's
+// device counts.
+struct EnergyTermArgs {
+    int T, V;
+    const int4* tets;
+    const double *x, *dminv, *vol, *xt, *vp, *m, *vrest;
+    Lame lame;
+    double drag, gd, hhat;
+    d3 gn;
+    const int* dc;
+    const int* env;
+    const int4 *self_set, *ee_set;
+    const int2 *ee_ie, *se;
+};
+constexpr int kFusedTerms = 5;
+__global__ __launch_bounds__(kEB) void e_terms(EnergyTermArgs a, double* __restrict__ P) {
+    double* part = P + (size_t)blockIdx.y * kEG;
+    switch (blockIdx.y) {
+    case 0: snh_sum(a.T, a.tets, a.x, a.dminv, a.vol, a.lame, part, nullptr); break;
+    case 1: inertia_sum(a.V, a.x, a.xt, a.vp, a.m, a.drag, part, nullptr); break;
+    case 2: ground_sum(0, a.dc + kDcEnv, a.env, a.x, a.gn, a.gd, a.hhat, part, nullptr); break;
+    case 3: self_sum(0, a.dc + kDcSelf, a.self_set, a.x, a.hhat, part, nullptr); break;
+    default: mollified_sum(0, a.dc + kDcMol, a.ee_set, a.ee_ie, a.se, a.x, a.vrest, a.hhat, part, nullptr); break;
+    }
+}
 __global__ __launch_bounds__(kEB) void e_soft(int n, const int* __restrict__ idx, const double* __restrict__ tgt, double w,
                                               const double* __restrict__ x, double* __restrict__ part) {
     double acc = 0.0;
     for (int i = blockIdx.x * kEB + threadIdx.x; i < n; i += gridDim.x * kEB) acc += 0.5 * w * sqn(ld3(x, idx[i]) - ld3(tgt, i));
     block_sum_store(acc, part, 0);
 }
-__global__ __launch_bounds__(kEB) void sum_partials(const double* __restrict__ part, int nterms, double* __restrict__ out) {
+// out[t] = the partials of term t summed in a fixed tree; a term whose bit is clear in
+// `present` was not launched (no elements): its sum is zero.
+__global__ __launch_bounds__(kEB) void sum_partials(const double* __restrict__ part, int nterms, unsigned present,
+                                                   double* __restrict__ out) {
     __shared__ double sh[kEB];
     for (int t = 0; t < nterms; ++t) {
+        if (!((present >> t) & 1u)) {
+            if (threadIdx.x == 0) out[t] = 0.0;
+            continue;
+        }
         sh[threadIdx.x] = part[t * kEG + threadIdx.x];
         __syncthreads();
         for (int s = kEB / 2; s > 0; s >>= 1) {
@@ -166,28 +226,33 @@ void energy_terms_launch(Context& c, hipStream_t st) {
 }
 
 // The partial sums of the ten energy terms (kEG per term) and, after them, the ten sums,
-// into P (kEnergyScratch doubles); every entry of P is written before it is read.
+// into P (kEnergyScratch doubles).  Terms with no elements in the scene (cloth, soft
+// targets) are not launched and their partials are neither written nor read; every other
+// entry of P is written before it is read.
 void energy_partials_launch(Context& c, hipStream_t st, double* P) {
     Scene& s = c.sc;
     static_assert(kEG == kEB, "partials are summed by one work-group");
-    hipLaunchKernelGGL(e_snh, dim3(kEG), dim3(kEB), 0, st, s.T, s.tets.p, s.x.p, s.dminv.p, s.vol.p, scene_lame(s), P,
-                       nullptr);
-    hipLaunchKernelGGL(e_inertia, dim3(kEG), dim3(kEB), 0, st, s.V, s.x.p, s.xt.p, s.vprev.p, s.mass.p, s.drag, P + kEG,
-                       nullptr);
-    // the contact counts are the active set's device counts
-    const int* dc = c.dcnt.p;
-    hipLaunchKernelGGL(e_ground, dim3(kEG), dim3(kEB), 0, st, 0, dc + kDcEnv, c.env_set.p, s.x.p, s.gn, s.gd, s.hhat, P + 2 * kEG,
-                       nullptr);
-    hipLaunchKernelGGL(e_self, dim3(kEG), dim3(kEB), 0, st, 0, dc + kDcSelf, c.self_set.p, s.x.p, s.hhat, P + 3 * kEG, nullptr);
-    hipLaunchKernelGGL(e_mollified, dim3(kEG), dim3(kEB), 0, st, 0, dc + kDcMol, c.ee_set.p, c.ee_ie.p, s.se.p, s.x.p, s.vrest.p,
-                       s.hhat, P + 4 * kEG, nullptr);
+    EnergyTermArgs a;
+    a.T = s.T; a.V = s.V; a.tets = s.tets.p; a.x = s.x.p; a.dminv = s.dminv.p; a.vol = s.vol.p; a.xt = s.xt.p;
+    a.vp = s.vprev.p; a.m = s.mass.p; a.vrest = s.vrest.p; a.lame = scene_lame(s); a.drag = s.drag; a.gd = s.gd;
+    a.hhat = s.hhat; a.gn = s.gn; a.dc = c.dcnt.p; a.env = c.env_set.p; a.self_set = c.self_set.p; a.ee_set = c.ee_set.p;
+    a.ee_ie = c.ee_ie.p; a.se = s.se.p;
+    hipLaunchKernelGGL(e_terms, dim3(kEG, kFusedTerms), dim3(kEB), 0, st, a, P);
+    unsigned present = (1u << kFusedTerms) - 1;
     // lagged friction: self part (lambda times clamped norm) and ground part (mu included)
     friction_energy_partials(c, P + 5 * kEG, nullptr, nullptr, st);
-    // cloth triangles and bending (IPCSolver.cpp:330-368)
-    cloth_energy_partials(c, P + 7 * kEG, P + 8 * kEG, st);
-    // soft targets (softEnergy, BoundaryConditions.cpp:210-229)
-    hipLaunchKernelGGL(e_soft, dim3(kEG), dim3(kEB), 0, st, c.n_soft, c.soft_idx.p, c.soft_t.p, c.soft_w, s.x.p, P + 9 * kEG);
-    hipLaunchKernelGGL(sum_partials, dim3(1), dim3(kEB), 0, st, P, 10, P + 10 * kEG);
+    present |= 3u << 5;
+    // cloth triangles and bending (IPCSolver.cpp:330-368), when the scene has them
+    if (s.Tr > 0 || s.Nb > 0) {
+        cloth_energy_partials(c, P + 7 * kEG, P + 8 * kEG, st);
+        present |= 3u << 7;
+    }
+    // soft targets (softEnergy, BoundaryConditions.cpp:210-229), when set
+    if (c.n_soft > 0) {
+        hipLaunchKernelGGL(e_soft, dim3(kEG), dim3(kEB), 0, st, c.n_soft, c.soft_idx.p, c.soft_t.p, c.soft_w, s.x.p, P + 9 * kEG);
+        present |= 1u << 9;
+    }
+    hipLaunchKernelGGL(sum_partials, dim3(1), dim3(kEB), 0, st, P, 10, present, P + 10 * kEG);
     CIPC_CHECK_LAUNCH();
 }
 

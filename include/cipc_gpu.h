@@ -102,6 +102,10 @@ typedef struct {
     /* 1 when the stages ran as one HIP graph launch (then only total_ms and, for a batch,
      * energy_ms are timed; the other stage times are zero) */
     int graph;
+    /* host time of the first round's enqueue (the launches one by one, or the graph launch
+     * with its capture when one is made): the iteration waits on the host when this is
+     * close to total_ms */
+    double enqueue_ms;
 } cipc_iteration_timing;
 
 /* Result of one Newton linear solve. */
