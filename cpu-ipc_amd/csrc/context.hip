@@ -939,12 +939,14 @@ void enqueue_iteration(cipc_gpu_context* c, double kappa, bool energy, const Ite
         energy_terms_launch(*c, c->aux);
         if (t) CIPC_HIP(hipEventRecord(t->en, c->aux));
     }
+    collision_count_launch(*c, c->aux);   // it reads the active set only
     CIPC_HIP(hipEventRecord(c->en_done, c->aux));
     elastic_join(c);
-    finalize_csr(*c);
+    finalize_csr(*c, false);
     if (t) CIPC_HIP(hipEventRecord(t->ev[2], c->stream));
     CIPC_HIP(hipStreamWaitEvent(c->stream, c->sb_done, 0));
     CIPC_HIP(hipStreamWaitEvent(c->stream, c->en_done, 0));
+    publish_nnz(*c, c->stream);
     if (t) CIPC_HIP(hipEventRecord(t->ev[3], c->stream));
 }
 
@@ -1028,7 +1030,10 @@ bool iteration_graph(cipc_gpu_context* c, double kappa, bool energy, hipEvent_t 
             (void)hipGetLastError();
         }
         if (g) (void)hipGraphDestroy(g);
-        if (!c->graph.exec) return false;   // this call runs eagerly
+        if (!c->graph.exec) {
+            c->bucket_cnt_clear = 0;   // the captured launches did not run
+            return false;              // this call runs eagerly
+        }
     } else {
         // the host side of a replayed iteration: what enqueue_iteration records on the host
         for (LBVH* t : {&c->face_tree, &c->edge_tree, &c->sface_tree, &c->sedge_tree})

@@ -206,6 +206,8 @@ struct Context {
     DBuf<int> gkey, gkey_sorted, gidx, gidx_sorted;
     // vertex buckets that order the records (sparse.hip): counts and cursors, starts, entries
     DBuf<int> bucket_cnt, bucket_start;
+    size_t bucket_cnt_clear = 0;        // leading entries of bucket_cnt known to be zero (left so by the last assembly)
+    const int* bucket_cnt_ptr = nullptr;  // the buffer they are known zero in
     DBuf<unsigned long long> bucket_entries;
     DBuf<double> gval;         // 3 per record
     DBuf<int> run_start;       // start of each run of equal sorted contact block keys
@@ -277,7 +279,10 @@ void contact_records_prepare(Context& c);
 void friction_records_launch(Context& c, hipStream_t st);
 void contact_assemble(Context& c, double kappa, bool friction_launched = false);
 // sparse.hip
-void finalize_csr(Context& c);
+// count_collisions false: the caller launches collision_count_launch and publish_nnz itself
+void finalize_csr(Context& c, bool count_collisions = true);
+void collision_count_launch(Context& c, hipStream_t st);
+void publish_nnz(Context& c, hipStream_t st);
 void exclusive_scan_int(Context& c, const int* in, int* out, int n);
 // broadphase.hip
 void build_tree(Context& c, LBVH& t, int n);

@@ -60,13 +60,7 @@ void exclusive_scan_int(Context& c, const int* in, int* out, int n) {
     CIPC_HIP(hipcub::DeviceScan::ExclusiveSum(c.tmp.p, bytes, in, out, n, c.stream));
 }
 
-// --- contact block records: heads of equal-key runs, their sums and nonzero masks -----
-__global__ void run_heads_u64(int n, const unsigned long long* __restrict__ k, int* __restrict__ head) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i > n) return;
-    if (i == n) { head[n] = 0; return; }
-    head[i] = (k[i] != kNoKey && (i == 0 || k[i] != k[i - 1])) ? 1 : 0;
-}
+// --- contact block records: equal-key runs (heads from bucket_tails), their sums and nonzero masks -----
 // End of the run of `key` that starts at i in the sorted keys (galloping, then bisection), so
 // that the summation loop has known bounds and its loads are independent.
 template <typename K>
@@ -221,11 +215,17 @@ __global__ void pair_count(const int* __restrict__ nm_ptr, const int* __restrict
     for (int r = r0; r < 3; ++r) n += (f >> (3 * r + cc)) & 1u;
     cnt[pair_index(mptr, b, cc, k)] = n;
 }
+// outer[j]: the column starts; the last one, the nonzero count, also into nnz_out
 __global__ void column_starts(int V, const int* __restrict__ mptr, const int* __restrict__ ppos, const int* __restrict__ nm_ptr,
-                              int* __restrict__ outer) {
+                              int* __restrict__ outer, int* __restrict__ nnz_out) {
     const int j = blockIdx.x * blockDim.x + threadIdx.x;
     if (j > 3 * V) return;
-    if (j == 3 * V) { outer[j] = ppos[3 * *nm_ptr]; return; }
+    if (j == 3 * V) {
+        const int nnz = ppos[3 * *nm_ptr];
+        outer[j] = nnz;
+        *nnz_out = nnz;
+        return;
+    }
     const int b = j / 3, cc = j % 3;
     outer[j] = ppos[pair_index(mptr, b, cc, mptr[b])];
 }
@@ -256,8 +256,9 @@ __global__ void masked_rhs(int V, const int* __restrict__ bt, const double* __re
 // multiplicity of a point-point / point-edge entry, one for every other Self_ActiveSet
 // entry.  One work-group, integer sums: the result does not depend on the order.
 // out[0] = the CSR's nonzeros (nnz_dev), out[1] = the collision count: read by the host together.
-__global__ void collision_count(const int* __restrict__ n_dev, const int4* __restrict__ set, const int* __restrict__ nnz_dev,
-                                int* __restrict__ out) {
+// *out: compute_g_dpt's count over Self_ActiveSet (one per entry, the multiplicity for a
+// point-point or point-edge map entry)
+__global__ void collision_count(const int* __restrict__ n_dev, const int4* __restrict__ set, int* __restrict__ out) {
     __shared__ long long sh[256];
     long long acc = 0;
     const int n = *n_dev;
@@ -271,7 +272,7 @@ __global__ void collision_count(const int* __restrict__ n_dev, const int4* __res
         if ((int)threadIdx.x < w) sh[threadIdx.x] += sh[threadIdx.x + w];
         __syncthreads();
     }
-    if (threadIdx.x == 0) { out[0] = *nnz_dev; out[1] = (int)sh[0]; }
+    if (threadIdx.x == 0) *out = (int)sh[0];
 }
 
 __global__ void zero_ints(int* __restrict__ p, int n) {
@@ -403,15 +404,43 @@ __global__ __launch_bounds__(64 * kBucketWaves) void bucket_sort(int nbk, const 
     }
 }
 // Records without a key, after the keyed ones: block keys kNoKey, gradient keys past every vertex.
+// After the bucket sort, in one launch: the unkeyed records after the keyed ones (as a stable
+// sort leaves them), the run heads of the sorted block keys (head[nb] = 0: the scan's total
+// is the run count), and the bucket counters cleared for the next assembly (nothing after
+// the bucket scatter reads them).  Keys past the keyed ones are the sentinel whether or not
+// this launch has written them yet.
 __global__ void bucket_tails(int nb, int ng, const int* __restrict__ starts, int V, unsigned long long* __restrict__ bkey,
-                             int* __restrict__ bidx, int* __restrict__ gkey, int* __restrict__ gidx) {
+                             int* __restrict__ bidx, int* __restrict__ gkey, int* __restrict__ gidx, int* __restrict__ head,
+                             int* __restrict__ counters, int ncounters) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     const int tb = starts[V], tg = starts[2 * V + 1] - starts[V + 1];
-    if (i >= tb && i < nb) { bkey[i] = kNoKey; bidx[i] = i; }
+    if (i < nb) {
+        const unsigned long long k = i < tb ? bkey[i] : kNoKey;
+        if (i >= tb) { bkey[i] = kNoKey; bidx[i] = i; }
+        const unsigned long long kp = i == 0 ? 0ull : (i - 1 < tb ? bkey[i - 1] : kNoKey);
+        head[i] = (k != kNoKey && (i == 0 || k != kp)) ? 1 : 0;
+    } else if (i == nb) {
+        head[nb] = 0;
+    }
     if (i >= tg && i < ng) { gkey[i] = 0x7fffffff; gidx[i] = i; }
+    if (i < ncounters) counters[i] = 0;
 }
 
-void finalize_csr(Context& c) {
+// The nonzero and collision counts (c.coll) to pinned memory, for the host's next
+// synchronisation.
+void publish_nnz(Context& c, hipStream_t st) {
+    c.pin_nnz.reserve(2);
+    CIPC_HIP(hipMemcpyAsync(c.pin_nnz.p, c.coll.p, 2 * sizeof(int), hipMemcpyDeviceToHost, st));
+    c.nnz_pending = true;
+}
+// The collision count of the assembly's set (c.coll[1]), on stream st.
+void collision_count_launch(Context& c, hipStream_t st) {
+    c.coll.resize(2);
+    hipLaunchKernelGGL(collision_count, dim3(1), dim3(256), 0, st, c.dcnt.p + kDcSelf, c.self_set.p, c.coll.p + 1);
+    CIPC_CHECK_LAUNCH();
+}
+
+void finalize_csr(Context& c, bool count_collisions) {
     Scene& s = c.sc;
     const int V = s.V;
     // contact block records
@@ -438,7 +467,13 @@ void finalize_csr(Context& c) {
         int* gstart = bstart + (V + 1);
         c.bucket_entries.resize((size_t)nb + ng + 1);
         c.gkey_sorted.resize(std::max(ng, 1)); c.gidx_sorted.resize(std::max(ng, 1));
-        hipLaunchKernelGGL(zero_ints, dim3(blocks_for(4 * ((size_t)V + 1), 256)), dim3(256), 0, c.stream, bcnt, 4 * (V + 1));
+        // the counters are left cleared by the previous assembly's bucket_tails; cleared here
+        // when that is not known (a new buffer, a larger scene, an enqueue cut short)
+        const size_t ncnt = 4 * ((size_t)V + 1);
+        if (c.bucket_cnt_clear < ncnt || c.bucket_cnt_ptr != bcnt)
+            hipLaunchKernelGGL(zero_ints, dim3(blocks_for(ncnt, 256)), dim3(256), 0, c.stream, bcnt, (int)ncnt);
+        c.bucket_cnt_clear = 0;
+        c.bucket_cnt_ptr = bcnt;
         const int nmax = std::max(nb, ng);
         if (nmax > 0)
             hipLaunchKernelGGL(bucket_count, dim3(blocks_for(nmax, 256)), dim3(256), 0, c.stream, nb, c.ckey.p, s.kb, ng, c.gkey.p, V,
@@ -451,14 +486,14 @@ void finalize_csr(Context& c) {
             const BucketOut bo{V, s.kb, 0, c.ckey_sorted.p, c.cidx_sorted.p, c.gkey_sorted.p, c.gidx_sorted.p};
             hipLaunchKernelGGL(bucket_sort, dim3(blocks_for(2 * V + 1, kBucketWaves)), dim3(64 * kBucketWaves), 0, c.stream, 2 * V + 1,
                                bstart, c.bucket_entries.p, bo);
-            hipLaunchKernelGGL(bucket_tails, dim3(blocks_for(nmax, 256)), dim3(256), 0, c.stream, nb, ng, bstart, V, c.ckey_sorted.p,
-                               c.cidx_sorted.p, c.gkey_sorted.p, c.gidx_sorted.p);
+            const size_t nt = std::max<size_t>(std::max<size_t>((size_t)nb + 1, (size_t)ng), ncnt);
+            hipLaunchKernelGGL(bucket_tails, dim3(blocks_for(nt, 256)), dim3(256), 0, c.stream, nb, ng, bstart, V, c.ckey_sorted.p,
+                               c.cidx_sorted.p, c.gkey_sorted.p, c.gidx_sorted.p, head, bcnt, (int)ncnt);
         }
         CIPC_CHECK_LAUNCH();
+        c.bucket_cnt_clear = ncnt;   // by bucket_tails, or never touched (no records)
     }
     if (nb > 0) {
-        hipLaunchKernelGGL(run_heads_u64, dim3(blocks_for(nb + 1, 256)), dim3(256), 0, c.stream, nb, c.ckey_sorted.p, head);
-        CIPC_CHECK_LAUNCH();
         exclusive_scan_int(c, head, pos, nb + 1);
         c.run_start.resize((size_t)nb + 2);
         hipLaunchKernelGGL(run_starts_u64, dim3(blocks_for(nb + 1, 256)), dim3(256), 0, c.stream, nb, c.ckey_sorted.p, head, pos,
@@ -499,16 +534,17 @@ void finalize_csr(Context& c) {
     // the scan covers the bound; entries past 3 nm + 1 are never read
     exclusive_scan_int(c, c.pcnt.p, c.ppos.p, (int)(3 * mcap + 1));
     c.outer.resize((size_t)3 * V + 1);
-    hipLaunchKernelGGL(column_starts, dim3(blocks_for(3 * V + 1, 256)), dim3(256), 0, c.stream, V, c.mptr.p, c.ppos.p, nm_ptr, c.outer.p);
-    CIPC_CHECK_LAUNCH();
-    // the count is read with the caller's next synchronisation; the entries are sized by
-    // their bound (9 per merged block)
-    c.pin_nnz.reserve(2);
     c.coll.resize(2);
-    hipLaunchKernelGGL(collision_count, dim3(1), dim3(256), 0, c.stream, c.dcnt.p + kDcSelf, c.self_set.p, c.outer.p + 3 * V, c.coll.p);
+    hipLaunchKernelGGL(column_starts, dim3(blocks_for(3 * V + 1, 256)), dim3(256), 0, c.stream, V, c.mptr.p, c.ppos.p, nm_ptr, c.outer.p,
+                       c.coll.p);
     CIPC_CHECK_LAUNCH();
-    CIPC_HIP(hipMemcpyAsync(c.pin_nnz.p, c.coll.p, 2 * sizeof(int), hipMemcpyDeviceToHost, c.stream));
-    c.nnz_pending = true;
+    // the counts are read with the caller's next synchronisation (the collision count is
+    // launched here, or by the caller on a stream of its own before it publishes them); the
+    // entries are sized by their bound (9 per merged block)
+    if (count_collisions) {
+        collision_count_launch(c, c.stream);
+        publish_nnz(c, c.stream);
+    }
     c.inner.resize(9 * mcap); c.val.resize(9 * mcap);
     hipLaunchKernelGGL(pair_fill, dim3(blocks_for(3 * mcap, 256)), dim3(256), 0, c.stream, nm_ptr, c.mptr.p, c.mrow.p, c.mcol.p,
                        c.mval.p, c.mflag.p, c.ppos.p, c.inner.p, c.val.p);
