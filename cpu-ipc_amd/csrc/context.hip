@@ -267,6 +267,17 @@ void upload_scene_impl(cipc_gpu_context* c, const cipc_scene_desc* d, bool plan 
         upload_counts(*c);
         c->as.capPT = c->as.capEE = c->as.capM = 0;   // capacities of an earlier scene
         c->capS = 0;
+        // Lanes per query of the four walks until the autotuner chooses: about 2^17 lanes
+        // in flight, floor(log2(2^17 / queries)) bits, 1 to 16 lanes (what the autotuner
+        // picks for the full CCD on the four-cube scene and for every walk of a 256-scene
+        // batch).
+        auto bits_for = [](int queries) {
+            int b = 0;
+            while (b < 4 && ((long long)queries << (b + 1)) <= (1ll << 17)) ++b;
+            return b;
+        };
+        c->tune.ccd_pt_bits = c->tune.as_pt_bits = bits_for(d->Vs);
+        c->tune.ccd_ee_bits = c->tune.as_ee_bits = bits_for(d->Es);
         if (c->graph.exec) (void)hipGraphExecDestroy(c->graph.exec);   // launches of an earlier scene
         c->graph.exec = nullptr;
         c->graph.key.clear();
