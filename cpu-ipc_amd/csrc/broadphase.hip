@@ -146,9 +146,9 @@ __global__ void skip_kernel(int n, const int* __restrict__ range_last, const int
 }
 // Internal boxes.  Node i of the Karras tree covers the sorted leaves [first, last], so its
 // box is the union of that range.  Power-of-two reduction levels over the sorted leaf boxes
-// (level l node j = leaves [j 2^l, (j+1) 2^l)), levels 1..8 built in LDS by the launch that
-// writes the leaves; each internal box is then the union of a few level nodes.  No
-// inter-work-group communication is needed.
+// (level l node j = leaves [j 2^l, (j+1) 2^l)) are built 8 levels per launch in LDS; each
+// internal box is then the union of at most 2 log2(n) level nodes.  No inter-work-group
+// communication is needed.
 constexpr int kSegBlock = 256;
 constexpr int kSegLevels = 8;   // log2(kSegBlock)
 __device__ inline Box empty_box() {
@@ -191,19 +191,16 @@ __global__ void prim_boxes(int n, PrimSrc src, Box* __restrict__ out) {
     if (i < n) out[i] = prim_box(src, i);
 }
 
-// Box of the sorted leaves [a, b] from reduction levels 0..kSegLevels: the union of at most
-// 2 kSegLevels level nodes below level kSegLevels and of the level-kSegLevels nodes the range
-// still spans there (ceil(n / 256) at most: only ranges of more than 512 leaves reach it).
-// Exact: the union is a min / max.
+// Box of the sorted leaves [a, b]: the union of at most 2 log2(n) level nodes (exact: the
+// union is a min / max).
 __device__ inline Box range_box(int a, int b, const SegLevels& L, const Box* __restrict__ seg) {
     int l = 0;
     Box acc = empty_box();
-    while (a <= b && l < kSegLevels) {
+    while (a <= b && l < 40) {
         if (a & 1) { const Box x = seg[L.offset[l] + a]; acc = Box{vmin(acc.lo, x.lo), vmax(acc.hi, x.hi)}; ++a; }
         if (a <= b && !(b & 1)) { const Box x = seg[L.offset[l] + b]; acc = Box{vmin(acc.lo, x.lo), vmax(acc.hi, x.hi)}; --b; }
         a >>= 1; b >>= 1; ++l;
     }
-    for (; a <= b; ++a) { const Box x = seg[L.offset[l] + a]; acc = Box{vmin(acc.lo, x.lo), vmax(acc.hi, x.hi)}; }
     return acc;
 }
 // A box into its slot of a four-wide record: slot code (record << 2 | k), or -1 for none.
@@ -214,10 +211,12 @@ __device__ inline void put_wide_box(Node4* __restrict__ wide, int code, const Bo
     r.lo[k][0] = __double2float_rd(x.lo.x); r.lo[k][1] = __double2float_rd(x.lo.y); r.lo[k][2] = __double2float_rd(x.lo.z);
     r.hi[k][0] = __double2float_ru(x.hi.x); r.hi[k][1] = __double2float_ru(x.hi.y); r.hi[k][2] = __double2float_ru(x.hi.z);
 }
-// A tree's refit in two launches, for one tree or two at once (a work-group range each):
+// A tree's refit, for one tree or two at once (a work-group range each):
 //   leaves: the leaf boxes in leaf order (node records and level 0) and reduction levels
 //     1..kSegLevels, each work-group over its kSegBlock leaves; with a primitive source the
 //     primitive boxes are computed here too (and stored, for the exact leaf tests).
+//   upper: the next kSegLevels levels from the last built, one launch per kSegLevels
+//     levels (one for up to 2^16 leaves).
 //   internal: each internal node's box, the union of the level boxes that cover its leaf
 //     range (range_box).
 // No work-group waits for another inside a launch: on this GPU that would take a device-wide
@@ -277,6 +276,36 @@ __global__ __launch_bounds__(kSegBlock) void refit_leaves(RefitSet rs) {
         if (l < 40 && t.L.count[l] > 0 && (threadIdx.x & ((1 << l) - 1)) == 0) {
             const int j = g >> l;
             if (j < t.L.count[l]) t.seg[t.L.offset[l] + j] = sh[threadIdx.x];
+        }
+    }
+}
+// Levels l0+1 .. l0+kSegLevels from level l0, for every tree of the set that has them: the
+// trees' work-groups one after the other (blocks_for(count[l0], kSegBlock) each).
+__device__ inline int upper_blocks(const RefitTree& t, int l0) {
+    return t.levels > l0 ? (t.L.count[l0] + kSegBlock - 1) / kSegBlock : 0;
+}
+__global__ __launch_bounds__(kSegBlock) void refit_upper(RefitSet rs, int l0) {
+    const int b0 = upper_blocks(rs.t[0], l0);
+    const int which = (rs.ntrees > 1 && (int)blockIdx.x >= b0) ? 1 : 0;
+    const RefitTree& t = rs.t[which];
+    const int blk = (int)blockIdx.x - (which ? b0 : 0);
+    __shared__ Box sh[kSegBlock];
+    const int nsrc = t.L.count[l0];
+    const Box* src = t.seg + t.L.offset[l0];
+    const int g = blk * kSegBlock + threadIdx.x;
+    sh[threadIdx.x] = g < nsrc ? src[g] : empty_box();
+    __syncthreads();
+    for (int l = 1; l <= kSegLevels; ++l) {
+        const int half = 1 << (l - 1);
+        if ((threadIdx.x & ((1 << l) - 1)) == 0) {
+            const Box a = sh[threadIdx.x], c = sh[threadIdx.x + half];
+            sh[threadIdx.x] = Box{vmin(a.lo, c.lo), vmax(a.hi, c.hi)};
+        }
+        __syncthreads();
+        const int lv = l0 + l;
+        if (lv < 40 && t.L.count[lv] > 0 && (threadIdx.x & ((1 << l) - 1)) == 0) {
+            const int j = g >> l;
+            if (j < t.L.count[lv]) t.seg[t.L.offset[lv] + j] = sh[threadIdx.x];
         }
     }
 }
@@ -401,9 +430,18 @@ static RefitTree refit_view(LBVH& t, const PrimSrc& src) {
     return r;
 }
 static void refit_set(const RefitSet& rs, hipStream_t st) {
-    int lb = 0, ib = 0;
-    for (int k = 0; k < rs.ntrees; ++k) { lb += rs.t[k].leaf_blocks; ib += rs.t[k].int_blocks; }
+    int lb = 0, ib = 0, levels = 0;
+    for (int k = 0; k < rs.ntrees; ++k) {
+        lb += rs.t[k].leaf_blocks;
+        ib += rs.t[k].int_blocks;
+        levels = std::max(levels, rs.t[k].levels);
+    }
     if (lb > 0) hipLaunchKernelGGL(refit_leaves, dim3(lb), dim3(kSegBlock), 0, st, rs);
+    for (int l0 = kSegLevels; l0 < levels; l0 += kSegLevels) {
+        int ub = 0;
+        for (int k = 0; k < rs.ntrees; ++k) ub += rs.t[k].levels > l0 ? (int)blocks_for(rs.t[k].L.count[l0], kSegBlock) : 0;
+        if (ub > 0) hipLaunchKernelGGL(refit_upper, dim3(ub), dim3(kSegBlock), 0, st, rs, l0);
+    }
     if (ib > 0) hipLaunchKernelGGL(refit_internal, dim3(ib), dim3(256), 0, st, rs);
     CIPC_CHECK_LAUNCH();
 }
