@@ -348,19 +348,12 @@ struct BucketOut {
         }
     }
 };
-// Bucket bk = [starts[bk], starts[bk + 1]) of the 2V + 1 buckets, one wavefront each: up to
-// 64 entries sorted in registers (bitonic network over the lanes), up to kBucketLds in LDS
-// (bitonic network, the wave ordering its own LDS accesses between steps), longer buckets by
-// one lane in place.
-__global__ __launch_bounds__(64 * kBucketWaves) void bucket_sort(int nbk, const int* __restrict__ starts,
-                                                                 unsigned long long* __restrict__ keys, BucketOut out) {
-    __shared__ unsigned long long sh[kBucketWaves][kBucketLds];
-    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-    const int bk = blockIdx.x * kBucketWaves + w;
-    if (bk >= nbk) return;   // the whole wavefront: no block-wide barrier follows
-    out.gbase = starts[out.V + 1];   // the gradient entries follow the block entries
-    const int s0 = starts[bk], k = starts[bk + 1] - s0;
-    if (k <= 0) return;
+// Bucket bk = [starts[bk], starts[bk + 1]) of the 2V + 1 buckets, k > 1 entries, sorted by
+// one wavefront: up to 64 entries in registers (bitonic network over the lanes), up to
+// kBucketLds in LDS (bitonic network, the wave ordering its own LDS accesses between steps),
+// longer buckets by one lane in place.
+__device__ inline void sort_bucket(int bk, int s0, int k, int lane, unsigned long long* __restrict__ a,
+                                   unsigned long long* __restrict__ keys, const BucketOut& out) {
     if (k <= 64) {
         unsigned long long v = lane < k ? keys[s0 + lane] : ~0ull;
 #pragma unroll
@@ -375,7 +368,6 @@ __global__ __launch_bounds__(64 * kBucketWaves) void bucket_sort(int nbk, const 
         return;
     }
     if (k <= kBucketLds) {
-        unsigned long long* a = sh[w];
         int P = 128;
         while (P < k) P <<= 1;
         for (int i = lane; i < P; i += 64) a[i] = i < k ? keys[s0 + i] : ~0ull;
@@ -391,6 +383,7 @@ __global__ __launch_bounds__(64 * kBucketWaves) void bucket_sort(int nbk, const 
                 bucket_wave_order();
             }
         for (int i = lane; i < k; i += 64) out.put(bk, s0 + i, a[i]);
+        bucket_wave_order();   // the LDS is the next bucket's
         return;
     }
     if (lane == 0) {
@@ -401,6 +394,32 @@ __global__ __launch_bounds__(64 * kBucketWaves) void bucket_sort(int nbk, const 
             keys[j + 1] = x;
         }
         for (int i = s0; i < s0 + k; ++i) out.put(bk, i, keys[i]);
+    }
+}
+// Each wavefront takes bpw consecutive buckets (1 or 64): a lane writes its bucket's entry
+// when it has one, and the wave sorts the longer ones together, one after the other.  64
+// when the record slots are fewer than the buckets (most buckets then have none or one: a
+// batch of scenes with few contacts), else 1, so that long buckets (vertices in many
+// contacts, often neighbours) are not sorted one after the other by one wavefront.
+__global__ __launch_bounds__(64 * kBucketWaves) void bucket_sort(int nbk, int bpw, const int* __restrict__ starts,
+                                                                 unsigned long long* __restrict__ keys, BucketOut out) {
+    __shared__ unsigned long long sh[kBucketWaves][kBucketLds];
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    const int bk0 = (blockIdx.x * kBucketWaves + w) * bpw;
+    if (bk0 >= nbk) return;   // the whole wavefront: no block-wide barrier follows
+    out.gbase = starts[out.V + 1];   // the gradient entries follow the block entries
+    const int bk = bk0 + lane;
+    int s0 = 0, k = 0;
+    if (lane < bpw && bk < nbk) {
+        s0 = starts[bk];
+        k = starts[bk + 1] - s0;
+    }
+    if (k == 1) out.put(bk, s0, keys[s0]);
+    unsigned long long big = __ballot(k > 1);
+    while (big) {
+        const int l = __ffsll((long long)big) - 1;
+        big &= big - 1;
+        sort_bucket(bk0 + l, __shfl(s0, l), __shfl(k, l), lane, sh[w], keys, out);
     }
 }
 // Records without a key, after the keyed ones: block keys kNoKey, gradient keys past every vertex.
@@ -484,8 +503,9 @@ void finalize_csr(Context& c, bool count_collisions) {
             hipLaunchKernelGGL(bucket_scatter, dim3(blocks_for(nmax, 256)), dim3(256), 0, c.stream, nb, c.ckey.p, s.kb, bstart, bfill,
                                ng, c.gkey.p, V, gstart, gfill, c.bucket_entries.p);
             const BucketOut bo{V, s.kb, 0, c.ckey_sorted.p, c.cidx_sorted.p, c.gkey_sorted.p, c.gidx_sorted.p};
-            hipLaunchKernelGGL(bucket_sort, dim3(blocks_for(2 * V + 1, kBucketWaves)), dim3(64 * kBucketWaves), 0, c.stream, 2 * V + 1,
-                               bstart, c.bucket_entries.p, bo);
+            const int bpw = (size_t)nb + ng < 2 * (size_t)V + 1 ? 64 : 1;
+            hipLaunchKernelGGL(bucket_sort, dim3(blocks_for(2 * V + 1, kBucketWaves * bpw)), dim3(64 * kBucketWaves), 0,
+                               c.stream, 2 * V + 1, bpw, bstart, c.bucket_entries.p, bo);
             const size_t nt = std::max<size_t>(std::max<size_t>((size_t)nb + 1, (size_t)ng), ncnt);
             hipLaunchKernelGGL(bucket_tails, dim3(blocks_for(nt, 256)), dim3(256), 0, c.stream, nb, ng, bstart, V, c.ckey_sorted.p,
                                c.cidx_sorted.p, c.gkey_sorted.p, c.gidx_sorted.p, head, bcnt, (int)ncnt);

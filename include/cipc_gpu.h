@@ -284,7 +284,9 @@ int cipc_gpu_step_bound(cipc_gpu_context* ctx, double out[5]);
  * for bit: a full-CCD walk's result is a minimum over pairs, and the active set ranks its
  * hits by primitive.  entries (may be NULL) receives up to max_entries records (4 at most),
  * n_entries the number of tuned kernels.
- * cipc_gpu_set_tune sets a choice directly (the default is 3, 8 lanes per query); its
+ * cipc_gpu_set_tune sets a choice directly (an uploaded scene starts with the largest c,
+ * at most 4, that keeps 2^c times the walk's query count (surface vertices or surface
+ * edges) within 2^17 lanes); its
  * kernel 2 switches the Newton iteration's single graph launch (cipc_gpu_newton_iteration,
  * cipc_gpu_batch_newton_iteration) off (choice 0: every stage launched on its own and timed)
  * or on (1); by default a batch context uses it and a single-scene context does not. */
