@@ -860,18 +860,16 @@ __device__ inline void map_tail(int i, int capM, const int* __restrict__ dc, uns
     if (i >= dc[kDcMapKeys]) mkeys[i] = ~0ull;
 }
 // After the run-length encoding of the sorted keys: the runs without the all-ones run (there
-// is one exactly when nm < capM) are the merged map entries.
-__global__ void map_total(int capM, const int* __restrict__ nruns, int* __restrict__ dc) {
-    if (blockIdx.x != 0 || threadIdx.x != 0 || dc[kDcFlags]) return;
-    const int nM = *nruns - (dc[kDcMapKeys] < capM ? 1 : 0);
-    dc[kDcSelf] = dc[kDcA] + dc[kDcB] + nM;
-}
-__global__ void map_decode(const int* __restrict__ dc, const unsigned long long* __restrict__ keys, const int* __restrict__ counts, int V, int bits,
-                           int4* __restrict__ out) {
+// is one exactly when nm < capM) are the merged map entries, decoded after the A and B parts
+// of Self_ActiveSet; the first thread publishes the set's size.
+__global__ void map_decode(int capM, const int* __restrict__ nruns, int* __restrict__ dc, const unsigned long long* __restrict__ keys,
+                           const int* __restrict__ counts, int V, int bits, int4* __restrict__ out) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (dc[kDcFlags]) return;
     const int base = dc[kDcA] + dc[kDcB];
-    if (i >= dc[kDcSelf] - base) return;
+    const int nM = *nruns - (dc[kDcMapKeys] < capM ? 1 : 0);
+    if (i == 0) dc[kDcSelf] = base + nM;
+    if (i >= nM) return;
     const unsigned long long k = keys[i], m = (1ull << bits) - 1;
     out[base + i] = make_int4((int)((k >> (2 * bits)) & m) - V, (int)((k >> bits) & m), (int)(k & m) - 1, -counts[i]);
 }
@@ -1109,9 +1107,8 @@ void compute_active_set(Context& c, bool publish) {
     CIPC_HIP(hipcub::DeviceRunLengthEncode::Encode(nullptr, bytes, b.mkeys_sorted.p, b.ukeys.p, b.ucounts.p, b.nruns.p, capM, c.stream));
     c.tmp.reserve(bytes);
     CIPC_HIP(hipcub::DeviceRunLengthEncode::Encode(c.tmp.p, bytes, b.mkeys_sorted.p, b.ukeys.p, b.ucounts.p, b.nruns.p, capM, c.stream));
-    hipLaunchKernelGGL(map_total, dim3(1), dim3(64), 0, c.stream, capM, b.nruns.p, dc);
-    hipLaunchKernelGGL(map_decode, dim3(blocks_for(capM, 256)), dim3(256), 0, c.stream, dc, b.ukeys.p, b.ucounts.p, s.V, bits,
-                       c.self_set.p);
+    hipLaunchKernelGGL(map_decode, dim3(blocks_for(capM, 256)), dim3(256), 0, c.stream, capM, b.nruns.p, dc, b.ukeys.p, b.ucounts.p,
+                       s.V, bits, c.self_set.p);
     CIPC_CHECK_LAUNCH();
     // the counts are read with the host's next synchronisation (settle_active_set)
     c.pin_dc.reserve(kDcWords);

@@ -216,9 +216,8 @@ __global__ void pair_count(const int* __restrict__ nm_ptr, const int* __restrict
     cnt[pair_index(mptr, b, cc, k)] = n;
 }
 // outer[j]: the column starts; the last one, the nonzero count, also into nnz_out
-__global__ void column_starts(int V, const int* __restrict__ mptr, const int* __restrict__ ppos, const int* __restrict__ nm_ptr,
-                              int* __restrict__ outer, int* __restrict__ nnz_out) {
-    const int j = blockIdx.x * blockDim.x + threadIdx.x;
+__device__ inline void column_starts(int j, int V, const int* __restrict__ mptr, const int* __restrict__ ppos,
+                                     const int* __restrict__ nm_ptr, int* __restrict__ outer, int* __restrict__ nnz_out) {
     if (j > 3 * V) return;
     if (j == 3 * V) {
         const int nnz = ppos[3 * *nm_ptr];
@@ -229,10 +228,9 @@ __global__ void column_starts(int V, const int* __restrict__ mptr, const int* __
     const int b = j / 3, cc = j % 3;
     outer[j] = ppos[pair_index(mptr, b, cc, mptr[b])];
 }
-__global__ void pair_fill(const int* __restrict__ nm_ptr, const int* __restrict__ mptr, const int* __restrict__ mrow,
-                          const int* __restrict__ mcol, const double* __restrict__ mval, const unsigned short* __restrict__ mflag,
-                          const int* __restrict__ ppos, int* __restrict__ inner, double* __restrict__ val) {
-    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+__device__ inline void pair_fill(int t, const int* __restrict__ nm_ptr, const int* __restrict__ mptr, const int* __restrict__ mrow,
+                                 const int* __restrict__ mcol, const double* __restrict__ mval, const unsigned short* __restrict__ mflag,
+                                 const int* __restrict__ ppos, int* __restrict__ inner, double* __restrict__ val) {
     if (t >= 3 * *nm_ptr) return;
     const int k = t / 3, cc = t % 3;
     const int b = mcol[k], a = mrow[k];
@@ -246,10 +244,34 @@ __global__ void pair_fill(const int* __restrict__ nm_ptr, const int* __restrict_
         }
     }
 }
-__global__ void masked_rhs(int V, const int* __restrict__ bt, const double* __restrict__ grad, double* __restrict__ rhs) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+__device__ inline void masked_rhs(int i, int V, const int* __restrict__ bt, const double* __restrict__ grad, double* __restrict__ rhs) {
     if (i >= 3 * V) return;
     rhs[i] = is_free(bt[i / 3]) ? grad[i] : 0.0;
+}
+// The CSR's last three steps in one launch (they read what earlier launches wrote and write
+// disjoint arrays): work-groups [0, bC) the column starts, the next bP the entries of the
+// merged blocks, the rest the right-hand side.
+struct CsrFinish {
+    int V, bC, bP;
+    const int *mptr, *ppos, *nm_ptr, *mrow, *mcol, *bt;
+    const double *mval, *grad;
+    const unsigned short* mflag;
+    int *outer, *nnz_out, *inner;
+    double *val, *rhs;
+};
+__global__ void csr_finish(CsrFinish a) {
+    int b = blockIdx.x;
+    if (b < a.bC) {
+        column_starts(b * blockDim.x + threadIdx.x, a.V, a.mptr, a.ppos, a.nm_ptr, a.outer, a.nnz_out);
+        return;
+    }
+    b -= a.bC;
+    if (b < a.bP) {
+        pair_fill(b * blockDim.x + threadIdx.x, a.nm_ptr, a.mptr, a.mrow, a.mcol, a.mval, a.mflag, a.ppos, a.inner, a.val);
+        return;
+    }
+    b -= a.bP;
+    masked_rhs(b * blockDim.x + threadIdx.x, a.V, a.bt, a.grad, a.rhs);
 }
 
 // Collision count as compute_g_dpt returns it (ContactMechanics.cpp:2970-3032): the
@@ -555,9 +577,17 @@ void finalize_csr(Context& c, bool count_collisions) {
     exclusive_scan_int(c, c.pcnt.p, c.ppos.p, (int)(3 * mcap + 1));
     c.outer.resize((size_t)3 * V + 1);
     c.coll.resize(2);
-    hipLaunchKernelGGL(column_starts, dim3(blocks_for(3 * V + 1, 256)), dim3(256), 0, c.stream, V, c.mptr.p, c.ppos.p, nm_ptr, c.outer.p,
-                       c.coll.p);
-    CIPC_CHECK_LAUNCH();
+    c.inner.resize(9 * mcap); c.val.resize(9 * mcap);
+    c.rhs.resize((size_t)3 * V);
+    {
+        CsrFinish a;
+        a.V = V; a.bC = (int)blocks_for(3 * V + 1, 256); a.bP = (int)blocks_for(3 * mcap, 256);
+        a.mptr = c.mptr.p; a.ppos = c.ppos.p; a.nm_ptr = nm_ptr; a.mrow = c.mrow.p; a.mcol = c.mcol.p; a.bt = s.bt.p;
+        a.mval = c.mval.p; a.grad = c.grad.p; a.mflag = c.mflag.p; a.outer = c.outer.p; a.nnz_out = c.coll.p;
+        a.inner = c.inner.p; a.val = c.val.p; a.rhs = c.rhs.p;
+        hipLaunchKernelGGL(csr_finish, dim3(a.bC + a.bP + (int)blocks_for(3 * V, 256)), dim3(256), 0, c.stream, a);
+        CIPC_CHECK_LAUNCH();
+    }
     // the counts are read with the caller's next synchronisation (the collision count is
     // launched here, or by the caller on a stream of its own before it publishes them); the
     // entries are sized by their bound (9 per merged block)
@@ -565,12 +595,6 @@ void finalize_csr(Context& c, bool count_collisions) {
         collision_count_launch(c, c.stream);
         publish_nnz(c, c.stream);
     }
-    c.inner.resize(9 * mcap); c.val.resize(9 * mcap);
-    hipLaunchKernelGGL(pair_fill, dim3(blocks_for(3 * mcap, 256)), dim3(256), 0, c.stream, nm_ptr, c.mptr.p, c.mrow.p, c.mcol.p,
-                       c.mval.p, c.mflag.p, c.ppos.p, c.inner.p, c.val.p);
-    CIPC_CHECK_LAUNCH();
-    c.rhs.resize((size_t)3 * V);
-    hipLaunchKernelGGL(masked_rhs, dim3(blocks_for(3 * V, 256)), dim3(256), 0, c.stream, V, s.bt.p, c.grad.p, c.rhs.p);
     CIPC_CHECK_LAUNCH();
 }
 
