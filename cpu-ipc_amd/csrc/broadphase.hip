@@ -28,6 +28,7 @@
 #include "lbvh.hpp"
 
 #include <hipcub/hipcub.hpp>
+#include <type_traits>
 
 namespace cipc {
 
@@ -485,19 +486,17 @@ void build_tree(Context& c, LBVH& t, int n) {
     refit(c, t, PrimSrc{}, c.stream);
 }
 
-// The static face tree on st_f and edge tree on st_e (each with its own temporary storage).
-static void build_static_trees_on(Context& c, hipStream_t st_f, DBuf<unsigned char>& tmp_f, hipStream_t st_e,
-                                  DBuf<unsigned char>& tmp_e) {
+// The static face and edge trees, on the context's stream.
+void build_static_trees(Context& c) {
     Scene& s = c.sc;
     c.face_tree.prim.resize(std::max(s.F, 1));
     c.edge_tree.prim.resize(std::max(s.Es, 1));
     PrimSrc fs, es;
     fs.kind = 1; fs.faces = s.faces.p; fs.x = s.x.p;
     es.kind = 2; es.se = s.se.p; es.x = s.x.p;
-    build_tree_from(c, c.face_tree, s.F, fs, st_f, tmp_f);
-    build_tree_from(c, c.edge_tree, s.Es, es, st_e, tmp_e);
+    build_tree_from(c, c.face_tree, s.F, fs, c.stream, c.tmp);
+    build_tree_from(c, c.edge_tree, s.Es, es, c.stream, c.tmp);
 }
-void build_static_trees(Context& c) { build_static_trees_on(c, c.stream, c.tmp, c.stream, c.tmp); }
 void build_swept_trees(Context& c, const unsigned long long* abits, hipStream_t st) {
     Scene& s = c.sc;
     c.sface_tree.prim.resize(std::max(s.F, 1));
@@ -547,15 +546,13 @@ __device__ inline int sum_over_parts(int v) {
 // The point walk also sets each surface vertex's ground flag (distance^2 to the ground below
 // Hhat, Ground::calculateActivateSet, CollisionBroadPhase.cpp:1076-1092), for the same scan.
 template <int PB>
-__global__ __launch_bounds__(64) void pt_traverse(int Vs, const int* __restrict__ perm, const int* __restrict__ sv,
-                                                   const double* __restrict__ x, TreeView tv, double sq, int* __restrict__ cnt,
-                                                   int* __restrict__ cntp, int* __restrict__ buf, int* __restrict__ fg, d3 gn,
-                                                   double gd, double hhat) {
+__device__ inline void pt_traverse_at(int t, int* stk, int Vs, const int* __restrict__ perm, const int* __restrict__ sv,
+                                      const double* __restrict__ x, TreeView tv, double sq, int* __restrict__ cnt,
+                                      int* __restrict__ cntp, int* __restrict__ buf, int* __restrict__ fg, d3 gn, double gd,
+                                      double hhat) {
     constexpr int P = 1 << PB;
-    const int t = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
     const bool active = t < Vs * P;
     const int s = active ? perm[t >> PB] : 0, part = t & (P - 1);
-    __shared__ int stk[kStack * 64];
     int n = 0;
     if (active) {
         const Box qb = box_pt(ld3(x, sv[s]));
@@ -569,16 +566,24 @@ __global__ __launch_bounds__(64) void pt_traverse(int Vs, const int* __restrict_
         const double dist = dot(gn, ld3(x, sv[s])) - gd;
         fg[s] = (dist * dist < hhat) ? 1 : 0;
     }
+}
+template <int PB>
+__global__ __launch_bounds__(64) void pt_traverse(int Vs, const int* __restrict__ perm, const int* __restrict__ sv,
+                                                   const double* __restrict__ x, TreeView tv, double sq, int* __restrict__ cnt,
+                                                   int* __restrict__ cntp, int* __restrict__ buf, int* __restrict__ fg, d3 gn,
+                                                   double gd, double hhat) {
+    __shared__ int stk[kStack * 64];
+    pt_traverse_at<PB>(xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x, stk, Vs, perm, sv, x, tv, sq, cnt, cntp, buf,
+                       fg, gn, gd, hhat);
     if (blockIdx.x == 0 && threadIdx.x == 0) { cnt[Vs] = 0; fg[Vs] = 0; }   // the scans' tails
 }
 
 template <int PB>
-__global__ __launch_bounds__(64) void pt_compact(int Vs, const int* __restrict__ perm, const int* __restrict__ sv, const double* __restrict__ x, TreeView tv,
-                           double sq, const int* __restrict__ off, const int* __restrict__ cntp, const int* __restrict__ buf,
-                           int* __restrict__ candU, int* __restrict__ own, const int* __restrict__ dc) {
+__device__ inline void pt_compact_at(int t, int* stk, int Vs, const int* __restrict__ perm, const int* __restrict__ sv,
+                                     const double* __restrict__ x, TreeView tv, double sq, const int* __restrict__ off,
+                                     const int* __restrict__ cntp, const int* __restrict__ buf, int* __restrict__ candU,
+                                     int* __restrict__ own, const int* __restrict__ dc) {
     constexpr int P = 1 << PB;
-    __shared__ int stk[kStack * 64];
-    const int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= Vs || dc[kDcFlags]) return;   // over capacity: the host builds the set again
     const int s = perm[t];
     const int o = off[s];
@@ -597,12 +602,19 @@ __global__ __launch_bounds__(64) void pt_compact(int Vs, const int* __restrict__
     for (int k = o; k < pos; ++k) own[k] = s;
 }
 
+template <int PB>
+__global__ __launch_bounds__(64) void pt_compact(int Vs, const int* __restrict__ perm, const int* __restrict__ sv, const double* __restrict__ x, TreeView tv,
+                           double sq, const int* __restrict__ off, const int* __restrict__ cntp, const int* __restrict__ buf,
+                           int* __restrict__ candU, int* __restrict__ own, const int* __restrict__ dc) {
+    __shared__ int stk[kStack * 64];
+    pt_compact_at<PB>(blockIdx.x * blockDim.x + threadIdx.x, stk, Vs, perm, sv, x, tv, sq, off, cntp, buf, candU, own, dc);
+}
+
 // Point-triangle hits: filters, then classification (CollisionBroadPhase.cpp:765-857).
-__global__ void pt_classify(const int* __restrict__ dc, int F, const int* __restrict__ own, const int* __restrict__ off, const int* __restrict__ sv,
-                            const double* __restrict__ x, const int* __restrict__ bt, const int4* __restrict__ faces,
-                            const int* __restrict__ candU, int* __restrict__ cand, double hhat,
-                            double sq, const int* __restrict__ vs, int4* __restrict__ rec, int* __restrict__ cat) {
-    const int u = blockIdx.x * blockDim.x + threadIdx.x;
+__device__ inline void pt_classify(int u, const int* __restrict__ dc, int F, const int* __restrict__ own, const int* __restrict__ off,
+                                   const int* __restrict__ sv, const double* __restrict__ x, const int* __restrict__ bt,
+                                   const int4* __restrict__ faces, const int* __restrict__ candU, int* __restrict__ cand, double hhat,
+                                   double sq, const int* __restrict__ vs, int4* __restrict__ rec, int* __restrict__ cat) {
     if (dc[kDcFlags] || u >= dc[kDcPT]) return;
     // sorted position: the hit's rank among its query's hits (all distinct)
     const int q = own[u], o = off[q], e = off[q + 1];
@@ -651,14 +663,12 @@ __global__ void pt_classify(const int* __restrict__ dc, int F, const int* __rest
 }
 
 template <int PB>
-__global__ __launch_bounds__(64) void ee_traverse(int Es, const int* __restrict__ perm, const int2* __restrict__ se,
-                                                   const double* __restrict__ x, TreeView tv, double sq, int* __restrict__ cnt,
-                                                   int* __restrict__ cntp, int* __restrict__ buf) {
+__device__ inline void ee_traverse_at(int t, int* stk, int Es, const int* __restrict__ perm, const int2* __restrict__ se,
+                                      const double* __restrict__ x, TreeView tv, double sq, int* __restrict__ cnt,
+                                      int* __restrict__ cntp, int* __restrict__ buf) {
     constexpr int P = 1 << PB;
-    const int t = xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
     const bool active = t < Es * P;
     const int eI = active ? perm[t >> PB] : 0, part = t & (P - 1);
-    __shared__ int stk[kStack * 64];
     int n = 0;
     if (active) {
         const int2 A = se[eI];
@@ -671,16 +681,22 @@ __global__ __launch_bounds__(64) void ee_traverse(int Es, const int* __restrict_
     }
     const int tot = sum_over_parts<PB>(n);
     if (active && part == 0) cnt[eI] = tot;
+}
+template <int PB>
+__global__ __launch_bounds__(64) void ee_traverse(int Es, const int* __restrict__ perm, const int2* __restrict__ se,
+                                                   const double* __restrict__ x, TreeView tv, double sq, int* __restrict__ cnt,
+                                                   int* __restrict__ cntp, int* __restrict__ buf) {
+    __shared__ int stk[kStack * 64];
+    ee_traverse_at<PB>(xcd_block(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x, stk, Es, perm, se, x, tv, sq, cnt, cntp, buf);
     if (blockIdx.x == 0 && threadIdx.x == 0) cnt[Es] = 0;   // the scan's tail
 }
 
 template <int PB>
-__global__ __launch_bounds__(64) void ee_compact(int Es, const int* __restrict__ perm, const int2* __restrict__ se, const double* __restrict__ x, TreeView tv,
-                           double sq, const int* __restrict__ off, const int* __restrict__ cntp, const int* __restrict__ buf,
-                           int* __restrict__ candU, int* __restrict__ own, const int* __restrict__ dc) {
+__device__ inline void ee_compact_at(int t, int* stk, int Es, const int* __restrict__ perm, const int2* __restrict__ se,
+                                     const double* __restrict__ x, TreeView tv, double sq, const int* __restrict__ off,
+                                     const int* __restrict__ cntp, const int* __restrict__ buf, int* __restrict__ candU,
+                                     int* __restrict__ own, const int* __restrict__ dc) {
     constexpr int P = 1 << PB;
-    __shared__ int stk[kStack * 64];
-    const int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= Es || dc[kDcFlags]) return;
     const int eI = perm[t];
     const int o = off[eI] - off[0];   // the joint scan's edge starts begin at the point total
@@ -702,13 +718,20 @@ __global__ __launch_bounds__(64) void ee_compact(int Es, const int* __restrict__
     for (int k = o; k < pos; ++k) own[k] = eI;
 }
 
+template <int PB>
+__global__ __launch_bounds__(64) void ee_compact(int Es, const int* __restrict__ perm, const int2* __restrict__ se, const double* __restrict__ x, TreeView tv,
+                           double sq, const int* __restrict__ off, const int* __restrict__ cntp, const int* __restrict__ buf,
+                           int* __restrict__ candU, int* __restrict__ own, const int* __restrict__ dc) {
+    __shared__ int stk[kStack * 64];
+    ee_compact_at<PB>(blockIdx.x * blockDim.x + threadIdx.x, stk, Es, perm, se, x, tv, sq, off, cntp, buf, candU, own, dc);
+}
+
 // Edge-edge hits: filters, then classification (CollisionBroadPhase.cpp:858-1052).
-__global__ void ee_classify(const int* __restrict__ dc, int Es, const int* __restrict__ own, const int* __restrict__ off, const int2* __restrict__ se,
-                            const double* __restrict__ x, const double* __restrict__ vrest, const int* __restrict__ bt,
-                            const int* __restrict__ candU, int* __restrict__ cand, double hhat,
-                            double sq, const int* __restrict__ vs, int4* __restrict__ rec, int2* __restrict__ aux,
-                            int* __restrict__ cat) {
-    const int u = blockIdx.x * blockDim.x + threadIdx.x;
+__device__ inline void ee_classify(int u, const int* __restrict__ dc, int Es, const int* __restrict__ own, const int* __restrict__ off,
+                                   const int2* __restrict__ se, const double* __restrict__ x, const double* __restrict__ vrest,
+                                   const int* __restrict__ bt, const int* __restrict__ candU, int* __restrict__ cand, double hhat,
+                                   double sq, const int* __restrict__ vs, int4* __restrict__ rec, int2* __restrict__ aux,
+                                   int* __restrict__ cat) {
     if (dc[kDcFlags] || u >= dc[kDcEE]) return;
     const int eI = own[u], o = off[eI] - off[0], e = off[eI + 1] - off[0];
     const int eJ = candU[u];
@@ -952,6 +975,111 @@ void size_active_walks(Context& c, int pt, int ee) {
     b.bufPT.resize((Vs << pt) * kPartCap); b.bufEE.resize((Es << ee) * kPartCap);
 }
 
+// Both walks of the active set in one launch, and both compactions in another: work-groups
+// [0, nbp) take point queries with 2^PBP lanes each, the rest edge queries with 2^PBE (a
+// work-group is one wavefront, so a wavefront's lanes share a query kind).  The traversal
+// maps work-groups to XCDs as the single walks do, over the joint grid.
+struct BothWalks {
+    int Vs, Es, nbp;
+    const int *pperm, *sv, *eperm;
+    const int2* se;
+    const double* x;
+    TreeView ft, et;
+    double sq, gd, hhat;
+    d3 gn;
+    int *cntPT, *cntEE, *fg, *cntpPT, *cntpEE, *bufPT, *bufEE;
+    const int *oPT, *oEE, *dc;
+    int *candUPT, *candUEE, *ownPT, *ownEE;
+};
+template <int PBP, int PBE>
+__global__ __launch_bounds__(64) void traverse_both(BothWalks a) {
+    __shared__ int stk[kStack * 64];
+    const int mb = xcd_block(blockIdx.x, gridDim.x);
+    if (mb < a.nbp)
+        pt_traverse_at<PBP>(mb * 64 + threadIdx.x, stk, a.Vs, a.pperm, a.sv, a.x, a.ft, a.sq, a.cntPT, a.cntpPT, a.bufPT, a.fg, a.gn,
+                            a.gd, a.hhat);
+    else
+        ee_traverse_at<PBE>((mb - a.nbp) * 64 + threadIdx.x, stk, a.Es, a.eperm, a.se, a.x, a.et, a.sq, a.cntEE, a.cntpEE, a.bufEE);
+    if (blockIdx.x == 0 && threadIdx.x == 0) { a.cntPT[a.Vs] = 0; a.fg[a.Vs] = 0; a.cntEE[a.Es] = 0; }   // the scans' tails
+}
+template <int PBP, int PBE>
+__global__ __launch_bounds__(64) void compact_both(BothWalks a) {
+    __shared__ int stk[kStack * 64];
+    const int b = blockIdx.x;
+    if (b < a.nbp)
+        pt_compact_at<PBP>(b * 64 + threadIdx.x, stk, a.Vs, a.pperm, a.sv, a.x, a.ft, a.sq, a.oPT, a.cntpPT, a.bufPT, a.candUPT,
+                           a.ownPT, a.dc);
+    else
+        ee_compact_at<PBE>((b - a.nbp) * 64 + threadIdx.x, stk, a.Es, a.eperm, a.se, a.x, a.et, a.sq, a.oEE, a.cntpEE, a.bufEE,
+                           a.candUEE, a.ownEE, a.dc);
+}
+// Both hit kinds' filters and classification in one launch: work-groups [0, nbp) the point
+// hits, the rest the edge hits.
+struct BothClassify {
+    int nbp, F, Es;
+    const int *dc, *ownPT, *oPT, *sv, *bt, *candUPT, *ownEE, *oEE, *candUEE, *vs;
+    const int4* faces;
+    const int2* se;
+    const double *x, *vrest;
+    double hhat, sq;
+    int *candPT, *catPT, *candEE, *catEE;
+    int4 *recPT, *recEE;
+    int2* auxEE;
+};
+__global__ void classify_both(BothClassify a) {
+    const int b = blockIdx.x;
+    if (b < a.nbp)
+        pt_classify(b * blockDim.x + threadIdx.x, a.dc, a.F, a.ownPT, a.oPT, a.sv, a.x, a.bt, a.faces, a.candUPT, a.candPT, a.hhat, a.sq,
+                    a.vs, a.recPT, a.catPT);
+    else
+        ee_classify((b - a.nbp) * blockDim.x + threadIdx.x, a.dc, a.Es, a.ownEE, a.oEE, a.se, a.x, a.vrest, a.bt, a.candUEE, a.candEE,
+                    a.hhat, a.sq, a.vs, a.recEE, a.auxEE, a.catEE);
+}
+template <int PBP, typename F>
+static void with_pbe(int pbe, F&& f) {
+    switch (pbe) {
+    case 0: f(std::integral_constant<int, PBP>{}, std::integral_constant<int, 0>{}); break;
+    case 1: f(std::integral_constant<int, PBP>{}, std::integral_constant<int, 1>{}); break;
+    case 2: f(std::integral_constant<int, PBP>{}, std::integral_constant<int, 2>{}); break;
+    case 3: f(std::integral_constant<int, PBP>{}, std::integral_constant<int, 3>{}); break;
+    default: f(std::integral_constant<int, PBP>{}, std::integral_constant<int, 4>{}); break;
+    }
+}
+template <typename F>
+static void with_splits(int pbp, int pbe, F&& f) {
+    switch (pbp) {
+    case 0: with_pbe<0>(pbe, f); break;
+    case 1: with_pbe<1>(pbe, f); break;
+    case 2: with_pbe<2>(pbe, f); break;
+    case 3: with_pbe<3>(pbe, f); break;
+    default: with_pbe<4>(pbe, f); break;
+    }
+}
+// Both walks (compact = false) or both compactions, with the context's splits, on st; needs
+// Vs > 0 and Es > 0.
+static void launch_both_walks(Context& c, bool compact, hipStream_t st) {
+    Scene& s = c.sc;
+    ActiveSetScratch& b = c.as;
+    const int pbp = c.tune.as_pt_bits, pbe = c.tune.as_ee_bits;
+    BothWalks a;
+    a.Vs = s.Vs; a.Es = s.Es;
+    a.nbp = (int)blocks_for(compact ? (size_t)s.Vs : (size_t)s.Vs << pbp, 64);
+    const int nbe = (int)blocks_for(compact ? (size_t)s.Es : (size_t)s.Es << pbe, 64);
+    a.pperm = s.sv_perm.p; a.sv = s.sv.p; a.eperm = s.se_perm.p; a.se = s.se.p; a.x = s.x.p;
+    a.ft = TreeView{c.face_tree.n, c.face_tree.nodes.p, c.face_tree.wide.p};
+    a.et = TreeView{c.edge_tree.n, c.edge_tree.nodes.p, c.edge_tree.wide.p};
+    a.sq = sqrt(s.hhat); a.gd = s.gd; a.hhat = s.hhat; a.gn = s.gn;
+    a.cntPT = b.cnt_all.p; a.cntEE = a.cntPT + (s.Vs + 1); a.fg = a.cntEE + (s.Es + 1);
+    a.cntpPT = b.cntpPT.p; a.cntpEE = b.cntpEE.p; a.bufPT = b.bufPT.p; a.bufEE = b.bufEE.p;
+    a.oPT = b.off_all.p; a.oEE = a.oPT + (s.Vs + 1); a.dc = c.dcnt.p;
+    a.candUPT = b.candUPT.p; a.candUEE = b.candUEE.p; a.ownPT = b.ownPT.p; a.ownEE = b.ownEE.p;
+    with_splits(pbp, pbe, [&](auto P, auto E) {
+        if (compact) hipLaunchKernelGGL((compact_both<decltype(P)::value, decltype(E)::value>), dim3(a.nbp + nbe), dim3(64), 0, st, a);
+        else hipLaunchKernelGGL((traverse_both<decltype(P)::value, decltype(E)::value>), dim3(a.nbp + nbe), dim3(64), 0, st, a);
+    });
+    CIPC_CHECK_LAUNCH();
+}
+
 // The active set's walks with 2^pb lanes per query: the traversal (compact = false) or the
 // compaction of its hits (compact = true), point-triangle (pt) or edge-edge.
 void launch_active_walk(Context& c, bool pt, bool compact, int pb, hipStream_t st) {
@@ -1003,20 +1131,24 @@ void launch_active_walk(Context& c, bool pt, bool compact, int pb, hipStream_t s
 // nE <= nEE, CCD pairs nPTk + nEEk <= nPT + nEE, ground nG <= Vs).
 void compute_active_set(Context& c, bool publish) {
     Scene& s = c.sc;
-    // the face tree and the point walk on the main stream, the edge tree and the edge walk on
-    // the second active-set stream, beside them (small launches that leave the GPU room)
-    CIPC_HIP(hipEventRecord(c.as2_fork, c.stream));
-    CIPC_HIP(hipStreamWaitEvent(c.as2, c.as2_fork, 0));
-    build_static_trees_on(c, c.stream, c.tmp, c.as2, c.tmp_as2);
+    // Everything on the main stream: both static trees refitted by the same launches, both
+    // walks in one launch, both compactions in one, both classifications in one.
+    const int Vs = s.Vs, Es = s.Es;
+    {
+        PrimSrc fs, es;
+        fs.kind = 1; fs.faces = s.faces.p; fs.x = s.x.p;
+        es.kind = 2; es.se = s.se.p; es.x = s.x.p;
+        c.face_tree.prim.resize(std::max(s.F, 1));
+        c.edge_tree.prim.resize(std::max(Es, 1));
+        if (refit_only(c.face_tree, s.F) && refit_only(c.edge_tree, Es)) refit_two(c.face_tree, fs, c.edge_tree, es, c.stream);
+        else build_static_trees(c);
+    }
     ActiveSetScratch& b = c.as;
     const double sq = sqrt(s.hhat);
-    const int Vs = s.Vs, Es = s.Es;
     if (b.capPT == 0) b.capPT = std::max(1024, 4 * Vs);
     if (b.capEE == 0) b.capEE = std::max(1024, 4 * Es);
     if (b.capM == 0) b.capM = 1024;
     const int capPT = b.capPT, capEE = b.capEE, capM = b.capM;
-    const TreeView ft{c.face_tree.n, c.face_tree.nodes.p, c.face_tree.wide.p};
-    const TreeView et{c.edge_tree.n, c.edge_tree.nodes.p, c.edge_tree.wide.p};
     c.dcnt.resize(kDcWords);
     int* dc = c.dcnt.p;
     hipLaunchKernelGGL(dc_reset, dim3(1), dim3(64), 0, c.stream, dc);
@@ -1030,15 +1162,18 @@ void compute_active_set(Context& c, bool publish) {
     const int* oEE = oPT + (Vs + 1);
     const int* oG = oEE + (Es + 1);
     size_active_walks(c, c.tune.as_pt_bits, c.tune.as_ee_bits);
-    // A: one walk per query part (the point walk also sets the ground flags)
-    if (Vs > 0) launch_active_walk(c, true, false, c.tune.as_pt_bits);
-    if (Es > 0) launch_active_walk(c, false, false, c.tune.as_ee_bits, c.as2);
-    // (the walks also zero the scan segments' tails)
-    if (Vs == 0) hipLaunchKernelGGL(point_tails, dim3(1), dim3(1), 0, c.stream, Vs, cPT, cG);
-    if (Es == 0) hipLaunchKernelGGL(fill_tail, dim3(1), dim3(1), 0, c.as2, cEE, Es);
+    const bool both = Vs > 0 && Es > 0;
+    // A: one walk per query part (the point walk also sets the ground flags; the walks zero
+    // the scan segments' tails)
+    if (both) {
+        launch_both_walks(c, false, c.stream);
+    } else {
+        if (Vs > 0) launch_active_walk(c, true, false, c.tune.as_pt_bits);
+        if (Es > 0) launch_active_walk(c, false, false, c.tune.as_ee_bits);
+        if (Vs == 0) hipLaunchKernelGGL(point_tails, dim3(1), dim3(1), 0, c.stream, Vs, cPT, cG);
+        if (Es == 0) hipLaunchKernelGGL(fill_tail, dim3(1), dim3(1), 0, c.stream, cEE, Es);
+    }
     CIPC_CHECK_LAUNCH();
-    CIPC_HIP(hipEventRecord(c.as2_join, c.as2));
-    CIPC_HIP(hipStreamWaitEvent(c.stream, c.as2_join, 0));
     // one scan: the edge and ground segments' starts come out offset by their first entry
     exclusive_scan_int(c, b.cnt_all.p, b.off_all.p, nall);
     hipLaunchKernelGGL(candidate_totals, dim3(1), dim3(64), 0, c.stream, Vs, Es, oPT, oEE, capPT, capEE, dc);
@@ -1047,24 +1182,23 @@ void compute_active_set(Context& c, bool publish) {
     b.candUPT.resize(capPT + 1); b.candUEE.resize(capEE + 1);
     b.candPT.resize(capPT + 1); b.recPT.resize(capPT + 1); b.catPT.resize(capPT + 1); b.ownPT.resize(capPT + 1);
     b.candEE.resize(capEE + 1); b.recEE.resize(capEE + 1); b.catEE.resize(capEE + 1); b.auxEE.resize(capEE + 1); b.ownEE.resize(capEE + 1);
-    // the edge hits on the second active-set stream (forked here, before the point hits are
-    // enqueued on the main stream), beside the point hits
-    if (Es > 0) {
-        CIPC_HIP(hipEventRecord(c.as2_fork2, c.stream));
-        CIPC_HIP(hipStreamWaitEvent(c.as2, c.as2_fork2, 0));
-        launch_active_walk(c, false, true, c.tune.as_ee_bits, c.as2);
-        hipLaunchKernelGGL(ee_classify, dim3(blocks_for(capEE, 128)), dim3(128), 0, c.as2, dc, Es, b.ownEE.p, oEE, s.se.p,
-                           s.x.p, s.vrest.p, s.bt.p, b.candUEE.p, b.candEE.p, s.hhat, sq, scene_of_vertex(s), b.recEE.p, b.auxEE.p,
-                           b.catEE.p);
-        CIPC_CHECK_LAUNCH();
-        CIPC_HIP(hipEventRecord(c.as2_join2, c.as2));
+    BothClassify cl;
+    cl.F = s.F; cl.Es = Es;
+    cl.dc = dc; cl.ownPT = b.ownPT.p; cl.oPT = oPT; cl.sv = s.sv.p; cl.bt = s.bt.p; cl.candUPT = b.candUPT.p;
+    cl.ownEE = b.ownEE.p; cl.oEE = oEE; cl.candUEE = b.candUEE.p; cl.vs = scene_of_vertex(s); cl.faces = s.faces.p; cl.se = s.se.p;
+    cl.x = s.x.p; cl.vrest = s.vrest.p; cl.hhat = s.hhat; cl.sq = sq;
+    cl.candPT = b.candPT.p; cl.catPT = b.catPT.p; cl.candEE = b.candEE.p; cl.catEE = b.catEE.p;
+    cl.recPT = b.recPT.p; cl.recEE = b.recEE.p; cl.auxEE = b.auxEE.p;
+    // the point hits' work-groups, then the edge hits' (none of a kind without queries)
+    cl.nbp = Vs > 0 ? (int)blocks_for(capPT, 128) : 0;
+    const int nbe = Es > 0 ? (int)blocks_for(capEE, 128) : 0;
+    if (both) {
+        launch_both_walks(c, true, c.stream);
+    } else {
+        if (Vs > 0) launch_active_walk(c, true, true, c.tune.as_pt_bits);
+        if (Es > 0) launch_active_walk(c, false, true, c.tune.as_ee_bits);
     }
-    if (Vs > 0) {
-        launch_active_walk(c, true, true, c.tune.as_pt_bits);
-        hipLaunchKernelGGL(pt_classify, dim3(blocks_for(capPT, 128)), dim3(128), 0, c.stream, dc, s.F, b.ownPT.p, oPT, s.sv.p, s.x.p,
-                           s.bt.p, s.faces.p, b.candUPT.p, b.candPT.p, s.hhat, sq, scene_of_vertex(s), b.recPT.p, b.catPT.p);
-    }
-    if (Es > 0) CIPC_HIP(hipStreamWaitEvent(c.stream, c.as2_join2, 0));
+    if (cl.nbp + nbe > 0) hipLaunchKernelGGL(classify_both, dim3(cl.nbp + nbe), dim3(128), 0, c.stream, cl);
     CIPC_CHECK_LAUNCH();
     // category positions
     const unsigned keptPT = (1u << kNone) | (1u << kSelfPT) | (1u << kMap);

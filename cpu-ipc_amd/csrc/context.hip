@@ -88,11 +88,6 @@ int cipc_gpu_context_create(int device, cipc_gpu_context** out) {
         CIPC_HIP(hipEventCreateWithFlags(&c->sb_done, hipEventDisableTiming));
         CIPC_HIP(hipEventCreateWithFlags(&c->it_start, hipEventDisableTiming));
         CIPC_HIP(hipEventCreateWithFlags(&c->en_done, hipEventDisableTiming));
-        CIPC_HIP(hipStreamCreateWithFlags(&c->as2, hipStreamNonBlocking));
-        CIPC_HIP(hipEventCreateWithFlags(&c->as2_fork, hipEventDisableTiming));
-        CIPC_HIP(hipEventCreateWithFlags(&c->as2_join, hipEventDisableTiming));
-        CIPC_HIP(hipEventCreateWithFlags(&c->as2_fork2, hipEventDisableTiming));
-        CIPC_HIP(hipEventCreateWithFlags(&c->as2_join2, hipEventDisableTiming));
         *out = c;
     });
 }
@@ -104,12 +99,6 @@ int cipc_gpu_context_destroy(cipc_gpu_context* c) {
         (void)hipStreamSynchronize(c->stream);
         (void)hipStreamSynchronize(c->aux);
         (void)hipStreamSynchronize(c->sbs);
-        (void)hipStreamSynchronize(c->as2);
-        (void)hipEventDestroy(c->as2_fork);
-        (void)hipEventDestroy(c->as2_join);
-        (void)hipEventDestroy(c->as2_fork2);
-        (void)hipEventDestroy(c->as2_join2);
-        (void)hipStreamDestroy(c->as2);
         (void)hipEventDestroy(c->fork);
         (void)hipEventDestroy(c->join);
         (void)hipEventDestroy(c->fr_fork);
@@ -918,8 +907,8 @@ struct IterEvents {
 // The streams fork from and join into the main stream through the context's events, so the
 // same launches can be captured into a graph.  t (eager only): per-stage timing events.
 void enqueue_iteration(cipc_gpu_context* c, double kappa, bool energy, const IterEvents* t) {
-    // The host enqueues the critical path first (the active set on the main stream and its
-    // edge half on `as2`), the elastic stage and the lagged friction records on `aux` (they
+    // The host enqueues the critical path first (the active set on the main stream), the
+    // elastic stage and the lagged friction records on `aux` (they
     // read the state and the lagged friction only), the barrier records on the main stream;
     // then what has slack: the ground bound, swept trees, full and partial CCD on `sbs`, the
     // energy on `aux`; then the CSR on the main stream.  Host enqueue time is close to half
@@ -1020,7 +1009,7 @@ bool iteration_graph(cipc_gpu_context* c, double kappa, bool energy, hipEvent_t 
             // a launch that cannot be captured: no graph for this context from now on; every
             // stream leaves capture and drains before the launches go eagerly
             c->graph.mode = 0;
-            for (hipStream_t st : {c->stream, c->aux, c->sbs, c->as2}) {
+            for (hipStream_t st : {c->stream, c->aux, c->sbs}) {
                 hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
                 if (hipStreamIsCapturing(st, &cs) == hipSuccess && cs != hipStreamCaptureStatusNone) {
                     hipGraph_t dropped = nullptr;

@@ -124,11 +124,6 @@ struct Context {
     // the trees (the assembly does not read them) and sorts with its own temporary storage.
     hipStream_t sbs = nullptr;
     hipEvent_t as_done = nullptr, sb_done = nullptr;
-    // The active set builds the edge tree and walks it on `as2` (default priority) while the
-    // main stream builds the face tree and walks it (fork / join events; own temporary storage).
-    hipStream_t as2 = nullptr;
-    hipEvent_t as2_fork = nullptr, as2_join = nullptr, as2_fork2 = nullptr, as2_join2 = nullptr;
-    DBuf<unsigned char> tmp_as2;
     // start of an iteration (the step-bound stream forks there) and end of its energy (joins
     // the main stream): a captured iteration forks and joins through these
     hipEvent_t it_start = nullptr, en_done = nullptr;
