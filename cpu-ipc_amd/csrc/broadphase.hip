@@ -988,7 +988,9 @@ struct BothWalks {
     double sq, gd, hhat;
     d3 gn;
     int *cntPT, *cntEE, *fg, *cntpPT, *cntpEE, *bufPT, *bufEE;
-    const int *oPT, *oEE, *dc;
+    const int *oPT, *oEE;
+    int* dc;
+    int capPT, capEE;
     int *candUPT, *candUEE, *ownPT, *ownEE;
 };
 template <int PBP, int PBE>
@@ -1001,11 +1003,24 @@ __global__ __launch_bounds__(64) void traverse_both(BothWalks a) {
     else
         ee_traverse_at<PBE>((mb - a.nbp) * 64 + threadIdx.x, stk, a.Es, a.eperm, a.se, a.x, a.et, a.sq, a.cntEE, a.cntpEE, a.bufEE);
     if (blockIdx.x == 0 && threadIdx.x == 0) { a.cntPT[a.Vs] = 0; a.fg[a.Vs] = 0; a.cntEE[a.Es] = 0; }   // the scans' tails
+    if (blockIdx.x == 0 && threadIdx.x < kDcWords) a.dc[threadIdx.x] = 0;   // the iteration's device counts start at zero
 }
+// The compaction also publishes the candidate totals (the per-query scans' last entries)
+// against their capacities, as candidate_totals does; every thread reads them from the scan,
+// so none waits for that write.
 template <int PBP, int PBE>
 __global__ __launch_bounds__(64) void compact_both(BothWalks a) {
     __shared__ int stk[kStack * 64];
     const int b = blockIdx.x;
+    const int nPT = a.oPT[a.Vs], nEE = a.oEE[a.Es] - a.oEE[0];
+    const int flags = (nPT > a.capPT ? 1 : 0) | (nEE > a.capEE ? 2 : 0);
+    if (b == 0 && threadIdx.x == 0) {
+        a.dc[kDcPT] = nPT;
+        a.dc[kDcEE] = nEE;
+        a.dc[kDcFlags] = flags;
+    }
+    if (flags) return;   // over capacity: the host builds the set again
+    // (the compactions' own check of dc[kDcFlags] reads zero here, before or after the write)
     if (b < a.nbp)
         pt_compact_at<PBP>(b * 64 + threadIdx.x, stk, a.Vs, a.pperm, a.sv, a.x, a.ft, a.sq, a.oPT, a.cntpPT, a.bufPT, a.candUPT,
                            a.ownPT, a.dc);
@@ -1071,7 +1086,7 @@ static void launch_both_walks(Context& c, bool compact, hipStream_t st) {
     a.sq = sqrt(s.hhat); a.gd = s.gd; a.hhat = s.hhat; a.gn = s.gn;
     a.cntPT = b.cnt_all.p; a.cntEE = a.cntPT + (s.Vs + 1); a.fg = a.cntEE + (s.Es + 1);
     a.cntpPT = b.cntpPT.p; a.cntpEE = b.cntpEE.p; a.bufPT = b.bufPT.p; a.bufEE = b.bufEE.p;
-    a.oPT = b.off_all.p; a.oEE = a.oPT + (s.Vs + 1); a.dc = c.dcnt.p;
+    a.oPT = b.off_all.p; a.oEE = a.oPT + (s.Vs + 1); a.dc = c.dcnt.p; a.capPT = b.capPT; a.capEE = b.capEE;
     a.candUPT = b.candUPT.p; a.candUEE = b.candUEE.p; a.ownPT = b.ownPT.p; a.ownEE = b.ownEE.p;
     with_splits(pbp, pbe, [&](auto P, auto E) {
         if (compact) hipLaunchKernelGGL((compact_both<decltype(P)::value, decltype(E)::value>), dim3(a.nbp + nbe), dim3(64), 0, st, a);
@@ -1151,7 +1166,8 @@ void compute_active_set(Context& c, bool publish) {
     const int capPT = b.capPT, capEE = b.capEE, capM = b.capM;
     c.dcnt.resize(kDcWords);
     int* dc = c.dcnt.p;
-    hipLaunchKernelGGL(dc_reset, dim3(1), dim3(64), 0, c.stream, dc);
+    const bool both = Vs > 0 && Es > 0;
+    if (!both) hipLaunchKernelGGL(dc_reset, dim3(1), dim3(64), 0, c.stream, dc);   // else the joint walk clears them
     // hit counts per point and per edge query and the ground flags, in one array for one scan
     const int nall = (Vs + 1) + (Es + 1) + (Vs + 1);
     b.cnt_all.resize(nall); b.off_all.resize(nall);
@@ -1162,7 +1178,6 @@ void compute_active_set(Context& c, bool publish) {
     const int* oEE = oPT + (Vs + 1);
     const int* oG = oEE + (Es + 1);
     size_active_walks(c, c.tune.as_pt_bits, c.tune.as_ee_bits);
-    const bool both = Vs > 0 && Es > 0;
     // A: one walk per query part (the point walk also sets the ground flags; the walks zero
     // the scan segments' tails)
     if (both) {
@@ -1176,7 +1191,7 @@ void compute_active_set(Context& c, bool publish) {
     CIPC_CHECK_LAUNCH();
     // one scan: the edge and ground segments' starts come out offset by their first entry
     exclusive_scan_int(c, b.cnt_all.p, b.off_all.p, nall);
-    hipLaunchKernelGGL(candidate_totals, dim3(1), dim3(64), 0, c.stream, Vs, Es, oPT, oEE, capPT, capEE, dc);
+    if (!both) hipLaunchKernelGGL(candidate_totals, dim3(1), dim3(64), 0, c.stream, Vs, Es, oPT, oEE, capPT, capEE, dc);   // else the joint compaction
     CIPC_CHECK_LAUNCH();
     // B + C: pack, sort, filter and classify the hits
     b.candUPT.resize(capPT + 1); b.candUEE.resize(capEE + 1);
