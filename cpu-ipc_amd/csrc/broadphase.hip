@@ -832,10 +832,10 @@ __device__ inline unsigned long long map_key(int4 r, int V, int bits) {
     return ((unsigned long long)(r.x + V) << (2 * bits)) | ((unsigned long long)r.y << bits) | (unsigned long long)(r.z + 1);
 }
 // Point-triangle hits -> Self_ActiveSet head (a), map keys (b), partial-CCD pairs (c).
-__device__ inline void scatter_pt(int i, const int* __restrict__ dc, const int* __restrict__ cat, const int4* __restrict__ pos,
+__device__ inline void scatter_pt(int i, int nPT, const int* __restrict__ cat, const int4* __restrict__ pos,
                                   const int4* __restrict__ rec, const int* __restrict__ own, const int* __restrict__ cand, int V, int bits,
                                   int4* __restrict__ self_set, unsigned long long* __restrict__ mkeys, int2* __restrict__ ccd) {
-    if (dc[kDcFlags] || i >= dc[kDcPT]) return;
+    if (i >= nPT) return;
     const int ct = cat[i];
     if (ct == kDrop) return;
     const int4 p = pos[i];
@@ -844,13 +844,15 @@ __device__ inline void scatter_pt(int i, const int* __restrict__ dc, const int* 
     else if (ct == kMap) mkeys[p.y] = map_key(rec[i], V, bits);
 }
 // Edge-edge hits -> Self_ActiveSet (a), map keys (b), Self_EE_ActiveSet + eIe (c), CCD pairs (d).
-__device__ inline void scatter_ee(int i, const int* __restrict__ dc, const int* __restrict__ cat, const int4* __restrict__ pos,
+// base: the point hits' totals (Self_ActiveSet, map keys, CCD pairs), where the edge hits'
+// entries start.
+__device__ inline void scatter_ee(int i, int nEE, int4 base, const int* __restrict__ cat, const int4* __restrict__ pos,
                                   const int4* __restrict__ rec, const int2* __restrict__ aux, const int* __restrict__ own,
                                   const int* __restrict__ cand, int V, int bits, int4* __restrict__ self_set,
                                   unsigned long long* __restrict__ mkeys, int4* __restrict__ ee_set, int2* __restrict__ ee_ie,
                                   int2* __restrict__ ccd) {
-    if (dc[kDcFlags] || i >= dc[kDcEE]) return;
-    const int base_self = dc[kDcA], base_map = dc[kDcA2], base_ccd = dc[kDcAK];
+    if (i >= nEE) return;
+    const int base_self = base.x, base_map = base.y, base_ccd = base.z;
     const int ct = cat[i];
     if (ct == kDrop) return;
     const int4 q = pos[i], z = pos[0];   // the joint scan's edge segment starts at the point totals
@@ -863,11 +865,7 @@ __device__ inline void scatter_ee(int i, const int* __restrict__ dc, const int* 
 // Counts after the category scans: the per-category totals are the scans' values at the hit
 // counts, the ground count the ground scan's total.  The map keys fill [0, nm) of a
 // buffer of capM; past nm they are the all-ones key, which sorts last.
-__global__ void category_totals(int Vs, const int4* __restrict__ posPT, const int4* __restrict__ posEE, const int* __restrict__ pg,
-                                int capM, int* __restrict__ dc) {
-    if (blockIdx.x != 0 || threadIdx.x != 0 || dc[kDcFlags]) return;
-    const int4 a = posPT[dc[kDcPT]], q = posEE[dc[kDcEE]], z = posEE[0];
-    const int4 b = make_int4(q.x - z.x, q.y - z.y, q.z - z.z, q.w - z.w);
+__device__ inline void category_totals(int Vs, const int4& a, const int4& b, const int* __restrict__ pg, int capM, int* __restrict__ dc) {
     const int nm = a.y + b.y;
     dc[kDcMapKeys] = nm;
     if (nm > capM) { dc[kDcFlags] |= 4; return; }
@@ -878,9 +876,9 @@ __global__ void category_totals(int Vs, const int4* __restrict__ posPT, const in
     dc[kDcCCD] = a.z + b.w;
     dc[kDcEnv] = pg[Vs] - pg[0];
 }
-__device__ inline void map_tail(int i, int capM, const int* __restrict__ dc, unsigned long long* __restrict__ mkeys) {
-    if (i >= capM || dc[kDcFlags]) return;
-    if (i >= dc[kDcMapKeys]) mkeys[i] = ~0ull;
+__device__ inline void map_tail(int i, int capM, int nm, unsigned long long* __restrict__ mkeys) {
+    if (i >= capM) return;
+    if (i >= nm) mkeys[i] = ~0ull;
 }
 // After the run-length encoding of the sorted keys: the runs without the all-ones run (there
 // is one exactly when nm < capM) are the merged map entries, decoded after the A and B parts
@@ -904,9 +902,11 @@ __device__ inline void ground_scatter(int s, int Vs, const int* __restrict__ sv,
 }
 // After the category scan, in one launch (the four parts write disjoint entries): work-groups
 // [0, bPT) place the point hits, the next bEE the edge hits, the next bG the ground entries,
-// the rest write the map keys' all-ones tail.
+// the rest write the map keys' all-ones tail.  Every thread reads the per-category totals
+// (the scan's values at the hit counts) itself; the first publishes them (category_totals).
+// Nothing is placed when a capacity was exceeded: the host builds the set again.
 struct ScatterArgs {
-    const int* dc;
+    int* dc;
     int V, bits, Vs, capM, bPT, bEE, bG;
     const int *catPT, *ownPT, *candPT, *catEE, *ownEE, *candEE, *sv, *fg, *pg;
     const int4 *posPT, *recPT, *posEE, *recEE;
@@ -917,15 +917,23 @@ struct ScatterArgs {
     int* env;
 };
 __global__ void scatter_all(ScatterArgs a) {
+    int* dc = a.dc;
+    if (dc[kDcFlags]) return;   // the candidates exceeded their capacity (an earlier launch)
+    const int nPT = dc[kDcPT], nEE = dc[kDcEE];
+    const int4 ta = a.posPT[nPT], q = a.posEE[nEE], z = a.posEE[0];
+    const int4 tb = make_int4(q.x - z.x, q.y - z.y, q.z - z.z, q.w - z.w);
+    if (blockIdx.x == 0 && threadIdx.x == 0) category_totals(a.Vs, ta, tb, a.pg, a.capM, dc);
+    const int nm = ta.y + tb.y;
+    if (nm > a.capM) return;
     int b = blockIdx.x;
     if (b < a.bPT) {
-        scatter_pt(b * blockDim.x + threadIdx.x, a.dc, a.catPT, a.posPT, a.recPT, a.ownPT, a.candPT, a.V, a.bits, a.self_set,
-                   a.mkeys, a.ccd);
+        scatter_pt(b * blockDim.x + threadIdx.x, nPT, a.catPT, a.posPT, a.recPT, a.ownPT, a.candPT, a.V, a.bits, a.self_set, a.mkeys,
+                   a.ccd);
         return;
     }
     b -= a.bPT;
     if (b < a.bEE) {
-        scatter_ee(b * blockDim.x + threadIdx.x, a.dc, a.catEE, a.posEE, a.recEE, a.auxEE, a.ownEE, a.candEE, a.V, a.bits,
+        scatter_ee(b * blockDim.x + threadIdx.x, nEE, ta, a.catEE, a.posEE, a.recEE, a.auxEE, a.ownEE, a.candEE, a.V, a.bits,
                    a.self_set, a.mkeys, a.ee_set, a.ee_ie, a.ccd);
         return;
     }
@@ -935,7 +943,7 @@ __global__ void scatter_all(ScatterArgs a) {
         return;
     }
     b -= a.bG;
-    map_tail(b * blockDim.x + threadIdx.x, a.capM, a.dc, a.mkeys);
+    map_tail(b * blockDim.x + threadIdx.x, a.capM, nm, a.mkeys);
 }
 // the tails of the joint scan's point and ground segments when there is no point walk
 __global__ void point_tails(int Vs, int* __restrict__ cnt, int* __restrict__ fg) {
@@ -1227,8 +1235,6 @@ void compute_active_set(Context& c, bool publish) {
     }
     const int4* posPT = b.pos_all.p;
     const int4* posEE = posPT + (capPT + 1);
-    hipLaunchKernelGGL(category_totals, dim3(1), dim3(64), 0, c.stream, Vs, posPT, posEE, oG, capM, dc);
-    CIPC_CHECK_LAUNCH();
     // PP/PE map keys need the vertex count to fit three fields in 64 bits
     int bits = 1;
     while ((1ll << bits) < (long long)s.V + 2) ++bits;
