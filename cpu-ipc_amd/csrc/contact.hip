@@ -209,12 +209,12 @@ __global__ __launch_bounds__(64) void contact_ee_kernel(
 // The packed index of (i, j) is the one kernel 1 writes.
 // range 0: the Self_ActiveSet contacts, 1: the Self_EE_ActiveSet ones.
 template <int M, bool Structured>
-__global__ __launch_bounds__(64) void contact_pd_kernel(const int* __restrict__ dc, int range, int n, const int* __restrict__ dims,
-                                                        double* __restrict__ Hs, const double* __restrict__ Gs) {
+__device__ inline void contact_pd(int t, const int* __restrict__ dc, int range, int n, const int* __restrict__ dims,
+                                  double* __restrict__ Hs, const double* __restrict__ Gs) {
     int ns;
     const int nc = record_contacts(dc, ns);
     const int c0 = range == 0 ? 0 : ns, c1 = range == 0 ? ns : nc;
-    const int c = c0 + (int)(blockIdx.x * blockDim.x + threadIdx.x);
+    const int c = c0 + t;
     if (c >= c1 || dims[c] != 3 * M) return;
     double* h = Hs + c;
     auto acc = [&](int k) -> double& { return h[(size_t)k * n]; };
@@ -226,6 +226,19 @@ __global__ __launch_bounds__(64) void contact_pd_kernel(const int* __restrict__ 
     } else {
         project_contact<M>(acc);
     }
+}
+template <int M, bool Structured>
+__global__ __launch_bounds__(64) void contact_pd_kernel(const int* __restrict__ dc, int range, int n, const int* __restrict__ dims,
+                                                        double* __restrict__ Hs, const double* __restrict__ Gs) {
+    contact_pd<M, Structured>(blockIdx.x * blockDim.x + threadIdx.x, dc, range, n, dims, Hs, Gs);
+}
+// Both 4-vertex kinds in one launch (the same registers either way): row y = 0 of work-groups
+// the Self_ActiveSet contacts (structured basis), y = 1 the Self_EE_ActiveSet ones.
+__global__ __launch_bounds__(64) void contact_pd4_kernel(const int* __restrict__ dc, int n, const int* __restrict__ dims,
+                                                         double* __restrict__ Hs, const double* __restrict__ Gs) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (blockIdx.y == 0) contact_pd<4, true>(t, dc, 0, n, dims, Hs, Gs);
+    else contact_pd<4, false>(t, dc, 1, n, dims, Hs, Gs);
 }
 
 // Kernel 3: block and gradient records.  Slots: 10 blocks and 4 gradients per contact.  With
@@ -716,8 +729,7 @@ void contact_assemble(Context& c, double kappa, bool friction_launched) {
     // projections of 2- and 3-vertex contacts.  Each projection kernel touches only the
     // contacts of its size.
     CIPC_HIP(hipEventRecord(c.cl_done, c.stream));
-    hipLaunchKernelGGL((contact_pd_kernel<4, true>), dim3(blocks_for(capS, 64)), dim3(64), 0, c.stream, dc, 0, capS, c.cdims.p, Hs, Gs);
-    hipLaunchKernelGGL((contact_pd_kernel<4, false>), dim3(blocks_for(capS, 64)), dim3(64), 0, c.stream, dc, 1, capS, c.cdims.p, Hs, Gs);
+    hipLaunchKernelGGL(contact_pd4_kernel, dim3(blocks_for(capS, 64), 2), dim3(64), 0, c.stream, dc, capS, c.cdims.p, Hs, Gs);
     CIPC_CHECK_LAUNCH();
     CIPC_HIP(hipStreamWaitEvent(c.aux, c.fr_fork, 0));
     if (!friction_launched) friction_records(c, nbc, ngc, c.aux);
